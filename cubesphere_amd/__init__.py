@@ -1,0 +1,322 @@
+"""cubesphere_amd -- MI355X (gfx950) cell-update path of the CubeSphere solver.
+
+Host-side mirror of the reference's per-patch driver interface
+(_I/_1000riemann_geomesh_cs_mhd/1000riemann_geomesh_cs_mhd.f:2007-3120) over the
+C-ABI in include/cubesphere_hip.h.  The library is loaded with ctypes from this
+package directory; there is no fallback: if libcubesphere_hip.so is missing or
+no GPU is present the device entry points raise.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libcubesphere_hip.so")
+
+RECON_PCM, RECON_WENO5 = 0, 1
+RS_HLL, RS_HLLD = 0, 1
+EMF_LLF, EMF_HLL = 0, 1
+FP64, MIXED = 0, 1
+RBC_PAD, RBC_WALL = 0, 1
+RMESH_UNIFORM, RMESH_LOG = 0, 1
+KERNEL_NAMES = ["halo", "prims", "flux_x", "flux_y", "flux_z", "emf_z", "emf_x", "emf_y",
+                "update_cells", "update_faces", "cleanup", "cfl", "step"]
+(K_HALO, K_PRIMS, K_FLUX_X, K_FLUX_Y, K_FLUX_Z, K_EMF_Z, K_EMF_X, K_EMF_Y, K_UPDATE_CELLS,
+ K_UPDATE_FACES, K_CLEANUP, K_CFL, K_STEP) = range(13)
+
+
+class Config(ctypes.Structure):
+    """Mirror of cs_config."""
+    _fields_ = ([(n, ctypes.c_int32) for n in
+                 ("ntp", "nr", "nghost", "recon", "riemann", "emf", "precision",
+                  "rk_order", "radial_bc", "radial_mesh")] +
+                [(n, ctypes.c_double) for n in
+                 ("rmin", "rmax", "gamma", "rhofloor", "prsfloor", "cfl", "grav")] +
+                [("imex", ctypes.c_int32), ("reserved", ctypes.c_int32)])
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Sizes(ctypes.Structure):
+    _fields_ = [("n_cell", ctypes.c_int64), ("n_face", ctypes.c_int64),
+                ("n_vert2d", ctypes.c_int64), ("n_interior", ctypes.c_int64)]
+
+
+def make_config(ntp, nr=1, nghost=3, recon=RECON_WENO5, riemann=RS_HLLD, emf=EMF_LLF,
+                precision=FP64, rk_order=3, radial_bc=None, radial_mesh=RMESH_UNIFORM,
+                rmin=1.0, rmax=None, gamma=5.0 / 3.0, rhofloor=1e-12, prsfloor=1e-9,
+                cfl=0.4, grav=0.0, imex=0):
+    """Build a Config.  nr == 1 is the thin-shell surface mode: rmax defaults
+    to rmin + (pi/2)/ntp and the radial boundaries are conducting walls."""
+    if rmax is None:
+        rmax = rmin + (0.5 * math.pi) / ntp if nr == 1 else 1.5 * rmin
+    if radial_bc is None:
+        radial_bc = RBC_WALL if nr == 1 else RBC_PAD
+    return Config(ntp, nr, nghost, recon, riemann, emf, precision, rk_order, radial_bc,
+                  radial_mesh, rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav, imex, 0)
+
+
+# The five workloads of the benchmark description (sizes, schemes, ICs).
+def workload(name, precision=FP64):
+    """Return (config, problem id, steps) of a named workload."""
+    if name == "C1":   # 6x32x32 surface, WENO5 + HLL + SSP-RK3, Gaussian bump + rotation
+        return make_config(32, 1, 3, RECON_WENO5, RS_HLL, EMF_LLF, precision, 3), 1, 100
+    if name == "C2":   # 6x256^2 surface, WENO + HLLD + SSP-RK3, CT, magnetised blast
+        return make_config(256, 1, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 3), 2, 500
+    if name == "C3":   # 6x192^2x64 shell r in [1,1.5], WENO5 + HLLD + CT, gravity 1/r^2
+        return make_config(192, 64, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 3,
+                           RBC_PAD, RMESH_UNIFORM, 1.0, 1.5, grav=1.0), 3, 10
+    if name == "C5":   # 6x512^2x128
+        return make_config(512, 128, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 3,
+                           RBC_PAD, RMESH_UNIFORM, 1.0, 1.5, grav=1.0), 3, 50
+    raise ValueError(name)
+
+
+_lib = None
+
+
+class CsError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libcubesphere_hip.so (built by cubesphere_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise CsError(f"{LIBPATH} is missing: run `python -m cubesphere_amd.build`")
+        L = ctypes.CDLL(LIBPATH)
+        vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+        cfgp = ctypes.POINTER(Config)
+        sig = {
+            "cs_sizes_of": [cfgp, ctypes.POINTER(Sizes)],
+            "cs_build_unit_lattice": [cfgp, vp],
+            "cs_build_radial_mesh": [cfgp, vp],
+            "cs_build_metric": [cfgp, vp, vp, vp, vp],
+            "cs_init_problem": [cfgp, i32, ctypes.c_uint64, vp, vp, vp, vp, vp],
+            "cs_create": [cfgp, ctypes.POINTER(vp)],
+            "cs_destroy": [vp],
+            "cs_upload_state": [vp, vp, vp, vp, vp, vp],
+            "cs_download_state": [vp, vp, vp, vp, vp, vp],
+            "cs_halo_exchange": [vp],
+            "cs_make_facial_fluxes": [vp],
+            "cs_make_edge_efields": [vp],
+            "cs_update_stage": [vp, i32],
+            "cs_evaluate_timestep": [vp],
+            "cs_set_dt": [vp, dbl],
+            "cs_get_dt": [vp, ctypes.POINTER(dbl)],
+            "cs_rk_stage": [vp, i32],
+            "cs_advance": [vp, i32, i32],
+            "cs_download_fluxes": [vp, i32, vp, vp],
+            "cs_download_emf": [vp, vp, vp, vp],
+            "cs_conservation_sums": [vp, vp],
+            "cs_max_divb": [vp, ctypes.POINTER(dbl)],
+            "cs_time_steps": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
+            "cs_time_kernel": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
+            "cs_kernel_units": [vp, i32, ctypes.POINTER(i64)],
+            "cs_last_error": [ctypes.c_char_p, ctypes.c_size_t],
+            "cs_device_info": [ctypes.c_char_p, ctypes.c_size_t],
+        }
+        for name, args in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def last_error():
+    buf = ctypes.create_string_buffer(512)
+    lib().cs_last_error(buf, 512)
+    return buf.value.decode()
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise CsError(f"{what} failed ({rc}): {last_error()}")
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def sizes(cfg):
+    s = Sizes()
+    _check(lib().cs_sizes_of(ctypes.byref(cfg), ctypes.byref(s)), "cs_sizes_of")
+    return s
+
+
+class HostState:
+    """Host arrays in the reference patch layout (Fortran order, 6 patches)."""
+
+    def __init__(self, cfg):
+        s = sizes(cfg)
+        self.cfg = cfg
+        self.U = np.zeros(s.n_cell * 8)
+        self.ent = np.zeros(s.n_cell)
+        self.Bx = np.zeros(s.n_face)
+        self.By = np.zeros(s.n_face)
+        self.Bz = np.zeros(s.n_face)
+
+    def arrays(self):
+        return [self.U, self.ent, self.Bx, self.By, self.Bz]
+
+    def copy(self):
+        o = HostState.__new__(HostState)
+        o.cfg = self.cfg
+        o.U, o.ent, o.Bx, o.By, o.Bz = (a.copy() for a in self.arrays())
+        return o
+
+    # views with named axes: U[s, var, k, j, i], faces[s, k, j, i]
+    def views(self):
+        c = self.cfg
+        nc, nrc = c.ntp + 2 * c.nghost, c.nr + 2 * c.nghost
+        nf, nrf = nc + 1, nrc + 1
+        return (self.U.reshape(6, 8, nrc, nc, nc), self.ent.reshape(6, nrc, nc, nc),
+                self.Bx.reshape(6, nrf, nf, nf), self.By.reshape(6, nrf, nf, nf),
+                self.Bz.reshape(6, nrf, nf, nf))
+
+    def interior(self):
+        """U (6,8,nr,N,N) and ent of the owned zones."""
+        c = self.cfg
+        g, N, NR = c.nghost, c.ntp, c.nr
+        U, ent = self.views()[:2]
+        return U[:, :, g:g + NR, g:g + N, g:g + N], ent[:, g:g + NR, g:g + N, g:g + N]
+
+
+def init_problem(cfg, problem, seed=1234):
+    st = HostState(cfg)
+    _check(lib().cs_init_problem(ctypes.byref(cfg), problem, seed, *[_p(a) for a in st.arrays()]),
+           "cs_init_problem")
+    return st
+
+
+def metric(cfg):
+    s = sizes(cfg)
+    vol, xa, ya, za = (np.zeros(n) for n in (s.n_cell, s.n_face, s.n_face, s.n_face))
+    _check(lib().cs_build_metric(ctypes.byref(cfg), _p(vol), _p(xa), _p(ya), _p(za)),
+           "cs_build_metric")
+    return vol, xa, ya, za
+
+
+def unit_lattice(cfg):
+    s = sizes(cfg)
+    a = np.zeros(s.n_vert2d * 3)
+    _check(lib().cs_build_unit_lattice(ctypes.byref(cfg), _p(a)), "cs_build_unit_lattice")
+    return a
+
+
+class CubeSphere:
+    """One shell (all six sectors) resident on the GPU.
+
+    Methods mirror the reference's per-patch call chain:
+      halo_exchange()        FETCH_BUFFERS_FILL_IN_UX/UY/UY_TOP/UZ (330-333)
+      make_facial_fluxes()   LIMIT_ALL_SLOPES_P1 + MAKE_FACIAL_FLUXES (410, 450)
+      make_edge_efields()    MAKE_EDGE_EFIELDS_P1 (460)
+      update_stage(irk)      UPDATE_IMEX_RK_STEP_P1 + CLEANUP_AFTER_UPDATE (484, 485)
+      evaluate_timestep()    EVALUATE_TIMESTEP (490)
+    """
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        h = ctypes.c_void_p()
+        _check(lib().cs_create(ctypes.byref(cfg), ctypes.byref(h)), "cs_create")
+        self.h = h
+        self.sizes = sizes(cfg)
+
+    def close(self):
+        if self.h:
+            lib().cs_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, st):
+        _check(lib().cs_upload_state(self.h, *[_p(a) for a in st.arrays()]), "cs_upload_state")
+
+    def download(self):
+        st = HostState(self.cfg)
+        _check(lib().cs_download_state(self.h, *[_p(a) for a in st.arrays()]), "cs_download_state")
+        return st
+
+    def halo_exchange(self):
+        _check(lib().cs_halo_exchange(self.h), "cs_halo_exchange")
+
+    def make_facial_fluxes(self):
+        _check(lib().cs_make_facial_fluxes(self.h), "cs_make_facial_fluxes")
+
+    def make_edge_efields(self):
+        _check(lib().cs_make_edge_efields(self.h), "cs_make_edge_efields")
+
+    def update_stage(self, irk):
+        _check(lib().cs_update_stage(self.h, irk), "cs_update_stage")
+
+    def evaluate_timestep(self):
+        _check(lib().cs_evaluate_timestep(self.h), "cs_evaluate_timestep")
+
+    def rk_stage(self, irk):
+        _check(lib().cs_rk_stage(self.h, irk), "cs_rk_stage")
+
+    def advance(self, nsteps, use_graph=True):
+        _check(lib().cs_advance(self.h, nsteps, 1 if use_graph else 0), "cs_advance")
+
+    @property
+    def dt(self):
+        v = ctypes.c_double()
+        _check(lib().cs_get_dt(self.h, ctypes.byref(v)), "cs_get_dt")
+        return v.value
+
+    @dt.setter
+    def dt(self, value):
+        _check(lib().cs_set_dt(self.h, float(value)), "cs_set_dt")
+
+    def fluxes(self, d):
+        F = np.zeros(self.sizes.n_face * 9)
+        S = np.zeros(self.sizes.n_face * 2)
+        _check(lib().cs_download_fluxes(self.h, d, _p(F), _p(S)), "cs_download_fluxes")
+        return F, S
+
+    def emf(self):
+        E = [np.zeros(self.sizes.n_face) for _ in range(3)]
+        _check(lib().cs_download_emf(self.h, *[_p(a) for a in E]), "cs_download_emf")
+        return E
+
+    def conservation_sums(self):
+        out = np.zeros(9)
+        _check(lib().cs_conservation_sums(self.h, _p(out)), "cs_conservation_sums")
+        return out
+
+    def max_divb(self):
+        v = ctypes.c_double()
+        _check(lib().cs_max_divb(self.h, ctypes.byref(v)), "cs_max_divb")
+        return v.value
+
+    def time_steps(self, nsteps, use_graph=True):
+        ms = ctypes.c_float()
+        _check(lib().cs_time_steps(self.h, nsteps, 1 if use_graph else 0, ctypes.byref(ms)),
+               "cs_time_steps")
+        return ms.value
+
+    def time_kernel(self, kernel, iters=10):
+        """Average duration (ms) of one kernel measured with hipEvents."""
+        ms = ctypes.c_float()
+        _check(lib().cs_time_kernel(self.h, kernel, iters, ctypes.byref(ms)), "cs_time_kernel")
+        return ms.value
+
+    def kernel_units(self, kernel):
+        n = ctypes.c_int64()
+        _check(lib().cs_kernel_units(self.h, kernel, ctypes.byref(n)), "cs_kernel_units")
+        return n.value
+
+
+def device_info():
+    buf = ctypes.create_string_buffer(256)
+    rc = lib().cs_device_info(buf, 256)
+    return buf.value.decode() if rc == 0 else None

@@ -1,0 +1,51 @@
+"""Build libcubesphere_hip.so (HIP for gfx950) in-tree.
+
+The host metric (cs_mesh.cpp) is compiled by g++ with -ffp-contract=off so the
+vertex lattice is bit-identical to the reference's Fortran arithmetic; the
+kernels are compiled by hipcc for gfx950 only.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+LIB = os.path.join(HERE, "libcubesphere_hip.so")
+BUILD = os.path.join(HERE, "_build")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("CS_OFFLOAD_ARCH", "gfx950")
+HIP_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result", "-Wno-unused-value",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} (exit {r.returncode})")
+    return r
+
+
+def build(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    mesh_o = os.path.join(BUILD, "cs_mesh.o")
+    kern_o = os.path.join(BUILD, "cs_kernels.o")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-c",
+          os.path.join(CSRC, "cs_mesh.cpp"), "-o", mesh_o])
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "cs_kernels.hip"), "-o", kern_o])
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", mesh_o, kern_o, "-o", tmp])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,1355 @@
+// cs_kernels.hip -- HIP kernels for gfx950 and the device half of the C-ABI.
+//
+// One handle holds all six cube sectors of one shell resident in HBM in the
+// reference's panel-major Fortran layout (x fastest).  A stage is
+//   halo gather -> prims -> face fluxes (x,y,z) -> edge EMFs (x,y,z)
+//   -> cell update -> face (CT) update -> cleanup,
+// and a step ends with the CFL min-reduction; dt stays on the device so a
+// whole step can be captured in one hipGraph.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cubesphere_hip.h"
+#include "cs_internal.h"
+#include "cs_physics.h"
+
+using namespace csp;
+
+namespace cs {
+
+static thread_local std::string g_err;
+void set_error(const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+int validate_config(const cs_config* c);
+int host_metric(const cs_config* c, std::vector<double>& rb, std::vector<double>& T);
+
+#define CS_HIP(call)                                                                    \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, __LINE__); \
+      return CS_E_HIP;                                                                  \
+    }                                                                                   \
+  } while (0)
+
+// ------------------------------------------------------------------ ranges
+struct Rng {
+  int i0, ni, j0, nj, k0, nk;
+  uint32_t total;
+};
+static Rng make_rng(int i0, int i1, int j0, int j1, int k0, int k1) {
+  Rng r;
+  r.i0 = i0; r.ni = i1 - i0 + 1;
+  r.j0 = j0; r.nj = j1 - j0 + 1;
+  r.k0 = k0; r.nk = k1 - k0 + 1;
+  r.total = (uint32_t)((uint64_t)r.ni * r.nj * r.nk * 6);
+  return r;
+}
+__device__ __forceinline__ bool decode(const Rng& r, uint32_t t, int& i, int& j, int& k, int& s) {
+  if (t >= r.total) return false;
+  uint32_t q = t / (uint32_t)r.ni;
+  i = r.i0 + (int)(t - q * (uint32_t)r.ni);
+  uint32_t q2 = q / (uint32_t)r.nj;
+  j = r.j0 + (int)(q - q2 * (uint32_t)r.nj);
+  uint32_t q3 = q2 / (uint32_t)r.nk;
+  k = r.k0 + (int)(q2 - q3 * (uint32_t)r.nk);
+  s = (int)q3;
+  return true;
+}
+__device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ------------------------------------------------------------------ device views
+struct Geo {
+  const double* T;   // 2-D tables
+  const double* rb;  // rb[k + g]
+  int64_t plane;     // t2 * 6
+  int64_t t2;
+  __device__ __forceinline__ double t(const CsDims& d, int f, int s, int i, int j) const {
+    return T[f * plane + s * t2 + d.t(i, j)];
+  }
+  __device__ __forceinline__ double r(const CsDims& d, int k) const { return rb[k + d.g]; }
+};
+
+struct State {
+  double* U;    // [s][8][cell]
+  double* ent;  // [s][cell]
+  double* Bx;   // [s][face]
+  double* By;
+  double* Bz;
+};
+
+struct Params {
+  double gamma, gm1, rhofloor, prsfloor, cfl, grav;
+  int recon, riemann, emf, wall;
+};
+
+// sector neighbours right, left, up, down (203fill_ghost2d_global.f:37-75), 0-based
+__constant__ int c_nbr[6][4] = {{2, 5, 1, 4}, {0, 3, 2, 5}, {1, 4, 0, 3},
+                                {4, 1, 5, 2}, {5, 2, 3, 0}, {3, 0, 4, 1}};
+
+// ------------------------------------------------------------------ halo exchange
+// Coinciding panel-edge faces: one thread per physical face writes both copies
+// with the averaging of 330fetch_buffers_fill_in_ux.f:337-350,
+// new = (mine + (-theirs)) * 0.5, which is negation-consistent bit for bit.
+__global__ void k_avg_coinciding(CsDims d, State st) {
+  uint32_t t = gtid();
+  const int N = d.N, NR = d.NR;
+  uint32_t per = (uint32_t)N * NR;
+  if (t >= 12u * per) return;
+  int pair = t / per;
+  uint32_t rem = t - pair * per;
+  int jj = 1 + (int)(rem % N);
+  int k = 1 + (int)(rem / N);
+  double *pa, *pb;
+  if (pair < 6) {  // right edge of sector `pair` with the top edge of its right neighbour
+    int s = pair, o = c_nbr[s][0];
+    pa = st.Bx + s * d.fpp + d.face(N, jj, k);
+    pb = st.By + o * d.fpp + d.face(jj, N, k);
+  } else if (pair < 9) {  // left edges 1-6, 2-4, 3-5 (1-based)
+    int s = pair - 6, o = c_nbr[s][1];
+    pa = st.Bx + s * d.fpp + d.face(0, jj, k);
+    pb = st.Bx + o * d.fpp + d.face(0, N + 1 - jj, k);
+  } else {  // bottom edges 1-5, 2-6, 3-4
+    int s = pair - 9, o = c_nbr[s][3];
+    pa = st.By + s * d.fpp + d.face(jj, 0, k);
+    pb = st.By + o * d.fpp + d.face(N + 1 - jj, 0, k);
+  }
+  double a = *pa, b = *pb;
+  *pa = (a + (-b)) * 0.5;
+  *pb = (b + (-a)) * 0.5;
+}
+
+// Zone-centred ghost strips (330-333 index flips): right, left, up, down.
+__global__ void k_exchange_cells(CsDims d, State st) {
+  uint32_t t = gtid();
+  const int N = d.N, NR = d.NR, g = d.g;
+  uint32_t per = (uint32_t)g * N * NR;
+  if (t >= 24u * per) return;
+  int s = t / (4 * per);
+  uint32_t r = t - s * 4 * per;
+  int strip = r / per;
+  r -= strip * per;
+  int a = 1 + (int)(r % g);
+  r /= g;
+  int tt = 1 + (int)(r % N);
+  int k = 1 + (int)(r / N);
+  int o = c_nbr[s][strip];
+  int di, dj, si, sj;
+  if (strip == 0) { di = N + a; dj = tt; si = tt; sj = N + 1 - a; }
+  else if (strip == 1) { di = 1 - a; dj = tt; si = a; sj = N + 1 - tt; }
+  else if (strip == 2) { di = tt; dj = N + a; si = N + 1 - a; sj = tt; }
+  else { di = tt; dj = 1 - a; si = N + 1 - tt; sj = a; }
+  int64_t dc = d.cell(di, dj, k), sc = d.cell(si, sj, k);
+  for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = st.U[(o * 8 + c) * d.cpp + sc];
+  st.ent[s * d.cpp + dc] = st.ent[o * d.cpp + sc];
+}
+
+// Face-centred ghost strips with the face-B swap and sign rules
+// (330fetch_buffers_fill_in_ux.f:232-294 and the y/z counterparts).
+__global__ void k_exchange_faces(CsDims d, State st) {
+  uint32_t t = gtid();
+  const int N = d.N, NR = d.NR, g = d.g;
+  // per (panel, strip, a): 3 families x (N+1) x (NR+1) slots
+  uint32_t per = 3u * (N + 1) * (NR + 1);
+  uint32_t tot = 6u * 4u * g * per;
+  if (t >= tot) return;
+  int s = t / (4u * g * per);
+  uint32_t r = t - s * (4u * g * per);
+  int strip = r / (g * per);
+  r -= strip * (g * per);
+  int a = 1 + (int)(r / per);
+  r -= (a - 1) * per;
+  int fam = r / ((N + 1) * (NR + 1));
+  r -= fam * ((N + 1) * (NR + 1));
+  int tt = (int)(r % (N + 1));
+  int k = (int)(r / (N + 1));
+  int o = c_nbr[s][strip];
+  double* dst = nullptr;
+  double val = 0.0;
+  // fam 0: the face family normal to the strip direction, t in [1,N], k in [1,NR]
+  // fam 1: the tangential family, t in [0,N], k in [1,NR]
+  // fam 2: Bzf, t in [1,N], k in [0,NR]
+  if (fam == 0 && (tt < 1 || k < 1)) return;
+  if (fam == 1 && k < 1) return;
+  if (fam == 2 && tt < 1) return;
+  if (strip == 0) {  // right
+    if (fam == 0) { dst = st.Bx + s * d.fpp + d.face(N + a, tt, k); val = -st.By[o * d.fpp + d.face(tt, N - a, k)]; }
+    else if (fam == 1) { dst = st.By + s * d.fpp + d.face(N + a, tt, k); val = st.Bx[o * d.fpp + d.face(tt, N + 1 - a, k)]; }
+    else { dst = st.Bz + s * d.fpp + d.face(N + a, tt, k); val = st.Bz[o * d.fpp + d.face(tt, N + 1 - a, k)]; }
+  } else if (strip == 1) {  // left
+    if (fam == 0) { dst = st.Bx + s * d.fpp + d.face(-a, tt, k); val = -st.Bx[o * d.fpp + d.face(a, N + 1 - tt, k)]; }
+    else if (fam == 1) { dst = st.By + s * d.fpp + d.face(1 - a, tt, k); val = -st.By[o * d.fpp + d.face(a, N - tt, k)]; }
+    else { dst = st.Bz + s * d.fpp + d.face(1 - a, tt, k); val = st.Bz[o * d.fpp + d.face(a, N + 1 - tt, k)]; }
+  } else if (strip == 2) {  // up
+    if (fam == 0) { dst = st.By + s * d.fpp + d.face(tt, N + a, k); val = -st.Bx[o * d.fpp + d.face(N - a, tt, k)]; }
+    else if (fam == 1) { dst = st.Bx + s * d.fpp + d.face(tt, N + a, k); val = st.By[o * d.fpp + d.face(N + 1 - a, tt, k)]; }
+    else { dst = st.Bz + s * d.fpp + d.face(tt, N + a, k); val = st.Bz[o * d.fpp + d.face(N + 1 - a, tt, k)]; }
+  } else {  // down
+    if (fam == 0) { dst = st.By + s * d.fpp + d.face(tt, -a, k); val = -st.By[o * d.fpp + d.face(N + 1 - tt, a, k)]; }
+    else if (fam == 1) { dst = st.Bx + s * d.fpp + d.face(tt, 1 - a, k); val = -st.Bx[o * d.fpp + d.face(N - tt, a, k)]; }
+    else { dst = st.Bz + s * d.fpp + d.face(tt, 1 - a, k); val = st.Bz[o * d.fpp + d.face(N + 1 - tt, a, k)]; }
+  }
+  *dst = val;
+}
+
+// Corner ghost blocks (no sector owns them: the tri-corners) take the value of
+// the panel's own corner zone.
+__global__ void k_corner_fill(CsDims d, State st) {
+  uint32_t t = gtid();
+  const int N = d.N, NR = d.NR, g = d.g;
+  uint32_t per = (uint32_t)g * g * NR;
+  if (t >= 24u * per) return;
+  int s = t / (4 * per);
+  uint32_t r = t - s * 4 * per;
+  int corner = r / per;
+  r -= corner * per;
+  int a = 1 + (int)(r % g);
+  r /= g;
+  int b = 1 + (int)(r % g);
+  int k = 1 + (int)(r / g);
+  int ci = (corner & 1) ? N : 1, cj = (corner & 2) ? N : 1;
+  int di = (corner & 1) ? N + a : 1 - a, dj = (corner & 2) ? N + b : 1 - b;
+  int64_t dc = d.cell(di, dj, k), sc = d.cell(ci, cj, k);
+  for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = st.U[(s * 8 + c) * d.cpp + sc];
+  st.ent[s * d.cpp + dc] = st.ent[s * d.cpp + sc];
+}
+
+// Radial ghost layers: zero-gradient pad (826pad_boundary_r.f:201-214 inner
+// form, used at both ends) or a reflecting, perfectly conducting wall.
+__global__ void k_radial_bc(CsDims d, Geo geo, State st, int wall) {
+  uint32_t t = gtid();
+  const int N = d.N, NR = d.NR, g = d.g;
+  const int nl = N + 2 * g + 1;  // lateral index range [-g, N+g]
+  uint32_t per = (uint32_t)nl * nl;
+  if (t >= 6u * 2u * g * per) return;
+  int s = t / (2u * g * per);
+  uint32_t r = t - s * 2u * g * per;
+  int side = r / (g * per);
+  r -= side * g * per;
+  int a = 1 + (int)(r / per);
+  r -= (a - 1) * per;
+  int i = -g + (int)(r % nl), j = -g + (int)(r / nl);
+  // faces: Bxf (x-vertex i, cell j), Byf (cell i, y-vertex j), Bzf (cell i, cell j)
+  // Sources always lie in the interior, so no thread reads a ghost another
+  // thread writes; a wall mirror of a thin shell (nr < g) reflects repeatedly
+  // between the two walls.
+  int kg = side == 0 ? 1 - a : NR + a;  // ghost cell layer
+  int ks = side == 0 ? 1 : NR;
+  int sgn = 1;
+  if (wall) {
+    ks = kg;
+    while (ks < 1 || ks > NR) { ks = ks < 1 ? 1 - ks : 2 * NR + 1 - ks; sgn = -sgn; }
+  }
+  if (j >= 1 - g) st.Bx[s * d.fpp + d.face(i, j, kg)] = st.Bx[s * d.fpp + d.face(i, j, ks)];
+  if (i >= 1 - g) st.By[s * d.fpp + d.face(i, j, kg)] = st.By[s * d.fpp + d.face(i, j, ks)];
+  if (i >= 1 - g && j >= 1 - g) {
+    int kzg = side == 0 ? -a : NR + a;  // ghost z-vertex
+    int kzs = side == 0 ? 0 : NR, zs = 1;
+    if (wall) {
+      kzs = kzg;
+      while (kzs < 0 || kzs > NR) { kzs = kzs < 0 ? -kzs : 2 * NR - kzs; zs = -zs; }
+    }
+    double bz = st.Bz[s * d.fpp + d.face(i, j, kzs)];
+    st.Bz[s * d.fpp + d.face(i, j, kzg)] = zs < 0 ? -bz : bz;
+    int64_t dc = d.cell(i, j, kg), sc = d.cell(i, j, ks);
+    double u[8];
+    for (int c = 0; c < 8; ++c) u[c] = st.U[(s * 8 + c) * d.cpp + sc];
+    if (sgn < 0) {
+      // zone-centre direction: well defined also in the degenerate tri-corner ghost blocks
+      double n0 = geo.t(d, T_ZCX, s, i, j), n1 = geo.t(d, T_ZCY, s, i, j), n2 = geo.t(d, T_ZCZ, s, i, j);
+      double m = u[1] * n0 + u[2] * n1 + u[3] * n2;
+      u[1] -= 2.0 * m * n0; u[2] -= 2.0 * m * n1; u[3] -= 2.0 * m * n2;
+      double b = u[5] * n0 + u[6] * n1 + u[7] * n2;
+      u[5] -= 2.0 * b * n0; u[6] -= 2.0 * b * n1; u[7] -= 2.0 * b * n2;
+    }
+    for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = u[c];
+    st.ent[s * d.cpp + dc] = st.ent[s * d.cpp + sc];
+  }
+}
+
+// ------------------------------------------------------------------ primitives
+template <typename P>
+__global__ void k_prims(CsDims d, Rng rg, State st, P* V, Params pr) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  int64_t c = d.cell(i, j, k);
+  double u[8], v[8];
+  for (int q = 0; q < 8; ++q) u[q] = st.U[(s * 8 + q) * d.cpp + c];
+  double ent = st.ent[s * d.cpp + c];
+  Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor};
+  cons_to_prim(gs, u, v, ent);
+  for (int q = 0; q < 8; ++q) V[(s * 9 + q) * d.cpp + c] = (P)v[q];
+  V[(s * 9 + 8) * d.cpp + c] = (P)ent;
+}
+
+// ------------------------------------------------------------------ face fluxes
+// DIR 0: x-faces (x-vertex i, cells i and i+1), 1: y, 2: z.
+// Output per face: flux(9) * area in the global frame, speeds SL, SR.
+template <int DIR, typename R, typename P>
+__global__ void k_face_flux(CsDims d, Rng rg, Geo geo, State st, const P* __restrict__ V,
+                            double* __restrict__ F, double* __restrict__ S, Params pr) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  const int64_t stride = DIR == 0 ? 1 : (DIR == 1 ? d.nc : (int64_t)d.nc * d.nc);
+  // left cell of the face
+  int64_t cl = DIR == 0 ? d.cell(i, j, k) : (DIR == 1 ? d.cell(i, j, k) : d.cell(i, j, k));
+  const P* Vs = V + (int64_t)s * 9 * d.cpp;
+  R VL[9], VR[9];
+  for (int q = 0; q < 9; ++q) {
+    const P* p = Vs + q * d.cpp + cl;
+    R m2 = (R)p[-2 * stride], m1 = (R)p[-stride], c0 = (R)p[0], p1 = (R)p[stride],
+      p2 = (R)p[2 * stride], p3 = (R)p[3 * stride];
+    if (pr.recon == CS_RECON_WENO5) {
+      VL[q] = weno5<R>(m2, m1, c0, p1, p2);
+      VR[q] = weno5<R>(p3, p2, p1, c0, m1);
+    } else {
+      VL[q] = c0;
+      VR[q] = p1;
+    }
+  }
+  if (!(VL[0] > R(0)) || !(VL[4] > R(0)) || !(VL[8] > R(0)))
+    for (int q = 0; q < 9; ++q) VL[q] = (R)Vs[q * d.cpp + cl];
+  if (!(VR[0] > R(0)) || !(VR[4] > R(0)) || !(VR[8] > R(0)))
+    for (int q = 0; q < 9; ++q) VR[q] = (R)Vs[q * d.cpp + cl + stride];
+  // face frame (A normal, B, C) and area
+  double A[3], Bv[3], Cv[3], area, bface;
+  int64_t f = s * d.fpp + d.face(i, j, k);
+  if (DIR == 0) {
+    for (int c = 0; c < 3; ++c) { A[c] = geo.t(d, T_XNX + c, s, i, j); Cv[c] = geo.t(d, T_XCX + c, s, i, j); }
+    Bv[0] = Cv[1] * A[2] - Cv[2] * A[1];
+    Bv[1] = Cv[2] * A[0] - Cv[0] * A[2];
+    Bv[2] = Cv[0] * A[1] - Cv[1] * A[0];
+    double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
+    area = 0.5 * (r1 * r1 - r0 * r0) * geo.t(d, T_THY, s, i, j);
+    bface = st.Bx[f];
+  } else if (DIR == 1) {
+    for (int c = 0; c < 3; ++c) { A[c] = geo.t(d, T_YNX + c, s, i, j); Cv[c] = geo.t(d, T_YCX + c, s, i, j); }
+    Bv[0] = Cv[1] * A[2] - Cv[2] * A[1];
+    Bv[1] = Cv[2] * A[0] - Cv[0] * A[2];
+    Bv[2] = Cv[0] * A[1] - Cv[1] * A[0];
+    double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
+    area = 0.5 * (r1 * r1 - r0 * r0) * geo.t(d, T_THX, s, i, j);
+    bface = st.By[f];
+  } else {
+    double zn = geo.t(d, T_ZSN, s, i, j);
+    for (int c = 0; c < 3; ++c) { A[c] = geo.t(d, T_ZSX + c, s, i, j) / zn; Bv[c] = geo.t(d, T_ZTX + c, s, i, j); }
+    Cv[0] = A[1] * Bv[2] - A[2] * Bv[1];
+    Cv[1] = A[2] * Bv[0] - A[0] * Bv[2];
+    Cv[2] = A[0] * Bv[1] - A[1] * Bv[0];
+    double r1 = geo.r(d, k);
+    area = r1 * r1 * zn;
+    bface = st.Bz[f];
+  }
+  R a0 = (R)A[0], a1 = (R)A[1], a2 = (R)A[2], b0 = (R)Bv[0], b1 = (R)Bv[1], b2 = (R)Bv[2],
+    c0 = (R)Cv[0], c1 = (R)Cv[1], c2 = (R)Cv[2];
+  R wl[8], wr[8];
+  wl[0] = VL[0]; wr[0] = VR[0];
+  wl[4] = VL[4]; wr[4] = VR[4];
+  wl[1] = a0 * VL[1] + a1 * VL[2] + a2 * VL[3];
+  wl[2] = b0 * VL[1] + b1 * VL[2] + b2 * VL[3];
+  wl[3] = c0 * VL[1] + c1 * VL[2] + c2 * VL[3];
+  wr[1] = a0 * VR[1] + a1 * VR[2] + a2 * VR[3];
+  wr[2] = b0 * VR[1] + b1 * VR[2] + b2 * VR[3];
+  wr[3] = c0 * VR[1] + c1 * VR[2] + c2 * VR[3];
+  wl[5] = (R)bface; wr[5] = (R)bface;
+  wl[6] = b0 * VL[5] + b1 * VL[6] + b2 * VL[7];
+  wl[7] = c0 * VL[5] + c1 * VL[6] + c2 * VL[7];
+  wr[6] = b0 * VR[5] + b1 * VR[6] + b2 * VR[7];
+  wr[7] = c0 * VR[5] + c1 * VR[6] + c2 * VR[7];
+  Gas<R> gs{(R)pr.gamma, (R)pr.gm1, (R)pr.rhofloor, (R)pr.prsfloor};
+  R Fs[8], entF, SL, SR;
+  if (pr.riemann == CS_RS_HLLD)
+    hlld<R>(gs, wl, wr, VL[8], VR[8], Fs, entF, SL, SR);
+  else
+    hll<R>(gs, wl, wr, VL[8], VR[8], Fs, entF, SL, SR);
+  // rotate back (450make_facial_fluxes.f:995-1030) and scale by the area
+  double out[9];
+  out[0] = (double)Fs[0];
+  out[4] = (double)Fs[4];
+  out[1] = A[0] * (double)Fs[1] + Bv[0] * (double)Fs[2] + Cv[0] * (double)Fs[3];
+  out[2] = A[1] * (double)Fs[1] + Bv[1] * (double)Fs[2] + Cv[1] * (double)Fs[3];
+  out[3] = A[2] * (double)Fs[1] + Bv[2] * (double)Fs[2] + Cv[2] * (double)Fs[3];
+  out[5] = A[0] * (double)Fs[5] + Bv[0] * (double)Fs[6] + Cv[0] * (double)Fs[7];
+  out[6] = A[1] * (double)Fs[5] + Bv[1] * (double)Fs[6] + Cv[1] * (double)Fs[7];
+  out[7] = A[2] * (double)Fs[5] + Bv[2] * (double)Fs[6] + Cv[2] * (double)Fs[7];
+  out[8] = (double)entF;
+  int64_t fo = (int64_t)s * 9 * d.fpp + d.face(i, j, k);
+  for (int q = 0; q < 9; ++q) F[fo + q * d.fpp] = out[q] * area;
+  int64_t so = (int64_t)s * 2 * d.fpp + d.face(i, j, k);
+  S[so] = (double)SL;
+  S[so + d.fpp] = (double)SR;
+}
+
+// ------------------------------------------------------------------ edge EMFs
+template <typename P>
+__device__ __forceinline__ void cell_E(const CsDims& d, const P* V, int s, int64_t c, double* E) {
+  const P* p = V + (int64_t)s * 9 * d.cpp + c;
+  double vx = p[1 * d.cpp], vy = p[2 * d.cpp], vz = p[3 * d.cpp];
+  double bx = p[5 * d.cpp], by = p[6 * d.cpp], bz = p[7 * d.cpp];
+  E[0] = -(vy * bz - vz * by);
+  E[1] = -(vz * bx - vx * bz);
+  E[2] = -(vx * by - vy * bx);
+}
+__device__ __forceinline__ double sp(const CsDims& d, const double* S, int s, int which, int i, int j, int k) {
+  return S[(int64_t)s * 2 * d.fpp + which * d.fpp + d.face(i, j, k)];
+}
+
+// z-edges (MAKE_EDGE_EFIELDS_P1 STEP 3, 460make_edge_fields_p1.f:2591-3721,
+// and the tri-corners of STEP 4, :3725-7035).  Stored as E * dr.
+template <typename P>
+__global__ void k_emf_z(CsDims d, Rng rg, Geo geo, State st, const P* V, const double* Sx,
+                        const double* Sy, double* Ez, Params pr) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  const int N = d.N;
+  double n3[3] = {geo.t(d, T_UX, s, i, j), geo.t(d, T_UY, s, i, j), geo.t(d, T_UZ, s, i, j)};
+  double dr = geo.r(d, k) - geo.r(d, k - 1);
+  const double* Bx = st.Bx + s * d.fpp;
+  const double* By = st.By + s * d.fpp;
+  bool ci = (i == 0 || i == N), cj = (j == 0 || j == N);
+  double ESSz;
+  if (ci && cj) {
+    // tri-corner: three active zones, 3-state LLF solver
+    int c1i, c1j, c2i, c2j, c3i, c3j;
+    double bn;
+    double S = 0.0;
+    auto sabs = [&](const double* SS, int ii, int jj) {
+      return fmax(fabs(sp(d, SS, s, 0, ii, jj, k)), fabs(sp(d, SS, s, 1, ii, jj, k)));
+    };
+    if (i == 0 && j == 0) {
+      c1i = 1; c1j = 1; c2i = 0; c2j = 1; c3i = 1; c3j = 0;
+      bn = -By[d.face(1, 0, k)] + Bx[d.face(0, 1, k)] + By[d.face(0, 0, k)];
+      S = fmax(sabs(Sx, 0, 1), sabs(Sy, 1, 0));
+    } else if (i == N && j == 0) {
+      c1i = N; c1j = 0; c2i = N; c2j = 1; c3i = N + 1; c3j = 1;
+      bn = Bx[d.face(N, 1, k)] + By[d.face(N, 0, k)] - By[d.face(N + 1, 0, k)];
+      S = fmax(sabs(Sx, N, 1), sabs(Sy, N, 0));
+    } else if (i == N && j == N) {
+      c1i = N; c1j = N; c2i = N + 1; c2j = N; c3i = N; c3j = N + 1;
+      bn = By[d.face(N, N, k)] - Bx[d.face(N, N, k)] - By[d.face(N + 1, N, k)];
+      S = fmax(sabs(Sx, N, N), sabs(Sy, N, N));
+    } else {
+      c1i = 0; c1j = N; c2i = 1; c2j = N; c3i = 1; c3j = N + 1;
+      bn = -By[d.face(1, N, k)] - Bx[d.face(0, N, k)] + By[d.face(0, N, k)];
+      S = fmax(sabs(Sx, 0, N), sabs(Sy, 1, N));
+    }
+    double e1[3], e2[3], e3[3];
+    cell_E(d, V, s, d.cell(c1i, c1j, k), e1);
+    cell_E(d, V, s, d.cell(c2i, c2j, k), e2);
+    cell_E(d, V, s, d.cell(c3i, c3j, k), e3);
+    double ob3 = 1.0 / 3.0;
+    double ex = (e1[0] + e2[0] + e3[0]) * ob3, ey = (e1[1] + e2[1] + e3[1]) * ob3,
+           ez = (e1[2] + e2[2] + e3[2]) * ob3;
+    double e = ex * n3[0] + ey * n3[1] + ez * n3[2];
+    ESSz = emf3_llf<double>(S, e, e, e, bn, 0.0, 0.0);
+  } else {
+    double eLD[3], eRD[3], eRU[3], eLU[3];
+    cell_E(d, V, s, d.cell(i, j, k), eLD);
+    cell_E(d, V, s, d.cell(i + 1, j, k), eRD);
+    cell_E(d, V, s, d.cell(i + 1, j + 1, k), eRU);
+    cell_E(d, V, s, d.cell(i, j + 1, k), eLU);
+    double zLD, zRD, zRU, zLU;
+    if (pr.emf == CS_EMF_LLF) {
+      double ex = (eLD[0] + eRD[0] + eRU[0] + eLU[0]) * 0.25;
+      double ey = (eLD[1] + eRD[1] + eRU[1] + eLU[1]) * 0.25;
+      double ez = (eLD[2] + eRD[2] + eRU[2] + eLU[2]) * 0.25;
+      zLD = zRD = zRU = zLU = ex * n3[0] + ey * n3[1] + ez * n3[2];
+    } else {
+      zLD = eLD[0] * n3[0] + eLD[1] * n3[1] + eLD[2] * n3[2];
+      zRD = eRD[0] * n3[0] + eRD[1] * n3[1] + eRD[2] * n3[2];
+      zRU = eRU[0] * n3[0] + eRU[1] * n3[1] + eRU[2] * n3[2];
+      zLU = eLU[0] * n3[0] + eLU[1] * n3[1] + eLU[2] * n3[2];
+    }
+    double By_R = By[d.face(i + 1, j, k)], By_L = By[d.face(i, j, k)];
+    double Bn_U = Bx[d.face(i, j + 1, k)], Bn_D = Bx[d.face(i, j, k)];
+    double S_R = fmax(sp(d, Sx, s, 1, i, j + 1, k), sp(d, Sx, s, 1, i, j, k));
+    double S_L = fmin(sp(d, Sx, s, 0, i, j + 1, k), sp(d, Sx, s, 0, i, j, k));
+    double S_U = fmax(sp(d, Sy, s, 1, i + 1, j, k), sp(d, Sy, s, 1, i, j, k));
+    double S_D = fmin(sp(d, Sy, s, 0, i + 1, j, k), sp(d, Sy, s, 0, i, j, k));
+    if (pr.emf == CS_EMF_LLF) {
+      double t = fmax(fmax(fabs(S_R), fabs(S_L)), fmax(fabs(S_U), fabs(S_D)));
+      S_R = t; S_L = -t; S_U = t; S_D = -t;
+    }
+    double ct = geo.t(d, T_ECOS, s, i, j), st_ = geo.t(d, T_ESIN, s, i, j);
+    ESSz = emf4_hll<double>(S_R, S_L, S_U, S_D, ct, st_, zRU, zLU, zLD, zRD, Bn_U, Bn_D, By_R, By_L);
+  }
+  Ez[s * d.fpp + d.face(i, j, k)] = ESSz * dr;
+}
+
+// x-edges (STEP 1, 460make_edge_fields_p1.f:393-1493) and y-edges (STEP 2,
+// :1497-2587): edges normal to the radial direction.  Stored as E * arc length.
+template <int DIR, typename P>
+__global__ void k_emf_xy(CsDims d, Rng rg, Geo geo, State st, const P* V, const double* Sa,
+                         const double* Sb, double* Eo, Params pr) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  int64_t fo = s * d.fpp + d.face(i, j, k);
+  if (pr.wall && (k == 0 || k == d.NR)) { Eo[fo] = 0.0; return; }
+  double n3[3];
+  int64_t cLD, cRD, cRU, cLU;
+  double By_R, By_L, Bn_U, Bn_D, S_R, S_L, S_U, S_D, len;
+  const double* Bx = st.Bx + s * d.fpp;
+  const double* By = st.By + s * d.fpp;
+  const double* Bz = st.Bz + s * d.fpp;
+  if (DIR == 0) {  // Efx: cell-x i, y-vertex j, z-vertex k; Sa = y-face speeds, Sb = z-face speeds
+    for (int c = 0; c < 3; ++c) n3[c] = geo.t(d, T_XETX + c, s, i, j);
+    cLD = d.cell(i, j, k); cRD = d.cell(i, j + 1, k); cRU = d.cell(i, j + 1, k + 1); cLU = d.cell(i, j, k + 1);
+    By_R = Bz[d.face(i, j + 1, k)]; By_L = Bz[d.face(i, j, k)];
+    Bn_U = By[d.face(i, j, k + 1)]; Bn_D = By[d.face(i, j, k)];
+    S_R = fmax(sp(d, Sa, s, 1, i, j, k + 1), sp(d, Sa, s, 1, i, j, k));
+    S_L = fmin(sp(d, Sa, s, 0, i, j, k + 1), sp(d, Sa, s, 0, i, j, k));
+    S_U = fmax(sp(d, Sb, s, 1, i, j + 1, k), sp(d, Sb, s, 1, i, j, k));
+    S_D = fmin(sp(d, Sb, s, 0, i, j + 1, k), sp(d, Sb, s, 0, i, j, k));
+    len = geo.r(d, k) * geo.t(d, T_THX, s, i, j);
+  } else {  // Efy: x-vertex i, cell-y j, z-vertex k; Sa = z-face speeds, Sb = x-face speeds
+    for (int c = 0; c < 3; ++c) n3[c] = geo.t(d, T_YETX + c, s, i, j);
+    cLD = d.cell(i, j, k); cRD = d.cell(i, j, k + 1); cLU = d.cell(i + 1, j, k); cRU = d.cell(i + 1, j, k + 1);
+    By_R = Bx[d.face(i, j, k + 1)]; By_L = Bx[d.face(i, j, k)];
+    Bn_U = Bz[d.face(i + 1, j, k)]; Bn_D = Bz[d.face(i, j, k)];
+    S_R = fmax(sp(d, Sa, s, 1, i + 1, j, k), sp(d, Sa, s, 1, i, j, k));
+    S_L = fmin(sp(d, Sa, s, 0, i + 1, j, k), sp(d, Sa, s, 0, i, j, k));
+    S_U = fmax(sp(d, Sb, s, 1, i, j, k), sp(d, Sb, s, 1, i, j, k + 1));
+    S_D = fmin(sp(d, Sb, s, 0, i, j, k), sp(d, Sb, s, 0, i, j, k + 1));
+    len = geo.r(d, k) * geo.t(d, T_THY, s, i, j);
+  }
+  double eLD[3], eRD[3], eRU[3], eLU[3];
+  cell_E(d, V, s, cLD, eLD);
+  cell_E(d, V, s, cRD, eRD);
+  cell_E(d, V, s, cRU, eRU);
+  cell_E(d, V, s, cLU, eLU);
+  double zLD, zRD, zRU, zLU;
+  if (pr.emf == CS_EMF_LLF) {
+    double ex = (eLD[0] + eRD[0] + eRU[0] + eLU[0]) * 0.25;
+    double ey = (eLD[1] + eRD[1] + eRU[1] + eLU[1]) * 0.25;
+    double ez = (eLD[2] + eRD[2] + eRU[2] + eLU[2]) * 0.25;
+    zLD = zRD = zRU = zLU = ex * n3[0] + ey * n3[1] + ez * n3[2];
+    double t = fmax(fmax(fabs(S_R), fabs(S_L)), fmax(fabs(S_U), fabs(S_D)));
+    S_R = t; S_L = -t; S_U = t; S_D = -t;
+  } else {
+    zLD = eLD[0] * n3[0] + eLD[1] * n3[1] + eLD[2] * n3[2];
+    zRD = eRD[0] * n3[0] + eRD[1] * n3[1] + eRD[2] * n3[2];
+    zRU = eRU[0] * n3[0] + eRU[1] * n3[1] + eRU[2] * n3[2];
+    zLU = eLU[0] * n3[0] + eLU[1] * n3[1] + eLU[2] * n3[2];
+  }
+  double e = emf4_hll<double>(S_R, S_L, S_U, S_D, 0.0, 1.0, zRU, zLU, zLD, zRD, Bn_U, Bn_D, By_R, By_L);
+  Eo[fo] = e * len;
+}
+
+// ------------------------------------------------------------------ update
+// U_out = a U0 + b (U_in + dt L(U_in)) for rho, rho v, E and entropy
+// (UPDATE_IMEX_RK_STEP_P1 484update_imex_rk_step_p1.f:483-497, 547-593 with the
+// Shu-Osher stage weights; the RK2 weights reproduce DRV:2661-2773).
+__global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
+                               const double* __restrict__ Fx, const double* __restrict__ Fy,
+                               const double* __restrict__ Fz, const double* dtp, double wa,
+                               double wb, Params pr) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  const double dt = dtp[0];
+  int64_t c = d.cell(i, j, k);
+  double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
+  double vol = geo.t(d, T_OM, s, i, j) * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0;
+  double ov = 1.0 / vol;
+  int64_t fxp = d.face(i, j, k), fxm = d.face(i - 1, j, k), fym = d.face(i, j - 1, k),
+          fzm = d.face(i, j, k - 1);
+  const int comps[6] = {0, 1, 2, 3, 4, 8};
+  double L[6];
+  for (int q = 0; q < 6; ++q) {
+    int64_t o = ((int64_t)s * 9 + comps[q]) * d.fpp;
+    L[q] = -ov * ((Fx[o + fxp] - Fx[o + fxm]) + (Fy[o + fxp] - Fy[o + fym]) + (Fz[o + fxp] - Fz[o + fzm]));
+  }
+  if (pr.grav != 0.0) {
+    // explicit point-mass gravity g = -grav / r^2 rhat (470build_non_stiff_sources.f:150-250)
+    double rc = 0.5 * (r0 + r1);
+    double zn = geo.t(d, T_ZSN, s, i, j);
+    double nx = geo.t(d, T_ZSX, s, i, j) / zn, ny = geo.t(d, T_ZSY, s, i, j) / zn, nz = geo.t(d, T_ZSZ, s, i, j) / zn;
+    double gr = -pr.grav / (rc * rc);
+    double rho = in.U[(s * 8 + 0) * d.cpp + c];
+    double mr = in.U[(s * 8 + 1) * d.cpp + c] * nx + in.U[(s * 8 + 2) * d.cpp + c] * ny + in.U[(s * 8 + 3) * d.cpp + c] * nz;
+    L[1] += rho * gr * nx;
+    L[2] += rho * gr * ny;
+    L[3] += rho * gr * nz;
+    L[4] += mr * gr;
+  }
+  for (int q = 0; q < 5; ++q) {
+    int64_t o = (s * 8 + q) * d.cpp + c;
+    double un = in.U[o] + dt * L[q];
+    out.U[o] = wa != 0.0 ? wa * u0.U[o] + wb * un : wb * un;
+  }
+  {
+    int64_t o = s * d.cpp + c;
+    double en = in.ent[o] + dt * L[5];
+    out.ent[o] = wa != 0.0 ? wa * u0.ent[o] + wb * en : wb * en;
+  }
+}
+
+// Constrained transport (484update_imex_rk_step_p1.f:646-908) with every edge
+// circulation element E*l computed once per edge, so the discrete div B of
+// each zone telescopes.
+__global__ void k_update_faces(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
+                               const double* __restrict__ Ex, const double* __restrict__ Ey,
+                               const double* __restrict__ Ez, const double* dtp, double wa,
+                               double wb) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  const double dt = dtp[0];
+  const int N = d.N, NR = d.NR;
+  const int64_t so = (int64_t)s * d.fpp;
+  int64_t f = so + d.face(i, j, k);
+  double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
+  // x-face: x-vertex i in [0,N], cell j in [1,N], cell k in [1,NR]
+  if (j >= 1 && k >= 1) {
+    double circ = Ez[f] - Ez[so + d.face(i, j - 1, k)] - Ey[f] + Ey[so + d.face(i, j, k - 1)];
+    double A = 0.5 * (r1 * r1 - r0 * r0) * geo.t(d, T_THY, s, i, j);
+    double bn = in.Bx[f] - dt * circ / A;
+    out.Bx[f] = wa != 0.0 ? wa * u0.Bx[f] + wb * bn : wb * bn;
+  }
+  // y-face: cell i in [1,N], y-vertex j in [0,N], cell k in [1,NR]
+  if (i >= 1 && k >= 1) {
+    double circ = -Ez[f] + Ez[so + d.face(i - 1, j, k)] + Ex[f] - Ex[so + d.face(i, j, k - 1)];
+    double A = 0.5 * (r1 * r1 - r0 * r0) * geo.t(d, T_THX, s, i, j);
+    double bn = in.By[f] - dt * circ / A;
+    out.By[f] = wa != 0.0 ? wa * u0.By[f] + wb * bn : wb * bn;
+  }
+  // z-face: cells i,j in [1,N], z-vertex k in [0,NR]
+  if (i >= 1 && j >= 1) {
+    double circ = Ey[f] - Ey[so + d.face(i - 1, j, k)] - Ex[f] + Ex[so + d.face(i, j - 1, k)];
+    double A = r1 * r1 * geo.t(d, T_ZSN, s, i, j);
+    double bn = in.Bz[f] - dt * circ / A;
+    out.Bz[f] = wa != 0.0 ? wa * u0.Bz[f] + wb * bn : wb * bn;
+  }
+  (void)N; (void)NR;
+}
+
+// Zone B from the faces and the cons->prim floors (CLEANUP_AFTER_UPDATE,
+// 485cleanup_after_update.f:4; CONS_TO_PRIM 662:69-151).
+__global__ void k_cleanup(CsDims d, Rng rg, Geo geo, State st, Params pr, int* flags) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  int64_t c = d.cell(i, j, k);
+  const int64_t so = (int64_t)s * d.fpp;
+  double b0 = 0.5 * (st.Bx[so + d.face(i - 1, j, k)] + st.Bx[so + d.face(i, j, k)]);
+  double b1 = 0.5 * (st.By[so + d.face(i, j - 1, k)] + st.By[so + d.face(i, j, k)]);
+  double b2 = 0.5 * (st.Bz[so + d.face(i, j, k - 1)] + st.Bz[so + d.face(i, j, k)]);
+  double u[8], v[8];
+  for (int q = 0; q < 8; ++q) u[q] = st.U[(s * 8 + q) * d.cpp + c];
+  for (int r = 0; r < 3; ++r)
+    u[5 + r] = geo.t(d, T_M00 + 3 * r, s, i, j) * b0 + geo.t(d, T_M00 + 3 * r + 1, s, i, j) * b1 +
+               geo.t(d, T_M00 + 3 * r + 2, s, i, j) * b2;
+  double ent = st.ent[s * d.cpp + c];
+  Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor};
+  int fl = cons_to_prim(gs, u, v, ent);
+  for (int q = 0; q < 8; ++q) st.U[(s * 8 + q) * d.cpp + c] = u[q];
+  st.ent[s * d.cpp + c] = ent;
+  if (fl && flags) flags[0] = 1;
+  if (!(u[0] == u[0]) || !(u[4] == u[4]) || isinf(u[4])) { if (flags) flags[1] = 1; }
+}
+
+// ------------------------------------------------------------------ reductions
+// CFL (490evaluate_timestep.f:118-278): per zone cfl * L / (|v| + c_ms),
+// reduced with a wavefront shuffle tree then one partial per block.
+__device__ __forceinline__ double wave_min(double v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+  return v;
+}
+__global__ void k_cfl_partial(CsDims d, Rng rg, Geo geo, State st, Params pr, double* part) {
+  __shared__ double sm[16];
+  int i, j, k, s;
+  double v = 1e300;
+  if (decode(rg, gtid(), i, j, k, s)) {
+    int64_t c = d.cell(i, j, k);
+    double u[8], w[8];
+    for (int q = 0; q < 8; ++q) u[q] = st.U[(s * 8 + q) * d.cpp + c];
+    double ent = st.ent[s * d.cpp + c];
+    Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor};
+    cons_to_prim(gs, u, w, ent);
+    double b2 = w[5] * w[5] + w[6] * w[6] + w[7] * w[7];
+    double ms = sqrt((pr.gamma * w[4] + C<double>::ob4pi * b2) / w[0]);
+    double a21 = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) + ms;
+    double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
+    double L = fmin(r1 - r0, r0 * geo.t(d, T_CHMIN, s, i, j));
+    v = pr.cfl * L / a21;
+  }
+  v = wave_min(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sm[w] = v;
+  __syncthreads();
+  if (threadIdx.x < 64) {
+    double x = threadIdx.x < (blockDim.x >> 6) ? sm[threadIdx.x] : 1e300;
+    x = wave_min(x);
+    if (threadIdx.x == 0) part[blockIdx.x] = x;
+  }
+}
+__global__ void k_cfl_final(const double* part, int n, double* dtp, double growth) {
+  __shared__ double sm[16];
+  double v = 1e300;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) v = fmin(v, part[t]);
+  v = wave_min(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sm[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 1e300;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) x = fmin(x, sm[q]);
+    double cur = dtp[0];
+    double nxt = cur > 0.0 ? fmin(growth * cur, x) : x;
+    dtp[0] = nxt;
+  }
+}
+
+// Compensated volume integrals: Neumaier two-sum per thread, two-sum tree.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  double bp = s - a;
+  e = (a - (s - bp)) + (b - bp);
+}
+__global__ void k_sums_partial(CsDims d, Rng rg, Geo geo, State st, double* part) {
+  // part[(blk * 9 + q) * 2 + {0,1}]
+  __shared__ double ss[256][2];
+  for (int q = 0; q < 9; ++q) {
+    int i, j, k, s;
+    double v = 0.0;
+    if (decode(rg, gtid(), i, j, k, s)) {
+      int64_t c = d.cell(i, j, k);
+      double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
+      double vol = geo.t(d, T_OM, s, i, j) * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0;
+      v = vol * (q < 8 ? st.U[(s * 8 + q) * d.cpp + c] : st.ent[s * d.cpp + c]);
+    }
+    ss[threadIdx.x][0] = v;
+    ss[threadIdx.x][1] = 0.0;
+    __syncthreads();
+    for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) {
+        double sa, ea;
+        two_sum(ss[threadIdx.x][0], ss[threadIdx.x + o][0], sa, ea);
+        ss[threadIdx.x][0] = sa;
+        ss[threadIdx.x][1] = ss[threadIdx.x][1] + ss[threadIdx.x + o][1] + ea;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      part[(blockIdx.x * 9 + q) * 2 + 0] = ss[0][0];
+      part[(blockIdx.x * 9 + q) * 2 + 1] = ss[0][1];
+    }
+    __syncthreads();
+  }
+}
+__global__ void k_sums_final(const double* part, int n, double* out) {
+  int q = threadIdx.x;
+  if (q >= 9) return;
+  double s = 0.0, e = 0.0;
+  for (int b = 0; b < n; ++b) {
+    double t, te;
+    two_sum(s, part[(b * 9 + q) * 2], t, te);
+    s = t;
+    e += te + part[(b * 9 + q) * 2 + 1];
+  }
+  out[q] = s + e;
+}
+
+// relative discrete divergence |sum A_f B_f| / sum |A_f B_f| per zone, max
+__global__ void k_divb_partial(CsDims d, Rng rg, Geo geo, State st, double* part) {
+  __shared__ double sm[16];
+  int i, j, k, s;
+  double v = 0.0;
+  if (decode(rg, gtid(), i, j, k, s)) {
+    const int64_t so = (int64_t)s * d.fpp;
+    double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
+    double hr = 0.5 * (r1 * r1 - r0 * r0);
+    double ax1 = hr * geo.t(d, T_THY, s, i, j), ax0 = hr * geo.t(d, T_THY, s, i - 1, j);
+    double ay1 = hr * geo.t(d, T_THX, s, i, j), ay0 = hr * geo.t(d, T_THX, s, i, j - 1);
+    double zn = geo.t(d, T_ZSN, s, i, j);
+    double az1 = r1 * r1 * zn, az0 = r0 * r0 * zn;
+    double p[6] = {ax1 * st.Bx[so + d.face(i, j, k)], -ax0 * st.Bx[so + d.face(i - 1, j, k)],
+                   ay1 * st.By[so + d.face(i, j, k)], -ay0 * st.By[so + d.face(i, j - 1, k)],
+                   az1 * st.Bz[so + d.face(i, j, k)], -az0 * st.Bz[so + d.face(i, j, k - 1)]};
+    double sum = 0.0, sa = 0.0;
+    for (int q = 0; q < 6; ++q) { sum += p[q]; sa += fabs(p[q]); }
+    v = sa > 0.0 ? fabs(sum) / sa : 0.0;
+  }
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sm[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) x = fmax(x, sm[q]);
+    part[blockIdx.x] = x;
+  }
+}
+
+}  // namespace cs
+
+// ====================================================================== host
+using namespace cs;
+
+struct cs_handle {
+  cs_config cfg;
+  CsDims d;
+  Params pr;
+  hipStream_t stream = nullptr;
+  double* T = nullptr;
+  double* rb = nullptr;
+  double* buf[2][5] = {};  // [stage buffer][U, ent, Bx, By, Bz]
+  void* V = nullptr;
+  double* F[3] = {};
+  double* S[3] = {};
+  double* E[3] = {};
+  double* dt = nullptr;
+  double* part = nullptr;
+  int* flags = nullptr;
+  int64_t npart = 0;
+  hipGraphExec_t graph = nullptr;
+  int cur = 0;  // which buffer holds U^n
+};
+
+namespace {
+
+const int kBlock = 256;
+inline unsigned nblk(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+State state_of(cs_handle* h, int b) {
+  return State{h->buf[b][0], h->buf[b][1], h->buf[b][2], h->buf[b][3], h->buf[b][4]};
+}
+Geo geo_of(cs_handle* h) { return Geo{h->T, h->rb, h->d.t2 * 6, h->d.t2}; }
+
+int launch_halo(cs_handle* h, int b) {
+  const CsDims& d = h->d;
+  State st = state_of(h, b);
+  uint64_t n1 = 12ull * d.N * d.NR;
+  k_avg_coinciding<<<nblk(n1), kBlock, 0, h->stream>>>(d, st);
+  uint64_t n2 = 24ull * d.g * d.N * d.NR;
+  k_exchange_cells<<<nblk(n2), kBlock, 0, h->stream>>>(d, st);
+  uint64_t n3 = 6ull * 4 * d.g * 3 * (d.N + 1) * (d.NR + 1);
+  k_exchange_faces<<<nblk(n3), kBlock, 0, h->stream>>>(d, st);
+  uint64_t n4 = 24ull * d.g * d.g * d.NR;
+  k_corner_fill<<<nblk(n4), kBlock, 0, h->stream>>>(d, st);
+  uint64_t nl = (uint64_t)(d.N + 2 * d.g + 1);
+  uint64_t n5 = 12ull * d.g * nl * nl;
+  k_radial_bc<<<nblk(n5), kBlock, 0, h->stream>>>(d, geo_of(h), st, h->pr.wall);
+  CS_HIP(hipGetLastError());
+  return CS_OK;
+}
+
+void stage_weights(int order, int irk, double& wa, double& wb, int& in, int& out) {
+  // buffers: 0 = U^n (relative to h->cur), 1 = scratch
+  if (order == 3) {
+    if (irk == 1) { wa = 0.0; wb = 1.0; in = 0; out = 1; }
+    else if (irk == 2) { wa = 0.75; wb = 0.25; in = 1; out = 1; }
+    else { wa = 1.0 / 3.0; wb = 2.0 / 3.0; in = 1; out = 0; }
+  } else {
+    if (irk == 1) { wa = 0.0; wb = 1.0; in = 0; out = 1; }
+    else { wa = 0.5; wb = 0.5; in = 1; out = 0; }
+  }
+}
+
+int stage_input(cs_handle* h, int irk) {
+  double wa, wb;
+  int bi, bo;
+  stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo);
+  return (h->cur + bi) & 1;
+}
+
+// index ranges each kernel covers (faces and edges needed by the update and
+// by the EMF speed look-ups, one layer into the ghost zones)
+struct Ranges {
+  Rng all, fx, fy, fz, ez, ex, ey, cells, faces;
+};
+Ranges ranges_of(const CsDims& d) {
+  const int N = d.N, NR = d.NR, g = d.g;
+  Ranges r;
+  r.all = make_rng(1 - g, N + g, 1 - g, N + g, 1 - g, NR + g);
+  r.fx = make_rng(0, N, 0, N + 1, 0, NR + 1);
+  r.fy = make_rng(0, N + 1, 0, N, 0, NR + 1);
+  r.fz = make_rng(0, N + 1, 0, N + 1, 0, NR);
+  r.ez = make_rng(0, N, 0, N, 1, NR);
+  r.ex = make_rng(1, N, 0, N, 0, NR);
+  r.ey = make_rng(0, N, 1, N, 0, NR);
+  r.cells = make_rng(1, N, 1, N, 1, NR);
+  r.faces = make_rng(0, N, 0, N, 0, NR);
+  return r;
+}
+
+// Launch one kernel of the stage on buffer b (update kernels use irk).
+template <typename R, typename P>
+int launch_kernel_t(cs_handle* h, int kid, int b, int irk) {
+  const CsDims& d = h->d;
+  const Ranges r = ranges_of(d);
+  State st = state_of(h, b);
+  Geo geo = geo_of(h);
+  P* V = (P*)h->V;
+  hipStream_t sm = h->stream;
+  double wa = 0, wb = 1;
+  int bi = 0, bo = 1;
+  stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo);
+  State in = state_of(h, (h->cur + bi) & 1), u0 = state_of(h, h->cur), out = state_of(h, (h->cur + bo) & 1);
+  switch (kid) {
+    case CS_K_HALO: return launch_halo(h, b);
+    case CS_K_PRIMS: k_prims<P><<<nblk(r.all.total), kBlock, 0, sm>>>(d, r.all, st, V, h->pr); break;
+    case CS_K_FLUX_X:
+      k_face_flux<0, R, P><<<nblk(r.fx.total), kBlock, 0, sm>>>(d, r.fx, geo, st, V, h->F[0], h->S[0], h->pr);
+      break;
+    case CS_K_FLUX_Y:
+      k_face_flux<1, R, P><<<nblk(r.fy.total), kBlock, 0, sm>>>(d, r.fy, geo, st, V, h->F[1], h->S[1], h->pr);
+      break;
+    case CS_K_FLUX_Z:
+      k_face_flux<2, R, P><<<nblk(r.fz.total), kBlock, 0, sm>>>(d, r.fz, geo, st, V, h->F[2], h->S[2], h->pr);
+      break;
+    case CS_K_EMF_Z:
+      k_emf_z<P><<<nblk(r.ez.total), kBlock, 0, sm>>>(d, r.ez, geo, st, V, h->S[0], h->S[1], h->E[2], h->pr);
+      break;
+    case CS_K_EMF_X:
+      k_emf_xy<0, P><<<nblk(r.ex.total), kBlock, 0, sm>>>(d, r.ex, geo, st, V, h->S[1], h->S[2], h->E[0], h->pr);
+      break;
+    case CS_K_EMF_Y:
+      k_emf_xy<1, P><<<nblk(r.ey.total), kBlock, 0, sm>>>(d, r.ey, geo, st, V, h->S[2], h->S[0], h->E[1], h->pr);
+      break;
+    case CS_K_UPDATE_CELLS:
+      k_update_cells<<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, h->F[0], h->F[1],
+                                                               h->F[2], h->dt, wa, wb, h->pr);
+      break;
+    case CS_K_UPDATE_FACES:
+      k_update_faces<<<nblk(r.faces.total), kBlock, 0, sm>>>(d, r.faces, geo, in, u0, out, h->E[0], h->E[1],
+                                                               h->E[2], h->dt, wa, wb);
+      break;
+    case CS_K_CLEANUP:
+      k_cleanup<<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, out, h->pr, h->flags);
+      break;
+    case CS_K_CFL: {
+      unsigned nb = nblk(r.cells.total);
+      k_cfl_partial<<<nb, kBlock, 0, sm>>>(d, r.cells, geo, state_of(h, h->cur), h->pr, h->part);
+      k_cfl_final<<<1, 1024, 0, sm>>>(h->part, (int)nb, h->dt, 1.1);
+      break;
+    }
+    default:
+      set_error("unknown kernel id %d", kid);
+      return CS_E_INVALID;
+  }
+  CS_HIP(hipGetLastError());
+  return CS_OK;
+}
+int launch_kernel(cs_handle* h, int kid, int b, int irk) {
+  return h->cfg.precision == CS_MIXED ? launch_kernel_t<float, float>(h, kid, b, irk)
+                                      : launch_kernel_t<double, double>(h, kid, b, irk);
+}
+int launch_seq(cs_handle* h, const int* ks, int n, int b, int irk) {
+  for (int q = 0; q < n; ++q) {
+    int rc = launch_kernel(h, ks[q], b, irk);
+    if (rc) return rc;
+  }
+  return CS_OK;
+}
+int launch_fluxes(cs_handle* h, int b) {
+  static const int ks[4] = {CS_K_PRIMS, CS_K_FLUX_X, CS_K_FLUX_Y, CS_K_FLUX_Z};
+  return launch_seq(h, ks, 4, b, 1);
+}
+int launch_emf(cs_handle* h, int b) {
+  static const int ks[3] = {CS_K_EMF_Z, CS_K_EMF_X, CS_K_EMF_Y};
+  return launch_seq(h, ks, 3, b, 1);
+}
+int launch_update(cs_handle* h, int irk) {
+  static const int ks[3] = {CS_K_UPDATE_CELLS, CS_K_UPDATE_FACES, CS_K_CLEANUP};
+  return launch_seq(h, ks, 3, h->cur, irk);
+}
+int launch_cfl(cs_handle* h) { return launch_kernel(h, CS_K_CFL, h->cur, 1); }
+
+int launch_stage(cs_handle* h, int irk) {
+  int b = stage_input(h, irk);
+  int rc;
+  if ((rc = launch_halo(h, b))) return rc;
+  if ((rc = launch_fluxes(h, b))) return rc;
+  if ((rc = launch_emf(h, b))) return rc;
+  return launch_update(h, irk);
+}
+
+int launch_step(cs_handle* h) {
+  int rc;
+  for (int irk = 1; irk <= h->cfg.rk_order; ++irk)
+    if ((rc = launch_stage(h, irk))) return rc;
+  return launch_cfl(h);
+}
+
+void free_all(cs_handle* h) {
+  if (h->graph) hipGraphExecDestroy(h->graph);
+  hipFree(h->T); hipFree(h->rb);
+  for (int b = 0; b < 2; ++b)
+    for (int q = 0; q < 5; ++q) hipFree(h->buf[b][q]);
+  hipFree(h->V);
+  for (int q = 0; q < 3; ++q) { hipFree(h->F[q]); hipFree(h->S[q]); hipFree(h->E[q]); }
+  hipFree(h->dt); hipFree(h->part); hipFree(h->flags);
+  if (h->stream) hipStreamDestroy(h->stream);
+}
+
+#define ALLOC(ptr, bytes)                                                           \
+  do {                                                                             \
+    if (hipMalloc((void**)&(ptr), (bytes)) != hipSuccess) {                        \
+      set_error("hipMalloc of %.3f GB failed", (double)(bytes) / 1e9);             \
+      free_all(h); delete h;                                                       \
+      return CS_E_NOMEM;                                                           \
+    }                                                                              \
+    if (hipMemset((ptr), 0, (bytes)) != hipSuccess) {                              \
+      set_error("hipMemset failed"); free_all(h); delete h; return CS_E_HIP;       \
+    }                                                                              \
+  } while (0)
+
+}  // namespace
+
+extern "C" int cs_create(const cs_config* cfg, cs_handle** out) {
+  if (!out) { set_error("null output"); return CS_E_INVALID; }
+  *out = nullptr;
+  std::vector<double> rb, T;
+  int rc = host_metric(cfg, rb, T);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_error("no HIP device available");
+    return CS_E_HIP;
+  }
+  cs_handle* h = new cs_handle();
+  h->cfg = *cfg;
+  h->d = cs_make_dims(cfg->ntp, cfg->nr, cfg->nghost);
+  h->pr = Params{cfg->gamma, cfg->gamma - 1.0, cfg->rhofloor, cfg->prsfloor, cfg->cfl, cfg->grav,
+                 cfg->recon, cfg->riemann, cfg->emf, cfg->radial_bc == CS_RBC_WALL ? 1 : 0};
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    delete h;
+    return CS_E_HIP;
+  }
+  const CsDims& d = h->d;
+  size_t cb = sizeof(double) * d.cpp * 6, fb = sizeof(double) * d.fpp * 6;
+  ALLOC(h->T, sizeof(double) * T.size());
+  ALLOC(h->rb, sizeof(double) * rb.size());
+  for (int b = 0; b < 2; ++b) {
+    ALLOC(h->buf[b][0], cb * 8);
+    ALLOC(h->buf[b][1], cb);
+    for (int q = 2; q < 5; ++q) ALLOC(h->buf[b][q], fb);
+  }
+  size_t vb = (cfg->precision == CS_MIXED ? sizeof(float) : sizeof(double)) * d.cpp * 6 * 9;
+  ALLOC(h->V, vb);
+  for (int q = 0; q < 3; ++q) {
+    ALLOC(h->F[q], fb * 9);
+    ALLOC(h->S[q], fb * 2);
+    ALLOC(h->E[q], fb);
+  }
+  ALLOC(h->dt, sizeof(double) * 2);
+  h->npart = (int64_t)nblk((uint64_t)6 * d.N * d.N * d.NR) + 16;
+  ALLOC(h->part, sizeof(double) * h->npart * 18);
+  ALLOC(h->flags, sizeof(int) * 4);
+  if (hipMemcpy(h->T, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(h->rb, rb.data(), sizeof(double) * rb.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("metric upload failed");
+    free_all(h);
+    delete h;
+    return CS_E_HIP;
+  }
+  *out = h;
+  return CS_OK;
+}
+
+extern "C" int cs_destroy(cs_handle* h) {
+  if (!h) return CS_OK;
+  hipStreamSynchronize(h->stream);
+  free_all(h);
+  delete h;
+  return CS_OK;
+}
+
+#define CHECK_H(h)                                  \
+  do {                                              \
+    if (!(h)) { set_error("null handle"); return CS_E_INVALID; } \
+  } while (0)
+
+extern "C" int cs_upload_state(cs_handle* h, const double* U, const double* ent, const double* Bxf,
+                               const double* Byf, const double* Bzf) {
+  CHECK_H(h);
+  if (!U || !ent || !Bxf || !Byf || !Bzf) { set_error("null array"); return CS_E_INVALID; }
+  const CsDims& d = h->d;
+  size_t cb = sizeof(double) * d.cpp * 6, fb = sizeof(double) * d.fpp * 6;
+  h->cur = 0;
+  CS_HIP(hipMemcpyAsync(h->buf[0][0], U, cb * 8, hipMemcpyHostToDevice, h->stream));
+  CS_HIP(hipMemcpyAsync(h->buf[0][1], ent, cb, hipMemcpyHostToDevice, h->stream));
+  CS_HIP(hipMemcpyAsync(h->buf[0][2], Bxf, fb, hipMemcpyHostToDevice, h->stream));
+  CS_HIP(hipMemcpyAsync(h->buf[0][3], Byf, fb, hipMemcpyHostToDevice, h->stream));
+  CS_HIP(hipMemcpyAsync(h->buf[0][4], Bzf, fb, hipMemcpyHostToDevice, h->stream));
+  // the scratch stage buffer starts as a copy so its ghost zones are defined
+  for (int q = 0; q < 5; ++q)
+    CS_HIP(hipMemcpyAsync(h->buf[1][q], h->buf[0][q], q == 0 ? cb * 8 : (q == 1 ? cb : fb),
+                          hipMemcpyDeviceToDevice, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_download_state(cs_handle* h, double* U, double* ent, double* Bxf, double* Byf,
+                                 double* Bzf) {
+  CHECK_H(h);
+  const CsDims& d = h->d;
+  size_t cb = sizeof(double) * d.cpp * 6, fb = sizeof(double) * d.fpp * 6;
+  int b = h->cur;
+  if (U) CS_HIP(hipMemcpyAsync(U, h->buf[b][0], cb * 8, hipMemcpyDeviceToHost, h->stream));
+  if (ent) CS_HIP(hipMemcpyAsync(ent, h->buf[b][1], cb, hipMemcpyDeviceToHost, h->stream));
+  if (Bxf) CS_HIP(hipMemcpyAsync(Bxf, h->buf[b][2], fb, hipMemcpyDeviceToHost, h->stream));
+  if (Byf) CS_HIP(hipMemcpyAsync(Byf, h->buf[b][3], fb, hipMemcpyDeviceToHost, h->stream));
+  if (Bzf) CS_HIP(hipMemcpyAsync(Bzf, h->buf[b][4], fb, hipMemcpyDeviceToHost, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_halo_exchange(cs_handle* h) {
+  CHECK_H(h);
+  int rc = launch_halo(h, h->cur);
+  if (rc) return rc;
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_make_facial_fluxes(cs_handle* h) {
+  CHECK_H(h);
+  int rc = launch_fluxes(h, h->cur);
+  if (rc) return rc;
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_make_edge_efields(cs_handle* h) {
+  CHECK_H(h);
+  int rc = launch_emf(h, h->cur);
+  if (rc) return rc;
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_update_stage(cs_handle* h, int32_t irk) {
+  CHECK_H(h);
+  if (irk < 1 || irk > h->cfg.rk_order) { set_error("irk %d out of range", irk); return CS_E_INVALID; }
+  int rc = launch_update(h, irk);
+  if (rc) return rc;
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_evaluate_timestep(cs_handle* h) {
+  CHECK_H(h);
+  int rc = launch_cfl(h);
+  if (rc) return rc;
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_set_dt(cs_handle* h, double dt) {
+  CHECK_H(h);
+  double v[2] = {dt, 0.0};
+  CS_HIP(hipMemcpyAsync(h->dt, v, sizeof(v), hipMemcpyHostToDevice, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_get_dt(cs_handle* h, double* dt) {
+  CHECK_H(h);
+  if (!dt) { set_error("null output"); return CS_E_INVALID; }
+  CS_HIP(hipMemcpyAsync(dt, h->dt, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_rk_stage(cs_handle* h, int32_t irk) {
+  CHECK_H(h);
+  if (irk < 1 || irk > h->cfg.rk_order) { set_error("irk %d out of range", irk); return CS_E_INVALID; }
+  int rc = launch_stage(h, irk);
+  if (rc) return rc;
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+static int check_flags(cs_handle* h) {
+  int fl[4];
+  CS_HIP(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  if (fl[1]) { set_error("NaN/Inf in the state"); return CS_E_NAN; }
+  return CS_OK;
+}
+
+static int run_steps(cs_handle* h, int nsteps, int use_graph) {
+  int rc;
+  if (!use_graph) {
+    for (int n = 0; n < nsteps; ++n)
+      if ((rc = launch_step(h))) return rc;
+    return CS_OK;
+  }
+  if (!h->graph) {
+    hipGraph_t gr;
+    CS_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    rc = launch_step(h);
+    hipError_t e = hipStreamEndCapture(h->stream, &gr);
+    if (rc) return rc;
+    if (e != hipSuccess) { set_error("capture failed: %s", hipGetErrorString(e)); return CS_E_HIP; }
+    CS_HIP(hipGraphInstantiate(&h->graph, gr, nullptr, nullptr, 0));
+    hipGraphDestroy(gr);
+  }
+  for (int n = 0; n < nsteps; ++n) CS_HIP(hipGraphLaunch(h->graph, h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_advance(cs_handle* h, int32_t nsteps, int32_t use_graph) {
+  CHECK_H(h);
+  if (nsteps < 0) { set_error("nsteps < 0"); return CS_E_INVALID; }
+  int rc = run_steps(h, nsteps, use_graph);
+  if (rc) return rc;
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return check_flags(h);
+}
+
+extern "C" int cs_download_fluxes(cs_handle* h, int32_t dir, double* flux, double* speeds) {
+  CHECK_H(h);
+  if (dir < 0 || dir > 2) { set_error("dir must be 0..2"); return CS_E_INVALID; }
+  size_t fb = sizeof(double) * h->d.fpp * 6;
+  if (flux) CS_HIP(hipMemcpyAsync(flux, h->F[dir], fb * 9, hipMemcpyDeviceToHost, h->stream));
+  if (speeds) CS_HIP(hipMemcpyAsync(speeds, h->S[dir], fb * 2, hipMemcpyDeviceToHost, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_download_emf(cs_handle* h, double* Efx, double* Efy, double* Efz) {
+  CHECK_H(h);
+  size_t fb = sizeof(double) * h->d.fpp * 6;
+  if (Efx) CS_HIP(hipMemcpyAsync(Efx, h->E[0], fb, hipMemcpyDeviceToHost, h->stream));
+  if (Efy) CS_HIP(hipMemcpyAsync(Efy, h->E[1], fb, hipMemcpyDeviceToHost, h->stream));
+  if (Efz) CS_HIP(hipMemcpyAsync(Efz, h->E[2], fb, hipMemcpyDeviceToHost, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_conservation_sums(cs_handle* h, double* sums9) {
+  CHECK_H(h);
+  if (!sums9) { set_error("null output"); return CS_E_INVALID; }
+  const CsDims& d = h->d;
+  Rng rc = make_rng(1, d.N, 1, d.N, 1, d.NR);
+  unsigned nb = nblk(rc.total);
+  k_sums_partial<<<nb, kBlock, 0, h->stream>>>(d, rc, geo_of(h), state_of(h, h->cur), h->part);
+  k_sums_final<<<1, 64, 0, h->stream>>>(h->part, (int)nb, h->part + (int64_t)h->npart * 18 - 16);
+  CS_HIP(hipGetLastError());
+  CS_HIP(hipMemcpyAsync(sums9, h->part + (int64_t)h->npart * 18 - 16, sizeof(double) * 9,
+                        hipMemcpyDeviceToHost, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_max_divb(cs_handle* h, double* out) {
+  CHECK_H(h);
+  if (!out) { set_error("null output"); return CS_E_INVALID; }
+  const CsDims& d = h->d;
+  Rng rc = make_rng(1, d.N, 1, d.N, 1, d.NR);
+  unsigned nb = nblk(rc.total);
+  k_divb_partial<<<nb, kBlock, 0, h->stream>>>(d, rc, geo_of(h), state_of(h, h->cur), h->part);
+  CS_HIP(hipGetLastError());
+  std::vector<double> p(nb);
+  CS_HIP(hipMemcpyAsync(p.data(), h->part, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  double m = 0.0;
+  for (double v : p) m = v > m ? v : m;
+  *out = m;
+  return CS_OK;
+}
+
+extern "C" int cs_time_steps(cs_handle* h, int32_t nsteps, int32_t use_graph, float* ms_total) {
+  CHECK_H(h);
+  if (!ms_total) { set_error("null output"); return CS_E_INVALID; }
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  if (use_graph) {  // build the graph outside the timed region
+    int rc = run_steps(h, 0, 1);
+    if (rc) return rc;
+  }
+  CS_HIP(hipStreamSynchronize(h->stream));
+  CS_HIP(hipEventRecord(e0, h->stream));
+  int rc = run_steps(h, nsteps, use_graph);
+  if (rc) return rc;
+  CS_HIP(hipEventRecord(e1, h->stream));
+  CS_HIP(hipEventSynchronize(e1));
+  CS_HIP(hipEventElapsedTime(ms_total, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return check_flags(h);
+}
+
+extern "C" int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float* ms_avg) {
+  CHECK_H(h);
+  if (!ms_avg || iters < 1 || kernel < 0 || kernel >= CS_K_COUNT) {
+    set_error("bad arguments");
+    return CS_E_INVALID;
+  }
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  // the update kernels are timed as stage 2 of SSP-RK3 (scratch buffer in
+  // place) or stage 1 of RK2 (scratch output), so U^n is never modified
+  int irk = h->cfg.rk_order == 3 ? 2 : 1;
+  CS_HIP(hipStreamSynchronize(h->stream));
+  CS_HIP(hipEventRecord(e0, h->stream));
+  for (int it = 0; it < iters; ++it) {
+    int rc = kernel == CS_K_STEP ? launch_step(h) : launch_kernel(h, kernel, h->cur, irk);
+    if (rc) return rc;
+  }
+  CS_HIP(hipEventRecord(e1, h->stream));
+  CS_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+  *ms_avg = ms / iters;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  return CS_OK;
+}
+
+extern "C" int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units) {
+  CHECK_H(h);
+  if (!units || kernel < 0 || kernel >= CS_K_COUNT) { set_error("bad arguments"); return CS_E_INVALID; }
+  const CsDims& d = h->d;
+  const Ranges r = ranges_of(d);
+  int64_t n = 0;
+  switch (kernel) {
+    case CS_K_HALO: n = 24ll * d.g * d.N * d.NR; break;
+    case CS_K_PRIMS: n = r.all.total; break;
+    case CS_K_FLUX_X: n = r.fx.total; break;
+    case CS_K_FLUX_Y: n = r.fy.total; break;
+    case CS_K_FLUX_Z: n = r.fz.total; break;
+    case CS_K_EMF_Z: n = r.ez.total; break;
+    case CS_K_EMF_X: n = r.ex.total; break;
+    case CS_K_EMF_Y: n = r.ey.total; break;
+    case CS_K_UPDATE_FACES: n = r.faces.total; break;
+    default: n = r.cells.total; break;
+  }
+  *units = n;
+  return CS_OK;
+}
+
+extern "C" int cs_last_error(char* buf, size_t len) {
+  if (!buf || len == 0) return CS_E_INVALID;
+  snprintf(buf, len, "%s", g_err.c_str());
+  return CS_OK;
+}
+
+extern "C" int cs_device_info(char* buf, size_t len) {
+  if (!buf || len == 0) return CS_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    snprintf(buf, len, "no device");
+    return CS_E_HIP;
+  }
+  hipDeviceProp_t p;
+  CS_HIP(hipGetDeviceProperties(&p, 0));
+  snprintf(buf, len, "%s %s CUs=%d mem=%.1fGB", p.name, p.gcnArchName, p.multiProcessorCount,
+           p.totalGlobalMem / 1e9);
+  return CS_OK;
+}

@@ -1,0 +1,213 @@
+/*
+ * cubesphere_hip.h -- C-ABI of the MI355X (gfx950) cell-update path of the
+ * CubeSphere cubed-sphere MHD solver.
+ *
+ * Array layout: every state array is the reference's per-patch Fortran
+ * (column-major) layout, one patch per cube sector, patches stacked last:
+ *
+ *   U   (1-g:N+g, 1-g:N+g, 1-g:nr+g, 8, 6)   conserved rho, rho*v(3), E, B(3)
+ *   ent (1-g:N+g, 1-g:N+g, 1-g:nr+g, 6)      entropy density p / rho^(gamma-1)
+ *   Bxf, Byf, Bzf (0-g:N+g, 0-g:N+g, 0-g:nr+g, 6)   facial normal B
+ *
+ * i.e. exactly U_amr / entropygr_amr / Bxf_amr ... of
+ * pkg/inc/cubesphere/300riemann_module/300riemann_module_3d.f:502-552 with
+ * Max_patches_per_proc = 6 (all sectors on one GPU).  Because the device keeps
+ * the same panel-major SoA layout (x fastest), upload and download are plain
+ * copies: no transposition happens at the boundary.
+ *
+ * Units are the reference's Gaussian units (B^2 / 8 pi).
+ *
+ * Contract: every entry point returns 0 on success or a negative CS_E_* code;
+ * nothing throws across the ABI; cs_last_error() returns the message of the
+ * last failure on the calling thread.  The handle owns its device memory;
+ * host arrays are borrowed for the duration of the call only.  Calls on one
+ * handle are not re-entrant; different handles are independent.
+ */
+#ifndef CUBESPHERE_HIP_H
+#define CUBESPHERE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes (counterparts of the reference's STOP / MPI_ABORT paths) */
+#define CS_OK 0
+#define CS_E_INVALID (-1)     /* bad argument or configuration               */
+#define CS_E_BAD_DECOMP (-2)  /* NGhost >= ntp etc. (203fill_ghost2d_global.f:27-31) */
+#define CS_E_HIP (-3)         /* HIP runtime failure                          */
+#define CS_E_NONPOSITIVE (-4) /* non-positive density/pressure after floors   */
+#define CS_E_NAN (-5)         /* NaN/Inf found in the state                   */
+#define CS_E_NOMEM (-6)       /* device allocation failed                     */
+
+/* reconstruction */
+#define CS_RECON_PCM 0   /* piecewise constant (first order)               */
+#define CS_RECON_WENO5 1 /* Jiang-Shu WENO5 of the primitive variables     */
+/* face Riemann solver */
+#define CS_RS_HLL 0      /* OneD_RS_MHD_Ptwise restated (610_1d_rs_oned_rs_mhd_ptwise.f:13) */
+#define CS_RS_HLLD 1     /* Miyoshi & Kusano (2005) HLLD                    */
+/* edge EMF solver */
+#define CS_EMF_LLF 0     /* 4-state HLL with symmetrised speeds, averaged E (USE_2D_LLF_RS_FOR_EZ 1) */
+#define CS_EMF_HLL 1     /* 4-state HLL with the face speeds (630_2d_rs_ez_two_d_rs_ez_4state_hll.f:10) */
+/* precision */
+#define CS_FP64 0        /* everything in FP64 (the drop-in)               */
+#define CS_MIXED 1       /* FP32 reconstruction + Riemann solves; FP64 metric, CT, update, sums */
+/* radial boundary */
+#define CS_RBC_PAD 0     /* zero-gradient pad (826pad_boundary_r.f:201 form) */
+#define CS_RBC_WALL 1    /* reflecting, perfectly conducting wall (surface mode) */
+/* radial mesh (202generate_global_radial_mesh.f) */
+#define CS_RMESH_UNIFORM 0
+#define CS_RMESH_LOG 1
+
+typedef struct cs_config {
+  int32_t ntp;         /* zones per panel edge (ntp_zones)                 */
+  int32_t nr;          /* radial zones (nr_zones)                           */
+  int32_t nghost;      /* ghost layers (NGhost), >= 3                       */
+  int32_t recon;       /* CS_RECON_*                                        */
+  int32_t riemann;     /* CS_RS_*                                           */
+  int32_t emf;         /* CS_EMF_*                                          */
+  int32_t precision;   /* CS_FP64 / CS_MIXED                                */
+  int32_t rk_order;    /* 2 = Heun RK2 (reference DRV:2661-2773), 3 = SSP-RK3 */
+  int32_t radial_bc;   /* CS_RBC_*                                          */
+  int32_t radial_mesh; /* CS_RMESH_*                                        */
+  double rmin, rmax;   /* shell radii                                       */
+  double gamma;        /* polytropic index                                  */
+  double rhofloor;     /* density floor (DRV:181: 1e-12)                    */
+  double prsfloor;     /* pressure floor (DRV:182: 1e-9)                    */
+  double cfl;          /* Courant number (cfl_coef)                         */
+  double grav;         /* radial gravity g = grav / r^2 (0 = off)           */
+  int32_t imex;        /* 1: gravity handled implicitly in the stage solve  */
+  int32_t reserved;
+} cs_config;
+
+/* element counts of the host arrays above (per array, all 6 panels) */
+typedef struct cs_sizes {
+  int64_t n_cell;  /* (N+2g)^2 (nr+2g) * 6      : ent, and per variable of U */
+  int64_t n_face;  /* (N+2g+1)^2 (nr+2g+1) * 6  : each of Bxf, Byf, Bzf, Ef* */
+  int64_t n_vert2d;/* (N+2g+1)^2 * 6            : 2-D vertex lattice        */
+  int64_t n_interior; /* 6 N^2 nr                                          */
+} cs_sizes;
+
+typedef struct cs_handle cs_handle;
+
+/* ---------------------------------------------------------------- host only */
+/* No GPU is touched by these. */
+
+int cs_sizes_of(const cs_config* cfg, cs_sizes* out);
+
+/* Unit-sphere vertex lattice of the six sectors including NGhost ghost rows,
+ * layout coordinates_gr_global(-g:N+g, -g:N+g, 3, 6): Mesh_Sector_General with
+ * ntp_patches = 1 (201mesh_sector_general.f:79) followed by FILL_GHOST2D_GLOBAL
+ * (203fill_ghost2d_global.f:7).  Corner ghost blocks, which the reference
+ * leaves unset, are filled by continuing the sector's own swivel lattice. */
+int cs_build_unit_lattice(const cs_config* cfg, double* coords);
+
+/* Radial vertex radii rb(-g:nr+g) (GENERATE_GLOBAL_RADIAL_{UNIFORM,LOG}_MESH,
+ * 202generate_global_radial_mesh.f:10,71). */
+int cs_build_radial_mesh(const cs_config* cfg, double* rb);
+
+/* Metric tables the kernels use (exact spherical geometry, separable into a
+ * 2-D panel table times 1-D radial factors).  Each output may be NULL.
+ * vol(1-g:N+g,1-g:N+g,1-g:nr+g,6), xarea/yarea/zarea face shaped. */
+int cs_build_metric(const cs_config* cfg, double* vol, double* xarea,
+                    double* yarea, double* zarea);
+
+/* Initial conditions written into host arrays of the layout above.
+ * problem: 0 closed-form coordinate fields (801problem_generator_u_a.f:105-229),
+ *          1 Gaussian bump + solid-body rotation + azimuthal B (surface test),
+ *          2 magnetised blast (p=10 inside a 0.1 rad cap, B = 0.5 east),
+ *          3 hydrostatic shell with seeded density noise, B = curl A.
+ * Face B comes from a vector potential by Stokes' theorem (820init_patch_cube_sphere.f:464+),
+ * so the discrete div B is zero to roundoff.  Ghost zones are left zero. */
+int cs_init_problem(const cs_config* cfg, int32_t problem, uint64_t seed,
+                    double* U, double* ent, double* Bxf, double* Byf,
+                    double* Bzf);
+
+/* ---------------------------------------------------------------- device */
+int cs_create(const cs_config* cfg, cs_handle** out);
+int cs_destroy(cs_handle* h);
+
+int cs_upload_state(cs_handle* h, const double* U, const double* ent,
+                    const double* Bxf, const double* Byf, const double* Bzf);
+int cs_download_state(cs_handle* h, double* U, double* ent, double* Bxf,
+                      double* Byf, double* Bzf);
+
+/* Ghost exchange between the six sectors: FETCH_BUFFERS_FILL_IN_UX/UY/UY_TOP/UZ
+ * (330-333) semantics on one device -- index flips, face-B swap and sign
+ * flips, averaging of coinciding faces -- then the radial boundary pad. */
+int cs_halo_exchange(cs_handle* h);
+
+/* Reconstruction + face Riemann solves for every x/y/z face
+ * (LIMIT_ALL_SLOPES_P1 reconstruction role + MAKE_FACIAL_FLUXES,
+ *  450make_facial_fluxes.f:5).  Reads the current stage state. */
+int cs_make_facial_fluxes(cs_handle* h);
+
+/* Edge EMFs by the multidimensional Riemann solver
+ * (MAKE_EDGE_EFIELDS_P1, 460make_edge_fields_p1.f:5). */
+int cs_make_edge_efields(cs_handle* h);
+
+/* Stage update: conserved variables, entropy, constrained-transport face B,
+ * then zone B from the faces and the cons->prim floors
+ * (UPDATE_IMEX_RK_STEP_P1 484:7 + CLEANUP_AFTER_UPDATE 485:4).
+ * irk is the 1-based stage; the stage weights follow cfg->rk_order. */
+int cs_update_stage(cs_handle* h, int32_t irk);
+
+/* CFL time step (EVALUATE_TIMESTEP, 490evaluate_timestep.f:7):
+ * dt_next = min(1.1 dt, cfl * L / (|v| + c_f)); kept on the device. */
+int cs_evaluate_timestep(cs_handle* h);
+
+int cs_set_dt(cs_handle* h, double dt);
+int cs_get_dt(cs_handle* h, double* dt);
+
+/* One RK stage: halo -> fluxes -> EMFs -> update (replaces the per-patch
+ * body DRV:2007-3120). */
+int cs_rk_stage(cs_handle* h, int32_t irk);
+
+/* nsteps full time steps (all stages + CFL), dt stays on the device.
+ * use_graph != 0 captures one step in a hipGraph and replays it. */
+int cs_advance(cs_handle* h, int32_t nsteps, int32_t use_graph);
+
+/* Intermediate results for parity checks (face shaped, all 6 panels).
+ * flux: (9 comps, face) per direction; speeds: (2, face) per direction. */
+int cs_download_fluxes(cs_handle* h, int32_t dir, double* flux, double* speeds);
+int cs_download_emf(cs_handle* h, double* Efx, double* Efy, double* Efz);
+
+/* Diagnostics on the device: compensated (two-sum) volume integrals of the
+ * 8 conserved variables + entropy, and max |div B| * V^(1/3) / max|B|. */
+int cs_conservation_sums(cs_handle* h, double* sums9);
+int cs_max_divb(cs_handle* h, double* out);
+
+/* Timing with hipEvents recorded on the handle's stream.
+ * cs_time_steps: nsteps whole steps (optionally graph replays).
+ * cs_time_kernel: average duration of one kernel (CS_K_* id) over iters
+ * launches on the current state; CS_K_STEP is a whole step.  Timing the
+ * update kernels overwrites the stage scratch buffer only.
+ * cs_kernel_units: number of work items (faces, edges, zones) one launch of
+ * the kernel processes, for algorithmic-byte accounting. */
+#define CS_K_HALO 0
+#define CS_K_PRIMS 1
+#define CS_K_FLUX_X 2
+#define CS_K_FLUX_Y 3
+#define CS_K_FLUX_Z 4
+#define CS_K_EMF_Z 5
+#define CS_K_EMF_X 6
+#define CS_K_EMF_Y 7
+#define CS_K_UPDATE_CELLS 8
+#define CS_K_UPDATE_FACES 9
+#define CS_K_CLEANUP 10
+#define CS_K_CFL 11
+#define CS_K_STEP 12
+#define CS_K_COUNT 13
+int cs_time_steps(cs_handle* h, int32_t nsteps, int32_t use_graph, float* ms_total);
+int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float* ms_avg);
+int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units);
+
+int cs_last_error(char* buf, size_t len);
+int cs_device_info(char* buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1272 @@
+/* cs_oracle.c -- CPU oracle of the CubeSphere cell-update path.
+ *
+ * TEST INFRASTRUCTURE ONLY (see cs_oracle.h).  Plain C, double precision,
+ * loop-per-zone.  The algorithm restated here is the one the GPU path
+ * implements: reference routines where the reference has them, and the
+ * documented new numerics (WENO5, HLLD, SSP-RK3, exact spherical metric)
+ * where it does not.  Ghost-exchange maps are NOT taken from index formulas:
+ * every ghost zone/face is matched geometrically to the zone/face of the
+ * neighbouring sector that occupies the same place on the sphere, so the
+ * index flips and face-B sign rules of 330fetch_buffers_fill_in_ux.f are
+ * checked against geometry.
+ */
+#include "cs_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OB8PI (1.0 / (8.0 * 3.14159265358979323846))
+#define OB4PI (1.0 / (4.0 * 3.14159265358979323846))
+#define PI_ 3.14159265358979323846
+
+/* ------------------------------------------------------------- lattice */
+/* SWIVEL_N, 201mesh_sector_general.f:5-59 */
+static void swivel(const double* a1, const double* a2, int n, int lo, int hi, double* arr) {
+  double dot = a1[0] * a2[0] + a1[1] * a2[1] + a1[2] * a2[2];
+  double theta = acos(dot), dth = theta / (double)n, x[3], y[3], t;
+  int i, c;
+  for (c = 0; c < 3; ++c) x[c] = a1[c];
+  for (c = 0; c < 3; ++c) y[c] = a2[c] - dot * x[c];
+  t = sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
+  for (c = 0; c < 3; ++c) y[c] = y[c] / t;
+  for (i = lo; i <= hi; ++i) {
+    double* o = arr + 3 * (i - lo);
+    if (i == 0) { for (c = 0; c < 3; ++c) o[c] = a1[c]; }
+    else if (i == n) { for (c = 0; c < 3; ++c) o[c] = a2[c]; }
+    else {
+      double tt = dth * (double)i, ct = cos(tt), st = sin(tt);
+      for (c = 0; c < 3; ++c) o[c] = ct * x[c] + st * y[c];
+    }
+  }
+}
+
+/* sector corners, 201mesh_sector_general.f:175-330 */
+static const int kCorner[6][4][3] = {
+    {{-1, -1, 1}, {1, -1, 1}, {-1, 1, 1}, {1, 1, 1}},   {{-1, 1, -1}, {-1, 1, 1}, {1, 1, -1}, {1, 1, 1}},
+    {{1, -1, -1}, {1, 1, -1}, {1, -1, 1}, {1, 1, 1}},   {{1, 1, -1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, -1}},
+    {{1, -1, 1}, {-1, -1, 1}, {1, -1, -1}, {-1, -1, -1}}, {{-1, 1, 1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}}};
+/* neighbours right, left, up, down (1-based), 203fill_ghost2d_global.f:37-75 */
+static const int kNb[6][4] = {{3, 6, 2, 5}, {1, 4, 3, 6}, {2, 5, 1, 4}, {5, 2, 6, 3}, {6, 3, 4, 1}, {4, 1, 5, 2}};
+
+static void cross3(const double* a, const double* b, double* o) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+static double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static void unit3(double* a) {
+  double n = sqrt(dot3(a, a));
+  if (n > 0) { a[0] /= n; a[1] /= n; a[2] /= n; }
+}
+
+void or_unit_lattice(int N, int g, double* coords) {
+  int nf = N + 2 * g + 1, s, i, j, c;
+  double* a1 = malloc(sizeof(double) * 3 * nf);
+  double* a2 = malloc(sizeof(double) * 3 * nf);
+  double* a3 = malloc(sizeof(double) * 3 * nf);
+  double* a4 = malloc(sizeof(double) * 3 * nf);
+  double* src;
+  double q = 1.0 / sqrt(3.0);
+#define CO(ii, jj, cc, ss) coords[(size_t)((ii) + g) + (size_t)nf * (((jj) + g) + (size_t)nf * ((cc) + 3 * (size_t)(ss)))]
+  memset(coords, 0, sizeof(double) * nf * nf * 18);
+  for (s = 0; s < 6; ++s) {
+    double v[4][3];
+    int a;
+    for (a = 0; a < 4; ++a)
+      for (c = 0; c < 3; ++c) v[a][c] = kCorner[s][a][c] > 0 ? q : -q;
+    swivel(v[0], v[1], N, 0, N, a1);
+    swivel(v[2], v[3], N, 0, N, a2);
+    swivel(v[0], v[2], N, 0, N, a3);
+    swivel(v[1], v[3], N, 0, N, a4);
+    for (j = 0; j <= N; ++j)
+      for (i = 0; i <= N; ++i) {
+        /* 201mesh_sector_general.f:735-815 */
+        const double *p1 = a1 + 3 * i, *p2 = a2 + 3 * i, *p3 = a3 + 3 * j, *p4 = a4 + 3 * j;
+        double Ax = p1[1] * p2[2] - p1[2] * p2[1], Ay = p1[2] * p2[0] - p1[0] * p2[2],
+               Az = p1[0] * p2[1] - p1[1] * p2[0];
+        double Bx = p3[1] * p4[2] - p3[2] * p4[1], By = p3[2] * p4[0] - p3[0] * p4[2],
+               Bz = p3[0] * p4[1] - p3[1] * p4[0];
+        double Cx = By * Az - Bz * Ay, Cy = Bz * Ax - Bx * Az, Cz = Bx * Ay - By * Ax, t;
+        int sec = s + 1;
+        if ((sec == 1 && Cz < 0.0) || (sec == 4 && Cz > 0.0) || (sec == 2 && Cy < 0.0) ||
+            (sec == 5 && Cy > 0.0) || (sec == 3 && Cx < 0.0) || (sec == 6 && Cx > 0.0)) {
+          Cx = -Cx; Cy = -Cy; Cz = -Cz;
+        }
+        t = 1.0 / sqrt(Cx * Cx + Cy * Cy + Cz * Cz);
+        CO(i, j, 0, s) = t * Cx;
+        CO(i, j, 1, s) = t * Cy;
+        CO(i, j, 2, s) = t * Cz;
+      }
+  }
+  /* FILL_GHOST2D_GLOBAL, 203fill_ghost2d_global.f:78-150 */
+  src = malloc(sizeof(double) * nf * nf * 18);
+  memcpy(src, coords, sizeof(double) * nf * nf * 18);
+#define SR(ii, jj, cc, ss) src[(size_t)((ii) + g) + (size_t)nf * (((jj) + g) + (size_t)nf * ((cc) + 3 * (size_t)(ss)))]
+  for (s = 0; s < 6; ++s) {
+    int rt = kNb[s][0] - 1, lf = kNb[s][1] - 1, up = kNb[s][2] - 1, dn = kNb[s][3] - 1, x, y;
+    for (x = N + 1; x <= N + g; ++x)
+      for (y = 0; y <= N; ++y)
+        for (c = 0; c < 3; ++c) CO(x, y, c, s) = SR(y, N - 1 - (x - N - 1), c, rt);
+    for (x = -1; x >= -g; --x)
+      for (y = 0; y <= N; ++y)
+        for (c = 0; c < 3; ++c) CO(x, y, c, s) = SR(1 - (x + 1), N - y, c, lf);
+    for (y = N + 1; y <= N + g; ++y)
+      for (x = 0; x <= N; ++x)
+        for (c = 0; c < 3; ++c) CO(x, y, c, s) = SR(N - 1 - (y - N - 1), x, c, up);
+    for (y = -1; y >= -g; --y)
+      for (x = 0; x <= N; ++x)
+        for (c = 0; c < 3; ++c) CO(x, y, c, s) = SR(N - x, 1 - (y + 1), c, dn);
+  }
+#undef SR
+  free(src); free(a1); free(a2); free(a3); free(a4);
+}
+
+/* continue the sector's own swivel lattice into the corner ghost blocks */
+static void corner_blocks(int N, int g, double* coords) {
+  int nf = N + 2 * g + 1, s, i, j, c;
+  double* a1 = malloc(sizeof(double) * 3 * nf);
+  double* a2 = malloc(sizeof(double) * 3 * nf);
+  double* a3 = malloc(sizeof(double) * 3 * nf);
+  double* a4 = malloc(sizeof(double) * 3 * nf);
+  double q = 1.0 / sqrt(3.0);
+  for (s = 0; s < 6; ++s) {
+    double v[4][3];
+    int a;
+    for (a = 0; a < 4; ++a)
+      for (c = 0; c < 3; ++c) v[a][c] = kCorner[s][a][c] > 0 ? q : -q;
+    swivel(v[0], v[1], N, -g, N + g, a1);
+    swivel(v[2], v[3], N, -g, N + g, a2);
+    swivel(v[0], v[2], N, -g, N + g, a3);
+    swivel(v[1], v[3], N, -g, N + g, a4);
+    for (j = -g; j <= N + g; ++j)
+      for (i = -g; i <= N + g; ++i) {
+        if ((i >= 0 && i <= N) || (j >= 0 && j <= N)) continue;
+        {
+          const double *p1 = a1 + 3 * (i + g), *p2 = a2 + 3 * (i + g), *p3 = a3 + 3 * (j + g), *p4 = a4 + 3 * (j + g);
+          double A[3], B[3], C[3];
+          int sec = s + 1;
+          cross3(p1, p2, A);
+          cross3(p3, p4, B);
+          cross3(B, A, C);
+          if ((sec == 1 && C[2] < 0.0) || (sec == 4 && C[2] > 0.0) || (sec == 2 && C[1] < 0.0) ||
+              (sec == 5 && C[1] > 0.0) || (sec == 3 && C[0] < 0.0) || (sec == 6 && C[0] > 0.0)) {
+            C[0] = -C[0]; C[1] = -C[1]; C[2] = -C[2];
+          }
+          unit3(C);
+          for (c = 0; c < 3; ++c) CO(i, j, c, s) = C[c];
+        }
+      }
+  }
+  free(a1); free(a2); free(a3); free(a4);
+}
+#undef CO
+
+/* GENERATE_GLOBAL_RADIAL_UNIFORM_MESH / _LOG_MESH, 202:10-60, 71-125 */
+void or_radial_mesh(const or_config* c, double* rb) {
+  int i;
+  for (i = -c->nghost; i <= c->nr + c->nghost; ++i) {
+    double p = (double)i / (double)c->nr;
+    rb[i + c->nghost] = c->radial_mesh == 1 ? c->rmin * pow(c->rmax / c->rmin, p)
+                                            : c->rmin + (c->rmax - c->rmin) * p;
+  }
+}
+
+/* ------------------------------------------------------------- grid */
+typedef struct { int fam, s, i, j, sfam, ss, si, sj; double sgn; } fmap_t;
+typedef struct { int s, i, j, ss, si, sj; } cmap_t;
+
+struct or_grid {
+  or_config c;
+  int N, NR, g, nc, nrc, nf, nrf;
+  int64_t cpp, fpp, t2;
+  double* lat;  /* lattice */
+  double* rb;
+  /* 2-D tables, each t2*6 */
+  double *u[3], *xn[3], *xc[3], *thy, *yn[3], *yc[3], *thx, *zs[3], *zsn, *zt[3], *om, *chmin, *mi[9];
+  double *en1[3], *etau[3], *ecos, *esin, *xet[3], *yet[3];
+  cmap_t* cmap; int ncmap;
+  fmap_t* fmap; int nfmap;
+  fmap_t* pairs; int npairs;
+  int fails;
+};
+
+#define CELL(G, i, j, k) ((int64_t)((i) - 1 + (G)->g) + (int64_t)(G)->nc * (((j) - 1 + (G)->g) + (int64_t)(G)->nc * ((k) - 1 + (G)->g)))
+#define FACE(G, i, j, k) ((int64_t)((i) + (G)->g) + (int64_t)(G)->nf * (((j) + (G)->g) + (int64_t)(G)->nf * ((k) + (G)->g)))
+#define T2(G, s, i, j) ((int64_t)(s) * (G)->t2 + (int64_t)((i) + (G)->g) + (int64_t)(G)->nf * ((j) + (G)->g))
+
+static void uv(const or_grid* G, int s, int i, int j, double* o) {
+  int c;
+  for (c = 0; c < 3; ++c) o[c] = G->lat[(size_t)(i + G->g) + (size_t)G->nf * ((j + G->g) + (size_t)G->nf * (c + 3 * (size_t)s))];
+}
+static double ang(const double* a, const double* b) {
+  double x[3];
+  cross3(a, b, x);
+  return atan2(sqrt(dot3(x, x)), dot3(a, b));
+}
+static void perp_unit(double* v, const double* n) {
+  double t = dot3(v, n);
+  v[0] -= t * n[0]; v[1] -= t * n[1]; v[2] -= t * n[2];
+  unit3(v);
+}
+static double tri_om(const double* a, const double* b, const double* c) {
+  double bc[3];
+  cross3(b, c, bc);
+  return 2.0 * atan2(dot3(a, bc), 1.0 + dot3(a, b) + dot3(b, c) + dot3(c, a));
+}
+
+static double* al(int64_t n) { return calloc((size_t)n, sizeof(double)); }
+
+static void build_geometry(or_grid* G) {
+  int N = G->N, g = G->g, s, i, j, c;
+  for (s = 0; s < 6; ++s) {
+    for (j = -g; j <= N + g; ++j)
+      for (i = -g; i <= N + g; ++i) {
+        double a[3];
+        uv(G, s, i, j, a);
+        for (c = 0; c < 3; ++c) G->u[c][T2(G, s, i, j)] = a[c];
+      }
+    for (j = 1 - g; j <= N + g; ++j) /* x-faces and y-edges */
+      for (i = -g; i <= N + g; ++i) {
+        double a[3], b[3], n[3], m[3], t[3];
+        int64_t q = T2(G, s, i, j);
+        uv(G, s, i, j - 1, a);
+        uv(G, s, i, j, b);
+        cross3(b, a, n);
+        unit3(n);
+        for (c = 0; c < 3; ++c) m[c] = a[c] + b[c];
+        unit3(m);
+        for (c = 0; c < 3; ++c) t[c] = b[c] - a[c];
+        perp_unit(t, m);
+        for (c = 0; c < 3; ++c) { G->xn[c][q] = n[c]; G->xc[c][q] = m[c]; G->yet[c][q] = t[c]; }
+        G->thy[q] = ang(a, b);
+      }
+    for (j = -g; j <= N + g; ++j) /* y-faces and x-edges */
+      for (i = 1 - g; i <= N + g; ++i) {
+        double a[3], b[3], n[3], m[3], t[3];
+        int64_t q = T2(G, s, i, j);
+        uv(G, s, i - 1, j, a);
+        uv(G, s, i, j, b);
+        cross3(a, b, n);
+        unit3(n);
+        for (c = 0; c < 3; ++c) m[c] = a[c] + b[c];
+        unit3(m);
+        for (c = 0; c < 3; ++c) t[c] = b[c] - a[c];
+        perp_unit(t, m);
+        for (c = 0; c < 3; ++c) { G->yn[c][q] = n[c]; G->yc[c][q] = m[c]; G->xet[c][q] = t[c]; }
+        G->thx[q] = ang(a, b);
+      }
+    for (j = 1 - g; j <= N + g; ++j) /* zones */
+      for (i = 1 - g; i <= N + g; ++i) {
+        double a[3], b[3], cc[3], e[3], zs[3], sh[3], t1[3], M[3][3], det, ch = 1e300, dd[3];
+        int64_t q = T2(G, s, i, j), qL = T2(G, s, i - 1, j), qD = T2(G, s, i, j - 1);
+        const double* pr[4][2];
+        int p;
+        uv(G, s, i - 1, j - 1, a);
+        uv(G, s, i, j - 1, b);
+        uv(G, s, i, j, cc);
+        uv(G, s, i - 1, j, e);
+        /* area vector of the spherical zone on the unit sphere, by closing
+           the cone: sum over its four edges of (theta/2) * unit normal */
+        for (c = 0; c < 3; ++c)
+          zs[c] = 0.5 * (G->thx[qD] * G->yn[c][qD] - G->thy[q] * G->xn[c][q] - G->thx[q] * G->yn[c][q] +
+                         G->thy[qL] * G->xn[c][qL]);
+        G->zsn[q] = sqrt(dot3(zs, zs));
+        for (c = 0; c < 3; ++c) { G->zs[c][q] = zs[c]; sh[c] = zs[c] / G->zsn[q]; }
+        for (c = 0; c < 3; ++c) t1[c] = (b[c] + cc[c]) - (a[c] + e[c]);
+        perp_unit(t1, sh);
+        for (c = 0; c < 3; ++c) G->zt[c][q] = t1[c];
+        G->om[q] = tri_om(a, b, cc) + tri_om(a, cc, e);
+        pr[0][0] = e; pr[0][1] = cc; pr[1][0] = a; pr[1][1] = b; pr[2][0] = b; pr[2][1] = cc; pr[3][0] = a; pr[3][1] = e;
+        for (p = 0; p < 4; ++p) {
+          double l;
+          for (c = 0; c < 3; ++c) dd[c] = pr[p][0][c] - pr[p][1][c];
+          l = sqrt(dot3(dd, dd));
+          if (l < ch) ch = l;
+        }
+        G->chmin[q] = ch;
+        for (c = 0; c < 3; ++c) {
+          M[0][c] = 0.5 * (G->xn[c][qL] + G->xn[c][q]);
+          M[1][c] = 0.5 * (G->yn[c][qD] + G->yn[c][q]);
+          M[2][c] = sh[c];
+        }
+        det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+              M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+        if (fabs(det) > 1e-300) {
+          /* inverse by cofactors */
+          int r, k2;
+          for (r = 0; r < 3; ++r)
+            for (k2 = 0; k2 < 3; ++k2) {
+              int r1 = (k2 + 1) % 3, r2 = (k2 + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+              G->mi[3 * r + k2][q] = (M[r1][c1] * M[r2][c2] - M[r1][c2] * M[r2][c1]) / det;
+            }
+        }
+      }
+    for (j = 1 - g; j <= N + g - 1; ++j) /* z-edge frames, 460make_edge_fields_p1.f:2700-2760 */
+      for (i = 1 - g; i <= N + g - 1; ++i) {
+        double n3[3], n1[3], tau[3], p[3], m[3], ct;
+        int64_t q = T2(G, s, i, j);
+        uv(G, s, i, j, n3);
+        uv(G, s, i + 1, j, p);
+        uv(G, s, i - 1, j, m);
+        for (c = 0; c < 3; ++c) n1[c] = p[c] - m[c];
+        perp_unit(n1, n3);
+        uv(G, s, i, j + 1, p);
+        uv(G, s, i, j - 1, m);
+        for (c = 0; c < 3; ++c) tau[c] = p[c] - m[c];
+        perp_unit(tau, n3);
+        ct = dot3(tau, n1);
+        for (c = 0; c < 3; ++c) { G->en1[c][q] = n1[c]; G->etau[c][q] = tau[c]; }
+        G->ecos[q] = ct;
+        G->esin[q] = sqrt(1.0 - ct * ct);
+      }
+  }
+}
+
+/* --- geometric matching of ghost zones / faces to owners --------------- */
+typedef struct { double p[3]; double n[3]; int fam, s, i, j; } gobj;
+
+static void zone_center(const or_grid* G, int s, int i, int j, double* o) {
+  double a[3], b[3], c3[3], d[3];
+  int c;
+  uv(G, s, i - 1, j - 1, a); uv(G, s, i, j - 1, b); uv(G, s, i, j, c3); uv(G, s, i - 1, j, d);
+  for (c = 0; c < 3; ++c) o[c] = a[c] + b[c] + c3[c] + d[c];
+  unit3(o);
+}
+/* face of family fam (0 x-face: vertex i, cell j; 1 y-face: cell i, vertex j):
+   midpoint and unit normal oriented toward increasing index */
+static void face_geo(const or_grid* G, int fam, int s, int i, int j, double* mid, double* nrm) {
+  double a[3], b[3], lo[3], hi[3], dir[3];
+  int c;
+  if (fam == 0) { uv(G, s, i, j - 1, a); uv(G, s, i, j, b); }
+  else { uv(G, s, i - 1, j, a); uv(G, s, i, j, b); }
+  for (c = 0; c < 3; ++c) mid[c] = a[c] + b[c];
+  unit3(mid);
+  cross3(a, b, nrm);
+  unit3(nrm);
+  /* orientation: from the zone below the face to the zone above it */
+  if (fam == 0) { zone_center(G, s, i, j, lo); zone_center(G, s, i + 1, j, hi); }
+  else { zone_center(G, s, i, j, lo); zone_center(G, s, i, j + 1, hi); }
+  for (c = 0; c < 3; ++c) dir[c] = hi[c] - lo[c];
+  if (dot3(dir, nrm) < 0) for (c = 0; c < 3; ++c) nrm[c] = -nrm[c];
+}
+
+static int nearest(const gobj* cand, int n, const double* p, int excl_s, double tol) {
+  int best = -1, q;
+  double bd = tol * tol;
+  for (q = 0; q < n; ++q) {
+    double dx, dy, dz, dd;
+    if (cand[q].s == excl_s) continue;
+    dx = cand[q].p[0] - p[0]; dy = cand[q].p[1] - p[1]; dz = cand[q].p[2] - p[2];
+    dd = dx * dx + dy * dy + dz * dz;
+    if (dd < bd) { bd = dd; best = q; }
+  }
+  return best;
+}
+
+/* sort candidates along x so the search is a window */
+static int cmp_gx(const void* a, const void* b) {
+  double x = ((const gobj*)a)->p[0], y = ((const gobj*)b)->p[0];
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+static int nearest_sorted(const gobj* cand, int n, const double* p, int excl_s, double tol) {
+  int lo = 0, hi = n, q, best = -1;
+  double bd = tol * tol;
+  while (lo < hi) { int m = (lo + hi) / 2; if (cand[m].p[0] < p[0] - tol) lo = m + 1; else hi = m; }
+  for (q = lo; q < n && cand[q].p[0] <= p[0] + tol; ++q) {
+    double dx, dy, dz, dd;
+    if (cand[q].s == excl_s) continue;
+    dx = cand[q].p[0] - p[0]; dy = cand[q].p[1] - p[1]; dz = cand[q].p[2] - p[2];
+    dd = dx * dx + dy * dy + dz * dz;
+    if (dd < bd || (dd == bd && best >= 0 && cand[q].s < cand[best].s)) { bd = dd; best = q; }
+  }
+  return best;
+}
+
+static void build_maps(or_grid* G) {
+  int N = G->N, g = G->g, s, i, j, a, t, nz = 0, nfc = 0;
+  double tol = 0.25 * PI_ / (2.0 * N);
+  gobj* zones = malloc(sizeof(gobj) * 6 * N * N);
+  gobj* faces = malloc(sizeof(gobj) * 6 * 2 * (N + 1) * N);
+  (void)nearest;
+  for (s = 0; s < 6; ++s)
+    for (j = 1; j <= N; ++j)
+      for (i = 1; i <= N; ++i) {
+        gobj* o = &zones[nz++];
+        zone_center(G, s, i, j, o->p);
+        o->s = s; o->i = i; o->j = j; o->fam = 2;
+      }
+  for (s = 0; s < 6; ++s) {
+    for (j = 1; j <= N; ++j)
+      for (i = 0; i <= N; ++i) {
+        gobj* o = &faces[nfc++];
+        face_geo(G, 0, s, i, j, o->p, o->n);
+        o->fam = 0; o->s = s; o->i = i; o->j = j;
+      }
+    for (j = 0; j <= N; ++j)
+      for (i = 1; i <= N; ++i) {
+        gobj* o = &faces[nfc++];
+        face_geo(G, 1, s, i, j, o->p, o->n);
+        o->fam = 1; o->s = s; o->i = i; o->j = j;
+      }
+  }
+  qsort(zones, nz, sizeof(gobj), cmp_gx);
+  qsort(faces, nfc, sizeof(gobj), cmp_gx);
+  G->cmap = malloc(sizeof(cmap_t) * 6 * 4 * g * N);
+  G->fmap = malloc(sizeof(fmap_t) * 6 * 4 * g * (3 * N + 2));
+  G->pairs = malloc(sizeof(fmap_t) * 6 * 4 * N);
+  G->ncmap = G->nfmap = G->npairs = 0;
+  G->fails = 0;
+  for (s = 0; s < 6; ++s)
+    for (a = 1; a <= g; ++a)
+      for (t = 1; t <= N; ++t) {
+        int gi[4] = {N + a, 1 - a, t, t}, gj[4] = {t, t, N + a, 1 - a}, st;
+        for (st = 0; st < 4; ++st) {
+          double p[3];
+          int q;
+          zone_center(G, s, gi[st], gj[st], p);
+          q = nearest_sorted(zones, nz, p, s, tol);
+          if (q < 0) { G->fails++; continue; }
+          {
+            cmap_t* m = &G->cmap[G->ncmap++];
+            m->s = s; m->i = gi[st]; m->j = gj[st]; m->ss = zones[q].s; m->si = zones[q].i; m->sj = zones[q].j;
+          }
+        }
+      }
+  /* ghost faces: normal family on t in [1,N], tangential family on t in [0,N] */
+  for (s = 0; s < 6; ++s)
+    for (a = 1; a <= g; ++a)
+      for (t = 0; t <= N; ++t) {
+        int fam[4][2] = {{0, 1}, {0, 1}, {1, 0}, {1, 0}};
+        int ni[4] = {N + a, -a, t, t}, nj[4] = {t, t, N + a, -a};        /* normal family */
+        int ti[4] = {N + a, 1 - a, t, t}, tj[4] = {t, t, N + a, 1 - a};  /* tangential family */
+        int st;
+        for (st = 0; st < 4; ++st) {
+          int w;
+          for (w = 0; w < 2; ++w) {
+            int f = fam[st][w], fi = w == 0 ? ni[st] : ti[st], fj = w == 0 ? nj[st] : tj[st], q;
+            double p[3], n[3];
+            if (w == 0 && t == 0) continue;
+            face_geo(G, f, s, fi, fj, p, n);
+            q = nearest_sorted(faces, nfc, p, s, tol);
+            if (q < 0) { G->fails++; continue; }
+            {
+              fmap_t* m = &G->fmap[G->nfmap++];
+              m->fam = f; m->s = s; m->i = fi; m->j = fj;
+              m->sfam = faces[q].fam; m->ss = faces[q].s; m->si = faces[q].i; m->sj = faces[q].j;
+              m->sgn = dot3(n, faces[q].n) < 0 ? -1.0 : 1.0;
+            }
+          }
+        }
+      }
+  /* coinciding boundary faces: each physical panel-edge face once */
+  for (s = 0; s < 6; ++s)
+    for (t = 1; t <= N; ++t) {
+      int fi[4] = {0, N, t, t}, fj[4] = {t, t, 0, N}, ff[4] = {0, 0, 1, 1}, e;
+      for (e = 0; e < 4; ++e) {
+        double p[3], n[3];
+        int q;
+        face_geo(G, ff[e], s, fi[e], fj[e], p, n);
+        q = nearest_sorted(faces, nfc, p, s, tol);
+        if (q < 0) { G->fails++; continue; }
+        if (faces[q].s < s) continue; /* the pair is recorded from the lower sector */
+        {
+          fmap_t* m = &G->pairs[G->npairs++];
+          m->fam = ff[e]; m->s = s; m->i = fi[e]; m->j = fj[e];
+          m->sfam = faces[q].fam; m->ss = faces[q].s; m->si = faces[q].i; m->sj = faces[q].j;
+          m->sgn = dot3(n, faces[q].n) < 0 ? -1.0 : 1.0;
+        }
+      }
+    }
+  free(zones);
+  free(faces);
+}
+
+or_grid* or_grid_new(const or_config* c) {
+  or_grid* G;
+  int q;
+  int64_t n;
+  if (c->ntp < 4 || c->nr < 1 || c->nghost < 3 || c->nghost >= c->ntp) return NULL;
+  G = calloc(1, sizeof(or_grid));
+  G->c = *c;
+  G->N = c->ntp; G->NR = c->nr; G->g = c->nghost;
+  G->nc = G->N + 2 * G->g; G->nrc = G->NR + 2 * G->g;
+  G->nf = G->nc + 1; G->nrf = G->nrc + 1;
+  G->cpp = (int64_t)G->nc * G->nc * G->nrc;
+  G->fpp = (int64_t)G->nf * G->nf * G->nrf;
+  G->t2 = (int64_t)G->nf * G->nf;
+  G->lat = al(G->t2 * 18);
+  or_unit_lattice(G->N, G->g, G->lat);
+  corner_blocks(G->N, G->g, G->lat);
+  G->rb = al(G->NR + 2 * G->g + 1);
+  or_radial_mesh(c, G->rb);
+  n = G->t2 * 6;
+  for (q = 0; q < 3; ++q) {
+    G->u[q] = al(n); G->xn[q] = al(n); G->xc[q] = al(n); G->yn[q] = al(n); G->yc[q] = al(n);
+    G->zs[q] = al(n); G->zt[q] = al(n); G->en1[q] = al(n); G->etau[q] = al(n); G->xet[q] = al(n); G->yet[q] = al(n);
+  }
+  G->thy = al(n); G->thx = al(n); G->zsn = al(n); G->om = al(n); G->chmin = al(n);
+  G->ecos = al(n); G->esin = al(n);
+  for (q = 0; q < 9; ++q) G->mi[q] = al(n);
+  build_geometry(G);
+  build_maps(G);
+  return G;
+}
+
+void or_grid_free(or_grid* G) {
+  int q;
+  if (!G) return;
+  free(G->lat); free(G->rb);
+  for (q = 0; q < 3; ++q) {
+    free(G->u[q]); free(G->xn[q]); free(G->xc[q]); free(G->yn[q]); free(G->yc[q]);
+    free(G->zs[q]); free(G->zt[q]); free(G->en1[q]); free(G->etau[q]); free(G->xet[q]); free(G->yet[q]);
+  }
+  free(G->thy); free(G->thx); free(G->zsn); free(G->om); free(G->chmin); free(G->ecos); free(G->esin);
+  for (q = 0; q < 9; ++q) free(G->mi[q]);
+  free(G->cmap); free(G->fmap); free(G->pairs);
+  free(G);
+}
+
+int or_grid_dims(const or_grid* G, int64_t* o) {
+  o[0] = G->cpp; o[1] = G->fpp; o[2] = G->t2; o[3] = G->ncmap + G->nfmap + G->npairs;
+  return 0;
+}
+int or_map_failures(const or_grid* G) { return G->fails; }
+
+static double R(const or_grid* G, int k) { return G->rb[k + G->g]; }
+static double vol_of(const or_grid* G, int s, int i, int j, int k) {
+  double r1 = R(G, k), r0 = R(G, k - 1);
+  return G->om[T2(G, s, i, j)] * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0;
+}
+static double xarea(const or_grid* G, int s, int i, int j, int k) {
+  double r1 = R(G, k), r0 = R(G, k - 1);
+  return 0.5 * (r1 * r1 - r0 * r0) * G->thy[T2(G, s, i, j)];
+}
+static double yarea(const or_grid* G, int s, int i, int j, int k) {
+  double r1 = R(G, k), r0 = R(G, k - 1);
+  return 0.5 * (r1 * r1 - r0 * r0) * G->thx[T2(G, s, i, j)];
+}
+static double zarea(const or_grid* G, int s, int i, int j, int k) {
+  double r = R(G, k);
+  return r * r * G->zsn[T2(G, s, i, j)];
+}
+
+void or_metric(const or_grid* G, double* vol, double* xa, double* ya, double* za) {
+  int s, i, j, k, g = G->g;
+  for (s = 0; s < 6; ++s) {
+    if (vol)
+      for (k = 1 - g; k <= G->NR + g; ++k)
+        for (j = 1 - g; j <= G->N + g; ++j)
+          for (i = 1 - g; i <= G->N + g; ++i) vol[s * G->cpp + CELL(G, i, j, k)] = vol_of(G, s, i, j, k);
+    for (k = -g; k <= G->NR + g; ++k)
+      for (j = -g; j <= G->N + g; ++j)
+        for (i = -g; i <= G->N + g; ++i) {
+          int64_t f = s * G->fpp + FACE(G, i, j, k);
+          int kc = k > -g;
+          if (xa) xa[f] = (kc && j > -g) ? xarea(G, s, i, j, k) : 0.0;
+          if (ya) ya[f] = (kc && i > -g) ? yarea(G, s, i, j, k) : 0.0;
+          if (za) za[f] = (i > -g && j > -g) ? zarea(G, s, i, j, k) : 0.0;
+        }
+  }
+}
+
+/* ------------------------------------------------------------- halo */
+#define UU(U, G, s, q, c) (U)[((int64_t)(s) * 8 + (q)) * (G)->cpp + (c)]
+
+void or_halo(const or_grid* G, double* U, double* ent, double* Bx, double* By, double* Bz) {
+  int N = G->N, NR = G->NR, g = G->g, m, k, s, i, j, a, q;
+  double* fam[2] = {Bx, By};
+  /* 1) coinciding faces, averaged from the pre-exchange values
+        (330fetch_buffers_fill_in_ux.f:337-350) */
+  for (m = 0; m < G->npairs; ++m) {
+    const fmap_t* p = &G->pairs[m];
+    for (k = 1; k <= NR; ++k) {
+      double* pa = &fam[p->fam][p->s * G->fpp + FACE(G, p->i, p->j, k)];
+      double* pb = &fam[p->sfam][p->ss * G->fpp + FACE(G, p->si, p->sj, k)];
+      double va = *pa, vb = *pb;
+      *pa = (va + p->sgn * vb) * 0.5;
+      *pb = (vb + p->sgn * va) * 0.5;
+    }
+  }
+  /* 2) ghost zones and their z-faces */
+  for (m = 0; m < G->ncmap; ++m) {
+    const cmap_t* p = &G->cmap[m];
+    for (k = 1; k <= NR; ++k) {
+      int64_t d = CELL(G, p->i, p->j, k), sidx = CELL(G, p->si, p->sj, k);
+      for (q = 0; q < 8; ++q) UU(U, G, p->s, q, d) = UU(U, G, p->ss, q, sidx);
+      ent[p->s * G->cpp + d] = ent[p->ss * G->cpp + sidx];
+    }
+    for (k = 0; k <= NR; ++k)
+      Bz[p->s * G->fpp + FACE(G, p->i, p->j, k)] = Bz[p->ss * G->fpp + FACE(G, p->si, p->sj, k)];
+  }
+  /* 3) ghost lateral faces */
+  for (m = 0; m < G->nfmap; ++m) {
+    const fmap_t* p = &G->fmap[m];
+    for (k = 1; k <= NR; ++k)
+      fam[p->fam][p->s * G->fpp + FACE(G, p->i, p->j, k)] =
+          p->sgn < 0 ? -fam[p->sfam][p->ss * G->fpp + FACE(G, p->si, p->sj, k)]
+                     : fam[p->sfam][p->ss * G->fpp + FACE(G, p->si, p->sj, k)];
+  }
+  /* 4) corner ghost blocks take the sector's corner zone */
+  for (s = 0; s < 6; ++s)
+    for (k = 1; k <= NR; ++k)
+      for (a = 1; a <= g; ++a)
+        for (j = 1; j <= g; ++j) {
+          int cc;
+          for (cc = 0; cc < 4; ++cc) {
+            int ci = (cc & 1) ? N : 1, cj = (cc & 2) ? N : 1;
+            int di = (cc & 1) ? N + a : 1 - a, dj = (cc & 2) ? N + j : 1 - j;
+            int64_t d = CELL(G, di, dj, k), sidx = CELL(G, ci, cj, k);
+            for (q = 0; q < 8; ++q) UU(U, G, s, q, d) = UU(U, G, s, q, sidx);
+            ent[s * G->cpp + d] = ent[s * G->cpp + sidx];
+          }
+        }
+  /* 5) radial ghosts: zero-gradient pad or reflecting conducting wall */
+  for (s = 0; s < 6; ++s)
+    for (a = 1; a <= g; ++a) {
+      int side;
+      for (side = 0; side < 2; ++side) {
+        int kg = side == 0 ? 1 - a : NR + a, ks = side == 0 ? 1 : NR, sg = 1;
+        int kzg = side == 0 ? -a : NR + a, kzs = side == 0 ? 0 : NR, zsg = 1;
+        if (G->c.radial_bc == 1) {
+          ks = kg;
+          while (ks < 1 || ks > NR) { ks = ks < 1 ? 1 - ks : 2 * NR + 1 - ks; sg = -sg; }
+          kzs = kzg;
+          while (kzs < 0 || kzs > NR) { kzs = kzs < 0 ? -kzs : 2 * NR - kzs; zsg = -zsg; }
+        }
+        for (j = -g; j <= N + g; ++j)
+          for (i = -g; i <= N + g; ++i) {
+            if (j >= 1 - g) Bx[s * G->fpp + FACE(G, i, j, kg)] = Bx[s * G->fpp + FACE(G, i, j, ks)];
+            if (i >= 1 - g) By[s * G->fpp + FACE(G, i, j, kg)] = By[s * G->fpp + FACE(G, i, j, ks)];
+            if (i >= 1 - g && j >= 1 - g) {
+              double u[8], bz = Bz[s * G->fpp + FACE(G, i, j, kzs)];
+              int64_t d = CELL(G, i, j, kg), sidx = CELL(G, i, j, ks);
+              Bz[s * G->fpp + FACE(G, i, j, kzg)] = zsg < 0 ? -bz : bz;
+              for (q = 0; q < 8; ++q) u[q] = UU(U, G, s, q, sidx);
+              if (sg < 0) {
+                double n[3];
+                double mm, bb;
+                zone_center(G, s, i, j, n);
+                mm = u[1] * n[0] + u[2] * n[1] + u[3] * n[2];
+                bb = u[5] * n[0] + u[6] * n[1] + u[7] * n[2];
+                for (q = 0; q < 3; ++q) { u[1 + q] -= 2.0 * mm * n[q]; u[5 + q] -= 2.0 * bb * n[q]; }
+              }
+              for (q = 0; q < 8; ++q) UU(U, G, s, q, d) = u[q];
+              ent[s * G->cpp + d] = ent[s * G->cpp + sidx];
+            }
+          }
+      }
+    }
+}
+
+/* ------------------------------------------------------------- pointwise */
+/* PRIM_TO_CONS (661pde_3d_prim_to_cons.f:67-125) */
+void or_prim_to_cons(double gam, double rhof, double prsf, double ent, double* V, double* U) {
+  V[0] = V[0] > rhof ? V[0] : rhof;
+  if (V[4] <= prsf) { double pe = ent * pow(V[0], gam - 1.0); V[4] = prsf > pe ? prsf : pe; }
+  U[0] = V[0]; U[1] = V[0] * V[1]; U[2] = V[0] * V[2]; U[3] = V[0] * V[3];
+  U[5] = V[5]; U[6] = V[6]; U[7] = V[7];
+  U[4] = V[4] / (gam - 1.0) + 0.5 * V[0] * (V[1] * V[1] + V[2] * V[2] + V[3] * V[3]) +
+         OB8PI * (V[5] * V[5] + V[6] * V[6] + V[7] * V[7]);
+}
+/* CONS_TO_PRIM (662pde_3d_cons_to_prim.f:69-151) */
+int or_cons_to_prim(double gam, double rhof, double prsf, double* ent, double* U, double* V) {
+  double a, V2, B2;
+  int flag = 0;
+  U[0] = U[0] > rhof ? U[0] : rhof;
+  V[0] = U[0];
+  a = V[0] / (V[0] * V[0] + rhof * rhof);
+  V[1] = U[1] * a; V[2] = U[2] * a; V[3] = U[3] * a;
+  V[5] = U[5]; V[6] = U[6]; V[7] = U[7];
+  V2 = V[1] * V[1] + V[2] * V[2] + V[3] * V[3];
+  B2 = V[5] * V[5] + V[6] * V[6] + V[7] * V[7];
+  V[4] = (gam - 1.0) * (U[4] - 0.5 * V[0] * V2 - OB8PI * B2);
+  if (V[4] <= prsf) {
+    double pe = *ent * pow(V[0], gam - 1.0);
+    V[4] = prsf > pe ? prsf : pe;
+    flag = 1;
+  }
+  *ent = V[4] / pow(V[0], gam - 1.0);
+  U[4] = V[4] / (gam - 1.0) + 0.5 * V[0] * V2 + OB8PI * B2;
+  return flag;
+}
+/* PRIM_TO_WAVSPD, x (660pde_3d_prim_to_wavspd.f:54-76) */
+void or_wavspd_x(double gam, const double* V, double* smin, double* smax) {
+  double cs = sqrt(gam * V[4] / V[0]);
+  double b2 = (V[5] * V[5] + V[6] * V[6] + V[7] * V[7]) / (4.0 * PI_ * V[0]);
+  double bb = -(cs * cs + b2), cc = cs * cs * V[5] * V[5] / (4.0 * PI_ * V[0]);
+  double disc = bb * bb - 4.0 * cc;
+  double mf = sqrt((-bb + sqrt(disc > 0 ? disc : 0.0)) / 2.0);
+  *smin = V[1] - mf;
+  *smax = V[1] + mf;
+}
+/* x-flux; 600_1d_rs_oned_rs_mhd_dad.f:194-230 with B0 = 0 */
+void or_flux_x(const double* V, const double* U, double* F) {
+  double pt = V[4] + OB8PI * (V[5] * V[5] + V[6] * V[6] + V[7] * V[7]);
+  double vb = V[1] * V[5] + V[2] * V[6] + V[3] * V[7];
+  int c;
+  for (c = 0; c < 8; ++c) F[c] = V[1] * U[c];
+  F[1] += pt - OB4PI * V[5] * V[5];
+  F[2] -= OB4PI * V[5] * V[6];
+  F[3] -= OB4PI * V[5] * V[7];
+  F[4] += pt * V[1] - OB4PI * V[5] * vb;
+  F[5] = 0.0;
+  F[6] -= V[5] * V[2];
+  F[7] -= V[5] * V[3];
+}
+static void speeds3(double gam, const double* VL, const double* VR, double* SL, double* SR) {
+  double a, b, Vm[8];
+  int c;
+  *SL = 1e30; *SR = -1e30;
+  or_wavspd_x(gam, VL, &a, &b); if (a < *SL) *SL = a; if (b > *SR) *SR = b;
+  or_wavspd_x(gam, VR, &a, &b); if (a < *SL) *SL = a; if (b > *SR) *SR = b;
+  for (c = 0; c < 8; ++c) Vm[c] = 0.5 * (VL[c] + VR[c]);
+  or_wavspd_x(gam, Vm, &a, &b); if (a < *SL) *SL = a; if (b > *SR) *SR = b;
+}
+/* OneD_RS_MHD_Ptwise (610_1d_rs_oned_rs_mhd_ptwise.f:96-220, 653-704) */
+void or_hll(double gam, double rhof, double prsf, double* VL, double* VR, double entL, double entR,
+            double* Fs, double* entF, double* SL, double* SR) {
+  double UL[8], UR[8], FL[8], FR[8], Us[8], t1, t2, t3, eU, pe, pg, ob;
+  int c;
+  double bx = 0.5 * (VL[5] + VR[5]);
+  VL[5] = bx; VR[5] = bx;
+  or_prim_to_cons(gam, rhof, prsf, entL, VL, UL);
+  or_prim_to_cons(gam, rhof, prsf, entR, VR, UR);
+  or_flux_x(VL, UL, FL);
+  or_flux_x(VR, UR, FR);
+  speeds3(gam, VL, VR, SL, SR);
+  if (*SL >= 0.0) { for (c = 0; c < 8; ++c) Fs[c] = FL[c]; *entF = entL * VL[1]; return; }
+  if (*SR <= 0.0) { for (c = 0; c < 8; ++c) Fs[c] = FR[c]; *entF = entR * VR[1]; return; }
+  t3 = 1.0 / (*SR - *SL); t1 = *SR * t3; t2 = *SL * t3;
+  for (c = 0; c < 8; ++c) Us[c] = t1 * UR[c] - t2 * UL[c] - t3 * (FR[c] - FL[c]);
+  eU = t1 * entR - t2 * entL - t3 * (entR * VR[1] - entL * VL[1]);
+  pe = eU * pow(Us[0], gam - 1.0);
+  pg = (gam - 1.0) * (Us[4] - 0.5 * (Us[1] * Us[1] + Us[2] * Us[2] + Us[3] * Us[3]) / Us[0] -
+                      OB8PI * (Us[5] * Us[5] + Us[6] * Us[6] + Us[7] * Us[7]));
+  if (pg <= prsf) Us[4] = Us[4] + (pe - pg) / (gam - 1.0);
+  for (c = 0; c < 8; ++c) Fs[c] = 0.5 * (FL[c] + FR[c]) + 0.5 * *SL * (Us[c] - UL[c]) - 0.5 * *SR * (UR[c] - Us[c]);
+  ob = 1.0 / (*SR - *SL);
+  *entF = (*SR * entL * VL[1] - *SL * entR * VR[1]) * ob + (entR - entL) * (*SR * *SL * ob);
+}
+/* HLLD, Miyoshi & Kusano (2005), rationalised field b = B / sqrt(4 pi) */
+void or_hlld(double gam, double rhof, double prsf, double* VL, double* VR, double entL, double entR,
+             double* Fs, double* entF, double* SL, double* SR) {
+  double UL[8], UR[8], FL[8], FR[8], s4 = sqrt(4.0 * PI_), F[8];
+  double bx, bL[3], bR[3], ptL, ptR, dL, dR, SM, pts, rLs, rRs, vLs[2], vRs[2], bLs[2], bRs[2];
+  double vbL, vbR, vbLs, vbRs, ELs, ERs, sqL, sqR, SLs, SRs, ULs[8], URs[8], ULr[8], URr[8], FLr[8], FRr[8];
+  int c, side;
+  double B = 0.5 * (VL[5] + VR[5]);
+  VL[5] = B; VR[5] = B;
+  or_prim_to_cons(gam, rhof, prsf, entL, VL, UL);
+  or_prim_to_cons(gam, rhof, prsf, entR, VR, UR);
+  or_flux_x(VL, UL, FL);
+  or_flux_x(VR, UR, FR);
+  speeds3(gam, VL, VR, SL, SR);
+  if (*SL >= 0.0) { for (c = 0; c < 8; ++c) Fs[c] = FL[c]; *entF = entL * VL[1]; return; }
+  if (*SR <= 0.0) { for (c = 0; c < 8; ++c) Fs[c] = FR[c]; *entF = entR * VR[1]; return; }
+  bx = B / s4;
+  for (c = 0; c < 3; ++c) { bL[c] = (c == 0 ? B : VL[5 + c]) / s4; bR[c] = (c == 0 ? B : VR[5 + c]) / s4; }
+  ptL = VL[4] + 0.5 * (bL[0] * bL[0] + bL[1] * bL[1] + bL[2] * bL[2]);
+  ptR = VR[4] + 0.5 * (bR[0] * bR[0] + bR[1] * bR[1] + bR[2] * bR[2]);
+  dL = (*SL - VL[1]) * VL[0];
+  dR = (*SR - VR[1]) * VR[0];
+  SM = (dR * VR[1] - dL * VL[1] - ptR + ptL) / (dR - dL);
+  pts = (dR * ptL - dL * ptR + dL * dR * (VR[1] - VL[1])) / (dR - dL);
+  rLs = dL / (*SL - SM);
+  rRs = dR / (*SR - SM);
+  for (side = 0; side < 2; ++side) {
+    const double* V = side ? VR : VL;
+    const double* bb = side ? bR : bL;
+    double d = side ? dR : dL, S = side ? *SR : *SL;
+    double* vs = side ? vRs : vLs;
+    double* bs = side ? bRs : bLs;
+    double den = d * (S - SM) - bx * bx;
+    if (fabs(den) > 1e-10 * (fabs(d * (S - SM)) + bx * bx) && den != 0.0) {
+      double f1 = bx * (SM - V[1]) / den, f2 = (d * (S - V[1]) - bx * bx) / den;
+      vs[0] = V[2] - bb[1] * f1; vs[1] = V[3] - bb[2] * f1;
+      bs[0] = bb[1] * f2; bs[1] = bb[2] * f2;
+    } else {
+      vs[0] = V[2]; vs[1] = V[3]; bs[0] = bb[1]; bs[1] = bb[2];
+    }
+  }
+  vbL = VL[1] * bx + VL[2] * bL[1] + VL[3] * bL[2];
+  vbR = VR[1] * bx + VR[2] * bR[1] + VR[3] * bR[2];
+  vbLs = SM * bx + vLs[0] * bLs[0] + vLs[1] * bLs[1];
+  vbRs = SM * bx + vRs[0] * bRs[0] + vRs[1] * bRs[1];
+  ELs = ((*SL - VL[1]) * UL[4] - ptL * VL[1] + pts * SM + bx * (vbL - vbLs)) / (*SL - SM);
+  ERs = ((*SR - VR[1]) * UR[4] - ptR * VR[1] + pts * SM + bx * (vbR - vbRs)) / (*SR - SM);
+  sqL = sqrt(rLs); sqR = sqrt(rRs);
+  SLs = SM - fabs(bx) / sqL;
+  SRs = SM + fabs(bx) / sqR;
+  ULs[0] = rLs; ULs[1] = rLs * SM; ULs[2] = rLs * vLs[0]; ULs[3] = rLs * vLs[1]; ULs[4] = ELs; ULs[5] = bx; ULs[6] = bLs[0]; ULs[7] = bLs[1];
+  URs[0] = rRs; URs[1] = rRs * SM; URs[2] = rRs * vRs[0]; URs[3] = rRs * vRs[1]; URs[4] = ERs; URs[5] = bx; URs[6] = bRs[0]; URs[7] = bRs[1];
+  for (c = 0; c < 5; ++c) { ULr[c] = UL[c]; URr[c] = UR[c]; FLr[c] = FL[c]; FRr[c] = FR[c]; }
+  ULr[5] = bx; ULr[6] = bL[1]; ULr[7] = bL[2];
+  URr[5] = bx; URr[6] = bR[1]; URr[7] = bR[2];
+  FLr[5] = 0.0; FRr[5] = 0.0;
+  FLr[6] = FL[6] / s4; FLr[7] = FL[7] / s4; FRr[6] = FR[6] / s4; FRr[7] = FR[7] / s4;
+  if (SLs >= 0.0) {
+    for (c = 0; c < 8; ++c) F[c] = FLr[c] + *SL * (ULs[c] - ULr[c]);
+  } else if (SRs <= 0.0) {
+    for (c = 0; c < 8; ++c) F[c] = FRr[c] + *SR * (URs[c] - URr[c]);
+  } else {
+    double sg = bx > 0 ? 1.0 : (bx < 0 ? -1.0 : 0.0), is = 1.0 / (sqL + sqR);
+    double vy = (sqL * vLs[0] + sqR * vRs[0] + (bRs[0] - bLs[0]) * sg) * is;
+    double vz = (sqL * vLs[1] + sqR * vRs[1] + (bRs[1] - bLs[1]) * sg) * is;
+    double by = (sqL * bRs[0] + sqR * bLs[0] + sqL * sqR * (vRs[0] - vLs[0]) * sg) * is;
+    double bz = (sqL * bRs[1] + sqR * bLs[1] + sqL * sqR * (vRs[1] - vLs[1]) * sg) * is;
+    double vbss = SM * bx + vy * by + vz * bz, Uss[8];
+    if (SM >= 0.0) {
+      Uss[0] = rLs; Uss[1] = rLs * SM; Uss[2] = rLs * vy; Uss[3] = rLs * vz;
+      Uss[4] = ELs - sqL * (vbLs - vbss) * sg; Uss[5] = bx; Uss[6] = by; Uss[7] = bz;
+      for (c = 0; c < 8; ++c) F[c] = FLr[c] + *SL * (ULs[c] - ULr[c]) + SLs * (Uss[c] - ULs[c]);
+    } else {
+      Uss[0] = rRs; Uss[1] = rRs * SM; Uss[2] = rRs * vy; Uss[3] = rRs * vz;
+      Uss[4] = ERs + sqR * (vbRs - vbss) * sg; Uss[5] = bx; Uss[6] = by; Uss[7] = bz;
+      for (c = 0; c < 8; ++c) F[c] = FRr[c] + *SR * (URs[c] - URr[c]) + SRs * (Uss[c] - URs[c]);
+    }
+  }
+  for (c = 0; c < 5; ++c) Fs[c] = F[c];
+  Fs[5] = 0.0;
+  Fs[6] = F[6] * s4;
+  Fs[7] = F[7] * s4;
+  *entF = Fs[0] >= 0.0 ? Fs[0] * (entL / VL[0]) : Fs[0] * (entR / VR[0]);
+}
+/* Jiang & Shu (1996) WENO5, left state at the right edge of v2 */
+double or_weno5(double v0, double v1, double v2, double v3, double v4) {
+  double e = 1e-6;
+  double b0 = 13.0 / 12.0 * (v0 - 2 * v1 + v2) * (v0 - 2 * v1 + v2) + 0.25 * (v0 - 4 * v1 + 3 * v2) * (v0 - 4 * v1 + 3 * v2);
+  double b1 = 13.0 / 12.0 * (v1 - 2 * v2 + v3) * (v1 - 2 * v2 + v3) + 0.25 * (v1 - v3) * (v1 - v3);
+  double b2 = 13.0 / 12.0 * (v2 - 2 * v3 + v4) * (v2 - 2 * v3 + v4) + 0.25 * (3 * v2 - 4 * v3 + v4) * (3 * v2 - 4 * v3 + v4);
+  double a0 = 0.1 / ((e + b0) * (e + b0)), a1 = 0.6 / ((e + b1) * (e + b1)), a2 = 0.3 / ((e + b2) * (e + b2));
+  double q0 = (2 * v0 - 7 * v1 + 11 * v2) / 6.0, q1 = (-v1 + 5 * v2 + 2 * v3) / 6.0, q2 = (2 * v2 + 5 * v3 - v4) / 6.0;
+  return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+}
+/* Two_D_RS_Ez_4State_HLL (630_2d_rs_ez_two_d_rs_ez_4state_hll.f:10-372);
+   s4 = (S_R,S_L,S_U,S_D), ez4 = (RU,LU,LD,RD), bn2 = (U,D), by2 = (R,L) */
+double or_emf4_hll(const double* s4, double ct, double st, const double* ez, const double* bn, const double* by) {
+  double SR = s4[0], SL = s4[1], SU = s4[2], SD = s4[3];
+  double ERU = ez[0], ELU = ez[1], ELD = ez[2], ERD = ez[3], BnU = bn[0], BnD = bn[1], ByR = by[0], ByL = by[1];
+  double ox = 1.0 / (SR - SL), oy = 1.0 / (SU - SD), csc = 1.0 / st, cot = ct / st;
+  double BxRU = (BnU + ByR * ct) / st, BxLU = (BnU + ByL * ct) / st, BxRD = (BnD + ByR * ct) / st, BxLD = (BnD + ByL * ct) / st;
+  double BtRU = BxRU * ct + ByR * st, BtLU = BxLU * ct + ByL * st, BtRD = BxRD * ct + ByR * st, BtLD = BxLD * ct + ByL * st;
+  double BSxL, BSyL, ESzL, BSxR, BSyR, ESzR, BSn, BSt, BSxD, BSyD, ESzD, BSxU, BSyU, ESzU, xc, yc, BSSx, BSSy, E1, E2;
+  if (SL >= 0.0 && SD >= 0.0) return ELD;
+  if (SR <= 0.0 && SD >= 0.0) return ERD;
+  if (SR <= 0.0 && SU <= 0.0) return ERU;
+  if (SL >= 0.0 && SU <= 0.0) return ELU;
+  if (SL >= 0.0) return (SU * ELD - SD * ELU) * oy + SU * SD * (BxLU - BxLD) * oy;
+  if (SR <= 0.0) return (SU * ERD - SD * ERU) * oy + SU * SD * (BxRU - BxRD) * oy;
+  if (SD >= 0.0) return (SR * ELD - SL * ERD) * ox - SR * SL * (BtRD - BtLD) * ox;
+  if (SU <= 0.0) return (SR * ELU - SL * ERU) * ox - SR * SL * (BtRU - BtLU) * ox;
+  BSxL = (SU * BxLU - SD * BxLD) * oy - (ELU - ELD) * oy; BSyL = ByL;
+  ESzL = (SU * ELD - SD * ELU) * oy + SU * SD * (BxLU - BxLD) * oy;
+  BSxR = (SU * BxRU - SD * BxRD) * oy - (ERU - ERD) * oy; BSyR = ByR;
+  ESzR = (SU * ERD - SD * ERU) * oy + SU * SD * (BxRU - BxRD) * oy;
+  BSn = BnD; BSt = (SR * BtRD - SL * BtLD) * ox + (ERD - ELD) * ox;
+  BSxD = BSn * st + BSt * ct; BSyD = -BSn * ct + BSt * st;
+  ESzD = (SR * ELD - SL * ERD) * ox - SR * SL * (BtRD - BtLD) * ox;
+  BSn = BnU; BSt = (SR * BtRU - SL * BtLU) * ox + (ERU - ELU) * ox;
+  BSxU = BSn * st + BSt * ct; BSyU = -BSn * ct + BSt * st;
+  ESzU = (SR * ELU - SL * ERU) * ox - SR * SL * (BtRU - BtLU) * ox;
+  xc = 0.5 * ((SU + SD) * ct + (SR + SL)) / st;
+  yc = 0.5 * (SU + SD);
+  BSSx = -0.5 * ((-ct * ESzR - SR * BSxR) * ox + (ESzU - SU * BSxU) * oy + (ct * ESzL + SL * BSxL) * ox + (-ESzD + SD * BSxD) * oy);
+  BSSy = -0.5 * ((-st * ESzR - SR * BSyR) * ox + (-SU * BSyU) * oy + (st * ESzL + SL * BSyL) * ox + (SD * BSyD) * oy);
+  E1 = -xc * BSSy - 0.5 * (csc * (-st * ESzR - SR * BSyR) + cot * (-SU * BSyU) - csc * (st * ESzL + SL * BSyL) - cot * (SD * BSyD));
+  E2 = yc * BSSx + 0.5 * ((ESzU - SU * BSxU) - (-ESzD + SD * BSxD));
+  return 0.5 * (E1 + E2);
+}
+/* Two_D_RS_Ez_3State_LLF (633_2d_rs_ez_two_d_rs_ez_3state_llf.f:59-84) */
+double or_emf3_llf(double S, const double* ez, const double* bn) {
+  double Sq3 = sqrt(3.0), ob3 = 1.0 / 3.0;
+  return ob3 * (ez[0] + ez[1] + ez[2]) - (4.0 * Sq3 * S / 9.0) * (bn[0] + bn[1] + bn[2]);
+}
+
+/* ------------------------------------------------------------- stage */
+static void prims_of(const or_grid* G, const double* U, const double* ent, int s, int64_t c, double* V) {
+  double u[8], e = ent[s * G->cpp + c];
+  int q;
+  for (q = 0; q < 8; ++q) u[q] = UU(U, G, s, q, c);
+  or_cons_to_prim(G->c.gamma, G->c.rhofloor, G->c.prsfloor, &e, u, V);
+  V[8] = e;
+}
+
+/* reconstruction + face Riemann solve for one face (450make_facial_fluxes.f:507-1152) */
+static void one_face(const or_grid* G, int dir, int s, int i, int j, int k, const double* Vc, int64_t stride,
+                     int64_t cl, double bface, double* out, double* SL, double* SR) {
+  double VL[9], VR[9], A[3], Bv[3], Cv[3], area, wl[8], wr[8], Fs[8], eF;
+  int q, c;
+  int64_t t = T2(G, s, i, j);
+  for (q = 0; q < 9; ++q) {
+    const double* p = Vc + q * G->cpp + cl;
+    if (G->c.recon == 1) {
+      VL[q] = or_weno5(p[-2 * stride], p[-stride], p[0], p[stride], p[2 * stride]);
+      VR[q] = or_weno5(p[3 * stride], p[2 * stride], p[stride], p[0], p[-stride]);
+    } else {
+      VL[q] = p[0];
+      VR[q] = p[stride];
+    }
+  }
+  if (!(VL[0] > 0) || !(VL[4] > 0) || !(VL[8] > 0)) for (q = 0; q < 9; ++q) VL[q] = Vc[q * G->cpp + cl];
+  if (!(VR[0] > 0) || !(VR[4] > 0) || !(VR[8] > 0)) for (q = 0; q < 9; ++q) VR[q] = Vc[q * G->cpp + cl + stride];
+  if (dir == 0) {
+    for (c = 0; c < 3; ++c) { A[c] = G->xn[c][t]; Cv[c] = G->xc[c][t]; }
+    cross3(Cv, A, Bv);
+    area = xarea(G, s, i, j, k);
+  } else if (dir == 1) {
+    for (c = 0; c < 3; ++c) { A[c] = G->yn[c][t]; Cv[c] = G->yc[c][t]; }
+    cross3(Cv, A, Bv);
+    area = yarea(G, s, i, j, k);
+  } else {
+    for (c = 0; c < 3; ++c) { A[c] = G->zs[c][t] / G->zsn[t]; Bv[c] = G->zt[c][t]; }
+    cross3(A, Bv, Cv);
+    area = zarea(G, s, i, j, k);
+  }
+  wl[0] = VL[0]; wr[0] = VR[0]; wl[4] = VL[4]; wr[4] = VR[4];
+  wl[1] = dot3(A, VL + 1); wl[2] = dot3(Bv, VL + 1); wl[3] = dot3(Cv, VL + 1);
+  wr[1] = dot3(A, VR + 1); wr[2] = dot3(Bv, VR + 1); wr[3] = dot3(Cv, VR + 1);
+  wl[5] = bface; wr[5] = bface;
+  wl[6] = dot3(Bv, VL + 5); wl[7] = dot3(Cv, VL + 5);
+  wr[6] = dot3(Bv, VR + 5); wr[7] = dot3(Cv, VR + 5);
+  if (G->c.riemann == 1) or_hlld(G->c.gamma, G->c.rhofloor, G->c.prsfloor, wl, wr, VL[8], VR[8], Fs, &eF, SL, SR);
+  else or_hll(G->c.gamma, G->c.rhofloor, G->c.prsfloor, wl, wr, VL[8], VR[8], Fs, &eF, SL, SR);
+  out[0] = Fs[0]; out[4] = Fs[4];
+  for (c = 0; c < 3; ++c) {
+    out[1 + c] = A[c] * Fs[1] + Bv[c] * Fs[2] + Cv[c] * Fs[3];
+    out[5 + c] = A[c] * Fs[5] + Bv[c] * Fs[6] + Cv[c] * Fs[7];
+  }
+  out[8] = eF;
+  for (q = 0; q < 9; ++q) out[q] *= area;
+}
+
+static double* prims_all(const or_grid* G, const double* U, const double* ent) {
+  double* V = malloc(sizeof(double) * 9 * 6 * G->cpp);
+  int s, i, j, k, q, g = G->g;
+  for (s = 0; s < 6; ++s)
+    for (k = 1 - g; k <= G->NR + g; ++k)
+      for (j = 1 - g; j <= G->N + g; ++j)
+        for (i = 1 - g; i <= G->N + g; ++i) {
+          int64_t c = CELL(G, i, j, k);
+          double v[9];
+          prims_of(G, U, ent, s, c, v);
+          for (q = 0; q < 9; ++q) V[((int64_t)s * 9 + q) * G->cpp + c] = v[q];
+        }
+  return V;
+}
+
+static void fluxes_from_V(const or_grid* G, const double* V, const double* Bx, const double* By, const double* Bz,
+                          double** F, double** S) {
+  int N = G->N, NR = G->NR, dir, s, i, j, k, q;
+  int lo[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  int hi[3][3] = {{N, N + 1, NR + 1}, {N + 1, N, NR + 1}, {N + 1, N + 1, NR}};
+  int64_t strides[3] = {1, G->nc, (int64_t)G->nc * G->nc};
+  const double* fb[3] = {Bx, By, Bz};
+  for (dir = 0; dir < 3; ++dir)
+    for (s = 0; s < 6; ++s)
+      for (k = lo[dir][2]; k <= hi[dir][2]; ++k)
+        for (j = lo[dir][1]; j <= hi[dir][1]; ++j)
+          for (i = lo[dir][0]; i <= hi[dir][0]; ++i) {
+            double out[9], sl, sr;
+            int64_t f = FACE(G, i, j, k);
+            one_face(G, dir, s, i, j, k, V + (int64_t)s * 9 * G->cpp, strides[dir], CELL(G, i, j, k),
+                     fb[dir][s * G->fpp + f], out, &sl, &sr);
+            for (q = 0; q < 9; ++q) F[dir][((int64_t)s * 9 + q) * G->fpp + f] = out[q];
+            S[dir][((int64_t)s * 2) * G->fpp + f] = sl;
+            S[dir][((int64_t)s * 2 + 1) * G->fpp + f] = sr;
+          }
+}
+
+void or_fluxes(const or_grid* G, const double* U, const double* ent, const double* Bx, const double* By,
+               const double* Bz, double** F, double** S) {
+  double* V = prims_all(G, U, ent);
+  fluxes_from_V(G, V, Bx, By, Bz, F, S);
+  free(V);
+}
+
+static void cellE(const or_grid* G, const double* V, int s, int64_t c, double* E) {
+  const double* p = V + (int64_t)s * 9 * G->cpp + c;
+  double vx = p[G->cpp], vy = p[2 * G->cpp], vz = p[3 * G->cpp], bx = p[5 * G->cpp], by = p[6 * G->cpp], bz = p[7 * G->cpp];
+  E[0] = -(vy * bz - vz * by);
+  E[1] = -(vz * bx - vx * bz);
+  E[2] = -(vx * by - vy * bx);
+}
+static double spd(const or_grid* G, double* const* S, int dir, int s, int w, int i, int j, int k) {
+  return S[dir][((int64_t)s * 2 + w) * G->fpp + FACE(G, i, j, k)];
+}
+static double emf_quad(const or_grid* G, const double* V, int s, const int64_t* cells, const double* n3, double S_R,
+                       double S_L, double S_U, double S_D, double ct, double st, double BnU, double BnD, double ByR,
+                       double ByL) {
+  /* cells: LD, RD, RU, LU.  460make_edge_fields_p1.f:3530-3700 */
+  double E[4][3], z[4], s4[4], ez4[4], bn2[2], by2[2];
+  int q;
+  for (q = 0; q < 4; ++q) cellE(G, V, s, cells[q], E[q]);
+  if (G->c.emf == 0) {
+    double ex = (E[0][0] + E[1][0] + E[2][0] + E[3][0]) * 0.25, ey = (E[0][1] + E[1][1] + E[2][1] + E[3][1]) * 0.25,
+           ez = (E[0][2] + E[1][2] + E[2][2] + E[3][2]) * 0.25, m;
+    z[0] = z[1] = z[2] = z[3] = ex * n3[0] + ey * n3[1] + ez * n3[2];
+    m = fmax(fmax(fabs(S_R), fabs(S_L)), fmax(fabs(S_U), fabs(S_D)));
+    S_R = m; S_L = -m; S_U = m; S_D = -m;
+  } else {
+    for (q = 0; q < 4; ++q) z[q] = dot3(E[q], n3);
+  }
+  s4[0] = S_R; s4[1] = S_L; s4[2] = S_U; s4[3] = S_D;
+  ez4[0] = z[2]; ez4[1] = z[3]; ez4[2] = z[0]; ez4[3] = z[1];
+  bn2[0] = BnU; bn2[1] = BnD; by2[0] = ByR; by2[1] = ByL;
+  return or_emf4_hll(s4, ct, st, ez4, bn2, by2);
+}
+
+static void emf_from_V(const or_grid* G, const double* V, const double* Bx, const double* By, const double* Bz,
+                       double* const* S, double** E) {
+  int N = G->N, NR = G->NR, s, i, j, k;
+  for (s = 0; s < 6; ++s) {
+    const double *bx = Bx + s * G->fpp, *by = By + s * G->fpp, *bz = Bz + s * G->fpp;
+    /* z-edges */
+    for (k = 1; k <= NR; ++k)
+      for (j = 0; j <= N; ++j)
+        for (i = 0; i <= N; ++i) {
+          int64_t t = T2(G, s, i, j);
+          double n3[3] = {G->u[0][t], G->u[1][t], G->u[2][t]}, e, dr = R(G, k) - R(G, k - 1);
+          int corner = (i == 0 || i == N) && (j == 0 || j == N);
+          if (corner) {
+            /* tri-corner, 460make_edge_fields_p1.f:3725-4540 and 633 */
+            int c[3][2];
+            double bn, Sx, Sy, E1[3], E2[3], E3[3], ez3[3], bn3[3] = {0, 0, 0}, em;
+            int sxi, sxj, syi, syj;
+            if (i == 0 && j == 0) {
+              c[0][0] = 1; c[0][1] = 1; c[1][0] = 0; c[1][1] = 1; c[2][0] = 1; c[2][1] = 0;
+              bn = -by[FACE(G, 1, 0, k)] + bx[FACE(G, 0, 1, k)] + by[FACE(G, 0, 0, k)];
+              sxi = 0; sxj = 1; syi = 1; syj = 0;
+            } else if (i == N && j == 0) {
+              c[0][0] = N; c[0][1] = 0; c[1][0] = N; c[1][1] = 1; c[2][0] = N + 1; c[2][1] = 1;
+              bn = bx[FACE(G, N, 1, k)] + by[FACE(G, N, 0, k)] - by[FACE(G, N + 1, 0, k)];
+              sxi = N; sxj = 1; syi = N; syj = 0;
+            } else if (i == N && j == N) {
+              c[0][0] = N; c[0][1] = N; c[1][0] = N + 1; c[1][1] = N; c[2][0] = N; c[2][1] = N + 1;
+              bn = by[FACE(G, N, N, k)] - bx[FACE(G, N, N, k)] - by[FACE(G, N + 1, N, k)];
+              sxi = N; sxj = N; syi = N; syj = N;
+            } else {
+              c[0][0] = 0; c[0][1] = N; c[1][0] = 1; c[1][1] = N; c[2][0] = 1; c[2][1] = N + 1;
+              bn = -by[FACE(G, 1, N, k)] - bx[FACE(G, 0, N, k)] + by[FACE(G, 0, N, k)];
+              sxi = 0; sxj = N; syi = 1; syj = N;
+            }
+            Sx = fmax(fabs(spd(G, S, 0, s, 0, sxi, sxj, k)), fabs(spd(G, S, 0, s, 1, sxi, sxj, k)));
+            Sy = fmax(fabs(spd(G, S, 1, s, 0, syi, syj, k)), fabs(spd(G, S, 1, s, 1, syi, syj, k)));
+            cellE(G, V, s, CELL(G, c[0][0], c[0][1], k), E1);
+            cellE(G, V, s, CELL(G, c[1][0], c[1][1], k), E2);
+            cellE(G, V, s, CELL(G, c[2][0], c[2][1], k), E3);
+            {
+              double ob3 = 1.0 / 3.0, ex = (E1[0] + E2[0] + E3[0]) * ob3, ey = (E1[1] + E2[1] + E3[1]) * ob3,
+                     ez = (E1[2] + E2[2] + E3[2]) * ob3;
+              em = ex * n3[0] + ey * n3[1] + ez * n3[2];
+            }
+            ez3[0] = ez3[1] = ez3[2] = em;
+            bn3[0] = bn;
+            e = or_emf3_llf(fmax(Sx, Sy), ez3, bn3);
+          } else {
+            int64_t cells[4] = {CELL(G, i, j, k), CELL(G, i + 1, j, k), CELL(G, i + 1, j + 1, k), CELL(G, i, j + 1, k)};
+            double SRv = fmax(spd(G, S, 0, s, 1, i, j + 1, k), spd(G, S, 0, s, 1, i, j, k));
+            double SLv = fmin(spd(G, S, 0, s, 0, i, j + 1, k), spd(G, S, 0, s, 0, i, j, k));
+            double SUv = fmax(spd(G, S, 1, s, 1, i + 1, j, k), spd(G, S, 1, s, 1, i, j, k));
+            double SDv = fmin(spd(G, S, 1, s, 0, i + 1, j, k), spd(G, S, 1, s, 0, i, j, k));
+            e = emf_quad(G, V, s, cells, n3, SRv, SLv, SUv, SDv, G->ecos[t], G->esin[t], bx[FACE(G, i, j + 1, k)],
+                         bx[FACE(G, i, j, k)], by[FACE(G, i + 1, j, k)], by[FACE(G, i, j, k)]);
+          }
+          E[2][s * G->fpp + FACE(G, i, j, k)] = e * dr;
+        }
+    /* x-edges (460:393-1493) */
+    for (k = 0; k <= NR; ++k)
+      for (j = 0; j <= N; ++j)
+        for (i = 1; i <= N; ++i) {
+          int64_t t = T2(G, s, i, j), f = s * G->fpp + FACE(G, i, j, k);
+          double n3[3] = {G->xet[0][t], G->xet[1][t], G->xet[2][t]};
+          int64_t cells[4] = {CELL(G, i, j, k), CELL(G, i, j + 1, k), CELL(G, i, j + 1, k + 1), CELL(G, i, j, k + 1)};
+          if (G->c.radial_bc == 1 && (k == 0 || k == NR)) { E[0][f] = 0.0; continue; }
+          E[0][f] = emf_quad(G, V, s, cells, n3, fmax(spd(G, S, 1, s, 1, i, j, k + 1), spd(G, S, 1, s, 1, i, j, k)),
+                             fmin(spd(G, S, 1, s, 0, i, j, k + 1), spd(G, S, 1, s, 0, i, j, k)),
+                             fmax(spd(G, S, 2, s, 1, i, j + 1, k), spd(G, S, 2, s, 1, i, j, k)),
+                             fmin(spd(G, S, 2, s, 0, i, j + 1, k), spd(G, S, 2, s, 0, i, j, k)), 0.0, 1.0,
+                             by[FACE(G, i, j, k + 1)], by[FACE(G, i, j, k)], bz[FACE(G, i, j + 1, k)], bz[FACE(G, i, j, k)]) *
+                    (R(G, k) * G->thx[t]);
+        }
+    /* y-edges (460:1497-2587) */
+    for (k = 0; k <= NR; ++k)
+      for (j = 1; j <= N; ++j)
+        for (i = 0; i <= N; ++i) {
+          int64_t t = T2(G, s, i, j), f = s * G->fpp + FACE(G, i, j, k);
+          double n3[3] = {G->yet[0][t], G->yet[1][t], G->yet[2][t]};
+          int64_t cells[4] = {CELL(G, i, j, k), CELL(G, i, j, k + 1), CELL(G, i + 1, j, k + 1), CELL(G, i + 1, j, k)};
+          if (G->c.radial_bc == 1 && (k == 0 || k == NR)) { E[1][f] = 0.0; continue; }
+          E[1][f] = emf_quad(G, V, s, cells, n3, fmax(spd(G, S, 2, s, 1, i + 1, j, k), spd(G, S, 2, s, 1, i, j, k)),
+                             fmin(spd(G, S, 2, s, 0, i + 1, j, k), spd(G, S, 2, s, 0, i, j, k)),
+                             fmax(spd(G, S, 0, s, 1, i, j, k), spd(G, S, 0, s, 1, i, j, k + 1)),
+                             fmin(spd(G, S, 0, s, 0, i, j, k), spd(G, S, 0, s, 0, i, j, k + 1)), 0.0, 1.0,
+                             bz[FACE(G, i + 1, j, k)], bz[FACE(G, i, j, k)], bx[FACE(G, i, j, k + 1)], bx[FACE(G, i, j, k)]) *
+                    (R(G, k) * G->thy[t]);
+        }
+  }
+}
+
+void or_emf(const or_grid* G, const double* U, const double* ent, const double* Bx, const double* By, const double* Bz,
+            double** S, double** E) {
+  double* V = prims_all(G, U, ent);
+  emf_from_V(G, V, Bx, By, Bz, S, E);
+  free(V);
+}
+
+static void weights(int order, int irk, double* wa, double* wb) {
+  if (order == 3) {
+    if (irk == 1) { *wa = 0.0; *wb = 1.0; }
+    else if (irk == 2) { *wa = 0.75; *wb = 0.25; }
+    else { *wa = 1.0 / 3.0; *wb = 2.0 / 3.0; }
+  } else {
+    if (irk == 1) { *wa = 0.0; *wb = 1.0; } else { *wa = 0.5; *wb = 0.5; }
+  }
+}
+
+/* UPDATE_IMEX_RK_STEP_P1 (484:483-497, 547-593, CT 646-908) and
+   CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:4) */
+void or_update(const or_grid* G, int irk, double dt, const double* const* in, const double* const* u0,
+               double* const* out, double** F, double** E) {
+  int N = G->N, NR = G->NR, s, i, j, k, q;
+  double wa, wb;
+  static const int comps[6] = {0, 1, 2, 3, 4, 8};
+  weights(G->c.rk_order, irk, &wa, &wb);
+  for (s = 0; s < 6; ++s)
+    for (k = 1; k <= NR; ++k)
+      for (j = 1; j <= N; ++j)
+        for (i = 1; i <= N; ++i) {
+          int64_t c = CELL(G, i, j, k);
+          double ov = 1.0 / vol_of(G, s, i, j, k), L[6];
+          int64_t fp = FACE(G, i, j, k), fxm = FACE(G, i - 1, j, k), fym = FACE(G, i, j - 1, k), fzm = FACE(G, i, j, k - 1);
+          for (q = 0; q < 6; ++q) {
+            int64_t o = ((int64_t)s * 9 + comps[q]) * G->fpp;
+            L[q] = -ov * ((F[0][o + fp] - F[0][o + fxm]) + (F[1][o + fp] - F[1][o + fym]) + (F[2][o + fp] - F[2][o + fzm]));
+          }
+          if (G->c.grav != 0.0) { /* 470build_non_stiff_sources.f:150-250, g = grav / r^2 */
+            int64_t t = T2(G, s, i, j);
+            double rc = 0.5 * (R(G, k - 1) + R(G, k)), n[3], gr = -G->c.grav / (rc * rc), rho, mr;
+            for (q = 0; q < 3; ++q) n[q] = G->zs[q][t] / G->zsn[t];
+            rho = UU(in[0], G, s, 0, c);
+            mr = UU(in[0], G, s, 1, c) * n[0] + UU(in[0], G, s, 2, c) * n[1] + UU(in[0], G, s, 3, c) * n[2];
+            for (q = 0; q < 3; ++q) L[1 + q] += rho * gr * n[q];
+            L[4] += mr * gr;
+          }
+          for (q = 0; q < 5; ++q) {
+            double un = UU(in[0], G, s, q, c) + dt * L[q];
+            UU(out[0], G, s, q, c) = wa != 0.0 ? wa * UU(u0[0], G, s, q, c) + wb * un : wb * un;
+          }
+          {
+            double en = in[1][s * G->cpp + c] + dt * L[5];
+            out[1][s * G->cpp + c] = wa != 0.0 ? wa * u0[1][s * G->cpp + c] + wb * en : wb * en;
+          }
+        }
+  for (s = 0; s < 6; ++s) {
+    const double *Ex = E[0] + s * G->fpp, *Ey = E[1] + s * G->fpp, *Ez = E[2] + s * G->fpp;
+    for (k = 1; k <= NR; ++k)
+      for (j = 1; j <= N; ++j)
+        for (i = 0; i <= N; ++i) {
+          int64_t f = FACE(G, i, j, k);
+          double circ = Ez[f] - Ez[FACE(G, i, j - 1, k)] - Ey[f] + Ey[FACE(G, i, j, k - 1)];
+          double bn = in[2][s * G->fpp + f] - dt * circ / xarea(G, s, i, j, k);
+          out[2][s * G->fpp + f] = wa != 0.0 ? wa * u0[2][s * G->fpp + f] + wb * bn : wb * bn;
+        }
+    for (k = 1; k <= NR; ++k)
+      for (j = 0; j <= N; ++j)
+        for (i = 1; i <= N; ++i) {
+          int64_t f = FACE(G, i, j, k);
+          double circ = -Ez[f] + Ez[FACE(G, i - 1, j, k)] + Ex[f] - Ex[FACE(G, i, j, k - 1)];
+          double bn = in[3][s * G->fpp + f] - dt * circ / yarea(G, s, i, j, k);
+          out[3][s * G->fpp + f] = wa != 0.0 ? wa * u0[3][s * G->fpp + f] + wb * bn : wb * bn;
+        }
+    for (k = 0; k <= NR; ++k)
+      for (j = 1; j <= N; ++j)
+        for (i = 1; i <= N; ++i) {
+          int64_t f = FACE(G, i, j, k);
+          double circ = Ey[f] - Ey[FACE(G, i - 1, j, k)] - Ex[f] + Ex[FACE(G, i, j - 1, k)];
+          double bn = in[4][s * G->fpp + f] - dt * circ / zarea(G, s, i, j, k);
+          out[4][s * G->fpp + f] = wa != 0.0 ? wa * u0[4][s * G->fpp + f] + wb * bn : wb * bn;
+        }
+  }
+  /* cleanup: zone B from faces, then CONS_TO_PRIM floors */
+  for (s = 0; s < 6; ++s)
+    for (k = 1; k <= NR; ++k)
+      for (j = 1; j <= N; ++j)
+        for (i = 1; i <= N; ++i) {
+          int64_t c = CELL(G, i, j, k), t = T2(G, s, i, j);
+          double b[3], u[8], v[8], e;
+          b[0] = 0.5 * (out[2][s * G->fpp + FACE(G, i - 1, j, k)] + out[2][s * G->fpp + FACE(G, i, j, k)]);
+          b[1] = 0.5 * (out[3][s * G->fpp + FACE(G, i, j - 1, k)] + out[3][s * G->fpp + FACE(G, i, j, k)]);
+          b[2] = 0.5 * (out[4][s * G->fpp + FACE(G, i, j, k - 1)] + out[4][s * G->fpp + FACE(G, i, j, k)]);
+          for (q = 0; q < 8; ++q) u[q] = UU(out[0], G, s, q, c);
+          for (q = 0; q < 3; ++q) u[5 + q] = G->mi[3 * q][t] * b[0] + G->mi[3 * q + 1][t] * b[1] + G->mi[3 * q + 2][t] * b[2];
+          e = out[1][s * G->cpp + c];
+          or_cons_to_prim(G->c.gamma, G->c.rhofloor, G->c.prsfloor, &e, u, v);
+          for (q = 0; q < 8; ++q) UU(out[0], G, s, q, c) = u[q];
+          out[1][s * G->cpp + c] = e;
+        }
+}
+
+/* EVALUATE_TIMESTEP (490evaluate_timestep.f:118-278) */
+double or_cfl(const or_grid* G, const double* U, const double* ent, double dt_prev) {
+  int s, i, j, k;
+  double m = 1e300;
+  for (s = 0; s < 6; ++s)
+    for (k = 1; k <= G->NR; ++k)
+      for (j = 1; j <= G->N; ++j)
+        for (i = 1; i <= G->N; ++i) {
+          double v[9], b2, ms, a21, L, r1 = R(G, k), r0 = R(G, k - 1), d;
+          prims_of(G, U, ent, s, CELL(G, i, j, k), v);
+          b2 = v[5] * v[5] + v[6] * v[6] + v[7] * v[7];
+          ms = sqrt((G->c.gamma * v[4] + OB4PI * b2) / v[0]);
+          a21 = sqrt(v[1] * v[1] + v[2] * v[2] + v[3] * v[3]) + ms;
+          L = fmin(r1 - r0, r0 * G->chmin[T2(G, s, i, j)]);
+          d = G->c.cfl * L / a21;
+          if (d < m) m = d;
+        }
+  return dt_prev > 0.0 ? fmin(1.1 * dt_prev, m) : m;
+}
+
+static double** alloc_set(const or_grid* G, int n, int64_t per) {
+  double** p = malloc(sizeof(double*) * n);
+  int q;
+  for (q = 0; q < n; ++q) p[q] = calloc((size_t)per, sizeof(double));
+  return p;
+}
+static void free_set(double** p, int n) {
+  int q;
+  for (q = 0; q < n; ++q) free(p[q]);
+  free(p);
+}
+
+void or_stage(const or_grid* G, int irk, double dt, double* const* in, double* const* u0, double* const* out) {
+  double **F = alloc_set(G, 3, 9 * 6 * G->fpp), **S = alloc_set(G, 3, 2 * 6 * G->fpp), **E = alloc_set(G, 3, 6 * G->fpp);
+  double* V;
+  or_halo(G, in[0], in[1], in[2], in[3], in[4]);
+  V = prims_all(G, in[0], in[1]);
+  fluxes_from_V(G, V, in[2], in[3], in[4], F, S);
+  emf_from_V(G, V, in[2], in[3], in[4], S, E);
+  free(V);
+  or_update(G, irk, dt, (const double* const*)in, (const double* const*)u0, out, F, E);
+  free_set(F, 3); free_set(S, 3); free_set(E, 3);
+}
+
+void or_advance(const or_grid* G, int nsteps, double* U, double* ent, double* Bx, double* By, double* Bz, double* dt) {
+  int64_t cs = 6 * G->cpp, fs = 6 * G->fpp;
+  double* A[5] = {U, ent, Bx, By, Bz};
+  double* B[5];
+  int n, q, irk;
+  B[0] = malloc(sizeof(double) * cs * 8); B[1] = malloc(sizeof(double) * cs);
+  for (q = 2; q < 5; ++q) B[q] = malloc(sizeof(double) * fs);
+  memcpy(B[0], U, sizeof(double) * cs * 8); memcpy(B[1], ent, sizeof(double) * cs);
+  for (q = 2; q < 5; ++q) memcpy(B[q], A[q], sizeof(double) * fs);
+  for (n = 0; n < nsteps; ++n) {
+    for (irk = 1; irk <= G->c.rk_order; ++irk) {
+      double* const* in = irk == 1 ? A : B;
+      double* const* out = (irk == G->c.rk_order) ? A : B;
+      or_stage(G, irk, *dt, in, A, out);
+    }
+    *dt = or_cfl(G, U, ent, *dt);
+  }
+  for (q = 0; q < 5; ++q) free(B[q]);
+}

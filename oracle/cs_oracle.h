@@ -1,0 +1,70 @@
+/* cs_oracle.h -- CPU oracle for the CubeSphere cell-update path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls
+ * this; only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * use it, as the checker.
+ *
+ * Plain C restatement of the reference's algorithm for this path; every
+ * function cites the reference file:line it follows.  Arrays use the same
+ * Fortran patch layout as include/cubesphere_hip.h.
+ */
+#ifndef CS_ORACLE_H
+#define CS_ORACLE_H
+#include <stdint.h>
+
+typedef struct or_config { /* field-for-field the same as cs_config */
+  int32_t ntp, nr, nghost, recon, riemann, emf, precision, rk_order, radial_bc, radial_mesh;
+  double rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav;
+  int32_t imex, reserved;
+} or_config;
+
+typedef struct or_grid or_grid;
+
+or_grid* or_grid_new(const or_config* c);
+void or_grid_free(or_grid* g);
+int or_grid_dims(const or_grid* g, int64_t* out4); /* cpp, fpp, t2, nmap */
+
+/* 201/203: unit lattice (-g:N+g, -g:N+g, 3, 6), reference corner blocks = 0 */
+void or_unit_lattice(int N, int g, double* coords);
+/* 202: rb(-g:nr+g) */
+void or_radial_mesh(const or_config* c, double* rb);
+
+/* metric (same layout as cs_build_metric) */
+void or_metric(const or_grid* g, double* vol, double* xarea, double* yarea, double* zarea);
+
+/* ghost exchange (maps found by geometric matching) + corner fill + radial bc */
+void or_halo(const or_grid* g, double* U, double* ent, double* Bx, double* By, double* Bz);
+/* number of ghost-map entries whose geometric match failed (0 expected) */
+int or_map_failures(const or_grid* g);
+
+/* face fluxes (9 comps * area, global frame) and speeds for dir 0..2 */
+void or_fluxes(const or_grid* g, const double* U, const double* ent, const double* Bx,
+               const double* By, const double* Bz, double** F, double** S);
+void or_emf(const or_grid* g, const double* U, const double* ent, const double* Bx,
+            const double* By, const double* Bz, double** S, double** E);
+void or_update(const or_grid* g, int irk, double dt, const double* const* in,
+               const double* const* u0, double* const* out, double** F, double** E);
+double or_cfl(const or_grid* g, const double* U, const double* ent, double dt_prev);
+
+/* whole steps on a state (U, ent, Bx, By, Bz); *dt is updated by the CFL */
+void or_advance(const or_grid* g, int nsteps, double* U, double* ent, double* Bx, double* By,
+                double* Bz, double* dt);
+/* single stage from a given input */
+void or_stage(const or_grid* g, int irk, double dt, double* const* in, double* const* u0,
+              double* const* out);
+
+/* pointwise (for known-answer tests against the reference build) */
+int or_cons_to_prim(double gam, double rhof, double prsf, double* ent, double* U, double* V);
+void or_prim_to_cons(double gam, double rhof, double prsf, double ent, double* V, double* U);
+void or_wavspd_x(double gam, const double* V, double* smin, double* smax);
+void or_flux_x(const double* V, const double* U, double* F);
+void or_hll(double gam, double rhof, double prsf, double* VL, double* VR, double entL,
+            double entR, double* F, double* entF, double* SL, double* SR);
+void or_hlld(double gam, double rhof, double prsf, double* VL, double* VR, double entL,
+             double entR, double* F, double* entF, double* SL, double* SR);
+double or_weno5(double v0, double v1, double v2, double v3, double v4);
+double or_emf4_hll(const double* s4, double cs_t, double sn_t, const double* ez4,
+                   const double* bn2, const double* by2);
+double or_emf3_llf(double S, const double* ez3, const double* bn3);
+
+#endif

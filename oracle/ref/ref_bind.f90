@@ -1,0 +1,110 @@
+! ref_bind.f90 -- C-callable wrappers around reference routines compiled from
+! /root/reference/pkg/inc/cubesphere (see oracle/ref/Makefile).  Test
+! infrastructure only: used by the CPU tests to pin the oracle's
+! restatement; never linked into the product.
+module cs_ref_bind
+  use iso_c_binding
+  implicit none
+contains
+
+  ! Mesh_Sector_General (201mesh_sector_general.f:79) for one patch per sector,
+  ! then FILL_GHOST2D_GLOBAL (203fill_ghost2d_global.f:7) for all six.
+  subroutine ref_unit_lattice(n, g, coords) bind(C, name="ref_unit_lattice")
+    integer(c_int), value :: n, g
+    real(c_double), intent(inout) :: coords(0-g:n+g, 0-g:n+g, 3, 6)
+    real(c_double) :: vertices(4, 3)
+    integer :: s
+    external Mesh_Sector_General, FILL_GHOST2D_GLOBAL
+    coords = 0.0d0
+    do s = 1, 6
+      call Mesh_Sector_General(s, 1, 1, 1, n, g, vertices, coords(:, :, :, s))
+    end do
+    do s = 1, 6
+      call FILL_GHOST2D_GLOBAL(s, n, g, coords)
+    end do
+  end subroutine
+
+  subroutine ref_radial_mesh(nr, g, rmin, rmax, ilog, rb) bind(C, name="ref_radial_mesh")
+    integer(c_int), value :: nr, g, ilog
+    real(c_double), value :: rmin, rmax
+    real(c_double), intent(out) :: rb(0-g:nr+g)
+    real(c_double) :: dr(1-g:nr+g), rc(1-g:nr+g), smallnum, ratio
+    external GENERATE_GLOBAL_RADIAL_UNIFORM_MESH, GENERATE_GLOBAL_RADIAL_LOG_MESH
+    smallnum = 1.0d-12
+    ratio = 2.0d0
+    if (ilog == 0) then
+      call GENERATE_GLOBAL_RADIAL_UNIFORM_MESH(nr, g, rmin, rmax, smallnum, ratio, rb, dr, rc)
+    else
+      call GENERATE_GLOBAL_RADIAL_LOG_MESH(nr, g, rmin, rmax, smallnum, ratio, rb, dr, rc)
+    end if
+  end subroutine
+
+  ! CONS_TO_PRIM / PRIM_TO_CONS / PRIM_TO_WAVSPD (662, 661, 660), MHD branch.
+  subroutine ref_cons_to_prim(gam, rhof, prsf, ent, u, v, flag) bind(C, name="ref_cons_to_prim")
+    real(c_double), value :: gam, rhof, prsf
+    real(c_double), intent(inout) :: ent, u(8)
+    real(c_double), intent(out) :: v(8)
+    integer(c_int), intent(out) :: flag
+    real(c_double) :: b0(3)
+    integer :: iflag
+    external CONS_TO_PRIM
+    b0 = 0.0d0
+    call CONS_TO_PRIM(gam, 1.4d0, 1.0d0, 0.0d0, 1.4d0, 1.0d0, rhof, prsf, 1.0d-5, &
+                      1.0d-12, 1.0d-12, iflag, ent, b0, u, v)
+    flag = iflag
+  end subroutine
+
+  subroutine ref_prim_to_cons(gam, rhof, prsf, ent, v, u) bind(C, name="ref_prim_to_cons")
+    real(c_double), value :: gam, rhof, prsf, ent
+    real(c_double), intent(inout) :: v(8)
+    real(c_double), intent(out) :: u(8)
+    real(c_double) :: b0(3), e
+    integer :: iflag
+    external PRIM_TO_CONS
+    b0 = 0.0d0
+    e = ent
+    call PRIM_TO_CONS(gam, 1.4d0, 1.0d0, 0.0d0, 1.4d0, 1.0d0, rhof, prsf, 1.0d-5, &
+                      1.0d-12, 1.0d-12, iflag, e, b0, u, v)
+  end subroutine
+
+  subroutine ref_wavspd(idir, gam, v, smin, smax) bind(C, name="ref_wavspd")
+    integer(c_int), value :: idir
+    real(c_double), value :: gam
+    real(c_double), intent(in) :: v(8)
+    real(c_double), intent(out) :: smin, smax
+    real(c_double) :: b0(3), vv(8)
+    external PRIM_TO_WAVSPD
+    b0 = 0.0d0
+    vv = v
+    call PRIM_TO_WAVSPD(idir, gam, b0, vv, smin, smax)
+  end subroutine
+
+  ! Two_D_RS_Ez_4State_HLL / _4State_LLF / _3State_LLF (630, 632, 633).
+  subroutine ref_emf4_hll(s, cs_t, sn_t, ez, bn, by, out) bind(C, name="ref_emf4_hll")
+    real(c_double), intent(in) :: s(4), ez(4), bn(2), by(2)
+    real(c_double), value :: cs_t, sn_t
+    real(c_double), intent(out) :: out(3)
+    external Two_D_RS_Ez_4State_HLL
+    ! s = (S_R, S_L, S_U, S_D); ez = (RU, LU, LD, RD); bn = (U, D); by = (R, L)
+    call Two_D_RS_Ez_4State_HLL(s(1), s(2), s(3), s(4), cs_t, sn_t, ez(1), ez(2), ez(3), ez(4), &
+                                bn(1), bn(2), by(1), by(2), out(1), out(2), out(3))
+  end subroutine
+
+  subroutine ref_emf4_llf(s, cs_t, sn_t, ez, bn, by, out) bind(C, name="ref_emf4_llf")
+    real(c_double), intent(in) :: s(4), ez(4), bn(2), by(2)
+    real(c_double), value :: cs_t, sn_t
+    real(c_double), intent(out) :: out(3)
+    external Two_D_RS_Ez_4State_LLF
+    call Two_D_RS_Ez_4State_LLF(s(1), s(2), s(3), s(4), cs_t, sn_t, ez(1), ez(2), ez(3), ez(4), &
+                                bn(1), bn(2), by(1), by(2), out(1), out(2), out(3))
+  end subroutine
+
+  subroutine ref_emf3_llf(s6, ez, bn, out) bind(C, name="ref_emf3_llf")
+    real(c_double), intent(in) :: s6(6), ez(3), bn(3)
+    real(c_double), intent(out) :: out(3)
+    external Two_D_RS_Ez_3State_LLF
+    call Two_D_RS_Ez_3State_LLF(s6(1), s6(2), s6(3), s6(4), s6(5), s6(6), ez(1), ez(2), ez(3), &
+                                bn(1), bn(2), bn(3), out(1), out(2), out(3))
+  end subroutine
+
+end module cs_ref_bind
