@@ -229,6 +229,9 @@ __global__ void k_corner_fill(CsDims d, State st) {
 // Radial ghost layers: zero-gradient pad (826pad_boundary_r.f:201-214 inner
 // form, used at both ends) or a reflecting, perfectly conducting wall.
 __global__ void k_radial_bc(CsDims d, Geo geo, State st, int wall) {
+  // no FMA contraction: the wall mirror rounds exactly as written, so the whole
+  // halo (exchange + boundary) is reproducible bit for bit on any host
+#pragma clang fp contract(off)
   uint32_t t = gtid();
   const int N = d.N, NR = d.NR, g = d.g;
   const int nl = N + 2 * g + 1;  // lateral index range [-g, N+g]
@@ -267,8 +270,16 @@ __global__ void k_radial_bc(CsDims d, Geo geo, State st, int wall) {
     double u[8];
     for (int c = 0; c < 8; ++c) u[c] = st.U[(s * 8 + c) * d.cpp + sc];
     if (sgn < 0) {
-      // zone-centre direction: well defined also in the degenerate tri-corner ghost blocks
-      double n0 = geo.t(d, T_ZCX, s, i, j), n1 = geo.t(d, T_ZCY, s, i, j), n2 = geo.t(d, T_ZCZ, s, i, j);
+      // mirror about the unit normal of the radial faces, so a wall face sees
+      // exactly mirrored states (no mass flux); the zone-centre direction
+      // stands in for it in the degenerate tri-corner ghost blocks
+      double zn = geo.t(d, T_ZSN, s, i, j);
+      double n0, n1, n2;
+      if (zn > 1e-300 && zn < 1e300) {
+        n0 = geo.t(d, T_ZSX, s, i, j) / zn; n1 = geo.t(d, T_ZSY, s, i, j) / zn; n2 = geo.t(d, T_ZSZ, s, i, j) / zn;
+      } else {
+        n0 = geo.t(d, T_ZCX, s, i, j); n1 = geo.t(d, T_ZCY, s, i, j); n2 = geo.t(d, T_ZCZ, s, i, j);
+      }
       double m = u[1] * n0 + u[2] * n1 + u[3] * n2;
       u[1] -= 2.0 * m * n0; u[2] -= 2.0 * m * n1; u[3] -= 2.0 * m * n2;
       double b = u[5] * n0 + u[6] * n1 + u[7] * n2;

@@ -153,8 +153,11 @@ static void corner_blocks(int N, int g, double* coords) {
               (sec == 5 && C[1] > 0.0) || (sec == 3 && C[0] < 0.0) || (sec == 6 && C[0] > 0.0)) {
             C[0] = -C[0]; C[1] = -C[1]; C[2] = -C[2];
           }
-          unit3(C);
-          for (c = 0; c < 3; ++c) CO(i, j, c, s) = C[c];
+          {
+            /* normalised as in 201mesh_sector_general.f:810-813 */
+            double t = 1.0 / sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]);
+            for (c = 0; c < 3; ++c) CO(i, j, c, s) = t * C[c];
+          }
         }
       }
   }
@@ -334,21 +337,20 @@ static void zone_center(const or_grid* G, int s, int i, int j, double* o) {
   unit3(o);
 }
 /* face of family fam (0 x-face: vertex i, cell j; 1 y-face: cell i, vertex j):
-   midpoint and unit normal oriented toward increasing index */
+   midpoint and unit normal oriented toward increasing index.  The panel's
+   (x, y, r) index frame is right handed, so an x-face, whose edge runs along
+   +y, has normal (edge) x r, and a y-face, whose edge runs along +x, has
+   normal r x (edge); only the face's own two vertices are used. */
 static void face_geo(const or_grid* G, int fam, int s, int i, int j, double* mid, double* nrm) {
-  double a[3], b[3], lo[3], hi[3], dir[3];
+  double a[3], b[3], e[3];
   int c;
   if (fam == 0) { uv(G, s, i, j - 1, a); uv(G, s, i, j, b); }
   else { uv(G, s, i - 1, j, a); uv(G, s, i, j, b); }
-  for (c = 0; c < 3; ++c) mid[c] = a[c] + b[c];
+  for (c = 0; c < 3; ++c) { mid[c] = a[c] + b[c]; e[c] = b[c] - a[c]; }
   unit3(mid);
-  cross3(a, b, nrm);
+  if (fam == 0) cross3(e, mid, nrm);
+  else cross3(mid, e, nrm);
   unit3(nrm);
-  /* orientation: from the zone below the face to the zone above it */
-  if (fam == 0) { zone_center(G, s, i, j, lo); zone_center(G, s, i + 1, j, hi); }
-  else { zone_center(G, s, i, j, lo); zone_center(G, s, i, j + 1, hi); }
-  for (c = 0; c < 3; ++c) dir[c] = hi[c] - lo[c];
-  if (dot3(dir, nrm) < 0) for (c = 0; c < 3; ++c) nrm[c] = -nrm[c];
 }
 
 static int nearest(const gobj* cand, int n, const double* p, int excl_s, double tol) {
@@ -646,7 +648,12 @@ void or_halo(const or_grid* G, double* U, double* ent, double* Bx, double* By, d
               if (sg < 0) {
                 double n[3];
                 double mm, bb;
-                zone_center(G, s, i, j, n);
+                int64_t t = T2(G, s, i, j);
+                if (G->zsn[t] > 1e-300 && G->zsn[t] < 1e300) {
+                  for (q = 0; q < 3; ++q) n[q] = G->zs[q][t] / G->zsn[t];
+                } else {
+                  zone_center(G, s, i, j, n);
+                }
                 mm = u[1] * n[0] + u[2] * n[1] + u[3] * n[2];
                 bb = u[5] * n[0] + u[6] * n[1] + u[7] * n[2];
                 for (q = 0; q < 3; ++q) { u[1 + q] -= 2.0 * mm * n[q]; u[5 + q] -= 2.0 * bb * n[q]; }
