@@ -44,7 +44,29 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
+
+
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+DEMO = os.path.join(HERE, "cs_dropin_demo")
+
+
+def build_fortran(verbose=False):
+    """The ISO_C_BINDING module and the Fortran drop-in demo (the reference's
+    own language); skipped when no Fortran compiler is installed."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("flang not found: Fortran interface not built")
+        return None
+    fdir = os.path.join(HERE, "fortran")
+    obj = os.path.join(BUILD, "cs_hip_iface.o")
+    _run([FLANG, "-c", "-O2", "-J", BUILD, os.path.join(fdir, "cs_hip_iface.f90"), "-o", obj])
+    _run([FLANG, "-O2", "-I", BUILD, os.path.join(fdir, "cs_dropin_demo.f90"), obj, "-L", HERE,
+          "-lcubesphere_hip", "-Wl,-rpath,$ORIGIN", "-o", DEMO])
+    if verbose:
+        print("built", DEMO)
+    return DEMO
 
 
 if __name__ == "__main__":

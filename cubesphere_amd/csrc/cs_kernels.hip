@@ -54,7 +54,7 @@ static Rng make_rng(int i0, int i1, int j0, int j1, int k0, int k1) {
   r.i0 = i0; r.ni = i1 - i0 + 1;
   r.j0 = j0; r.nj = j1 - j0 + 1;
   r.k0 = k0; r.nk = k1 - k0 + 1;
-  r.total = (uint32_t)((uint64_t)r.ni * r.nj * r.nk * 6);
+  r.total = (r.ni > 0 && r.nj > 0 && r.nk > 0) ? (uint32_t)((uint64_t)r.ni * r.nj * r.nk * 6) : 0u;
   return r;
 }
 __device__ __forceinline__ bool decode(const Rng& r, uint32_t t, int& i, int& j, int& k, int& s) {
@@ -313,21 +313,37 @@ __global__ void k_face_flux(CsDims d, Rng rg, Geo geo, State st, const P* __rest
                             double* __restrict__ F, double* __restrict__ S, Params pr) {
   int i, j, k, s;
   if (!decode(rg, gtid(), i, j, k, s)) return;
+  if (DIR == 2 && (i == 0 || i == d.N + 1) && (j == 0 || j == d.N + 1)) {
+    // radial faces of the tri-corner ghost columns have zero area (two of the
+    // four vertices coincide); nothing reads them
+    int64_t fo = (int64_t)s * 9 * d.fpp + d.face(i, j, k);
+    for (int q = 0; q < 9; ++q) F[fo + q * d.fpp] = 0.0;
+    int64_t so = (int64_t)s * 2 * d.fpp + d.face(i, j, k);
+    S[so] = 0.0;
+    S[so + d.fpp] = 0.0;
+    return;
+  }
   const int64_t stride = DIR == 0 ? 1 : (DIR == 1 ? d.nc : (int64_t)d.nc * d.nc);
   // left cell of the face
   int64_t cl = DIR == 0 ? d.cell(i, j, k) : (DIR == 1 ? d.cell(i, j, k) : d.cell(i, j, k));
   const P* Vs = V + (int64_t)s * 9 * d.cpp;
+  // Faces in the ghost rows/columns (outside [1,N] transversally) only supply
+  // wave speeds to the EMFs of the panel-edge edges; their stencils would
+  // reach into the tri-corner ghost blocks, so they use the first-order states.
+  const bool weno = pr.recon == CS_RECON_WENO5 &&
+                    (DIR == 0 ? (j >= 1 && j <= d.N)
+                              : DIR == 1 ? (i >= 1 && i <= d.N) : (i >= 1 && i <= d.N && j >= 1 && j <= d.N));
   R VL[9], VR[9];
   for (int q = 0; q < 9; ++q) {
     const P* p = Vs + q * d.cpp + cl;
-    R m2 = (R)p[-2 * stride], m1 = (R)p[-stride], c0 = (R)p[0], p1 = (R)p[stride],
-      p2 = (R)p[2 * stride], p3 = (R)p[3 * stride];
-    if (pr.recon == CS_RECON_WENO5) {
+    if (weno) {
+      R m2 = (R)p[-2 * stride], m1 = (R)p[-stride], c0 = (R)p[0], p1 = (R)p[stride],
+        p2 = (R)p[2 * stride], p3 = (R)p[3 * stride];
       VL[q] = weno5<R>(m2, m1, c0, p1, p2);
       VR[q] = weno5<R>(p3, p2, p1, c0, m1);
     } else {
-      VL[q] = c0;
-      VR[q] = p1;
+      VL[q] = (R)p[0];
+      VR[q] = (R)p[stride];
     }
   }
   if (!(VL[0] > R(0)) || !(VL[4] > R(0)) || !(VL[8] > R(0)))
@@ -878,16 +894,21 @@ int stage_input(cs_handle* h, int irk) {
 struct Ranges {
   Rng all, fx, fy, fz, ez, ex, ey, cells, faces;
 };
-Ranges ranges_of(const CsDims& d) {
+// With conducting walls (surface mode) the x/y-edge EMFs on the walls are
+// zero, so neither they nor the lateral faces of the radial ghost layers that
+// would feed their speeds are computed.
+Ranges ranges_of(const CsDims& d, int wall) {
   const int N = d.N, NR = d.NR, g = d.g;
+  const int kl0 = wall ? 1 : 0, kl1 = wall ? NR : NR + 1;
+  const int ke0 = wall ? 1 : 0, ke1 = wall ? NR - 1 : NR;
   Ranges r;
   r.all = make_rng(1 - g, N + g, 1 - g, N + g, 1 - g, NR + g);
-  r.fx = make_rng(0, N, 0, N + 1, 0, NR + 1);
-  r.fy = make_rng(0, N + 1, 0, N, 0, NR + 1);
+  r.fx = make_rng(0, N, 0, N + 1, kl0, kl1);
+  r.fy = make_rng(0, N + 1, 0, N, kl0, kl1);
   r.fz = make_rng(0, N + 1, 0, N + 1, 0, NR);
   r.ez = make_rng(0, N, 0, N, 1, NR);
-  r.ex = make_rng(1, N, 0, N, 0, NR);
-  r.ey = make_rng(0, N, 1, N, 0, NR);
+  r.ex = make_rng(1, N, 0, N, ke0, ke1);
+  r.ey = make_rng(0, N, 1, N, ke0, ke1);
   r.cells = make_rng(1, N, 1, N, 1, NR);
   r.faces = make_rng(0, N, 0, N, 0, NR);
   return r;
@@ -897,7 +918,7 @@ Ranges ranges_of(const CsDims& d) {
 template <typename R, typename P>
 int launch_kernel_t(cs_handle* h, int kid, int b, int irk) {
   const CsDims& d = h->d;
-  const Ranges r = ranges_of(d);
+  const Ranges r = ranges_of(d, h->pr.wall);
   State st = state_of(h, b);
   Geo geo = geo_of(h);
   P* V = (P*)h->V;
@@ -922,10 +943,10 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk) {
       k_emf_z<P><<<nblk(r.ez.total), kBlock, 0, sm>>>(d, r.ez, geo, st, V, h->S[0], h->S[1], h->E[2], h->pr);
       break;
     case CS_K_EMF_X:
-      k_emf_xy<0, P><<<nblk(r.ex.total), kBlock, 0, sm>>>(d, r.ex, geo, st, V, h->S[1], h->S[2], h->E[0], h->pr);
+      if (r.ex.total) k_emf_xy<0, P><<<nblk(r.ex.total), kBlock, 0, sm>>>(d, r.ex, geo, st, V, h->S[1], h->S[2], h->E[0], h->pr);
       break;
     case CS_K_EMF_Y:
-      k_emf_xy<1, P><<<nblk(r.ey.total), kBlock, 0, sm>>>(d, r.ey, geo, st, V, h->S[2], h->S[0], h->E[1], h->pr);
+      if (r.ey.total) k_emf_xy<1, P><<<nblk(r.ey.total), kBlock, 0, sm>>>(d, r.ey, geo, st, V, h->S[2], h->S[0], h->E[1], h->pr);
       break;
     case CS_K_UPDATE_CELLS:
       k_update_cells<<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, h->F[0], h->F[1],
@@ -1327,7 +1348,7 @@ extern "C" int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units) {
   CHECK_H(h);
   if (!units || kernel < 0 || kernel >= CS_K_COUNT) { set_error("bad arguments"); return CS_E_INVALID; }
   const CsDims& d = h->d;
-  const Ranges r = ranges_of(d);
+  const Ranges r = ranges_of(d, h->pr.wall);
   int64_t n = 0;
   switch (kernel) {
     case CS_K_HALO: n = 24ll * d.g * d.N * d.NR; break;

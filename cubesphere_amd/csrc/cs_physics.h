@@ -205,14 +205,20 @@ __device__ __forceinline__ void hlld(const Gas<R>& gs, R* VL, R* VR, R entL, R e
   R EL = UL[4], ER = UR[4];
   R dL = (SL - uL) * rL, dR = (SR - uR) * rR;
   R SM = (dR * uR - dL * uL - ptR + ptL) / (dR - dL);
+  // the contact speed lies strictly between the outer speeds; in rounding it may not
+  {
+    R margin = (SR - SL) * (sizeof(R) == 4 ? R(1e-6) : R(1e-12));
+    SM = rmin(rmax(SM, SL + margin), SR - margin);
+  }
   R pts = (dR * ptL - dL * ptR + dL * dR * (uR - uL)) / (dR - dL);
-  // star states
+  // star states; the degenerate case (Miyoshi & Kusano sec. 3.3) is detected
+  // relative to the rounding of the arithmetic type
+  const R degen = sizeof(R) == 4 ? R(1e-5) : R(1e-10);
   R rLs = dL / (SL - SM), rRs = dR / (SR - SM);
   R vyLs, vzLs, byLs, bzLs, vyRs, vzRs, byRs, bzRs;
   {
     R den = dL * (SL - SM) - bx * bx;
-    R scale = rL * (SL - uL) * (SL - uL) + R(0);
-    if (rabs(den) > R(1e-10) * (rabs(dL * (SL - SM)) + bx * bx) && den != R(0)) {
+    if (rabs(den) > degen * (rabs(dL * (SL - SM)) + bx * bx) && den != R(0)) {
       R f1 = bx * (SM - uL) / den;
       vyLs = VL[2] - byL * f1;
       vzLs = VL[3] - bzL * f1;
@@ -222,11 +228,10 @@ __device__ __forceinline__ void hlld(const Gas<R>& gs, R* VL, R* VR, R entL, R e
     } else {
       vyLs = VL[2]; vzLs = VL[3]; byLs = byL; bzLs = bzL;
     }
-    (void)scale;
   }
   {
     R den = dR * (SR - SM) - bx * bx;
-    if (rabs(den) > R(1e-10) * (rabs(dR * (SR - SM)) + bx * bx) && den != R(0)) {
+    if (rabs(den) > degen * (rabs(dR * (SR - SM)) + bx * bx) && den != R(0)) {
       R f1 = bx * (SM - uR) / den;
       vyRs = VR[2] - byR * f1;
       vzRs = VR[3] - bzR * f1;

@@ -1,0 +1,135 @@
+! cs_hip_iface.f90 -- ISO_C_BINDING interface to libcubesphere_hip.so for the
+! reference's Fortran driver (_I/_1000riemann_geomesh_cs_mhd/1000riemann_geomesh_cs_mhd.f).
+!
+! The driver keeps its module/subroutine API; under one preprocessor switch it
+! calls these entry points instead of the per-patch routines:
+!   cs_halo_exchange       <- LOAD_LOCAL_BUFFERS_* / FETCH_BUFFERS_FILL_IN_* (DRV:1746-1756)
+!   cs_make_facial_fluxes  <- LIMIT_ALL_SLOPES_P1 + MAKE_FACIAL_FLUXES      (DRV:2396, 2485)
+!   cs_make_edge_efields   <- MAKE_EDGE_EFIELDS_P1                          (DRV:2548)
+!   cs_update_stage        <- UPDATE_IMEX_RK_STEP_P1 + CLEANUP_AFTER_UPDATE (DRV:2782, 2815)
+!   cs_evaluate_timestep   <- EVALUATE_TIMESTEP                             (DRV:2887)
+!   cs_rk_stage            <- the whole per-patch loop body                (DRV:2007-3120)
+! State arrays keep the driver's layout U_amr(1-g:N+g,1-g:N+g,1-g:nr+g,8,6),
+! entropygr_amr(...,6), Bxf_amr(0-g:N+g,0-g:N+g,0-g:nr+g,6) (MOD:502-552).
+module cs_hip_iface
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: CS_RECON_PCM = 0, CS_RECON_WENO5 = 1
+  integer(c_int), parameter :: CS_RS_HLL = 0, CS_RS_HLLD = 1
+  integer(c_int), parameter :: CS_EMF_LLF = 0, CS_EMF_HLL = 1
+  integer(c_int), parameter :: CS_FP64 = 0, CS_MIXED = 1
+  integer(c_int), parameter :: CS_RBC_PAD = 0, CS_RBC_WALL = 1
+
+  type, bind(C) :: cs_config
+    integer(c_int32_t) :: ntp, nr, nghost, recon, riemann, emf, precision, rk_order, radial_bc, radial_mesh
+    real(c_double) :: rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav
+    integer(c_int32_t) :: imex, reserved
+  end type
+
+  interface
+    integer(c_int) function cs_create(cfg, h) bind(C, name="cs_create")
+      import :: c_int, c_ptr, cs_config
+      type(cs_config), intent(in) :: cfg
+      type(c_ptr), intent(out) :: h
+    end function
+    integer(c_int) function cs_destroy(h) bind(C, name="cs_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cs_init_problem(cfg, problem, seed, u, ent, bxf, byf, bzf) &
+        bind(C, name="cs_init_problem")
+      import :: c_int, c_int32_t, c_int64_t, c_double, cs_config
+      type(cs_config), intent(in) :: cfg
+      integer(c_int32_t), value :: problem
+      integer(c_int64_t), value :: seed
+      real(c_double), intent(out) :: u(*), ent(*), bxf(*), byf(*), bzf(*)
+    end function
+    integer(c_int) function cs_upload_state(h, u, ent, bxf, byf, bzf) bind(C, name="cs_upload_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(in) :: u(*), ent(*), bxf(*), byf(*), bzf(*)
+    end function
+    integer(c_int) function cs_download_state(h, u, ent, bxf, byf, bzf) bind(C, name="cs_download_state")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: u(*), ent(*), bxf(*), byf(*), bzf(*)
+    end function
+    integer(c_int) function cs_halo_exchange(h) bind(C, name="cs_halo_exchange")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cs_make_facial_fluxes(h) bind(C, name="cs_make_facial_fluxes")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cs_make_edge_efields(h) bind(C, name="cs_make_edge_efields")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cs_update_stage(h, irk) bind(C, name="cs_update_stage")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: irk
+    end function
+    integer(c_int) function cs_evaluate_timestep(h) bind(C, name="cs_evaluate_timestep")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+    end function
+    integer(c_int) function cs_set_dt(h, dt) bind(C, name="cs_set_dt")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), value :: dt
+    end function
+    integer(c_int) function cs_get_dt(h, dt) bind(C, name="cs_get_dt")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: dt
+    end function
+    integer(c_int) function cs_rk_stage(h, irk) bind(C, name="cs_rk_stage")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: irk
+    end function
+    integer(c_int) function cs_advance(h, nsteps, use_graph) bind(C, name="cs_advance")
+      import :: c_int, c_ptr, c_int32_t
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: nsteps, use_graph
+    end function
+    integer(c_int) function cs_conservation_sums(h, sums) bind(C, name="cs_conservation_sums")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: sums(9)
+    end function
+    integer(c_int) function cs_max_divb(h, v) bind(C, name="cs_max_divb")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: h
+      real(c_double), intent(out) :: v
+    end function
+    integer(c_int) function cs_last_error(buf, n) bind(C, name="cs_last_error")
+      import :: c_int, c_char, c_size_t
+      character(kind=c_char), intent(out) :: buf(*)
+      integer(c_size_t), value :: n
+    end function
+  end interface
+
+contains
+
+  ! Stop the way the reference stops (WRITE(6,*) + STOP) on a non-zero status.
+  subroutine cs_check(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: what
+    character(kind=c_char) :: buf(512)
+    integer :: i, n
+    if (rc == 0) return
+    i = cs_last_error(buf, int(512, c_size_t))
+    n = 0
+    do while (n < 511)
+      if (buf(n + 1) == c_null_char) exit
+      n = n + 1
+    end do
+    write (6, *) "Stopping in ", what, " status ", rc, ": ", (buf(i), i = 1, n)
+    stop 1
+  end subroutine
+
+end module cs_hip_iface

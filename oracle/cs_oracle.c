@@ -778,6 +778,11 @@ void or_hlld(double gam, double rhof, double prsf, double* VL, double* VR, doubl
   dL = (*SL - VL[1]) * VL[0];
   dR = (*SR - VR[1]) * VR[0];
   SM = (dR * VR[1] - dL * VL[1] - ptR + ptL) / (dR - dL);
+  { /* keep the contact strictly inside the fan, as rounding may not */
+    double margin = (*SR - *SL) * 1e-12;
+    if (SM < *SL + margin) SM = *SL + margin;
+    if (SM > *SR - margin) SM = *SR - margin;
+  }
   pts = (dR * ptL - dL * ptR + dL * dR * (VR[1] - VL[1])) / (dR - dL);
   rLs = dL / (*SL - SM);
   rRs = dR / (*SR - SM);
@@ -903,11 +908,14 @@ static void prims_of(const or_grid* G, const double* U, const double* ent, int s
 static void one_face(const or_grid* G, int dir, int s, int i, int j, int k, const double* Vc, int64_t stride,
                      int64_t cl, double bface, double* out, double* SL, double* SR) {
   double VL[9], VR[9], A[3], Bv[3], Cv[3], area, wl[8], wr[8], Fs[8], eF;
-  int q, c;
+  int q, c, N = G->N;
   int64_t t = T2(G, s, i, j);
+  /* faces of the ghost rows/columns only feed EMF speeds: first-order states */
+  int weno = G->c.recon == 1 && (dir == 0 ? (j >= 1 && j <= N)
+                                 : dir == 1 ? (i >= 1 && i <= N) : (i >= 1 && i <= N && j >= 1 && j <= N));
   for (q = 0; q < 9; ++q) {
     const double* p = Vc + q * G->cpp + cl;
-    if (G->c.recon == 1) {
+    if (weno) {
       VL[q] = or_weno5(p[-2 * stride], p[-stride], p[0], p[stride], p[2 * stride]);
       VR[q] = or_weno5(p[3 * stride], p[2 * stride], p[stride], p[0], p[-stride]);
     } else {
@@ -965,8 +973,11 @@ static double* prims_all(const or_grid* G, const double* U, const double* ent) {
 static void fluxes_from_V(const or_grid* G, const double* V, const double* Bx, const double* By, const double* Bz,
                           double** F, double** S) {
   int N = G->N, NR = G->NR, dir, s, i, j, k, q;
-  int lo[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-  int hi[3][3] = {{N, N + 1, NR + 1}, {N + 1, N, NR + 1}, {N + 1, N + 1, NR}};
+  /* conducting walls: lateral faces of the radial ghost layers only fed the
+     (zero) wall EMFs, so they are not computed */
+  int wall = G->c.radial_bc == 1;
+  int lo[3][3] = {{0, 0, wall}, {0, 0, wall}, {0, 0, 0}};
+  int hi[3][3] = {{N, N + 1, wall ? NR : NR + 1}, {N + 1, N, wall ? NR : NR + 1}, {N + 1, N + 1, NR}};
   int64_t strides[3] = {1, G->nc, (int64_t)G->nc * G->nc};
   const double* fb[3] = {Bx, By, Bz};
   for (dir = 0; dir < 3; ++dir)
@@ -976,6 +987,7 @@ static void fluxes_from_V(const or_grid* G, const double* V, const double* Bx, c
           for (i = lo[dir][0]; i <= hi[dir][0]; ++i) {
             double out[9], sl, sr;
             int64_t f = FACE(G, i, j, k);
+            if (dir == 2 && (i == 0 || i == N + 1) && (j == 0 || j == N + 1)) continue; /* zero-area tri-corner column */
             one_face(G, dir, s, i, j, k, V + (int64_t)s * 9 * G->cpp, strides[dir], CELL(G, i, j, k),
                      fb[dir][s * G->fpp + f], out, &sl, &sr);
             for (q = 0; q < 9; ++q) F[dir][((int64_t)s * 9 + q) * G->fpp + f] = out[q];
