@@ -305,11 +305,52 @@ __global__ void k_prims(CsDims d, Rng rg, State st, P* V, Params pr) {
   V[(s * 9 + 8) * d.cpp + c] = (P)ent;
 }
 
+// One face: rotate the reconstructed states into the face frame (A normal,
+// Bv, Cv), put the facial B in the normal slot, Riemann-solve, rotate the
+// resolved flux back to the global frame (450make_facial_fluxes.f:890-1030).
+template <typename R>
+__device__ __forceinline__ void solve_face(const R* VL, const R* VR, const double* A, const double* Bv,
+                                           const double* Cv, double bface, const Params& pr, double* out,
+                                           R& SL, R& SR) {
+  R a0 = (R)A[0], a1 = (R)A[1], a2 = (R)A[2], b0 = (R)Bv[0], b1 = (R)Bv[1], b2 = (R)Bv[2],
+    c0 = (R)Cv[0], c1 = (R)Cv[1], c2 = (R)Cv[2];
+  R wl[8], wr[8];
+  wl[0] = VL[0]; wr[0] = VR[0];
+  wl[4] = VL[4]; wr[4] = VR[4];
+  wl[1] = a0 * VL[1] + a1 * VL[2] + a2 * VL[3];
+  wl[2] = b0 * VL[1] + b1 * VL[2] + b2 * VL[3];
+  wl[3] = c0 * VL[1] + c1 * VL[2] + c2 * VL[3];
+  wr[1] = a0 * VR[1] + a1 * VR[2] + a2 * VR[3];
+  wr[2] = b0 * VR[1] + b1 * VR[2] + b2 * VR[3];
+  wr[3] = c0 * VR[1] + c1 * VR[2] + c2 * VR[3];
+  wl[5] = (R)bface; wr[5] = (R)bface;
+  wl[6] = b0 * VL[5] + b1 * VL[6] + b2 * VL[7];
+  wl[7] = c0 * VL[5] + c1 * VL[6] + c2 * VL[7];
+  wr[6] = b0 * VR[5] + b1 * VR[6] + b2 * VR[7];
+  wr[7] = c0 * VR[5] + c1 * VR[6] + c2 * VR[7];
+  Gas<R> gs{(R)pr.gamma, (R)pr.gm1, (R)pr.rhofloor, (R)pr.prsfloor};
+  R Fs[8], entF;
+  if (pr.riemann == CS_RS_HLLD)
+    hlld<R>(gs, wl, wr, VL[8], VR[8], Fs, entF, SL, SR);
+  else
+    hll<R>(gs, wl, wr, VL[8], VR[8], Fs, entF, SL, SR);
+  // rotate back (450make_facial_fluxes.f:995-1030); the caller scales by the area
+  out[0] = (double)Fs[0];
+  out[4] = (double)Fs[4];
+  out[1] = A[0] * (double)Fs[1] + Bv[0] * (double)Fs[2] + Cv[0] * (double)Fs[3];
+  out[2] = A[1] * (double)Fs[1] + Bv[1] * (double)Fs[2] + Cv[1] * (double)Fs[3];
+  out[3] = A[2] * (double)Fs[1] + Bv[2] * (double)Fs[2] + Cv[2] * (double)Fs[3];
+  out[5] = A[0] * (double)Fs[5] + Bv[0] * (double)Fs[6] + Cv[0] * (double)Fs[7];
+  out[6] = A[1] * (double)Fs[5] + Bv[1] * (double)Fs[6] + Cv[1] * (double)Fs[7];
+  out[7] = A[2] * (double)Fs[5] + Bv[2] * (double)Fs[6] + Cv[2] * (double)Fs[7];
+  out[8] = (double)entF;
+}
+
 // ------------------------------------------------------------------ face fluxes
 // DIR 0: x-faces (x-vertex i, cells i and i+1), 1: y, 2: z.
 // Output per face: flux(9) * area in the global frame, speeds SL, SR.
 template <int DIR, typename R, typename P>
-__global__ void k_face_flux(CsDims d, Rng rg, Geo geo, State st, const P* __restrict__ V,
+__global__ void __launch_bounds__(256, 2) k_face_flux(CsDims d, Rng rg, Geo geo, State st, const P* __restrict__ V,
                             double* __restrict__ F, double* __restrict__ S, Params pr) {
   int i, j, k, s;
   if (!decode(rg, gtid(), i, j, k, s)) return;
@@ -379,39 +420,9 @@ __global__ void k_face_flux(CsDims d, Rng rg, Geo geo, State st, const P* __rest
     area = r1 * r1 * zn;
     bface = st.Bz[f];
   }
-  R a0 = (R)A[0], a1 = (R)A[1], a2 = (R)A[2], b0 = (R)Bv[0], b1 = (R)Bv[1], b2 = (R)Bv[2],
-    c0 = (R)Cv[0], c1 = (R)Cv[1], c2 = (R)Cv[2];
-  R wl[8], wr[8];
-  wl[0] = VL[0]; wr[0] = VR[0];
-  wl[4] = VL[4]; wr[4] = VR[4];
-  wl[1] = a0 * VL[1] + a1 * VL[2] + a2 * VL[3];
-  wl[2] = b0 * VL[1] + b1 * VL[2] + b2 * VL[3];
-  wl[3] = c0 * VL[1] + c1 * VL[2] + c2 * VL[3];
-  wr[1] = a0 * VR[1] + a1 * VR[2] + a2 * VR[3];
-  wr[2] = b0 * VR[1] + b1 * VR[2] + b2 * VR[3];
-  wr[3] = c0 * VR[1] + c1 * VR[2] + c2 * VR[3];
-  wl[5] = (R)bface; wr[5] = (R)bface;
-  wl[6] = b0 * VL[5] + b1 * VL[6] + b2 * VL[7];
-  wl[7] = c0 * VL[5] + c1 * VL[6] + c2 * VL[7];
-  wr[6] = b0 * VR[5] + b1 * VR[6] + b2 * VR[7];
-  wr[7] = c0 * VR[5] + c1 * VR[6] + c2 * VR[7];
-  Gas<R> gs{(R)pr.gamma, (R)pr.gm1, (R)pr.rhofloor, (R)pr.prsfloor};
-  R Fs[8], entF, SL, SR;
-  if (pr.riemann == CS_RS_HLLD)
-    hlld<R>(gs, wl, wr, VL[8], VR[8], Fs, entF, SL, SR);
-  else
-    hll<R>(gs, wl, wr, VL[8], VR[8], Fs, entF, SL, SR);
-  // rotate back (450make_facial_fluxes.f:995-1030) and scale by the area
   double out[9];
-  out[0] = (double)Fs[0];
-  out[4] = (double)Fs[4];
-  out[1] = A[0] * (double)Fs[1] + Bv[0] * (double)Fs[2] + Cv[0] * (double)Fs[3];
-  out[2] = A[1] * (double)Fs[1] + Bv[1] * (double)Fs[2] + Cv[1] * (double)Fs[3];
-  out[3] = A[2] * (double)Fs[1] + Bv[2] * (double)Fs[2] + Cv[2] * (double)Fs[3];
-  out[5] = A[0] * (double)Fs[5] + Bv[0] * (double)Fs[6] + Cv[0] * (double)Fs[7];
-  out[6] = A[1] * (double)Fs[5] + Bv[1] * (double)Fs[6] + Cv[1] * (double)Fs[7];
-  out[7] = A[2] * (double)Fs[5] + Bv[2] * (double)Fs[6] + Cv[2] * (double)Fs[7];
-  out[8] = (double)entF;
+  R SL, SR;
+  solve_face<R>(VL, VR, A, Bv, Cv, bface, pr, out, SL, SR);
   int64_t fo = (int64_t)s * 9 * d.fpp + d.face(i, j, k);
   for (int q = 0; q < 9; ++q) F[fo + q * d.fpp] = out[q] * area;
   int64_t so = (int64_t)s * 2 * d.fpp + d.face(i, j, k);
@@ -608,6 +619,94 @@ __global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, St
     L[1] += rho * gr * nx;
     L[2] += rho * gr * ny;
     L[3] += rho * gr * nz;
+    L[4] += mr * gr;
+  }
+  for (int q = 0; q < 5; ++q) {
+    int64_t o = (s * 8 + q) * d.cpp + c;
+    double un = in.U[o] + dt * L[q];
+    out.U[o] = wa != 0.0 ? wa * u0.U[o] + wb * un : wb * un;
+  }
+  {
+    int64_t o = s * d.cpp + c;
+    double en = in.ent[o] + dt * L[5];
+    out.ent[o] = wa != 0.0 ? wa * u0.ent[o] + wb * en : wb * en;
+  }
+}
+
+// Surface mode (one radial layer between conducting walls): the radial ghost
+// column of a zone is its own state mirrored about the radial face normal,
+// alternating (..., M(u), u, M(u), u | M(u), u, ...), so the two wall fluxes
+// are functions of the zone alone.  They are evaluated here, with exactly the
+// reconstruction and Riemann solve k_face_flux<2> applies to that column, and
+// neither the radial ghost fill nor the z-face pass is needed.
+template <typename R, typename P>
+__global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
+                                    const P* __restrict__ V, const double* __restrict__ Fx,
+                                    const double* __restrict__ Fy, const double* dtp, double wa,
+                                    double wb, Params pr) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  const double dt = dtp[0];
+  const int64_t c = d.cell(i, j, k);
+  const double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
+  const double ov = 1.0 / (geo.t(d, T_OM, s, i, j) * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0);
+  const int64_t fxp = d.face(i, j, k), fxm = d.face(i - 1, j, k), fym = d.face(i, j - 1, k);
+  const int comps[6] = {0, 1, 2, 3, 4, 8};
+  double L[6];
+  for (int q = 0; q < 6; ++q) {
+    int64_t o = ((int64_t)s * 9 + comps[q]) * d.fpp;
+    L[q] = -ov * ((Fx[o + fxp] - Fx[o + fxm]) + (Fy[o + fxp] - Fy[o + fym]));
+  }
+  // the zone and its mirror image (k_radial_bc arithmetic, then k_prims)
+  const double zn = geo.t(d, T_ZSN, s, i, j);
+  double A[3], Bv[3], Cv[3];
+  for (int q = 0; q < 3; ++q) { A[q] = geo.t(d, T_ZSX + q, s, i, j) / zn; Bv[q] = geo.t(d, T_ZTX + q, s, i, j); }
+  Cv[0] = A[1] * Bv[2] - A[2] * Bv[1];
+  Cv[1] = A[2] * Bv[0] - A[0] * Bv[2];
+  Cv[2] = A[0] * Bv[1] - A[1] * Bv[0];
+  double um[8];
+  for (int q = 0; q < 8; ++q) um[q] = in.U[(s * 8 + q) * d.cpp + c];
+  {
+#pragma clang fp contract(off)
+    double n0 = geo.t(d, T_ZSX, s, i, j) / zn, n1 = geo.t(d, T_ZSY, s, i, j) / zn, n2 = geo.t(d, T_ZSZ, s, i, j) / zn;
+    double m = um[1] * n0 + um[2] * n1 + um[3] * n2;
+    um[1] -= 2.0 * m * n0; um[2] -= 2.0 * m * n1; um[3] -= 2.0 * m * n2;
+    double b = um[5] * n0 + um[6] * n1 + um[7] * n2;
+    um[5] -= 2.0 * b * n0; um[6] -= 2.0 * b * n1; um[7] -= 2.0 * b * n2;
+  }
+  R Vc[9], Vm[9];
+  {
+    double vm[8], em = in.ent[s * d.cpp + c];
+    Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor};
+    cons_to_prim(gs, um, vm, em);
+    for (int q = 0; q < 8; ++q) Vm[q] = (R)(P)vm[q];
+    Vm[8] = (R)(P)em;
+    for (int q = 0; q < 9; ++q) Vc[q] = (R)V[((int64_t)s * 9 + q) * d.cpp + c];
+  }
+  // states of the outer wall face (k = 1: zone | ghost) and the inner one (ghost | zone)
+  R Wp[9], Wm[9];
+  const bool weno = pr.recon == CS_RECON_WENO5;
+  for (int q = 0; q < 9; ++q) {
+    Wp[q] = weno ? weno5<R>(Vc[q], Vm[q], Vc[q], Vm[q], Vc[q]) : Vc[q];
+    Wm[q] = weno ? weno5<R>(Vm[q], Vc[q], Vm[q], Vc[q], Vm[q]) : Vm[q];
+  }
+  const bool okp = (Wp[0] > R(0)) && (Wp[4] > R(0)) && (Wp[8] > R(0));
+  const bool okm = (Wm[0] > R(0)) && (Wm[4] > R(0)) && (Wm[8] > R(0));
+  const int64_t so = (int64_t)s * d.fpp;
+  double fo[9], fi[9];
+  R sl, sr;
+  solve_face<R>(okp ? Wp : Vc, okm ? Wm : Vm, A, Bv, Cv, in.Bz[so + d.face(i, j, k)], pr, fo, sl, sr);
+  solve_face<R>(okm ? Wm : Vm, okp ? Wp : Vc, A, Bv, Cv, in.Bz[so + d.face(i, j, k - 1)], pr, fi, sl, sr);
+  const double a1 = r1 * r1 * zn, a0 = r0 * r0 * zn;
+  for (int q = 0; q < 6; ++q) L[q] -= ov * (fo[comps[q]] * a1 - fi[comps[q]] * a0);
+  if (pr.grav != 0.0) {  // as in k_update_cells
+    double rc = 0.5 * (r0 + r1), gr = -pr.grav / (rc * rc);
+    double rho = in.U[(s * 8 + 0) * d.cpp + c];
+    double mr = in.U[(s * 8 + 1) * d.cpp + c] * A[0] + in.U[(s * 8 + 2) * d.cpp + c] * A[1] +
+                in.U[(s * 8 + 3) * d.cpp + c] * A[2];
+    L[1] += rho * gr * A[0];
+    L[2] += rho * gr * A[1];
+    L[3] += rho * gr * A[2];
     L[4] += mr * gr;
   }
   for (int q = 0; q < 5; ++q) {
@@ -840,6 +939,7 @@ struct cs_handle {
   int64_t npart = 0;
   hipGraphExec_t graph = nullptr;
   int cur = 0;  // which buffer holds U^n
+  int surf = 0; // one radial layer between walls: surface-mode stage kernels
 };
 
 namespace {
@@ -852,7 +952,7 @@ State state_of(cs_handle* h, int b) {
 }
 Geo geo_of(cs_handle* h) { return Geo{h->T, h->rb, h->d.t2 * 6, h->d.t2}; }
 
-int launch_halo(cs_handle* h, int b) {
+int launch_halo(cs_handle* h, int b, bool full = true) {
   const CsDims& d = h->d;
   State st = state_of(h, b);
   uint64_t n1 = 12ull * d.N * d.NR;
@@ -865,7 +965,8 @@ int launch_halo(cs_handle* h, int b) {
   k_corner_fill<<<nblk(n4), kBlock, 0, h->stream>>>(d, st);
   uint64_t nl = (uint64_t)(d.N + 2 * d.g + 1);
   uint64_t n5 = 12ull * d.g * nl * nl;
-  k_radial_bc<<<nblk(n5), kBlock, 0, h->stream>>>(d, geo_of(h), st, h->pr.wall);
+  // surface-mode stages never read the radial ghosts
+  if (full || !h->surf) k_radial_bc<<<nblk(n5), kBlock, 0, h->stream>>>(d, geo_of(h), st, h->pr.wall);
   CS_HIP(hipGetLastError());
   return CS_OK;
 }
@@ -916,7 +1017,7 @@ Ranges ranges_of(const CsDims& d, int wall) {
 
 // Launch one kernel of the stage on buffer b (update kernels use irk).
 template <typename R, typename P>
-int launch_kernel_t(cs_handle* h, int kid, int b, int irk) {
+int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
   const CsDims& d = h->d;
   const Ranges r = ranges_of(d, h->pr.wall);
   State st = state_of(h, b);
@@ -928,8 +1029,15 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk) {
   stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo);
   State in = state_of(h, (h->cur + bi) & 1), u0 = state_of(h, h->cur), out = state_of(h, (h->cur + bo) & 1);
   switch (kid) {
-    case CS_K_HALO: return launch_halo(h, b);
-    case CS_K_PRIMS: k_prims<P><<<nblk(r.all.total), kBlock, 0, sm>>>(d, r.all, st, V, h->pr); break;
+    case CS_K_HALO: return launch_halo(h, b, !surf);
+    case CS_K_PRIMS:
+      if (surf) {
+        Rng r1 = make_rng(1 - d.g, d.N + d.g, 1 - d.g, d.N + d.g, 1, 1);
+        k_prims<P><<<nblk(r1.total), kBlock, 0, sm>>>(d, r1, st, V, h->pr);
+      } else {
+        k_prims<P><<<nblk(r.all.total), kBlock, 0, sm>>>(d, r.all, st, V, h->pr);
+      }
+      break;
     case CS_K_FLUX_X:
       k_face_flux<0, R, P><<<nblk(r.fx.total), kBlock, 0, sm>>>(d, r.fx, geo, st, V, h->F[0], h->S[0], h->pr);
       break;
@@ -949,8 +1057,12 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk) {
       if (r.ey.total) k_emf_xy<1, P><<<nblk(r.ey.total), kBlock, 0, sm>>>(d, r.ey, geo, st, V, h->S[2], h->S[0], h->E[1], h->pr);
       break;
     case CS_K_UPDATE_CELLS:
-      k_update_cells<<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, h->F[0], h->F[1],
-                                                               h->F[2], h->dt, wa, wb, h->pr);
+      if (surf)
+        k_update_cells_surf<R, P><<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, V, h->F[0],
+                                                                          h->F[1], h->dt, wa, wb, h->pr);
+      else
+        k_update_cells<<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, h->F[0], h->F[1],
+                                                                 h->F[2], h->dt, wa, wb, h->pr);
       break;
     case CS_K_UPDATE_FACES:
       k_update_faces<<<nblk(r.faces.total), kBlock, 0, sm>>>(d, r.faces, geo, in, u0, out, h->E[0], h->E[1],
@@ -972,38 +1084,40 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk) {
   CS_HIP(hipGetLastError());
   return CS_OK;
 }
-int launch_kernel(cs_handle* h, int kid, int b, int irk) {
-  return h->cfg.precision == CS_MIXED ? launch_kernel_t<float, float>(h, kid, b, irk)
-                                      : launch_kernel_t<double, double>(h, kid, b, irk);
+// surf: use the surface-mode stage kernels (only from the stage path)
+int launch_kernel(cs_handle* h, int kid, int b, int irk, bool surf = false) {
+  return h->cfg.precision == CS_MIXED ? launch_kernel_t<float, float>(h, kid, b, irk, surf)
+                                      : launch_kernel_t<double, double>(h, kid, b, irk, surf);
 }
-int launch_seq(cs_handle* h, const int* ks, int n, int b, int irk) {
+int launch_seq(cs_handle* h, const int* ks, int n, int b, int irk, bool surf = false) {
   for (int q = 0; q < n; ++q) {
-    int rc = launch_kernel(h, ks[q], b, irk);
+    int rc = launch_kernel(h, ks[q], b, irk, surf);
     if (rc) return rc;
   }
   return CS_OK;
 }
-int launch_fluxes(cs_handle* h, int b) {
+int launch_fluxes(cs_handle* h, int b, bool surf = false) {
   static const int ks[4] = {CS_K_PRIMS, CS_K_FLUX_X, CS_K_FLUX_Y, CS_K_FLUX_Z};
-  return launch_seq(h, ks, 4, b, 1);
+  return launch_seq(h, ks, surf ? 3 : 4, b, 1, surf);
 }
-int launch_emf(cs_handle* h, int b) {
+int launch_emf(cs_handle* h, int b, bool surf = false) {
   static const int ks[3] = {CS_K_EMF_Z, CS_K_EMF_X, CS_K_EMF_Y};
-  return launch_seq(h, ks, 3, b, 1);
+  return launch_seq(h, ks, 3, b, 1, surf);
 }
-int launch_update(cs_handle* h, int irk) {
+int launch_update(cs_handle* h, int irk, bool surf = false) {
   static const int ks[3] = {CS_K_UPDATE_CELLS, CS_K_UPDATE_FACES, CS_K_CLEANUP};
-  return launch_seq(h, ks, 3, h->cur, irk);
+  return launch_seq(h, ks, 3, h->cur, irk, surf);
 }
 int launch_cfl(cs_handle* h) { return launch_kernel(h, CS_K_CFL, h->cur, 1); }
 
 int launch_stage(cs_handle* h, int irk) {
   int b = stage_input(h, irk);
   int rc;
-  if ((rc = launch_halo(h, b))) return rc;
-  if ((rc = launch_fluxes(h, b))) return rc;
-  if ((rc = launch_emf(h, b))) return rc;
-  return launch_update(h, irk);
+  const bool surf = h->surf != 0;
+  if ((rc = launch_halo(h, b, !surf))) return rc;
+  if ((rc = launch_fluxes(h, b, surf))) return rc;
+  if ((rc = launch_emf(h, b, surf))) return rc;
+  return launch_update(h, irk, surf);
 }
 
 int launch_step(cs_handle* h) {
@@ -1054,6 +1168,7 @@ extern "C" int cs_create(const cs_config* cfg, cs_handle** out) {
   h->d = cs_make_dims(cfg->ntp, cfg->nr, cfg->nghost);
   h->pr = Params{cfg->gamma, cfg->gamma - 1.0, cfg->rhofloor, cfg->prsfloor, cfg->cfl, cfg->grav,
                  cfg->recon, cfg->riemann, cfg->emf, cfg->radial_bc == CS_RBC_WALL ? 1 : 0};
+  h->surf = (cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1) ? 1 : 0;
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
     delete h;
@@ -1331,7 +1446,7 @@ extern "C" int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float
   CS_HIP(hipStreamSynchronize(h->stream));
   CS_HIP(hipEventRecord(e0, h->stream));
   for (int it = 0; it < iters; ++it) {
-    int rc = kernel == CS_K_STEP ? launch_step(h) : launch_kernel(h, kernel, h->cur, irk);
+    int rc = kernel == CS_K_STEP ? launch_step(h) : launch_kernel(h, kernel, h->cur, irk, h->surf != 0);
     if (rc) return rc;
   }
   CS_HIP(hipEventRecord(e1, h->stream));

@@ -98,9 +98,20 @@ def test_steps_match_oracle(name):
     Uo, eo = ref.interior()
     rel = np.abs(Ug - Uo).sum() / np.abs(Uo).sum()
     assert rel < 1e-10, f"U rel L1 {rel:.3e}"
+    assert rel_l1(eg, eo) < 1e-10
     assert abs(dt_g - dt_o) <= 1e-12 * dt_o
-    for a, b in zip(got.arrays()[2:], ref.arrays()[2:]):
-        assert _rel(a, b) < 1e-9
+    # owned faces (ghost layers are stage scratch; the surface-mode stage does
+    # not refresh the radial ghosts it never reads)
+    g, N, NR = cfg.nghost, cfg.ntp, cfg.nr
+    owned = [(slice(g + 1, g + NR + 1), slice(g + 1, g + N + 1), slice(g, g + N + 1)),
+             (slice(g + 1, g + NR + 1), slice(g, g + N + 1), slice(g + 1, g + N + 1)),
+             (slice(g, g + NR + 1), slice(g + 1, g + N + 1), slice(g + 1, g + N + 1))]
+    for a, b, sl in zip(got.views()[2:], ref.views()[2:], owned):
+        assert _rel(a[(slice(None),) + sl], b[(slice(None),) + sl]) < 1e-9
+
+
+def rel_l1(a, b):
+    return np.abs(a - b).sum() / np.abs(b).sum()
 
 
 def test_graph_replay_equals_eager():
