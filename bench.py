@@ -36,26 +36,37 @@ def word_bytes(prec):
     return 8 if prec == 0 else 4
 
 
-def algorithmic_bytes_per_unit(cs, kernel, prec):
-    """Compulsory HBM bytes per work item of a kernel (state in, result out).
-
-    face flux: the zone primitives it reconstructs from (9 words, each zone is
-    read once per sweep), the face B (1 FP64 word), the flux (9 FP64) and the
-    two speeds (2 FP64) written.
+def algorithmic_bytes_per_unit(cs, kernel, prec, surface):
+    """Compulsory HBM bytes per work item of a kernel: every input it must
+    read once and every output it must write once, with data shared between
+    neighbouring items (a zone read by two faces, a face flux read by two
+    zones) counted once.  Words of the primitive array V are FP32 in mixed
+    precision, everything else is FP64.  With one radial layer (surface) the
+    2-D metric tables are per zone and count; in a 3-D shell they are reused
+    by every layer and do not.
     """
     w = word_bytes(prec)
+    t2 = 1 if surface else 0
     if kernel in (cs.K_FLUX_X, cs.K_FLUX_Y, cs.K_FLUX_Z):
-        return 9 * w + 8 + 11 * 8
+        # V of one zone, face B, face frame (normal, radial dir, angle), flux (9) + speeds (2) out
+        return 9 * w + 8 + t2 * 7 * 8 + 11 * 8
     if kernel == cs.K_PRIMS:
-        return 9 * 8 + 9 * w
+        return 9 * 8 + 9 * w                      # U + entropy in, V out
     if kernel in (cs.K_EMF_X, cs.K_EMF_Y, cs.K_EMF_Z):
-        return 6 * w + 4 * 8 + 8           # v, B of one zone, 4 face B... amortised, E out
+        # v and B of one zone, two face B, four speeds, edge frame (5), EMF out
+        return 6 * w + 2 * 8 + 4 * 8 + t2 * 5 * 8 + 8
     if kernel == cs.K_UPDATE_CELLS:
-        return (6 + 6 + 6) * 8 + 3 * 6 * 8  # in, U0, out of 6 words + 3 face fluxes of 6 words
+        if surface:
+            # two lateral fluxes of 6 components, U + entropy in (9), U0 and out (6 each),
+            # V (9), the two wall z-face B, zone metric (solid angle, area vector, frame: 8)
+            return 12 * 8 + 9 * 8 + 6 * 8 + 6 * 8 + 9 * w + 2 * 8 + 8 * 8
+        return 18 * 8 + 3 * 6 * 8                 # three lateral+radial fluxes of 6; in, U0, out of 6
     if kernel == cs.K_UPDATE_FACES:
-        return 3 * (3 * 8) + 3 * 8          # in, U0, out of 3 face B + 3 edge EMF
+        return 4 * 3 * 8 + t2 * 3 * 8             # in, U0, out B and the EMF of 3 families; areas
     if kernel == cs.K_CLEANUP:
-        return 9 * 8 + 3 * 8 + 9 * 8
+        return (8 + 1 + 3) * 8 + (8 + 1) * 8 + t2 * 9 * 8   # U, entropy, face B in; U, entropy out; M
+    if kernel == cs.K_HALO:
+        return 2 * 9 * 8                          # one ghost zone: U + entropy read and written
     return 0
 
 
@@ -132,6 +143,31 @@ def cpu_baseline(cs, name, max_seconds):
                       f"single thread, {el:.1f} s"}
 
 
+def probe(cs, name, prec, kernel, iters):
+    """Hardware-counter run: `iters` launches of one kernel on the workload's
+    state, then one plain device copy of 1 GiB (read) + 1 GiB (write), whose
+    counter readings calibrate the byte counters."""
+    import torch
+    cfg, prob, _ = cs.workload(name, prec)
+    st = cs.init_problem(cfg, prob, seed=20261202)
+    dev = cs.CubeSphere(cfg)
+    dev.upload(st)
+    dev.dt = 0.0
+    dev.evaluate_timestep()
+    dev.advance(2, use_graph=False)
+    kid = cs.KERNEL_NAMES.index(kernel)
+    ms = dev.time_kernel(kid, iters)
+    units = dev.kernel_units(kid)
+    dev.close()
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"probe {kernel} ({'fp64' if prec == 0 else 'mixed'}): {iters} launches, {ms * 1e3:.2f} us avg, "
+        f"{units} units; calibration copy {a.numel() * 8} bytes read and written")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,6 +178,10 @@ def main():
     ap.add_argument("--gate-steps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", default=None, metavar="KERNEL",
+                    help="only launch this kernel --profile-iters times plus a calibration copy "
+                         "(for rocprofv3 --pmc runs); prints no JSON line")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64", help="precision of --probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design "
@@ -151,6 +191,10 @@ def main():
 
     import cubesphere_amd as cs
     log("device:", cs.device_info())
+    if args.probe:
+        probe(cs, args.workload, cs.FP64 if args.precision == "fp64" else cs.MIXED, args.probe,
+              args.profile_iters)
+        return
     res = {}
     for prec in (cs.FP64, cs.MIXED):
         try:
@@ -166,8 +210,9 @@ def main():
         res[prec] = dict(cfg=cfg, ms=ms, kern=kern, divb=divb, sums=sums, dt=dt, cups=cups, n_int=n_int)
         log(f"prec={'fp64' if prec == 0 else 'mixed'}: {ms / args.steps:.4f} ms/step, "
             f"{cups:.3e} cell-updates/s, max rel divB {divb:.2e}, dt {dt:.3e}")
+        surface = cfg.nr == 1 and cfg.radial_bc == cs.RBC_WALL
         for k, (t, u, kid) in sorted(kern.items(), key=lambda kv: -kv[1][0]):
-            b = algorithmic_bytes_per_unit(cs, kid, prec) * u
+            b = algorithmic_bytes_per_unit(cs, kid, prec, surface) * u
             log(f"   {k:14s} {t * 1e3:9.2f} us  units {u:10d}  alg {b / (t * 1e-3) / 1e9 if t > 0 else 0:8.1f} GB/s")
 
     try:
@@ -180,16 +225,19 @@ def main():
     r = res[head]
 
     # dominant kernel of the headline precision
+    cfg = r["cfg"]
+    surface = cfg.nr == 1 and cfg.radial_bc == cs.RBC_WALL
     kname, (kt, ku, kid) = max(r["kern"].items(), key=lambda kv: kv[1][0])
-    alg = algorithmic_bytes_per_unit(cs, kid, head) * ku
+    alg = algorithmic_bytes_per_unit(cs, kid, head, surface) * ku
     achieved = alg / (kt * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "alg_bytes_per_launch": alg, "avg_launch_us": kt * 1e3}
-    # whole-step view against the step's compulsory traffic
-    cfg = r["cfg"]
-    surface = cfg.nr == 1
-    b_alg = 1392 if surface else 768
+    # whole-step view: the compulsory bytes of every kernel of a stage, times
+    # the stages of a step, per zone update
+    stage_bytes = sum(algorithmic_bytes_per_unit(cs, kid_, head, surface) * u_
+                      for (_, u_, kid_) in r["kern"].values())
+    b_alg = stage_bytes * cfg.rk_order / r["n_int"]
     step_gbs = r["cups"] * b_alg / 1e9
 
     cpu = None

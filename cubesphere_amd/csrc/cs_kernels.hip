@@ -585,6 +585,108 @@ __global__ void k_emf_xy(CsDims d, Rng rg, Geo geo, State st, const P* V, const 
   Eo[fo] = e * len;
 }
 
+// ------------------------------------------------------------------ shared faces and edges
+// Both sectors that meet at a panel edge compute its faces and edges, from
+// states that differ only where a stencil reaches a ghost row.  The reference
+// makes them unique after the update by its defect synchronisation: the two
+// copies of a coinciding face flux or edge EMF are averaged
+// (334fetch_buffers_fill_in_flux_x.f:290-400, 335/336 for y and z) and the
+// radial edge at a cube corner takes the mean of its three copies
+// (334...:371-380 accumulation, 340synchronize_efz_quad_corner.f).  Here the
+// same averages are taken before the update, so the defect is zero and div B
+// is kept at roundoff.  One thread per physical face/edge writes every copy:
+//   flux pairs   (a + (-b)) * 0.5: the two face normals are opposite;
+//   z-edge pairs (a + b) * 0.5: both store E . rhat * dr;
+//   x/y-edge pairs (a + sg b) * 0.5 with sg = -1 where the two sectors run
+//   the edge in opposite directions (left-left and down-down panel edges);
+//   cube corners (e0 + e1 + e2) * (1/3) in ascending sector order.
+struct CornerTab {
+  int s[8][3], i[8][3], j[8][3];
+};
+
+__device__ __forceinline__ void avg_pair(double* pa, double* pb, double sg) {
+  double a = *pa, b = *pb;
+  *pa = (a + sg * b) * 0.5;
+  *pb = (b + sg * a) * 0.5;
+}
+
+template <bool FLUX, bool EDGE>
+__global__ void k_sync_shared(CsDims d, double* Fx, double* Fy, double* Ex, double* Ey, double* Ez,
+                              CornerTab ct, int kh0, int nkh) {
+  uint32_t t = gtid();
+  const int N = d.N, NR = d.NR;
+  const uint32_t nF = FLUX ? 12u * N * NR : 0u;
+  const uint32_t nZ = EDGE ? 12u * (N - 1) * NR : 0u;
+  const uint32_t nH = EDGE ? 12u * N * nkh : 0u;
+  const uint32_t nC = EDGE ? 8u * NR : 0u;
+  if (t < nF) {
+    uint32_t per = (uint32_t)N * NR;
+    int p = t / per;
+    uint32_t rem = t - p * per;
+    int tt = 1 + (int)(rem % N), k = 1 + (int)(rem / N);
+    double *pa, *pb;
+    if (p < 6) {         // right edge of sector p with the top edge of its right neighbour
+      int s = p, o = c_nbr[s][0];
+      pa = Fx + (int64_t)s * 9 * d.fpp + d.face(N, tt, k);
+      pb = Fy + (int64_t)o * 9 * d.fpp + d.face(tt, N, k);
+    } else if (p < 9) {  // left edges 1-6, 2-4, 3-5 (1-based)
+      int s = p - 6, o = c_nbr[s][1];
+      pa = Fx + (int64_t)s * 9 * d.fpp + d.face(0, tt, k);
+      pb = Fx + (int64_t)o * 9 * d.fpp + d.face(0, N + 1 - tt, k);
+    } else {             // bottom edges 1-5, 2-6, 3-4
+      int s = p - 9, o = c_nbr[s][3];
+      pa = Fy + (int64_t)s * 9 * d.fpp + d.face(tt, 0, k);
+      pb = Fy + (int64_t)o * 9 * d.fpp + d.face(N + 1 - tt, 0, k);
+    }
+    for (int q = 0; q < 9; ++q) avg_pair(pa + q * d.fpp, pb + q * d.fpp, -1.0);
+    return;
+  }
+  t -= nF;
+  if (t < nZ) {          // radial edges along panel edges, vertices 1..N-1
+    uint32_t per = (uint32_t)(N - 1) * NR;
+    int p = t / per;
+    uint32_t rem = t - p * per;
+    int tt = 1 + (int)(rem % (N - 1)), k = 1 + (int)(rem / (N - 1));
+    int64_t a, b;
+    if (p < 6) { int s = p, o = c_nbr[s][0]; a = s * d.fpp + d.face(N, tt, k); b = o * d.fpp + d.face(tt, N, k); }
+    else if (p < 9) { int s = p - 6, o = c_nbr[s][1]; a = s * d.fpp + d.face(0, tt, k); b = o * d.fpp + d.face(0, N - tt, k); }
+    else { int s = p - 9, o = c_nbr[s][3]; a = s * d.fpp + d.face(tt, 0, k); b = o * d.fpp + d.face(N - tt, 0, k); }
+    avg_pair(Ez + a, Ez + b, 1.0);
+    return;
+  }
+  t -= nZ;
+  if (t < nH) {          // x/y-edges along panel edges, levels kh0..kh0+nkh-1
+    uint32_t per = (uint32_t)N * nkh;
+    int p = t / per;
+    uint32_t rem = t - p * per;
+    int tt = 1 + (int)(rem % N), k = kh0 + (int)(rem / N);
+    double *pa, *pb, sg;
+    if (p < 6) {
+      int s = p, o = c_nbr[s][0];
+      pa = Ey + s * d.fpp + d.face(N, tt, k); pb = Ex + o * d.fpp + d.face(tt, N, k); sg = 1.0;
+    } else if (p < 9) {
+      int s = p - 6, o = c_nbr[s][1];
+      pa = Ey + s * d.fpp + d.face(0, tt, k); pb = Ey + o * d.fpp + d.face(0, N + 1 - tt, k); sg = -1.0;
+    } else {
+      int s = p - 9, o = c_nbr[s][3];
+      pa = Ex + s * d.fpp + d.face(tt, 0, k); pb = Ex + o * d.fpp + d.face(N + 1 - tt, 0, k); sg = -1.0;
+    }
+    avg_pair(pa, pb, sg);
+    return;
+  }
+  t -= nH;
+  if (t < nC) {          // radial edges at the eight cube corners
+    int c = t / NR, k = 1 + (int)(t % NR);
+    double* p0 = Ez + ct.s[c][0] * d.fpp + d.face(ct.i[c][0], ct.j[c][0], k);
+    double* p1 = Ez + ct.s[c][1] * d.fpp + d.face(ct.i[c][1], ct.j[c][1], k);
+    double* p2 = Ez + ct.s[c][2] * d.fpp + d.face(ct.i[c][2], ct.j[c][2], k);
+    double v = (*p0 + *p1 + *p2) * (1.0 / 3.0);
+    *p0 = v;
+    *p1 = v;
+    *p2 = v;
+  }
+}
+
 // ------------------------------------------------------------------ update
 // U_out = a U0 + b (U_in + dt L(U_in)) for rho, rho v, E and entropy
 // (UPDATE_IMEX_RK_STEP_P1 484update_imex_rk_step_p1.f:483-497, 547-593 with the
@@ -940,6 +1042,7 @@ struct cs_handle {
   hipGraphExec_t graph = nullptr;
   int cur = 0;  // which buffer holds U^n
   int surf = 0; // one radial layer between walls: surface-mode stage kernels
+  CornerTab ct{}; // the three (sector, i, j) copies of each cube-corner vertex
 };
 
 namespace {
@@ -1015,6 +1118,38 @@ Ranges ranges_of(const CsDims& d, int wall) {
   return r;
 }
 
+// Number of x/y-edge levels along panel edges that carry an EMF: walls hold
+// zero EMF on their tangential edges.
+void sync_levels(const cs_handle* h, int& kh0, int& nkh) {
+  kh0 = h->pr.wall ? 1 : 0;
+  nkh = h->pr.wall ? h->d.NR - 1 : h->d.NR + 1;
+}
+uint64_t sync_units(const cs_handle* h, bool flux, bool edge) {
+  const CsDims& d = h->d;
+  int kh0, nkh;
+  sync_levels(h, kh0, nkh);
+  uint64_t n = 0;
+  if (flux) n += 12ull * d.N * d.NR;
+  if (edge) n += 12ull * (d.N - 1) * d.NR + 12ull * d.N * nkh + 8ull * d.NR;
+  return n;
+}
+int launch_sync(cs_handle* h, bool flux, bool edge) {
+  int kh0, nkh;
+  sync_levels(h, kh0, nkh);
+  uint64_t n = sync_units(h, flux, edge);
+  if (flux && edge)
+    k_sync_shared<true, true><<<nblk(n), kBlock, 0, h->stream>>>(h->d, h->F[0], h->F[1], h->E[0], h->E[1], h->E[2],
+                                                                 h->ct, kh0, nkh);
+  else if (flux)
+    k_sync_shared<true, false><<<nblk(n), kBlock, 0, h->stream>>>(h->d, h->F[0], h->F[1], h->E[0], h->E[1], h->E[2],
+                                                                  h->ct, kh0, nkh);
+  else
+    k_sync_shared<false, true><<<nblk(n), kBlock, 0, h->stream>>>(h->d, h->F[0], h->F[1], h->E[0], h->E[1], h->E[2],
+                                                                  h->ct, kh0, nkh);
+  CS_HIP(hipGetLastError());
+  return CS_OK;
+}
+
 // Launch one kernel of the stage on buffer b (update kernels use irk).
 template <typename R, typename P>
 int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
@@ -1071,6 +1206,8 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
     case CS_K_CLEANUP:
       k_cleanup<<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, out, h->pr, h->flags);
       break;
+    case CS_K_SYNC:
+      return launch_sync(h, true, true);
     case CS_K_CFL: {
       unsigned nb = nblk(r.cells.total);
       k_cfl_partial<<<nb, kBlock, 0, sm>>>(d, r.cells, geo, state_of(h, h->cur), h->pr, h->part);
@@ -1096,13 +1233,20 @@ int launch_seq(cs_handle* h, const int* ks, int n, int b, int irk, bool surf = f
   }
   return CS_OK;
 }
-int launch_fluxes(cs_handle* h, int b, bool surf = false) {
+// sync: make the panel-edge fluxes unique here (the stage path defers it to
+// the one shared-face/edge launch after the EMFs)
+int launch_fluxes(cs_handle* h, int b, bool surf = false, bool sync = true) {
   static const int ks[4] = {CS_K_PRIMS, CS_K_FLUX_X, CS_K_FLUX_Y, CS_K_FLUX_Z};
-  return launch_seq(h, ks, surf ? 3 : 4, b, 1, surf);
+  int rc = launch_seq(h, ks, surf ? 3 : 4, b, 1, surf);
+  if (rc || !sync) return rc;
+  return launch_sync(h, true, false);
 }
-int launch_emf(cs_handle* h, int b, bool surf = false) {
+// with_fluxes: also make the panel-edge fluxes unique in the same launch
+int launch_emf(cs_handle* h, int b, bool surf = false, bool with_fluxes = false) {
   static const int ks[3] = {CS_K_EMF_Z, CS_K_EMF_X, CS_K_EMF_Y};
-  return launch_seq(h, ks, 3, b, 1, surf);
+  int rc = launch_seq(h, ks, 3, b, 1, surf);
+  if (rc) return rc;
+  return launch_sync(h, with_fluxes, true);
 }
 int launch_update(cs_handle* h, int irk, bool surf = false) {
   static const int ks[3] = {CS_K_UPDATE_CELLS, CS_K_UPDATE_FACES, CS_K_CLEANUP};
@@ -1115,8 +1259,8 @@ int launch_stage(cs_handle* h, int irk) {
   int rc;
   const bool surf = h->surf != 0;
   if ((rc = launch_halo(h, b, !surf))) return rc;
-  if ((rc = launch_fluxes(h, b, surf))) return rc;
-  if ((rc = launch_emf(h, b, surf))) return rc;
+  if ((rc = launch_fluxes(h, b, surf, false))) return rc;
+  if ((rc = launch_emf(h, b, surf, true))) return rc;
   return launch_update(h, irk, surf);
 }
 
@@ -1136,6 +1280,41 @@ void free_all(cs_handle* h) {
   for (int q = 0; q < 3; ++q) { hipFree(h->F[q]); hipFree(h->S[q]); hipFree(h->E[q]); }
   hipFree(h->dt); hipFree(h->part); hipFree(h->flags);
   if (h->stream) hipStreamDestroy(h->stream);
+}
+
+// The eight cube corners: group the 24 panel-corner vertices by position on
+// the unit sphere (each corner is shared by three sectors), ascending sector
+// order within a group.
+int build_corner_table(const CsDims& d, const std::vector<double>& T, CornerTab& ct) {
+  const int64_t plane = d.t2 * 6;
+  struct V { int s, i, j; double u[3]; bool used; } v[24];
+  int n = 0;
+  for (int s = 0; s < 6; ++s)
+    for (int c = 0; c < 4; ++c) {
+      V& x = v[n++];
+      x.s = s; x.i = (c & 1) ? d.N : 0; x.j = (c & 2) ? d.N : 0; x.used = false;
+      for (int q = 0; q < 3; ++q) x.u[q] = T[(T_UX + q) * plane + s * d.t2 + d.t(x.i, x.j)];
+    }
+  int nc = 0;
+  for (int a = 0; a < 24; ++a) {
+    if (v[a].used) continue;
+    int m[3] = {a, -1, -1}, nm = 1;
+    for (int b = a + 1; b < 24 && nm <= 3; ++b) {
+      double dx = v[a].u[0] - v[b].u[0], dy = v[a].u[1] - v[b].u[1], dz = v[a].u[2] - v[b].u[2];
+      if (!v[b].used && dx * dx + dy * dy + dz * dz < 1e-20) {
+        if (nm == 3) { nm = 4; break; }
+        m[nm++] = b;
+      }
+    }
+    if (nm != 3 || nc == 8) { set_error("cube-corner vertices do not group in threes"); return CS_E_INVALID; }
+    for (int q = 0; q < 3; ++q) {  // v is in ascending sector order already
+      v[m[q]].used = true;
+      ct.s[nc][q] = v[m[q]].s; ct.i[nc][q] = v[m[q]].i; ct.j[nc][q] = v[m[q]].j;
+    }
+    ++nc;
+  }
+  if (nc != 8) { set_error("found %d cube corners", nc); return CS_E_INVALID; }
+  return CS_OK;
 }
 
 #define ALLOC(ptr, bytes)                                                           \
@@ -1169,6 +1348,10 @@ extern "C" int cs_create(const cs_config* cfg, cs_handle** out) {
   h->pr = Params{cfg->gamma, cfg->gamma - 1.0, cfg->rhofloor, cfg->prsfloor, cfg->cfl, cfg->grav,
                  cfg->recon, cfg->riemann, cfg->emf, cfg->radial_bc == CS_RBC_WALL ? 1 : 0};
   h->surf = (cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1) ? 1 : 0;
+  if ((rc = build_corner_table(h->d, T, h->ct))) {
+    delete h;
+    return rc;
+  }
   if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
     set_error("hipStreamCreate failed");
     delete h;
@@ -1475,6 +1658,7 @@ extern "C" int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units) {
     case CS_K_EMF_X: n = r.ex.total; break;
     case CS_K_EMF_Y: n = r.ey.total; break;
     case CS_K_UPDATE_FACES: n = r.faces.total; break;
+    case CS_K_SYNC: n = (int64_t)sync_units(h, true, true); break;
     default: n = r.cells.total; break;
   }
   *units = n;

@@ -191,6 +191,9 @@ struct or_grid {
   cmap_t* cmap; int ncmap;
   fmap_t* fmap; int nfmap;
   fmap_t* pairs; int npairs;
+  fmap_t* epairs; int nepairs;   /* panel-edge edges: fam 0 Efx, 1 Efy, 2 Efz */
+  int corner[8][3][3];           /* cube corners: 3 x (sector, i, j), ascending sector */
+  int ncorner;
   int fails;
 };
 
@@ -484,6 +487,142 @@ static void build_maps(or_grid* G) {
   free(faces);
 }
 
+/* Panel-edge edges shared by two sectors and the cube-corner vertices shared
+   by three, found by position on the unit sphere (no index formulas).  An
+   x-edge (i, j) runs from vertex (i-1, j) to (i, j), a y-edge (i, j) from
+   (i, j-1) to (i, j); two copies of one edge that run in opposite directions
+   pair with sign -1.  A radial edge lies on the panel-edge vertex it starts
+   from and always points along +r. */
+typedef struct { double a[3], b[3]; int fam, s, i, j; } eobj;
+static void build_edge_maps(or_grid* G) {
+  int N = G->N, s, t, e, q, nv = 0, ne = 0;
+  double tol2 = 1e-20;
+  eobj* v = malloc(sizeof(eobj) * 6 * 4 * (N + 1));
+  eobj* ed = malloc(sizeof(eobj) * 6 * 4 * N);
+  for (s = 0; s < 6; ++s)
+    for (t = 0; t <= N; ++t) {
+      int vi[4] = {0, N, t, t}, vj[4] = {t, t, 0, N};
+      for (e = 0; e < 4; ++e) {
+        if (e >= 2 && (t == 0 || t == N)) continue; /* corners listed once */
+        v[nv].fam = 2; v[nv].s = s; v[nv].i = vi[e]; v[nv].j = vj[e];
+        uv(G, s, vi[e], vj[e], v[nv].a);
+        ++nv;
+      }
+    }
+  for (s = 0; s < 6; ++s)
+    for (t = 1; t <= N; ++t) {
+      /* y-edges on i = 0, N; x-edges on j = 0, N */
+      int fi[4] = {0, N, t, t}, fj[4] = {t, t, 0, N}, ff[4] = {1, 1, 0, 0};
+      for (e = 0; e < 4; ++e) {
+        eobj* o = &ed[ne++];
+        o->fam = ff[e]; o->s = s; o->i = fi[e]; o->j = fj[e];
+        if (ff[e] == 1) { uv(G, s, fi[e], fj[e] - 1, o->a); uv(G, s, fi[e], fj[e], o->b); }
+        else { uv(G, s, fi[e] - 1, fj[e], o->a); uv(G, s, fi[e], fj[e], o->b); }
+      }
+    }
+  G->epairs = malloc(sizeof(fmap_t) * (nv + ne));
+  G->nepairs = 0;
+  G->ncorner = 0;
+  for (q = 0; q < nv; ++q) {
+    int m[4], nm = 0, r;
+    for (r = 0; r < nv; ++r) {
+      double dx = v[r].a[0] - v[q].a[0], dy = v[r].a[1] - v[q].a[1], dz = v[r].a[2] - v[q].a[2];
+      if (v[r].s != v[q].s && dx * dx + dy * dy + dz * dz < tol2 && nm < 4) m[nm++] = r;
+    }
+    if (nm == 1) {
+      if (v[m[0]].s < v[q].s) continue;
+      {
+        fmap_t* p = &G->epairs[G->nepairs++];
+        p->fam = 2; p->s = v[q].s; p->i = v[q].i; p->j = v[q].j;
+        p->sfam = 2; p->ss = v[m[0]].s; p->si = v[m[0]].i; p->sj = v[m[0]].j; p->sgn = 1.0;
+      }
+    } else if (nm == 2) {
+      /* record from the lowest sector of the three; v is in sector order */
+      if (v[m[0]].s < v[q].s || v[m[1]].s < v[q].s || G->ncorner == 8) continue;
+      {
+        int (*c)[3] = G->corner[G->ncorner++];
+        int lo = v[m[0]].s < v[m[1]].s ? 0 : 1;
+        c[0][0] = v[q].s; c[0][1] = v[q].i; c[0][2] = v[q].j;
+        c[1][0] = v[m[lo]].s; c[1][1] = v[m[lo]].i; c[1][2] = v[m[lo]].j;
+        c[2][0] = v[m[1 - lo]].s; c[2][1] = v[m[1 - lo]].i; c[2][2] = v[m[1 - lo]].j;
+      }
+    } else {
+      G->fails++;
+    }
+  }
+  for (q = 0; q < ne; ++q) {
+    int r, found = 0;
+    for (r = 0; r < ne; ++r) {
+      double d1 = 0, d2 = 0, d3 = 0, d4 = 0;
+      int c;
+      if (ed[r].s == ed[q].s) continue;
+      for (c = 0; c < 3; ++c) {
+        d1 += (ed[r].a[c] - ed[q].a[c]) * (ed[r].a[c] - ed[q].a[c]);
+        d2 += (ed[r].b[c] - ed[q].b[c]) * (ed[r].b[c] - ed[q].b[c]);
+        d3 += (ed[r].a[c] - ed[q].b[c]) * (ed[r].a[c] - ed[q].b[c]);
+        d4 += (ed[r].b[c] - ed[q].a[c]) * (ed[r].b[c] - ed[q].a[c]);
+      }
+      if ((d1 < tol2 && d2 < tol2) || (d3 < tol2 && d4 < tol2)) {
+        found = 1;
+        if (ed[r].s < ed[q].s) break;
+        {
+          fmap_t* p = &G->epairs[G->nepairs++];
+          p->fam = ed[q].fam; p->s = ed[q].s; p->i = ed[q].i; p->j = ed[q].j;
+          p->sfam = ed[r].fam; p->ss = ed[r].s; p->si = ed[r].i; p->sj = ed[r].j;
+          p->sgn = (d1 < tol2 && d2 < tol2) ? 1.0 : -1.0;
+        }
+        break;
+      }
+    }
+    if (!found) G->fails++;
+  }
+  if (G->ncorner != 8) G->fails++;
+  free(v);
+  free(ed);
+}
+
+/* Shared-face fluxes and shared-edge EMFs made unique across sectors: the
+   averages of the reference's defect synchronisation
+   (334fetch_buffers_fill_in_flux_x.f:290-400, 335, 336;
+   340synchronize_efz_quad_corner.f), taken before the update. */
+static void sync_fluxes(const or_grid* G, double** F) {
+  int m, k, q;
+  for (m = 0; m < G->npairs; ++m) {
+    const fmap_t* p = &G->pairs[m];
+    for (k = 1; k <= G->NR; ++k)
+      for (q = 0; q < 9; ++q) {
+        double* pa = &F[p->fam][(p->s * 9 + q) * G->fpp + FACE(G, p->i, p->j, k)];
+        double* pb = &F[p->sfam][(p->ss * 9 + q) * G->fpp + FACE(G, p->si, p->sj, k)];
+        double va = *pa, vb = *pb;
+        *pa = (va + p->sgn * vb) * 0.5;
+        *pb = (vb + p->sgn * va) * 0.5;
+      }
+  }
+}
+static void sync_edges(const or_grid* G, double** E) {
+  int m, k;
+  for (m = 0; m < G->nepairs; ++m) {
+    const fmap_t* p = &G->epairs[m];
+    int k0 = p->fam == 2 ? 1 : 0;
+    for (k = k0; k <= G->NR; ++k) {
+      double* pa = &E[p->fam][p->s * G->fpp + FACE(G, p->i, p->j, k)];
+      double* pb = &E[p->sfam][p->ss * G->fpp + FACE(G, p->si, p->sj, k)];
+      double va = *pa, vb = *pb;
+      *pa = (va + p->sgn * vb) * 0.5;
+      *pb = (vb + p->sgn * va) * 0.5;
+    }
+  }
+  for (m = 0; m < G->ncorner; ++m)
+    for (k = 1; k <= G->NR; ++k) {
+      const int(*c)[3] = G->corner[m];
+      double* p0 = &E[2][c[0][0] * G->fpp + FACE(G, c[0][1], c[0][2], k)];
+      double* p1 = &E[2][c[1][0] * G->fpp + FACE(G, c[1][1], c[1][2], k)];
+      double* p2 = &E[2][c[2][0] * G->fpp + FACE(G, c[2][1], c[2][2], k)];
+      double val = (*p0 + *p1 + *p2) * (1.0 / 3.0);
+      *p0 = *p1 = *p2 = val;
+    }
+}
+
 or_grid* or_grid_new(const or_config* c) {
   or_grid* G;
   int q;
@@ -512,6 +651,7 @@ or_grid* or_grid_new(const or_config* c) {
   for (q = 0; q < 9; ++q) G->mi[q] = al(n);
   build_geometry(G);
   build_maps(G);
+  build_edge_maps(G);
   return G;
 }
 
@@ -525,7 +665,7 @@ void or_grid_free(or_grid* G) {
   }
   free(G->thy); free(G->thx); free(G->zsn); free(G->om); free(G->chmin); free(G->ecos); free(G->esin);
   for (q = 0; q < 9; ++q) free(G->mi[q]);
-  free(G->cmap); free(G->fmap); free(G->pairs);
+  free(G->cmap); free(G->fmap); free(G->pairs); free(G->epairs);
   free(G);
 }
 
@@ -1000,6 +1140,7 @@ void or_fluxes(const or_grid* G, const double* U, const double* ent, const doubl
                const double* Bz, double** F, double** S) {
   double* V = prims_all(G, U, ent);
   fluxes_from_V(G, V, Bx, By, Bz, F, S);
+  sync_fluxes(G, F);
   free(V);
 }
 
@@ -1130,6 +1271,7 @@ void or_emf(const or_grid* G, const double* U, const double* ent, const double* 
             double** S, double** E) {
   double* V = prims_all(G, U, ent);
   emf_from_V(G, V, Bx, By, Bz, S, E);
+  sync_edges(G, E);
   free(V);
 }
 
@@ -1264,7 +1406,9 @@ void or_stage(const or_grid* G, int irk, double dt, double* const* in, double* c
   or_halo(G, in[0], in[1], in[2], in[3], in[4]);
   V = prims_all(G, in[0], in[1]);
   fluxes_from_V(G, V, in[2], in[3], in[4], F, S);
+  sync_fluxes(G, F);
   emf_from_V(G, V, in[2], in[3], in[4], S, E);
+  sync_edges(G, E);
   free(V);
   or_update(G, irk, dt, (const double* const*)in, (const double* const*)u0, out, F, E);
   free_set(F, 3); free_set(S, 3); free_set(E, 3);

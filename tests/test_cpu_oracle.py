@@ -285,7 +285,69 @@ def test_divb_and_mass_preserved_by_oracle_steps():
         return np.abs(div).max() / scale, float((V * U[:, 0, 0]).sum())
     d0, m0 = divb_and_mass(st)
     dt = G.cfl(st, 0.0)
-    G.advance(st, 4, dt)
+    # 20 steps: the panel-edge EMFs of the two sectors sharing an edge must
+    # be identical, or the boundary zones drift by ~1e-10 within this many steps
+    G.advance(st, 20, dt)
     d1, m1 = divb_and_mass(st)
     assert d0 < 1e-14 and d1 < 1e-13, (d0, d1)
     assert abs(m1 - m0) < 1e-13 * m0
+
+
+def test_shared_edges_and_faces_are_unique():
+    """After the synchronisation every physical panel-edge face flux and edge
+    EMF has one value: the copies held by the sectors that share it agree
+    (with the orientation sign), checked against geometry, not index maps."""
+    cfg = cs.make_config(10, 2, 3, cs.RECON_WENO5, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 3, cs.RBC_PAD,
+                         cs.RMESH_UNIFORM, 1.0, 1.4)
+    st = cs.init_problem(cfg, 3, seed=4)
+    G = orc.Grid(cfg)
+    G.halo(st)
+    F, S = G.fluxes(st)
+    E = G.emf(st, S)
+    g, N, NR = cfg.nghost, cfg.ntp, cfg.nr
+    nf, nrf = N + 2 * g + 1, NR + 2 * g + 1
+    Ex, Ey, Ez = (e.reshape(6, nrf, nf, nf) for e in E)
+    lat = cs.unit_lattice(cfg).reshape(6, 3, nf, nf)   # unit vertices [s, c, j+g, i+g]
+
+    def key(s, i, j):
+        return tuple(np.round(lat[s, :, j + g, i + g], 9))
+
+    pos = {}
+    for s in range(6):
+        for t in range(N + 1):
+            for (i, j) in ((0, t), (N, t), (t, 0), (t, N)):
+                pos.setdefault(key(s, i, j), set()).add((s, i, j))
+    groups = [sorted(v) for v in pos.values()]
+    assert sorted(len(v) for v in groups).count(3) == 8
+    assert all(len(v) in (2, 3) for v in groups)
+    for v in groups:  # radial edges on each shared vertex
+        vals = [Ez[s, g + 1:g + NR + 1, j + g, i + g] for (s, i, j) in v]
+        for w in vals[1:]:
+            assert np.array_equal(w, vals[0])
+    # x/y-edges along panel edges: same endpoints, sign by direction
+    edges = {}
+    for s in range(6):
+        for t in range(1, N + 1):
+            for fam, (i, j), a, b in ((1, (0, t), (0, t - 1), (0, t)), (1, (N, t), (N, t - 1), (N, t)),
+                                      (0, (t, 0), (t - 1, 0), (t, 0)), (0, (t, N), (t - 1, N), (t, N))):
+                ka, kb = key(s, *a), key(s, *b)
+                arr = Ex if fam == 0 else Ey
+                edges.setdefault(frozenset((ka, kb)), []).append((ka, arr[s, :, j + g, i + g]))
+    assert len(edges) == 12 * N
+    for (ka0, v0), (ka1, v1) in edges.values():
+        sg = 1.0 if ka0 == ka1 else -1.0
+        assert np.array_equal(v0[g:g + NR + 1], sg * v1[g:g + NR + 1])
+    # coinciding face fluxes: opposite normals
+    Fx, Fy = (f.reshape(6, 9, nrf, nf, nf) for f in F[:2])
+    faces = {}
+    for s in range(6):
+        for t in range(1, N + 1):
+            for fam, (i, j), a, b in ((0, (0, t), (0, t - 1), (0, t)), (0, (N, t), (N, t - 1), (N, t)),
+                                      (1, (t, 0), (t - 1, 0), (t, 0)), (1, (t, N), (t - 1, N), (t, N))):
+                arr = Fx if fam == 0 else Fy
+                faces.setdefault(frozenset((key(s, *a), key(s, *b))), []).append(
+                    arr[s, :, g + 1:g + NR + 1, j + g, i + g])
+    assert len(faces) == 12 * N
+    for v0, v1 in faces.values():
+        assert np.array_equal(v0, -v1)
+    assert np.abs(Ez).max() > 0 and np.abs(Ex).max() > 0 and np.abs(Ey).max() > 0
