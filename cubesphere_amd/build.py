@@ -44,8 +44,16 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    # hardware-counter probe: one kernel + a calibration copy, no Python in the process
+    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "cs_probe.hip"), "-L", HERE, "-lcubesphere_hip",
+          "-Wl,-rpath,$ORIGIN", "-o", PROBE])
+    if verbose:
+        print("built", PROBE)
     build_fortran(verbose)
     return LIB
+
+
+PROBE = os.path.join(HERE, "cs_probe")
 
 
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")

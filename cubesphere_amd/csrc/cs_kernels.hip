@@ -91,7 +91,7 @@ struct State {
 };
 
 struct Params {
-  double gamma, gm1, rhofloor, prsfloor, cfl, grav;
+  double gamma, gm1, rhofloor, prsfloor, cfl, grav, igm1;
   int recon, riemann, emf, wall;
 };
 
@@ -132,11 +132,10 @@ __global__ void k_avg_coinciding(CsDims d, State st) {
 }
 
 // Zone-centred ghost strips (330-333 index flips): right, left, up, down.
-__global__ void k_exchange_cells(CsDims d, State st) {
-  uint32_t t = gtid();
+template <typename P>
+__device__ __forceinline__ void exchange_cells(const CsDims& d, const State& st, P* V, uint32_t t) {
   const int N = d.N, NR = d.NR, g = d.g;
   uint32_t per = (uint32_t)g * N * NR;
-  if (t >= 24u * per) return;
   int s = t / (4 * per);
   uint32_t r = t - s * 4 * per;
   int strip = r / per;
@@ -152,19 +151,22 @@ __global__ void k_exchange_cells(CsDims d, State st) {
   else if (strip == 2) { di = tt; dj = N + a; si = N + 1 - a; sj = tt; }
   else { di = tt; dj = 1 - a; si = N + 1 - tt; sj = a; }
   int64_t dc = d.cell(di, dj, k), sc = d.cell(si, sj, k);
-  for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = st.U[(o * 8 + c) * d.cpp + sc];
-  st.ent[s * d.cpp + dc] = st.ent[o * d.cpp + sc];
+  double v[9];  // all loads in flight before the first store
+  P w[9];       // primitives: the same function of the same zone
+  for (int c = 0; c < 8; ++c) v[c] = st.U[(o * 8 + c) * d.cpp + sc];
+  v[8] = st.ent[o * d.cpp + sc];
+  for (int c = 0; c < 9; ++c) w[c] = V[((int64_t)o * 9 + c) * d.cpp + sc];
+  for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = v[c];
+  st.ent[s * d.cpp + dc] = v[8];
+  for (int c = 0; c < 9; ++c) V[((int64_t)s * 9 + c) * d.cpp + dc] = w[c];
 }
 
 // Face-centred ghost strips with the face-B swap and sign rules
 // (330fetch_buffers_fill_in_ux.f:232-294 and the y/z counterparts).
-__global__ void k_exchange_faces(CsDims d, State st) {
-  uint32_t t = gtid();
+__device__ __forceinline__ void exchange_faces(const CsDims& d, const State& st, uint32_t t) {
   const int N = d.N, NR = d.NR, g = d.g;
   // per (panel, strip, a): 3 families x (N+1) x (NR+1) slots
   uint32_t per = 3u * (N + 1) * (NR + 1);
-  uint32_t tot = 6u * 4u * g * per;
-  if (t >= tot) return;
   int s = t / (4u * g * per);
   uint32_t r = t - s * (4u * g * per);
   int strip = r / (g * per);
@@ -206,11 +208,10 @@ __global__ void k_exchange_faces(CsDims d, State st) {
 
 // Corner ghost blocks (no sector owns them: the tri-corners) take the value of
 // the panel's own corner zone.
-__global__ void k_corner_fill(CsDims d, State st) {
-  uint32_t t = gtid();
+template <typename P>
+__device__ __forceinline__ void corner_fill(const CsDims& d, const State& st, P* V, uint32_t t) {
   const int N = d.N, NR = d.NR, g = d.g;
   uint32_t per = (uint32_t)g * g * NR;
-  if (t >= 24u * per) return;
   int s = t / (4 * per);
   uint32_t r = t - s * 4 * per;
   int corner = r / per;
@@ -222,13 +223,42 @@ __global__ void k_corner_fill(CsDims d, State st) {
   int ci = (corner & 1) ? N : 1, cj = (corner & 2) ? N : 1;
   int di = (corner & 1) ? N + a : 1 - a, dj = (corner & 2) ? N + b : 1 - b;
   int64_t dc = d.cell(di, dj, k), sc = d.cell(ci, cj, k);
-  for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = st.U[(s * 8 + c) * d.cpp + sc];
-  st.ent[s * d.cpp + dc] = st.ent[s * d.cpp + sc];
+  double v[9];
+  P w[9];
+  for (int c = 0; c < 8; ++c) v[c] = st.U[(s * 8 + c) * d.cpp + sc];
+  v[8] = st.ent[s * d.cpp + sc];
+  for (int c = 0; c < 9; ++c) w[c] = V[((int64_t)s * 9 + c) * d.cpp + sc];
+  for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = v[c];
+  st.ent[s * d.cpp + dc] = v[8];
+  for (int c = 0; c < 9; ++c) V[((int64_t)s * 9 + c) * d.cpp + dc] = w[c];
+}
+
+// Thread counts of the three parts of the lateral exchange.
+CS_HD void exchange_counts(const CsDims& d, uint32_t& nc, uint32_t& nf, uint32_t& nk) {
+  nc = 24u * d.g * d.N * d.NR;
+  nf = 6u * 4u * d.g * 3u * (d.N + 1) * (d.NR + 1);
+  nk = 24u * d.g * d.g * d.NR;
+}
+
+// The lateral ghost exchange in one launch: zone strips, face strips and the
+// corner blocks read only owned zones and faces and write only ghosts, so
+// the three parts are independent.  Runs after the coinciding-face average.
+// The zone primitives V travel with U, so no ghost zone is converted again.
+template <typename P>
+__global__ void k_exchange(CsDims d, State st, P* V) {
+  uint32_t t = gtid(), nc, nf, nk;
+  exchange_counts(d, nc, nf, nk);
+  if (t < nc) { exchange_cells<P>(d, st, V, t); return; }
+  t -= nc;
+  if (t < nf) { exchange_faces(d, st, t); return; }
+  t -= nf;
+  if (t < nk) corner_fill<P>(d, st, V, t);
 }
 
 // Radial ghost layers: zero-gradient pad (826pad_boundary_r.f:201-214 inner
 // form, used at both ends) or a reflecting, perfectly conducting wall.
-__global__ void k_radial_bc(CsDims d, Geo geo, State st, int wall) {
+template <typename P>
+__global__ void k_radial_bc(CsDims d, Geo geo, State st, int wall, P* V, Params pr) {
   // no FMA contraction: the wall mirror rounds exactly as written, so the whole
   // halo (exchange + boundary) is reproducible bit for bit on any host
 #pragma clang fp contract(off)
@@ -286,7 +316,14 @@ __global__ void k_radial_bc(CsDims d, Geo geo, State st, int wall) {
       u[5] -= 2.0 * b * n0; u[6] -= 2.0 * b * n1; u[7] -= 2.0 * b * n2;
     }
     for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = u[c];
-    st.ent[s * d.cpp + dc] = st.ent[s * d.cpp + sc];
+    double ent = st.ent[s * d.cpp + sc];
+    st.ent[s * d.cpp + dc] = ent;
+    // primitives of the ghost zone, as k_prims converts
+    Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor, pr.igm1};
+    double v[8];
+    cons_to_prim(gs, u, v, ent);
+    for (int c = 0; c < 8; ++c) V[((int64_t)s * 9 + c) * d.cpp + dc] = (P)v[c];
+    V[((int64_t)s * 9 + 8) * d.cpp + dc] = (P)ent;
   }
 }
 
@@ -299,7 +336,7 @@ __global__ void k_prims(CsDims d, Rng rg, State st, P* V, Params pr) {
   double u[8], v[8];
   for (int q = 0; q < 8; ++q) u[q] = st.U[(s * 8 + q) * d.cpp + c];
   double ent = st.ent[s * d.cpp + c];
-  Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor};
+  Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor, pr.igm1};
   cons_to_prim(gs, u, v, ent);
   for (int q = 0; q < 8; ++q) V[(s * 9 + q) * d.cpp + c] = (P)v[q];
   V[(s * 9 + 8) * d.cpp + c] = (P)ent;
@@ -328,7 +365,7 @@ __device__ __forceinline__ void solve_face(const R* VL, const R* VR, const doubl
   wl[7] = c0 * VL[5] + c1 * VL[6] + c2 * VL[7];
   wr[6] = b0 * VR[5] + b1 * VR[6] + b2 * VR[7];
   wr[7] = c0 * VR[5] + c1 * VR[6] + c2 * VR[7];
-  Gas<R> gs{(R)pr.gamma, (R)pr.gm1, (R)pr.rhofloor, (R)pr.prsfloor};
+  Gas<R> gs{(R)pr.gamma, (R)pr.gm1, (R)pr.rhofloor, (R)pr.prsfloor, (R)pr.igm1};
   R Fs[8], entF;
   if (pr.riemann == CS_RS_HLLD)
     hlld<R>(gs, wl, wr, VL[8], VR[8], Fs, entF, SL, SR);
@@ -638,7 +675,12 @@ __global__ void k_sync_shared(CsDims d, double* Fx, double* Fy, double* Ex, doub
       pa = Fy + (int64_t)s * 9 * d.fpp + d.face(tt, 0, k);
       pb = Fy + (int64_t)o * 9 * d.fpp + d.face(N + 1 - tt, 0, k);
     }
-    for (int q = 0; q < 9; ++q) avg_pair(pa + q * d.fpp, pb + q * d.fpp, -1.0);
+    double va[9], vb[9];  // all loads in flight before the first store
+    for (int q = 0; q < 9; ++q) { va[q] = pa[q * d.fpp]; vb[q] = pb[q * d.fpp]; }
+    for (int q = 0; q < 9; ++q) {
+      pa[q * d.fpp] = (va[q] + (-vb[q])) * 0.5;
+      pb[q * d.fpp] = (vb[q] + (-va[q])) * 0.5;
+    }
     return;
   }
   t -= nF;
@@ -688,13 +730,43 @@ __global__ void k_sync_shared(CsDims d, double* Fx, double* Fy, double* Ex, doub
 }
 
 // ------------------------------------------------------------------ update
+// CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:4) of one zone of the stage
+// output, run by the zone-update kernels after the faces are updated: the
+// zone-centred B from the zone's six faces through the zone-B operator, then
+// CONS_TO_PRIM (662pde_3d_cons_to_prim.f:69-151) with its floors; U and the
+// entropy are stored back and the same conversion gives the primitives V the
+// next stage reconstructs from.  u[0..4] and ent hold the updated zone.
+template <typename P>
+__device__ __forceinline__ void cleanup_zone(const CsDims& d, const Geo& geo, const State& st, P* V,
+                                             const Params& pr, int* flags, int s, int i, int j, int k,
+                                             double* u, double ent) {
+  const int64_t c = d.cell(i, j, k);
+  const int64_t so = (int64_t)s * d.fpp;
+  double b0 = 0.5 * (st.Bx[so + d.face(i - 1, j, k)] + st.Bx[so + d.face(i, j, k)]);
+  double b1 = 0.5 * (st.By[so + d.face(i, j - 1, k)] + st.By[so + d.face(i, j, k)]);
+  double b2 = 0.5 * (st.Bz[so + d.face(i, j, k - 1)] + st.Bz[so + d.face(i, j, k)]);
+  for (int r = 0; r < 3; ++r)
+    u[5 + r] = geo.t(d, T_M00 + 3 * r, s, i, j) * b0 + geo.t(d, T_M00 + 3 * r + 1, s, i, j) * b1 +
+               geo.t(d, T_M00 + 3 * r + 2, s, i, j) * b2;
+  Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor, pr.igm1};
+  double v[8];
+  int fl = cons_to_prim(gs, u, v, ent);
+  for (int q = 0; q < 8; ++q) st.U[(s * 8 + q) * d.cpp + c] = u[q];
+  st.ent[s * d.cpp + c] = ent;
+  for (int q = 0; q < 8; ++q) V[((int64_t)s * 9 + q) * d.cpp + c] = (P)v[q];
+  V[((int64_t)s * 9 + 8) * d.cpp + c] = (P)ent;
+  if (fl && flags) flags[0] = 1;
+  if (!(u[0] == u[0]) || !(u[4] == u[4]) || isinf(u[4])) { if (flags) flags[1] = 1; }
+}
+
 // U_out = a U0 + b (U_in + dt L(U_in)) for rho, rho v, E and entropy
 // (UPDATE_IMEX_RK_STEP_P1 484update_imex_rk_step_p1.f:483-497, 547-593 with the
 // Shu-Osher stage weights; the RK2 weights reproduce DRV:2661-2773).
+template <typename P>
 __global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
                                const double* __restrict__ Fx, const double* __restrict__ Fy,
                                const double* __restrict__ Fz, const double* dtp, double wa,
-                               double wb, Params pr) {
+                               double wb, Params pr, P* Vout, int* flags) {
   int i, j, k, s;
   if (!decode(rg, gtid(), i, j, k, s)) return;
   const double dt = dtp[0];
@@ -723,16 +795,18 @@ __global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, St
     L[3] += rho * gr * nz;
     L[4] += mr * gr;
   }
+  double u[8], en;
   for (int q = 0; q < 5; ++q) {
     int64_t o = (s * 8 + q) * d.cpp + c;
     double un = in.U[o] + dt * L[q];
-    out.U[o] = wa != 0.0 ? wa * u0.U[o] + wb * un : wb * un;
+    u[q] = wa != 0.0 ? wa * u0.U[o] + wb * un : wb * un;
   }
   {
     int64_t o = s * d.cpp + c;
-    double en = in.ent[o] + dt * L[5];
-    out.ent[o] = wa != 0.0 ? wa * u0.ent[o] + wb * en : wb * en;
+    double e1 = in.ent[o] + dt * L[5];
+    en = wa != 0.0 ? wa * u0.ent[o] + wb * e1 : wb * e1;
   }
+  cleanup_zone<P>(d, geo, out, Vout, pr, flags, s, i, j, k, u, en);
 }
 
 // Surface mode (one radial layer between conducting walls): the radial ghost
@@ -743,9 +817,9 @@ __global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, St
 // neither the radial ghost fill nor the z-face pass is needed.
 template <typename R, typename P>
 __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
-                                    const P* __restrict__ V, const double* __restrict__ Fx,
+                                    P* V, const double* __restrict__ Fx,
                                     const double* __restrict__ Fy, const double* dtp, double wa,
-                                    double wb, Params pr) {
+                                    double wb, Params pr, int* flags) {
   int i, j, k, s;
   if (!decode(rg, gtid(), i, j, k, s)) return;
   const double dt = dtp[0];
@@ -759,38 +833,35 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
     int64_t o = ((int64_t)s * 9 + comps[q]) * d.fpp;
     L[q] = -ov * ((Fx[o + fxp] - Fx[o + fxm]) + (Fy[o + fxp] - Fy[o + fym]));
   }
-  // the zone and its mirror image (k_radial_bc arithmetic, then k_prims)
+  // the zone's primitives and their mirror image about the radial face
+  // normal: rho, p and the entropy are unchanged, v and B lose twice their
+  // normal part (the primitives of the mirrored ghost zone of k_radial_bc)
   const double zn = geo.t(d, T_ZSN, s, i, j);
   double A[3], Bv[3], Cv[3];
   for (int q = 0; q < 3; ++q) { A[q] = geo.t(d, T_ZSX + q, s, i, j) / zn; Bv[q] = geo.t(d, T_ZTX + q, s, i, j); }
   Cv[0] = A[1] * Bv[2] - A[2] * Bv[1];
   Cv[1] = A[2] * Bv[0] - A[0] * Bv[2];
   Cv[2] = A[0] * Bv[1] - A[1] * Bv[0];
-  double um[8];
-  for (int q = 0; q < 8; ++q) um[q] = in.U[(s * 8 + q) * d.cpp + c];
-  {
-#pragma clang fp contract(off)
-    double n0 = geo.t(d, T_ZSX, s, i, j) / zn, n1 = geo.t(d, T_ZSY, s, i, j) / zn, n2 = geo.t(d, T_ZSZ, s, i, j) / zn;
-    double m = um[1] * n0 + um[2] * n1 + um[3] * n2;
-    um[1] -= 2.0 * m * n0; um[2] -= 2.0 * m * n1; um[3] -= 2.0 * m * n2;
-    double b = um[5] * n0 + um[6] * n1 + um[7] * n2;
-    um[5] -= 2.0 * b * n0; um[6] -= 2.0 * b * n1; um[7] -= 2.0 * b * n2;
-  }
   R Vc[9], Vm[9];
+  for (int q = 0; q < 9; ++q) Vc[q] = (R)V[((int64_t)s * 9 + q) * d.cpp + c];
   {
-    double vm[8], em = in.ent[s * d.cpp + c];
-    Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor};
-    cons_to_prim(gs, um, vm, em);
-    for (int q = 0; q < 8; ++q) Vm[q] = (R)(P)vm[q];
-    Vm[8] = (R)(P)em;
-    for (int q = 0; q < 9; ++q) Vc[q] = (R)V[((int64_t)s * 9 + q) * d.cpp + c];
+    const R n0 = (R)A[0], n1 = (R)A[1], n2 = (R)A[2];
+    R m = Vc[1] * n0 + Vc[2] * n1 + Vc[3] * n2;
+    R b = Vc[5] * n0 + Vc[6] * n1 + Vc[7] * n2;
+    Vm[0] = Vc[0]; Vm[4] = Vc[4]; Vm[8] = Vc[8];
+    Vm[1] = Vc[1] - R(2) * m * n0; Vm[2] = Vc[2] - R(2) * m * n1; Vm[3] = Vc[3] - R(2) * m * n2;
+    Vm[5] = Vc[5] - R(2) * b * n0; Vm[6] = Vc[6] - R(2) * b * n1; Vm[7] = Vc[7] - R(2) * b * n2;
   }
   // states of the outer wall face (k = 1: zone | ghost) and the inner one (ghost | zone)
   R Wp[9], Wm[9];
   const bool weno = pr.recon == CS_RECON_WENO5;
   for (int q = 0; q < 9; ++q) {
-    Wp[q] = weno ? weno5<R>(Vc[q], Vm[q], Vc[q], Vm[q], Vc[q]) : Vc[q];
-    Wm[q] = weno ? weno5<R>(Vm[q], Vc[q], Vm[q], Vc[q], Vm[q]) : Vm[q];
+    if (weno) {
+      weno5_mirror<R>(Vc[q], Vm[q], Wp[q], Wm[q]);  // weno5(c,m,c,m,c), weno5(m,c,m,c,m)
+    } else {
+      Wp[q] = Vc[q];
+      Wm[q] = Vm[q];
+    }
   }
   const bool okp = (Wp[0] > R(0)) && (Wp[4] > R(0)) && (Wp[8] > R(0));
   const bool okm = (Wm[0] > R(0)) && (Wm[4] > R(0)) && (Wm[8] > R(0));
@@ -811,25 +882,31 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
     L[3] += rho * gr * A[2];
     L[4] += mr * gr;
   }
+  double u[8], en;
   for (int q = 0; q < 5; ++q) {
     int64_t o = (s * 8 + q) * d.cpp + c;
     double un = in.U[o] + dt * L[q];
-    out.U[o] = wa != 0.0 ? wa * u0.U[o] + wb * un : wb * un;
+    u[q] = wa != 0.0 ? wa * u0.U[o] + wb * un : wb * un;
   }
   {
     int64_t o = s * d.cpp + c;
-    double en = in.ent[o] + dt * L[5];
-    out.ent[o] = wa != 0.0 ? wa * u0.ent[o] + wb * en : wb * en;
+    double e1 = in.ent[o] + dt * L[5];
+    en = wa != 0.0 ? wa * u0.ent[o] + wb * e1 : wb * e1;
   }
+  // V was read above for this zone only, so it is overwritten in place
+  cleanup_zone<P>(d, geo, out, V, pr, flags, s, i, j, k, u, en);
 }
 
 // Constrained transport (484update_imex_rk_step_p1.f:646-908) with every edge
 // circulation element E*l computed once per edge, so the discrete div B of
 // each zone telescopes.
+// With conducting walls the tangential EMFs on the walls vanish, so the wall
+// z-faces keep their value; they are left untouched, which also lets the
+// surface-mode zone update read them after this kernel has run in place.
 __global__ void k_update_faces(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
                                const double* __restrict__ Ex, const double* __restrict__ Ey,
                                const double* __restrict__ Ez, const double* dtp, double wa,
-                               double wb) {
+                               double wb, int wall) {
   int i, j, k, s;
   if (!decode(rg, gtid(), i, j, k, s)) return;
   const double dt = dtp[0];
@@ -852,7 +929,7 @@ __global__ void k_update_faces(CsDims d, Rng rg, Geo geo, State in, State u0, St
     out.By[f] = wa != 0.0 ? wa * u0.By[f] + wb * bn : wb * bn;
   }
   // z-face: cells i,j in [1,N], z-vertex k in [0,NR]
-  if (i >= 1 && j >= 1) {
+  if (i >= 1 && j >= 1 && !(wall && (k == 0 || k == NR))) {
     double circ = Ey[f] - Ey[so + d.face(i - 1, j, k)] - Ex[f] + Ex[so + d.face(i, j - 1, k)];
     double A = r1 * r1 * geo.t(d, T_ZSN, s, i, j);
     double bn = in.Bz[f] - dt * circ / A;
@@ -861,29 +938,6 @@ __global__ void k_update_faces(CsDims d, Rng rg, Geo geo, State in, State u0, St
   (void)N; (void)NR;
 }
 
-// Zone B from the faces and the cons->prim floors (CLEANUP_AFTER_UPDATE,
-// 485cleanup_after_update.f:4; CONS_TO_PRIM 662:69-151).
-__global__ void k_cleanup(CsDims d, Rng rg, Geo geo, State st, Params pr, int* flags) {
-  int i, j, k, s;
-  if (!decode(rg, gtid(), i, j, k, s)) return;
-  int64_t c = d.cell(i, j, k);
-  const int64_t so = (int64_t)s * d.fpp;
-  double b0 = 0.5 * (st.Bx[so + d.face(i - 1, j, k)] + st.Bx[so + d.face(i, j, k)]);
-  double b1 = 0.5 * (st.By[so + d.face(i, j - 1, k)] + st.By[so + d.face(i, j, k)]);
-  double b2 = 0.5 * (st.Bz[so + d.face(i, j, k - 1)] + st.Bz[so + d.face(i, j, k)]);
-  double u[8], v[8];
-  for (int q = 0; q < 8; ++q) u[q] = st.U[(s * 8 + q) * d.cpp + c];
-  for (int r = 0; r < 3; ++r)
-    u[5 + r] = geo.t(d, T_M00 + 3 * r, s, i, j) * b0 + geo.t(d, T_M00 + 3 * r + 1, s, i, j) * b1 +
-               geo.t(d, T_M00 + 3 * r + 2, s, i, j) * b2;
-  double ent = st.ent[s * d.cpp + c];
-  Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor};
-  int fl = cons_to_prim(gs, u, v, ent);
-  for (int q = 0; q < 8; ++q) st.U[(s * 8 + q) * d.cpp + c] = u[q];
-  st.ent[s * d.cpp + c] = ent;
-  if (fl && flags) flags[0] = 1;
-  if (!(u[0] == u[0]) || !(u[4] == u[4]) || isinf(u[4])) { if (flags) flags[1] = 1; }
-}
 
 // ------------------------------------------------------------------ reductions
 // CFL (490evaluate_timestep.f:118-278): per zone cfl * L / (|v| + c_ms),
@@ -901,7 +955,7 @@ __global__ void k_cfl_partial(CsDims d, Rng rg, Geo geo, State st, Params pr, do
     double u[8], w[8];
     for (int q = 0; q < 8; ++q) u[q] = st.U[(s * 8 + q) * d.cpp + c];
     double ent = st.ent[s * d.cpp + c];
-    Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor};
+    Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor, pr.igm1};
     cons_to_prim(gs, u, w, ent);
     double b2 = w[5] * w[5] + w[6] * w[6] + w[7] * w[7];
     double ms = sqrt((pr.gamma * w[4] + C<double>::ob4pi * b2) / w[0]);
@@ -1043,6 +1097,7 @@ struct cs_handle {
   int cur = 0;  // which buffer holds U^n
   int surf = 0; // one radial layer between walls: surface-mode stage kernels
   CornerTab ct{}; // the three (sector, i, j) copies of each cube-corner vertex
+  int vbuf = -1;  // stage buffer whose owned-zone primitives V holds (-1: none)
 };
 
 namespace {
@@ -1055,23 +1110,26 @@ State state_of(cs_handle* h, int b) {
 }
 Geo geo_of(cs_handle* h) { return Geo{h->T, h->rb, h->d.t2 * 6, h->d.t2}; }
 
-int launch_halo(cs_handle* h, int b, bool full = true) {
+template <typename P>
+int launch_halo_t(cs_handle* h, int b, bool full) {
   const CsDims& d = h->d;
   State st = state_of(h, b);
+  P* V = (P*)h->V;
   uint64_t n1 = 12ull * d.N * d.NR;
   k_avg_coinciding<<<nblk(n1), kBlock, 0, h->stream>>>(d, st);
-  uint64_t n2 = 24ull * d.g * d.N * d.NR;
-  k_exchange_cells<<<nblk(n2), kBlock, 0, h->stream>>>(d, st);
-  uint64_t n3 = 6ull * 4 * d.g * 3 * (d.N + 1) * (d.NR + 1);
-  k_exchange_faces<<<nblk(n3), kBlock, 0, h->stream>>>(d, st);
-  uint64_t n4 = 24ull * d.g * d.g * d.NR;
-  k_corner_fill<<<nblk(n4), kBlock, 0, h->stream>>>(d, st);
+  uint32_t nc, nf, nk;
+  exchange_counts(d, nc, nf, nk);
+  k_exchange<P><<<nblk((uint64_t)nc + nf + nk), kBlock, 0, h->stream>>>(d, st, V);
   uint64_t nl = (uint64_t)(d.N + 2 * d.g + 1);
   uint64_t n5 = 12ull * d.g * nl * nl;
   // surface-mode stages never read the radial ghosts
-  if (full || !h->surf) k_radial_bc<<<nblk(n5), kBlock, 0, h->stream>>>(d, geo_of(h), st, h->pr.wall);
+  if (full || !h->surf)
+    k_radial_bc<P><<<nblk(n5), kBlock, 0, h->stream>>>(d, geo_of(h), st, h->pr.wall, V, h->pr);
   CS_HIP(hipGetLastError());
   return CS_OK;
+}
+int launch_halo(cs_handle* h, int b, bool full = true) {
+  return h->cfg.precision == CS_MIXED ? launch_halo_t<float>(h, b, full) : launch_halo_t<double>(h, b, full);
 }
 
 void stage_weights(int order, int irk, double& wa, double& wb, int& in, int& out) {
@@ -1191,21 +1249,21 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
     case CS_K_EMF_Y:
       if (r.ey.total) k_emf_xy<1, P><<<nblk(r.ey.total), kBlock, 0, sm>>>(d, r.ey, geo, st, V, h->S[2], h->S[0], h->E[1], h->pr);
       break;
-    case CS_K_UPDATE_CELLS:
+    case CS_K_UPDATE_CELLS:  // zone update + cleanup + primitives of the output
       if (surf)
         k_update_cells_surf<R, P><<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, V, h->F[0],
-                                                                          h->F[1], h->dt, wa, wb, h->pr);
+                                                                          h->F[1], h->dt, wa, wb, h->pr, h->flags);
       else
-        k_update_cells<<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, h->F[0], h->F[1],
-                                                                 h->F[2], h->dt, wa, wb, h->pr);
+        k_update_cells<P><<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, h->F[0], h->F[1],
+                                                                    h->F[2], h->dt, wa, wb, h->pr, V, h->flags);
       break;
     case CS_K_UPDATE_FACES:
       k_update_faces<<<nblk(r.faces.total), kBlock, 0, sm>>>(d, r.faces, geo, in, u0, out, h->E[0], h->E[1],
-                                                               h->E[2], h->dt, wa, wb);
+                                                               h->E[2], h->dt, wa, wb, h->pr.wall);
       break;
     case CS_K_CLEANUP:
-      k_cleanup<<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, out, h->pr, h->flags);
-      break;
+      set_error("the cleanup runs inside the zone update (CS_K_UPDATE_CELLS)");
+      return CS_E_INVALID;
     case CS_K_SYNC:
       return launch_sync(h, true, true);
     case CS_K_CFL: {
@@ -1233,12 +1291,15 @@ int launch_seq(cs_handle* h, const int* ks, int n, int b, int irk, bool surf = f
   }
   return CS_OK;
 }
-// sync: make the panel-edge fluxes unique here (the stage path defers it to
-// the one shared-face/edge launch after the EMFs)
-int launch_fluxes(cs_handle* h, int b, bool surf = false, bool sync = true) {
+// The API path converts every zone (ghosts included) before the fluxes and
+// makes the panel-edge fluxes unique at once; the stage path has the
+// primitives already (from the previous zone update and the exchange) and
+// defers the flux averaging to the one shared-face/edge launch after the EMFs.
+int launch_fluxes(cs_handle* h, int b, bool surf = false, bool api = true) {
   static const int ks[4] = {CS_K_PRIMS, CS_K_FLUX_X, CS_K_FLUX_Y, CS_K_FLUX_Z};
-  int rc = launch_seq(h, ks, surf ? 3 : 4, b, 1, surf);
-  if (rc || !sync) return rc;
+  int rc = api ? launch_seq(h, ks, surf ? 3 : 4, b, 1, surf) : launch_seq(h, ks + 1, surf ? 2 : 3, b, 1, surf);
+  if (rc || !api) return rc;
+  h->vbuf = b;
   return launch_sync(h, true, false);
 }
 // with_fluxes: also make the panel-edge fluxes unique in the same launch
@@ -1248,9 +1309,31 @@ int launch_emf(cs_handle* h, int b, bool surf = false, bool with_fluxes = false)
   if (rc) return rc;
   return launch_sync(h, with_fluxes, true);
 }
+// Faces first: the zone update finishes with the cleanup, which needs the
+// updated faces.  The primitives then belong to the stage output.
 int launch_update(cs_handle* h, int irk, bool surf = false) {
-  static const int ks[3] = {CS_K_UPDATE_CELLS, CS_K_UPDATE_FACES, CS_K_CLEANUP};
-  return launch_seq(h, ks, 3, h->cur, irk, surf);
+  static const int ks[2] = {CS_K_UPDATE_FACES, CS_K_UPDATE_CELLS};
+  int rc = launch_seq(h, ks, 2, h->cur, irk, surf);
+  if (rc) return rc;
+  double wa, wb;
+  int bi, bo;
+  stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo);
+  h->vbuf = (h->cur + bo) & 1;
+  return CS_OK;
+}
+// Primitives of the owned zones of buffer b when V does not hold them (after
+// an upload or a timing run); the exchange carries them into the ghosts.
+template <typename P>
+int launch_prims_interior_t(cs_handle* h, int b) {
+  const Ranges r = ranges_of(h->d, h->pr.wall);
+  k_prims<P><<<nblk(r.cells.total), kBlock, 0, h->stream>>>(h->d, r.cells, state_of(h, b), (P*)h->V, h->pr);
+  CS_HIP(hipGetLastError());
+  h->vbuf = b;
+  return CS_OK;
+}
+int ensure_prims(cs_handle* h, int b) {
+  if (h->vbuf == b) return CS_OK;
+  return h->cfg.precision == CS_MIXED ? launch_prims_interior_t<float>(h, b) : launch_prims_interior_t<double>(h, b);
 }
 int launch_cfl(cs_handle* h) { return launch_kernel(h, CS_K_CFL, h->cur, 1); }
 
@@ -1258,6 +1341,7 @@ int launch_stage(cs_handle* h, int irk) {
   int b = stage_input(h, irk);
   int rc;
   const bool surf = h->surf != 0;
+  if ((rc = ensure_prims(h, b))) return rc;
   if ((rc = launch_halo(h, b, !surf))) return rc;
   if ((rc = launch_fluxes(h, b, surf, false))) return rc;
   if ((rc = launch_emf(h, b, surf, true))) return rc;
@@ -1346,7 +1430,7 @@ extern "C" int cs_create(const cs_config* cfg, cs_handle** out) {
   h->cfg = *cfg;
   h->d = cs_make_dims(cfg->ntp, cfg->nr, cfg->nghost);
   h->pr = Params{cfg->gamma, cfg->gamma - 1.0, cfg->rhofloor, cfg->prsfloor, cfg->cfl, cfg->grav,
-                 cfg->recon, cfg->riemann, cfg->emf, cfg->radial_bc == CS_RBC_WALL ? 1 : 0};
+                 1.0 / (cfg->gamma - 1.0), cfg->recon, cfg->riemann, cfg->emf, cfg->radial_bc == CS_RBC_WALL ? 1 : 0};
   h->surf = (cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1) ? 1 : 0;
   if ((rc = build_corner_table(h->d, T, h->ct))) {
     delete h;
@@ -1408,6 +1492,7 @@ extern "C" int cs_upload_state(cs_handle* h, const double* U, const double* ent,
   const CsDims& d = h->d;
   size_t cb = sizeof(double) * d.cpp * 6, fb = sizeof(double) * d.fpp * 6;
   h->cur = 0;
+  h->vbuf = -1;
   CS_HIP(hipMemcpyAsync(h->buf[0][0], U, cb * 8, hipMemcpyHostToDevice, h->stream));
   CS_HIP(hipMemcpyAsync(h->buf[0][1], ent, cb, hipMemcpyHostToDevice, h->stream));
   CS_HIP(hipMemcpyAsync(h->buf[0][2], Bxf, fb, hipMemcpyHostToDevice, h->stream));
@@ -1517,6 +1602,8 @@ static int run_steps(cs_handle* h, int nsteps, int use_graph) {
       if ((rc = launch_step(h))) return rc;
     return CS_OK;
   }
+  // the captured step expects the primitives of U^n in V and leaves them there
+  if ((rc = ensure_prims(h, h->cur))) return rc;
   if (!h->graph) {
     hipGraph_t gr;
     CS_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -1634,6 +1721,7 @@ extern "C" int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float
   }
   CS_HIP(hipEventRecord(e1, h->stream));
   CS_HIP(hipEventSynchronize(e1));
+  h->vbuf = -1;  // a timed zone update or conversion leaves V holding another state
   float ms = 0.f;
   CS_HIP(hipEventElapsedTime(&ms, e0, e1));
   *ms_avg = ms / iters;
@@ -1650,7 +1738,8 @@ extern "C" int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units) {
   int64_t n = 0;
   switch (kernel) {
     case CS_K_HALO: n = 24ll * d.g * d.N * d.NR; break;
-    case CS_K_PRIMS: n = r.all.total; break;
+    case CS_K_PRIMS: n = h->surf ? 6ll * d.nc * d.nc : r.all.total; break;
+    case CS_K_CLEANUP: n = 0; break;  // part of CS_K_UPDATE_CELLS
     case CS_K_FLUX_X: n = r.fx.total; break;
     case CS_K_FLUX_Y: n = r.fy.total; break;
     case CS_K_FLUX_Z: n = r.fz.total; break;

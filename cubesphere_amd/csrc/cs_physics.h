@@ -33,17 +33,31 @@ __device__ __forceinline__ double rpow(double a, double b) { return pow(a, b); }
 __device__ __forceinline__ float rpow(float a, float b) { return powf(a, b); }
 __device__ __forceinline__ double rabs(double a) { return fabs(a); }
 __device__ __forceinline__ float rabs(float a) { return fabsf(a); }
+// Reciprocal: IEEE division in FP64; the hardware v_rcp_f32 (1 ulp) in FP32,
+// where the whole evaluation is already carried at single precision.
+__device__ __forceinline__ double rrcp(double a) { return 1.0 / a; }
+__device__ __forceinline__ float rrcp(float a) { return __builtin_amdgcn_rcpf(a); }
 
 template <typename R> struct Gas {
-  R gamma, gm1, rhofloor, prsfloor;
+  R gamma, gm1, rhofloor, prsfloor, igm1;  // igm1 = 1 / (gamma - 1)
 };
+
+// rho^(gamma - 1) for the entropy p / rho^(gamma - 1): cbrt(rho^2) for the
+// monatomic gamma = 5/3 (a uniform branch), exp2((gamma - 1) log2 rho)
+// otherwise; both within a few ulps of pow at a fraction of its cost.
+__device__ __forceinline__ double rpow_gm1(const Gas<double>& gs, double x) {
+  return gs.gamma == 5.0 / 3.0 ? cbrt(x * x) : exp2(gs.gm1 * log2(x));
+}
+__device__ __forceinline__ float rpow_gm1(const Gas<float>& gs, float x) {
+  return gs.gamma == (float)(5.0 / 3.0) ? cbrtf(x * x) : exp2f(gs.gm1 * log2f(x));
+}
 
 // PRIM_TO_CONS, MHD branch with PRESSURE_POSITIVITY 1
 // (661pde_3d_prim_to_cons.f:67-125).  V may be modified by the floors.
 template <typename R>
 __device__ __forceinline__ void prim_to_cons(const Gas<R>& gs, R ent, R* V, R* U) {
   V[0] = rmax(V[0], gs.rhofloor);
-  if (V[4] <= gs.prsfloor) V[4] = rmax(gs.prsfloor, ent * rpow(V[0], gs.gm1));
+  if (V[4] <= gs.prsfloor) V[4] = rmax(gs.prsfloor, ent * rpow_gm1(gs, V[0]));
   U[0] = V[0];
   U[1] = V[0] * V[1];
   U[2] = V[0] * V[2];
@@ -51,7 +65,7 @@ __device__ __forceinline__ void prim_to_cons(const Gas<R>& gs, R ent, R* V, R* U
   U[5] = V[5];
   U[6] = V[6];
   U[7] = V[7];
-  U[4] = V[4] / gs.gm1 + R(0.5) * V[0] * (V[1] * V[1] + V[2] * V[2] + V[3] * V[3]) +
+  U[4] = V[4] * gs.igm1 + R(0.5) * V[0] * (V[1] * V[1] + V[2] * V[2] + V[3] * V[3]) +
          C<R>::ob8pi * (V[5] * V[5] + V[6] * V[6] + V[7] * V[7]);
 }
 
@@ -74,22 +88,25 @@ __device__ __forceinline__ int cons_to_prim(const Gas<R>& gs, R* U, R* V, R& ent
   R B2 = V[5] * V[5] + V[6] * V[6] + V[7] * V[7];
   V[4] = gs.gm1 * (U[4] - R(0.5) * V[0] * V2 - C<R>::ob8pi * B2);
   if (V[4] <= gs.prsfloor) {
-    V[4] = rmax(gs.prsfloor, ent * rpow(V[0], gs.gm1));
+    V[4] = rmax(gs.prsfloor, ent * rpow_gm1(gs, V[0]));
     flag = 1;
   }
-  ent = V[4] / rpow(V[0], gs.gm1);
-  U[4] = V[4] / gs.gm1 + R(0.5) * V[0] * V2 + C<R>::ob8pi * B2;
+  ent = V[4] / rpow_gm1(gs, V[0]);
+  U[4] = V[4] * gs.igm1 + R(0.5) * V[0] * V2 + C<R>::ob8pi * B2;
   return flag;
 }
 
 // PRIM_TO_WAVSPD, x-direction (660pde_3d_prim_to_wavspd.f:54-76)
 template <typename R>
 __device__ __forceinline__ void wavspd_x(const Gas<R>& gs, const R* V, R& smin, R& smax) {
-  R cs = rsqrt_(gs.gamma * V[4] / V[0]);
-  R b2 = (V[5] * V[5] + V[6] * V[6] + V[7] * V[7]) / (R(4) * C<R>::pi * V[0]);
-  R bb = -(cs * cs + b2);
-  R cc = cs * cs * V[5] * V[5] / (R(4) * C<R>::pi * V[0]);
-  R mf = rsqrt_((-bb + rsqrt_(rmax(bb * bb - R(4) * cc, R(0)))) / R(2));
+  // one reciprocal of rho serves the sound speed and both Alfven terms
+  R irho = rrcp(V[0]);
+  R cs2 = gs.gamma * V[4] * irho;
+  R ob4pr = C<R>::ob4pi * irho;
+  R b2 = (V[5] * V[5] + V[6] * V[6] + V[7] * V[7]) * ob4pr;
+  R bb = -(cs2 + b2);
+  R cc = cs2 * V[5] * V[5] * ob4pr;
+  R mf = rsqrt_((-bb + rsqrt_(rmax(bb * bb - R(4) * cc, R(0)))) * R(0.5));
   smin = V[1] - mf;
   smax = V[1] + mf;
 }
@@ -151,10 +168,10 @@ __device__ __forceinline__ void hll(const Gas<R>& gs, R* VL, R* VR, R entL, R en
   R Us[8];
   for (int c = 0; c < 8; ++c) Us[c] = t1 * UR[c] - t2 * UL[c] - t3 * (FR[c] - FL[c]);
   R entUs = t1 * entR - t2 * entL - t3 * (entR * VR[1] - entL * VL[1]);
-  R pent = entUs * rpow(Us[0], gs.gm1);
+  R pent = entUs * rpow_gm1(gs, Us[0]);
   R peng = gs.gm1 * (Us[4] - R(0.5) * (Us[1] * Us[1] + Us[2] * Us[2] + Us[3] * Us[3]) / Us[0] -
                      C<R>::ob8pi * (Us[5] * Us[5] + Us[6] * Us[6] + Us[7] * Us[7]));
-  if (peng <= gs.prsfloor) Us[4] = Us[4] + (pent - peng) / gs.gm1;
+  if (peng <= gs.prsfloor) Us[4] = Us[4] + (pent - peng) * gs.igm1;
   for (int c = 0; c < 8; ++c)
     Fs[c] = R(0.5) * (FL[c] + FR[c]) + R(0.5) * SL * (Us[c] - UL[c]) - R(0.5) * SR * (UR[c] - Us[c]);
   R ob = R(1) / (SR - SL);
@@ -204,25 +221,28 @@ __device__ __forceinline__ void hlld(const Gas<R>& gs, R* VL, R* VR, R entL, R e
   R ptR = VR[4] + R(0.5) * (bx * bx + byR * byR + bzR * bzR);
   R EL = UL[4], ER = UR[4];
   R dL = (SL - uL) * rL, dR = (SR - uR) * rR;
-  R SM = (dR * uR - dL * uL - ptR + ptL) / (dR - dL);
+  const R idd = rrcp(dR - dL);
+  R SM = (dR * uR - dL * uL - ptR + ptL) * idd;
   // the contact speed lies strictly between the outer speeds; in rounding it may not
   {
     R margin = (SR - SL) * (sizeof(R) == 4 ? R(1e-6) : R(1e-12));
     SM = rmin(rmax(SM, SL + margin), SR - margin);
   }
-  R pts = (dR * ptL - dL * ptR + dL * dR * (uR - uL)) / (dR - dL);
+  R pts = (dR * ptL - dL * ptR + dL * dR * (uR - uL)) * idd;
   // star states; the degenerate case (Miyoshi & Kusano sec. 3.3) is detected
   // relative to the rounding of the arithmetic type
   const R degen = sizeof(R) == 4 ? R(1e-5) : R(1e-10);
-  R rLs = dL / (SL - SM), rRs = dR / (SR - SM);
+  const R iLm = rrcp(SL - SM), iRm = rrcp(SR - SM);
+  R rLs = dL * iLm, rRs = dR * iRm;
   R vyLs, vzLs, byLs, bzLs, vyRs, vzRs, byRs, bzRs;
   {
     R den = dL * (SL - SM) - bx * bx;
     if (rabs(den) > degen * (rabs(dL * (SL - SM)) + bx * bx) && den != R(0)) {
-      R f1 = bx * (SM - uL) / den;
+      R iden = rrcp(den);
+      R f1 = bx * (SM - uL) * iden;
       vyLs = VL[2] - byL * f1;
       vzLs = VL[3] - bzL * f1;
-      R f2 = (dL * (SL - uL) - bx * bx) / den;
+      R f2 = (dL * (SL - uL) - bx * bx) * iden;
       byLs = byL * f2;
       bzLs = bzL * f2;
     } else {
@@ -232,10 +252,11 @@ __device__ __forceinline__ void hlld(const Gas<R>& gs, R* VL, R* VR, R entL, R e
   {
     R den = dR * (SR - SM) - bx * bx;
     if (rabs(den) > degen * (rabs(dR * (SR - SM)) + bx * bx) && den != R(0)) {
-      R f1 = bx * (SM - uR) / den;
+      R iden = rrcp(den);
+      R f1 = bx * (SM - uR) * iden;
       vyRs = VR[2] - byR * f1;
       vzRs = VR[3] - bzR * f1;
-      R f2 = (dR * (SR - uR) - bx * bx) / den;
+      R f2 = (dR * (SR - uR) - bx * bx) * iden;
       byRs = byR * f2;
       bzRs = bzR * f2;
     } else {
@@ -246,10 +267,10 @@ __device__ __forceinline__ void hlld(const Gas<R>& gs, R* VL, R* VR, R entL, R e
   R vbR = uR * bx + VR[2] * byR + VR[3] * bzR;
   R vbLs = SM * bx + vyLs * byLs + vzLs * bzLs;
   R vbRs = SM * bx + vyRs * byRs + vzRs * bzRs;
-  R ELs = ((SL - uL) * EL - ptL * uL + pts * SM + bx * (vbL - vbLs)) / (SL - SM);
-  R ERs = ((SR - uR) * ER - ptR * uR + pts * SM + bx * (vbR - vbRs)) / (SR - SM);
+  R ELs = ((SL - uL) * EL - ptL * uL + pts * SM + bx * (vbL - vbLs)) * iLm;
+  R ERs = ((SR - uR) * ER - ptR * uR + pts * SM + bx * (vbR - vbRs)) * iRm;
   R sqL = rsqrt_(rLs), sqR = rsqrt_(rRs);
-  R SLs = SM - rabs(bx) / sqL, SRs = SM + rabs(bx) / sqR;
+  R SLs = SM - rabs(bx) * rrcp(sqL), SRs = SM + rabs(bx) * rrcp(sqR);
   // conserved star states in the rationalised field
   R ULs[8] = {rLs, rLs * SM, rLs * vyLs, rLs * vzLs, ELs, bx, byLs, bzLs};
   R URs[8] = {rRs, rRs * SM, rRs * vyRs, rRs * vzRs, ERs, bx, byRs, bzRs};
@@ -267,7 +288,7 @@ __device__ __forceinline__ void hlld(const Gas<R>& gs, R* VL, R* VR, R entL, R e
     for (int c = 0; c < 8; ++c) F[c] = FRr[c] + SR * (URs[c] - URr[c]);
   } else {
     R sg = bx > R(0) ? R(1) : (bx < R(0) ? R(-1) : R(0));
-    R isum = R(1) / (sqL + sqR);
+    R isum = rrcp(sqL + sqR);
     R vyss = (sqL * vyLs + sqR * vyRs + (byRs - byLs) * sg) * isum;
     R vzss = (sqL * vzLs + sqR * vzRs + (bzRs - bzLs) * sg) * isum;
     R byss = (sqL * byRs + sqR * byLs + sqL * sqR * (vyRs - vyLs) * sg) * isum;
@@ -289,27 +310,61 @@ __device__ __forceinline__ void hlld(const Gas<R>& gs, R* VL, R* VR, R entL, R e
   Fs[5] = R(0);
   Fs[6] = F[6] * s4;
   Fs[7] = F[7] * s4;
-  entF = Fs[0] >= R(0) ? Fs[0] * (entL / VL[0]) : Fs[0] * (entR / VR[0]);
+  const bool up = Fs[0] >= R(0);
+  entF = Fs[0] * ((up ? entL : entR) * rrcp(up ? VL[0] : VR[0]));
 }
 
 // Jiang & Shu (1996) WENO5 value at the right edge of cell v2 from
-// v0..v4 = cells i-2..i+2 (the left state of face i+1/2).
+// v0..v4 = cells i-2..i+2 (the left state of face i+1/2):
+//   w_k = a_k / sum a,  a_k = g_k / (eps + beta_k)^2,  g = (0.1, 0.6, 0.3).
+// FP64 multiplies every a_k by d0 d1 d2 (d_k = (eps + beta_k)^2), which
+// leaves the weights unchanged and needs one division instead of four; the
+// products stay far inside the FP64 range.  FP32, whose range the products
+// could leave, uses four hardware reciprocals.
 template <typename R>
 __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4) {
   const R eps = R(1e-6);
-  R b0 = R(13.0 / 12.0) * (v0 - R(2) * v1 + v2) * (v0 - R(2) * v1 + v2) +
-         R(0.25) * (v0 - R(4) * v1 + R(3) * v2) * (v0 - R(4) * v1 + R(3) * v2);
-  R b1 = R(13.0 / 12.0) * (v1 - R(2) * v2 + v3) * (v1 - R(2) * v2 + v3) +
-         R(0.25) * (v1 - v3) * (v1 - v3);
-  R b2 = R(13.0 / 12.0) * (v2 - R(2) * v3 + v4) * (v2 - R(2) * v3 + v4) +
-         R(0.25) * (R(3) * v2 - R(4) * v3 + v4) * (R(3) * v2 - R(4) * v3 + v4);
-  R a0 = R(0.1) / ((eps + b0) * (eps + b0));
-  R a1 = R(0.6) / ((eps + b1) * (eps + b1));
-  R a2 = R(0.3) / ((eps + b2) * (eps + b2));
-  R q0 = (R(2) * v0 - R(7) * v1 + R(11) * v2) * R(1.0 / 6.0);
-  R q1 = (-v1 + R(5) * v2 + R(2) * v3) * R(1.0 / 6.0);
-  R q2 = (R(2) * v2 + R(5) * v3 - v4) * R(1.0 / 6.0);
-  return (a0 * q0 + a1 * q1 + a2 * q2) / (a0 + a1 + a2);
+  R s0 = v0 - R(2) * v1 + v2, t0 = v0 - R(4) * v1 + R(3) * v2;
+  R s1 = v1 - R(2) * v2 + v3, t1 = v1 - v3;
+  R s2 = v2 - R(2) * v3 + v4, t2 = R(3) * v2 - R(4) * v3 + v4;
+  R e0 = eps + (R(13.0 / 12.0) * s0 * s0 + R(0.25) * t0 * t0);
+  R e1 = eps + (R(13.0 / 12.0) * s1 * s1 + R(0.25) * t1 * t1);
+  R e2 = eps + (R(13.0 / 12.0) * s2 * s2 + R(0.25) * t2 * t2);
+  R d0 = e0 * e0, d1 = e1 * e1, d2 = e2 * e2;
+  // 6 x the three candidate values
+  R q0 = R(2) * v0 - R(7) * v1 + R(11) * v2;
+  R q1 = -v1 + R(5) * v2 + R(2) * v3;
+  R q2 = R(2) * v2 + R(5) * v3 - v4;
+  R a0, a1, a2;
+  if (sizeof(R) == 8) {
+    a0 = R(0.1) * d1 * d2; a1 = R(0.6) * d0 * d2; a2 = R(0.3) * d0 * d1;
+  } else {
+    a0 = R(0.1) * rrcp(d0); a1 = R(0.6) * rrcp(d1); a2 = R(0.3) * rrcp(d2);
+  }
+  R num = a0 * q0 + a1 * q1 + a2 * q2, den = R(6) * (a0 + a1 + a2);
+  return sizeof(R) == 8 ? num / den : num * rrcp(den);
+}
+
+// The two WENO5 values at a wall whose ghost zones mirror the zone:
+// stencils (a, b, a, b, a) and (b, a, b, a, b) with a the zone and b its
+// mirror image, component by component.  Then beta_0 = beta_2 = 25/3 D and
+// beta_1 = 13/3 D with D = (a - b)^2, the candidates are (13a - 7b)/6,
+// (5a + b)/6, (a + 5b)/6, and the weights depend on D only, so
+//   W(a,b) = (e1 (1.6 a + 0.8 b) + e0 (3 a + 0.6 b)) / (2.4 e1 + 3.6 e0),
+//   e0 = (eps + 25/3 D)^2, e1 = (eps + 13/3 D)^2,
+// and W(b,a) = a + b - W(a,b) (the candidates of the swapped stencil sum
+// with those of the original to a + b under the same weights).
+template <typename R>
+__device__ __forceinline__ void weno5_mirror(R a, R b, R& wab, R& wba) {
+  const R eps = R(1e-6);
+  R D = (a - b) * (a - b);
+  R e0 = eps + R(25.0 / 3.0) * D, e1 = eps + R(13.0 / 3.0) * D;
+  e0 = e0 * e0;
+  e1 = e1 * e1;
+  R num = e1 * (R(1.6) * a + R(0.8) * b) + e0 * (R(3) * a + R(0.6) * b);
+  R den = R(2.4) * e1 + R(3.6) * e0;
+  wab = sizeof(R) == 8 ? num / den : num * rrcp(den);
+  wba = (a + b) - wab;
 }
 
 // Two_D_RS_Ez_4State_HLL (630_2d_rs_ez_two_d_rs_ez_4state_hll.f:10-372):
