@@ -47,12 +47,12 @@ def algorithmic_bytes_per_unit(cs, kernel, prec, surface):
     """
     w = word_bytes(prec)
     t2 = 1 if surface else 0
-    if kernel in (cs.K_FLUX_X, cs.K_FLUX_Y, cs.K_FLUX_Z):
+    if kernel in (cs.K_FLUX_X, cs.K_FLUX_Y, cs.K_FLUX_Z, cs.K_FLUX_XY):
         # V of one zone, face B, face frame (normal, radial dir, angle), flux (9) + speeds (2) out
         return 9 * w + 8 + t2 * 7 * 8 + 11 * 8
     if kernel == cs.K_PRIMS:
         return 9 * 8 + 9 * w                      # U + entropy in, V out
-    if kernel in (cs.K_EMF_X, cs.K_EMF_Y, cs.K_EMF_Z):
+    if kernel in (cs.K_EMF_X, cs.K_EMF_Y, cs.K_EMF_Z, cs.K_EMF):
         # v and B of one zone, two face B, four speeds, edge frame (5), EMF out
         return 6 * w + 2 * 8 + 4 * 8 + t2 * 5 * 8 + 8
     if kernel == cs.K_UPDATE_CELLS:
@@ -71,9 +71,9 @@ def algorithmic_bytes_per_unit(cs, kernel, prec, surface):
     return 0
 
 
-# the launches of one stage (halo, fluxes, EMFs + shared-edge averaging, face
+# the launches of one stage (exchange, lateral + radial fluxes, all EMFs, face
 # and zone update) and the CFL reduction of a step
-STEP_KERNELS = (0, 2, 3, 4, 5, 6, 7, 13, 9, 8, 11)
+STEP_KERNELS = (0, 14, 4, 15, 9, 8, 11)
 
 
 def run_gpu(cs, name, prec, steps, warmup, profile_iters):

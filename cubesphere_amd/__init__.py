@@ -22,9 +22,9 @@ FP64, MIXED = 0, 1
 RBC_PAD, RBC_WALL = 0, 1
 RMESH_UNIFORM, RMESH_LOG = 0, 1
 KERNEL_NAMES = ["halo", "prims", "flux_x", "flux_y", "flux_z", "emf_z", "emf_x", "emf_y",
-                "update_cells", "update_faces", "cleanup", "cfl", "step", "sync"]
+                "update_cells", "update_faces", "cleanup", "cfl", "step", "sync", "flux_xy", "emf"]
 (K_HALO, K_PRIMS, K_FLUX_X, K_FLUX_Y, K_FLUX_Z, K_EMF_Z, K_EMF_X, K_EMF_Y, K_UPDATE_CELLS,
- K_UPDATE_FACES, K_CLEANUP, K_CFL, K_STEP, K_SYNC) = range(14)
+ K_UPDATE_FACES, K_CLEANUP, K_CFL, K_STEP, K_SYNC, K_FLUX_XY, K_EMF) = range(16)
 
 
 class Config(ctypes.Structure):

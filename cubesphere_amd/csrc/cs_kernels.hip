@@ -68,7 +68,20 @@ __device__ __forceinline__ bool decode(const Rng& r, uint32_t t, int& i, int& j,
   s = (int)q3;
   return true;
 }
-__device__ __forceinline__ uint32_t gtid() { return blockIdx.x * blockDim.x + threadIdx.x; }
+// Workgroups are handed to the 8 XCDs of an MI355X round robin (workgroup b
+// runs on XCD b mod 8), and each XCD has its own L2.  Renumbering them so that
+// XCD x takes one contiguous range of the index space keeps the rows a
+// stencil shares (j-2..j+3 of a y-sweep, the two rows of a face pair) inside
+// one L2 instead of fetching them from HBM once per XCD.  The map is a
+// bijection of [0, gridDim.x) for any grid size.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block() {
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t q0 = nb / kXcds, r = nb % kXcds;
+  const uint32_t x = b % kXcds, q = b / kXcds;
+  return x * q0 + (x < r ? x : r) + q;
+}
+__device__ __forceinline__ uint32_t gtid() { return xcd_block() * blockDim.x + threadIdx.x; }
 
 // ------------------------------------------------------------------ device views
 struct Geo {
@@ -387,10 +400,9 @@ __device__ __forceinline__ void solve_face(const R* VL, const R* VR, const doubl
 // DIR 0: x-faces (x-vertex i, cells i and i+1), 1: y, 2: z.
 // Output per face: flux(9) * area in the global frame, speeds SL, SR.
 template <int DIR, typename R, typename P>
-__global__ void __launch_bounds__(256, 2) k_face_flux(CsDims d, Rng rg, Geo geo, State st, const P* __restrict__ V,
-                            double* __restrict__ F, double* __restrict__ S, Params pr) {
-  int i, j, k, s;
-  if (!decode(rg, gtid(), i, j, k, s)) return;
+__device__ __forceinline__ void face_flux(const CsDims& d, int i, int j, int k, int s, const Geo& geo,
+                                          const State& st, const P* __restrict__ V, double* __restrict__ F,
+                                          double* __restrict__ S, const Params& pr) {
   if (DIR == 2 && (i == 0 || i == d.N + 1) && (j == 0 || j == d.N + 1)) {
     // radial faces of the tri-corner ghost columns have zero area (two of the
     // four vertices coincide); nothing reads them
@@ -467,6 +479,30 @@ __global__ void __launch_bounds__(256, 2) k_face_flux(CsDims d, Rng rg, Geo geo,
   S[so + d.fpp] = (double)SR;
 }
 
+template <int DIR, typename R, typename P>
+__global__ void __launch_bounds__(256, 2) k_face_flux(CsDims d, Rng rg, Geo geo, State st, const P* __restrict__ V,
+                                                      double* __restrict__ F, double* __restrict__ S, Params pr) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  face_flux<DIR, R, P>(d, i, j, k, s, geo, st, V, F, S, pr);
+}
+
+// x- and y-faces of a stage in one launch (the two sweeps are independent)
+template <typename R, typename P>
+__global__ void __launch_bounds__(256, 2) k_face_flux_xy(CsDims d, Rng rx, Rng ry, Geo geo, State st,
+                                                         const P* __restrict__ V, double* __restrict__ Fx,
+                                                         double* __restrict__ Sx, double* __restrict__ Fy,
+                                                         double* __restrict__ Sy, Params pr) {
+  const uint32_t t = gtid();
+  int i, j, k, s;
+  if (t < rx.total) {
+    decode(rx, t, i, j, k, s);
+    face_flux<0, R, P>(d, i, j, k, s, geo, st, V, Fx, Sx, pr);
+  } else if (decode(ry, t - rx.total, i, j, k, s)) {
+    face_flux<1, R, P>(d, i, j, k, s, geo, st, V, Fy, Sy, pr);
+  }
+}
+
 // ------------------------------------------------------------------ edge EMFs
 template <typename P>
 __device__ __forceinline__ void cell_E(const CsDims& d, const P* V, int s, int64_t c, double* E) {
@@ -484,10 +520,9 @@ __device__ __forceinline__ double sp(const CsDims& d, const double* S, int s, in
 // z-edges (MAKE_EDGE_EFIELDS_P1 STEP 3, 460make_edge_fields_p1.f:2591-3721,
 // and the tri-corners of STEP 4, :3725-7035).  Stored as E * dr.
 template <typename P>
-__global__ void k_emf_z(CsDims d, Rng rg, Geo geo, State st, const P* V, const double* Sx,
-                        const double* Sy, double* Ez, Params pr) {
-  int i, j, k, s;
-  if (!decode(rg, gtid(), i, j, k, s)) return;
+__device__ __forceinline__ void emf_z(const CsDims& d, int i, int j, int k, int s, const Geo& geo, const State& st,
+                                      const P* V, const double* Sx, const double* Sy, double* Ez,
+                                      const Params& pr) {
   const int N = d.N;
   double n3[3] = {geo.t(d, T_UX, s, i, j), geo.t(d, T_UY, s, i, j), geo.t(d, T_UZ, s, i, j)};
   double dr = geo.r(d, k) - geo.r(d, k - 1);
@@ -566,10 +601,9 @@ __global__ void k_emf_z(CsDims d, Rng rg, Geo geo, State st, const P* V, const d
 // x-edges (STEP 1, 460make_edge_fields_p1.f:393-1493) and y-edges (STEP 2,
 // :1497-2587): edges normal to the radial direction.  Stored as E * arc length.
 template <int DIR, typename P>
-__global__ void k_emf_xy(CsDims d, Rng rg, Geo geo, State st, const P* V, const double* Sa,
-                         const double* Sb, double* Eo, Params pr) {
-  int i, j, k, s;
-  if (!decode(rg, gtid(), i, j, k, s)) return;
+__device__ __forceinline__ void emf_xy(const CsDims& d, int i, int j, int k, int s, const Geo& geo,
+                                       const State& st, const P* V, const double* Sa, const double* Sb,
+                                       double* Eo, const Params& pr) {
   int64_t fo = s * d.fpp + d.face(i, j, k);
   if (pr.wall && (k == 0 || k == d.NR)) { Eo[fo] = 0.0; return; }
   double n3[3];
@@ -622,6 +656,41 @@ __global__ void k_emf_xy(CsDims d, Rng rg, Geo geo, State st, const P* V, const 
   Eo[fo] = e * len;
 }
 
+template <typename P>
+__global__ void k_emf_z(CsDims d, Rng rg, Geo geo, State st, const P* V, const double* Sx,
+                        const double* Sy, double* Ez, Params pr) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  emf_z<P>(d, i, j, k, s, geo, st, V, Sx, Sy, Ez, pr);
+}
+template <int DIR, typename P>
+__global__ void k_emf_xy(CsDims d, Rng rg, Geo geo, State st, const P* V, const double* Sa,
+                         const double* Sb, double* Eo, Params pr) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  emf_xy<DIR, P>(d, i, j, k, s, geo, st, V, Sa, Sb, Eo, pr);
+}
+// the three edge families of a stage in one launch; S: face speeds x, y, z
+template <typename P>
+__global__ void k_emf_all(CsDims d, Rng rz, Rng rx, Rng ry, Geo geo, State st, const P* V, const double* S0,
+                          const double* S1, const double* S2, double* Ex, double* Ey, double* Ez, Params pr) {
+  uint32_t t = gtid();
+  int i, j, k, s;
+  if (t < rz.total) {
+    decode(rz, t, i, j, k, s);
+    emf_z<P>(d, i, j, k, s, geo, st, V, S0, S1, Ez, pr);
+    return;
+  }
+  t -= rz.total;
+  if (t < rx.total) {
+    decode(rx, t, i, j, k, s);
+    emf_xy<0, P>(d, i, j, k, s, geo, st, V, S1, S2, Ex, pr);
+    return;
+  }
+  t -= rx.total;
+  if (decode(ry, t, i, j, k, s)) emf_xy<1, P>(d, i, j, k, s, geo, st, V, S2, S0, Ey, pr);
+}
+
 // ------------------------------------------------------------------ shared faces and edges
 // Both sectors that meet at a panel edge compute its faces and edges, from
 // states that differ only where a stencil reaches a ghost row.  The reference
@@ -637,9 +706,20 @@ __global__ void k_emf_xy(CsDims d, Rng rg, Geo geo, State st, const P* V, const 
 //   x/y-edge pairs (a + sg b) * 0.5 with sg = -1 where the two sectors run
 //   the edge in opposite directions (left-left and down-down panel edges);
 //   cube corners (e0 + e1 + e2) * (1/3) in ascending sector order.
-struct CornerTab {
-  int s[8][3], i[8][3], j[8][3];
-};
+// The eight cube corners, from the grouping of the 24 panel-corner vertices
+// by position at cs_create (the same for every N): c_corner[c][m] is copy m
+// (ascending sector) of corner c as 4 * sector + corner bits (bit 0: i = N,
+// bit 1: j = N); c_cornerof[s][bits] is the corner a panel corner belongs to.
+__constant__ int c_corner[8][3];
+__constant__ int c_cornerof[6][4];
+__device__ __forceinline__ int64_t corner_edge(const CsDims& d, int code, int k) {
+  const int sc = code >> 2, b = code & 3;
+  return (int64_t)sc * d.fpp + d.face((b & 1) ? d.N : 0, (b & 2) ? d.N : 0, k);
+}
+__device__ __forceinline__ double corner_mean(const CsDims& d, const double* Ez, int c, int k) {
+  return (Ez[corner_edge(d, c_corner[c][0], k)] + Ez[corner_edge(d, c_corner[c][1], k)] +
+          Ez[corner_edge(d, c_corner[c][2], k)]) * (1.0 / 3.0);
+}
 
 __device__ __forceinline__ void avg_pair(double* pa, double* pb, double sg) {
   double a = *pa, b = *pb;
@@ -649,7 +729,7 @@ __device__ __forceinline__ void avg_pair(double* pa, double* pb, double sg) {
 
 template <bool FLUX, bool EDGE>
 __global__ void k_sync_shared(CsDims d, double* Fx, double* Fy, double* Ex, double* Ey, double* Ez,
-                              CornerTab ct, int kh0, int nkh) {
+                              int kh0, int nkh) {
   uint32_t t = gtid();
   const int N = d.N, NR = d.NR;
   const uint32_t nF = FLUX ? 12u * N * NR : 0u;
@@ -719,14 +799,83 @@ __global__ void k_sync_shared(CsDims d, double* Fx, double* Fy, double* Ex, doub
   t -= nH;
   if (t < nC) {          // radial edges at the eight cube corners
     int c = t / NR, k = 1 + (int)(t % NR);
-    double* p0 = Ez + ct.s[c][0] * d.fpp + d.face(ct.i[c][0], ct.j[c][0], k);
-    double* p1 = Ez + ct.s[c][1] * d.fpp + d.face(ct.i[c][1], ct.j[c][1], k);
-    double* p2 = Ez + ct.s[c][2] * d.fpp + d.face(ct.i[c][2], ct.j[c][2], k);
-    double v = (*p0 + *p1 + *p2) * (1.0 / 3.0);
-    *p0 = v;
-    *p1 = v;
-    *p2 = v;
+    double v = corner_mean(d, Ez, c, k);
+    Ez[corner_edge(d, c_corner[c][0], k)] = v;
+    Ez[corner_edge(d, c_corner[c][1], k)] = v;
+    Ez[corner_edge(d, c_corner[c][2], k)] = v;
   }
+}
+
+// ------------------------------------------------------------------ unique on read
+// The stage path makes panel-edge fluxes and EMFs unique where they are
+// read, with the averages of k_sync_shared written from the reader's side:
+// own' = (own + sg * other) / 2 is the same expression for both copies (the
+// addition commutes), so both sectors see one value bit for bit, and the
+// separate synchronisation launch disappears.  Interior items return the
+// stored value.
+// Zone fluxes: the four lateral faces of zone (i, j, k) read with their
+// panel-edge copies averaged.  The partner copies are located once per zone
+// (null inside the panel); the loads are predicated on the boundary lanes.
+struct ZoneFaces {
+  int64_t xr, xl, yr, yl;                 // own faces
+  const double *pxr, *pxl, *pyr, *pyl;    // partner copies (component 0) or null
+};
+__device__ __forceinline__ ZoneFaces zone_faces(const CsDims& d, const double* Fx, const double* Fy, int s, int i,
+                                                int j, int k) {
+  const int N = d.N;
+  ZoneFaces z;
+  z.xr = (int64_t)s * 9 * d.fpp + d.face(i, j, k);
+  z.xl = (int64_t)s * 9 * d.fpp + d.face(i - 1, j, k);
+  z.yr = z.xr;
+  z.yl = (int64_t)s * 9 * d.fpp + d.face(i, j - 1, k);
+  z.pxr = i == N ? Fy + (int64_t)c_nbr[s][0] * 9 * d.fpp + d.face(j, N, k) : nullptr;
+  z.pxl = i == 1 ? Fx + (int64_t)c_nbr[s][1] * 9 * d.fpp + d.face(0, N + 1 - j, k) : nullptr;
+  z.pyr = j == N ? Fx + (int64_t)c_nbr[s][2] * 9 * d.fpp + d.face(N, i, k) : nullptr;
+  z.pyl = j == 1 ? Fy + (int64_t)c_nbr[s][3] * 9 * d.fpp + d.face(N + 1 - i, 0, k) : nullptr;
+  return z;
+}
+// (fx(i) - fx(i-1)) + (fy(j) - fy(j-1)) of component q
+__device__ __forceinline__ double lateral_div(const CsDims& d, const ZoneFaces& z, const double* __restrict__ Fx,
+                                              const double* __restrict__ Fy, int q) {
+  const int64_t qo = (int64_t)q * d.fpp;
+  double fxr = Fx[z.xr + qo], fxl = Fx[z.xl + qo], fyr = Fy[z.yr + qo], fyl = Fy[z.yl + qo];
+  if (z.pxr) fxr = (fxr + (-z.pxr[qo])) * 0.5;
+  if (z.pxl) fxl = (fxl + (-z.pxl[qo])) * 0.5;
+  if (z.pyr) fyr = (fyr + (-z.pyr[qo])) * 0.5;
+  if (z.pyl) fyl = (fyl + (-z.pyl[qo])) * 0.5;
+  return (fxr - fxl) + (fyr - fyl);
+}
+// radial edge at vertex (i, j): two copies on a panel edge, three at a cube corner
+__device__ __forceinline__ double ez_unique(const CsDims& d, const double* __restrict__ Ez, int s, int i, int j,
+                                            int k) {
+  const int N = d.N;
+  const double a = Ez[(int64_t)s * d.fpp + d.face(i, j, k)];
+  const bool ci = i == 0 || i == N, cj = j == 0 || j == N;
+  if (!ci && !cj) return a;
+  if (ci && cj) return corner_mean(d, Ez, c_cornerof[s][(i == N ? 1 : 0) | (j == N ? 2 : 0)], k);
+  int64_t o;
+  if (i == N) o = (int64_t)c_nbr[s][0] * d.fpp + d.face(j, N, k);
+  else if (i == 0) o = (int64_t)c_nbr[s][1] * d.fpp + d.face(0, N - j, k);
+  else if (j == N) o = (int64_t)c_nbr[s][2] * d.fpp + d.face(N, i, k);
+  else o = (int64_t)c_nbr[s][3] * d.fpp + d.face(N - i, 0, k);
+  return (a + Ez[o]) * 0.5;
+}
+// y-edge (x-vertex i, cell j) and x-edge (cell i, y-vertex j) on panel edges
+__device__ __forceinline__ double ey_unique(const CsDims& d, const double* __restrict__ Ex,
+                                            const double* __restrict__ Ey, int s, int i, int j, int k) {
+  const int N = d.N;
+  const double a = Ey[(int64_t)s * d.fpp + d.face(i, j, k)];
+  if (i != 0 && i != N) return a;
+  if (i == N) return (a + Ex[(int64_t)c_nbr[s][0] * d.fpp + d.face(j, N, k)]) * 0.5;
+  return (a + (-Ey[(int64_t)c_nbr[s][1] * d.fpp + d.face(0, N + 1 - j, k)])) * 0.5;
+}
+__device__ __forceinline__ double ex_unique(const CsDims& d, const double* __restrict__ Ex,
+                                            const double* __restrict__ Ey, int s, int i, int j, int k) {
+  const int N = d.N;
+  const double a = Ex[(int64_t)s * d.fpp + d.face(i, j, k)];
+  if (j != 0 && j != N) return a;
+  if (j == N) return (a + Ey[(int64_t)c_nbr[s][2] * d.fpp + d.face(N, i, k)]) * 0.5;
+  return (a + (-Ex[(int64_t)c_nbr[s][3] * d.fpp + d.face(N + 1 - i, 0, k)])) * 0.5;
 }
 
 // ------------------------------------------------------------------ update
@@ -774,13 +923,13 @@ __global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, St
   double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
   double vol = geo.t(d, T_OM, s, i, j) * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0;
   double ov = 1.0 / vol;
-  int64_t fxp = d.face(i, j, k), fxm = d.face(i - 1, j, k), fym = d.face(i, j - 1, k),
-          fzm = d.face(i, j, k - 1);
+  const int64_t fxp = d.face(i, j, k), fzm = d.face(i, j, k - 1);
   const int comps[6] = {0, 1, 2, 3, 4, 8};
+  const ZoneFaces zf = zone_faces(d, Fx, Fy, s, i, j, k);
   double L[6];
   for (int q = 0; q < 6; ++q) {
     int64_t o = ((int64_t)s * 9 + comps[q]) * d.fpp;
-    L[q] = -ov * ((Fx[o + fxp] - Fx[o + fxm]) + (Fy[o + fxp] - Fy[o + fym]) + (Fz[o + fxp] - Fz[o + fzm]));
+    L[q] = -ov * (lateral_div(d, zf, Fx, Fy, comps[q]) + (Fz[o + fxp] - Fz[o + fzm]));
   }
   if (pr.grav != 0.0) {
     // explicit point-mass gravity g = -grav / r^2 rhat (470build_non_stiff_sources.f:150-250)
@@ -826,13 +975,10 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
   const int64_t c = d.cell(i, j, k);
   const double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
   const double ov = 1.0 / (geo.t(d, T_OM, s, i, j) * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0);
-  const int64_t fxp = d.face(i, j, k), fxm = d.face(i - 1, j, k), fym = d.face(i, j - 1, k);
   const int comps[6] = {0, 1, 2, 3, 4, 8};
+  const ZoneFaces zf = zone_faces(d, Fx, Fy, s, i, j, k);
   double L[6];
-  for (int q = 0; q < 6; ++q) {
-    int64_t o = ((int64_t)s * 9 + comps[q]) * d.fpp;
-    L[q] = -ov * ((Fx[o + fxp] - Fx[o + fxm]) + (Fy[o + fxp] - Fy[o + fym]));
-  }
+  for (int q = 0; q < 6; ++q) L[q] = -ov * lateral_div(d, zf, Fx, Fy, comps[q]);
   // the zone's primitives and their mirror image about the radial face
   // normal: rho, p and the entropy are unchanged, v and B lose twice their
   // normal part (the primitives of the mirrored ghost zone of k_radial_bc)
@@ -914,28 +1060,42 @@ __global__ void k_update_faces(CsDims d, Rng rg, Geo geo, State in, State u0, St
   const int64_t so = (int64_t)s * d.fpp;
   int64_t f = so + d.face(i, j, k);
   double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
+  const bool xf = j >= 1 && k >= 1, yf = i >= 1 && k >= 1, zf = i >= 1 && j >= 1;
+  // edge circulation elements, each read once (unique across sectors)
+  const double ez = (xf || yf) ? ez_unique(d, Ez, s, i, j, k) : 0.0;
+  // A face on a panel edge has two copies (k_avg_coinciding's pairs).  One
+  // sector, the one that holds the pair's first copy there, updates it and
+  // writes the negated value into the other copy, so the pair stays exactly
+  // negation-consistent and the stage needs no averaging at the exchange.
   // x-face: x-vertex i in [0,N], cell j in [1,N], cell k in [1,NR]
-  if (j >= 1 && k >= 1) {
-    double circ = Ez[f] - Ez[so + d.face(i, j - 1, k)] - Ey[f] + Ey[so + d.face(i, j, k - 1)];
+  if (xf && !(i == 0 && s >= 3)) {
+    double circ = ez - ez_unique(d, Ez, s, i, j - 1, k) - ey_unique(d, Ex, Ey, s, i, j, k) +
+                  ey_unique(d, Ex, Ey, s, i, j, k - 1);
     double A = 0.5 * (r1 * r1 - r0 * r0) * geo.t(d, T_THY, s, i, j);
     double bn = in.Bx[f] - dt * circ / A;
-    out.Bx[f] = wa != 0.0 ? wa * u0.Bx[f] + wb * bn : wb * bn;
+    double v = wa != 0.0 ? wa * u0.Bx[f] + wb * bn : wb * bn;
+    out.Bx[f] = v;
+    if (i == N) out.By[(int64_t)c_nbr[s][0] * d.fpp + d.face(j, N, k)] = -v;
+    else if (i == 0) out.Bx[(int64_t)c_nbr[s][1] * d.fpp + d.face(0, N + 1 - j, k)] = -v;
   }
   // y-face: cell i in [1,N], y-vertex j in [0,N], cell k in [1,NR]
-  if (i >= 1 && k >= 1) {
-    double circ = -Ez[f] + Ez[so + d.face(i - 1, j, k)] + Ex[f] - Ex[so + d.face(i, j, k - 1)];
+  if (yf && j != N && !(j == 0 && s >= 3)) {
+    double circ = -ez + ez_unique(d, Ez, s, i - 1, j, k) + ex_unique(d, Ex, Ey, s, i, j, k) -
+                  ex_unique(d, Ex, Ey, s, i, j, k - 1);
     double A = 0.5 * (r1 * r1 - r0 * r0) * geo.t(d, T_THX, s, i, j);
     double bn = in.By[f] - dt * circ / A;
-    out.By[f] = wa != 0.0 ? wa * u0.By[f] + wb * bn : wb * bn;
+    double v = wa != 0.0 ? wa * u0.By[f] + wb * bn : wb * bn;
+    out.By[f] = v;
+    if (j == 0) out.By[(int64_t)c_nbr[s][3] * d.fpp + d.face(N + 1 - i, 0, k)] = -v;
   }
   // z-face: cells i,j in [1,N], z-vertex k in [0,NR]
-  if (i >= 1 && j >= 1 && !(wall && (k == 0 || k == NR))) {
-    double circ = Ey[f] - Ey[so + d.face(i - 1, j, k)] - Ex[f] + Ex[so + d.face(i, j - 1, k)];
+  if (zf && !(wall && (k == 0 || k == NR))) {
+    double circ = ey_unique(d, Ex, Ey, s, i, j, k) - ey_unique(d, Ex, Ey, s, i - 1, j, k) -
+                  ex_unique(d, Ex, Ey, s, i, j, k) + ex_unique(d, Ex, Ey, s, i, j - 1, k);
     double A = r1 * r1 * geo.t(d, T_ZSN, s, i, j);
     double bn = in.Bz[f] - dt * circ / A;
     out.Bz[f] = wa != 0.0 ? wa * u0.Bz[f] + wb * bn : wb * bn;
   }
-  (void)N; (void)NR;
 }
 
 
@@ -1096,7 +1256,6 @@ struct cs_handle {
   hipGraphExec_t graph = nullptr;
   int cur = 0;  // which buffer holds U^n
   int surf = 0; // one radial layer between walls: surface-mode stage kernels
-  CornerTab ct{}; // the three (sector, i, j) copies of each cube-corner vertex
   int vbuf = -1;  // stage buffer whose owned-zone primitives V holds (-1: none)
 };
 
@@ -1110,13 +1269,18 @@ State state_of(cs_handle* h, int b) {
 }
 Geo geo_of(cs_handle* h) { return Geo{h->T, h->rb, h->d.t2 * 6, h->d.t2}; }
 
+void launch_avg(cs_handle* h, int b) {
+  const CsDims& d = h->d;
+  k_avg_coinciding<<<nblk(12ull * d.N * d.NR), kBlock, 0, h->stream>>>(d, state_of(h, b));
+}
+// avg: average the two copies of every panel-edge face first.  The stage
+// path leaves it out: its face update writes both copies from one value.
 template <typename P>
-int launch_halo_t(cs_handle* h, int b, bool full) {
+int launch_halo_t(cs_handle* h, int b, bool full, bool avg) {
   const CsDims& d = h->d;
   State st = state_of(h, b);
   P* V = (P*)h->V;
-  uint64_t n1 = 12ull * d.N * d.NR;
-  k_avg_coinciding<<<nblk(n1), kBlock, 0, h->stream>>>(d, st);
+  if (avg) launch_avg(h, b);
   uint32_t nc, nf, nk;
   exchange_counts(d, nc, nf, nk);
   k_exchange<P><<<nblk((uint64_t)nc + nf + nk), kBlock, 0, h->stream>>>(d, st, V);
@@ -1128,8 +1292,9 @@ int launch_halo_t(cs_handle* h, int b, bool full) {
   CS_HIP(hipGetLastError());
   return CS_OK;
 }
-int launch_halo(cs_handle* h, int b, bool full = true) {
-  return h->cfg.precision == CS_MIXED ? launch_halo_t<float>(h, b, full) : launch_halo_t<double>(h, b, full);
+int launch_halo(cs_handle* h, int b, bool full = true, bool avg = true) {
+  return h->cfg.precision == CS_MIXED ? launch_halo_t<float>(h, b, full, avg)
+                                      : launch_halo_t<double>(h, b, full, avg);
 }
 
 void stage_weights(int order, int irk, double& wa, double& wb, int& in, int& out) {
@@ -1197,13 +1362,13 @@ int launch_sync(cs_handle* h, bool flux, bool edge) {
   uint64_t n = sync_units(h, flux, edge);
   if (flux && edge)
     k_sync_shared<true, true><<<nblk(n), kBlock, 0, h->stream>>>(h->d, h->F[0], h->F[1], h->E[0], h->E[1], h->E[2],
-                                                                 h->ct, kh0, nkh);
+                                                                 kh0, nkh);
   else if (flux)
     k_sync_shared<true, false><<<nblk(n), kBlock, 0, h->stream>>>(h->d, h->F[0], h->F[1], h->E[0], h->E[1], h->E[2],
-                                                                  h->ct, kh0, nkh);
+                                                                  kh0, nkh);
   else
     k_sync_shared<false, true><<<nblk(n), kBlock, 0, h->stream>>>(h->d, h->F[0], h->F[1], h->E[0], h->E[1], h->E[2],
-                                                                  h->ct, kh0, nkh);
+                                                                  kh0, nkh);
   CS_HIP(hipGetLastError());
   return CS_OK;
 }
@@ -1222,7 +1387,7 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
   stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo);
   State in = state_of(h, (h->cur + bi) & 1), u0 = state_of(h, h->cur), out = state_of(h, (h->cur + bo) & 1);
   switch (kid) {
-    case CS_K_HALO: return launch_halo(h, b, !surf);
+    case CS_K_HALO: return launch_halo(h, b, !surf, false);  // the stage's exchange
     case CS_K_PRIMS:
       if (surf) {
         Rng r1 = make_rng(1 - d.g, d.N + d.g, 1 - d.g, d.N + d.g, 1, 1);
@@ -1239,6 +1404,14 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
       break;
     case CS_K_FLUX_Z:
       k_face_flux<2, R, P><<<nblk(r.fz.total), kBlock, 0, sm>>>(d, r.fz, geo, st, V, h->F[2], h->S[2], h->pr);
+      break;
+    case CS_K_FLUX_XY:
+      k_face_flux_xy<R, P><<<nblk((uint64_t)r.fx.total + r.fy.total), kBlock, 0, sm>>>(
+          d, r.fx, r.fy, geo, st, V, h->F[0], h->S[0], h->F[1], h->S[1], h->pr);
+      break;
+    case CS_K_EMF:
+      k_emf_all<P><<<nblk((uint64_t)r.ez.total + r.ex.total + r.ey.total), kBlock, 0, sm>>>(
+          d, r.ez, r.ex, r.ey, geo, st, V, h->S[0], h->S[1], h->S[2], h->E[0], h->E[1], h->E[2], h->pr);
       break;
     case CS_K_EMF_Z:
       k_emf_z<P><<<nblk(r.ez.total), kBlock, 0, sm>>>(d, r.ez, geo, st, V, h->S[0], h->S[1], h->E[2], h->pr);
@@ -1342,9 +1515,14 @@ int launch_stage(cs_handle* h, int irk) {
   int rc;
   const bool surf = h->surf != 0;
   if ((rc = ensure_prims(h, b))) return rc;
-  if ((rc = launch_halo(h, b, !surf))) return rc;
-  if ((rc = launch_fluxes(h, b, surf, false))) return rc;
-  if ((rc = launch_emf(h, b, surf, true))) return rc;
+  // panel-edge faces are consistent (written once by the previous face update)
+  if ((rc = launch_halo(h, b, !surf, false))) return rc;
+  // lateral fluxes in one launch, radial ones (shells), all edge EMFs in one
+  // launch; panel-edge fluxes and EMFs are made unique where they are read
+  static const int ks[3] = {CS_K_FLUX_XY, CS_K_FLUX_Z, CS_K_EMF};
+  if ((rc = launch_kernel(h, ks[0], b, 1, surf))) return rc;
+  if (!surf && (rc = launch_kernel(h, ks[1], b, 1, surf))) return rc;
+  if ((rc = launch_kernel(h, ks[2], b, 1, surf))) return rc;
   return launch_update(h, irk, surf);
 }
 
@@ -1368,18 +1546,19 @@ void free_all(cs_handle* h) {
 
 // The eight cube corners: group the 24 panel-corner vertices by position on
 // the unit sphere (each corner is shared by three sectors), ascending sector
-// order within a group.
-int build_corner_table(const CsDims& d, const std::vector<double>& T, CornerTab& ct) {
+// order within a group, and load the tables every kernel reads.
+int load_corner_table(const CsDims& d, const std::vector<double>& T) {
   const int64_t plane = d.t2 * 6;
-  struct V { int s, i, j; double u[3]; bool used; } v[24];
+  struct Vx { int s, b; double u[3]; bool used; } v[24];
   int n = 0;
   for (int s = 0; s < 6; ++s)
-    for (int c = 0; c < 4; ++c) {
-      V& x = v[n++];
-      x.s = s; x.i = (c & 1) ? d.N : 0; x.j = (c & 2) ? d.N : 0; x.used = false;
-      for (int q = 0; q < 3; ++q) x.u[q] = T[(T_UX + q) * plane + s * d.t2 + d.t(x.i, x.j)];
+    for (int b = 0; b < 4; ++b) {
+      Vx& x = v[n++];
+      x.s = s; x.b = b; x.used = false;
+      const int i = (b & 1) ? d.N : 0, j = (b & 2) ? d.N : 0;
+      for (int q = 0; q < 3; ++q) x.u[q] = T[(T_UX + q) * plane + s * d.t2 + d.t(i, j)];
     }
-  int nc = 0;
+  int corner[8][3], cornerof[6][4], nc = 0;
   for (int a = 0; a < 24; ++a) {
     if (v[a].used) continue;
     int m[3] = {a, -1, -1}, nm = 1;
@@ -1393,11 +1572,14 @@ int build_corner_table(const CsDims& d, const std::vector<double>& T, CornerTab&
     if (nm != 3 || nc == 8) { set_error("cube-corner vertices do not group in threes"); return CS_E_INVALID; }
     for (int q = 0; q < 3; ++q) {  // v is in ascending sector order already
       v[m[q]].used = true;
-      ct.s[nc][q] = v[m[q]].s; ct.i[nc][q] = v[m[q]].i; ct.j[nc][q] = v[m[q]].j;
+      corner[nc][q] = 4 * v[m[q]].s + v[m[q]].b;
+      cornerof[v[m[q]].s][v[m[q]].b] = nc;
     }
     ++nc;
   }
   if (nc != 8) { set_error("found %d cube corners", nc); return CS_E_INVALID; }
+  CS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_corner), corner, sizeof(corner)));
+  CS_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_cornerof), cornerof, sizeof(cornerof)));
   return CS_OK;
 }
 
@@ -1432,7 +1614,7 @@ extern "C" int cs_create(const cs_config* cfg, cs_handle** out) {
   h->pr = Params{cfg->gamma, cfg->gamma - 1.0, cfg->rhofloor, cfg->prsfloor, cfg->cfl, cfg->grav,
                  1.0 / (cfg->gamma - 1.0), cfg->recon, cfg->riemann, cfg->emf, cfg->radial_bc == CS_RBC_WALL ? 1 : 0};
   h->surf = (cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1) ? 1 : 0;
-  if ((rc = build_corner_table(h->d, T, h->ct))) {
+  if ((rc = load_corner_table(h->d, T))) {
     delete h;
     return rc;
   }
@@ -1498,6 +1680,10 @@ extern "C" int cs_upload_state(cs_handle* h, const double* U, const double* ent,
   CS_HIP(hipMemcpyAsync(h->buf[0][2], Bxf, fb, hipMemcpyHostToDevice, h->stream));
   CS_HIP(hipMemcpyAsync(h->buf[0][3], Byf, fb, hipMemcpyHostToDevice, h->stream));
   CS_HIP(hipMemcpyAsync(h->buf[0][4], Bzf, fb, hipMemcpyHostToDevice, h->stream));
+  // the two copies of each panel-edge face are made one (negation-consistent)
+  // value here; from then on the face update keeps them so
+  launch_avg(h, 0);
+  CS_HIP(hipGetLastError());
   // the scratch stage buffer starts as a copy so its ghost zones are defined
   for (int q = 0; q < 5; ++q)
     CS_HIP(hipMemcpyAsync(h->buf[1][q], h->buf[0][q], q == 0 ? cb * 8 : (q == 1 ? cb : fb),
@@ -1748,6 +1934,8 @@ extern "C" int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units) {
     case CS_K_EMF_Y: n = r.ey.total; break;
     case CS_K_UPDATE_FACES: n = r.faces.total; break;
     case CS_K_SYNC: n = (int64_t)sync_units(h, true, true); break;
+    case CS_K_FLUX_XY: n = (int64_t)r.fx.total + r.fy.total; break;
+    case CS_K_EMF: n = (int64_t)r.ez.total + r.ex.total + r.ey.total; break;
     default: n = r.cells.total; break;
   }
   *units = n;

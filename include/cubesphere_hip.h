@@ -200,7 +200,9 @@ int cs_max_divb(cs_handle* h, double* out);
 #define CS_K_CFL 11
 #define CS_K_STEP 12
 #define CS_K_SYNC 13   /* panel-edge faces and edges made unique across sectors */
-#define CS_K_COUNT 14
+#define CS_K_FLUX_XY 14 /* x- and y-face fluxes in one launch (the stage's form) */
+#define CS_K_EMF 15     /* all edge EMFs in one launch (the stage's form)        */
+#define CS_K_COUNT 16
 int cs_time_steps(cs_handle* h, int32_t nsteps, int32_t use_graph, float* ms_total);
 int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float* ms_avg);
 int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units);
