@@ -71,12 +71,34 @@ def algorithmic_bytes_per_unit(cs, kernel, prec, surface):
     return 0
 
 
-# the launches of one stage (exchange, lateral + radial fluxes, all EMFs, face
-# and zone update) and the CFL reduction of a step
-STEP_KERNELS = (0, 14, 4, 15, 9, 8, 11)
+def measured_traffic(kernel, prec, workload):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter read
+    for a copy kernel of known size), or None when it holds no such line."""
+    path = os.path.join(ROOT, "profiles", "c2_surface256_hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    cal, rec = {}, None
+    for line in open(path):
+        f = line.split()
+        if not f or f[0].startswith("#"):
+            continue
+        if f[0].startswith("calibration_"):
+            cal[f[0]] = float(f[1])
+        elif f[0] == "kernel" and f[1:4] == [kernel, prec, workload]:
+            rec = {f[4]: float(f[5]), f[6]: float(f[7])}
+    if rec is None or len(cal) != 3:
+        return None
+    copy = cal["calibration_copy_bytes"]
+    return (rec["fetch_kib"] * 1024 * copy / (cal["calibration_fetch_kib"] * 1024) +
+            rec["write_kib"] * 1024 * copy / (cal["calibration_write_kib"] * 1024))
 
 
-def run_gpu(cs, name, prec, steps, warmup, profile_iters):
+def run_gpu(cs, name, prec, steps, warmup, profile_steps):
+    """Time `steps` graph-replayed steps (the headline), then run
+    `profile_steps` more steps launch by launch with a hipEvent pair around
+    every launch on the library's stream, for the per-kernel durations as the
+    kernels run inside a step."""
     cfg, prob, _ = cs.workload(name, prec)
     st = cs.init_problem(cfg, prob, seed=20261202)
     dev = cs.CubeSphere(cfg)
@@ -86,12 +108,8 @@ def run_gpu(cs, name, prec, steps, warmup, profile_iters):
     dev.advance(warmup, use_graph=True)
     ms = dev.time_steps(steps, use_graph=True)
     kern = {}
-    surface = cfg.nr == 1 and cfg.radial_bc == cs.RBC_WALL
-    for k in STEP_KERNELS:
-        # kernels that are not part of this configuration's step
-        if (surface and k == cs.K_FLUX_Z) or dev.kernel_units(k) == 0:
-            continue
-        kern[cs.KERNEL_NAMES[k]] = (dev.time_kernel(k, profile_iters), dev.kernel_units(k), k)
+    for k, (ms_avg, calls) in dev.profile_steps(profile_steps).items():
+        kern[cs.KERNEL_NAMES[k]] = (ms_avg, dev.kernel_units(k), k)
     divb = dev.max_divb()
     sums = dev.conservation_sums()
     dt = dev.dt
@@ -180,7 +198,9 @@ def main():
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="C2")
-    ap.add_argument("--profile-iters", type=int, default=20)
+    ap.add_argument("--profile-steps", type=int, default=20,
+                    help="steps run launch by launch for the per-kernel durations")
+    ap.add_argument("--probe-iters", type=int, default=5, help="launches of the --probe kernel")
     ap.add_argument("--gate-steps", type=int, default=100)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -199,13 +219,13 @@ def main():
     log("device:", cs.device_info())
     if args.probe:
         probe(cs, args.workload, cs.FP64 if args.precision == "fp64" else cs.MIXED, args.probe,
-              args.profile_iters)
+              args.probe_iters)
         return
     res = {}
     for prec in (cs.FP64, cs.MIXED):
         try:
             cfg, ms, kern, divb, sums, dt = run_gpu(cs, args.workload, prec, args.steps, args.warmup,
-                                                    args.profile_iters)
+                                                    args.profile_steps)
         except cs.CsError as e:
             if prec == cs.FP64:
                 raise
@@ -237,8 +257,10 @@ def main():
     alg = algorithmic_bytes_per_unit(cs, kid, head, surface) * ku
     achieved = alg / (kt * 1e-3) / 1e9
     roof = {"bound": "hbm", "kernel": kname, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "alg_bytes_per_launch": alg, "avg_launch_us": kt * 1e3}
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": measured_traffic(kname, "fp64" if head == cs.FP64 else "mixed", args.workload),
+            "alg_bytes_per_launch": alg, "avg_launch_us": kt * 1e3,
+            "launch_time": f"hipEvent pair around each launch, {args.profile_steps} steps run launch by launch"}
     # whole-step view: the compulsory bytes of every kernel of a stage, times
     # the stages of a step, per zone update
     stage_bytes = sum(algorithmic_bytes_per_unit(cs, kid_, head, surface) * u_

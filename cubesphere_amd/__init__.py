@@ -116,6 +116,7 @@ def lib():
             "cs_max_divb": [vp, ctypes.POINTER(dbl)],
             "cs_time_steps": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
             "cs_time_kernel": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
+            "cs_profile_steps": [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32)],
             "cs_kernel_units": [vp, i32, ctypes.POINTER(i64)],
             "cs_last_error": [ctypes.c_char_p, ctypes.c_size_t],
             "cs_device_info": [ctypes.c_char_p, ctypes.c_size_t],
@@ -309,6 +310,14 @@ class CubeSphere:
         ms = ctypes.c_float()
         _check(lib().cs_time_kernel(self.h, kernel, iters, ctypes.byref(ms)), "cs_time_kernel")
         return ms.value
+
+    def profile_steps(self, nsteps):
+        """Run nsteps steps launch by launch with an event pair around every
+        launch; returns {kernel id: (average ms per launch, launches)}."""
+        ms = (ctypes.c_float * len(KERNEL_NAMES))()
+        calls = (ctypes.c_int32 * len(KERNEL_NAMES))()
+        _check(lib().cs_profile_steps(self.h, nsteps, ms, calls), "cs_profile_steps")
+        return {k: (ms[k] / calls[k], calls[k]) for k in range(len(KERNEL_NAMES)) if calls[k]}
 
     def kernel_units(self, kernel):
         n = ctypes.c_int64()

@@ -1257,6 +1257,15 @@ struct cs_handle {
   int cur = 0;  // which buffer holds U^n
   int surf = 0; // one radial layer between walls: surface-mode stage kernels
   int vbuf = -1;  // stage buffer whose owned-zone primitives V holds (-1: none)
+  struct Prof* prof = nullptr;  // per-launch event pairs while cs_profile_steps runs
+};
+
+// Event pairs around the launches of one eagerly run step (cs_profile_steps).
+constexpr int kMaxProf = 64;
+struct Prof {
+  hipEvent_t ev[2 * kMaxProf];
+  int kid[kMaxProf];
+  int n = 0;
 };
 
 namespace {
@@ -1454,8 +1463,16 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
 }
 // surf: use the surface-mode stage kernels (only from the stage path)
 int launch_kernel(cs_handle* h, int kid, int b, int irk, bool surf = false) {
-  return h->cfg.precision == CS_MIXED ? launch_kernel_t<float, float>(h, kid, b, irk, surf)
-                                      : launch_kernel_t<double, double>(h, kid, b, irk, surf);
+  Prof* p = h->prof;
+  const bool rec = p && p->n < kMaxProf;
+  if (rec) CS_HIP(hipEventRecord(p->ev[2 * p->n], h->stream));
+  int rc = h->cfg.precision == CS_MIXED ? launch_kernel_t<float, float>(h, kid, b, irk, surf)
+                                        : launch_kernel_t<double, double>(h, kid, b, irk, surf);
+  if (rec) {
+    CS_HIP(hipEventRecord(p->ev[2 * p->n + 1], h->stream));
+    p->kid[p->n++] = kid;
+  }
+  return rc;
 }
 int launch_seq(cs_handle* h, const int* ks, int n, int b, int irk, bool surf = false) {
   for (int q = 0; q < n; ++q) {
@@ -1516,7 +1533,7 @@ int launch_stage(cs_handle* h, int irk) {
   const bool surf = h->surf != 0;
   if ((rc = ensure_prims(h, b))) return rc;
   // panel-edge faces are consistent (written once by the previous face update)
-  if ((rc = launch_halo(h, b, !surf, false))) return rc;
+  if ((rc = launch_kernel(h, CS_K_HALO, b, 1, surf))) return rc;
   // lateral fluxes in one launch, radial ones (shells), all edge EMFs in one
   // launch; panel-edge fluxes and EMFs are made unique where they are read
   static const int ks[3] = {CS_K_FLUX_XY, CS_K_FLUX_Z, CS_K_EMF};
@@ -1884,6 +1901,39 @@ extern "C" int cs_time_steps(cs_handle* h, int32_t nsteps, int32_t use_graph, fl
   CS_HIP(hipEventElapsedTime(ms_total, e0, e1));
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  return check_flags(h);
+}
+
+extern "C" int cs_profile_steps(cs_handle* h, int32_t nsteps, float* ms_sum, int32_t* calls) {
+  CHECK_H(h);
+  if (!ms_sum || !calls || nsteps < 1) { set_error("bad arguments"); return CS_E_INVALID; }
+  for (int q = 0; q < CS_K_COUNT; ++q) { ms_sum[q] = 0.f; calls[q] = 0; }
+  Prof* p = new Prof();
+  int made = 0, rc = CS_OK;
+  for (; made < 2 * kMaxProf; ++made)
+    if (hipEventCreate(&p->ev[made]) != hipSuccess) { rc = CS_E_HIP; set_error("hipEventCreate failed"); break; }
+  if (rc == CS_OK) rc = ensure_prims(h, h->cur);
+  if (rc == CS_OK) rc = hipStreamSynchronize(h->stream) == hipSuccess ? CS_OK : CS_E_HIP;
+  h->prof = p;
+  for (int n = 0; n < nsteps && rc == CS_OK; ++n) {
+    p->n = 0;
+    rc = launch_step(h);
+    if (rc == CS_OK && hipStreamSynchronize(h->stream) != hipSuccess) { rc = CS_E_HIP; set_error("step failed"); }
+    for (int q = 0; q < p->n && rc == CS_OK; ++q) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, p->ev[2 * q], p->ev[2 * q + 1]) != hipSuccess) {
+        rc = CS_E_HIP;
+        set_error("hipEventElapsedTime failed");
+        break;
+      }
+      ms_sum[p->kid[q]] += ms;
+      calls[p->kid[q]] += 1;
+    }
+  }
+  h->prof = nullptr;
+  for (int q = 0; q < made; ++q) hipEventDestroy(p->ev[q]);
+  delete p;
+  if (rc) return rc;
   return check_flags(h);
 }
 

@@ -205,6 +205,12 @@ int cs_max_divb(cs_handle* h, double* out);
 #define CS_K_COUNT 16
 int cs_time_steps(cs_handle* h, int32_t nsteps, int32_t use_graph, float* ms_total);
 int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float* ms_avg);
+/* cs_profile_steps: nsteps whole steps run launch by launch (no graph) with a
+ * hipEvent pair around every launch on the handle's stream; ms_sum[id] and
+ * calls[id] (arrays of CS_K_COUNT) accumulate the time and the launches per
+ * kernel id, as the kernels run inside the step (caches as the step leaves
+ * them).  Advances the state by nsteps. */
+int cs_profile_steps(cs_handle* h, int32_t nsteps, float* ms_sum, int32_t* calls);
 int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units);
 
 int cs_last_error(char* buf, size_t len);
