@@ -71,6 +71,21 @@ def algorithmic_bytes_per_unit(cs, kernel, prec, surface):
     return 0
 
 
+def describe(cs, cfg):
+    """Scheme and initial state of a workload, in words."""
+    rs = "HLLD" if cfg.riemann == cs.RS_HLLD else "HLL"
+    rec = "WENO5" if cfg.recon == cs.RECON_WENO5 else "PCM"
+    if cfg.imex:
+        tint = "IMEX-SSP2(2,2,2) with implicit gravity 1/r^2"
+    else:
+        tint = "SSP-RK3" if cfg.rk_order == 3 else "Heun RK2"
+        if cfg.grav:
+            tint += " with explicit gravity 1/r^2"
+    ic = "surface, magnetised blast" if cfg.nr == 1 else "shell r in [%g, %g], hydrostatic + 1e-2 density noise" % (
+        cfg.rmin, cfg.rmax)
+    return f"{rec} + {rs} + 2-D HLL/LLF EMF + CT, {tint}, {ic}"
+
+
 def measured_traffic(kernel, prec, workload):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter read
@@ -287,8 +302,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if head == cs.MIXED else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: 6x{cfg.ntp}x{cfg.ntp}x{cfg.nr} zones, WENO5 + HLLD + "
-                               f"2-D HLL/LLF EMF + CT, SSP-RK3, magnetised blast",
+        "config": {"workload": f"{args.workload}: 6x{cfg.ntp}x{cfg.ntp}x{cfg.nr} zones, {describe(cs, cfg)}",
                    "zones": r["n_int"], "precision": "mixed" if head == cs.MIXED else "fp64",
                    "graph": True},
         "roofline": roof,

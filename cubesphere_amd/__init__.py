@@ -66,12 +66,12 @@ def workload(name, precision=FP64):
         return make_config(32, 1, 3, RECON_WENO5, RS_HLL, EMF_LLF, precision, 3), 1, 100
     if name == "C2":   # 6x256^2 surface, WENO + HLLD + SSP-RK3, CT, magnetised blast
         return make_config(256, 1, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 3), 2, 500
-    if name == "C3":   # 6x192^2x64 shell r in [1,1.5], WENO5 + HLLD + CT, gravity 1/r^2
-        return make_config(192, 64, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 3,
-                           RBC_PAD, RMESH_UNIFORM, 1.0, 1.5, grav=1.0), 3, 10
-    if name == "C5":   # 6x512^2x128
-        return make_config(512, 128, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 3,
-                           RBC_PAD, RMESH_UNIFORM, 1.0, 1.5, grav=1.0), 3, 50
+    if name == "C3":   # 6x192^2x64 shell r in [1,1.5], WENO5 + HLLD + CT, IMEX-RK with gravity 1/r^2 implicit
+        return make_config(192, 64, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 2,
+                           RBC_PAD, RMESH_UNIFORM, 1.0, 1.5, grav=1.0, imex=1), 3, 10
+    if name == "C5":   # 6x512^2x128, the same scheme
+        return make_config(512, 128, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 2,
+                           RBC_PAD, RMESH_UNIFORM, 1.0, 1.5, grav=1.0, imex=1), 3, 50
     raise ValueError(name)
 
 

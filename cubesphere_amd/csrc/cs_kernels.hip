@@ -105,7 +105,7 @@ struct State {
 
 struct Params {
   double gamma, gm1, rhofloor, prsfloor, cfl, grav, igm1;
-  int recon, riemann, emf, wall;
+  int recon, riemann, emf, wall, imex;
 };
 
 // sector neighbours right, left, up, down (203fill_ghost2d_global.f:37-75), 0-based
@@ -908,6 +908,82 @@ __device__ __forceinline__ void cleanup_zone(const CsDims& d, const Geo& geo, co
   if (!(u[0] == u[0]) || !(u[4] == u[4]) || isinf(u[4])) { if (flags) flags[1] = 1; }
 }
 
+// Combination of a zone's stage update with the radial gravity source
+// S(U) = (0, rho gr n, gr (rho v . n)), gr = -grav / rc^2 at the zone's mid
+// radius, n the unit normal of its radial faces (the point-mass gravity of
+// 470build_non_stiff_sources.f:150-250).  The entropy carries no source.
+//  mode 0: explicit source, Shu-Osher  u = wa u0 + wb (in + dt (L + S(in))).
+//  IMEX-SSP2(2,2,2) (Pareschi & Russo 2005; explicit part = Heun, the
+//  reference's RK2), gravity implicit, gi = 1 - 1/sqrt(2):
+//  mode 1: in = U1, u0 = Un:  U2 = Un + dt L(U1) + (1 - 2 gi) dt S(U1) + gi dt S(U2).
+//          S leaves rho unchanged and is linear in rho v, so U2 is explicit:
+//          rho v then E take the implicit term in turn.
+//  mode 2: in = U2, u0 = Un:  Un+1 = (Un + U2 + dt L(U2)) / 2 + dt/2 (2 gi S(U1) + (1 - gi) S(U2)),
+//          with U1 = Un + gi dt S(U1) rebuilt from Un.
+__device__ __forceinline__ void finish_zone(const CsDims& d, const State& in, const State& u0, int s, int64_t c,
+                                            double dt, double wa, double wb, int mode, double gr, const double* n,
+                                            const double* L, double* u, double& en) {
+  double ui[5];
+  for (int q = 0; q < 5; ++q) ui[q] = in.U[(s * 8 + q) * d.cpp + c];
+  const double ei = in.ent[s * d.cpp + c];
+  if (mode == 0) {
+    double Lg[5] = {L[0], L[1], L[2], L[3], L[4]};
+    if (gr != 0.0) {
+      const double mr = ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2];
+      for (int q = 0; q < 3; ++q) Lg[1 + q] += ui[0] * gr * n[q];
+      Lg[4] += mr * gr;
+    }
+    for (int q = 0; q < 5; ++q) {
+      double un = ui[q] + dt * Lg[q];
+      u[q] = wa != 0.0 ? wa * u0.U[(s * 8 + q) * d.cpp + c] + wb * un : wb * un;
+    }
+    double e1 = ei + dt * L[5];
+    en = wa != 0.0 ? wa * u0.ent[s * d.cpp + c] + wb * e1 : wb * e1;
+    return;
+  }
+  const double gi = 1.0 - 0.70710678118654752440;
+  double uz[5];
+  for (int q = 0; q < 5; ++q) uz[q] = u0.U[(s * 8 + q) * d.cpp + c];
+  const double ez = u0.ent[s * d.cpp + c];
+  if (mode == 1) {
+    const double s1m = ui[0] * gr, s1e = gr * (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]);
+    u[0] = uz[0] + dt * L[0];
+    for (int q = 1; q < 4; ++q) u[q] = uz[q] + dt * L[q] + (1.0 - 2.0 * gi) * dt * s1m * n[q - 1];
+    u[4] = uz[4] + dt * L[4] + (1.0 - 2.0 * gi) * dt * s1e;
+    for (int q = 1; q < 4; ++q) u[q] += gi * dt * u[0] * gr * n[q - 1];
+    u[4] += gi * dt * gr * (u[1] * n[0] + u[2] * n[1] + u[3] * n[2]);
+    en = ez + dt * L[5];
+    return;
+  }
+  const double s2m = ui[0] * gr, s2e = gr * (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]);
+  const double s1m = uz[0] * gr, s1e = gr * ((uz[1] * n[0] + uz[2] * n[1] + uz[3] * n[2]) + gi * dt * uz[0] * gr);
+  for (int q = 0; q < 5; ++q) u[q] = 0.5 * uz[q] + 0.5 * (ui[q] + dt * L[q]);
+  for (int q = 1; q < 4; ++q) u[q] += 0.5 * dt * (2.0 * gi * s1m + (1.0 - gi) * s2m) * n[q - 1];
+  u[4] += 0.5 * dt * (2.0 * gi * s1e + (1.0 - gi) * s2e);
+  en = 0.5 * ez + 0.5 * (ei + dt * L[5]);
+}
+
+// First, purely implicit stage of IMEX-SSP2(2,2,2): U1 = Un + gi dt S(U1) in
+// closed form (rho unchanged, then rho v, then E), written to the stage buffer
+// whose faces hold a copy of Un's; the cleanup then gives U1 its zone B and
+// primitives, like any stage output.
+template <typename P>
+__global__ void k_imex_source(CsDims d, Rng rg, Geo geo, State un, State u1, P* V, const double* dtp, Params pr,
+                              int* flags) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  const double dt = dtp[0], gi = 1.0 - 0.70710678118654752440;
+  const int64_t c = d.cell(i, j, k);
+  const double r1 = geo.r(d, k), r0 = geo.r(d, k - 1), rc = 0.5 * (r0 + r1), gr = -pr.grav / (rc * rc);
+  const double zn = geo.t(d, T_ZSN, s, i, j);
+  double n[3], u[8];
+  for (int q = 0; q < 3; ++q) n[q] = geo.t(d, T_ZSX + q, s, i, j) / zn;
+  for (int q = 0; q < 5; ++q) u[q] = un.U[(s * 8 + q) * d.cpp + c];
+  for (int q = 1; q < 4; ++q) u[q] += gi * dt * u[0] * gr * n[q - 1];
+  u[4] += gi * dt * gr * (u[1] * n[0] + u[2] * n[1] + u[3] * n[2]);
+  cleanup_zone<P>(d, geo, u1, V, pr, flags, s, i, j, k, u, un.ent[s * d.cpp + c]);
+}
+
 // U_out = a U0 + b (U_in + dt L(U_in)) for rho, rho v, E and entropy
 // (UPDATE_IMEX_RK_STEP_P1 484update_imex_rk_step_p1.f:483-497, 547-593 with the
 // Shu-Osher stage weights; the RK2 weights reproduce DRV:2661-2773).
@@ -915,7 +991,7 @@ template <typename P>
 __global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
                                const double* __restrict__ Fx, const double* __restrict__ Fy,
                                const double* __restrict__ Fz, const double* dtp, double wa,
-                               double wb, Params pr, P* Vout, int* flags) {
+                               double wb, Params pr, P* Vout, int* flags, int mode) {
   int i, j, k, s;
   if (!decode(rg, gtid(), i, j, k, s)) return;
   const double dt = dtp[0];
@@ -931,30 +1007,14 @@ __global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, St
     int64_t o = ((int64_t)s * 9 + comps[q]) * d.fpp;
     L[q] = -ov * (lateral_div(d, zf, Fx, Fy, comps[q]) + (Fz[o + fxp] - Fz[o + fzm]));
   }
+  double n[3] = {0.0, 0.0, 0.0}, gr = 0.0;
   if (pr.grav != 0.0) {
-    // explicit point-mass gravity g = -grav / r^2 rhat (470build_non_stiff_sources.f:150-250)
-    double rc = 0.5 * (r0 + r1);
-    double zn = geo.t(d, T_ZSN, s, i, j);
-    double nx = geo.t(d, T_ZSX, s, i, j) / zn, ny = geo.t(d, T_ZSY, s, i, j) / zn, nz = geo.t(d, T_ZSZ, s, i, j) / zn;
-    double gr = -pr.grav / (rc * rc);
-    double rho = in.U[(s * 8 + 0) * d.cpp + c];
-    double mr = in.U[(s * 8 + 1) * d.cpp + c] * nx + in.U[(s * 8 + 2) * d.cpp + c] * ny + in.U[(s * 8 + 3) * d.cpp + c] * nz;
-    L[1] += rho * gr * nx;
-    L[2] += rho * gr * ny;
-    L[3] += rho * gr * nz;
-    L[4] += mr * gr;
+    const double zn = geo.t(d, T_ZSN, s, i, j), rc = 0.5 * (r0 + r1);
+    for (int q = 0; q < 3; ++q) n[q] = geo.t(d, T_ZSX + q, s, i, j) / zn;
+    gr = -pr.grav / (rc * rc);
   }
   double u[8], en;
-  for (int q = 0; q < 5; ++q) {
-    int64_t o = (s * 8 + q) * d.cpp + c;
-    double un = in.U[o] + dt * L[q];
-    u[q] = wa != 0.0 ? wa * u0.U[o] + wb * un : wb * un;
-  }
-  {
-    int64_t o = s * d.cpp + c;
-    double e1 = in.ent[o] + dt * L[5];
-    en = wa != 0.0 ? wa * u0.ent[o] + wb * e1 : wb * e1;
-  }
+  finish_zone(d, in, u0, s, c, dt, wa, wb, mode, gr, n, L, u, en);
   cleanup_zone<P>(d, geo, out, Vout, pr, flags, s, i, j, k, u, en);
 }
 
@@ -968,7 +1028,7 @@ template <typename R, typename P>
 __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
                                     P* V, const double* __restrict__ Fx,
                                     const double* __restrict__ Fy, const double* dtp, double wa,
-                                    double wb, Params pr, int* flags) {
+                                    double wb, Params pr, int* flags, int mode) {
   int i, j, k, s;
   if (!decode(rg, gtid(), i, j, k, s)) return;
   const double dt = dtp[0];
@@ -1018,27 +1078,9 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
   solve_face<R>(okm ? Wm : Vm, okp ? Wp : Vc, A, Bv, Cv, in.Bz[so + d.face(i, j, k - 1)], pr, fi, sl, sr);
   const double a1 = r1 * r1 * zn, a0 = r0 * r0 * zn;
   for (int q = 0; q < 6; ++q) L[q] -= ov * (fo[comps[q]] * a1 - fi[comps[q]] * a0);
-  if (pr.grav != 0.0) {  // as in k_update_cells
-    double rc = 0.5 * (r0 + r1), gr = -pr.grav / (rc * rc);
-    double rho = in.U[(s * 8 + 0) * d.cpp + c];
-    double mr = in.U[(s * 8 + 1) * d.cpp + c] * A[0] + in.U[(s * 8 + 2) * d.cpp + c] * A[1] +
-                in.U[(s * 8 + 3) * d.cpp + c] * A[2];
-    L[1] += rho * gr * A[0];
-    L[2] += rho * gr * A[1];
-    L[3] += rho * gr * A[2];
-    L[4] += mr * gr;
-  }
+  const double rcz = 0.5 * (r0 + r1), gr = pr.grav != 0.0 ? -pr.grav / (rcz * rcz) : 0.0;
   double u[8], en;
-  for (int q = 0; q < 5; ++q) {
-    int64_t o = (s * 8 + q) * d.cpp + c;
-    double un = in.U[o] + dt * L[q];
-    u[q] = wa != 0.0 ? wa * u0.U[o] + wb * un : wb * un;
-  }
-  {
-    int64_t o = s * d.cpp + c;
-    double e1 = in.ent[o] + dt * L[5];
-    en = wa != 0.0 ? wa * u0.ent[o] + wb * e1 : wb * e1;
-  }
+  finish_zone(d, in, u0, s, c, dt, wa, wb, mode, gr, A, L, u, en);
   // V was read above for this zone only, so it is overwritten in place
   cleanup_zone<P>(d, geo, out, V, pr, flags, s, i, j, k, u, en);
 }
@@ -1306,8 +1348,14 @@ int launch_halo(cs_handle* h, int b, bool full = true, bool avg = true) {
                                       : launch_halo_t<double>(h, b, full, avg);
 }
 
-void stage_weights(int order, int irk, double& wa, double& wb, int& in, int& out) {
-  // buffers: 0 = U^n (relative to h->cur), 1 = scratch
+void stage_weights(int order, int irk, double& wa, double& wb, int& in, int& out, int imex = 0) {
+  // buffers: 0 = U^n (relative to h->cur), 1 = scratch.  IMEX-SSP2(2,2,2)
+  // reads its first flux stage from U1 = Un + gi dt S(U1) in the scratch buffer.
+  if (order == 2 && imex) {
+    if (irk == 1) { wa = 0.0; wb = 1.0; in = 1; out = 1; }
+    else { wa = 0.5; wb = 0.5; in = 1; out = 0; }
+    return;
+  }
   if (order == 3) {
     if (irk == 1) { wa = 0.0; wb = 1.0; in = 0; out = 1; }
     else if (irk == 2) { wa = 0.75; wb = 0.25; in = 1; out = 1; }
@@ -1321,7 +1369,7 @@ void stage_weights(int order, int irk, double& wa, double& wb, int& in, int& out
 int stage_input(cs_handle* h, int irk) {
   double wa, wb;
   int bi, bo;
-  stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo);
+  stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo, h->pr.imex);
   return (h->cur + bi) & 1;
 }
 
@@ -1393,7 +1441,7 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
   hipStream_t sm = h->stream;
   double wa = 0, wb = 1;
   int bi = 0, bo = 1;
-  stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo);
+  stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo, h->pr.imex);
   State in = state_of(h, (h->cur + bi) & 1), u0 = state_of(h, h->cur), out = state_of(h, (h->cur + bo) & 1);
   switch (kid) {
     case CS_K_HALO: return launch_halo(h, b, !surf, false);  // the stage's exchange
@@ -1434,10 +1482,12 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
     case CS_K_UPDATE_CELLS:  // zone update + cleanup + primitives of the output
       if (surf)
         k_update_cells_surf<R, P><<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, V, h->F[0],
-                                                                          h->F[1], h->dt, wa, wb, h->pr, h->flags);
+                                                                          h->F[1], h->dt, wa, wb, h->pr, h->flags,
+                                                                          h->pr.imex ? irk : 0);
       else
         k_update_cells<P><<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, h->F[0], h->F[1],
-                                                                    h->F[2], h->dt, wa, wb, h->pr, V, h->flags);
+                                                                    h->F[2], h->dt, wa, wb, h->pr, V, h->flags,
+                                                                    h->pr.imex ? irk : 0);
       break;
     case CS_K_UPDATE_FACES:
       k_update_faces<<<nblk(r.faces.total), kBlock, 0, sm>>>(d, r.faces, geo, in, u0, out, h->E[0], h->E[1],
@@ -1507,7 +1557,7 @@ int launch_update(cs_handle* h, int irk, bool surf = false) {
   if (rc) return rc;
   double wa, wb;
   int bi, bo;
-  stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo);
+  stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo, h->pr.imex);
   h->vbuf = (h->cur + bo) & 1;
   return CS_OK;
 }
@@ -1527,10 +1577,31 @@ int ensure_prims(cs_handle* h, int b) {
 }
 int launch_cfl(cs_handle* h) { return launch_kernel(h, CS_K_CFL, h->cur, 1); }
 
+// IMEX-SSP2(2,2,2) implicit first stage into the scratch buffer: its faces
+// are copied from U^n (the stage leaves B alone), then U1 with its cleanup.
+template <typename P>
+int launch_imex_source_t(cs_handle* h) {
+  const int sb = (h->cur + 1) & 1;
+  const size_t fb = sizeof(double) * h->d.fpp * 6;
+  for (int q = 2; q < 5; ++q)
+    CS_HIP(hipMemcpyAsync(h->buf[sb][q], h->buf[h->cur][q], fb, hipMemcpyDeviceToDevice, h->stream));
+  const Ranges r = ranges_of(h->d, h->pr.wall);
+  k_imex_source<P><<<nblk(r.cells.total), kBlock, 0, h->stream>>>(h->d, r.cells, geo_of(h), state_of(h, h->cur),
+                                                                   state_of(h, sb), (P*)h->V, h->dt, h->pr,
+                                                                   h->flags);
+  CS_HIP(hipGetLastError());
+  h->vbuf = sb;
+  return CS_OK;
+}
+int launch_imex_source(cs_handle* h) {
+  return h->cfg.precision == CS_MIXED ? launch_imex_source_t<float>(h) : launch_imex_source_t<double>(h);
+}
+
 int launch_stage(cs_handle* h, int irk) {
   int b = stage_input(h, irk);
   int rc;
   const bool surf = h->surf != 0;
+  if (h->pr.imex && irk == 1 && (rc = launch_imex_source(h))) return rc;
   if ((rc = ensure_prims(h, b))) return rc;
   // panel-edge faces are consistent (written once by the previous face update)
   if ((rc = launch_kernel(h, CS_K_HALO, b, 1, surf))) return rc;
@@ -1629,7 +1700,8 @@ extern "C" int cs_create(const cs_config* cfg, cs_handle** out) {
   h->cfg = *cfg;
   h->d = cs_make_dims(cfg->ntp, cfg->nr, cfg->nghost);
   h->pr = Params{cfg->gamma, cfg->gamma - 1.0, cfg->rhofloor, cfg->prsfloor, cfg->cfl, cfg->grav,
-                 1.0 / (cfg->gamma - 1.0), cfg->recon, cfg->riemann, cfg->emf, cfg->radial_bc == CS_RBC_WALL ? 1 : 0};
+                 1.0 / (cfg->gamma - 1.0), cfg->recon, cfg->riemann, cfg->emf, cfg->radial_bc == CS_RBC_WALL ? 1 : 0,
+                 cfg->imex};
   h->surf = (cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1) ? 1 : 0;
   if ((rc = load_corner_table(h->d, T))) {
     delete h;
@@ -1751,6 +1823,10 @@ extern "C" int cs_make_edge_efields(cs_handle* h) {
 extern "C" int cs_update_stage(cs_handle* h, int32_t irk) {
   CHECK_H(h);
   if (irk < 1 || irk > h->cfg.rk_order) { set_error("irk %d out of range", irk); return CS_E_INVALID; }
+  if (h->pr.imex) {
+    set_error("implicit gravity needs the implicit first stage: advance with cs_rk_stage or cs_advance");
+    return CS_E_INVALID;
+  }
   int rc = launch_update(h, irk);
   if (rc) return rc;
   CS_HIP(hipStreamSynchronize(h->stream));

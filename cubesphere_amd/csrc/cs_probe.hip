@@ -59,6 +59,7 @@ int main(int argc, char** argv) {
     problem = !strcmp(wl, "C1") ? 1 : 2;
   } else if (!strcmp(wl, "C3")) {
     c.ntp = 192; c.nr = 64; c.radial_bc = CS_RBC_PAD; c.rmax = 1.5; c.grav = 1.0;
+    c.rk_order = 2; c.imex = 1;  // IMEX-SSP2(2,2,2), gravity implicit
     problem = 3;
   } else {
     fprintf(stderr, "unknown workload %s\n", wl);

@@ -1287,6 +1287,73 @@ static void weights(int order, int irk, double* wa, double* wb) {
 
 /* UPDATE_IMEX_RK_STEP_P1 (484:483-497, 547-593, CT 646-908) and
    CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:4) */
+/* CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:4): zone B from the six
+   faces through the zone-B operator, then CONS_TO_PRIM with its floors
+   (662pde_3d_cons_to_prim.f:69-151); U and the entropy are stored back. */
+static void cleanup(const or_grid* G, double* const* out) {
+  int N = G->N, NR = G->NR, s, i, j, k, q;
+  for (s = 0; s < 6; ++s)
+    for (k = 1; k <= NR; ++k)
+      for (j = 1; j <= N; ++j)
+        for (i = 1; i <= N; ++i) {
+          int64_t c = CELL(G, i, j, k), t = T2(G, s, i, j);
+          double b[3], u[8], v[8], e;
+          b[0] = 0.5 * (out[2][s * G->fpp + FACE(G, i - 1, j, k)] + out[2][s * G->fpp + FACE(G, i, j, k)]);
+          b[1] = 0.5 * (out[3][s * G->fpp + FACE(G, i, j - 1, k)] + out[3][s * G->fpp + FACE(G, i, j, k)]);
+          b[2] = 0.5 * (out[4][s * G->fpp + FACE(G, i, j, k - 1)] + out[4][s * G->fpp + FACE(G, i, j, k)]);
+          for (q = 0; q < 8; ++q) u[q] = UU(out[0], G, s, q, c);
+          for (q = 0; q < 3; ++q) u[5 + q] = G->mi[3 * q][t] * b[0] + G->mi[3 * q + 1][t] * b[1] + G->mi[3 * q + 2][t] * b[2];
+          e = out[1][s * G->cpp + c];
+          or_cons_to_prim(G->c.gamma, G->c.rhofloor, G->c.prsfloor, &e, u, v);
+          for (q = 0; q < 8; ++q) UU(out[0], G, s, q, c) = u[q];
+          out[1][s * G->cpp + c] = e;
+        }
+}
+
+/* Radial gravity at zone (s, i, j, k): gr = -grav / rc^2 along the unit
+   normal n of the radial faces (470build_non_stiff_sources.f:150-250). */
+static double gravity_at(const or_grid* G, int s, int i, int j, int k, double* n) {
+  int64_t t = T2(G, s, i, j);
+  double rc = 0.5 * (R(G, k - 1) + R(G, k));
+  int q;
+  for (q = 0; q < 3; ++q) n[q] = G->zs[q][t] / G->zsn[t];
+  return -G->c.grav / (rc * rc);
+}
+#define IMEX_GI (1.0 - 0.70710678118654752440) /* 1 - 1/sqrt(2) */
+
+/* IMEX-SSP2(2,2,2) (Pareschi & Russo 2005; explicit part Heun, the
+   reference's RK2 of DRV:2661-2773) with the gravity source
+   S(U) = (0, rho gr n, gr (rho v . n)) implicit.  u = (rho, rho v, E).
+   First stage U1 = Un + gi dt S(U1): rho is unchanged and S is linear in
+   rho v, so rho v and then E follow in closed form. */
+void or_imex_pre(double dt, double gr, const double* n, double* u) {
+  int q;
+  for (q = 1; q < 4; ++q) u[q] += IMEX_GI * dt * u[0] * gr * n[q - 1];
+  u[4] += IMEX_GI * dt * gr * (u[1] * n[0] + u[2] * n[1] + u[3] * n[2]);
+}
+/* stage 1 (in = U1, uz = Un): U2 = Un + dt L(U1) + (1-2gi) dt S(U1) + gi dt S(U2);
+   stage 2 (in = U2, uz = Un): Un+1 = (Un + U2 + dt L(U2))/2 + dt/2 (2gi S(U1) + (1-gi) S(U2)),
+   with S(U1) rebuilt from Un. */
+void or_imex_combine(int stage, double dt, double gr, const double* n, const double* ui, const double* uz,
+                     const double* L, double* u) {
+  double gi = IMEX_GI;
+  int q;
+  if (stage == 1) {
+    double s1m = ui[0] * gr, s1e = gr * (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]);
+    u[0] = uz[0] + dt * L[0];
+    for (q = 1; q < 4; ++q) u[q] = uz[q] + dt * L[q] + (1.0 - 2.0 * gi) * dt * s1m * n[q - 1];
+    u[4] = uz[4] + dt * L[4] + (1.0 - 2.0 * gi) * dt * s1e;
+    for (q = 1; q < 4; ++q) u[q] += gi * dt * u[0] * gr * n[q - 1];
+    u[4] += gi * dt * gr * (u[1] * n[0] + u[2] * n[1] + u[3] * n[2]);
+  } else {
+    double s2m = ui[0] * gr, s2e = gr * (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]);
+    double s1m = uz[0] * gr, s1e = gr * ((uz[1] * n[0] + uz[2] * n[1] + uz[3] * n[2]) + gi * dt * uz[0] * gr);
+    for (q = 0; q < 5; ++q) u[q] = 0.5 * uz[q] + 0.5 * (ui[q] + dt * L[q]);
+    for (q = 1; q < 4; ++q) u[q] += 0.5 * dt * (2.0 * gi * s1m + (1.0 - gi) * s2m) * n[q - 1];
+    u[4] += 0.5 * dt * (2.0 * gi * s1e + (1.0 - gi) * s2e);
+  }
+}
+
 void or_update(const or_grid* G, int irk, double dt, const double* const* in, const double* const* u0,
                double* const* out, double** F, double** E) {
   int N = G->N, NR = G->NR, s, i, j, k, q;
@@ -1304,22 +1371,30 @@ void or_update(const or_grid* G, int irk, double dt, const double* const* in, co
             int64_t o = ((int64_t)s * 9 + comps[q]) * G->fpp;
             L[q] = -ov * ((F[0][o + fp] - F[0][o + fxm]) + (F[1][o + fp] - F[1][o + fym]) + (F[2][o + fp] - F[2][o + fzm]));
           }
-          if (G->c.grav != 0.0) { /* 470build_non_stiff_sources.f:150-250, g = grav / r^2 */
-            int64_t t = T2(G, s, i, j);
-            double rc = 0.5 * (R(G, k - 1) + R(G, k)), n[3], gr = -G->c.grav / (rc * rc), rho, mr;
-            for (q = 0; q < 3; ++q) n[q] = G->zs[q][t] / G->zsn[t];
-            rho = UU(in[0], G, s, 0, c);
-            mr = UU(in[0], G, s, 1, c) * n[0] + UU(in[0], G, s, 2, c) * n[1] + UU(in[0], G, s, 3, c) * n[2];
-            for (q = 0; q < 3; ++q) L[1 + q] += rho * gr * n[q];
-            L[4] += mr * gr;
-          }
-          for (q = 0; q < 5; ++q) {
-            double un = UU(in[0], G, s, q, c) + dt * L[q];
-            UU(out[0], G, s, q, c) = wa != 0.0 ? wa * UU(u0[0], G, s, q, c) + wb * un : wb * un;
-          }
-          {
-            double en = in[1][s * G->cpp + c] + dt * L[5];
-            out[1][s * G->cpp + c] = wa != 0.0 ? wa * u0[1][s * G->cpp + c] + wb * en : wb * en;
+          if (!G->c.imex) {
+            if (G->c.grav != 0.0) { /* explicit gravity */
+              double n[3], gr = gravity_at(G, s, i, j, k, n), rho, mr;
+              rho = UU(in[0], G, s, 0, c);
+              mr = UU(in[0], G, s, 1, c) * n[0] + UU(in[0], G, s, 2, c) * n[1] + UU(in[0], G, s, 3, c) * n[2];
+              for (q = 0; q < 3; ++q) L[1 + q] += rho * gr * n[q];
+              L[4] += mr * gr;
+            }
+            for (q = 0; q < 5; ++q) {
+              double un = UU(in[0], G, s, q, c) + dt * L[q];
+              UU(out[0], G, s, q, c) = wa != 0.0 ? wa * UU(u0[0], G, s, q, c) + wb * un : wb * un;
+            }
+            {
+              double en = in[1][s * G->cpp + c] + dt * L[5];
+              out[1][s * G->cpp + c] = wa != 0.0 ? wa * u0[1][s * G->cpp + c] + wb * en : wb * en;
+            }
+          } else {
+            /* IMEX-SSP2(2,2,2), gravity implicit; the entropy has no source */
+            double n[3], gr = gravity_at(G, s, i, j, k, n), ui[5], uz[5], u[5];
+            for (q = 0; q < 5; ++q) { ui[q] = UU(in[0], G, s, q, c); uz[q] = UU(u0[0], G, s, q, c); }
+            or_imex_combine(irk, dt, gr, n, ui, uz, L, u);
+            if (irk == 1) out[1][s * G->cpp + c] = u0[1][s * G->cpp + c] + dt * L[5];
+            else out[1][s * G->cpp + c] = 0.5 * u0[1][s * G->cpp + c] + 0.5 * (in[1][s * G->cpp + c] + dt * L[5]);
+            for (q = 0; q < 5; ++q) UU(out[0], G, s, q, c) = u[q];
           }
         }
   for (s = 0; s < 6; ++s) {
@@ -1349,23 +1424,7 @@ void or_update(const or_grid* G, int irk, double dt, const double* const* in, co
           out[4][s * G->fpp + f] = wa != 0.0 ? wa * u0[4][s * G->fpp + f] + wb * bn : wb * bn;
         }
   }
-  /* cleanup: zone B from faces, then CONS_TO_PRIM floors */
-  for (s = 0; s < 6; ++s)
-    for (k = 1; k <= NR; ++k)
-      for (j = 1; j <= N; ++j)
-        for (i = 1; i <= N; ++i) {
-          int64_t c = CELL(G, i, j, k), t = T2(G, s, i, j);
-          double b[3], u[8], v[8], e;
-          b[0] = 0.5 * (out[2][s * G->fpp + FACE(G, i - 1, j, k)] + out[2][s * G->fpp + FACE(G, i, j, k)]);
-          b[1] = 0.5 * (out[3][s * G->fpp + FACE(G, i, j - 1, k)] + out[3][s * G->fpp + FACE(G, i, j, k)]);
-          b[2] = 0.5 * (out[4][s * G->fpp + FACE(G, i, j, k - 1)] + out[4][s * G->fpp + FACE(G, i, j, k)]);
-          for (q = 0; q < 8; ++q) u[q] = UU(out[0], G, s, q, c);
-          for (q = 0; q < 3; ++q) u[5 + q] = G->mi[3 * q][t] * b[0] + G->mi[3 * q + 1][t] * b[1] + G->mi[3 * q + 2][t] * b[2];
-          e = out[1][s * G->cpp + c];
-          or_cons_to_prim(G->c.gamma, G->c.rhofloor, G->c.prsfloor, &e, u, v);
-          for (q = 0; q < 8; ++q) UU(out[0], G, s, q, c) = u[q];
-          out[1][s * G->cpp + c] = e;
-        }
+  cleanup(G, out);
 }
 
 /* EVALUATE_TIMESTEP (490evaluate_timestep.f:118-278) */
@@ -1424,8 +1483,23 @@ void or_advance(const or_grid* G, int nsteps, double* U, double* ent, double* Bx
   memcpy(B[0], U, sizeof(double) * cs * 8); memcpy(B[1], ent, sizeof(double) * cs);
   for (q = 2; q < 5; ++q) memcpy(B[q], A[q], sizeof(double) * fs);
   for (n = 0; n < nsteps; ++n) {
+    if (G->c.imex) { /* U1 = Un + gi dt S(U1), faces copied, then its cleanup */
+      int s, i, j, k;
+      for (q = 0; q < 5; ++q) memcpy(B[q], A[q], sizeof(double) * (q == 0 ? cs * 8 : (q == 1 ? cs : fs)));
+      for (s = 0; s < 6; ++s)
+        for (k = 1; k <= G->NR; ++k)
+          for (j = 1; j <= G->N; ++j)
+            for (i = 1; i <= G->N; ++i) {
+              int64_t c = CELL(G, i, j, k);
+              double nn[3], gr = gravity_at(G, s, i, j, k, nn), u[5];
+              for (q = 0; q < 5; ++q) u[q] = UU(A[0], G, s, q, c);
+              or_imex_pre(*dt, gr, nn, u);
+              for (q = 0; q < 5; ++q) UU(B[0], G, s, q, c) = u[q];
+            }
+      cleanup(G, B);
+    }
     for (irk = 1; irk <= G->c.rk_order; ++irk) {
-      double* const* in = irk == 1 ? A : B;
+      double* const* in = (irk == 1 && !G->c.imex) ? A : B;
       double* const* out = (irk == G->c.rk_order) ? A : B;
       or_stage(G, irk, *dt, in, A, out);
     }

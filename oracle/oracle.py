@@ -56,6 +56,8 @@ def olib():
         L.or_emf4_hll.restype = dbl
         L.or_emf3_llf.argtypes = [dbl, vp, vp]
         L.or_emf3_llf.restype = dbl
+        L.or_imex_pre.argtypes = [dbl, dbl, vp, vp]
+        L.or_imex_combine.argtypes = [i32, dbl, dbl, vp, vp, vp, vp, vp]
         _o = L
     return _o
 

@@ -23,6 +23,8 @@ def _cases():
         "shell8x3_hll_emfhll_log": (cs.make_config(8, 3, 3, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_HLL, cs.FP64, 3,
                                                    cs.RBC_PAD, cs.RMESH_LOG, 1.0, 2.0), 3),
         "surf8_pcm_wall": (cs.make_config(8, 1, 3, cs.RECON_PCM, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 3), 1),
+        "shell8x3_imex_grav": (cs.make_config(8, 3, 3, cs.RECON_WENO5, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 2,
+                                              cs.RBC_PAD, cs.RMESH_UNIFORM, 1.0, 1.5, grav=1.0, imex=1), 3),
     }
 
 
