@@ -71,7 +71,7 @@ def algorithmic_bytes_per_unit(cs, kernel, prec, surface):
     return 0
 
 
-def describe(cs, cfg):
+def describe(cs, cfg, problem):
     """Scheme and initial state of a workload, in words."""
     rs = "HLLD" if cfg.riemann == cs.RS_HLLD else "HLL"
     rec = "WENO5" if cfg.recon == cs.RECON_WENO5 else "PCM"
@@ -81,8 +81,9 @@ def describe(cs, cfg):
         tint = "SSP-RK3" if cfg.rk_order == 3 else "Heun RK2"
         if cfg.grav:
             tint += " with explicit gravity 1/r^2"
-    ic = "surface, magnetised blast" if cfg.nr == 1 else "shell r in [%g, %g], hydrostatic + 1e-2 density noise" % (
-        cfg.rmin, cfg.rmax)
+    ic = {0: "coordinate fields", 1: "Gaussian bump + solid-body rotation", 2: "magnetised blast",
+          3: "hydrostatic + 1e-2 density noise"}[problem]
+    ic = ("surface, " if cfg.nr == 1 else "shell r in [%g, %g], " % (cfg.rmin, cfg.rmax)) + ic
     return f"{rec} + {rs} + 2-D HLL/LLF EMF + CT, {tint}, {ic}"
 
 
@@ -302,7 +303,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32+f64" if head == cs.MIXED else "f64",
         "data": "synthetic",
-        "config": {"workload": f"{args.workload}: 6x{cfg.ntp}x{cfg.ntp}x{cfg.nr} zones, {describe(cs, cfg)}",
+        "config": {"workload": f"{args.workload}: 6x{cfg.ntp}x{cfg.ntp}x{cfg.nr} zones, {describe(cs, cfg, cs.workload(args.workload)[1])}",
                    "zones": r["n_int"], "precision": "mixed" if head == cs.MIXED else "fp64",
                    "graph": True},
         "roofline": roof,

@@ -67,7 +67,13 @@ enum CsT2 {
   T_OM,                          // solid angle of cell (i,j)
   T_ZCX, T_ZCY, T_ZCZ,           // unit direction of the zone centre (vertex sum)
   T_CHMIN,                       // min lateral chord |u_a - u_b| of cell (i,j)
-  T_M00, T_M01, T_M02, T_M10, T_M11, T_M12, T_M20, T_M21, T_M22, // zone-B operator
+  T_M00, T_M01, T_M02, T_M10, T_M11, T_M12, T_M20, T_M21, T_M22, // surface-mode zone-B operator
+  // zone-centred B from the six face values, least squares over the face
+  // normals (485cleanup_after_update.f:692-860): B = sum_f W_f b_f with
+  // W_f = (A^T A)^-1 n_f for the x+, x-, y+, y- faces and W_z for both z faces
+  T_BWXP, T_BWXPY, T_BWXPZ, T_BWXM, T_BWXMY, T_BWXMZ,
+  T_BWYP, T_BWYPY, T_BWYPZ, T_BWYM, T_BWYMY, T_BWYMZ,
+  T_BWZ, T_BWZY, T_BWZZ,
   T_EN1X, T_EN1Y, T_EN1Z,        // z-edge frame n1 (along mesh x)
   T_ETAX, T_ETAY, T_ETAZ,        // z-edge frame tau (along mesh y)
   T_ECOS, T_ESIN,                // z-edge cos/sin of the angle between n1 and tau

@@ -105,7 +105,7 @@ struct State {
 
 struct Params {
   double gamma, gm1, rhofloor, prsfloor, cfl, grav, igm1;
-  int recon, riemann, emf, wall, imex;
+  int recon, riemann, emf, wall, imex, surf;
 };
 
 // sector neighbours right, left, up, down (203fill_ghost2d_global.f:37-75), 0-based
@@ -880,23 +880,46 @@ __device__ __forceinline__ double ex_unique(const CsDims& d, const double* __res
 
 // ------------------------------------------------------------------ update
 // CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:4) of one zone of the stage
-// output, run by the zone-update kernels after the faces are updated: the
-// zone-centred B from the zone's six faces through the zone-B operator, then
-// CONS_TO_PRIM (662pde_3d_cons_to_prim.f:69-151) with its floors; U and the
-// entropy are stored back and the same conversion gives the primitives V the
-// next stage reconstructs from.  u[0..4] and ent hold the updated zone.
+// output, run by the zone-update kernels after the faces are updated.
+// Shells: the zone-centred B is replaced by the least-squares fit to the
+// zone's six face values over their unit normals (:692-860,
+// B = (A^T A)^-1 A^T b, kept as per-face weights W_f = (A^T A)^-1 n_f), and
+// the total energy takes the change of magnetic energy so the thermal and
+// kinetic parts are kept (:845-851).  Surface mode (one layer between walls):
+// the fit would couple the lateral face values into a radial B, which the
+// wall mirror turns into mass flux through the walls, so each face family's
+// average is reproduced along its mean normal instead and the total energy
+// is kept.  Then CONS_TO_PRIM (662pde_3d_cons_to_prim.f:69-151) with its
+// floors; U and the entropy are stored back and the same conversion gives
+// the primitives V the next stage reconstructs from.  u[0..7] and ent hold
+// the updated zone (u[5..7] advanced by the induction fluxes, as the
+// reference's dU_dt).
 template <typename P>
 __device__ __forceinline__ void cleanup_zone(const CsDims& d, const Geo& geo, const State& st, P* V,
                                              const Params& pr, int* flags, int s, int i, int j, int k,
                                              double* u, double ent) {
   const int64_t c = d.cell(i, j, k);
   const int64_t so = (int64_t)s * d.fpp;
-  double b0 = 0.5 * (st.Bx[so + d.face(i - 1, j, k)] + st.Bx[so + d.face(i, j, k)]);
-  double b1 = 0.5 * (st.By[so + d.face(i, j - 1, k)] + st.By[so + d.face(i, j, k)]);
-  double b2 = 0.5 * (st.Bz[so + d.face(i, j, k - 1)] + st.Bz[so + d.face(i, j, k)]);
-  for (int r = 0; r < 3; ++r)
-    u[5 + r] = geo.t(d, T_M00 + 3 * r, s, i, j) * b0 + geo.t(d, T_M00 + 3 * r + 1, s, i, j) * b1 +
-               geo.t(d, T_M00 + 3 * r + 2, s, i, j) * b2;
+  const double bxp = st.Bx[so + d.face(i, j, k)], bxm = st.Bx[so + d.face(i - 1, j, k)];
+  const double byp = st.By[so + d.face(i, j, k)], bym = st.By[so + d.face(i, j - 1, k)];
+  const double bzp = st.Bz[so + d.face(i, j, k)], bzm = st.Bz[so + d.face(i, j, k - 1)];
+  if (pr.surf) {
+    const double b0 = 0.5 * (bxm + bxp), b1 = 0.5 * (bym + byp), b2 = 0.5 * (bzm + bzp);
+    for (int r = 0; r < 3; ++r)
+      u[5 + r] = geo.t(d, T_M00 + 3 * r, s, i, j) * b0 + geo.t(d, T_M00 + 3 * r + 1, s, i, j) * b1 +
+                 geo.t(d, T_M00 + 3 * r + 2, s, i, j) * b2;
+  } else {
+    double bn[3];
+    for (int r = 0; r < 3; ++r)
+      bn[r] = geo.t(d, T_BWXP + r, s, i, j) * bxp + geo.t(d, T_BWXM + r, s, i, j) * bxm +
+              geo.t(d, T_BWYP + r, s, i, j) * byp + geo.t(d, T_BWYM + r, s, i, j) * bym +
+              geo.t(d, T_BWZ + r, s, i, j) * (bzp + bzm);
+    u[4] = u[4] - (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) * C<double>::ob8pi +
+           (bn[0] * bn[0] + bn[1] * bn[1] + bn[2] * bn[2]) * C<double>::ob8pi;
+    u[5] = bn[0];
+    u[6] = bn[1];
+    u[7] = bn[2];
+  }
   Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor, pr.igm1};
   double v[8];
   int fl = cons_to_prim(gs, u, v, ent);
@@ -923,8 +946,10 @@ __device__ __forceinline__ void cleanup_zone(const CsDims& d, const Geo& geo, co
 __device__ __forceinline__ void finish_zone(const CsDims& d, const State& in, const State& u0, int s, int64_t c,
                                             double dt, double wa, double wb, int mode, double gr, const double* n,
                                             const double* L, double* u, double& en) {
-  double ui[5];
-  for (int q = 0; q < 5; ++q) ui[q] = in.U[(s * 8 + q) * d.cpp + c];
+  // L[0..7]: rho, rho v, E, B (zone B advanced by the induction fluxes, as the
+  // reference's dU_dt; the cleanup replaces it); L[8]: entropy
+  double ui[8];
+  for (int q = 0; q < 8; ++q) ui[q] = in.U[(s * 8 + q) * d.cpp + c];
   const double ei = in.ent[s * d.cpp + c];
   if (mode == 0) {
     double Lg[5] = {L[0], L[1], L[2], L[3], L[4]};
@@ -933,17 +958,17 @@ __device__ __forceinline__ void finish_zone(const CsDims& d, const State& in, co
       for (int q = 0; q < 3; ++q) Lg[1 + q] += ui[0] * gr * n[q];
       Lg[4] += mr * gr;
     }
-    for (int q = 0; q < 5; ++q) {
-      double un = ui[q] + dt * Lg[q];
+    for (int q = 0; q < 8; ++q) {
+      double un = ui[q] + dt * (q < 5 ? Lg[q] : L[q]);
       u[q] = wa != 0.0 ? wa * u0.U[(s * 8 + q) * d.cpp + c] + wb * un : wb * un;
     }
-    double e1 = ei + dt * L[5];
+    double e1 = ei + dt * L[8];
     en = wa != 0.0 ? wa * u0.ent[s * d.cpp + c] + wb * e1 : wb * e1;
     return;
   }
   const double gi = 1.0 - 0.70710678118654752440;
-  double uz[5];
-  for (int q = 0; q < 5; ++q) uz[q] = u0.U[(s * 8 + q) * d.cpp + c];
+  double uz[8];
+  for (int q = 0; q < 8; ++q) uz[q] = u0.U[(s * 8 + q) * d.cpp + c];
   const double ez = u0.ent[s * d.cpp + c];
   if (mode == 1) {
     const double s1m = ui[0] * gr, s1e = gr * (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]);
@@ -952,15 +977,16 @@ __device__ __forceinline__ void finish_zone(const CsDims& d, const State& in, co
     u[4] = uz[4] + dt * L[4] + (1.0 - 2.0 * gi) * dt * s1e;
     for (int q = 1; q < 4; ++q) u[q] += gi * dt * u[0] * gr * n[q - 1];
     u[4] += gi * dt * gr * (u[1] * n[0] + u[2] * n[1] + u[3] * n[2]);
-    en = ez + dt * L[5];
+    for (int q = 5; q < 8; ++q) u[q] = uz[q] + dt * L[q];
+    en = ez + dt * L[8];
     return;
   }
   const double s2m = ui[0] * gr, s2e = gr * (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]);
   const double s1m = uz[0] * gr, s1e = gr * ((uz[1] * n[0] + uz[2] * n[1] + uz[3] * n[2]) + gi * dt * uz[0] * gr);
-  for (int q = 0; q < 5; ++q) u[q] = 0.5 * uz[q] + 0.5 * (ui[q] + dt * L[q]);
+  for (int q = 0; q < 8; ++q) u[q] = 0.5 * uz[q] + 0.5 * (ui[q] + dt * L[q]);
   for (int q = 1; q < 4; ++q) u[q] += 0.5 * dt * (2.0 * gi * s1m + (1.0 - gi) * s2m) * n[q - 1];
   u[4] += 0.5 * dt * (2.0 * gi * s1e + (1.0 - gi) * s2e);
-  en = 0.5 * ez + 0.5 * (ei + dt * L[5]);
+  en = 0.5 * ez + 0.5 * (ei + dt * L[8]);
 }
 
 // First, purely implicit stage of IMEX-SSP2(2,2,2): U1 = Un + gi dt S(U1) in
@@ -978,7 +1004,7 @@ __global__ void k_imex_source(CsDims d, Rng rg, Geo geo, State un, State u1, P* 
   const double zn = geo.t(d, T_ZSN, s, i, j);
   double n[3], u[8];
   for (int q = 0; q < 3; ++q) n[q] = geo.t(d, T_ZSX + q, s, i, j) / zn;
-  for (int q = 0; q < 5; ++q) u[q] = un.U[(s * 8 + q) * d.cpp + c];
+  for (int q = 0; q < 8; ++q) u[q] = un.U[(s * 8 + q) * d.cpp + c];
   for (int q = 1; q < 4; ++q) u[q] += gi * dt * u[0] * gr * n[q - 1];
   u[4] += gi * dt * gr * (u[1] * n[0] + u[2] * n[1] + u[3] * n[2]);
   cleanup_zone<P>(d, geo, u1, V, pr, flags, s, i, j, k, u, un.ent[s * d.cpp + c]);
@@ -1000,12 +1026,11 @@ __global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, St
   double vol = geo.t(d, T_OM, s, i, j) * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0;
   double ov = 1.0 / vol;
   const int64_t fxp = d.face(i, j, k), fzm = d.face(i, j, k - 1);
-  const int comps[6] = {0, 1, 2, 3, 4, 8};
   const ZoneFaces zf = zone_faces(d, Fx, Fy, s, i, j, k);
-  double L[6];
-  for (int q = 0; q < 6; ++q) {
-    int64_t o = ((int64_t)s * 9 + comps[q]) * d.fpp;
-    L[q] = -ov * (lateral_div(d, zf, Fx, Fy, comps[q]) + (Fz[o + fxp] - Fz[o + fzm]));
+  double L[9];
+  for (int q = 0; q < 9; ++q) {
+    int64_t o = ((int64_t)s * 9 + q) * d.fpp;
+    L[q] = -ov * (lateral_div(d, zf, Fx, Fy, q) + (Fz[o + fxp] - Fz[o + fzm]));
   }
   double n[3] = {0.0, 0.0, 0.0}, gr = 0.0;
   if (pr.grav != 0.0) {
@@ -1035,10 +1060,9 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
   const int64_t c = d.cell(i, j, k);
   const double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
   const double ov = 1.0 / (geo.t(d, T_OM, s, i, j) * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0);
-  const int comps[6] = {0, 1, 2, 3, 4, 8};
   const ZoneFaces zf = zone_faces(d, Fx, Fy, s, i, j, k);
-  double L[6];
-  for (int q = 0; q < 6; ++q) L[q] = -ov * lateral_div(d, zf, Fx, Fy, comps[q]);
+  double L[9];
+  for (int q = 0; q < 9; ++q) L[q] = -ov * lateral_div(d, zf, Fx, Fy, q);
   // the zone's primitives and their mirror image about the radial face
   // normal: rho, p and the entropy are unchanged, v and B lose twice their
   // normal part (the primitives of the mirrored ghost zone of k_radial_bc)
@@ -1077,7 +1101,7 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
   solve_face<R>(okp ? Wp : Vc, okm ? Wm : Vm, A, Bv, Cv, in.Bz[so + d.face(i, j, k)], pr, fo, sl, sr);
   solve_face<R>(okm ? Wm : Vm, okp ? Wp : Vc, A, Bv, Cv, in.Bz[so + d.face(i, j, k - 1)], pr, fi, sl, sr);
   const double a1 = r1 * r1 * zn, a0 = r0 * r0 * zn;
-  for (int q = 0; q < 6; ++q) L[q] -= ov * (fo[comps[q]] * a1 - fi[comps[q]] * a0);
+  for (int q = 0; q < 9; ++q) L[q] -= ov * (fo[q] * a1 - fi[q] * a0);
   const double rcz = 0.5 * (r0 + r1), gr = pr.grav != 0.0 ? -pr.grav / (rcz * rcz) : 0.0;
   double u[8], en;
   finish_zone(d, in, u0, s, c, dt, wa, wb, mode, gr, A, L, u, en);
@@ -1701,7 +1725,7 @@ extern "C" int cs_create(const cs_config* cfg, cs_handle** out) {
   h->d = cs_make_dims(cfg->ntp, cfg->nr, cfg->nghost);
   h->pr = Params{cfg->gamma, cfg->gamma - 1.0, cfg->rhofloor, cfg->prsfloor, cfg->cfl, cfg->grav,
                  1.0 / (cfg->gamma - 1.0), cfg->recon, cfg->riemann, cfg->emf, cfg->radial_bc == CS_RBC_WALL ? 1 : 0,
-                 cfg->imex};
+                 cfg->imex, (cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1) ? 1 : 0};
   h->surf = (cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1) ? 1 : 0;
   if ((rc = load_corner_table(h->d, T))) {
     delete h;

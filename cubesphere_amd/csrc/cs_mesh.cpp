@@ -12,6 +12,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "../../include/cubesphere_hip.h"
@@ -282,6 +283,8 @@ static void build_tables(const CsDims& d, const double* coords, double* T) {
           ch = l < ch ? l : ch;
         }
         put(T_CHMIN, i, j, ch);
+        // surface mode: the inverse of the face-family mean normals, so each
+        // family's face average is reproduced along its mean normal
         double M[3][3];
         for (int c = 0; c < 3; ++c) {
           M[0][c] = 0.5 * (get(T_XNX + c, i - 1, j) + get(T_XNX + c, i, j));
@@ -308,6 +311,37 @@ static void build_tables(const CsDims& d, const double* coords, double* T) {
         }
         for (int r = 0; r < 3; ++r)
           for (int c = 0; c < 3; ++c) put(T_M00 + 3 * r + c, i, j, Mi[r][c]);
+        // zone-B least squares of CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:740-800):
+        // rows = unit normals of the x+, x-, y+, y- faces and the radial unit
+        // vector (for both z faces); A33 = A^T A, inverted by cofactors with
+        // the reference's guard sign(det) / max(1e-32, |det|)
+        double rows[5][3];
+        for (int c = 0; c < 3; ++c) {
+          rows[0][c] = get(T_XNX + c, i, j);
+          rows[1][c] = get(T_XNX + c, i - 1, j);
+          rows[2][c] = get(T_YNX + c, i, j);
+          rows[3][c] = get(T_YNX + c, i, j - 1);
+          rows[4][c] = zsn > 0.0 ? sh[c] : zc[c];
+        }
+        double m33[3][3];
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c)
+            m33[r][c] = rows[0][r] * rows[0][c] + rows[1][r] * rows[1][c] + rows[2][r] * rows[2][c] +
+                      rows[3][r] * rows[3][c] + rows[4][r] * rows[4][c] + rows[4][r] * rows[4][c];
+        double d33 = m33[0][0] * (m33[1][1] * m33[2][2] - m33[2][1] * m33[1][2]) -
+                     m33[0][1] * (m33[1][0] * m33[2][2] - m33[2][0] * m33[1][2]) +
+                     m33[0][2] * (m33[1][0] * m33[2][1] - m33[2][0] * m33[1][1]);
+        double od = (d33 < 0.0 ? -1.0 : 1.0) / std::max(1e-32, fabs(d33));
+        double bi[3][3] = {{(m33[1][1] * m33[2][2] - m33[1][2] * m33[2][1]) * od, (m33[0][2] * m33[2][1] - m33[0][1] * m33[2][2]) * od,
+                            (m33[0][1] * m33[1][2] - m33[1][1] * m33[0][2]) * od},
+                           {(m33[1][2] * m33[2][0] - m33[1][0] * m33[2][2]) * od, (m33[0][0] * m33[2][2] - m33[0][2] * m33[2][0]) * od,
+                            (m33[0][2] * m33[1][0] - m33[0][0] * m33[1][2]) * od},
+                           {(m33[1][0] * m33[2][1] - m33[1][1] * m33[2][0]) * od, (m33[0][1] * m33[2][0] - m33[0][0] * m33[2][1]) * od,
+                            (m33[0][0] * m33[1][1] - m33[0][1] * m33[1][0]) * od}};
+        const int slot[5] = {T_BWXP, T_BWXM, T_BWYP, T_BWYM, T_BWZ};
+        for (int f = 0; f < 5; ++f)
+          for (int r = 0; r < 3; ++r)
+            put(slot[f] + r, i, j, bi[r][0] * rows[f][0] + bi[r][1] * rows[f][1] + bi[r][2] * rows[f][2]);
       }
     // z-edges at vertex (i,j): frame of 460make_edge_fields_p1.f:2700-2760
     for (int j = 1 - g; j <= N + g - 1; ++j)
@@ -558,6 +592,7 @@ extern "C" int cs_init_problem(const cs_config* cfg, int32_t problem, uint64_t s
   CsDims d = cs_make_dims(cfg->ntp, cfg->nr, cfg->nghost);
   const int N = d.N, NR = d.NR, g = d.g;
   const int64_t plane = d.t2 * 6;
+  const bool surface = cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1;
   memset(Uo, 0, sizeof(double) * d.cpp * 6 * 8);
   memset(ento, 0, sizeof(double) * d.cpp * 6);
   memset(Bxf, 0, sizeof(double) * d.fpp * 6);
@@ -632,13 +667,21 @@ extern "C" int cs_init_problem(const cs_config* cfg, int32_t problem, uint64_t s
           double x[3] = {rc2 * uc[0] / un, rc2 * uc[1] / un, rc2 * uc[2] / un};
           uint64_t gid = (uint64_t)s * N * N * NR + (uint64_t)(k - 1) * N * N + (uint64_t)(j - 1) * N + (uint64_t)(i - 1);
           Prim P = prim_at(problem, cfg, x, seed, gid);
-          // zone B from the faces (same operator as the stage cleanup)
-          double b0 = 0.5 * (Bxf[s * d.fpp + d.face(i - 1, j, k)] + Bxf[s * d.fpp + d.face(i, j, k)]);
-          double b1 = 0.5 * (Byf[s * d.fpp + d.face(i, j - 1, k)] + Byf[s * d.fpp + d.face(i, j, k)]);
-          double b2 = 0.5 * (Bzf[s * d.fpp + d.face(i, j, k - 1)] + Bzf[s * d.fpp + d.face(i, j, k)]);
+          // zone B from the faces, with the operator of the stage cleanup
+          const double bxp = Bxf[s * d.fpp + d.face(i, j, k)], bxm = Bxf[s * d.fpp + d.face(i - 1, j, k)];
+          const double byp = Byf[s * d.fpp + d.face(i, j, k)], bym = Byf[s * d.fpp + d.face(i, j - 1, k)];
+          const double bzp = Bzf[s * d.fpp + d.face(i, j, k)], bzm = Bzf[s * d.fpp + d.face(i, j, k - 1)];
           double B[3];
-          for (int rr = 0; rr < 3; ++rr)
-            B[rr] = tab(T_M00 + 3 * rr, i, j) * b0 + tab(T_M00 + 3 * rr + 1, i, j) * b1 + tab(T_M00 + 3 * rr + 2, i, j) * b2;
+          if (surface) {
+            const double b0 = 0.5 * (bxm + bxp), b1 = 0.5 * (bym + byp), b2 = 0.5 * (bzm + bzp);
+            for (int rr = 0; rr < 3; ++rr)
+              B[rr] = tab(T_M00 + 3 * rr, i, j) * b0 + tab(T_M00 + 3 * rr + 1, i, j) * b1 +
+                      tab(T_M00 + 3 * rr + 2, i, j) * b2;
+          } else {
+            for (int rr = 0; rr < 3; ++rr)
+              B[rr] = tab(T_BWXP + rr, i, j) * bxp + tab(T_BWXM + rr, i, j) * bxm + tab(T_BWYP + rr, i, j) * byp +
+                      tab(T_BWYM + rr, i, j) * bym + tab(T_BWZ + rr, i, j) * (bzp + bzm);
+          }
           int64_t cidx = s * d.cpp * 8 + d.cell(i, j, k);
           Uo[cidx + 0 * d.cpp] = P.rho;
           Uo[cidx + 1 * d.cpp] = P.rho * P.vx;

@@ -186,7 +186,7 @@ struct or_grid {
   double* lat;  /* lattice */
   double* rb;
   /* 2-D tables, each t2*6 */
-  double *u[3], *xn[3], *xc[3], *thy, *yn[3], *yc[3], *thx, *zs[3], *zsn, *zt[3], *om, *chmin, *mi[9];
+  double *u[3], *xn[3], *xc[3], *thy, *yn[3], *yc[3], *thx, *zs[3], *zsn, *zt[3], *om, *chmin, *mi[9], *bw[15];
   double *en1[3], *etau[3], *ecos, *esin, *xet[3], *yet[3];
   cmap_t* cmap; int ncmap;
   fmap_t* fmap; int nfmap;
@@ -264,7 +264,7 @@ static void build_geometry(or_grid* G) {
       }
     for (j = 1 - g; j <= N + g; ++j) /* zones */
       for (i = 1 - g; i <= N + g; ++i) {
-        double a[3], b[3], cc[3], e[3], zs[3], sh[3], t1[3], M[3][3], det, ch = 1e300, dd[3];
+        double a[3], b[3], cc[3], e[3], zs[3], sh[3], t1[3], zc3[3], ch = 1e300, dd[3];
         int64_t q = T2(G, s, i, j), qL = T2(G, s, i - 1, j), qD = T2(G, s, i, j - 1);
         const double* pr[4][2];
         int p;
@@ -272,6 +272,8 @@ static void build_geometry(or_grid* G) {
         uv(G, s, i, j - 1, b);
         uv(G, s, i, j, cc);
         uv(G, s, i - 1, j, e);
+        for (c = 0; c < 3; ++c) zc3[c] = a[c] + b[c] + cc[c] + e[c];
+        unit3(zc3);
         /* area vector of the spherical zone on the unit sphere, by closing
            the cone: sum over its four edges of (theta/2) * unit normal */
         for (c = 0; c < 3; ++c)
@@ -291,21 +293,57 @@ static void build_geometry(or_grid* G) {
           if (l < ch) ch = l;
         }
         G->chmin[q] = ch;
+        /* surface mode: inverse of the face-family mean normals */
+        {
+          double M[3][3], det;
         for (c = 0; c < 3; ++c) {
-          M[0][c] = 0.5 * (G->xn[c][qL] + G->xn[c][q]);
-          M[1][c] = 0.5 * (G->yn[c][qD] + G->yn[c][q]);
-          M[2][c] = sh[c];
+            M[0][c] = 0.5 * (G->xn[c][qL] + G->xn[c][q]);
+            M[1][c] = 0.5 * (G->yn[c][qD] + G->yn[c][q]);
+            M[2][c] = sh[c];
+          }
+          det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+                M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
+          if (fabs(det) > 1e-300) {
+            /* inverse by cofactors */
+            int r, k2;
+            for (r = 0; r < 3; ++r)
+              for (k2 = 0; k2 < 3; ++k2) {
+                int r1 = (k2 + 1) % 3, r2 = (k2 + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
+                G->mi[3 * r + k2][q] = (M[r1][c1] * M[r2][c2] - M[r1][c2] * M[r2][c1]) / det;
+              }
+          }
         }
-        det = M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
-              M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0]);
-        if (fabs(det) > 1e-300) {
-          /* inverse by cofactors */
-          int r, k2;
+        { /* zone-B least squares, 485cleanup_after_update.f:740-800: rows are the
+             unit normals of the x+, x-, y+, y- faces and the radial unit vector
+             (both z faces); W_f = (A^T A)^-1 n_f */
+          double rows[5][3], m3[3][3], bi[3][3], det, od;
+          int r, f;
+          for (c = 0; c < 3; ++c) {
+            rows[0][c] = G->xn[c][q];
+            rows[1][c] = G->xn[c][qL];
+            rows[2][c] = G->yn[c][q];
+            rows[3][c] = G->yn[c][qD];
+            rows[4][c] = G->zsn[q] > 0.0 ? sh[c] : zc3[c];
+          }
           for (r = 0; r < 3; ++r)
-            for (k2 = 0; k2 < 3; ++k2) {
-              int r1 = (k2 + 1) % 3, r2 = (k2 + 2) % 3, c1 = (r + 1) % 3, c2 = (r + 2) % 3;
-              G->mi[3 * r + k2][q] = (M[r1][c1] * M[r2][c2] - M[r1][c2] * M[r2][c1]) / det;
-            }
+            for (c = 0; c < 3; ++c)
+              m3[r][c] = rows[0][r] * rows[0][c] + rows[1][r] * rows[1][c] + rows[2][r] * rows[2][c] +
+                         rows[3][r] * rows[3][c] + rows[4][r] * rows[4][c] + rows[4][r] * rows[4][c];
+          det = m3[0][0] * (m3[1][1] * m3[2][2] - m3[2][1] * m3[1][2]) - m3[0][1] * (m3[1][0] * m3[2][2] - m3[2][0] * m3[1][2]) +
+                m3[0][2] * (m3[1][0] * m3[2][1] - m3[2][0] * m3[1][1]);
+          od = (det < 0.0 ? -1.0 : 1.0) / fmax(1e-32, fabs(det));
+          bi[0][0] = (m3[1][1] * m3[2][2] - m3[1][2] * m3[2][1]) * od;
+          bi[0][1] = (m3[0][2] * m3[2][1] - m3[0][1] * m3[2][2]) * od;
+          bi[0][2] = (m3[0][1] * m3[1][2] - m3[1][1] * m3[0][2]) * od;
+          bi[1][0] = (m3[1][2] * m3[2][0] - m3[1][0] * m3[2][2]) * od;
+          bi[1][1] = (m3[0][0] * m3[2][2] - m3[0][2] * m3[2][0]) * od;
+          bi[1][2] = (m3[0][2] * m3[1][0] - m3[0][0] * m3[1][2]) * od;
+          bi[2][0] = (m3[1][0] * m3[2][1] - m3[1][1] * m3[2][0]) * od;
+          bi[2][1] = (m3[0][1] * m3[2][0] - m3[0][0] * m3[2][1]) * od;
+          bi[2][2] = (m3[0][0] * m3[1][1] - m3[0][1] * m3[1][0]) * od;
+          for (f = 0; f < 5; ++f)
+            for (r = 0; r < 3; ++r)
+              G->bw[3 * f + r][q] = bi[r][0] * rows[f][0] + bi[r][1] * rows[f][1] + bi[r][2] * rows[f][2];
         }
       }
     for (j = 1 - g; j <= N + g - 1; ++j) /* z-edge frames, 460make_edge_fields_p1.f:2700-2760 */
@@ -649,6 +687,7 @@ or_grid* or_grid_new(const or_config* c) {
   G->thy = al(n); G->thx = al(n); G->zsn = al(n); G->om = al(n); G->chmin = al(n);
   G->ecos = al(n); G->esin = al(n);
   for (q = 0; q < 9; ++q) G->mi[q] = al(n);
+  for (q = 0; q < 15; ++q) G->bw[q] = al(n);
   build_geometry(G);
   build_maps(G);
   build_edge_maps(G);
@@ -665,6 +704,7 @@ void or_grid_free(or_grid* G) {
   }
   free(G->thy); free(G->thx); free(G->zsn); free(G->om); free(G->chmin); free(G->ecos); free(G->esin);
   for (q = 0; q < 9; ++q) free(G->mi[q]);
+  for (q = 0; q < 15; ++q) free(G->bw[q]);
   free(G->cmap); free(G->fmap); free(G->pairs); free(G->epairs);
   free(G);
 }
@@ -1287,27 +1327,48 @@ static void weights(int order, int irk, double* wa, double* wb) {
 
 /* UPDATE_IMEX_RK_STEP_P1 (484:483-497, 547-593, CT 646-908) and
    CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:4) */
-/* CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:4): zone B from the six
-   faces through the zone-B operator, then CONS_TO_PRIM with its floors
-   (662pde_3d_cons_to_prim.f:69-151); U and the entropy are stored back. */
+/* CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:4): zone B replaced by the
+   least-squares fit to the six face values over their unit normals
+   (:692-860), the total energy corrected by the change of magnetic energy
+   (:845-851), then CONS_TO_PRIM with its floors (662pde_3d_cons_to_prim.f:69-151);
+   U and the entropy are stored back. */
 static void cleanup(const or_grid* G, double* const* out) {
-  int N = G->N, NR = G->NR, s, i, j, k, q;
+  int N = G->N, NR = G->NR, s, i, j, k, q, surf = G->c.radial_bc == 1 && G->NR == 1;
   for (s = 0; s < 6; ++s)
     for (k = 1; k <= NR; ++k)
       for (j = 1; j <= N; ++j)
         for (i = 1; i <= N; ++i) {
           int64_t c = CELL(G, i, j, k), t = T2(G, s, i, j);
-          double b[3], u[8], v[8], e;
-          b[0] = 0.5 * (out[2][s * G->fpp + FACE(G, i - 1, j, k)] + out[2][s * G->fpp + FACE(G, i, j, k)]);
-          b[1] = 0.5 * (out[3][s * G->fpp + FACE(G, i, j - 1, k)] + out[3][s * G->fpp + FACE(G, i, j, k)]);
-          b[2] = 0.5 * (out[4][s * G->fpp + FACE(G, i, j, k - 1)] + out[4][s * G->fpp + FACE(G, i, j, k)]);
+          double b[6], bn[3], u[8], v[8], e;
+          b[0] = out[2][s * G->fpp + FACE(G, i, j, k)];
+          b[1] = out[2][s * G->fpp + FACE(G, i - 1, j, k)];
+          b[2] = out[3][s * G->fpp + FACE(G, i, j, k)];
+          b[3] = out[3][s * G->fpp + FACE(G, i, j - 1, k)];
+          b[4] = out[4][s * G->fpp + FACE(G, i, j, k)];
+          b[5] = out[4][s * G->fpp + FACE(G, i, j, k - 1)];
           for (q = 0; q < 8; ++q) u[q] = UU(out[0], G, s, q, c);
-          for (q = 0; q < 3; ++q) u[5 + q] = G->mi[3 * q][t] * b[0] + G->mi[3 * q + 1][t] * b[1] + G->mi[3 * q + 2][t] * b[2];
+          if (surf) { /* face-family averages along their mean normals; energy kept */
+            double m0 = 0.5 * (b[1] + b[0]), m1 = 0.5 * (b[3] + b[2]), m2 = 0.5 * (b[5] + b[4]);
+            for (q = 0; q < 3; ++q) u[5 + q] = G->mi[3 * q][t] * m0 + G->mi[3 * q + 1][t] * m1 + G->mi[3 * q + 2][t] * m2;
+          } else {
+            for (q = 0; q < 3; ++q)
+              bn[q] = G->bw[q][t] * b[0] + G->bw[3 + q][t] * b[1] + G->bw[6 + q][t] * b[2] + G->bw[9 + q][t] * b[3] +
+                      G->bw[12 + q][t] * (b[4] + b[5]);
+            /* the total energy takes the change of magnetic energy (:845-851) */
+            u[4] = u[4] - (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) * OB8PI +
+                   (bn[0] * bn[0] + bn[1] * bn[1] + bn[2] * bn[2]) * OB8PI;
+            for (q = 0; q < 3; ++q) u[5 + q] = bn[q];
+          }
           e = out[1][s * G->cpp + c];
           or_cons_to_prim(G->c.gamma, G->c.rhofloor, G->c.prsfloor, &e, u, v);
           for (q = 0; q < 8; ++q) UU(out[0], G, s, q, c) = u[q];
           out[1][s * G->cpp + c] = e;
         }
+}
+
+void or_cleanup(const or_grid* G, double* U, double* ent, double* Bx, double* By, double* Bz) {
+  double* st[5] = {U, ent, Bx, By, Bz};
+  cleanup(G, st);
 }
 
 /* Radial gravity at zone (s, i, j, k): gr = -grav / rc^2 along the unit
@@ -1358,17 +1419,16 @@ void or_update(const or_grid* G, int irk, double dt, const double* const* in, co
                double* const* out, double** F, double** E) {
   int N = G->N, NR = G->NR, s, i, j, k, q;
   double wa, wb;
-  static const int comps[6] = {0, 1, 2, 3, 4, 8};
   weights(G->c.rk_order, irk, &wa, &wb);
   for (s = 0; s < 6; ++s)
     for (k = 1; k <= NR; ++k)
       for (j = 1; j <= N; ++j)
         for (i = 1; i <= N; ++i) {
           int64_t c = CELL(G, i, j, k);
-          double ov = 1.0 / vol_of(G, s, i, j, k), L[6];
+          double ov = 1.0 / vol_of(G, s, i, j, k), L[9]; /* rho, rho v, E, B (advanced as the reference's dU_dt), entropy */
           int64_t fp = FACE(G, i, j, k), fxm = FACE(G, i - 1, j, k), fym = FACE(G, i, j - 1, k), fzm = FACE(G, i, j, k - 1);
-          for (q = 0; q < 6; ++q) {
-            int64_t o = ((int64_t)s * 9 + comps[q]) * G->fpp;
+          for (q = 0; q < 9; ++q) {
+            int64_t o = ((int64_t)s * 9 + q) * G->fpp;
             L[q] = -ov * ((F[0][o + fp] - F[0][o + fxm]) + (F[1][o + fp] - F[1][o + fym]) + (F[2][o + fp] - F[2][o + fzm]));
           }
           if (!G->c.imex) {
@@ -1379,12 +1439,12 @@ void or_update(const or_grid* G, int irk, double dt, const double* const* in, co
               for (q = 0; q < 3; ++q) L[1 + q] += rho * gr * n[q];
               L[4] += mr * gr;
             }
-            for (q = 0; q < 5; ++q) {
+            for (q = 0; q < 8; ++q) {
               double un = UU(in[0], G, s, q, c) + dt * L[q];
               UU(out[0], G, s, q, c) = wa != 0.0 ? wa * UU(u0[0], G, s, q, c) + wb * un : wb * un;
             }
             {
-              double en = in[1][s * G->cpp + c] + dt * L[5];
+              double en = in[1][s * G->cpp + c] + dt * L[8];
               out[1][s * G->cpp + c] = wa != 0.0 ? wa * u0[1][s * G->cpp + c] + wb * en : wb * en;
             }
           } else {
@@ -1392,9 +1452,12 @@ void or_update(const or_grid* G, int irk, double dt, const double* const* in, co
             double n[3], gr = gravity_at(G, s, i, j, k, n), ui[5], uz[5], u[5];
             for (q = 0; q < 5; ++q) { ui[q] = UU(in[0], G, s, q, c); uz[q] = UU(u0[0], G, s, q, c); }
             or_imex_combine(irk, dt, gr, n, ui, uz, L, u);
-            if (irk == 1) out[1][s * G->cpp + c] = u0[1][s * G->cpp + c] + dt * L[5];
-            else out[1][s * G->cpp + c] = 0.5 * u0[1][s * G->cpp + c] + 0.5 * (in[1][s * G->cpp + c] + dt * L[5]);
+            if (irk == 1) out[1][s * G->cpp + c] = u0[1][s * G->cpp + c] + dt * L[8];
+            else out[1][s * G->cpp + c] = 0.5 * u0[1][s * G->cpp + c] + 0.5 * (in[1][s * G->cpp + c] + dt * L[8]);
             for (q = 0; q < 5; ++q) UU(out[0], G, s, q, c) = u[q];
+            for (q = 5; q < 8; ++q)
+              UU(out[0], G, s, q, c) = irk == 1 ? UU(u0[0], G, s, q, c) + dt * L[q]
+                                                : 0.5 * UU(u0[0], G, s, q, c) + 0.5 * (UU(in[0], G, s, q, c) + dt * L[q]);
           }
         }
   for (s = 0; s < 6; ++s) {

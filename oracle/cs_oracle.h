@@ -66,6 +66,8 @@ double or_weno5(double v0, double v1, double v2, double v3, double v4);
 double or_emf4_hll(const double* s4, double cs_t, double sn_t, const double* ez4,
                    const double* bn2, const double* by2);
 double or_emf3_llf(double S, const double* ez3, const double* bn3);
+/* CLEANUP_AFTER_UPDATE on the owned zones of a state */
+void or_cleanup(const or_grid* G, double* U, double* ent, double* Bx, double* By, double* Bz);
 /* IMEX-SSP2(2,2,2) with the implicit gravity source, u = (rho, rho v, E) */
 void or_imex_pre(double dt, double gr, const double* n, double* u);
 void or_imex_combine(int stage, double dt, double gr, const double* n, const double* ui, const double* uz,

@@ -57,6 +57,7 @@ def olib():
         L.or_emf3_llf.argtypes = [dbl, vp, vp]
         L.or_emf3_llf.restype = dbl
         L.or_imex_pre.argtypes = [dbl, dbl, vp, vp]
+        L.or_cleanup.argtypes = [vp] + [vp] * 5
         L.or_imex_combine.argtypes = [i32, dbl, dbl, vp, vp, vp, vp, vp]
         _o = L
     return _o
@@ -130,6 +131,9 @@ class Grid:
         b, _k2 = ptrs(st_u0.arrays())
         c, _k3 = ptrs(st_out.arrays())
         olib().or_stage(self.g, irk, dt, a, b, c)
+
+    def cleanup(self, st):
+        olib().or_cleanup(self.g, *[P(a) for a in st.arrays()])
 
     def metric(self):
         vol = np.zeros(6 * self.cpp)

@@ -400,3 +400,56 @@ def test_imex_is_second_order_in_time():
         res[m] = st.interior()[0].copy()
     e = [np.abs(res[m] - res[16]).max() for m in (1, 2, 4)]
     assert e[0] / e[1] > 3.5 and e[1] / e[2] > 3.3, e
+
+
+# ------------------------------------------------------------------ cleanup
+@pytest.mark.parametrize("nr", [1, 3])
+def test_cleanup_reproduces_a_uniform_field(nr):
+    """CLEANUP_AFTER_UPDATE: for a uniform field B0 every face holds its
+    normal flux density n_f . B0, so the least-squares zone B (shells) and the
+    surface-mode operator both return B0; in a shell the pressure is kept
+    (the energy takes the magnetic-energy change), on the surface the total
+    energy is kept."""
+    cfg = cs.make_config(8, nr, 3)
+    G = orc.Grid(cfg)
+    st = cs.init_problem(cfg, 0, seed=1)
+    g, N, NR = cfg.nghost, cfg.ntp, cfg.nr
+    nf = N + 2 * g + 1
+    B0 = np.array([0.3, -0.7, 0.45])
+    # face normal flux densities of B0 from the oracle's own face normals: a
+    # uniform field's flux through a planar x/y face is B0 . n A, through the
+    # spherical z face B0 . (area vector); the metric gives A, so use the
+    # lattice for the normals
+    lat = cs.unit_lattice(cfg).reshape(6, 3, nf, nf)
+    U, ent, Bx, By, Bz = st.views()
+    U[:, 5:8] = 0.0
+    for s in range(6):
+        for j in range(-g, N + g + 1):
+            for i in range(-g, N + g + 1):
+                if j > -g:   # x-face (vertex i, cell j): normal (b - a) x mid
+                    a, b = lat[s, :, j - 1 + g, i + g], lat[s, :, j + g, i + g]
+                    n = np.cross(b - a, a + b)
+                    Bx[s, :, j + g, i + g] = B0 @ (n / np.linalg.norm(n))
+                if i > -g:   # y-face (cell i, vertex j): normal mid x (b - a)
+                    a, b = lat[s, :, j + g, i - 1 + g], lat[s, :, j + g, i + g]
+                    n = np.cross(a + b, b - a)
+                    By[s, :, j + g, i + g] = B0 @ (n / np.linalg.norm(n))
+                if i > -g and j > -g:  # z-face: unit area vector of the spherical zone
+                    c4 = [lat[s, :, jj + g, ii + g] for ii, jj in ((i - 1, j - 1), (i, j - 1), (i, j), (i - 1, j))]
+                    zs = np.zeros(3)
+                    for q in range(4):   # sum of theta/2 * unit edge-plane normals closes the cone
+                        p0, p1 = c4[q], c4[(q + 1) % 4]
+                        nn = np.cross(p0, p1)
+                        zs += 0.5 * math.atan2(np.linalg.norm(nn), p0 @ p1) * nn / np.linalg.norm(nn)
+                    Bz[s, :, j + g, i + g] = B0 @ (zs / np.linalg.norm(zs))
+    E_before = U[:, 4].copy()
+    G.cleanup(st)
+    Ui, _ = st.interior()
+    for c in range(3):
+        np.testing.assert_allclose(Ui[:, 5 + c], B0[c], rtol=0, atol=1e-12)
+    Bm2 = (B0 @ B0) / (8 * math.pi)
+    Ei = E_before[:, g:g + NR, g:g + N, g:g + N]
+    if nr > 1:
+        np.testing.assert_allclose(Ui[:, 4], Ei + Bm2, rtol=1e-12)     # p kept, E gains |B0|^2/8pi
+    else:
+        np.testing.assert_allclose(Ui[:, 4], Ei, rtol=1e-14)           # E kept
