@@ -1062,7 +1062,11 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
   const double ov = 1.0 / (geo.t(d, T_OM, s, i, j) * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0);
   const ZoneFaces zf = zone_faces(d, Fx, Fy, s, i, j, k);
   double L[9];
-  for (int q = 0; q < 9; ++q) L[q] = -ov * lateral_div(d, zf, Fx, Fy, q);
+  // the surface cleanup rebuilds the zone B from the faces and keeps the
+  // total energy, so the induction components (5-7) are not needed here
+  L[5] = L[6] = L[7] = 0.0;
+  for (int q = 0; q < 9; ++q)
+    if (q < 5 || q == 8) L[q] = -ov * lateral_div(d, zf, Fx, Fy, q);
   // the zone's primitives and their mirror image about the radial face
   // normal: rho, p and the entropy are unchanged, v and B lose twice their
   // normal part (the primitives of the mirrored ghost zone of k_radial_bc)
@@ -1101,7 +1105,8 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
   solve_face<R>(okp ? Wp : Vc, okm ? Wm : Vm, A, Bv, Cv, in.Bz[so + d.face(i, j, k)], pr, fo, sl, sr);
   solve_face<R>(okm ? Wm : Vm, okp ? Wp : Vc, A, Bv, Cv, in.Bz[so + d.face(i, j, k - 1)], pr, fi, sl, sr);
   const double a1 = r1 * r1 * zn, a0 = r0 * r0 * zn;
-  for (int q = 0; q < 9; ++q) L[q] -= ov * (fo[q] * a1 - fi[q] * a0);
+  for (int q = 0; q < 9; ++q)
+    if (q < 5 || q == 8) L[q] -= ov * (fo[q] * a1 - fi[q] * a0);
   const double rcz = 0.5 * (r0 + r1), gr = pr.grav != 0.0 ? -pr.grav / (rcz * rcz) : 0.0;
   double u[8], en;
   finish_zone(d, in, u0, s, c, dt, wa, wb, mode, gr, A, L, u, en);
