@@ -183,6 +183,76 @@ def cpu_baseline(cs, name, max_seconds):
                       f"single thread, {el:.1f} s"}
 
 
+def c4_workload(cs, prec, seed=20261202):
+    """48 independent shells: panel edge N from {32, 48, 64, 96, 128, 192},
+    radial zones from {16, 32, 64}, drawn with a seeded generator; blast and
+    hydrostatic (with 1/r^2 gravity) initial states alternate."""
+    rng = np.random.default_rng(seed)
+    cfgs, probs = [], []
+    for b in range(48):
+        N = int(rng.choice([32, 48, 64, 96, 128, 192]))
+        nr = int(rng.choice([16, 32, 64]))
+        prob = 2 if b % 2 == 0 else 3
+        cfgs.append(cs.make_config(N, nr, 3, cs.RECON_WENO5, cs.RS_HLLD, cs.EMF_LLF, prec, 3, cs.RBC_PAD,
+                                   cs.RMESH_UNIFORM, 1.0, 1.5, grav=1.0 if prob == 3 else 0.0))
+        probs.append(prob)
+    return cfgs, probs
+
+
+def run_batch(cs, prec, steps, warmup, states=None):
+    """The 48-shell batch: autotuned workgroup sizes, graph-replayed steps;
+    then every shell on its own (its own per-shell graph), for comparison."""
+    from concurrent.futures import ThreadPoolExecutor
+    cfgs, probs = c4_workload(cs, prec)
+    if states is None:
+        with ThreadPoolExecutor(16) as ex:
+            states = list(ex.map(lambda a: cs.init_problem(a[0], a[1], seed=20261202 + a[2]),
+                                 [(c, p, i) for i, (c, p) in enumerate(zip(cfgs, probs))]))
+    t0 = time.perf_counter()
+    B = cs.Batch(cfgs)
+    for sh, st in zip(B.shells, states):
+        sh.upload(st)
+        sh.dt = 0.0
+    B.evaluate_timestep()
+    tuned = B.autotune(2)
+    B.advance(warmup, use_graph=True)
+    ms = B.time_steps(steps)
+    ms_single = sum(sh.time_steps(steps, use_graph=True) for sh in B.shells)
+    divb = max(sh.max_divb() for sh in B.shells)
+    B.close()
+    cells = sum(6 * c.ntp * c.ntp * c.nr for c in cfgs)
+    log(f"batch {'fp64' if prec == 0 else 'mixed'}: {len(cfgs)} shells, {cells} zones, {ms / steps:.3f} ms/step "
+        f"batched, {ms_single / steps:.3f} ms/step shell by shell; workgroups {tuned}; max rel divB {divb:.2e} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    return dict(cells=cells, ms=ms, ms_single=ms_single, tuned=tuned, divb=divb, states=states)
+
+
+def bench_batch(cs, args):
+    res = {}
+    states = None
+    for prec in (cs.FP64, cs.MIXED):
+        r = run_batch(cs, prec, args.steps, args.warmup, states)
+        states = r.pop("states")
+        res[prec] = r
+    r = res[cs.FP64]
+    cups = r["cells"] * args.steps / (r["ms"] * 1e-3)
+    line = {
+        "metric": METRIC, "value": cups, "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms"] / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "C4: 48 shells, N in {32..192}, nr in {16, 32, 64}, WENO5 + HLLD + CT, SSP-RK3, "
+                               "blast / hydrostatic alternating; one launch per phase over all shells, one graph "
+                               "per step, autotuned workgroups", "zones": r["cells"]},
+        "roofline": None, "cpu_baseline": None,
+        "batched_vs_shell_by_shell": r["ms_single"] / r["ms"],
+        "mixed": {"cell_updates_per_s": res[cs.MIXED]["cells"] * args.steps / (res[cs.MIXED]["ms"] * 1e-3),
+                  "speedup_vs_fp64": r["ms"] / res[cs.MIXED]["ms"]},
+        "autotuned_workgroups": {"fp64": r["tuned"], "mixed": res[cs.MIXED]["tuned"]},
+        "max_rel_divb": max(r["divb"], res[cs.MIXED]["divb"]),
+    }
+    print(json.dumps(line))
+
+
 def probe(cs, name, prec, kernel, iters):
     """Hardware-counter run: `iters` launches of one kernel on the workload's
     state, then one plain device copy of 1 GiB (read) + 1 GiB (write), whose
@@ -236,6 +306,9 @@ def main():
     if args.probe:
         probe(cs, args.workload, cs.FP64 if args.precision == "fp64" else cs.MIXED, args.probe,
               args.probe_iters)
+        return
+    if args.workload == "C4":
+        bench_batch(cs, args)
         return
     res = {}
     for prec in (cs.FP64, cs.MIXED):

@@ -117,6 +117,13 @@ def lib():
             "cs_time_steps": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
             "cs_time_kernel": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
             "cs_profile_steps": [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32)],
+            "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
+            "cs_batch_destroy": [vp],
+            "cs_batch_shell": [vp, i32, ctypes.POINTER(vp)],
+            "cs_batch_evaluate_timestep": [vp],
+            "cs_batch_advance": [vp, i32, i32],
+            "cs_batch_time_steps": [vp, i32, ctypes.POINTER(ctypes.c_float)],
+            "cs_batch_autotune": [vp, i32, ctypes.POINTER(i32)],
             "cs_kernel_units": [vp, i32, ctypes.POINTER(i64)],
             "cs_last_error": [ctypes.c_char_p, ctypes.c_size_t],
             "cs_device_info": [ctypes.c_char_p, ctypes.c_size_t],
@@ -221,17 +228,21 @@ class CubeSphere:
       evaluate_timestep()    EVALUATE_TIMESTEP (490)
     """
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, _borrowed=None):
         self.cfg = cfg
-        h = ctypes.c_void_p()
-        _check(lib().cs_create(ctypes.byref(cfg), ctypes.byref(h)), "cs_create")
+        self.owner = _borrowed is None
+        if _borrowed is None:
+            h = ctypes.c_void_p()
+            _check(lib().cs_create(ctypes.byref(cfg), ctypes.byref(h)), "cs_create")
+        else:
+            h = _borrowed
         self.h = h
         self.sizes = sizes(cfg)
 
     def close(self):
-        if self.h:
+        if self.h and self.owner:
             lib().cs_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:
@@ -323,6 +334,58 @@ class CubeSphere:
         n = ctypes.c_int64()
         _check(lib().cs_kernel_units(self.h, kernel, ctypes.byref(n)), "cs_kernel_units")
         return n.value
+
+
+BATCH_PHASES = ["exchange", "radial_pad", "flux_xy", "flux_z", "emf", "update_faces", "update_cells",
+                "implicit_source", "cfl"]
+
+
+class Batch:
+    """Independent shells advanced together: every phase of a stage is one
+    launch over all shells and a step is one hipGraph.  The shells share
+    the precision and the time integrator."""
+
+    def __init__(self, cfgs):
+        self.cfgs = list(cfgs)
+        arr = (Config * len(self.cfgs))(*self.cfgs)
+        b = ctypes.c_void_p()
+        _check(lib().cs_batch_create(arr, len(self.cfgs), ctypes.byref(b)), "cs_batch_create")
+        self.b = b
+        self.shells = []
+        for i, c in enumerate(self.cfgs):
+            h = ctypes.c_void_p()
+            _check(lib().cs_batch_shell(self.b, i, ctypes.byref(h)), "cs_batch_shell")
+            self.shells.append(CubeSphere(c, _borrowed=h))
+
+    def close(self):
+        if self.b:
+            for s in self.shells:
+                s.close()
+            lib().cs_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def evaluate_timestep(self):
+        _check(lib().cs_batch_evaluate_timestep(self.b), "cs_batch_evaluate_timestep")
+
+    def advance(self, nsteps, use_graph=True):
+        _check(lib().cs_batch_advance(self.b, nsteps, 1 if use_graph else 0), "cs_batch_advance")
+
+    def time_steps(self, nsteps):
+        ms = ctypes.c_float()
+        _check(lib().cs_batch_time_steps(self.b, nsteps, ctypes.byref(ms)), "cs_batch_time_steps")
+        return ms.value
+
+    def autotune(self, iters=3):
+        """Pick each phase's workgroup size by timing; returns {phase: size}."""
+        bs = (ctypes.c_int32 * len(BATCH_PHASES))()
+        _check(lib().cs_batch_autotune(self.b, iters, bs), "cs_batch_autotune")
+        return dict(zip(BATCH_PHASES, list(bs)))
 
 
 def device_info():

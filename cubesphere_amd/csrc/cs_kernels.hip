@@ -271,11 +271,11 @@ __global__ void k_exchange(CsDims d, State st, P* V) {
 // Radial ghost layers: zero-gradient pad (826pad_boundary_r.f:201-214 inner
 // form, used at both ends) or a reflecting, perfectly conducting wall.
 template <typename P>
-__global__ void k_radial_bc(CsDims d, Geo geo, State st, int wall, P* V, Params pr) {
+__device__ __forceinline__ void radial_bc(const CsDims& d, const Geo& geo, const State& st, int wall, P* V,
+                                          const Params& pr, uint32_t t) {
   // no FMA contraction: the wall mirror rounds exactly as written, so the whole
   // halo (exchange + boundary) is reproducible bit for bit on any host
 #pragma clang fp contract(off)
-  uint32_t t = gtid();
   const int N = d.N, NR = d.NR, g = d.g;
   const int nl = N + 2 * g + 1;  // lateral index range [-g, N+g]
   uint32_t per = (uint32_t)nl * nl;
@@ -338,6 +338,10 @@ __global__ void k_radial_bc(CsDims d, Geo geo, State st, int wall, P* V, Params 
     for (int c = 0; c < 8; ++c) V[((int64_t)s * 9 + c) * d.cpp + dc] = (P)v[c];
     V[((int64_t)s * 9 + 8) * d.cpp + dc] = (P)ent;
   }
+}
+template <typename P>
+__global__ void k_radial_bc(CsDims d, Geo geo, State st, int wall, P* V, Params pr) {
+  radial_bc<P>(d, geo, st, wall, V, pr, gtid());
 }
 
 // ------------------------------------------------------------------ primitives
@@ -994,11 +998,10 @@ __device__ __forceinline__ void finish_zone(const CsDims& d, const State& in, co
 // whose faces hold a copy of Un's; the cleanup then gives U1 its zone B and
 // primitives, like any stage output.
 template <typename P>
-__global__ void k_imex_source(CsDims d, Rng rg, Geo geo, State un, State u1, P* V, const double* dtp, Params pr,
-                              int* flags) {
-  int i, j, k, s;
-  if (!decode(rg, gtid(), i, j, k, s)) return;
-  const double dt = dtp[0], gi = 1.0 - 0.70710678118654752440;
+__device__ __forceinline__ void imex_source(const CsDims& d, int i, int j, int k, int s, const Geo& geo,
+                                            const State& un, const State& u1, P* V, double dt, const Params& pr,
+                                            int* flags) {
+  const double gi = 1.0 - 0.70710678118654752440;
   const int64_t c = d.cell(i, j, k);
   const double r1 = geo.r(d, k), r0 = geo.r(d, k - 1), rc = 0.5 * (r0 + r1), gr = -pr.grav / (rc * rc);
   const double zn = geo.t(d, T_ZSN, s, i, j);
@@ -1009,18 +1012,23 @@ __global__ void k_imex_source(CsDims d, Rng rg, Geo geo, State un, State u1, P* 
   u[4] += gi * dt * gr * (u[1] * n[0] + u[2] * n[1] + u[3] * n[2]);
   cleanup_zone<P>(d, geo, u1, V, pr, flags, s, i, j, k, u, un.ent[s * d.cpp + c]);
 }
+template <typename P>
+__global__ void k_imex_source(CsDims d, Rng rg, Geo geo, State un, State u1, P* V, const double* dtp, Params pr,
+                              int* flags) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  imex_source<P>(d, i, j, k, s, geo, un, u1, V, dtp[0], pr, flags);
+}
 
 // U_out = a U0 + b (U_in + dt L(U_in)) for rho, rho v, E and entropy
 // (UPDATE_IMEX_RK_STEP_P1 484update_imex_rk_step_p1.f:483-497, 547-593 with the
 // Shu-Osher stage weights; the RK2 weights reproduce DRV:2661-2773).
 template <typename P>
-__global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
-                               const double* __restrict__ Fx, const double* __restrict__ Fy,
-                               const double* __restrict__ Fz, const double* dtp, double wa,
-                               double wb, Params pr, P* Vout, int* flags, int mode) {
-  int i, j, k, s;
-  if (!decode(rg, gtid(), i, j, k, s)) return;
-  const double dt = dtp[0];
+__device__ __forceinline__ void update_cells(const CsDims& d, int i, int j, int k, int s, const Geo& geo,
+                                             const State& in, const State& u0, const State& out,
+                                             const double* __restrict__ Fx, const double* __restrict__ Fy,
+                                             const double* __restrict__ Fz, double dt, double wa, double wb,
+                                             const Params& pr, P* Vout, int* flags, int mode) {
   int64_t c = d.cell(i, j, k);
   double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
   double vol = geo.t(d, T_OM, s, i, j) * (r1 * r1 * r1 - r0 * r0 * r0) / 3.0;
@@ -1041,6 +1049,15 @@ __global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, St
   double u[8], en;
   finish_zone(d, in, u0, s, c, dt, wa, wb, mode, gr, n, L, u, en);
   cleanup_zone<P>(d, geo, out, Vout, pr, flags, s, i, j, k, u, en);
+}
+template <typename P>
+__global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
+                               const double* __restrict__ Fx, const double* __restrict__ Fy,
+                               const double* __restrict__ Fz, const double* dtp, double wa,
+                               double wb, Params pr, P* Vout, int* flags, int mode) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  update_cells<P>(d, i, j, k, s, geo, in, u0, out, Fx, Fy, Fz, dtp[0], wa, wb, pr, Vout, flags, mode);
 }
 
 // Surface mode (one radial layer between conducting walls): the radial ghost
@@ -1120,13 +1137,11 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
 // With conducting walls the tangential EMFs on the walls vanish, so the wall
 // z-faces keep their value; they are left untouched, which also lets the
 // surface-mode zone update read them after this kernel has run in place.
-__global__ void k_update_faces(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
-                               const double* __restrict__ Ex, const double* __restrict__ Ey,
-                               const double* __restrict__ Ez, const double* dtp, double wa,
-                               double wb, int wall) {
-  int i, j, k, s;
-  if (!decode(rg, gtid(), i, j, k, s)) return;
-  const double dt = dtp[0];
+__device__ __forceinline__ void update_faces(const CsDims& d, int i, int j, int k, int s, const Geo& geo,
+                                             const State& in, const State& u0, const State& out,
+                                             const double* __restrict__ Ex, const double* __restrict__ Ey,
+                                             const double* __restrict__ Ez, double dt, double wa, double wb,
+                                             int wall) {
   const int N = d.N, NR = d.NR;
   const int64_t so = (int64_t)s * d.fpp;
   int64_t f = so + d.face(i, j, k);
@@ -1168,6 +1183,14 @@ __global__ void k_update_faces(CsDims d, Rng rg, Geo geo, State in, State u0, St
     out.Bz[f] = wa != 0.0 ? wa * u0.Bz[f] + wb * bn : wb * bn;
   }
 }
+__global__ void k_update_faces(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
+                               const double* __restrict__ Ex, const double* __restrict__ Ey,
+                               const double* __restrict__ Ez, const double* dtp, double wa,
+                               double wb, int wall) {
+  int i, j, k, s;
+  if (!decode(rg, gtid(), i, j, k, s)) return;
+  update_faces(d, i, j, k, s, geo, in, u0, out, Ex, Ey, Ez, dtp[0], wa, wb, wall);
+}
 
 
 // ------------------------------------------------------------------ reductions
@@ -1177,24 +1200,26 @@ __device__ __forceinline__ double wave_min(double v) {
   for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
   return v;
 }
-__global__ void k_cfl_partial(CsDims d, Rng rg, Geo geo, State st, Params pr, double* part) {
+// EVALUATE_TIMESTEP (490evaluate_timestep.f:118-278) for one zone:
+// cfl * L / (|v| + c_fast), L the shortest of its radial and lateral extents
+__device__ __forceinline__ double cfl_zone(const CsDims& d, int i, int j, int k, int s, const Geo& geo,
+                                           const State& st, const Params& pr) {
+  int64_t c = d.cell(i, j, k);
+  double u[8], w[8];
+  for (int q = 0; q < 8; ++q) u[q] = st.U[(s * 8 + q) * d.cpp + c];
+  double ent = st.ent[s * d.cpp + c];
+  Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor, pr.igm1};
+  cons_to_prim(gs, u, w, ent);
+  double b2 = w[5] * w[5] + w[6] * w[6] + w[7] * w[7];
+  double ms = sqrt((pr.gamma * w[4] + C<double>::ob4pi * b2) / w[0]);
+  double a21 = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) + ms;
+  double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
+  double L = fmin(r1 - r0, r0 * geo.t(d, T_CHMIN, s, i, j));
+  return pr.cfl * L / a21;
+}
+// minimum over the block's waves, written by thread 0
+__device__ __forceinline__ void block_min_store(double v, double* out) {
   __shared__ double sm[16];
-  int i, j, k, s;
-  double v = 1e300;
-  if (decode(rg, gtid(), i, j, k, s)) {
-    int64_t c = d.cell(i, j, k);
-    double u[8], w[8];
-    for (int q = 0; q < 8; ++q) u[q] = st.U[(s * 8 + q) * d.cpp + c];
-    double ent = st.ent[s * d.cpp + c];
-    Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor, pr.igm1};
-    cons_to_prim(gs, u, w, ent);
-    double b2 = w[5] * w[5] + w[6] * w[6] + w[7] * w[7];
-    double ms = sqrt((pr.gamma * w[4] + C<double>::ob4pi * b2) / w[0]);
-    double a21 = sqrt(w[1] * w[1] + w[2] * w[2] + w[3] * w[3]) + ms;
-    double r1 = geo.r(d, k), r0 = geo.r(d, k - 1);
-    double L = fmin(r1 - r0, r0 * geo.t(d, T_CHMIN, s, i, j));
-    v = pr.cfl * L / a21;
-  }
   v = wave_min(v);
   int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) sm[w] = v;
@@ -1202,8 +1227,14 @@ __global__ void k_cfl_partial(CsDims d, Rng rg, Geo geo, State st, Params pr, do
   if (threadIdx.x < 64) {
     double x = threadIdx.x < (blockDim.x >> 6) ? sm[threadIdx.x] : 1e300;
     x = wave_min(x);
-    if (threadIdx.x == 0) part[blockIdx.x] = x;
+    if (threadIdx.x == 0) *out = x;
   }
+}
+__global__ void k_cfl_partial(CsDims d, Rng rg, Geo geo, State st, Params pr, double* part) {
+  int i, j, k, s;
+  double v = 1e300;
+  if (decode(rg, gtid(), i, j, k, s)) v = cfl_zone(d, i, j, k, s, geo, st, pr);
+  block_min_store(v, part + blockIdx.x);
 }
 __global__ void k_cfl_final(const double* part, int n, double* dtp, double growth) {
   __shared__ double sm[16];
@@ -1303,6 +1334,108 @@ __global__ void k_divb_partial(CsDims d, Rng rg, Geo geo, State st, double* part
   }
 }
 
+// ------------------------------------------------------------------ batched shells
+// Many independent shells of different sizes advance together: each phase of
+// a stage is one launch over all shells.  Every shell's work of a phase starts
+// on a workgroup boundary (boff: prefix sums of its workgroup counts), so a
+// workgroup belongs to one shell, found by a binary search of boff; per-shell
+// reductions (the CFL step) stay per workgroup.
+struct ShellDev {
+  CsDims d;
+  Geo geo;
+  Params pr;
+  State st[2];       // [0] U^n, [1] stage scratch
+  void* V;
+  double *F[3], *S[3], *E[3];
+  double *dt, *part;
+  int* flags;
+  Rng all, fx, fy, fz, ez, ex, ey, cells, faces;
+};
+enum BatchPhase { BP_EXCH = 0, BP_RAD, BP_FLUXXY, BP_FLUXZ, BP_EMF, BP_FACES, BP_CELLS, BP_IMEX, BP_CFL, BP_COUNT };
+
+__device__ __forceinline__ int shell_of(const uint32_t* boff, int nsh, uint32_t blk) {
+  int lo = 0, hi = nsh - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (boff[mid] <= blk) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// bin / bout / b0: stage input, output and U^n buffers; mode: stage combination
+template <int PH, typename R, typename P>
+__global__ void __launch_bounds__(256, 2) k_batch(const ShellDev* __restrict__ sh, const uint32_t* __restrict__ boff,
+                                                  int nsh, int bin, int bout, double wa, double wb, int mode) {
+  const uint32_t lb = xcd_block();
+  const int b = shell_of(boff, nsh, lb);
+  const ShellDev& S = sh[b];
+  const uint32_t t = (lb - boff[b]) * blockDim.x + threadIdx.x;
+  const State& in = S.st[bin];
+  P* V = (P*)S.V;
+  int i, j, k, s;
+  if (PH == BP_EXCH) {
+    uint32_t nc, nf, nk;
+    exchange_counts(S.d, nc, nf, nk);
+    if (t < nc) exchange_cells<P>(S.d, in, V, t);
+    else if (t < nc + nf) exchange_faces(S.d, in, t - nc);
+    else if (t < nc + nf + nk) corner_fill<P>(S.d, in, V, t - nc - nf);
+  } else if (PH == BP_RAD) {
+    const uint32_t nl = S.d.N + 2 * S.d.g + 1;
+    if (t < 12u * S.d.g * nl * nl) radial_bc<P>(S.d, S.geo, in, S.pr.wall, V, S.pr, t);
+  } else if (PH == BP_FLUXXY) {
+    if (t < S.fx.total) {
+      decode(S.fx, t, i, j, k, s);
+      face_flux<0, R, P>(S.d, i, j, k, s, S.geo, in, V, S.F[0], S.S[0], S.pr);
+    } else if (decode(S.fy, t - S.fx.total, i, j, k, s)) {
+      face_flux<1, R, P>(S.d, i, j, k, s, S.geo, in, V, S.F[1], S.S[1], S.pr);
+    }
+  } else if (PH == BP_FLUXZ) {
+    if (decode(S.fz, t, i, j, k, s)) face_flux<2, R, P>(S.d, i, j, k, s, S.geo, in, V, S.F[2], S.S[2], S.pr);
+  } else if (PH == BP_EMF) {
+    if (t < S.ez.total) {
+      decode(S.ez, t, i, j, k, s);
+      emf_z<P>(S.d, i, j, k, s, S.geo, in, V, S.S[0], S.S[1], S.E[2], S.pr);
+    } else if (t < S.ez.total + S.ex.total) {
+      decode(S.ex, t - S.ez.total, i, j, k, s);
+      emf_xy<0, P>(S.d, i, j, k, s, S.geo, in, V, S.S[1], S.S[2], S.E[0], S.pr);
+    } else if (decode(S.ey, t - S.ez.total - S.ex.total, i, j, k, s)) {
+      emf_xy<1, P>(S.d, i, j, k, s, S.geo, in, V, S.S[2], S.S[0], S.E[1], S.pr);
+    }
+  } else if (PH == BP_FACES) {
+    if (decode(S.faces, t, i, j, k, s))
+      update_faces(S.d, i, j, k, s, S.geo, in, S.st[0], S.st[bout], S.E[0], S.E[1], S.E[2], S.dt[0], wa, wb,
+                   S.pr.wall);
+  } else if (PH == BP_CELLS) {
+    if (decode(S.cells, t, i, j, k, s))
+      update_cells<P>(S.d, i, j, k, s, S.geo, in, S.st[0], S.st[bout], S.F[0], S.F[1], S.F[2], S.dt[0], wa, wb,
+                      S.pr, V, S.flags, mode);
+  } else if (PH == BP_IMEX) {
+    if (decode(S.cells, t, i, j, k, s)) imex_source<P>(S.d, i, j, k, s, S.geo, S.st[0], S.st[1], V, S.dt[0], S.pr, S.flags);
+  } else if (PH == BP_CFL) {
+    double v = 1e300;
+    if (decode(S.cells, t, i, j, k, s)) v = cfl_zone(S.d, i, j, k, s, S.geo, S.st[0], S.pr);
+    block_min_store(v, S.part + (lb - boff[b]));
+  }
+}
+
+// per shell: dt = min(1.1 dt, min over its CFL workgroups)
+__global__ void k_batch_cfl_final(const ShellDev* __restrict__ sh, const uint32_t* __restrict__ boff) {
+  const ShellDev& S = sh[blockIdx.x];
+  const int n = (int)(boff[blockIdx.x + 1] - boff[blockIdx.x]);
+  __shared__ double sm[16];
+  double v = 1e300;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) v = fmin(v, S.part[t]);
+  v = wave_min(v);
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sm[w] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 1e300;
+    for (int q = 0; q < (int)(blockDim.x >> 6); ++q) x = fmin(x, sm[q]);
+    double cur = S.dt[0];
+    S.dt[0] = cur > 0.0 ? fmin(1.1 * cur, x) : x;
+  }
+}
 }  // namespace cs
 
 // ====================================================================== host
@@ -2114,5 +2247,310 @@ extern "C" int cs_device_info(char* buf, size_t len) {
   CS_HIP(hipGetDeviceProperties(&p, 0));
   snprintf(buf, len, "%s %s CUs=%d mem=%.1fGB", p.name, p.gcnArchName, p.multiProcessorCount,
            p.totalGlobalMem / 1e9);
+  return CS_OK;
+}
+
+// ====================================================================== batched shells (host)
+// A batch owns one cs_handle per shell (its buffers and metric) and advances
+// them together: each phase of a stage is one launch over all shells
+// (k_batch), the whole step one hipGraph.  The shells share the precision and
+// the time integrator; sizes, schemes and radii are free.
+struct cs_batch {
+  std::vector<cs_handle*> sh;
+  int n = 0, precision = 0, rk_order = 3, imex = 0;
+  hipStream_t stream = nullptr;
+  ShellDev* dsh = nullptr;
+  uint32_t* dboff[BP_COUNT] = {};
+  std::vector<uint32_t> hoff[BP_COUNT];
+  int bs[BP_COUNT];
+  hipGraphExec_t graph = nullptr;
+};
+
+namespace {
+
+uint64_t batch_work(const cs_handle* h, int ph) {
+  const CsDims& d = h->d;
+  const Ranges r = ranges_of(d, h->pr.wall);
+  uint32_t nc, nf, nk;
+  switch (ph) {
+    case BP_EXCH: exchange_counts(d, nc, nf, nk); return (uint64_t)nc + nf + nk;
+    case BP_RAD: { uint64_t nl = d.N + 2 * d.g + 1; return 12ull * d.g * nl * nl; }
+    case BP_FLUXXY: return (uint64_t)r.fx.total + r.fy.total;
+    case BP_FLUXZ: return r.fz.total;
+    case BP_EMF: return (uint64_t)r.ez.total + r.ex.total + r.ey.total;
+    case BP_FACES: return r.faces.total;
+    default: return r.cells.total;  // BP_CELLS, BP_IMEX, BP_CFL
+  }
+}
+
+int batch_offsets(cs_batch* B, int ph, int bs) {
+  std::vector<uint32_t>& o = B->hoff[ph];
+  o.assign(B->n + 1, 0);
+  for (int b = 0; b < B->n; ++b) {
+    uint64_t nb = (batch_work(B->sh[b], ph) + bs - 1) / bs;
+    if ((uint64_t)o[b] + nb > 0x7fffffffull) { set_error("batch too large for one launch"); return CS_E_INVALID; }
+    o[b + 1] = o[b] + (uint32_t)nb;
+  }
+  B->bs[ph] = bs;
+  CS_HIP(hipMemcpy(B->dboff[ph], o.data(), sizeof(uint32_t) * (B->n + 1), hipMemcpyHostToDevice));
+  return CS_OK;
+}
+
+ShellDev shell_desc(cs_handle* h) {
+  ShellDev s;
+  const Ranges r = ranges_of(h->d, h->pr.wall);
+  s.d = h->d;
+  s.geo = geo_of(h);
+  s.pr = h->pr;
+  s.st[0] = state_of(h, h->cur);
+  s.st[1] = state_of(h, (h->cur + 1) & 1);
+  s.V = h->V;
+  for (int q = 0; q < 3; ++q) { s.F[q] = h->F[q]; s.S[q] = h->S[q]; s.E[q] = h->E[q]; }
+  s.dt = h->dt;
+  s.part = h->part;
+  s.flags = h->flags;
+  s.all = r.all; s.fx = r.fx; s.fy = r.fy; s.fz = r.fz; s.ez = r.ez; s.ex = r.ex; s.ey = r.ey;
+  s.cells = r.cells; s.faces = r.faces;
+  return s;
+}
+
+template <int PH, typename R, typename P>
+int batch_launch_t(cs_batch* B, int bin, int bout, double wa, double wb, int mode) {
+  const unsigned nb = B->hoff[PH][B->n];
+  if (nb == 0) return CS_OK;
+  k_batch<PH, R, P><<<nb, B->bs[PH], 0, B->stream>>>(B->dsh, B->dboff[PH], B->n, bin, bout, wa, wb, mode);
+  CS_HIP(hipGetLastError());
+  return CS_OK;
+}
+template <typename R, typename P>
+int batch_phase_t(cs_batch* B, int ph, int bin, int bout, double wa, double wb, int mode) {
+  switch (ph) {
+    case BP_EXCH: return batch_launch_t<BP_EXCH, R, P>(B, bin, bout, wa, wb, mode);
+    case BP_RAD: return batch_launch_t<BP_RAD, R, P>(B, bin, bout, wa, wb, mode);
+    case BP_FLUXXY: return batch_launch_t<BP_FLUXXY, R, P>(B, bin, bout, wa, wb, mode);
+    case BP_FLUXZ: return batch_launch_t<BP_FLUXZ, R, P>(B, bin, bout, wa, wb, mode);
+    case BP_EMF: return batch_launch_t<BP_EMF, R, P>(B, bin, bout, wa, wb, mode);
+    case BP_FACES: return batch_launch_t<BP_FACES, R, P>(B, bin, bout, wa, wb, mode);
+    case BP_CELLS: return batch_launch_t<BP_CELLS, R, P>(B, bin, bout, wa, wb, mode);
+    case BP_IMEX: return batch_launch_t<BP_IMEX, R, P>(B, bin, bout, wa, wb, mode);
+    case BP_CFL: return batch_launch_t<BP_CFL, R, P>(B, bin, bout, wa, wb, mode);
+  }
+  set_error("bad batch phase %d", ph);
+  return CS_E_INVALID;
+}
+int batch_phase(cs_batch* B, int ph, int bin, int bout, double wa, double wb, int mode) {
+  return B->precision == CS_MIXED ? batch_phase_t<float, float>(B, ph, bin, bout, wa, wb, mode)
+                                  : batch_phase_t<double, double>(B, ph, bin, bout, wa, wb, mode);
+}
+
+int batch_stage(cs_batch* B, int irk) {
+  double wa, wb;
+  int bi, bo, rc;
+  stage_weights(B->rk_order, irk, wa, wb, bi, bo, B->imex);
+  const int mode = B->imex ? irk : 0;
+  if (B->imex && irk == 1) {
+    for (cs_handle* h : B->sh) {
+      const size_t fb = sizeof(double) * h->d.fpp * 6;
+      for (int q = 2; q < 5; ++q)
+        CS_HIP(hipMemcpyAsync(h->buf[(h->cur + 1) & 1][q], h->buf[h->cur][q], fb, hipMemcpyDeviceToDevice,
+                              B->stream));
+    }
+    if ((rc = batch_phase(B, BP_IMEX, 0, 1, wa, wb, mode))) return rc;
+  }
+  static const int ph[7] = {BP_EXCH, BP_RAD, BP_FLUXXY, BP_FLUXZ, BP_EMF, BP_FACES, BP_CELLS};
+  for (int q = 0; q < 7; ++q)
+    if ((rc = batch_phase(B, ph[q], bi, bo, wa, wb, mode))) return rc;
+  return CS_OK;
+}
+
+int batch_step(cs_batch* B) {
+  int rc;
+  for (int irk = 1; irk <= B->rk_order; ++irk)
+    if ((rc = batch_stage(B, irk))) return rc;
+  if ((rc = batch_phase(B, BP_CFL, 0, 0, 0.0, 1.0, 0))) return rc;
+  k_batch_cfl_final<<<B->n, 1024, 0, B->stream>>>(B->dsh, B->dboff[BP_CFL]);
+  CS_HIP(hipGetLastError());
+  return CS_OK;
+}
+
+// the primitives of U^n of every shell, before a run (its step leaves them)
+int batch_prepare(cs_batch* B) {
+  for (cs_handle* h : B->sh) {
+    int rc = ensure_prims(h, h->cur);
+    if (rc) return rc;
+    CS_HIP(hipStreamSynchronize(h->stream));
+  }
+  return CS_OK;
+}
+
+int batch_run(cs_batch* B, int nsteps, int use_graph) {
+  int rc;
+  if ((rc = batch_prepare(B))) return rc;
+  if (!use_graph) {
+    for (int s = 0; s < nsteps; ++s)
+      if ((rc = batch_step(B))) return rc;
+  } else {
+    if (!B->graph) {
+      hipGraph_t gr;
+      CS_HIP(hipStreamBeginCapture(B->stream, hipStreamCaptureModeThreadLocal));
+      rc = batch_step(B);
+      hipError_t e = hipStreamEndCapture(B->stream, &gr);
+      if (rc) return rc;
+      if (e != hipSuccess) { set_error("capture failed: %s", hipGetErrorString(e)); return CS_E_HIP; }
+      CS_HIP(hipGraphInstantiate(&B->graph, gr, nullptr, nullptr, 0));
+      hipGraphDestroy(gr);
+    }
+    for (int s = 0; s < nsteps; ++s) CS_HIP(hipGraphLaunch(B->graph, B->stream));
+  }
+  for (cs_handle* h : B->sh) h->vbuf = h->cur;
+  return CS_OK;
+}
+
+void batch_free(cs_batch* B) {
+  if (B->graph) hipGraphExecDestroy(B->graph);
+  for (int q = 0; q < BP_COUNT; ++q) hipFree(B->dboff[q]);
+  hipFree(B->dsh);
+  if (B->stream) hipStreamDestroy(B->stream);
+  for (cs_handle* h : B->sh) cs_destroy(h);
+}
+
+}  // namespace
+
+extern "C" int cs_batch_create(const cs_config* cfgs, int32_t n, cs_batch** out) {
+  if (!out || !cfgs || n < 1) { set_error("bad arguments"); return CS_E_INVALID; }
+  *out = nullptr;
+  for (int b = 1; b < n; ++b)
+    if (cfgs[b].precision != cfgs[0].precision || cfgs[b].rk_order != cfgs[0].rk_order ||
+        cfgs[b].imex != cfgs[0].imex) {
+      set_error("shell %d: the shells of a batch share precision, rk_order and imex", b);
+      return CS_E_INVALID;
+    }
+  cs_batch* B = new cs_batch();
+  B->n = n;
+  B->precision = cfgs[0].precision;
+  B->rk_order = cfgs[0].rk_order;
+  B->imex = cfgs[0].imex;
+  int rc = CS_OK;
+  for (int b = 0; b < n && rc == CS_OK; ++b) {
+    cs_handle* h = nullptr;
+    rc = cs_create(&cfgs[b], &h);
+    if (rc == CS_OK) B->sh.push_back(h);
+  }
+  std::vector<ShellDev> desc;
+  for (cs_handle* h : B->sh) desc.push_back(shell_desc(h));
+  if (rc == CS_OK && (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
+                      hipMalloc((void**)&B->dsh, sizeof(ShellDev) * n) != hipSuccess ||
+                      hipMemcpy(B->dsh, desc.data(), sizeof(ShellDev) * n, hipMemcpyHostToDevice) != hipSuccess)) {
+    set_error("batch set-up failed");
+    rc = CS_E_HIP;
+  }
+  for (int q = 0; q < BP_COUNT && rc == CS_OK; ++q) {
+    if (hipMalloc((void**)&B->dboff[q], sizeof(uint32_t) * (n + 1)) != hipSuccess) {
+      set_error("batch set-up failed");
+      rc = CS_E_NOMEM;
+      break;
+    }
+    rc = batch_offsets(B, q, kBlock);
+  }
+  if (rc) {
+    batch_free(B);
+    delete B;
+    return rc;
+  }
+  *out = B;
+  return CS_OK;
+}
+
+extern "C" int cs_batch_destroy(cs_batch* B) {
+  if (!B) return CS_OK;
+  if (B->stream) hipStreamSynchronize(B->stream);
+  batch_free(B);
+  delete B;
+  return CS_OK;
+}
+
+extern "C" int cs_batch_shell(cs_batch* B, int32_t i, cs_handle** h) {
+  if (!B || !h || i < 0 || i >= B->n) { set_error("bad arguments"); return CS_E_INVALID; }
+  *h = B->sh[i];
+  return CS_OK;
+}
+
+extern "C" int cs_batch_evaluate_timestep(cs_batch* B) {
+  if (!B) { set_error("null batch"); return CS_E_INVALID; }
+  for (cs_handle* h : B->sh) {
+    int rc = cs_evaluate_timestep(h);
+    if (rc) return rc;
+  }
+  return CS_OK;
+}
+
+extern "C" int cs_batch_advance(cs_batch* B, int32_t nsteps, int32_t use_graph) {
+  if (!B || nsteps < 0) { set_error("bad arguments"); return CS_E_INVALID; }
+  int rc = batch_run(B, nsteps, use_graph);
+  if (rc) return rc;
+  CS_HIP(hipStreamSynchronize(B->stream));
+  for (cs_handle* h : B->sh)
+    if ((rc = check_flags(h))) return rc;
+  return CS_OK;
+}
+
+extern "C" int cs_batch_time_steps(cs_batch* B, int32_t nsteps, float* ms_total) {
+  if (!B || !ms_total || nsteps < 1) { set_error("bad arguments"); return CS_E_INVALID; }
+  int rc = batch_run(B, 0, 1);  // graph built outside the timed region
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  CS_HIP(hipStreamSynchronize(B->stream));
+  CS_HIP(hipEventRecord(e0, B->stream));
+  if ((rc = batch_run(B, nsteps, 1))) return rc;
+  CS_HIP(hipEventRecord(e1, B->stream));
+  CS_HIP(hipEventSynchronize(e1));
+  CS_HIP(hipEventElapsedTime(ms_total, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  for (cs_handle* h : B->sh)
+    if ((rc = check_flags(h))) return rc;
+  return CS_OK;
+}
+
+extern "C" int cs_batch_autotune(cs_batch* B, int32_t iters, int32_t* block_sizes) {
+  if (!B || iters < 1) { set_error("bad arguments"); return CS_E_INVALID; }
+  static const int cand[3] = {64, 128, 256};
+  int rc;
+  if ((rc = batch_prepare(B))) return rc;
+  // a stage that writes only the scratch buffer: RK3 stage 2, RK2 / IMEX stage 1
+  double wa, wb;
+  int bi, bo;
+  const int irk = B->rk_order == 3 ? 2 : 1;
+  stage_weights(B->rk_order, irk, wa, wb, bi, bo, B->imex);
+  const int mode = B->imex ? irk : 0;
+  hipEvent_t e0, e1;
+  CS_HIP(hipEventCreate(&e0));
+  CS_HIP(hipEventCreate(&e1));
+  for (int ph = 0; ph < BP_COUNT; ++ph) {
+    if (ph == BP_CFL) continue;  // its partial buffers are sized for 256-thread workgroups
+    int best = kBlock;
+    float tbest = 1e30f;
+    for (int c = 0; c < 3; ++c) {
+      if ((rc = batch_offsets(B, ph, cand[c]))) return rc;
+      if ((rc = batch_phase(B, ph, ph == BP_IMEX ? 0 : bi, ph == BP_IMEX ? 1 : bo, wa, wb, mode))) return rc;
+      CS_HIP(hipEventRecord(e0, B->stream));
+      for (int it = 0; it < iters; ++it)
+        if ((rc = batch_phase(B, ph, ph == BP_IMEX ? 0 : bi, ph == BP_IMEX ? 1 : bo, wa, wb, mode))) return rc;
+      CS_HIP(hipEventRecord(e1, B->stream));
+      CS_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      CS_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < tbest) { tbest = ms; best = cand[c]; }
+    }
+    if ((rc = batch_offsets(B, ph, best))) return rc;
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (B->graph) { hipGraphExecDestroy(B->graph); B->graph = nullptr; }
+  for (cs_handle* h : B->sh) h->vbuf = -1;  // the timed zone updates rewrote V
+  if (block_sizes)
+    for (int ph = 0; ph < BP_COUNT; ++ph) block_sizes[ph] = B->bs[ph];
   return CS_OK;
 }

@@ -213,6 +213,25 @@ int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float* ms_avg);
 int cs_profile_steps(cs_handle* h, int32_t nsteps, float* ms_sum, int32_t* calls);
 int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units);
 
+/* ---------------------------------------------------------------- batches */
+/* Independent shells of different sizes advanced together (no counterpart
+ * in the reference, whose driver runs one shell): each phase of a stage is
+ * one launch over all shells, a step one hipGraph.  The shells must share
+ * precision, rk_order and imex.  cs_batch_shell lends the handle of shell i
+ * for upload, download and diagnostics (the batch owns it).
+ * cs_batch_autotune times every phase at 64/128/256-thread workgroups on the
+ * batch's current state, keeps the fastest (block_sizes[CS_BP_COUNT] may be
+ * NULL), and leaves the stage scratch buffers modified. */
+typedef struct cs_batch cs_batch;
+#define CS_BP_COUNT 9 /* exchange, radial pad, x/y fluxes, z fluxes, EMFs, faces, zones, implicit source, CFL */
+int cs_batch_create(const cs_config* cfgs, int32_t n, cs_batch** out);
+int cs_batch_destroy(cs_batch* b);
+int cs_batch_shell(cs_batch* b, int32_t i, cs_handle** h);
+int cs_batch_evaluate_timestep(cs_batch* b);
+int cs_batch_advance(cs_batch* b, int32_t nsteps, int32_t use_graph);
+int cs_batch_time_steps(cs_batch* b, int32_t nsteps, float* ms_total);
+int cs_batch_autotune(cs_batch* b, int32_t iters, int32_t* block_sizes);
+
 int cs_last_error(char* buf, size_t len);
 int cs_device_info(char* buf, size_t len);
 
