@@ -122,6 +122,9 @@ def run_gpu(cs, name, prec, steps, warmup, profile_steps):
     dev.dt = 0.0
     dev.evaluate_timestep()
     dev.advance(warmup, use_graph=True)
+    tuned = dev.autotune(3)
+    log("autotuned workgroups:", {k: tuned[k] for k in ("halo", "flux_xy", "flux_z", "emf", "update_faces",
+                                                         "update_cells")})
     ms = dev.time_steps(steps, use_graph=True)
     kern = {}
     for k, (ms_avg, calls) in dev.profile_steps(profile_steps).items():

@@ -117,6 +117,7 @@ def lib():
             "cs_time_steps": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
             "cs_time_kernel": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
             "cs_profile_steps": [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32)],
+            "cs_autotune": [vp, i32, ctypes.POINTER(i32)],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
             "cs_batch_shell": [vp, i32, ctypes.POINTER(vp)],
@@ -329,6 +330,12 @@ class CubeSphere:
         calls = (ctypes.c_int32 * len(KERNEL_NAMES))()
         _check(lib().cs_profile_steps(self.h, nsteps, ms, calls), "cs_profile_steps")
         return {k: (ms[k] / calls[k], calls[k]) for k in range(len(KERNEL_NAMES)) if calls[k]}
+
+    def autotune(self, iters=3):
+        """Pick each stage launch's workgroup size by timing; returns {kernel name: size}."""
+        bs = (ctypes.c_int32 * len(KERNEL_NAMES))()
+        _check(lib().cs_autotune(self.h, iters, bs), "cs_autotune")
+        return {KERNEL_NAMES[k]: bs[k] for k in range(len(KERNEL_NAMES))}
 
     def kernel_units(self, kernel):
         n = ctypes.c_int64()

@@ -1462,6 +1462,7 @@ struct cs_handle {
   int surf = 0; // one radial layer between walls: surface-mode stage kernels
   int vbuf = -1;  // stage buffer whose owned-zone primitives V holds (-1: none)
   struct Prof* prof = nullptr;  // per-launch event pairs while cs_profile_steps runs
+  int bs[CS_K_COUNT];           // workgroup size per kernel id (cs_autotune)
 };
 
 // Event pairs around the launches of one eagerly run step (cs_profile_steps).
@@ -1476,6 +1477,7 @@ namespace {
 
 const int kBlock = 256;
 inline unsigned nblk(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline unsigned nblk_b(uint64_t n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 State state_of(cs_handle* h, int b) {
   return State{h->buf[b][0], h->buf[b][1], h->buf[b][2], h->buf[b][3], h->buf[b][4]};
@@ -1496,12 +1498,13 @@ int launch_halo_t(cs_handle* h, int b, bool full, bool avg) {
   if (avg) launch_avg(h, b);
   uint32_t nc, nf, nk;
   exchange_counts(d, nc, nf, nk);
-  k_exchange<P><<<nblk((uint64_t)nc + nf + nk), kBlock, 0, h->stream>>>(d, st, V);
+  const int K = h->bs[CS_K_HALO];
+  k_exchange<P><<<nblk_b((uint64_t)nc + nf + nk, K), K, 0, h->stream>>>(d, st, V);
   uint64_t nl = (uint64_t)(d.N + 2 * d.g + 1);
   uint64_t n5 = 12ull * d.g * nl * nl;
   // surface-mode stages never read the radial ghosts
   if (full || !h->surf)
-    k_radial_bc<P><<<nblk(n5), kBlock, 0, h->stream>>>(d, geo_of(h), st, h->pr.wall, V, h->pr);
+    k_radial_bc<P><<<nblk_b(n5, K), K, 0, h->stream>>>(d, geo_of(h), st, h->pr.wall, V, h->pr);
   CS_HIP(hipGetLastError());
   return CS_OK;
 }
@@ -1604,6 +1607,7 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
   double wa = 0, wb = 1;
   int bi = 0, bo = 1;
   stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo, h->pr.imex);
+  const int K = h->bs[kid];
   State in = state_of(h, (h->cur + bi) & 1), u0 = state_of(h, h->cur), out = state_of(h, (h->cur + bo) & 1);
   switch (kid) {
     case CS_K_HALO: return launch_halo(h, b, !surf, false);  // the stage's exchange
@@ -1616,43 +1620,43 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
       }
       break;
     case CS_K_FLUX_X:
-      k_face_flux<0, R, P><<<nblk(r.fx.total), kBlock, 0, sm>>>(d, r.fx, geo, st, V, h->F[0], h->S[0], h->pr);
+      k_face_flux<0, R, P><<<nblk_b(r.fx.total, K), K, 0, sm>>>(d, r.fx, geo, st, V, h->F[0], h->S[0], h->pr);
       break;
     case CS_K_FLUX_Y:
-      k_face_flux<1, R, P><<<nblk(r.fy.total), kBlock, 0, sm>>>(d, r.fy, geo, st, V, h->F[1], h->S[1], h->pr);
+      k_face_flux<1, R, P><<<nblk_b(r.fy.total, K), K, 0, sm>>>(d, r.fy, geo, st, V, h->F[1], h->S[1], h->pr);
       break;
     case CS_K_FLUX_Z:
-      k_face_flux<2, R, P><<<nblk(r.fz.total), kBlock, 0, sm>>>(d, r.fz, geo, st, V, h->F[2], h->S[2], h->pr);
+      k_face_flux<2, R, P><<<nblk_b(r.fz.total, K), K, 0, sm>>>(d, r.fz, geo, st, V, h->F[2], h->S[2], h->pr);
       break;
     case CS_K_FLUX_XY:
-      k_face_flux_xy<R, P><<<nblk((uint64_t)r.fx.total + r.fy.total), kBlock, 0, sm>>>(
+      k_face_flux_xy<R, P><<<nblk_b((uint64_t)r.fx.total + r.fy.total, K), K, 0, sm>>>(
           d, r.fx, r.fy, geo, st, V, h->F[0], h->S[0], h->F[1], h->S[1], h->pr);
       break;
     case CS_K_EMF:
-      k_emf_all<P><<<nblk((uint64_t)r.ez.total + r.ex.total + r.ey.total), kBlock, 0, sm>>>(
+      k_emf_all<P><<<nblk_b((uint64_t)r.ez.total + r.ex.total + r.ey.total, K), K, 0, sm>>>(
           d, r.ez, r.ex, r.ey, geo, st, V, h->S[0], h->S[1], h->S[2], h->E[0], h->E[1], h->E[2], h->pr);
       break;
     case CS_K_EMF_Z:
-      k_emf_z<P><<<nblk(r.ez.total), kBlock, 0, sm>>>(d, r.ez, geo, st, V, h->S[0], h->S[1], h->E[2], h->pr);
+      k_emf_z<P><<<nblk_b(r.ez.total, K), K, 0, sm>>>(d, r.ez, geo, st, V, h->S[0], h->S[1], h->E[2], h->pr);
       break;
     case CS_K_EMF_X:
-      if (r.ex.total) k_emf_xy<0, P><<<nblk(r.ex.total), kBlock, 0, sm>>>(d, r.ex, geo, st, V, h->S[1], h->S[2], h->E[0], h->pr);
+      if (r.ex.total) k_emf_xy<0, P><<<nblk_b(r.ex.total, K), K, 0, sm>>>(d, r.ex, geo, st, V, h->S[1], h->S[2], h->E[0], h->pr);
       break;
     case CS_K_EMF_Y:
-      if (r.ey.total) k_emf_xy<1, P><<<nblk(r.ey.total), kBlock, 0, sm>>>(d, r.ey, geo, st, V, h->S[2], h->S[0], h->E[1], h->pr);
+      if (r.ey.total) k_emf_xy<1, P><<<nblk_b(r.ey.total, K), K, 0, sm>>>(d, r.ey, geo, st, V, h->S[2], h->S[0], h->E[1], h->pr);
       break;
     case CS_K_UPDATE_CELLS:  // zone update + cleanup + primitives of the output
       if (surf)
-        k_update_cells_surf<R, P><<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, V, h->F[0],
+        k_update_cells_surf<R, P><<<nblk_b(r.cells.total, K), K, 0, sm>>>(d, r.cells, geo, in, u0, out, V, h->F[0],
                                                                           h->F[1], h->dt, wa, wb, h->pr, h->flags,
                                                                           h->pr.imex ? irk : 0);
       else
-        k_update_cells<P><<<nblk(r.cells.total), kBlock, 0, sm>>>(d, r.cells, geo, in, u0, out, h->F[0], h->F[1],
+        k_update_cells<P><<<nblk_b(r.cells.total, K), K, 0, sm>>>(d, r.cells, geo, in, u0, out, h->F[0], h->F[1],
                                                                     h->F[2], h->dt, wa, wb, h->pr, V, h->flags,
                                                                     h->pr.imex ? irk : 0);
       break;
     case CS_K_UPDATE_FACES:
-      k_update_faces<<<nblk(r.faces.total), kBlock, 0, sm>>>(d, r.faces, geo, in, u0, out, h->E[0], h->E[1],
+      k_update_faces<<<nblk_b(r.faces.total, K), K, 0, sm>>>(d, r.faces, geo, in, u0, out, h->E[0], h->E[1],
                                                                h->E[2], h->dt, wa, wb, h->pr.wall);
       break;
     case CS_K_CLEANUP:
@@ -1859,6 +1863,7 @@ extern "C" int cs_create(const cs_config* cfg, cs_handle** out) {
     return CS_E_HIP;
   }
   cs_handle* h = new cs_handle();
+  for (int q = 0; q < CS_K_COUNT; ++q) h->bs[q] = kBlock;
   h->cfg = *cfg;
   h->d = cs_make_dims(cfg->ntp, cfg->nr, cfg->nghost);
   h->pr = Params{cfg->gamma, cfg->gamma - 1.0, cfg->rhofloor, cfg->prsfloor, cfg->cfl, cfg->grav,
@@ -2201,6 +2206,32 @@ extern "C" int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float
   *ms_avg = ms / iters;
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  return CS_OK;
+}
+
+extern "C" int cs_autotune(cs_handle* h, int32_t iters, int32_t* block_sizes) {
+  CHECK_H(h);
+  if (iters < 1) { set_error("bad arguments"); return CS_E_INVALID; }
+  // the launches of a stage; the CFL reduction keeps 256 (its partial sums
+  // are sized for it)
+  static const int ks[6] = {CS_K_HALO, CS_K_FLUX_XY, CS_K_FLUX_Z, CS_K_EMF, CS_K_UPDATE_FACES, CS_K_UPDATE_CELLS};
+  static const int cand[3] = {64, 128, 256};
+  for (int q = 0; q < 6; ++q) {
+    if (ks[q] == CS_K_FLUX_Z && h->surf) continue;
+    int best = kBlock;
+    float tbest = 1e30f;
+    for (int c = 0; c < 3; ++c) {
+      h->bs[ks[q]] = cand[c];
+      float ms = 0.f;
+      int rc = cs_time_kernel(h, ks[q], iters, &ms);
+      if (rc) return rc;
+      if (ms < tbest) { tbest = ms; best = cand[c]; }
+    }
+    h->bs[ks[q]] = best;
+  }
+  if (h->graph) { hipGraphExecDestroy(h->graph); h->graph = nullptr; }
+  if (block_sizes)
+    for (int q = 0; q < CS_K_COUNT; ++q) block_sizes[q] = h->bs[q];
   return CS_OK;
 }
 

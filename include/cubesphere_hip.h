@@ -212,6 +212,12 @@ int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float* ms_avg);
  * them).  Advances the state by nsteps. */
 int cs_profile_steps(cs_handle* h, int32_t nsteps, float* ms_sum, int32_t* calls);
 int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units);
+/* cs_autotune: time every launch of a stage at 64/128/256-thread workgroups
+ * (iters launches each, through cs_time_kernel) and keep the fastest; later
+ * steps and graphs use them.  block_sizes[CS_K_COUNT] (may be NULL) receives
+ * the workgroup size of every kernel id.  Only the stage scratch buffer and
+ * the primitives are modified. */
+int cs_autotune(cs_handle* h, int32_t iters, int32_t* block_sizes);
 
 /* ---------------------------------------------------------------- batches */
 /* Independent shells of different sizes advanced together (no counterpart

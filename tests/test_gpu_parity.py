@@ -171,3 +171,28 @@ def test_mixed_precision_close_to_fp64():
     rel = np.abs(U32 - U64).sum() / np.abs(U64).sum()
     print(f"mixed vs fp64 rel L1 after 20 steps: {rel:.3e}")
     assert rel < 1e-5
+
+
+@pytest.mark.parametrize("name", ["surf16_hlld_blast", "shell10x4_hlld_grav"])
+def test_autotune_keeps_results(name):
+    """Workgroup sizes change only the schedule: the same steps give the same bits."""
+    cfg, prob = CASES[name]
+    print(f"case {name}", flush=True)
+    st = cs.init_problem(cfg, prob, seed=21)
+    outs = []
+    for tune in (False, True):
+        dev = cs.CubeSphere(cfg)
+        dev.upload(st)
+        dev.dt = 0.0
+        dev.evaluate_timestep()
+        if tune:
+            sizes = dev.autotune(2)
+            assert sizes["flux_xy"] in (64, 128, 256)
+            dev.upload(st)
+            dev.dt = 0.0
+            dev.evaluate_timestep()
+        dev.advance(3, use_graph=True)
+        outs.append(dev.download())
+        dev.close()
+    for a, b in zip(outs[0].arrays(), outs[1].arrays()):
+        assert np.array_equal(a, b)
