@@ -36,11 +36,11 @@ def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     mesh_o = os.path.join(BUILD, "cs_mesh.o")
     kern_o = os.path.join(BUILD, "cs_kernels.o")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-c",
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-c",
           os.path.join(CSRC, "cs_mesh.cpp"), "-o", mesh_o])
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "cs_kernels.hip"), "-o", kern_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", mesh_o, kern_o, "-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", mesh_o, kern_o, "-pthread", "-o", tmp])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

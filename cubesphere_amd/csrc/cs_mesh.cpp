@@ -13,6 +13,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "../../include/cubesphere_hip.h"
@@ -600,7 +601,9 @@ extern "C" int cs_init_problem(const cs_config* cfg, int32_t problem, uint64_t s
   memset(Bzf, 0, sizeof(double) * d.fpp * 6);
   const double gam = cfg->gamma, ob8pi = 1.0 / (8.0 * (4.0 * atan(1.0)));
   auto r = [&](int k) { return rb[(size_t)(k + g)]; };
-  for (int s = 0; s < 6; ++s) {
+  // the six sectors are independent (each writes only its own arrays): one
+  // host thread per sector
+  auto sector = [&](int s) {
     auto tab = [&](int f, int i, int j) { return T[f * plane + s * d.t2 + d.t(i, j)]; };
     auto uvec = [&](int i, int j, double* o) {
       for (int c = 0; c < 3; ++c) o[c] = tab(T_UX + c, i, j);
@@ -694,6 +697,9 @@ extern "C" int cs_init_problem(const cs_config* cfg, int32_t problem, uint64_t s
           Uo[cidx + 7 * d.cpp] = B[2];
           ento[s * d.cpp + d.cell(i, j, k)] = P.p / pow(P.rho, gam - 1.0);
         }
-  }
+  };
+  std::vector<std::thread> th;
+  for (int s = 0; s < 6; ++s) th.emplace_back(sector, s);
+  for (auto& t : th) t.join();
   return CS_OK;
 }
