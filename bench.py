@@ -56,14 +56,19 @@ def algorithmic_bytes_per_unit(cs, kernel, prec, surface):
         # v and B of one zone, two face B, four speeds, edge frame (5), EMF out
         return 6 * w + 2 * 8 + 4 * 8 + t2 * 5 * 8 + 8
     if kernel == cs.K_UPDATE_CELLS:
-        # zone update + cleanup + primitives of the output: fluxes of 6 components
-        # (2 directions on the surface, 3 in a shell), stage input U + entropy, U0 (6),
-        # output U + entropy (9), V out (and in, for the surface walls), zone face B
-        # (the surface also reads its two wall z-faces), zone metric (solid angle,
-        # area vector, frame, zone-B operator: 17)
+        # zone update + cleanup + primitives of the output.
+        # Surface: lateral fluxes of the 6 components the face-average cleanup
+        # keeps (2 directions), stage input U + entropy (9), U0 of those 6,
+        # output U + entropy (9), V in and out, 2 lateral face B, the two wall
+        # z-faces, zone metric (solid angle, area vector, frame, zone-B
+        # operator: 17).
+        # Shell: fluxes of all 9 components (the induction fluxes advance the
+        # zone B the least-squares cleanup replaces; 3 directions), stage input,
+        # U^n (read by every implicit-explicit stage) and output U + entropy
+        # (9 each), V out, 3 face B; the 2-D metric is reused by every layer.
         if surface:
             return 12 * 8 + 9 * 8 + 6 * 8 + 9 * 8 + 18 * w + 2 * 8 + 2 * 8 + 17 * 8
-        return 18 * 8 + 6 * 8 + 6 * 8 + 9 * 8 + 9 * w + 3 * 8
+        return 27 * 8 + 9 * 8 + 9 * 8 + 9 * 8 + 9 * w + 3 * 8
     if kernel == cs.K_UPDATE_FACES:
         return 4 * 3 * 8 + t2 * 3 * 8             # in, U0, out B and the EMF of 3 families; areas
     if kernel == cs.K_HALO:

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/cubesphere_hip.h"
+#include "cs_halo_maps.h"
 #include "cs_internal.h"
 #include "cs_physics.h"
 
@@ -45,27 +46,42 @@ int host_metric(const cs_config* c, std::vector<double>& rb, std::vector<double>
   } while (0)
 
 // ------------------------------------------------------------------ ranges
+// Work-item order: i fastest (coalesced rows), then j then k, or, with kj
+// set, k before j.  The k-before-j order suits kernels whose neighbours lie
+// along i and k only and that read per-column 2-D metric entries: the radial
+// levels of one column follow each other a row apart, so the column's metric
+// (and the neighbour row in k) is still in L2 when the next level reads it.
 struct Rng {
   int i0, ni, j0, nj, k0, nk;
   uint32_t total;
+  int kj;
 };
-static Rng make_rng(int i0, int i1, int j0, int j1, int k0, int k1) {
+static Rng make_rng(int i0, int i1, int j0, int j1, int k0, int k1, int kj = 0) {
   Rng r;
   r.i0 = i0; r.ni = i1 - i0 + 1;
   r.j0 = j0; r.nj = j1 - j0 + 1;
   r.k0 = k0; r.nk = k1 - k0 + 1;
   r.total = (r.ni > 0 && r.nj > 0 && r.nk > 0) ? (uint32_t)((uint64_t)r.ni * r.nj * r.nk * 6) : 0u;
+  r.kj = kj;
   return r;
 }
 __device__ __forceinline__ bool decode(const Rng& r, uint32_t t, int& i, int& j, int& k, int& s) {
   if (t >= r.total) return false;
   uint32_t q = t / (uint32_t)r.ni;
   i = r.i0 + (int)(t - q * (uint32_t)r.ni);
-  uint32_t q2 = q / (uint32_t)r.nj;
-  j = r.j0 + (int)(q - q2 * (uint32_t)r.nj);
-  uint32_t q3 = q2 / (uint32_t)r.nk;
-  k = r.k0 + (int)(q2 - q3 * (uint32_t)r.nk);
-  s = (int)q3;
+  if (r.kj) {
+    uint32_t q2 = q / (uint32_t)r.nk;
+    k = r.k0 + (int)(q - q2 * (uint32_t)r.nk);
+    uint32_t q3 = q2 / (uint32_t)r.nj;
+    j = r.j0 + (int)(q2 - q3 * (uint32_t)r.nj);
+    s = (int)q3;
+  } else {
+    uint32_t q2 = q / (uint32_t)r.nj;
+    j = r.j0 + (int)(q - q2 * (uint32_t)r.nj);
+    uint32_t q3 = q2 / (uint32_t)r.nk;
+    k = r.k0 + (int)(q2 - q3 * (uint32_t)r.nk);
+    s = (int)q3;
+  }
   return true;
 }
 // Workgroups are handed to the 8 XCDs of an MI355X round robin (workgroup b
@@ -159,10 +175,7 @@ __device__ __forceinline__ void exchange_cells(const CsDims& d, const State& st,
   int k = 1 + (int)(r / N);
   int o = c_nbr[s][strip];
   int di, dj, si, sj;
-  if (strip == 0) { di = N + a; dj = tt; si = tt; sj = N + 1 - a; }
-  else if (strip == 1) { di = 1 - a; dj = tt; si = a; sj = N + 1 - tt; }
-  else if (strip == 2) { di = tt; dj = N + a; si = N + 1 - a; sj = tt; }
-  else { di = tt; dj = 1 - a; si = N + 1 - tt; sj = a; }
+  ghost_zone_map(N, strip, a, tt, di, dj, si, sj);
   int64_t dc = d.cell(di, dj, k), sc = d.cell(si, sj, k);
   double v[9];  // all loads in flight before the first store
   P w[9];       // primitives: the same function of the same zone
@@ -191,32 +204,18 @@ __device__ __forceinline__ void exchange_faces(const CsDims& d, const State& st,
   int tt = (int)(r % (N + 1));
   int k = (int)(r / (N + 1));
   int o = c_nbr[s][strip];
-  double* dst = nullptr;
-  double val = 0.0;
   // fam 0: the face family normal to the strip direction, t in [1,N], k in [1,NR]
   // fam 1: the tangential family, t in [0,N], k in [1,NR]
   // fam 2: Bzf, t in [1,N], k in [0,NR]
   if (fam == 0 && (tt < 1 || k < 1)) return;
   if (fam == 1 && k < 1) return;
   if (fam == 2 && tt < 1) return;
-  if (strip == 0) {  // right
-    if (fam == 0) { dst = st.Bx + s * d.fpp + d.face(N + a, tt, k); val = -st.By[o * d.fpp + d.face(tt, N - a, k)]; }
-    else if (fam == 1) { dst = st.By + s * d.fpp + d.face(N + a, tt, k); val = st.Bx[o * d.fpp + d.face(tt, N + 1 - a, k)]; }
-    else { dst = st.Bz + s * d.fpp + d.face(N + a, tt, k); val = st.Bz[o * d.fpp + d.face(tt, N + 1 - a, k)]; }
-  } else if (strip == 1) {  // left
-    if (fam == 0) { dst = st.Bx + s * d.fpp + d.face(-a, tt, k); val = -st.Bx[o * d.fpp + d.face(a, N + 1 - tt, k)]; }
-    else if (fam == 1) { dst = st.By + s * d.fpp + d.face(1 - a, tt, k); val = -st.By[o * d.fpp + d.face(a, N - tt, k)]; }
-    else { dst = st.Bz + s * d.fpp + d.face(1 - a, tt, k); val = st.Bz[o * d.fpp + d.face(a, N + 1 - tt, k)]; }
-  } else if (strip == 2) {  // up
-    if (fam == 0) { dst = st.By + s * d.fpp + d.face(tt, N + a, k); val = -st.Bx[o * d.fpp + d.face(N - a, tt, k)]; }
-    else if (fam == 1) { dst = st.Bx + s * d.fpp + d.face(tt, N + a, k); val = st.By[o * d.fpp + d.face(N + 1 - a, tt, k)]; }
-    else { dst = st.Bz + s * d.fpp + d.face(tt, N + a, k); val = st.Bz[o * d.fpp + d.face(N + 1 - a, tt, k)]; }
-  } else {  // down
-    if (fam == 0) { dst = st.By + s * d.fpp + d.face(tt, -a, k); val = -st.By[o * d.fpp + d.face(N + 1 - tt, a, k)]; }
-    else if (fam == 1) { dst = st.Bx + s * d.fpp + d.face(tt, 1 - a, k); val = -st.Bx[o * d.fpp + d.face(N - tt, a, k)]; }
-    else { dst = st.Bz + s * d.fpp + d.face(tt, 1 - a, k); val = st.Bz[o * d.fpp + d.face(N + 1 - tt, a, k)]; }
-  }
-  *dst = val;
+  int dfam, di, dj, sfam, si, sj;
+  double sign;
+  ghost_face_map(N, strip, a, fam, tt, dfam, di, dj, sfam, si, sj, sign);
+  const double* src = sfam == 0 ? st.Bx : (sfam == 1 ? st.By : st.Bz);
+  double* dst = dfam == 0 ? st.Bx : (dfam == 1 ? st.By : st.Bz);
+  dst[s * d.fpp + d.face(di, dj, k)] = sign * src[o * d.fpp + d.face(si, sj, k)];
 }
 
 // Corner ghost blocks (no sector owns them: the tri-corners) take the value of
@@ -927,10 +926,19 @@ __device__ __forceinline__ void cleanup_zone(const CsDims& d, const Geo& geo, co
   Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor, pr.igm1};
   double v[8];
   int fl = cons_to_prim(gs, u, v, ent);
-  for (int q = 0; q < 8; ++q) st.U[(s * 8 + q) * d.cpp + c] = u[q];
-  st.ent[s * d.cpp + c] = ent;
-  for (int q = 0; q < 8; ++q) V[((int64_t)s * 9 + q) * d.cpp + c] = (P)v[q];
-  V[((int64_t)s * 9 + 8) * d.cpp + c] = (P)ent;
+  // the zone and every lateral ghost slot that copies it (neighbour strips,
+  // and the corner block for a panel corner zone): the exchange is done here
+  auto put = [&](int ds, int64_t dc) {
+    for (int q = 0; q < 8; ++q) st.U[(ds * 8 + q) * d.cpp + dc] = u[q];
+    st.ent[ds * d.cpp + dc] = ent;
+    for (int q = 0; q < 8; ++q) V[((int64_t)ds * 9 + q) * d.cpp + dc] = (P)v[q];
+    V[((int64_t)ds * 9 + 8) * d.cpp + dc] = (P)ent;
+  };
+  put(s, c);
+  if (i <= d.g || i > d.N - d.g || j <= d.g || j > d.N - d.g) {
+    zone_pushes(c_nbr, d.N, d.g, s, i, j, [&](int ds, int di, int dj) { put(ds, d.cell(di, dj, k)); });
+    corner_pushes(d.N, d.g, i, j, [&](int di, int dj) { put(s, d.cell(di, dj, k)); });
+  }
   if (fl && flags) flags[0] = 1;
   if (!(u[0] == u[0]) || !(u[4] == u[4]) || isinf(u[4])) { if (flags) flags[1] = 1; }
 }
@@ -1050,8 +1058,10 @@ __device__ __forceinline__ void update_cells(const CsDims& d, int i, int j, int 
   finish_zone(d, in, u0, s, c, dt, wa, wb, mode, gr, n, L, u, en);
   cleanup_zone<P>(d, geo, out, Vout, pr, flags, s, i, j, k, u, en);
 }
+// at most 256 work-items per workgroup: the register budget that keeps the
+// zone's fluxes, states and cleanup weights out of scratch
 template <typename P>
-__global__ void k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
+__global__ void __launch_bounds__(256, 2) k_update_cells(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
                                const double* __restrict__ Fx, const double* __restrict__ Fy,
                                const double* __restrict__ Fz, const double* dtp, double wa,
                                double wb, Params pr, P* Vout, int* flags, int mode) {
@@ -1149,6 +1159,16 @@ __device__ __forceinline__ void update_faces(const CsDims& d, int i, int j, int 
   const bool xf = j >= 1 && k >= 1, yf = i >= 1 && k >= 1, zf = i >= 1 && j >= 1;
   // edge circulation elements, each read once (unique across sectors)
   const double ez = (xf || yf) ? ez_unique(d, Ez, s, i, j, k) : 0.0;
+  // a face value and every lateral ghost face that copies it: the exchange
+  // of the faces is done here
+  auto put = [&](int fam, int fs, int fi, int fj, double v) {
+    double* dst = fam == 0 ? out.Bx : (fam == 1 ? out.By : out.Bz);
+    dst[(int64_t)fs * d.fpp + d.face(fi, fj, k)] = v;
+    face_pushes(c_nbr, N, d.g, fam, fs, fi, fj, [&](int gf, int gs, int gi, int gj, double sg) {
+      double* gd = gf == 0 ? out.Bx : (gf == 1 ? out.By : out.Bz);
+      gd[(int64_t)gs * d.fpp + d.face(gi, gj, k)] = sg * v;
+    });
+  };
   // A face on a panel edge has two copies (k_avg_coinciding's pairs).  One
   // sector, the one that holds the pair's first copy there, updates it and
   // writes the negated value into the other copy, so the pair stays exactly
@@ -1160,9 +1180,9 @@ __device__ __forceinline__ void update_faces(const CsDims& d, int i, int j, int 
     double A = 0.5 * (r1 * r1 - r0 * r0) * geo.t(d, T_THY, s, i, j);
     double bn = in.Bx[f] - dt * circ / A;
     double v = wa != 0.0 ? wa * u0.Bx[f] + wb * bn : wb * bn;
-    out.Bx[f] = v;
-    if (i == N) out.By[(int64_t)c_nbr[s][0] * d.fpp + d.face(j, N, k)] = -v;
-    else if (i == 0) out.Bx[(int64_t)c_nbr[s][1] * d.fpp + d.face(0, N + 1 - j, k)] = -v;
+    put(0, s, i, j, v);
+    if (i == N) put(1, c_nbr[s][0], j, N, -v);
+    else if (i == 0) put(0, c_nbr[s][1], 0, N + 1 - j, -v);
   }
   // y-face: cell i in [1,N], y-vertex j in [0,N], cell k in [1,NR]
   if (yf && j != N && !(j == 0 && s >= 3)) {
@@ -1171,8 +1191,8 @@ __device__ __forceinline__ void update_faces(const CsDims& d, int i, int j, int 
     double A = 0.5 * (r1 * r1 - r0 * r0) * geo.t(d, T_THX, s, i, j);
     double bn = in.By[f] - dt * circ / A;
     double v = wa != 0.0 ? wa * u0.By[f] + wb * bn : wb * bn;
-    out.By[f] = v;
-    if (j == 0) out.By[(int64_t)c_nbr[s][3] * d.fpp + d.face(N + 1 - i, 0, k)] = -v;
+    put(1, s, i, j, v);
+    if (j == 0) put(1, c_nbr[s][3], N + 1 - i, 0, -v);
   }
   // z-face: cells i,j in [1,N], z-vertex k in [0,NR]
   if (zf && !(wall && (k == 0 || k == NR))) {
@@ -1180,7 +1200,7 @@ __device__ __forceinline__ void update_faces(const CsDims& d, int i, int j, int 
                   ex_unique(d, Ex, Ey, s, i, j, k) + ex_unique(d, Ex, Ey, s, i, j - 1, k);
     double A = r1 * r1 * geo.t(d, T_ZSN, s, i, j);
     double bn = in.Bz[f] - dt * circ / A;
-    out.Bz[f] = wa != 0.0 ? wa * u0.Bz[f] + wb * bn : wb * bn;
+    put(2, s, i, j, wa != 0.0 ? wa * u0.Bz[f] + wb * bn : wb * bn);
   }
 }
 __global__ void k_update_faces(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
@@ -1461,6 +1481,10 @@ struct cs_handle {
   int cur = 0;  // which buffer holds U^n
   int surf = 0; // one radial layer between walls: surface-mode stage kernels
   int vbuf = -1;  // stage buffer whose owned-zone primitives V holds (-1: none)
+  // stage buffer whose lateral ghost zones (with their primitives in V) and
+  // ghost faces were written by the zone and face updates that produced it,
+  // so the next stage needs no exchange (-1: none)
+  int vghost = -1;
   struct Prof* prof = nullptr;  // per-launch event pairs while cs_profile_steps runs
   int bs[CS_K_COUNT];           // workgroup size per kernel id (cs_autotune)
 };
@@ -1490,8 +1514,10 @@ void launch_avg(cs_handle* h, int b) {
 }
 // avg: average the two copies of every panel-edge face first.  The stage
 // path leaves it out: its face update writes both copies from one value.
+// lateral: the exchange between sectors; radial: the radial ghost layers
+// (after the exchange: they pad the lateral ghost columns too).
 template <typename P>
-int launch_halo_t(cs_handle* h, int b, bool full, bool avg) {
+int launch_halo_t(cs_handle* h, int b, bool lateral, bool radial, bool avg) {
   const CsDims& d = h->d;
   State st = state_of(h, b);
   P* V = (P*)h->V;
@@ -1499,18 +1525,16 @@ int launch_halo_t(cs_handle* h, int b, bool full, bool avg) {
   uint32_t nc, nf, nk;
   exchange_counts(d, nc, nf, nk);
   const int K = h->bs[CS_K_HALO];
-  k_exchange<P><<<nblk_b((uint64_t)nc + nf + nk, K), K, 0, h->stream>>>(d, st, V);
+  if (lateral) k_exchange<P><<<nblk_b((uint64_t)nc + nf + nk, K), K, 0, h->stream>>>(d, st, V);
   uint64_t nl = (uint64_t)(d.N + 2 * d.g + 1);
   uint64_t n5 = 12ull * d.g * nl * nl;
-  // surface-mode stages never read the radial ghosts
-  if (full || !h->surf)
-    k_radial_bc<P><<<nblk_b(n5, K), K, 0, h->stream>>>(d, geo_of(h), st, h->pr.wall, V, h->pr);
+  if (radial) k_radial_bc<P><<<nblk_b(n5, K), K, 0, h->stream>>>(d, geo_of(h), st, h->pr.wall, V, h->pr);
   CS_HIP(hipGetLastError());
   return CS_OK;
 }
-int launch_halo(cs_handle* h, int b, bool full = true, bool avg = true) {
-  return h->cfg.precision == CS_MIXED ? launch_halo_t<float>(h, b, full, avg)
-                                      : launch_halo_t<double>(h, b, full, avg);
+int launch_halo(cs_handle* h, int b, bool lateral = true, bool radial = true, bool avg = true) {
+  return h->cfg.precision == CS_MIXED ? launch_halo_t<float>(h, b, lateral, radial, avg)
+                                      : launch_halo_t<double>(h, b, lateral, radial, avg);
 }
 
 void stage_weights(int order, int irk, double& wa, double& wb, int& in, int& out, int imex = 0) {
@@ -1552,13 +1576,15 @@ Ranges ranges_of(const CsDims& d, int wall) {
   const int ke0 = wall ? 1 : 0, ke1 = wall ? NR - 1 : NR;
   Ranges r;
   r.all = make_rng(1 - g, N + g, 1 - g, N + g, 1 - g, NR + g);
-  r.fx = make_rng(0, N, 0, N + 1, kl0, kl1);
+  // x-faces (stencil along i), z-faces (along k), y-edges (zones at i-1, i and
+  // k-1, k) and zones (update, implicit source, CFL) go k before j
+  r.fx = make_rng(0, N, 0, N + 1, kl0, kl1, 1);
   r.fy = make_rng(0, N + 1, 0, N, kl0, kl1);
-  r.fz = make_rng(0, N + 1, 0, N + 1, 0, NR);
+  r.fz = make_rng(0, N + 1, 0, N + 1, 0, NR, 1);
   r.ez = make_rng(0, N, 0, N, 1, NR);
   r.ex = make_rng(1, N, 0, N, ke0, ke1);
-  r.ey = make_rng(0, N, 1, N, ke0, ke1);
-  r.cells = make_rng(1, N, 1, N, 1, NR);
+  r.ey = make_rng(0, N, 1, N, ke0, ke1, 1);
+  r.cells = make_rng(1, N, 1, N, 1, NR, 1);
   r.faces = make_rng(0, N, 0, N, 0, NR);
   return r;
 }
@@ -1597,7 +1623,7 @@ int launch_sync(cs_handle* h, bool flux, bool edge) {
 
 // Launch one kernel of the stage on buffer b (update kernels use irk).
 template <typename R, typename P>
-int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
+int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf, int halo) {
   const CsDims& d = h->d;
   const Ranges r = ranges_of(d, h->pr.wall);
   State st = state_of(h, b);
@@ -1610,7 +1636,7 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
   const int K = h->bs[kid];
   State in = state_of(h, (h->cur + bi) & 1), u0 = state_of(h, h->cur), out = state_of(h, (h->cur + bo) & 1);
   switch (kid) {
-    case CS_K_HALO: return launch_halo(h, b, !surf, false);  // the stage's exchange
+    case CS_K_HALO: return launch_halo(h, b, halo & 1, (halo & 2) != 0, false);  // the stage's halo
     case CS_K_PRIMS:
       if (surf) {
         Rng r1 = make_rng(1 - d.g, d.N + d.g, 1 - d.g, d.N + d.g, 1, 1);
@@ -1677,13 +1703,14 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf) {
   CS_HIP(hipGetLastError());
   return CS_OK;
 }
-// surf: use the surface-mode stage kernels (only from the stage path)
-int launch_kernel(cs_handle* h, int kid, int b, int irk, bool surf = false) {
+// surf: use the surface-mode stage kernels (only from the stage path);
+// halo: parts of CS_K_HALO, bit 0 the lateral exchange, bit 1 the radial layers
+int launch_kernel(cs_handle* h, int kid, int b, int irk, bool surf = false, int halo = 3) {
   Prof* p = h->prof;
   const bool rec = p && p->n < kMaxProf;
   if (rec) CS_HIP(hipEventRecord(p->ev[2 * p->n], h->stream));
-  int rc = h->cfg.precision == CS_MIXED ? launch_kernel_t<float, float>(h, kid, b, irk, surf)
-                                        : launch_kernel_t<double, double>(h, kid, b, irk, surf);
+  int rc = h->cfg.precision == CS_MIXED ? launch_kernel_t<float, float>(h, kid, b, irk, surf, halo)
+                                        : launch_kernel_t<double, double>(h, kid, b, irk, surf, halo);
   if (rec) {
     CS_HIP(hipEventRecord(p->ev[2 * p->n + 1], h->stream));
     p->kid[p->n++] = kid;
@@ -1724,7 +1751,7 @@ int launch_update(cs_handle* h, int irk, bool surf = false) {
   double wa, wb;
   int bi, bo;
   stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo, h->pr.imex);
-  h->vbuf = (h->cur + bo) & 1;
+  h->vbuf = h->vghost = (h->cur + bo) & 1;
   return CS_OK;
 }
 // Primitives of the owned zones of buffer b when V does not hold them (after
@@ -1735,6 +1762,7 @@ int launch_prims_interior_t(cs_handle* h, int b) {
   k_prims<P><<<nblk(r.cells.total), kBlock, 0, h->stream>>>(h->d, r.cells, state_of(h, b), (P*)h->V, h->pr);
   CS_HIP(hipGetLastError());
   h->vbuf = b;
+  h->vghost = -1;
   return CS_OK;
 }
 int ensure_prims(cs_handle* h, int b) {
@@ -1756,7 +1784,7 @@ int launch_imex_source_t(cs_handle* h) {
                                                                    state_of(h, sb), (P*)h->V, h->dt, h->pr,
                                                                    h->flags);
   CS_HIP(hipGetLastError());
-  h->vbuf = sb;
+  h->vbuf = h->vghost = sb;
   return CS_OK;
 }
 int launch_imex_source(cs_handle* h) {
@@ -1768,9 +1796,14 @@ int launch_stage(cs_handle* h, int irk) {
   int rc;
   const bool surf = h->surf != 0;
   if (h->pr.imex && irk == 1 && (rc = launch_imex_source(h))) return rc;
-  if ((rc = ensure_prims(h, b))) return rc;
-  // panel-edge faces are consistent (written once by the previous face update)
-  if ((rc = launch_kernel(h, CS_K_HALO, b, 1, surf))) return rc;
+  // the updates that produced b wrote its lateral ghosts as they went; after
+  // an upload or a timing run the primitives and the exchange are made here.
+  // Panel-edge faces are consistent (written once by the previous face update).
+  const bool lateral = !(h->vbuf == b && h->vghost == b);
+  if (lateral && (rc = ensure_prims(h, b))) return rc;
+  const int halo = (lateral ? 1 : 0) | (surf ? 0 : 2);  // surface stages never read the radial ghosts
+  if (halo && (rc = launch_kernel(h, CS_K_HALO, b, 1, surf, halo))) return rc;
+  if (lateral) h->vghost = b;
   // lateral fluxes in one launch, radial ones (shells), all edge EMFs in one
   // launch; panel-edge fluxes and EMFs are made unique where they are read
   static const int ks[3] = {CS_K_FLUX_XY, CS_K_FLUX_Z, CS_K_EMF};
@@ -1930,16 +1963,18 @@ extern "C" int cs_upload_state(cs_handle* h, const double* U, const double* ent,
   const CsDims& d = h->d;
   size_t cb = sizeof(double) * d.cpp * 6, fb = sizeof(double) * d.fpp * 6;
   h->cur = 0;
-  h->vbuf = -1;
+  h->vbuf = h->vghost = -1;
   CS_HIP(hipMemcpyAsync(h->buf[0][0], U, cb * 8, hipMemcpyHostToDevice, h->stream));
   CS_HIP(hipMemcpyAsync(h->buf[0][1], ent, cb, hipMemcpyHostToDevice, h->stream));
   CS_HIP(hipMemcpyAsync(h->buf[0][2], Bxf, fb, hipMemcpyHostToDevice, h->stream));
   CS_HIP(hipMemcpyAsync(h->buf[0][3], Byf, fb, hipMemcpyHostToDevice, h->stream));
   CS_HIP(hipMemcpyAsync(h->buf[0][4], Bzf, fb, hipMemcpyHostToDevice, h->stream));
   // the two copies of each panel-edge face are made one (negation-consistent)
-  // value here; from then on the face update keeps them so
-  launch_avg(h, 0);
-  CS_HIP(hipGetLastError());
+  // value here; from then on the face update keeps them so.  The ghost zones
+  // and faces are filled once: faces the updates never write (conducting
+  // walls) keep these ghost copies in both stage buffers.
+  int rc = launch_halo(h, 0, true, true, true);
+  if (rc) return rc;
   // the scratch stage buffer starts as a copy so its ghost zones are defined
   for (int q = 0; q < 5; ++q)
     CS_HIP(hipMemcpyAsync(h->buf[1][q], h->buf[0][q], q == 0 ? cb * 8 : (q == 1 ? cb : fb),
@@ -2195,12 +2230,15 @@ extern "C" int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float
   CS_HIP(hipStreamSynchronize(h->stream));
   CS_HIP(hipEventRecord(e0, h->stream));
   for (int it = 0; it < iters; ++it) {
-    int rc = kernel == CS_K_STEP ? launch_step(h) : launch_kernel(h, kernel, h->cur, irk, h->surf != 0);
+    // the halo is timed as the full exchange of a stage whose input was not
+    // produced by the updates (with the radial layers of a shell)
+    int rc = kernel == CS_K_STEP ? launch_step(h)
+                                 : launch_kernel(h, kernel, h->cur, irk, h->surf != 0, h->surf ? 1 : 3);
     if (rc) return rc;
   }
   CS_HIP(hipEventRecord(e1, h->stream));
   CS_HIP(hipEventSynchronize(e1));
-  h->vbuf = -1;  // a timed zone update or conversion leaves V holding another state
+  h->vbuf = h->vghost = -1;  // a timed zone update or conversion leaves V holding another state
   float ms = 0.f;
   CS_HIP(hipEventElapsedTime(&ms, e0, e1));
   *ms_avg = ms / iters;
@@ -2388,8 +2426,9 @@ int batch_stage(cs_batch* B, int irk) {
     }
     if ((rc = batch_phase(B, BP_IMEX, 0, 1, wa, wb, mode))) return rc;
   }
-  static const int ph[7] = {BP_EXCH, BP_RAD, BP_FLUXXY, BP_FLUXZ, BP_EMF, BP_FACES, BP_CELLS};
-  for (int q = 0; q < 7; ++q)
+  // no exchange phase: the previous stage's updates wrote the lateral ghosts
+  static const int ph[6] = {BP_RAD, BP_FLUXXY, BP_FLUXZ, BP_EMF, BP_FACES, BP_CELLS};
+  for (int q = 0; q < 6; ++q)
     if ((rc = batch_phase(B, ph[q], bi, bo, wa, wb, mode))) return rc;
   return CS_OK;
 }
@@ -2404,11 +2443,15 @@ int batch_step(cs_batch* B) {
   return CS_OK;
 }
 
-// the primitives of U^n of every shell, before a run (its step leaves them)
+// the primitives of U^n of every shell and its lateral ghosts, before a run
+// (its step leaves them: the updates write the ghosts they feed)
 int batch_prepare(cs_batch* B) {
   for (cs_handle* h : B->sh) {
+    if (h->vbuf == h->cur && h->vghost == h->cur) continue;
     int rc = ensure_prims(h, h->cur);
+    if (!rc) rc = launch_halo(h, h->cur, true, false, false);
     if (rc) return rc;
+    h->vghost = h->cur;
     CS_HIP(hipStreamSynchronize(h->stream));
   }
   return CS_OK;
@@ -2433,7 +2476,7 @@ int batch_run(cs_batch* B, int nsteps, int use_graph) {
     }
     for (int s = 0; s < nsteps; ++s) CS_HIP(hipGraphLaunch(B->graph, B->stream));
   }
-  for (cs_handle* h : B->sh) h->vbuf = h->cur;
+  for (cs_handle* h : B->sh) h->vbuf = h->vghost = h->cur;
   return CS_OK;
 }
 
@@ -2580,7 +2623,7 @@ extern "C" int cs_batch_autotune(cs_batch* B, int32_t iters, int32_t* block_size
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (B->graph) { hipGraphExecDestroy(B->graph); B->graph = nullptr; }
-  for (cs_handle* h : B->sh) h->vbuf = -1;  // the timed zone updates rewrote V
+  for (cs_handle* h : B->sh) h->vbuf = h->vghost = -1;  // the timed zone updates rewrote V
   if (block_sizes)
     for (int ph = 0; ph < BP_COUNT; ++ph) block_sizes[ph] = B->bs[ph];
   return CS_OK;
