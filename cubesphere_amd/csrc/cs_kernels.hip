@@ -897,7 +897,7 @@ __device__ __forceinline__ double ex_unique(const CsDims& d, const double* __res
 // the primitives V the next stage reconstructs from.  u[0..7] and ent hold
 // the updated zone (u[5..7] advanced by the induction fluxes, as the
 // reference's dU_dt).
-template <typename P>
+template <typename P, bool PUSH = true>
 __device__ __forceinline__ void cleanup_zone(const CsDims& d, const Geo& geo, const State& st, P* V,
                                              const Params& pr, int* flags, int s, int i, int j, int k,
                                              double* u, double ent) {
@@ -926,8 +926,11 @@ __device__ __forceinline__ void cleanup_zone(const CsDims& d, const Geo& geo, co
   Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor, pr.igm1};
   double v[8];
   int fl = cons_to_prim(gs, u, v, ent);
-  // the zone and every lateral ghost slot that copies it (neighbour strips,
-  // and the corner block for a panel corner zone): the exchange is done here
+  // the zone and, in a shell, every lateral ghost slot that copies it
+  // (neighbour strips, and the corner block for a panel corner zone): the
+  // exchange is done here.  Surface mode keeps the exchange launch: its zone
+  // update is register bound and the scattered ghost writes cost it more than
+  // the launch they replace.
   auto put = [&](int ds, int64_t dc) {
     for (int q = 0; q < 8; ++q) st.U[(ds * 8 + q) * d.cpp + dc] = u[q];
     st.ent[ds * d.cpp + dc] = ent;
@@ -935,7 +938,7 @@ __device__ __forceinline__ void cleanup_zone(const CsDims& d, const Geo& geo, co
     V[((int64_t)ds * 9 + 8) * d.cpp + dc] = (P)ent;
   };
   put(s, c);
-  if (i <= d.g || i > d.N - d.g || j <= d.g || j > d.N - d.g) {
+  if (PUSH && !pr.surf && (i <= d.g || i > d.N - d.g || j <= d.g || j > d.N - d.g)) {
     zone_pushes(c_nbr, d.N, d.g, s, i, j, [&](int ds, int di, int dj) { put(ds, d.cell(di, dj, k)); });
     corner_pushes(d.N, d.g, i, j, [&](int di, int dj) { put(s, d.cell(di, dj, k)); });
   }
@@ -1138,7 +1141,7 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
   double u[8], en;
   finish_zone(d, in, u0, s, c, dt, wa, wb, mode, gr, A, L, u, en);
   // V was read above for this zone only, so it is overwritten in place
-  cleanup_zone<P>(d, geo, out, V, pr, flags, s, i, j, k, u, en);
+  cleanup_zone<P, false>(d, geo, out, V, pr, flags, s, i, j, k, u, en);
 }
 
 // Constrained transport (484update_imex_rk_step_p1.f:646-908) with every edge
@@ -1151,7 +1154,7 @@ __device__ __forceinline__ void update_faces(const CsDims& d, int i, int j, int 
                                              const State& in, const State& u0, const State& out,
                                              const double* __restrict__ Ex, const double* __restrict__ Ey,
                                              const double* __restrict__ Ez, double dt, double wa, double wb,
-                                             int wall) {
+                                             int wall, int push) {
   const int N = d.N, NR = d.NR;
   const int64_t so = (int64_t)s * d.fpp;
   int64_t f = so + d.face(i, j, k);
@@ -1159,12 +1162,12 @@ __device__ __forceinline__ void update_faces(const CsDims& d, int i, int j, int 
   const bool xf = j >= 1 && k >= 1, yf = i >= 1 && k >= 1, zf = i >= 1 && j >= 1;
   // edge circulation elements, each read once (unique across sectors)
   const double ez = (xf || yf) ? ez_unique(d, Ez, s, i, j, k) : 0.0;
-  // a face value and every lateral ghost face that copies it: the exchange
-  // of the faces is done here
+  // a face value and (push: shells) every lateral ghost face that copies it:
+  // the exchange of the faces is done here
   auto put = [&](int fam, int fs, int fi, int fj, double v) {
     double* dst = fam == 0 ? out.Bx : (fam == 1 ? out.By : out.Bz);
     dst[(int64_t)fs * d.fpp + d.face(fi, fj, k)] = v;
-    face_pushes(c_nbr, N, d.g, fam, fs, fi, fj, [&](int gf, int gs, int gi, int gj, double sg) {
+    if (push) face_pushes(c_nbr, N, d.g, fam, fs, fi, fj, [&](int gf, int gs, int gi, int gj, double sg) {
       double* gd = gf == 0 ? out.Bx : (gf == 1 ? out.By : out.Bz);
       gd[(int64_t)gs * d.fpp + d.face(gi, gj, k)] = sg * v;
     });
@@ -1206,10 +1209,10 @@ __device__ __forceinline__ void update_faces(const CsDims& d, int i, int j, int 
 __global__ void k_update_faces(CsDims d, Rng rg, Geo geo, State in, State u0, State out,
                                const double* __restrict__ Ex, const double* __restrict__ Ey,
                                const double* __restrict__ Ez, const double* dtp, double wa,
-                               double wb, int wall) {
+                               double wb, int wall, int push) {
   int i, j, k, s;
   if (!decode(rg, gtid(), i, j, k, s)) return;
-  update_faces(d, i, j, k, s, geo, in, u0, out, Ex, Ey, Ez, dtp[0], wa, wb, wall);
+  update_faces(d, i, j, k, s, geo, in, u0, out, Ex, Ey, Ez, dtp[0], wa, wb, wall, push);
 }
 
 
@@ -1424,7 +1427,7 @@ __global__ void __launch_bounds__(256, 2) k_batch(const ShellDev* __restrict__ s
   } else if (PH == BP_FACES) {
     if (decode(S.faces, t, i, j, k, s))
       update_faces(S.d, i, j, k, s, S.geo, in, S.st[0], S.st[bout], S.E[0], S.E[1], S.E[2], S.dt[0], wa, wb,
-                   S.pr.wall);
+                   S.pr.wall, !S.pr.surf);
   } else if (PH == BP_CELLS) {
     if (decode(S.cells, t, i, j, k, s))
       update_cells<P>(S.d, i, j, k, s, S.geo, in, S.st[0], S.st[bout], S.F[0], S.F[1], S.F[2], S.dt[0], wa, wb,
@@ -1576,9 +1579,9 @@ Ranges ranges_of(const CsDims& d, int wall) {
   const int ke0 = wall ? 1 : 0, ke1 = wall ? NR - 1 : NR;
   Ranges r;
   r.all = make_rng(1 - g, N + g, 1 - g, N + g, 1 - g, NR + g);
-  // x-faces (stencil along i), z-faces (along k), y-edges (zones at i-1, i and
-  // k-1, k) and zones (update, implicit source, CFL) go k before j
-  r.fx = make_rng(0, N, 0, N + 1, kl0, kl1, 1);
+  // z-faces (stencil along k), y-edges (zones at i-1, i and k-1, k) and zones
+  // (update, implicit source, CFL) go k before j
+  r.fx = make_rng(0, N, 0, N + 1, kl0, kl1);
   r.fy = make_rng(0, N + 1, 0, N, kl0, kl1);
   r.fz = make_rng(0, N + 1, 0, N + 1, 0, NR, 1);
   r.ez = make_rng(0, N, 0, N, 1, NR);
@@ -1683,7 +1686,7 @@ int launch_kernel_t(cs_handle* h, int kid, int b, int irk, bool surf, int halo) 
       break;
     case CS_K_UPDATE_FACES:
       k_update_faces<<<nblk_b(r.faces.total, K), K, 0, sm>>>(d, r.faces, geo, in, u0, out, h->E[0], h->E[1],
-                                                               h->E[2], h->dt, wa, wb, h->pr.wall);
+                                                               h->E[2], h->dt, wa, wb, h->pr.wall, !h->surf);
       break;
     case CS_K_CLEANUP:
       set_error("the cleanup runs inside the zone update (CS_K_UPDATE_CELLS)");
@@ -1751,7 +1754,8 @@ int launch_update(cs_handle* h, int irk, bool surf = false) {
   double wa, wb;
   int bi, bo;
   stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo, h->pr.imex);
-  h->vbuf = h->vghost = (h->cur + bo) & 1;
+  h->vbuf = (h->cur + bo) & 1;
+  h->vghost = h->surf ? -1 : h->vbuf;  // shells: the updates wrote the lateral ghosts
   return CS_OK;
 }
 // Primitives of the owned zones of buffer b when V does not hold them (after
@@ -1784,7 +1788,8 @@ int launch_imex_source_t(cs_handle* h) {
                                                                    state_of(h, sb), (P*)h->V, h->dt, h->pr,
                                                                    h->flags);
   CS_HIP(hipGetLastError());
-  h->vbuf = h->vghost = sb;
+  h->vbuf = sb;
+  h->vghost = h->surf ? -1 : sb;
   return CS_OK;
 }
 int launch_imex_source(cs_handle* h) {
@@ -2327,6 +2332,7 @@ extern "C" int cs_device_info(char* buf, size_t len) {
 struct cs_batch {
   std::vector<cs_handle*> sh;
   int n = 0, precision = 0, rk_order = 3, imex = 0;
+  int nsurf = 0;  // surface-mode shells: their stages need the exchange phase
   hipStream_t stream = nullptr;
   ShellDev* dsh = nullptr;
   uint32_t* dboff[BP_COUNT] = {};
@@ -2426,9 +2432,10 @@ int batch_stage(cs_batch* B, int irk) {
     }
     if ((rc = batch_phase(B, BP_IMEX, 0, 1, wa, wb, mode))) return rc;
   }
-  // no exchange phase: the previous stage's updates wrote the lateral ghosts
-  static const int ph[6] = {BP_RAD, BP_FLUXXY, BP_FLUXZ, BP_EMF, BP_FACES, BP_CELLS};
-  for (int q = 0; q < 6; ++q)
+  // the updates of shells write the lateral ghosts they feed, so the exchange
+  // phase runs only when the batch holds surface-mode shells
+  static const int ph[7] = {BP_EXCH, BP_RAD, BP_FLUXXY, BP_FLUXZ, BP_EMF, BP_FACES, BP_CELLS};
+  for (int q = B->nsurf ? 0 : 1; q < 7; ++q)
     if ((rc = batch_phase(B, ph[q], bi, bo, wa, wb, mode))) return rc;
   return CS_OK;
 }
@@ -2476,7 +2483,10 @@ int batch_run(cs_batch* B, int nsteps, int use_graph) {
     }
     for (int s = 0; s < nsteps; ++s) CS_HIP(hipGraphLaunch(B->graph, B->stream));
   }
-  for (cs_handle* h : B->sh) h->vbuf = h->vghost = h->cur;
+  for (cs_handle* h : B->sh) {
+    h->vbuf = h->cur;
+    h->vghost = h->surf ? -1 : h->cur;
+  }
   return CS_OK;
 }
 
@@ -2511,7 +2521,10 @@ extern "C" int cs_batch_create(const cs_config* cfgs, int32_t n, cs_batch** out)
     if (rc == CS_OK) B->sh.push_back(h);
   }
   std::vector<ShellDev> desc;
-  for (cs_handle* h : B->sh) desc.push_back(shell_desc(h));
+  for (cs_handle* h : B->sh) {
+    desc.push_back(shell_desc(h));
+    B->nsurf += h->surf;
+  }
   if (rc == CS_OK && (hipStreamCreateWithFlags(&B->stream, hipStreamNonBlocking) != hipSuccess ||
                       hipMalloc((void**)&B->dsh, sizeof(ShellDev) * n) != hipSuccess ||
                       hipMemcpy(B->dsh, desc.data(), sizeof(ShellDev) * n, hipMemcpyHostToDevice) != hipSuccess)) {

@@ -116,12 +116,13 @@ def rel_l1(a, b):
     return np.abs(a - b).sum() / np.abs(b).sum()
 
 
-@pytest.mark.parametrize("name", list(CASES))
+@pytest.mark.parametrize("name", [n for n in CASES if n.startswith("shell")])
 def test_pushed_ghosts_equal_exchange(name):
-    """The zone and face updates write the lateral ghosts they feed; after
-    whole steps (graph and eager) every lateral ghost zone and face must equal
-    what a full exchange then copies into it, bit for bit (radial ghost layers,
-    which the next stage's boundary kernel fills, are left out)."""
+    """In a shell the zone and face updates write the lateral ghosts they
+    feed; after whole steps (graph and eager) every lateral ghost zone and face
+    must equal what a full exchange then copies into it, bit for bit (radial
+    ghost layers, which the next stage's boundary kernel fills, are left out).
+    Surface mode keeps the exchange launch at the start of each stage."""
     cfg, prob = CASES[name]
     print(f"case {name}", flush=True)
     g, NR = cfg.nghost, cfg.nr
