@@ -275,6 +275,41 @@ def bench_batch(cs, args):
     print(json.dumps(line))
 
 
+def bench_geometry(cs, args):
+    """The P1 quadrature geometry (vertices and mask, face quadrature and scale
+    length, volume quadrature, basis moments) of the 6 x 192^2 x 64 shell built
+    on the GPU; HBM bytes per item are the outputs written plus the compulsory
+    inputs read."""
+    cfg = cs.make_config(192, 64, 3, cs.RECON_WENO5, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
+                         cs.RMESH_UNIFORM, 1.0, 1.5)
+    z = cs.geometry_sizes(cfg)
+    runs = [cs.build_geometry(cfg, download=False)[1] for _ in range(max(1, args.steps))]
+    ms = [min(r[q] for r in runs) for q in range(4)]
+    items = [z["vert"], z["xf"] + z["yf"] + z["zf"], z["zone"], z["zone"]]
+    nbytes = [24 * z["vert"] + 4 * z["zone"],
+              216 * (z["xf"] + z["yf"]) + 312 * z["zf"] + 8 * z["zone"] + 24 * z["vert"],
+              864 * z["zone"] + 24 * z["vert"],
+              (292 + 32) * z["zone"]]
+    names = ["vertices+mask", "faces+scale", "volume", "moments"]
+    kern = {n: {"ms": t, "items": it, "GBps": b / (t * 1e-3) / 1e9, "frac": b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            for n, t, it, b in zip(names, ms, items, nbytes)}
+    for n, k in kern.items():
+        log(f"   {n:14s} {k['ms']:8.3f} ms  {k['items']:11d} items  {k['GBps']:7.1f} GB/s ({100 * k['frac']:.0f} % of HBM)")
+    total = sum(ms)
+    dom = max(kern, key=lambda n: kern[n]["ms"])
+    line = {
+        "metric": "P1 geometry zones/s", "value": z["zone"] / (total * 1e-3), "unit": "zones/s", "n_gpus": 1,
+        "steps": len(runs), "warmup": 0, "ms_per_step": total, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "P1 quadrature geometry of the 6x192^2x64 shell (ghosts included): vertices, "
+                               "face and volume quadrature, scale length, basis moments", "zones": z["zone"]},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom]["GBps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": kern[dom]["frac"], "traffic": None},
+        "cpu_baseline": None, "kernels": kern,
+    }
+    print(json.dumps(line))
+
+
 def probe(cs, name, prec, kernel, iters):
     """Hardware-counter run: `iters` launches of one kernel on the workload's
     state, then one plain device copy of 1 GiB (read) + 1 GiB (write), whose
@@ -331,6 +366,9 @@ def main():
         return
     if args.workload == "C4":
         bench_batch(cs, args)
+        return
+    if args.workload == "G3":
+        bench_geometry(cs, args)
         return
     res = {}
     for prec in (cs.FP64, cs.MIXED):

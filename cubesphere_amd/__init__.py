@@ -118,6 +118,9 @@ def lib():
             "cs_time_kernel": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
             "cs_profile_steps": [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32)],
             "cs_autotune": [vp, i32, ctypes.POINTER(i32)],
+            "cs_geometry_sizes": [cfgp, vp],
+            "cs_build_geometry": [cfgp, vp, vp],
+            "cs_mode_indices": [i32, ctypes.POINTER(i32), vp, vp, vp],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
             "cs_batch_shell": [vp, i32, ctypes.POINTER(vp)],
@@ -399,3 +402,60 @@ def device_info():
     buf = ctypes.create_string_buffer(256)
     rc = lib().cs_device_info(buf, 256)
     return buf.value.decode() if rc == 0 else None
+
+
+# ------------------------------------------------------------------ P1 geometry
+GEOMETRY_FACE_KEYS = ("pts", "wts", "normal", "area", "centroid")
+
+
+class FaceQuad(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in GEOMETRY_FACE_KEYS]
+
+
+class Geometry(ctypes.Structure):
+    _fields_ = [("vert", ctypes.c_void_p), ("mask", ctypes.c_void_p), ("xf", FaceQuad), ("yf", FaceQuad),
+                ("zf", FaceQuad), ("scale", ctypes.c_void_p), ("v_pts", ctypes.c_void_p),
+                ("v_wts", ctypes.c_void_p), ("v_tan", ctypes.c_void_p), ("v_vol", ctypes.c_void_p),
+                ("v_cen", ctypes.c_void_p), ("moments", ctypes.c_void_p)]
+
+
+def geometry_sizes(cfg):
+    """Element counts over the six sectors: vertices, zones, x-, y-, z-faces."""
+    n = np.zeros(5, np.int64)
+    _check(lib().cs_geometry_sizes(ctypes.byref(cfg), _p(n)), "cs_geometry_sizes")
+    return dict(zip(("vert", "zone", "xf", "yf", "zf"), (int(x) for x in n)))
+
+
+def build_geometry(cfg, download=True):
+    """The quadratic isoparametric geometry of the six sectors built on the GPU
+    (EXTRACT_VERT_COORDS_PATCH, COORDS_TO_WT_QUADPTS_FACES_ISO2,
+    COORDS_TO_WT_QUADPTS_VOLUME_ISO2, INIT_BASIS_FUNCTION_COEFS).  Returns
+    (arrays, kernel milliseconds); with download=False only the times."""
+    z = geometry_sizes(cfg)
+    ms = (ctypes.c_float * 4)()
+    if not download:
+        _check(lib().cs_build_geometry(ctypes.byref(cfg), None, ms), "cs_build_geometry")
+        return None, list(ms)
+    a = dict(vert=np.zeros(3 * z["vert"]), mask=np.zeros(z["zone"], np.int32), scale=np.zeros(z["zone"]),
+             v_pts=np.zeros(24 * z["zone"]), v_wts=np.zeros(8 * z["zone"]), v_tan=np.zeros(72 * z["zone"]),
+             v_vol=np.zeros(z["zone"]), v_cen=np.zeros(3 * z["zone"]), moments=np.zeros(4 * z["zone"]))
+    G = Geometry()
+    for k in ("vert", "mask", "scale", "v_pts", "v_wts", "v_tan", "v_vol", "v_cen", "moments"):
+        setattr(G, k, a[k].ctypes.data)
+    for f in ("xf", "yf", "zf"):
+        n = z[f]
+        a[f] = dict(pts=np.zeros(15 * n), wts=np.zeros(5 * n), normal=np.zeros((15 if f == "zf" else 3) * n),
+                    area=np.zeros(n), centroid=np.zeros(3 * n))
+        fq = getattr(G, f)
+        for k in GEOMETRY_FACE_KEYS:
+            setattr(fq, k, a[f][k].ctypes.data)
+    _check(lib().cs_build_geometry(ctypes.byref(cfg), ctypes.byref(G), ms), "cs_build_geometry")
+    return a, list(ms)
+
+
+def mode_indices(order):
+    """BUILD_MODE_INDICES: the monomial powers (i, j, k) of each mode."""
+    ii, jj, kk = (np.zeros(20, np.int32) for _ in range(3))
+    n = ctypes.c_int32()
+    _check(lib().cs_mode_indices(order, ctypes.byref(n), _p(ii), _p(jj), _p(kk)), "cs_mode_indices")
+    return [(int(ii[m]), int(jj[m]), int(kk[m])) for m in range(n.value)]

@@ -39,8 +39,10 @@ def build(verbose=False):
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-c",
           os.path.join(CSRC, "cs_mesh.cpp"), "-o", mesh_o])
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "cs_kernels.hip"), "-o", kern_o])
+    geom_o = os.path.join(BUILD, "cs_geometry.o")
+    _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "cs_geometry.hip"), "-o", geom_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", mesh_o, kern_o, "-pthread", "-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", mesh_o, kern_o, geom_o, "-pthread", "-o", tmp])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -238,6 +238,45 @@ int cs_batch_advance(cs_batch* b, int32_t nsteps, int32_t use_graph);
 int cs_batch_time_steps(cs_batch* b, int32_t nsteps, float* ms_total);
 int cs_batch_autotune(cs_batch* b, int32_t iters, int32_t* block_sizes);
 
+/* ---------------------------------------------------------------- P1 geometry
+ * The quadratic isoparametric zone and face geometry the reference's P1
+ * reconstruction reads, built on the GPU for the six sectors (one patch each)
+ * and returned in the reference's per-patch Fortran layouts, sector slowest:
+ *   vert    (0-g:N+g, 0-g:N+g, 0-g:nr+g, 3)     mask  (1-g:N+g)^2 x (1-g:nr+g)
+ *   zones   (1-g:N+g)^2 x (1-g:nr+g) x trailing dims
+ *   x-faces (0-g:N+g, 1-g:N+g, 1-g:nr+g) x trailing dims
+ *   y-faces (1-g:N+g, 0-g:N+g, 1-g:nr+g) x trailing dims
+ *   z-faces (1-g:N+g, 1-g:N+g, 0-g:nr+g) x trailing dims
+ * Any null pointer skips that output. */
+typedef struct cs_face_quad {
+  double* pts;      /* (face, 3, 5) quadrature points                      */
+  double* wts;      /* (face, 5) weights divided by the area               */
+  double* normal;   /* x, y: (face, 3) at the centre; z: (face, 3, 5)       */
+  double* area;     /* (face)                                              */
+  double* centroid; /* (face, 3)                                           */
+} cs_face_quad;
+typedef struct cs_geometry {
+  double* vert;     /* EXTRACT_VERT_COORDS_PATCH (210extract_vert_coords_patch.f:7) */
+  int32_t* mask;
+  cs_face_quad xf, yf, zf;  /* COORDS_TO_WT_QUADPTS_FACES_ISO2 (220coords_to_wt_quadpts_faces_iso2.f:7) */
+  double* scale;            /* v_scale_length, same routine (:2212-2290)            */
+  double* v_pts;            /* COORDS_TO_WT_QUADPTS_VOLUME_ISO2 (240coords_to_wt_quadpts_volume_iso2.f:7): (zone, 3, 8) */
+  double* v_wts;            /* (zone, 8)                                            */
+  double* v_tan;            /* (zone, 3, 3, 8): d x_c / d xi_q at each point         */
+  double* v_vol;            /* (zone)                                               */
+  double* v_cen;            /* (zone, 3)                                            */
+  double* moments;          /* INIT_BASIS_FUNCTION_COEFS (250init_basis_function_coefs.f:5): (zone, 4) */
+} cs_geometry;
+/* element counts of the six sectors: vertices, zones, x-, y-, z-faces */
+int cs_geometry_sizes(const cs_config* cfg, int64_t* sizes5);
+/* builds every output on the GPU and copies the requested ones back;
+ * kernel_ms (optional, 4 floats): vertices + mask, faces + scale length,
+ * volume quadrature, basis moments */
+int cs_build_geometry(const cs_config* cfg, const cs_geometry* out, float* kernel_ms);
+/* BUILD_MODE_INDICES (204build_mode_indices.f:7): monomial powers of the
+ * modes of a reconstruction of the given order (1-4) */
+int cs_mode_indices(int32_t order, int32_t* n_modes, int32_t* ii, int32_t* jj, int32_t* kk);
+
 int cs_last_error(char* buf, size_t len);
 int cs_device_info(char* buf, size_t len);
 

@@ -1570,3 +1570,382 @@ void or_advance(const or_grid* G, int nsteps, double* U, double* ent, double* Bx
   }
   for (q = 0; q < 5; ++q) free(B[q]);
 }
+
+/* ------------------------------------------------------------- quadrature geometry
+ * The quadratic isoparametric zone and face geometry of the reference's
+ * P1 reconstruction, for the six sectors (one patch each).  Layouts are the
+ * reference's per-patch Fortran arrays with the sector as the slowest index:
+ *   vert  (0-g:N+g, 0-g:N+g, 0-g:nr+g, 3)          mask (1-g:N+g)^2 x (1-g:nr+g)
+ *   zone arrays (1-g:N+g)^2 x (1-g:nr+g) x trailing dims
+ *   x-faces (0-g:N+g, 1-g:N+g, 1-g:nr+g), y-faces (1-g:N+g, 0-g:N+g, 1-g:nr+g),
+ *   z-faces (1-g:N+g, 1-g:N+g, 0-g:nr+g), each x trailing dims.
+ * The element maps are the 20-node (zones) and 8-node (faces) serendipity
+ * elements on [-1/2, 1/2]^d, written generically from their node tables. */
+#define SMALLNUM 1.0e-12
+
+typedef struct { int N, NR, g, nv, nvr, nc, nrc; size_t vs, zs, xs, ys, zfs; } geo_dims;
+static geo_dims geo_dims_of(const or_config* c) {
+  geo_dims d;
+  d.N = c->ntp; d.NR = c->nr; d.g = c->nghost;
+  d.nv = d.N + 2 * d.g + 1; d.nvr = d.NR + 2 * d.g + 1;
+  d.nc = d.N + 2 * d.g; d.nrc = d.NR + 2 * d.g;
+  d.vs = (size_t)d.nv * d.nv * d.nvr;          /* vertices per sector (one component) */
+  d.zs = (size_t)d.nc * d.nc * d.nrc;          /* zones per sector */
+  d.xs = (size_t)d.nv * d.nc * d.nrc;          /* x-faces per sector */
+  d.ys = d.xs;
+  d.zfs = (size_t)d.nc * d.nc * d.nvr;         /* z-faces per sector */
+  return d;
+}
+static const double* vtx(const geo_dims* d, const double* vert, int s, int i, int j, int k, double* out) {
+  size_t o = (size_t)(i + d->g) + (size_t)d->nv * ((j + d->g) + (size_t)d->nv * (k + d->g));
+  int c;
+  for (c = 0; c < 3; ++c) out[c] = vert[(size_t)(3 * s + c) * d->vs + o];
+  return out;
+}
+static double norm3(const double* v) { return sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+/* midpoint of two vertices, optionally pushed onto the sphere of radius r */
+static void midpoint(const double* a, const double* b, int project, double r, double* m) {
+  int c;
+  for (c = 0; c < 3; ++c) m[c] = 0.5 * (a[c] + b[c]);
+  if (project) {
+    double t = r / fmax(SMALLNUM, norm3(m));
+    for (c = 0; c < 3; ++c) m[c] = m[c] * t;
+  }
+}
+
+/* EXTRACT_VERT_COORDS_PATCH (210extract_vert_coords_patch.f:7, :285-343):
+ * vert(i, j, k) = rb(k) u(i, j); a zone is inactive when one of its four
+ * lateral vertices is the zero of a tri-corner ghost block. */
+void or_vert_coords(const or_config* c, double* vert, int32_t* mask) {
+  geo_dims d = geo_dims_of(c);
+  int N = d.N, g = d.g, nl = N + 2 * g + 1, s, i, j, k, q;
+  double* lat = malloc(sizeof(double) * (size_t)nl * nl * 18);
+  double* rb = malloc(sizeof(double) * (size_t)d.nvr);
+  or_unit_lattice(N, g, lat);
+  or_radial_mesh(c, rb);
+#define LAT(ii, jj, cc, ss) lat[(size_t)((ii) + g) + (size_t)nl * (((jj) + g) + (size_t)nl * ((cc) + 3 * (size_t)(ss)))]
+  for (s = 0; s < 6; ++s) {
+    for (k = -g; k <= d.NR + g; ++k)
+      for (j = -g; j <= N + g; ++j)
+        for (i = -g; i <= N + g; ++i)
+          for (q = 0; q < 3; ++q)
+            vert[(size_t)(3 * s + q) * d.vs + (size_t)(i + g) + (size_t)d.nv * ((j + g) + (size_t)d.nv * (k + g))] =
+                rb[k + g] * LAT(i, j, q, s);
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        int on = 1, a;
+        for (a = 0; a < 4; ++a) {
+          int ii = i - (a & 1), jj = j - (a >> 1);
+          double u[3] = {LAT(ii, jj, 0, s), LAT(ii, jj, 1, s), LAT(ii, jj, 2, s)};
+          if (norm3(u) < SMALLNUM) on = 0;
+        }
+        for (k = 1 - g; k <= d.NR + g; ++k)
+          mask[(size_t)s * d.zs + (size_t)(i - 1 + g) + (size_t)d.nc * ((j - 1 + g) + (size_t)d.nc * (k - 1 + g))] = on;
+      }
+  }
+#undef LAT
+  free(lat);
+  free(rb);
+}
+
+/* 8-node serendipity quadrilateral on [-1/2,1/2]^2: corner nodes (sx, sy)
+ * first (--, +-, ++, -+), then the edge nodes (0,-), (+,0), (0,+), (-,0) */
+static const int kQ8[8][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}, {0, -1}, {1, 0}, {0, 1}, {-1, 0}};
+static void q8_basis(double x, double y, double* N8, double* dx, double* dy) {
+  int n;
+  for (n = 0; n < 8; ++n) {
+    double sx = kQ8[n][0], sy = kQ8[n][1];
+    if (n < 4) {
+      double a = 0.5 + sx * x, b = 0.5 + sy * y, e = 2.0 * sx * x + 2.0 * sy * y - 1.0;
+      N8[n] = a * b * e;
+      dx[n] = sx * b * e + a * b * 2.0 * sx;
+      dy[n] = sy * a * e + a * b * 2.0 * sy;
+    } else if (sx == 0) {  /* edge node along x at y = sy / 2 */
+      double b = 0.5 + sy * y;
+      N8[n] = (1.0 - 4.0 * x * x) * b;
+      dx[n] = -8.0 * x * b;
+      dy[n] = (1.0 - 4.0 * x * x) * sy;
+    } else {               /* edge node along y at x = sx / 2 */
+      double a = 0.5 + sx * x;
+      N8[n] = a * (1.0 - 4.0 * y * y);
+      dx[n] = sx * (1.0 - 4.0 * y * y);
+      dy[n] = -8.0 * y * a;
+    }
+  }
+}
+/* 5-point face rule (220coords_to_wt_quadpts_faces_iso2.f:212-226): centre 1/3, edge midpoints 1/6 */
+static const double kFq[5][3] = {{0.0, 0.0, 1.0 / 3.0}, {0.5, 0.0, 1.0 / 6.0}, {-0.5, 0.0, 1.0 / 6.0},
+                                 {0.0, 0.5, 1.0 / 6.0}, {0.0, -0.5, 1.0 / 6.0}};
+
+/* One face from its eight nodes: quadrature points, weights / area, normal(s),
+ * area, centroid.  nall: normals at all five points (curved z-faces). */
+static void face_quad(double X[8][3], double* pts, double* wts, double* nrm, double* area, double* cen,
+                      int nall, size_t st) {
+  int p, n, c;
+  double C[5][3], a = 0.0;
+  for (p = 0; p < 5; ++p) {
+    double N8[8], dx[8], dy[8], A[3], B[3], cm;
+    q8_basis(kFq[p][0], kFq[p][1], N8, dx, dy);
+    for (c = 0; c < 3; ++c) {
+      double x = 0.0, ax = 0.0, bx = 0.0;
+      for (n = 0; n < 8; ++n) { x = x + X[n][c] * N8[n]; ax = ax + X[n][c] * dx[n]; bx = bx + X[n][c] * dy[n]; }
+      pts[(c + 3 * p) * st] = x;
+      A[c] = ax;
+      B[c] = bx;
+    }
+    C[p][0] = A[1] * B[2] - A[2] * B[1];
+    C[p][1] = A[2] * B[0] - A[0] * B[2];
+    C[p][2] = A[0] * B[1] - A[1] * B[0];
+    cm = sqrt(C[p][0] * C[p][0] + C[p][1] * C[p][1] + C[p][2] * C[p][2]);
+    wts[p * st] = kFq[p][2] * cm;
+    if (p == 0 || nall) {
+      double inv = 1.0 / fmax(cm, SMALLNUM);
+      for (c = 0; c < 3; ++c) nrm[(c + 3 * (nall ? p : 0)) * st] = inv * C[p][c];
+    }
+  }
+  for (p = 0; p < 5; ++p) a = a + wts[p * st];
+  a = fmax(SMALLNUM, a);
+  *area = a;
+  for (p = 0; p < 5; ++p) wts[p * st] = wts[p * st] / a;
+  for (c = 0; c < 3; ++c) {
+    double s = 0.0;
+    for (p = 0; p < 5; ++p) s = s + wts[p * st] * pts[(c + 3 * p) * st];
+    cen[c * st] = s;
+  }
+}
+
+/* COORDS_TO_WT_QUADPTS_FACES_ISO2 (220coords_to_wt_quadpts_faces_iso2.f:7):
+ * x-faces (:178-348), y-faces (:426-599), z-faces (:677-878, centroid
+ * projected back onto the sphere), scale length (:2212-2290).
+ * xf/yf/zf = {pts, wts, normal, area, centroid}. */
+void or_quad_faces(const or_config* c, const double* vert, double* const* xf, double* const* yf,
+                   double* const* zf, double* scl) {
+  geo_dims d = geo_dims_of(c);
+  int N = d.N, NR = d.NR, g = d.g, s, i, j, k, n, q;
+  for (s = 0; s < 6; ++s) {
+    /* x-faces: nodes v(i, j-1, k-1), v(i, j, k-1), v(i, j, k), v(i, j-1, k);
+       the edges along j lie on the spheres of their radius, those along k are straight */
+    for (k = 1 - g; k <= NR + g; ++k)
+      for (j = 1 - g; j <= N + g; ++j)
+        for (i = -g; i <= N + g; ++i) {
+          double X[8][3];
+          size_t o = (size_t)(i + g) + (size_t)d.nv * ((j - 1 + g) + (size_t)d.nc * (k - 1 + g));
+          size_t S = d.xs, b = (size_t)s * S;
+          vtx(&d, vert, s, i, j - 1, k - 1, X[0]); vtx(&d, vert, s, i, j, k - 1, X[1]);
+          vtx(&d, vert, s, i, j, k, X[2]); vtx(&d, vert, s, i, j - 1, k, X[3]);
+          midpoint(X[0], X[1], 1, norm3(X[0]), X[4]);
+          midpoint(X[1], X[2], 0, 0.0, X[5]);
+          midpoint(X[2], X[3], 1, norm3(X[2]), X[6]);
+          midpoint(X[3], X[0], 0, 0.0, X[7]);
+          face_quad(X, xf[0] + 15 * b + o, xf[1] + 5 * b + o, xf[2] + 3 * b + o, xf[3] + b + o,
+                    xf[4] + 3 * b + o, 0, S);
+        }
+    /* y-faces: nodes v(i-1, j, k-1), v(i-1, j, k), v(i, j, k), v(i, j, k-1) */
+    for (k = 1 - g; k <= NR + g; ++k)
+      for (j = -g; j <= N + g; ++j)
+        for (i = 1 - g; i <= N + g; ++i) {
+          double X[8][3];
+          size_t o = (size_t)(i - 1 + g) + (size_t)d.nc * ((j + g) + (size_t)d.nv * (k - 1 + g));
+          size_t S = d.ys, b = (size_t)s * S;
+          vtx(&d, vert, s, i - 1, j, k - 1, X[0]); vtx(&d, vert, s, i - 1, j, k, X[1]);
+          vtx(&d, vert, s, i, j, k, X[2]); vtx(&d, vert, s, i, j, k - 1, X[3]);
+          midpoint(X[0], X[1], 0, 0.0, X[4]);
+          midpoint(X[1], X[2], 1, norm3(X[2]), X[5]);
+          midpoint(X[2], X[3], 0, 0.0, X[6]);
+          midpoint(X[3], X[0], 1, norm3(X[0]), X[7]);
+          face_quad(X, yf[0] + 15 * b + o, yf[1] + 5 * b + o, yf[2] + 3 * b + o, yf[3] + b + o,
+                    yf[4] + 3 * b + o, 0, S);
+        }
+    /* z-faces: nodes v(i-1, j-1, k), v(i, j-1, k), v(i, j, k), v(i-1, j, k), all
+       edges on the sphere; normals at every point; the centroid is put back
+       on the sphere of the face */
+    for (k = -g; k <= NR + g; ++k)
+      for (j = 1 - g; j <= N + g; ++j)
+        for (i = 1 - g; i <= N + g; ++i) {
+          double X[8][3], r0, r1, r2;
+          size_t o = (size_t)(i - 1 + g) + (size_t)d.nc * ((j - 1 + g) + (size_t)d.nc * (k + g));
+          size_t S = d.zfs, b = (size_t)s * S;
+          vtx(&d, vert, s, i - 1, j - 1, k, X[0]); vtx(&d, vert, s, i, j - 1, k, X[1]);
+          vtx(&d, vert, s, i, j, k, X[2]); vtx(&d, vert, s, i - 1, j, k, X[3]);
+          r0 = norm3(X[0]);
+          for (n = 0; n < 4; ++n) midpoint(X[n], X[(n + 1) & 3], 1, r0, X[4 + n]);
+          face_quad(X, zf[0] + 15 * b + o, zf[1] + 5 * b + o, zf[2] + 15 * b + o, zf[3] + b + o,
+                    zf[4] + 3 * b + o, 1, S);
+          {
+            double* cn = zf[4] + 3 * b + o;
+            double cv[3] = {cn[0], cn[S], cn[2 * S]};
+            r1 = fmax(SMALLNUM, norm3(cv));
+            r2 = norm3(X[2]);
+            for (q = 0; q < 3; ++q) cn[q * S] = r2 * cv[q] / r1;
+          }
+        }
+    /* scale length: the radial edge, then the shorter lateral edges */
+    for (k = 1 - g; k <= NR + g; ++k)
+      for (j = 1 - g; j <= N + g; ++j)
+        for (i = 1 - g; i <= N + g; ++i) {
+          double a[3], b[3], e[4][3], L;
+          vtx(&d, vert, s, i, j, k, a);
+          vtx(&d, vert, s, i, j, k - 1, b);
+          L = sqrt((a[0] - b[0]) * (a[0] - b[0]) + (a[1] - b[1]) * (a[1] - b[1]) + (a[2] - b[2]) * (a[2] - b[2]));
+          vtx(&d, vert, s, i - 1, j, k - 1, e[0]); vtx(&d, vert, s, i, j, k - 1, e[1]);
+          vtx(&d, vert, s, i - 1, j - 1, k - 1, e[2]); vtx(&d, vert, s, i, j - 1, k - 1, e[3]);
+          for (n = 0; n < 2; ++n) {
+            double* p0 = e[2 * n], * p1 = e[2 * n + 1];
+            double t = sqrt((p0[0] - p1[0]) * (p0[0] - p1[0]) + (p0[1] - p1[1]) * (p0[1] - p1[1]) +
+                            (p0[2] - p1[2]) * (p0[2] - p1[2]));
+            L = fmin(L, t);
+          }
+          vtx(&d, vert, s, i, j - 1, k - 1, e[0]); vtx(&d, vert, s, i, j, k - 1, e[1]);
+          vtx(&d, vert, s, i - 1, j - 1, k - 1, e[2]); vtx(&d, vert, s, i - 1, j, k - 1, e[3]);
+          for (n = 0; n < 2; ++n) {
+            double* p0 = e[2 * n], * p1 = e[2 * n + 1];
+            double t = sqrt((p0[0] - p1[0]) * (p0[0] - p1[0]) + (p0[1] - p1[1]) * (p0[1] - p1[1]) +
+                            (p0[2] - p1[2]) * (p0[2] - p1[2]));
+            L = fmin(L, t);
+          }
+          scl[(size_t)s * d.zs + (size_t)(i - 1 + g) + (size_t)d.nc * ((j - 1 + g) + (size_t)d.nc * (k - 1 + g))] = L;
+        }
+  }
+}
+
+/* 20-node serendipity hexahedron on [-1/2,1/2]^3: corners (x fastest within
+ * a layer, lower layer first, counter-clockwise), then the edge nodes along x,
+ * along y and along z (coordinate 0 on the edge's own axis) */
+static const int kH20[20][3] = {
+    {-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1},
+    {0, -1, -1}, {0, 1, -1}, {0, 1, 1}, {0, -1, 1},
+    {-1, 0, -1}, {-1, 0, 1}, {1, 0, 1}, {1, 0, -1},
+    {-1, -1, 0}, {1, -1, 0}, {1, 1, 0}, {-1, 1, 0}};
+static void h20_basis(const double* x, double* N, double (*D)[3]) {
+  int n, a;
+  for (n = 0; n < 20; ++n) {
+    double sg[3] = {kH20[n][0], kH20[n][1], kH20[n][2]}, l[3], dl[3];
+    int e = -1;
+    for (a = 0; a < 3; ++a)
+      if (sg[a] == 0.0) e = a;
+    for (a = 0; a < 3; ++a) {
+      if (a == e) { l[a] = 1.0 - 4.0 * x[a] * x[a]; dl[a] = -8.0 * x[a]; }
+      else { l[a] = 0.5 + sg[a] * x[a]; dl[a] = sg[a]; }
+    }
+    if (e < 0) {
+      double t = 2.0 * sg[0] * x[0] + 2.0 * sg[1] * x[1] + 2.0 * sg[2] * x[2] - 2.0;
+      N[n] = l[0] * l[1] * l[2] * t;
+      for (a = 0; a < 3; ++a) {
+        double p = 1.0;
+        int b;
+        for (b = 0; b < 3; ++b) if (b != a) p *= l[b];
+        D[n][a] = dl[a] * p * t + l[0] * l[1] * l[2] * 2.0 * sg[a];
+      }
+    } else {
+      N[n] = l[0] * l[1] * l[2];
+      for (a = 0; a < 3; ++a) {
+        double p = 1.0;
+        int b;
+        for (b = 0; b < 3; ++b) if (b != a) p *= l[b];
+        D[n][a] = dl[a] * p;
+      }
+    }
+  }
+}
+
+/* COORDS_TO_WT_QUADPTS_VOLUME_ISO2 (240coords_to_wt_quadpts_volume_iso2.f:7):
+ * 2x2x2 Gauss points (+-1/(2 sqrt 3)), quadrature points, weights Det/8
+ * normalised by the volume, the three tangent vectors d x/d(xi, eta, zeta)
+ * at each point, volume, and the centroid (vertex mean + weighted offsets). */
+void or_quad_volume(const or_config* c, const double* vert, double* pts, double* wts, double* tan,
+                    double* vol, double* cen) {
+  geo_dims d = geo_dims_of(c);
+  int N = d.N, NR = d.NR, g = d.g, s, i, j, k, p, n, q;
+  const double xg = 0.5 * sqrt(1.0 / 3.0);
+  double Nb[8][20], Db[8][20][3];
+  for (p = 0; p < 8; ++p) {
+    double x[3] = {(p & 1) ? xg : -xg, (p & 2) ? xg : -xg, (p & 4) ? xg : -xg};
+    h20_basis(x, Nb[p], Db[p]);
+  }
+  for (s = 0; s < 6; ++s)
+    for (k = 1 - g; k <= NR + g; ++k)
+      for (j = 1 - g; j <= N + g; ++j)
+        for (i = 1 - g; i <= N + g; ++i) {
+          double X[20][3], r1, r2, v = 0.0, c0[3], tmp[3] = {0.0, 0.0, 0.0};
+          size_t S = d.zs, b = (size_t)s * S;
+          size_t o = (size_t)(i - 1 + g) + (size_t)d.nc * ((j - 1 + g) + (size_t)d.nc * (k - 1 + g));
+          for (n = 0; n < 8; ++n)
+            vtx(&d, vert, s, i - 1 + ((n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0),
+                j - 1 + ((n == 2 || n == 3 || n == 6 || n == 7) ? 1 : 0), k - 1 + (n >= 4 ? 1 : 0), X[n]);
+          r1 = norm3(X[0]);
+          r2 = norm3(X[4]);
+          midpoint(X[0], X[1], 1, r1, X[8]);  midpoint(X[2], X[3], 1, r1, X[9]);
+          midpoint(X[6], X[7], 1, r2, X[10]); midpoint(X[4], X[5], 1, r2, X[11]);
+          midpoint(X[0], X[3], 1, r1, X[12]); midpoint(X[4], X[7], 1, r2, X[13]);
+          midpoint(X[5], X[6], 1, r2, X[14]); midpoint(X[1], X[2], 1, r1, X[15]);
+          midpoint(X[0], X[4], 0, 0.0, X[16]); midpoint(X[1], X[5], 0, 0.0, X[17]);
+          midpoint(X[2], X[6], 0, 0.0, X[18]); midpoint(X[3], X[7], 0, 0.0, X[19]);
+          for (p = 0; p < 8; ++p) {
+            double T[3][3], Dx, Dy, Dz, det;
+            for (q = 0; q < 3; ++q) {
+              double x = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
+              for (n = 0; n < 20; ++n) {
+                x = x + X[n][q] * Nb[p][n];
+                t0 = t0 + X[n][q] * Db[p][n][0];
+                t1 = t1 + X[n][q] * Db[p][n][1];
+                t2 = t2 + X[n][q] * Db[p][n][2];
+              }
+              pts[b * 24 + o + (q + 3 * p) * S] = x;
+              T[0][q] = t0; T[1][q] = t1; T[2][q] = t2;
+            }
+            Dx = T[0][1] * T[1][2] - T[0][2] * T[1][1];
+            Dy = T[0][2] * T[1][0] - T[0][0] * T[1][2];
+            Dz = T[0][0] * T[1][1] - T[0][1] * T[1][0];
+            det = fabs(T[2][0] * Dx + T[2][1] * Dy + T[2][2] * Dz);
+            wts[b * 8 + o + p * S] = 0.125 * det;
+            for (q = 0; q < 3; ++q)
+              for (n = 0; n < 3; ++n) tan[b * 72 + o + (q + 3 * (n + 3 * p)) * S] = T[n][q];
+          }
+          for (p = 0; p < 8; ++p) v = v + wts[b * 8 + o + p * S];
+          v = fmax(SMALLNUM, v);
+          vol[b + o] = v;
+          for (p = 0; p < 8; ++p) wts[b * 8 + o + p * S] = wts[b * 8 + o + p * S] / v;
+          for (q = 0; q < 3; ++q)
+            c0[q] = 0.125 * (X[0][q] + X[1][q] + X[2][q] + X[3][q] + X[4][q] + X[5][q] + X[6][q] + X[7][q]);
+          for (p = 0; p < 8; ++p)
+            for (q = 0; q < 3; ++q) tmp[q] = tmp[q] + wts[b * 8 + o + p * S] * (pts[b * 24 + o + (q + 3 * p) * S] - c0[q]);
+          for (q = 0; q < 3; ++q) cen[b * 3 + o + q * S] = c0[q] + tmp[q];
+        }
+}
+
+/* BUILD_MODE_INDICES (204build_mode_indices.f:7) at second order and
+ * INIT_BASIS_FUNCTION_COEFS (250init_basis_function_coefs.f:5, :150-267):
+ * moments <x^a y^b z^c> of the zone's quadrature points about the centroid in
+ * units of the scale length, mode 1 zero; inactive zones keep zeros. */
+void or_mode_indices(int32_t* ii, int32_t* jj, int32_t* kk) {
+  int m;
+  for (m = 0; m < 4; ++m) { ii[m] = m == 1; jj[m] = m == 2; kk[m] = m == 3; }
+}
+void or_basis_moments(const or_config* c, const int32_t* mask, const double* pts, const double* wts,
+                      const double* cen, const double* scl, double* coef) {
+  geo_dims d = geo_dims_of(c);
+  int32_t ii[4], jj[4], kk[4];
+  size_t S = d.zs, z, s;
+  int m, p, q;
+  or_mode_indices(ii, jj, kk);
+  memset(coef, 0, sizeof(double) * S * 6 * 4);
+  for (s = 0; s < 6; ++s)
+    for (z = 0; z < S; ++z) {
+      size_t o = s * S + z;
+      double aa;
+      if (mask[o] != 1) continue;
+      aa = 1.0 / fmax(SMALLNUM, fabs(scl[o]));
+      for (m = 1; m < 4; ++m) {
+        double t = 0.0;
+        for (p = 0; p < 8; ++p) {
+          double x[3], v = 1.0;
+          for (q = 0; q < 3; ++q) x[q] = (pts[s * S * 24 + z + (q + 3 * p) * S] - cen[s * S * 3 + z + q * S]) * aa;
+          if (ii[m]) v *= x[0];
+          if (jj[m]) v *= x[1];
+          if (kk[m]) v *= x[2];
+          t = t + wts[s * S * 8 + z + p * S] * v;
+        }
+        coef[s * S * 4 + z + m * S] = coef[s * S * 4 + z + m * S] + t;
+      }
+    }
+}

@@ -73,4 +73,15 @@ void or_imex_pre(double dt, double gr, const double* n, double* u);
 void or_imex_combine(int stage, double dt, double gr, const double* n, const double* ui, const double* uz,
                      const double* L, double* u);
 
+/* quadratic isoparametric geometry of the P1 reconstruction (210, 220, 240,
+ * 204/250), six sectors; layouts in cs_oracle.c */
+void or_vert_coords(const or_config* c, double* vert, int32_t* mask);
+void or_quad_faces(const or_config* c, const double* vert, double* const* xf, double* const* yf,
+                   double* const* zf, double* scl);
+void or_quad_volume(const or_config* c, const double* vert, double* pts, double* wts, double* tan,
+                    double* vol, double* cen);
+void or_mode_indices(int32_t* ii, int32_t* jj, int32_t* kk);
+void or_basis_moments(const or_config* c, const int32_t* mask, const double* pts, const double* wts,
+                      const double* cen, const double* scl, double* coef);
+
 #endif

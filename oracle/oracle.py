@@ -140,3 +140,68 @@ class Grid:
         xa, ya, za = (np.zeros(6 * self.fpp) for _ in range(3))
         olib().or_metric(self.g, P(vol), P(xa), P(ya), P(za))
         return vol, xa, ya, za
+
+
+# ------------------------------------------------------------------ quadrature geometry
+GEOM_FACE_KEYS = ("pts", "wts", "normal", "area", "centroid")
+
+
+def geometry_dims(cfg):
+    """Per-sector element counts of the quadrature-geometry arrays (see
+    cs_oracle.c): vertices, zones, x-faces, y-faces, z-faces."""
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    nv, nvr, nc, nrc = N + 2 * g + 1, NR + 2 * g + 1, N + 2 * g, NR + 2 * g
+    return dict(vert=nv * nv * nvr, zone=nc * nc * nrc, xf=nv * nc * nrc, yf=nc * nv * nrc, zf=nc * nc * nvr)
+
+
+def geometry_arrays(cfg):
+    d = geometry_dims(cfg)
+    a = dict(vert=np.zeros(6 * 3 * d["vert"]), mask=np.zeros(6 * d["zone"], np.int32),
+             scale=np.zeros(6 * d["zone"]), v_pts=np.zeros(6 * 24 * d["zone"]), v_wts=np.zeros(6 * 8 * d["zone"]),
+             v_tan=np.zeros(6 * 72 * d["zone"]), v_vol=np.zeros(6 * d["zone"]), v_cen=np.zeros(6 * 3 * d["zone"]),
+             moments=np.zeros(6 * 4 * d["zone"]))
+    for f in ("xf", "yf", "zf"):
+        n = d[f]
+        a[f] = dict(pts=np.zeros(6 * 15 * n), wts=np.zeros(6 * 5 * n),
+                    normal=np.zeros(6 * (15 if f == "zf" else 3) * n), area=np.zeros(6 * n),
+                    centroid=np.zeros(6 * 3 * n))
+    return a
+
+
+def quad_geometry(cfg, use_reference=False):
+    """The vertex coordinates, active mask, face and volume quadrature, scale
+    length and basis moments of every sector, from the oracle or (when built)
+    from the reference's own routines."""
+    a = geometry_arrays(cfg)
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    vp = ctypes.c_void_p
+    fptr = lambda f: [P(a[f][k]) for k in GEOM_FACE_KEYS]
+    if use_reference:
+        R = rlib()
+        if R is None:
+            raise RuntimeError("reference build not present")
+        lat = np.zeros((N + 2 * g + 1) ** 2 * 18)
+        R.ref_unit_lattice(N, g, P(lat))
+        rb = np.zeros(NR + 2 * g + 1)
+        R.ref_radial_mesh(NR, g, ctypes.c_double(cfg.rmin), ctypes.c_double(cfg.rmax), 1 if cfg.radial_mesh == 1 else 0,
+                          P(rb))
+        R.ref_vert_coords(N, NR, g, P(lat), P(rb), P(a["vert"]), P(a["mask"]))
+        R.ref_quad_faces(N, NR, g, P(a["vert"]), P(a["mask"]), *fptr("xf"), *fptr("yf"), *fptr("zf"), P(a["scale"]))
+        R.ref_quad_volume(N, NR, g, P(a["vert"]), P(a["mask"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_tan"]),
+                          P(a["v_vol"]), P(a["v_cen"]))
+        R.ref_basis_moments(N, NR, g, P(a["mask"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_vol"]), P(a["v_cen"]),
+                            P(a["scale"]), P(a["moments"]))
+        return a
+    L = olib()
+    L.or_vert_coords.argtypes = [vp, vp, vp]
+    L.or_quad_faces.argtypes = [vp, vp, vp, vp, vp, vp]
+    L.or_quad_volume.argtypes = [vp] * 7
+    L.or_basis_moments.argtypes = [vp] * 7
+    L.or_vert_coords(ctypes.byref(cfg), P(a["vert"]), P(a["mask"]))
+    keep = [ptrs([a[f][k] for k in GEOM_FACE_KEYS]) for f in ("xf", "yf", "zf")]
+    L.or_quad_faces(ctypes.byref(cfg), P(a["vert"]), keep[0][0], keep[1][0], keep[2][0], P(a["scale"]))
+    L.or_quad_volume(ctypes.byref(cfg), P(a["vert"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_tan"]), P(a["v_vol"]),
+                     P(a["v_cen"]))
+    L.or_basis_moments(ctypes.byref(cfg), P(a["mask"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_cen"]), P(a["scale"]),
+                       P(a["moments"]))
+    return a

@@ -107,4 +107,91 @@ contains
                                 bn(1), bn(2), bn(3), out(1), out(2), out(3))
   end subroutine
 
+  ! EXTRACT_VERT_COORDS_PATCH (210extract_vert_coords_patch.f:7) for the six
+  ! sectors, one patch each: vertices r(k) * u(i, j) and the active-zone mask.
+  subroutine ref_vert_coords(n, nr, g, coords, rb, vert, mask) bind(C, name="ref_vert_coords")
+    integer(c_int), value :: n, nr, g
+    real(c_double), intent(in) :: coords(0-g:n+g, 0-g:n+g, 3, 6), rb(0-g:nr+g)
+    real(c_double), intent(out) :: vert(0-g:n+g, 0-g:n+g, 0-g:nr+g, 3, 6)
+    integer(c_int), intent(out) :: mask(1-g:n+g, 1-g:n+g, 1-g:nr+g, 6)
+    integer :: s
+    external EXTRACT_VERT_COORDS_PATCH, REF_INIT_COMMON
+    call REF_INIT_COMMON()
+    do s = 1, 6
+      call EXTRACT_VERT_COORDS_PATCH(n, nr, g, n, nr, 1, 1, g, s, 1, 1, 1, 1, 1, coords, rb, &
+                                     vert(:, :, :, :, s), mask(:, :, :, s))
+    end do
+  end subroutine
+
+  ! COORDS_TO_WT_QUADPTS_VOLUME_ISO2 (240coords_to_wt_quadpts_volume_iso2.f:7), six sectors.
+  subroutine ref_quad_volume(n, nr, g, vert, mask, pts, wts, tang, vol, cen) bind(C, name="ref_quad_volume")
+    integer(c_int), value :: n, nr, g
+    real(c_double), intent(in) :: vert(0-g:n+g, 0-g:n+g, 0-g:nr+g, 3, 6)
+    integer(c_int), intent(in) :: mask(1-g:n+g, 1-g:n+g, 1-g:nr+g, 6)
+    real(c_double), intent(out) :: pts(1-g:n+g, 1-g:n+g, 1-g:nr+g, 3, 8, 6), wts(1-g:n+g, 1-g:n+g, 1-g:nr+g, 8, 6)
+    real(c_double), intent(out) :: tang(1-g:n+g, 1-g:n+g, 1-g:nr+g, 3, 3, 8, 6)
+    real(c_double), intent(out) :: vol(1-g:n+g, 1-g:n+g, 1-g:nr+g, 6), cen(1-g:n+g, 1-g:n+g, 1-g:nr+g, 3, 6)
+    integer :: s
+    external COORDS_TO_WT_QUADPTS_VOLUME_ISO2, REF_INIT_COMMON
+    call REF_INIT_COMMON()
+    do s = 1, 6
+      call COORDS_TO_WT_QUADPTS_VOLUME_ISO2(nr, n, g, vert(:, :, :, :, s), mask(:, :, :, s), &
+           pts(:, :, :, :, :, s), wts(:, :, :, :, s), tang(:, :, :, :, :, :, s), vol(:, :, :, s), cen(:, :, :, :, s))
+    end do
+  end subroutine
+
+  ! COORDS_TO_WT_QUADPTS_FACES_ISO2 (220coords_to_wt_quadpts_faces_iso2.f:7), six sectors.
+  subroutine ref_quad_faces(n, nr, g, vert, mask, xp, xw, xn, xa, xc, yp, yw, yn, ya, yc, &
+                            zp, zw, zn, za, zc, scl) bind(C, name="ref_quad_faces")
+    integer(c_int), value :: n, nr, g
+    real(c_double), intent(in) :: vert(0-g:n+g, 0-g:n+g, 0-g:nr+g, 3, 6)
+    integer(c_int), intent(in) :: mask(1-g:n+g, 1-g:n+g, 1-g:nr+g, 6)
+    real(c_double), intent(out) :: xp(0-g:n+g, 1-g:n+g, 1-g:nr+g, 3, 5, 6), xw(0-g:n+g, 1-g:n+g, 1-g:nr+g, 5, 6)
+    real(c_double), intent(out) :: xn(0-g:n+g, 1-g:n+g, 1-g:nr+g, 3, 6), xa(0-g:n+g, 1-g:n+g, 1-g:nr+g, 6)
+    real(c_double), intent(out) :: xc(0-g:n+g, 1-g:n+g, 1-g:nr+g, 3, 6)
+    real(c_double), intent(out) :: yp(1-g:n+g, 0-g:n+g, 1-g:nr+g, 3, 5, 6), yw(1-g:n+g, 0-g:n+g, 1-g:nr+g, 5, 6)
+    real(c_double), intent(out) :: yn(1-g:n+g, 0-g:n+g, 1-g:nr+g, 3, 6), ya(1-g:n+g, 0-g:n+g, 1-g:nr+g, 6)
+    real(c_double), intent(out) :: yc(1-g:n+g, 0-g:n+g, 1-g:nr+g, 3, 6)
+    real(c_double), intent(out) :: zp(1-g:n+g, 1-g:n+g, 0-g:nr+g, 3, 5, 6), zw(1-g:n+g, 1-g:n+g, 0-g:nr+g, 5, 6)
+    real(c_double), intent(out) :: zn(1-g:n+g, 1-g:n+g, 0-g:nr+g, 3, 5, 6), za(1-g:n+g, 1-g:n+g, 0-g:nr+g, 6)
+    real(c_double), intent(out) :: zc(1-g:n+g, 1-g:n+g, 0-g:nr+g, 3, 6)
+    real(c_double), intent(out) :: scl(1-g:n+g, 1-g:n+g, 1-g:nr+g, 6)
+    integer :: s
+    external COORDS_TO_WT_QUADPTS_FACES_ISO2, REF_INIT_COMMON
+    call REF_INIT_COMMON()
+    do s = 1, 6
+      call COORDS_TO_WT_QUADPTS_FACES_ISO2(nr, n, g, vert(:, :, :, :, s), mask(:, :, :, s), &
+           xp(:, :, :, :, :, s), xw(:, :, :, :, s), xn(:, :, :, :, s), xa(:, :, :, s), xc(:, :, :, :, s), &
+           yp(:, :, :, :, :, s), yw(:, :, :, :, s), yn(:, :, :, :, s), ya(:, :, :, s), yc(:, :, :, :, s), &
+           zp(:, :, :, :, :, s), zw(:, :, :, :, s), zn(:, :, :, :, :, s), za(:, :, :, s), zc(:, :, :, :, s), &
+           scl(:, :, :, s))
+    end do
+  end subroutine
+
+  ! BUILD_MODE_INDICES (204build_mode_indices.f:7) and INIT_BASIS_FUNCTION_COEFS
+  ! (250init_basis_function_coefs.f:5), second order: four modes.
+  subroutine ref_basis_moments(n, nr, g, mask, pts, wts, vol, cen, scl, coef) bind(C, name="ref_basis_moments")
+    integer(c_int), value :: n, nr, g
+    integer(c_int), intent(in) :: mask(1-g:n+g, 1-g:n+g, 1-g:nr+g, 6)
+    real(c_double), intent(in) :: pts(1-g:n+g, 1-g:n+g, 1-g:nr+g, 3, 8, 6), wts(1-g:n+g, 1-g:n+g, 1-g:nr+g, 8, 6)
+    real(c_double), intent(in) :: vol(1-g:n+g, 1-g:n+g, 1-g:nr+g, 6), cen(1-g:n+g, 1-g:n+g, 1-g:nr+g, 3, 6)
+    real(c_double), intent(in) :: scl(1-g:n+g, 1-g:n+g, 1-g:nr+g, 6)
+    real(c_double), intent(out) :: coef(1-g:n+g, 1-g:n+g, 1-g:nr+g, 4, 6)
+    integer :: s, ii(4), jj(4), kk(4)
+    external BUILD_MODE_INDICES, INIT_BASIS_FUNCTION_COEFS, REF_INIT_COMMON
+    call REF_INIT_COMMON()
+    call BUILD_MODE_INDICES(4, ii, jj, kk)
+    do s = 1, 6
+      call INIT_BASIS_FUNCTION_COEFS(nr, n, g, 4, ii, jj, kk, mask(:, :, :, s), pts(:, :, :, :, :, s), &
+           wts(:, :, :, :, s), vol(:, :, :, s), cen(:, :, :, :, s), scl(:, :, :, s), coef(:, :, :, :, s))
+    end do
+  end subroutine
+
+  subroutine ref_mode_indices(nm, ii, jj, kk) bind(C, name="ref_mode_indices")
+    integer(c_int), value :: nm
+    integer(c_int), intent(out) :: ii(nm), jj(nm), kk(nm)
+    external BUILD_MODE_INDICES
+    call BUILD_MODE_INDICES(nm, ii, jj, kk)
+  end subroutine
+
 end module cs_ref_bind

@@ -1,0 +1,119 @@
+"""The P1 geometry (EXTRACT_VERT_COORDS_PATCH, COORDS_TO_WT_QUADPTS_FACES_ISO2,
+COORDS_TO_WT_QUADPTS_VOLUME_ISO2, BUILD_MODE_INDICES, INIT_BASIS_FUNCTION_COEFS)
+of the C oracle, pinned against the reference's own routines (flang build,
+when present) and against what the geometry fixes: the quadratic isoparametric
+zones fill the spherical shell, face weights and normals are normalised, the
+radial faces' centroids lie on their spheres, and second-order moments about
+the centroid vanish.  The reference STOPs for nr < NGhost, so it is compared
+on shells only."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import cubesphere_amd as cs
+from oracle import oracle as orc
+
+SHELLS = {
+    "shell6x3_g3": cs.make_config(6, 3, 3, radial_bc=cs.RBC_PAD, rmin=1.0, rmax=1.5),
+    "shell8x4_g3": cs.make_config(8, 4, 3, radial_bc=cs.RBC_PAD, rmin=1.0, rmax=1.2),
+    "shell5x3_g2_log": cs.make_config(5, 3, 2, radial_bc=cs.RBC_PAD, radial_mesh=cs.RMESH_LOG, rmin=1.0, rmax=2.0),
+}
+
+
+def _rel(a, b):
+    s = np.abs(b).max()
+    return float(np.abs(a - b).max() / s) if s > 0 else float(np.abs(a - b).max())
+
+
+@pytest.mark.parametrize("name", list(SHELLS))
+def test_geometry_matches_reference(name):
+    if orc.rlib() is None:
+        pytest.skip("reference build not present")
+    cfg = SHELLS[name]
+    o, r = orc.quad_geometry(cfg), orc.quad_geometry(cfg, use_reference=True)
+    # vertices, mask, scale length and the face quadrature: bit for bit
+    for k in ("vert", "mask", "scale"):
+        assert np.array_equal(o[k], r[k]), k
+    for f in ("xf", "yf", "zf"):
+        for k in orc.GEOM_FACE_KEYS:
+            assert np.array_equal(o[f][k], r[f][k]), f"{f}.{k}"
+    # volume quadrature: the points bit for bit; the Jacobians through the
+    # basis derivatives at the irrational Gauss points to rounding
+    assert np.array_equal(o["v_pts"], r["v_pts"])
+    for k in ("v_wts", "v_tan", "v_vol", "v_cen"):
+        assert _rel(o[k], r[k]) < 1e-13, k
+    # first moments about the centroid are rounding-level numbers of the
+    # weights' rounding: compared absolutely (dimensionless, scale-length units)
+    assert np.abs(o["moments"] - r["moments"]).max() < 1e-13
+
+
+def test_mode_indices():
+    """Second order against the reference build; higher orders: every
+    monomial of degree < order once, by degree (204build_mode_indices.f:40-150)."""
+    assert cs.mode_indices(2) == [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)]
+    R = orc.rlib()
+    if R is not None:
+        ii, jj, kk = (np.zeros(4, np.int32) for _ in range(3))
+        R.ref_mode_indices(4, orc.P(ii), orc.P(jj), orc.P(kk))
+        assert list(zip(ii.tolist(), jj.tolist(), kk.tolist())) == cs.mode_indices(2)
+    for order, n in ((1, 1), (2, 4), (3, 10), (4, 20)):
+        m = cs.mode_indices(order)
+        assert len(m) == n and len(set(m)) == n
+        assert all(sum(p) < order for p in m)
+        assert [sum(p) for p in m] == sorted(sum(p) for p in m)
+
+
+@pytest.mark.parametrize("name", list(SHELLS) + ["surface8"])
+def test_geometry_invariants(name):
+    cfg = SHELLS.get(name) or cs.make_config(8, 1, 3)
+    a = orc.quad_geometry(cfg)
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    nc, nrc, nv, nvr = N + 2 * g, NR + 2 * g, N + 2 * g + 1, NR + 2 * g + 1
+    rb = np.zeros(nvr)
+    orc.olib().or_radial_mesh(ctypes.byref(cfg), orc.P(rb))
+    r0, r1 = rb[g], rb[g + NR]
+    vol = a["v_vol"].reshape(6, nrc, nc, nc)[:, g:g + NR, g:g + N, g:g + N]
+    exact = 4.0 * np.pi / 3.0 * (r1 ** 3 - r0 ** 3)
+    assert abs(vol.sum() - exact) / exact < 0.1 * (np.pi / (2 * N)) ** 4
+    for f in ("xf", "yf", "zf"):
+        S = a[f]["area"].size // 6
+        w = a[f]["wts"].reshape(6, 5, S)
+        live = a[f]["area"].reshape(6, S) > 1e-11          # faces with four distinct vertices
+        assert np.allclose(w.sum(axis=1)[live], 1.0, rtol=0, atol=1e-14)
+        nrm = a[f]["normal"].reshape(6, 5 if f == "zf" else 1, 3, S)
+        ln = np.sqrt((nrm ** 2).sum(axis=2))
+        # unit normals on the faces of the sector's own columns (the
+        # tri-corner ghost faces collapse and keep shorter or zero normals)
+        assert (ln <= 1.0 + 1e-14).all()
+        if f == "xf":
+            own = ln.reshape(6, 1, nrc, nc, nv)[:, :, g:g + NR, g:g + N, g:g + N + 1]
+        elif f == "yf":
+            own = ln.reshape(6, 1, nrc, nv, nc)[:, :, g:g + NR, g:g + N + 1, g:g + N]
+        else:
+            own = ln.reshape(6, 5, nvr, nc, nc)[:, :, g:g + NR + 1, g:g + N, g:g + N]
+        assert np.allclose(own, 1.0, rtol=0, atol=1e-14)
+    # z-face centroids on their spheres
+    zc = a["zf"]["centroid"].reshape(6, 3, nvr, nc, nc)
+    rad = np.sqrt((zc ** 2).sum(axis=1))[:, g:g + NR + 1, g:g + N, g:g + N]
+    assert np.allclose(rad, rb[g:g + NR + 1][None, :, None, None], rtol=1e-14, atol=0)
+    # volume weights sum to one, first moments about the centroid vanish
+    wv = a["v_wts"].reshape(6, 8, nrc, nc, nc)[:, :, g:g + NR, g:g + N, g:g + N]
+    assert np.allclose(wv.sum(axis=1), 1.0, rtol=0, atol=1e-14)
+    m = a["moments"].reshape(6, 4, -1)
+    assert np.abs(m).max() < 1e-13
+    assert (a["scale"].reshape(6, nrc, nc, nc)[:, g:g + NR, g:g + N, g:g + N] > 0).all()
+
+
+def test_shell_volume_converges_fourth_order():
+    """The quadratic isoparametric zones fill the shell with an error falling
+    as h^4 (quadratic interpolation of the great circles)."""
+    errs = []
+    for N in (6, 12):
+        cfg = cs.make_config(N, 3, 3, radial_bc=cs.RBC_PAD, rmin=1.0, rmax=1.5)
+        a = orc.quad_geometry(cfg)
+        g, NR, nc, nrc = 3, 3, N + 6, 9
+        vol = a["v_vol"].reshape(6, nrc, nc, nc)[:, g:g + NR, g:g + N, g:g + N].sum()
+        exact = 4.0 * np.pi / 3.0 * (1.5 ** 3 - 1.0)
+        errs.append(abs(vol - exact) / exact)
+    assert 12.0 < errs[0] / errs[1] < 20.0, errs
