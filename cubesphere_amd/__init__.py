@@ -121,6 +121,7 @@ def lib():
             "cs_geometry_sizes": [cfgp, vp],
             "cs_build_geometry": [cfgp, vp, vp],
             "cs_mode_indices": [i32, ctypes.POINTER(i32), vp, vp, vp],
+            "cs_stencil_indexing": [cfgp, vp],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
             "cs_batch_shell": [vp, i32, ctypes.POINTER(vp)],
@@ -459,3 +460,12 @@ def mode_indices(order):
     n = ctypes.c_int32()
     _check(lib().cs_mode_indices(order, ctypes.byref(n), _p(ii), _p(jj), _p(kk)), "cs_mode_indices")
     return [(int(ii[m]), int(jj[m]), int(kk[m])) for m in range(n.value)]
+
+
+def stencil_indexing(cfg):
+    """STENCIL_INDEXING_TRICORNERS: int32 (12, 2, N+2g, N+2g) view of the
+    reference's istencil(1-g:N+g, 1-g:N+g, 2, 12): [dir-1, (x, y), y, x]."""
+    nc = cfg.ntp + 2 * cfg.nghost
+    a = np.zeros(nc * nc * 24, np.int32)
+    _check(lib().cs_stencil_indexing(ctypes.byref(cfg), _p(a)), "cs_stencil_indexing")
+    return a

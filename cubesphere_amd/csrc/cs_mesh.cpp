@@ -436,6 +436,66 @@ extern "C" int cs_build_unit_lattice(const cs_config* cfg, double* coords) {
   return CS_OK;
 }
 
+// STENCIL_INDEXING_TRICORNERS (211stencil_indexing_tricorners.f:6) for a
+// sector that is one patch, so all four corners are tri-corners.  Each zone's
+// twelve stencil neighbours (1 R, 2 L, 3 U, 4 D, 5 RR, 6 LL, 7 UU, 8 DD,
+// 9 RU, 10 LU, 11 LD, 12 RD) are first the index neighbours clamped to the
+// patch; around each tri-corner ghost block they are re-pointed to the zones
+// that are the neighbours on the sphere.  The four corners' remaps are one
+// pattern under the reflections of the patch: written once for the left-down
+// corner and mirrored (x -> N+1-x swaps R/L, RR/LL, RU/LU, LD/RD; y -> N+1-y
+// swaps U/D, UU/DD, RU/RD, LU/LD), corners applied in the reference's order
+// LD, RD, RU, LU (:84, 127, 173, 219).
+extern "C" int cs_stencil_indexing(const cs_config* cfg, int32_t* st) {
+  int rc = check_cfg(cfg);
+  if (rc) return rc;
+  if (!st) { set_error("null output"); return CS_E_INVALID; }
+  const int N = cfg->ntp, g = cfg->nghost, nc = N + 2 * g, lo = 1 - g, hi = N + g;
+  auto set = [&](int x, int y, int dir, int vx, int vy) {
+    const size_t o = (size_t)(x - lo) + (size_t)nc * ((size_t)(y - lo) + (size_t)nc * 2 * (size_t)(dir - 1));
+    st[o] = vx;
+    st[o + (size_t)nc * nc] = vy;
+  };
+  static const int dx[13] = {0, 1, -1, 0, 0, 2, -2, 0, 0, 1, -1, -1, 1};
+  static const int dy[13] = {0, 0, 0, 1, -1, 0, 0, 2, -2, 1, 1, -1, -1};
+  for (int y = lo; y <= hi; ++y)
+    for (int x = lo; x <= hi; ++x)
+      for (int dir = 1; dir <= 12; ++dir)
+        set(x, y, dir, std::min(std::max(x + dx[dir], lo), hi), std::min(std::max(y + dy[dir], lo), hi));
+  static const int mirror_x[13] = {0, 2, 1, 3, 4, 6, 5, 7, 8, 10, 9, 12, 11};
+  static const int mirror_y[13] = {0, 1, 2, 4, 3, 5, 6, 8, 7, 12, 11, 10, 9};
+  static const int corner[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};  // LD, RD, RU, LU
+  for (const auto& c : corner) {
+    auto fx = [&](int x) { return c[0] ? N + 1 - x : x; };
+    auto fy = [&](int y) { return c[1] ? N + 1 - y : y; };
+    auto put = [&](int x, int y, int dir, int vx, int vy) {
+      if (c[0]) dir = mirror_x[dir];
+      if (c[1]) dir = mirror_y[dir];
+      set(fx(x), fy(y), dir, fx(vx), fy(vy));
+    };
+    // left-down pattern: the ghost row below zone row 1 left of the corner
+    // wraps onto the zones left of column 1, and vice versa
+    for (int x = lo; x <= 0; ++x) {
+      put(x, 1, 4, 1, x);
+      put(x, 1, 8, 2, x);
+      put(x, 1, 11, 1, std::max(x - 1, lo));
+      if (x == 0) put(x, 1, 12, 2, x - 1);
+      else put(x, 1, 12, 1, x + 1);
+      put(x, 2, 8, 1, x);
+    }
+    for (int y = lo; y <= 0; ++y) {
+      put(1, y, 2, y, 1);
+      put(1, y, 6, y, 2);
+      if (y == 0) put(1, y, 10, y - 1, 2);
+      else put(1, y, 10, y + 1, 1);
+      put(1, y, 11, std::max(y - 1, lo), 1);
+      put(2, y, 6, y, 1);
+    }
+    put(1, 1, 11, 2, -1);
+  }
+  return CS_OK;
+}
+
 extern "C" int cs_build_radial_mesh(const cs_config* cfg, double* rb) {
   int rc = check_cfg(cfg);
   if (rc) return rc;

@@ -273,6 +273,11 @@ int cs_geometry_sizes(const cs_config* cfg, int64_t* sizes5);
  * kernel_ms (optional, 4 floats): vertices + mask, faces + scale length,
  * volume quadrature, basis moments */
 int cs_build_geometry(const cs_config* cfg, const cs_geometry* out, float* kernel_ms);
+/* STENCIL_INDEXING_TRICORNERS (211stencil_indexing_tricorners.f:6) for a
+ * sector that is one patch: istencil (1-g:N+g, 1-g:N+g, 2, 12), the zone (x, y)
+ * reached in each of the twelve directions (R, L, U, D, RR, LL, UU, DD, RU, LU,
+ * LD, RD), re-pointed around the tri-corner blocks; the same for every sector */
+int cs_stencil_indexing(const cs_config* cfg, int32_t* istencil);
 /* BUILD_MODE_INDICES (204build_mode_indices.f:7): monomial powers of the
  * modes of a reconstruction of the given order (1-4) */
 int cs_mode_indices(int32_t order, int32_t* n_modes, int32_t* ii, int32_t* jj, int32_t* kk);

@@ -1949,3 +1949,96 @@ void or_basis_moments(const or_config* c, const int32_t* mask, const double* pts
       }
     }
 }
+
+/* STENCIL_INDEXING_TRICORNERS (211stencil_indexing_tricorners.f:6) for a
+ * sector that is one patch (all four corners are tri-corners):
+ * st(1-g:N+g, 1-g:N+g, 2, 12), the (x, y) zone each of the twelve stencil
+ * directions reaches (1 R, 2 L, 3 U, 4 D, 5 RR, 6 LL, 7 UU, 8 DD, 9 RU, 10 LU,
+ * 11 LD, 12 RD), clamped to the patch, then re-pointed around the tri-corner
+ * blocks (:84-271, each corner's assignments in the reference's order). */
+void or_stencil_indexing(int N, int g, int32_t* st) {
+  const int nc = N + 2 * g, lo = 1 - g, hi = N + g;
+  int x, y;
+#define SET(xx, yy, d, vx, vy)                                                            \
+  do {                                                                                    \
+    size_t o_ = (size_t)((xx) - lo) + (size_t)nc * ((size_t)((yy) - lo) + (size_t)nc * 2 * ((d) - 1)); \
+    st[o_] = (vx);                                                                        \
+    st[o_ + (size_t)nc * nc] = (vy);                                                      \
+  } while (0)
+#define MN(a, b) ((a) < (b) ? (a) : (b))
+#define MX(a, b) ((a) > (b) ? (a) : (b))
+  for (x = lo; x <= hi; ++x)
+    for (y = lo; y <= hi; ++y) {
+      int xp1 = MN(x + 1, hi), yp1 = MN(y + 1, hi), xp2 = MN(x + 2, hi), yp2 = MN(y + 2, hi);
+      int xm1 = MX(x - 1, lo), ym1 = MX(y - 1, lo), xm2 = MX(x - 2, lo), ym2 = MX(y - 2, lo);
+      SET(x, y, 1, xp1, y); SET(x, y, 2, xm1, y); SET(x, y, 3, x, yp1); SET(x, y, 4, x, ym1);
+      SET(x, y, 5, xp2, y); SET(x, y, 6, xm2, y); SET(x, y, 7, x, yp2); SET(x, y, 8, x, ym2);
+      SET(x, y, 9, xp1, yp1); SET(x, y, 10, xm1, yp1); SET(x, y, 11, xm1, ym1); SET(x, y, 12, xp1, ym1);
+    }
+  /* left-down (:84-122) */
+  y = 1;
+  for (x = lo; x <= 0; ++x) {
+    SET(x, y, 4, 1, x); SET(x, y, 8, 2, x); SET(x, y, 11, 1, MX(x - 1, lo));
+    if (x == 0) SET(x, y, 12, 2, x - 1); else SET(x, y, 12, 1, x + 1);
+    SET(x, y + 1, 8, 1, x);
+  }
+  x = 1;
+  for (y = lo; y <= 0; ++y) {
+    SET(x, y, 2, y, 1); SET(x, y, 6, y, 2);
+    if (y == 0) SET(x, y, 10, y - 1, 2); else SET(x, y, 10, y + 1, 1);
+    SET(x, y, 11, MX(y - 1, lo), 1);
+    SET(x + 1, y, 6, y, 1);
+  }
+  SET(1, 1, 11, 2, -1);
+  /* right-down (:127-168) */
+  y = 1;
+  for (x = N + 1; x <= hi; ++x) {
+    SET(x, y, 4, N, N - x + 1); SET(x, y, 8, N - 1, N - x + 1);
+    if (x == N + 1) SET(x, y, 11, N - 1, N - x); else SET(x, y, 11, N, N - x + 2);
+    SET(x, y, 12, N, MX(N - x, lo));
+    SET(x, y + 1, 8, N, N - x + 1);
+  }
+  x = N;
+  for (y = lo; y <= 0; ++y) {
+    SET(x, y, 1, N + 1 - y, 1); SET(x, y, 5, N + 1 - y, 2);
+    if (y == 0) SET(x, y, 9, N + 2 - y, 2); else SET(x, y, 9, N - y, 1);
+    SET(x, y, 12, MN(N + 2 - y, hi), 1);
+    SET(x - 1, y, 5, N + 1 - y, 1);
+  }
+  SET(N, 1, 12, N - 1, -1);
+  /* right-up (:173-214) */
+  y = N;
+  for (x = N + 1; x <= hi; ++x) {
+    SET(x, y, 3, N, x); SET(x, y, 7, N - 1, x);
+    if (x == N + 1) SET(x, y, 10, N - 1, x + 1); else SET(x, y, 10, N, x - 1);
+    SET(x, y, 9, N, MN(x + 1, hi));
+    SET(x, y - 1, 7, N, x);
+  }
+  x = N;
+  for (y = N + 1; y <= hi; ++y) {
+    SET(x, y, 1, y, N); SET(x, y, 5, y, N - 1);
+    if (y == N + 1) SET(x, y, 12, y + 1, N - 1); else SET(x, y, 12, y - 1, N);
+    SET(x, y, 9, MN(y + 1, hi), N);
+    SET(x - 1, y, 5, y, N);
+  }
+  SET(N, N, 9, N - 1, N + 2);
+  /* left-up (:219-260) */
+  x = 1;
+  for (y = N + 1; y <= hi; ++y) {
+    SET(x, y, 2, N - y + 1, N); SET(x, y, 6, N - y + 1, N - 1);
+    if (y == N + 1) SET(x, y, 11, N - y, N - 1); else SET(x, y, 11, N - y + 2, N);
+    SET(x, y, 10, MX(N - y, lo), N);
+    SET(x + 1, y, 6, N - y + 1, N);
+  }
+  y = N;
+  for (x = lo; x <= 0; ++x) {
+    SET(x, y, 3, 1, N - x + 1); SET(x, y, 7, 2, N - x + 1);
+    if (x == 0) SET(x, y, 9, 2, N - x + 2); else SET(x, y, 9, 1, N - x);
+    SET(x, y, 10, 1, MN(N - x + 2, hi));
+    SET(x, y - 1, 7, 1, N - x + 1);
+  }
+  SET(1, N, 10, 2, N + 2);
+#undef SET
+#undef MN
+#undef MX
+}

@@ -81,6 +81,9 @@ void or_quad_faces(const or_config* c, const double* vert, double* const* xf, do
 void or_quad_volume(const or_config* c, const double* vert, double* pts, double* wts, double* tan,
                     double* vol, double* cen);
 void or_mode_indices(int32_t* ii, int32_t* jj, int32_t* kk);
+/* 211: twelve-direction stencil zone table with the tri-corner remaps,
+ * (1-g:N+g, 1-g:N+g, 2, 12) for a one-patch sector */
+void or_stencil_indexing(int N, int g, int32_t* st);
 void or_basis_moments(const or_config* c, const int32_t* mask, const double* pts, const double* wts,
                       const double* cen, const double* scl, double* coef);
 

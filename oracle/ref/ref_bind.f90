@@ -194,4 +194,19 @@ contains
     call BUILD_MODE_INDICES(nm, ii, jj, kk)
   end subroutine
 
+  ! STENCIL_INDEXING_TRICORNERS (211stencil_indexing_tricorners.f:6) for a
+  ! one-patch sector.
+  subroutine ref_stencil_indexing(n, nr, g, st) bind(C, name="ref_stencil_indexing")
+    integer(c_int), value :: n, nr, g
+    integer(c_int), intent(out) :: st(1-g:n+g, 1-g:n+g, 2, 12)
+    integer(c_int), allocatable :: mask(:, :, :)
+    integer :: ld, rd, ru, lu
+    external STENCIL_INDEXING_TRICORNERS, REF_INIT_COMMON
+    call REF_INIT_COMMON()
+    allocate(mask(1-g:n+g, 1-g:n+g, 1-g:nr+g))
+    mask = 1
+    call STENCIL_INDEXING_TRICORNERS(n, nr, 1, 1, g, 1, 1, ld, rd, ru, lu, mask, st)
+    deallocate(mask)
+  end subroutine
+
 end module cs_ref_bind

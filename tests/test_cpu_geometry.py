@@ -117,3 +117,44 @@ def test_shell_volume_converges_fourth_order():
         exact = 4.0 * np.pi / 3.0 * (1.5 ** 3 - 1.0)
         errs.append(abs(vol - exact) / exact)
     assert 12.0 < errs[0] / errs[1] < 20.0, errs
+
+
+@pytest.mark.parametrize("N,g", [(4, 3), (8, 3), (12, 4), (16, 6)])
+def test_stencil_indexing(N, g):
+    """STENCIL_INDEXING_TRICORNERS: the library's table (the left-down remaps
+    mirrored onto the other corners) equals the oracle's literal restatement,
+    which equals the reference build; and every one-step neighbour it names
+    around the tri-corners is a zone adjacent on the sphere."""
+    cfg = cs.make_config(N, 3, g, radial_bc=cs.RBC_PAD, rmin=1.0, rmax=1.5)
+    got = cs.stencil_indexing(cfg)
+    L = orc.olib()
+    L.or_stencil_indexing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    o = np.zeros_like(got)
+    L.or_stencil_indexing(N, g, orc.P(o))
+    assert np.array_equal(got, o)
+    R = orc.rlib()
+    if R is not None:
+        r = np.zeros_like(got)
+        R.ref_stencil_indexing(N, 3, g, orc.P(r))
+        assert np.array_equal(o, r)
+    # adjacency on the sphere: zone centres from the vertices of sector 0
+    a = orc.quad_geometry(cfg)
+    nc, nv = N + 2 * g, N + 2 * g + 1
+    vert = a["vert"].reshape(6, 3, 3 + 2 * g + 1, nv, nv)[0, :, g]  # (3, j, i) at k = 0
+    mask = a["mask"].reshape(6, 3 + 2 * g, nc, nc)[0, g]
+    cen = 0.25 * (vert[:, :-1, :-1] + vert[:, 1:, :-1] + vert[:, :-1, 1:] + vert[:, 1:, 1:])  # (3, y, x)
+    st = got.reshape(12, 2, nc, nc)
+    h = np.pi / 2 / N
+    checked = 0
+    for y in range(nc):
+        for x in range(nc):
+            if not mask[y, x]:
+                continue
+            for d in range(4):
+                vx, vy = st[d, 0, y, x] - (1 - g), st[d, 1, y, x] - (1 - g)
+                if (vx, vy) == (x, y) or not (0 <= vx < nc and 0 <= vy < nc) or not mask[vy, vx]:
+                    continue
+                dist = np.linalg.norm(cen[:, y, x] - cen[:, vy, vx])
+                assert 0.4 * h < dist < 1.6 * h, (x, y, d, vx, vy, dist / h)
+                checked += 1
+    assert checked > 4 * N * N
