@@ -276,10 +276,11 @@ def bench_batch(cs, args):
 
 
 def bench_geometry(cs, args):
-    """The P1 quadrature geometry (vertices and mask, face quadrature and scale
-    length, volume quadrature, basis moments) of the 6 x 192^2 x 64 shell built
-    on the GPU; HBM bytes per item are the outputs written plus the compulsory
-    inputs read."""
+    """The P1 quadrature geometry (vertices and mask, face quadrature with the
+    B-field weights and the scale length, volume quadrature, basis moments) of
+    the 6 x 192^2 x 64 shell built on the GPU; HBM bytes per item are the
+    outputs written plus the compulsory inputs read (x/y faces 32 words, z-faces
+    44, zones 108 for the volume quadrature)."""
     cfg = cs.make_config(192, 64, 3, cs.RECON_WENO5, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
                          cs.RMESH_UNIFORM, 1.0, 1.5)
     z = cs.geometry_sizes(cfg)
@@ -287,7 +288,7 @@ def bench_geometry(cs, args):
     ms = [min(r[q] for r in runs) for q in range(4)]
     items = [z["vert"], z["xf"] + z["yf"] + z["zf"], z["zone"], z["zone"]]
     nbytes = [24 * z["vert"] + 4 * z["zone"],
-              216 * (z["xf"] + z["yf"]) + 312 * z["zf"] + 8 * z["zone"] + 24 * z["vert"],
+              256 * (z["xf"] + z["yf"]) + 352 * z["zf"] + 8 * z["zone"] + 24 * z["vert"],
               864 * z["zone"] + 24 * z["vert"],
               (292 + 32) * z["zone"]]
     names = ["vertices+mask", "faces+scale", "volume", "moments"]

@@ -415,7 +415,8 @@ class FaceQuad(ctypes.Structure):
 
 class Geometry(ctypes.Structure):
     _fields_ = [("vert", ctypes.c_void_p), ("mask", ctypes.c_void_p), ("xf", FaceQuad), ("yf", FaceQuad),
-                ("zf", FaceQuad), ("scale", ctypes.c_void_p), ("v_pts", ctypes.c_void_p),
+                ("zf", FaceQuad), ("scale", ctypes.c_void_p), ("xb", FaceQuad), ("yb", FaceQuad),
+                ("zb", FaceQuad), ("v_pts", ctypes.c_void_p),
                 ("v_wts", ctypes.c_void_p), ("v_tan", ctypes.c_void_p), ("v_vol", ctypes.c_void_p),
                 ("v_cen", ctypes.c_void_p), ("moments", ctypes.c_void_p)]
 
@@ -450,6 +451,11 @@ def build_geometry(cfg, download=True):
         fq = getattr(G, f)
         for k in GEOMETRY_FACE_KEYS:
             setattr(fq, k, a[f][k].ctypes.data)
+        bname = f[0] + "b"   # B-field points of the same faces (no area, centroid)
+        a[bname] = dict(pts=np.zeros(15 * n), wts=np.zeros(5 * n), normal=np.zeros((15 if f == "zf" else 3) * n))
+        bq = getattr(G, bname)
+        for k in GEOMETRY_FACE_KEYS[:3]:
+            setattr(bq, k, a[bname][k].ctypes.data)
     _check(lib().cs_build_geometry(ctypes.byref(cfg), ctypes.byref(G), ms), "cs_build_geometry")
     return a, list(ms)
 

@@ -184,7 +184,7 @@ __global__ void k_mask(GDims d, const double* __restrict__ lat, int32_t* mask) {
 // the area, the unit normal at the centre (all five on curved z-faces), the
 // area and the weighted centroid.  S: element stride of the face arrays.
 __device__ void face_element(double (*X)[3], int64_t o, int64_t S, bool all_normals, double* pts, double* wts,
-                             double* nrm, double* area, double* cen) {
+                             double* nrm, double* area, double* cen, double* bwts) {
   double w[5], P[5][3];
   for (int p = 0; p < 5; ++p) {
     double A[3], B[3];
@@ -210,6 +210,10 @@ __device__ void face_element(double (*X)[3], int64_t o, int64_t S, bool all_norm
   }
   double a = 0.0;
   for (int p = 0; p < 5; ++p) a = a + w[p];
+  // the B-field points of COORDS_TO_WT_QUADPTS_BFLD_ISO2 (230…:8) are these
+  // points and normals with the weights times the reciprocal of their sum
+  const double rb = 1.0 / fmax(a, kSmall);
+  for (int p = 0; p < 5; ++p) bwts[o + p * S] = rb * w[p];
   a = fmax(kSmall, a);
   area[o] = a;
   for (int p = 0; p < 5; ++p) {
@@ -223,7 +227,7 @@ __device__ void face_element(double (*X)[3], int64_t o, int64_t S, bool all_norm
   }
 }
 struct FaceOut {
-  double *pts, *wts, *nrm, *area, *cen;
+  double *pts, *wts, *nrm, *area, *cen, *bwts;
 };
 // fam 0: x-faces, 1: y-faces, 2: z-faces
 __global__ void __launch_bounds__(256) k_faces(GDims d, const double* __restrict__ vert, int fam, FaceOut f) {
@@ -257,7 +261,7 @@ __global__ void __launch_bounds__(256) k_faces(GDims d, const double* __restrict
   }
   const int64_t b = (int64_t)s * S;
   face_element(X, o, S, fam == 2, f.pts + 15 * b, f.wts + 5 * b, f.nrm + (fam == 2 ? 15 : 3) * b, f.area + b,
-               f.cen + 3 * b);
+               f.cen + 3 * b, f.bwts + 5 * b);
   if (fam == 2) {  // the centroid goes back onto the face's sphere
     double* cn = f.cen + 3 * b;
     double v[3] = {cn[o], cn[o + S], cn[o + 2 * S]};
@@ -458,6 +462,7 @@ extern "C" int cs_build_geometry(const cs_config* cfg, const cs_geometry* out, f
     F[f].nrm = B.get<double>((f == 2 ? 15 : 3) * S);
     F[f].area = B.get<double>(S);
     F[f].cen = B.get<double>(3 * S);
+    F[f].bwts = B.get<double>(5 * S);
   }
   VolOut Vo;
   Vo.pts = B.get<double>(144 * d.zs);
@@ -493,6 +498,7 @@ extern "C" int cs_build_geometry(const cs_config* cfg, const cs_geometry* out, f
     rc = copy_out(out->vert, vert, sizeof(double) * 18 * d.vs, st);
     if (!rc) rc = copy_out(out->mask, mask, sizeof(int32_t) * 6 * d.zs, st);
     const cs_face_quad* fo[3] = {&out->xf, &out->yf, &out->zf};
+    const cs_face_quad* bo[3] = {&out->xb, &out->yb, &out->zb};
     for (int f = 0; f < 3 && !rc; ++f) {
       const size_t fb = sizeof(double) * 6 * (f == 2 ? d.zfs : d.xs);
       rc = copy_out(fo[f]->pts, F[f].pts, 15 * fb, st);
@@ -500,6 +506,9 @@ extern "C" int cs_build_geometry(const cs_config* cfg, const cs_geometry* out, f
       if (!rc) rc = copy_out(fo[f]->normal, F[f].nrm, (f == 2 ? 15 : 3) * fb, st);
       if (!rc) rc = copy_out(fo[f]->area, F[f].area, fb, st);
       if (!rc) rc = copy_out(fo[f]->centroid, F[f].cen, 3 * fb, st);
+      if (!rc) rc = copy_out(bo[f]->pts, F[f].pts, 15 * fb, st);
+      if (!rc) rc = copy_out(bo[f]->wts, F[f].bwts, 5 * fb, st);
+      if (!rc) rc = copy_out(bo[f]->normal, F[f].nrm, (f == 2 ? 15 : 3) * fb, st);
     }
     if (!rc) rc = copy_out(out->scale, scale, zb, st);
     if (!rc) rc = copy_out(out->v_pts, Vo.pts, 24 * zb, st);

@@ -260,6 +260,9 @@ typedef struct cs_geometry {
   int32_t* mask;
   cs_face_quad xf, yf, zf;  /* COORDS_TO_WT_QUADPTS_FACES_ISO2 (220coords_to_wt_quadpts_faces_iso2.f:7) */
   double* scale;            /* v_scale_length, same routine (:2212-2290)            */
+  cs_face_quad xb, yb, zb;  /* COORDS_TO_WT_QUADPTS_BFLD_ISO2 (230coords_to_wt_quadpts_bfld_iso2.f:8):
+                               pts, wts (times the reciprocal sum), normal; area and
+                               centroid are not used */
   double* v_pts;            /* COORDS_TO_WT_QUADPTS_VOLUME_ISO2 (240coords_to_wt_quadpts_volume_iso2.f:7): (zone, 3, 8) */
   double* v_wts;            /* (zone, 8)                                            */
   double* v_tan;            /* (zone, 3, 3, 8): d x_c / d xi_q at each point         */

@@ -1680,7 +1680,7 @@ static const double kFq[5][3] = {{0.0, 0.0, 1.0 / 3.0}, {0.5, 0.0, 1.0 / 6.0}, {
 /* One face from its eight nodes: quadrature points, weights / area, normal(s),
  * area, centroid.  nall: normals at all five points (curved z-faces). */
 static void face_quad(double X[8][3], double* pts, double* wts, double* nrm, double* area, double* cen,
-                      int nall, size_t st) {
+                      int nall, size_t st, int bnorm) {
   int p, n, c;
   double C[5][3], a = 0.0;
   for (p = 0; p < 5; ++p) {
@@ -1704,6 +1704,11 @@ static void face_quad(double X[8][3], double* pts, double* wts, double* nrm, dou
     }
   }
   for (p = 0; p < 5; ++p) a = a + wts[p * st];
+  if (bnorm) {  /* 230coords_to_wt_quadpts_bfld_iso2.f: weights times the reciprocal sum */
+    double r = 1.0 / fmax(a, SMALLNUM);
+    for (p = 0; p < 5; ++p) wts[p * st] = r * wts[p * st];
+    return;
+  }
   a = fmax(SMALLNUM, a);
   *area = a;
   for (p = 0; p < 5; ++p) wts[p * st] = wts[p * st] / a;
@@ -1738,7 +1743,7 @@ void or_quad_faces(const or_config* c, const double* vert, double* const* xf, do
           midpoint(X[2], X[3], 1, norm3(X[2]), X[6]);
           midpoint(X[3], X[0], 0, 0.0, X[7]);
           face_quad(X, xf[0] + 15 * b + o, xf[1] + 5 * b + o, xf[2] + 3 * b + o, xf[3] + b + o,
-                    xf[4] + 3 * b + o, 0, S);
+                    xf[4] + 3 * b + o, 0, S, 0);
         }
     /* y-faces: nodes v(i-1, j, k-1), v(i-1, j, k), v(i, j, k), v(i, j, k-1) */
     for (k = 1 - g; k <= NR + g; ++k)
@@ -1754,7 +1759,7 @@ void or_quad_faces(const or_config* c, const double* vert, double* const* xf, do
           midpoint(X[2], X[3], 0, 0.0, X[6]);
           midpoint(X[3], X[0], 1, norm3(X[0]), X[7]);
           face_quad(X, yf[0] + 15 * b + o, yf[1] + 5 * b + o, yf[2] + 3 * b + o, yf[3] + b + o,
-                    yf[4] + 3 * b + o, 0, S);
+                    yf[4] + 3 * b + o, 0, S, 0);
         }
     /* z-faces: nodes v(i-1, j-1, k), v(i, j-1, k), v(i, j, k), v(i-1, j, k), all
        edges on the sphere; normals at every point; the centroid is put back
@@ -1770,7 +1775,7 @@ void or_quad_faces(const or_config* c, const double* vert, double* const* xf, do
           r0 = norm3(X[0]);
           for (n = 0; n < 4; ++n) midpoint(X[n], X[(n + 1) & 3], 1, r0, X[4 + n]);
           face_quad(X, zf[0] + 15 * b + o, zf[1] + 5 * b + o, zf[2] + 15 * b + o, zf[3] + b + o,
-                    zf[4] + 3 * b + o, 1, S);
+                    zf[4] + 3 * b + o, 1, S, 0);
           {
             double* cn = zf[4] + 3 * b + o;
             double cv[3] = {cn[0], cn[S], cn[2 * S]};
@@ -2041,4 +2046,56 @@ void or_stencil_indexing(int N, int g, int32_t* st) {
 #undef SET
 #undef MN
 #undef MX
+}
+
+/* COORDS_TO_WT_QUADPTS_BFLD_ISO2 (230coords_to_wt_quadpts_bfld_iso2.f:8): the
+ * face points and normals of 220 with the weights scaled by the reciprocal of
+ * their sum; xb/yb/zb = {pts, wts, normal}. */
+void or_quad_bfield(const or_config* c, const double* vert, double* const* xb, double* const* yb,
+                    double* const* zb) {
+  geo_dims d = geo_dims_of(c);
+  int N = d.N, NR = d.NR, g = d.g, s, i, j, k, n;
+  double area, cen[3];
+  for (s = 0; s < 6; ++s) {
+    for (k = 1 - g; k <= NR + g; ++k)
+      for (j = 1 - g; j <= N + g; ++j)
+        for (i = -g; i <= N + g; ++i) {
+          double X[8][3];
+          size_t o = (size_t)(i + g) + (size_t)d.nv * ((j - 1 + g) + (size_t)d.nc * (k - 1 + g));
+          size_t S = d.xs, b = (size_t)s * S;
+          vtx(&d, vert, s, i, j - 1, k - 1, X[0]); vtx(&d, vert, s, i, j, k - 1, X[1]);
+          vtx(&d, vert, s, i, j, k, X[2]); vtx(&d, vert, s, i, j - 1, k, X[3]);
+          midpoint(X[0], X[1], 1, norm3(X[0]), X[4]);
+          midpoint(X[1], X[2], 0, 0.0, X[5]);
+          midpoint(X[2], X[3], 1, norm3(X[2]), X[6]);
+          midpoint(X[3], X[0], 0, 0.0, X[7]);
+          face_quad(X, xb[0] + 15 * b + o, xb[1] + 5 * b + o, xb[2] + 3 * b + o, &area, cen, 0, S, 1);
+        }
+    for (k = 1 - g; k <= NR + g; ++k)
+      for (j = -g; j <= N + g; ++j)
+        for (i = 1 - g; i <= N + g; ++i) {
+          double X[8][3];
+          size_t o = (size_t)(i - 1 + g) + (size_t)d.nc * ((j + g) + (size_t)d.nv * (k - 1 + g));
+          size_t S = d.ys, b = (size_t)s * S;
+          vtx(&d, vert, s, i - 1, j, k - 1, X[0]); vtx(&d, vert, s, i - 1, j, k, X[1]);
+          vtx(&d, vert, s, i, j, k, X[2]); vtx(&d, vert, s, i, j, k - 1, X[3]);
+          midpoint(X[0], X[1], 0, 0.0, X[4]);
+          midpoint(X[1], X[2], 1, norm3(X[2]), X[5]);
+          midpoint(X[2], X[3], 0, 0.0, X[6]);
+          midpoint(X[3], X[0], 1, norm3(X[0]), X[7]);
+          face_quad(X, yb[0] + 15 * b + o, yb[1] + 5 * b + o, yb[2] + 3 * b + o, &area, cen, 0, S, 1);
+        }
+    for (k = -g; k <= NR + g; ++k)
+      for (j = 1 - g; j <= N + g; ++j)
+        for (i = 1 - g; i <= N + g; ++i) {
+          double X[8][3], r0;
+          size_t o = (size_t)(i - 1 + g) + (size_t)d.nc * ((j - 1 + g) + (size_t)d.nc * (k + g));
+          size_t S = d.zfs, b = (size_t)s * S;
+          vtx(&d, vert, s, i - 1, j - 1, k, X[0]); vtx(&d, vert, s, i, j - 1, k, X[1]);
+          vtx(&d, vert, s, i, j, k, X[2]); vtx(&d, vert, s, i - 1, j, k, X[3]);
+          r0 = norm3(X[0]);
+          for (n = 0; n < 4; ++n) midpoint(X[n], X[(n + 1) & 3], 1, r0, X[4 + n]);
+          face_quad(X, zb[0] + 15 * b + o, zb[1] + 5 * b + o, zb[2] + 15 * b + o, &area, cen, 1, S, 1);
+        }
+  }
 }

@@ -78,6 +78,8 @@ void or_imex_combine(int stage, double dt, double gr, const double* n, const dou
 void or_vert_coords(const or_config* c, double* vert, int32_t* mask);
 void or_quad_faces(const or_config* c, const double* vert, double* const* xf, double* const* yf,
                    double* const* zf, double* scl);
+void or_quad_bfield(const or_config* c, const double* vert, double* const* xb, double* const* yb,
+                    double* const* zb);
 void or_quad_volume(const or_config* c, const double* vert, double* pts, double* wts, double* tan,
                     double* vol, double* cen);
 void or_mode_indices(int32_t* ii, int32_t* jj, int32_t* kk);

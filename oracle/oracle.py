@@ -165,6 +165,9 @@ def geometry_arrays(cfg):
         a[f] = dict(pts=np.zeros(6 * 15 * n), wts=np.zeros(6 * 5 * n),
                     normal=np.zeros(6 * (15 if f == "zf" else 3) * n), area=np.zeros(6 * n),
                     centroid=np.zeros(6 * 3 * n))
+        b = f[0] + "b"   # B-field points of the same faces
+        a[b] = dict(pts=np.zeros(6 * 15 * n), wts=np.zeros(6 * 5 * n),
+                    normal=np.zeros(6 * (15 if f == "zf" else 3) * n))
     return a
 
 
@@ -175,7 +178,7 @@ def quad_geometry(cfg, use_reference=False):
     a = geometry_arrays(cfg)
     N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
     vp = ctypes.c_void_p
-    fptr = lambda f: [P(a[f][k]) for k in GEOM_FACE_KEYS]
+    fptr = lambda f: [P(a[f][k]) for k in GEOM_FACE_KEYS if k in a[f]]
     if use_reference:
         R = rlib()
         if R is None:
@@ -187,6 +190,7 @@ def quad_geometry(cfg, use_reference=False):
                           P(rb))
         R.ref_vert_coords(N, NR, g, P(lat), P(rb), P(a["vert"]), P(a["mask"]))
         R.ref_quad_faces(N, NR, g, P(a["vert"]), P(a["mask"]), *fptr("xf"), *fptr("yf"), *fptr("zf"), P(a["scale"]))
+        R.ref_quad_bfield(N, NR, g, P(a["vert"]), P(a["mask"]), *fptr("xb")[:3], *fptr("yb")[:3], *fptr("zb")[:3])
         R.ref_quad_volume(N, NR, g, P(a["vert"]), P(a["mask"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_tan"]),
                           P(a["v_vol"]), P(a["v_cen"]))
         R.ref_basis_moments(N, NR, g, P(a["mask"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_vol"]), P(a["v_cen"]),
@@ -200,6 +204,9 @@ def quad_geometry(cfg, use_reference=False):
     L.or_vert_coords(ctypes.byref(cfg), P(a["vert"]), P(a["mask"]))
     keep = [ptrs([a[f][k] for k in GEOM_FACE_KEYS]) for f in ("xf", "yf", "zf")]
     L.or_quad_faces(ctypes.byref(cfg), P(a["vert"]), keep[0][0], keep[1][0], keep[2][0], P(a["scale"]))
+    L.or_quad_bfield.argtypes = [vp] * 5
+    kb = [ptrs([a[f][k] for k in GEOM_FACE_KEYS[:3]]) for f in ("xb", "yb", "zb")]
+    L.or_quad_bfield(ctypes.byref(cfg), P(a["vert"]), kb[0][0], kb[1][0], kb[2][0])
     L.or_quad_volume(ctypes.byref(cfg), P(a["vert"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_tan"]), P(a["v_vol"]),
                      P(a["v_cen"]))
     L.or_basis_moments(ctypes.byref(cfg), P(a["mask"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_cen"]), P(a["scale"]),

@@ -209,4 +209,27 @@ contains
     deallocate(mask)
   end subroutine
 
+  ! COORDS_TO_WT_QUADPTS_BFLD_ISO2 (230coords_to_wt_quadpts_bfld_iso2.f:8), six sectors.
+  subroutine ref_quad_bfield(n, nr, g, vert, mask, xp, xw, xn, yp, yw, yn, zp, zw, zn) &
+      bind(C, name="ref_quad_bfield")
+    integer(c_int), value :: n, nr, g
+    real(c_double), intent(in) :: vert(0-g:n+g, 0-g:n+g, 0-g:nr+g, 3, 6)
+    integer(c_int), intent(in) :: mask(1-g:n+g, 1-g:n+g, 1-g:nr+g, 6)
+    real(c_double), intent(out) :: xp(0-g:n+g, 1-g:n+g, 1-g:nr+g, 3, 5, 6), xw(0-g:n+g, 1-g:n+g, 1-g:nr+g, 5, 6)
+    real(c_double), intent(out) :: xn(0-g:n+g, 1-g:n+g, 1-g:nr+g, 3, 6)
+    real(c_double), intent(out) :: yp(1-g:n+g, 0-g:n+g, 1-g:nr+g, 3, 5, 6), yw(1-g:n+g, 0-g:n+g, 1-g:nr+g, 5, 6)
+    real(c_double), intent(out) :: yn(1-g:n+g, 0-g:n+g, 1-g:nr+g, 3, 6)
+    real(c_double), intent(out) :: zp(1-g:n+g, 1-g:n+g, 0-g:nr+g, 3, 5, 6), zw(1-g:n+g, 1-g:n+g, 0-g:nr+g, 5, 6)
+    real(c_double), intent(out) :: zn(1-g:n+g, 1-g:n+g, 0-g:nr+g, 3, 5, 6)
+    integer :: s
+    external COORDS_TO_WT_QUADPTS_BFLD_ISO2, REF_INIT_COMMON
+    call REF_INIT_COMMON()
+    do s = 1, 6
+      call COORDS_TO_WT_QUADPTS_BFLD_ISO2(nr, n, g, vert(:, :, :, :, s), mask(:, :, :, s), &
+           xp(:, :, :, :, :, s), xw(:, :, :, :, s), xn(:, :, :, :, s), &
+           yp(:, :, :, :, :, s), yw(:, :, :, :, s), yn(:, :, :, :, s), &
+           zp(:, :, :, :, :, s), zw(:, :, :, :, s), zn(:, :, :, :, :, s))
+    end do
+  end subroutine
+
 end module cs_ref_bind

@@ -38,6 +38,9 @@ def test_geometry_matches_reference(name):
     for f in ("xf", "yf", "zf"):
         for k in orc.GEOM_FACE_KEYS:
             assert np.array_equal(o[f][k], r[f][k]), f"{f}.{k}"
+    for f in ("xb", "yb", "zb"):   # B-field points (230)
+        for k in orc.GEOM_FACE_KEYS[:3]:
+            assert np.array_equal(o[f][k], r[f][k]), f"{f}.{k}"
     # volume quadrature: the points bit for bit; the Jacobians through the
     # basis derivatives at the irrational Gauss points to rounding
     assert np.array_equal(o["v_pts"], r["v_pts"])

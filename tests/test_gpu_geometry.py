@@ -38,6 +38,9 @@ def test_geometry_matches_oracle(name):
     for f in ("xf", "yf", "zf"):
         for k in cs.GEOMETRY_FACE_KEYS:
             assert _rel(got[f][k], ref[f][k]) <= 1e-15, f"{f}.{k}"
+    for f in ("xb", "yb", "zb"):
+        for k in cs.GEOMETRY_FACE_KEYS[:3]:
+            assert _rel(got[f][k], ref[f][k]) <= 1e-15, f"{f}.{k}"
     for k in ("v_wts", "v_tan", "v_vol", "v_cen"):
         assert _rel(got[k], ref[k]) < 1e-13, k
     assert np.abs(got["moments"] - ref["moments"]).max() < 1e-13
