@@ -34,7 +34,8 @@ class Config(ctypes.Structure):
                   "rk_order", "radial_bc", "radial_mesh")] +
                 [(n, ctypes.c_double) for n in
                  ("rmin", "rmax", "gamma", "rhofloor", "prsfloor", "cfl", "grav")] +
-                [("imex", ctypes.c_int32), ("reserved", ctypes.c_int32)])
+                [("imex", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                 ("omega", ctypes.c_double * 3)])
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
@@ -48,7 +49,7 @@ class Sizes(ctypes.Structure):
 def make_config(ntp, nr=1, nghost=3, recon=RECON_WENO5, riemann=RS_HLLD, emf=EMF_LLF,
                 precision=FP64, rk_order=3, radial_bc=None, radial_mesh=RMESH_UNIFORM,
                 rmin=1.0, rmax=None, gamma=5.0 / 3.0, rhofloor=1e-12, prsfloor=1e-9,
-                cfl=0.4, grav=0.0, imex=0):
+                cfl=0.4, grav=0.0, imex=0, omega=(0.0, 0.0, 0.0)):
     """Build a Config.  nr == 1 is the thin-shell surface mode: rmax defaults
     to rmin + (pi/2)/ntp and the radial boundaries are conducting walls."""
     if rmax is None:
@@ -56,7 +57,8 @@ def make_config(ntp, nr=1, nghost=3, recon=RECON_WENO5, riemann=RS_HLLD, emf=EMF
     if radial_bc is None:
         radial_bc = RBC_WALL if nr == 1 else RBC_PAD
     return Config(ntp, nr, nghost, recon, riemann, emf, precision, rk_order, radial_bc,
-                  radial_mesh, rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav, imex, 0)
+                  radial_mesh, rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav, imex, 0,
+                  (ctypes.c_double * 3)(*omega))
 
 
 # The five workloads of the benchmark description (sizes, schemes, ICs).

@@ -122,6 +122,8 @@ struct State {
 struct Params {
   double gamma, gm1, rhofloor, prsfloor, cfl, grav, igm1;
   int recon, riemann, emf, wall, imex, surf;
+  double om[3];  // rotating frame (omega_passive)
+  int rot;       // any component of om nonzero
 };
 
 // sector neighbours right, left, up, down (203fill_ghost2d_global.f:37-75), 0-based
@@ -946,6 +948,26 @@ __device__ __forceinline__ void cleanup_zone(const CsDims& d, const Geo& geo, co
   if (!(u[0] == u[0]) || !(u[4] == u[4]) || isinf(u[4])) { if (flags) flags[1] = 1; }
 }
 
+// Rotating-frame sources of BUILD_NON_STIFF_SOURCES (470build_non_stiff_sources.f:
+// 150-250), explicit in every scheme: centrifugal -rho Om x (Om x r) and
+// Coriolis -2 rho Om x v on the momentum, their work v . S on the energy, at
+// the zone position r (mid radius along the radial normal, where the gravity
+// acts) with v from the stage input as CONS_TO_PRIM forms it.  Added to L.
+__device__ __forceinline__ void rotation_source(const Params& pr, const State& in, const CsDims& d, int s,
+                                                int64_t c, const double* r, double* L) {
+  const double rho = in.U[(s * 8) * d.cpp + c];
+  const double a = rho / (rho * rho + pr.rhofloor * pr.rhofloor);
+  double v[3];
+  for (int q = 0; q < 3; ++q) v[q] = in.U[(s * 8 + 1 + q) * d.cpp + c] * a;
+  const double* w = pr.om;
+  const double wr[3] = {w[1] * r[2] - w[2] * r[1], w[2] * r[0] - w[0] * r[2], w[0] * r[1] - w[1] * r[0]};
+  const double S[3] = {-rho * (w[1] * wr[2] - w[2] * wr[1]) - 2.0 * rho * (w[1] * v[2] - w[2] * v[1]),
+                       -rho * (w[2] * wr[0] - w[0] * wr[2]) - 2.0 * rho * (w[2] * v[0] - w[0] * v[2]),
+                       -rho * (w[0] * wr[1] - w[1] * wr[0]) - 2.0 * rho * (w[0] * v[1] - w[1] * v[0])};
+  for (int q = 0; q < 3; ++q) L[1 + q] += S[q];
+  L[4] += v[0] * S[0] + v[1] * S[1] + v[2] * S[2];
+}
+
 // Combination of a zone's stage update with the radial gravity source
 // S(U) = (0, rho gr n, gr (rho v . n)), gr = -grav / rc^2 at the zone's mid
 // radius, n the unit normal of its radial faces (the point-mass gravity of
@@ -1052,10 +1074,14 @@ __device__ __forceinline__ void update_cells(const CsDims& d, int i, int j, int 
     L[q] = -ov * (lateral_div(d, zf, Fx, Fy, q) + (Fz[o + fxp] - Fz[o + fzm]));
   }
   double n[3] = {0.0, 0.0, 0.0}, gr = 0.0;
-  if (pr.grav != 0.0) {
+  if (pr.grav != 0.0 || pr.rot) {
     const double zn = geo.t(d, T_ZSN, s, i, j), rc = 0.5 * (r0 + r1);
     for (int q = 0; q < 3; ++q) n[q] = geo.t(d, T_ZSX + q, s, i, j) / zn;
-    gr = -pr.grav / (rc * rc);
+    if (pr.grav != 0.0) gr = -pr.grav / (rc * rc);
+    if (pr.rot) {
+      const double rp[3] = {rc * n[0], rc * n[1], rc * n[2]};
+      rotation_source(pr, in, d, s, c, rp, L);
+    }
   }
   double u[8], en;
   finish_zone(d, in, u0, s, c, dt, wa, wb, mode, gr, n, L, u, en);
@@ -1138,6 +1164,10 @@ __global__ void __launch_bounds__(256, 2) k_update_cells_surf(CsDims d, Rng rg, 
   for (int q = 0; q < 9; ++q)
     if (q < 5 || q == 8) L[q] -= ov * (fo[q] * a1 - fi[q] * a0);
   const double rcz = 0.5 * (r0 + r1), gr = pr.grav != 0.0 ? -pr.grav / (rcz * rcz) : 0.0;
+  if (pr.rot) {
+    const double rp[3] = {rcz * A[0], rcz * A[1], rcz * A[2]};
+    rotation_source(pr, in, d, s, c, rp, L);
+  }
   double u[8], en;
   finish_zone(d, in, u0, s, c, dt, wa, wb, mode, gr, A, L, u, en);
   // V was read above for this zone only, so it is overwritten in place
@@ -1906,7 +1936,9 @@ extern "C" int cs_create(const cs_config* cfg, cs_handle** out) {
   h->d = cs_make_dims(cfg->ntp, cfg->nr, cfg->nghost);
   h->pr = Params{cfg->gamma, cfg->gamma - 1.0, cfg->rhofloor, cfg->prsfloor, cfg->cfl, cfg->grav,
                  1.0 / (cfg->gamma - 1.0), cfg->recon, cfg->riemann, cfg->emf, cfg->radial_bc == CS_RBC_WALL ? 1 : 0,
-                 cfg->imex, (cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1) ? 1 : 0};
+                 cfg->imex, (cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1) ? 1 : 0,
+                 {cfg->omega[0], cfg->omega[1], cfg->omega[2]},
+                 (cfg->omega[0] != 0.0 || cfg->omega[1] != 0.0 || cfg->omega[2] != 0.0) ? 1 : 0};
   h->surf = (cfg->radial_bc == CS_RBC_WALL && cfg->nr == 1) ? 1 : 0;
   if ((rc = load_corner_table(h->d, T))) {
     delete h;

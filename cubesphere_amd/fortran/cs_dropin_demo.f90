@@ -14,7 +14,8 @@ program cs_dropin_demo
   integer(c_int32_t) :: irk
 
   cfg = cs_config(n, nr, g, CS_RECON_WENO5, CS_RS_HLLD, CS_EMF_LLF, CS_FP64, 3, CS_RBC_WALL, 0, &
-                  1.0d0, 1.0d0 + 2.0d0 * atan(1.0d0) / n, 5.0d0 / 3.0d0, 1.0d-12, 1.0d-9, 0.4d0, 0.0d0, 0, 0)
+                  1.0d0, 1.0d0 + 2.0d0 * atan(1.0d0) / n, 5.0d0 / 3.0d0, 1.0d-12, 1.0d-9, 0.4d0, 0.0d0, 0, 0, &
+                  [0.0d0, 0.0d0, 0.0d0])
   allocate (u(1 - g:n + g, 1 - g:n + g, 1 - g:nr + g, 8, 6), ent(1 - g:n + g, 1 - g:n + g, 1 - g:nr + g, 6))
   allocate (bxf(0 - g:n + g, 0 - g:n + g, 0 - g:nr + g, 6), byf(0 - g:n + g, 0 - g:n + g, 0 - g:nr + g, 6), &
             bzf(0 - g:n + g, 0 - g:n + g, 0 - g:nr + g, 6))

@@ -25,6 +25,7 @@ module cs_hip_iface
     integer(c_int32_t) :: ntp, nr, nghost, recon, riemann, emf, precision, rk_order, radial_bc, radial_mesh
     real(c_double) :: rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav
     integer(c_int32_t) :: imex, reserved
+    real(c_double) :: omega(3)
   end type
 
   interface

@@ -80,6 +80,9 @@ typedef struct cs_config {
   double grav;         /* radial gravity g = grav / r^2 (0 = off)           */
   int32_t imex;        /* 1: gravity handled implicitly in the stage solve  */
   int32_t reserved;
+  double omega[3];     /* angular velocity of the rotating frame (omega_passive,
+                          470build_non_stiff_sources.f): centrifugal and
+                          Coriolis sources, explicit; 0 = off               */
 } cs_config;
 
 /* element counts of the host arrays above (per array, all 6 panels) */

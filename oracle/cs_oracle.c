@@ -1415,6 +1415,23 @@ void or_imex_combine(int stage, double dt, double gr, const double* n, const dou
   }
 }
 
+/* Rotating frame (470build_non_stiff_sources.f:150-250): centrifugal
+   -rho Om x (Om x r) and Coriolis -2 rho Om x v added to the momentum rows of
+   L, their work v . S to the energy; r the zone position, v = m rho / (rho^2 + rho_f^2). */
+void or_rotation_source(const double* om, double rhofloor, const double* r, const double* u, double* L) {
+  double rho = u[0], a = rho / (rho * rho + rhofloor * rhofloor), v[3], oxr[3], S[3], cen, cor;
+  int q;
+  for (q = 0; q < 3; ++q) v[q] = u[1 + q] * a;
+  for (q = 0; q < 3; ++q) oxr[q] = om[(q + 1) % 3] * r[(q + 2) % 3] - om[(q + 2) % 3] * r[(q + 1) % 3];
+  for (q = 0; q < 3; ++q) {
+    cen = -rho * (om[(q + 1) % 3] * oxr[(q + 2) % 3] - om[(q + 2) % 3] * oxr[(q + 1) % 3]);
+    cor = -2.0 * rho * (om[(q + 1) % 3] * v[(q + 2) % 3] - om[(q + 2) % 3] * v[(q + 1) % 3]);
+    S[q] = cen + cor;
+    L[1 + q] += S[q];
+  }
+  L[4] += v[0] * S[0] + v[1] * S[1] + v[2] * S[2];
+}
+
 void or_update(const or_grid* G, int irk, double dt, const double* const* in, const double* const* u0,
                double* const* out, double** F, double** E) {
   int N = G->N, NR = G->NR, s, i, j, k, q;
@@ -1430,6 +1447,13 @@ void or_update(const or_grid* G, int irk, double dt, const double* const* in, co
           for (q = 0; q < 9; ++q) {
             int64_t o = ((int64_t)s * 9 + q) * G->fpp;
             L[q] = -ov * ((F[0][o + fp] - F[0][o + fxm]) + (F[1][o + fp] - F[1][o + fym]) + (F[2][o + fp] - F[2][o + fzm]));
+          }
+          if (G->c.omega[0] != 0.0 || G->c.omega[1] != 0.0 || G->c.omega[2] != 0.0) {
+            double n[3], r[3], u[4], rc = 0.5 * (R(G, k - 1) + R(G, k));
+            gravity_at(G, s, i, j, k, n);
+            for (q = 0; q < 3; ++q) r[q] = rc * n[q];
+            for (q = 0; q < 4; ++q) u[q] = UU(in[0], G, s, q, c);
+            or_rotation_source(G->c.omega, G->c.rhofloor, r, u, L);
           }
           if (!G->c.imex) {
             if (G->c.grav != 0.0) { /* explicit gravity */

@@ -16,6 +16,7 @@ typedef struct or_config { /* field-for-field the same as cs_config */
   int32_t ntp, nr, nghost, recon, riemann, emf, precision, rk_order, radial_bc, radial_mesh;
   double rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav;
   int32_t imex, reserved;
+  double omega[3];
 } or_config;
 
 typedef struct or_grid or_grid;
@@ -66,6 +67,8 @@ double or_weno5(double v0, double v1, double v2, double v3, double v4);
 double or_emf4_hll(const double* s4, double cs_t, double sn_t, const double* ez4,
                    const double* bn2, const double* by2);
 double or_emf3_llf(double S, const double* ez3, const double* bn3);
+/* rotating-frame centrifugal + Coriolis sources added to L (rho, rho v, E rows) */
+void or_rotation_source(const double* om, double rhofloor, const double* r, const double* u, double* L);
 /* CLEANUP_AFTER_UPDATE on the owned zones of a state */
 void or_cleanup(const or_grid* G, double* U, double* ent, double* Bx, double* By, double* Bz);
 /* IMEX-SSP2(2,2,2) with the implicit gravity source, u = (rho, rho v, E) */

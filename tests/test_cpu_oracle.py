@@ -453,3 +453,37 @@ def test_cleanup_reproduces_a_uniform_field(nr):
         np.testing.assert_allclose(Ui[:, 4], Ei + Bm2, rtol=1e-12)     # p kept, E gains |B0|^2/8pi
     else:
         np.testing.assert_allclose(Ui[:, 4], Ei, rtol=1e-14)           # E kept
+
+
+def test_rotating_frame_source():
+    """Centrifugal and Coriolis sources of BUILD_NON_STIFF_SOURCES
+    (470build_non_stiff_sources.f:150-250): for rotation about z the
+    centrifugal force is rho w^2 (x, y, 0), the Coriolis force is normal to v
+    (no work), and the energy takes the work of the centrifugal force only."""
+    import ctypes
+    L = orc.olib()
+    L.or_rotation_source.argtypes = [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_void_p] * 3
+    rng = np.random.default_rng(5)
+    for _ in range(20):
+        w = rng.uniform(0.1, 2.0)
+        r = rng.normal(size=3)
+        rho = rng.uniform(0.5, 2.0)
+        v = rng.normal(size=3)
+        u = np.array([rho, *(rho * v)])
+        Lv = np.zeros(9)
+        L.or_rotation_source(orc.P(np.array([0.0, 0.0, w])), 1e-12, orc.P(r), orc.P(u), orc.P(Lv))
+        cen = rho * w * w * np.array([r[0], r[1], 0.0])
+        cor = Lv[1:4] - cen
+        assert np.allclose(Lv[1:4] - cor, cen, rtol=1e-13)
+        assert abs(cor @ v) < 1e-12 * np.linalg.norm(cor) * np.linalg.norm(v) + 1e-15
+        assert np.allclose(np.linalg.norm(cor), 2 * rho * w * np.linalg.norm(v[:2]), rtol=1e-12)
+        assert np.isclose(Lv[4], cen @ v, rtol=1e-12, atol=1e-14)
+        assert Lv[0] == 0.0 and (Lv[5:] == 0.0).all()
+    # a general axis: the centrifugal force is rho (w^2 r - (w.r) w)
+    om = np.array([0.3, -0.4, 0.5])
+    r = np.array([0.7, 0.2, -1.1])
+    u = np.array([1.3, 0.0, 0.0, 0.0])
+    Lv = np.zeros(9)
+    L.or_rotation_source(orc.P(om), 1e-12, orc.P(r), orc.P(u), orc.P(Lv))
+    assert np.allclose(Lv[1:4], 1.3 * ((om @ om) * r - (om @ r) * om), rtol=1e-13)
+    assert Lv[4] == 0.0

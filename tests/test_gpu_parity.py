@@ -25,6 +25,11 @@ def _cases():
         "surf8_pcm_wall": (cs.make_config(8, 1, 3, cs.RECON_PCM, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 3), 1),
         "shell8x3_imex_grav": (cs.make_config(8, 3, 3, cs.RECON_WENO5, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 2,
                                               cs.RBC_PAD, cs.RMESH_UNIFORM, 1.0, 1.5, grav=1.0, imex=1), 3),
+        "shell8x3_rot_grav": (cs.make_config(8, 3, 3, cs.RECON_WENO5, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 3,
+                                             cs.RBC_PAD, cs.RMESH_UNIFORM, 1.0, 1.5, grav=1.0,
+                                             omega=(0.1, -0.2, 0.4)), 3),
+        "surf12_rot": (cs.make_config(12, 1, 3, cs.RECON_WENO5, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 3,
+                                      omega=(0.0, 0.0, 0.5)), 1),
     }
 
 
