@@ -487,3 +487,17 @@ def test_rotating_frame_source():
     L.or_rotation_source(orc.P(om), 1e-12, orc.P(r), orc.P(u), orc.P(Lv))
     assert np.allclose(Lv[1:4], 1.3 * ((om @ om) * r - (om @ r) * om), rtol=1e-13)
     assert Lv[4] == 0.0
+
+
+def test_rotating_frame_coriolis_sign():
+    """Coriolis -2 rho Om x v: for Om = w z and v = x (at the axis, so no
+    centrifugal term) the force points along -y."""
+    import ctypes
+    L = orc.olib()
+    L.or_rotation_source.argtypes = [ctypes.c_void_p, ctypes.c_double] + [ctypes.c_void_p] * 3
+    w, rho = 0.7, 1.9
+    Lv = np.zeros(9)
+    L.or_rotation_source(orc.P(np.array([0.0, 0.0, w])), 0.0, orc.P(np.zeros(3)),
+                         orc.P(np.array([rho, rho, 0.0, 0.0])), orc.P(Lv))
+    assert np.allclose(Lv[1:4], [0.0, -2.0 * rho * w, 0.0], rtol=1e-15, atol=0.0)
+    assert Lv[4] == 0.0
