@@ -92,4 +92,34 @@ void or_stencil_indexing(int N, int g, int32_t* st);
 void or_basis_moments(const or_config* c, const int32_t* mask, const double* pts, const double* wts,
                       const double* cen, const double* scl, double* coef);
 
+/* LIMIT_ALL_SLOPES_P1 (410limit_all_slopes_p1.f:9) with the pressure
+ * positivity of 440 for one patch (cs_oracle_p1.c).  Field order and types
+ * are those of cs_p1_patch in the product's C header, so one structure serves
+ * both. */
+typedef struct or_p1_patch {
+  int32_t ntp, nr, nghost;
+  int32_t itri[4]; /* itri_patch LD, RD, RU, LU */
+  int32_t pad;
+  double gamma, molewt, rhofloor, prsfloor, smallnum, flatten_coef, prs_flatten_coef;
+  const double* vert;
+  const int32_t* mask;
+  const int32_t* stencil;
+  const double *xf_normal, *xf_area, *xf_centroid;
+  const double *yf_normal, *yf_area, *yf_centroid;
+  const double *zf_area, *zf_centroid;
+  const double *xb_pts, *xb_wts, *xb_normal;
+  const double *yb_pts, *yb_wts, *yb_normal;
+  const double *zb_pts, *zb_wts, *zb_normal;
+  double* scale;
+  const double* v_centroid;
+  double *U, *ent, *Bxf, *Byf, *Bzf;
+  double *V, *B1_zc, *B2_zc, *B3_zc, *Ex_zc, *Ey_zc, *Ez_zc;
+  double *gamma_eos, *gamma_sound, *mean_mol_wt, *divvel, *msonic, *flatten, *work;
+  double *Bxf_q, *Byf_q, *Bzf_q;
+  double *V_min_local, *V_max_local;
+} or_p1_patch;
+int or_limit_all_slopes_p1(const or_p1_patch* p);
+/* zones of the last call with reduced slopes, and with slopes removed */
+void or_p1_positivity_stats(long* out);
+
 #endif

@@ -1,0 +1,732 @@
+/* cs_oracle_p1.c -- CPU restatement of the reference's second-order zone
+ * reconstruction for one patch: LIMIT_ALL_SLOPES_P1
+ * (cubesphere/400riemann_integrate/410limit_all_slopes_p1.f:9) with
+ * ENFORCE_PRESSURE_POSITIVITY_PRIM (440enforce_pressure_positivity_prim.f:6)
+ * and MATSOLVE (100math/120matsolve.f, LU of Numerical Recipes' ludcmp/lubksb,
+ * 100math/110numerical_recipes.f).
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker of the HIP path, never linked by the
+ * product.  Configuration: ideal MHD (8 components), adiabatic EOS, P1 modes
+ * (mean + 3 gradients), PRESSURE_POSITIVITY = 1, no B0 field, no helio
+ * application branches, 5 B-field points per face.
+ *
+ * Arrays are the reference's, column-major, for one patch with NGhost = g:
+ *   zone   (1-g:N+g, 1-g:N+g, 1-g:nr+g)       [, comp][, mode]
+ *   faceB  (0-g:N+g, 0-g:N+g, 0-g:nr+g)       Bxf, Byf, Bzf [, point]
+ *   x-face (0-g:N+g, 1-g:N+g, 1-g:nr+g) y-face (1-g:N+g, 0-g:N+g, 1-g:nr+g)
+ *   z-face (1-g:N+g, 1-g:N+g, 0-g:nr+g)       [, xyz][, point]
+ *   vert   (0-g:N+g, 0-g:N+g, 0-g:nr+g, 3)
+ *   stencil (1-g:N+g, 1-g:N+g, 2, 12)
+ * The ranges below are the reference's loop bounds; they need g >= 4.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <string.h>
+
+#include "cs_oracle.h"
+
+typedef struct {
+  int N, NR, g, nz, nzr, nf, nfr;
+  long NZ, NXF, NYF, NZF, NFC, NV;
+} p1dims;
+
+static p1dims p1_dims(const or_p1_patch* p) {
+  p1dims d;
+  d.N = p->ntp;
+  d.NR = p->nr;
+  d.g = p->nghost;
+  d.nz = d.N + 2 * d.g;
+  d.nzr = d.NR + 2 * d.g;
+  d.nf = d.nz + 1;
+  d.nfr = d.nzr + 1;
+  d.NZ = (long)d.nz * d.nz * d.nzr;
+  d.NXF = (long)d.nf * d.nz * d.nzr;
+  d.NYF = d.NXF;
+  d.NZF = (long)d.nz * d.nz * d.nfr;
+  d.NFC = (long)d.nf * d.nf * d.nfr;
+  d.NV = d.NFC;
+  return d;
+}
+
+/* element offsets of the layouts above (first index fastest) */
+static long zi(const p1dims* d, int i, int j, int k) {
+  return (i - 1 + d->g) + (long)d->nz * ((j - 1 + d->g) + (long)d->nz * (k - 1 + d->g));
+}
+static long fci(const p1dims* d, int i, int j, int k) {
+  return (i + d->g) + (long)d->nf * ((j + d->g) + (long)d->nf * (k + d->g));
+}
+static long xfi(const p1dims* d, int i, int j, int k) {
+  return (i + d->g) + (long)d->nf * ((j - 1 + d->g) + (long)d->nz * (k - 1 + d->g));
+}
+static long yfi(const p1dims* d, int i, int j, int k) {
+  return (i - 1 + d->g) + (long)d->nz * ((j + d->g) + (long)d->nf * (k - 1 + d->g));
+}
+static long zfi(const p1dims* d, int i, int j, int k) {
+  return (i - 1 + d->g) + (long)d->nz * ((j - 1 + d->g) + (long)d->nz * (k + d->g));
+}
+/* stencil entry n (0-based: R L U D RR LL UU DD RU LU LD RD) of zone column (i, j) */
+static void nbr(const or_p1_patch* p, const p1dims* d, int i, int j, int n, int* ni, int* nj) {
+  long s = (i - 1 + d->g) + (long)d->nz * (j - 1 + d->g), plane = (long)d->nz * d->nz;
+  *ni = p->stencil[s + plane * (2 * n)];
+  *nj = p->stencil[s + plane * (2 * n + 1)];
+}
+static int active(const or_p1_patch* p, const p1dims* d, int i, int j, int k) {
+  return p->mask[zi(d, i, j, k)] == 1;
+}
+
+#define VV(p, d, c, m, z) (p)->V[(z) + (d)->NZ * ((c) + 8 * (m))]
+#define UU(p, d, c, m, z) (p)->U[(z) + (d)->NZ * ((c) + 8 * (m))]
+
+static double omax(double a, double b) { return a > b ? a : b; }
+static double omin(double a, double b) { return a < b ? a : b; }
+
+/* relative position of point x to zone centroid, scaled: (x - c) / l */
+static void rel(const or_p1_patch* p, const p1dims* d, long z, const double* x, long xs, double obl, double* r) {
+  for (int c = 0; c < 3; ++c) r[c] = (x[xs * c] - p->v_centroid[z + d->NZ * c]) * obl;
+}
+
+static double obl_of(const or_p1_patch* p, long z) { return 1.0 / omax(p->scale[z], p->smallnum); }
+
+/* ------------------------------------------------------------------------ */
+/* Second-order multi-D WENO of 410...f:2421-3761 (and :5437-7100): eight
+ * one-sided three-zone stencils {R,L} x {U,D} x {z+1, z-1}, each a 3x3 solve
+ * for the gradient; the ninth, central stencil averages the eight and weighs
+ * central_weight = 100; weights 1/(IS + smallnum)^4 with IS = |grad|^2. */
+typedef struct {
+  double A[8][3][3];
+} weno_mats;
+
+/* stencil j uses (x-neighbour, y-neighbour, z-neighbour):
+ * 1 (R,U,+) 2 (R,U,-) 3 (L,U,+) 4 (L,U,-) 5 (R,D,+) 6 (R,D,-) 7 (L,D,+) 8 (L,D,-) */
+static const int c_sx[8] = {0, 0, 1, 1, 0, 0, 1, 1};  /* 0: R, 1: L */
+static const int c_sy[8] = {0, 0, 0, 0, 1, 1, 1, 1};  /* 0: U, 1: D */
+static const int c_sz[8] = {0, 1, 0, 1, 0, 1, 0, 1};  /* 0: +1, 1: -1 */
+
+/* neighbour zones of (i, j, k) in the order R, L, U, D, top, bottom */
+static void six_nbrs(const or_p1_patch* p, const p1dims* d, int i, int j, int k, long* z6) {
+  int a, b;
+  nbr(p, d, i, j, 0, &a, &b); z6[0] = zi(d, a, b, k);
+  nbr(p, d, i, j, 1, &a, &b); z6[1] = zi(d, a, b, k);
+  nbr(p, d, i, j, 2, &a, &b); z6[2] = zi(d, a, b, k);
+  nbr(p, d, i, j, 3, &a, &b); z6[3] = zi(d, a, b, k);
+  z6[4] = zi(d, i, j, k + 1);
+  z6[5] = zi(d, i, j, k - 1);
+}
+
+/* adjugate / determinant inverse of the 3x3 matrix of scaled centroid offsets */
+static void inv3(const double a[3][3], double tiny, double r[3][3]) {
+  double a11 = a[0][0], a12 = a[0][1], a13 = a[0][2], a21 = a[1][0], a22 = a[1][1], a23 = a[1][2], a31 = a[2][0],
+         a32 = a[2][1], a33 = a[2][2];
+  double det = -(a13 * a22 * a31) + a12 * a23 * a31 + a13 * a21 * a32 - a11 * a23 * a32 - a12 * a21 * a33 + a11 * a22 * a33;
+  double od = (signbit(det) ? -1.0 : 1.0) / omax(tiny, fabs(det));
+  r[0][0] = (-(a23 * a32) + a22 * a33) * od;
+  r[0][1] = (a13 * a32 - a12 * a33) * od;
+  r[0][2] = (-(a13 * a22) + a12 * a23) * od;
+  r[1][0] = (a23 * a31 - a21 * a33) * od;
+  r[1][1] = (-(a13 * a31) + a11 * a33) * od;
+  r[1][2] = (a13 * a21 - a11 * a23) * od;
+  r[2][0] = (-(a22 * a31) + a21 * a32) * od;
+  r[2][1] = (a12 * a31 - a11 * a32) * od;
+  r[2][2] = (-(a12 * a21) + a11 * a22) * od;
+}
+
+static void weno_build(const or_p1_patch* p, const p1dims* d, long z0, const long* z6, weno_mats* W) {
+  double obl = obl_of(p, z0), off[6][3];
+  for (int n = 0; n < 6; ++n)
+    for (int c = 0; c < 3; ++c) off[n][c] = (p->v_centroid[z6[n] + d->NZ * c] - p->v_centroid[z0 + d->NZ * c]) * obl;
+  for (int s = 0; s < 8; ++s) {
+    double a[3][3];
+    for (int c = 0; c < 3; ++c) {
+      a[0][c] = off[c_sx[s]][c];
+      a[1][c] = off[2 + c_sy[s]][c];
+      a[2][c] = off[4 + c_sz[s]][c];
+    }
+    inv3(a, 1.0e-32, W->A[s]);
+  }
+}
+
+/* gradient modes of a field with value f0 at the zone and f6 at its six
+ * neighbours (R, L, U, D, top, bottom), times (1 - flatten) */
+static void weno_apply(const weno_mats* W, double f0, const double* f6, double smallnum, double flat, double* g3) {
+  double X[9][3], IS[9], w[9], sum;
+  for (int s = 0; s < 8; ++s) {
+    double r0 = f6[c_sx[s]] - f0, r1 = f6[2 + c_sy[s]] - f0, r2 = f6[4 + c_sz[s]] - f0;
+    for (int m = 0; m < 3; ++m) X[s][m] = W->A[s][m][0] * r0 + W->A[s][m][1] * r1 + W->A[s][m][2] * r2;
+  }
+  for (int m = 0; m < 3; ++m)
+    X[8][m] = 0.125 * (X[0][m] + X[1][m] + X[2][m] + X[3][m] + X[4][m] + X[5][m] + X[6][m] + X[7][m]);
+  for (int s = 0; s < 9; ++s) {
+    IS[s] = X[s][0] * X[s][0] + X[s][1] * X[s][1] + X[s][2] * X[s][2];
+    double t = IS[s] + smallnum;
+    t = t * t;
+    w[s] = (s == 8 ? 100.0 : 1.0) / (t * t);
+  }
+  sum = 1.0 / (w[0] + w[1] + w[2] + w[3] + w[4] + w[5] + w[6] + w[7] + w[8]);
+  for (int s = 0; s < 9; ++s) w[s] = w[s] * sum;
+  for (int m = 0; m < 3; ++m) {
+    double v = w[0] * X[0][m];
+    for (int s = 1; s < 9; ++s) v = v + w[s] * X[s][m];
+    g3[m] = v * (1.0 - flat);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Numerical Recipes' LU decomposition with implicit (row-scaled) partial
+ * pivoting (Crout), then forward/back substitution: Press et al., Numerical
+ * Recipes, 2nd ed., sec. 2.3, as called through MATSOLVE (120matsolve.f). */
+#define LUMAX 18
+static int lu_solve(int n, double a[LUMAX][LUMAX], double* b) {
+  int indx[LUMAX], imax = 0;
+  double vv[LUMAX];
+  for (int i = 0; i < n; ++i) {
+    double big = 0.0;
+    for (int j = 0; j < n; ++j)
+      if (fabs(a[i][j]) > big) big = fabs(a[i][j]);
+    if (fabs(big) < 1.0e-30) return 1;
+    vv[i] = 1.0 / big;
+  }
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i < j; ++i) {
+      double s = a[i][j];
+      for (int k = 0; k < i; ++k) s = s - a[i][k] * a[k][j];
+      a[i][j] = s;
+    }
+    double big = 0.0;
+    for (int i = j; i < n; ++i) {
+      double s = a[i][j];
+      for (int k = 0; k < j; ++k) s = s - a[i][k] * a[k][j];
+      a[i][j] = s;
+      double t = vv[i] * fabs(s);
+      if (t >= big) {
+        big = t;
+        imax = i;
+      }
+    }
+    if (j != imax) {
+      for (int k = 0; k < n; ++k) {
+        double t = a[imax][k];
+        a[imax][k] = a[j][k];
+        a[j][k] = t;
+      }
+      vv[imax] = vv[j];
+    }
+    indx[j] = imax;
+    if (fabs(a[j][j]) < 1.0e-30) return 1;
+    if (j != n - 1) {
+      double r = 1.0 / a[j][j];
+      for (int i = j + 1; i < n; ++i) a[i][j] = a[i][j] * r;
+    }
+  }
+  int ii = -1;
+  for (int i = 0; i < n; ++i) {
+    int ip = indx[i];
+    double s = b[ip];
+    b[ip] = b[i];
+    if (ii >= 0)
+      for (int j = ii; j < i; ++j) s = s - a[i][j] * b[j];
+    else if (s != 0.0)
+      ii = i;
+    b[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int j = i + 1; j < n; ++j) s = s - a[i][j] * b[j];
+    b[i] = s / a[i][i];
+  }
+  return 0;
+}
+
+/* one row [n_x p, n_y p, n_z p], p = (1, x, y, z) */
+static void bfield_row(const double* n, const double* r, double* row) {
+  for (int c = 0; c < 3; ++c) {
+    row[4 * c + 0] = n[c];
+    row[4 * c + 1] = n[c] * r[0];
+    row[4 * c + 2] = n[c] * r[1];
+    row[4 * c + 3] = n[c] * r[2];
+  }
+}
+
+/* zones of the last call whose slopes the positivity limiter reduced, and
+   those it flattened completely */
+static long g_p1_stats[2];
+void or_p1_positivity_stats(long* out) {
+  out[0] = g_p1_stats[0];
+  out[1] = g_p1_stats[1];
+}
+
+/* ------------------------------------------------------------------------ */
+int or_limit_all_slopes_p1(const or_p1_patch* p) {
+  if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1) return 1;
+  const p1dims D = p1_dims(p), *d = &D;
+  const int N = d->N, NR = d->NR, g = d->g;
+  g_p1_stats[0] = g_p1_stats[1] = 0;
+  const double pi = 4.0 * atan(1.0), ob8pi = 1.0 / (8.0 * pi), ob4pi = 1.0 / (4.0 * pi), gam = p->gamma;
+  int i, j, k, c, m, q;
+
+  /* STEP 2 (410...f:529-720): constants, zero the higher modes, make the
+     tri-corner (inactive) zones safe; zero face B between two inactive zones */
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        long z = zi(d, i, j, k);
+        p->gamma_eos[z] = gam;
+        p->gamma_sound[z] = gam;
+        p->mean_mol_wt[z] = p->molewt;
+        p->msonic[z] = p->smallnum;
+        for (c = 0; c < 8; ++c)
+          for (m = 1; m < 4; ++m) UU(p, d, c, m, z) = 0.0;
+        for (m = 1; m < 4; ++m) p->ent[z + d->NZ * m] = 0.0;
+        for (c = 0; c < 8; ++c)
+          for (m = 0; m < 4; ++m) VV(p, d, c, m, z) = 0.0;
+        if (p->mask[z] == 0) {
+          for (c = 0; c < 8; ++c) UU(p, d, c, 0, z) = 0.0;
+          UU(p, d, 0, 0, z) = p->rhofloor;
+          UU(p, d, 4, 0, z) = p->prsfloor / (gam - 1.0);
+          p->scale[z] = p->smallnum;
+        }
+      }
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = -g; i <= N + g; ++i) {
+        int a = i < 1 - g ? 1 - g : i, b = i + 1 > N + g ? N + g : i + 1;
+        if (!active(p, d, a, j, k) && !active(p, d, b, j, k)) p->Bxf[fci(d, i, j, k)] = 0.0;
+      }
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = -g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        int a = j < 1 - g ? 1 - g : j, b = j + 1 > N + g ? N + g : j + 1;
+        if (!active(p, d, i, a, k) && !active(p, d, i, b, k)) p->Byf[fci(d, i, j, k)] = 0.0;
+      }
+  for (k = -g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i)
+        if (!active(p, d, i, j, 1)) p->Bzf[fci(d, i, j, k)] = 0.0;
+
+  /* STEP 3 (410...f:722-920): at a tri-corner, the y-face B of the ghost row
+     next to the panel edge and the x-face B it coincides with (the Down or Up
+     neighbour's) are made equal, with the sign each corner's orientation gives */
+  for (int corner = 0; corner < 4; ++corner) {
+    if (!p->itri[corner]) continue;
+    int jj = (corner < 2) ? 1 : N, lo = (corner == 0 || corner == 3) ? 1 - g : N + 1,
+        hi = (corner == 0 || corner == 3) ? 0 : N + g;
+    for (i = lo; i <= hi; ++i) {
+      int a, b;
+      nbr(p, d, i, jj, corner < 2 ? 3 : 2, &a, &b);
+      for (k = 1 - g; k <= NR + g; ++k) {
+        double* by = &p->Byf[fci(d, i, corner < 2 ? jj - 1 : jj, k)];
+        double* bx = &p->Bxf[fci(d, (corner == 0 || corner == 3) ? a - 1 : a, b, k)];
+        if (corner == 0 || corner == 2) { /* LD, RU: opposite signs */
+          double t = 0.5 * (*by - *bx);
+          *by = t;
+          *bx = -t;
+        } else { /* RD, LU: same signs */
+          double t = 0.5 * (*by + *bx);
+          *by = t;
+          *bx = t;
+        }
+      }
+    }
+  }
+
+  /* STEP 5 (410...f:1421-1740): primitives of every zone, magnetosonic speed */
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        long z = zi(d, i, j, k);
+        double Up[8], Vp[8], e = p->ent[z];
+        for (c = 0; c < 8; ++c) Up[c] = UU(p, d, c, 0, z);
+        or_cons_to_prim(gam, p->rhofloor, p->prsfloor, &e, Up, Vp);
+        for (c = 0; c < 8; ++c) {
+          VV(p, d, c, 0, z) = Vp[c];
+          UU(p, d, c, 0, z) = Up[c];
+        }
+        p->ent[z] = e;
+        for (m = 1; m < 4; ++m) p->ent[z + d->NZ * m] = 0.0;
+        double b2 = (Vp[5] * Vp[5] + Vp[6] * Vp[6] + Vp[7] * Vp[7]) * ob4pi;
+        double ms = p->gamma_sound[z] * Vp[4] + b2;
+        p->msonic[z] = sqrt(ms / Vp[0]);
+      }
+
+  /* STEP 6.a (410...f:1748-2090): divergence of the velocity through the six
+     faces, scaled to the zone; flattener from |div v| / (0.4 min c_f) */
+  const int f0 = 2 - g, fx1 = N + g - 1, fz1 = NR + g - 1;
+  for (long z = 0; z < d->NZ; ++z) p->flatten[z] = 0.0;
+  for (k = f0; k <= fz1; ++k)
+    for (j = f0; j <= fx1; ++j)
+      for (i = f0; i <= fx1; ++i) {
+        long z = zi(d, i, j, k), z6[6];
+        six_nbrs(p, d, i, j, k, z6);
+        double obl = obl_of(p, z), dv = 0.0, wmin = p->msonic[z];
+        for (q = 0; q < 6; ++q) wmin = omin(wmin, p->msonic[z6[q]]);
+        p->work[z] = wmin;
+        for (q = 0; q < 6; ++q) {
+          /* faces R, L, U, D use the face normal; top and bottom the unit z-face centroid */
+          long zo = (q % 2 == 0) ? z6[q] : z, zi_ = (q % 2 == 0) ? z : z6[q];
+          double r[3], n[3], s2 = 0.0;
+          for (c = 0; c < 3; ++c) r[c] = (p->v_centroid[zo + d->NZ * c] - p->v_centroid[zi_ + d->NZ * c]) * obl;
+          double a11 = 0.5 / omax(sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), p->smallnum);
+          if (q < 2) {
+            long f = xfi(d, q == 0 ? i : i - 1, j, k);
+            for (c = 0; c < 3; ++c) n[c] = p->xf_normal[f + d->NXF * c];
+          } else if (q < 4) {
+            long f = yfi(d, i, q == 2 ? j : j - 1, k);
+            for (c = 0; c < 3; ++c) n[c] = p->yf_normal[f + d->NYF * c];
+          } else {
+            long f = zfi(d, i, j, k);
+            for (c = 0; c < 3; ++c) n[c] = p->zf_centroid[f + d->NZF * c];
+            double a12 = 1.0 / omax(sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]), p->smallnum);
+            for (c = 0; c < 3; ++c) n[c] = n[c] * a12;
+          }
+          (void)s2;
+          dv = dv + n[0] * a11 * (VV(p, d, 1, 0, zo) - VV(p, d, 1, 0, zi_)) +
+               n[1] * a11 * (VV(p, d, 2, 0, zo) - VV(p, d, 2, 0, zi_)) +
+               n[2] * a11 * (VV(p, d, 3, 0, zo) - VV(p, d, 3, 0, zi_));
+        }
+        p->divvel[z] = dv;
+        double a13 = fabs(dv) / (p->flatten_coef * p->work[z]) - 1.0;
+        p->flatten[z] = a13 <= 0.0 ? 0.0 : omin(1.0, a13);
+        if (p->mask[z] != 1) p->flatten[z] = 0.0;
+      }
+  /* STEP 6.b (:2094-2310): take a neighbour's flattener where its pressure
+     exceeds prs_flatten_coef times this zone's */
+  memcpy(p->work, p->flatten, sizeof(double) * d->NZ);
+  for (k = f0; k <= fz1; ++k)
+    for (j = f0; j <= fx1; ++j)
+      for (i = f0; i <= fx1; ++i) {
+        long z = zi(d, i, j, k), z6[6];
+        six_nbrs(p, d, i, j, k, z6);
+        for (q = 0; q < 6; ++q)
+          if (VV(p, d, 4, 0, z6[q]) > p->prs_flatten_coef * VV(p, d, 4, 0, z))
+            p->flatten[z] = omax(p->flatten[z], p->work[z6[q]]);
+      }
+  /* STEP 6.c (:2317-2418): spread the flattener to the six neighbours */
+  memcpy(p->work, p->flatten, sizeof(double) * d->NZ);
+  for (k = f0; k <= fz1; ++k)
+    for (j = f0; j <= fx1; ++j)
+      for (i = f0; i <= fx1; ++i) {
+        long z = zi(d, i, j, k), z6[6];
+        six_nbrs(p, d, i, j, k, z6);
+        for (q = 0; q < 6; ++q) p->flatten[z] = omax(p->flatten[z], p->work[z6[q]]);
+      }
+
+  /* STEP 7.a (:2421-3761): WENO gradients of the 8 primitives and the entropy */
+  for (k = -2; k <= NR + 3; ++k)
+    for (j = -2; j <= N + 3; ++j)
+      for (i = -2; i <= N + 3; ++i) {
+        if (!active(p, d, i, j, 1)) continue;
+        long z = zi(d, i, j, k), z6[6];
+        weno_mats W;
+        six_nbrs(p, d, i, j, k, z6);
+        weno_build(p, d, z, z6, &W);
+        for (c = 0; c < 9; ++c) {
+          double f6[6], g3[3], f0v;
+          if (c < 8) {
+            f0v = VV(p, d, c, 0, z);
+            for (q = 0; q < 6; ++q) f6[q] = VV(p, d, c, 0, z6[q]);
+          } else {
+            f0v = p->ent[z];
+            for (q = 0; q < 6; ++q) f6[q] = p->ent[z6[q]];
+          }
+          weno_apply(&W, f0v, f6, p->smallnum, p->flatten[z], g3);
+          for (m = 0; m < 3; ++m) {
+            if (c < 8)
+              VV(p, d, c, m + 1, z) = g3[m];
+            else
+              p->ent[z + d->NZ * (m + 1)] = g3[m];
+          }
+        }
+      }
+
+  /* STEP 7.b: ENFORCE_PRESSURE_POSITIVITY_PRIM (440...f:6-830) */
+  {
+    const int n_iter = 15;
+    const double ratio = pow(0.10, 1.0 / (double)n_iter), expandfac = 0.2;
+    for (k = -2; k <= NR + 3; ++k)
+      for (j = -2; j <= N + 3; ++j)
+        for (i = -2; i <= N + 3; ++i) {
+          long z = zi(d, i, j, k), col[9];
+          int a, b;
+          /* the 27 zones around: the column itself and its 8 stencil neighbours
+             (R, L, U, D, RU, LU, LD, RD), each at k-1, k, k+1 */
+          static const int dirs[8] = {0, 1, 2, 3, 8, 9, 10, 11};
+          col[0] = zi(d, i, j, 0);
+          for (q = 0; q < 8; ++q) {
+            nbr(p, d, i, j, dirs[q], &a, &b);
+            col[q + 1] = zi(d, a, b, 0);
+          }
+          double rmin = 1e300, rmax = -1e300, pmin = 1e300, pmax = -1e300, emin = 1e300, emax = -1e300;
+          const long kstride = (long)d->nz * d->nz;
+          for (q = 0; q < 9; ++q)
+            for (int dk = -1; dk <= 1; ++dk) {
+              long zz = col[q] + kstride * (k - 0 + dk);
+              rmin = omin(rmin, VV(p, d, 0, 0, zz));
+              rmax = omax(rmax, VV(p, d, 0, 0, zz));
+              pmin = omin(pmin, VV(p, d, 4, 0, zz));
+              pmax = omax(pmax, VV(p, d, 4, 0, zz));
+              emin = omin(emin, p->ent[zz]);
+              emax = omax(emax, p->ent[zz]);
+            }
+          double ff = omax(omin((p->flatten[z] - 0.1) / 0.4, 1.0), 0.0), omff = 1.0 - ff;
+          rmin = omax(rmin * (1.0 - expandfac * omff), p->rhofloor);
+          rmax = omax(rmax * (1.0 + expandfac * omff), p->rhofloor);
+          pmin = omax(pmin * (1.0 - expandfac * omff), p->prsfloor);
+          pmax = omax(pmax * (1.0 + expandfac * omff), p->prsfloor);
+          emin = emin * (1.0 - expandfac * omff);
+          emax = emax * (1.0 + expandfac * omff);
+          /* 27 nodes: 8 vertices, 6 face centroids, 12 edge midpoints, centroid */
+          double node[27][3], cen[3], obl = obl_of(p, z);
+          for (c = 0; c < 3; ++c) cen[c] = p->v_centroid[z + d->NZ * c];
+          for (int v = 0; v < 8; ++v) {
+            int vi = (v & 1) ? i : i - 1, vj = (v & 2) ? j : j - 1, vk = (v & 4) ? k : k - 1;
+            for (c = 0; c < 3; ++c) node[v][c] = p->vert[fci(d, vi, vj, vk) + d->NV * c];
+          }
+          for (c = 0; c < 3; ++c) {
+            node[8][c] = p->xf_centroid[xfi(d, i, j, k) + d->NXF * c];
+            node[9][c] = p->xf_centroid[xfi(d, i - 1, j, k) + d->NXF * c];
+            node[10][c] = p->yf_centroid[yfi(d, i, j, k) + d->NYF * c];
+            node[11][c] = p->yf_centroid[yfi(d, i, j - 1, k) + d->NYF * c];
+            node[12][c] = p->zf_centroid[zfi(d, i, j, k) + d->NZF * c];
+            node[13][c] = p->zf_centroid[zfi(d, i, j, k - 1) + d->NZF * c];
+          }
+          static const int edge[12][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3},
+                                          {4, 6}, {5, 7}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+          for (int e = 0; e < 12; ++e)
+            for (c = 0; c < 3; ++c) node[14 + e][c] = (node[edge[e][0]][c] + node[edge[e][1]][c]) * 0.5;
+          for (c = 0; c < 3; ++c) node[26][c] = cen[c];
+          for (int n = 0; n < 27; ++n)
+            for (c = 0; c < 3; ++c) node[n][c] = (node[n][c] - cen[c]) * obl;
+          double Vm[8][4], em[4];
+          for (c = 0; c < 8; ++c)
+            for (m = 0; m < 4; ++m) Vm[c][m] = VV(p, d, c, m, z);
+          for (m = 0; m < 4; ++m) em[m] = p->ent[z + d->NZ * m];
+          for (int it = 1; it <= n_iter + 1; ++it) {
+            if (it == 2) ++g_p1_stats[0];
+            if (it == n_iter + 1) {
+              ++g_p1_stats[1];
+              for (c = 0; c < 8; ++c)
+                for (m = 1; m < 4; ++m) Vm[c][m] = 0.0;
+              for (m = 0; m < 4; ++m) em[m] = 0.0;
+              break;
+            }
+            int pass = 1;
+            for (int n = 0; n < 27; ++n) {
+              double x = node[n][0], y = node[n][1], zz = node[n][2];
+              double rn = Vm[0][0] + Vm[0][1] * x + Vm[0][2] * y + Vm[0][3] * zz;
+              double pn = Vm[4][0] + Vm[4][1] * x + Vm[4][2] * y + Vm[4][3] * zz;
+              double en = em[0] + em[1] * x + em[2] * y + em[3] * zz;
+              if (rn < rmin || rn > rmax) pass = 0;
+              if (pn < pmin || pn > pmax) pass = 0;
+              if (en < emin || en > emax) pass = 0;
+            }
+            if (pass) break;
+            for (c = 0; c < 8; ++c)
+              for (m = 1; m < 4; ++m) Vm[c][m] = ratio * Vm[c][m];
+            for (m = 1; m < 4; ++m) em[m] = ratio * em[m];
+          }
+          for (c = 0; c < 8; ++c)
+            for (m = 0; m < 4; ++m) VV(p, d, c, m, z) = Vm[c][m];
+          for (m = 0; m < 4; ++m) {
+            p->ent[z + d->NZ * m] = em[m];
+            p->B1_zc[z + d->NZ * m] = Vm[5][m];
+            p->B2_zc[z + d->NZ * m] = Vm[6][m];
+            p->B3_zc[z + d->NZ * m] = Vm[7][m];
+          }
+        }
+  }
+
+  /* STEP 8.a-c (:3874-4640): normal B at the five B-field points of each face
+     from the two zones' linear fields, averaged, then shifted so the weighted
+     mean is the face's own Bf */
+  for (int fam = 0; fam < 3; ++fam) {
+    const int ilo = fam == 0 ? -3 : -2, jlo = fam == 1 ? -3 : -2, klo = fam == 2 ? -3 : -2;
+    double* Bq = fam == 0 ? p->Bxf_q : (fam == 1 ? p->Byf_q : p->Bzf_q);
+    const double* Bf = fam == 0 ? p->Bxf : (fam == 1 ? p->Byf : p->Bzf);
+    const double* pts = fam == 0 ? p->xb_pts : (fam == 1 ? p->yb_pts : p->zb_pts);
+    const double* wts = fam == 0 ? p->xb_wts : (fam == 1 ? p->yb_wts : p->zb_wts);
+    const double* nrm = fam == 0 ? p->xb_normal : (fam == 1 ? p->yb_normal : p->zb_normal);
+    const long NF = fam == 0 ? d->NXF : (fam == 1 ? d->NYF : d->NZF);
+    for (k = klo; k <= NR + 3; ++k)
+      for (j = jlo; j <= N + 3; ++j)
+        for (i = ilo; i <= N + 3; ++i) {
+          long fb = fci(d, i, j, k), fg = fam == 0 ? xfi(d, i, j, k) : (fam == 1 ? yfi(d, i, j, k) : zfi(d, i, j, k));
+          int lo_ok, zl_i = i, zl_j = j, zr_i = i + (fam == 0), zr_j = j + (fam == 1), zr_k = k + (fam == 2);
+          int skip = 0;
+          if (fam < 2) {
+            int al = active(p, d, zl_i, zl_j, k), ar = active(p, d, zr_i, zr_j, k);
+            if (al && ar) {
+            } else if (!al && ar) {
+              nbr(p, d, zr_i, zr_j, fam == 0 ? 1 : 3, &zl_i, &zl_j);
+            } else if (al && !ar) {
+              nbr(p, d, zl_i, zl_j, fam == 0 ? 0 : 2, &zr_i, &zr_j);
+            } else {
+              skip = 1;
+            }
+          } else {
+            skip = !active(p, d, i, j, k);
+          }
+          (void)lo_ok;
+          for (q = 0; q < 5; ++q) {
+            if (skip) {
+              Bq[fb + d->NFC * q] = 0.0;
+              continue;
+            }
+            long zl = zi(d, zl_i, zl_j, k), zr = zi(d, zr_i, zr_j, fam == 2 ? zr_k : k);
+            double n[3], rl[3], rr[3], bl[3], br[3];
+            for (c = 0; c < 3; ++c) n[c] = fam == 2 ? nrm[fg + NF * (c + 3 * q)] : nrm[fg + NF * c];
+            rel(p, d, zl, pts + fg + NF * (3 * q), NF, obl_of(p, zl), rl);
+            rel(p, d, zr, pts + fg + NF * (3 * q), NF, obl_of(p, zr), rr);
+            for (c = 0; c < 3; ++c) {
+              bl[c] = VV(p, d, 5 + c, 0, zl) + VV(p, d, 5 + c, 1, zl) * rl[0] + VV(p, d, 5 + c, 2, zl) * rl[1] +
+                      VV(p, d, 5 + c, 3, zl) * rl[2];
+              br[c] = VV(p, d, 5 + c, 0, zr) + VV(p, d, 5 + c, 1, zr) * rr[0] + VV(p, d, 5 + c, 2, zr) * rr[1] +
+                      VV(p, d, 5 + c, 3, zr) * rr[2];
+            }
+            Bq[fb + d->NFC * q] = 0.5 * (n[0] * (bl[0] + br[0]) + n[1] * (bl[1] + br[1]) + n[2] * (bl[2] + br[2]));
+          }
+          double bd = Bf[fb];
+          for (q = 0; q < 5; ++q) bd = bd - wts[fg + NF * q] * Bq[fb + d->NFC * q];
+          for (q = 0; q < 5; ++q) Bq[fb + d->NFC * q] = Bq[fb + d->NFC * q] + bd;
+        }
+  }
+
+  /* STEP 8.d (:4643-5360): ADP constrained least squares for the 12 linear
+     coefficients of B: the six face means exactly (constraints), points 2 and
+     4 of each face and the WENO modes in the least-squares sense */
+  for (k = -2; k <= NR + 3; ++k)
+    for (j = -2; j <= N + 3; ++j)
+      for (i = -2; i <= N + 3; ++i) {
+        long z = zi(d, i, j, k);
+        if (p->mask[z] == 0) continue;
+        double obl = obl_of(p, z), Cm[6][12], dv[6], Am[24][12], bv[24], r[3], n[3];
+        memset(Am, 0, sizeof Am);
+        memset(bv, 0, sizeof bv);
+        /* faces in the order right x, left x, up y, down y, top z, bottom z */
+        for (int f = 0; f < 6; ++f) {
+          int fam = f / 2, hi = f % 2 == 0;
+          int fi_ = i - (fam == 0 && !hi), fj_ = j - (fam == 1 && !hi), fk_ = k - (fam == 2 && !hi);
+          long fg = fam == 0 ? xfi(d, fi_, fj_, fk_) : (fam == 1 ? yfi(d, fi_, fj_, fk_) : zfi(d, fi_, fj_, fk_));
+          long NF = fam == 0 ? d->NXF : (fam == 1 ? d->NYF : d->NZF);
+          long fb = fci(d, fi_, fj_, fk_);
+          const double* nrm = fam == 0 ? p->xb_normal : (fam == 1 ? p->yb_normal : p->zb_normal);
+          const double* cen = fam == 0 ? p->xf_centroid : (fam == 1 ? p->yf_centroid : p->zf_centroid);
+          const double* pts = fam == 0 ? p->xb_pts : (fam == 1 ? p->yb_pts : p->zb_pts);
+          const double* Bf = fam == 0 ? p->Bxf : (fam == 1 ? p->Byf : p->Bzf);
+          const double* Bq = fam == 0 ? p->Bxf_q : (fam == 1 ? p->Byf_q : p->Bzf_q);
+          for (c = 0; c < 3; ++c) n[c] = nrm[fg + NF * c];  /* z: the first (central) point's normal */
+          rel(p, d, z, cen + fg, NF, obl, r);
+          bfield_row(n, r, Cm[f]);
+          dv[f] = Bf[fb];
+          for (int t = 0; t < 2; ++t) {
+            int pt = t == 0 ? 1 : 3;  /* points 2 and 4 */
+            if (fam == 2)
+              for (c = 0; c < 3; ++c) n[c] = nrm[fg + NF * (c + 3 * pt)];
+            rel(p, d, z, pts + fg + NF * (3 * pt), NF, obl, r);
+            bfield_row(n, r, Am[2 * f + t]);
+            bv[2 * f + t] = Bq[fb + d->NFC * pt];
+          }
+        }
+        for (q = 0; q < 12; ++q) {
+          Am[12 + q][q] = 1.0;
+          bv[12 + q] = VV(p, d, 5 + q / 4, q % 4, z);
+        }
+        double K[LUMAX][LUMAX], rhs[LUMAX];
+        for (int a = 0; a < 12; ++a) {
+          for (int b = 0; b < 12; ++b) {
+            double s = 0.0;
+            for (q = 0; q < 24; ++q) s = s + Am[q][a] * Am[q][b];
+            K[a][b] = s;
+          }
+          double s = 0.0;
+          for (q = 0; q < 24; ++q) s = s + Am[q][a] * bv[q];
+          rhs[a] = s;
+          for (int f = 0; f < 6; ++f) {
+            K[a][12 + f] = Cm[f][a];
+            K[12 + f][a] = Cm[f][a];
+          }
+        }
+        for (int f = 0; f < 6; ++f) {
+          for (int h = 0; h < 6; ++h) K[12 + f][12 + h] = 0.0;
+          rhs[12 + f] = dv[f];
+        }
+        lu_solve(18, K, rhs);
+        double aa = (UU(p, d, 5, 0, z) * UU(p, d, 5, 0, z) + UU(p, d, 6, 0, z) * UU(p, d, 6, 0, z) +
+                     UU(p, d, 7, 0, z) * UU(p, d, 7, 0, z)) * ob8pi;
+        for (m = 0; m < 4; ++m) {
+          p->B1_zc[z + d->NZ * m] = rhs[m];
+          p->B2_zc[z + d->NZ * m] = rhs[4 + m];
+          p->B3_zc[z + d->NZ * m] = rhs[8 + m];
+          for (c = 0; c < 3; ++c) VV(p, d, 5 + c, m, z) = rhs[4 * c + m];
+        }
+        for (c = 0; c < 3; ++c) UU(p, d, 5 + c, 0, z) = VV(p, d, 5 + c, 0, z);
+        double bb = (UU(p, d, 5, 0, z) * UU(p, d, 5, 0, z) + UU(p, d, 6, 0, z) * UU(p, d, 6, 0, z) +
+                     UU(p, d, 7, 0, z) * UU(p, d, 7, 0, z)) * ob8pi;
+        UU(p, d, 4, 0, z) = UU(p, d, 4, 0, z) - aa + bb;
+      }
+
+  /* STEP 9.a (:5373-5430): zone electric field E = -v x B */
+  for (long z = 0; z < d->NZ; ++z) {
+    double vx = VV(p, d, 1, 0, z), vy = VV(p, d, 2, 0, z), vz = VV(p, d, 3, 0, z);
+    double Bx = 0.0 + VV(p, d, 5, 0, z), By = 0.0 + VV(p, d, 6, 0, z), Bz = 0.0 + VV(p, d, 7, 0, z);
+    p->Ex_zc[z] = -(vy * Bz - vz * By);
+    p->Ey_zc[z] = -(vz * Bx - vx * Bz);
+    p->Ez_zc[z] = -(vx * By - vy * Bx);
+  }
+  /* STEP 9.b (:5437-7100): WENO gradients of E, same stencils and weights */
+  for (k = -2; k <= NR + 3; ++k)
+    for (j = -2; j <= N + 3; ++j)
+      for (i = -2; i <= N + 3; ++i) {
+        if (!active(p, d, i, j, 1)) continue;
+        long z = zi(d, i, j, k), z6[6];
+        weno_mats W;
+        six_nbrs(p, d, i, j, k, z6);
+        weno_build(p, d, z, z6, &W);
+        double* E[3] = {p->Ex_zc, p->Ey_zc, p->Ez_zc};
+        for (c = 0; c < 3; ++c) {
+          double f6[6], g3[3];
+          for (q = 0; q < 6; ++q) f6[q] = E[c][z6[q]];
+          weno_apply(&W, E[c][z], f6, p->smallnum, p->flatten[z], g3);
+          for (m = 0; m < 3; ++m) E[c][z + d->NZ * (m + 1)] = g3[m];
+        }
+      }
+
+  /* STEP 10 (:7101-7700): min and max of the primitives over the owned zones,
+     the face-B divergence (kept in work) normalised by |B| and the
+     magnetosonic speed (recomputed from the new primitives) */
+  double vmin[9], vmax[9];
+  for (q = 0; q < 9; ++q) {
+    vmin[q] = 1.0e30;
+    vmax[q] = -1.0e30;
+  }
+  for (long z = 0; z < d->NZ; ++z) p->work[z] = 0.0;
+  for (k = 1; k <= NR; ++k)
+    for (j = 1; j <= N; ++j)
+      for (i = 1; i <= N; ++i) {
+        long z = zi(d, i, j, k);
+        if (p->mask[z] != 1) continue;
+        for (c = 0; c < 8; ++c) {
+          vmin[c] = omin(vmin[c], VV(p, d, c, 0, z));
+          vmax[c] = omax(vmax[c], VV(p, d, c, 0, z));
+        }
+        long xp = xfi(d, i, j, k), xm = xfi(d, i - 1, j, k), yp = yfi(d, i, j, k), ym = yfi(d, i, j - 1, k),
+             zp = zfi(d, i, j, k), zm = zfi(d, i, j, k - 1);
+        double bxp = p->Bxf[fci(d, i, j, k)], bxm = p->Bxf[fci(d, i - 1, j, k)], byp = p->Byf[fci(d, i, j, k)],
+               bym = p->Byf[fci(d, i, j - 1, k)], bzp = p->Bzf[fci(d, i, j, k)], bzm = p->Bzf[fci(d, i, j, k - 1)];
+        double a21 = p->xf_area[xp] * bxp + p->yf_area[yp] * byp + p->zf_area[zp] * bzp - p->xf_area[xm] * bxm -
+                     p->yf_area[ym] * bym - p->zf_area[zm] * bzm;
+        p->work[z] = a21;
+        double v6 = VV(p, d, 5, 0, z), v7 = VV(p, d, 6, 0, z), v8 = VV(p, d, 7, 0, z);
+        double a11 = (v6 + 0.0) * (v6 + 0.0) + (v7 + 0.0) * (v7 + 0.0) + (v8 + 0.0) * (v8 + 0.0);
+        double ms = p->gamma_sound[z] * VV(p, d, 4, 0, z) + ob4pi * a11;
+        p->msonic[z] = sqrt(ms / VV(p, d, 0, 0, z));
+        double den = omax(omax(sqrt(bxp * bxp + byp * byp + bzp * bzp), sqrt(bxm * bxm + bym * bym + bzm * bzm)),
+                          p->msonic[z] * sqrt(4.0 * 3.14 * VV(p, d, 0, 0, z)));
+        a21 = a21 / den;
+        vmin[8] = omin(vmin[8], a21);
+        vmax[8] = omax(vmax[8], a21);
+      }
+  if (p->V_min_local)
+    for (q = 0; q < 9; ++q) p->V_min_local[q] = vmin[q];
+  if (p->V_max_local)
+    for (q = 0; q < 9; ++q) p->V_max_local[q] = vmax[q];
+  return 0;
+}

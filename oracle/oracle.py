@@ -212,3 +212,157 @@ def quad_geometry(cfg, use_reference=False):
     L.or_basis_moments(ctypes.byref(cfg), P(a["mask"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_cen"]), P(a["scale"]),
                        P(a["moments"]))
     return a
+
+
+# ---------------------------------------------------------------------------
+# LIMIT_ALL_SLOPES_P1 on one patch (or_limit_all_slopes_p1 / the reference's
+# routine / cubesphere_amd.limit_all_slopes_p1 share this structure layout).
+
+P1_GEOM_FIELDS = ("vert", "mask", "stencil", "xf_normal", "xf_area", "xf_centroid", "yf_normal", "yf_area",
+                  "yf_centroid", "zf_area", "zf_centroid", "xb_pts", "xb_wts", "xb_normal", "yb_pts", "yb_wts",
+                  "yb_normal", "zb_pts", "zb_wts", "zb_normal", "scale", "v_centroid")
+P1_STATE_FIELDS = ("U", "ent", "Bxf", "Byf", "Bzf", "V", "B1_zc", "B2_zc", "B3_zc", "Ex_zc", "Ey_zc", "Ez_zc",
+                   "gamma_eos", "gamma_sound", "mean_mol_wt", "divvel", "msonic", "flatten", "work",
+                   "Bxf_q", "Byf_q", "Bzf_q")
+P1_SCALARS = ("gamma", "molewt", "rhofloor", "prsfloor", "smallnum", "flatten_coef", "prs_flatten_coef")
+
+
+class P1Patch(ctypes.Structure):
+    _fields_ = ([("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                 ("itri", ctypes.c_int32 * 4), ("pad", ctypes.c_int32)]
+                + [(k, ctypes.c_double) for k in P1_SCALARS]
+                + [(k, ctypes.c_void_p) for k in P1_GEOM_FIELDS + P1_STATE_FIELDS]
+                + [("V_min_local", ctypes.c_void_p), ("V_max_local", ctypes.c_void_p)])
+
+
+def p1_dims(N, NR, g):
+    nz, nzr = N + 2 * g, NR + 2 * g
+    nf, nfr = nz + 1, nzr + 1
+    return dict(zone=nz * nz * nzr, xf=nf * nz * nzr, zf=nz * nz * nfr, fc=nf * nf * nfr)
+
+
+def p1_struct(prob, arrays, addr=None):
+    """A P1Patch over the arrays of a problem; addr(array) gives each array's
+    address (host numpy by default)."""
+    addr = addr or (lambda a: a.ctypes.data)
+    s = P1Patch()
+    s.ntp, s.nr, s.nghost = prob["N"], prob["NR"], prob["g"]
+    for q in range(4):
+        s.itri[q] = 1
+    for k in P1_SCALARS:
+        setattr(s, k, prob[k])
+    for k in P1_GEOM_FIELDS + P1_STATE_FIELDS:
+        setattr(s, k, addr(arrays[k]))
+    s.V_min_local = arrays["V_min_local"].ctypes.data
+    s.V_max_local = arrays["V_max_local"].ctypes.data
+    return s
+
+
+def p1_problem(N, NR, g=4, sector=0, seed=0, shock=True, rmin=1.0, rmax=1.6, gamma=5.0 / 3.0, junk=True,
+               noise=0.0):
+    """One sector of an (N, NR, g) shell as a patch: the quadrature geometry of
+    the oracle (pinned to the reference's own routines), the stencil table,
+    and a state with smooth flow, a radial-velocity jump and a pressure jump
+    (so the flattener acts); noise > 0 multiplies density, pressure and
+    entropy by log-normal zone noise (so the positivity limiter acts).  Output arrays start
+    filled with seeded junk so that untouched entries are compared too."""
+    import cubesphere_amd as cs  # Config layout only
+    cfg = cs.make_config(N, NR, g, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2, rmin=rmin, rmax=rmax,
+                         gamma=gamma)
+    geo = quad_geometry(cfg)
+    gd = geometry_dims(cfg)
+    sl = lambda a, n, m: np.ascontiguousarray(a[sector * n * m:(sector + 1) * n * m])
+    d = p1_dims(N, NR, g)
+    st = np.zeros((N + 2 * g) ** 2 * 24, np.int32)
+    L = olib()
+    L.or_stencil_indexing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    L.or_stencil_indexing(N, g, P(st))
+    A = dict(vert=sl(geo["vert"], gd["vert"], 3), mask=sl(geo["mask"], gd["zone"], 1), stencil=st,
+             scale=sl(geo["scale"], gd["zone"], 1), v_centroid=sl(geo["v_cen"], gd["zone"], 3))
+    for f, n in (("xf", "xf"), ("yf", "yf"), ("zf", "zf")):
+        nn = gd[n]
+        A[f + "_normal"] = sl(geo[f]["normal"], nn, 15 if f == "zf" else 3)
+        A[f + "_area"] = sl(geo[f]["area"], nn, 1)
+        A[f + "_centroid"] = sl(geo[f]["centroid"], nn, 3)
+        b = f[0] + "b"
+        A[b + "_pts"] = sl(geo[b]["pts"], nn, 15)
+        A[b + "_wts"] = sl(geo[b]["wts"], nn, 5)
+        A[b + "_normal"] = sl(geo[b]["normal"], nn, 15 if f == "zf" else 3)
+    del A["zf_normal"]
+    rng = np.random.default_rng(seed)
+    nzone, nfc = d["zone"], d["fc"]
+    cen = A["v_centroid"].reshape(3, nzone)
+    r = np.sqrt((cen ** 2).sum(0))
+    x, y, z = cen / np.maximum(r, 1e-30)
+    rho = 1.0 + 0.3 * np.sin(3.0 * x + 1.0) * np.cos(2.0 * y) + 0.2 * (r - rmin)
+    prs = 1.0 + 0.2 * np.cos(2.5 * z - 0.5 * x)
+    vel = np.stack([0.3 * y - 0.2 * z, -0.3 * x + 0.1 * z, 0.2 * x * y])
+    if shock:
+        # a front across the middle of the patch: s = r_hat . t, t tangent to the
+        # patch centre; converging flow within a band around it
+        act = A["mask"] == 1
+        c0 = np.array([x[act].mean(), y[act].mean(), z[act].mean()])
+        c0 /= np.linalg.norm(c0)
+        t = np.cross(c0, [0.3, 0.5, 0.8])
+        t /= np.linalg.norm(t)
+        sfr = x * t[0] + y * t[1] + z * t[2] - 0.1
+        front = sfr > 0.0
+        rho = np.where(front, 3.5 * rho, rho)
+        prs = np.where(front, 12.0 * prs, prs)
+        vel = vel - 2.5 * np.where(front, 1.0, -1.0)[None] * t[:, None] * (np.abs(sfr) < 0.12)[None]
+    if noise > 0.0:
+        rho = rho * np.exp(noise * rng.normal(size=nzone))
+        prs = prs * np.exp(noise * rng.normal(size=nzone))
+    Bc = np.stack([0.5 + 0.2 * y, -0.3 + 0.1 * z * x, 0.2 * np.cos(2 * x)])
+    Ub = np.zeros((8, nzone))
+    Ub[0] = rho
+    Ub[1:4] = rho * vel
+    Ub[4] = prs / (gamma - 1.0) + 0.5 * rho * (vel ** 2).sum(0) + (Bc ** 2).sum(0) / (8.0 * np.pi)
+    Ub[5:8] = Bc
+    U = rng.normal(size=(4, 8, nzone))
+    U[0] = Ub
+    ent = rng.normal(size=(4, nzone))
+    ent[0] = prs / rho ** (gamma - 1.0) * (1.0 + 0.01 * rng.uniform(-1, 1, nzone))
+    if noise > 0.0:
+        ent[0] = ent[0] * np.exp(noise * rng.normal(size=nzone))
+    A["U"] = U.ravel().copy()
+    A["ent"] = ent.ravel().copy()
+    for q, nm in enumerate(("Bxf", "Byf", "Bzf")):
+        A[nm] = (0.4 * np.sin(np.arange(nfc) * (0.37 + 0.11 * q)) + 0.1 * q).astype(np.float64)
+    sizes = dict(V=32 * nzone, B1_zc=4 * nzone, B2_zc=4 * nzone, B3_zc=4 * nzone, Ex_zc=4 * nzone,
+                 Ey_zc=4 * nzone, Ez_zc=4 * nzone, gamma_eos=nzone, gamma_sound=nzone, mean_mol_wt=nzone,
+                 divvel=nzone, msonic=nzone, flatten=nzone, work=nzone, Bxf_q=5 * nfc, Byf_q=5 * nfc,
+                 Bzf_q=5 * nfc)
+    for k, n in sizes.items():
+        A[k] = rng.normal(size=n) if junk else np.zeros(n)
+    A["V_min_local"] = np.zeros(9)
+    A["V_max_local"] = np.zeros(9)
+    prob = dict(N=N, NR=NR, g=g, gamma=gamma, molewt=1.0, rhofloor=1e-12, prsfloor=1e-9, smallnum=1e-12,
+                flatten_coef=0.4, prs_flatten_coef=1.5)
+    return prob, A
+
+
+def p1_copy(A):
+    return {k: v.copy() for k, v in A.items()}
+
+
+def p1_oracle(prob, A):
+    """or_limit_all_slopes_p1 in place on the arrays A."""
+    L = olib()
+    L.or_limit_all_slopes_p1.argtypes = [ctypes.c_void_p]
+    s = p1_struct(prob, A)
+    rc = L.or_limit_all_slopes_p1(ctypes.byref(s))
+    if rc != 0:
+        raise RuntimeError("oracle rejected the patch")
+    return A
+
+
+def p1_reference(prob, A):
+    """The reference's LIMIT_ALL_SLOPES_P1 (flang build) in place on A."""
+    R = rlib()
+    if R is None:
+        raise RuntimeError("reference build not present")
+    R.ref_limit_all_slopes_p1.argtypes = [ctypes.c_void_p]
+    s = p1_struct(prob, A)
+    R.ref_limit_all_slopes_p1(ctypes.byref(s))
+    return A

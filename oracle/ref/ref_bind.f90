@@ -5,6 +5,18 @@
 module cs_ref_bind
   use iso_c_binding
   implicit none
+
+  ! mirror of or_p1_patch (oracle/cs_oracle.h) and cs_p1_patch (include/cubesphere_hip.h)
+  type, bind(C) :: ref_p1_patch
+    integer(c_int32_t) :: ntp, nr, nghost, itri(4), pad
+    real(c_double) :: gamma, molewt, rhofloor, prsfloor, smallnum, flatten_coef, prs_flatten_coef
+    type(c_ptr) :: vert, mask, stencil, xf_normal, xf_area, xf_centroid, yf_normal, yf_area, yf_centroid
+    type(c_ptr) :: zf_area, zf_centroid, xb_pts, xb_wts, xb_normal, yb_pts, yb_wts, yb_normal
+    type(c_ptr) :: zb_pts, zb_wts, zb_normal, scale, v_centroid, U, ent, Bxf, Byf, Bzf
+    type(c_ptr) :: V, B1_zc, B2_zc, B3_zc, Ex_zc, Ey_zc, Ez_zc
+    type(c_ptr) :: gamma_eos, gamma_sound, mean_mol_wt, divvel, msonic, flatten, work
+    type(c_ptr) :: Bxf_q, Byf_q, Bzf_q, V_min_local, V_max_local
+  end type
 contains
 
   ! Mesh_Sector_General (201mesh_sector_general.f:79) for one patch per sector,
@@ -230,6 +242,71 @@ contains
            yp(:, :, :, :, :, s), yw(:, :, :, :, s), yn(:, :, :, :, s), &
            zp(:, :, :, :, :, s), zw(:, :, :, :, s), zn(:, :, :, :, :, s))
     end do
+  end subroutine
+
+  ! LIMIT_ALL_SLOPES_P1 (410limit_all_slopes_p1.f:9) on one patch, from the
+  ! fields of a C structure laid out as or_p1_patch / cs_p1_patch.  The arrays
+  ! the routine declares but does not read at this order (area quadrature
+  ! points and weights, z-face normals, volumes, basis moments, Wzc) are
+  ! passed as zero-filled stand-ins of their declared sizes.
+  subroutine ref_limit_all_slopes_p1(p) bind(C, name="ref_limit_all_slopes_p1")
+    type(ref_p1_patch), intent(in) :: p
+    integer :: n, nr, g, nz, nzr, nf, nfr, nzone, nxf, nzf, nfc, nv
+    integer :: md(4)
+    real(c_double), pointer :: vert(:), xfn(:), xfa(:), xfc(:), yfn(:), yfa(:), yfc(:), zfa(:), zfc(:)
+    real(c_double), pointer :: xbp(:), xbw(:), xbn(:), ybp(:), ybw(:), ybn(:), zbp(:), zbw(:), zbn(:)
+    real(c_double), pointer :: scl(:), cen(:), u(:), ent(:), bxf(:), byf(:), bzf(:), v(:)
+    real(c_double), pointer :: b1(:), b2(:), b3(:), ex(:), ey(:), ez(:), geos(:), gsnd(:), mmw(:)
+    real(c_double), pointer :: divv(:), msn(:), flat(:), work(:), bxq(:), byq(:), bzq(:), vmin(:), vmax(:)
+    integer(c_int), pointer :: mask(:), st(:)
+    real(c_double), allocatable :: xq(:), xw(:), yq(:), yw(:), zq(:), zw(:), zn(:), vol(:), coef(:), wzc(:)
+    external LIMIT_ALL_SLOPES_P1, REF_INIT_COMMON, REF_SET_P1_COMMON
+    n = p%ntp; nr = p%nr; g = p%nghost
+    nz = n + 2 * g; nzr = nr + 2 * g; nf = nz + 1; nfr = nzr + 1
+    nzone = nz * nz * nzr; nxf = nf * nz * nzr; nzf = nz * nz * nfr; nfc = nf * nf * nfr; nv = nfc
+    md = (/ 0, 1, 0, 0 /)
+    call REF_INIT_COMMON()
+    call REF_SET_P1_COMMON(p%gamma, p%molewt, p%rhofloor, p%prsfloor, p%smallnum, p%flatten_coef, &
+                           p%prs_flatten_coef)
+    call c_f_pointer(p%vert, vert, [3 * nv])
+    call c_f_pointer(p%mask, mask, [nzone])
+    call c_f_pointer(p%stencil, st, [nz * nz * 24])
+    call c_f_pointer(p%xf_normal, xfn, [3 * nxf]); call c_f_pointer(p%xf_area, xfa, [nxf])
+    call c_f_pointer(p%xf_centroid, xfc, [3 * nxf])
+    call c_f_pointer(p%yf_normal, yfn, [3 * nxf]); call c_f_pointer(p%yf_area, yfa, [nxf])
+    call c_f_pointer(p%yf_centroid, yfc, [3 * nxf])
+    call c_f_pointer(p%zf_area, zfa, [nzf]); call c_f_pointer(p%zf_centroid, zfc, [3 * nzf])
+    call c_f_pointer(p%xb_pts, xbp, [15 * nxf]); call c_f_pointer(p%xb_wts, xbw, [5 * nxf])
+    call c_f_pointer(p%xb_normal, xbn, [3 * nxf])
+    call c_f_pointer(p%yb_pts, ybp, [15 * nxf]); call c_f_pointer(p%yb_wts, ybw, [5 * nxf])
+    call c_f_pointer(p%yb_normal, ybn, [3 * nxf])
+    call c_f_pointer(p%zb_pts, zbp, [15 * nzf]); call c_f_pointer(p%zb_wts, zbw, [5 * nzf])
+    call c_f_pointer(p%zb_normal, zbn, [15 * nzf])
+    call c_f_pointer(p%scale, scl, [nzone]); call c_f_pointer(p%v_centroid, cen, [3 * nzone])
+    call c_f_pointer(p%U, u, [32 * nzone]); call c_f_pointer(p%ent, ent, [4 * nzone])
+    call c_f_pointer(p%Bxf, bxf, [nfc]); call c_f_pointer(p%Byf, byf, [nfc]); call c_f_pointer(p%Bzf, bzf, [nfc])
+    call c_f_pointer(p%V, v, [32 * nzone])
+    call c_f_pointer(p%B1_zc, b1, [4 * nzone]); call c_f_pointer(p%B2_zc, b2, [4 * nzone])
+    call c_f_pointer(p%B3_zc, b3, [4 * nzone])
+    call c_f_pointer(p%Ex_zc, ex, [4 * nzone]); call c_f_pointer(p%Ey_zc, ey, [4 * nzone])
+    call c_f_pointer(p%Ez_zc, ez, [4 * nzone])
+    call c_f_pointer(p%gamma_eos, geos, [nzone]); call c_f_pointer(p%gamma_sound, gsnd, [nzone])
+    call c_f_pointer(p%mean_mol_wt, mmw, [nzone]); call c_f_pointer(p%divvel, divv, [nzone])
+    call c_f_pointer(p%msonic, msn, [nzone]); call c_f_pointer(p%flatten, flat, [nzone])
+    call c_f_pointer(p%work, work, [nzone])
+    call c_f_pointer(p%Bxf_q, bxq, [5 * nfc]); call c_f_pointer(p%Byf_q, byq, [5 * nfc])
+    call c_f_pointer(p%Bzf_q, bzq, [5 * nfc])
+    call c_f_pointer(p%V_min_local, vmin, [9]); call c_f_pointer(p%V_max_local, vmax, [9])
+    allocate(xq(15 * nxf), xw(5 * nxf), yq(15 * nxf), yw(5 * nxf), zq(15 * nzf), zw(5 * nzf), zn(15 * nzf))
+    allocate(vol(nzone), coef(4 * nzone), wzc(4 * nzone))
+    xq = 0; xw = 0; yq = 0; yw = 0; zq = 0; zw = 0; zn = 0; vol = 1; coef = 0; wzc = 0
+    call LIMIT_ALL_SLOPES_P1(nr, n, g, 8, 4, 1, 1, 1, md, md, md, vert, mask, &
+         xq, xw, xfn, xfa, xfc, yq, yw, yfn, yfa, yfc, zq, zw, zn, zfa, zfc, &
+         xbp, xbw, xbn, ybp, ybw, ybn, zbp, zbw, zbn, scl, vol, cen, coef, &
+         p%itri(1), p%itri(2), p%itri(3), p%itri(4), st, &
+         u, ent, bxf, byf, bzf, v, b1, b2, b3, ex, ey, ez, wzc, &
+         geos, gsnd, mmw, divv, msn, flat, work, bxq, byq, bzq, vmin, vmax)
+    deallocate(xq, xw, yq, yw, zq, zw, zn, vol, coef, wzc)
   end subroutine
 
 end module cs_ref_bind

@@ -1,0 +1,134 @@
+"""CPU checks of the oracle's LIMIT_ALL_SLOPES_P1 (410limit_all_slopes_p1.f:9,
+with 440enforce_pressure_positivity_prim.f and 120matsolve.f): bit-exact
+against the reference's own routine built from its sources (when present),
+and closed-form cases that hold without it."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+
+CASES = [
+    # N, NR, sector, seed, shock, noise: smooth + front (flattener), log-normal
+    # noise (positivity limiter, including complete removal), other sectors
+    (8, 3, 0, 0, True, 0.0),
+    (8, 4, 3, 1, True, 0.3),
+    (12, 5, 1, 2, True, 1.0),
+    (10, 4, 5, 3, False, 2.0),
+]
+
+
+def _ref_or_skip():
+    if orc.rlib() is None:
+        pytest.skip("reference build (oracle/_ref) not present")
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "N%d_nr%d_s%d_seed%d_%s_noise%g" % (
+    c[0], c[1], c[2], c[3], "front" if c[4] else "smooth", c[5]))
+def test_p1_oracle_bit_exact_with_reference(case):
+    _ref_or_skip()
+    N, NR, s, seed, shock, noise = case
+    prob, A0 = orc.p1_problem(N, NR, 4, sector=s, seed=seed, shock=shock, noise=noise)
+    A = orc.p1_oracle(prob, orc.p1_copy(A0))
+    B = orc.p1_reference(prob, orc.p1_copy(A0))
+    bad = {k: int((A[k] != B[k]).sum()) for k in A if not np.array_equal(A[k], B[k])}
+    assert not bad, bad
+
+
+def test_p1_cases_reach_every_branch():
+    """The flattener, the positivity limiter and its complete removal all act
+    in the cases above (else the bit-exact comparison would not cover them)."""
+    L = orc.olib()
+    st = np.zeros(2, np.int64)
+    prob, A = orc.p1_problem(8, 3, 4, seed=0, shock=True)
+    orc.p1_oracle(prob, A)
+    assert (A["flatten"] > 0).sum() > 100 and (A["flatten"] >= 1.0).sum() > 10
+    prob, A = orc.p1_problem(12, 5, 4, sector=1, seed=2, shock=True, noise=1.0)
+    orc.p1_oracle(prob, A)
+    L.or_p1_positivity_stats(orc.P(st))
+    assert st[0] > 100 and st[1] > 10
+
+
+def _uniform(N=8, NR=3, s=2):
+    prob, A = orc.p1_problem(N, NR, 4, sector=s, shock=False)
+    d = orc.p1_dims(N, NR, 4)
+    nz, nfc = d["zone"], d["fc"]
+    rho, prs, v, B = 1.3, 0.7, np.array([0.2, -0.1, 0.3]), np.array([0.4, -0.25, 0.15])
+    g = prob["gamma"]
+    U = A["U"].reshape(4, 8, nz)
+    U[0, 0] = rho
+    U[0, 1:4] = (rho * v)[:, None]
+    U[0, 4] = prs / (g - 1) + 0.5 * rho * v @ v + B @ B / (8 * np.pi)
+    U[0, 5:8] = B[:, None]
+    A["ent"].reshape(4, nz)[0] = prs / rho ** (g - 1)
+    return prob, A, d, (rho, prs, v, B)
+
+
+def test_p1_uniform_state_has_no_slopes():
+    """A uniform state: zero WENO slopes, no flattening, E = -v x B; with face
+    B set to B . n of a uniform field the CLSQ returns that field."""
+    prob, A, d, (rho, prs, v, B) = _uniform()
+    nz, nfc = d["zone"], d["fc"]
+    # face means of the uniform field, each family from its own B-field normals
+    g, N, NR = 4, prob["N"], prob["NR"]
+    nf, nzc = N + 2 * g + 1, N + 2 * g
+    xn = A["xb_normal"].reshape(3, NR + 2 * g, nzc, nf)
+    yn = A["yb_normal"].reshape(3, NR + 2 * g, nf, nzc)
+    zn = A["zb_normal"].reshape(5, 3, NR + 2 * g + 1, nzc, nzc)[0]
+    Bxf = np.zeros((NR + 2 * g + 1, nf, nf))
+    Byf = np.zeros_like(Bxf)
+    Bzf = np.zeros_like(Bxf)
+    Bxf[1:, 1:, :] = np.einsum("c,ckji->kji", B, xn)
+    Byf[1:, :, 1:] = np.einsum("c,ckji->kji", B, yn)
+    Bzf[:, 1:, 1:] = np.einsum("c,ckji->kji", B, zn)
+    A["Bxf"], A["Byf"], A["Bzf"] = Bxf.ravel().copy(), Byf.ravel().copy(), Bzf.ravel().copy()
+    orc.p1_oracle(prob, A)
+    mask = A["mask"] == 1
+    V = A["V"].reshape(4, 8, nz)
+    ixs = np.arange(nz)
+    i = ixs % nzc - (g - 1)
+    j = (ixs // nzc) % nzc - (g - 1)
+    k = ixs // (nzc * nzc) - (g - 1)
+    inner = mask & (i >= -2) & (i <= N + 3) & (j >= -2) & (j <= N + 3) & (k >= -2) & (k <= NR + 3)
+    assert np.abs(V[1:, :5][:, :, inner]).max() == 0.0
+    assert np.abs(A["flatten"]).max() == 0.0
+    # E = -v x B of the reconstructed B (step 9.a), slopes ~ 0 (step 9.b)
+    Bout = V[0, 5:8]
+    E = -np.cross(v[:, None], Bout, axis=0)
+    for q, nm in enumerate(("Ex_zc", "Ey_zc", "Ez_zc")):
+        e = A[nm].reshape(4, nz)
+        np.testing.assert_allclose(e[0, mask], E[q, mask], rtol=0, atol=1e-15)
+        assert np.abs(e[1:, inner]).max() < 1e-4
+    # face B = B . n of a uniform field: the constrained fit returns that field
+    # up to the curvature of the z-faces (their mean is constrained through the
+    # central normal, their points use their own normals), O(h^2)
+    for q, nm in enumerate(("B1_zc", "B2_zc", "B3_zc")):
+        b = A[nm].reshape(4, nz)
+        np.testing.assert_allclose(b[0, inner], B[q], rtol=0, atol=2e-4)
+        assert np.abs(b[1:, inner]).max() < 2e-3
+        np.testing.assert_array_equal(b[:, inner], V[:, 5 + q][:, inner])
+
+
+def test_p1_linear_field_weno_exact():
+    """A field linear in position has the same gradient in every stencil, so
+    the WENO modes are that gradient times the zone's scale length."""
+    prob, A, d, (rho, prs, v, B) = _uniform(N=10, NR=4, s=4)
+    nz = d["zone"]
+    cen = A["v_centroid"].reshape(3, nz)
+    grad = np.array([0.05, -0.03, 0.02])
+    U = A["U"].reshape(4, 8, nz)
+    U[0, 0] = rho + cen.T @ grad
+    U[0, 1:4] = U[0, 0] * v[:, None]
+    U[0, 4] = prs / (prob["gamma"] - 1) + 0.5 * U[0, 0] * (v @ v) + B @ B / (8 * np.pi)
+    orc.p1_oracle(prob, A)
+    V = A["V"].reshape(4, 8, nz)
+    g, N, NR = 4, 10, 4
+    nzc = N + 2 * g
+    ixs = np.arange(nz)
+    i = ixs % nzc - (g - 1)
+    j = (ixs // nzc) % nzc - (g - 1)
+    k = ixs // (nzc * nzc) - (g - 1)
+    inner = (A["mask"] == 1) & (i >= 1) & (i <= N) & (j >= 1) & (j <= N) & (k >= 1) & (k <= NR)
+    want = grad[:, None] * A["scale"][None, inner]
+    np.testing.assert_allclose(V[1:, 0][:, inner], want, rtol=1e-9, atol=1e-13)
