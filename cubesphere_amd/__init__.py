@@ -124,6 +124,7 @@ def lib():
             "cs_build_geometry": [cfgp, vp, vp],
             "cs_mode_indices": [i32, ctypes.POINTER(i32), vp, vp, vp],
             "cs_stencil_indexing": [cfgp, vp],
+            "cs_limit_all_slopes_p1": [vp, vp],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
             "cs_batch_shell": [vp, i32, ctypes.POINTER(vp)],
@@ -477,3 +478,75 @@ def stencil_indexing(cfg):
     a = np.zeros(nc * nc * 24, np.int32)
     _check(lib().cs_stencil_indexing(ctypes.byref(cfg), _p(a)), "cs_stencil_indexing")
     return a
+
+
+# ---------------------------------------------------------------------------
+# LIMIT_ALL_SLOPES_P1 of one patch (cs_limit_all_slopes_p1)
+
+P1_SCALARS = ("gamma", "molewt", "rhofloor", "prsfloor", "smallnum", "flatten_coef", "prs_flatten_coef")
+P1_ARRAYS = ("vert", "mask", "stencil", "xf_normal", "xf_area", "xf_centroid", "yf_normal", "yf_area",
+             "yf_centroid", "zf_area", "zf_centroid", "xb_pts", "xb_wts", "xb_normal", "yb_pts", "yb_wts",
+             "yb_normal", "zb_pts", "zb_wts", "zb_normal", "scale", "v_centroid",
+             "U", "ent", "Bxf", "Byf", "Bzf", "V", "B1_zc", "B2_zc", "B3_zc", "Ex_zc", "Ey_zc", "Ez_zc",
+             "gamma_eos", "gamma_sound", "mean_mol_wt", "divvel", "msonic", "flatten", "work",
+             "Bxf_q", "Byf_q", "Bzf_q")
+
+
+class P1Patch(ctypes.Structure):
+    """cs_p1_patch: the arguments of LIMIT_ALL_SLOPES_P1 for one patch."""
+    _fields_ = ([("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                 ("itri", ctypes.c_int32 * 4), ("pad", ctypes.c_int32)]
+                + [(k, ctypes.c_double) for k in P1_SCALARS]
+                + [(k, ctypes.c_void_p) for k in P1_ARRAYS]
+                + [("V_min_local", ctypes.c_void_p), ("V_max_local", ctypes.c_void_p)])
+
+
+def p1_counts(N, NR, g):
+    """Element count of every array of a patch, from the reference's layouts."""
+    nz, nzr = N + 2 * g, NR + 2 * g
+    nf, nfr = nz + 1, nzr + 1
+    zone, xf, zf, fc = nz * nz * nzr, nf * nz * nzr, nz * nz * nfr, nf * nf * nfr
+    c = dict(vert=3 * fc, mask=zone, stencil=nz * nz * 24, xf_normal=3 * xf, xf_area=xf, xf_centroid=3 * xf,
+             yf_normal=3 * xf, yf_area=xf, yf_centroid=3 * xf, zf_area=zf, zf_centroid=3 * zf,
+             xb_pts=15 * xf, xb_wts=5 * xf, xb_normal=3 * xf, yb_pts=15 * xf, yb_wts=5 * xf, yb_normal=3 * xf,
+             zb_pts=15 * zf, zb_wts=5 * zf, zb_normal=15 * zf, scale=zone, v_centroid=3 * zone,
+             U=32 * zone, ent=4 * zone, Bxf=fc, Byf=fc, Bzf=fc, V=32 * zone)
+    for k in ("B1_zc", "B2_zc", "B3_zc", "Ex_zc", "Ey_zc", "Ez_zc"):
+        c[k] = 4 * zone
+    for k in ("gamma_eos", "gamma_sound", "mean_mol_wt", "divvel", "msonic", "flatten", "work"):
+        c[k] = zone
+    for k in ("Bxf_q", "Byf_q", "Bzf_q"):
+        c[k] = 5 * fc
+    return c
+
+
+def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=None, minmax=True):
+    """Run LIMIT_ALL_SLOPES_P1 on one patch whose arrays are torch tensors on
+    the GPU (float64, int32 for mask and stencil), contiguous, in the
+    reference's layouts; they are updated in place.  Every tensor's size is
+    checked against the layout before the launch.  Returns (V_min_local,
+    V_max_local) as numpy arrays of 9 when minmax."""
+    import torch
+    want = p1_counts(N, NR, g)
+    s = P1Patch()
+    s.ntp, s.nr, s.nghost = N, NR, g
+    for q in range(4):
+        s.itri[q] = int(itri[q])
+    for k in P1_SCALARS:
+        setattr(s, k, float(scalars[k]))
+    for k in P1_ARRAYS:
+        t = arrays[k]
+        dt = torch.int32 if k in ("mask", "stencil") else torch.float64
+        if not (t.is_cuda and t.dtype == dt and t.is_contiguous() and t.numel() == want[k]):
+            raise CsError(f"limit_all_slopes_p1: {k} must be a contiguous {dt} CUDA tensor of {want[k]} "
+                          f"elements (got {t.dtype}, {t.numel()}, cuda={t.is_cuda})")
+        setattr(s, k, t.data_ptr())
+    vmin = np.zeros(9)
+    vmax = np.zeros(9)
+    if minmax:
+        s.V_min_local = vmin.ctypes.data
+        s.V_max_local = vmax.ctypes.data
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    _check(lib().cs_limit_all_slopes_p1(ctypes.byref(s), st), "cs_limit_all_slopes_p1")
+    return vmin, vmax
+

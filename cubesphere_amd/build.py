@@ -41,8 +41,12 @@ def build(verbose=False):
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "cs_kernels.hip"), "-o", kern_o])
     geom_o = os.path.join(BUILD, "cs_geometry.o")
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "cs_geometry.hip"), "-o", geom_o])
+    p1_o = os.path.join(BUILD, "cs_slopes_p1.o")
+    _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, "cs_slopes_p1.hip"),
+          "-o", p1_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", mesh_o, kern_o, geom_o, "-pthread", "-o", tmp])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", mesh_o, kern_o, geom_o, p1_o, "-pthread", "-o",
+          tmp])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -1,0 +1,1130 @@
+// cs_slopes_p1.hip -- the reference's second-order zone reconstruction
+// LIMIT_ALL_SLOPES_P1 (cubesphere/400riemann_integrate/410limit_all_slopes_p1.f:9)
+// for one patch on the GPU, FP64, behind cs_limit_all_slopes_p1.
+//
+// The patch arrays keep the reference's column-major layouts (first index
+// fastest), so a thread per zone or face reads consecutive addresses along x.
+// Each kernel is one dependency level of the routine:
+//   k_prim        steps 2 and 5: constants, higher modes cleared, tri-corner
+//                 zones made safe, CONS_TO_PRIM, magnetosonic speed
+//   k_face_zero   step 2 (faces): face B between two inactive zones cleared
+//   k_tricorner   step 3: face B synchronised at the tri-corners
+//   k_flat_a/b/c  step 6.a-c: velocity-divergence flattener, its pressure-
+//                 ratio extension, its spread to the six neighbours
+//   k_weno_pos    step 7.a and 7.b: multi-D WENO of the 8 primitives and the
+//                 entropy, then ENFORCE_PRESSURE_POSITIVITY_PRIM (440...f)
+//   k_face_pts    step 8.a-c: normal B at the five B-field points of each face
+//   k_clsq_e      step 8.d and 9.a: ADP constrained least squares for the 12
+//                 linear B coefficients; zone E = -v x B
+//   k_weno_e      step 9.b: WENO of E
+//   k_diag(+final) step 10: min/max of the primitives, face-B divergence
+//
+// The constrained least squares of 8.d (the reference assembles the 18 x 18
+// KKT matrix and factors it with Numerical Recipes' LU) is solved here through
+// the Schur complement: M = A^T A (12 x 12, >= identity) by Cholesky, then the
+// 6 x 6 system C M^-1 C^T lambda = C M^-1 A^T b - d, all in registers.
+//
+// The positivity limiter of the reference runs in loop order and, when a zone
+// fails every reduction, also zeroes its entropy mean (440...f:553-600), which
+// later zones of the loop then see in their entropy range.  k_weno_pos runs
+// all zones with the incoming entropy means and flags the zones that fail
+// completely; when any do, the zones with such a neighbour earlier in the
+// loop order are recomputed with those means taken as zero, repeating until
+// the flags no longer change (the dependence only runs forward in the loop
+// order, so this reaches the sequential result), and the flagged means are
+// then cleared.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "cs_physics.h"
+#include "../../include/cubesphere_hip.h"
+
+namespace cs {
+void set_error(const char* fmt, ...);
+}
+
+using namespace csp;
+
+namespace csp1 {
+
+struct P1 {
+  cs_p1_patch a;
+  int N, NR, g, nz, nzr, nf, nfr;
+  int64_t NZ, NXF, NZF, NFC;
+
+  __device__ __forceinline__ int64_t zi(int i, int j, int k) const {
+    return (int64_t)(i - 1 + g) + (int64_t)nz * ((j - 1 + g) + (int64_t)nz * (k - 1 + g));
+  }
+  __device__ __forceinline__ int64_t fci(int i, int j, int k) const {
+    return (int64_t)(i + g) + (int64_t)nf * ((j + g) + (int64_t)nf * (k + g));
+  }
+  __device__ __forceinline__ int64_t xfi(int i, int j, int k) const {
+    return (int64_t)(i + g) + (int64_t)nf * ((j - 1 + g) + (int64_t)nz * (k - 1 + g));
+  }
+  __device__ __forceinline__ int64_t yfi(int i, int j, int k) const {
+    return (int64_t)(i - 1 + g) + (int64_t)nz * ((j + g) + (int64_t)nf * (k - 1 + g));
+  }
+  __device__ __forceinline__ int64_t zfi(int i, int j, int k) const {
+    return (int64_t)(i - 1 + g) + (int64_t)nz * ((j - 1 + g) + (int64_t)nz * (k + g));
+  }
+  // stencil entry n (R L U D RR LL UU DD RU LU LD RD) of zone column (i, j),
+  // clamped to the zone range so that no table entry can address outside it
+  __device__ __forceinline__ void nbr(int i, int j, int n, int& ni, int& nj) const {
+    const int64_t s = (int64_t)(i - 1 + g) + (int64_t)nz * (j - 1 + g), plane = (int64_t)nz * nz;
+    ni = min(max(a.stencil[s + plane * (2 * n)], 1 - g), N + g);
+    nj = min(max(a.stencil[s + plane * (2 * n + 1)], 1 - g), N + g);
+  }
+  __device__ __forceinline__ bool active(int i, int j, int k) const { return a.mask[zi(i, j, k)] == 1; }
+  __device__ __forceinline__ double V(int c, int m, int64_t z) const { return a.V[z + NZ * (c + 8 * m)]; }
+  __device__ __forceinline__ double& Vr(int c, int m, int64_t z) const { return a.V[z + NZ * (c + 8 * m)]; }
+  __device__ __forceinline__ double& Ur(int c, int m, int64_t z) const { return a.U[z + NZ * (c + 8 * m)]; }
+  __device__ __forceinline__ double obl(int64_t z) const { return 1.0 / fmax(a.scale[z], a.smallnum); }
+  __device__ __forceinline__ double cen(int64_t z, int c) const { return a.v_centroid[z + NZ * c]; }
+};
+
+// a box of zone or face indices, decoded from a flat thread index
+struct Box {
+  int i0, j0, k0, ni, nj, nk;
+  __device__ __forceinline__ bool decode(int64_t t, int& i, int& j, int& k) const {
+    if (t >= (int64_t)ni * nj * nk) return false;
+    i = i0 + (int)(t % ni);
+    j = j0 + (int)((t / ni) % nj);
+    k = k0 + (int)(t / ((int64_t)ni * nj));
+    return true;
+  }
+  int64_t count() const { return (int64_t)ni * nj * nk; }
+};
+static Box make_box(int i0, int i1, int j0, int j1, int k0, int k1) {
+  return Box{i0, j0, k0, i1 - i0 + 1, j1 - j0 + 1, k1 - k0 + 1};
+}
+
+__device__ __forceinline__ int64_t tid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ---------------------------------------------------------------------------
+// steps 2 and 5
+__global__ void __launch_bounds__(256) k_prim(const P1 P, const Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k);
+  const double gam = a.gamma;
+  a.gamma_eos[z] = gam;
+  a.gamma_sound[z] = gam;
+  a.mean_mol_wt[z] = a.molewt;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int m = 1; m < 4; ++m) {
+      P.Ur(c, m, z) = 0.0;
+      P.Vr(c, m, z) = 0.0;
+    }
+#pragma unroll
+  for (int m = 1; m < 4; ++m) a.ent[z + P.NZ * m] = 0.0;
+  double U[8], V[8];
+  if (a.mask[z] == 0) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) U[c] = 0.0;
+    U[0] = a.rhofloor;
+    U[4] = a.prsfloor / (gam - 1.0);
+    a.scale[z] = a.smallnum;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) U[c] = P.Ur(c, 0, z);
+  }
+  double e = a.ent[z];
+  Gas<double> gs{gam, gam - 1.0, a.rhofloor, a.prsfloor, 1.0 / (gam - 1.0)};
+  V[4] = 0.0;
+  cons_to_prim(gs, U, V, e);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    P.Vr(c, 0, z) = V[c];
+    P.Ur(c, 0, z) = U[c];
+  }
+  a.ent[z] = e;
+  const double b2 = (V[5] * V[5] + V[6] * V[6] + V[7] * V[7]) * C<double>::ob4pi;
+  a.msonic[z] = sqrt((gam * V[4] + b2) / V[0]);
+}
+
+// step 2, faces: blockIdx.y = family
+__global__ void __launch_bounds__(256) k_face_zero(const P1 P) {
+  const int fam = blockIdx.y;
+  const int n = P.nf;
+  const int64_t t = tid();
+  if (t >= P.NFC) return;
+  const int i = (int)(t % n) - P.g, j = (int)((t / n) % n) - P.g, k = (int)(t / ((int64_t)n * n)) - P.g;
+  const int lo = 1 - P.g, hi = P.N + P.g, hr = P.NR + P.g;
+  if (fam == 0) {
+    if (j < lo || k < lo || k > hr) return;
+    if (!P.active(max(i, lo), j, k) && !P.active(min(i + 1, hi), j, k)) P.a.Bxf[t] = 0.0;
+  } else if (fam == 1) {
+    if (i < lo || k < lo || k > hr) return;
+    if (!P.active(i, max(j, lo), k) && !P.active(i, min(j + 1, hi), k)) P.a.Byf[t] = 0.0;
+  } else {
+    if (i < lo || j < lo) return;
+    if (!P.active(i, j, 1)) P.a.Bzf[t] = 0.0;
+  }
+}
+
+// step 3: one thread per (corner, ghost column, radial layer)
+__global__ void __launch_bounds__(256) k_tricorner(const P1 P) {
+  const int64_t t = tid();
+  const int g = P.g, nzr = P.nzr;
+  if (t >= 4 * (int64_t)g * nzr) return;
+  const int corner = (int)(t / ((int64_t)g * nzr)), ii = (int)((t / nzr) % g), k = (int)(t % nzr) + 1 - g;
+  if (!P.a.itri[corner]) return;
+  const bool left = corner == 0 || corner == 3, down = corner < 2;
+  const int i = left ? 1 - g + ii : P.N + 1 + ii, jj = down ? 1 : P.N;
+  int a, b;
+  P.nbr(i, jj, down ? 3 : 2, a, b);
+  double& by = P.a.Byf[P.fci(i, down ? jj - 1 : jj, k)];
+  double& bx = P.a.Bxf[P.fci(left ? a - 1 : a, b, k)];
+  if (corner == 0 || corner == 2) {
+    const double s = 0.5 * (by - bx);
+    by = s;
+    bx = -s;
+  } else {
+    const double s = 0.5 * (by + bx);
+    by = s;
+    bx = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// step 6
+__device__ __forceinline__ void six(const P1& P, int i, int j, int k, int64_t* z6) {
+  int a, b;
+  P.nbr(i, j, 0, a, b);
+  z6[0] = P.zi(a, b, k);
+  P.nbr(i, j, 1, a, b);
+  z6[1] = P.zi(a, b, k);
+  P.nbr(i, j, 2, a, b);
+  z6[2] = P.zi(a, b, k);
+  P.nbr(i, j, 3, a, b);
+  z6[3] = P.zi(a, b, k);
+  z6[4] = P.zi(i, j, k + 1);
+  z6[5] = P.zi(i, j, k - 1);
+}
+
+__device__ __forceinline__ bool in_flat_range(const P1& P, int i, int j, int k) {
+  const int lo = 2 - P.g, hx = P.N + P.g - 1, hr = P.NR + P.g - 1;
+  return i >= lo && i <= hx && j >= lo && j <= hx && k >= lo && k <= hr;
+}
+
+__global__ void __launch_bounds__(256) k_flat_a(const P1 P, const Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k);
+  if (!in_flat_range(P, i, j, k)) {
+    a.flatten[z] = 0.0;
+    return;
+  }
+  int64_t z6[6];
+  six(P, i, j, k, z6);
+  const double obl = P.obl(z);
+  double wmin = a.msonic[z];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) wmin = fmin(wmin, a.msonic[z6[q]]);
+  // the unit radial direction of the top z-face serves both z-faces
+  double nzv[3];
+  {
+    const int64_t f = P.zfi(i, j, k);
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      nzv[c] = a.zf_centroid[f + P.NZF * c];
+    }
+    s = 1.0 / fmax(sqrt(nzv[0] * nzv[0] + nzv[1] * nzv[1] + nzv[2] * nzv[2]), a.smallnum);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nzv[c] = nzv[c] * s;
+  }
+  double dv = 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const bool up = (q % 2) == 0;
+    const int64_t zo = up ? z6[q] : z, zn = up ? z : z6[q];
+    double r[3], n[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = (P.cen(zo, c) - P.cen(zn, c)) * obl;
+    const double a11 = 0.5 / fmax(sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), a.smallnum);
+    if (q < 2) {
+      const int64_t f = P.xfi(q == 0 ? i : i - 1, j, k);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) n[c] = a.xf_normal[f + P.NXF * c];
+    } else if (q < 4) {
+      const int64_t f = P.yfi(i, q == 2 ? j : j - 1, k);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) n[c] = a.yf_normal[f + P.NXF * c];
+    } else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) n[c] = nzv[c];
+    }
+    dv = dv + n[0] * a11 * (P.V(1, 0, zo) - P.V(1, 0, zn)) + n[1] * a11 * (P.V(2, 0, zo) - P.V(2, 0, zn)) +
+         n[2] * a11 * (P.V(3, 0, zo) - P.V(3, 0, zn));
+  }
+  a.divvel[z] = dv;
+  const double a13 = fabs(dv) / (a.flatten_coef * wmin) - 1.0;
+  double f = a13 <= 0.0 ? 0.0 : fmin(1.0, a13);
+  if (a.mask[z] != 1) f = 0.0;
+  a.flatten[z] = f;
+}
+
+// flatten (stage a) -> work (stage b)
+__global__ void __launch_bounds__(256) k_flat_b(const P1 P, const Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k);
+  double f = a.flatten[z];
+  if (in_flat_range(P, i, j, k)) {
+    int64_t z6[6];
+    six(P, i, j, k, z6);
+    const double p0 = a.prs_flatten_coef * P.V(4, 0, z);
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      if (P.V(4, 0, z6[q]) > p0) f = fmax(f, a.flatten[z6[q]]);
+  }
+  a.work[z] = f;
+}
+
+// work (stage b) -> flatten (stage c)
+__global__ void __launch_bounds__(256) k_flat_c(const P1 P, const Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k);
+  double f = a.work[z];
+  if (in_flat_range(P, i, j, k)) {
+    int64_t z6[6];
+    six(P, i, j, k, z6);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) f = fmax(f, a.work[z6[q]]);
+  }
+  a.flatten[z] = f;
+}
+
+// ---------------------------------------------------------------------------
+// WENO (410...f:2421-3761, 5437-7100): stencil s uses the x-neighbour
+// (R for s in {0,1,4,5}, else L), the y-neighbour (U for s < 4, else D) and the
+// z-neighbour (top for even s, else bottom)
+struct Weno {
+  double A[8][3][3];
+};
+
+__device__ __forceinline__ void inv3(const double (&m)[3][3], double (&r)[3][3]) {
+  const double a11 = m[0][0], a12 = m[0][1], a13 = m[0][2], a21 = m[1][0], a22 = m[1][1], a23 = m[1][2],
+               a31 = m[2][0], a32 = m[2][1], a33 = m[2][2];
+  const double det =
+      -(a13 * a22 * a31) + a12 * a23 * a31 + a13 * a21 * a32 - a11 * a23 * a32 - a12 * a21 * a33 + a11 * a22 * a33;
+  const double od = (signbit(det) ? -1.0 : 1.0) / fmax(1.0e-32, fabs(det));
+  r[0][0] = (-(a23 * a32) + a22 * a33) * od;
+  r[0][1] = (a13 * a32 - a12 * a33) * od;
+  r[0][2] = (-(a13 * a22) + a12 * a23) * od;
+  r[1][0] = (a23 * a31 - a21 * a33) * od;
+  r[1][1] = (-(a13 * a31) + a11 * a33) * od;
+  r[1][2] = (a13 * a21 - a11 * a23) * od;
+  r[2][0] = (-(a22 * a31) + a21 * a32) * od;
+  r[2][1] = (a12 * a31 - a11 * a32) * od;
+  r[2][2] = (-(a12 * a21) + a11 * a22) * od;
+}
+
+__device__ __forceinline__ void weno_build(const P1& P, int64_t z0, const int64_t* z6, Weno& W) {
+  const double obl = P.obl(z0);
+  double off[6][3];
+#pragma unroll
+  for (int n = 0; n < 6; ++n)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) off[n][c] = (P.cen(z6[n], c) - P.cen(z0, c)) * obl;
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    double m[3][3];
+    const int sx = (s >> 1) & 1, sy = (s >> 2) & 1, sz = s & 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      m[0][c] = off[sx][c];
+      m[1][c] = off[2 + sy][c];
+      m[2][c] = off[4 + sz][c];
+    }
+    inv3(m, W.A[s]);
+  }
+}
+
+__device__ __forceinline__ void weno_apply(const Weno& W, double f0, const double* f6, double smallnum,
+                                           double omflat, double* g3) {
+  double X[9][3];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int sx = (s >> 1) & 1, sy = (s >> 2) & 1, sz = s & 1;
+    const double r0 = f6[sx] - f0, r1 = f6[2 + sy] - f0, r2 = f6[4 + sz] - f0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) X[s][m] = W.A[s][m][0] * r0 + W.A[s][m][1] * r1 + W.A[s][m][2] * r2;
+  }
+#pragma unroll
+  for (int m = 0; m < 3; ++m)
+    X[8][m] = 0.125 * (X[0][m] + X[1][m] + X[2][m] + X[3][m] + X[4][m] + X[5][m] + X[6][m] + X[7][m]);
+  double w[9], sum = 0.0;
+#pragma unroll
+  for (int s = 0; s < 9; ++s) {
+    double t = X[s][0] * X[s][0] + X[s][1] * X[s][1] + X[s][2] * X[s][2] + smallnum;
+    t = t * t;
+    w[s] = (s == 8 ? 100.0 : 1.0) / (t * t);
+    sum = s == 0 ? w[0] : sum + w[s];
+  }
+  const double os = 1.0 / sum;
+#pragma unroll
+  for (int s = 0; s < 9; ++s) w[s] = w[s] * os;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    double v = w[0] * X[0][m];
+#pragma unroll
+    for (int s = 1; s < 9; ++s) v = v + w[s] * X[s][m];
+    g3[m] = v * omflat;
+  }
+}
+
+__device__ __forceinline__ bool in_weno_range(const P1& P, int i, int j, int k) {
+  return i >= -2 && i <= P.N + 3 && j >= -2 && j <= P.N + 3 && k >= -2 && k <= P.NR + 3;
+}
+
+// ---------------------------------------------------------------------------
+// step 7: WENO then pressure positivity (440...f:6-830).
+// mode 0: every zone of the range, incoming entropy means, writes flagA.
+// mode 1: only zones with a flagged (prev) zone earlier in the loop order among
+// their 27 neighbours, whose entropy means count as zero; writes next.
+struct Flags {
+  uint8_t *fa, *prev, *next;
+  unsigned int* counter;
+};
+
+__global__ void __launch_bounds__(128) k_weno_pos(const P1 P, const Box b, const Flags F, int mode) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k);
+  // the 27 zones: the column and its 8 stencil neighbours, at k-1, k, k+1
+  int64_t col[9];
+  {
+    col[0] = P.zi(i, j, 0);
+    const int dirs[8] = {0, 1, 2, 3, 8, 9, 10, 11};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      int ai, bj;
+      P.nbr(i, j, dirs[q], ai, bj);
+      col[q + 1] = P.zi(ai, bj, 0);
+    }
+  }
+  const int64_t ks = (int64_t)P.nz * P.nz;
+  if (mode == 1) {
+    bool cand = false;
+    for (int q = 0; q < 9; ++q)
+      for (int dk = -1; dk <= 1; ++dk) {
+        const int64_t zz = col[q] + ks * (k + dk);
+        cand = cand || (zz < z && F.fa[zz]);
+      }
+    if (!cand) {
+      F.next[z] = F.fa[z];
+      return;
+    }
+  }
+  double vm[8][3], em[3];
+  if (P.active(i, j, 1)) {
+    int64_t z6[6];
+    six(P, i, j, k, z6);
+    Weno W;
+    weno_build(P, z, z6, W);
+    const double omf = 1.0 - a.flatten[z];
+#pragma unroll 1
+    for (int c = 0; c < 9; ++c) {
+      double f6[6], g3[3];
+      const double* src = c < 8 ? a.V + P.NZ * c : a.ent;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) f6[q] = src[z6[q]];
+      weno_apply(W, src[z], f6, a.smallnum, omf, g3);
+      if (c < 8) {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) P.Vr(c, m + 1, z) = g3[m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < 3; ++m) em[m] = g3[m];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) em[m] = 0.0;
+  }
+  // ranges of rho, p and the entropy over the 27 zones, widened by 20 % where
+  // the flattener is below 0.1 (440...f:169-280)
+  double rmin = 1e300, rmax = -1e300, pmin = 1e300, pmax = -1e300, emin = 1e300, emax = -1e300;
+  for (int q = 0; q < 9; ++q)
+    for (int dk = -1; dk <= 1; ++dk) {
+      const int64_t zz = col[q] + ks * (k + dk);
+      const double r = P.V(0, 0, zz), p = P.V(4, 0, zz);
+      double e = a.ent[zz];
+      if (mode == 1 && zz < z && F.prev[zz]) e = 0.0;
+      rmin = fmin(rmin, r);
+      rmax = fmax(rmax, r);
+      pmin = fmin(pmin, p);
+      pmax = fmax(pmax, p);
+      emin = fmin(emin, e);
+      emax = fmax(emax, e);
+    }
+  const double expandfac = 0.2;
+  const double ff = fmax(fmin((a.flatten[z] - 0.1) / 0.4, 1.0), 0.0), omff = 1.0 - ff;
+  rmin = fmax(rmin * (1.0 - expandfac * omff), a.rhofloor);
+  rmax = fmax(rmax * (1.0 + expandfac * omff), a.rhofloor);
+  pmin = fmax(pmin * (1.0 - expandfac * omff), a.prsfloor);
+  pmax = fmax(pmax * (1.0 + expandfac * omff), a.prsfloor);
+  emin = emin * (1.0 - expandfac * omff);
+  emax = emax * (1.0 + expandfac * omff);
+  // modes checked at the 27 nodes: rho, p and the entropy
+  double r0 = P.V(0, 0, z), p0 = P.V(4, 0, z), e0 = a.ent[z];
+  double rm[3], pm[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    rm[m] = P.V(0, m + 1, z);
+    pm[m] = P.V(4, m + 1, z);
+  }
+  // node coordinates relative to the centroid, scaled
+  const double obl = P.obl(z);
+  double c0[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) c0[c] = P.cen(z, c);
+  const double ratio = 0.85769589859089412;  // 0.1^(1/15)
+  int nred = 0;                              // reductions applied
+  bool removed = false;
+  for (int it = 1; it <= 16; ++it) {
+    if (it == 16) {
+      removed = true;
+      break;
+    }
+    bool pass = true;
+    for (int n = 0; n < 27; ++n) {
+      double x[3];
+      if (n < 8) {
+        const int vi = (n & 1) ? i : i - 1, vj = (n & 2) ? j : j - 1, vk = (n & 4) ? k : k - 1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[c] = a.vert[P.fci(vi, vj, vk) + P.NFC * c];
+      } else if (n < 14) {
+        const int f = n - 8;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          x[c] = f == 0 ? a.xf_centroid[P.xfi(i, j, k) + P.NXF * c]
+               : f == 1 ? a.xf_centroid[P.xfi(i - 1, j, k) + P.NXF * c]
+               : f == 2 ? a.yf_centroid[P.yfi(i, j, k) + P.NXF * c]
+               : f == 3 ? a.yf_centroid[P.yfi(i, j - 1, k) + P.NXF * c]
+               : f == 4 ? a.zf_centroid[P.zfi(i, j, k) + P.NZF * c]
+                        : a.zf_centroid[P.zfi(i, j, k - 1) + P.NZF * c];
+      } else if (n < 26) {
+        // edge midpoints (vertex pairs along x, then y, then z)
+        const int e = n - 14;
+        int va, vb;
+        if (e < 4) {
+          va = 2 * e;
+          vb = va + 1;
+        } else if (e < 8) {
+          const int q = e - 4;
+          va = (q & 1) + 4 * (q >> 1);
+          vb = va + 2;
+        } else {
+          va = e - 8;
+          vb = va + 4;
+        }
+        const int ai = (va & 1) ? i : i - 1, aj = (va & 2) ? j : j - 1, ak = (va & 4) ? k : k - 1;
+        const int bi = (vb & 1) ? i : i - 1, bj = (vb & 2) ? j : j - 1, bk = (vb & 4) ? k : k - 1;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          x[c] = (a.vert[P.fci(ai, aj, ak) + P.NFC * c] + a.vert[P.fci(bi, bj, bk) + P.NFC * c]) * 0.5;
+      } else {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[c] = c0[c];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) x[c] = (x[c] - c0[c]) * obl;
+      const double rn = r0 + rm[0] * x[0] + rm[1] * x[1] + rm[2] * x[2];
+      const double pn = p0 + pm[0] * x[0] + pm[1] * x[1] + pm[2] * x[2];
+      const double en = e0 + em[0] * x[0] + em[1] * x[1] + em[2] * x[2];
+      if (rn < rmin || rn > rmax || pn < pmin || pn > pmax || en < emin || en > emax) pass = false;
+    }
+    if (pass) break;
+    ++nred;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      rm[m] = ratio * rm[m];
+      pm[m] = ratio * pm[m];
+      em[m] = ratio * em[m];
+    }
+  }
+  // apply the same reductions to every component (repeated, as the reference
+  // multiplies once per iteration) and store; B_zc copies the B modes
+  for (int c = 0; c < 8; ++c) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      double v = P.V(c, m + 1, z);
+      if (removed) {
+        v = 0.0;
+      } else {
+        for (int r = 0; r < nred; ++r) v = ratio * v;
+      }
+      P.Vr(c, m + 1, z) = v;
+      if (c >= 5) (c == 5 ? a.B1_zc : c == 6 ? a.B2_zc : a.B3_zc)[z + P.NZ * (m + 1)] = v;
+    }
+  }
+  a.B1_zc[z] = P.V(5, 0, z);
+  a.B2_zc[z] = P.V(6, 0, z);
+  a.B3_zc[z] = P.V(7, 0, z);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) a.ent[z + P.NZ * (m + 1)] = removed ? 0.0 : em[m];
+  uint8_t* out = mode == 0 ? F.fa : F.next;
+  out[z] = removed ? 1 : 0;
+  if (mode == 0 && removed) atomicAdd(F.counter, 1u);
+  if (mode == 1 && (out[z] != F.prev[z])) atomicAdd(F.counter, 1u);
+}
+
+// the entropy means of the zones that lost all their slopes
+__global__ void __launch_bounds__(256) k_clear_ent(const P1 P, const Box b, const uint8_t* flags) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int64_t z = P.zi(i, j, k);
+  if (flags[z]) P.a.ent[z] = 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// step 8.a-c: blockIdx.y = family
+__global__ void __launch_bounds__(256) k_face_pts(const P1 P, const Box bx, const Box by, const Box bz) {
+  const int fam = blockIdx.y;
+  const Box& b = fam == 0 ? bx : (fam == 1 ? by : bz);
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const cs_p1_patch& a = P.a;
+  double* Bq = fam == 0 ? a.Bxf_q : (fam == 1 ? a.Byf_q : a.Bzf_q);
+  const double* Bf = fam == 0 ? a.Bxf : (fam == 1 ? a.Byf : a.Bzf);
+  const double* pts = fam == 0 ? a.xb_pts : (fam == 1 ? a.yb_pts : a.zb_pts);
+  const double* wts = fam == 0 ? a.xb_wts : (fam == 1 ? a.yb_wts : a.zb_wts);
+  const double* nrm = fam == 0 ? a.xb_normal : (fam == 1 ? a.yb_normal : a.zb_normal);
+  const int64_t NF = fam == 2 ? P.NZF : P.NXF;
+  const int64_t fb = P.fci(i, j, k), fg = fam == 0 ? P.xfi(i, j, k) : (fam == 1 ? P.yfi(i, j, k) : P.zfi(i, j, k));
+  int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1);
+  bool skip = false;
+  if (fam < 2) {
+    const bool al = P.active(li, lj, k), ar = P.active(ri, rj, k);
+    if (!al && ar) {
+      P.nbr(ri, rj, fam == 0 ? 1 : 3, li, lj);
+    } else if (al && !ar) {
+      P.nbr(li, lj, fam == 0 ? 0 : 2, ri, rj);
+    } else if (!al && !ar) {
+      skip = true;
+    }
+  } else {
+    skip = !P.active(i, j, k);
+  }
+  double q5[5];
+  if (skip) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) q5[q] = 0.0;
+  } else {
+    const int64_t zl = P.zi(li, lj, k), zr = P.zi(ri, rj, fam == 2 ? k + 1 : k);
+    const double oll = P.obl(zl), olr = P.obl(zr);
+    double vl[3][4], vr[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        vl[c][m] = P.V(5 + c, m, zl);
+        vr[c][m] = P.V(5 + c, m, zr);
+      }
+    double n[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) n[c] = nrm[fg + NF * c];
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      if (fam == 2) {
+#pragma unroll
+        for (int c = 0; c < 3; ++c) n[c] = nrm[fg + NF * (c + 3 * q)];
+      }
+      double rl[3], rr[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double x = pts[fg + NF * (c + 3 * q)];
+        rl[c] = (x - P.cen(zl, c)) * oll;
+        rr[c] = (x - P.cen(zr, c)) * olr;
+      }
+      double s[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double bl = vl[c][0] + vl[c][1] * rl[0] + vl[c][2] * rl[1] + vl[c][3] * rl[2];
+        const double br = vr[c][0] + vr[c][1] * rr[0] + vr[c][2] * rr[1] + vr[c][3] * rr[2];
+        s[c] = bl + br;
+      }
+      q5[q] = 0.5 * (n[0] * s[0] + n[1] * s[1] + n[2] * s[2]);
+    }
+  }
+  double bd = Bf[fb];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) bd = bd - wts[fg + NF * q] * q5[q];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) Bq[fb + P.NFC * q] = q5[q] + bd;
+}
+
+// ---------------------------------------------------------------------------
+// step 8.d: packed lower-triangular 12 x 12 Cholesky and solves
+#define LIX(r, c) ((r) * ((r) + 1) / 2 + (c))
+
+__device__ __forceinline__ void chol12(double* L) {
+#pragma unroll
+  for (int j = 0; j < 12; ++j) {
+    double s = L[LIX(j, j)];
+#pragma unroll
+    for (int k = 0; k < j; ++k) s -= L[LIX(j, k)] * L[LIX(j, k)];
+    const double d = sqrt(s), od = 1.0 / d;
+    L[LIX(j, j)] = d;
+#pragma unroll
+    for (int i = j + 1; i < 12; ++i) {
+      double t = L[LIX(i, j)];
+#pragma unroll
+      for (int k = 0; k < j; ++k) t -= L[LIX(i, k)] * L[LIX(j, k)];
+      L[LIX(i, j)] = t * od;
+    }
+  }
+}
+// x <- (L L^T)^-1 x
+__device__ __forceinline__ void cholsolve12(const double* L, double* x) {
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    double s = x[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= L[LIX(i, k)] * x[k];
+    x[i] = s / L[LIX(i, i)];
+  }
+#pragma unroll
+  for (int i = 11; i >= 0; --i) {
+    double s = x[i];
+#pragma unroll
+    for (int k = i + 1; k < 12; ++k) s -= L[LIX(k, i)] * x[k];
+    x[i] = s / L[LIX(i, i)];
+  }
+}
+
+// the constraint or LSQ row of a face: n (x) (1, r)
+__device__ __forceinline__ void brow(const double* n, const double* r, double* row) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    row[4 * c] = n[c];
+    row[4 * c + 1] = n[c] * r[0];
+    row[4 * c + 2] = n[c] * r[1];
+    row[4 * c + 3] = n[c] * r[2];
+  }
+}
+
+// face f of zone (i, j, k): 0 right x, 1 left x, 2 up y, 3 down y, 4 top z, 5 bottom z
+struct FaceRef {
+  const double *nrm, *cen, *pts, *Bf, *Bq;
+  int64_t fg, fb, NF;
+};
+__device__ __forceinline__ FaceRef face_of(const P1& P, int f, int i, int j, int k) {
+  const cs_p1_patch& a = P.a;
+  const int fam = f >> 1;
+  const bool hi = (f & 1) == 0;
+  const int fi = i - (fam == 0 && !hi), fj = j - (fam == 1 && !hi), fk = k - (fam == 2 && !hi);
+  FaceRef r;
+  r.fg = fam == 0 ? P.xfi(fi, fj, fk) : (fam == 1 ? P.yfi(fi, fj, fk) : P.zfi(fi, fj, fk));
+  r.fb = P.fci(fi, fj, fk);
+  r.NF = fam == 2 ? P.NZF : P.NXF;
+  r.nrm = fam == 0 ? a.xb_normal : (fam == 1 ? a.yb_normal : a.zb_normal);
+  r.cen = fam == 0 ? a.xf_centroid : (fam == 1 ? a.yf_centroid : a.zf_centroid);
+  r.pts = fam == 0 ? a.xb_pts : (fam == 1 ? a.yb_pts : a.zb_pts);
+  r.Bf = fam == 0 ? a.Bxf : (fam == 1 ? a.Byf : a.Bzf);
+  r.Bq = fam == 0 ? a.Bxf_q : (fam == 1 ? a.Byf_q : a.Bzf_q);
+  return r;
+}
+
+// constraint row f (face normal, at the face centroid) and its right-hand side
+__device__ __forceinline__ double crow(const P1& P, int f, int i, int j, int k, int64_t z, double obl, double* row) {
+  const FaceRef F = face_of(P, f, i, j, k);
+  double n[3], r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    n[c] = F.nrm[F.fg + F.NF * c];
+    r[c] = (F.cen[F.fg + F.NF * c] - P.cen(z, c)) * obl;
+  }
+  brow(n, r, row);
+  return F.Bf[F.fb];
+}
+
+__global__ void __launch_bounds__(64) k_clsq_e(const P1 P, const Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k);
+  if (in_weno_range(P, i, j, k) && a.mask[z] != 0) {
+    const double obl = P.obl(z);
+    double L[78], u[12], row[12];
+#pragma unroll
+    for (int r = 0; r < 78; ++r) L[r] = 0.0;
+#pragma unroll
+    for (int r = 0; r < 12; ++r) {
+      L[LIX(r, r)] = 1.0;
+      u[r] = P.V(5 + r / 4, r % 4, z);
+    }
+    // least-squares rows: points 2 and 4 of each face
+#pragma unroll 1
+    for (int f = 0; f < 6; ++f) {
+      const FaceRef F = face_of(P, f, i, j, k);
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int pt = 1 + 2 * t;
+        double n[3], r[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          n[c] = (f >> 1) == 2 ? F.nrm[F.fg + F.NF * (c + 3 * pt)] : F.nrm[F.fg + F.NF * c];
+          r[c] = (F.pts[F.fg + F.NF * (c + 3 * pt)] - P.cen(z, c)) * obl;
+        }
+        brow(n, r, row);
+        const double bv = F.Bq[F.fb + P.NFC * pt];
+#pragma unroll
+        for (int p = 0; p < 12; ++p) {
+          u[p] += row[p] * bv;
+#pragma unroll
+          for (int q = 0; q <= p; ++q) L[LIX(p, q)] += row[p] * row[q];
+        }
+      }
+    }
+    chol12(L);
+    cholsolve12(L, u);  // unconstrained least-squares solution
+    // Schur complement S = C M^-1 C^T, s = C u - d
+    double S[21], s[6];
+#pragma unroll 1
+    for (int ci = 0; ci < 6; ++ci) {
+      double y[12];
+      const double dci = crow(P, ci, i, j, k, z, obl, y);
+      double cu = 0.0;
+#pragma unroll
+      for (int p = 0; p < 12; ++p) cu += y[p] * u[p];
+      s[ci] = cu - dci;
+      cholsolve12(L, y);
+#pragma unroll
+      for (int cj = 0; cj < 6; ++cj) {
+        if (cj < ci) continue;
+        crow(P, cj, i, j, k, z, obl, row);
+        double t = 0.0;
+#pragma unroll
+        for (int p = 0; p < 12; ++p) t += row[p] * y[p];
+        S[cj * (cj + 1) / 2 + ci] = t;
+      }
+    }
+    // S lambda = s (6 x 6 Cholesky)
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double d = S[c * (c + 1) / 2 + c];
+#pragma unroll
+      for (int q = 0; q < c; ++q) d -= S[c * (c + 1) / 2 + q] * S[c * (c + 1) / 2 + q];
+      d = sqrt(d);
+      S[c * (c + 1) / 2 + c] = d;
+#pragma unroll
+      for (int r = c + 1; r < 6; ++r) {
+        double t = S[r * (r + 1) / 2 + c];
+#pragma unroll
+        for (int q = 0; q < c; ++q) t -= S[r * (r + 1) / 2 + q] * S[c * (c + 1) / 2 + q];
+        S[r * (r + 1) / 2 + c] = t / d;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 6; ++r) {
+      double t = s[r];
+#pragma unroll
+      for (int q = 0; q < r; ++q) t -= S[r * (r + 1) / 2 + q] * s[q];
+      s[r] = t / S[r * (r + 1) / 2 + r];
+    }
+#pragma unroll
+    for (int r = 5; r >= 0; --r) {
+      double t = s[r];
+#pragma unroll
+      for (int q = r + 1; q < 6; ++q) t -= S[q * (q + 1) / 2 + r] * s[q];
+      s[r] = t / S[r * (r + 1) / 2 + r];
+    }
+    // x = u - M^-1 C^T lambda
+#pragma unroll 1
+    for (int ci = 0; ci < 6; ++ci) {
+      crow(P, ci, i, j, k, z, obl, row);
+      cholsolve12(L, row);
+#pragma unroll
+      for (int p = 0; p < 12; ++p) u[p] -= s[ci] * row[p];
+    }
+    const double aa = (P.Ur(5, 0, z) * P.Ur(5, 0, z) + P.Ur(6, 0, z) * P.Ur(6, 0, z) + P.Ur(7, 0, z) * P.Ur(7, 0, z)) *
+                      C<double>::ob8pi;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      a.B1_zc[z + P.NZ * m] = u[m];
+      a.B2_zc[z + P.NZ * m] = u[4 + m];
+      a.B3_zc[z + P.NZ * m] = u[8 + m];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) P.Vr(5 + c, m, z) = u[4 * c + m];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) P.Ur(5 + c, 0, z) = u[4 * c];
+    const double bb = (u[0] * u[0] + u[4] * u[4] + u[8] * u[8]) * C<double>::ob8pi;
+    P.Ur(4, 0, z) = P.Ur(4, 0, z) - aa + bb;
+  }
+  // step 9.a: E = -v x B from the zone means
+  const double vx = P.V(1, 0, z), vy = P.V(2, 0, z), vz = P.V(3, 0, z);
+  const double Bx = P.V(5, 0, z), By = P.V(6, 0, z), Bz = P.V(7, 0, z);
+  a.Ex_zc[z] = -(vy * Bz - vz * By);
+  a.Ey_zc[z] = -(vz * Bx - vx * Bz);
+  a.Ez_zc[z] = -(vx * By - vy * Bx);
+}
+
+// step 9.b
+__global__ void __launch_bounds__(128) k_weno_e(const P1 P, const Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  if (!P.active(i, j, 1)) return;
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k);
+  int64_t z6[6];
+  six(P, i, j, k, z6);
+  Weno W;
+  weno_build(P, z, z6, W);
+  const double omf = 1.0 - a.flatten[z];
+#pragma unroll 1
+  for (int c = 0; c < 3; ++c) {
+    double* E = c == 0 ? a.Ex_zc : (c == 1 ? a.Ey_zc : a.Ez_zc);
+    double f6[6], g3[3];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) f6[q] = E[z6[q]];
+    weno_apply(W, E[z], f6, a.smallnum, omf, g3);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) E[z + P.NZ * (m + 1)] = g3[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// step 10: per-block minima and maxima of the 8 primitives and the
+// normalised divergence over the owned active zones (18 values per block)
+constexpr int DIAG_THREADS = 256;
+__global__ void __launch_bounds__(DIAG_THREADS) k_diag(const P1 P, const Box b, double* part) {
+  __shared__ double sh[18][DIAG_THREADS];
+  double mn[9], mx[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    mn[q] = 1.0e30;
+    mx[q] = -1.0e30;
+  }
+  int i, j, k;
+  if (b.decode(tid(), i, j, k)) {
+    const cs_p1_patch& a = P.a;
+    const int64_t z = P.zi(i, j, k);
+    a.work[z] = 0.0;
+    const bool own = i >= 1 && i <= P.N && j >= 1 && j <= P.N && k >= 1 && k <= P.NR && a.mask[z] == 1;
+    if (own) {
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        mn[c] = fmin(mn[c], P.V(c, 0, z));
+        mx[c] = fmax(mx[c], P.V(c, 0, z));
+      }
+      const double bxp = a.Bxf[P.fci(i, j, k)], bxm = a.Bxf[P.fci(i - 1, j, k)];
+      const double byp = a.Byf[P.fci(i, j, k)], bym = a.Byf[P.fci(i, j - 1, k)];
+      const double bzp = a.Bzf[P.fci(i, j, k)], bzm = a.Bzf[P.fci(i, j, k - 1)];
+      double a21 = a.xf_area[P.xfi(i, j, k)] * bxp + a.yf_area[P.yfi(i, j, k)] * byp +
+                   a.zf_area[P.zfi(i, j, k)] * bzp - a.xf_area[P.xfi(i - 1, j, k)] * bxm -
+                   a.yf_area[P.yfi(i, j - 1, k)] * bym - a.zf_area[P.zfi(i, j, k - 1)] * bzm;
+      a.work[z] = a21;
+      const double v6 = P.V(5, 0, z), v7 = P.V(6, 0, z), v8 = P.V(7, 0, z), rho = P.V(0, 0, z);
+      const double a11 = v6 * v6 + v7 * v7 + v8 * v8;
+      const double ms = sqrt((a.gamma_sound[z] * P.V(4, 0, z) + C<double>::ob4pi * a11) / rho);
+      a.msonic[z] = ms;
+      const double den = fmax(fmax(sqrt(bxp * bxp + byp * byp + bzp * bzp), sqrt(bxm * bxm + bym * bym + bzm * bzm)),
+                              ms * sqrt(4.0 * 3.14 * rho));
+      a21 = a21 / den;
+      mn[8] = a21;
+      mx[8] = a21;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    sh[q][threadIdx.x] = mn[q];
+    sh[9 + q][threadIdx.x] = mx[q];
+  }
+  __syncthreads();
+  for (int w = DIAG_THREADS / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        sh[q][threadIdx.x] = fmin(sh[q][threadIdx.x], sh[q][threadIdx.x + w]);
+        sh[9 + q][threadIdx.x] = fmax(sh[9 + q][threadIdx.x], sh[9 + q][threadIdx.x + w]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 18) part[(int64_t)blockIdx.x * 18 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(64) k_diag_final(const double* part, int nblocks, double* out) {
+  const int q = threadIdx.x;
+  if (q >= 18) return;
+  double v = q < 9 ? 1.0e30 : -1.0e30;
+  for (int b = 0; b < nblocks; ++b) v = q < 9 ? fmin(v, part[(int64_t)b * 18 + q]) : fmax(v, part[(int64_t)b * 18 + q]);
+  out[q] = v;
+}
+
+// ---------------------------------------------------------------------------
+// per-device work memory: flags (3 x zones), counter, block partials
+struct Scratch {
+  void* mem = nullptr;
+  size_t bytes = 0;
+};
+static std::mutex g_mu;
+static std::vector<Scratch> g_scratch;
+
+}  // namespace csp1
+
+using namespace csp1;
+
+static int p1_check(const cs_p1_patch* p) {
+  if (!p) {
+    cs::set_error("cs_limit_all_slopes_p1: null patch");
+    return CS_E_INVALID;
+  }
+  if (p->ntp < 1 || p->nr < 1 || p->nghost < 4) {
+    cs::set_error("cs_limit_all_slopes_p1: need ntp >= 1, nr >= 1, nghost >= 4 (got %d, %d, %d)", p->ntp, p->nr,
+                  p->nghost);
+    return CS_E_INVALID;
+  }
+  if (p->nghost >= p->ntp) {
+    cs::set_error("cs_limit_all_slopes_p1: nghost (%d) must be below ntp (%d)", p->nghost, p->ntp);
+    return CS_E_BAD_DECOMP;
+  }
+  if (!(p->gamma > 1.0) || !(p->smallnum > 0.0) || !(p->rhofloor > 0.0) || !(p->prsfloor > 0.0)) {
+    cs::set_error("cs_limit_all_slopes_p1: gamma > 1 and positive floors and smallnum required");
+    return CS_E_INVALID;
+  }
+  const void* ptrs[] = {p->vert, p->mask, p->stencil, p->xf_normal, p->xf_area, p->xf_centroid, p->yf_normal,
+                        p->yf_area, p->yf_centroid, p->zf_area, p->zf_centroid, p->xb_pts, p->xb_wts,
+                        p->xb_normal, p->yb_pts, p->yb_wts, p->yb_normal, p->zb_pts, p->zb_wts, p->zb_normal,
+                        p->scale, p->v_centroid, p->U, p->ent, p->Bxf, p->Byf, p->Bzf, p->V, p->B1_zc, p->B2_zc,
+                        p->B3_zc, p->Ex_zc, p->Ey_zc, p->Ez_zc, p->gamma_eos, p->gamma_sound, p->mean_mol_wt,
+                        p->divvel, p->msonic, p->flatten, p->work, p->Bxf_q, p->Byf_q, p->Bzf_q};
+  for (const void* q : ptrs)
+    if (!q) {
+      cs::set_error("cs_limit_all_slopes_p1: null array in the patch");
+      return CS_E_INVALID;
+    }
+  return 0;
+}
+
+#define P1_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      cs::set_error("cs_limit_all_slopes_p1: %s: %s", #call, hipGetErrorString(e_)); \
+      return CS_E_HIP;                                                                \
+    }                                                                                 \
+  } while (0)
+
+static inline unsigned grid_of(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
+
+extern "C" int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream_) {
+  int rc = p1_check(p);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream_;
+  P1 P;
+  P.a = *p;
+  P.N = p->ntp;
+  P.NR = p->nr;
+  P.g = p->nghost;
+  P.nz = P.N + 2 * P.g;
+  P.nzr = P.NR + 2 * P.g;
+  P.nf = P.nz + 1;
+  P.nfr = P.nzr + 1;
+  P.NZ = (int64_t)P.nz * P.nz * P.nzr;
+  P.NXF = (int64_t)P.nf * P.nz * P.nzr;
+  P.NZF = (int64_t)P.nz * P.nz * P.nfr;
+  P.NFC = (int64_t)P.nf * P.nf * P.nfr;
+  const int N = P.N, NR = P.NR, g = P.g;
+  const Box all = make_box(1 - g, N + g, 1 - g, N + g, 1 - g, NR + g);
+  const Box wr = make_box(-2, N + 3, -2, N + 3, -2, NR + 3);
+  const Box own = make_box(1, N, 1, N, 1, NR);
+  const Box fx = make_box(-3, N + 3, -2, N + 3, -2, NR + 3), fy = make_box(-2, N + 3, -3, N + 3, -2, NR + 3),
+            fz = make_box(-2, N + 3, -2, N + 3, -3, NR + 3);
+  const int nblk_diag = (int)grid_of(all.count(), DIAG_THREADS);
+
+  int dev = 0;
+  P1_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_mu);
+  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
+  Scratch& S = g_scratch[dev];
+  const size_t flag_bytes = ((size_t)P.NZ + 255) / 256 * 256;
+  const size_t need = 3 * flag_bytes + 256 + (size_t)nblk_diag * 18 * sizeof(double) + 18 * sizeof(double);
+  if (S.bytes < need) {
+    if (S.mem) P1_HIP(hipFree(S.mem));
+    S.mem = nullptr;
+    S.bytes = 0;
+    if (hipMalloc(&S.mem, need) != hipSuccess) {
+      cs::set_error("cs_limit_all_slopes_p1: cannot allocate %zu bytes of work memory", need);
+      return CS_E_NOMEM;
+    }
+    S.bytes = need;
+  }
+  uint8_t* base = (uint8_t*)S.mem;
+  Flags F;
+  F.fa = base;
+  F.prev = base + flag_bytes;
+  F.next = base + 2 * flag_bytes;
+  F.counter = (unsigned int*)(base + 3 * flag_bytes);
+  double* part = (double*)(base + 3 * flag_bytes + 256);
+  double* red = part + (size_t)nblk_diag * 18;
+
+  k_prim<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  k_face_zero<<<dim3(grid_of(P.NFC, 256), 3), 256, 0, st>>>(P);
+  k_tricorner<<<grid_of(4 * (int64_t)g * P.nzr, 256), 256, 0, st>>>(P);
+  k_flat_a<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  k_flat_b<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  k_flat_c<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  // flags of zones outside the limiter's range stay clear
+  P1_HIP(hipMemsetAsync(F.fa, 0, P.NZ, st));
+  P1_HIP(hipMemsetAsync(F.counter, 0, sizeof(unsigned int), st));
+  k_weno_pos<<<grid_of(wr.count(), 128), 128, 0, st>>>(P, wr, F, 0);
+  P1_HIP(hipGetLastError());
+  unsigned int nfail = 0;
+  P1_HIP(hipMemcpyAsync(&nfail, F.counter, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+  P1_HIP(hipStreamSynchronize(st));
+  if (nfail > 0) {
+    // replay the loop-order dependence until the flags settle
+    P1_HIP(hipMemcpyAsync(F.prev, F.fa, P.NZ, hipMemcpyDeviceToDevice, st));
+    bool settled = false;
+    for (int it = 0; it < 256 && !settled; ++it) {
+      P1_HIP(hipMemsetAsync(F.counter, 0, sizeof(unsigned int), st));
+      P1_HIP(hipMemsetAsync(F.next, 0, P.NZ, st));
+      k_weno_pos<<<grid_of(wr.count(), 128), 128, 0, st>>>(P, wr, F, 1);
+      P1_HIP(hipGetLastError());
+      unsigned int nchg = 0;
+      P1_HIP(hipMemcpyAsync(&nchg, F.counter, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+      P1_HIP(hipStreamSynchronize(st));
+      std::swap(F.prev, F.next);
+      settled = nchg == 0;
+    }
+    if (!settled) {
+      cs::set_error("cs_limit_all_slopes_p1: positivity replay did not settle");
+      return CS_E_INVALID;
+    }
+    k_clear_ent<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, F.prev);
+  }
+  k_face_pts<<<dim3(grid_of(std::max(fx.count(), std::max(fy.count(), fz.count())), 256), 3), 256, 0, st>>>(
+      P, fx, fy, fz);
+  k_clsq_e<<<grid_of(all.count(), 64), 64, 0, st>>>(P, all);
+  k_weno_e<<<grid_of(wr.count(), 128), 128, 0, st>>>(P, wr);
+  k_diag<<<nblk_diag, DIAG_THREADS, 0, st>>>(P, all, part);
+  k_diag_final<<<1, 64, 0, st>>>(part, nblk_diag, red);
+  P1_HIP(hipGetLastError());
+  if (p->V_min_local || p->V_max_local) {
+    double h[18];
+    P1_HIP(hipMemcpyAsync(h, red, sizeof(h), hipMemcpyDeviceToHost, st));
+    P1_HIP(hipStreamSynchronize(st));
+    for (int q = 0; q < 9; ++q) {
+      if (p->V_min_local) p->V_min_local[q] = h[q];
+      if (p->V_max_local) p->V_max_local[q] = h[9 + q];
+    }
+  }
+  (void)own;
+  return 0;
+}

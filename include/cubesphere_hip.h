@@ -288,6 +288,49 @@ int cs_stencil_indexing(const cs_config* cfg, int32_t* istencil);
  * modes of a reconstruction of the given order (1-4) */
 int cs_mode_indices(int32_t order, int32_t* n_modes, int32_t* ii, int32_t* jj, int32_t* kk);
 
+/* ---- P1 zone reconstruction of one patch ---------------------------------
+ * LIMIT_ALL_SLOPES_P1 (400riemann_integrate/410limit_all_slopes_p1.f:9-160,
+ * called from the driver's stage loop) with ENFORCE_PRESSURE_POSITIVITY_PRIM
+ * (440enforce_pressure_positivity_prim.f:6): ideal MHD, P1 modes (mean and
+ * three gradients), PRESSURE_POSITIVITY 1, no B0.  Every array is a DEVICE
+ * pointer in the reference's own column-major layout for one patch of
+ * NGhost = nghost >= 4 (the reference's loop bounds reach 4 ghost layers):
+ *   zone    (1-g:N+g, 1-g:N+g, 1-g:nr+g) [, component 8][, mode 4]
+ *   face B  (0-g:N+g, 0-g:N+g, 0-g:nr+g) Bxf, Byf, Bzf (mode 1 only) [, point 5]
+ *   x-face  (0-g:N+g, 1-g:N+g, 1-g:nr+g), y-face (1-g:N+g, 0-g:N+g, 1-g:nr+g),
+ *   z-face  (1-g:N+g, 1-g:N+g, 0-g:nr+g) [, xyz][, point]
+ *   vert    (0-g:N+g, 0-g:N+g, 0-g:nr+g, 3); stencil (1-g:N+g, 1-g:N+g, 2, 12)
+ * (the layouts cs_build_geometry and cs_stencil_indexing produce, one sector
+ * per patch).  As in the reference, scale (v_scale_length) is written: the
+ * inactive tri-corner zones get smallnum.  V_min_local / V_max_local are HOST
+ * arrays of 9 (may be NULL).  stream: a hipStream_t, NULL = default stream.
+ * The call waits on the stream once mid-way (the positivity limiter's
+ * loop-order dependence) and at the end when the min/max are requested; calls
+ * on one device share its work memory and are serialised. */
+typedef struct cs_p1_patch {
+  int32_t ntp, nr, nghost;
+  int32_t itri[4];                 /* itri_patch_LD, _RD, _RU, _LU (1 = tri-corner) */
+  int32_t pad;
+  double gamma, molewt, rhofloor, prsfloor, smallnum, flatten_coef, prs_flatten_coef;
+  const double* vert;              /* vert_coords_patch */
+  const int32_t* mask;             /* imask_active_patch */
+  const int32_t* stencil;          /* istencil_patch */
+  const double *xf_normal, *xf_area, *xf_centroid; /* xf_quadnorml_patch, xf_area_patch, xf_centroid_patch */
+  const double *yf_normal, *yf_area, *yf_centroid;
+  const double *zf_area, *zf_centroid;
+  const double *xb_pts, *xb_wts, *xb_normal;       /* xb_quadpts_patch, xb_quadwt_patch, xb_quadnorml_patch */
+  const double *yb_pts, *yb_wts, *yb_normal;
+  const double *zb_pts, *zb_wts, *zb_normal;       /* zb normal: (face, 3, 5) */
+  double* scale;                   /* v_scale_length (in/out) */
+  const double* v_centroid;        /* v_centroid_patch */
+  double *U, *ent, *Bxf, *Byf, *Bzf;               /* in/out: U, entropygr, Bxf, Byf, Bzf */
+  double *V, *B1_zc, *B2_zc, *B3_zc, *Ex_zc, *Ey_zc, *Ez_zc;
+  double *gamma_eos, *gamma_sound, *mean_mol_wt, *divvel, *msonic, *flatten, *work;
+  double *Bxf_q, *Byf_q, *Bzf_q;   /* Bxf_quadpts, Byf_quadpts, Bzf_quadpts */
+  double *V_min_local, *V_max_local;
+} cs_p1_patch;
+int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream);
+
 int cs_last_error(char* buf, size_t len);
 int cs_device_info(char* buf, size_t len);
 
