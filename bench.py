@@ -11,6 +11,7 @@ One JSON line on stdout; details on stderr.
 """
 import argparse
 import json
+import math
 import os
 import subprocess
 import sys
@@ -311,6 +312,181 @@ def bench_geometry(cs, args):
     print(json.dumps(line))
 
 
+def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None):
+    """One sector of an N^2 x NR shell as a LIMIT_ALL_SLOPES_P1 patch on the
+    GPU: quadrature geometry from cs_build_geometry, the stencil table, and a
+    state with smooth flow, a front across the patch and log-normal noise on
+    density, pressure and entropy (so the flattener and the positivity limiter
+    act), generated with torch on the device."""
+    import torch
+    cfg = cs.make_config(N, NR, g, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
+                         cs.RMESH_UNIFORM, 1.0, 1.6)
+    if geo is None:
+        geo, _ = cs.build_geometry(cfg, skip=("v_pts", "v_wts", "v_tan", "v_vol", "moments"))
+    z = {k: v // 6 for k, v in cs.geometry_sizes(cfg).items()}  # one sector
+    dev = torch.device(device)
+    sec = lambda a, n, m: torch.from_numpy(a[: n * m]).to(dev)
+    T = dict(vert=sec(geo["vert"], z["vert"], 3), mask=sec(geo["mask"], z["zone"], 1),
+             stencil=torch.from_numpy(cs.stencil_indexing(cfg)).to(dev), scale=sec(geo["scale"], z["zone"], 1),
+             v_centroid=sec(geo["v_cen"], z["zone"], 3))
+    for f in ("xf", "yf", "zf"):
+        n = z[f]
+        if f != "zf":
+            T[f + "_normal"] = sec(geo[f]["normal"], n, 3)
+        T[f + "_area"] = sec(geo[f]["area"], n, 1)
+        T[f + "_centroid"] = sec(geo[f]["centroid"], n, 3)
+        b = f[0] + "b"
+        T[b + "_pts"] = sec(geo[b]["pts"], n, 15)
+        T[b + "_wts"] = sec(geo[b]["wts"], n, 5)
+        T[b + "_normal"] = sec(geo[b]["normal"], n, 15 if f == "zf" else 3)
+    del geo
+    cnt = cs.p1_counts(N, NR, g)
+    nz = cnt["mask"]
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    cen = T["v_centroid"].view(3, nz)
+    r = cen.norm(dim=0).clamp_min(1e-30)
+    x, y, w = cen / r
+    gam = 5.0 / 3.0
+    rho = 1.0 + 0.3 * torch.sin(3.0 * x + 1.0) * torch.cos(2.0 * y) + 0.2 * (r - 1.0)
+    prs = 1.0 + 0.2 * torch.cos(2.5 * w - 0.5 * x)
+    vel = torch.stack([0.3 * y - 0.2 * w, -0.3 * x + 0.1 * w, 0.2 * x * y])
+    act = T["mask"] == 1
+    c0 = torch.stack([x[act].mean(), y[act].mean(), w[act].mean()])
+    c0 = c0 / c0.norm()
+    t = torch.linalg.cross(c0, torch.tensor([0.3, 0.5, 0.8], dtype=torch.float64, device=dev), dim=0)
+    t = t / t.norm()
+    sfr = x * t[0] + y * t[1] + w * t[2] - 0.1
+    front = sfr > 0
+    rho = torch.where(front, 3.5 * rho, rho) * torch.exp(noise * torch.randn(nz, generator=gen, device=dev,
+                                                                             dtype=torch.float64))
+    prs = torch.where(front, 12.0 * prs, prs) * torch.exp(noise * torch.randn(nz, generator=gen, device=dev,
+                                                                              dtype=torch.float64))
+    sgn = torch.where(front, 1.0, -1.0).to(torch.float64)
+    vel = vel - 2.5 * sgn[None] * t[:, None] * (sfr.abs() < 0.12)[None]
+    Bc = torch.stack([0.5 + 0.2 * y, -0.3 + 0.1 * w * x, 0.2 * torch.cos(2 * x)])
+    U = torch.zeros(4, 8, nz, dtype=torch.float64, device=dev)
+    U[0, 0] = rho
+    U[0, 1:4] = rho * vel
+    U[0, 4] = prs / (gam - 1.0) + 0.5 * rho * (vel ** 2).sum(0) + (Bc ** 2).sum(0) / (8.0 * math.pi)
+    U[0, 5:8] = Bc
+    ent = torch.zeros(4, nz, dtype=torch.float64, device=dev)
+    ent[0] = prs / rho ** (gam - 1.0) * torch.exp(noise * torch.randn(nz, generator=gen, device=dev,
+                                                                      dtype=torch.float64))
+    T["U"] = U.view(-1)
+    T["ent"] = ent.view(-1)
+    nfc = cnt["Bxf"]
+    ar = torch.arange(nfc, device=dev, dtype=torch.float64)
+    for q, nm in enumerate(("Bxf", "Byf", "Bzf")):
+        T[nm] = 0.4 * torch.sin(ar * (0.37 + 0.11 * q)) + 0.1 * q
+    for k in cs.P1_ARRAYS:
+        if k not in T:
+            T[k] = torch.zeros(cnt[k], dtype=torch.float64, device=dev)
+    scal = dict(gamma=gam, molewt=1.0, rhofloor=1e-12, prsfloor=1e-9, smallnum=1e-12, flatten_coef=0.4,
+                prs_flatten_coef=1.5)
+    if dev.type == "cuda":
+        torch.cuda.synchronize()
+    return cfg, scal, T
+
+
+# algorithmic HBM bytes per item of the P1 stages (words read + written once):
+# WENO + positivity per zone in its range: V means 8 + entropy 1 of the zone
+# (its 6 neighbours' are re-reads, counted once), flattener 1, centroid 3,
+# scale 1, the 27-node geometry of the zone (vertex 3 and face centroids
+# 3 x 3, counted once per zone), V slopes 24 + entropy slopes 3 + B_zc 12
+# written; CLSQ + E per zone: V B modes 12 read and written, U 4 read and
+# written, scale 1, centroid 3, the zone's 3 own faces (B-field points 15,
+# normals 3 or 15, centroids 3, face B 1, face-point B 5), E 3 written.
+P1_BYTES = {"weno+positivity": 8 * (8 + 1 + 1 + 3 + 1 + 12 + 24 + 3 + 12),
+            "clsq+E": 8 * (24 + 8 + 1 + 3 + 2 * (15 + 3 + 3 + 1 + 5) + (15 + 15 + 3 + 1 + 5) + 3)}
+
+
+def bench_p1(cs, args):
+    """LIMIT_ALL_SLOPES_P1 on one sector of the 6 x 192^2 x 64 shell as a
+    patch (NGhost 4): owned-zone reconstructions per second, per-stage times."""
+    import torch
+    N, NR, g = 192, 64, 4
+    cfg, scal, T = p1_patch_on_gpu(cs, N, NR, g)
+    U0 = T["U"].clone()
+    e0 = T["ent"].clone()
+    sc0 = T["scale"].clone()
+    Bf0 = [T[k].clone() for k in ("Bxf", "Byf", "Bzf")]
+
+    def reset():
+        T["U"].copy_(U0)
+        T["ent"].copy_(e0)
+        T["scale"].copy_(sc0)
+        for k, b in zip(("Bxf", "Byf", "Bzf"), Bf0):
+            T[k].copy_(b)
+
+    for _ in range(max(1, args.warmup)):
+        reset()
+        cs.limit_all_slopes_p1(N, NR, g, scal, T)
+    torch.cuda.synchronize()
+    steps = max(1, min(args.steps, 50))
+    tot = 0.0
+    stage = None
+    for _ in range(steps):
+        reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        vmin, vmax, ms = cs.limit_all_slopes_p1(N, NR, g, scal, T, timed=True)
+        torch.cuda.synchronize()
+        tot += time.perf_counter() - t0
+        stage = ms if stage is None else [a + b for a, b in zip(stage, ms)]
+    ms_call = tot / steps * 1e3
+    stage = [s / steps for s in stage]
+    own = N * N * NR
+    cnt = cs.p1_counts(N, NR, g)
+    wr = (N + 6) ** 2 * (NR + 6)
+    kern = {}
+    for name, t in zip(cs.P1_STAGES, stage):
+        k = {"ms": t}
+        if name in P1_BYTES:
+            items = wr if name == "weno+positivity" else cnt["mask"]
+            k["items"] = items
+            k["GBps"] = P1_BYTES[name] * items / (t * 1e-3) / 1e9
+            k["frac"] = k["GBps"] / HBM_PEAK_GBS
+        kern[name] = k
+        log(f"   {name:20s} {t:8.3f} ms" + (f"  {k['GBps']:7.1f} GB/s algorithmic" if "GBps" in k else ""))
+    dom = max(kern, key=lambda n: kern[n]["ms"])
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_p1(args.cpu_seconds)
+    line = {
+        "metric": "P1 zone reconstructions/s", "value": own / (ms_call * 1e-3), "unit": "zones/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": ms_call, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"P1: LIMIT_ALL_SLOPES_P1 on one sector of the 6x{N}^2x{NR} shell as a patch, "
+                               f"NGhost {g} ({own} owned zones, {cnt['mask']} with ghosts); front + log-normal "
+                               "noise 0.3", "zones": own},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom].get("GBps"), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": kern[dom].get("frac"), "traffic": None},
+        "cpu_baseline": cpu, "kernels": kern,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline_p1(max_seconds):
+    """The reference's own LIMIT_ALL_SLOPES_P1 (built from its sources) on the
+    host, single thread, on a 32^2 x 16 patch of the same kind of data; the
+    oracle's restatement when that build is absent."""
+    from oracle import oracle as orc
+    prob, A0 = orc.p1_problem(32, 16, 4, sector=0, seed=7, shock=True, noise=0.3)
+    kind = "reference" if orc.rlib() is not None else "port"
+    fn = orc.p1_reference if kind == "reference" else orc.p1_oracle
+    calls, t0 = 0, time.perf_counter()
+    while True:
+        fn(prob, orc.p1_copy(A0))
+        calls += 1
+        el = time.perf_counter() - t0
+        if el > max_seconds or calls >= 200:
+            break
+    own = 32 * 32 * 16
+    return {"value": own * calls / el, "unit": "zones/s", "cores": 1, "kind": kind,
+            "sample": f"{calls} calls on a 32^2x16 patch (NGhost 4), single thread, {el:.1f} s"}
+
+
 def probe(cs, name, prec, kernel, iters):
     """Hardware-counter run: `iters` launches of one kernel on the workload's
     state, then one plain device copy of 1 GiB (read) + 1 GiB (write), whose
@@ -359,6 +535,11 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.workload == "P1":
+        # the P1 leg keeps its patch in torch tensors: torch's HIP runtime has to
+        # be initialised before the library's first HIP call
+        import torch
+        torch.zeros(1, device="cuda")
     import cubesphere_amd as cs
     log("device:", cs.device_info())
     if args.probe:
@@ -370,6 +551,9 @@ def main():
         return
     if args.workload == "G3":
         bench_geometry(cs, args)
+        return
+    if args.workload == "P1":
+        bench_p1(cs, args)
         return
     res = {}
     for prec in (cs.FP64, cs.MIXED):

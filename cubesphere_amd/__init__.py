@@ -125,6 +125,7 @@ def lib():
             "cs_mode_indices": [i32, ctypes.POINTER(i32), vp, vp, vp],
             "cs_stencil_indexing": [cfgp, vp],
             "cs_limit_all_slopes_p1": [vp, vp],
+            "cs_limit_all_slopes_p1_timed": [vp, vp, ctypes.POINTER(ctypes.c_float)],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
             "cs_batch_shell": [vp, i32, ctypes.POINTER(vp)],
@@ -431,22 +432,24 @@ def geometry_sizes(cfg):
     return dict(zip(("vert", "zone", "xf", "yf", "zf"), (int(x) for x in n)))
 
 
-def build_geometry(cfg, download=True):
+def build_geometry(cfg, download=True, skip=()):
     """The quadratic isoparametric geometry of the six sectors built on the GPU
     (EXTRACT_VERT_COORDS_PATCH, COORDS_TO_WT_QUADPTS_FACES_ISO2,
     COORDS_TO_WT_QUADPTS_VOLUME_ISO2, INIT_BASIS_FUNCTION_COEFS).  Returns
-    (arrays, kernel milliseconds); with download=False only the times."""
+    (arrays, kernel milliseconds); with download=False only the times.  Zone
+    arrays named in skip (e.g. "v_tan", "v_pts") are not copied back."""
     z = geometry_sizes(cfg)
     ms = (ctypes.c_float * 4)()
     if not download:
         _check(lib().cs_build_geometry(ctypes.byref(cfg), None, ms), "cs_build_geometry")
         return None, list(ms)
-    a = dict(vert=np.zeros(3 * z["vert"]), mask=np.zeros(z["zone"], np.int32), scale=np.zeros(z["zone"]),
-             v_pts=np.zeros(24 * z["zone"]), v_wts=np.zeros(8 * z["zone"]), v_tan=np.zeros(72 * z["zone"]),
-             v_vol=np.zeros(z["zone"]), v_cen=np.zeros(3 * z["zone"]), moments=np.zeros(4 * z["zone"]))
+    per = dict(vert=3 * z["vert"], scale=z["zone"], v_pts=24 * z["zone"], v_wts=8 * z["zone"],
+               v_tan=72 * z["zone"], v_vol=z["zone"], v_cen=3 * z["zone"], moments=4 * z["zone"])
+    a = {k: np.zeros(n) for k, n in per.items() if k not in skip}
+    a["mask"] = np.zeros(z["zone"], np.int32)
     G = Geometry()
     for k in ("vert", "mask", "scale", "v_pts", "v_wts", "v_tan", "v_vol", "v_cen", "moments"):
-        setattr(G, k, a[k].ctypes.data)
+        setattr(G, k, a[k].ctypes.data if k in a else None)
     for f in ("xf", "yf", "zf"):
         n = z[f]
         a[f] = dict(pts=np.zeros(15 * n), wts=np.zeros(5 * n), normal=np.zeros((15 if f == "zf" else 3) * n),
@@ -520,12 +523,17 @@ def p1_counts(N, NR, g):
     return c
 
 
-def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=None, minmax=True):
+P1_STAGES = ("prim", "face_zero+tricorner", "flatten", "weno+positivity", "face_pts", "clsq+E", "weno_E",
+             "minmax")
+
+
+def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=None, minmax=True, timed=False):
     """Run LIMIT_ALL_SLOPES_P1 on one patch whose arrays are torch tensors on
     the GPU (float64, int32 for mask and stencil), contiguous, in the
     reference's layouts; they are updated in place.  Every tensor's size is
     checked against the layout before the launch.  Returns (V_min_local,
-    V_max_local) as numpy arrays of 9 when minmax."""
+    V_max_local) as numpy arrays of 9 when minmax, and with timed=True also
+    the per-stage milliseconds (P1_STAGES)."""
     import torch
     want = p1_counts(N, NR, g)
     s = P1Patch()
@@ -547,6 +555,10 @@ def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=Non
         s.V_min_local = vmin.ctypes.data
         s.V_max_local = vmax.ctypes.data
     st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    if timed:
+        ms = (ctypes.c_float * len(P1_STAGES))()
+        _check(lib().cs_limit_all_slopes_p1_timed(ctypes.byref(s), st, ms), "cs_limit_all_slopes_p1_timed")
+        return vmin, vmax, list(ms)
     _check(lib().cs_limit_all_slopes_p1(ctypes.byref(s), st), "cs_limit_all_slopes_p1")
     return vmin, vmax
 

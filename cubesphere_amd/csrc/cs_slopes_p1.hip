@@ -11,12 +11,13 @@
 //   k_tricorner   step 3: face B synchronised at the tri-corners
 //   k_flat_a/b/c  step 6.a-c: velocity-divergence flattener, its pressure-
 //                 ratio extension, its spread to the six neighbours
-//   k_weno_pos    step 7.a and 7.b: multi-D WENO of the 8 primitives and the
-//                 entropy, then ENFORCE_PRESSURE_POSITIVITY_PRIM (440...f)
+//   k_weno9       step 7.a: multi-D WENO of the 8 primitives and the entropy,
+//                 one thread per (field, zone)
+//   k_pos         step 7.b: ENFORCE_PRESSURE_POSITIVITY_PRIM (440...f)
 //   k_face_pts    step 8.a-c: normal B at the five B-field points of each face
 //   k_clsq_e      step 8.d and 9.a: ADP constrained least squares for the 12
 //                 linear B coefficients; zone E = -v x B
-//   k_weno_e      step 9.b: WENO of E
+//   k_weno9       step 9.b: WENO of E (the same kernel, three fields)
 //   k_diag(+final) step 10: min/max of the primitives, face-B divergence
 //
 // The constrained least squares of 8.d (the reference assembles the 18 x 18
@@ -26,13 +27,14 @@
 //
 // The positivity limiter of the reference runs in loop order and, when a zone
 // fails every reduction, also zeroes its entropy mean (440...f:553-600), which
-// later zones of the loop then see in their entropy range.  k_weno_pos runs
-// all zones with the incoming entropy means and flags the zones that fail
+// later zones of the loop then see in their entropy range.  k_pos runs all
+// zones with the incoming entropy means and flags the zones that fail
 // completely; when any do, the zones with such a neighbour earlier in the
-// loop order are recomputed with those means taken as zero, repeating until
-// the flags no longer change (the dependence only runs forward in the loop
-// order, so this reaches the sequential result), and the flagged means are
-// then cleared.
+// loop order (k_cands) are recomputed (k_weno9, k_pos on that list) with the
+// means of the flagged earlier zones taken as zero, repeating until the flags
+// no longer change (the dependence only runs forward in the loop order, so
+// this reaches the sequential result), and the flagged means are then
+// cleared.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -95,7 +97,7 @@ struct Box {
     k = k0 + (int)(t / ((int64_t)ni * nj));
     return true;
   }
-  int64_t count() const { return (int64_t)ni * nj * nk; }
+  __host__ __device__ int64_t count() const { return (int64_t)ni * nj * nk; }
 };
 static Box make_box(int i0, int i1, int j0, int j1, int k0, int k1) {
   return Box{i0, j0, k0, i1 - i0 + 1, j1 - j0 + 1, k1 - k0 + 1};
@@ -390,80 +392,107 @@ __device__ __forceinline__ bool in_weno_range(const P1& P, int i, int j, int k) 
 }
 
 // ---------------------------------------------------------------------------
-// step 7: WENO then pressure positivity (440...f:6-830).
-// mode 0: every zone of the range, incoming entropy means, writes flagA.
-// mode 1: only zones with a flagged (prev) zone earlier in the loop order among
-// their 27 neighbours, whose entropy means count as zero; writes next.
+// step 7.a: WENO of the 8 primitives and the entropy, one thread per (field,
+// zone): thread t takes field t / n of zone t % n of the range (or of a zone
+// list).  Inactive columns keep the zero modes k_prim left.
+__device__ __forceinline__ void zone_of(const P1& P, int64_t z, int& i, int& j, int& k) {
+  i = (int)(z % P.nz) + 1 - P.g;
+  j = (int)((z / P.nz) % P.nz) + 1 - P.g;
+  k = (int)(z / ((int64_t)P.nz * P.nz)) + 1 - P.g;
+}
+
+// the fields of one WENO launch: the mean of field f at src[f][z], its three
+// gradient modes at dst[f][z + m * stride[f]], m = 0, 1, 2
+struct WenoFields {
+  const double* src[9];
+  double* dst[9];
+  int64_t stride[9];
+  int n;
+};
+
+__global__ void __launch_bounds__(128) k_weno9(const P1 P, const Box b, const int64_t* list, int64_t nlist,
+                                               const WenoFields W9) {
+  const int64_t t = tid(), n = list ? nlist : b.count();
+  if (t >= W9.n * n) return;
+  const int c = (int)(t / n);
+  int i, j, k;
+  int64_t z;
+  if (list) {
+    z = list[t % n];
+    zone_of(P, z, i, j, k);
+  } else {
+    b.decode(t % n, i, j, k);
+    z = P.zi(i, j, k);
+  }
+  if (!P.active(i, j, 1)) return;
+  const cs_p1_patch& a = P.a;
+  int64_t z6[6];
+  six(P, i, j, k, z6);
+  Weno W;
+  weno_build(P, z, z6, W);
+  const double* src = W9.src[c];
+  double f6[6], g3[3];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) f6[q] = src[z6[q]];
+  weno_apply(W, src[z], f6, a.smallnum, 1.0 - a.flatten[z], g3);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) W9.dst[c][z + W9.stride[c] * m] = g3[m];
+}
+
+// ---------------------------------------------------------------------------
+// step 7.b: ENFORCE_PRESSURE_POSITIVITY_PRIM (440...f:6-830).
+// pass 0: every zone of the range with the incoming entropy means; sets fa and
+// counts the zones whose slopes were removed.
+// pass 1: the zones of a list (those with an fa zone earlier in the loop order
+// among their 27), taking the entropy means of the prev-flagged earlier zones
+// as zero; writes next and counts the zones whose flag changed.
 struct Flags {
   uint8_t *fa, *prev, *next;
   unsigned int* counter;
 };
 
-__global__ void __launch_bounds__(128) k_weno_pos(const P1 P, const Box b, const Flags F, int mode) {
+// the 27 zones around (i, j, k): the column and its 8 stencil neighbours
+// (R, L, U, D, RU, LU, LD, RD), each at k-1, k, k+1, as the zone index of
+// layer 0 of each column
+__device__ __forceinline__ void columns9(const P1& P, int i, int j, int64_t* col) {
+  col[0] = P.zi(i, j, 0);
+  const int dirs[8] = {0, 1, 2, 3, 8, 9, 10, 11};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    int ai, bj;
+    P.nbr(i, j, dirs[q], ai, bj);
+    col[q + 1] = P.zi(ai, bj, 0);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_pos(const P1 P, const Box b, const int64_t* list, int64_t nlist,
+                                             const Flags F) {
+  const int64_t t = tid();
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
-  const cs_p1_patch& a = P.a;
-  const int64_t z = P.zi(i, j, k);
-  // the 27 zones: the column and its 8 stencil neighbours, at k-1, k, k+1
-  int64_t col[9];
-  {
-    col[0] = P.zi(i, j, 0);
-    const int dirs[8] = {0, 1, 2, 3, 8, 9, 10, 11};
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      int ai, bj;
-      P.nbr(i, j, dirs[q], ai, bj);
-      col[q + 1] = P.zi(ai, bj, 0);
-    }
-  }
-  const int64_t ks = (int64_t)P.nz * P.nz;
-  if (mode == 1) {
-    bool cand = false;
-    for (int q = 0; q < 9; ++q)
-      for (int dk = -1; dk <= 1; ++dk) {
-        const int64_t zz = col[q] + ks * (k + dk);
-        cand = cand || (zz < z && F.fa[zz]);
-      }
-    if (!cand) {
-      F.next[z] = F.fa[z];
-      return;
-    }
-  }
-  double vm[8][3], em[3];
-  if (P.active(i, j, 1)) {
-    int64_t z6[6];
-    six(P, i, j, k, z6);
-    Weno W;
-    weno_build(P, z, z6, W);
-    const double omf = 1.0 - a.flatten[z];
-#pragma unroll 1
-    for (int c = 0; c < 9; ++c) {
-      double f6[6], g3[3];
-      const double* src = c < 8 ? a.V + P.NZ * c : a.ent;
-#pragma unroll
-      for (int q = 0; q < 6; ++q) f6[q] = src[z6[q]];
-      weno_apply(W, src[z], f6, a.smallnum, omf, g3);
-      if (c < 8) {
-#pragma unroll
-        for (int m = 0; m < 3; ++m) P.Vr(c, m + 1, z) = g3[m];
-      } else {
-#pragma unroll
-        for (int m = 0; m < 3; ++m) em[m] = g3[m];
-      }
-    }
+  int64_t z;
+  if (list) {
+    if (t >= nlist) return;
+    z = list[t];
+    zone_of(P, z, i, j, k);
   } else {
-#pragma unroll
-    for (int m = 0; m < 3; ++m) em[m] = 0.0;
+    if (!b.decode(t, i, j, k)) return;
+    z = P.zi(i, j, k);
   }
+  const cs_p1_patch& a = P.a;
+  int64_t col[9];
+  columns9(P, i, j, col);
+  const int64_t ks = (int64_t)P.nz * P.nz;
   // ranges of rho, p and the entropy over the 27 zones, widened by 20 % where
   // the flattener is below 0.1 (440...f:169-280)
   double rmin = 1e300, rmax = -1e300, pmin = 1e300, pmax = -1e300, emin = 1e300, emax = -1e300;
+#pragma unroll 1
   for (int q = 0; q < 9; ++q)
+#pragma unroll
     for (int dk = -1; dk <= 1; ++dk) {
       const int64_t zz = col[q] + ks * (k + dk);
       const double r = P.V(0, 0, zz), p = P.V(4, 0, zz);
       double e = a.ent[zz];
-      if (mode == 1 && zz < z && F.prev[zz]) e = 0.0;
+      if (list && zz < z && F.prev[zz]) e = 0.0;
       rmin = fmin(rmin, r);
       rmax = fmax(rmax, r);
       pmin = fmin(pmin, p);
@@ -479,19 +508,37 @@ __global__ void __launch_bounds__(128) k_weno_pos(const P1 P, const Box b, const
   pmax = fmax(pmax * (1.0 + expandfac * omff), a.prsfloor);
   emin = emin * (1.0 - expandfac * omff);
   emax = emax * (1.0 + expandfac * omff);
-  // modes checked at the 27 nodes: rho, p and the entropy
-  double r0 = P.V(0, 0, z), p0 = P.V(4, 0, z), e0 = a.ent[z];
-  double rm[3], pm[3];
+  // the modes checked at the 27 nodes: rho, p and the entropy
+  const double r0 = P.V(0, 0, z), p0 = P.V(4, 0, z), e0 = a.ent[z];
+  double rm[3], pm[3], em[3];
 #pragma unroll
   for (int m = 0; m < 3; ++m) {
     rm[m] = P.V(0, m + 1, z);
     pm[m] = P.V(4, m + 1, z);
+    em[m] = a.ent[z + P.NZ * (m + 1)];
   }
-  // node coordinates relative to the centroid, scaled
   const double obl = P.obl(z);
   double c0[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) c0[c] = P.cen(z, c);
+  // the zone's 8 vertices and 6 face centroids, loaded once; the 27 nodes
+  // (vertices, face centroids, edge midpoints, centroid) are formed from them
+  double vx[8][3], fcx[6][3];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const int vi = (n & 1) ? i : i - 1, vj = (n & 2) ? j : j - 1, vk = (n & 4) ? k : k - 1;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) vx[n][c] = a.vert[P.fci(vi, vj, vk) + P.NFC * c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    fcx[0][c] = a.xf_centroid[P.xfi(i, j, k) + P.NXF * c];
+    fcx[1][c] = a.xf_centroid[P.xfi(i - 1, j, k) + P.NXF * c];
+    fcx[2][c] = a.yf_centroid[P.yfi(i, j, k) + P.NXF * c];
+    fcx[3][c] = a.yf_centroid[P.yfi(i, j - 1, k) + P.NXF * c];
+    fcx[4][c] = a.zf_centroid[P.zfi(i, j, k) + P.NZF * c];
+    fcx[5][c] = a.zf_centroid[P.zfi(i, j, k - 1) + P.NZF * c];
+  }
   const double ratio = 0.85769589859089412;  // 0.1^(1/15)
   int nred = 0;                              // reductions applied
   bool removed = false;
@@ -501,53 +548,31 @@ __global__ void __launch_bounds__(128) k_weno_pos(const P1 P, const Box b, const
       break;
     }
     bool pass = true;
-    for (int n = 0; n < 27; ++n) {
+    // one node: scaled offset from the centroid, then the three checks
+    auto node = [&](const double* x0) {
       double x[3];
-      if (n < 8) {
-        const int vi = (n & 1) ? i : i - 1, vj = (n & 2) ? j : j - 1, vk = (n & 4) ? k : k - 1;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) x[c] = a.vert[P.fci(vi, vj, vk) + P.NFC * c];
-      } else if (n < 14) {
-        const int f = n - 8;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          x[c] = f == 0 ? a.xf_centroid[P.xfi(i, j, k) + P.NXF * c]
-               : f == 1 ? a.xf_centroid[P.xfi(i - 1, j, k) + P.NXF * c]
-               : f == 2 ? a.yf_centroid[P.yfi(i, j, k) + P.NXF * c]
-               : f == 3 ? a.yf_centroid[P.yfi(i, j - 1, k) + P.NXF * c]
-               : f == 4 ? a.zf_centroid[P.zfi(i, j, k) + P.NZF * c]
-                        : a.zf_centroid[P.zfi(i, j, k - 1) + P.NZF * c];
-      } else if (n < 26) {
-        // edge midpoints (vertex pairs along x, then y, then z)
-        const int e = n - 14;
-        int va, vb;
-        if (e < 4) {
-          va = 2 * e;
-          vb = va + 1;
-        } else if (e < 8) {
-          const int q = e - 4;
-          va = (q & 1) + 4 * (q >> 1);
-          vb = va + 2;
-        } else {
-          va = e - 8;
-          vb = va + 4;
-        }
-        const int ai = (va & 1) ? i : i - 1, aj = (va & 2) ? j : j - 1, ak = (va & 4) ? k : k - 1;
-        const int bi = (vb & 1) ? i : i - 1, bj = (vb & 2) ? j : j - 1, bk = (vb & 4) ? k : k - 1;
-#pragma unroll
-        for (int c = 0; c < 3; ++c)
-          x[c] = (a.vert[P.fci(ai, aj, ak) + P.NFC * c] + a.vert[P.fci(bi, bj, bk) + P.NFC * c]) * 0.5;
-      } else {
-#pragma unroll
-        for (int c = 0; c < 3; ++c) x[c] = c0[c];
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) x[c] = (x[c] - c0[c]) * obl;
+      for (int c = 0; c < 3; ++c) x[c] = (x0[c] - c0[c]) * obl;
       const double rn = r0 + rm[0] * x[0] + rm[1] * x[1] + rm[2] * x[2];
       const double pn = p0 + pm[0] * x[0] + pm[1] * x[1] + pm[2] * x[2];
       const double en = e0 + em[0] * x[0] + em[1] * x[1] + em[2] * x[2];
       if (rn < rmin || rn > rmax || pn < pmin || pn > pmax || en < emin || en > emax) pass = false;
+    };
+#pragma unroll
+    for (int n = 0; n < 8; ++n) node(vx[n]);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) node(fcx[f]);
+    // edge midpoints: vertex pairs along x, then y, then z
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+      const int va = e < 4 ? 2 * e : (e < 8 ? ((e - 4) & 1) + 4 * ((e - 4) >> 1) : e - 8);
+      const int vb = e < 4 ? va + 1 : (e < 8 ? va + 2 : va + 4);
+      double m[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) m[c] = (vx[va][c] + vx[vb][c]) * 0.5;
+      node(m);
     }
+    node(c0);
     if (pass) break;
     ++nred;
 #pragma unroll
@@ -557,30 +582,56 @@ __global__ void __launch_bounds__(128) k_weno_pos(const P1 P, const Box b, const
       em[m] = ratio * em[m];
     }
   }
-  // apply the same reductions to every component (repeated, as the reference
-  // multiplies once per iteration) and store; B_zc copies the B modes
-  for (int c = 0; c < 8; ++c) {
+  // the same reductions on every component (repeated, as the reference
+  // multiplies once per iteration); B_zc copies the B modes
+  if (nred > 0 || removed) {
+    for (int c = 0; c < 8; ++c)
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      double v = P.V(c, m + 1, z);
-      if (removed) {
-        v = 0.0;
-      } else {
-        for (int r = 0; r < nred; ++r) v = ratio * v;
+      for (int m = 0; m < 3; ++m) {
+        double v = P.V(c, m + 1, z);
+        if (removed) {
+          v = 0.0;
+        } else {
+          for (int r = 0; r < nred; ++r) v = ratio * v;
+        }
+        P.Vr(c, m + 1, z) = v;
       }
-      P.Vr(c, m + 1, z) = v;
-      if (c >= 5) (c == 5 ? a.B1_zc : c == 6 ? a.B2_zc : a.B3_zc)[z + P.NZ * (m + 1)] = v;
-    }
-  }
-  a.B1_zc[z] = P.V(5, 0, z);
-  a.B2_zc[z] = P.V(6, 0, z);
-  a.B3_zc[z] = P.V(7, 0, z);
 #pragma unroll
-  for (int m = 0; m < 3; ++m) a.ent[z + P.NZ * (m + 1)] = removed ? 0.0 : em[m];
-  uint8_t* out = mode == 0 ? F.fa : F.next;
-  out[z] = removed ? 1 : 0;
-  if (mode == 0 && removed) atomicAdd(F.counter, 1u);
-  if (mode == 1 && (out[z] != F.prev[z])) atomicAdd(F.counter, 1u);
+    for (int m = 0; m < 3; ++m) a.ent[z + P.NZ * (m + 1)] = removed ? 0.0 : em[m];
+  }
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    a.B1_zc[z + P.NZ * m] = P.V(5, m, z);
+    a.B2_zc[z + P.NZ * m] = P.V(6, m, z);
+    a.B3_zc[z + P.NZ * m] = P.V(7, m, z);
+  }
+  const uint8_t flag = removed ? 1 : 0;
+  if (!list) {
+    F.fa[z] = flag;
+    if (removed) atomicAdd(F.counter, 1u);
+  } else {
+    F.next[z] = flag;
+    if (flag != F.prev[z]) atomicAdd(F.counter, 1u);
+  }
+}
+
+// the zones of the range with an fa zone earlier in the loop order among their
+// 27: appended to list (order immaterial)
+__global__ void __launch_bounds__(256) k_cands(const P1 P, const Box b, const uint8_t* fa, int64_t* list,
+                                               unsigned int* count) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int64_t z = P.zi(i, j, k), ks = (int64_t)P.nz * P.nz;
+  int64_t col[9];
+  columns9(P, i, j, col);
+  bool cand = false;
+  for (int q = 0; q < 9; ++q)
+#pragma unroll
+    for (int dk = -1; dk <= 1; ++dk) {
+      const int64_t zz = col[q] + ks * (k + dk);
+      cand = cand || (zz < z && fa[zz]);
+    }
+  if (cand) list[atomicAdd(count, 1u)] = z;
 }
 
 // the entropy means of the zones that lost all their slopes
@@ -875,30 +926,6 @@ __global__ void __launch_bounds__(64) k_clsq_e(const P1 P, const Box b) {
   a.Ez_zc[z] = -(vx * By - vy * Bx);
 }
 
-// step 9.b
-__global__ void __launch_bounds__(128) k_weno_e(const P1 P, const Box b) {
-  int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
-  if (!P.active(i, j, 1)) return;
-  const cs_p1_patch& a = P.a;
-  const int64_t z = P.zi(i, j, k);
-  int64_t z6[6];
-  six(P, i, j, k, z6);
-  Weno W;
-  weno_build(P, z, z6, W);
-  const double omf = 1.0 - a.flatten[z];
-#pragma unroll 1
-  for (int c = 0; c < 3; ++c) {
-    double* E = c == 0 ? a.Ex_zc : (c == 1 ? a.Ey_zc : a.Ez_zc);
-    double f6[6], g3[3];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) f6[q] = E[z6[q]];
-    weno_apply(W, E[z], f6, a.smallnum, omf, g3);
-#pragma unroll
-    for (int m = 0; m < 3; ++m) E[z + P.NZ * (m + 1)] = g3[m];
-  }
-}
-
 // ---------------------------------------------------------------------------
 // step 10: per-block minima and maxima of the 8 primitives and the
 // normalised divergence over the owned active zones (18 values per block)
@@ -959,12 +986,21 @@ __global__ void __launch_bounds__(DIAG_THREADS) k_diag(const P1 P, const Box b, 
   if (threadIdx.x < 18) part[(int64_t)blockIdx.x * 18 + threadIdx.x] = sh[threadIdx.x][0];
 }
 
-__global__ void __launch_bounds__(64) k_diag_final(const double* part, int nblocks, double* out) {
-  const int q = threadIdx.x;
-  if (q >= 18) return;
+// one block per quantity (9 minima, then 9 maxima)
+__global__ void __launch_bounds__(256) k_diag_final(const double* part, int nblocks, double* out) {
+  __shared__ double sh[256];
+  const int q = blockIdx.x;
   double v = q < 9 ? 1.0e30 : -1.0e30;
-  for (int b = 0; b < nblocks; ++b) v = q < 9 ? fmin(v, part[(int64_t)b * 18 + q]) : fmax(v, part[(int64_t)b * 18 + q]);
-  out[q] = v;
+  for (int b = threadIdx.x; b < nblocks; b += 256)
+    v = q < 9 ? fmin(v, part[(int64_t)b * 18 + q]) : fmax(v, part[(int64_t)b * 18 + q]);
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      sh[threadIdx.x] = q < 9 ? fmin(sh[threadIdx.x], sh[threadIdx.x + w]) : fmax(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = sh[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -1023,7 +1059,8 @@ static int p1_check(const cs_p1_patch* p) {
 
 static inline unsigned grid_of(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
-extern "C" int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream_) {
+// kernel_ms (optional): per-stage times from HIP events on the stream
+static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms) {
   int rc = p1_check(p);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream_;
@@ -1054,7 +1091,9 @@ extern "C" int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream_) {
   if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
   Scratch& S = g_scratch[dev];
   const size_t flag_bytes = ((size_t)P.NZ + 255) / 256 * 256;
-  const size_t need = 3 * flag_bytes + 256 + (size_t)nblk_diag * 18 * sizeof(double) + 18 * sizeof(double);
+  const size_t list_bytes = (size_t)wr.count() * sizeof(int64_t);
+  const size_t need = 3 * flag_bytes + 256 + (size_t)nblk_diag * 18 * sizeof(double) + 18 * sizeof(double) +
+                      list_bytes;
   if (S.bytes < need) {
     if (S.mem) P1_HIP(hipFree(S.mem));
     S.mem = nullptr;
@@ -1073,34 +1112,75 @@ extern "C" int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream_) {
   F.counter = (unsigned int*)(base + 3 * flag_bytes);
   double* part = (double*)(base + 3 * flag_bytes + 256);
   double* red = part + (size_t)nblk_diag * 18;
+  int64_t* list = (int64_t*)(red + 18);
+  unsigned int* lcount = F.counter + 1;
 
+  constexpr int NEV = CS_P1_STAGES + 1;
+  hipEvent_t ev[NEV];
+  int nev = 0;
+  if (kernel_ms) {
+    for (int q = 0; q < NEV; ++q) P1_HIP(hipEventCreate(&ev[q]));
+  }
+  auto mark = [&]() -> int {
+    if (kernel_ms) P1_HIP(hipEventRecord(ev[nev++], st));
+    return 0;
+  };
+  if ((rc = mark())) return rc;
   k_prim<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  if ((rc = mark())) return rc;
   k_face_zero<<<dim3(grid_of(P.NFC, 256), 3), 256, 0, st>>>(P);
   k_tricorner<<<grid_of(4 * (int64_t)g * P.nzr, 256), 256, 0, st>>>(P);
+  if ((rc = mark())) return rc;
   k_flat_a<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
   k_flat_b<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
   k_flat_c<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  if ((rc = mark())) return rc;
   // flags of zones outside the limiter's range stay clear
   P1_HIP(hipMemsetAsync(F.fa, 0, P.NZ, st));
-  P1_HIP(hipMemsetAsync(F.counter, 0, sizeof(unsigned int), st));
-  k_weno_pos<<<grid_of(wr.count(), 128), 128, 0, st>>>(P, wr, F, 0);
+  P1_HIP(hipMemsetAsync(F.counter, 0, 2 * sizeof(unsigned int), st));
+  WenoFields wv, we;
+  wv.n = 9;
+  for (int c = 0; c < 8; ++c) {
+    wv.src[c] = p->V + P.NZ * c;
+    wv.dst[c] = p->V + P.NZ * (c + 8);
+    wv.stride[c] = 8 * P.NZ;
+  }
+  wv.src[8] = p->ent;
+  wv.dst[8] = p->ent + P.NZ;
+  wv.stride[8] = P.NZ;
+  we.n = 3;
+  double* E3[3] = {p->Ex_zc, p->Ey_zc, p->Ez_zc};
+  for (int c = 0; c < 3; ++c) {
+    we.src[c] = E3[c];
+    we.dst[c] = E3[c] + P.NZ;
+    we.stride[c] = P.NZ;
+  }
+  k_weno9<<<grid_of(9 * wr.count(), 128), 128, 0, st>>>(P, wr, nullptr, 0, wv);
+  k_pos<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, nullptr, 0, F);
   P1_HIP(hipGetLastError());
   unsigned int nfail = 0;
   P1_HIP(hipMemcpyAsync(&nfail, F.counter, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
   P1_HIP(hipStreamSynchronize(st));
   if (nfail > 0) {
-    // replay the loop-order dependence until the flags settle
+    // replay the loop-order dependence on the zones it can reach until the
+    // flags settle
+    k_cands<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, F.fa, list, lcount);
+    unsigned int ncand = 0;
+    P1_HIP(hipMemcpyAsync(&ncand, lcount, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    P1_HIP(hipStreamSynchronize(st));
     P1_HIP(hipMemcpyAsync(F.prev, F.fa, P.NZ, hipMemcpyDeviceToDevice, st));
-    bool settled = false;
+    P1_HIP(hipMemcpyAsync(F.next, F.fa, P.NZ, hipMemcpyDeviceToDevice, st));
+    bool settled = ncand == 0;
     for (int it = 0; it < 256 && !settled; ++it) {
       P1_HIP(hipMemsetAsync(F.counter, 0, sizeof(unsigned int), st));
-      P1_HIP(hipMemsetAsync(F.next, 0, P.NZ, st));
-      k_weno_pos<<<grid_of(wr.count(), 128), 128, 0, st>>>(P, wr, F, 1);
+      k_weno9<<<grid_of(9 * (int64_t)ncand, 128), 128, 0, st>>>(P, wr, list, ncand, wv);
+      k_pos<<<grid_of(ncand, 256), 256, 0, st>>>(P, wr, list, ncand, F);
       P1_HIP(hipGetLastError());
       unsigned int nchg = 0;
       P1_HIP(hipMemcpyAsync(&nchg, F.counter, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
       P1_HIP(hipStreamSynchronize(st));
-      std::swap(F.prev, F.next);
+      // next now holds fa outside the list and the new flags on it
+      P1_HIP(hipMemcpyAsync(F.prev, F.next, P.NZ, hipMemcpyDeviceToDevice, st));
       settled = nchg == 0;
     }
     if (!settled) {
@@ -1109,13 +1189,23 @@ extern "C" int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream_) {
     }
     k_clear_ent<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, F.prev);
   }
+  if ((rc = mark())) return rc;
   k_face_pts<<<dim3(grid_of(std::max(fx.count(), std::max(fy.count(), fz.count())), 256), 3), 256, 0, st>>>(
       P, fx, fy, fz);
+  if ((rc = mark())) return rc;
   k_clsq_e<<<grid_of(all.count(), 64), 64, 0, st>>>(P, all);
-  k_weno_e<<<grid_of(wr.count(), 128), 128, 0, st>>>(P, wr);
+  if ((rc = mark())) return rc;
+  k_weno9<<<grid_of(3 * wr.count(), 128), 128, 0, st>>>(P, wr, nullptr, 0, we);
+  if ((rc = mark())) return rc;
   k_diag<<<nblk_diag, DIAG_THREADS, 0, st>>>(P, all, part);
-  k_diag_final<<<1, 64, 0, st>>>(part, nblk_diag, red);
+  k_diag_final<<<18, 256, 0, st>>>(part, nblk_diag, red);
+  if ((rc = mark())) return rc;
   P1_HIP(hipGetLastError());
+  if (kernel_ms) {
+    P1_HIP(hipEventSynchronize(ev[nev - 1]));
+    for (int q = 0; q + 1 < nev; ++q) P1_HIP(hipEventElapsedTime(&kernel_ms[q], ev[q], ev[q + 1]));
+    for (int q = 0; q < NEV; ++q) (void)hipEventDestroy(ev[q]);
+  }
   if (p->V_min_local || p->V_max_local) {
     double h[18];
     P1_HIP(hipMemcpyAsync(h, red, sizeof(h), hipMemcpyDeviceToHost, st));
@@ -1127,4 +1217,14 @@ extern "C" int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream_) {
   }
   (void)own;
   return 0;
+}
+
+extern "C" int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream) { return run_p1(p, stream, nullptr); }
+
+extern "C" int cs_limit_all_slopes_p1_timed(const cs_p1_patch* p, void* stream, float* kernel_ms) {
+  if (!kernel_ms) {
+    cs::set_error("cs_limit_all_slopes_p1_timed: kernel_ms is null");
+    return CS_E_INVALID;
+  }
+  return run_p1(p, stream, kernel_ms);
 }

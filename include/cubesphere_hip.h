@@ -330,6 +330,13 @@ typedef struct cs_p1_patch {
   double *V_min_local, *V_max_local;
 } cs_p1_patch;
 int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream);
+/* the same, timing its stages with HIP events on the stream (no counterpart
+ * in the reference: measurement only).  kernel_ms[CS_P1_STAGES]: primitives;
+ * face-B clearing and tri-corner sync; flattener; WENO + positivity (with any
+ * replay); face B points; constrained least squares + zone E; WENO of E;
+ * min/max */
+#define CS_P1_STAGES 8
+int cs_limit_all_slopes_p1_timed(const cs_p1_patch* p, void* stream, float* kernel_ms);
 
 int cs_last_error(char* buf, size_t len);
 int cs_device_info(char* buf, size_t len);
