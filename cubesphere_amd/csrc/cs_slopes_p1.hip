@@ -740,6 +740,26 @@ __device__ __forceinline__ void chol12(double* L) {
     }
   }
 }
+// x <- L^-1 x
+__device__ __forceinline__ void fwd12(const double* L, double* x) {
+#pragma unroll
+  for (int i = 0; i < 12; ++i) {
+    double s = x[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) s -= L[LIX(i, k)] * x[k];
+    x[i] = s / L[LIX(i, i)];
+  }
+}
+// x <- L^-T x
+__device__ __forceinline__ void bwd12(const double* L, double* x) {
+#pragma unroll
+  for (int i = 11; i >= 0; --i) {
+    double s = x[i];
+#pragma unroll
+    for (int k = i + 1; k < 12; ++k) s -= L[LIX(k, i)] * x[k];
+    x[i] = s / L[LIX(i, i)];
+  }
+}
 // x <- (L L^T)^-1 x
 __device__ __forceinline__ void cholsolve12(const double* L, double* x) {
 #pragma unroll
@@ -805,6 +825,7 @@ __device__ __forceinline__ double crow(const P1& P, int f, int i, int j, int k, 
 }
 
 __global__ void __launch_bounds__(64) k_clsq_e(const P1 P, const Box b) {
+  __shared__ double Zs[72][64];  // each thread's own column: no barrier needed
   int i, j, k;
   if (!b.decode(tid(), i, j, k)) return;
   const cs_p1_patch& a = P.a;
@@ -843,28 +864,31 @@ __global__ void __launch_bounds__(64) k_clsq_e(const P1 P, const Box b) {
       }
     }
     chol12(L);
-    cholsolve12(L, u);  // unconstrained least-squares solution
-    // Schur complement S = C M^-1 C^T, s = C u - d
+    // with Z = L^-1 C^T (kept in LDS) and zb = L^-1 A^T b: S = Z^T Z,
+    // s = Z^T zb - d, then x = L^-T (zb - Z lambda)
+    fwd12(L, u);
     double S[21], s[6];
 #pragma unroll 1
     for (int ci = 0; ci < 6; ++ci) {
-      double y[12];
-      const double dci = crow(P, ci, i, j, k, z, obl, y);
-      double cu = 0.0;
+      const double dci = crow(P, ci, i, j, k, z, obl, row);
+      fwd12(L, row);
+      double t = 0.0;
 #pragma unroll
-      for (int p = 0; p < 12; ++p) cu += y[p] * u[p];
-      s[ci] = cu - dci;
-      cholsolve12(L, y);
+      for (int p = 0; p < 12; ++p) {
+        t += row[p] * u[p];
+        Zs[ci * 12 + p][threadIdx.x] = row[p];
+      }
+      s[ci] = t - dci;
+    }
 #pragma unroll
-      for (int cj = 0; cj < 6; ++cj) {
-        if (cj < ci) continue;
-        crow(P, cj, i, j, k, z, obl, row);
+    for (int r = 0; r < 6; ++r)
+#pragma unroll
+      for (int c = 0; c <= r; ++c) {
         double t = 0.0;
 #pragma unroll
-        for (int p = 0; p < 12; ++p) t += row[p] * y[p];
-        S[cj * (cj + 1) / 2 + ci] = t;
+        for (int p = 0; p < 12; ++p) t += Zs[r * 12 + p][threadIdx.x] * Zs[c * 12 + p][threadIdx.x];
+        S[r * (r + 1) / 2 + c] = t;
       }
-    }
     // S lambda = s (6 x 6 Cholesky)
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
@@ -895,14 +919,15 @@ __global__ void __launch_bounds__(64) k_clsq_e(const P1 P, const Box b) {
       for (int q = r + 1; q < 6; ++q) t -= S[q * (q + 1) / 2 + r] * s[q];
       s[r] = t / S[r * (r + 1) / 2 + r];
     }
-    // x = u - M^-1 C^T lambda
-#pragma unroll 1
-    for (int ci = 0; ci < 6; ++ci) {
-      crow(P, ci, i, j, k, z, obl, row);
-      cholsolve12(L, row);
+    // x = L^-T (zb - Z lambda)
 #pragma unroll
-      for (int p = 0; p < 12; ++p) u[p] -= s[ci] * row[p];
+    for (int p = 0; p < 12; ++p) {
+      double t = u[p];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) t -= s[c] * Zs[c * 12 + p][threadIdx.x];
+      u[p] = t;
     }
+    bwd12(L, u);
     const double aa = (P.Ur(5, 0, z) * P.Ur(5, 0, z) + P.Ur(6, 0, z) * P.Ur(6, 0, z) + P.Ur(7, 0, z) * P.Ur(7, 0, z)) *
                       C<double>::ob8pi;
 #pragma unroll
