@@ -12,7 +12,7 @@
 //   k_flat_a/b/c  step 6.a-c: velocity-divergence flattener, its pressure-
 //                 ratio extension, its spread to the six neighbours
 //   k_weno9       step 7.a: multi-D WENO of the 8 primitives and the entropy,
-//                 one thread per (field, zone)
+//                 one thread per three fields of a zone
 //   k_pos         step 7.b: ENFORCE_PRESSURE_POSITIVITY_PRIM (440...f)
 //   k_face_pts    step 8.a-c: normal B at the five B-field points of each face
 //   k_clsq_e      step 8.d and 9.a: ADP constrained least squares for the 12
@@ -392,9 +392,10 @@ __device__ __forceinline__ bool in_weno_range(const P1& P, int i, int j, int k) 
 }
 
 // ---------------------------------------------------------------------------
-// step 7.a: WENO of the 8 primitives and the entropy, one thread per (field,
-// zone): thread t takes field t / n of zone t % n of the range (or of a zone
-// list).  Inactive columns keep the zero modes k_prim left.
+// step 7.a: WENO of the 8 primitives and the entropy, one thread per three
+// fields of a zone of the range (or of a zone list), so each zone's stencil
+// inverses are formed three times.  Inactive columns keep the zero modes
+// k_prim left.
 __device__ __forceinline__ void zone_of(const P1& P, int64_t z, int& i, int& j, int& k) {
   i = (int)(z % P.nz) + 1 - P.g;
   j = (int)((z / P.nz) % P.nz) + 1 - P.g;
@@ -412,9 +413,11 @@ struct WenoFields {
 
 __global__ void __launch_bounds__(128) k_weno9(const P1 P, const Box b, const int64_t* list, int64_t nlist,
                                                const WenoFields W9) {
+  // thread t: fields 3 (t / n) .. 3 (t / n) + 2 of zone t % n
   const int64_t t = tid(), n = list ? nlist : b.count();
-  if (t >= W9.n * n) return;
-  const int c = (int)(t / n);
+  const int ngroups = (W9.n + 2) / 3;
+  if (t >= ngroups * n) return;
+  const int c0 = 3 * (int)(t / n);
   int i, j, k;
   int64_t z;
   if (list) {
@@ -430,13 +433,17 @@ __global__ void __launch_bounds__(128) k_weno9(const P1 P, const Box b, const in
   six(P, i, j, k, z6);
   Weno W;
   weno_build(P, z, z6, W);
-  const double* src = W9.src[c];
-  double f6[6], g3[3];
+  const double omf = 1.0 - a.flatten[z];
+#pragma unroll 1
+  for (int c = c0; c < c0 + 3 && c < W9.n; ++c) {
+    const double* src = W9.src[c];
+    double f6[6], g3[3];
 #pragma unroll
-  for (int q = 0; q < 6; ++q) f6[q] = src[z6[q]];
-  weno_apply(W, src[z], f6, a.smallnum, 1.0 - a.flatten[z], g3);
+    for (int q = 0; q < 6; ++q) f6[q] = src[z6[q]];
+    weno_apply(W, src[z], f6, a.smallnum, omf, g3);
 #pragma unroll
-  for (int m = 0; m < 3; ++m) W9.dst[c][z + W9.stride[c] * m] = g3[m];
+    for (int m = 0; m < 3; ++m) W9.dst[c][z + W9.stride[c] * m] = g3[m];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1180,7 +1187,7 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms) {
     we.dst[c] = E3[c] + P.NZ;
     we.stride[c] = P.NZ;
   }
-  k_weno9<<<grid_of(9 * wr.count(), 128), 128, 0, st>>>(P, wr, nullptr, 0, wv);
+  k_weno9<<<grid_of(3 * wr.count(), 128), 128, 0, st>>>(P, wr, nullptr, 0, wv);
   k_pos<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, nullptr, 0, F);
   P1_HIP(hipGetLastError());
   unsigned int nfail = 0;
@@ -1198,7 +1205,7 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms) {
     bool settled = ncand == 0;
     for (int it = 0; it < 256 && !settled; ++it) {
       P1_HIP(hipMemsetAsync(F.counter, 0, sizeof(unsigned int), st));
-      k_weno9<<<grid_of(9 * (int64_t)ncand, 128), 128, 0, st>>>(P, wr, list, ncand, wv);
+      k_weno9<<<grid_of(3 * (int64_t)ncand, 128), 128, 0, st>>>(P, wr, list, ncand, wv);
       k_pos<<<grid_of(ncand, 256), 256, 0, st>>>(P, wr, list, ncand, F);
       P1_HIP(hipGetLastError());
       unsigned int nchg = 0;
@@ -1220,7 +1227,7 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms) {
   if ((rc = mark())) return rc;
   k_clsq_e<<<grid_of(all.count(), 64), 64, 0, st>>>(P, all);
   if ((rc = mark())) return rc;
-  k_weno9<<<grid_of(3 * wr.count(), 128), 128, 0, st>>>(P, wr, nullptr, 0, we);
+  k_weno9<<<grid_of(wr.count(), 128), 128, 0, st>>>(P, wr, nullptr, 0, we);
   if ((rc = mark())) return rc;
   k_diag<<<nblk_diag, DIAG_THREADS, 0, st>>>(P, all, part);
   k_diag_final<<<18, 256, 0, st>>>(part, nblk_diag, red);
