@@ -5,6 +5,7 @@
 ! calls these entry points instead of the per-patch routines:
 !   cs_halo_exchange       <- LOAD_LOCAL_BUFFERS_* / FETCH_BUFFERS_FILL_IN_* (DRV:1746-1756)
 !   cs_make_facial_fluxes  <- LIMIT_ALL_SLOPES_P1 + MAKE_FACIAL_FLUXES      (DRV:2396, 2485)
+!   cs_limit_all_slopes_p1 <- LIMIT_ALL_SLOPES_P1 alone, argument for argument (DRV:2396)
 !   cs_make_edge_efields   <- MAKE_EDGE_EFIELDS_P1                          (DRV:2548)
 !   cs_update_stage        <- UPDATE_IMEX_RK_STEP_P1 + CLEANUP_AFTER_UPDATE (DRV:2782, 2815)
 !   cs_evaluate_timestep   <- EVALUATE_TIMESTEP                             (DRV:2887)
@@ -28,7 +29,26 @@ module cs_hip_iface
     real(c_double) :: omega(3)
   end type
 
+  ! cs_p1_patch: the arguments of LIMIT_ALL_SLOPES_P1 (410limit_all_slopes_p1.f:9-60)
+  ! for one patch, as device addresses (c_loc of the arrays inside an
+  ! "!$acc host_data use_device(...)" region, or of HIP allocations).
+  type, bind(C) :: cs_p1_patch
+    integer(c_int32_t) :: ntp, nr, nghost, itri(4), pad
+    real(c_double) :: gamma, molewt, rhofloor, prsfloor, smallnum, flatten_coef, prs_flatten_coef
+    type(c_ptr) :: vert, mask, stencil, xf_normal, xf_area, xf_centroid, yf_normal, yf_area, yf_centroid
+    type(c_ptr) :: zf_area, zf_centroid, xb_pts, xb_wts, xb_normal, yb_pts, yb_wts, yb_normal
+    type(c_ptr) :: zb_pts, zb_wts, zb_normal, scale, v_centroid, U, ent, Bxf, Byf, Bzf
+    type(c_ptr) :: V, B1_zc, B2_zc, B3_zc, Ex_zc, Ey_zc, Ez_zc
+    type(c_ptr) :: gamma_eos, gamma_sound, mean_mol_wt, divvel, msonic, flatten, work
+    type(c_ptr) :: Bxf_q, Byf_q, Bzf_q, V_min_local, V_max_local
+  end type
+
   interface
+    integer(c_int) function cs_limit_all_slopes_p1(p, stream) bind(C, name="cs_limit_all_slopes_p1")
+      import :: c_int, c_ptr, cs_p1_patch
+      type(cs_p1_patch), intent(in) :: p
+      type(c_ptr), value :: stream
+    end function
     integer(c_int) function cs_create(cfg, h) bind(C, name="cs_create")
       import :: c_int, c_ptr, cs_config
       type(cs_config), intent(in) :: cfg
