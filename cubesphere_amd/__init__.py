@@ -126,6 +126,7 @@ def lib():
             "cs_stencil_indexing": [cfgp, vp],
             "cs_limit_all_slopes_p1": [vp, vp],
             "cs_limit_all_slopes_p1_timed": [vp, vp, ctypes.POINTER(ctypes.c_float)],
+            "cs_make_facial_fluxes_p1": [vp, vp],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
             "cs_batch_shell": [vp, i32, ctypes.POINTER(vp)],
@@ -561,4 +562,56 @@ def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=Non
         return vmin, vmax, list(ms)
     _check(lib().cs_limit_all_slopes_p1(ctypes.byref(s), st), "cs_limit_all_slopes_p1")
     return vmin, vmax
+
+
+# ---------------------------------------------------------------------------
+# MAKE_FACIAL_FLUXES of one patch (cs_make_facial_fluxes_p1)
+
+P1F_SCALARS = ("gamma", "rhofloor", "prsfloor", "smallnum")
+P1F_IN = ("vert", "mask", "stencil", "xf_pts", "xf_normal", "xf_area", "xf_centroid", "yf_pts", "yf_normal",
+          "yf_area", "yf_centroid", "zf_pts", "zf_normal", "zf_area", "zf_centroid", "scale", "v_volume",
+          "v_centroid", "V", "ent", "flatten")
+P1F_OUT = ("S_L", "S_R", "S_D", "S_U", "S_B", "S_T", "Fluct_x_Pl", "Fluct_y_Pl", "Fluct_z_Pl", "Fluct_x_Mi",
+           "Fluct_y_Mi", "Fluct_z_Mi", "dU_dt", "EntFlux_x", "EntFlux_y", "EntFlux_z", "Efx", "Efy", "Efz")
+
+
+class P1Flux(ctypes.Structure):
+    """cs_p1_flux: the arguments of MAKE_FACIAL_FLUXES for one patch."""
+    _fields_ = ([("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                 ("pad", ctypes.c_int32)]
+                + [(k, ctypes.c_double) for k in P1F_SCALARS]
+                + [(k, ctypes.c_void_p) for k in P1F_IN + P1F_OUT])
+
+
+def p1f_counts(N, NR, g):
+    c = p1_counts(N, NR, g)
+    zone, fc = c["mask"], c["Bxf"]
+    xf, zf = c["xf_area"], c["zf_area"]
+    c.update(xf_pts=15 * xf, yf_pts=15 * xf, zf_pts=15 * zf, zf_normal=15 * zf, v_volume=zone, dU_dt=8 * zone)
+    for k in ("S_L", "S_R", "S_D", "S_U", "S_B", "S_T", "EntFlux_x", "EntFlux_y", "EntFlux_z", "Efx", "Efy", "Efz"):
+        c[k] = fc
+    for k in ("Fluct_x_Pl", "Fluct_y_Pl", "Fluct_z_Pl", "Fluct_x_Mi", "Fluct_y_Mi", "Fluct_z_Mi"):
+        c[k] = 8 * fc
+    return c
+
+
+def make_facial_fluxes_p1(N, NR, g, scalars, arrays, stream=None):
+    """MAKE_FACIAL_FLUXES on one patch whose arrays are contiguous torch
+    tensors on the GPU in the reference's layouts (float64; int32 for mask and
+    stencil); outputs are written in place.  Sizes are checked first."""
+    import torch
+    want = p1f_counts(N, NR, g)
+    s = P1Flux()
+    s.ntp, s.nr, s.nghost = N, NR, g
+    for k in P1F_SCALARS:
+        setattr(s, k, float(scalars[k]))
+    for k in P1F_IN + P1F_OUT:
+        t = arrays[k]
+        dt = torch.int32 if k in ("mask", "stencil") else torch.float64
+        if not (t.is_cuda and t.dtype == dt and t.is_contiguous() and t.numel() == want[k]):
+            raise CsError(f"make_facial_fluxes_p1: {k} must be a contiguous {dt} CUDA tensor of {want[k]} "
+                          f"elements (got {t.dtype}, {t.numel()}, cuda={t.is_cuda})")
+        setattr(s, k, t.data_ptr())
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    _check(lib().cs_make_facial_fluxes_p1(ctypes.byref(s), st), "cs_make_facial_fluxes_p1")
 

@@ -1260,3 +1260,457 @@ extern "C" int cs_limit_all_slopes_p1_timed(const cs_p1_patch* p, void* stream, 
   }
   return run_p1(p, stream, kernel_ms);
 }
+
+// ===========================================================================
+// MAKE_FACIAL_FLUXES (400riemann_integrate/450make_facial_fluxes.f:5) for one
+// patch at second order, behind cs_make_facial_fluxes_p1.
+//
+//   k_ff_face   one face of one family (blockIdx.y): the two zones' linear
+//               fields at the face centroid, the face frame, the point Riemann
+//               solver OneD_RS_MHD_Ptwise; S, fluctuations and entropy flux
+//               written, and the two zones' shares of the update kept
+//   k_ff_gather one zone: dU_dt from the shares of its six faces, in the
+//               reference's order (x-faces, then y, then z; lower face first)
+//   k_ff_remap  the few zones that also take the share of a face whose own
+//               zone on that side is a tri-corner zone (the face then points
+//               at the stencil neighbour): dU_dt rebuilt with those shares in
+//               the order of the reference's loops
+namespace csp1 {
+
+struct P1F {
+  cs_p1_flux a;
+  int N, NR, g, nz, nzr, nf, nfr;
+  int64_t NZ, NXF, NZF, NFC;
+  __device__ __forceinline__ int64_t zi(int i, int j, int k) const {
+    return (int64_t)(i - 1 + g) + (int64_t)nz * ((j - 1 + g) + (int64_t)nz * (k - 1 + g));
+  }
+  __device__ __forceinline__ int64_t fci(int i, int j, int k) const {
+    return (int64_t)(i + g) + (int64_t)nf * ((j + g) + (int64_t)nf * (k + g));
+  }
+  __device__ __forceinline__ int64_t xfi(int i, int j, int k) const {
+    return (int64_t)(i + g) + (int64_t)nf * ((j - 1 + g) + (int64_t)nz * (k - 1 + g));
+  }
+  __device__ __forceinline__ int64_t yfi(int i, int j, int k) const {
+    return (int64_t)(i - 1 + g) + (int64_t)nz * ((j + g) + (int64_t)nf * (k - 1 + g));
+  }
+  __device__ __forceinline__ int64_t zfi(int i, int j, int k) const {
+    return (int64_t)(i - 1 + g) + (int64_t)nz * ((j - 1 + g) + (int64_t)nz * (k + g));
+  }
+  __device__ __forceinline__ void nbr(int i, int j, int n, int& ni, int& nj) const {
+    const int64_t s = (int64_t)(i - 1 + g) + (int64_t)nz * (j - 1 + g), plane = (int64_t)nz * nz;
+    ni = min(max(a.stencil[s + plane * (2 * n)], 1 - g), N + g);
+    nj = min(max(a.stencil[s + plane * (2 * n + 1)], 1 - g), N + g);
+  }
+  __device__ __forceinline__ bool active(int i, int j, int k) const { return a.mask[zi(i, j, k)] == 1; }
+  __device__ __forceinline__ double obl(int64_t z) const { return 1.0 / fmax(a.scale[z], a.smallnum); }
+  // face range of family fam (the reference's loops with iexpand = 1)
+  __device__ __forceinline__ bool in_range(int fam, int i, int j, int k) const {
+    return i >= (fam == 0 ? -1 : 0) && i <= N + 1 && j >= (fam == 1 ? -1 : 0) && j <= N + 1 &&
+           k >= (fam == 2 ? -1 : 0) && k <= NR + 1;
+  }
+};
+
+// OneD_RS_MHD_Ptwise with every output (610...f:96-220, 644-704), face frame
+__device__ __forceinline__ void rs_ptwise(const Gas<double>& gs, double* VL, double* VR, double entL, double entR,
+                                          double& SL, double& SR, double& entF, double* FlL, double* FlR,
+                                          double* FL, double* FR) {
+  const double bx = 0.5 * (VL[5] + VR[5]);
+  VL[5] = bx;
+  VR[5] = bx;
+  double UL[8], UR[8], Vm[8];
+  prim_to_cons(gs, entL, VL, UL);
+  prim_to_cons(gs, entR, VR, UR);
+  flux_x(VL, UL, FL);
+  flux_x(VR, UR, FR);
+  double smin, smax;
+  SR = -1e30;
+  SL = 1e30;
+  wavspd_x(gs, VL, smin, smax);
+  SL = fmin(SL, smin);
+  SR = fmax(SR, smax);
+  wavspd_x(gs, VR, smin, smax);
+  SL = fmin(SL, smin);
+  SR = fmax(SR, smax);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) Vm[c] = 0.5 * (VL[c] + VR[c]);
+  wavspd_x(gs, Vm, smin, smax);
+  SL = fmin(SL, smin);
+  SR = fmax(SR, smax);
+  if (SL >= 0.0) {
+    // U* = U_L before the one-sided fluctuation is formed (610...f:653-658)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      FlR[c] = SR * (UR[c] - UL[c]);
+      FlL[c] = 0.0;
+    }
+    entF = entL * VL[1];
+    return;
+  }
+  if (SR <= 0.0) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      FlL[c] = SL * (UR[c] - UL[c]);
+      FlR[c] = 0.0;
+    }
+    entF = entR * VR[1];
+    return;
+  }
+  const double t3 = 1.0 / (SR - SL), t1 = SR * t3, t2 = SL * t3;
+  double Us[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) Us[c] = t1 * UR[c] - t2 * UL[c] - t3 * (FR[c] - FL[c]);
+  const double entUs = t1 * entR - t2 * entL - t3 * (entR * VR[1] - entL * VL[1]);
+  const double pent = entUs * rpow_gm1(gs, Us[0]);
+  const double peng = gs.gm1 * (Us[4] - 0.5 * (Us[1] * Us[1] + Us[2] * Us[2] + Us[3] * Us[3]) / Us[0] -
+                                C<double>::ob8pi * (Us[5] * Us[5] + Us[6] * Us[6] + Us[7] * Us[7]));
+  if (peng <= gs.prsfloor) Us[4] = Us[4] + (pent - peng) * gs.igm1;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    FlL[c] = SL * (Us[c] - UL[c]);
+    FlR[c] = SR * (UR[c] - Us[c]);
+  }
+  const double o = 1.0 / (SR - SL);
+  entF = (SR * entL * VL[1] - SL * entR * VR[1]) * o + (entR - entL) * (SR * SL * o);
+}
+
+// one face; the shares cl (to the left zone) and cr (to the right) are kept
+// in share[fam][c][face] (c < 8 left, c >= 8 right); faces whose side zone was
+// re-pointed through the stencil are appended to the remap list
+__global__ void __launch_bounds__(128) k_ff_face(const P1F P, double* share, int64_t* rlist,
+                                                 unsigned int* rcount) {
+  const int fam = blockIdx.y;
+  const int64_t t = tid();
+  if (t >= P.NFC) return;
+  const int n = P.nf;
+  const int i = (int)(t % n) - P.g, j = (int)((t / n) % n) - P.g, k = (int)(t / ((int64_t)n * n)) - P.g;
+  if (!P.in_range(fam, i, j, k)) return;
+  const cs_p1_flux& a = P.a;
+  const int64_t fb = t;
+  int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1);
+  const int rk = k + (fam == 2);
+  int remapped = 0;  // 1: left side re-pointed, 2: right side
+  if (fam < 2) {
+    const bool al = P.active(li, lj, k), ar = P.active(ri, rj, k);
+    if (!al && ar) {
+      P.nbr(ri, rj, fam == 0 ? 1 : 3, li, lj);
+      remapped = 1;
+    } else if (al && !ar) {
+      P.nbr(li, lj, fam == 0 ? 0 : 2, ri, rj);
+      remapped = 2;
+    } else if (!al && !ar) {
+      return;
+    }
+  } else if (!P.active(i, j, k)) {
+    return;
+  }
+  const int64_t zl = P.zi(li, lj, k), zr = P.zi(ri, rj, fam == 2 ? rk : k);
+  const int64_t fg = fam == 0 ? P.xfi(i, j, k) : (fam == 1 ? P.yfi(i, j, k) : P.zfi(i, j, k));
+  const int64_t NF = fam == 2 ? P.NZF : P.NXF;
+  const double* nrm = fam == 0 ? a.xf_normal : (fam == 1 ? a.yf_normal : a.zf_normal);
+  const double* fcen = fam == 0 ? a.xf_centroid : (fam == 1 ? a.yf_centroid : a.zf_centroid);
+  const double* pts = fam == 0 ? a.xf_pts : (fam == 1 ? a.yf_pts : a.zf_pts);
+  const double sm = a.smallnum;
+  double A[3], B[3], Cv[3];
+  if (fam == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      A[c] = nrm[fg + NF * c];
+      Cv[c] = fcen[fg + NF * c];
+    }
+    const double s = 1.0 / fmax(sqrt(Cv[0] * Cv[0] + Cv[1] * Cv[1] + Cv[2] * Cv[2]), sm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Cv[c] = Cv[c] * s;
+    B[0] = Cv[1] * A[2] - Cv[2] * A[1];
+    B[1] = Cv[2] * A[0] - Cv[0] * A[2];
+    B[2] = Cv[0] * A[1] - Cv[1] * A[0];
+  } else if (fam == 1) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      A[c] = nrm[fg + NF * c];
+      B[c] = fcen[fg + NF * c];
+    }
+    const double s = 1.0 / fmax(sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]), sm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) B[c] = B[c] * s;
+    Cv[0] = A[1] * B[2] - A[2] * B[1];
+    Cv[1] = A[2] * B[0] - A[0] * B[2];
+    Cv[2] = A[0] * B[1] - A[1] * B[0];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) A[c] = nrm[fg + NF * c];
+    double s = 1.0 / fmax(sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]), sm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) A[c] = A[c] * s;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      B[c] = a.vert[P.fci(i, j, k) + P.NFC * c] + a.vert[P.fci(i, j - 1, k) + P.NFC * c] -
+             a.vert[P.fci(i - 1, j, k) + P.NFC * c] - a.vert[P.fci(i - 1, j - 1, k) + P.NFC * c];
+    s = B[0] * A[0] + B[1] * A[1] + B[2] * A[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) B[c] = B[c] - s * A[c];
+    s = 1.0 / fmax(sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]), sm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) B[c] = B[c] * s;
+    Cv[0] = A[1] * B[2] - A[2] * B[1];
+    Cv[1] = A[2] * B[0] - A[0] * B[2];
+    Cv[2] = A[0] * B[1] - A[1] * B[0];
+    s = 1.0 / fmax(sqrt(Cv[0] * Cv[0] + Cv[1] * Cv[1] + Cv[2] * Cv[2]), sm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Cv[c] = Cv[c] * s;
+    B[0] = Cv[1] * A[2] - Cv[2] * A[1];
+    B[1] = Cv[2] * A[0] - Cv[0] * A[2];
+    B[2] = Cv[0] * A[1] - Cv[1] * A[0];
+  }
+  // both zones' linear fields at the face centroid, into the face frame
+  double VL[8], VR[8], eL, eR;
+  {
+    const double ol = P.obl(zl), orr = P.obl(zr);
+    double rl[3], rr[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double x = pts[fg + NF * c];
+      rl[c] = (x - a.v_centroid[zl + P.NZ * c]) * ol;
+      rr[c] = (x - a.v_centroid[zr + P.NZ * c]) * orr;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const double* v = a.V + P.NZ * c;
+      VL[c] = v[zl] + v[zl + 8 * P.NZ] * rl[0] + v[zl + 16 * P.NZ] * rl[1] + v[zl + 24 * P.NZ] * rl[2];
+      VR[c] = v[zr] + v[zr + 8 * P.NZ] * rr[0] + v[zr + 16 * P.NZ] * rr[1] + v[zr + 24 * P.NZ] * rr[2];
+    }
+    eL = a.ent[zl] + a.ent[zl + P.NZ] * rl[0] + a.ent[zl + 2 * P.NZ] * rl[1] + a.ent[zl + 3 * P.NZ] * rl[2];
+    eR = a.ent[zr] + a.ent[zr + P.NZ] * rr[0] + a.ent[zr + 2 * P.NZ] * rr[1] + a.ent[zr + 3 * P.NZ] * rr[2];
+#pragma unroll
+    for (int o = 1; o <= 5; o += 4) {
+      const double la = A[0] * VL[o] + A[1] * VL[o + 1] + A[2] * VL[o + 2];
+      const double lb = B[0] * VL[o] + B[1] * VL[o + 1] + B[2] * VL[o + 2];
+      const double lc = Cv[0] * VL[o] + Cv[1] * VL[o + 1] + Cv[2] * VL[o + 2];
+      VL[o] = la;
+      VL[o + 1] = lb;
+      VL[o + 2] = lc;
+      const double ra = A[0] * VR[o] + A[1] * VR[o + 1] + A[2] * VR[o + 2];
+      const double rb = B[0] * VR[o] + B[1] * VR[o + 1] + B[2] * VR[o + 2];
+      const double rc = Cv[0] * VR[o] + Cv[1] * VR[o + 1] + Cv[2] * VR[o + 2];
+      VR[o] = ra;
+      VR[o + 1] = rb;
+      VR[o + 2] = rc;
+    }
+  }
+  const Gas<double> gs{a.gamma, a.gamma - 1.0, a.rhofloor, a.prsfloor, 1.0 / (a.gamma - 1.0)};
+  double SL, SR, eF, FlL[8], FlR[8], FL[8], FR[8];
+  rs_ptwise(gs, VL, VR, eL, eR, SL, SR, eF, FlL, FlR, FL, FR);
+  (fam == 0 ? a.S_L : fam == 1 ? a.S_D : a.S_B)[fb] = SL;
+  (fam == 0 ? a.S_R : fam == 1 ? a.S_U : a.S_T)[fb] = SR;
+  (fam == 0 ? a.EntFlux_x : fam == 1 ? a.EntFlux_y : a.EntFlux_z)[fb] = eF;
+  double* FPl = fam == 0 ? a.Fluct_x_Pl : (fam == 1 ? a.Fluct_y_Pl : a.Fluct_z_Pl);
+  double* FMi = fam == 0 ? a.Fluct_x_Mi : (fam == 1 ? a.Fluct_y_Mi : a.Fluct_z_Mi);
+  const double ar = (fam == 0 ? a.xf_area : fam == 1 ? a.yf_area : a.zf_area)[fg];
+  const double vl = a.v_volume[zl], vr = a.v_volume[zr];
+  double* sh = share + (int64_t)fam * 16 * P.NFC;
+  // back to the global frame: scalars as they are, vectors through (A, B, C)
+  auto back = [&](const double* f, int c) -> double {
+    if (c == 0 || c == 4) return f[c];
+    const int o = c < 4 ? 1 : 5, q = c - o;
+    return A[q] * f[o] + B[q] * f[o + 1] + Cv[q] * f[o + 2];
+  };
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    const double pl = back(FlR, c), mi = back(FlL, c), fll = back(FL, c), frr = back(FR, c);
+    FPl[fb + P.NFC * c] = pl;
+    FMi[fb + P.NFC * c] = mi;
+    sh[fb + P.NFC * c] = (mi + fll) * ar / vl;
+    sh[fb + P.NFC * (8 + c)] = (pl - frr) * ar / vr;
+  }
+  if (remapped) {
+    const unsigned int e = atomicAdd(rcount, 1u);
+    // target zone, then (family, face, side) packed
+    rlist[2 * (int64_t)e] = remapped == 1 ? zl : zr;
+    rlist[2 * (int64_t)e + 1] = (((int64_t)fam * P.NFC + fb) << 1) | (remapped == 1 ? 0 : 1);
+  }
+}
+
+// dU_dt of one zone from its own faces: x lower (as right zone), x upper (as
+// left zone), then y, then z
+__global__ void __launch_bounds__(256) k_ff_gather(const P1F P, const double* share) {
+  const int64_t z = tid();
+  if (z >= P.NZ) return;
+  const int i = (int)(z % P.nz) + 1 - P.g, j = (int)((z / P.nz) % P.nz) + 1 - P.g,
+            k = (int)(z / ((int64_t)P.nz * P.nz)) + 1 - P.g;
+  if (!P.active(i, j, k)) return;
+  double d[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) d[c] = 0.0;
+#pragma unroll
+  for (int fam = 0; fam < 3; ++fam) {
+    const double* sh = share + (int64_t)fam * 16 * P.NFC;
+    const int di = fam == 0, dj = fam == 1, dk = fam == 2;
+    if (P.in_range(fam, i - di, j - dj, k - dk)) {
+      const int64_t f = P.fci(i - di, j - dj, k - dk);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) d[c] = d[c] - sh[f + P.NFC * (8 + c)];
+    }
+    if (P.in_range(fam, i, j, k)) {
+      const int64_t f = P.fci(i, j, k);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) d[c] = d[c] - sh[f + P.NFC * c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) P.a.dU_dt[z + P.NZ * c] = d[c];
+}
+
+// zones that also take re-pointed shares: rebuilt in loop order (one thread
+// per list entry; the first entry of each zone does the work)
+__global__ void __launch_bounds__(64) k_ff_remap(const P1F P, const double* share, const int64_t* rlist,
+                                                 unsigned int n) {
+  const int64_t e = tid();
+  if (e >= n) return;
+  const int64_t T = rlist[2 * e];
+  for (unsigned int q = 0; q < (unsigned int)e; ++q)
+    if (rlist[2 * (int64_t)q] == T) return;
+  const int i = (int)(T % P.nz) + 1 - P.g, j = (int)((T / P.nz) % P.nz) + 1 - P.g,
+            k = (int)(T / ((int64_t)P.nz * P.nz)) + 1 - P.g;
+  double d[8];
+  for (int c = 0; c < 8; ++c) d[c] = 0.0;
+  for (int fam = 0; fam < 3; ++fam) {
+    // the shares reaching T in this family: (face, side), sorted by face
+    int64_t fc[8];
+    int sd[8], m = 0;
+    const int di = fam == 0, dj = fam == 1, dk = fam == 2;
+    if (P.active(i, j, k)) {
+      if (P.in_range(fam, i - di, j - dj, k - dk)) {
+        fc[m] = P.fci(i - di, j - dj, k - dk);
+        sd[m++] = 1;
+      }
+      if (P.in_range(fam, i, j, k)) {
+        fc[m] = P.fci(i, j, k);
+        sd[m++] = 0;
+      }
+    }
+    for (unsigned int q = 0; q < n && m < 8; ++q) {
+      if (rlist[2 * (int64_t)q] != T) continue;
+      const int64_t code = rlist[2 * (int64_t)q + 1], ff = code >> 1;
+      if (ff / P.NFC != fam) continue;
+      fc[m] = ff % P.NFC;
+      sd[m++] = (int)(code & 1);
+    }
+    for (int a1 = 1; a1 < m; ++a1)  // insertion sort by face index
+      for (int b1 = a1; b1 > 0 && fc[b1] < fc[b1 - 1]; --b1) {
+        const int64_t tf = fc[b1];
+        fc[b1] = fc[b1 - 1];
+        fc[b1 - 1] = tf;
+        const int ts = sd[b1];
+        sd[b1] = sd[b1 - 1];
+        sd[b1 - 1] = ts;
+      }
+    const double* sh = share + (int64_t)fam * 16 * P.NFC;
+    for (int q = 0; q < m; ++q)
+      for (int c = 0; c < 8; ++c) d[c] = d[c] - sh[fc[q] + P.NFC * (8 * sd[q] + c)];
+  }
+  for (int c = 0; c < 8; ++c) P.a.dU_dt[T + P.NZ * c] = d[c];
+}
+
+__global__ void __launch_bounds__(256) k_ff_zero_e(const P1F P) {
+  const int64_t z = tid();
+  if (z >= P.NZ) return;
+  const int i = (int)(z % P.nz) + 1 - P.g, j = (int)((z / P.nz) % P.nz) + 1 - P.g,
+            k = (int)(z / ((int64_t)P.nz * P.nz)) + 1 - P.g;
+  const int64_t f = P.fci(i, j, k);
+  P.a.Efx[f] = 0.0;
+  P.a.Efy[f] = 0.0;
+  P.a.Efz[f] = 0.0;
+}
+
+struct FfScratch {
+  void* mem = nullptr;
+  size_t bytes = 0;
+};
+static std::vector<FfScratch> g_ff_scratch;
+
+}  // namespace csp1
+
+extern "C" int cs_make_facial_fluxes_p1(const cs_p1_flux* p, void* stream_) {
+  if (!p) {
+    cs::set_error("cs_make_facial_fluxes_p1: null argument");
+    return CS_E_INVALID;
+  }
+  if (p->ntp < 1 || p->nr < 1 || p->nghost < 4) {
+    cs::set_error("cs_make_facial_fluxes_p1: need ntp >= 1, nr >= 1, nghost >= 4 (got %d, %d, %d)", p->ntp, p->nr,
+                  p->nghost);
+    return CS_E_INVALID;
+  }
+  if (p->nghost >= p->ntp) {
+    cs::set_error("cs_make_facial_fluxes_p1: nghost (%d) must be below ntp (%d)", p->nghost, p->ntp);
+    return CS_E_BAD_DECOMP;
+  }
+  if (!(p->gamma > 1.0) || !(p->smallnum > 0.0) || !(p->rhofloor > 0.0) || !(p->prsfloor > 0.0)) {
+    cs::set_error("cs_make_facial_fluxes_p1: gamma > 1 and positive floors and smallnum required");
+    return CS_E_INVALID;
+  }
+  const void* ptrs[] = {p->vert, p->mask, p->stencil, p->xf_pts, p->xf_normal, p->xf_area, p->xf_centroid,
+                        p->yf_pts, p->yf_normal, p->yf_area, p->yf_centroid, p->zf_pts, p->zf_normal, p->zf_area,
+                        p->zf_centroid, p->scale, p->v_volume, p->v_centroid, p->V, p->ent, p->flatten, p->S_L,
+                        p->S_R, p->S_D, p->S_U, p->S_B, p->S_T, p->Fluct_x_Pl, p->Fluct_y_Pl, p->Fluct_z_Pl,
+                        p->Fluct_x_Mi, p->Fluct_y_Mi, p->Fluct_z_Mi, p->dU_dt, p->EntFlux_x, p->EntFlux_y,
+                        p->EntFlux_z, p->Efx, p->Efy, p->Efz};
+  for (const void* q : ptrs)
+    if (!q) {
+      cs::set_error("cs_make_facial_fluxes_p1: null array");
+      return CS_E_INVALID;
+    }
+  hipStream_t st = (hipStream_t)stream_;
+  P1F P;
+  P.a = *p;
+  P.N = p->ntp;
+  P.NR = p->nr;
+  P.g = p->nghost;
+  P.nz = P.N + 2 * P.g;
+  P.nzr = P.NR + 2 * P.g;
+  P.nf = P.nz + 1;
+  P.nfr = P.nzr + 1;
+  P.NZ = (int64_t)P.nz * P.nz * P.nzr;
+  P.NXF = (int64_t)P.nf * P.nz * P.nzr;
+  P.NZF = (int64_t)P.nz * P.nz * P.nfr;
+  P.NFC = (int64_t)P.nf * P.nf * P.nfr;
+  int dev = 0;
+  P1_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_mu);
+  if ((int)g_ff_scratch.size() <= dev) g_ff_scratch.resize(dev + 1);
+  FfScratch& S = g_ff_scratch[dev];
+  const size_t share_bytes = (size_t)48 * P.NFC * sizeof(double);
+  const size_t list_cap = (size_t)2 * P.NFC;  // faces of two families at most
+  const size_t need = share_bytes + 256 + list_cap * 2 * sizeof(int64_t);
+  if (S.bytes < need) {
+    if (S.mem) P1_HIP(hipFree(S.mem));
+    S.mem = nullptr;
+    S.bytes = 0;
+    if (hipMalloc(&S.mem, need) != hipSuccess) {
+      cs::set_error("cs_make_facial_fluxes_p1: cannot allocate %zu bytes of work memory", need);
+      return CS_E_NOMEM;
+    }
+    S.bytes = need;
+  }
+  double* share = (double*)S.mem;
+  unsigned int* rcount = (unsigned int*)((uint8_t*)S.mem + share_bytes);
+  int64_t* rlist = (int64_t*)((uint8_t*)S.mem + share_bytes + 256);
+  // every fluctuation and the whole update start at zero (450...f:360-380);
+  // the electric fields of the zone range too (:400-420)
+  P1_HIP(hipMemsetAsync(p->dU_dt, 0, sizeof(double) * 8 * P.NZ, st));
+  double* fl[6] = {p->Fluct_x_Pl, p->Fluct_y_Pl, p->Fluct_z_Pl, p->Fluct_x_Mi, p->Fluct_y_Mi, p->Fluct_z_Mi};
+  for (double* q : fl) P1_HIP(hipMemsetAsync(q, 0, sizeof(double) * 8 * P.NFC, st));
+  P1_HIP(hipMemsetAsync(rcount, 0, sizeof(unsigned int), st));
+  k_ff_zero_e<<<grid_of(P.NZ, 256), 256, 0, st>>>(P);
+  k_ff_face<<<dim3(grid_of(P.NFC, 128), 3), 128, 0, st>>>(P, share, rlist, rcount);
+  k_ff_gather<<<grid_of(P.NZ, 256), 256, 0, st>>>(P, share);
+  P1_HIP(hipGetLastError());
+  unsigned int nrm = 0;
+  P1_HIP(hipMemcpyAsync(&nrm, rcount, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+  P1_HIP(hipStreamSynchronize(st));
+  if (nrm > list_cap) {
+    cs::set_error("cs_make_facial_fluxes_p1: %u re-pointed faces exceed the list", nrm);
+    return CS_E_INVALID;
+  }
+  if (nrm > 0) k_ff_remap<<<grid_of(nrm, 64), 64, 0, st>>>(P, share, rlist, nrm);
+  P1_HIP(hipGetLastError());
+  return 0;
+}

@@ -338,6 +338,36 @@ int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream);
 #define CS_P1_STAGES 8
 int cs_limit_all_slopes_p1_timed(const cs_p1_patch* p, void* stream, float* kernel_ms);
 
+/* ---- P1 facial fluxes of one patch -----------------------------------------
+ * MAKE_FACIAL_FLUXES (400riemann_integrate/450make_facial_fluxes.f:5-60) at
+ * second order: per face, the two zones' P1 fields (V, entropygr from
+ * cs_limit_all_slopes_p1) at the face centroid, rotated into the face frame,
+ * OneD_RS_MHD_Ptwise (600riemann_1d/610...f:13), the fluctuations rotated back
+ * and scattered into dU_dt of both zones.  MHD, no facial electric fields
+ * (Efx, Efy, Efz are zeroed over the zone range, as the routine does), no B0.
+ * Device arrays in the layouts of cs_p1_patch; face outputs (S_*, Fluct_*,
+ * EntFlux_*) in the face-B layout (0-g:N+g)^3-shaped [, 8]; dU_dt (zone, 8).
+ * The call waits on the stream once (the count of faces re-pointed at the
+ * tri-corners). */
+typedef struct cs_p1_flux {
+  int32_t ntp, nr, nghost, pad;
+  double gamma, rhofloor, prsfloor, smallnum;
+  const double* vert;
+  const int32_t* mask;
+  const int32_t* stencil;
+  const double *xf_pts, *xf_normal, *xf_area, *xf_centroid; /* x-face area quadrature (220...f) */
+  const double *yf_pts, *yf_normal, *yf_area, *yf_centroid;
+  const double *zf_pts, *zf_normal, *zf_area, *zf_centroid; /* zf normal: (face, 3, 5) */
+  const double *scale, *v_volume, *v_centroid;
+  const double *V, *ent, *flatten;                        /* from LIMIT_ALL_SLOPES_P1 */
+  double *S_L, *S_R, *S_D, *S_U, *S_B, *S_T;             /* S_L_gr ... S_T_gr */
+  double *Fluct_x_Pl, *Fluct_y_Pl, *Fluct_z_Pl, *Fluct_x_Mi, *Fluct_y_Mi, *Fluct_z_Mi;
+  double* dU_dt;
+  double *EntFlux_x, *EntFlux_y, *EntFlux_z;
+  double *Efx, *Efy, *Efz;
+} cs_p1_flux;
+int cs_make_facial_fluxes_p1(const cs_p1_flux* p, void* stream);
+
 int cs_last_error(char* buf, size_t len);
 int cs_device_info(char* buf, size_t len);
 

@@ -122,4 +122,29 @@ int or_limit_all_slopes_p1(const or_p1_patch* p);
 /* zones of the last call with reduced slopes, and with slopes removed */
 void or_p1_positivity_stats(long* out);
 
+/* MAKE_FACIAL_FLUXES (450make_facial_fluxes.f:5) of one patch at second order
+ * (cs_oracle_p1.c); the same field order as cs_p1_flux in the product header */
+typedef struct or_p1_flux {
+  int32_t ntp, nr, nghost, pad;
+  double gamma, rhofloor, prsfloor, smallnum;
+  const double* vert;
+  const int32_t* mask;
+  const int32_t* stencil;
+  const double *xf_pts, *xf_normal, *xf_area, *xf_centroid;
+  const double *yf_pts, *yf_normal, *yf_area, *yf_centroid;
+  const double *zf_pts, *zf_normal, *zf_area, *zf_centroid;
+  const double *scale, *v_volume, *v_centroid;
+  const double *V, *ent, *flatten;
+  double *S_L, *S_R, *S_D, *S_U, *S_B, *S_T;
+  double *Fluct_x_Pl, *Fluct_y_Pl, *Fluct_z_Pl, *Fluct_x_Mi, *Fluct_y_Mi, *Fluct_z_Mi;
+  double* dU_dt;
+  double *EntFlux_x, *EntFlux_y, *EntFlux_z;
+  double *Efx, *Efy, *Efz;
+} or_p1_flux;
+int or_make_facial_fluxes_p1(const or_p1_flux* f);
+/* OneD_RS_MHD_Ptwise with all its outputs (face frame, x-direction) */
+void or_rs_ptwise(double gam, double rhof, double prsf, double* VL, double* VR, double entL, double entR,
+                  double* SL, double* SR, double* Us, double* Fs, double* entF, double* FlL, double* FlR,
+                  double* UL, double* UR, double* FL, double* FR);
+
 #endif

@@ -730,3 +730,237 @@ int or_limit_all_slopes_p1(const or_p1_patch* p) {
     for (q = 0; q < 9; ++q) p->V_max_local[q] = vmax[q];
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* OneD_RS_MHD_Ptwise (600riemann_1d/610_1d_rs_oned_rs_mhd_ptwise.f:96-220,
+ * 644-704), MHD, no eigensystem: every output of the routine.  The x-flux is
+ * the ideal-MHD flux with B0 = 0 (the routine calls CONS_PRIM_TO_FLUX_3D, which
+ * is absent from the reference tree; or_flux_x writes the flux of
+ * 600_1d_rs_oned_rs_mhd_dad.f:194-230).  In the supersonic branches the state
+ * U* is set to the upwind state before the one-sided fluctuation is formed
+ * (:653-666), as there. */
+void or_rs_ptwise(double gam, double rhof, double prsf, double* VL, double* VR, double entL, double entR,
+                  double* SL, double* SR, double* Us, double* Fs, double* entF, double* FlL, double* FlR,
+                  double* UL, double* UR, double* FL, double* FR) {
+  const double pi = 4.0 * atan(1.0), ob8pi = 1.0 / (8.0 * pi);
+  double Vm[8], a, b, t1, t2, t3, eU, pe, pg;
+  int c;
+  double bx = 0.5 * (VL[5] + VR[5]);
+  VL[5] = bx;
+  VR[5] = bx;
+  or_prim_to_cons(gam, rhof, prsf, entL, VL, UL);
+  or_prim_to_cons(gam, rhof, prsf, entR, VR, UR);
+  or_flux_x(VL, UL, FL);
+  or_flux_x(VR, UR, FR);
+  *SR = -1.0e30;
+  *SL = 1.0e30;
+  or_wavspd_x(gam, VL, &a, &b);
+  *SL = omin(*SL, a);
+  *SR = omax(*SR, b);
+  or_wavspd_x(gam, VR, &a, &b);
+  *SL = omin(*SL, a);
+  *SR = omax(*SR, b);
+  for (c = 0; c < 8; ++c) Vm[c] = 0.5 * (VL[c] + VR[c]);
+  or_wavspd_x(gam, Vm, &a, &b);
+  *SL = omin(*SL, a);
+  *SR = omax(*SR, b);
+  t3 = 1.0 / (*SR - *SL);
+  t1 = *SR * t3;
+  t2 = *SL * t3;
+  for (c = 0; c < 8; ++c) Us[c] = t1 * UR[c] - t2 * UL[c] - t3 * (FR[c] - FL[c]);
+  eU = t1 * entR - t2 * entL - t3 * (entR * VR[1] - entL * VL[1]);
+  pe = eU * pow(Us[0], gam - 1.0);
+  pg = (gam - 1.0) * (Us[4] - 0.5 * (Us[1] * Us[1] + Us[2] * Us[2] + Us[3] * Us[3]) / Us[0] -
+                      ob8pi * (Us[5] * Us[5] + Us[6] * Us[6] + Us[7] * Us[7]));
+  if (pg <= prsf) Us[4] = Us[4] + (pe - pg) / (gam - 1.0);
+  if (*SL >= 0.0) {
+    for (c = 0; c < 8; ++c) {
+      Us[c] = UL[c];
+      Fs[c] = FL[c];
+    }
+    *entF = entL * VL[1];
+    for (c = 0; c < 8; ++c) {
+      FlR[c] = *SL * (Us[c] - UL[c]) + *SR * (UR[c] - Us[c]);
+      FlL[c] = 0.0;
+    }
+  } else if (*SR <= 0.0) {
+    for (c = 0; c < 8; ++c) {
+      Us[c] = UR[c];
+      Fs[c] = FR[c];
+    }
+    *entF = entR * VR[1];
+    for (c = 0; c < 8; ++c) {
+      FlL[c] = *SL * (Us[c] - UL[c]) + *SR * (UR[c] - Us[c]);
+      FlR[c] = 0.0;
+    }
+  } else {
+    double o = 1.0 / (*SR - *SL), s2 = *SR * *SL * o;
+    for (c = 0; c < 8; ++c) {
+      Fs[c] = 0.5 * (FL[c] + FR[c]) + 0.5 * *SL * (Us[c] - UL[c]) - 0.5 * *SR * (UR[c] - Us[c]);
+      FlL[c] = *SL * (Us[c] - UL[c]);
+      FlR[c] = *SR * (UR[c] - Us[c]);
+    }
+    *entF = (*SR * entL * VL[1] - *SL * entR * VR[1]) * o + (entR - entL) * s2;
+  }
+}
+
+/* vector components 1-3 and 5-7 from the global frame to (A, B, C) */
+static void to_frame(const double* A, const double* B, const double* C, double* v) {
+  for (int o = 1; o <= 5; o += 4) {
+    double a = A[0] * v[o] + A[1] * v[o + 1] + A[2] * v[o + 2];
+    double b = B[0] * v[o] + B[1] * v[o + 1] + B[2] * v[o + 2];
+    double c = C[0] * v[o] + C[1] * v[o + 1] + C[2] * v[o + 2];
+    v[o] = a;
+    v[o + 1] = b;
+    v[o + 2] = c;
+  }
+}
+/* and back: out = f with the vector parts rotated into the global frame */
+static void from_frame(const double* A, const double* B, const double* C, const double* f, double* out) {
+  for (int c = 0; c < 8; ++c) out[c] = f[c];
+  for (int o = 1; o <= 5; o += 4)
+    for (int q = 0; q < 3; ++q) out[o + q] = A[q] * f[o] + B[q] * f[o + 1] + C[q] * f[o + 2];
+}
+
+/* MAKE_FACIAL_FLUXES (400riemann_integrate/450make_facial_fluxes.f:5) at
+ * second order (one Riemann problem per face, at the face centroid, weight 1),
+ * MHD, no facial electric fields, no B0. */
+int or_make_facial_fluxes_p1(const or_p1_flux* f) {
+  if (!f || f->nghost < 4 || f->ntp < 1 || f->nr < 1) return 1;
+  or_p1_patch pp;
+  memset(&pp, 0, sizeof pp);
+  pp.ntp = f->ntp;
+  pp.nr = f->nr;
+  pp.nghost = f->nghost;
+  pp.mask = f->mask;
+  pp.stencil = f->stencil;
+  pp.scale = (double*)f->scale;
+  pp.v_centroid = f->v_centroid;
+  pp.V = (double*)f->V;
+  const p1dims D = p1_dims(&pp), *d = &D;
+  const int N = d->N, NR = d->NR, g = d->g;
+  const double gam = f->gamma, sm = f->smallnum;
+  int i, j, k, c, q;
+  /* initial values (450...f:370-480) */
+  for (long z = 0; z < d->NZ * 8; ++z) f->dU_dt[z] = 0.0;
+  for (long z = 0; z < d->NFC * 8; ++z) {
+    f->Fluct_x_Pl[z] = f->Fluct_y_Pl[z] = f->Fluct_z_Pl[z] = 0.0;
+    f->Fluct_x_Mi[z] = f->Fluct_y_Mi[z] = f->Fluct_z_Mi[z] = 0.0;
+  }
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        long fb = fci(d, i, j, k);
+        f->Efx[fb] = f->Efy[fb] = f->Efz[fb] = 0.0;
+      }
+  for (int fam = 0; fam < 3; ++fam) {
+    const int ilo = fam == 0 ? -1 : 0, jlo = fam == 1 ? -1 : 0, klo = fam == 2 ? -1 : 0;
+    double* SLg = fam == 0 ? f->S_L : (fam == 1 ? f->S_D : f->S_B);
+    double* SRg = fam == 0 ? f->S_R : (fam == 1 ? f->S_U : f->S_T);
+    double* FPl = fam == 0 ? f->Fluct_x_Pl : (fam == 1 ? f->Fluct_y_Pl : f->Fluct_z_Pl);
+    double* FMi = fam == 0 ? f->Fluct_x_Mi : (fam == 1 ? f->Fluct_y_Mi : f->Fluct_z_Mi);
+    double* EF = fam == 0 ? f->EntFlux_x : (fam == 1 ? f->EntFlux_y : f->EntFlux_z);
+    const double* pts = fam == 0 ? f->xf_pts : (fam == 1 ? f->yf_pts : f->zf_pts);
+    const double* nrm = fam == 0 ? f->xf_normal : (fam == 1 ? f->yf_normal : f->zf_normal);
+    const double* area = fam == 0 ? f->xf_area : (fam == 1 ? f->yf_area : f->zf_area);
+    const double* fcen = fam == 0 ? f->xf_centroid : (fam == 1 ? f->yf_centroid : f->zf_centroid);
+    const long NF = fam == 2 ? d->NZF : d->NXF;
+    for (k = klo; k <= NR + 1; ++k)
+      for (j = jlo; j <= N + 1; ++j)
+        for (i = ilo; i <= N + 1; ++i) {
+          long fb = fci(d, i, j, k), fg = fam == 0 ? xfi(d, i, j, k) : (fam == 1 ? yfi(d, i, j, k) : zfi(d, i, j, k));
+          int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1), rk = k + (fam == 2);
+          if (fam < 2) {
+            int al = active(&pp, d, li, lj, k), ar = active(&pp, d, ri, rj, k);
+            if (al && ar) {
+            } else if (!al && ar) {
+              nbr(&pp, d, ri, rj, fam == 0 ? 1 : 3, &li, &lj);
+            } else if (al && !ar) {
+              nbr(&pp, d, li, lj, fam == 0 ? 0 : 2, &ri, &rj);
+            } else {
+              continue;
+            }
+          } else if (!active(&pp, d, i, j, k)) {
+            continue;
+          }
+          long zl = zi(d, li, lj, k), zr = zi(d, ri, rj, fam == 2 ? rk : k);
+          /* the face frame (A normal, B, C) */
+          double A[3], B[3], C[3], t;
+          if (fam == 0) {
+            for (c = 0; c < 3; ++c) {
+              A[c] = nrm[fg + NF * c];
+              C[c] = fcen[fg + NF * c];
+            }
+            t = 1.0 / omax(sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]), sm);
+            for (c = 0; c < 3; ++c) C[c] = C[c] * t;
+            B[0] = C[1] * A[2] - C[2] * A[1];
+            B[1] = C[2] * A[0] - C[0] * A[2];
+            B[2] = C[0] * A[1] - C[1] * A[0];
+          } else if (fam == 1) {
+            for (c = 0; c < 3; ++c) {
+              A[c] = nrm[fg + NF * c];
+              B[c] = fcen[fg + NF * c];
+            }
+            t = 1.0 / omax(sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]), sm);
+            for (c = 0; c < 3; ++c) B[c] = B[c] * t;
+            C[0] = A[1] * B[2] - A[2] * B[1];
+            C[1] = A[2] * B[0] - A[0] * B[2];
+            C[2] = A[0] * B[1] - A[1] * B[0];
+          } else {
+            for (c = 0; c < 3; ++c) A[c] = nrm[fg + NF * c]; /* the first (central) point's normal */
+            t = 1.0 / omax(sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]), sm);
+            for (c = 0; c < 3; ++c) A[c] = A[c] * t;
+            for (c = 0; c < 3; ++c)
+              B[c] = f->vert[fci(d, i, j, k) + d->NV * c] + f->vert[fci(d, i, j - 1, k) + d->NV * c] -
+                     f->vert[fci(d, i - 1, j, k) + d->NV * c] - f->vert[fci(d, i - 1, j - 1, k) + d->NV * c];
+            t = B[0] * A[0] + B[1] * A[1] + B[2] * A[2];
+            for (c = 0; c < 3; ++c) B[c] = B[c] - t * A[c];
+            t = 1.0 / omax(sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]), sm);
+            for (c = 0; c < 3; ++c) B[c] = B[c] * t;
+            C[0] = A[1] * B[2] - A[2] * B[1];
+            C[1] = A[2] * B[0] - A[0] * B[2];
+            C[2] = A[0] * B[1] - A[1] * B[0];
+            t = 1.0 / omax(sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]), sm);
+            for (c = 0; c < 3; ++c) C[c] = C[c] * t;
+            B[0] = C[1] * A[2] - C[2] * A[1];
+            B[1] = C[2] * A[0] - C[0] * A[2];
+            B[2] = C[0] * A[1] - C[1] * A[0];
+          }
+          /* the two zones' linear fields at the face centroid (point 1) */
+          double rl[3], rr[3], VL[8], VR[8], eL, eR;
+          rel(&pp, d, zl, pts + fg, NF, obl_of(&pp, zl), rl);
+          rel(&pp, d, zr, pts + fg, NF, obl_of(&pp, zr), rr);
+          for (c = 0; c < 8; ++c) {
+            VL[c] = VV(&pp, d, c, 0, zl) + VV(&pp, d, c, 1, zl) * rl[0] + VV(&pp, d, c, 2, zl) * rl[1] +
+                    VV(&pp, d, c, 3, zl) * rl[2];
+            VR[c] = VV(&pp, d, c, 0, zr) + VV(&pp, d, c, 1, zr) * rr[0] + VV(&pp, d, c, 2, zr) * rr[1] +
+                    VV(&pp, d, c, 3, zr) * rr[2];
+          }
+          eL = f->ent[zl] + f->ent[zl + d->NZ] * rl[0] + f->ent[zl + 2 * d->NZ] * rl[1] + f->ent[zl + 3 * d->NZ] * rl[2];
+          eR = f->ent[zr] + f->ent[zr + d->NZ] * rr[0] + f->ent[zr + 2 * d->NZ] * rr[1] + f->ent[zr + 3 * d->NZ] * rr[2];
+          to_frame(A, B, C, VL);
+          to_frame(A, B, C, VR);
+          double SL, SR, Us[8], Fs[8], eF, FlL[8], FlR[8], UL[8], UR[8], FL[8], FR[8];
+          or_rs_ptwise(gam, f->rhofloor, f->prsfloor, VL, VR, eL, eR, &SL, &SR, Us, Fs, &eF, FlL, FlR, UL, UR, FL, FR);
+          SLg[fb] = SL;
+          SRg[fb] = SR;
+          double Pl[8], Mi[8], FLL[8], FRR[8];
+          from_frame(A, B, C, FlR, Pl);
+          from_frame(A, B, C, FlL, Mi);
+          from_frame(A, B, C, FL, FLL);
+          from_frame(A, B, C, FR, FRR);
+          for (c = 0; c < 8; ++c) {
+            FPl[fb + d->NFC * c] = FPl[fb + d->NFC * c] + 1.0 * Pl[c];
+            FMi[fb + d->NFC * c] = FMi[fb + d->NFC * c] + 1.0 * Mi[c];
+          }
+          const double ar_ = area[fg];
+          for (c = 0; c < 8; ++c) {
+            f->dU_dt[zl + d->NZ * c] = f->dU_dt[zl + d->NZ * c] - 1.0 * (Mi[c] + FLL[c]) * ar_ / f->v_volume[zl];
+            f->dU_dt[zr + d->NZ * c] = f->dU_dt[zr + d->NZ * c] - 1.0 * (Pl[c] - FRR[c]) * ar_ / f->v_volume[zr];
+          }
+          EF[fb] = eF;
+          (void)q;
+        }
+  }
+  return 0;
+}

@@ -132,3 +132,90 @@ def test_p1_linear_field_weno_exact():
     inner = (A["mask"] == 1) & (i >= 1) & (i <= N) & (j >= 1) & (j <= N) & (k >= 1) & (k <= NR)
     want = grad[:, None] * A["scale"][None, inner]
     np.testing.assert_allclose(V[1:, 0][:, inner], want, rtol=1e-9, atol=1e-13)
+
+
+# ---------------------------------------------------------------------------
+# MAKE_FACIAL_FLUXES (450make_facial_fluxes.f:5) and OneD_RS_MHD_Ptwise
+# (610_1d_rs_oned_rs_mhd_ptwise.f).  The reference's routine cannot be built
+# here (it calls CONS_PRIM_TO_FLUX_3D, absent from the reference tree), so the
+# restatement is fixed by the identities of the HLL solver and by conservation.
+
+def _rs(VL, VR, eL=1.0, eR=1.2, gam=5.0 / 3.0):
+    L = orc.olib()
+    L.or_rs_ptwise.argtypes = [ctypes.c_double] * 3 + [ctypes.c_void_p] * 2 + [ctypes.c_double] * 2 + \
+        [ctypes.c_void_p] * 11
+    SL, SR, eF = (np.zeros(1) for _ in range(3))
+    out = {k: np.zeros(8) for k in ("Us", "Fs", "FlL", "FlR", "UL", "UR", "FL", "FR")}
+    VL, VR = VL.copy(), VR.copy()
+    L.or_rs_ptwise(gam, 1e-12, 1e-9, orc.P(VL), orc.P(VR), eL, eR, orc.P(SL), orc.P(SR), orc.P(out["Us"]),
+                   orc.P(out["Fs"]), orc.P(eF), orc.P(out["FlL"]), orc.P(out["FlR"]), orc.P(out["UL"]),
+                   orc.P(out["UR"]), orc.P(out["FL"]), orc.P(out["FR"]))
+    return SL[0], SR[0], eF[0], out
+
+
+def test_rs_ptwise_identities():
+    rng = np.random.default_rng(11)
+    nsub = 0
+    for _ in range(200):
+        VL = np.array([rng.uniform(0.5, 2), *rng.normal(0, 0.5, 3), rng.uniform(0.5, 2), *rng.normal(0, 1, 3)])
+        VR = np.array([rng.uniform(0.5, 2), *rng.normal(0, 0.5, 3), rng.uniform(0.5, 2), *rng.normal(0, 1, 3)])
+        SL, SR, eF, o = _rs(VL, VR)
+        assert SL < SR
+        scale = np.abs(o["FR"]).max() + np.abs(o["FL"]).max() + 1.0
+        if SL < 0.0 < SR:
+            nsub += 1
+            # subsonic: the two fluctuations add up to the flux jump, and the
+            # resolved flux is either side's flux plus its fluctuation
+            np.testing.assert_allclose(o["FlL"] + o["FlR"], o["FR"] - o["FL"], atol=1e-12 * scale)
+            np.testing.assert_allclose(o["Fs"], o["FL"] + o["FlL"], atol=1e-12 * scale)
+            np.testing.assert_allclose(o["Fs"], o["FR"] - o["FlR"], atol=1e-12 * scale)
+            assert o["FL"][5] == 0.0 and o["FR"][5] == 0.0  # no normal-B flux
+    assert nsub > 150
+    # consistency: equal states give the physical flux and no fluctuation
+    V = np.array([1.3, 0.2, -0.1, 0.3, 0.8, 0.4, -0.2, 0.1])
+    SL, SR, eF, o = _rs(V, V, 0.9, 0.9)
+    np.testing.assert_allclose(o["Fs"], o["FL"], rtol=1e-13, atol=1e-15)
+    assert np.abs(o["FlL"]).max() < 1e-14 and np.abs(o["FlR"]).max() < 1e-14
+    np.testing.assert_allclose(eF, 0.9 * V[1], rtol=1e-13)
+    # supersonic to the right: the left flux, no left-going fluctuation, and
+    # the right-going one formed after U* is set to U_L (610...f:653-658)
+    V2 = V.copy()
+    V2[1] = 9.0
+    SL, SR, eF, o = _rs(V2, V2 * [1, 1, 1, 1, 1.1, 1, 1, 1])
+    assert SL >= 0.0
+    np.testing.assert_array_equal(o["Fs"], o["FL"])
+    np.testing.assert_array_equal(o["FlL"], np.zeros(8))
+    np.testing.assert_allclose(o["FlR"], SR * (o["UR"] - o["UL"]), rtol=1e-14, atol=1e-14)
+
+
+def test_facial_fluxes_conserve_in_subsonic_flow():
+    """Every face gives its left zone -(Fluct_Mi + F_LL) A and its right zone
+    -(Fluct_Pl - F_RR) A; when every face is subsonic these cancel, so the
+    volume-weighted sum of dU_dt over the patch vanishes component by
+    component."""
+    prob, A = orc.p1_flux_problem(10, 4, 4, sector=2, seed=5, shock=False, noise=0.0)
+    orc.p1f_oracle(prob, A)
+    nz = orc.p1_dims(10, 4, 4)["zone"]
+    # the state is slow: every processed face is subsonic
+    tot = (A["dU_dt"].reshape(8, nz) * A["v_volume"][None]).sum(1)
+    mag = (np.abs(A["dU_dt"].reshape(8, nz)) * A["v_volume"][None]).sum(1)
+    assert np.all(np.abs(tot) <= 1e-12 * mag), (tot, mag)
+
+
+def test_facial_fluxes_fill_the_reference_ranges():
+    """dU_dt is written for every zone (zero where no face reaches), the face
+    outputs for the faces of the reference's ranges only."""
+    prob, A0 = orc.p1_flux_problem(8, 3, 4, seed=1)
+    A = orc.p1f_oracle(prob, orc.p1_copy(A0))
+    g, N, NR = 4, 8, 3
+    nf = N + 2 * g + 1
+    S_L = A["S_L"].reshape(NR + 2 * g + 1, nf, nf)
+    S0 = A0["S_L"].reshape(NR + 2 * g + 1, nf, nf)
+    # x-faces (i in [-1, N+1], j in [0, N+1], k in [0, NR+1]) that are not
+    # between two tri-corner zones are written; the others keep their input
+    written = S_L != S0
+    ks, js, is_ = np.nonzero(written)
+    assert is_.min() - g == -1 and is_.max() - g == N + 1
+    assert js.min() - g == 0 and js.max() - g == N + 1
+    assert ks.min() - g == 0 and ks.max() - g == NR + 1
+    assert np.all(np.isfinite(A["dU_dt"]))

@@ -63,7 +63,7 @@ def test_p1_gpu_matches_oracle(case):
     prob, A0 = orc.p1_problem(N, NR, 4, sector=s, seed=seed, shock=shock, noise=noise)
     O = orc.p1_oracle(prob, orc.p1_copy(A0))
     G = _run_gpu(prob, orc.p1_copy(A0))
-    bad = _compare(G, O)
+    bad = _compare(G, {k: O[k] for k in G})
     assert not bad, bad
     # discrete decisions agree exactly
     nz = orc.p1_dims(N, NR, 4)["zone"]
@@ -83,3 +83,31 @@ def test_p1_gpu_rejects_bad_sizes():
     T = {k: torch.from_numpy(A[k]).to(dev) for k in cs.P1_ARRAYS}
     with pytest.raises(cs.CsError):
         cs.limit_all_slopes_p1(8, 3, 3, prob, T)  # nghost < 4: sizes and the entry point refuse
+
+
+FLUX_CASES = [
+    (8, 3, 0, 0, True, 0.0),
+    (12, 5, 1, 2, True, 1.0),
+    (10, 4, 5, 3, False, 0.5),
+    (24, 8, 2, 4, True, 0.3),
+]
+
+
+@pytest.mark.parametrize("case", FLUX_CASES, ids=lambda c: "N%d_nr%d_s%d_seed%d_%s_noise%g" % (
+    c[0], c[1], c[2], c[3], "front" if c[4] else "smooth", c[5]))
+def test_facial_fluxes_gpu_matches_oracle(case):
+    """cs_make_facial_fluxes_p1 against or_make_facial_fluxes_p1 on a patch the
+    oracle's LIMIT_ALL_SLOPES_P1 reconstructed: every output to 1e-11 of its
+    largest magnitude (the wave speeds use a reciprocal, the rest the same
+    formulas), the tri-corner re-pointed shares included."""
+    N, NR, s, seed, shock, noise = case
+    print(f"\nflux case N={N} NR={NR} sector={s} seed={seed} noise={noise}", flush=True)
+    prob, A0 = orc.p1_flux_problem(N, NR, 4, sector=s, seed=seed, shock=shock, noise=noise)
+    O = orc.p1f_oracle(prob, orc.p1_copy(A0))
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(A0[k]).to(dev) for k in cs.P1F_IN + cs.P1F_OUT}
+    cs.make_facial_fluxes_p1(N, NR, 4, prob, T)
+    torch.cuda.synchronize()
+    G = {k: T[k].cpu().numpy() for k in cs.P1F_OUT}
+    bad = _compare(G, {k: O[k] for k in cs.P1F_OUT}, rtol=1e-11)
+    assert not bad, bad
