@@ -322,17 +322,18 @@ def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None
     cfg = cs.make_config(N, NR, g, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
                          cs.RMESH_UNIFORM, 1.0, 1.6)
     if geo is None:
-        geo, _ = cs.build_geometry(cfg, skip=("v_pts", "v_wts", "v_tan", "v_vol", "moments"))
+        geo, _ = cs.build_geometry(cfg, skip=("v_pts", "v_wts", "v_tan", "moments"))
     z = {k: v // 6 for k, v in cs.geometry_sizes(cfg).items()}  # one sector
     dev = torch.device(device)
     sec = lambda a, n, m: torch.from_numpy(a[: n * m]).to(dev)
     T = dict(vert=sec(geo["vert"], z["vert"], 3), mask=sec(geo["mask"], z["zone"], 1),
              stencil=torch.from_numpy(cs.stencil_indexing(cfg)).to(dev), scale=sec(geo["scale"], z["zone"], 1),
              v_centroid=sec(geo["v_cen"], z["zone"], 3))
+    T["v_volume"] = sec(geo["v_vol"], z["zone"], 1)
     for f in ("xf", "yf", "zf"):
         n = z[f]
-        if f != "zf":
-            T[f + "_normal"] = sec(geo[f]["normal"], n, 3)
+        T[f + "_normal"] = sec(geo[f]["normal"], n, 15 if f == "zf" else 3)
+        T[f + "_pts"] = sec(geo[f]["pts"], n, 15)
         T[f + "_area"] = sec(geo[f]["area"], n, 1)
         T[f + "_centroid"] = sec(geo[f]["centroid"], n, 3)
         b = f[0] + "b"
@@ -379,9 +380,10 @@ def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None
     ar = torch.arange(nfc, device=dev, dtype=torch.float64)
     for q, nm in enumerate(("Bxf", "Byf", "Bzf")):
         T[nm] = 0.4 * torch.sin(ar * (0.37 + 0.11 * q)) + 0.1 * q
-    for k in cs.P1_ARRAYS:
+    cntf = cs.p1f_counts(N, NR, g)
+    for k in cs.P1_ARRAYS + cs.P1F_OUT:
         if k not in T:
-            T[k] = torch.zeros(cnt[k], dtype=torch.float64, device=dev)
+            T[k] = torch.zeros(cntf[k], dtype=torch.float64, device=dev)
     scal = dict(gamma=gam, molewt=1.0, rhofloor=1e-12, prsfloor=1e-9, smallnum=1e-12, flatten_coef=0.4,
                 prs_flatten_coef=1.5)
     if dev.type == "cuda":
@@ -422,9 +424,11 @@ def bench_p1(cs, args):
     for _ in range(max(1, args.warmup)):
         reset()
         cs.limit_all_slopes_p1(N, NR, g, scal, T)
+        cs.make_facial_fluxes_p1(N, NR, g, scal, T)
     torch.cuda.synchronize()
     steps = max(1, min(args.steps, 50))
     tot = 0.0
+    totf = 0.0
     stage = None
     for _ in range(steps):
         reset()
@@ -432,9 +436,14 @@ def bench_p1(cs, args):
         t0 = time.perf_counter()
         vmin, vmax, ms = cs.limit_all_slopes_p1(N, NR, g, scal, T, timed=True)
         torch.cuda.synchronize()
-        tot += time.perf_counter() - t0
+        t1 = time.perf_counter()
+        cs.make_facial_fluxes_p1(N, NR, g, scal, T)
+        torch.cuda.synchronize()
+        tot += t1 - t0
+        totf += time.perf_counter() - t1
         stage = ms if stage is None else [a + b for a, b in zip(stage, ms)]
     ms_call = tot / steps * 1e3
+    ms_flux = totf / steps * 1e3
     stage = [s / steps for s in stage]
     own = N * N * NR
     cnt = cs.p1_counts(N, NR, g)
@@ -449,6 +458,7 @@ def bench_p1(cs, args):
             k["frac"] = k["GBps"] / HBM_PEAK_GBS
         kern[name] = k
         log(f"   {name:20s} {t:8.3f} ms" + (f"  {k['GBps']:7.1f} GB/s algorithmic" if "GBps" in k else ""))
+    log(f"   MAKE_FACIAL_FLUXES   {ms_flux:8.3f} ms  ({own / (ms_flux * 1e-3):.3e} zones/s)")
     dom = max(kern, key=lambda n: kern[n]["ms"])
     cpu = None
     if not args.no_cpu:
@@ -463,6 +473,7 @@ def bench_p1(cs, args):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom].get("GBps"), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kern[dom].get("frac"), "traffic": None},
         "cpu_baseline": cpu, "kernels": kern,
+        "facial_fluxes": {"ms": ms_flux, "zones_per_s": own / (ms_flux * 1e-3)},
     }
     print(json.dumps(line))
 

@@ -1376,16 +1376,30 @@ __device__ __forceinline__ void rs_ptwise(const Gas<double>& gs, double* VL, dou
 // one face; the shares cl (to the left zone) and cr (to the right) are kept
 // in share[fam][c][face] (c < 8 left, c >= 8 right); faces whose side zone was
 // re-pointed through the stencil are appended to the remap list
-__global__ void __launch_bounds__(128) k_ff_face(const P1F P, double* share, int64_t* rlist,
+__global__ void __launch_bounds__(128) k_ff_face(const P1F P, double* share, int64_t* rlist, int64_t cap,
                                                  unsigned int* rcount) {
   const int fam = blockIdx.y;
   const int64_t t = tid();
   if (t >= P.NFC) return;
   const int n = P.nf;
   const int i = (int)(t % n) - P.g, j = (int)((t / n) % n) - P.g, k = (int)(t / ((int64_t)n * n)) - P.g;
-  if (!P.in_range(fam, i, j, k)) return;
   const cs_p1_flux& a = P.a;
   const int64_t fb = t;
+  double* FPl = fam == 0 ? a.Fluct_x_Pl : (fam == 1 ? a.Fluct_y_Pl : a.Fluct_z_Pl);
+  double* FMi = fam == 0 ? a.Fluct_x_Mi : (fam == 1 ? a.Fluct_y_Mi : a.Fluct_z_Mi);
+  // faces outside the range or between two tri-corner zones keep zero
+  // fluctuations (450...f:370-380)
+  auto zero = [&]() {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      FPl[fb + P.NFC * c] = 0.0;
+      FMi[fb + P.NFC * c] = 0.0;
+    }
+  };
+  if (!P.in_range(fam, i, j, k)) {
+    zero();
+    return;
+  }
   int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1);
   const int rk = k + (fam == 2);
   int remapped = 0;  // 1: left side re-pointed, 2: right side
@@ -1398,9 +1412,11 @@ __global__ void __launch_bounds__(128) k_ff_face(const P1F P, double* share, int
       P.nbr(li, lj, fam == 0 ? 0 : 2, ri, rj);
       remapped = 2;
     } else if (!al && !ar) {
+      zero();
       return;
     }
   } else if (!P.active(i, j, k)) {
+    zero();
     return;
   }
   const int64_t zl = P.zi(li, lj, k), zr = P.zi(ri, rj, fam == 2 ? rk : k);
@@ -1502,8 +1518,6 @@ __global__ void __launch_bounds__(128) k_ff_face(const P1F P, double* share, int
   (fam == 0 ? a.S_L : fam == 1 ? a.S_D : a.S_B)[fb] = SL;
   (fam == 0 ? a.S_R : fam == 1 ? a.S_U : a.S_T)[fb] = SR;
   (fam == 0 ? a.EntFlux_x : fam == 1 ? a.EntFlux_y : a.EntFlux_z)[fb] = eF;
-  double* FPl = fam == 0 ? a.Fluct_x_Pl : (fam == 1 ? a.Fluct_y_Pl : a.Fluct_z_Pl);
-  double* FMi = fam == 0 ? a.Fluct_x_Mi : (fam == 1 ? a.Fluct_y_Mi : a.Fluct_z_Mi);
   const double ar = (fam == 0 ? a.xf_area : fam == 1 ? a.yf_area : a.zf_area)[fg];
   const double vl = a.v_volume[zl], vr = a.v_volume[zr];
   double* sh = share + (int64_t)fam * 16 * P.NFC;
@@ -1523,9 +1537,9 @@ __global__ void __launch_bounds__(128) k_ff_face(const P1F P, double* share, int
   }
   if (remapped) {
     const unsigned int e = atomicAdd(rcount, 1u);
-    // target zone, then (family, face, side) packed
-    rlist[2 * (int64_t)e] = remapped == 1 ? zl : zr;
-    rlist[2 * (int64_t)e + 1] = (((int64_t)fam * P.NFC + fb) << 1) | (remapped == 1 ? 0 : 1);
+    // target zone; (family, face, side) packed in the second half
+    rlist[e] = remapped == 1 ? zl : zr;
+    rlist[cap + e] = (((int64_t)fam * P.NFC + fb) << 1) | (remapped == 1 ? 0 : 1);
   }
 }
 
@@ -1536,12 +1550,14 @@ __global__ void __launch_bounds__(256) k_ff_gather(const P1F P, const double* sh
   if (z >= P.NZ) return;
   const int i = (int)(z % P.nz) + 1 - P.g, j = (int)((z / P.nz) % P.nz) + 1 - P.g,
             k = (int)(z / ((int64_t)P.nz * P.nz)) + 1 - P.g;
-  if (!P.active(i, j, k)) return;
   double d[8];
 #pragma unroll
   for (int c = 0; c < 8; ++c) d[c] = 0.0;
+  // a tri-corner zone takes no share of any face
+  const bool act = P.active(i, j, k);
 #pragma unroll
   for (int fam = 0; fam < 3; ++fam) {
+    if (!act) break;
     const double* sh = share + (int64_t)fam * 16 * P.NFC;
     const int di = fam == 0, dj = fam == 1, dk = fam == 2;
     if (P.in_range(fam, i - di, j - dj, k - dk)) {
@@ -1559,15 +1575,15 @@ __global__ void __launch_bounds__(256) k_ff_gather(const P1F P, const double* sh
   for (int c = 0; c < 8; ++c) P.a.dU_dt[z + P.NZ * c] = d[c];
 }
 
-// zones that also take re-pointed shares: rebuilt in loop order (one thread
-// per list entry; the first entry of each zone does the work)
-__global__ void __launch_bounds__(64) k_ff_remap(const P1F P, const double* share, const int64_t* rlist,
-                                                 unsigned int n) {
-  const int64_t e = tid();
-  if (e >= n) return;
-  const int64_t T = rlist[2 * e];
-  for (unsigned int q = 0; q < (unsigned int)e; ++q)
-    if (rlist[2 * (int64_t)q] == T) return;
+// zones that also take re-pointed shares: dU_dt rebuilt in loop order, one
+// thread per such zone.  grp[2 q], grp[2 q + 1]: the zone and the first of
+// its entries in codes (the host groups the list by zone); codes packs
+// (family, face, side).
+__global__ void __launch_bounds__(64) k_ff_remap(const P1F P, const double* share, const int64_t* grp,
+                                                 const int64_t* codes, int ngrp, int ncodes) {
+  const int q0 = (int)tid();
+  if (q0 >= ngrp) return;
+  const int64_t T = grp[2 * q0], e0 = grp[2 * q0 + 1], e1 = q0 + 1 < ngrp ? grp[2 * q0 + 3] : ncodes;
   const int i = (int)(T % P.nz) + 1 - P.g, j = (int)((T / P.nz) % P.nz) + 1 - P.g,
             k = (int)(T / ((int64_t)P.nz * P.nz)) + 1 - P.g;
   double d[8];
@@ -1587,9 +1603,8 @@ __global__ void __launch_bounds__(64) k_ff_remap(const P1F P, const double* shar
         sd[m++] = 0;
       }
     }
-    for (unsigned int q = 0; q < n && m < 8; ++q) {
-      if (rlist[2 * (int64_t)q] != T) continue;
-      const int64_t code = rlist[2 * (int64_t)q + 1], ff = code >> 1;
+    for (int64_t q = e0; q < e1 && m < 8; ++q) {
+      const int64_t code = codes[q], ff = code >> 1;
       if (ff / P.NFC != fam) continue;
       fc[m] = ff % P.NFC;
       sd[m++] = (int)(code & 1);
@@ -1693,24 +1708,47 @@ extern "C" int cs_make_facial_fluxes_p1(const cs_p1_flux* p, void* stream_) {
   double* share = (double*)S.mem;
   unsigned int* rcount = (unsigned int*)((uint8_t*)S.mem + share_bytes);
   int64_t* rlist = (int64_t*)((uint8_t*)S.mem + share_bytes + 256);
-  // every fluctuation and the whole update start at zero (450...f:360-380);
-  // the electric fields of the zone range too (:400-420)
-  P1_HIP(hipMemsetAsync(p->dU_dt, 0, sizeof(double) * 8 * P.NZ, st));
-  double* fl[6] = {p->Fluct_x_Pl, p->Fluct_y_Pl, p->Fluct_z_Pl, p->Fluct_x_Mi, p->Fluct_y_Mi, p->Fluct_z_Mi};
-  for (double* q : fl) P1_HIP(hipMemsetAsync(q, 0, sizeof(double) * 8 * P.NFC, st));
+  // the face kernel writes every fluctuation (zero where no Riemann problem
+  // is solved) and the gather every zone's update; the electric fields of
+  // the zone range are zeroed (450...f:400-420)
   P1_HIP(hipMemsetAsync(rcount, 0, sizeof(unsigned int), st));
   k_ff_zero_e<<<grid_of(P.NZ, 256), 256, 0, st>>>(P);
-  k_ff_face<<<dim3(grid_of(P.NFC, 128), 3), 128, 0, st>>>(P, share, rlist, rcount);
+  k_ff_face<<<dim3(grid_of(P.NFC, 128), 3), 128, 0, st>>>(P, share, rlist, (int64_t)list_cap, rcount);
   k_ff_gather<<<grid_of(P.NZ, 256), 256, 0, st>>>(P, share);
   P1_HIP(hipGetLastError());
   unsigned int nrm = 0;
   P1_HIP(hipMemcpyAsync(&nrm, rcount, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
   P1_HIP(hipStreamSynchronize(st));
-  if (nrm > list_cap) {
-    cs::set_error("cs_make_facial_fluxes_p1: %u re-pointed faces exceed the list", nrm);
-    return CS_E_INVALID;
+  if (nrm > 0) {
+    // group the entries by zone on the host (a few per radial layer), then
+    // one thread per zone
+    std::vector<int64_t> tz(nrm), code(nrm), order(nrm);
+    P1_HIP(hipMemcpy(tz.data(), rlist, sizeof(int64_t) * nrm, hipMemcpyDeviceToHost));
+    P1_HIP(hipMemcpy(code.data(), rlist + list_cap, sizeof(int64_t) * nrm, hipMemcpyDeviceToHost));
+    for (unsigned int q = 0; q < nrm; ++q) order[q] = q;
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
+      return tz[x] != tz[y] ? tz[x] < tz[y] : code[x] < code[y];
+    });
+    std::vector<int64_t> grp, codes(nrm);
+    for (unsigned int q = 0; q < nrm; ++q) {
+      codes[q] = code[order[q]];
+      if (q == 0 || tz[order[q]] != tz[order[q - 1]]) {
+        grp.push_back(tz[order[q]]);
+        grp.push_back(q);
+      }
+    }
+    const int ngrp = (int)(grp.size() / 2);
+    if (grp.size() > list_cap) {
+      cs::set_error("cs_make_facial_fluxes_p1: %d zones take re-pointed shares", ngrp);
+      return CS_E_INVALID;
+    }
+    int64_t* dgrp = rlist;  // the list has been read: its space holds the groups
+    int64_t* dcodes = rlist + list_cap;
+    P1_HIP(hipMemcpyAsync(dgrp, grp.data(), sizeof(int64_t) * grp.size(), hipMemcpyHostToDevice, st));
+    P1_HIP(hipMemcpyAsync(dcodes, codes.data(), sizeof(int64_t) * nrm, hipMemcpyHostToDevice, st));
+    k_ff_remap<<<grid_of(ngrp, 64), 64, 0, st>>>(P, share, dgrp, dcodes, ngrp, (int)nrm);
+    P1_HIP(hipStreamSynchronize(st));  // the host buffers above are released on return
   }
-  if (nrm > 0) k_ff_remap<<<grid_of(nrm, 64), 64, 0, st>>>(P, share, rlist, nrm);
   P1_HIP(hipGetLastError());
   return 0;
 }
