@@ -6,6 +6,7 @@
 !   cs_halo_exchange       <- LOAD_LOCAL_BUFFERS_* / FETCH_BUFFERS_FILL_IN_* (DRV:1746-1756)
 !   cs_make_facial_fluxes  <- LIMIT_ALL_SLOPES_P1 + MAKE_FACIAL_FLUXES      (DRV:2396, 2485)
 !   cs_limit_all_slopes_p1 <- LIMIT_ALL_SLOPES_P1 alone, argument for argument (DRV:2396)
+!   cs_make_facial_fluxes_p1 <- MAKE_FACIAL_FLUXES alone, argument for argument (DRV:2485)
 !   cs_make_edge_efields   <- MAKE_EDGE_EFIELDS_P1                          (DRV:2548)
 !   cs_update_stage        <- UPDATE_IMEX_RK_STEP_P1 + CLEANUP_AFTER_UPDATE (DRV:2782, 2815)
 !   cs_evaluate_timestep   <- EVALUATE_TIMESTEP                             (DRV:2887)
@@ -43,7 +44,24 @@ module cs_hip_iface
     type(c_ptr) :: Bxf_q, Byf_q, Bzf_q, V_min_local, V_max_local
   end type
 
+  ! cs_p1_flux: the arguments of MAKE_FACIAL_FLUXES (450make_facial_fluxes.f:5-60)
+  type, bind(C) :: cs_p1_flux
+    integer(c_int32_t) :: ntp, nr, nghost, pad
+    real(c_double) :: gamma, rhofloor, prsfloor, smallnum
+    type(c_ptr) :: vert, mask, stencil, xf_pts, xf_normal, xf_area, xf_centroid
+    type(c_ptr) :: yf_pts, yf_normal, yf_area, yf_centroid, zf_pts, zf_normal, zf_area, zf_centroid
+    type(c_ptr) :: scale, v_volume, v_centroid, V, ent, flatten
+    type(c_ptr) :: S_L, S_R, S_D, S_U, S_B, S_T
+    type(c_ptr) :: Fluct_x_Pl, Fluct_y_Pl, Fluct_z_Pl, Fluct_x_Mi, Fluct_y_Mi, Fluct_z_Mi
+    type(c_ptr) :: dU_dt, EntFlux_x, EntFlux_y, EntFlux_z, Efx, Efy, Efz
+  end type
+
   interface
+    integer(c_int) function cs_make_facial_fluxes_p1(p, stream) bind(C, name="cs_make_facial_fluxes_p1")
+      import :: c_int, c_ptr, cs_p1_flux
+      type(cs_p1_flux), intent(in) :: p
+      type(c_ptr), value :: stream
+    end function
     integer(c_int) function cs_limit_all_slopes_p1(p, stream) bind(C, name="cs_limit_all_slopes_p1")
       import :: c_int, c_ptr, cs_p1_patch
       type(cs_p1_patch), intent(in) :: p
