@@ -127,6 +127,7 @@ def lib():
             "cs_limit_all_slopes_p1": [vp, vp],
             "cs_limit_all_slopes_p1_timed": [vp, vp, ctypes.POINTER(ctypes.c_float)],
             "cs_make_facial_fluxes_p1": [vp, vp],
+            "cs_update_imex_rk_step_p1": [vp, vp],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
             "cs_batch_shell": [vp, i32, ctypes.POINTER(vp)],
@@ -614,4 +615,40 @@ def make_facial_fluxes_p1(N, NR, g, scalars, arrays, stream=None):
         setattr(s, k, t.data_ptr())
     st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
     _check(lib().cs_make_facial_fluxes_p1(ctypes.byref(s), st), "cs_make_facial_fluxes_p1")
+
+
+# ---------------------------------------------------------------------------
+# UPDATE_IMEX_RK_STEP_P1 of one patch (cs_update_imex_rk_step_p1)
+
+P1U_SCALARS = ("dtmove", "gamma", "rhofloor", "prsfloor")
+P1U_ARRAYS = ("vert", "xf_area", "yf_area", "zf_area", "v_volume", "U", "ent", "Bxf", "Byf", "Bzf", "dU_dt",
+              "EntFlux_x", "EntFlux_y", "EntFlux_z", "Efx", "Efy", "Efz", "flag_entprs")
+
+
+class P1Update(ctypes.Structure):
+    """cs_p1_update: the arguments of UPDATE_IMEX_RK_STEP_P1 for one patch."""
+    _fields_ = ([("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                 ("pad", ctypes.c_int32)]
+                + [(k, ctypes.c_double) for k in P1U_SCALARS]
+                + [(k, ctypes.c_void_p) for k in P1U_ARRAYS])
+
+
+def update_imex_rk_step_p1(N, NR, g, scalars, arrays, stream=None):
+    """UPDATE_IMEX_RK_STEP_P1 on one patch of contiguous float64 torch tensors
+    on the GPU in the reference's layouts (sizes checked first)."""
+    import torch
+    want = p1f_counts(N, NR, g)
+    want["flag_entprs"] = want["mask"]
+    s = P1Update()
+    s.ntp, s.nr, s.nghost = N, NR, g
+    for k in P1U_SCALARS:
+        setattr(s, k, float(scalars[k]))
+    for k in P1U_ARRAYS:
+        t = arrays[k]
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == want[k]):
+            raise CsError(f"update_imex_rk_step_p1: {k} must be a contiguous float64 CUDA tensor of {want[k]} "
+                          f"elements (got {t.dtype}, {t.numel()}, cuda={t.is_cuda})")
+        setattr(s, k, t.data_ptr())
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    _check(lib().cs_update_imex_rk_step_p1(ctypes.byref(s), st), "cs_update_imex_rk_step_p1")
 

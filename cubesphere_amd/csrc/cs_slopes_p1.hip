@@ -1752,3 +1752,146 @@ extern "C" int cs_make_facial_fluxes_p1(const cs_p1_flux* p, void* stream_) {
   P1_HIP(hipGetLastError());
   return 0;
 }
+
+// ===========================================================================
+// UPDATE_IMEX_RK_STEP_P1 (400riemann_integrate/484update_imex_rk_step_p1.f:7),
+// MHD, behind cs_update_imex_rk_step_p1:
+//   k_upd_zone  every zone: U += dt dU_dt (:483-489); owned zones: entropy from
+//               the face entropy fluxes, CONS_TO_PRIM, its flag (:540-628)
+//   k_upd_face  owned faces of one family (blockIdx.y): constrained transport
+//               with the edges' radial lengths and arc lengths (:630-906)
+namespace csp1 {
+
+struct P1U {
+  cs_p1_update a;
+  int N, NR, g, nz, nf;
+  int64_t NZ, NXF, NZF, NFC;
+  __device__ __forceinline__ int64_t fci(int i, int j, int k) const {
+    return (int64_t)(i + g) + (int64_t)nf * ((j + g) + (int64_t)nf * (k + g));
+  }
+  __device__ __forceinline__ int64_t xfi(int i, int j, int k) const {
+    return (int64_t)(i + g) + (int64_t)nf * ((j - 1 + g) + (int64_t)nz * (k - 1 + g));
+  }
+  __device__ __forceinline__ int64_t yfi(int i, int j, int k) const {
+    return (int64_t)(i - 1 + g) + (int64_t)nz * ((j + g) + (int64_t)nf * (k - 1 + g));
+  }
+  __device__ __forceinline__ int64_t zfi(int i, int j, int k) const {
+    return (int64_t)(i - 1 + g) + (int64_t)nz * ((j - 1 + g) + (int64_t)nz * (k + g));
+  }
+  __device__ __forceinline__ double vx(int i, int j, int k, int c) const { return a.vert[fci(i, j, k) + NFC * c]; }
+  __device__ __forceinline__ double rad(int i, int j, int k) const {
+    const double x = vx(i, j, k, 0), y = vx(i, j, k, 1), z = vx(i, j, k, 2);
+    return sqrt(x * x + y * y + z * z);
+  }
+  // arc between vertices (i1, j1) and (i2, j2) of level k: both directions
+  // scaled by t = 1 / (radius of the face's own vertex), as the reference does
+  __device__ __forceinline__ double angle(int i1, int j1, int i2, int j2, int k, double t) const {
+    const double d = (vx(i1, j1, k, 0) * t) * (vx(i2, j2, k, 0) * t) + (vx(i1, j1, k, 1) * t) * (vx(i2, j2, k, 1) * t) +
+                     (vx(i1, j1, k, 2) * t) * (vx(i2, j2, k, 2) * t);
+    return acos(d);
+  }
+};
+
+__global__ void __launch_bounds__(256) k_upd_zone(const P1U P) {
+  const int64_t z = tid();
+  if (z >= P.NZ) return;
+  const cs_p1_update& a = P.a;
+  const int i = (int)(z % P.nz) + 1 - P.g, j = (int)((z / P.nz) % P.nz) + 1 - P.g,
+            k = (int)(z / ((int64_t)P.nz * P.nz)) + 1 - P.g;
+  const double dt = a.dtmove;
+  double U[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) U[c] = a.U[z + P.NZ * c] + dt * a.dU_dt[z + P.NZ * c];
+  double flag = 0.0;
+  if (i >= 1 && i <= P.N && j >= 1 && j <= P.N && k >= 1 && k <= P.NR) {
+    const double s = a.xf_area[P.xfi(i, j, k)] * a.EntFlux_x[P.fci(i, j, k)] -
+                     a.xf_area[P.xfi(i - 1, j, k)] * a.EntFlux_x[P.fci(i - 1, j, k)] +
+                     a.yf_area[P.yfi(i, j, k)] * a.EntFlux_y[P.fci(i, j, k)] -
+                     a.yf_area[P.yfi(i, j - 1, k)] * a.EntFlux_y[P.fci(i, j - 1, k)] +
+                     a.zf_area[P.zfi(i, j, k)] * a.EntFlux_z[P.fci(i, j, k)] -
+                     a.zf_area[P.zfi(i, j, k - 1)] * a.EntFlux_z[P.fci(i, j, k - 1)];
+    double e = a.ent[z] - dt * s / a.v_volume[z];
+    const Gas<double> gs{a.gamma, a.gamma - 1.0, a.rhofloor, a.prsfloor, 1.0 / (a.gamma - 1.0)};
+    double V[8];
+    flag = (double)cons_to_prim(gs, U, V, e);
+    a.ent[z] = e;
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) a.U[z + P.NZ * c] = U[c];
+  a.flag_entprs[z] = flag;
+}
+
+__global__ void __launch_bounds__(256) k_upd_face(const P1U P) {
+  const int fam = blockIdx.y;
+  const int64_t t = tid();
+  const int N = P.N, NR = P.NR;
+  // owned faces: x (0..N, 1..N, 1..NR), y (1..N, 0..N, 1..NR), z (1..N, 1..N, 0..NR)
+  const int ni = fam == 0 ? N + 1 : N, nj = fam == 1 ? N + 1 : N, nk = fam == 2 ? NR + 1 : NR;
+  if (t >= (int64_t)ni * nj * nk) return;
+  const int i = (int)(t % ni) + (fam == 0 ? 0 : 1), j = (int)((t / ni) % nj) + (fam == 1 ? 0 : 1),
+            k = (int)(t / ((int64_t)ni * nj)) + (fam == 2 ? 0 : 1);
+  const cs_p1_update& a = P.a;
+  const double dt = a.dtmove;
+  const int64_t f = P.fci(i, j, k);
+  const double rout = P.rad(i, j, k), tt = 1.0 / rout;
+  if (fam == 0) {
+    const double rin = P.rad(i, j, k - 1), dr = rout - rin, th = P.angle(i, j, i, j - 1, k, tt);
+    const double lin = rin * th, lout = rout * th;
+    a.Bxf[f] = a.Bxf[f] - dt * (dr * a.Efz[f] - dr * a.Efz[P.fci(i, j - 1, k)] - lout * a.Efy[f] +
+                                lin * a.Efy[P.fci(i, j, k - 1)]) / a.xf_area[P.xfi(i, j, k)];
+  } else if (fam == 1) {
+    const double rin = P.rad(i, j, k - 1), dr = rout - rin, th = P.angle(i, j, i - 1, j, k, tt);
+    const double lin = rin * th, lout = rout * th;
+    a.Byf[f] = a.Byf[f] - dt * (-(dr * a.Efz[f]) + dr * a.Efz[P.fci(i - 1, j, k)] + lout * a.Efx[f] -
+                                lin * a.Efx[P.fci(i, j, k - 1)]) / a.yf_area[P.yfi(i, j, k)];
+  } else {
+    const double lxp = rout * P.angle(i, j, i - 1, j, k, tt), lxm = rout * P.angle(i, j - 1, i - 1, j - 1, k, tt);
+    const double lyp = rout * P.angle(i, j, i, j - 1, k, tt), lym = rout * P.angle(i - 1, j, i - 1, j - 1, k, tt);
+    a.Bzf[f] = a.Bzf[f] - dt * (lyp * a.Efy[f] - lym * a.Efy[P.fci(i - 1, j, k)] - lxp * a.Efx[f] +
+                                lxm * a.Efx[P.fci(i, j - 1, k)]) / a.zf_area[P.zfi(i, j, k)];
+  }
+}
+
+}  // namespace csp1
+
+extern "C" int cs_update_imex_rk_step_p1(const cs_p1_update* p, void* stream_) {
+  if (!p) {
+    cs::set_error("cs_update_imex_rk_step_p1: null argument");
+    return CS_E_INVALID;
+  }
+  if (p->ntp < 1 || p->nr < 1 || p->nghost < 1) {
+    cs::set_error("cs_update_imex_rk_step_p1: need ntp, nr, nghost >= 1 (got %d, %d, %d)", p->ntp, p->nr,
+                  p->nghost);
+    return CS_E_INVALID;
+  }
+  if (!(p->gamma > 1.0) || !(p->rhofloor > 0.0) || !(p->prsfloor > 0.0)) {
+    cs::set_error("cs_update_imex_rk_step_p1: gamma > 1 and positive floors required");
+    return CS_E_INVALID;
+  }
+  const void* ptrs[] = {p->vert, p->xf_area, p->yf_area, p->zf_area, p->v_volume, p->U, p->ent, p->Bxf,
+                        p->Byf, p->Bzf, p->dU_dt, p->EntFlux_x, p->EntFlux_y, p->EntFlux_z, p->Efx, p->Efy,
+                        p->Efz, p->flag_entprs};
+  for (const void* q : ptrs)
+    if (!q) {
+      cs::set_error("cs_update_imex_rk_step_p1: null array");
+      return CS_E_INVALID;
+    }
+  hipStream_t st = (hipStream_t)stream_;
+  P1U P;
+  P.a = *p;
+  P.N = p->ntp;
+  P.NR = p->nr;
+  P.g = p->nghost;
+  P.nz = P.N + 2 * P.g;
+  P.nf = P.nz + 1;
+  const int nzr = P.NR + 2 * P.g;
+  P.NZ = (int64_t)P.nz * P.nz * nzr;
+  P.NXF = (int64_t)P.nf * P.nz * nzr;
+  P.NZF = (int64_t)P.nz * P.nz * (nzr + 1);
+  P.NFC = (int64_t)P.nf * P.nf * (nzr + 1);
+  const int64_t nfaces = (int64_t)(P.N + 1) * (P.N + 1) * (P.NR + 1);
+  k_upd_zone<<<grid_of(P.NZ, 256), 256, 0, st>>>(P);
+  k_upd_face<<<dim3(grid_of(nfaces, 256), 3), 256, 0, st>>>(P);
+  P1_HIP(hipGetLastError());
+  return 0;
+}

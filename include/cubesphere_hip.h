@@ -368,6 +368,28 @@ typedef struct cs_p1_flux {
 } cs_p1_flux;
 int cs_make_facial_fluxes_p1(const cs_p1_flux* p, void* stream);
 
+/* ---- P1 stage update of one patch ------------------------------------------
+ * UPDATE_IMEX_RK_STEP_P1 (400riemann_integrate/484update_imex_rk_step_p1.f:7-26),
+ * MHD: U(:, mode 1) += dtmove dU_dt on every zone; on the owned zones the
+ * entropy from the face entropy fluxes (EntFlux_* of cs_make_facial_fluxes_p1),
+ * then CONS_TO_PRIM (U updated, flag_entprs = 1 where the pressure was taken
+ * from the floor or the entropy); on the owned faces constrained transport of
+ * Bxf, Byf, Bzf with the edge electric fields Efx, Efy, Efz over the edges'
+ * radial lengths and arc lengths r acos(u1 . u2).  Device arrays in the
+ * layouts of cs_p1_patch / cs_p1_flux; U and entropy with their 4 modes
+ * (only mode 1 is touched).  No wait on the stream. */
+typedef struct cs_p1_update {
+  int32_t ntp, nr, nghost, pad;
+  double dtmove, gamma, rhofloor, prsfloor;
+  const double* vert;
+  const double *xf_area, *yf_area, *zf_area, *v_volume;
+  double *U, *ent, *Bxf, *Byf, *Bzf;
+  const double* dU_dt;
+  const double *EntFlux_x, *EntFlux_y, *EntFlux_z, *Efx, *Efy, *Efz;
+  double* flag_entprs;
+} cs_p1_update;
+int cs_update_imex_rk_step_p1(const cs_p1_update* p, void* stream);
+
 int cs_last_error(char* buf, size_t len);
 int cs_device_info(char* buf, size_t len);
 

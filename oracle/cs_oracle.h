@@ -147,4 +147,18 @@ void or_rs_ptwise(double gam, double rhof, double prsf, double* VL, double* VR, 
                   double* SL, double* SR, double* Us, double* Fs, double* entF, double* FlL, double* FlR,
                   double* UL, double* UR, double* FL, double* FR);
 
+/* UPDATE_IMEX_RK_STEP_P1 (484update_imex_rk_step_p1.f:7), MHD; the same field
+ * order as cs_p1_update in the product header */
+typedef struct or_p1_update {
+  int32_t ntp, nr, nghost, pad;
+  double dtmove, gamma, rhofloor, prsfloor;
+  const double* vert;
+  const double *xf_area, *yf_area, *zf_area, *v_volume;
+  double *U, *ent, *Bxf, *Byf, *Bzf;
+  const double* dU_dt;
+  const double *EntFlux_x, *EntFlux_y, *EntFlux_z, *Efx, *Efy, *Efz;
+  double* flag_entprs;
+} or_p1_update;
+int or_update_imex_rk_step_p1(const or_p1_update* u);
+
 #endif

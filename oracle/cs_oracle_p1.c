@@ -964,3 +964,90 @@ int or_make_facial_fluxes_p1(const or_p1_flux* f) {
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* UPDATE_IMEX_RK_STEP_P1 (400riemann_integrate/484update_imex_rk_step_p1.f:7),
+ * MHD: every zone's U += dt dU_dt (:483-489); the owned zones' entropy from the
+ * face entropy fluxes, then CONS_TO_PRIM and its flag (:540-628); constrained
+ * transport of the owned faces with the edges' arc lengths (:630-906). */
+int or_update_imex_rk_step_p1(const or_p1_update* u) {
+  if (!u || u->nghost < 1 || u->ntp < 1 || u->nr < 1) return 1;
+  or_p1_patch pp;
+  memset(&pp, 0, sizeof pp);
+  pp.ntp = u->ntp;
+  pp.nr = u->nr;
+  pp.nghost = u->nghost;
+  const p1dims D = p1_dims(&pp), *d = &D;
+  const int N = d->N, NR = d->NR, g = d->g;
+  const double dt = u->dtmove;
+  int i, j, k, c;
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        long z = zi(d, i, j, k);
+        for (c = 0; c < 8; ++c) u->U[z + d->NZ * c] = u->U[z + d->NZ * c] + dt * u->dU_dt[z + d->NZ * c];
+      }
+  for (long z = 0; z < d->NZ; ++z) u->flag_entprs[z] = 0.0;
+  for (k = 1; k <= NR; ++k)
+    for (j = 1; j <= N; ++j)
+      for (i = 1; i <= N; ++i) {
+        long z = zi(d, i, j, k);
+        double s = u->xf_area[xfi(d, i, j, k)] * u->EntFlux_x[fci(d, i, j, k)] -
+                   u->xf_area[xfi(d, i - 1, j, k)] * u->EntFlux_x[fci(d, i - 1, j, k)] +
+                   u->yf_area[yfi(d, i, j, k)] * u->EntFlux_y[fci(d, i, j, k)] -
+                   u->yf_area[yfi(d, i, j - 1, k)] * u->EntFlux_y[fci(d, i, j - 1, k)] +
+                   u->zf_area[zfi(d, i, j, k)] * u->EntFlux_z[fci(d, i, j, k)] -
+                   u->zf_area[zfi(d, i, j, k - 1)] * u->EntFlux_z[fci(d, i, j, k - 1)];
+        u->ent[z] = u->ent[z] - dt * s / u->v_volume[z];
+        double Up[8], Vp[8], e = u->ent[z];
+        for (c = 0; c < 8; ++c) Up[c] = u->U[z + d->NZ * c];
+        int flag = or_cons_to_prim(u->gamma, u->rhofloor, u->prsfloor, &e, Up, Vp);
+        for (c = 0; c < 8; ++c) u->U[z + d->NZ * c] = Up[c];
+        u->ent[z] = e;
+        u->flag_entprs[z] = (double)flag;
+      }
+  /* vertex (i, j, k) and its unit direction scaled by 1/r of another vertex */
+#define VX(ii, jj, kk, cc) u->vert[fci(d, ii, jj, kk) + d->NV * (cc)]
+#define RAD(ii, jj, kk) sqrt(VX(ii, jj, kk, 0) * VX(ii, jj, kk, 0) + VX(ii, jj, kk, 1) * VX(ii, jj, kk, 1) + \
+                             VX(ii, jj, kk, 2) * VX(ii, jj, kk, 2))
+  for (k = 1; k <= NR; ++k)
+    for (j = 1; j <= N; ++j)
+      for (i = 0; i <= N; ++i) {
+        double rout = RAD(i, j, k), rin = RAD(i, j, k - 1), dr = rout - rin, t = 1.0 / rout;
+        double dot = (VX(i, j, k, 0) * t) * (VX(i, j - 1, k, 0) * t) + (VX(i, j, k, 1) * t) * (VX(i, j - 1, k, 1) * t) +
+                     (VX(i, j, k, 2) * t) * (VX(i, j - 1, k, 2) * t);
+        double th = acos(dot), lin = rin * th, lout = rout * th;
+        long f = fci(d, i, j, k);
+        u->Bxf[f] = u->Bxf[f] - dt * (dr * u->Efz[f] - dr * u->Efz[fci(d, i, j - 1, k)] - lout * u->Efy[f] +
+                                      lin * u->Efy[fci(d, i, j, k - 1)]) / u->xf_area[xfi(d, i, j, k)];
+      }
+  for (k = 1; k <= NR; ++k)
+    for (j = 0; j <= N; ++j)
+      for (i = 1; i <= N; ++i) {
+        double rout = RAD(i, j, k), rin = RAD(i, j, k - 1), dr = rout - rin, t = 1.0 / rout;
+        double dot = (VX(i, j, k, 0) * t) * (VX(i - 1, j, k, 0) * t) + (VX(i, j, k, 1) * t) * (VX(i - 1, j, k, 1) * t) +
+                     (VX(i, j, k, 2) * t) * (VX(i - 1, j, k, 2) * t);
+        double th = acos(dot), lin = rin * th, lout = rout * th;
+        long f = fci(d, i, j, k);
+        u->Byf[f] = u->Byf[f] - dt * (-(dr * u->Efz[f]) + dr * u->Efz[fci(d, i - 1, j, k)] + lout * u->Efx[f] -
+                                      lin * u->Efx[fci(d, i, j, k - 1)]) / u->yf_area[yfi(d, i, j, k)];
+      }
+  for (k = 0; k <= NR; ++k)
+    for (j = 1; j <= N; ++j)
+      for (i = 1; i <= N; ++i) {
+        double rout = RAD(i, j, k), t = 1.0 / rout;
+        /* arc between two vertices of this radial level, both scaled by 1/rout */
+#define ARC(a1, b1, a2, b2)                                                                                  \
+  (rout * acos((VX(a1, b1, k, 0) * t) * (VX(a2, b2, k, 0) * t) + (VX(a1, b1, k, 1) * t) * (VX(a2, b2, k, 1) * t) + \
+               (VX(a1, b1, k, 2) * t) * (VX(a2, b2, k, 2) * t)))
+        double lxp = ARC(i, j, i - 1, j), lxm = ARC(i, j - 1, i - 1, j - 1), lyp = ARC(i, j, i, j - 1),
+               lym = ARC(i - 1, j, i - 1, j - 1);
+#undef ARC
+        long f = fci(d, i, j, k);
+        u->Bzf[f] = u->Bzf[f] - dt * (lyp * u->Efy[f] - lym * u->Efy[fci(d, i - 1, j, k)] - lxp * u->Efx[f] +
+                                      lxm * u->Efx[fci(d, i, j - 1, k)]) / u->zf_area[zfi(d, i, j, k)];
+      }
+#undef VX
+#undef RAD
+  return 0;
+}

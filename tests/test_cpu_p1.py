@@ -219,3 +219,94 @@ def test_facial_fluxes_fill_the_reference_ranges():
     assert js.min() - g == 0 and js.max() - g == N + 1
     assert ks.min() - g == 0 and ks.max() - g == NR + 1
     assert np.all(np.isfinite(A["dU_dt"]))
+
+
+# ---------------------------------------------------------------------------
+# UPDATE_IMEX_RK_STEP_P1 (484update_imex_rk_step_p1.f:7).  The reference's file
+# does not preprocess (an #endif without #if at :920), so the restatement is
+# fixed by what constrained transport and the update must satisfy.
+
+def _face_div(A, N, NR, g):
+    """Outward sum A_f B_f over the six faces of every owned zone."""
+    nf, nz = N + 2 * g + 1, N + 2 * g
+    nzr = NR + 2 * g
+    Bx = A["Bxf"].reshape(nzr + 1, nf, nf)
+    By = A["Byf"].reshape(nzr + 1, nf, nf)
+    Bz = A["Bzf"].reshape(nzr + 1, nf, nf)
+    ax = A["xf_area"].reshape(nzr, nz, nf)
+    ay = A["yf_area"].reshape(nzr, nf, nz)
+    az = A["zf_area"].reshape(nzr + 1, nz, nz)
+    K, J, I = np.meshgrid(np.arange(1, NR + 1), np.arange(1, N + 1), np.arange(1, N + 1), indexing="ij")
+    # cube index of (i, j, k) is [k + g, j + g, i + g]; x-face area [k-1+g, j-1+g, i+g]
+    fx = lambda i, j, k: Bx[k + g, j + g, i + g] * ax[k - 1 + g, j - 1 + g, i + g]
+    fy = lambda i, j, k: By[k + g, j + g, i + g] * ay[k - 1 + g, j + g, i - 1 + g]
+    fz = lambda i, j, k: Bz[k + g, j + g, i + g] * az[k + g, j - 1 + g, i - 1 + g]
+    return (fx(I, J, K) - fx(I - 1, J, K) + fy(I, J, K) - fy(I, J - 1, K) + fz(I, J, K) - fz(I, J, K - 1))
+
+
+def test_update_ct_keeps_face_divergence():
+    """Whatever the edge fields, constrained transport with edge lengths shared
+    by the faces around each edge leaves every owned zone's sum A_f B_f
+    unchanged (to rounding: the arc of an edge is scaled by the radius of a
+    neighbouring vertex of the same sphere)."""
+    N, NR, g = 10, 4, 4
+    prob, A = orc.p1_update_problem(N, NR, g, sector=3, seed=2, dt=0.1)
+    before = _face_div(A, N, NR, g)
+    scale = np.abs(A["Efx"]).max() * 0.1 * np.abs(A["xf_area"]).max() ** 0.5
+    orc.p1u_oracle(prob, A)
+    after = _face_div(A, N, NR, g)
+    assert np.abs(after - before).max() < 1e-12 * max(scale, np.abs(before).max()), np.abs(after - before).max()
+
+
+def test_update_arc_lengths_are_great_circles():
+    """With Efy = 1 on one lateral edge and zero elsewhere, the x-face whose
+    outer edge it is changes by dt l / A and the x-face whose inner edge it is
+    by -dt l / A, with l the great-circle length r atan2(|u1 x u2|, u1 . u2)
+    between the edge's two vertices (computed here independently)."""
+    N, NR, g = 8, 3, 4
+    prob, A = orc.p1_update_problem(N, NR, g, sector=1, seed=3, dt=0.5)
+    for k in ("Efx", "Efy", "Efz"):
+        A[k][:] = 0.0
+    nf, nz, nzr = N + 2 * g + 1, N + 2 * g, NR + 2 * g
+    i, j, k = 3, 4, 2
+    fc = lambda ii, jj, kk: (ii + g) + nf * ((jj + g) + nf * (kk + g))
+    A["Efy"][fc(i, j, k)] = 1.0  # the y-edge at x-vertex i, cell-y j, z-vertex k
+    B0 = A["Bxf"].copy()
+    orc.p1u_oracle(prob, A)
+    vert = A["vert"].reshape(3, nzr + 1, nf, nf)
+    u1 = vert[:, k + g, j + g, i + g]
+    u2 = vert[:, k + g, j - 1 + g, i + g]
+    r = np.linalg.norm(u1)
+    l_out = r * np.arctan2(np.linalg.norm(np.cross(u1, u2)), u1 @ u2)
+    area = A["xf_area"].reshape(nzr, nz, nf)[k - 1 + g, j - 1 + g, i + g]
+    dB = A["Bxf"][fc(i, j, k)] - B0[fc(i, j, k)]
+    np.testing.assert_allclose(dB, 0.5 * l_out / area, rtol=1e-12)
+    # the face below the edge sees it as its inner arc, with the opposite sign
+    area2 = A["xf_area"].reshape(nzr, nz, nf)[k + g, j - 1 + g, i + g]
+    dB2 = A["Bxf"][fc(i, j, k + 1)] - B0[fc(i, j, k + 1)]
+    np.testing.assert_allclose(dB2, -0.5 * l_out / area2, rtol=1e-12)
+
+
+def test_update_zone_state():
+    """Ghost zones: U += dt dU_dt exactly, flag 0; owned zones: the entropy
+    telescopes (sum of V ent changes = boundary entropy fluxes) and U goes
+    through CONS_TO_PRIM (energy consistent with the reset entropy when the
+    flag is set)."""
+    N, NR, g = 8, 3, 4
+    prob, A0 = orc.p1_update_problem(N, NR, g, sector=0, seed=4, dt=1e-3)
+    A = orc.p1u_oracle(prob, orc.p1_copy(A0))
+    nz, nzr = N + 2 * g, NR + 2 * g
+    NZ = nz * nz * nzr
+    idx = np.arange(NZ)
+    i = idx % nz - (g - 1)
+    j = (idx // nz) % nz - (g - 1)
+    k = idx // (nz * nz) - (g - 1)
+    own = (i >= 1) & (i <= N) & (j >= 1) & (j <= N) & (k >= 1) & (k <= NR)
+    U0 = A0["U"].reshape(4, 8, NZ)[0]
+    U1 = A["U"].reshape(4, 8, NZ)[0]
+    dU = A0["dU_dt"].reshape(8, NZ)
+    np.testing.assert_array_equal(U1[:, ~own], U0[:, ~own] + 1e-3 * dU[:, ~own])
+    assert np.all(A["flag_entprs"][~own] == 0.0)
+    np.testing.assert_array_equal(A["U"].reshape(4, 8, NZ)[1:], A0["U"].reshape(4, 8, NZ)[1:])
+    # mass: owned zones take U + dt dU (CONS_TO_PRIM leaves a positive density alone)
+    np.testing.assert_allclose(U1[0, own], U0[0, own] + 1e-3 * dU[0, own], rtol=1e-15)

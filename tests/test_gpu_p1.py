@@ -111,3 +111,24 @@ def test_facial_fluxes_gpu_matches_oracle(case):
     G = {k: T[k].cpu().numpy() for k in cs.P1F_OUT}
     bad = _compare(G, {k: O[k] for k in cs.P1F_OUT}, rtol=1e-11)
     assert not bad, bad
+
+
+@pytest.mark.parametrize("case", [(8, 3, 0, 0, 0.0), (12, 5, 1, 2, 1.0), (24, 8, 2, 4, 0.3)],
+                         ids=lambda c: "N%d_nr%d_s%d_seed%d_noise%g" % c)
+def test_update_gpu_matches_oracle(case):
+    """cs_update_imex_rk_step_p1 against or_update_imex_rk_step_p1 on a patch
+    after the oracle's reconstruction and facial fluxes: U, entropy and face B
+    to 1e-12 of their largest magnitude (acos and the entropy's power round
+    within an ulp of the host's), the flags exactly."""
+    N, NR, s, seed, noise = case
+    print(f"\nupdate case N={N} NR={NR} sector={s} seed={seed}", flush=True)
+    prob, A0 = orc.p1_update_problem(N, NR, 4, sector=s, seed=seed, dt=2e-3, noise=noise)
+    O = orc.p1u_oracle(prob, orc.p1_copy(A0))
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(A0[k]).to(dev) for k in cs.P1U_ARRAYS}
+    cs.update_imex_rk_step_p1(N, NR, 4, prob, T)
+    torch.cuda.synchronize()
+    G = {k: T[k].cpu().numpy() for k in ("U", "ent", "Bxf", "Byf", "Bzf", "flag_entprs")}
+    bad = _compare(G, {k: O[k] for k in G}, rtol=1e-12)
+    assert not bad, bad
+    np.testing.assert_array_equal(G["flag_entprs"], O["flag_entprs"])
