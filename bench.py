@@ -399,13 +399,24 @@ def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None
 # written; CLSQ + E per zone: V B modes 12 read and written, U 4 read and
 # written, scale 1, centroid 3, the zone's 3 own faces (B-field points 15,
 # normals 3 or 15, centroids 3, face B 1, face-point B 5), E 3 written.
+# CLEANUP_AFTER_UPDATE per zone with ghosts: U mode 1 read 8, U written 32 (the
+# higher modes cleared), entropy read 1 and written 4, V written 32, gamma_eos,
+# gamma_sound, mean_mol_wt written 3, the zone's own x, y face normals 6 and
+# z-face centroid 3, its 3 own face B, B_zc written 3, the mask 4 bytes.
+# EVALUATE_TIMESTEP per owned zone: U read and written 16, entropy 1, V written
+# 8, gamma_sound 1, scale 1, msonic written 1, the mask 4 bytes.
+P1_CLEANUP_BYTES = 8 * (8 + 32 + 1 + 4 + 32 + 3 + 6 + 3 + 3 + 3) + 4
+P1_TIMESTEP_BYTES = 8 * (16 + 1 + 8 + 1 + 1 + 1) + 4
 P1_BYTES = {"weno+positivity": 8 * (8 + 1 + 1 + 3 + 1 + 12 + 24 + 3 + 12),
             "clsq+E": 8 * (24 + 8 + 1 + 3 + 2 * (15 + 3 + 3 + 1 + 5) + (15 + 15 + 3 + 1 + 5) + 3)}
 
 
 def bench_p1(cs, args):
     """LIMIT_ALL_SLOPES_P1 on one sector of the 6 x 192^2 x 64 shell as a
-    patch (NGhost 4): owned-zone reconstructions per second, per-stage times."""
+    patch (NGhost 4): owned-zone reconstructions per second, per-stage times;
+    then the rest of the P1 stage on the same patch, each call timed:
+    MAKE_FACIAL_FLUXES, UPDATE_IMEX_RK_STEP_P1, CLEANUP_AFTER_UPDATE,
+    EVALUATE_TIMESTEP."""
     import torch
     N, NR, g = 192, 64, 4
     cfg, scal, T = p1_patch_on_gpu(cs, N, NR, g)
@@ -421,14 +432,31 @@ def bench_p1(cs, args):
         for k, b in zip(("Bxf", "Byf", "Bzf"), Bf0):
             T[k].copy_(b)
 
+    T["flag_entprs"] = torch.zeros(cs.p1_counts(N, NR, g)["mask"], dtype=torch.float64, device=T["U"].device)
+    uscal = dict(scal, dtmove=1e-4)
+    dt_out = torch.empty(1, dtype=torch.float64, device=T["U"].device)
+
+    def chain():
+        """the rest of one P1 stage after the reconstruction, each call timed"""
+        out = []
+        for fn in (lambda: cs.make_facial_fluxes_p1(N, NR, g, scal, T),
+                   lambda: cs.update_imex_rk_step_p1(N, NR, g, uscal, T),
+                   lambda: cs.cleanup_after_update_p1(N, NR, g, scal, T),
+                   lambda: cs.evaluate_timestep_p1(N, NR, g, scal, T, 0.4, 1.0, out=dt_out)):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            out.append(time.perf_counter() - t0)
+        return out
+
     for _ in range(max(1, args.warmup)):
         reset()
         cs.limit_all_slopes_p1(N, NR, g, scal, T)
-        cs.make_facial_fluxes_p1(N, NR, g, scal, T)
+        chain()
     torch.cuda.synchronize()
     steps = max(1, min(args.steps, 50))
     tot = 0.0
-    totf = 0.0
+    tchain = [0.0] * 4
     stage = None
     for _ in range(steps):
         reset()
@@ -436,14 +464,11 @@ def bench_p1(cs, args):
         t0 = time.perf_counter()
         vmin, vmax, ms = cs.limit_all_slopes_p1(N, NR, g, scal, T, timed=True)
         torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        cs.make_facial_fluxes_p1(N, NR, g, scal, T)
-        torch.cuda.synchronize()
-        tot += t1 - t0
-        totf += time.perf_counter() - t1
+        tot += time.perf_counter() - t0
+        tchain = [a + b for a, b in zip(tchain, chain())]
         stage = ms if stage is None else [a + b for a, b in zip(stage, ms)]
     ms_call = tot / steps * 1e3
-    ms_flux = totf / steps * 1e3
+    ms_flux, ms_upd, ms_clean, ms_dt = (t / steps * 1e3 for t in tchain)
     stage = [s / steps for s in stage]
     own = N * N * NR
     cnt = cs.p1_counts(N, NR, g)
@@ -458,7 +483,18 @@ def bench_p1(cs, args):
             k["frac"] = k["GBps"] / HBM_PEAK_GBS
         kern[name] = k
         log(f"   {name:20s} {t:8.3f} ms" + (f"  {k['GBps']:7.1f} GB/s algorithmic" if "GBps" in k else ""))
-    log(f"   MAKE_FACIAL_FLUXES   {ms_flux:8.3f} ms  ({own / (ms_flux * 1e-3):.3e} zones/s)")
+    rest = {}
+    for name, t, nb, items in (("MAKE_FACIAL_FLUXES", ms_flux, None, own),
+                               ("UPDATE_IMEX_RK_STEP_P1", ms_upd, None, own),
+                               ("CLEANUP_AFTER_UPDATE", ms_clean, P1_CLEANUP_BYTES, cnt["mask"]),
+                               ("EVALUATE_TIMESTEP", ms_dt, P1_TIMESTEP_BYTES, own)):
+        r = {"ms": t, "zones_per_s": own / (t * 1e-3)}
+        if nb:
+            r["GBps"] = nb * items / (t * 1e-3) / 1e9
+            r["frac"] = r["GBps"] / HBM_PEAK_GBS
+        rest[name] = r
+        log(f"   {name:22s} {t:8.3f} ms  ({own / (t * 1e-3):.3e} zones/s)"
+            + (f"  {r['GBps']:7.1f} GB/s algorithmic" if nb else ""))
     dom = max(kern, key=lambda n: kern[n]["ms"])
     cpu = None
     if not args.no_cpu:
@@ -473,7 +509,7 @@ def bench_p1(cs, args):
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": kern[dom].get("GBps"), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": kern[dom].get("frac"), "traffic": None},
         "cpu_baseline": cpu, "kernels": kern,
-        "facial_fluxes": {"ms": ms_flux, "zones_per_s": own / (ms_flux * 1e-3)},
+        "facial_fluxes": rest["MAKE_FACIAL_FLUXES"], "stage_chain": rest,
     }
     print(json.dumps(line))
 

@@ -128,6 +128,8 @@ def lib():
             "cs_limit_all_slopes_p1_timed": [vp, vp, ctypes.POINTER(ctypes.c_float)],
             "cs_make_facial_fluxes_p1": [vp, vp],
             "cs_update_imex_rk_step_p1": [vp, vp],
+            "cs_cleanup_after_update_p1": [vp, vp],
+            "cs_evaluate_timestep_p1": [vp, ctypes.c_double, ctypes.c_double, vp, vp],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
             "cs_batch_shell": [vp, i32, ctypes.POINTER(vp)],
@@ -652,3 +654,54 @@ def update_imex_rk_step_p1(N, NR, g, scalars, arrays, stream=None):
     st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
     _check(lib().cs_update_imex_rk_step_p1(ctypes.byref(s), st), "cs_update_imex_rk_step_p1")
 
+
+# ---------------------------------------------------------------------------
+# CLEANUP_AFTER_UPDATE and EVALUATE_TIMESTEP of one patch, on the fields of
+# cs_p1_patch (cs_cleanup_after_update_p1, cs_evaluate_timestep_p1)
+
+P1C_ARRAYS = ("mask", "stencil", "xf_normal", "yf_normal", "zf_centroid", "scale", "U", "ent", "Bxf", "Byf", "Bzf",
+              "V", "B1_zc", "B2_zc", "B3_zc", "gamma_eos", "gamma_sound", "mean_mol_wt")
+P1T_ARRAYS = ("mask", "scale", "U", "ent", "V", "gamma_sound", "msonic")
+
+
+def _p1_subset(who, N, NR, g, scalars, arrays, names, itri=(1, 1, 1, 1)):
+    import torch
+    want = p1_counts(N, NR, g)
+    s = P1Patch()
+    s.ntp, s.nr, s.nghost = N, NR, g
+    for q in range(4):
+        s.itri[q] = int(itri[q])
+    for k in P1_SCALARS:
+        setattr(s, k, float(scalars[k]))
+    for k in names:
+        t = arrays[k]
+        dt = torch.int32 if k in ("mask", "stencil") else torch.float64
+        if not (t.is_cuda and t.dtype == dt and t.is_contiguous() and t.numel() == want[k]):
+            raise CsError(f"{who}: {k} must be a contiguous {dt} CUDA tensor of {want[k]} "
+                          f"elements (got {t.dtype}, {t.numel()}, cuda={t.is_cuda})")
+        setattr(s, k, t.data_ptr())
+    return s
+
+
+def cleanup_after_update_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=None):
+    """CLEANUP_AFTER_UPDATE on one patch of contiguous torch tensors on the GPU
+    in the reference's layouts (P1C_ARRAYS are read; sizes checked first)."""
+    s = _p1_subset("cleanup_after_update_p1", N, NR, g, scalars, arrays, P1C_ARRAYS, itri)
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    _check(lib().cs_cleanup_after_update_p1(ctypes.byref(s), st), "cs_cleanup_after_update_p1")
+
+
+def evaluate_timestep_p1(N, NR, g, scalars, arrays, cfl, dtcur, out=None, stream=None):
+    """EVALUATE_TIMESTEP on one patch: returns the new time step as a float64
+    CUDA tensor of one element (out, when given, receives it) without waiting
+    on the stream."""
+    import torch
+    s = _p1_subset("evaluate_timestep_p1", N, NR, g, scalars, arrays, P1T_ARRAYS)
+    if out is None:
+        out = torch.empty(1, dtype=torch.float64, device=arrays["U"].device)
+    if not (out.is_cuda and out.dtype == torch.float64 and out.numel() >= 1):
+        raise CsError("evaluate_timestep_p1: out must be a float64 CUDA tensor")
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    _check(lib().cs_evaluate_timestep_p1(ctypes.byref(s), float(cfl), float(dtcur), ctypes.c_void_p(out.data_ptr()),
+                                         st), "cs_evaluate_timestep_p1")
+    return out

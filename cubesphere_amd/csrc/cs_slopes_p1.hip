@@ -1895,3 +1895,254 @@ extern "C" int cs_update_imex_rk_step_p1(const cs_p1_update* p, void* stream_) {
   P1_HIP(hipGetLastError());
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// CLEANUP_AFTER_UPDATE (cubesphere/400riemann_integrate/485cleanup_after_update.f:1)
+// and EVALUATE_TIMESTEP (490evaluate_timestep.f:1) of one patch, MHD, on the
+// fields of cs_p1_patch:
+//   k_face_zero, k_tricorner  steps 2 (faces) and 3 of the cleanup: the
+//                             reconstruction's kernels, whose ranges and rules
+//                             are the same (485...f:366-686)
+//   k_cleanup_zone            step 2 (zones), step 4.a: the zone B as the least-
+//                             squares fit of the six face values over the face
+//                             normals, the energy rebalanced (:691-865), and
+//                             step 5: CONS_TO_PRIM of every zone (:1180-1280)
+//   k_dt_init, k_dt           CONS_TO_PRIM of the owned active zones, their
+//                             magnetosonic speed, and the minimum of
+//                             cfl l / (|v| + c_f) with 1.1 dt (490...f:72-163):
+//                             a block minimum, folded into the output by a
+//                             compare-and-swap minimum (positive values, so
+//                             the result does not depend on the block order)
+namespace csp1 {
+
+__global__ void __launch_bounds__(256) k_cleanup_zone(const P1 P) {
+  const int64_t z = tid();
+  if (z >= P.NZ) return;
+  const cs_p1_patch& a = P.a;
+  const int i = (int)(z % P.nz) + 1 - P.g, j = (int)((z / P.nz) % P.nz) + 1 - P.g,
+            k = (int)(z / ((int64_t)P.nz * P.nz)) + 1 - P.g;
+  const double gam = a.gamma;
+  a.gamma_eos[z] = gam;
+  a.gamma_sound[z] = gam;
+  a.mean_mol_wt[z] = a.molewt;
+#pragma unroll
+  for (int c = 0; c < 8; ++c)
+#pragma unroll
+    for (int m = 1; m < 4; ++m) {
+      P.Ur(c, m, z) = 0.0;
+      P.Vr(c, m, z) = 0.0;
+    }
+#pragma unroll
+  for (int m = 1; m < 4; ++m) a.ent[z + P.NZ * m] = 0.0;
+  double U[8], V[8];
+  if (a.mask[z] == 0) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) U[c] = 0.0;
+    U[0] = a.rhofloor;
+    U[4] = a.prsfloor / (gam - 1.0);
+    a.scale[z] = a.smallnum;
+  } else {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) U[c] = P.Ur(c, 0, z);
+  }
+  if (P.active(i, j, 1)) {
+    // rows: x+ and x- face normals, y+ and y-, and the radial direction of
+    // the zone's upper z-face centroid for both z-faces
+    double A[6][3], R[6];
+    const int64_t fz = P.zfi(i, j, k), xp = P.xfi(i, j, k), xm = P.xfi(i - 1, j, k), yp = P.yfi(i, j, k),
+                  ym = P.yfi(i, j - 1, k);
+    double nv[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nv[c] = a.zf_centroid[fz + P.NZF * c];
+    const double t = 1.0 / fmax(a.smallnum, sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      A[0][c] = a.xf_normal[xp + P.NXF * c];
+      A[1][c] = a.xf_normal[xm + P.NXF * c];
+      A[2][c] = a.yf_normal[yp + P.NXF * c];
+      A[3][c] = a.yf_normal[ym + P.NXF * c];
+      A[4][c] = nv[c] * t;
+      A[5][c] = A[4][c];
+    }
+    R[0] = a.Bxf[P.fci(i, j, k)];
+    R[1] = a.Bxf[P.fci(i - 1, j, k)];
+    R[2] = a.Byf[P.fci(i, j, k)];
+    R[3] = a.Byf[P.fci(i, j - 1, k)];
+    R[4] = a.Bzf[P.fci(i, j, k)];
+    R[5] = a.Bzf[P.fci(i, j, k - 1)];
+    double M[3][3], r3[3];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        double s = 0.0;
+#pragma unroll
+        for (int r = 0; r < 6; ++r) s += A[r][p] * A[r][q];
+        M[p][q] = s;
+      }
+      double s = 0.0;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) s += A[r][p] * R[r];
+      r3[p] = s;
+    }
+    const double det = M[0][0] * (M[1][1] * M[2][2] - M[2][1] * M[1][2]) -
+                       M[0][1] * (M[1][0] * M[2][2] - M[2][0] * M[1][2]) +
+                       M[0][2] * (M[1][0] * M[2][1] - M[2][0] * M[1][1]);
+    const double od = copysign(1.0, det) / fmax(1.0e-32, fabs(det));
+    const double b[3][3] = {{(M[1][1] * M[2][2] - M[1][2] * M[2][1]) * od, (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * od,
+                             (M[0][1] * M[1][2] - M[1][1] * M[0][2]) * od},
+                            {(M[1][2] * M[2][0] - M[1][0] * M[2][2]) * od, (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * od,
+                             (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * od},
+                            {(M[1][0] * M[2][1] - M[1][1] * M[2][0]) * od, (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * od,
+                             (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * od}};
+    const double b2old = U[5] * U[5] + U[6] * U[6] + U[7] * U[7];
+#pragma unroll
+    for (int p = 0; p < 3; ++p) U[5 + p] = b[p][0] * r3[0] + b[p][1] * r3[1] + b[p][2] * r3[2];
+    const double ob8pi = 0.5 * C<double>::ob4pi;
+    U[4] = U[4] - b2old * ob8pi + (U[5] * U[5] + U[6] * U[6] + U[7] * U[7]) * ob8pi;
+    a.B1_zc[z] = U[5];
+    a.B2_zc[z] = U[6];
+    a.B3_zc[z] = U[7];
+  }
+  double e = a.ent[z];
+  Gas<double> gs{gam, gam - 1.0, a.rhofloor, a.prsfloor, 1.0 / (gam - 1.0)};
+  V[4] = 0.0;
+  cons_to_prim(gs, U, V, e);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    P.Vr(c, 0, z) = V[c];
+    P.Ur(c, 0, z) = U[c];
+  }
+  a.ent[z] = e;
+}
+
+constexpr int DT_THREADS = 256, DT_BLOCKS = 1024;
+
+__global__ void k_dt_init(double dt0, double* out) { out[0] = dt0; }
+
+// min of a positive double into *out by compare-and-swap (order-independent)
+__device__ __forceinline__ void atomic_min_pos(double* out, double v) {
+  unsigned long long* w = (unsigned long long*)out;
+  unsigned long long cur = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  while (v < __longlong_as_double((long long)cur)) {
+    const unsigned long long seen = atomicCAS(w, cur, (unsigned long long)__double_as_longlong(v));
+    if (seen == cur) break;
+    cur = seen;
+  }
+}
+
+__global__ void __launch_bounds__(DT_THREADS) k_dt(const P1 P, const Box own, double cfl, double dt0,
+                                                   double* out) {
+  __shared__ double red[DT_THREADS];
+  const cs_p1_patch& a = P.a;
+  const Gas<double> gs{a.gamma, a.gamma - 1.0, a.rhofloor, a.prsfloor, 1.0 / (a.gamma - 1.0)};
+  double best = dt0;
+  const int64_t n = own.count(), stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = tid(); t < n; t += stride) {
+    int i, j, k;
+    own.decode(t, i, j, k);
+    const int64_t z = P.zi(i, j, k);
+    if (a.mask[z] == 0) continue;
+    double U[8], V[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) U[c] = P.Ur(c, 0, z);
+    double e = a.ent[z];  // the routine keeps its entropy array as it is
+    V[4] = 0.0;
+    cons_to_prim(gs, U, V, e);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      P.Ur(c, 0, z) = U[c];
+      P.Vr(c, 0, z) = V[c];
+    }
+    const double b2 = V[5] * V[5] + V[6] * V[6] + V[7] * V[7];
+    const double ms = sqrt((a.gamma_sound[z] * V[4] + C<double>::ob4pi * b2) / V[0]);
+    a.msonic[z] = ms;
+    const double s = sqrt(V[1] * V[1] + V[2] * V[2] + V[3] * V[3]) + ms;
+    best = fmin(best, cfl * a.scale[z] / s);
+  }
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int h = DT_THREADS / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomic_min_pos(out, red[0]);
+}
+
+static int p1_dims_check(const cs_p1_patch* p, const char* who, P1& P) {
+  if (!p) {
+    cs::set_error("%s: null patch", who);
+    return CS_E_INVALID;
+  }
+  if (p->ntp < 1 || p->nr < 1 || p->nghost < 1) {
+    cs::set_error("%s: need ntp, nr, nghost >= 1 (got %d, %d, %d)", who, p->ntp, p->nr, p->nghost);
+    return CS_E_INVALID;
+  }
+  if (p->nghost >= p->ntp) {
+    cs::set_error("%s: nghost (%d) must be below ntp (%d)", who, p->nghost, p->ntp);
+    return CS_E_BAD_DECOMP;
+  }
+  if (!(p->gamma > 1.0) || !(p->rhofloor > 0.0) || !(p->prsfloor > 0.0) || !(p->smallnum > 0.0)) {
+    cs::set_error("%s: gamma > 1 and positive floors and smallnum required", who);
+    return CS_E_INVALID;
+  }
+  P.a = *p;
+  P.N = p->ntp;
+  P.NR = p->nr;
+  P.g = p->nghost;
+  P.nz = P.N + 2 * P.g;
+  P.nzr = P.NR + 2 * P.g;
+  P.nf = P.nz + 1;
+  P.nfr = P.nzr + 1;
+  P.NZ = (int64_t)P.nz * P.nz * P.nzr;
+  P.NXF = (int64_t)P.nf * P.nz * P.nzr;
+  P.NZF = (int64_t)P.nz * P.nz * P.nfr;
+  P.NFC = (int64_t)P.nf * P.nf * P.nfr;
+  return 0;
+}
+
+}  // namespace csp1
+
+extern "C" int cs_cleanup_after_update_p1(const cs_p1_patch* p, void* stream_) {
+  P1 P;
+  int rc = p1_dims_check(p, "cs_cleanup_after_update_p1", P);
+  if (rc) return rc;
+  const void* ptrs[] = {p->mask, p->stencil, p->xf_normal, p->yf_normal, p->zf_centroid, p->scale, p->U,
+                        p->ent, p->Bxf, p->Byf, p->Bzf, p->V, p->B1_zc, p->B2_zc, p->B3_zc, p->gamma_eos,
+                        p->gamma_sound, p->mean_mol_wt};
+  for (const void* q : ptrs)
+    if (!q) {
+      cs::set_error("cs_cleanup_after_update_p1: null array in the patch");
+      return CS_E_INVALID;
+    }
+  hipStream_t st = (hipStream_t)stream_;
+  k_face_zero<<<dim3(grid_of(P.NFC, 256), 3), 256, 0, st>>>(P);
+  k_tricorner<<<grid_of(4 * (int64_t)P.g * P.nzr, 256), 256, 0, st>>>(P);
+  k_cleanup_zone<<<grid_of(P.NZ, 256), 256, 0, st>>>(P);
+  P1_HIP(hipGetLastError());
+  return 0;
+}
+
+extern "C" int cs_evaluate_timestep_p1(const cs_p1_patch* p, double cfl, double dtcur, double* dt_new,
+                                       void* stream_) {
+  P1 P;
+  int rc = p1_dims_check(p, "cs_evaluate_timestep_p1", P);
+  if (rc) return rc;
+  const void* ptrs[] = {p->mask, p->scale, p->U, p->ent, p->V, p->gamma_sound, p->msonic, dt_new};
+  for (const void* q : ptrs)
+    if (!q) {
+      cs::set_error("cs_evaluate_timestep_p1: null array in the patch or null dt_new");
+      return CS_E_INVALID;
+    }
+  if (!(cfl > 0.0) || !(dtcur > 0.0)) {
+    cs::set_error("cs_evaluate_timestep_p1: cfl and dtcur must be positive (got %g, %g)", cfl, dtcur);
+    return CS_E_INVALID;
+  }
+  hipStream_t st = (hipStream_t)stream_;
+  const Box own = make_box(1, P.N, 1, P.N, 1, P.NR);
+  const int nblk = (int)std::min<int64_t>(grid_of(own.count(), DT_THREADS), DT_BLOCKS);
+  k_dt_init<<<1, 1, 0, st>>>(1.1 * dtcur, dt_new);
+  k_dt<<<nblk, DT_THREADS, 0, st>>>(P, own, cfl, 1.1 * dtcur, dt_new);
+  P1_HIP(hipGetLastError());
+  return 0;
+}

@@ -390,6 +390,28 @@ typedef struct cs_p1_update {
 } cs_p1_update;
 int cs_update_imex_rk_step_p1(const cs_p1_update* p, void* stream);
 
+/* ---- P1 cleanup and time step of one patch ---------------------------------
+ * CLEANUP_AFTER_UPDATE (400riemann_integrate/485cleanup_after_update.f:1-25),
+ * MHD, on the fields of cs_p1_patch: gamma_eos, gamma_sound, mean_mol_wt set;
+ * modes 2-4 of U, entropygr and V cleared; inactive (tri-corner) zones made
+ * safe (scale = smallnum); face B between two inactive zones cleared and
+ * synchronised at the tri-corners; on every zone of an active column the zone
+ * B as the least-squares fit of its six face values over the face normals
+ * (x and y faces: xf_normal, yf_normal; both z faces: the radial direction of
+ * zf_centroid), the energy rebalanced by the change of magnetic energy, and
+ * B1_zc..B3_zc (mode 1) set; then CONS_TO_PRIM on every zone (V, U, entropy
+ * mode 1).  Reads mask, stencil, xf_normal, yf_normal, zf_centroid; the other
+ * geometry pointers may be NULL.  nghost >= 1.  No wait on the stream. */
+int cs_cleanup_after_update_p1(const cs_p1_patch* p, void* stream);
+/* EVALUATE_TIMESTEP (400riemann_integrate/490evaluate_timestep.f:1-23), MHD:
+ * CONS_TO_PRIM of the owned active zones (U and V mode 1 rewritten, the
+ * entropy array left as it is), msonic = sqrt((gamma_sound p + B^2 / 4pi) /
+ * rho), and *dt_new = min(1.1 dtcur, min cfl scale / (|v| + msonic)).  dt_new
+ * is a DEVICE pointer to one double, written on the stream (no wait), so the
+ * minimum over many patches can stay on the device.  Reads mask, scale,
+ * gamma_sound; writes U, V, msonic. */
+int cs_evaluate_timestep_p1(const cs_p1_patch* p, double cfl, double dtcur, double* dt_new, void* stream);
+
 int cs_last_error(char* buf, size_t len);
 int cs_device_info(char* buf, size_t len);
 

@@ -161,4 +161,9 @@ typedef struct or_p1_update {
 } or_p1_update;
 int or_update_imex_rk_step_p1(const or_p1_update* u);
 
+/* CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:4) and EVALUATE_TIMESTEP
+ * (490evaluate_timestep.f:7) of one patch, MHD, on the fields of or_p1_patch */
+int or_cleanup_after_update_p1(const or_p1_patch* p);
+int or_evaluate_timestep_p1(const or_p1_patch* p, double cfl, double dtcur, double* dtnew);
+
 #endif

@@ -1051,3 +1051,173 @@ int or_update_imex_rk_step_p1(const or_p1_update* u) {
 #undef RAD
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* CLEANUP_AFTER_UPDATE (400riemann_integrate/485cleanup_after_update.f:4), MHD:
+ * the constants and the safe tri-corner zones of STEP 2 (:305-490), the
+ * tri-corner face-B sync of STEP 3 (:491-686), the zone B as the least-squares
+ * fit of the six face values over the face normals of STEP 4.a (:691-865, the
+ * branch the routine compiles) with the energy rebalanced, and CONS_TO_PRIM of
+ * every zone (STEP 5, :1180-1280). */
+int or_cleanup_after_update_p1(const or_p1_patch* p) {
+  if (!p || p->nghost < 1 || p->ntp < 1 || p->nr < 1) return 1;
+  const p1dims D = p1_dims(p), *d = &D;
+  const int N = d->N, NR = d->NR, g = d->g;
+  const double pi = 4.0 * atan(1.0), ob8pi = 1.0 / (8.0 * pi), gam = p->gamma;
+  int i, j, k, c, m;
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        long z = zi(d, i, j, k);
+        p->gamma_eos[z] = gam;
+        p->gamma_sound[z] = gam;
+        p->mean_mol_wt[z] = p->molewt;
+        for (c = 0; c < 8; ++c)
+          for (m = 1; m < 4; ++m) UU(p, d, c, m, z) = 0.0;
+        for (m = 1; m < 4; ++m) p->ent[z + d->NZ * m] = 0.0;
+        for (c = 0; c < 8; ++c)
+          for (m = 0; m < 4; ++m) VV(p, d, c, m, z) = 0.0;
+        if (p->mask[z] == 0) {
+          for (c = 0; c < 8; ++c) UU(p, d, c, 0, z) = 0.0;
+          UU(p, d, 0, 0, z) = p->rhofloor;
+          UU(p, d, 4, 0, z) = p->prsfloor / (gam - 1.0);
+          p->scale[z] = p->smallnum;
+        }
+      }
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = -g; i <= N + g; ++i) {
+        int a = i < 1 - g ? 1 - g : i, b = i + 1 > N + g ? N + g : i + 1;
+        if (!active(p, d, a, j, k) && !active(p, d, b, j, k)) p->Bxf[fci(d, i, j, k)] = 0.0;
+      }
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = -g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        int a = j < 1 - g ? 1 - g : j, b = j + 1 > N + g ? N + g : j + 1;
+        if (!active(p, d, i, a, k) && !active(p, d, i, b, k)) p->Byf[fci(d, i, j, k)] = 0.0;
+      }
+  for (k = -g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i)
+        if (!active(p, d, i, j, 1)) p->Bzf[fci(d, i, j, k)] = 0.0;
+  for (int corner = 0; corner < 4; ++corner) {
+    if (!p->itri[corner]) continue;
+    int jj = (corner < 2) ? 1 : N, lo = (corner == 0 || corner == 3) ? 1 - g : N + 1,
+        hi = (corner == 0 || corner == 3) ? 0 : N + g;
+    for (i = lo; i <= hi; ++i) {
+      int a, b;
+      nbr(p, d, i, jj, corner < 2 ? 3 : 2, &a, &b);
+      for (k = 1 - g; k <= NR + g; ++k) {
+        double* by = &p->Byf[fci(d, i, corner < 2 ? jj - 1 : jj, k)];
+        double* bx = &p->Bxf[fci(d, (corner == 0 || corner == 3) ? a - 1 : a, b, k)];
+        if (corner == 0 || corner == 2) {
+          double t = 0.5 * (*by - *bx);
+          *by = t;
+          *bx = -t;
+        } else {
+          double t = 0.5 * (*by + *bx);
+          *by = t;
+          *bx = t;
+        }
+      }
+    }
+  }
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        if (!active(p, d, i, j, 1)) continue;
+        long z = zi(d, i, j, k);
+        double nv[3], A6[6][3], R6[6], A33[3][3], R3[3], t;
+        long fz = zfi(d, i, j, k);
+        for (c = 0; c < 3; ++c) nv[c] = p->zf_centroid[fz + d->NZF * c];
+        t = 1.0 / omax(p->smallnum, sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]));
+        for (c = 0; c < 3; ++c) nv[c] = nv[c] * t;
+        for (c = 0; c < 3; ++c) {
+          A6[0][c] = p->xf_normal[xfi(d, i, j, k) + d->NXF * c];
+          A6[1][c] = p->xf_normal[xfi(d, i - 1, j, k) + d->NXF * c];
+          A6[2][c] = p->yf_normal[yfi(d, i, j, k) + d->NYF * c];
+          A6[3][c] = p->yf_normal[yfi(d, i, j - 1, k) + d->NYF * c];
+          A6[4][c] = nv[c];
+          A6[5][c] = nv[c];
+        }
+        R6[0] = p->Bxf[fci(d, i, j, k)];
+        R6[1] = p->Bxf[fci(d, i - 1, j, k)];
+        R6[2] = p->Byf[fci(d, i, j, k)];
+        R6[3] = p->Byf[fci(d, i, j - 1, k)];
+        R6[4] = p->Bzf[fci(d, i, j, k)];
+        R6[5] = p->Bzf[fci(d, i, j, k - 1)];
+        for (int a = 0; a < 3; ++a) {
+          for (int b = 0; b < 3; ++b) {
+            double s = 0.0;
+            for (int r = 0; r < 6; ++r) s = s + A6[r][a] * A6[r][b];
+            A33[a][b] = s;
+          }
+          double s = 0.0;
+          for (int r = 0; r < 6; ++r) s = s + A6[r][a] * R6[r];
+          R3[a] = s;
+        }
+        double a11 = A33[0][0], a12 = A33[0][1], a13 = A33[0][2], a21 = A33[1][0], a22 = A33[1][1], a23 = A33[1][2],
+               a31 = A33[2][0], a32 = A33[2][1], a33 = A33[2][2];
+        double det = a11 * (a22 * a33 - a32 * a23) - a12 * (a21 * a33 - a31 * a23) + a13 * (a21 * a32 - a31 * a22);
+        double od = (signbit(det) ? -1.0 : 1.0) / omax(1.0e-32, fabs(det));
+        double b11 = (a22 * a33 - a23 * a32) * od, b12 = (a13 * a32 - a12 * a33) * od, b13 = (a12 * a23 - a22 * a13) * od;
+        double b21 = (a23 * a31 - a21 * a33) * od, b22 = (a11 * a33 - a13 * a31) * od, b23 = (a13 * a21 - a11 * a23) * od;
+        double b31 = (a21 * a32 - a22 * a31) * od, b32 = (a12 * a31 - a11 * a32) * od, b33 = (a11 * a22 - a12 * a21) * od;
+        double Bx = UU(p, d, 5, 0, z), By = UU(p, d, 6, 0, z), Bz = UU(p, d, 7, 0, z);
+        UU(p, d, 5, 0, z) = b11 * R3[0] + b12 * R3[1] + b13 * R3[2];
+        UU(p, d, 6, 0, z) = b21 * R3[0] + b22 * R3[1] + b23 * R3[2];
+        UU(p, d, 7, 0, z) = b31 * R3[0] + b32 * R3[1] + b33 * R3[2];
+        UU(p, d, 4, 0, z) = UU(p, d, 4, 0, z) - (Bx * Bx + By * By + Bz * Bz) * ob8pi +
+                            (UU(p, d, 5, 0, z) * UU(p, d, 5, 0, z) + UU(p, d, 6, 0, z) * UU(p, d, 6, 0, z) +
+                             UU(p, d, 7, 0, z) * UU(p, d, 7, 0, z)) * ob8pi;
+        p->B1_zc[z] = UU(p, d, 5, 0, z);
+        p->B2_zc[z] = UU(p, d, 6, 0, z);
+        p->B3_zc[z] = UU(p, d, 7, 0, z);
+        for (c = 5; c < 8; ++c) VV(p, d, c, 0, z) = UU(p, d, c, 0, z);
+      }
+  for (k = 1 - g; k <= NR + g; ++k)
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        long z = zi(d, i, j, k);
+        double Up[8], Vp[8], e = p->ent[z];
+        for (c = 0; c < 8; ++c) Up[c] = UU(p, d, c, 0, z);
+        or_cons_to_prim(gam, p->rhofloor, p->prsfloor, &e, Up, Vp);
+        for (c = 0; c < 8; ++c) {
+          VV(p, d, c, 0, z) = Vp[c];
+          UU(p, d, c, 0, z) = Up[c];
+        }
+        p->ent[z] = e;
+      }
+  return 0;
+}
+
+/* EVALUATE_TIMESTEP (400riemann_integrate/490evaluate_timestep.f:7-306), MHD:
+ * CONS_TO_PRIM of the owned active zones (U, V mode 1 rewritten), their
+ * magnetosonic speed, and dt_new = min(1.1 dt, cfl l / (|v| + c_f)). */
+int or_evaluate_timestep_p1(const or_p1_patch* p, double cfl, double dtcur, double* dtnew) {
+  if (!p || !dtnew) return 1;
+  const p1dims D = p1_dims(p), *d = &D;
+  const double pi = 4.0 * atan(1.0), ob4pi = 1.0 / (4.0 * pi);
+  double dt = 1.1 * dtcur;
+  for (int k = 1; k <= d->NR; ++k)
+    for (int j = 1; j <= d->N; ++j)
+      for (int i = 1; i <= d->N; ++i) {
+        long z = zi(d, i, j, k);
+        if (p->mask[z] == 0) continue;
+        double Up[8], Vp[8], e = p->ent[z];
+        int c;
+        for (c = 0; c < 8; ++c) Up[c] = UU(p, d, c, 0, z);
+        or_cons_to_prim(p->gamma, p->rhofloor, p->prsfloor, &e, Up, Vp);
+        for (c = 0; c < 8; ++c) {
+          UU(p, d, c, 0, z) = Up[c];
+          VV(p, d, c, 0, z) = Vp[c];
+        }
+        double b2 = (Vp[5] + 0.0) * (Vp[5] + 0.0) + (Vp[6] + 0.0) * (Vp[6] + 0.0) + (Vp[7] + 0.0) * (Vp[7] + 0.0);
+        double ms = p->gamma_sound[z] * Vp[4] + ob4pi * b2;
+        p->msonic[z] = sqrt(ms / Vp[0]);
+        double a21 = sqrt(Vp[1] * Vp[1] + Vp[2] * Vp[2] + Vp[3] * Vp[3]) + p->msonic[z];
+        dt = omin(dt, cfl * p->scale[z] / a21);
+      }
+  *dtnew = dt;
+  return 0;
+}

@@ -476,3 +476,72 @@ def p1u_oracle(prob, A):
     if L.or_update_imex_rk_step_p1(ctypes.byref(s)) != 0:
         raise RuntimeError("oracle rejected the patch")
     return A
+
+
+# ---------------------------------------------------------------------------
+# CLEANUP_AFTER_UPDATE and EVALUATE_TIMESTEP of one patch, on the P1Patch
+# structure (cubesphere_amd.cleanup_after_update_p1 / evaluate_timestep_p1)
+
+def p1_cleanup_problem(N, NR, g=4, sector=0, seed=0, dt=2e-3, noise=0.0, floors=True):
+    """A patch after the oracle's reconstruction, facial fluxes and update;
+    with floors, a few owned zones get a negative internal energy or density
+    so that CONS_TO_PRIM takes its entropy and floor branches."""
+    prob, A = p1_update_problem(N, NR, g, sector=sector, seed=seed, dt=dt, noise=noise)
+    p1u_oracle(prob, A)
+    d = p1_dims(N, NR, g)
+    nz = d["zone"]
+    if floors:
+        rng = np.random.default_rng(seed + 300)
+        U = A["U"].reshape(4, 8, nz)
+        own = np.flatnonzero(A["mask"] == 1)
+        pick = rng.choice(own, size=max(2, own.size // 50), replace=False)
+        half = pick.size // 2
+        U[0, 4, pick[:half]] = 0.5 * (U[0, 1:4, pick[:half]] ** 2).sum(-1) / U[0, 0, pick[:half]] * 0.9
+        # a negative density, with a momentum that keeps the velocity O(1) once
+        # the density is floored (else the pressure is the difference of two
+        # 1e11 energies: rounding noise)
+        U[0, 0, pick[half:half + 2]] = -1e-3
+        U[0, 1:4, pick[half:half + 2]] *= prob["rhofloor"]
+    return prob, A
+
+
+def p1c_oracle(prob, A):
+    L = olib()
+    L.or_cleanup_after_update_p1.argtypes = [ctypes.c_void_p]
+    s = p1_struct(prob, A)
+    if L.or_cleanup_after_update_p1(ctypes.byref(s)) != 0:
+        raise RuntimeError("oracle rejected the patch")
+    return A
+
+
+def p1c_reference(prob, A):
+    """The reference's CLEANUP_AFTER_UPDATE (flang build) in place on A."""
+    R = rlib()
+    if R is None:
+        raise RuntimeError("reference build not present")
+    R.ref_cleanup_after_update_p1.argtypes = [ctypes.c_void_p]
+    s = p1_struct(prob, A)
+    R.ref_cleanup_after_update_p1(ctypes.byref(s))
+    return A
+
+
+def p1_timestep_oracle(prob, A, cfl, dtcur):
+    L = olib()
+    L.or_evaluate_timestep_p1.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+    s = p1_struct(prob, A)
+    out = ctypes.c_double(0.0)
+    if L.or_evaluate_timestep_p1(ctypes.byref(s), cfl, dtcur, ctypes.byref(out)) != 0:
+        raise RuntimeError("oracle rejected the patch")
+    return out.value
+
+
+def p1_timestep_reference(prob, A, cfl, dtcur):
+    """The reference's EVALUATE_TIMESTEP (flang build) in place on A."""
+    R = rlib()
+    if R is None:
+        raise RuntimeError("reference build not present")
+    R.ref_evaluate_timestep_p1.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_void_p]
+    s = p1_struct(prob, A)
+    out = ctypes.c_double(0.0)
+    R.ref_evaluate_timestep_p1(ctypes.byref(s), cfl, dtcur, ctypes.byref(out))
+    return out.value

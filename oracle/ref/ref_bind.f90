@@ -309,4 +309,74 @@ contains
     deallocate(xq, xw, yq, yw, zq, zw, zn, vol, coef, wzc)
   end subroutine
 
+  ! CLEANUP_AFTER_UPDATE (485cleanup_after_update.f:1) on the fields of a
+  ! ref_p1_patch; z-face normals are declared but unused in the MHD branch
+  subroutine ref_cleanup_after_update_p1(p) bind(C, name="ref_cleanup_after_update_p1")
+    type(ref_p1_patch), intent(in) :: p
+    integer :: n, nr, g, nz, nzr, nf, nfr, nzone, nxf, nzf, nfc
+    real(c_double), pointer :: vert(:), xfn(:), xfc(:), yfn(:), yfc(:), zfc(:), scl(:), cen(:)
+    real(c_double), pointer :: u(:), ent(:), bxf(:), byf(:), bzf(:), v(:), b1(:), b2(:), b3(:)
+    real(c_double), pointer :: geos(:), gsnd(:), mmw(:)
+    integer(c_int), pointer :: mask(:), st(:)
+    real(c_double), allocatable :: zn(:)
+    external CLEANUP_AFTER_UPDATE, REF_INIT_COMMON, REF_SET_P1_COMMON
+    n = p%ntp; nr = p%nr; g = p%nghost
+    nz = n + 2 * g; nzr = nr + 2 * g; nf = nz + 1; nfr = nzr + 1
+    nzone = nz * nz * nzr; nxf = nf * nz * nzr; nzf = nz * nz * nfr; nfc = nf * nf * nfr
+    call REF_INIT_COMMON()
+    call REF_SET_P1_COMMON(p%gamma, p%molewt, p%rhofloor, p%prsfloor, p%smallnum, p%flatten_coef, &
+                           p%prs_flatten_coef)
+    call c_f_pointer(p%vert, vert, [3 * nfc]); call c_f_pointer(p%mask, mask, [nzone])
+    call c_f_pointer(p%stencil, st, [nz * nz * 24])
+    call c_f_pointer(p%xf_normal, xfn, [3 * nxf]); call c_f_pointer(p%xf_centroid, xfc, [3 * nxf])
+    call c_f_pointer(p%yf_normal, yfn, [3 * nxf]); call c_f_pointer(p%yf_centroid, yfc, [3 * nxf])
+    call c_f_pointer(p%zf_centroid, zfc, [3 * nzf])
+    call c_f_pointer(p%scale, scl, [nzone]); call c_f_pointer(p%v_centroid, cen, [3 * nzone])
+    call c_f_pointer(p%U, u, [32 * nzone]); call c_f_pointer(p%ent, ent, [4 * nzone])
+    call c_f_pointer(p%Bxf, bxf, [nfc]); call c_f_pointer(p%Byf, byf, [nfc]); call c_f_pointer(p%Bzf, bzf, [nfc])
+    call c_f_pointer(p%V, v, [32 * nzone])
+    call c_f_pointer(p%B1_zc, b1, [4 * nzone]); call c_f_pointer(p%B2_zc, b2, [4 * nzone])
+    call c_f_pointer(p%B3_zc, b3, [4 * nzone])
+    call c_f_pointer(p%gamma_eos, geos, [nzone]); call c_f_pointer(p%gamma_sound, gsnd, [nzone])
+    call c_f_pointer(p%mean_mol_wt, mmw, [nzone])
+    allocate(zn(15 * nzf)); zn = 0
+    call CLEANUP_AFTER_UPDATE(nr, n, g, 8, 4, 1, 1, 1, vert, mask, xfn, xfc, yfn, yfc, zn, zfc, &
+         scl, cen, p%itri(1), p%itri(2), p%itri(3), p%itri(4), st, u, ent, bxf, byf, bzf, &
+         v, b1, b2, b3, geos, gsnd, mmw)
+    deallocate(zn)
+  end subroutine
+
+  ! EVALUATE_TIMESTEP (490evaluate_timestep.f:1) on the fields of a ref_p1_patch
+  subroutine ref_evaluate_timestep_p1(p, cfl, dtcur, dtnew) bind(C, name="ref_evaluate_timestep_p1")
+    type(ref_p1_patch), intent(in) :: p
+    real(c_double), value :: cfl, dtcur
+    real(c_double), intent(out) :: dtnew
+    integer :: n, nr, g, nz, nzr, nf, nfr, nzone, nfc
+    real(c_double) :: t
+    real(c_double), pointer :: vert(:), scl(:), cen(:), u(:), ent(:), bxf(:), byf(:), bzf(:), v(:)
+    real(c_double), pointer :: b1(:), b2(:), b3(:), geos(:), gsnd(:), mmw(:), divv(:), msn(:), flat(:)
+    integer(c_int), pointer :: mask(:)
+    external EVALUATE_TIMESTEP, REF_INIT_COMMON, REF_SET_P1_COMMON, REF_SET_CFL
+    n = p%ntp; nr = p%nr; g = p%nghost
+    nz = n + 2 * g; nzr = nr + 2 * g; nf = nz + 1; nfr = nzr + 1
+    nzone = nz * nz * nzr; nfc = nf * nf * nfr
+    call REF_INIT_COMMON()
+    call REF_SET_P1_COMMON(p%gamma, p%molewt, p%rhofloor, p%prsfloor, p%smallnum, p%flatten_coef, &
+                           p%prs_flatten_coef)
+    call REF_SET_CFL(cfl)
+    call c_f_pointer(p%vert, vert, [3 * nfc]); call c_f_pointer(p%mask, mask, [nzone])
+    call c_f_pointer(p%scale, scl, [nzone]); call c_f_pointer(p%v_centroid, cen, [3 * nzone])
+    call c_f_pointer(p%U, u, [32 * nzone]); call c_f_pointer(p%ent, ent, [4 * nzone])
+    call c_f_pointer(p%Bxf, bxf, [nfc]); call c_f_pointer(p%Byf, byf, [nfc]); call c_f_pointer(p%Bzf, bzf, [nfc])
+    call c_f_pointer(p%V, v, [32 * nzone])
+    call c_f_pointer(p%B1_zc, b1, [4 * nzone]); call c_f_pointer(p%B2_zc, b2, [4 * nzone])
+    call c_f_pointer(p%B3_zc, b3, [4 * nzone])
+    call c_f_pointer(p%gamma_eos, geos, [nzone]); call c_f_pointer(p%gamma_sound, gsnd, [nzone])
+    call c_f_pointer(p%mean_mol_wt, mmw, [nzone]); call c_f_pointer(p%divvel, divv, [nzone])
+    call c_f_pointer(p%msonic, msn, [nzone]); call c_f_pointer(p%flatten, flat, [nzone])
+    t = 0
+    call EVALUATE_TIMESTEP(nr, n, g, 8, 4, 1, t, dtcur, dtnew, vert, mask, scl, cen, &
+         u, ent, bxf, byf, bzf, v, b1, b2, b3, geos, gsnd, mmw, divv, msn, flat)
+  end subroutine
+
 end module cs_ref_bind

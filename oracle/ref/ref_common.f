@@ -32,3 +32,11 @@
       magB_small_cgl = 1.0e-12
       eps_bdry_cgl = 1.0e-12
       END SUBROUTINE REF_SET_P1_COMMON
+
+! CFL number of EVALUATE_TIMESTEP (490evaluate_timestep.f:158)
+      SUBROUTINE REF_SET_CFL ( cfl)
+      IMPLICIT NONE
+#include <cubesphere/000com/riemann.com>
+      REAL cfl
+      cfl_coef = cfl
+      END SUBROUTINE REF_SET_CFL

@@ -310,3 +310,116 @@ def test_update_zone_state():
     np.testing.assert_array_equal(A["U"].reshape(4, 8, NZ)[1:], A0["U"].reshape(4, 8, NZ)[1:])
     # mass: owned zones take U + dt dU (CONS_TO_PRIM leaves a positive density alone)
     np.testing.assert_allclose(U1[0, own], U0[0, own] + 1e-3 * dU[0, own], rtol=1e-15)
+
+
+CLEANUP_CASES = [
+    # N, NR, sector, seed, noise, floors
+    (8, 3, 0, 0, 0.0, True),
+    (12, 5, 1, 2, 1.0, True),
+    (10, 4, 5, 3, 0.5, False),
+    (9, 4, 3, 1, 0.3, True),
+]
+
+
+@pytest.mark.parametrize("case", CLEANUP_CASES, ids=lambda c: "N%d_nr%d_s%d_seed%d_noise%g_floors%d" % c)
+def test_cleanup_and_timestep_bit_exact_with_reference(case):
+    """or_cleanup_after_update_p1 against CLEANUP_AFTER_UPDATE
+    (485cleanup_after_update.f:1) and or_evaluate_timestep_p1 against
+    EVALUATE_TIMESTEP (490evaluate_timestep.f:1) on the cleaned patch: every
+    array and the new time step bit for bit."""
+    _ref_or_skip()
+    N, NR, s, seed, noise, floors = case
+    prob, A0 = orc.p1_cleanup_problem(N, NR, 4, sector=s, seed=seed, noise=noise, floors=floors)
+    A = orc.p1c_oracle(prob, orc.p1_copy(A0))
+    B = orc.p1c_reference(prob, orc.p1_copy(A0))
+    bad = {k: int((A[k] != B[k]).sum()) for k in A if not np.array_equal(A[k], B[k])}
+    assert not bad, bad
+    for dtcur in (1.0, 1e-3):
+        ta = orc.p1_timestep_oracle(prob, A, 0.4, dtcur)
+        tb = orc.p1_timestep_reference(prob, B, 0.4, dtcur)
+        assert ta == tb
+        bad = {k: int((A[k] != B[k]).sum()) for k in A if not np.array_equal(A[k], B[k])}
+        assert not bad, bad
+    assert orc.p1_timestep_oracle(prob, A, 0.4, 1e-18) == 1.1 * 1e-18
+
+
+def test_cleanup_cases_reach_every_branch():
+    """The cases above make the tri-corner zones safe, floor a density and
+    restore a pressure from the entropy in CONS_TO_PRIM."""
+    prob, A0 = orc.p1_cleanup_problem(12, 5, 4, sector=1, seed=2, noise=1.0, floors=True)
+    d = orc.p1_dims(12, 5, 4)
+    nz = d["zone"]
+    A = orc.p1c_oracle(prob, orc.p1_copy(A0))
+    U0, U = A0["U"].reshape(4, 8, nz), A["U"].reshape(4, 8, nz)
+    V = A["V"].reshape(4, 8, nz)
+    g = prob["gamma"]
+    assert (A["mask"] == 0).sum() > 0
+    assert np.all(A["scale"][A["mask"] == 0] == prob["smallnum"])
+    floored = U0[0, 0] < prob["rhofloor"]
+    assert floored.sum() >= 2 and np.all(U[0, 0, floored] == prob["rhofloor"])
+    e0 = U0[0, 4] - 0.5 * (U0[0, 1:4] ** 2).sum(0) / np.maximum(U0[0, 0], 1e-300) \
+        - (U[0, 5:8] ** 2).sum(0) / (8 * np.pi)
+    from_ent = (e0 * (g - 1) <= prob["prsfloor"]) & ~floored & (A["mask"] == 1)
+    assert from_ent.sum() >= 2
+    np.testing.assert_allclose(V[0, 4, from_ent], (A0["ent"][:nz] * U[0, 0] ** (g - 1))[from_ent], rtol=1e-12)
+    assert np.all(V[1:] == 0) and np.all(U[1:] == 0) and np.all(A["ent"][nz:] == 0)
+
+
+def test_cleanup_recovers_a_uniform_field():
+    """Face values n . B0 of a uniform field B0 on every face: the six-face
+    least squares returns B0 in each zone with an active column, and the
+    energy changes by exactly the magnetic energy difference."""
+    N, NR, g0 = 8, 3, 4
+    prob, A = orc.p1_cleanup_problem(N, NR, g0, sector=2, seed=5, floors=False)
+    d = orc.p1_dims(N, NR, g0)
+    nz, nf, nzs, nfr = d["zone"], N + 2 * g0 + 1, N + 2 * g0, NR + 2 * g0 + 1
+    B0 = np.array([0.31, -0.42, 0.17])
+    xn = A["xf_normal"].reshape(3, -1)
+    yn = A["yf_normal"].reshape(3, -1)
+    zc = A["zf_centroid"].reshape(3, -1)
+    zc = zc / np.maximum(np.sqrt((zc ** 2).sum(0)), 1e-300)
+    # face-B cube (0-g:N+g)^2 (0-g:nr+g); x-face (0-g:N+g, 1-g:N+g, 1-g:nr+g);
+    # y-face (1-g:N+g, 0-g:N+g, 1-g:nr+g); z-face (1-g:N+g, 1-g:N+g, 0-g:nr+g)
+    Bx = np.zeros((nfr, nf, nf)); By = np.zeros((nfr, nf, nf)); Bz = np.zeros((nfr, nf, nf))
+    Bx[1:, 1:, :] = (B0 @ xn).reshape(nfr - 1, nzs, nf)
+    By[1:, :, 1:] = (B0 @ yn).reshape(nfr - 1, nf, nzs)
+    Bz[:, 1:, 1:] = (B0 @ zc).reshape(nfr, nzs, nzs)
+    A["Bxf"], A["Byf"], A["Bzf"] = Bx.ravel(), By.ravel(), Bz.ravel()
+    U0 = A["U"].reshape(4, 8, nz)[0].copy()
+    e_in = U0[4] - 0.5 * (U0[1:4] ** 2).sum(0) / U0[0] - (U0[5:8] ** 2).sum(0) / (8 * np.pi)
+    orc.p1c_oracle(prob, A)
+    U = A["U"].reshape(4, 8, nz)[0]
+    col = np.tile(A["mask"].reshape(-1, nzs * nzs)[g0], NR + 2 * g0) == 1
+    # tri-corner synchronisation averages the re-pointed faces; zones away from
+    # those faces keep the exact field
+    ij = np.arange(nz) % (nzs * nzs)
+    ii, jj = ij % nzs - g0 + 1, ij // nzs - g0 + 1
+    inner = col & (ii >= 1) & (ii <= N) & (jj >= 1) & (jj <= N)
+    np.testing.assert_allclose(U[5:8, inner], B0[:, None] * np.ones(inner.sum()), rtol=0, atol=1e-13)
+    np.testing.assert_allclose(A["B1_zc"][:nz][inner], B0[0], rtol=0, atol=1e-13)
+    e_out = U[4] - 0.5 * (U[1:4] ** 2).sum(0) / U[0] - (U[5:8] ** 2).sum(0) / (8 * np.pi)
+    ok = inner & (e_in * (prob["gamma"] - 1) > prob["prsfloor"])
+    np.testing.assert_allclose(e_out[ok], e_in[ok], rtol=1e-12)
+
+
+def test_timestep_of_a_gas_at_rest():
+    """A uniform unmagnetised gas at rest: dt = cfl * min(scale) / c_s with
+    c_s = sqrt(gamma p / rho), unless 1.1 dt_old is smaller."""
+    prob, A = orc.p1_cleanup_problem(8, 3, 4, sector=0, seed=1, floors=False)
+    nz = orc.p1_dims(8, 3, 4)["zone"]
+    U = A["U"].reshape(4, 8, nz)
+    rho, prs, g = 1.7, 0.9, prob["gamma"]
+    U[0] = 0.0
+    U[0, 0] = rho
+    U[0, 4] = prs / (g - 1)
+    A["gamma_sound"][:] = g
+    cs_ = np.sqrt(g * prs / rho)
+    act = np.zeros(nz, bool)
+    zz = np.arange(nz)
+    nzs = 8 + 8
+    i, j, k = zz % nzs - 3, (zz // nzs) % nzs - 3, zz // (nzs * nzs) - 3
+    act = (A["mask"] == 1) & (i >= 1) & (i <= 8) & (j >= 1) & (j <= 8) & (k >= 1) & (k <= 3)
+    want = 0.4 * A["scale"][act].min() / cs_
+    dt = orc.p1_timestep_oracle(prob, A, 0.4, 10.0)
+    assert abs(dt - want) <= 1e-14 * want
+    assert orc.p1_timestep_oracle(prob, A, 0.4, 0.5 * want) == 1.1 * (0.5 * want)

@@ -40,18 +40,24 @@ def _run_gpu(prob, A):
     return out
 
 
-def _compare(G, O, rtol=1e-10):
+def _compare(G, O, rtol=1e-10, rows=None):
+    """Each array to rtol of its largest magnitude; with rows = {name: n}, the
+    array is split into n equal rows (components, modes) and each row is
+    scaled by its own largest magnitude."""
     bad = []
+    rows = rows or {}
     for k in O:
         g, o = G[k], O[k]
         if o.dtype.kind != "f":
             if not np.array_equal(g, o):
                 bad.append((k, "int mismatch", int((g != o).sum())))
             continue
-        scale = max(np.abs(o).max(), 1e-300)
-        err = np.abs(g - o).max()
-        if not err <= rtol * scale:
-            bad.append((k, float(err), float(scale), int((np.abs(g - o) > rtol * scale).sum())))
+        n = rows.get(k, 1)
+        for r, (gr, orow) in enumerate(zip(g.reshape(n, -1), o.reshape(n, -1))):
+            scale = max(np.abs(orow).max(), 1e-300)
+            err = np.abs(gr - orow).max()
+            if not err <= rtol * scale:
+                bad.append((k, r, float(err), float(scale), int((np.abs(gr - orow) > rtol * scale).sum())))
     return bad
 
 
@@ -132,3 +138,52 @@ def test_update_gpu_matches_oracle(case):
     bad = _compare(G, {k: O[k] for k in G}, rtol=1e-12)
     assert not bad, bad
     np.testing.assert_array_equal(G["flag_entprs"], O["flag_entprs"])
+
+
+CLEANUP_CASES = [(8, 3, 0, 0, 0.0, True), (12, 5, 1, 2, 1.0, True), (10, 4, 5, 3, 0.5, False),
+                 (24, 8, 2, 4, 0.3, True)]
+
+
+@pytest.mark.parametrize("case", CLEANUP_CASES, ids=lambda c: "N%d_nr%d_s%d_seed%d_noise%g_floors%d" % c)
+def test_cleanup_and_timestep_gpu_match_oracle(case):
+    """cs_cleanup_after_update_p1 against or_cleanup_after_update_p1 (bit-exact
+    with the reference's CLEANUP_AFTER_UPDATE) on a patch after the oracle's
+    reconstruction, fluxes and update, with floored zones: every array it
+    writes to 1e-12 of the largest magnitude of its component and mode (the
+    3 x 3 normal equations and CONS_TO_PRIM with fused multiply-adds), then cs_evaluate_timestep_p1 on the
+    cleaned patch: the time step to 1e-13 relative, U, V and msonic to 1e-12."""
+    N, NR, s, seed, noise, floors = case
+    print(f"\ncleanup case N={N} NR={NR} sector={s} seed={seed} floors={floors}", flush=True)
+    prob, A0 = orc.p1_cleanup_problem(N, NR, 4, sector=s, seed=seed, noise=noise, floors=floors)
+    O = orc.p1c_oracle(prob, orc.p1_copy(A0))
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(A0[k]).to(dev) for k in cs.P1_ARRAYS}
+    cs.cleanup_after_update_p1(N, NR, 4, prob, T)
+    torch.cuda.synchronize()
+    keys = ("U", "ent", "V", "Bxf", "Byf", "Bzf", "B1_zc", "B2_zc", "B3_zc", "scale", "gamma_eos", "gamma_sound",
+            "mean_mol_wt")
+    G = {k: T[k].cpu().numpy() for k in keys}
+    rows = dict(U=32, V=32, ent=4, B1_zc=4, B2_zc=4, B3_zc=4)
+    bad = _compare(G, {k: O[k] for k in keys}, rtol=1e-12, rows=rows)
+    assert not bad, bad
+    # the time step from the oracle's cleaned state on both sides
+    for dtcur in (1.0, 1e-18):
+        A1 = orc.p1_copy(O)
+        T = {k: torch.from_numpy(A1[k]).to(dev) for k in cs.P1_ARRAYS}
+        dt_g = float(cs.evaluate_timestep_p1(N, NR, 4, prob, T, 0.4, dtcur).item())
+        dt_o = orc.p1_timestep_oracle(prob, A1, 0.4, dtcur)
+        assert abs(dt_g - dt_o) <= 1e-13 * dt_o, (dt_g, dt_o)
+        G = {k: T[k].cpu().numpy() for k in ("U", "V", "msonic", "ent")}
+        bad = _compare(G, {k: A1[k] for k in G}, rtol=1e-12, rows=rows)
+        assert not bad, bad
+
+
+def test_timestep_gpu_rejects_bad_arguments():
+    prob, A = orc.p1_problem(8, 3, 4)
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(A[k]).to(dev) for k in cs.P1_ARRAYS}
+    with pytest.raises(cs.CsError):
+        cs.evaluate_timestep_p1(8, 3, 4, prob, T, 0.0, 1.0)
+    T["msonic"] = T["msonic"][:-1].contiguous()
+    with pytest.raises(cs.CsError):
+        cs.evaluate_timestep_p1(8, 3, 4, prob, T, 0.4, 1.0)
