@@ -8,6 +8,9 @@
 !   cs_limit_all_slopes_p1 <- LIMIT_ALL_SLOPES_P1 alone, argument for argument (DRV:2396)
 !   cs_make_facial_fluxes_p1 <- MAKE_FACIAL_FLUXES alone, argument for argument (DRV:2485)
 !   cs_make_edge_efields   <- MAKE_EDGE_EFIELDS_P1                          (DRV:2548)
+!   cs_update_imex_rk_step_p1 <- UPDATE_IMEX_RK_STEP_P1 alone, argument for argument (DRV:2782)
+!   cs_cleanup_after_update_p1 <- CLEANUP_AFTER_UPDATE alone, on the cs_p1_patch (DRV:2815)
+!   cs_evaluate_timestep_p1 <- EVALUATE_TIMESTEP alone, dt into device memory  (DRV:2887)
 !   cs_update_stage        <- UPDATE_IMEX_RK_STEP_P1 + CLEANUP_AFTER_UPDATE (DRV:2782, 2815)
 !   cs_evaluate_timestep   <- EVALUATE_TIMESTEP                             (DRV:2887)
 !   cs_rk_stage            <- the whole per-patch loop body                (DRV:2007-3120)
@@ -56,7 +59,35 @@ module cs_hip_iface
     type(c_ptr) :: dU_dt, EntFlux_x, EntFlux_y, EntFlux_z, Efx, Efy, Efz
   end type
 
+  ! cs_p1_update: the arguments of UPDATE_IMEX_RK_STEP_P1 (484update_imex_rk_step_p1.f:7-26)
+  type, bind(C) :: cs_p1_update
+    integer(c_int32_t) :: ntp, nr, nghost, pad
+    real(c_double) :: dtmove, gamma, rhofloor, prsfloor
+    type(c_ptr) :: vert, xf_area, yf_area, zf_area, v_volume
+    type(c_ptr) :: U, ent, Bxf, Byf, Bzf, dU_dt
+    type(c_ptr) :: EntFlux_x, EntFlux_y, EntFlux_z, Efx, Efy, Efz, flag_entprs
+  end type
+
   interface
+    integer(c_int) function cs_update_imex_rk_step_p1(p, stream) bind(C, name="cs_update_imex_rk_step_p1")
+      import :: c_int, c_ptr, cs_p1_update
+      type(cs_p1_update), intent(in) :: p
+      type(c_ptr), value :: stream
+    end function
+    integer(c_int) function cs_cleanup_after_update_p1(p, stream) bind(C, name="cs_cleanup_after_update_p1")
+      import :: c_int, c_ptr, cs_p1_patch
+      type(cs_p1_patch), intent(in) :: p
+      type(c_ptr), value :: stream
+    end function
+    ! dt_new: the device address of one double (c_loc of a device array under
+    ! host_data), written on the stream
+    integer(c_int) function cs_evaluate_timestep_p1(p, cfl, dtcur, dt_new, stream) &
+        bind(C, name="cs_evaluate_timestep_p1")
+      import :: c_int, c_ptr, c_double, cs_p1_patch
+      type(cs_p1_patch), intent(in) :: p
+      real(c_double), value :: cfl, dtcur
+      type(c_ptr), value :: dt_new, stream
+    end function
     integer(c_int) function cs_make_facial_fluxes_p1(p, stream) bind(C, name="cs_make_facial_fluxes_p1")
       import :: c_int, c_ptr, cs_p1_flux
       type(cs_p1_flux), intent(in) :: p

@@ -310,6 +310,32 @@ def test_update_zone_state():
     np.testing.assert_array_equal(A["U"].reshape(4, 8, NZ)[1:], A0["U"].reshape(4, 8, NZ)[1:])
     # mass: owned zones take U + dt dU (CONS_TO_PRIM leaves a positive density alone)
     np.testing.assert_allclose(U1[0, own], U0[0, own] + 1e-3 * dU[0, own], rtol=1e-15)
+    # entropy: with every owned zone's energy far below the floor, CONS_TO_PRIM
+    # takes the pressure from the updated entropy, so the output entropy is
+    # that update; summed with the volumes over the owned zones the interior
+    # faces cancel and only the patch boundary's entropy fluxes remain
+    dt = 1e-3
+    B0 = orc.p1_copy(A0)
+    B0["U"].reshape(4, 8, NZ)[0, 4, own] = -1e3
+    B = orc.p1u_oracle(prob, orc.p1_copy(B0))
+    assert np.all(B["flag_entprs"][own] == 1.0)
+    e0, e1 = B0["ent"].reshape(4, NZ)[0], B["ent"].reshape(4, NZ)[0]
+    nf, nfr = nz + 1, nzr + 1
+    F = {q: B0["EntFlux_" + q].reshape(nfr, nf, nf) for q in "xyz"}  # [k+g, j+g, i+g]
+    Ax = B0["xf_area"].reshape(nzr, nz, nf)  # [k-1+g, j-1+g, i+g]
+    Ay = B0["yf_area"].reshape(nzr, nf, nz)  # [k-1+g, j+g, i-1+g]
+    Az = B0["zf_area"].reshape(nfr, nz, nz)  # [k+g, j-1+g, i-1+g]
+    r = np.arange(1, N + 1)
+    kr = np.arange(1, NR + 1)
+    bx = sum(sgn * (Ax[np.ix_(kr - 1 + g, r - 1 + g, [i0 + g])] * F["x"][np.ix_(kr + g, r + g, [i0 + g])]).sum()
+             for sgn, i0 in ((1, N), (-1, 0)))
+    by = sum(sgn * (Ay[np.ix_(kr - 1 + g, [j0 + g], r - 1 + g)] * F["y"][np.ix_(kr + g, [j0 + g], r + g)]).sum()
+             for sgn, j0 in ((1, N), (-1, 0)))
+    bz = sum(sgn * (Az[np.ix_([k0 + g], r - 1 + g, r - 1 + g)] * F["z"][np.ix_([k0 + g], r + g, r + g)]).sum()
+             for sgn, k0 in ((1, NR), (-1, 0)))
+    lhs = (B0["v_volume"] * (e1 - e0))[own].sum()
+    scale = (B0["v_volume"] * np.abs(e1 - e0))[own].sum()
+    assert scale > 0 and abs(lhs + dt * (bx + by + bz)) <= 1e-12 * scale, (lhs, -dt * (bx + by + bz))
 
 
 CLEANUP_CASES = [
