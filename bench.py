@@ -405,6 +405,11 @@ def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None
 # z-face centroid 3, its 3 own face B, B_zc written 3, the mask 4 bytes.
 # EVALUATE_TIMESTEP per owned zone: U read and written 16, entropy 1, V written
 # 8, gamma_sound 1, scale 1, msonic written 1, the mask 4 bytes.
+# MAKE_EDGE_EFIELDS_P1 per owned zone (one edge of each family): the zone's
+# mode-1 E 3, the four edge points of its three own faces 12, its six face
+# speeds 6, its vertex 3; plus the three edge arrays written whole (zeros off
+# the owned edges), counted per entry.
+P1_EDGE_BYTES = 8 * (3 + 12 + 6 + 3)
 P1_CLEANUP_BYTES = 8 * (8 + 32 + 1 + 4 + 32 + 3 + 6 + 3 + 3 + 3) + 4
 P1_TIMESTEP_BYTES = 8 * (16 + 1 + 8 + 1 + 1 + 1) + 4
 P1_BYTES = {"weno+positivity": 8 * (8 + 1 + 1 + 3 + 1 + 12 + 24 + 3 + 12),
@@ -415,8 +420,8 @@ def bench_p1(cs, args):
     """LIMIT_ALL_SLOPES_P1 on one sector of the 6 x 192^2 x 64 shell as a
     patch (NGhost 4): owned-zone reconstructions per second, per-stage times;
     then the rest of the P1 stage on the same patch, each call timed:
-    MAKE_FACIAL_FLUXES, UPDATE_IMEX_RK_STEP_P1, CLEANUP_AFTER_UPDATE,
-    EVALUATE_TIMESTEP."""
+    MAKE_FACIAL_FLUXES, MAKE_EDGE_EFIELDS_P1, UPDATE_IMEX_RK_STEP_P1,
+    CLEANUP_AFTER_UPDATE, EVALUATE_TIMESTEP."""
     import torch
     N, NR, g = 192, 64, 4
     cfg, scal, T = p1_patch_on_gpu(cs, N, NR, g)
@@ -440,6 +445,7 @@ def bench_p1(cs, args):
         """the rest of one P1 stage after the reconstruction, each call timed"""
         out = []
         for fn in (lambda: cs.make_facial_fluxes_p1(N, NR, g, scal, T),
+                   lambda: cs.make_edge_efields_p1(N, NR, g, T),
                    lambda: cs.update_imex_rk_step_p1(N, NR, g, uscal, T),
                    lambda: cs.cleanup_after_update_p1(N, NR, g, scal, T),
                    lambda: cs.evaluate_timestep_p1(N, NR, g, scal, T, 0.4, 1.0, out=dt_out)):
@@ -456,7 +462,7 @@ def bench_p1(cs, args):
     torch.cuda.synchronize()
     steps = max(1, min(args.steps, 50))
     tot = 0.0
-    tchain = [0.0] * 4
+    tchain = [0.0] * 5
     stage = None
     for _ in range(steps):
         reset()
@@ -468,7 +474,7 @@ def bench_p1(cs, args):
         tchain = [a + b for a, b in zip(tchain, chain())]
         stage = ms if stage is None else [a + b for a, b in zip(stage, ms)]
     ms_call = tot / steps * 1e3
-    ms_flux, ms_upd, ms_clean, ms_dt = (t / steps * 1e3 for t in tchain)
+    ms_flux, ms_edge, ms_upd, ms_clean, ms_dt = (t / steps * 1e3 for t in tchain)
     stage = [s / steps for s in stage]
     own = N * N * NR
     cnt = cs.p1_counts(N, NR, g)
@@ -484,7 +490,9 @@ def bench_p1(cs, args):
         kern[name] = k
         log(f"   {name:20s} {t:8.3f} ms" + (f"  {k['GBps']:7.1f} GB/s algorithmic" if "GBps" in k else ""))
     rest = {}
+    nfc = cnt["Bxf"]
     for name, t, nb, items in (("MAKE_FACIAL_FLUXES", ms_flux, None, own),
+                               ("MAKE_EDGE_EFIELDS_P1", ms_edge, P1_EDGE_BYTES + 3 * 8 * nfc / own, own),
                                ("UPDATE_IMEX_RK_STEP_P1", ms_upd, None, own),
                                ("CLEANUP_AFTER_UPDATE", ms_clean, P1_CLEANUP_BYTES, cnt["mask"]),
                                ("EVALUATE_TIMESTEP", ms_dt, P1_TIMESTEP_BYTES, own)):

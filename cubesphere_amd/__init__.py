@@ -129,6 +129,7 @@ def lib():
             "cs_make_facial_fluxes_p1": [vp, vp],
             "cs_update_imex_rk_step_p1": [vp, vp],
             "cs_cleanup_after_update_p1": [vp, vp],
+            "cs_make_edge_efields_p1": [vp, vp],
             "cs_evaluate_timestep_p1": [vp, ctypes.c_double, ctypes.c_double, vp, vp],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
@@ -705,3 +706,40 @@ def evaluate_timestep_p1(N, NR, g, scalars, arrays, cfl, dtcur, out=None, stream
     _check(lib().cs_evaluate_timestep_p1(ctypes.byref(s), float(cfl), float(dtcur), ctypes.c_void_p(out.data_ptr()),
                                          st), "cs_evaluate_timestep_p1")
     return out
+
+
+# ---------------------------------------------------------------------------
+# MAKE_EDGE_EFIELDS_P1 of one patch (cs_make_edge_efields_p1)
+
+P1E_IN = ("vert", "mask", "Ex_zc", "Ey_zc", "Ez_zc", "Bxf_q", "Byf_q", "Bzf_q", "S_L", "S_R", "S_D", "S_U", "S_B",
+          "S_T")
+P1E_OUT = ("Efx", "Efy", "Efz")
+
+
+class P1Edge(ctypes.Structure):
+    """cs_p1_edge: the arguments of MAKE_EDGE_EFIELDS_P1 that it reads at
+    second order, for one patch."""
+    _fields_ = ([("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                 ("pad", ctypes.c_int32), ("itri", ctypes.c_int32 * 4)]
+                + [(k, ctypes.c_void_p) for k in P1E_IN + P1E_OUT])
+
+
+def make_edge_efields_p1(N, NR, g, arrays, itri=(1, 1, 1, 1), stream=None):
+    """MAKE_EDGE_EFIELDS_P1 on one patch of contiguous torch tensors on the GPU
+    in the reference's layouts (float64; int32 for mask); Efx, Efy, Efz are
+    written.  Sizes are checked first."""
+    import torch
+    want = p1f_counts(N, NR, g)
+    s = P1Edge()
+    s.ntp, s.nr, s.nghost = N, NR, g
+    for q in range(4):
+        s.itri[q] = int(itri[q])
+    for k in P1E_IN + P1E_OUT:
+        t = arrays[k]
+        dt = torch.int32 if k == "mask" else torch.float64
+        if not (t.is_cuda and t.dtype == dt and t.is_contiguous() and t.numel() == want[k]):
+            raise CsError(f"make_edge_efields_p1: {k} must be a contiguous {dt} CUDA tensor of {want[k]} "
+                          f"elements (got {t.dtype}, {t.numel()}, cuda={t.is_cuda})")
+        setattr(s, k, t.data_ptr())
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    _check(lib().cs_make_edge_efields_p1(ctypes.byref(s), st), "cs_make_edge_efields_p1")

@@ -2146,3 +2146,284 @@ extern "C" int cs_evaluate_timestep_p1(const cs_p1_patch* p, double cfl, double 
   P1_HIP(hipGetLastError());
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// MAKE_EDGE_EFIELDS_P1 (cubesphere/400riemann_integrate/460make_edge_fields_p1.f:5)
+// of one patch at second order, FP64.  k_edge: one work item per entry of the
+// face-B-shaped edge arrays, the family (x, y, z edges) in blockIdx.y; items
+// on an edge of the owned faces compute it, all others write the zero the
+// routine leaves there, so the arrays need no separate clearing.  An edge's
+// field: the mean mode-1 E of its four zones along the edge's tangent, the
+// face-point normal B of the four faces around it (LIMIT_ALL_SLOPES_P1) and
+// their wave speeds (MAKE_FACIAL_FLUXES) symmetrised to +-max|S|, and the
+// 4-state HLL of 630...f (csp::emf4_hll).  z-edges with an inactive zone keep
+// zero; at a flagged tri-corner the edge where the three active zones meet
+// adds the 3-state HLL of 631...f (emf3_hll below).
+namespace csp1 {
+
+struct P1E {
+  cs_p1_edge a;
+  int N, NR, g, nz, nf;
+  int64_t NZ, NFC;
+  __device__ __forceinline__ int64_t zi(int i, int j, int k) const {
+    return (int64_t)(i - 1 + g) + (int64_t)nz * ((j - 1 + g) + (int64_t)nz * (k - 1 + g));
+  }
+  __device__ __forceinline__ int64_t fci(int i, int j, int k) const {
+    return (int64_t)(i + g) + (int64_t)nf * ((j + g) + (int64_t)nf * (k + g));
+  }
+  __device__ __forceinline__ double vx(int i, int j, int k, int c) const { return a.vert[fci(i, j, k) + NFC * c]; }
+  __device__ __forceinline__ double emean(int c, const int64_t* z, int n) const {
+    const double* E = c == 0 ? a.Ex_zc : c == 1 ? a.Ey_zc : a.Ez_zc;
+    return n == 4 ? (E[z[0]] + E[z[1]] + E[z[2]] + E[z[3]]) * 0.25 : (E[z[0]] + E[z[1]] + E[z[2]]) * (1.0 / 3.0);
+  }
+};
+
+// Two_D_RS_Ez_3State_HLL (620riemann_2d/631...f:1-216): three zones around an
+// edge, the faces 13, 21, 32 between them at 120 degrees in the solver frame;
+// the HLL field from the wave triangle A, B, C of the six speeds.
+__device__ double emf3_hll(const double* sr, const double* sl, const double* ez, const double* bn) {
+  const double h3 = 0.5 * sqrt(3.0), r3 = 1.0 / sqrt(3.0);
+  const double o1 = 1.0 / (sr[0] - sl[0]), o2 = 1.0 / (sr[1] - sl[1]), o3 = 1.0 / (sr[2] - sl[2]);
+  const double Ax = (sl[0] + 2.0 * sr[1]) * r3, Ay = -sl[0], Bx = (sl[1] - sr[2]) * r3, By = sl[1] + sr[2];
+  const double Cx = -(sr[0] + 2.0 * sl[2]) * r3, Cy = -sr[0];
+  const double xc = (Ax + Bx + Cx) / 3.0, yc = (Ay + By + Cy) / 3.0;
+  const double lx[3] = {Ax - Cx, Bx - Ax, Cx - Bx}, ly[3] = {Ay - Cy, By - Ay, Cy - By};
+  const double px[3] = {Ax, Bx, Cx}, py[3] = {Ay, By, Cy};
+  const double mx[3] = {0.5 * (Ax + Cx), 0.5 * (Bx + Ax), 0.5 * (Cx + Bx)},
+               my[3] = {0.5 * (Ay + Cy), 0.5 * (By + Ay), 0.5 * (Cy + By)};
+  double l[3], nx[3], ny[3], Sn[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    l[q] = sqrt(lx[q] * lx[q] + ly[q] * ly[q]);
+    nx[q] = ly[q] / l[q];
+    ny[q] = -lx[q] / l[q];
+    Sn[q] = nx[q] * px[q] + ny[q] * py[q];
+  }
+  const double s = 0.5 * (l[0] + l[1] + l[2]);
+  const double area = sqrt(s * (s - l[0]) * (s - l[1]) * (s - l[2]));
+  // zone fields in the solver frame, their components along each face
+  const double Bx1 = (2.0 * bn[1] + bn[0]) * r3, Bx2 = (bn[1] - bn[2]) * r3, Bx3 = -(2.0 * bn[2] + bn[0]) * r3;
+  const double By1 = -bn[0], By2 = bn[1] + bn[2];
+  const double t21 = -0.5 * Bx1 + h3 * By1, t22 = -0.5 * Bx2 + h3 * By2;
+  const double t32 = -0.5 * Bx2 - h3 * By2, t33 = -0.5 * Bx3 - h3 * By1;
+  // the 1-D HLL state of each face: tangential B, E, the frame components
+  const double Bt13 = (sr[0] * Bx3 - sl[0] * Bx1 + ez[2] - ez[0]) * o1;
+  const double E13 = (sr[0] * ez[0] - sl[0] * ez[2] - sr[0] * sl[0] * (Bx3 - Bx1)) * o1;
+  const double Bt21 = (sr[1] * t21 - sl[1] * t22 + ez[0] - ez[1]) * o2;
+  const double E21 = (sr[1] * ez[1] - sl[1] * ez[0] - sr[1] * sl[1] * (t21 - t22)) * o2;
+  const double Bt32 = (sr[2] * t32 - sl[2] * t33 + ez[1] - ez[2]) * o3;
+  const double E32 = (sr[2] * ez[2] - sl[2] * ez[1] - sr[2] * sl[2] * (t32 - t33)) * o3;
+  const double Es[3] = {E13, E21, E32};
+  const double Bsx[3] = {Bt13, -0.5 * Bt21 + h3 * bn[1], -0.5 * Bt32 - h3 * bn[2]};
+  const double Bsy[3] = {-bn[0], h3 * Bt21 + 0.5 * bn[1], -h3 * Bt32 + 0.5 * bn[2]};
+  if (sl[0] >= 0.0 && sr[1] <= 0.0) return ez[0];
+  if (sl[1] >= 0.0 && sr[2] <= 0.0) return ez[1];
+  if (sl[2] >= 0.0 && sr[0] <= 0.0) return ez[2];
+  if (sl[0] * (sr[0] + 2.0 * sl[2]) + sr[0] * (sl[0] + 2.0 * sr[1]) <= 0.0) return E13;
+  if ((sl[1] + sr[2]) * (sl[0] + 2.0 * sr[1]) + (sl[1] - sr[2]) * sl[0] <= 0.0) return E21;
+  if ((sl[1] + sr[2]) * (sr[0] + 2.0 * sl[2]) - (sl[1] - sr[2]) * sr[0] <= 0.0) return E32;
+  // strongly interacting state: the face states' fluxes around the triangle
+  double sx = 0.0, sy = 0.0, mxs = 0.0, mys = 0.0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double fx = ny[q] * Es[q] - Sn[q] * Bsx[q], fy = -nx[q] * Es[q] - Sn[q] * Bsy[q];
+    sx += fx * l[q];
+    sy += fy * l[q];
+    mxs += (mx[q] - xc) * fy * l[q];
+    mys += (my[q] - yc) * fx * l[q];
+  }
+  const double BSSx = sx / (-2.0 * area), BSSy = sy / (-2.0 * area);
+  return 0.5 * ((-xc * BSSy - mxs / area) + (yc * BSSx + mys / area));
+}
+
+__global__ void __launch_bounds__(256) k_edge(const P1E P) {
+  const int fam = blockIdx.y;
+  const int64_t t = tid();
+  if (t >= P.NFC) return;
+  const cs_p1_edge& a = P.a;
+  const int n = P.nf, N = P.N, NR = P.NR;
+  const int i = (int)(t % n) - P.g, j = (int)((t / n) % n) - P.g, k = (int)(t / ((int64_t)n * n)) - P.g;
+  double* out = fam == 0 ? a.Efx : fam == 1 ? a.Efy : a.Efz;
+  const bool on = fam == 0   ? (i >= 1 && i <= N && j >= 0 && j <= N && k >= 0 && k <= NR)
+                  : fam == 1 ? (i >= 0 && i <= N && j >= 1 && j <= N && k >= 0 && k <= NR)
+                             : (i >= 0 && i <= N && j >= 0 && j <= N && k >= 1 && k <= NR);
+  double val = 0.0;
+  if (on) {
+    // zones LD, RD, RU, LU around the edge, and the edge's other vertex
+    int zc[4][3];
+    int pi = i, pj = j, pk = k;
+    if (fam == 0) {
+      const int q[4][3] = {{i, j, k}, {i, j + 1, k}, {i, j + 1, k + 1}, {i, j, k + 1}};
+      memcpy(zc, q, sizeof(zc));
+      pi = i - 1;
+    } else if (fam == 1) {
+      const int q[4][3] = {{i, j, k}, {i, j, k + 1}, {i + 1, j, k + 1}, {i + 1, j, k}};
+      memcpy(zc, q, sizeof(zc));
+      pj = j - 1;
+    } else {
+      const int q[4][3] = {{i, j, k}, {i + 1, j, k}, {i + 1, j + 1, k}, {i, j + 1, k}};
+      memcpy(zc, q, sizeof(zc));
+      pk = k - 1;
+    }
+    int64_t z[4], f[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      z[q] = P.zi(zc[q][0], zc[q][1], zc[q][2]);
+      f[q] = P.fci(zc[q][0], zc[q][1], zc[q][2]);
+    }
+    const bool active = fam < 2 || (a.mask[z[0]] && a.mask[z[1]] && a.mask[z[2]] && a.mask[z[3]]);
+    double ec[3], tg[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ec[c] = 0.5 * (P.vx(i, j, k, c) + P.vx(pi, pj, pk, c));
+    if (active) {
+      double ct = 0.0, st = 1.0;
+      if (fam < 2) {
+        // tangent: the chord, made normal to the radius at the edge centre
+        // (the centre lifted to the vertices' sphere)
+        const double rv = sqrt(P.vx(i, j, k, 0) * P.vx(i, j, k, 0) + P.vx(i, j, k, 1) * P.vx(i, j, k, 1) +
+                               P.vx(i, j, k, 2) * P.vx(i, j, k, 2));
+        const double lift = rv / sqrt(ec[0] * ec[0] + ec[1] * ec[1] + ec[2] * ec[2]);
+        double r[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] = ec[c] * lift;
+        const double ir = 1.0 / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          r[c] *= ir;
+          tg[c] = P.vx(i, j, k, c) - P.vx(pi, pj, pk, c);
+        }
+        const double d = tg[0] * r[0] + tg[1] * r[1] + tg[2] * r[2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) tg[c] -= d * r[c];
+        const double it = 1.0 / sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) tg[c] *= it;
+      } else {
+        // tangent: radial; the angle between the x and y lattice directions
+        const double ir = 1.0 / sqrt(ec[0] * ec[0] + ec[1] * ec[1] + ec[2] * ec[2]);
+        double n1[3], tau[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          tg[c] = ec[c] * ir;
+          n1[c] = P.vx(i + 1, j, k, c) - P.vx(i - 1, j, k, c);
+          tau[c] = P.vx(i, j + 1, k, c) - P.vx(i, j - 1, k, c);
+        }
+        const double d1 = n1[0] * tg[0] + n1[1] * tg[1] + n1[2] * tg[2];
+        const double d2 = tau[0] * tg[0] + tau[1] * tg[1] + tau[2] * tg[2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          n1[c] -= d1 * tg[c];
+          tau[c] -= d2 * tg[c];
+        }
+        const double i1 = 1.0 / sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
+        const double i2 = 1.0 / sqrt(tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]);
+        ct = (tau[0] * i2) * (n1[0] * i1) + (tau[1] * i2) * (n1[1] * i1) + (tau[2] * i2) * (n1[2] * i1);
+        st = sqrt(1.0 - ct * ct);
+      }
+      const double Et = P.emean(0, z, 4) * tg[0] + P.emean(1, z, 4) * tg[1] + P.emean(2, z, 4) * tg[2];
+      // face-point normal B (points 2..5 of the faces around the edge) and speeds
+      const double* byq = fam == 0 ? a.Bzf_q : fam == 1 ? a.Bxf_q : a.Byf_q;
+      const double* bnq = fam == 0 ? a.Byf_q : fam == 1 ? a.Bzf_q : a.Bxf_q;
+      const double By_R = byq[f[1] + 4 * P.NFC], By_L = byq[f[0] + 3 * P.NFC];
+      const double Bn_U = bnq[f[3] + 2 * P.NFC], Bn_D = bnq[f[0] + P.NFC];
+      const double *sHi1, *sLo1, *sHi2, *sLo2;
+      int a1, b1, a2, b2;  // zones whose speeds make S_R, S_L (a1, b1) and S_U, S_D (a2, b2)
+      if (fam == 0) {
+        sHi1 = a.S_U; sLo1 = a.S_D; sHi2 = a.S_T; sLo2 = a.S_B; a1 = 3; b1 = 0; a2 = 1; b2 = 0;
+      } else if (fam == 1) {
+        sHi1 = a.S_T; sLo1 = a.S_B; sHi2 = a.S_R; sLo2 = a.S_L; a1 = 3; b1 = 0; a2 = 0; b2 = 1;
+      } else {
+        sHi1 = a.S_R; sLo1 = a.S_L; sHi2 = a.S_U; sLo2 = a.S_D; a1 = 3; b1 = 0; a2 = 1; b2 = 0;
+      }
+      const double sR = fmax(sHi1[f[a1]], sHi1[f[b1]]), sL = fmin(sLo1[f[a1]], sLo1[f[b1]]);
+      const double sU = fmax(sHi2[f[a2]], sHi2[f[b2]]), sD = fmin(sLo2[f[a2]], sLo2[f[b2]]);
+      const double S = fmax(fmax(fabs(sR), fabs(sL)), fmax(fabs(sU), fabs(sD)));
+      val = emf4_hll<double>(S, -S, S, -S, ct, st, Et, Et, Et, Et, Bn_U, Bn_D, By_R, By_L);
+    }
+    if (fam == 2) {
+      const int corner = (i == 0 && j == 0) ? 0 : (i == N && j == 0) ? 1 : (i == N && j == N) ? 2
+                                                : (i == 0 && j == N) ? 3 : -1;
+      if (corner >= 0 && a.itri[corner]) {
+        // the three active zones 1, 2, 3 of the corner (460...f tri-corner blocks)
+        int q1[2], q2[2], q3[2];
+        if (corner == 0) {
+          q1[0] = i + 1; q1[1] = j + 1; q2[0] = i; q2[1] = j + 1; q3[0] = i + 1; q3[1] = j;
+        } else if (corner == 1) {
+          q1[0] = i + 1; q1[1] = j + 1; q2[0] = i; q2[1] = j + 1; q3[0] = i; q3[1] = j;
+        } else if (corner == 2) {
+          q1[0] = i; q1[1] = j + 1; q2[0] = i; q2[1] = j; q3[0] = i + 1; q3[1] = j;
+        } else {
+          q1[0] = i + 1; q1[1] = j + 1; q2[0] = i; q2[1] = j; q3[0] = i + 1; q3[1] = j;
+        }
+        const int64_t z3[3] = {P.zi(q1[0], q1[1], k), P.zi(q2[0], q2[1], k), P.zi(q3[0], q3[1], k)};
+        const double ir = 1.0 / sqrt(ec[0] * ec[0] + ec[1] * ec[1] + ec[2] * ec[2]);
+        const double Et = P.emean(0, z3, 3) * (ec[0] * ir) + P.emean(1, z3, 3) * (ec[1] * ir) +
+                          P.emean(2, z3, 3) * (ec[2] * ir);
+        const int64_t f1 = P.fci(q1[0], q1[1], k), f2 = P.fci(q2[0], q2[1], k), f3 = P.fci(q3[0], q3[1], k),
+                      f2d = P.fci(q2[0], q2[1] - 1, k), f1d = P.fci(q1[0], q1[1] - 1, k);
+        const int64_t NF = P.NFC;
+        double bn[3], sr[3], sl[3];
+        if (corner == 0) {
+          bn[0] = -a.Byf_q[f3 + 4 * NF]; bn[1] = a.Bxf_q[f2 + 2 * NF]; bn[2] = a.Byf_q[f2d + 3 * NF];
+          sr[0] = -a.S_D[f3]; sl[0] = -a.S_U[f3]; sr[1] = a.S_R[f2]; sl[1] = a.S_L[f2];
+          sr[2] = a.S_U[f2d]; sl[2] = a.S_D[f2d];
+        } else if (corner == 1) {
+          bn[0] = -a.Byf_q[f1d + 4 * NF]; bn[1] = a.Bxf_q[f2 + 2 * NF]; bn[2] = a.Byf_q[f3 + 3 * NF];
+          sr[0] = -a.S_L[f3]; sl[0] = -a.S_R[f3]; sr[1] = a.S_R[f2]; sl[1] = a.S_L[f2];
+          sr[2] = a.S_U[f3]; sl[2] = a.S_D[f3];
+        } else if (corner == 2) {
+          bn[0] = a.Bxf_q[f1 + 2 * NF]; bn[1] = a.Byf_q[f2 + 3 * NF]; bn[2] = -a.Bxf_q[f2 + NF];
+          sr[0] = a.S_R[f1]; sl[0] = a.S_L[f1]; sr[1] = a.S_U[f2]; sl[1] = a.S_D[f2];
+          sr[2] = -a.S_L[f2]; sl[2] = -a.S_R[f2];
+        } else {
+          bn[0] = -a.Byf_q[f3 + 4 * NF]; bn[1] = a.Byf_q[f2 + 3 * NF]; bn[2] = -a.Bxf_q[f2 + NF];
+          sr[0] = -a.S_D[f3]; sl[0] = -a.S_U[f3]; sr[1] = a.S_U[f2]; sl[1] = a.S_D[f2];
+          sr[2] = -a.S_L[f2]; sl[2] = -a.S_R[f2];
+        }
+        const double S = fmax(fmax(fmax(fabs(sr[0]), fabs(sl[0])), fmax(fabs(sr[1]), fabs(sl[1]))),
+                              fmax(fabs(sr[2]), fabs(sl[2])));
+        const double Sr[3] = {S, S, S}, Sl[3] = {-S, -S, -S}, E3[3] = {Et, Et, Et};
+        val += emf3_hll(Sr, Sl, E3, bn);
+      }
+    }
+  }
+  out[t] = val;
+}
+
+}  // namespace csp1
+
+extern "C" int cs_make_edge_efields_p1(const cs_p1_edge* p, void* stream_) {
+  if (!p) {
+    cs::set_error("cs_make_edge_efields_p1: null argument");
+    return CS_E_INVALID;
+  }
+  if (p->ntp < 1 || p->nr < 1 || p->nghost < 1) {
+    cs::set_error("cs_make_edge_efields_p1: need ntp, nr, nghost >= 1 (got %d, %d, %d)", p->ntp, p->nr,
+                  p->nghost);
+    return CS_E_INVALID;
+  }
+  if (p->nghost >= p->ntp) {
+    cs::set_error("cs_make_edge_efields_p1: nghost (%d) must be below ntp (%d)", p->nghost, p->ntp);
+    return CS_E_BAD_DECOMP;
+  }
+  const void* ptrs[] = {p->vert, p->mask, p->Ex_zc, p->Ey_zc, p->Ez_zc, p->Bxf_q, p->Byf_q, p->Bzf_q, p->S_L,
+                        p->S_R, p->S_D, p->S_U, p->S_B, p->S_T, p->Efx, p->Efy, p->Efz};
+  for (const void* q : ptrs)
+    if (!q) {
+      cs::set_error("cs_make_edge_efields_p1: null array");
+      return CS_E_INVALID;
+    }
+  hipStream_t st = (hipStream_t)stream_;
+  P1E P;
+  P.a = *p;
+  P.N = p->ntp;
+  P.NR = p->nr;
+  P.g = p->nghost;
+  P.nz = P.N + 2 * P.g;
+  P.nf = P.nz + 1;
+  P.NZ = (int64_t)P.nz * P.nz * (P.NR + 2 * P.g);
+  P.NFC = (int64_t)P.nf * P.nf * (P.NR + 2 * P.g + 1);
+  k_edge<<<dim3(grid_of(P.NFC, 256), 3), 256, 0, st>>>(P);
+  P1_HIP(hipGetLastError());
+  return 0;
+}

@@ -390,6 +390,31 @@ typedef struct cs_p1_update {
 } cs_p1_update;
 int cs_update_imex_rk_step_p1(const cs_p1_update* p, void* stream);
 
+/* ---- P1 edge electric fields of one patch ----------------------------------
+ * MAKE_EDGE_EFIELDS_P1 (400riemann_integrate/460make_edge_fields_p1.f:5-20) at
+ * second order (NUMORDER 2, USE_FACIAL_EFIELDS 0, no B0): for every edge of
+ * the owned faces (x-edges i 1..N, j 0..N, k 0..nr; y-edges i 0..N, j 1..N;
+ * z-edges i, j 0..N, k 1..nr) the mean mode-1 E of its four zones along the
+ * edge, the face-point normal B (Bxf_q, Byf_q, Bzf_q of
+ * cs_limit_all_slopes_p1) and the face wave speeds (S_* of
+ * cs_make_facial_fluxes_p1) of the four faces around it, symmetrised to
+ * +-max|S|, into Two_D_RS_Ez_4State_HLL (620riemann_2d/630...f:10); z-edges
+ * with an inactive zone get zero, and at a flagged tri-corner the edge of the
+ * three active zones adds Two_D_RS_Ez_3State_HLL (631...f:1).  Efx, Efy, Efz
+ * (face-B layout) are written over their whole arrays, zero off those edges.
+ * Device arrays in the layouts of cs_p1_patch.  No wait on the stream. */
+typedef struct cs_p1_edge {
+  int32_t ntp, nr, nghost, pad;
+  int32_t itri[4];                 /* itri_patch_LD, _RD, _RU, _LU */
+  const double* vert;
+  const int32_t* mask;
+  const double *Ex_zc, *Ey_zc, *Ez_zc;             /* zone E (mode 1 read) */
+  const double *Bxf_q, *Byf_q, *Bzf_q;             /* Bxf_quadpts ... (face-B layout, 5 points) */
+  const double *S_L, *S_R, *S_D, *S_U, *S_B, *S_T; /* S_L_gr ... S_T_gr */
+  double *Efx, *Efy, *Efz;
+} cs_p1_edge;
+int cs_make_edge_efields_p1(const cs_p1_edge* p, void* stream);
+
 /* ---- P1 cleanup and time step of one patch ---------------------------------
  * CLEANUP_AFTER_UPDATE (400riemann_integrate/485cleanup_after_update.f:1-25),
  * MHD, on the fields of cs_p1_patch: gamma_eos, gamma_sound, mean_mol_wt set;

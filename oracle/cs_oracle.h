@@ -166,4 +166,20 @@ int or_update_imex_rk_step_p1(const or_p1_update* u);
 int or_cleanup_after_update_p1(const or_p1_patch* p);
 int or_evaluate_timestep_p1(const or_p1_patch* p, double cfl, double dtcur, double* dtnew);
 
+/* MAKE_EDGE_EFIELDS_P1 (460make_edge_fields_p1.f:5) of one patch at second
+ * order, and its tri-corner solver Two_D_RS_Ez_3State_HLL (631...f:1); the
+ * same field order as cs_p1_edge in the product header */
+typedef struct or_p1_edge {
+  int32_t ntp, nr, nghost, pad;
+  int32_t itri[4];
+  const double* vert;
+  const int32_t* mask;
+  const double *Ex_zc, *Ey_zc, *Ez_zc;
+  const double *Bxf_q, *Byf_q, *Bzf_q;
+  const double *S_L, *S_R, *S_D, *S_U, *S_B, *S_T;
+  double *Efx, *Efy, *Efz;
+} or_p1_edge;
+int or_make_edge_efields_p1(const or_p1_edge* e);
+double or_emf3_hll(const double* sr, const double* sl, const double* ez, const double* bn);
+
 #endif

@@ -1221,3 +1221,276 @@ int or_evaluate_timestep_p1(const or_p1_patch* p, double cfl, double dtcur, doub
   *dtnew = dt;
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Two_D_RS_Ez_3State_HLL (620riemann_2d/631_2d_rs_ez_two_d_rs_ez_3state_hll.f:1-216):
+ * the 2-D HLL electric field where three zones meet, each pair across a face
+ * at 120 degrees in the solver's frame.  sr, sl: the speeds of the three
+ * faces (13, 21, 32); ez: the zones' E; bn: the faces' normal B.  The
+ * operations keep the routine's order so that results match it bit for bit. */
+double or_emf3_hll(const double* sr, const double* sl, const double* ez, const double* bn) {
+  const double SR1 = sr[0], SL1 = sl[0], SR2 = sr[1], SL2 = sl[1], SR3 = sr[2], SL3 = sl[2];
+  const double E1 = ez[0], E2 = ez[1], E3 = ez[2], Bn1 = bn[0], Bn2 = bn[1], Bn3 = bn[2];
+  const double h3 = sqrt(3.0) / 2.0, r3 = 1.0 / sqrt(3.0);
+  const double o1 = 1.0 / (SR1 - SL1), o2 = 1.0 / (SR2 - SL2), o3 = 1.0 / (SR3 - SL3);
+  /* corners A, B, C of the wave triangle, its centre, sides, outward normals */
+  const double Ax = (SL1 + 2.0 * SR2) * r3, Ay = -SL1, Bx = (SL2 - SR3) * r3, By = SL2 + SR3;
+  const double Cx = -((SR1 + 2.0 * SL3) * r3), Cy = -SR1;
+  const double xc = (Ax + Bx + Cx) / 3.0, yc = (Ay + By + Cy) / 3.0;
+  const double l1x = Ax - Cx, l2x = Bx - Ax, l3x = Cx - Bx, l1y = Ay - Cy, l2y = By - Ay, l3y = Cy - By;
+  const double l1 = sqrt(l1x * l1x + l1y * l1y), l2 = sqrt(l2x * l2x + l2y * l2y), l3 = sqrt(l3x * l3x + l3y * l3y);
+  const double n13x = l1y / l1, n13y = -(l1x / l1), n21x = l2y / l2, n21y = -(l2x / l2), n32x = l3y / l3,
+               n32y = -(l3x / l3);
+  const double S13 = n13x * Ax + n13y * Ay, S21 = n21x * Bx + n21y * By, S32 = n32x * Cx + n32y * Cy;
+  const double xb13 = 0.5 * (Ax + Cx), yb13 = 0.5 * (Ay + Cy), xb21 = 0.5 * (Bx + Ax), yb21 = 0.5 * (By + Ay);
+  const double xb32 = 0.5 * (Cx + Bx), yb32 = 0.5 * (Cy + By);
+  const double s = (l1 + l2 + l3) / 2.0, area = sqrt(s * (s - l1) * (s - l2) * (s - l3));
+  /* the three zones' field in the solver frame and along each face */
+  const double Bx1 = (2.0 * Bn2 + Bn1) * r3, Bx2 = (Bn2 - Bn3) * r3, Bx3 = -((2.0 * Bn3 + Bn1) * r3);
+  const double By1 = -Bn1, By2 = Bn2 + Bn3;
+  const double t1U1 = Bx1, t1U3 = Bx3, t2U1 = -0.5 * Bx1 + h3 * By1, t2U2 = -0.5 * Bx2 + h3 * By2;
+  const double t3U2 = -0.5 * Bx2 - h3 * By2, t3U3 = -0.5 * Bx3 - h3 * By1;
+  /* the 1-D HLL state of each face */
+  const double Bt13 = (SR1 * t1U3 - SL1 * t1U1 + E3 - E1) * o1;
+  const double Ez13 = (SR1 * E1 - SL1 * E3 - SR1 * SL1 * (t1U3 - t1U1)) * o1;
+  const double Bx13 = Bt13, By13 = -Bn1;
+  const double Bt21 = (SR2 * t2U1 - SL2 * t2U2 + E1 - E2) * o2;
+  const double Ez21 = (SR2 * E2 - SL2 * E1 - SR2 * SL2 * (t2U1 - t2U2)) * o2;
+  const double Bx21 = -0.5 * Bt21 + h3 * Bn2, By21 = h3 * Bt21 + 0.5 * Bn2;
+  const double Bt32 = (SR3 * t3U2 - SL3 * t3U3 + E2 - E3) * o3;
+  const double Ez32 = (SR3 * E3 - SL3 * E2 - SR3 * SL3 * (t3U2 - t3U3)) * o3;
+  const double Bx32 = -0.5 * Bt32 - h3 * Bn3, By32 = -(h3 * Bt32) + 0.5 * Bn3;
+  if (SL1 >= 0.0 && SR2 <= 0.0) return E1;
+  if (SL2 >= 0.0 && SR3 <= 0.0) return E2;
+  if (SL3 >= 0.0 && SR1 <= 0.0) return E3;
+  if (SL1 * (SR1 + 2.0 * SL3) + SR1 * (SL1 + 2.0 * SR2) <= 0) return Ez13;
+  if ((SL2 + SR3) * (SL1 + 2.0 * SR2) + (SL2 - SR3) * SL1 <= 0) return Ez21;
+  if ((SL2 + SR3) * (SR1 + 2.0 * SL3) - (SL2 - SR3) * SR1 <= 0) return Ez32;
+  /* the strongly interacting state: the face states' fluxes around the triangle */
+  const double fx13 = n13y * Ez13 - S13 * Bx13, fx21 = n21y * Ez21 - S21 * Bx21, fx32 = n32y * Ez32 - S32 * Bx32;
+  const double fy13 = -(n13x * Ez13) - S13 * By13, fy21 = -(n21x * Ez21) - S21 * By21,
+               fy32 = -(n32x * Ez32) - S32 * By32;
+  const double BSSx = (fx13 * l1 + fx21 * l2 + fx32 * l3) / (-2.0 * area);
+  const double BSSy = (fy13 * l1 + fy21 * l2 + fy32 * l3) / (-2.0 * area);
+  const double e1 = -(xc * BSSy) -
+                    ((xb13 - xc) * fy13 * l1 + (xb21 - xc) * fy21 * l2 + (xb32 - xc) * fy32 * l3) / area;
+  const double e2 = yc * BSSx + ((yb13 - yc) * fx13 * l1 + (yb21 - yc) * fx21 * l2 + (yb32 - yc) * fx32 * l3) / area;
+  return 0.5 * (e1 + e2);
+}
+
+/* MAKE_EDGE_EFIELDS_P1 (400riemann_integrate/460make_edge_fields_p1.f:5) at
+ * second order (NUMORDER 2, no facial electric fields, no B0): every edge of
+ * the owned faces takes the mean E of its four zones along the edge's
+ * tangent and the 2-D HLL field of Two_D_RS_Ez_4State_HLL (630) with the
+ * face-point normal B of the four faces around it (from the reconstruction)
+ * and their wave speeds (from the facial fluxes), all four symmetrised to
+ * +-max|S|; z-edges with an inactive zone are skipped, and at the tri-corners
+ * the three zones' edge takes the 3-state HLL (631).  Efx, Efy, Efz are set
+ * to zero first over their whole arrays. */
+int or_make_edge_efields_p1(const or_p1_edge* e) {
+  if (!e || e->nghost < 2 || e->ntp < 1 || e->nr < 1) return 1;
+  or_p1_patch pp;
+  memset(&pp, 0, sizeof(pp));
+  pp.ntp = e->ntp;
+  pp.nr = e->nr;
+  pp.nghost = e->nghost;
+  const p1dims D = p1_dims(&pp), *d = &D;
+  const int N = d->N, NR = d->NR;
+  const long NFC = d->NFC, NZ = d->NZ;
+  long q;
+  int i, j, k, c;
+  for (q = 0; q < NFC; ++q) {
+    e->Efx[q] = 0.0;
+    e->Efy[q] = 0.0;
+    e->Efz[q] = 0.0;
+  }
+#define EV(ii, jj, kk, cc) e->vert[fci(d, ii, jj, kk) + NFC * (cc)]
+#define BQ(a, ii, jj, kk, pt) e->a[fci(d, ii, jj, kk) + NFC * ((pt)-1)]
+#define SG(a, ii, jj, kk) e->a[fci(d, ii, jj, kk)]
+  /* the mean E of four zones (listed LD, RD, RU, LU) along t */
+  double Eq[4], ez4[4], s4[4], bn2[2], by2[2], t3[3], n[3], ec[3], a, b;
+  long zq[4];
+  for (int fam = 0; fam < 3; ++fam) {
+    const int i0 = fam == 0 ? 1 : 0, j0 = fam == 1 ? 1 : 0, k0 = fam == 2 ? 1 : 0;
+    for (k = k0; k <= NR; ++k)
+      for (j = j0; j <= N; ++j)
+        for (i = i0; i <= N; ++i) {
+          int zz[4][3];
+          if (fam == 0) { /* LD (i,j,k) RD (i,j+1,k) RU (i,j+1,k+1) LU (i,j,k+1) */
+            int t[4][3] = {{i, j, k}, {i, j + 1, k}, {i, j + 1, k + 1}, {i, j, k + 1}};
+            memcpy(zz, t, sizeof(zz));
+          } else if (fam == 1) { /* LD (i,j,k) RD (i,j,k+1) RU (i+1,j,k+1) LU (i+1,j,k) */
+            int t[4][3] = {{i, j, k}, {i, j, k + 1}, {i + 1, j, k + 1}, {i + 1, j, k}};
+            memcpy(zz, t, sizeof(zz));
+          } else { /* LD (i,j,k) RD (i+1,j,k) RU (i+1,j+1,k) LU (i,j+1,k) */
+            int t[4][3] = {{i, j, k}, {i + 1, j, k}, {i + 1, j + 1, k}, {i, j + 1, k}};
+            memcpy(zz, t, sizeof(zz));
+          }
+          for (c = 0; c < 4; ++c) zq[c] = zi(d, zz[c][0], zz[c][1], zz[c][2]);
+          if (fam == 2 && (e->mask[zq[0]] == 0 || e->mask[zq[1]] == 0 || e->mask[zq[2]] == 0 || e->mask[zq[3]] == 0))
+            continue;
+          /* edge centre, the radial direction there, the tangent */
+          const int pi_ = fam == 0 ? i - 1 : i, pj = fam == 1 ? j - 1 : j, pk = fam == 2 ? k - 1 : k;
+          for (c = 0; c < 3; ++c) ec[c] = 0.5 * (EV(i, j, k, c) + EV(pi_, pj, pk, c));
+          double ct = 0.0, st = 1.0;
+          if (fam < 2) {
+            a = sqrt(EV(i, j, k, 0) * EV(i, j, k, 0) + EV(i, j, k, 1) * EV(i, j, k, 1) + EV(i, j, k, 2) * EV(i, j, k, 2)) /
+                sqrt(ec[0] * ec[0] + ec[1] * ec[1] + ec[2] * ec[2]);
+            for (c = 0; c < 3; ++c) ec[c] = ec[c] * a;
+            a = 1.0 / sqrt(ec[0] * ec[0] + ec[1] * ec[1] + ec[2] * ec[2]);
+            for (c = 0; c < 3; ++c) n[c] = ec[c] * a;
+            for (c = 0; c < 3; ++c) t3[c] = EV(i, j, k, c) - EV(pi_, pj, pk, c);
+            a = t3[0] * n[0] + t3[1] * n[1] + t3[2] * n[2];
+            for (c = 0; c < 3; ++c) t3[c] = t3[c] - a * n[c];
+            a = 1.0 / sqrt(t3[0] * t3[0] + t3[1] * t3[1] + t3[2] * t3[2]);
+            for (c = 0; c < 3; ++c) t3[c] = t3[c] * a;
+          } else {
+            double n1[3], tau[3];
+            a = 1.0 / sqrt(ec[0] * ec[0] + ec[1] * ec[1] + ec[2] * ec[2]);
+            for (c = 0; c < 3; ++c) t3[c] = ec[c] * a;
+            for (c = 0; c < 3; ++c) n1[c] = EV(i + 1, j, k, c) - EV(i - 1, j, k, c);
+            a = n1[0] * t3[0] + n1[1] * t3[1] + n1[2] * t3[2];
+            for (c = 0; c < 3; ++c) n1[c] = n1[c] - a * t3[c];
+            a = 1.0 / sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
+            for (c = 0; c < 3; ++c) n1[c] = n1[c] * a;
+            for (c = 0; c < 3; ++c) tau[c] = EV(i, j + 1, k, c) - EV(i, j - 1, k, c);
+            a = tau[0] * t3[0] + tau[1] * t3[1] + tau[2] * t3[2];
+            for (c = 0; c < 3; ++c) tau[c] = tau[c] - a * t3[c];
+            a = 1.0 / sqrt(tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]);
+            for (c = 0; c < 3; ++c) tau[c] = tau[c] * a;
+            ct = tau[0] * n1[0] + tau[1] * n1[1] + tau[2] * n1[2];
+            st = sqrt(1.0 - ct * ct);
+          }
+          const double* Ec[3] = {e->Ex_zc, e->Ey_zc, e->Ez_zc};
+          for (c = 0; c < 3; ++c) Eq[c] = (Ec[c][zq[0]] + Ec[c][zq[1]] + Ec[c][zq[2]] + Ec[c][zq[3]]) * 0.25;
+          b = Eq[0] * t3[0] + Eq[1] * t3[1] + Eq[2] * t3[2];
+          ez4[0] = b; /* RU */
+          ez4[1] = b; /* LU */
+          ez4[2] = b; /* LD */
+          ez4[3] = b; /* RD */
+          double sR, sL, sU, sD;
+          long fz[4];
+          for (c = 0; c < 4; ++c) fz[c] = fci(d, zz[c][0], zz[c][1], zz[c][2]);
+          if (fam == 0) {
+            by2[0] = e->Bzf_q[fz[1] + NFC * 4];
+            by2[1] = e->Bzf_q[fz[0] + NFC * 3];
+            bn2[0] = e->Byf_q[fz[3] + NFC * 2];
+            bn2[1] = e->Byf_q[fz[0] + NFC * 1];
+            sR = omax(e->S_U[fz[3]], e->S_U[fz[0]]);
+            sL = omin(e->S_D[fz[3]], e->S_D[fz[0]]);
+            sU = omax(e->S_T[fz[1]], e->S_T[fz[0]]);
+            sD = omin(e->S_B[fz[1]], e->S_B[fz[0]]);
+          } else if (fam == 1) {
+            by2[0] = e->Bxf_q[fz[1] + NFC * 4];
+            by2[1] = e->Bxf_q[fz[0] + NFC * 3];
+            bn2[0] = e->Bzf_q[fz[3] + NFC * 2];
+            bn2[1] = e->Bzf_q[fz[0] + NFC * 1];
+            sR = omax(e->S_T[fz[3]], e->S_T[fz[0]]);
+            sL = omin(e->S_B[fz[3]], e->S_B[fz[0]]);
+            sU = omax(e->S_R[fz[0]], e->S_R[fz[1]]);
+            sD = omin(e->S_L[fz[0]], e->S_L[fz[1]]);
+          } else {
+            by2[0] = e->Byf_q[fz[1] + NFC * 4];
+            by2[1] = e->Byf_q[fz[0] + NFC * 3];
+            bn2[0] = e->Bxf_q[fz[3] + NFC * 2];
+            bn2[1] = e->Bxf_q[fz[0] + NFC * 1];
+            sR = omax(e->S_R[fz[3]], e->S_R[fz[0]]);
+            sL = omin(e->S_L[fz[3]], e->S_L[fz[0]]);
+            sU = omax(e->S_U[fz[1]], e->S_U[fz[0]]);
+            sD = omin(e->S_D[fz[1]], e->S_D[fz[0]]);
+          }
+          a = omax(omax(fabs(sR), fabs(sL)), omax(fabs(sU), fabs(sD)));
+          s4[0] = a;
+          s4[1] = -a;
+          s4[2] = a;
+          s4[3] = -a;
+          double* out = fam == 0 ? e->Efx : fam == 1 ? e->Efy : e->Efz;
+          q = fci(d, i, j, k);
+          out[q] = out[q] + 1.0 * or_emf4_hll(s4, ct, st, ez4, bn2, by2);
+        }
+  }
+  /* tri-corners: the edge where the three active zones meet */
+  for (int corner = 0; corner < 4; ++corner) {
+    if (!e->itri[corner]) continue;
+    /* corner vertex (ixx, iyy); zones 1, 2, 3; per face: the field (array,
+     * zone, point, sign) and the speeds (right and left array, zone, sign) */
+    const int ix = (corner == 1 || corner == 2) ? N : 0, iy = (corner >= 2) ? N : 0;
+    int z1[2], z2[2], z3[2];
+    if (corner == 0) {
+      z1[0] = ix + 1; z1[1] = iy + 1; z2[0] = ix; z2[1] = iy + 1; z3[0] = ix + 1; z3[1] = iy;
+    } else if (corner == 1) {
+      z1[0] = ix + 1; z1[1] = iy + 1; z2[0] = ix; z2[1] = iy + 1; z3[0] = ix; z3[1] = iy;
+    } else if (corner == 2) {
+      z1[0] = ix; z1[1] = iy + 1; z2[0] = ix; z2[1] = iy; z3[0] = ix + 1; z3[1] = iy;
+    } else {
+      z1[0] = ix + 1; z1[1] = iy + 1; z2[0] = ix; z2[1] = iy; z3[0] = ix + 1; z3[1] = iy;
+    }
+    for (k = 1; k <= NR; ++k) {
+      double sr[3], sl[3], ez3[3], bn3[3];
+      for (c = 0; c < 3; ++c) ec[c] = 0.5 * (EV(ix, iy, k, c) + EV(ix, iy, k - 1, c));
+      a = 1.0 / sqrt(ec[0] * ec[0] + ec[1] * ec[1] + ec[2] * ec[2]);
+      for (c = 0; c < 3; ++c) t3[c] = ec[c] * a;
+      const long q1 = zi(d, z1[0], z1[1], k), q2 = zi(d, z2[0], z2[1], k), q3 = zi(d, z3[0], z3[1], k);
+      const double* Ec[3] = {e->Ex_zc, e->Ey_zc, e->Ez_zc};
+      for (c = 0; c < 3; ++c) Eq[c] = (Ec[c][q1] + Ec[c][q2] + Ec[c][q3]) * (1.0 / 3.0);
+      b = Eq[0] * t3[0] + Eq[1] * t3[1] + Eq[2] * t3[2];
+      ez3[0] = b;
+      ez3[1] = b;
+      ez3[2] = b;
+      if (corner == 0) { /* 460...f LD block */
+        bn3[0] = -BQ(Byf_q, z3[0], z3[1], k, 5);
+        bn3[1] = BQ(Bxf_q, z2[0], z2[1], k, 3);
+        bn3[2] = BQ(Byf_q, z2[0], z2[1] - 1, k, 4);
+        sr[0] = -SG(S_D, z3[0], z3[1], k);
+        sl[0] = -SG(S_U, z3[0], z3[1], k);
+        sr[1] = SG(S_R, z2[0], z2[1], k);
+        sl[1] = SG(S_L, z2[0], z2[1], k);
+        sr[2] = SG(S_U, z2[0], z2[1] - 1, k);
+        sl[2] = SG(S_D, z2[0], z2[1] - 1, k);
+      } else if (corner == 1) { /* RD block */
+        bn3[0] = -BQ(Byf_q, z1[0], z1[1] - 1, k, 5);
+        bn3[1] = BQ(Bxf_q, z2[0], z2[1], k, 3);
+        bn3[2] = BQ(Byf_q, z3[0], z3[1], k, 4);
+        sr[0] = -SG(S_L, z3[0], z3[1], k);
+        sl[0] = -SG(S_R, z3[0], z3[1], k);
+        sr[1] = SG(S_R, z2[0], z2[1], k);
+        sl[1] = SG(S_L, z2[0], z2[1], k);
+        sr[2] = SG(S_U, z3[0], z3[1], k);
+        sl[2] = SG(S_D, z3[0], z3[1], k);
+      } else if (corner == 2) { /* RU block */
+        bn3[0] = BQ(Bxf_q, z1[0], z1[1], k, 3);
+        bn3[1] = BQ(Byf_q, z2[0], z2[1], k, 4);
+        bn3[2] = -BQ(Bxf_q, z2[0], z2[1], k, 2);
+        sr[0] = SG(S_R, z1[0], z1[1], k);
+        sl[0] = SG(S_L, z1[0], z1[1], k);
+        sr[1] = SG(S_U, z2[0], z2[1], k);
+        sl[1] = SG(S_D, z2[0], z2[1], k);
+        sr[2] = -SG(S_L, z2[0], z2[1], k);
+        sl[2] = -SG(S_R, z2[0], z2[1], k);
+      } else { /* LU block */
+        bn3[0] = -BQ(Byf_q, z3[0], z3[1], k, 5);
+        bn3[1] = BQ(Byf_q, z2[0], z2[1], k, 4);
+        bn3[2] = -BQ(Bxf_q, z2[0], z2[1], k, 2);
+        sr[0] = -SG(S_D, z3[0], z3[1], k);
+        sl[0] = -SG(S_U, z3[0], z3[1], k);
+        sr[1] = SG(S_U, z2[0], z2[1], k);
+        sl[1] = SG(S_D, z2[0], z2[1], k);
+        sr[2] = -SG(S_L, z2[0], z2[1], k);
+        sl[2] = -SG(S_R, z2[0], z2[1], k);
+      }
+      a = omax(omax(omax(fabs(sr[0]), fabs(sl[0])), omax(fabs(sr[1]), fabs(sl[1]))),
+               omax(fabs(sr[2]), fabs(sl[2])));
+      for (c = 0; c < 3; ++c) {
+        sr[c] = a;
+        sl[c] = -a;
+      }
+      q = fci(d, ix, iy, k);
+      e->Efz[q] = e->Efz[q] + 1.0 * or_emf3_hll(sr, sl, ez3, bn3);
+    }
+  }
+#undef EV
+#undef BQ
+#undef SG
+  (void)NZ;
+  return 0;
+}

@@ -545,3 +545,60 @@ def p1_timestep_reference(prob, A, cfl, dtcur):
     out = ctypes.c_double(0.0)
     R.ref_evaluate_timestep_p1(ctypes.byref(s), cfl, dtcur, ctypes.byref(out))
     return out.value
+
+
+# ---------------------------------------------------------------------------
+# MAKE_EDGE_EFIELDS_P1 of one patch (or_make_edge_efields_p1 /
+# cubesphere_amd.make_edge_efields_p1 share this structure layout)
+
+P1E_IN = ("vert", "mask", "Ex_zc", "Ey_zc", "Ez_zc", "Bxf_q", "Byf_q", "Bzf_q", "S_L", "S_R", "S_D", "S_U", "S_B",
+          "S_T")
+P1E_OUT = ("Efx", "Efy", "Efz")
+
+
+class P1Edge(ctypes.Structure):
+    _fields_ = ([("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                 ("pad", ctypes.c_int32), ("itri", ctypes.c_int32 * 4)]
+                + [(k, ctypes.c_void_p) for k in P1E_IN + P1E_OUT])
+
+
+def p1e_struct(prob, A, itri=(1, 1, 1, 1)):
+    s = P1Edge()
+    s.ntp, s.nr, s.nghost = prob["N"], prob["NR"], prob["g"]
+    for q in range(4):
+        s.itri[q] = int(itri[q])
+    for k in P1E_IN + P1E_OUT:
+        setattr(s, k, A[k].ctypes.data)
+    return s
+
+
+def p1_edge_problem(N, NR, g=4, sector=0, seed=0, shock=True, noise=0.0):
+    """A patch after the oracle's LIMIT_ALL_SLOPES_P1 (zone E, face-point B)
+    and MAKE_FACIAL_FLUXES (the face wave speeds); the edge arrays hold junk
+    that the routine must clear."""
+    prob, A = p1_flux_problem(N, NR, g, sector=sector, seed=seed, shock=shock, noise=noise)
+    p1f_oracle(prob, A)
+    rng = np.random.default_rng(seed + 400)
+    for k in P1E_OUT:
+        A[k] = rng.normal(size=A[k].size)
+    return prob, A
+
+
+def p1e_oracle(prob, A, itri=(1, 1, 1, 1)):
+    L = olib()
+    L.or_make_edge_efields_p1.argtypes = [ctypes.c_void_p]
+    s = p1e_struct(prob, A, itri)
+    if L.or_make_edge_efields_p1(ctypes.byref(s)) != 0:
+        raise RuntimeError("oracle rejected the patch")
+    return A
+
+
+def p1e_reference(prob, A, itri=(1, 1, 1, 1)):
+    """The reference's MAKE_EDGE_EFIELDS_P1 (flang build) in place on A."""
+    R = rlib()
+    if R is None:
+        raise RuntimeError("reference build not present")
+    R.ref_make_edge_efields_p1.argtypes = [ctypes.c_void_p]
+    s = p1e_struct(prob, A, itri)
+    R.ref_make_edge_efields_p1(ctypes.byref(s))
+    return A

@@ -17,6 +17,14 @@ module cs_ref_bind
     type(c_ptr) :: gamma_eos, gamma_sound, mean_mol_wt, divvel, msonic, flatten, work
     type(c_ptr) :: Bxf_q, Byf_q, Bzf_q, V_min_local, V_max_local
   end type
+
+  ! mirror of or_p1_edge / cs_p1_edge
+  type, bind(C) :: ref_p1_edge
+    integer(c_int32_t) :: ntp, nr, nghost, pad
+    integer(c_int32_t) :: itri(4)
+    type(c_ptr) :: vert, mask, Ex_zc, Ey_zc, Ez_zc, Bxf_q, Byf_q, Bzf_q
+    type(c_ptr) :: S_L, S_R, S_D, S_U, S_B, S_T, Efx, Efy, Efz
+  end type
 contains
 
   ! Mesh_Sector_General (201mesh_sector_general.f:79) for one patch per sector,
@@ -377,6 +385,48 @@ contains
     t = 0
     call EVALUATE_TIMESTEP(nr, n, g, 8, 4, 1, t, dtcur, dtnew, vert, mask, scl, cen, &
          u, ent, bxf, byf, bzf, v, b1, b2, b3, geos, gsnd, mmw, divv, msn, flat)
+  end subroutine
+
+  ! MAKE_EDGE_EFIELDS_P1 (460make_edge_fields_p1.f:5) on the fields of a
+  ! ref_p1_edge; the arguments the routine declares but does not read at second
+  ! order are zero-filled arrays of their declared sizes
+  subroutine ref_make_edge_efields_p1(p) bind(C, name="ref_make_edge_efields_p1")
+    type(ref_p1_edge), intent(in) :: p
+    integer :: n, nr, g, nz, nzr, nf, nfr, nzone, nxf, nzf, nfc
+    integer :: md(4)
+    real(c_double), pointer :: vert(:), ex(:), ey(:), ez(:), bxq(:), byq(:), bzq(:)
+    real(c_double), pointer :: sl(:), sr(:), sd(:), su(:), sb(:), st_(:), efx(:), efy(:), efz(:)
+    integer(c_int), pointer :: mask(:)
+    real(c_double), allocatable :: fq(:), fw(:), fn(:), fa(:), fc(:), zq(:), zw(:), zn(:), za(:), zc(:)
+    real(c_double), allocatable :: zone1(:), zone3(:), zone4(:), zone32(:), ent(:), bf(:)
+    integer(c_int), allocatable :: st(:)
+    external MAKE_EDGE_EFIELDS_P1, REF_INIT_COMMON
+    n = p%ntp; nr = p%nr; g = p%nghost
+    nz = n + 2 * g; nzr = nr + 2 * g; nf = nz + 1; nfr = nzr + 1
+    nzone = nz * nz * nzr; nxf = nf * nz * nzr; nzf = nz * nz * nfr; nfc = nf * nf * nfr
+    md = (/ 0, 1, 0, 0 /)
+    call REF_INIT_COMMON()
+    call c_f_pointer(p%vert, vert, [3 * nfc]); call c_f_pointer(p%mask, mask, [nzone])
+    call c_f_pointer(p%Ex_zc, ex, [4 * nzone]); call c_f_pointer(p%Ey_zc, ey, [4 * nzone])
+    call c_f_pointer(p%Ez_zc, ez, [4 * nzone])
+    call c_f_pointer(p%Bxf_q, bxq, [5 * nfc]); call c_f_pointer(p%Byf_q, byq, [5 * nfc])
+    call c_f_pointer(p%Bzf_q, bzq, [5 * nfc])
+    call c_f_pointer(p%S_L, sl, [nfc]); call c_f_pointer(p%S_R, sr, [nfc]); call c_f_pointer(p%S_D, sd, [nfc])
+    call c_f_pointer(p%S_U, su, [nfc]); call c_f_pointer(p%S_B, sb, [nfc]); call c_f_pointer(p%S_T, st_, [nfc])
+    call c_f_pointer(p%Efx, efx, [nfc]); call c_f_pointer(p%Efy, efy, [nfc]); call c_f_pointer(p%Efz, efz, [nfc])
+    allocate(fq(15 * nxf), fw(5 * nxf), fn(3 * nxf), fa(nxf), fc(3 * nxf))
+    allocate(zq(15 * nzf), zw(5 * nzf), zn(15 * nzf), za(nzf), zc(3 * nzf))
+    allocate(zone1(nzone), zone3(3 * nzone), zone4(4 * nzone), zone32(32 * nzone), ent(4 * nzone), bf(nfc))
+    allocate(st(nz * nz * 24))
+    fq = 0; fw = 0; fn = 0; fa = 0; fc = 0; zq = 0; zw = 0; zn = 0; za = 0; zc = 0
+    zone1 = 1; zone3 = 0; zone4 = 0; zone32 = 0; ent = 0; bf = 0; st = 1
+    call MAKE_EDGE_EFIELDS_P1(nr, n, g, 8, 4, 1, md, md, md, vert, mask, &
+         fq, fw, fn, fa, fc, fq, fw, fn, fa, fc, zq, zw, zn, za, zc, &
+         fq, fw, fn, fq, fw, fn, zq, zw, zn, zone1, zone1, zone3, zone4, &
+         p%itri(1), p%itri(2), p%itri(3), p%itri(4), st, &
+         zone32, ent, bf, bf, bf, zone32, zone4, zone4, zone4, ex, ey, ez, &
+         zone1, zone1, zone1, zone1, zone1, zone1, bxq, byq, bzq, sl, sr, sd, su, sb, st_, efx, efy, efz)
+    deallocate(fq, fw, fn, fa, fc, zq, zw, zn, za, zc, zone1, zone3, zone4, zone32, ent, bf, st)
   end subroutine
 
 end module cs_ref_bind

@@ -449,3 +449,125 @@ def test_timestep_of_a_gas_at_rest():
     dt = orc.p1_timestep_oracle(prob, A, 0.4, 10.0)
     assert abs(dt - want) <= 1e-14 * want
     assert orc.p1_timestep_oracle(prob, A, 0.4, 0.5 * want) == 1.1 * (0.5 * want)
+
+
+# ---------------------------------------------------------------------------
+# MAKE_EDGE_EFIELDS_P1 (460make_edge_fields_p1.f:5) with the 2-D solvers
+# Two_D_RS_Ez_4State_HLL (630) and Two_D_RS_Ez_3State_HLL (631)
+
+EDGE_CASES = [
+    # N, NR, sector, seed, shock, noise, itri
+    (8, 3, 0, 0, True, 0.0, (1, 1, 1, 1)),
+    (12, 5, 1, 2, True, 1.0, (1, 1, 1, 1)),
+    (10, 4, 5, 3, False, 0.5, (1, 1, 1, 1)),
+    (9, 4, 3, 1, True, 0.3, (1, 0, 1, 0)),
+]
+
+
+@pytest.mark.parametrize("case", EDGE_CASES, ids=lambda c: "N%d_nr%d_s%d_seed%d_%s_noise%g_itri%s" % (
+    c[0], c[1], c[2], c[3], "front" if c[4] else "smooth", c[5], "".join(map(str, c[6]))))
+def test_edge_efields_bit_exact_with_reference(case):
+    """or_make_edge_efields_p1 against MAKE_EDGE_EFIELDS_P1 on a patch after the
+    oracle's reconstruction and facial fluxes: Efx, Efy, Efz bit for bit, the
+    tri-corner edges (3-state solver) included."""
+    _ref_or_skip()
+    N, NR, s, seed, shock, noise, itri = case
+    prob, A0 = orc.p1_edge_problem(N, NR, 4, sector=s, seed=seed, shock=shock, noise=noise)
+    A = orc.p1e_oracle(prob, orc.p1_copy(A0), itri)
+    B = orc.p1e_reference(prob, orc.p1_copy(A0), itri)
+    for k in orc.P1E_OUT:
+        np.testing.assert_array_equal(A[k], B[k], err_msg=k)
+    # every edge of the owned faces is set; the tri-corner edges only where
+    # their flag is
+    nz_ = (N + 1) * (N + 1) * NR - (4 - sum(itri)) * NR
+    assert int((A["Efz"] != 0).sum()) == nz_
+    assert int((A["Efx"] != 0).sum()) == N * (N + 1) * (NR + 1)
+
+
+def _emf3_ref():
+    R = orc.rlib()
+    f = R.two_d_rs_ez_3state_hll_
+    f.argtypes = [ctypes.c_void_p] * 15
+    f.restype = None
+    return f
+
+
+def test_emf3_hll_bit_exact_with_reference_and_every_branch():
+    """or_emf3_hll against the reference's Two_D_RS_Ez_3State_HLL on random
+    states whose speeds reach each of its seven branches."""
+    _ref_or_skip()
+    L = orc.olib()
+    L.or_emf3_hll.restype = ctypes.c_double
+    L.or_emf3_hll.argtypes = [ctypes.c_void_p] * 4
+    f = _emf3_ref()
+    rng = np.random.default_rng(21)
+    hits = np.zeros(7, int)
+    for n in range(3000):
+        sr = rng.uniform(-1, 3, 3)
+        sl = sr - rng.uniform(0.1, 3, 3)
+        if n % 3 == 0:  # symmetric, the form the edge routine uses
+            a = rng.uniform(0.1, 3)
+            sr, sl = np.full(3, a), np.full(3, -a)
+        ez, bn = rng.normal(size=3), rng.normal(size=3)
+        args = [np.array([v]) for v in (sr[0], sl[0], sr[1], sl[1], sr[2], sl[2], *ez, *bn)]
+        out = [np.zeros(1) for _ in range(3)]
+        f(*[orc.P(a) for a in args + out])
+        got = L.or_emf3_hll(orc.P(sr.copy()), orc.P(sl.copy()), orc.P(ez.copy()), orc.P(bn.copy()))
+        assert got == out[2][0], (n, got, out[2][0])
+        SR1, SL1, SR2, SL2, SR3, SL3 = sr[0], sl[0], sr[1], sl[1], sr[2], sl[2]
+        br = [SL1 >= 0 and SR2 <= 0, SL2 >= 0 and SR3 <= 0, SL3 >= 0 and SR1 <= 0,
+              SL1 * (SR1 + 2 * SL3) + SR1 * (SL1 + 2 * SR2) <= 0,
+              (SL2 + SR3) * (SL1 + 2 * SR2) + (SL2 - SR3) * SL1 <= 0,
+              (SL2 + SR3) * (SR1 + 2 * SL3) - (SL2 - SR3) * SR1 <= 0, True]
+        hits[br.index(True)] += 1
+    assert np.all(hits > 0), hits
+
+
+def test_emf3_hll_symmetric_speeds_equal_llf_closed_form():
+    """With the speeds symmetrised to +-S and one E in the three zones (as at
+    the tri-corners), the 3-state HLL field is the 3-state LLF closed form
+    E - (4 sqrt(3) S / 9)(Bn1 + Bn2 + Bn3) of 633...f:59-84."""
+    L = orc.olib()
+    L.or_emf3_hll.restype = ctypes.c_double
+    L.or_emf3_hll.argtypes = [ctypes.c_void_p] * 4
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        S = rng.uniform(0.01, 5)
+        ez = np.full(3, rng.normal())
+        bn = rng.normal(size=3)
+        got = L.or_emf3_hll(orc.P(np.full(3, S)), orc.P(np.full(3, -S)), orc.P(ez), orc.P(bn))
+        want = ez[0] - 4 * np.sqrt(3) * S / 9 * bn.sum()
+        assert abs(got - want) <= 1e-13 * (abs(ez[0]) + S * np.abs(bn).sum())
+
+
+def test_edge_efields_of_a_uniform_e_without_face_field():
+    """One E in every zone and no face-point B: every edge field is E . t, with
+    t the chord between the edge's vertices for x and y edges (both on one
+    sphere, so the chord is normal to the radius at the midpoint) and the
+    radial direction for z edges, the tri-corner edges included."""
+    N, NR, g = 8, 3, 4
+    prob, A = orc.p1_edge_problem(N, NR, g, sector=4, seed=9)
+    d = orc.p1_dims(N, NR, g)
+    nz, nfc = d["zone"], d["fc"]
+    E0 = np.array([0.3, -0.7, 0.45])
+    for c, k in enumerate(("Ex_zc", "Ey_zc", "Ez_zc")):
+        A[k].reshape(4, nz)[0] = E0[c]
+    for k in ("Bxf_q", "Byf_q", "Bzf_q"):
+        A[k][:] = 0.0
+    orc.p1e_oracle(prob, A)
+    nf, nfr = N + 2 * g + 1, NR + 2 * g + 1
+    vx = A["vert"].reshape(3, nfr, nf, nf)  # [c, k+g, j+g, i+g]
+    V = lambda i, j, k: vx[:, k + g, j + g, i + g]
+    Ef = {q: A["Ef" + q].reshape(nfr, nf, nf) for q in "xyz"}
+    for k in range(0, NR + 1):
+        for j in range(0, N + 1):
+            for i in range(1, N + 1):
+                t = V(i, j, k) - V(i - 1, j, k)
+                assert abs(Ef["x"][k + g, j + g, i + g] - E0 @ t / np.linalg.norm(t)) < 1e-13
+                t = V(j, i, k) - V(j, i - 1, k)
+                assert abs(Ef["y"][k + g, i + g, j + g] - E0 @ t / np.linalg.norm(t)) < 1e-13
+    for k in range(1, NR + 1):
+        for j in range(0, N + 1):
+            for i in range(0, N + 1):
+                t = V(i, j, k)
+                assert abs(Ef["z"][k + g, j + g, i + g] - E0 @ t / np.linalg.norm(t)) < 1e-13

@@ -187,3 +187,30 @@ def test_timestep_gpu_rejects_bad_arguments():
     T["msonic"] = T["msonic"][:-1].contiguous()
     with pytest.raises(cs.CsError):
         cs.evaluate_timestep_p1(8, 3, 4, prob, T, 0.4, 1.0)
+
+
+EDGE_CASES = [(8, 3, 0, 0, True, 0.0, (1, 1, 1, 1)), (12, 5, 1, 2, True, 1.0, (1, 1, 1, 1)),
+              (10, 4, 5, 3, False, 0.5, (1, 1, 1, 1)), (24, 8, 2, 4, True, 0.3, (1, 0, 1, 0))]
+
+
+@pytest.mark.parametrize("case", EDGE_CASES, ids=lambda c: "N%d_nr%d_s%d_seed%d_%s_noise%g_itri%s" % (
+    c[0], c[1], c[2], c[3], "front" if c[4] else "smooth", c[5], "".join(map(str, c[6]))))
+def test_edge_efields_gpu_match_oracle(case):
+    """cs_make_edge_efields_p1 against or_make_edge_efields_p1 (bit-exact with
+    the reference's MAKE_EDGE_EFIELDS_P1) on a patch after the oracle's
+    reconstruction and facial fluxes, the edge arrays starting as junk: Efx,
+    Efy, Efz to 1e-12 of their largest magnitude (the tangents normalised
+    with fused multiply-adds), the zeros off the owned edges exactly."""
+    N, NR, s, seed, shock, noise, itri = case
+    print(f"\nedge case N={N} NR={NR} sector={s} seed={seed} itri={itri}", flush=True)
+    prob, A0 = orc.p1_edge_problem(N, NR, 4, sector=s, seed=seed, shock=shock, noise=noise)
+    O = orc.p1e_oracle(prob, orc.p1_copy(A0), itri)
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(A0[k]).to(dev) for k in cs.P1E_IN + cs.P1E_OUT}
+    cs.make_edge_efields_p1(N, NR, 4, T, itri)
+    torch.cuda.synchronize()
+    G = {k: T[k].cpu().numpy() for k in cs.P1E_OUT}
+    bad = _compare(G, {k: O[k] for k in cs.P1E_OUT}, rtol=1e-12)
+    assert not bad, bad
+    for k in cs.P1E_OUT:
+        np.testing.assert_array_equal(G[k] == 0, O[k] == 0)
