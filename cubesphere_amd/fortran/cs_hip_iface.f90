@@ -8,6 +8,7 @@
 !   cs_limit_all_slopes_p1 <- LIMIT_ALL_SLOPES_P1 alone, argument for argument (DRV:2396)
 !   cs_make_facial_fluxes_p1 <- MAKE_FACIAL_FLUXES alone, argument for argument (DRV:2485)
 !   cs_make_edge_efields   <- MAKE_EDGE_EFIELDS_P1                          (DRV:2548)
+!   cs_make_edge_efields_p1 <- MAKE_EDGE_EFIELDS_P1 alone, on the arguments it reads (DRV:2548)
 !   cs_update_imex_rk_step_p1 <- UPDATE_IMEX_RK_STEP_P1 alone, argument for argument (DRV:2782)
 !   cs_cleanup_after_update_p1 <- CLEANUP_AFTER_UPDATE alone, on the cs_p1_patch (DRV:2815)
 !   cs_evaluate_timestep_p1 <- EVALUATE_TIMESTEP alone, dt into device memory  (DRV:2887)
@@ -59,6 +60,15 @@ module cs_hip_iface
     type(c_ptr) :: dU_dt, EntFlux_x, EntFlux_y, EntFlux_z, Efx, Efy, Efz
   end type
 
+  ! cs_p1_edge: the arguments MAKE_EDGE_EFIELDS_P1 (460make_edge_fields_p1.f:5-20)
+  ! reads at second order
+  type, bind(C) :: cs_p1_edge
+    integer(c_int32_t) :: ntp, nr, nghost, pad
+    integer(c_int32_t) :: itri(4)
+    type(c_ptr) :: vert, mask, Ex_zc, Ey_zc, Ez_zc, Bxf_q, Byf_q, Bzf_q
+    type(c_ptr) :: S_L, S_R, S_D, S_U, S_B, S_T, Efx, Efy, Efz
+  end type
+
   ! cs_p1_update: the arguments of UPDATE_IMEX_RK_STEP_P1 (484update_imex_rk_step_p1.f:7-26)
   type, bind(C) :: cs_p1_update
     integer(c_int32_t) :: ntp, nr, nghost, pad
@@ -69,6 +79,11 @@ module cs_hip_iface
   end type
 
   interface
+    integer(c_int) function cs_make_edge_efields_p1(p, stream) bind(C, name="cs_make_edge_efields_p1")
+      import :: c_int, c_ptr, cs_p1_edge
+      type(cs_p1_edge), intent(in) :: p
+      type(c_ptr), value :: stream
+    end function
     integer(c_int) function cs_update_imex_rk_step_p1(p, stream) bind(C, name="cs_update_imex_rk_step_p1")
       import :: c_int, c_ptr, cs_p1_update
       type(cs_p1_update), intent(in) :: p

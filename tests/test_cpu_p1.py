@@ -1,7 +1,10 @@
-"""CPU checks of the oracle's LIMIT_ALL_SLOPES_P1 (410limit_all_slopes_p1.f:9,
-with 440enforce_pressure_positivity_prim.f and 120matsolve.f): bit-exact
-against the reference's own routine built from its sources (when present),
-and closed-form cases that hold without it."""
+"""CPU checks of the oracle's P1 stage routines of one patch:
+LIMIT_ALL_SLOPES_P1 (410limit_all_slopes_p1.f:9, with 440 and 120matsolve.f),
+MAKE_FACIAL_FLUXES (450), MAKE_EDGE_EFIELDS_P1 (460, with 630 and 631),
+UPDATE_IMEX_RK_STEP_P1 (484), CLEANUP_AFTER_UPDATE (485) and
+EVALUATE_TIMESTEP (490): bit-exact against the reference's own routines built
+from their sources where they build (when that build is present), and
+closed-form cases and invariants that hold without it."""
 import ctypes
 
 import numpy as np

@@ -214,3 +214,45 @@ def test_edge_efields_gpu_match_oracle(case):
     assert not bad, bad
     for k in cs.P1E_OUT:
         np.testing.assert_array_equal(G[k] == 0, O[k] == 0)
+
+
+@pytest.mark.parametrize("case", [(12, 5, 1, 3, 0.3), (16, 6, 4, 8, 1.0)], ids=lambda c: "N%d_nr%d_s%d_seed%d_noise%g" % c)
+def test_p1_stage_gpu_matches_oracle(case):
+    """The reference's per-patch stage on one patch, from the raw state:
+    LIMIT_ALL_SLOPES_P1, MAKE_FACIAL_FLUXES, MAKE_EDGE_EFIELDS_P1,
+    UPDATE_IMEX_RK_STEP_P1, CLEANUP_AFTER_UPDATE, EVALUATE_TIMESTEP, every
+    routine on the GPU through the C-ABI against the same sequence of the
+    oracle's routines (each bit-exact with the reference's or pinned by its
+    invariants).  Rounding differences pass through the limiters, so the
+    state after the stage is compared to 1e-9 of each component's and mode's
+    largest magnitude, the new time step to 1e-9 relative."""
+    N, NR, s, seed, noise = case
+    print(f"\nstage case N={N} NR={NR} sector={s} seed={seed} noise={noise}", flush=True)
+    prob, A0 = orc.p1_problem(N, NR, 4, sector=s, seed=seed, shock=True, noise=noise)
+    d = orc.p1_dims(N, NR, 4)
+    for k in cs.P1F_OUT:
+        A0[k] = np.zeros((8 if k.startswith("Fluct") else 1) * d["fc"])
+    A0["dU_dt"] = np.zeros(8 * d["zone"])
+    A0["flag_entprs"] = np.zeros(d["zone"])
+    uprob = dict(prob, dtmove=2e-3)
+    O = orc.p1_copy(A0)
+    orc.p1_oracle(prob, O)
+    orc.p1f_oracle(prob, O)
+    orc.p1e_oracle(prob, O)
+    orc.p1u_oracle(uprob, O)
+    orc.p1c_oracle(prob, O)
+    dt_o = orc.p1_timestep_oracle(prob, O, 0.4, 1.0)
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(v).to(dev) for k, v in A0.items()}
+    cs.limit_all_slopes_p1(N, NR, 4, prob, T)
+    cs.make_facial_fluxes_p1(N, NR, 4, prob, T)
+    cs.make_edge_efields_p1(N, NR, 4, T)
+    cs.update_imex_rk_step_p1(N, NR, 4, uprob, T)
+    cs.cleanup_after_update_p1(N, NR, 4, prob, T)
+    dt_g = float(cs.evaluate_timestep_p1(N, NR, 4, prob, T, 0.4, 1.0).item())
+    keys = ("U", "ent", "V", "Bxf", "Byf", "Bzf", "Efx", "Efy", "Efz", "flag_entprs")
+    G = {k: T[k].cpu().numpy() for k in keys}
+    bad = _compare(G, {k: O[k] for k in keys}, rtol=1e-9, rows=dict(U=32, V=32, ent=4))
+    assert not bad, bad
+    assert abs(dt_g - dt_o) <= 1e-9 * dt_o, (dt_g, dt_o)
+    np.testing.assert_array_equal(G["flag_entprs"], O["flag_entprs"])
