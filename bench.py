@@ -410,6 +410,11 @@ def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None
 # speeds 6, its vertex 3; plus the three edge arrays written whole (zeros off
 # the owned edges), counted per entry.
 P1_EDGE_BYTES = 8 * (3 + 12 + 6 + 3)
+# BUILD_NON_STIFF_SOURCES per zone with ghosts: Source written 8, and on the
+# zones of its range centroid 3, V mode 1 density and velocity 4, dU_dt read
+# and written 16, the mask 4 bytes (counted over all zones: the range holds
+# all but the outer ghost layers).
+P1_SOURCE_BYTES = 8 * (8 + 3 + 4 + 16) + 4
 P1_CLEANUP_BYTES = 8 * (8 + 32 + 1 + 4 + 32 + 3 + 6 + 3 + 3 + 3) + 4
 P1_TIMESTEP_BYTES = 8 * (16 + 1 + 8 + 1 + 1 + 1) + 4
 P1_BYTES = {"weno+positivity": 8 * (8 + 1 + 1 + 3 + 1 + 12 + 24 + 3 + 12),
@@ -420,8 +425,9 @@ def bench_p1(cs, args):
     """LIMIT_ALL_SLOPES_P1 on one sector of the 6 x 192^2 x 64 shell as a
     patch (NGhost 4): owned-zone reconstructions per second, per-stage times;
     then the rest of the P1 stage on the same patch, each call timed:
-    MAKE_FACIAL_FLUXES, MAKE_EDGE_EFIELDS_P1, UPDATE_IMEX_RK_STEP_P1,
-    CLEANUP_AFTER_UPDATE, EVALUATE_TIMESTEP."""
+    MAKE_FACIAL_FLUXES, MAKE_EDGE_EFIELDS_P1, BUILD_NON_STIFF_SOURCES (point
+    mass and rotation on), UPDATE_IMEX_RK_STEP_P1, CLEANUP_AFTER_UPDATE,
+    EVALUATE_TIMESTEP."""
     import torch
     N, NR, g = 192, 64, 4
     cfg, scal, T = p1_patch_on_gpu(cs, N, NR, g)
@@ -437,6 +443,7 @@ def bench_p1(cs, args):
         for k, b in zip(("Bxf", "Byf", "Bzf"), Bf0):
             T[k].copy_(b)
 
+    T["Source"] = torch.zeros(8 * cs.p1_counts(N, NR, g)["mask"], dtype=torch.float64, device=T["U"].device)
     T["flag_entprs"] = torch.zeros(cs.p1_counts(N, NR, g)["mask"], dtype=torch.float64, device=T["U"].device)
     uscal = dict(scal, dtmove=1e-4)
     dt_out = torch.empty(1, dtype=torch.float64, device=T["U"].device)
@@ -446,6 +453,7 @@ def bench_p1(cs, args):
         out = []
         for fn in (lambda: cs.make_facial_fluxes_p1(N, NR, g, scal, T),
                    lambda: cs.make_edge_efields_p1(N, NR, g, T),
+                   lambda: cs.build_non_stiff_sources_p1(N, NR, g, 1.0, 0.8, 1e-12, (0.0, 0.0, 0.1), T),
                    lambda: cs.update_imex_rk_step_p1(N, NR, g, uscal, T),
                    lambda: cs.cleanup_after_update_p1(N, NR, g, scal, T),
                    lambda: cs.evaluate_timestep_p1(N, NR, g, scal, T, 0.4, 1.0, out=dt_out)):
@@ -462,7 +470,7 @@ def bench_p1(cs, args):
     torch.cuda.synchronize()
     steps = max(1, min(args.steps, 50))
     tot = 0.0
-    tchain = [0.0] * 5
+    tchain = [0.0] * 6
     stage = None
     for _ in range(steps):
         reset()
@@ -474,7 +482,7 @@ def bench_p1(cs, args):
         tchain = [a + b for a, b in zip(tchain, chain())]
         stage = ms if stage is None else [a + b for a, b in zip(stage, ms)]
     ms_call = tot / steps * 1e3
-    ms_flux, ms_edge, ms_upd, ms_clean, ms_dt = (t / steps * 1e3 for t in tchain)
+    ms_flux, ms_edge, ms_src, ms_upd, ms_clean, ms_dt = (t / steps * 1e3 for t in tchain)
     stage = [s / steps for s in stage]
     own = N * N * NR
     cnt = cs.p1_counts(N, NR, g)
@@ -493,6 +501,7 @@ def bench_p1(cs, args):
     nfc = cnt["Bxf"]
     for name, t, nb, items in (("MAKE_FACIAL_FLUXES", ms_flux, None, own),
                                ("MAKE_EDGE_EFIELDS_P1", ms_edge, P1_EDGE_BYTES + 3 * 8 * nfc / own, own),
+                               ("BUILD_NON_STIFF_SOURCES", ms_src, P1_SOURCE_BYTES, cnt["mask"]),
                                ("UPDATE_IMEX_RK_STEP_P1", ms_upd, None, own),
                                ("CLEANUP_AFTER_UPDATE", ms_clean, P1_CLEANUP_BYTES, cnt["mask"]),
                                ("EVALUATE_TIMESTEP", ms_dt, P1_TIMESTEP_BYTES, own)):

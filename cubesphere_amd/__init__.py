@@ -130,6 +130,7 @@ def lib():
             "cs_update_imex_rk_step_p1": [vp, vp],
             "cs_cleanup_after_update_p1": [vp, vp],
             "cs_make_edge_efields_p1": [vp, vp],
+            "cs_build_non_stiff_sources_p1": [vp, vp],
             "cs_evaluate_timestep_p1": [vp, ctypes.c_double, ctypes.c_double, vp, vp],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
@@ -743,3 +744,38 @@ def make_edge_efields_p1(N, NR, g, arrays, itri=(1, 1, 1, 1), stream=None):
         setattr(s, k, t.data_ptr())
     st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
     _check(lib().cs_make_edge_efields_p1(ctypes.byref(s), st), "cs_make_edge_efields_p1")
+
+
+# ---------------------------------------------------------------------------
+# BUILD_NON_STIFF_SOURCES of one patch (cs_build_non_stiff_sources_p1)
+
+class P1Source(ctypes.Structure):
+    """cs_p1_source: the arguments of BUILD_NON_STIFF_SOURCES for one patch."""
+    _fields_ = [("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("newtonsc", ctypes.c_double), ("mstar", ctypes.c_double),
+                ("smallnum", ctypes.c_double), ("omega", ctypes.c_double * 3), ("mask", ctypes.c_void_p),
+                ("v_centroid", ctypes.c_void_p), ("V", ctypes.c_void_p), ("Source", ctypes.c_void_p),
+                ("dU_dt", ctypes.c_void_p)]
+
+
+def build_non_stiff_sources_p1(N, NR, g, newtonsc, mstar, smallnum, omega, arrays, stream=None):
+    """BUILD_NON_STIFF_SOURCES on one patch of contiguous torch tensors on the
+    GPU (mask int32; v_centroid, V, Source, dU_dt float64); Source is written
+    and dU_dt updated in place.  Sizes are checked first."""
+    import torch
+    want = p1_counts(N, NR, g)
+    want.update(Source=8 * want["mask"], dU_dt=8 * want["mask"])
+    s = P1Source()
+    s.ntp, s.nr, s.nghost = N, NR, g
+    s.newtonsc, s.mstar, s.smallnum = float(newtonsc), float(mstar), float(smallnum)
+    for q in range(3):
+        s.omega[q] = float(omega[q])
+    for k in ("mask", "v_centroid", "V", "Source", "dU_dt"):
+        t = arrays[k]
+        dt = torch.int32 if k == "mask" else torch.float64
+        if not (t.is_cuda and t.dtype == dt and t.is_contiguous() and t.numel() == want[k]):
+            raise CsError(f"build_non_stiff_sources_p1: {k} must be a contiguous {dt} CUDA tensor of {want[k]} "
+                          f"elements (got {t.dtype}, {t.numel()}, cuda={t.is_cuda})")
+        setattr(s, k, t.data_ptr())
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    _check(lib().cs_build_non_stiff_sources_p1(ctypes.byref(s), st), "cs_build_non_stiff_sources_p1")

@@ -2427,3 +2427,61 @@ extern "C" int cs_make_edge_efields_p1(const cs_p1_edge* p, void* stream_) {
   P1_HIP(hipGetLastError());
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// BUILD_NON_STIFF_SOURCES (cubesphere/400riemann_integrate/470build_non_stiff_sources.f:6)
+// of one patch, MHD, adiabatic, no external source.  k_src: one work item per
+// zone; every zone's Source is written (zero outside the routine's range), and
+// on the active zones of (0..N+1)^2 x (0..nr+1) the point-mass gravity at the
+// zone centroid, the centrifugal and the Coriolis terms of omega_passive and
+// the energy source v . S go into Source and are added to dU_dt (:150-250).
+namespace csp1 {
+
+__global__ void __launch_bounds__(256) k_src(const cs_p1_source a, int N, int NR, int g, int64_t NZ) {
+  const int64_t z = tid();
+  if (z >= NZ) return;
+  const int nz = N + 2 * g;
+  const int i = (int)(z % nz) + 1 - g, j = (int)((z / nz) % nz) + 1 - g, k = (int)(z / ((int64_t)nz * nz)) + 1 - g;
+  const bool on = i >= 0 && i <= N + 1 && j >= 0 && j <= N + 1 && k >= 0 && k <= NR + 1 && a.mask[z] == 1;
+  double S[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const double ox = a.omega[0], oy = a.omega[1], oz = a.omega[2];
+  if (on && (a.mstar > a.smallnum || sqrt(ox * ox + oy * oy + oz * oz) > a.smallnum)) {
+    const double x = a.v_centroid[z], y = a.v_centroid[z + NZ], w = a.v_centroid[z + 2 * NZ];
+    const double rho = a.V[z], vx = a.V[z + NZ], vy = a.V[z + 2 * NZ], vz = a.V[z + 3 * NZ];
+    const double rad = sqrt(x * x + y * y + w * w);
+    const double gmr3 = a.newtonsc * a.mstar / (rad * rad * rad);
+    const double cx = oy * w - oz * y, cy = oz * x - ox * w, cz = ox * y - oy * x;  // omega x r
+    S[1] = -gmr3 * x * rho - rho * (oy * cz - oz * cy) - 2.0 * rho * (oy * vz - oz * vy);
+    S[2] = -gmr3 * y * rho - rho * (oz * cx - ox * cz) - 2.0 * rho * (oz * vx - ox * vz);
+    S[3] = -gmr3 * w * rho - rho * (ox * cy - oy * cx) - 2.0 * rho * (ox * vy - oy * vx);
+    S[4] = vx * S[1] + vy * S[2] + vz * S[3];
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    a.Source[z + NZ * c] = S[c];
+    if (on) a.dU_dt[z + NZ * c] += S[c];
+  }
+}
+
+}  // namespace csp1
+
+extern "C" int cs_build_non_stiff_sources_p1(const cs_p1_source* p, void* stream_) {
+  if (!p) {
+    cs::set_error("cs_build_non_stiff_sources_p1: null argument");
+    return CS_E_INVALID;
+  }
+  if (p->ntp < 1 || p->nr < 1 || p->nghost < 1) {
+    cs::set_error("cs_build_non_stiff_sources_p1: need ntp, nr, nghost >= 1 (got %d, %d, %d)", p->ntp, p->nr,
+                  p->nghost);
+    return CS_E_INVALID;
+  }
+  if (!p->mask || !p->v_centroid || !p->V || !p->Source || !p->dU_dt) {
+    cs::set_error("cs_build_non_stiff_sources_p1: null array");
+    return CS_E_INVALID;
+  }
+  const int64_t nz = p->ntp + 2 * p->nghost;
+  const int64_t NZ = nz * nz * (p->nr + 2 * p->nghost);
+  k_src<<<grid_of(NZ, 256), 256, 0, (hipStream_t)stream_>>>(*p, p->ntp, p->nr, p->nghost, NZ);
+  P1_HIP(hipGetLastError());
+  return 0;
+}

@@ -368,6 +368,25 @@ typedef struct cs_p1_flux {
 } cs_p1_flux;
 int cs_make_facial_fluxes_p1(const cs_p1_flux* p, void* stream);
 
+/* ---- P1 non-stiff sources of one patch -------------------------------------
+ * BUILD_NON_STIFF_SOURCES (400riemann_integrate/470build_non_stiff_sources.f:6-7),
+ * MHD, adiabatic, with no external source (the routine's application hook
+ * MAKE_EXTERNAL_SOURCE contributes nothing): Source (zone, 8) is cleared; on
+ * the active zones of (0..N+1)^2 x (0..nr+1), when mstar or |omega| exceeds
+ * smallnum, the point-mass gravity -newtonsc mstar rho r / |r|^3 at the zone
+ * centroid, the centrifugal -rho omega x (omega x r) and the Coriolis
+ * -2 rho omega x v (omega = omega_passive, v and rho from V mode 1), the
+ * energy source v . S, stored in Source and added to dU_dt.  Device arrays in
+ * the layouts of cs_p1_patch.  No wait on the stream. */
+typedef struct cs_p1_source {
+  int32_t ntp, nr, nghost, pad;
+  double newtonsc, mstar, smallnum, omega[3];
+  const int32_t* mask;
+  const double *v_centroid, *V;    /* v_centroid_patch, V (mode 1 read) */
+  double *Source, *dU_dt;          /* (zone, 8) each */
+} cs_p1_source;
+int cs_build_non_stiff_sources_p1(const cs_p1_source* p, void* stream);
+
 /* ---- P1 stage update of one patch ------------------------------------------
  * UPDATE_IMEX_RK_STEP_P1 (400riemann_integrate/484update_imex_rk_step_p1.f:7-26),
  * MHD: U(:, mode 1) += dtmove dU_dt on every zone; on the owned zones the

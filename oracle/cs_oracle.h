@@ -182,4 +182,16 @@ typedef struct or_p1_edge {
 int or_make_edge_efields_p1(const or_p1_edge* e);
 double or_emf3_hll(const double* sr, const double* sl, const double* ez, const double* bn);
 
+/* BUILD_NON_STIFF_SOURCES (470build_non_stiff_sources.f:6) of one patch with
+ * no external source; the same field order as cs_p1_source in the product
+ * header */
+typedef struct or_p1_source {
+  int32_t ntp, nr, nghost, pad;
+  double newtonsc, mstar, smallnum, omega[3];
+  const int32_t* mask;
+  const double *v_centroid, *V;
+  double *Source, *dU_dt;
+} or_p1_source;
+int or_build_non_stiff_sources_p1(const or_p1_source* s);
+
 #endif

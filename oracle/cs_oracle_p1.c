@@ -1494,3 +1494,53 @@ int or_make_edge_efields_p1(const or_p1_edge* e) {
   (void)NZ;
   return 0;
 }
+
+/* BUILD_NON_STIFF_SOURCES (400riemann_integrate/470build_non_stiff_sources.f:6)
+ * of one patch, MHD, adiabatic, with no external source (the routine's
+ * MAKE_EXTERNAL_SOURCE hook is not in the reference tree): Source cleared;
+ * on the active zones of (0..N+1)^2 x (0..nr+1), when Mstar or |omega|
+ * exceeds smallnum, the point-mass gravity -G Mstar rho r / |r|^3 at the
+ * zone centroid plus the centrifugal -rho omega x (omega x r) and Coriolis
+ * -2 rho omega x v terms, the energy source v . S, added to Source and
+ * Source added to dU_dt (:150-250). */
+int or_build_non_stiff_sources_p1(const or_p1_source* s) {
+  if (!s || s->nghost < 1 || s->ntp < 1 || s->nr < 1) return 1;
+  or_p1_patch pp;
+  memset(&pp, 0, sizeof(pp));
+  pp.ntp = s->ntp;
+  pp.nr = s->nr;
+  pp.nghost = s->nghost;
+  const p1dims D = p1_dims(&pp), *d = &D;
+  const long NZ = d->NZ;
+  long q;
+  int i, j, k, c;
+  for (q = 0; q < 8 * NZ; ++q) s->Source[q] = 0.0;
+  const double ox = s->omega[0], oy = s->omega[1], oz = s->omega[2];
+  const double aa = sqrt(ox * ox + oy * oy + oz * oz);
+  for (k = 0; k <= d->NR + 1; ++k)
+    for (j = 0; j <= d->N + 1; ++j)
+      for (i = 0; i <= d->N + 1; ++i) {
+        const long z = zi(d, i, j, k);
+        if (s->mask[z] != 1) continue;
+        const double x = s->v_centroid[z], y = s->v_centroid[z + NZ], w = s->v_centroid[z + 2 * NZ];
+        const double rho = s->V[z], vx = s->V[z + NZ], vy = s->V[z + 2 * NZ], vz = s->V[z + 3 * NZ];
+        double S[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const double rad = sqrt(x * x + y * y + w * w);
+        const double gmr3 = s->newtonsc * s->mstar * (1.0 / (rad * rad * rad));
+        if (s->mstar > s->smallnum || aa > s->smallnum) {
+          const double cx = oy * w - oz * y, cy = oz * x - ox * w, cz = ox * y - oy * x;
+          const double fx = -rho * (oy * cz - oz * cy), fy = -rho * (oz * cx - ox * cz), fz = -rho * (ox * cy - oy * cx);
+          const double kx = -2.0 * rho * (oy * vz - oz * vy), ky = -2.0 * rho * (oz * vx - ox * vz),
+                       kz = -2.0 * rho * (ox * vy - oy * vx);
+          S[1] = -(gmr3 * x * rho) + fx + kx;
+          S[2] = -(gmr3 * y * rho) + fy + ky;
+          S[3] = -(gmr3 * w * rho) + fz + kz;
+          S[4] = vx * S[1] + vy * S[2] + vz * S[3];
+        }
+        for (c = 0; c < 8; ++c) {
+          s->Source[z + NZ * c] = s->Source[z + NZ * c] + S[c];
+          s->dU_dt[z + NZ * c] = s->dU_dt[z + NZ * c] + s->Source[z + NZ * c];
+        }
+      }
+  return 0;
+}

@@ -602,3 +602,50 @@ def p1e_reference(prob, A, itri=(1, 1, 1, 1)):
     s = p1e_struct(prob, A, itri)
     R.ref_make_edge_efields_p1(ctypes.byref(s))
     return A
+
+
+# ---------------------------------------------------------------------------
+# BUILD_NON_STIFF_SOURCES of one patch (or_build_non_stiff_sources_p1 /
+# cubesphere_amd.build_non_stiff_sources_p1 share this structure layout)
+
+class P1Source(ctypes.Structure):
+    _fields_ = [("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("newtonsc", ctypes.c_double), ("mstar", ctypes.c_double),
+                ("smallnum", ctypes.c_double), ("omega", ctypes.c_double * 3), ("mask", ctypes.c_void_p),
+                ("v_centroid", ctypes.c_void_p), ("V", ctypes.c_void_p), ("Source", ctypes.c_void_p),
+                ("dU_dt", ctypes.c_void_p)]
+
+
+P1S_ARRAYS = ("mask", "v_centroid", "V", "Source", "dU_dt")
+
+
+def p1_source_problem(N, NR, g=4, sector=0, seed=0, gm=(1.0, 0.8), omega=(0.1, -0.2, 0.3)):
+    """A reconstructed patch (the oracle's LIMIT_ALL_SLOPES_P1) with seeded
+    dU_dt and junk in Source; gm = (newtonsc, Mstar)."""
+    prob, A = p1_problem(N, NR, g, sector=sector, seed=seed, shock=True, noise=0.3)
+    p1_oracle(prob, A)
+    d = p1_dims(N, NR, g)
+    rng = np.random.default_rng(seed + 500)
+    A["Source"] = rng.normal(size=8 * d["zone"])
+    A["dU_dt"] = rng.normal(size=8 * d["zone"])
+    prob = dict(prob, newtonsc=gm[0], mstar=gm[1], omega=tuple(omega))
+    return prob, A
+
+
+def p1s_struct(prob, A):
+    s = P1Source()
+    s.ntp, s.nr, s.nghost = prob["N"], prob["NR"], prob["g"]
+    s.newtonsc, s.mstar, s.smallnum = prob["newtonsc"], prob["mstar"], prob["smallnum"]
+    for q in range(3):
+        s.omega[q] = prob["omega"][q]
+    for k in P1S_ARRAYS:
+        setattr(s, k, A[k].ctypes.data)
+    return s
+
+
+def p1s_oracle(prob, A):
+    L = olib()
+    L.or_build_non_stiff_sources_p1.argtypes = [ctypes.c_void_p]
+    if L.or_build_non_stiff_sources_p1(ctypes.byref(p1s_struct(prob, A))) != 0:
+        raise RuntimeError("oracle rejected the patch")
+    return A

@@ -574,3 +574,77 @@ def test_edge_efields_of_a_uniform_e_without_face_field():
             for i in range(0, N + 1):
                 t = V(i, j, k)
                 assert abs(Ef["z"][k + g, j + g, i + g] - E0 @ t / np.linalg.norm(t)) < 1e-13
+
+
+# ---------------------------------------------------------------------------
+# BUILD_NON_STIFF_SOURCES (470build_non_stiff_sources.f:6).  The reference's
+# routine cannot be built here (it calls MAKE_EXTERNAL_SOURCE, absent from the
+# reference tree), so the restatement is fixed by the closed forms of the
+# three forces and by the zone range, mask and switch of the routine.
+
+def _src_setup(N=8, NR=3, gm=(1.0, 0.8), omega=(0.1, -0.2, 0.3), seed=0, uniform_v=None):
+    prob, A = orc.p1_source_problem(N, NR, 4, sector=2, seed=seed, gm=gm, omega=omega)
+    d = orc.p1_dims(N, NR, 4)
+    nz = d["zone"]
+    V = A["V"].reshape(4, 8, nz)
+    if uniform_v is not None:
+        V[0, 1:4] = np.asarray(uniform_v, float)[:, None]
+    i = np.arange(nz) % (N + 8) - 3
+    j = (np.arange(nz) // (N + 8)) % (N + 8) - 3
+    k = np.arange(nz) // ((N + 8) ** 2) - 3
+    rng_ = (i >= 0) & (i <= N + 1) & (j >= 0) & (j <= N + 1) & (k >= 0) & (k <= NR + 1) & (A["mask"] == 1)
+    return prob, A, nz, rng_
+
+
+def test_sources_gravity_is_radial_inverse_square():
+    """Mstar > 0, no rotation, gas at rest: the force is along -r with
+    magnitude G Mstar rho / r^2 at the zone centroid, no energy source."""
+    prob, A, nz, act = _src_setup(gm=(2.0, 0.7), omega=(0, 0, 0), uniform_v=(0, 0, 0))
+    dU0 = A["dU_dt"].copy()
+    orc.p1s_oracle(prob, A)
+    S = A["Source"].reshape(8, nz)
+    r = A["v_centroid"].reshape(3, nz)
+    rho = A["V"].reshape(4, 8, nz)[0, 0]
+    rn = np.sqrt((r ** 2).sum(0))
+    F = S[1:4, act]
+    mag = np.sqrt((F ** 2).sum(0))
+    np.testing.assert_allclose(mag, 2.0 * 0.7 * rho[act] / rn[act] ** 2, rtol=1e-13)
+    np.testing.assert_allclose((F * r[:, act]).sum(0), -mag * rn[act], rtol=1e-13)
+    assert np.all(S[4, act] == 0) and np.all(S[[0, 5, 6, 7]] == 0) and np.all(S[:, ~act] == 0)
+    np.testing.assert_array_equal(A["dU_dt"].reshape(8, nz)[:, act], (dU0.reshape(8, nz) + S)[:, act])
+    np.testing.assert_array_equal(A["dU_dt"].reshape(8, nz)[:, ~act], dU0.reshape(8, nz)[:, ~act])
+
+
+def test_sources_centrifugal_and_coriolis():
+    """No gravity: at rest the force is the centrifugal rho(|w|^2 r - (w.r)w);
+    with a velocity the Coriolis part is normal to v with magnitude
+    2 rho |w x v|, and the energy source is v . S (the centrifugal work)."""
+    om = np.array([0.1, -0.2, 0.3])
+    prob, A, nz, act = _src_setup(gm=(1.0, 0.0), omega=om, uniform_v=(0, 0, 0))
+    orc.p1s_oracle(prob, A)
+    S = A["Source"].reshape(8, nz)
+    r = A["v_centroid"].reshape(3, nz)[:, act]
+    rho = A["V"].reshape(4, 8, nz)[0, 0, act]
+    want = rho * ((om @ om) * r - (om @ r) * om[:, None])
+    np.testing.assert_allclose(S[1:4, act], want, rtol=1e-12, atol=1e-14 * np.abs(want).max())
+    assert np.all(S[4, act] == 0)
+    v = np.array([0.4, 0.1, -0.25])
+    prob, B, nz, act = _src_setup(gm=(1.0, 0.0), omega=om, uniform_v=v)
+    orc.p1s_oracle(prob, B)
+    S2 = B["Source"].reshape(8, nz)
+    cor = S2[1:4, act] - want
+    np.testing.assert_allclose(v @ cor, 0.0, atol=1e-13 * np.abs(cor).max())
+    np.testing.assert_allclose(np.sqrt((cor ** 2).sum(0)), 2 * rho * np.linalg.norm(np.cross(om, v)), rtol=1e-12)
+    # and points along -(w x v): the Coriolis force of a frame rotating with w
+    assert np.all(np.cross(om, v) @ cor < 0)
+    np.testing.assert_allclose(S2[4, act], v @ S2[1:4, act], rtol=1e-14, atol=1e-15)
+
+
+def test_sources_switch_off_below_smallnum():
+    """Mstar and |omega| both at most smallnum: no source at all, dU_dt
+    unchanged (the routine's switch), but Source still cleared."""
+    prob, A, nz, act = _src_setup(gm=(1.0, 0.0), omega=(0, 0, 0))
+    dU0 = A["dU_dt"].copy()
+    orc.p1s_oracle(prob, A)
+    assert np.all(A["Source"] == 0)
+    np.testing.assert_array_equal(A["dU_dt"], dU0)

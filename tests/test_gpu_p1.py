@@ -220,6 +220,7 @@ def test_edge_efields_gpu_match_oracle(case):
 def test_p1_stage_gpu_matches_oracle(case):
     """The reference's per-patch stage on one patch, from the raw state:
     LIMIT_ALL_SLOPES_P1, MAKE_FACIAL_FLUXES, MAKE_EDGE_EFIELDS_P1,
+    BUILD_NON_STIFF_SOURCES (point mass and rotation on),
     UPDATE_IMEX_RK_STEP_P1, CLEANUP_AFTER_UPDATE, EVALUATE_TIMESTEP, every
     routine on the GPU through the C-ABI against the same sequence of the
     oracle's routines (each bit-exact with the reference's or pinned by its
@@ -234,11 +235,14 @@ def test_p1_stage_gpu_matches_oracle(case):
         A0[k] = np.zeros((8 if k.startswith("Fluct") else 1) * d["fc"])
     A0["dU_dt"] = np.zeros(8 * d["zone"])
     A0["flag_entprs"] = np.zeros(d["zone"])
+    A0["Source"] = np.zeros(8 * d["zone"])
     uprob = dict(prob, dtmove=2e-3)
+    sprob = dict(prob, newtonsc=1.0, mstar=0.5, omega=(0.05, 0.0, 0.2))
     O = orc.p1_copy(A0)
     orc.p1_oracle(prob, O)
     orc.p1f_oracle(prob, O)
     orc.p1e_oracle(prob, O)
+    orc.p1s_oracle(sprob, O)
     orc.p1u_oracle(uprob, O)
     orc.p1c_oracle(prob, O)
     dt_o = orc.p1_timestep_oracle(prob, O, 0.4, 1.0)
@@ -247,12 +251,34 @@ def test_p1_stage_gpu_matches_oracle(case):
     cs.limit_all_slopes_p1(N, NR, 4, prob, T)
     cs.make_facial_fluxes_p1(N, NR, 4, prob, T)
     cs.make_edge_efields_p1(N, NR, 4, T)
+    cs.build_non_stiff_sources_p1(N, NR, 4, 1.0, 0.5, prob["smallnum"], sprob["omega"], T)
     cs.update_imex_rk_step_p1(N, NR, 4, uprob, T)
     cs.cleanup_after_update_p1(N, NR, 4, prob, T)
     dt_g = float(cs.evaluate_timestep_p1(N, NR, 4, prob, T, 0.4, 1.0).item())
-    keys = ("U", "ent", "V", "Bxf", "Byf", "Bzf", "Efx", "Efy", "Efz", "flag_entprs")
+    keys = ("U", "ent", "V", "Bxf", "Byf", "Bzf", "Efx", "Efy", "Efz", "Source", "flag_entprs")
     G = {k: T[k].cpu().numpy() for k in keys}
-    bad = _compare(G, {k: O[k] for k in keys}, rtol=1e-9, rows=dict(U=32, V=32, ent=4))
+    bad = _compare(G, {k: O[k] for k in keys}, rtol=1e-9, rows=dict(U=32, V=32, ent=4, Source=8))
     assert not bad, bad
     assert abs(dt_g - dt_o) <= 1e-9 * dt_o, (dt_g, dt_o)
     np.testing.assert_array_equal(G["flag_entprs"], O["flag_entprs"])
+
+
+@pytest.mark.parametrize("gm,omega", [((1.0, 0.8), (0.1, -0.2, 0.3)), ((2.0, 0.5), (0.0, 0.0, 0.0)),
+                                      ((1.0, 0.0), (0.0, 0.4, -0.1)), ((1.0, 0.0), (0.0, 0.0, 0.0))])
+def test_sources_gpu_match_oracle(gm, omega):
+    """cs_build_non_stiff_sources_p1 against or_build_non_stiff_sources_p1 on a
+    reconstructed patch (gravity and rotation, each alone, and neither):
+    Source and dU_dt to 1e-13 of each component's largest magnitude, Source
+    exactly zero where the oracle's is."""
+    print(f"\nsources gm={gm} omega={omega}", flush=True)
+    N, NR = 12, 5
+    prob, A0 = orc.p1_source_problem(N, NR, 4, sector=3, seed=6, gm=gm, omega=omega)
+    O = orc.p1s_oracle(prob, orc.p1_copy(A0))
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(A0[k]).to(dev) for k in orc.P1S_ARRAYS}
+    cs.build_non_stiff_sources_p1(N, NR, 4, gm[0], gm[1], prob["smallnum"], omega, T)
+    torch.cuda.synchronize()
+    G = {k: T[k].cpu().numpy() for k in ("Source", "dU_dt")}
+    bad = _compare(G, {k: O[k] for k in G}, rtol=1e-13, rows=dict(Source=8, dU_dt=8))
+    assert not bad, bad
+    np.testing.assert_array_equal(G["Source"] == 0, O["Source"] == 0)
