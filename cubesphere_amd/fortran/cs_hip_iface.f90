@@ -9,6 +9,7 @@
 !   cs_make_facial_fluxes_p1 <- MAKE_FACIAL_FLUXES alone, argument for argument (DRV:2485)
 !   cs_make_edge_efields   <- MAKE_EDGE_EFIELDS_P1                          (DRV:2548)
 !   cs_make_edge_efields_p1 <- MAKE_EDGE_EFIELDS_P1 alone, on the arguments it reads (DRV:2548)
+!   cs_build_non_stiff_sources_p1 <- BUILD_NON_STIFF_SOURCES alone, the routine's arguments (DRV:2603)
 !   cs_update_imex_rk_step_p1 <- UPDATE_IMEX_RK_STEP_P1 alone, argument for argument (DRV:2782)
 !   cs_cleanup_after_update_p1 <- CLEANUP_AFTER_UPDATE alone, on the cs_p1_patch (DRV:2815)
 !   cs_evaluate_timestep_p1 <- EVALUATE_TIMESTEP alone, dt into device memory  (DRV:2887)
@@ -69,6 +70,13 @@ module cs_hip_iface
     type(c_ptr) :: S_L, S_R, S_D, S_U, S_B, S_T, Efx, Efy, Efz
   end type
 
+  ! cs_p1_source: the arguments of BUILD_NON_STIFF_SOURCES (470build_non_stiff_sources.f:6-7)
+  type, bind(C) :: cs_p1_source
+    integer(c_int32_t) :: ntp, nr, nghost, pad
+    real(c_double) :: newtonsc, mstar, smallnum, omega(3)
+    type(c_ptr) :: mask, v_centroid, V, Source, dU_dt
+  end type
+
   ! cs_p1_update: the arguments of UPDATE_IMEX_RK_STEP_P1 (484update_imex_rk_step_p1.f:7-26)
   type, bind(C) :: cs_p1_update
     integer(c_int32_t) :: ntp, nr, nghost, pad
@@ -79,6 +87,11 @@ module cs_hip_iface
   end type
 
   interface
+    integer(c_int) function cs_build_non_stiff_sources_p1(p, stream) bind(C, name="cs_build_non_stiff_sources_p1")
+      import :: c_int, c_ptr, cs_p1_source
+      type(cs_p1_source), intent(in) :: p
+      type(c_ptr), value :: stream
+    end function
     integer(c_int) function cs_make_edge_efields_p1(p, stream) bind(C, name="cs_make_edge_efields_p1")
       import :: c_int, c_ptr, cs_p1_edge
       type(cs_p1_edge), intent(in) :: p
