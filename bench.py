@@ -312,7 +312,7 @@ def bench_geometry(cs, args):
     print(json.dumps(line))
 
 
-def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None):
+def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None, sector=0, front_axis=None):
     """One sector of an N^2 x NR shell as a LIMIT_ALL_SLOPES_P1 patch on the
     GPU: quadrature geometry from cs_build_geometry, the stencil table, and a
     state with smooth flow, a front across the patch and log-normal noise on
@@ -325,7 +325,7 @@ def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None
         geo, _ = cs.build_geometry(cfg, skip=("v_pts", "v_wts", "v_tan", "moments"))
     z = {k: v // 6 for k, v in cs.geometry_sizes(cfg).items()}  # one sector
     dev = torch.device(device)
-    sec = lambda a, n, m: torch.from_numpy(a[: n * m]).to(dev)
+    sec = lambda a, n, m: torch.from_numpy(a[sector * n * m:(sector + 1) * n * m]).to(dev)
     T = dict(vert=sec(geo["vert"], z["vert"], 3), mask=sec(geo["mask"], z["zone"], 1),
              stencil=torch.from_numpy(cs.stencil_indexing(cfg)).to(dev), scale=sec(geo["scale"], z["zone"], 1),
              v_centroid=sec(geo["v_cen"], z["zone"], 3))
@@ -353,7 +353,10 @@ def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None
     prs = 1.0 + 0.2 * torch.cos(2.5 * w - 0.5 * x)
     vel = torch.stack([0.3 * y - 0.2 * w, -0.3 * x + 0.1 * w, 0.2 * x * y])
     act = T["mask"] == 1
-    c0 = torch.stack([x[act].mean(), y[act].mean(), w[act].mean()])
+    if front_axis is None:  # the front through this sector's centre
+        c0 = torch.stack([x[act].mean(), y[act].mean(), w[act].mean()])
+    else:  # one front for every sector of a shell
+        c0 = torch.tensor(front_axis, dtype=torch.float64, device=dev)
     c0 = c0 / c0.norm()
     t = torch.linalg.cross(c0, torch.tensor([0.3, 0.5, 0.8], dtype=torch.float64, device=dev), dim=0)
     t = t / t.norm()
@@ -386,6 +389,7 @@ def p1_patch_on_gpu(cs, N, NR, g=4, noise=0.3, seed=7, device="cuda:0", geo=None
             T[k] = torch.zeros(cntf[k], dtype=torch.float64, device=dev)
     scal = dict(gamma=gam, molewt=1.0, rhofloor=1e-12, prsfloor=1e-9, smallnum=1e-12, flatten_coef=0.4,
                 prs_flatten_coef=1.5)
+    T["gamma_sound"].fill_(gam)  # as CLEANUP_AFTER_UPDATE leaves it
     if dev.type == "cuda":
         torch.cuda.synchronize()
     return cfg, scal, T
@@ -531,6 +535,54 @@ def bench_p1(cs, args):
     print(json.dumps(line))
 
 
+def bench_p1_shell(cs, args):
+    """The reference's own time step on the 6 x 192^2 x 64 shell (the C3
+    size): six P1 patches, the driver's loop (cubesphere_amd.P1Shell) of
+    exchange, LIMIT_ALL_SLOPES_P1, MAKE_FACIAL_FLUXES, MAKE_EDGE_EFIELDS_P1,
+    BUILD_NON_STIFF_SOURCES, UPDATE_IMEX_RK_STEP_P1, CLEANUP_AFTER_UPDATE per
+    patch and stage, Heun RK2, EVALUATE_TIMESTEP per step; no defect
+    correction.  Cell-updates per second of whole steps."""
+    import torch
+    N, NR, g = 192, 64, 4
+    cfg0 = cs.make_config(N, NR, g, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
+                          cs.RMESH_UNIFORM, 1.0, 1.6)
+    geo, _ = cs.build_geometry(cfg0, skip=("v_pts", "v_wts", "v_tan", "moments"))
+    patches = []
+    for s in range(6):
+        cfg, scal, T = p1_patch_on_gpu(cs, N, NR, g, geo=geo, sector=s, front_axis=(0.3, 0.5, 0.8), seed=7 + s)
+        T["Source"] = torch.zeros(8 * cs.p1_counts(N, NR, g)["mask"], dtype=torch.float64, device=T["U"].device)
+        T["flag_entprs"] = torch.zeros(cs.p1_counts(N, NR, g)["mask"], dtype=torch.float64, device=T["U"].device)
+        patches.append(T)
+    del geo
+    shell = cs.P1Shell(cfg, patches, scal, sources=(1.0, 0.0, (0.0, 0.0, 0.0)))
+    try:
+        dt = shell.timestep(0.4, 1.0)
+        for _ in range(max(1, args.warmup)):
+            shell.step(dt)
+            dt = shell.timestep(0.4, dt)
+        steps = max(1, min(args.steps, 10))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            shell.step(dt)
+            dt = shell.timestep(0.4, dt)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+    finally:
+        shell.close()
+    cells = 6 * N * N * NR
+    log(f"   P1 shell step {ms:8.2f} ms  ({cells / (ms * 1e-3):.3e} cell-updates/s), dt {dt:.3e}")
+    line = {
+        "metric": "cell-updates/s on 1 MI355X, the reference's P1 scheme", "value": cells / (ms * 1e-3),
+        "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"P1 shell: 6x{N}^2x{NR} shell (C3 size), six P1 patches, NGhost {g}, the driver's "
+                               "Heun RK2 step of the reference's routines, no defect correction; front + log-normal "
+                               "noise 0.3", "zones": cells},
+    }
+    print(json.dumps(line))
+
+
 def cpu_baseline_p1(max_seconds):
     """The reference's own LIMIT_ALL_SLOPES_P1 (built from its sources) on the
     host, single thread, on a 32^2 x 16 patch of the same kind of data; the
@@ -599,9 +651,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    if args.workload == "P1":
-        # the P1 leg keeps its patch in torch tensors: torch's HIP runtime has to
-        # be initialised before the library's first HIP call
+    if args.workload in ("P1", "P1SHELL"):
+        # the P1 legs keep their patches in torch tensors: torch's HIP runtime has
+        # to be initialised before the library's first HIP call
         import torch
         torch.zeros(1, device="cuda")
     import cubesphere_amd as cs
@@ -618,6 +670,9 @@ def main():
         return
     if args.workload == "P1":
         bench_p1(cs, args)
+        return
+    if args.workload == "P1SHELL":
+        bench_p1_shell(cs, args)
         return
     res = {}
     for prec in (cs.FP64, cs.MIXED):

@@ -104,6 +104,7 @@ def lib():
             "cs_upload_state": [vp, vp, vp, vp, vp, vp],
             "cs_download_state": [vp, vp, vp, vp, vp, vp],
             "cs_halo_exchange": [vp],
+            "cs_halo_exchange_p1": [vp, vp, vp, vp, vp, vp, vp],
             "cs_make_facial_fluxes": [vp],
             "cs_make_edge_efields": [vp],
             "cs_update_stage": [vp, i32],
@@ -275,6 +276,17 @@ class CubeSphere:
 
     def halo_exchange(self):
         _check(lib().cs_halo_exchange(self.h), "cs_halo_exchange")
+
+    def halo_exchange_p1(self, patches, stream=None):
+        """cs_halo_exchange_p1 over six P1 patches (dicts of CUDA tensors with
+        U, ent, Bxf, Byf, Bzf in the layouts of cs_p1_patch)."""
+        lists = []
+        for k in ("U", "ent", "Bxf", "Byf", "Bzf"):
+            arr = (ctypes.c_void_p * 6)(*[p[k].data_ptr() for p in patches])
+            lists.append(arr)
+        st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+        _check(lib().cs_halo_exchange_p1(self.h, *[ctypes.cast(a, ctypes.c_void_p) for a in lists], st),
+               "cs_halo_exchange_p1")
 
     def make_facial_fluxes(self):
         _check(lib().cs_make_facial_fluxes(self.h), "cs_make_facial_fluxes")
@@ -779,3 +791,75 @@ def build_non_stiff_sources_p1(N, NR, g, newtonsc, mstar, smallnum, omega, array
         setattr(s, k, t.data_ptr())
     st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
     _check(lib().cs_build_non_stiff_sources_p1(ctypes.byref(s), st), "cs_build_non_stiff_sources_p1")
+
+
+# ---------------------------------------------------------------------------
+# The reference's time step on the six P1 patches of a shell
+
+class P1Shell:
+    """The reference driver's per-step loop over the six P1 patches of one
+    shell (1000riemann_geomesh_cs_mhd.f:1746-2890), on the GPU: before each
+    stage the ghost exchange (cs_halo_exchange_p1), then per patch
+    LIMIT_ALL_SLOPES_P1, MAKE_FACIAL_FLUXES, MAKE_EDGE_EFIELDS_P1,
+    BUILD_NON_STIFF_SOURCES, UPDATE_IMEX_RK_STEP_P1, CLEANUP_AFTER_UPDATE,
+    with the driver's Heun RK2 bookkeeping (DRV:2645-2773): stage 1 saves U,
+    entropy and face B (mode 1) and moves by dt; stage 2 averages them with
+    the saved state after the fluxes are formed and moves by dt / 2.
+    EVALUATE_TIMESTEP over the patches gives the next step.
+
+    patches: six dicts of contiguous CUDA tensors holding every array of the
+    P1 entry points (cs_p1_patch, cs_p1_flux, cs_p1_edge, cs_p1_source,
+    cs_p1_update); scalars: gamma, molewt, rhofloor, prsfloor, smallnum,
+    flatten_coef, prs_flatten_coef; sources: (newtonsc, mstar, omega).  The
+    patches' face-flux speeds and fluctuations, edge fields, Source and
+    dU_dt arrays are work arrays of the step."""
+
+    def __init__(self, cfg, patches, scalars, sources=(1.0, 0.0, (0.0, 0.0, 0.0)), itri=(1, 1, 1, 1)):
+        import torch
+        if len(patches) != 6:
+            raise CsError("P1Shell: six patches (one per cube sector) required")
+        self.cfg, self.P, self.sc, self.src, self.itri = cfg, patches, dict(scalars), sources, itri
+        self.N, self.NR, self.g = cfg.ntp, cfg.nr, cfg.nghost
+        self.solver = CubeSphere(cfg)
+        self.saved = [{k: torch.empty_like(p[k][: n]) for k, n in self._mode1(p).items()} for p in patches]
+        self.dt_dev = torch.empty(6, dtype=torch.float64, device=patches[0]["U"].device)
+
+    def _mode1(self, p):
+        nz = p["ent"].numel() // 4
+        return {"U": 8 * nz, "ent": nz, "Bxf": p["Bxf"].numel(), "Byf": p["Byf"].numel(), "Bzf": p["Bzf"].numel()}
+
+    def exchange(self):
+        self.solver.halo_exchange_p1(self.P)
+
+    def stage(self, irk, dt):
+        N, NR, g = self.N, self.NR, self.g
+        self.exchange()
+        for p, sv in zip(self.P, self.saved):
+            limit_all_slopes_p1(N, NR, g, self.sc, p, self.itri, minmax=False)
+            make_facial_fluxes_p1(N, NR, g, self.sc, p)
+            make_edge_efields_p1(N, NR, g, p, self.itri)
+            build_non_stiff_sources_p1(N, NR, g, self.src[0], self.src[1], self.sc["smallnum"], self.src[2], p)
+            if irk == 1:
+                for k, n in self._mode1(p).items():
+                    sv[k].copy_(p[k][:n])
+                move = dt
+            else:
+                for k, n in self._mode1(p).items():
+                    p[k][:n].add_(sv[k]).mul_(0.5)
+                move = 0.5 * dt
+            update_imex_rk_step_p1(N, NR, g, dict(self.sc, dtmove=move), p)
+            cleanup_after_update_p1(N, NR, g, self.sc, p, self.itri)
+
+    def step(self, dt):
+        self.stage(1, dt)
+        self.stage(2, dt)
+
+    def timestep(self, cfl, dt_old):
+        """EVALUATE_TIMESTEP on each patch, the minimum over the six (the
+        driver's MPI_ALLREDUCE(MIN), DRV:3515)."""
+        for s, p in enumerate(self.P):
+            evaluate_timestep_p1(self.N, self.NR, self.g, self.sc, p, cfl, dt_old, out=self.dt_dev[s:s + 1])
+        return float(self.dt_dev.min().item())
+
+    def close(self):
+        self.solver.close()

@@ -2043,6 +2043,54 @@ extern "C" int cs_halo_exchange(cs_handle* h) {
   return CS_OK;
 }
 
+// Ghost exchange of six P1 patches (one per cube sector, the layouts of
+// cs_p1_patch) through the handle's exchange: mode 1 of each patch's U and
+// entropy and its face B are gathered into the handle's current state (the
+// per-sector layouts are the same, so these are plain copies), exchanged
+// (FETCH_BUFFERS_FILL_IN_* and PAD_BOUNDARY_R, as cs_halo_exchange), and
+// scattered back.  The work is ordered after the caller's stream and the
+// caller's stream after it, through events.
+extern "C" int cs_halo_exchange_p1(cs_handle* h, double* const* U, double* const* ent, double* const* Bxf,
+                                   double* const* Byf, double* const* Bzf, void* stream) {
+  CHECK_H(h);
+  if (!U || !ent || !Bxf || !Byf || !Bzf) {
+    set_error("cs_halo_exchange_p1: null array list");
+    return CS_E_INVALID;
+  }
+  for (int s = 0; s < 6; ++s)
+    if (!U[s] || !ent[s] || !Bxf[s] || !Byf[s] || !Bzf[s]) {
+      set_error("cs_halo_exchange_p1: null array for sector %d", s);
+      return CS_E_INVALID;
+    }
+  const CsDims& d = h->d;
+  const int b = h->cur;
+  double* const* dst = h->buf[b];
+  hipStream_t cst = (hipStream_t)stream;
+  hipEvent_t ev[2];
+  for (int q = 0; q < 2; ++q) CS_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
+  CS_HIP(hipEventRecord(ev[0], cst));
+  CS_HIP(hipStreamWaitEvent(h->stream, ev[0], 0));
+  const size_t zb = (size_t)d.cpp * sizeof(double), fb = (size_t)d.fpp * sizeof(double);
+  double* const* src[5] = {U, ent, Bxf, Byf, Bzf};
+  const size_t per[5] = {8 * zb, zb, fb, fb, fb};
+  for (int dir = 0; dir < 2; ++dir) {
+    if (dir == 1) {
+      int rc = launch_halo(h, b);
+      if (rc) return rc;
+    }
+    for (int s = 0; s < 6; ++s)
+      for (int a = 0; a < 5; ++a) {
+        char* hs = (char*)dst[a] + per[a] * s;
+        char* ps = (char*)src[a][s];
+        CS_HIP(hipMemcpyAsync(dir == 0 ? hs : ps, dir == 0 ? ps : hs, per[a], hipMemcpyDeviceToDevice, h->stream));
+      }
+  }
+  CS_HIP(hipEventRecord(ev[1], h->stream));
+  CS_HIP(hipStreamWaitEvent(cst, ev[1], 0));
+  for (int q = 0; q < 2; ++q) CS_HIP(hipEventDestroy(ev[q]));
+  return CS_OK;
+}
+
 extern "C" int cs_make_facial_fluxes(cs_handle* h) {
   CHECK_H(h);
   int rc = launch_fluxes(h, h->cur);

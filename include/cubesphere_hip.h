@@ -141,6 +141,16 @@ int cs_download_state(cs_handle* h, double* U, double* ent, double* Bxf,
  * (330-333) semantics on one device -- index flips, face-B swap and sign
  * flips, averaging of coinciding faces -- then the radial boundary pad. */
 int cs_halo_exchange(cs_handle* h);
+/* The same exchange for six P1 patches, one per cube sector, each in the
+ * layouts of cs_p1_patch (DEVICE arrays): for sector s, U[s] (mode 1 is
+ * exchanged; modes 2-4 untouched), ent[s] (mode 1), Bxf[s], Byf[s], Bzf[s].
+ * FETCH_BUFFERS_FILL_IN_UX/UY/UY_TOP/UZ (330-333) + PAD_BOUNDARY_R over the
+ * six patches, as the driver does before each stage (DRV:1746-1756); the
+ * handle (same ntp, nr, nghost, radial_bc) lends its state buffer and its
+ * exchange kernels.  Ordered after the work already on `stream` and before
+ * the work put on it afterwards (events); no host wait. */
+int cs_halo_exchange_p1(cs_handle* h, double* const* U, double* const* ent, double* const* Bxf,
+                        double* const* Byf, double* const* Bzf, void* stream);
 
 /* Reconstruction + face Riemann solves for every x/y/z face
  * (LIMIT_ALL_SLOPES_P1 reconstruction role + MAKE_FACIAL_FLUXES,

@@ -649,3 +649,74 @@ def p1s_oracle(prob, A):
     if L.or_build_non_stiff_sources_p1(ctypes.byref(p1s_struct(prob, A))) != 0:
         raise RuntimeError("oracle rejected the patch")
     return A
+
+
+# ---------------------------------------------------------------------------
+# the driver's time step over the six P1 patches of one shell, on the oracle
+
+class _Stacked:
+    def __init__(self, arrs):
+        self.a = arrs
+
+    def arrays(self):
+        return self.a
+
+
+def p1_shell_problem(N, NR, g=4, seed=0, noise=0.0, rmin=1.0, rmax=1.6):
+    """Six P1 patches (one per sector) of one shell with a smooth state that
+    is one function of position (so the ghost zones agree with the
+    neighbouring sectors' zones), plus every work array of the stage."""
+    import cubesphere_amd as cs  # Config layout only
+    probs, As = [], []
+    d = p1_dims(N, NR, g)
+    for s in range(6):
+        prob, A = p1_problem(N, NR, g, sector=s, seed=seed, shock=False, rmin=rmin, rmax=rmax, junk=False,
+                             noise=noise)
+        for k in P1F_OUT:
+            A[k] = np.zeros((8 if k.startswith("Fluct") else 1) * d["fc"])
+        A["dU_dt"] = np.zeros(8 * d["zone"])
+        A["flag_entprs"] = np.zeros(d["zone"])
+        A["Source"] = np.zeros(8 * d["zone"])
+        A["gamma_sound"][:] = prob["gamma"]  # as CLEANUP_AFTER_UPDATE leaves it
+        probs.append(prob)
+        As.append(A)
+    cfg = cs.make_config(N, NR, g, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2, rmin=rmin, rmax=rmax,
+                         radial_bc=cs.RBC_PAD)
+    return cfg, probs, As
+
+
+def p1_shell_exchange_oracle(grid, As):
+    nz = As[0]["ent"].size // 4
+    keys = (("U", 8 * nz), ("ent", nz), ("Bxf", As[0]["Bxf"].size), ("Byf", As[0]["Byf"].size),
+            ("Bzf", As[0]["Bzf"].size))
+    st = _Stacked([np.concatenate([A[k][:n] for A in As]) for k, n in keys])
+    grid.halo(st)
+    for q, (k, n) in enumerate(keys):
+        for s, A in enumerate(As):
+            A[k][:n] = st.a[q][s * n:(s + 1) * n]
+
+
+def p1_shell_step_oracle(grid, probs, As, dt, sources=(1.0, 0.0, (0.0, 0.0, 0.0))):
+    """One Heun RK2 step of the driver (DRV:1746-2890) over the six patches,
+    every routine the oracle's."""
+    nz = As[0]["ent"].size // 4
+    keys = (("U", 8 * nz), ("ent", nz), ("Bxf", As[0]["Bxf"].size), ("Byf", As[0]["Byf"].size),
+            ("Bzf", As[0]["Bzf"].size))
+    saved = [None] * 6
+    for irk in (1, 2):
+        p1_shell_exchange_oracle(grid, As)
+        for s, (prob, A) in enumerate(zip(probs, As)):
+            p1_oracle(prob, A)
+            p1f_oracle(prob, A)
+            p1e_oracle(prob, A)
+            p1s_oracle(dict(prob, newtonsc=sources[0], mstar=sources[1], omega=tuple(sources[2])), A)
+            if irk == 1:
+                saved[s] = {k: A[k][:n].copy() for k, n in keys}
+                move = dt
+            else:
+                for k, n in keys:
+                    A[k][:n] = (A[k][:n] + saved[s][k]) * 0.5
+                move = 0.5 * dt
+            p1u_oracle(dict(prob, dtmove=move), A)
+            p1c_oracle(prob, A)
+    return As

@@ -282,3 +282,35 @@ def test_sources_gpu_match_oracle(gm, omega):
     bad = _compare(G, {k: O[k] for k in G}, rtol=1e-13, rows=dict(Source=8, dU_dt=8))
     assert not bad, bad
     np.testing.assert_array_equal(G["Source"] == 0, O["Source"] == 0)
+
+
+def test_p1_shell_steps_gpu_match_oracle():
+    """Two Heun RK2 steps of the driver's loop over the six P1 patches of a
+    shell (cubesphere_amd.P1Shell: cs_halo_exchange_p1 and every P1 entry
+    point through the C-ABI) against the same loop of the oracle's routines
+    (oracle.p1_shell_step_oracle, the exchange being the oracle's geometric
+    one): every patch's U, entropy and face B to 1e-9 of each component's and
+    mode's largest magnitude, the next time step to 1e-9 relative."""
+    N, NR, g = 8, 4, 4
+    print(f"\nshell N={N} NR={NR}", flush=True)
+    cfg, probs, As = orc.p1_shell_problem(N, NR, g, seed=3)
+    grid = orc.Grid(cfg)
+    src = (1.0, 0.3, (0.0, 0.1, 0.2))
+    dt = min(orc.p1_timestep_oracle(p, orc.p1_copy(A), 0.4, 1.0) for p, A in zip(probs, As))
+    dev = torch.device("cuda:0")
+    Ts = [{k: torch.from_numpy(v.copy()).to(dev) for k, v in A.items()} for A in As]
+    shell = cs.P1Shell(cfg, Ts, probs[0], sources=src)
+    try:
+        for _ in range(2):
+            orc.p1_shell_step_oracle(grid, probs, As, dt, sources=src)
+            shell.step(dt)
+        torch.cuda.synchronize()
+        for s in range(6):
+            G = {k: Ts[s][k].cpu().numpy() for k in ("U", "ent", "Bxf", "Byf", "Bzf")}
+            bad = _compare(G, {k: As[s][k] for k in G}, rtol=1e-9, rows=dict(U=32, ent=4))
+            assert not bad, (s, bad)
+        dt_g = shell.timestep(0.4, dt)
+        dt_o = min(orc.p1_timestep_oracle(p, A, 0.4, dt) for p, A in zip(probs, As))
+        assert abs(dt_g - dt_o) <= 1e-9 * dt_o, (dt_g, dt_o)
+    finally:
+        shell.close()
