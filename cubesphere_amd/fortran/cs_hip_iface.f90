@@ -5,6 +5,7 @@
 ! calls these entry points instead of the per-patch routines:
 !   cs_halo_exchange       <- LOAD_LOCAL_BUFFERS_* / FETCH_BUFFERS_FILL_IN_* (DRV:1746-1756)
 !   cs_make_facial_fluxes  <- LIMIT_ALL_SLOPES_P1 + MAKE_FACIAL_FLUXES      (DRV:2396, 2485)
+!   cs_halo_exchange_p1    <- FETCH_BUFFERS_FILL_IN_* on six P1 patches' arrays (DRV:1746-1756)
 !   cs_limit_all_slopes_p1 <- LIMIT_ALL_SLOPES_P1 alone, argument for argument (DRV:2396)
 !   cs_make_facial_fluxes_p1 <- MAKE_FACIAL_FLUXES alone, argument for argument (DRV:2485)
 !   cs_make_edge_efields   <- MAKE_EDGE_EFIELDS_P1                          (DRV:2548)
@@ -87,6 +88,13 @@ module cs_hip_iface
   end type
 
   interface
+    ! U, ent, Bxf, Byf, Bzf: arrays of six device addresses (c_loc under host_data)
+    integer(c_int) function cs_halo_exchange_p1(h, U, ent, Bxf, Byf, Bzf, stream) &
+        bind(C, name="cs_halo_exchange_p1")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h, stream
+      type(c_ptr), intent(in) :: U(6), ent(6), Bxf(6), Byf(6), Bzf(6)
+    end function
     integer(c_int) function cs_build_non_stiff_sources_p1(p, stream) bind(C, name="cs_build_non_stiff_sources_p1")
       import :: c_int, c_ptr, cs_p1_source
       type(cs_p1_source), intent(in) :: p
