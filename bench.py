@@ -539,9 +539,10 @@ def bench_p1_shell(cs, args):
     """The reference's own time step on the 6 x 192^2 x 64 shell (the C3
     size): six P1 patches, the driver's loop (cubesphere_amd.P1Shell) of
     exchange, LIMIT_ALL_SLOPES_P1, MAKE_FACIAL_FLUXES, MAKE_EDGE_EFIELDS_P1,
-    BUILD_NON_STIFF_SOURCES, UPDATE_IMEX_RK_STEP_P1, CLEANUP_AFTER_UPDATE per
-    patch and stage, Heun RK2, EVALUATE_TIMESTEP per step; no defect
-    correction.  Cell-updates per second of whole steps."""
+    BUILD_NON_STIFF_SOURCES per patch, the defect correction over the shared
+    faces and edges (cs_sync_shared_p1), UPDATE_IMEX_RK_STEP_P1,
+    CLEANUP_AFTER_UPDATE per patch, Heun RK2, EVALUATE_TIMESTEP per step.
+    Cell-updates per second of whole steps."""
     import torch
     N, NR, g = 192, 64, 4
     cfg0 = cs.make_config(N, NR, g, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
@@ -577,7 +578,7 @@ def bench_p1_shell(cs, args):
         "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"P1 shell: 6x{N}^2x{NR} shell (C3 size), six P1 patches, NGhost {g}, the driver's "
-                               "Heun RK2 step of the reference's routines, no defect correction; front + log-normal "
+                               "Heun RK2 step of the reference's routines with its defect correction; front + log-normal "
                                "noise 0.3", "zones": cells},
     }
     print(json.dumps(line))

@@ -132,6 +132,7 @@ def lib():
             "cs_cleanup_after_update_p1": [vp, vp],
             "cs_make_edge_efields_p1": [vp, vp],
             "cs_build_non_stiff_sources_p1": [vp, vp],
+            "cs_sync_shared_p1": [vp, vp],
             "cs_evaluate_timestep_p1": [vp, ctypes.c_double, ctypes.c_double, vp, vp],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
             "cs_batch_destroy": [vp],
@@ -794,15 +795,156 @@ def build_non_stiff_sources_p1(N, NR, g, newtonsc, mstar, smallnum, omega, array
 
 
 # ---------------------------------------------------------------------------
+# Faces and edges the six P1 patches of a shell share (cs_sync_shared_p1)
+
+FACE_PAIR = np.dtype([("pa", "<i4"), ("fa", "<i4"), ("sa", "<i4"), ("pb", "<i4"), ("fb", "<i4"), ("sb", "<i4"),
+                      ("flip", "<i4"), ("pad", "<i4"), ("ia", "<i8"), ("za", "<i8"), ("aa", "<i8"), ("ib", "<i8"),
+                      ("zb", "<i8"), ("ab", "<i8")])
+EDGE_GROUP = np.dtype([("n", "<i4"), ("pad", "<i4"), ("p", "<i4", 3), ("f", "<i4", 3), ("sign", "<i4", 3),
+                       ("pad2", "<i4"), ("idx", "<i8", 3)])
+
+
+def p1_shell_pairings(geom, N, NR, g, tol=1e-9):
+    """The faces on the lateral boundary of each patch (x faces i = 0, N and y
+    faces j = 0, N of the owned range) paired with the same face of the
+    neighbouring patch, and the lateral-boundary edges grouped with the same
+    edge of the other patches (two, or three at the tri-corners), matched by
+    position: geom[s] holds patch s's vert, xf_centroid, yf_centroid,
+    xf_normal, yf_normal as host arrays.  Returns (faces, edges) as arrays of
+    FACE_PAIR and EDGE_GROUP."""
+    from scipy.spatial import cKDTree
+    nz, nzr, nf, nfr = N + 2 * g, NR + 2 * g, N + 2 * g + 1, NR + 2 * g + 1
+    NXF, NFC = nf * nz * nzr, nf * nf * nfr
+    fci = lambda i, j, k: (i + g) + nf * ((j + g) + nf * (k + g))
+    xfi = lambda i, j, k: (i + g) + nf * ((j - 1 + g) + nz * (k - 1 + g))
+    yfi = lambda i, j, k: (i - 1 + g) + nz * ((j + g) + nf * (k - 1 + g))
+    zi = lambda i, j, k: (i - 1 + g) + nz * ((j - 1 + g) + nz * (k - 1 + g))
+    r = np.arange(1, N + 1)
+    kk = np.arange(1, NR + 1)
+    # faces: (patch, family, face index, zone index, area index, side, position, normal)
+    rows = []
+    for s, G in enumerate(geom):
+        for fam, edge_vals in ((0, (0, N)), (1, (0, N))):
+            cen = G["xf_centroid" if fam == 0 else "yf_centroid"].reshape(3, -1)
+            nrm = G["xf_normal" if fam == 0 else "yf_normal"].reshape(3, -1)
+            for b in edge_vals:
+                K, T = np.meshgrid(kk, r, indexing="ij")
+                K, T = K.ravel(), T.ravel()
+                if fam == 0:
+                    i, j = np.full_like(T, b), T
+                    zone = zi(np.where(b == 0, 1, N), j, K)
+                    ai = xfi(i, j, K)
+                else:
+                    i, j = T, np.full_like(T, b)
+                    zone = zi(i, np.where(b == 0, 1, N), K)
+                    ai = yfi(i, j, K)
+                rows.append(dict(p=np.full_like(T, s), f=np.full_like(T, fam), idx=fci(i, j, K), zone=zone, area=ai,
+                                 side=np.full_like(T, 1 if b == 0 else 0), x=cen[:, ai].T, n=nrm[:, ai].T))
+    F = {k: np.concatenate([q[k] for q in rows]) for k in rows[0]}
+    tree = cKDTree(F["x"])
+    pairs = np.array(sorted(tree.query_pairs(tol)), dtype=np.int64).reshape(-1, 2)
+    if pairs.shape[0] * 2 != F["p"].size or np.unique(pairs).size != F["p"].size:
+        raise CsError("p1_shell_pairings: the patches' boundary faces do not pair up")
+    faces = np.zeros(pairs.shape[0], FACE_PAIR)
+    a, b = pairs[:, 0], pairs[:, 1]
+    for dst, src in (("pa", "p"), ("fa", "f"), ("sa", "side"), ("ia", "idx"), ("za", "zone"), ("aa", "area")):
+        faces[dst] = F[src][a]
+        faces[dst[0] + "b"] = F[src][b]
+    faces["flip"] = ((F["n"][a] * F["n"][b]).sum(1) < 0).astype(np.int32)
+    # edges: (patch, family, index, midpoint, direction)
+    rows = []
+    for s, G in enumerate(geom):
+        V = G["vert"].reshape(3, nfr, nf, nf)
+        P = lambda i, j, k: V[:, k + g, j + g, i + g].T
+        k0 = np.arange(0, NR + 1)
+        for fam in range(3):
+            if fam < 2:
+                K, T = np.meshgrid(k0, r, indexing="ij")
+                K, T = K.ravel(), T.ravel()
+                for b in (0, N):
+                    i, j = (T, np.full_like(T, b)) if fam == 0 else (np.full_like(T, b), T)
+                    a0 = P(i - 1, j, K) if fam == 0 else P(i, j - 1, K)
+                    a1 = P(i, j, K)
+                    rows.append(dict(p=np.full_like(T, s), f=np.full_like(T, fam), idx=fci(i, j, K),
+                                     x=0.5 * (a0 + a1), t=a1 - a0))
+            else:
+                ij = [(i_, j_) for i_ in range(N + 1) for j_ in range(N + 1) if i_ in (0, N) or j_ in (0, N)]
+                I = np.array([q[0] for q in ij])
+                J = np.array([q[1] for q in ij])
+                K, Q = np.meshgrid(kk, np.arange(I.size), indexing="ij")
+                i, j, K = I[Q.ravel()], J[Q.ravel()], K.ravel()
+                a0, a1 = P(i, j, K - 1), P(i, j, K)
+                rows.append(dict(p=np.full_like(i, s), f=np.full_like(i, 2), idx=fci(i, j, K), x=0.5 * (a0 + a1),
+                                 t=a1 - a0))
+    E = {k: np.concatenate([q[k] for q in rows]) for k in rows[0]}
+    tree = cKDTree(E["x"])
+    parent = np.arange(E["p"].size)
+
+    def root(u):
+        while parent[u] != u:
+            parent[u] = parent[parent[u]]
+            u = parent[u]
+        return u
+    for u, v in tree.query_pairs(tol):
+        ru, rv = root(u), root(v)
+        if ru != rv:
+            parent[max(ru, rv)] = min(ru, rv)
+    roots = np.array([root(u) for u in range(parent.size)])
+    order = np.argsort(roots, kind="stable")
+    _, starts, counts = np.unique(roots[order], return_index=True, return_counts=True)
+    if counts.min() < 2 or counts.max() > 3:
+        raise CsError("p1_shell_pairings: the patches' boundary edges do not group by two or three")
+    edges = np.zeros(starts.size, EDGE_GROUP)
+    edges["n"] = counts
+    for m in range(3):
+        has = counts > m
+        mem = order[np.minimum(starts + m, order.size - 1)]
+        first = order[starts]
+        edges["p"][:, m] = np.where(has, E["p"][mem], 0)
+        edges["f"][:, m] = np.where(has, E["f"][mem], 0)
+        edges["idx"][:, m] = np.where(has, E["idx"][mem], 0)
+        edges["sign"][:, m] = np.where(has, np.where((E["t"][mem] * E["t"][first]).sum(1) < 0, -1, 1), 0)
+    return faces, edges
+
+
+class P1PatchFluxes(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in ("Fluct_x_Pl", "Fluct_y_Pl", "Fluct_z_Pl", "Fluct_x_Mi", "Fluct_y_Mi",
+                                                "Fluct_z_Mi", "EntFlux_x", "EntFlux_y", "EntFlux_z", "Efx", "Efy",
+                                                "Efz", "dU_dt", "xf_area", "yf_area", "zf_area", "v_volume")]
+
+
+class P1ShellSync(ctypes.Structure):
+    _fields_ = [("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("patch", P1PatchFluxes * 6), ("faces", ctypes.c_void_p), ("nface", ctypes.c_int64),
+                ("edges", ctypes.c_void_p), ("nedge", ctypes.c_int64)]
+
+
+def sync_shared_p1(N, NR, g, patches, faces_dev, nface, edges_dev, nedge, stream=None):
+    """cs_sync_shared_p1 over six patches (dicts of CUDA tensors) with the
+    pairings of p1_shell_pairings already on the device."""
+    s = P1ShellSync()
+    s.ntp, s.nr, s.nghost = N, NR, g
+    for q, p in enumerate(patches):
+        for k, _ in P1PatchFluxes._fields_:
+            setattr(s.patch[q], k, p[k].data_ptr())
+    s.faces, s.nface = faces_dev.data_ptr(), nface
+    s.edges, s.nedge = edges_dev.data_ptr(), nedge
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    _check(lib().cs_sync_shared_p1(ctypes.byref(s), st), "cs_sync_shared_p1")
+
+
+# ---------------------------------------------------------------------------
 # The reference's time step on the six P1 patches of a shell
 
 class P1Shell:
     """The reference driver's per-step loop over the six P1 patches of one
-    shell (1000riemann_geomesh_cs_mhd.f:1746-2890), on the GPU: before each
+    shell (1000riemann_geomesh_cs_mhd.f:1746-3461), on the GPU: before each
     stage the ghost exchange (cs_halo_exchange_p1), then per patch
     LIMIT_ALL_SLOPES_P1, MAKE_FACIAL_FLUXES, MAKE_EDGE_EFIELDS_P1,
-    BUILD_NON_STIFF_SOURCES, UPDATE_IMEX_RK_STEP_P1, CLEANUP_AFTER_UPDATE,
-    with the driver's Heun RK2 bookkeeping (DRV:2645-2773): stage 1 saves U,
+    BUILD_NON_STIFF_SOURCES; the defect correction in its before-the-update
+    form (cs_sync_shared_p1, with defect=True); then per patch
+    UPDATE_IMEX_RK_STEP_P1, CLEANUP_AFTER_UPDATE, with the driver's Heun RK2
+    bookkeeping (DRV:2645-2773): stage 1 saves U,
     entropy and face B (mode 1) and moves by dt; stage 2 averages them with
     the saved state after the fluxes are formed and moves by dt / 2.
     EVALUATE_TIMESTEP over the patches gives the next step.
@@ -814,7 +956,8 @@ class P1Shell:
     patches' face-flux speeds and fluctuations, edge fields, Source and
     dU_dt arrays are work arrays of the step."""
 
-    def __init__(self, cfg, patches, scalars, sources=(1.0, 0.0, (0.0, 0.0, 0.0)), itri=(1, 1, 1, 1)):
+    def __init__(self, cfg, patches, scalars, sources=(1.0, 0.0, (0.0, 0.0, 0.0)), itri=(1, 1, 1, 1),
+                 defect=True):
         import torch
         if len(patches) != 6:
             raise CsError("P1Shell: six patches (one per cube sector) required")
@@ -823,6 +966,15 @@ class P1Shell:
         self.solver = CubeSphere(cfg)
         self.saved = [{k: torch.empty_like(p[k][: n]) for k, n in self._mode1(p).items()} for p in patches]
         self.dt_dev = torch.empty(6, dtype=torch.float64, device=patches[0]["U"].device)
+        self.defect = defect
+        if defect:
+            geom = [{k: p[k].cpu().numpy() for k in ("vert", "xf_centroid", "yf_centroid", "xf_normal", "yf_normal")}
+                    for p in patches]
+            faces, edges = p1_shell_pairings(geom, self.N, self.NR, self.g)
+            dev = patches[0]["U"].device
+            self.faces = torch.from_numpy(faces.view(np.uint8).copy()).to(dev)
+            self.edges = torch.from_numpy(edges.view(np.uint8).copy()).to(dev)
+            self.nface, self.nedge = faces.size, edges.size
 
     def _mode1(self, p):
         nz = p["ent"].numel() // 4
@@ -834,11 +986,14 @@ class P1Shell:
     def stage(self, irk, dt):
         N, NR, g = self.N, self.NR, self.g
         self.exchange()
-        for p, sv in zip(self.P, self.saved):
+        for p in self.P:
             limit_all_slopes_p1(N, NR, g, self.sc, p, self.itri, minmax=False)
             make_facial_fluxes_p1(N, NR, g, self.sc, p)
             make_edge_efields_p1(N, NR, g, p, self.itri)
             build_non_stiff_sources_p1(N, NR, g, self.src[0], self.src[1], self.sc["smallnum"], self.src[2], p)
+        if self.defect:
+            sync_shared_p1(N, NR, g, self.P, self.faces, self.nface, self.edges, self.nedge)
+        for p, sv in zip(self.P, self.saved):
             if irk == 1:
                 for k, n in self._mode1(p).items():
                     sv[k].copy_(p[k][:n])

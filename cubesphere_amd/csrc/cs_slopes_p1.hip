@@ -2485,3 +2485,134 @@ extern "C" int cs_build_non_stiff_sources_p1(const cs_p1_source* p, void* stream
   P1_HIP(hipGetLastError());
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// The defect correction of the driver over the P1 patches of one shell
+// (FILL_FLUX_SND_PATCH 599..., FETCH_BUFFERS_FILL_IN_FLUX_X/Y/Z 334-336...,
+// SYNCHRONIZE_EFZ_QUAD_CORNER 340..., UNPACK_DEFECT_FLUXES 341...,
+// UPDATE_DEFECT_IMEX_RK 485update_defect_imex_rk.f): the fluctuations and
+// entropy fluxes of every face that two patches share take the mean of the
+// two patches' values (334...f:100-116), the electric field of every shared
+// edge the mean of its two or three patches' values (334...f:128-160, the
+// tri-corner edges a third of three), and each patch's boundary zones get the
+// difference of their fluctuations, -(mean - own) A / V, in dU_dt.  Done
+// before the update instead of re-applied after it; the update is linear in
+// these, so the result is the same.  The face and edge pairings, with their
+// orientations, come from the host (matched geometrically once).
+namespace csp1 {
+
+struct P1Sync {
+  double* Fl_Pl[6][3];  // [patch][family]
+  double* Fl_Mi[6][3];
+  double* EntF[6][3];
+  double* Ef[6][3];
+  double* dU_dt[6];
+  const double* area[6][3];
+  const double* vol[6];
+  int N, g, nz, nf;
+  int64_t NZ, NFC, NXF, NZF;
+};
+
+using FacePair = cs_p1_face_pair;
+using EdgeGroup = cs_p1_edge_group;
+
+__global__ void __launch_bounds__(256) k_sync_faces(const P1Sync S, const FacePair* fp, int64_t n) {
+  const int64_t t = tid();
+  if (t >= n) return;
+  const FacePair q = fp[t];
+  const int64_t NFC = S.NFC;
+  double ePl[2][8], eMi[2][8];
+  double ent[2];
+  const int pp[2] = {q.pa, q.pb}, ff[2] = {q.fa, q.fb};
+  const int64_t ii[2] = {q.ia, q.ib};
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      ePl[s][c] = S.Fl_Pl[pp[s]][ff[s]][ii[s] + NFC * c];
+      eMi[s][c] = S.Fl_Mi[pp[s]][ff[s]][ii[s] + NFC * c];
+    }
+    ent[s] = S.EntF[pp[s]][ff[s]][ii[s]];
+  }
+  // the mean in a's orientation: with opposite normals b's plus side is a's
+  // minus side and its entropy flux has the other sign
+  double mPl[8], mMi[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    mPl[c] = 0.5 * (ePl[0][c] + (q.flip ? eMi[1][c] : ePl[1][c]));
+    mMi[c] = 0.5 * (eMi[0][c] + (q.flip ? ePl[1][c] : eMi[1][c]));
+  }
+  const double mEnt = 0.5 * (ent[0] + (q.flip ? -ent[1] : ent[1]));
+  const int64_t zz[2] = {q.za, q.zb}, ar[2] = {q.aa, q.ab};
+  const int sd[2] = {q.sa, q.sb};
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    const bool swap = s == 1 && q.flip;
+    const double* nPl = swap ? mMi : mPl;
+    const double* nMi = swap ? mPl : mMi;
+    const double w = S.area[pp[s]][ff[s]][ar[s]] / S.vol[pp[s]][zz[s]];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      const double own = sd[s] ? ePl[s][c] : eMi[s][c], mean = sd[s] ? nPl[c] : nMi[c];
+      atomicAdd(&S.dU_dt[pp[s]][zz[s] + S.NZ * c], -(mean - own) * w);
+      S.Fl_Pl[pp[s]][ff[s]][ii[s] + NFC * c] = nPl[c];
+      S.Fl_Mi[pp[s]][ff[s]][ii[s] + NFC * c] = nMi[c];
+    }
+    S.EntF[pp[s]][ff[s]][ii[s]] = (s == 1 && q.flip) ? -mEnt : mEnt;
+  }
+}
+
+__global__ void __launch_bounds__(256) k_sync_edges(const P1Sync S, const EdgeGroup* eg, int64_t n) {
+  const int64_t t = tid();
+  if (t >= n) return;
+  const EdgeGroup q = eg[t];
+  double sum = 0.0;
+  for (int m = 0; m < q.n; ++m) sum += q.sign[m] * S.Ef[q.p[m]][q.f[m]][q.idx[m]];
+  const double mean = q.n == 3 ? sum * (1.0 / 3.0) : 0.5 * sum;
+  for (int m = 0; m < q.n; ++m) S.Ef[q.p[m]][q.f[m]][q.idx[m]] = q.sign[m] * mean;
+}
+
+}  // namespace csp1
+
+extern "C" int cs_sync_shared_p1(const cs_p1_shell_sync* p, void* stream_) {
+  if (!p || !p->faces || !p->edges) {
+    cs::set_error("cs_sync_shared_p1: null argument");
+    return CS_E_INVALID;
+  }
+  if (p->ntp < 1 || p->nr < 1 || p->nghost < 1 || p->nface < 0 || p->nedge < 0) {
+    cs::set_error("cs_sync_shared_p1: bad sizes");
+    return CS_E_INVALID;
+  }
+  P1Sync S;
+  for (int s = 0; s < 6; ++s) {
+    const cs_p1_patch_fluxes& q = p->patch[s];
+    const void* ptrs[] = {q.Fluct_x_Pl, q.Fluct_y_Pl, q.Fluct_z_Pl, q.Fluct_x_Mi, q.Fluct_y_Mi, q.Fluct_z_Mi,
+                          q.EntFlux_x, q.EntFlux_y, q.EntFlux_z, q.Efx, q.Efy, q.Efz, q.dU_dt,
+                          q.xf_area, q.yf_area, q.zf_area, q.v_volume};
+    for (const void* r : ptrs)
+      if (!r) {
+        cs::set_error("cs_sync_shared_p1: null array of patch %d", s);
+        return CS_E_INVALID;
+      }
+    S.Fl_Pl[s][0] = q.Fluct_x_Pl; S.Fl_Pl[s][1] = q.Fluct_y_Pl; S.Fl_Pl[s][2] = q.Fluct_z_Pl;
+    S.Fl_Mi[s][0] = q.Fluct_x_Mi; S.Fl_Mi[s][1] = q.Fluct_y_Mi; S.Fl_Mi[s][2] = q.Fluct_z_Mi;
+    S.EntF[s][0] = q.EntFlux_x; S.EntF[s][1] = q.EntFlux_y; S.EntF[s][2] = q.EntFlux_z;
+    S.Ef[s][0] = q.Efx; S.Ef[s][1] = q.Efy; S.Ef[s][2] = q.Efz;
+    S.dU_dt[s] = q.dU_dt;
+    S.area[s][0] = q.xf_area; S.area[s][1] = q.yf_area; S.area[s][2] = q.zf_area;
+    S.vol[s] = q.v_volume;
+  }
+  S.N = p->ntp;
+  S.g = p->nghost;
+  S.nz = p->ntp + 2 * p->nghost;
+  S.nf = S.nz + 1;
+  S.NZ = (int64_t)S.nz * S.nz * (p->nr + 2 * p->nghost);
+  S.NFC = (int64_t)S.nf * S.nf * (p->nr + 2 * p->nghost + 1);
+  hipStream_t st = (hipStream_t)stream_;
+  if (p->nface > 0)
+    k_sync_faces<<<grid_of(p->nface, 256), 256, 0, st>>>(S, (const FacePair*)p->faces, p->nface);
+  if (p->nedge > 0)
+    k_sync_edges<<<grid_of(p->nedge, 256), 256, 0, st>>>(S, (const EdgeGroup*)p->edges, p->nedge);
+  P1_HIP(hipGetLastError());
+  return 0;
+}

@@ -444,6 +444,48 @@ typedef struct cs_p1_edge {
 } cs_p1_edge;
 int cs_make_edge_efields_p1(const cs_p1_edge* p, void* stream);
 
+/* ---- P1 defect correction over the six patches of a shell -------------------
+ * The driver's defect correction (FILL_FLUX_SND_PATCH 500riemann_main/599...,
+ * FETCH_BUFFERS_FILL_IN_FLUX_X/Y/Z 300riemann_module/334-336...,
+ * SYNCHRONIZE_EFZ_QUAD_CORNER 340..., UNPACK_DEFECT_FLUXES 341...,
+ * UPDATE_DEFECT_IMEX_RK 400riemann_integrate/485update_defect_imex_rk.f) in the
+ * form that comes before the update: for every face two patches share, both
+ * patches' Fluct_*_Pl, Fluct_*_Mi and EntFlux_* become the mean of the two
+ * (334...f:100-116) and the patch's zone next to it gets -(mean - own) A / V
+ * added to dU_dt; for every edge two or three patches share, Ef* becomes the
+ * mean (334...f:128-160; three at the tri-corners, 340...).  The update is
+ * linear in these, so updating after this equals the reference's update plus
+ * its defect step.  faces / edges: DEVICE arrays of the pairings (built once
+ * by the host from the patches' geometry): face a and face b with their
+ * family (0 x, 1 y, 2 z), face-B-cube index, zone and area index, the side of
+ * the face the patch's zone is on (0 minus, 1 plus) and whether the two
+ * normals are opposite (then b's plus side is a's minus side and its entropy
+ * flux changes sign); an edge group lists its 2 or 3 edges with the sign that
+ * turns each into the first one's direction.  No wait on the stream. */
+typedef struct cs_p1_face_pair {
+  int32_t pa, fa, sa, pb, fb, sb, flip, pad;
+  int64_t ia, za, aa, ib, zb, ab;
+} cs_p1_face_pair;
+typedef struct cs_p1_edge_group {
+  int32_t n, pad;
+  int32_t p[3], f[3], sign[3], pad2;
+  int64_t idx[3];
+} cs_p1_edge_group;
+typedef struct cs_p1_patch_fluxes {
+  double *Fluct_x_Pl, *Fluct_y_Pl, *Fluct_z_Pl, *Fluct_x_Mi, *Fluct_y_Mi, *Fluct_z_Mi;
+  double *EntFlux_x, *EntFlux_y, *EntFlux_z, *Efx, *Efy, *Efz, *dU_dt;
+  const double *xf_area, *yf_area, *zf_area, *v_volume;
+} cs_p1_patch_fluxes;
+typedef struct cs_p1_shell_sync {
+  int32_t ntp, nr, nghost, pad;
+  cs_p1_patch_fluxes patch[6];
+  const cs_p1_face_pair* faces;
+  int64_t nface;
+  const cs_p1_edge_group* edges;
+  int64_t nedge;
+} cs_p1_shell_sync;
+int cs_sync_shared_p1(const cs_p1_shell_sync* p, void* stream);
+
 /* ---- P1 cleanup and time step of one patch ---------------------------------
  * CLEANUP_AFTER_UPDATE (400riemann_integrate/485cleanup_after_update.f:1-25),
  * MHD, on the fields of cs_p1_patch: gamma_eos, gamma_sound, mean_mol_wt set;

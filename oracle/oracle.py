@@ -696,9 +696,54 @@ def p1_shell_exchange_oracle(grid, As):
             A[k][:n] = st.a[q][s * n:(s + 1) * n]
 
 
-def p1_shell_step_oracle(grid, probs, As, dt, sources=(1.0, 0.0, (0.0, 0.0, 0.0))):
-    """One Heun RK2 step of the driver (DRV:1746-2890) over the six patches,
-    every routine the oracle's."""
+_FAM = ("x", "y", "z")
+
+
+def p1_shell_sync_oracle(As, faces, edges):
+    """The driver's defect correction in its before-the-update form: the
+    fluctuations and entropy flux of each shared face become the mean of the
+    two patches' (334fetch_buffers_fill_in_flux_x.f:100-116, 341...f), the
+    patch's boundary zone takes -(mean - own) A / V in dU_dt (485update_defect
+    _imex_rk.f), each shared edge's field the mean of its two or three
+    patches' (334...f:128-160, 340...f).  faces, edges: the pairings of
+    cubesphere_amd.p1_shell_pairings."""
+    nz = As[0]["ent"].size // 4
+    nfc = As[0]["Bxf"].size
+    for q in faces:
+        fa, fb = _FAM[q["fa"]], _FAM[q["fb"]]
+        A, B = As[q["pa"]], As[q["pb"]]
+        ia, ib = int(q["ia"]), int(q["ib"])
+        sl = lambda i: slice(i, i + 8 * nfc, nfc)
+        Pa, Ma, Ea = A["Fluct_%s_Pl" % fa][sl(ia)].copy(), A["Fluct_%s_Mi" % fa][sl(ia)].copy(), A["EntFlux_" + fa][ia]
+        Pb, Mb, Eb = B["Fluct_%s_Pl" % fb][sl(ib)].copy(), B["Fluct_%s_Mi" % fb][sl(ib)].copy(), B["EntFlux_" + fb][ib]
+        flip = bool(q["flip"])
+        mP = 0.5 * (Pa + (Mb if flip else Pb))
+        mM = 0.5 * (Ma + (Pb if flip else Mb))
+        mE = 0.5 * (Ea + (-Eb if flip else Eb))
+        for X, fam, i, z, ar, side, own_P, own_M, nP, nM, nE in (
+                (A, fa, ia, int(q["za"]), int(q["aa"]), int(q["sa"]), Pa, Ma, mP, mM, mE),
+                (B, fb, ib, int(q["zb"]), int(q["ab"]), int(q["sb"]), Pb, Mb, mM if flip else mP, mP if flip else mM,
+                 -mE if flip else mE)):
+            w = X[fam + "f_area"][ar] / X["v_volume"][z]
+            own, mean = (own_P, nP) if side else (own_M, nM)
+            X["dU_dt"][z::nz][:8] += -(mean - own) * w
+            X["Fluct_%s_Pl" % fam][sl(i)] = nP
+            X["Fluct_%s_Mi" % fam][sl(i)] = nM
+            X["EntFlux_" + fam][i] = nE
+    for q in edges:
+        n = int(q["n"])
+        tot = 0.0
+        for m in range(n):
+            tot += q["sign"][m] * As[q["p"][m]]["Ef" + _FAM[q["f"][m]]][q["idx"][m]]
+        mean = tot * (1.0 / 3.0) if n == 3 else 0.5 * tot
+        for m in range(n):
+            As[q["p"][m]]["Ef" + _FAM[q["f"][m]]][q["idx"][m]] = q["sign"][m] * mean
+
+
+def p1_shell_step_oracle(grid, probs, As, dt, sources=(1.0, 0.0, (0.0, 0.0, 0.0)), pairings=None):
+    """One Heun RK2 step of the driver (DRV:1746-3461) over the six patches,
+    every routine the oracle's; with pairings (faces, edges) the defect
+    correction in its before-the-update form."""
     nz = As[0]["ent"].size // 4
     keys = (("U", 8 * nz), ("ent", nz), ("Bxf", As[0]["Bxf"].size), ("Byf", As[0]["Byf"].size),
             ("Bzf", As[0]["Bzf"].size))
@@ -710,6 +755,9 @@ def p1_shell_step_oracle(grid, probs, As, dt, sources=(1.0, 0.0, (0.0, 0.0, 0.0)
             p1f_oracle(prob, A)
             p1e_oracle(prob, A)
             p1s_oracle(dict(prob, newtonsc=sources[0], mstar=sources[1], omega=tuple(sources[2])), A)
+        if pairings is not None:
+            p1_shell_sync_oracle(As, *pairings)
+        for s, (prob, A) in enumerate(zip(probs, As)):
             if irk == 1:
                 saved[s] = {k: A[k][:n].copy() for k, n in keys}
                 move = dt

@@ -286,11 +286,13 @@ def test_sources_gpu_match_oracle(gm, omega):
 
 def test_p1_shell_steps_gpu_match_oracle():
     """Two Heun RK2 steps of the driver's loop over the six P1 patches of a
-    shell (cubesphere_amd.P1Shell: cs_halo_exchange_p1 and every P1 entry
-    point through the C-ABI) against the same loop of the oracle's routines
-    (oracle.p1_shell_step_oracle, the exchange being the oracle's geometric
-    one): every patch's U, entropy and face B to 1e-9 of each component's and
-    mode's largest magnitude, the next time step to 1e-9 relative."""
+    shell (cubesphere_amd.P1Shell: cs_halo_exchange_p1, every P1 entry point
+    and the defect correction cs_sync_shared_p1 through the C-ABI) against
+    the same loop of the oracle's routines (oracle.p1_shell_step_oracle, the
+    exchange being the oracle's geometric one, the same face and edge
+    pairings): every patch's U, entropy and face B to 1e-9 of each
+    component's and mode's largest magnitude, the next time step to 1e-9
+    relative."""
     N, NR, g = 8, 4, 4
     print(f"\nshell N={N} NR={NR}", flush=True)
     cfg, probs, As = orc.p1_shell_problem(N, NR, g, seed=3)
@@ -300,9 +302,10 @@ def test_p1_shell_steps_gpu_match_oracle():
     dev = torch.device("cuda:0")
     Ts = [{k: torch.from_numpy(v.copy()).to(dev) for k, v in A.items()} for A in As]
     shell = cs.P1Shell(cfg, Ts, probs[0], sources=src)
+    pairings = cs.p1_shell_pairings(As, N, NR, g)
     try:
         for _ in range(2):
-            orc.p1_shell_step_oracle(grid, probs, As, dt, sources=src)
+            orc.p1_shell_step_oracle(grid, probs, As, dt, sources=src, pairings=pairings)
             shell.step(dt)
         torch.cuda.synchronize()
         for s in range(6):
