@@ -6,6 +6,7 @@
 !   cs_halo_exchange       <- LOAD_LOCAL_BUFFERS_* / FETCH_BUFFERS_FILL_IN_* (DRV:1746-1756)
 !   cs_make_facial_fluxes  <- LIMIT_ALL_SLOPES_P1 + MAKE_FACIAL_FLUXES      (DRV:2396, 2485)
 !   cs_halo_exchange_p1    <- FETCH_BUFFERS_FILL_IN_* on six P1 patches' arrays (DRV:1746-1756)
+!   cs_sync_shared_p1      <- the defect correction over six P1 patches, before the update (DRV:3027-3461)
 !   cs_limit_all_slopes_p1 <- LIMIT_ALL_SLOPES_P1 alone, argument for argument (DRV:2396)
 !   cs_make_facial_fluxes_p1 <- MAKE_FACIAL_FLUXES alone, argument for argument (DRV:2485)
 !   cs_make_edge_efields   <- MAKE_EDGE_EFIELDS_P1                          (DRV:2548)
@@ -78,6 +79,22 @@ module cs_hip_iface
     type(c_ptr) :: mask, v_centroid, V, Source, dU_dt
   end type
 
+  ! cs_p1_shell_sync: the six patches' face and edge arrays and the pairings
+  ! (device arrays of cs_p1_face_pair / cs_p1_edge_group records)
+  type, bind(C) :: cs_p1_patch_fluxes
+    type(c_ptr) :: Fluct_x_Pl, Fluct_y_Pl, Fluct_z_Pl, Fluct_x_Mi, Fluct_y_Mi, Fluct_z_Mi
+    type(c_ptr) :: EntFlux_x, EntFlux_y, EntFlux_z, Efx, Efy, Efz, dU_dt
+    type(c_ptr) :: xf_area, yf_area, zf_area, v_volume
+  end type
+  type, bind(C) :: cs_p1_shell_sync
+    integer(c_int32_t) :: ntp, nr, nghost, pad
+    type(cs_p1_patch_fluxes) :: patch(6)
+    type(c_ptr) :: faces
+    integer(c_int64_t) :: nface
+    type(c_ptr) :: edges
+    integer(c_int64_t) :: nedge
+  end type
+
   ! cs_p1_update: the arguments of UPDATE_IMEX_RK_STEP_P1 (484update_imex_rk_step_p1.f:7-26)
   type, bind(C) :: cs_p1_update
     integer(c_int32_t) :: ntp, nr, nghost, pad
@@ -88,6 +105,11 @@ module cs_hip_iface
   end type
 
   interface
+    integer(c_int) function cs_sync_shared_p1(p, stream) bind(C, name="cs_sync_shared_p1")
+      import :: c_int, c_ptr, cs_p1_shell_sync
+      type(cs_p1_shell_sync), intent(in) :: p
+      type(c_ptr), value :: stream
+    end function
     ! U, ent, Bxf, Byf, Bzf: arrays of six device addresses (c_loc under host_data)
     integer(c_int) function cs_halo_exchange_p1(h, U, ent, Bxf, Byf, Bzf, stream) &
         bind(C, name="cs_halo_exchange_p1")
