@@ -311,10 +311,8 @@ __global__ void __launch_bounds__(256) k_flat_c(const P1 P, const Box b) {
 // ---------------------------------------------------------------------------
 // WENO (410...f:2421-3761, 5437-7100): stencil s uses the x-neighbour
 // (R for s in {0,1,4,5}, else L), the y-neighbour (U for s < 4, else D) and the
-// z-neighbour (top for even s, else bottom)
-struct Weno {
-  double A[8][3][3];
-};
+// z-neighbour (top for even s, else bottom), each stencil's 3 x 3 offset
+// matrix inverted by its adjugate
 
 __device__ __forceinline__ void inv3(const double (&m)[3][3], double (&r)[3][3]) {
   const double a11 = m[0][0], a12 = m[0][1], a13 = m[0][2], a21 = m[1][0], a22 = m[1][1], a23 = m[1][2],
@@ -333,69 +331,14 @@ __device__ __forceinline__ void inv3(const double (&m)[3][3], double (&r)[3][3])
   r[2][2] = (-(a12 * a21) + a11 * a22) * od;
 }
 
-__device__ __forceinline__ void weno_build(const P1& P, int64_t z0, const int64_t* z6, Weno& W) {
-  const double obl = P.obl(z0);
-  double off[6][3];
-#pragma unroll
-  for (int n = 0; n < 6; ++n)
-#pragma unroll
-    for (int c = 0; c < 3; ++c) off[n][c] = (P.cen(z6[n], c) - P.cen(z0, c)) * obl;
-#pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    double m[3][3];
-    const int sx = (s >> 1) & 1, sy = (s >> 2) & 1, sz = s & 1;
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      m[0][c] = off[sx][c];
-      m[1][c] = off[2 + sy][c];
-      m[2][c] = off[4 + sz][c];
-    }
-    inv3(m, W.A[s]);
-  }
-}
-
-__device__ __forceinline__ void weno_apply(const Weno& W, double f0, const double* f6, double smallnum,
-                                           double omflat, double* g3) {
-  double X[9][3];
-#pragma unroll
-  for (int s = 0; s < 8; ++s) {
-    const int sx = (s >> 1) & 1, sy = (s >> 2) & 1, sz = s & 1;
-    const double r0 = f6[sx] - f0, r1 = f6[2 + sy] - f0, r2 = f6[4 + sz] - f0;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) X[s][m] = W.A[s][m][0] * r0 + W.A[s][m][1] * r1 + W.A[s][m][2] * r2;
-  }
-#pragma unroll
-  for (int m = 0; m < 3; ++m)
-    X[8][m] = 0.125 * (X[0][m] + X[1][m] + X[2][m] + X[3][m] + X[4][m] + X[5][m] + X[6][m] + X[7][m]);
-  double w[9], sum = 0.0;
-#pragma unroll
-  for (int s = 0; s < 9; ++s) {
-    double t = X[s][0] * X[s][0] + X[s][1] * X[s][1] + X[s][2] * X[s][2] + smallnum;
-    t = t * t;
-    w[s] = (s == 8 ? 100.0 : 1.0) / (t * t);
-    sum = s == 0 ? w[0] : sum + w[s];
-  }
-  const double os = 1.0 / sum;
-#pragma unroll
-  for (int s = 0; s < 9; ++s) w[s] = w[s] * os;
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    double v = w[0] * X[0][m];
-#pragma unroll
-    for (int s = 1; s < 9; ++s) v = v + w[s] * X[s][m];
-    g3[m] = v * omflat;
-  }
-}
-
 __device__ __forceinline__ bool in_weno_range(const P1& P, int i, int j, int k) {
   return i >= -2 && i <= P.N + 3 && j >= -2 && j <= P.N + 3 && k >= -2 && k <= P.NR + 3;
 }
 
 // ---------------------------------------------------------------------------
 // step 7.a: WENO of the 8 primitives and the entropy, one thread per three
-// fields of a zone of the range (or of a zone list), so each zone's stencil
-// inverses are formed three times.  Inactive columns keep the zero modes
-// k_prim left.
+// fields of a zone of the range (or of a zone list).  Inactive columns keep
+// the zero modes k_prim left.
 __device__ __forceinline__ void zone_of(const P1& P, int64_t z, int& i, int& j, int& k) {
   i = (int)(z % P.nz) + 1 - P.g;
   j = (int)((z / P.nz) % P.nz) + 1 - P.g;
@@ -410,6 +353,59 @@ struct WenoFields {
   int64_t stride[9];
   int n;
 };
+
+// The WENO of a field of one zone in one pass over the 8 stencils: per
+// stencil its 3 x 3 inverse, the gradient X_s, its weight
+// w_s = 1 / (|X_s|^2 + smallnum)^4 and the running sums of w_s X_s, w_s and
+// X_s; the central candidate (the mean of the 8 gradients, weight 100) is
+// added at the end and the sum normalised.  Only one inverse and three sums
+// are live, so the kernel runs at several waves per SIMD (the inverses are
+// formed once per field, which the FP64 units absorb).
+__device__ __forceinline__ void weno_stream(const double (&off)[6][3], const int64_t* z6, int64_t z0,
+                                            const double* src, double smallnum, double omflat, double* dst,
+                                            int64_t stride) {
+  const double f0 = src[z0];
+  double f6[6];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) f6[q] = src[z6[q]] - f0;
+  double num[3] = {0.0, 0.0, 0.0}, sx[3] = {0.0, 0.0, 0.0}, den = 0.0;
+#pragma unroll 1
+  for (int s = 0; s < 8; ++s) {
+    // the stencil's x, y and z neighbours, chosen by selects (no indexing of
+    // the small arrays by the loop counter, which would put them in scratch)
+    const bool bx = (s >> 1) & 1, by = (s >> 2) & 1, bz = s & 1;
+    double m3[3][3], A[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      m3[0][c] = bx ? off[1][c] : off[0][c];
+      m3[1][c] = by ? off[3][c] : off[2][c];
+      m3[2][c] = bz ? off[5][c] : off[4][c];
+    }
+    inv3(m3, A);
+    const double r0 = bx ? f6[1] : f6[0], r1 = by ? f6[3] : f6[2], r2 = bz ? f6[5] : f6[4];
+    double X[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) X[m] = A[m][0] * r0 + A[m][1] * r1 + A[m][2] * r2;
+    double t = X[0] * X[0] + X[1] * X[1] + X[2] * X[2] + smallnum;
+    t = t * t;
+    const double w = 1.0 / (t * t);
+    den = s == 0 ? w : den + w;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      num[m] += w * X[m];
+      sx[m] += X[m];
+    }
+  }
+  double X8[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) X8[m] = 0.125 * sx[m];
+  double t = X8[0] * X8[0] + X8[1] * X8[1] + X8[2] * X8[2] + smallnum;
+  t = t * t;
+  const double w8 = 100.0 / (t * t);
+  const double os = 1.0 / (den + w8);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) dst[z0 + stride * m] = (num[m] + w8 * X8[m]) * os * omflat;
+}
 
 __global__ void __launch_bounds__(128) k_weno9(const P1 P, const Box b, const int64_t* list, int64_t nlist,
                                                const WenoFields W9) {
@@ -431,19 +427,15 @@ __global__ void __launch_bounds__(128) k_weno9(const P1 P, const Box b, const in
   const cs_p1_patch& a = P.a;
   int64_t z6[6];
   six(P, i, j, k, z6);
-  Weno W;
-  weno_build(P, z, z6, W);
+  const double obl = P.obl(z);
+  double off[6][3];
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) off[q][c] = (P.cen(z6[q], c) - P.cen(z, c)) * obl;
   const double omf = 1.0 - a.flatten[z];
 #pragma unroll 1
-  for (int c = c0; c < c0 + 3 && c < W9.n; ++c) {
-    const double* src = W9.src[c];
-    double f6[6], g3[3];
-#pragma unroll
-    for (int q = 0; q < 6; ++q) f6[q] = src[z6[q]];
-    weno_apply(W, src[z], f6, a.smallnum, omf, g3);
-#pragma unroll
-    for (int m = 0; m < 3; ++m) W9.dst[c][z + W9.stride[c] * m] = g3[m];
-  }
+  for (int c = c0; c < c0 + 3 && c < W9.n; ++c) weno_stream(off, z6, z, W9.src[c], a.smallnum, omf, W9.dst[c], W9.stride[c]);
 }
 
 // ---------------------------------------------------------------------------
