@@ -354,59 +354,14 @@ struct WenoFields {
   int n;
 };
 
-// The WENO of a field of one zone in one pass over the 8 stencils: per
-// stencil its 3 x 3 inverse, the gradient X_s, its weight
-// w_s = 1 / (|X_s|^2 + smallnum)^4 and the running sums of w_s X_s, w_s and
+// The WENO of up to three fields of one zone in one pass over the 8
+// stencils: per stencil its 3 x 3 inverse (formed once for the three
+// fields), each field's gradient X_s, its weight
+// w_s = 1 / (|X_s|^2 + smallnum)^4, and the running sums of w_s X_s, w_s and
 // X_s; the central candidate (the mean of the 8 gradients, weight 100) is
-// added at the end and the sum normalised.  Only one inverse and three sums
-// are live, so the kernel runs at several waves per SIMD (the inverses are
-// formed once per field, which the FP64 units absorb).
-__device__ __forceinline__ void weno_stream(const double (&off)[6][3], const int64_t* z6, int64_t z0,
-                                            const double* src, double smallnum, double omflat, double* dst,
-                                            int64_t stride) {
-  const double f0 = src[z0];
-  double f6[6];
-#pragma unroll
-  for (int q = 0; q < 6; ++q) f6[q] = src[z6[q]] - f0;
-  double num[3] = {0.0, 0.0, 0.0}, sx[3] = {0.0, 0.0, 0.0}, den = 0.0;
-#pragma unroll 1
-  for (int s = 0; s < 8; ++s) {
-    // the stencil's x, y and z neighbours, chosen by selects (no indexing of
-    // the small arrays by the loop counter, which would put them in scratch)
-    const bool bx = (s >> 1) & 1, by = (s >> 2) & 1, bz = s & 1;
-    double m3[3][3], A[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      m3[0][c] = bx ? off[1][c] : off[0][c];
-      m3[1][c] = by ? off[3][c] : off[2][c];
-      m3[2][c] = bz ? off[5][c] : off[4][c];
-    }
-    inv3(m3, A);
-    const double r0 = bx ? f6[1] : f6[0], r1 = by ? f6[3] : f6[2], r2 = bz ? f6[5] : f6[4];
-    double X[3];
-#pragma unroll
-    for (int m = 0; m < 3; ++m) X[m] = A[m][0] * r0 + A[m][1] * r1 + A[m][2] * r2;
-    double t = X[0] * X[0] + X[1] * X[1] + X[2] * X[2] + smallnum;
-    t = t * t;
-    const double w = 1.0 / (t * t);
-    den = s == 0 ? w : den + w;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      num[m] += w * X[m];
-      sx[m] += X[m];
-    }
-  }
-  double X8[3];
-#pragma unroll
-  for (int m = 0; m < 3; ++m) X8[m] = 0.125 * sx[m];
-  double t = X8[0] * X8[0] + X8[1] * X8[1] + X8[2] * X8[2] + smallnum;
-  t = t * t;
-  const double w8 = 100.0 / (t * t);
-  const double os = 1.0 / (den + w8);
-#pragma unroll
-  for (int m = 0; m < 3; ++m) dst[z0 + stride * m] = (num[m] + w8 * X8[m]) * os * omflat;
-}
-
+// added at the end and the sum normalised.  The stencil's neighbours are
+// chosen by selects: indexing the small arrays by the loop counter would put
+// them in scratch.
 __global__ void __launch_bounds__(128) k_weno9(const P1 P, const Box b, const int64_t* list, int64_t nlist,
                                                const WenoFields W9) {
   // thread t: fields 3 (t / n) .. 3 (t / n) + 2 of zone t % n
@@ -433,9 +388,80 @@ __global__ void __launch_bounds__(128) k_weno9(const P1 P, const Box b, const in
   for (int q = 0; q < 6; ++q)
 #pragma unroll
     for (int c = 0; c < 3; ++c) off[q][c] = (P.cen(z6[q], c) - P.cen(z, c)) * obl;
-  const double omf = 1.0 - a.flatten[z];
+  const int nf = 3;  // both launches hold 9 or 3 fields: every group has three
+  const double smallnum = a.smallnum, omf = 1.0 - a.flatten[z];
+  // the group's field pointers by selects among the table's static entries
+  // (c0 is 0, 3 or 6; indexing the argument table at run time would copy it
+  // to scratch)
+  const double* srcs[3];
+  double* dsts[3];
+  int64_t strides[3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    srcs[f] = c0 == 0 ? W9.src[f] : (c0 == 3 ? W9.src[3 + f] : W9.src[6 + f]);
+    dsts[f] = c0 == 0 ? W9.dst[f] : (c0 == 3 ? W9.dst[3 + f] : W9.dst[6 + f]);
+    strides[f] = c0 == 0 ? W9.stride[f] : (c0 == 3 ? W9.stride[3 + f] : W9.stride[6 + f]);
+  }
+  // differences to the six neighbours, one array per neighbour (a select
+  // between two entries of one array would be turned into an indexed load)
+  double dR[3], dL[3], dU[3], dD[3], dT[3], dB[3], num[3][3], den[3], sx[3][3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const double* src = srcs[f];
+    const double f0 = src[z];
+    dR[f] = src[z6[0]] - f0;
+    dL[f] = src[z6[1]] - f0;
+    dU[f] = src[z6[2]] - f0;
+    dD[f] = src[z6[3]] - f0;
+    dT[f] = src[z6[4]] - f0;
+    dB[f] = src[z6[5]] - f0;
+    den[f] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) num[f][m] = sx[f][m] = 0.0;
+  }
 #pragma unroll 1
-  for (int c = c0; c < c0 + 3 && c < W9.n; ++c) weno_stream(off, z6, z, W9.src[c], a.smallnum, omf, W9.dst[c], W9.stride[c]);
+  for (int s = 0; s < 8; ++s) {
+    const bool bx = (s >> 1) & 1, by = (s >> 2) & 1, bz = s & 1;
+    double m3[3][3], A[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      m3[0][c] = bx ? off[1][c] : off[0][c];
+      m3[1][c] = by ? off[3][c] : off[2][c];
+      m3[2][c] = bz ? off[5][c] : off[4][c];
+    }
+    inv3(m3, A);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const double r0 = bx ? dL[f] : dR[f], r1 = by ? dD[f] : dU[f], r2 = bz ? dB[f] : dT[f];
+      double X[3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) X[m] = A[m][0] * r0 + A[m][1] * r1 + A[m][2] * r2;
+      double tt = X[0] * X[0] + X[1] * X[1] + X[2] * X[2] + smallnum;
+      tt = tt * tt;
+      const double w = 1.0 / (tt * tt);
+      den[f] = s == 0 ? w : den[f] + w;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        num[f][m] += w * X[m];
+        sx[f][m] += X[m];
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    if (f >= nf) break;
+    double X8[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) X8[m] = 0.125 * sx[f][m];
+    double tt = X8[0] * X8[0] + X8[1] * X8[1] + X8[2] * X8[2] + smallnum;
+    tt = tt * tt;
+    const double w8 = 100.0 / (tt * tt);
+    const double os = 1.0 / (den[f] + w8);
+    double* dst = dsts[f];
+    const int64_t stride = strides[f];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) dst[z + stride * m] = (num[f][m] + w8 * X8[m]) * os * omf;
+  }
 }
 
 // ---------------------------------------------------------------------------
