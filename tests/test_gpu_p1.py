@@ -317,3 +317,69 @@ def test_p1_shell_steps_gpu_match_oracle():
         assert abs(dt_g - dt_o) <= 1e-9 * dt_o, (dt_g, dt_o)
     finally:
         shell.close()
+
+
+def _shell_after_fluxes(N, NR, g, seed):
+    """six patches after the oracle's exchange, reconstruction, facial fluxes
+    and edge fields"""
+    cfg, probs, As = orc.p1_shell_problem(N, NR, g, seed=seed)
+    grid = orc.Grid(cfg)
+    orc.p1_shell_exchange_oracle(grid, As)
+    for p, A in zip(probs, As):
+        orc.p1_oracle(p, A)
+        orc.p1f_oracle(p, A)
+        orc.p1e_oracle(p, A)
+    return cfg, probs, As, grid
+
+
+def test_p1_halo_exchange_gpu_bit_exact():
+    """cs_halo_exchange_p1 over six P1 patches against the oracle's geometric
+    exchange of the same mode-1 arrays: U, entropy and face B bit for bit
+    (modes 2-4 untouched)."""
+    N, NR, g = 8, 4, 4
+    cfg, probs, As = orc.p1_shell_problem(N, NR, g, seed=4)
+    rng = np.random.default_rng(9)
+    for A in As:  # junk in the ghosts, so the exchange has something to overwrite
+        for k in ("U", "ent", "Bxf", "Byf", "Bzf"):
+            A[k] = A[k] + rng.normal(0, 0.1, A[k].size)
+    dev = torch.device("cuda:0")
+    Ts = [{k: torch.from_numpy(A[k].copy()).to(dev) for k in ("U", "ent", "Bxf", "Byf", "Bzf")} for A in As]
+    orc.p1_shell_exchange_oracle(orc.Grid(cfg), As)
+    solver = cs.CubeSphere(cfg)
+    try:
+        solver.halo_exchange_p1(Ts)
+        torch.cuda.synchronize()
+    finally:
+        solver.close()
+    for s in range(6):
+        for k in ("U", "ent", "Bxf", "Byf", "Bzf"):
+            np.testing.assert_array_equal(Ts[s][k].cpu().numpy(), As[s][k], err_msg=f"patch {s} {k}")
+
+
+def test_p1_sync_shared_gpu_matches_oracle():
+    """cs_sync_shared_p1 against p1_shell_sync_oracle on six patches after
+    their fluxes and edge fields: the fluctuations, entropy fluxes and edge
+    fields exactly (means of the same two or three values), dU_dt to 1e-14 of
+    each component's largest magnitude (the corrections of a corner zone's
+    two shared faces are added in either order)."""
+    N, NR, g = 8, 4, 4
+    cfg, probs, As, grid = _shell_after_fluxes(N, NR, g, seed=5)
+    faces, edges = cs.p1_shell_pairings(As, N, NR, g)
+    dev = torch.device("cuda:0")
+    keys = [k for k, _ in cs.P1PatchFluxes._fields_]
+    Ts = [{k: torch.from_numpy(A[k].copy()).to(dev) for k in keys} for A in As]
+    fd = torch.from_numpy(faces.view(np.uint8).copy()).to(dev)
+    ed = torch.from_numpy(edges.view(np.uint8).copy()).to(dev)
+    orc.p1_shell_sync_oracle(As, faces, edges)
+    cs.sync_shared_p1(N, NR, g, Ts, fd, faces.size, ed, edges.size)
+    torch.cuda.synchronize()
+    for s in range(6):
+        for k in keys:
+            if k in ("xf_area", "yf_area", "zf_area", "v_volume"):
+                continue
+            G = Ts[s][k].cpu().numpy()
+            if k == "dU_dt":
+                bad = _compare({k: G}, {k: As[s][k]}, rtol=1e-14, rows={k: 8})
+                assert not bad, (s, bad)
+            else:
+                np.testing.assert_array_equal(G, As[s][k], err_msg=f"patch {s} {k}")
