@@ -573,6 +573,7 @@ def bench_p1_shell(cs, args):
         shell.close()
     cells = 6 * N * N * NR
     log(f"   P1 shell step {ms:8.2f} ms  ({cells / (ms * 1e-3):.3e} cell-updates/s), dt {dt:.3e}")
+    cpu = None if args.no_cpu else cpu_baseline_p1_shell(args.cpu_seconds)
     line = {
         "metric": "cell-updates/s on 1 MI355X, the reference's P1 scheme", "value": cells / (ms * 1e-3),
         "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms,
@@ -580,8 +581,33 @@ def bench_p1_shell(cs, args):
         "config": {"workload": f"P1 shell: 6x{N}^2x{NR} shell (C3 size), six P1 patches, NGhost {g}, the driver's "
                                "Heun RK2 step of the reference's routines with its defect correction; front + log-normal "
                                "noise 0.3", "zones": cells},
+        "cpu_baseline": cpu,
     }
     print(json.dumps(line))
+
+
+def cpu_baseline_p1_shell(max_seconds):
+    """The oracle's restatement of the same step (the reference's routines 450,
+    470 and 484 do not build from its sources, so no reference build of the
+    whole step exists) on a 6 x 16^2 x 8 shell, one host thread."""
+    import cubesphere_amd as cs
+    from oracle import oracle as orc
+    N, NR = 16, 8
+    cfg, probs, As = orc.p1_shell_problem(N, NR, 4, seed=7, noise=0.3)
+    grid = orc.Grid(cfg)
+    pairings = cs.p1_shell_pairings(As, N, NR, 4)
+    dt = min(orc.p1_timestep_oracle(p, orc.p1_copy(A), 0.4, 1.0) for p, A in zip(probs, As))
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        orc.p1_shell_step_oracle(grid, probs, As, dt, pairings=pairings)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > max_seconds or steps >= 50:
+            break
+    cells = 6 * N * N * NR
+    return {"value": cells * steps / el, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} steps of the oracle's shell step on a 6x16^2x8 shell (six P1 patches, defect "
+                      f"correction), single thread, {el:.1f} s"}
 
 
 def cpu_baseline_p1(max_seconds):
