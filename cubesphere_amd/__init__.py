@@ -894,6 +894,13 @@ def p1_shell_pairings(geom, N, NR, g, tol=1e-9):
     _, starts, counts = np.unique(roots[order], return_index=True, return_counts=True)
     if counts.min() < 2 or counts.max() > 3:
         raise CsError("p1_shell_pairings: the patches' boundary edges do not group by two or three")
+    if E["idx"].min() < 0 or E["idx"].max() >= NFC:
+        raise CsError("p1_shell_pairings: an edge index is outside the patch layouts")
+    for q, n in (("i", NFC), ("z", nz * nz * nzr), ("a", NXF)):
+        for side in "ab":
+            v = faces[q + side]
+            if v.min() < 0 or v.max() >= n:
+                raise CsError("p1_shell_pairings: a face index is outside the patch layouts")
     edges = np.zeros(starts.size, EDGE_GROUP)
     edges["n"] = counts
     for m in range(3):

@@ -2066,29 +2066,36 @@ extern "C" int cs_halo_exchange_p1(cs_handle* h, double* const* U, double* const
   const int b = h->cur;
   double* const* dst = h->buf[b];
   hipStream_t cst = (hipStream_t)stream;
-  hipEvent_t ev[2];
-  for (int q = 0; q < 2; ++q) CS_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
-  CS_HIP(hipEventRecord(ev[0], cst));
-  CS_HIP(hipStreamWaitEvent(h->stream, ev[0], 0));
-  const size_t zb = (size_t)d.cpp * sizeof(double), fb = (size_t)d.fpp * sizeof(double);
-  double* const* src[5] = {U, ent, Bxf, Byf, Bzf};
-  const size_t per[5] = {8 * zb, zb, fb, fb, fb};
-  for (int dir = 0; dir < 2; ++dir) {
-    if (dir == 1) {
-      int rc = launch_halo(h, b);
-      if (rc) return rc;
-    }
-    for (int s = 0; s < 6; ++s)
-      for (int a = 0; a < 5; ++a) {
-        char* hs = (char*)dst[a] + per[a] * s;
-        char* ps = (char*)src[a][s];
-        CS_HIP(hipMemcpyAsync(dir == 0 ? hs : ps, dir == 0 ? ps : hs, per[a], hipMemcpyDeviceToDevice, h->stream));
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  // the events are destroyed on every path out
+  auto run = [&]() -> int {
+    for (int q = 0; q < 2; ++q) CS_HIP(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
+    CS_HIP(hipEventRecord(ev[0], cst));
+    CS_HIP(hipStreamWaitEvent(h->stream, ev[0], 0));
+    const size_t zb = (size_t)d.cpp * sizeof(double), fb = (size_t)d.fpp * sizeof(double);
+    double* const* src[5] = {U, ent, Bxf, Byf, Bzf};
+    const size_t per[5] = {8 * zb, zb, fb, fb, fb};
+    for (int dir = 0; dir < 2; ++dir) {
+      if (dir == 1) {
+        int rc = launch_halo(h, b);
+        if (rc) return rc;
       }
-  }
-  CS_HIP(hipEventRecord(ev[1], h->stream));
-  CS_HIP(hipStreamWaitEvent(cst, ev[1], 0));
-  for (int q = 0; q < 2; ++q) CS_HIP(hipEventDestroy(ev[q]));
-  return CS_OK;
+      for (int s = 0; s < 6; ++s)
+        for (int a = 0; a < 5; ++a) {
+          char* hs = (char*)dst[a] + per[a] * s;
+          char* ps = (char*)src[a][s];
+          CS_HIP(hipMemcpyAsync(dir == 0 ? hs : ps, dir == 0 ? ps : hs, per[a], hipMemcpyDeviceToDevice,
+                                h->stream));
+        }
+    }
+    CS_HIP(hipEventRecord(ev[1], h->stream));
+    CS_HIP(hipStreamWaitEvent(cst, ev[1], 0));
+    return CS_OK;
+  };
+  const int rc = run();
+  for (int q = 0; q < 2; ++q)
+    if (ev[q]) (void)hipEventDestroy(ev[q]);
+  return rc;
 }
 
 extern "C" int cs_make_facial_fluxes(cs_handle* h) {

@@ -461,7 +461,10 @@ int cs_make_edge_efields_p1(const cs_p1_edge* p, void* stream);
  * the face the patch's zone is on (0 minus, 1 plus) and whether the two
  * normals are opposite (then b's plus side is a's minus side and its entropy
  * flux changes sign); an edge group lists its 2 or 3 edges with the sign that
- * turns each into the first one's direction.  No wait on the stream. */
+ * turns each into the first one's direction.  The table entries are not
+ * checked on the device: they must index the patches' arrays (patch 0-5,
+ * family 0-2, indices within the layouts), as the host builder's are.  No
+ * wait on the stream. */
 typedef struct cs_p1_face_pair {
   int32_t pa, fa, sa, pb, fb, sb, flip, pad;
   int64_t ia, za, aa, ib, zb, ab;
