@@ -565,48 +565,72 @@ __global__ void __launch_bounds__(256) k_pos(const P1 P, const Box b, const int6
     fcx[5][c] = a.zf_centroid[P.zfi(i, j, k - 1) + P.NZF * c];
   }
   const double ratio = 0.85769589859089412;  // 0.1^(1/15)
-  int nred = 0;                              // reductions applied
-  bool removed = false;
-  for (int it = 1; it <= 16; ++it) {
-    if (it == 16) {
-      removed = true;
-      break;
+  // does every node pass with the slopes reduced n times?  The reduced slopes
+  // are formed by n multiplications, as the reference's loop forms them
+  auto passes = [&](int n) -> bool {
+    double r[3], p[3], e[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      r[m] = rm[m];
+      p[m] = pm[m];
+      e[m] = em[m];
     }
+    for (int q = 0; q < n; ++q)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        r[m] = ratio * r[m];
+        p[m] = ratio * p[m];
+        e[m] = ratio * e[m];
+      }
     bool pass = true;
     // one node: scaled offset from the centroid, then the three checks
     auto node = [&](const double* x0) {
       double x[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) x[c] = (x0[c] - c0[c]) * obl;
-      const double rn = r0 + rm[0] * x[0] + rm[1] * x[1] + rm[2] * x[2];
-      const double pn = p0 + pm[0] * x[0] + pm[1] * x[1] + pm[2] * x[2];
-      const double en = e0 + em[0] * x[0] + em[1] * x[1] + em[2] * x[2];
+      const double rn = r0 + r[0] * x[0] + r[1] * x[1] + r[2] * x[2];
+      const double pn = p0 + p[0] * x[0] + p[1] * x[1] + p[2] * x[2];
+      const double en = e0 + e[0] * x[0] + e[1] * x[1] + e[2] * x[2];
       if (rn < rmin || rn > rmax || pn < pmin || pn > pmax || en < emin || en > emax) pass = false;
     };
 #pragma unroll
-    for (int n = 0; n < 8; ++n) node(vx[n]);
+    for (int v = 0; v < 8; ++v) node(vx[v]);
 #pragma unroll
     for (int f = 0; f < 6; ++f) node(fcx[f]);
     // edge midpoints: vertex pairs along x, then y, then z
 #pragma unroll
-    for (int e = 0; e < 12; ++e) {
-      const int va = e < 4 ? 2 * e : (e < 8 ? ((e - 4) & 1) + 4 * ((e - 4) >> 1) : e - 8);
-      const int vb = e < 4 ? va + 1 : (e < 8 ? va + 2 : va + 4);
+    for (int q = 0; q < 12; ++q) {
+      const int va = q < 4 ? 2 * q : (q < 8 ? ((q - 4) & 1) + 4 * ((q - 4) >> 1) : q - 8);
+      const int vb = q < 4 ? va + 1 : (q < 8 ? va + 2 : va + 4);
       double m[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) m[c] = (vx[va][c] + vx[vb][c]) * 0.5;
       node(m);
     }
     node(c0);
-    if (pass) break;
-    ++nred;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) {
-      rm[m] = ratio * rm[m];
-      pm[m] = ratio * pm[m];
-      em[m] = ratio * em[m];
-    }
+    return pass;
+  };
+  // the reference (440...f:553-600) checks 0, 1, ..., 14 reductions in turn and
+  // takes the first that passes, removing the slopes when none does.  With
+  // the zone's mean inside the bounds, passing is monotone in the number of
+  // reductions, so the first passing count is found by bisection: 2 to 6
+  // checks instead of up to 15
+  // one call site (the check is large): probe 0, then 14, then bisect; lo is
+  // the largest count known to fail, hi the smallest known to pass (15: none)
+  int lo = -1, hi = 15, n = 0;
+#pragma unroll 1
+  while (hi - lo > 1) {
+    const bool ok = passes(n);
+    if (ok) hi = n;
+    else lo = n;
+    n = lo < 0 ? 0 : (lo == 0 && hi == 15 ? 14 : (lo + hi) >> 1);
   }
+  const bool removed = hi == 15;
+  const int nred = removed ? 0 : hi;
+  if (!removed)
+    for (int q = 0; q < nred; ++q)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) em[m] = ratio * em[m];
   // the same reductions on every component (repeated, as the reference
   // multiplies once per iteration); B_zc copies the B modes
   if (nred > 0 || removed) {
