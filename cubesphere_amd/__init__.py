@@ -132,6 +132,7 @@ def lib():
             "cs_cleanup_after_update_p1": [vp, vp],
             "cs_make_edge_efields_p1": [vp, vp],
             "cs_build_non_stiff_sources_p1": [vp, vp],
+            "cs_pad_boundary_r_p1": [vp, vp],
             "cs_sync_shared_p1": [vp, vp],
             "cs_evaluate_timestep_p1": [vp, ctypes.c_double, ctypes.c_double, vp, vp],
             "cs_batch_create": [vp, i32, ctypes.POINTER(vp)],
@@ -795,6 +796,42 @@ def build_non_stiff_sources_p1(N, NR, g, newtonsc, mstar, smallnum, omega, array
 
 
 # ---------------------------------------------------------------------------
+# PAD_BOUNDARY_R of one patch (cs_pad_boundary_r_p1)
+
+class P1Pad(ctypes.Structure):
+    _fields_ = [("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("mask", ctypes.c_void_p), ("bcarr_r", ctypes.c_void_p),
+                ("xf_area", ctypes.c_void_p), ("yf_area", ctypes.c_void_p), ("zf_area", ctypes.c_void_p),
+                ("v_volume", ctypes.c_void_p), ("U", ctypes.c_void_p), ("Bxf", ctypes.c_void_p),
+                ("Byf", ctypes.c_void_p), ("Bzf", ctypes.c_void_p)]
+
+
+def pad_boundary_r_p1(N, NR, g, arrays, stream=None):
+    """PAD_BOUNDARY_R (826pad_boundary_r.f:1) on one patch of contiguous torch
+    tensors on the GPU: mask and bcarr_r (1-g:N+g, 1-g:N+g, 2) int32; the face
+    areas, v_volume, U (mode 1 first, any number of modes), Bxf, Byf, Bzf
+    float64.  Outflow pads (bcarr_r = 2) in place.  Sizes are checked first."""
+    import torch
+    want = p1_counts(N, NR, g)
+    nz = N + 2 * g
+    want.update(bcarr_r=2 * nz * nz, v_volume=want["mask"])
+    s = P1Pad()
+    s.ntp, s.nr, s.nghost = N, NR, g
+    for k in ("mask", "bcarr_r", "xf_area", "yf_area", "zf_area", "v_volume", "U", "Bxf", "Byf", "Bzf"):
+        t = arrays[k]
+        dt = torch.int32 if k in ("mask", "bcarr_r") else torch.float64
+        n = t.numel()
+        ok = n >= 8 * want["mask"] and n % want["mask"] == 0 if k == "U" else n == want[k]
+        if not (t.is_cuda and t.dtype == dt and t.is_contiguous() and ok):
+            raise CsError(f"pad_boundary_r_p1: {k} must be a contiguous {dt} CUDA tensor of "
+                          f"{'8 or more modes of ' if k == 'U' else ''}{want[k]} elements "
+                          f"(got {t.dtype}, {n}, cuda={t.is_cuda})")
+        setattr(s, k, t.data_ptr())
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    _check(lib().cs_pad_boundary_r_p1(ctypes.byref(s), st), "cs_pad_boundary_r_p1")
+
+
+# ---------------------------------------------------------------------------
 # Faces and edges the six P1 patches of a shell share (cs_sync_shared_p1)
 
 FACE_PAIR = np.dtype([("pa", "<i4"), ("fa", "<i4"), ("sa", "<i4"), ("pb", "<i4"), ("fb", "<i4"), ("sb", "<i4"),
@@ -954,7 +991,9 @@ class P1Shell:
     bookkeeping (DRV:2645-2773): stage 1 saves U,
     entropy and face B (mode 1) and moves by dt; stage 2 averages them with
     the saved state after the fluxes are formed and moves by dt / 2.
-    EVALUATE_TIMESTEP over the patches gives the next step.
+    EVALUATE_TIMESTEP over the patches gives the next step.  With pad=True
+    (the driver's single radial patch, DRV:2238-2275) PAD_BOUNDARY_R with
+    outflow at both radial ends follows each exchange.
 
     patches: six dicts of contiguous CUDA tensors holding every array of the
     P1 entry points (cs_p1_patch, cs_p1_flux, cs_p1_edge, cs_p1_source,
@@ -964,7 +1003,7 @@ class P1Shell:
     dU_dt arrays are work arrays of the step."""
 
     def __init__(self, cfg, patches, scalars, sources=(1.0, 0.0, (0.0, 0.0, 0.0)), itri=(1, 1, 1, 1),
-                 defect=True):
+                 defect=True, pad=True):
         import torch
         if len(patches) != 6:
             raise CsError("P1Shell: six patches (one per cube sector) required")
@@ -974,6 +1013,8 @@ class P1Shell:
         self.saved = [{k: torch.empty_like(p[k][: n]) for k, n in self._mode1(p).items()} for p in patches]
         self.dt_dev = torch.empty(6, dtype=torch.float64, device=patches[0]["U"].device)
         self.defect = defect
+        nz = self.N + 2 * self.g
+        self.bcarr = torch.full((2 * nz * nz,), 2, dtype=torch.int32, device=patches[0]["U"].device) if pad else None
         if defect:
             geom = [{k: p[k].cpu().numpy() for k in ("vert", "xf_centroid", "yf_centroid", "xf_normal", "yf_normal")}
                     for p in patches]
@@ -989,6 +1030,9 @@ class P1Shell:
 
     def exchange(self):
         self.solver.halo_exchange_p1(self.P)
+        if self.bcarr is not None:
+            for p in self.P:
+                pad_boundary_r_p1(self.N, self.NR, self.g, dict(p, bcarr_r=self.bcarr))
 
     def stage(self, irk, dt):
         N, NR, g = self.N, self.NR, self.g

@@ -2658,3 +2658,72 @@ extern "C" int cs_sync_shared_p1(const cs_p1_shell_sync* p, void* stream_) {
   P1_HIP(hipGetLastError());
   return 0;
 }
+
+// ---------------------------------------------------------------------------
+// PAD_BOUNDARY_R (800applications_cs/826pad_boundary_r.f:1) of one patch,
+// outflow pads.  One work item per (column, ghost layer izz); every write
+// reads only layer 1 / face 0 (inner) or layer nr / face nr (outer), and the
+// faces two columns share get the same value from both, so the items are
+// independent.
+namespace csp1 {
+
+__global__ void __launch_bounds__(256) k_pad_r(const cs_p1_pad P, int N, int NR, int g, int64_t NZ) {
+  const int64_t nz = N + 2 * g, nf = nz + 1;
+  const int64_t t = tid();
+  if (t >= nz * nz * g) return;
+  const int ix = (int)(t % nz) + 1 - g, iy = (int)((t / nz) % nz) + 1 - g, izz = (int)(t / (nz * nz)) + 1;
+  auto zi = [&](int i, int j, int k) { return (int64_t)(i - 1 + g) + nz * ((j - 1 + g) + nz * (k - 1 + g)); };
+  auto fc = [&](int i, int j, int k) { return (int64_t)(i + g) + nf * ((j + g) + nf * (k + g)); };
+  auto xa = [&](int i, int j, int k) { return (int64_t)(i + g) + nf * ((j - 1 + g) + nz * (k - 1 + g)); };
+  auto ya = [&](int i, int j, int k) { return (int64_t)(i - 1 + g) + nz * ((j + g) + nf * (k - 1 + g)); };
+  auto za = [&](int i, int j, int k) { return (int64_t)(i - 1 + g) + nz * ((j - 1 + g) + nz * (k + g)); };
+  if (P.mask[zi(ix, iy, izz)] != 1) return;
+  const int64_t col = (int64_t)(ix - 1 + g) + nz * (iy - 1 + g);
+  if (P.bcarr_r[col] == 2) {
+    const int l = 1 - izz, r = 1;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) P.U[zi(ix, iy, l) + NZ * c] = P.U[zi(ix, iy, r) + NZ * c];
+    P.Bxf[fc(ix, iy, l)] = P.Bxf[fc(ix, iy, r)];
+    P.Bxf[fc(ix - 1, iy, l)] = P.Bxf[fc(ix - 1, iy, r)];
+    P.Byf[fc(ix, iy, l)] = P.Byf[fc(ix, iy, r)];
+    P.Byf[fc(ix, iy - 1, l)] = P.Byf[fc(ix, iy - 1, r)];
+    P.Bzf[fc(ix, iy, l - 1)] = P.Bzf[fc(ix, iy, r - 1)];
+  }
+  if (P.bcarr_r[col + nz * nz] == 2) {
+    const int l = NR + izz, r = NR;
+    const double vr = P.v_volume[zi(ix, iy, r)], vl = P.v_volume[zi(ix, iy, l)];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) P.U[zi(ix, iy, l) + NZ * c] = P.U[zi(ix, iy, r) + NZ * c] * vr / vl;
+    P.Bzf[fc(ix, iy, l)] = P.Bzf[fc(ix, iy, r)] * P.zf_area[za(ix, iy, r)] / P.zf_area[za(ix, iy, l)];
+    P.Bxf[fc(ix, iy, l)] = P.Bxf[fc(ix, iy, r)] * P.xf_area[xa(ix, iy, r)] / P.xf_area[xa(ix, iy, l)];
+    P.Byf[fc(ix, iy, l)] = P.Byf[fc(ix, iy, r)] * P.yf_area[ya(ix, iy, r)] / P.yf_area[ya(ix, iy, l)];
+    P.Bxf[fc(ix - 1, iy, l)] =
+        P.Bxf[fc(ix - 1, iy, r)] * P.xf_area[xa(ix - 1, iy, r)] / P.xf_area[xa(ix - 1, iy, l)];
+    P.Byf[fc(ix, iy - 1, l)] =
+        P.Byf[fc(ix, iy - 1, r)] * P.yf_area[ya(ix, iy - 1, r)] / P.yf_area[ya(ix, iy - 1, l)];
+  }
+}
+
+}  // namespace csp1
+
+extern "C" int cs_pad_boundary_r_p1(const cs_p1_pad* p, void* stream_) {
+  if (!p) {
+    cs::set_error("cs_pad_boundary_r_p1: null argument");
+    return CS_E_INVALID;
+  }
+  if (p->ntp < 1 || p->nr < 1 || p->nghost < 1) {
+    cs::set_error("cs_pad_boundary_r_p1: need ntp, nr, nghost >= 1 (got %d, %d, %d)", p->ntp, p->nr, p->nghost);
+    return CS_E_INVALID;
+  }
+  if (!p->mask || !p->bcarr_r || !p->xf_area || !p->yf_area || !p->zf_area || !p->v_volume || !p->U || !p->Bxf ||
+      !p->Byf || !p->Bzf) {
+    cs::set_error("cs_pad_boundary_r_p1: null array");
+    return CS_E_INVALID;
+  }
+  const int64_t nz = p->ntp + 2 * p->nghost;
+  const int64_t NZ = nz * nz * (p->nr + 2 * p->nghost);
+  csp1::k_pad_r<<<grid_of(nz * nz * p->nghost, 256), 256, 0, (hipStream_t)stream_>>>(*p, p->ntp, p->nr,
+                                                                                       p->nghost, NZ);
+  P1_HIP(hipGetLastError());
+  return 0;
+}

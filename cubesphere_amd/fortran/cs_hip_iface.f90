@@ -12,6 +12,7 @@
 !   cs_make_edge_efields   <- MAKE_EDGE_EFIELDS_P1                          (DRV:2548)
 !   cs_make_edge_efields_p1 <- MAKE_EDGE_EFIELDS_P1 alone, on the arguments it reads (DRV:2548)
 !   cs_build_non_stiff_sources_p1 <- BUILD_NON_STIFF_SOURCES alone, the routine's arguments (DRV:2603)
+!   cs_pad_boundary_r_p1      <- PAD_BOUNDARY_R with outflow bcarr_r, one patch (DRV:2262)
 !   cs_update_imex_rk_step_p1 <- UPDATE_IMEX_RK_STEP_P1 alone, argument for argument (DRV:2782)
 !   cs_cleanup_after_update_p1 <- CLEANUP_AFTER_UPDATE alone, on the cs_p1_patch (DRV:2815)
 !   cs_evaluate_timestep_p1 <- EVALUATE_TIMESTEP alone, dt into device memory  (DRV:2887)
@@ -79,6 +80,13 @@ module cs_hip_iface
     type(c_ptr) :: mask, v_centroid, V, Source, dU_dt
   end type
 
+  ! cs_p1_pad: the arguments of PAD_BOUNDARY_R that it reads or writes
+  ! (826pad_boundary_r.f:1-16); bcarr_r is (1-g:N+g, 1-g:N+g, 2)
+  type, bind(C) :: cs_p1_pad
+    integer(c_int32_t) :: ntp, nr, nghost, pad
+    type(c_ptr) :: mask, bcarr_r, xf_area, yf_area, zf_area, v_volume, U, Bxf, Byf, Bzf
+  end type
+
   ! cs_p1_shell_sync: the six patches' face and edge arrays and the pairings
   ! (device arrays of cs_p1_face_pair / cs_p1_edge_group records)
   type, bind(C) :: cs_p1_patch_fluxes
@@ -120,6 +128,11 @@ module cs_hip_iface
     integer(c_int) function cs_build_non_stiff_sources_p1(p, stream) bind(C, name="cs_build_non_stiff_sources_p1")
       import :: c_int, c_ptr, cs_p1_source
       type(cs_p1_source), intent(in) :: p
+      type(c_ptr), value :: stream
+    end function
+    integer(c_int) function cs_pad_boundary_r_p1(p, stream) bind(C, name="cs_pad_boundary_r_p1")
+      import :: c_int, c_ptr, cs_p1_pad
+      type(cs_p1_pad), intent(in) :: p
       type(c_ptr), value :: stream
     end function
     integer(c_int) function cs_make_edge_efields_p1(p, stream) bind(C, name="cs_make_edge_efields_p1")

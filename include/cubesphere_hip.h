@@ -397,6 +397,27 @@ typedef struct cs_p1_source {
 } cs_p1_source;
 int cs_build_non_stiff_sources_p1(const cs_p1_source* p, void* stream);
 
+/* PAD_BOUNDARY_R (800applications_cs/826pad_boundary_r.f:1; the driver calls
+ * it after the exchange, 1000riemann_geomesh_cs_mhd.f:2238-2275) of one
+ * patch, outflow pads (bcarr_r = 2), no application branch.  For the ghost
+ * layers 1..NGhost below layer 1 and above layer nr of every zone column whose
+ * mask is 1: inner side, mode 1 of U, the column's Bxf (x-faces i-1, i) and
+ * Byf (y-faces j-1, j) copied from layer 1 and Bzf at radial faces -1..-NGhost
+ * from face 0; outer side, mode 1 of U scaled by V(nr) / V(ghost), Bzf from
+ * face nr and Bxf, Byf from layer nr scaled by the ratio of the face areas.
+ * bcarr_r is (1-g:N+g, 1-g:N+g, 2), inner then outer; a side whose code is
+ * not 2 is left alone.  The entropy is not touched (the reference routine
+ * takes it but does not write it).  Device arrays in the layouts of
+ * cs_p1_patch; U with any number of modes (only mode 1 is touched).  No wait
+ * on the stream. */
+typedef struct cs_p1_pad {
+  int32_t ntp, nr, nghost, pad;
+  const int32_t *mask, *bcarr_r;
+  const double *xf_area, *yf_area, *zf_area, *v_volume;
+  double *U, *Bxf, *Byf, *Bzf;
+} cs_p1_pad;
+int cs_pad_boundary_r_p1(const cs_p1_pad* p, void* stream);
+
 /* ---- P1 stage update of one patch ------------------------------------------
  * UPDATE_IMEX_RK_STEP_P1 (400riemann_integrate/484update_imex_rk_step_p1.f:7-26),
  * MHD: U(:, mode 1) += dtmove dU_dt on every zone; on the owned zones the

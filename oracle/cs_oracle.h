@@ -194,4 +194,15 @@ typedef struct or_p1_source {
 } or_p1_source;
 int or_build_non_stiff_sources_p1(const or_p1_source* s);
 
+/* PAD_BOUNDARY_R (826pad_boundary_r.f:1) of one patch, outflow pads; the same
+ * field order as cs_p1_pad in the product header.  bcarr_r is
+ * (1-g:N+g, 1-g:N+g, 2): inner, outer; 2 pads, anything else leaves the side */
+typedef struct or_p1_pad {
+  int32_t ntp, nr, nghost, pad;
+  const int32_t *mask, *bcarr_r;
+  const double *xf_area, *yf_area, *zf_area, *v_volume;
+  double *U, *Bxf, *Byf, *Bzf;
+} or_p1_pad;
+int or_pad_boundary_r_p1(const or_p1_pad* p);
+
 #endif

@@ -1544,3 +1544,62 @@ int or_build_non_stiff_sources_p1(const or_p1_source* s) {
       }
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* PAD_BOUNDARY_R (800applications_cs/826pad_boundary_r.f:1) of one patch,
+ * outflow (bcarr_r = 2) only, no application branch (CAK, helio, B0B1 off).
+ * For izz = 1..NGhost and every zone column whose mask at layer izz is 1:
+ *   inner (bcarr_r(.,.,1) = 2, :219-268, the "#else" copy): mode 1 of U at
+ *     layer 1 - izz copied from layer 1; Bxf at x-faces ixx-1 and ixx, Byf at
+ *     y-faces iyy-1 and iyy, copied from layer 1; Bzf at radial face -izz from
+ *     face 0;
+ *   outer (bcarr_r(.,.,2) = 2, :296-337): mode 1 of U at layer nr + izz from
+ *     layer nr scaled by V(nr) / V(nr + izz); Bzf at face nr + izz from face
+ *     nr, Bxf and Byf from layer nr, each scaled by the ratio of the face areas.
+ * The entropy argument is not written by the routine and is not an argument
+ * here.  Every write reads only layers 1 and nr and faces 0 and nr, so the
+ * order of the loop does not matter. */
+int or_pad_boundary_r_p1(const or_p1_pad* p) {
+  const int N = p->ntp, NR = p->nr, g = p->nghost;
+  const int64_t nz = N + 2 * g, nf = nz + 1, NZ = nz * nz * (NR + 2 * g);
+#define ZI(i, j, k) ((int64_t)((i) - 1 + g) + nz * (((j) - 1 + g) + nz * ((k) - 1 + g)))
+#define FC(i, j, k) ((int64_t)((i) + g) + nf * (((j) + g) + nf * ((k) + g)))
+#define XA(i, j, k) ((int64_t)((i) + g) + nf * (((j) - 1 + g) + nz * ((k) - 1 + g)))
+#define YA(i, j, k) ((int64_t)((i) - 1 + g) + nz * (((j) + g) + nf * ((k) - 1 + g)))
+#define ZA(i, j, k) ((int64_t)((i) - 1 + g) + nz * (((j) - 1 + g) + nz * ((k) + g)))
+#define BC(i, j, s) ((int64_t)((i) - 1 + g) + nz * (((j) - 1 + g) + nz * (s)))
+  for (int izz = 1; izz <= g; ++izz)
+    for (int iy = 1 - g; iy <= N + g; ++iy)
+      for (int ix = 1 - g; ix <= N + g; ++ix) {
+        if (p->mask[ZI(ix, iy, izz)] != 1) continue;
+        if (p->bcarr_r[BC(ix, iy, 0)] == 2) {
+          const int l = 1 - izz, r = 1;
+          for (int c = 0; c < 8; ++c) p->U[ZI(ix, iy, l) + NZ * c] = p->U[ZI(ix, iy, r) + NZ * c];
+          p->Bxf[FC(ix, iy, l)] = p->Bxf[FC(ix, iy, r)];
+          p->Bxf[FC(ix - 1, iy, l)] = p->Bxf[FC(ix - 1, iy, r)];
+          p->Byf[FC(ix, iy, l)] = p->Byf[FC(ix, iy, r)];
+          p->Byf[FC(ix, iy - 1, l)] = p->Byf[FC(ix, iy - 1, r)];
+          p->Bzf[FC(ix, iy, l - 1)] = p->Bzf[FC(ix, iy, r - 1)];
+        }
+        if (p->bcarr_r[BC(ix, iy, 1)] == 2) {
+          const int l = NR + izz, r = NR;
+          for (int c = 0; c < 8; ++c)
+            p->U[ZI(ix, iy, l) + NZ * c] =
+                p->U[ZI(ix, iy, r) + NZ * c] * p->v_volume[ZI(ix, iy, r)] / p->v_volume[ZI(ix, iy, l)];
+          p->Bzf[FC(ix, iy, l)] = p->Bzf[FC(ix, iy, r)] * p->zf_area[ZA(ix, iy, r)] / p->zf_area[ZA(ix, iy, l)];
+          p->Bxf[FC(ix, iy, l)] = p->Bxf[FC(ix, iy, r)] * p->xf_area[XA(ix, iy, r)] / p->xf_area[XA(ix, iy, l)];
+          p->Byf[FC(ix, iy, l)] = p->Byf[FC(ix, iy, r)] * p->yf_area[YA(ix, iy, r)] / p->yf_area[YA(ix, iy, l)];
+          p->Bxf[FC(ix - 1, iy, l)] =
+              p->Bxf[FC(ix - 1, iy, r)] * p->xf_area[XA(ix - 1, iy, r)] / p->xf_area[XA(ix - 1, iy, l)];
+          p->Byf[FC(ix, iy - 1, l)] =
+              p->Byf[FC(ix, iy - 1, r)] * p->yf_area[YA(ix, iy - 1, r)] / p->yf_area[YA(ix, iy - 1, l)];
+        }
+      }
+#undef ZI
+#undef FC
+#undef XA
+#undef YA
+#undef ZA
+#undef BC
+  return 0;
+}

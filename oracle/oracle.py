@@ -652,6 +652,67 @@ def p1s_oracle(prob, A):
 
 
 # ---------------------------------------------------------------------------
+# PAD_BOUNDARY_R of one patch (or_pad_boundary_r_p1 /
+# cubesphere_amd.pad_boundary_r_p1 share this structure layout)
+
+class P1Pad(ctypes.Structure):
+    _fields_ = [("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("mask", ctypes.c_void_p), ("bcarr_r", ctypes.c_void_p),
+                ("xf_area", ctypes.c_void_p), ("yf_area", ctypes.c_void_p), ("zf_area", ctypes.c_void_p),
+                ("v_volume", ctypes.c_void_p), ("U", ctypes.c_void_p), ("Bxf", ctypes.c_void_p),
+                ("Byf", ctypes.c_void_p), ("Bzf", ctypes.c_void_p)]
+
+
+P1P_ARRAYS = ("mask", "bcarr_r", "xf_area", "yf_area", "zf_area", "v_volume", "U", "Bxf", "Byf", "Bzf")
+
+
+def p1_bcarr_r(N, g, inner=2, outer=2):
+    """bcarr_r (1-g:N+g, 1-g:N+g, 2) with one code per side; the driver's
+    single radial patch pads both sides with 2 (DRV:2238-2247)."""
+    nz = N + 2 * g
+    b = np.empty(2 * nz * nz, dtype=np.int32)
+    b[:nz * nz], b[nz * nz:] = inner, outer
+    return b
+
+
+def p1_pad_problem(N, NR, g=4, sector=0, seed=0, inner=2, outer=2):
+    """A patch with seeded state everywhere (so every pad write is visible)
+    and bcarr_r set to (inner, outer) over the whole column range."""
+    prob, A = p1_problem(N, NR, g, sector=sector, seed=seed, shock=True, noise=0.3)
+    rng = np.random.default_rng(seed + 700)
+    for k in ("U", "Bxf", "Byf", "Bzf"):
+        A[k] = A[k] + 0.01 * rng.normal(size=A[k].size)
+    A["bcarr_r"] = p1_bcarr_r(N, g, inner, outer)
+    return prob, A
+
+
+def p1p_struct(prob, A):
+    s = P1Pad()
+    s.ntp, s.nr, s.nghost = prob["N"], prob["NR"], prob["g"]
+    for k in P1P_ARRAYS:
+        setattr(s, k, A[k].ctypes.data)
+    return s
+
+
+def p1p_oracle(prob, A):
+    L = olib()
+    L.or_pad_boundary_r_p1.argtypes = [ctypes.c_void_p]
+    if L.or_pad_boundary_r_p1(ctypes.byref(p1p_struct(prob, A))) != 0:
+        raise RuntimeError("oracle rejected the patch")
+    return A
+
+
+def p1p_reference(prob, A):
+    """The reference's PAD_BOUNDARY_R (flang build) in place on A."""
+    R = rlib()
+    if R is None:
+        raise RuntimeError("reference build not present")
+    R.ref_pad_boundary_r_p1.argtypes = [ctypes.c_void_p]
+    R.ref_pad_boundary_r_p1(ctypes.byref(p1p_struct(prob, A)))
+    return A
+
+
+# ---------------------------------------------------------------------------
 # the driver's time step over the six P1 patches of one shell, on the oracle
 
 class _Stacked:
@@ -740,16 +801,20 @@ def p1_shell_sync_oracle(As, faces, edges):
             As[q["p"][m]]["Ef" + _FAM[q["f"][m]]][q["idx"][m]] = q["sign"][m] * mean
 
 
-def p1_shell_step_oracle(grid, probs, As, dt, sources=(1.0, 0.0, (0.0, 0.0, 0.0)), pairings=None):
+def p1_shell_step_oracle(grid, probs, As, dt, sources=(1.0, 0.0, (0.0, 0.0, 0.0)), pairings=None, pad=True):
     """One Heun RK2 step of the driver (DRV:1746-3461) over the six patches,
     every routine the oracle's; with pairings (faces, edges) the defect
-    correction in its before-the-update form."""
+    correction in its before-the-update form; with pad the outflow
+    PAD_BOUNDARY_R at both radial ends after each exchange (DRV:2238-2275)."""
     nz = As[0]["ent"].size // 4
     keys = (("U", 8 * nz), ("ent", nz), ("Bxf", As[0]["Bxf"].size), ("Byf", As[0]["Byf"].size),
             ("Bzf", As[0]["Bzf"].size))
     saved = [None] * 6
     for irk in (1, 2):
         p1_shell_exchange_oracle(grid, As)
+        if pad:
+            for prob, A in zip(probs, As):
+                p1p_oracle(prob, dict(A, bcarr_r=p1_bcarr_r(prob["N"], prob["g"])))
         for s, (prob, A) in enumerate(zip(probs, As)):
             p1_oracle(prob, A)
             p1f_oracle(prob, A)

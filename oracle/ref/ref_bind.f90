@@ -19,6 +19,10 @@ module cs_ref_bind
   end type
 
   ! mirror of or_p1_edge / cs_p1_edge
+  type, bind(C) :: ref_p1_pad
+    integer(c_int32_t) :: ntp, nr, nghost, pad
+    type(c_ptr) :: mask, bcarr_r, xf_area, yf_area, zf_area, v_volume, U, Bxf, Byf, Bzf
+  end type
   type, bind(C) :: ref_p1_edge
     integer(c_int32_t) :: ntp, nr, nghost, pad
     integer(c_int32_t) :: itri(4)
@@ -388,6 +392,30 @@ contains
   end subroutine
 
   ! MAKE_EDGE_EFIELDS_P1 (460make_edge_fields_p1.f:5) on the fields of a
+  ! PAD_BOUNDARY_R (826pad_boundary_r.f:1) on the fields of a ref_p1_pad; U
+  ! carries mode 1 only (n_cc_modes 1), the arguments the routine does not read
+  ! (vertices, centroids, entropy) are zero-filled arrays of their sizes
+  subroutine ref_pad_boundary_r_p1(p) bind(C, name="ref_pad_boundary_r_p1")
+    type(ref_p1_pad), intent(in) :: p
+    integer :: n, nr, g, nz, nzr, nf, nfr, nzone, nxf, nzf, nfc
+    real(c_double), pointer :: xa(:), ya(:), za(:), vol(:), u(:), bxf(:), byf(:), bzf(:)
+    integer(c_int), pointer :: mask(:), bc(:)
+    real(c_double), allocatable :: vert(:), cen(:), ent(:)
+    external PAD_BOUNDARY_R
+    n = p%ntp; nr = p%nr; g = p%nghost
+    nz = n + 2 * g; nzr = nr + 2 * g; nf = nz + 1; nfr = nzr + 1
+    nzone = nz * nz * nzr; nxf = nf * nz * nzr; nzf = nz * nz * nfr; nfc = nf * nf * nfr
+    call c_f_pointer(p%mask, mask, [nzone]); call c_f_pointer(p%bcarr_r, bc, [2 * nz * nz])
+    call c_f_pointer(p%xf_area, xa, [nxf]); call c_f_pointer(p%yf_area, ya, [nxf])
+    call c_f_pointer(p%zf_area, za, [nzf]); call c_f_pointer(p%v_volume, vol, [nzone])
+    call c_f_pointer(p%U, u, [8 * nzone])
+    call c_f_pointer(p%Bxf, bxf, [nfc]); call c_f_pointer(p%Byf, byf, [nfc]); call c_f_pointer(p%Bzf, bzf, [nfc])
+    allocate(vert(3 * nfc), cen(3 * nzone), ent(4 * nzone))
+    vert = 0; cen = 0; ent = 0
+    call PAD_BOUNDARY_R(nr, n, g, 8, 1, 1, vert, mask, xa, ya, za, vol, cen, u, ent, bxf, byf, bzf, bc)
+    deallocate(vert, cen, ent)
+  end subroutine
+
   ! ref_p1_edge; the arguments the routine declares but does not read at second
   ! order are zero-filled arrays of their declared sizes
   subroutine ref_make_edge_efields_p1(p) bind(C, name="ref_make_edge_efields_p1")

@@ -714,3 +714,75 @@ def test_p1_shell_step_keeps_div_b():
         assert not any(np.isnan(A["U"]).any() for A in As)
     assert drift[True] < 1e-13, drift
     assert drift[False] > 1e-8, drift  # the defect is real, and the correction removes it
+
+
+# ---------------------------------------------------------------------------
+# PAD_BOUNDARY_R (826pad_boundary_r.f:1), outflow pads
+
+PAD_CASES = [
+    # N, NR, sector, seed, (inner, outer)
+    (8, 3, 0, 0, (2, 2)),
+    (10, 5, 4, 1, (2, 17)),
+    (9, 4, 2, 2, (17, 2)),
+    (7, 6, 5, 3, (17, 17)),
+]
+
+
+def _pad_ids(c):
+    return "N%d_nr%d_s%d_seed%d_bc%d-%d" % (c[0], c[1], c[2], c[3], c[4][0], c[4][1])
+
+
+@pytest.mark.parametrize("case", PAD_CASES, ids=_pad_ids)
+def test_pad_boundary_r_bit_exact_with_reference(case):
+    """or_pad_boundary_r_p1 against PAD_BOUNDARY_R: U (mode 1), Bxf, Byf, Bzf
+    bit for bit for each combination of padded sides."""
+    _ref_or_skip()
+    N, NR, s, seed, bc = case
+    prob, A0 = orc.p1_pad_problem(N, NR, 4, sector=s, seed=seed, inner=bc[0], outer=bc[1])
+    A = orc.p1p_oracle(prob, orc.p1_copy(A0))
+    B = orc.p1p_reference(prob, orc.p1_copy(A0))
+    for k in ("U", "Bxf", "Byf", "Bzf"):
+        np.testing.assert_array_equal(A[k], B[k], err_msg=k)
+    changed = sum(int((A[k] != A0[k]).sum()) for k in ("U", "Bxf", "Byf", "Bzf"))
+    assert (changed > 0) == (2 in bc)
+
+
+@pytest.mark.parametrize("case", PAD_CASES[:3], ids=_pad_ids)
+def test_pad_boundary_r_invariants(case):
+    """Without the reference build: the inner pad copies layer 1 and face 0,
+    the outer pad keeps U V and B A of layer nr (face nr), only radial ghost
+    layers of active columns change, and padding twice equals padding once."""
+    N, NR, s, seed, bc = case
+    g = 4
+    prob, A0 = orc.p1_pad_problem(N, NR, g, sector=s, seed=seed, inner=bc[0], outer=bc[1])
+    A = orc.p1p_oracle(prob, orc.p1_copy(A0))
+    nz, nf = N + 2 * g, N + 2 * g + 1
+    NZ = nz * nz * (NR + 2 * g)
+    U0 = A0["U"][:8 * NZ].reshape(8, NR + 2 * g, nz, nz)
+    U = A["U"][:8 * NZ].reshape(8, NR + 2 * g, nz, nz)
+    vol = A["v_volume"].reshape(NR + 2 * g, nz, nz)
+    act = A["mask"].reshape(NR + 2 * g, nz, nz)[g] == 1
+    Bz = A["Bzf"].reshape(NR + 2 * g + 1, nf, nf)
+    Bz0 = A0["Bzf"].reshape(NR + 2 * g + 1, nf, nf)
+    za = A["zf_area"].reshape(NR + 2 * g + 1, nz, nz)
+    for izz in range(1, g + 1):
+        lo, hi = g - izz, NR + g - 1 + izz  # zone layers 1 - izz and nr + izz
+        if bc[0] == 2:
+            np.testing.assert_array_equal(U[:, lo][:, act], U0[:, g][:, act])
+            np.testing.assert_array_equal(Bz[g - izz, 1:, 1:][act], Bz0[g, 1:, 1:][act])
+        else:
+            np.testing.assert_array_equal(U[:, lo], U0[:, lo])
+        if bc[1] == 2:
+            np.testing.assert_allclose((U[:, hi] * vol[hi])[:, act], (U0[:, NR + g - 1] * vol[NR + g - 1])[:, act],
+                                       rtol=1e-14, atol=1e-300)
+            np.testing.assert_allclose((Bz[NR + g + izz, 1:, 1:] * za[NR + g + izz])[act],
+                                       (Bz0[NR + g, 1:, 1:] * za[NR + g])[act], rtol=1e-14)
+        else:
+            np.testing.assert_array_equal(U[:, hi], U0[:, hi])
+    # owned layers untouched, inactive columns untouched
+    np.testing.assert_array_equal(U[:, g:NR + g], U0[:, g:NR + g])
+    np.testing.assert_array_equal(U[:, :, ~act], U0[:, :, ~act])
+    np.testing.assert_array_equal(Bz[g:NR + g + 1], Bz0[g:NR + g + 1])
+    B = orc.p1p_oracle(prob, orc.p1_copy(A))
+    for k in ("U", "Bxf", "Byf", "Bzf"):
+        np.testing.assert_array_equal(A[k], B[k], err_msg=k)

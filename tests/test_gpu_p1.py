@@ -284,6 +284,34 @@ def test_sources_gpu_match_oracle(gm, omega):
     np.testing.assert_array_equal(G["Source"] == 0, O["Source"] == 0)
 
 
+@pytest.mark.parametrize("N,NR,sector,seed,bc", [(12, 5, 0, 1, (2, 2)), (10, 4, 4, 2, (2, 17)),
+                                                (9, 6, 2, 3, (17, 2))])
+def test_pad_boundary_r_gpu_bit_exact(N, NR, sector, seed, bc):
+    """cs_pad_boundary_r_p1 against or_pad_boundary_r_p1 (which matches the
+    reference's PAD_BOUNDARY_R bit for bit): U with its four modes, Bxf, Byf,
+    Bzf bit for bit, for both sides padded and each side alone."""
+    print(f"\npad N={N} NR={NR} sector={sector} bc={bc}", flush=True)
+    prob, A0 = orc.p1_pad_problem(N, NR, 4, sector=sector, seed=seed, inner=bc[0], outer=bc[1])
+    O = orc.p1p_oracle(prob, orc.p1_copy(A0))
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(A0[k]).to(dev) for k in orc.P1P_ARRAYS}
+    cs.pad_boundary_r_p1(N, NR, 4, T)
+    torch.cuda.synchronize()
+    for k in ("U", "Bxf", "Byf", "Bzf"):
+        np.testing.assert_array_equal(T[k].cpu().numpy(), O[k], err_msg=k)
+
+
+def test_pad_boundary_r_gpu_rejects_bad_sizes():
+    """A U of fewer than eight components, or a bcarr_r of the wrong size, is
+    refused on the host."""
+    prob, A0 = orc.p1_pad_problem(8, 3, 4)
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(A0[k]).to(dev) for k in orc.P1P_ARRAYS}
+    for k, bad in (("U", T["U"][:7 * T["mask"].numel()]), ("bcarr_r", T["bcarr_r"][:-1])):
+        with pytest.raises(cs.CsError):
+            cs.pad_boundary_r_p1(8, 3, 4, dict(T, **{k: bad}))
+
+
 def test_p1_shell_steps_gpu_match_oracle():
     """Two Heun RK2 steps of the driver's loop over the six P1 patches of a
     shell (cubesphere_amd.P1Shell: cs_halo_exchange_p1, every P1 entry point
