@@ -579,7 +579,7 @@ def bench_p1_shell(cs, args):
         "unit": "cell-updates/s", "n_gpus": 1, "steps": steps, "warmup": args.warmup, "ms_per_step": ms,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"P1 shell: 6x{N}^2x{NR} shell (C3 size), six P1 patches, NGhost {g}, the driver's "
-                               "Heun RK2 step of the reference's routines with its defect correction; front + log-normal "
+                               "Heun RK2 step of the reference's routines with its defect correction and outflow radial pads; front + log-normal "
                                "noise 0.3", "zones": cells},
         "cpu_baseline": cpu,
     }
