@@ -15,7 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libcubesphere_hip.so")
 
-RECON_PCM, RECON_WENO5 = 0, 1
+RECON_PCM, RECON_WENO5, RECON_P1ADP = 0, 1, 2
 RS_HLL, RS_HLLD = 0, 1
 EMF_LLF, EMF_HLL = 0, 1
 FP64, MIXED = 0, 1
@@ -66,12 +66,17 @@ def workload(name, precision=FP64):
     """Return (config, problem id, steps) of a named workload."""
     if name == "C1":   # 6x32x32 surface, WENO5 + HLL + SSP-RK3, Gaussian bump + rotation
         return make_config(32, 1, 3, RECON_WENO5, RS_HLL, EMF_LLF, precision, 3), 1, 100
-    if name == "C2":   # 6x256^2 surface, WENO + HLLD + SSP-RK3, CT, magnetised blast
+    if name == "C2":   # 6x256^2 surface, WENO-ADP (the reference's P1 scheme) + HLLD + SSP-RK3, CT, magnetised blast
+        return make_config(256, 1, 4, RECON_P1ADP, RS_HLLD, EMF_LLF, precision, 3), 2, 500
+    if name == "C2W5":  # the same surface with the stage path's own WENO5
         return make_config(256, 1, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 3), 2, 500
     if name == "C3":   # 6x192^2x64 shell r in [1,1.5], WENO5 + HLLD + CT, IMEX-RK with gravity 1/r^2 implicit
         return make_config(192, 64, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 2,
                            RBC_PAD, RMESH_UNIFORM, 1.0, 1.5, grav=1.0, imex=1), 3, 10
-    if name == "C5":   # 6x512^2x128, the same scheme
+    if name == "C5":   # 6x512^2x128, WENO-ADP (the reference's P1 scheme) + HLLD + CT + IMEX-RK, gravity implicit
+        return make_config(512, 128, 4, RECON_P1ADP, RS_HLLD, EMF_LLF, precision, 2,
+                           RBC_PAD, RMESH_UNIFORM, 1.0, 1.5, grav=1.0, imex=1), 3, 50
+    if name == "C5W5":  # the same shell with the stage path's own WENO5
         return make_config(512, 128, 3, RECON_WENO5, RS_HLLD, EMF_LLF, precision, 2,
                            RBC_PAD, RMESH_UNIFORM, 1.0, 1.5, grav=1.0, imex=1), 3, 50
     raise ValueError(name)

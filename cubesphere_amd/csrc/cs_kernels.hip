@@ -12,11 +12,13 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <vector>
 
 #include "../../include/cubesphere_hip.h"
 #include "cs_halo_maps.h"
+#include "cs_p1_stage.h"
 #include "cs_internal.h"
 #include "cs_physics.h"
 
@@ -183,10 +185,12 @@ __device__ __forceinline__ void exchange_cells(const CsDims& d, const State& st,
   P w[9];       // primitives: the same function of the same zone
   for (int c = 0; c < 8; ++c) v[c] = st.U[(o * 8 + c) * d.cpp + sc];
   v[8] = st.ent[o * d.cpp + sc];
-  for (int c = 0; c < 9; ++c) w[c] = V[((int64_t)o * 9 + c) * d.cpp + sc];
+  if (V)
+    for (int c = 0; c < 9; ++c) w[c] = V[((int64_t)o * 9 + c) * d.cpp + sc];
   for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = v[c];
   st.ent[s * d.cpp + dc] = v[8];
-  for (int c = 0; c < 9; ++c) V[((int64_t)s * 9 + c) * d.cpp + dc] = w[c];
+  if (V)
+    for (int c = 0; c < 9; ++c) V[((int64_t)s * 9 + c) * d.cpp + dc] = w[c];
 }
 
 // Face-centred ghost strips with the face-B swap and sign rules
@@ -241,10 +245,12 @@ __device__ __forceinline__ void corner_fill(const CsDims& d, const State& st, P*
   P w[9];
   for (int c = 0; c < 8; ++c) v[c] = st.U[(s * 8 + c) * d.cpp + sc];
   v[8] = st.ent[s * d.cpp + sc];
-  for (int c = 0; c < 9; ++c) w[c] = V[((int64_t)s * 9 + c) * d.cpp + sc];
+  if (V)
+    for (int c = 0; c < 9; ++c) w[c] = V[((int64_t)s * 9 + c) * d.cpp + sc];
   for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = v[c];
   st.ent[s * d.cpp + dc] = v[8];
-  for (int c = 0; c < 9; ++c) V[((int64_t)s * 9 + c) * d.cpp + dc] = w[c];
+  if (V)
+    for (int c = 0; c < 9; ++c) V[((int64_t)s * 9 + c) * d.cpp + dc] = w[c];
 }
 
 // Thread counts of the three parts of the lateral exchange.
@@ -332,6 +338,7 @@ __device__ __forceinline__ void radial_bc(const CsDims& d, const Geo& geo, const
     for (int c = 0; c < 8; ++c) st.U[(s * 8 + c) * d.cpp + dc] = u[c];
     double ent = st.ent[s * d.cpp + sc];
     st.ent[s * d.cpp + dc] = ent;
+    if (!V) return;  // no primitives kept (the P1 stage converts every zone itself)
     // primitives of the ghost zone, as k_prims converts
     Gas<double> gs{pr.gamma, pr.gm1, pr.rhofloor, pr.prsfloor, pr.igm1};
     double v[8];
@@ -985,45 +992,15 @@ __device__ __forceinline__ void finish_zone(const CsDims& d, const State& in, co
                                             const double* L, double* u, double& en) {
   // L[0..7]: rho, rho v, E, B (zone B advanced by the induction fluxes, as the
   // reference's dU_dt; the cleanup replaces it); L[8]: entropy
-  double ui[8];
+  double ui[8], uz[8];
   for (int q = 0; q < 8; ++q) ui[q] = in.U[(s * 8 + q) * d.cpp + c];
   const double ei = in.ent[s * d.cpp + c];
-  if (mode == 0) {
-    double Lg[5] = {L[0], L[1], L[2], L[3], L[4]};
-    if (gr != 0.0) {
-      const double mr = ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2];
-      for (int q = 0; q < 3; ++q) Lg[1 + q] += ui[0] * gr * n[q];
-      Lg[4] += mr * gr;
-    }
-    for (int q = 0; q < 8; ++q) {
-      double un = ui[q] + dt * (q < 5 ? Lg[q] : L[q]);
-      u[q] = wa != 0.0 ? wa * u0.U[(s * 8 + q) * d.cpp + c] + wb * un : wb * un;
-    }
-    double e1 = ei + dt * L[8];
-    en = wa != 0.0 ? wa * u0.ent[s * d.cpp + c] + wb * e1 : wb * e1;
-    return;
+  double ez = 0.0;
+  if (mode != 0 || wa != 0.0) {
+    for (int q = 0; q < 8; ++q) uz[q] = u0.U[(s * 8 + q) * d.cpp + c];
+    ez = u0.ent[s * d.cpp + c];
   }
-  const double gi = 1.0 - 0.70710678118654752440;
-  double uz[8];
-  for (int q = 0; q < 8; ++q) uz[q] = u0.U[(s * 8 + q) * d.cpp + c];
-  const double ez = u0.ent[s * d.cpp + c];
-  if (mode == 1) {
-    const double s1m = ui[0] * gr, s1e = gr * (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]);
-    u[0] = uz[0] + dt * L[0];
-    for (int q = 1; q < 4; ++q) u[q] = uz[q] + dt * L[q] + (1.0 - 2.0 * gi) * dt * s1m * n[q - 1];
-    u[4] = uz[4] + dt * L[4] + (1.0 - 2.0 * gi) * dt * s1e;
-    for (int q = 1; q < 4; ++q) u[q] += gi * dt * u[0] * gr * n[q - 1];
-    u[4] += gi * dt * gr * (u[1] * n[0] + u[2] * n[1] + u[3] * n[2]);
-    for (int q = 5; q < 8; ++q) u[q] = uz[q] + dt * L[q];
-    en = ez + dt * L[8];
-    return;
-  }
-  const double s2m = ui[0] * gr, s2e = gr * (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]);
-  const double s1m = uz[0] * gr, s1e = gr * ((uz[1] * n[0] + uz[2] * n[1] + uz[3] * n[2]) + gi * dt * uz[0] * gr);
-  for (int q = 0; q < 8; ++q) u[q] = 0.5 * uz[q] + 0.5 * (ui[q] + dt * L[q]);
-  for (int q = 1; q < 4; ++q) u[q] += 0.5 * dt * (2.0 * gi * s1m + (1.0 - gi) * s2m) * n[q - 1];
-  u[4] += 0.5 * dt * (2.0 * gi * s1e + (1.0 - gi) * s2e);
-  en = 0.5 * ez + 0.5 * (ei + dt * L[8]);
+  rk_combine(ui, ei, uz, ez, dt, wa, wb, mode, gr, n, L, u, en);
 }
 
 // First, purely implicit stage of IMEX-SSP2(2,2,2): U1 = Un + gi dt S(U1) in
@@ -1519,6 +1496,7 @@ struct cs_handle {
   // so the next stage needs no exchange (-1: none)
   int vghost = -1;
   struct Prof* prof = nullptr;  // per-launch event pairs while cs_profile_steps runs
+  p1s::Stage* p1 = nullptr;     // the P1 stage's buffers (recon CS_RECON_P1ADP)
   int bs[CS_K_COUNT];           // workgroup size per kernel id (cs_autotune)
 };
 
@@ -1540,6 +1518,9 @@ State state_of(cs_handle* h, int b) {
   return State{h->buf[b][0], h->buf[b][1], h->buf[b][2], h->buf[b][3], h->buf[b][4]};
 }
 Geo geo_of(cs_handle* h) { return Geo{h->T, h->rb, h->d.t2 * 6, h->d.t2}; }
+p1s::Buf p1buf(cs_handle* h, int b) {
+  return p1s::Buf{h->buf[b][0], h->buf[b][1], h->buf[b][2], h->buf[b][3], h->buf[b][4]};
+}
 
 void launch_avg(cs_handle* h, int b) {
   const CsDims& d = h->d;
@@ -1803,7 +1784,16 @@ int ensure_prims(cs_handle* h, int b) {
   if (h->vbuf == b) return CS_OK;
   return h->cfg.precision == CS_MIXED ? launch_prims_interior_t<float>(h, b) : launch_prims_interior_t<double>(h, b);
 }
-int launch_cfl(cs_handle* h) { return launch_kernel(h, CS_K_CFL, h->cur, 1); }
+int launch_cfl(cs_handle* h) {
+  if (h->p1) {
+    int rc = p1s::cfl_partial(h->p1, p1buf(h, h->cur), h->part, h->stream);
+    if (rc) return rc;
+    k_cfl_final<<<1, 1024, 0, h->stream>>>(h->part, p1s::nblocks_cfl(), h->dt, 1.1);
+    CS_HIP(hipGetLastError());
+    return CS_OK;
+  }
+  return launch_kernel(h, CS_K_CFL, h->cur, 1);
+}
 
 // IMEX-SSP2(2,2,2) implicit first stage into the scratch buffer: its faces
 // are copied from U^n (the stage leaves B alone), then U1 with its cleanup.
@@ -1826,7 +1816,30 @@ int launch_imex_source(cs_handle* h) {
   return h->cfg.precision == CS_MIXED ? launch_imex_source_t<float>(h) : launch_imex_source_t<double>(h);
 }
 
+// The stage of the reference's P1 scheme (recon CS_RECON_P1ADP): the full
+// exchange of the stage input, the reconstruction, face fluxes and edge
+// fields of the six sectors, the panel-edge averages of the fluxes and edge
+// fields (k_sync_shared), the update; the IMEX stages begin with the
+// implicit source stage.
+int launch_stage_p1(cs_handle* h, int irk) {
+  double wa, wb;
+  int bi, bo, rc;
+  stage_weights(h->cfg.rk_order, irk, wa, wb, bi, bo, h->pr.imex);
+  const int b = (h->cur + bi) & 1, bz = h->cur, bt = (h->cur + bo) & 1;
+  if (h->pr.imex && irk == 1 &&
+      (rc = p1s::imex_source(h->p1, p1buf(h, h->cur), p1buf(h, (h->cur + 1) & 1), h->dt, h->stream)))
+    return rc;
+  // with the average of the two copies of every panel-edge face, as each
+  // exchange of the reference (330fetch_buffers_fill_in_ux.f:337-350)
+  if ((rc = launch_halo(h, b, true, true, true))) return rc;
+  if ((rc = p1s::front(h->p1, p1buf(h, b), h->F, h->S, h->E, h->flags + 2, h->stream))) return rc;
+  if ((rc = launch_sync(h, true, true))) return rc;
+  return p1s::update(h->p1, p1buf(h, b), p1buf(h, bz), p1buf(h, bt), h->F, h->E, h->dt, wa, wb,
+                     h->pr.imex ? irk : 0, h->stream);
+}
+
 int launch_stage(cs_handle* h, int irk) {
+  if (h->p1) return launch_stage_p1(h, irk);
   int b = stage_input(h, irk);
   int rc;
   const bool surf = h->surf != 0;
@@ -1863,6 +1876,8 @@ void free_all(cs_handle* h) {
   hipFree(h->V);
   for (int q = 0; q < 3; ++q) { hipFree(h->F[q]); hipFree(h->S[q]); hipFree(h->E[q]); }
   hipFree(h->dt); hipFree(h->part); hipFree(h->flags);
+  p1s::destroy(h->p1);
+  h->p1 = nullptr;
   if (h->stream) hipStreamDestroy(h->stream);
 }
 
@@ -1958,15 +1973,24 @@ extern "C" int cs_create(const cs_config* cfg, cs_handle** out) {
     ALLOC(h->buf[b][1], cb);
     for (int q = 2; q < 5; ++q) ALLOC(h->buf[b][q], fb);
   }
-  size_t vb = (cfg->precision == CS_MIXED ? sizeof(float) : sizeof(double)) * d.cpp * 6 * 9;
-  ALLOC(h->V, vb);
+  if (cfg->recon == CS_RECON_P1ADP) {
+    size_t pb = 0;
+    if ((rc = p1s::create(cfg, &h->p1, &pb))) {
+      free_all(h);
+      delete h;
+      return rc;
+    }
+  } else {
+    size_t vb = (cfg->precision == CS_MIXED ? sizeof(float) : sizeof(double)) * d.cpp * 6 * 9;
+    ALLOC(h->V, vb);
+  }
   for (int q = 0; q < 3; ++q) {
     ALLOC(h->F[q], fb * 9);
     ALLOC(h->S[q], fb * 2);
     ALLOC(h->E[q], fb);
   }
   ALLOC(h->dt, sizeof(double) * 2);
-  h->npart = (int64_t)nblk((uint64_t)6 * d.N * d.N * d.NR) + 16;
+  h->npart = std::max<int64_t>((int64_t)nblk((uint64_t)6 * d.N * d.N * d.NR) + 16, p1s::nblocks_cfl());
   ALLOC(h->part, sizeof(double) * h->npart * 18);
   ALLOC(h->flags, sizeof(int) * 4);
   if (hipMemcpy(h->T, T.data(), sizeof(double) * T.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1991,6 +2015,11 @@ extern "C" int cs_destroy(cs_handle* h) {
 #define CHECK_H(h)                                  \
   do {                                              \
     if (!(h)) { set_error("null handle"); return CS_E_INVALID; } \
+  } while (0)
+// entry points of the WENO5/PCM stage path only
+#define STAGE_PATH_ONLY(h, name)                                                             \
+  do {                                                                                      \
+    if ((h)->p1) { set_error("%s: not available with CS_RECON_P1ADP", name); return CS_E_INVALID; } \
   } while (0)
 
 extern "C" int cs_upload_state(cs_handle* h, const double* U, const double* ent, const double* Bxf,
@@ -2100,6 +2129,7 @@ extern "C" int cs_halo_exchange_p1(cs_handle* h, double* const* U, double* const
 
 extern "C" int cs_make_facial_fluxes(cs_handle* h) {
   CHECK_H(h);
+  STAGE_PATH_ONLY(h, "cs_make_facial_fluxes");
   int rc = launch_fluxes(h, h->cur);
   if (rc) return rc;
   CS_HIP(hipStreamSynchronize(h->stream));
@@ -2108,6 +2138,7 @@ extern "C" int cs_make_facial_fluxes(cs_handle* h) {
 
 extern "C" int cs_make_edge_efields(cs_handle* h) {
   CHECK_H(h);
+  STAGE_PATH_ONLY(h, "cs_make_edge_efields");
   int rc = launch_emf(h, h->cur);
   if (rc) return rc;
   CS_HIP(hipStreamSynchronize(h->stream));
@@ -2116,6 +2147,7 @@ extern "C" int cs_make_edge_efields(cs_handle* h) {
 
 extern "C" int cs_update_stage(cs_handle* h, int32_t irk) {
   CHECK_H(h);
+  STAGE_PATH_ONLY(h, "cs_update_stage");
   if (irk < 1 || irk > h->cfg.rk_order) { set_error("irk %d out of range", irk); return CS_E_INVALID; }
   if (h->pr.imex) {
     set_error("implicit gravity needs the implicit first stage: advance with cs_rk_stage or cs_advance");
@@ -2165,6 +2197,7 @@ static int check_flags(cs_handle* h) {
   CS_HIP(hipMemcpyAsync(fl, h->flags, sizeof(fl), hipMemcpyDeviceToHost, h->stream));
   CS_HIP(hipStreamSynchronize(h->stream));
   if (fl[1]) { set_error("NaN/Inf in the state"); return CS_E_NAN; }
+  if (fl[2]) { set_error("the positivity limiter's loop-order replay did not settle within its sweeps"); return CS_E_INVALID; }
   return CS_OK;
 }
 
@@ -2176,7 +2209,7 @@ static int run_steps(cs_handle* h, int nsteps, int use_graph) {
     return CS_OK;
   }
   // the captured step expects the primitives of U^n in V and leaves them there
-  if ((rc = ensure_prims(h, h->cur))) return rc;
+  if (!h->p1 && (rc = ensure_prims(h, h->cur))) return rc;
   if (!h->graph) {
     hipGraph_t gr;
     CS_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
@@ -2276,6 +2309,7 @@ extern "C" int cs_time_steps(cs_handle* h, int32_t nsteps, int32_t use_graph, fl
 
 extern "C" int cs_profile_steps(cs_handle* h, int32_t nsteps, float* ms_sum, int32_t* calls) {
   CHECK_H(h);
+  STAGE_PATH_ONLY(h, "cs_profile_steps");
   if (!ms_sum || !calls || nsteps < 1) { set_error("bad arguments"); return CS_E_INVALID; }
   for (int q = 0; q < CS_K_COUNT; ++q) { ms_sum[q] = 0.f; calls[q] = 0; }
   Prof* p = new Prof();
@@ -2309,6 +2343,7 @@ extern "C" int cs_profile_steps(cs_handle* h, int32_t nsteps, float* ms_sum, int
 
 extern "C" int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float* ms_avg) {
   CHECK_H(h);
+  STAGE_PATH_ONLY(h, "cs_time_kernel");
   if (!ms_avg || iters < 1 || kernel < 0 || kernel >= CS_K_COUNT) {
     set_error("bad arguments");
     return CS_E_INVALID;
@@ -2341,6 +2376,7 @@ extern "C" int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float
 
 extern "C" int cs_autotune(cs_handle* h, int32_t iters, int32_t* block_sizes) {
   CHECK_H(h);
+  STAGE_PATH_ONLY(h, "cs_autotune");
   if (iters < 1) { set_error("bad arguments"); return CS_E_INVALID; }
   // the launches of a stage; the CFL reduction keeps 256 (its partial sums
   // are sized for it)
@@ -2367,6 +2403,7 @@ extern "C" int cs_autotune(cs_handle* h, int32_t iters, int32_t* block_sizes) {
 
 extern "C" int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units) {
   CHECK_H(h);
+  STAGE_PATH_ONLY(h, "cs_kernel_units");
   if (!units || kernel < 0 || kernel >= CS_K_COUNT) { set_error("bad arguments"); return CS_E_INVALID; }
   const CsDims& d = h->d;
   const Ranges r = ranges_of(d, h->pr.wall);
@@ -2590,6 +2627,11 @@ void batch_free(cs_batch* B) {
 extern "C" int cs_batch_create(const cs_config* cfgs, int32_t n, cs_batch** out) {
   if (!out || !cfgs || n < 1) { set_error("bad arguments"); return CS_E_INVALID; }
   *out = nullptr;
+  for (int b = 0; b < n; ++b)
+    if (cfgs[b].recon == CS_RECON_P1ADP) {
+      set_error("shell %d: batches run the WENO5/PCM stage path (CS_RECON_P1ADP shells step on their own handles)", b);
+      return CS_E_INVALID;
+    }
   for (int b = 1; b < n; ++b)
     if (cfgs[b].precision != cfgs[0].precision || cfgs[b].rk_order != cfgs[0].rk_order ||
         cfgs[b].imex != cfgs[0].imex) {

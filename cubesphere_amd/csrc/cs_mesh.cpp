@@ -389,7 +389,7 @@ static int check_cfg(const cs_config* c) {
     set_error("implicit gravity (imex = 1) is the IMEX-SSP2(2,2,2) scheme: it needs rk_order = 2");
     return CS_E_INVALID;
   }
-  if (c->recon < 0 || c->recon > 1 || c->riemann < 0 || c->riemann > 1 || c->emf < 0 || c->emf > 1 ||
+  if (c->recon < 0 || c->recon > 2 || c->riemann < 0 || c->riemann > 1 || c->emf < 0 || c->emf > 1 ||
       c->precision < 0 || c->precision > 1 || c->radial_bc < 0 || c->radial_bc > 1 ||
       c->radial_mesh < 0 || c->radial_mesh > 1) {
     set_error("bad scheme selector");

@@ -1,0 +1,1672 @@
+// cs_p1_stage.hip -- the reference's second-order P1 scheme as the RK stage of
+// a cs_handle (recon CS_RECON_P1ADP): the multi-D WENO of the primitives and
+// the entropy with the flattener and the pressure-positivity limiter, the
+// divergence-preserving constrained least squares for the zone B
+// (LIMIT_ALL_SLOPES_P1, 410limit_all_slopes_p1.f:9 with 440...f:6), the face
+// fluxes at the face centroids (MAKE_FACIAL_FLUXES, 450make_facial_fluxes.f:5,
+// with the HLL of OneD_RS_MHD_Ptwise or HLLD), the edge fields
+// (MAKE_EDGE_EFIELDS_P1, 460make_edge_fields_p1.f:5), the update with the
+// arc-length constrained transport (UPDATE_IMEX_RK_STEP_P1, 484...f:7),
+// CLEANUP_AFTER_UPDATE (485...f:4) and EVALUATE_TIMESTEP (490...f:7).
+//
+// Differences from the per-patch entry points of cs_slopes_p1.hip, which keep
+// the reference's routines argument for argument:
+//  * all six sectors in each launch (the sector in blockIdx.y, or packed in a
+//    list entry), no host synchronisation anywhere: the positivity limiter's
+//    loop-order replay is a bounded number of device-side sweeps that return
+//    at once when there is nothing to replay, so a whole step is one graph;
+//  * the face geometry comes from separable tables (cs_p1geo.h) instead of
+//    ~90 stored words per zone; the zone centroid, volume and scale length
+//    are stored (5 words);
+//  * the state is the handle's (mode 1 only); the slopes of the 8 primitives
+//    and the entropy live in V (4 modes x 9 fields); the WENO of the zone E
+//    (410...f:5437-7100) is not formed: at second order nothing reads it;
+//  * the face fluxes are stored once per face as flux x area (the resolved
+//    flux of the HLL or HLLD solve in the face frame, rotated back), so both
+//    zones of a face take the same flux; the handle averages the two copies
+//    of each panel-edge face and edge before the update (the reference's
+//    defect correction in its before-the-update form) and the update gathers
+//    the six faces of each owned zone.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/cubesphere_hip.h"
+#include "cs_internal.h"
+#include "cs_p1_common.h"
+#include "cs_p1_stage.h"
+#include "cs_p1geo.h"
+#include "cs_p1s_host.h"
+#include "cs_physics.h"
+
+using namespace csp;
+using namespace csp1;
+
+namespace cs {
+void set_error(const char* fmt, ...);
+}
+
+namespace p1s {
+
+constexpr int kSweeps = 12;          // bound on the positivity replay sweeps per stage
+constexpr int kListBlocks = 1024;    // grid of the list kernels (grid-stride)
+constexpr int kCflBlocks = 1024;     // grid of the time-step reduction
+// counters: [0] zones whose slopes pass 0 removed, [1] list length,
+// [2] settled, [3 .. 3 + kSweeps) flag changes per sweep
+constexpr int kCtr = 3 + kSweeps;
+
+struct Dev {
+  int N, NR, g, nz, nzr, nf, nfr;
+  int64_t NZ, NFC, NZ2;
+  csg1::P1Geo G;
+  const double* zg;        // [s][5][zone]: centroid (3), volume, scale length
+  const int32_t* mask;     // [nz * nz] active zone columns (the same in every sector)
+  const int32_t* stencil;  // [24][nz * nz] (STENCIL_INDEXING_TRICORNERS)
+  double* V;               // [m][s][9][zone]: mode m of the 8 primitives and the entropy
+  double *flat, *work, *ms;  // [s][zone]
+  double* Bq;              // [s][fam][5][face]: normal B at the B-field points
+  uint8_t *fa, *prev, *next, *ever, *inlist;  // [s][zone]
+  uint32_t* list;          // packed s * NZ + zone
+  unsigned int* ctr;
+  double gamma, rhofloor, prsfloor, smallnum, flatten_coef, prs_flatten_coef, grav;
+  int wall, riemann;
+
+  __device__ __forceinline__ int64_t zi(int i, int j, int k) const {
+    return (int64_t)(i - 1 + g) + (int64_t)nz * ((j - 1 + g) + (int64_t)nz * (k - 1 + g));
+  }
+  __device__ __forceinline__ int64_t fci(int i, int j, int k) const {
+    return (int64_t)(i + g) + (int64_t)nf * ((j + g) + (int64_t)nf * (k + g));
+  }
+  __device__ __forceinline__ int64_t col(int i, int j) const { return (int64_t)(i - 1 + g) + (int64_t)nz * (j - 1 + g); }
+  __device__ __forceinline__ bool active(int i, int j) const { return mask[col(i, j)] == 1; }
+  // stencil entry n (R L U D RR LL UU DD RU LU LD RD) of zone column (i, j),
+  // clamped to the zone range
+  __device__ __forceinline__ void nbr(int i, int j, int n, int& ni, int& nj) const {
+    const int64_t c = col(i, j);
+    ni = min(max(stencil[c + NZ2 * (2 * n)], 1 - g), N + g);
+    nj = min(max(stencil[c + NZ2 * (2 * n + 1)], 1 - g), N + g);
+  }
+  __device__ __forceinline__ double& v(int m, int s, int c, int64_t z) const {
+    return V[(((int64_t)m * 6 + s) * 9 + c) * NZ + z];
+  }
+  __device__ __forceinline__ double cen(int s, int64_t z, int c) const { return zg[((int64_t)s * 5 + c) * NZ + z]; }
+  __device__ __forceinline__ double vol(int s, int64_t z) const { return zg[((int64_t)s * 5 + 3) * NZ + z]; }
+  __device__ __forceinline__ double obl(int s, int64_t z) const {
+    return 1.0 / fmax(zg[((int64_t)s * 5 + 4) * NZ + z], smallnum);
+  }
+  __device__ __forceinline__ double scale(int s, int64_t z) const { return zg[((int64_t)s * 5 + 4) * NZ + z]; }
+  __device__ __forceinline__ double* bq(int s, int fam, int p) const {
+    return Bq + (((int64_t)s * 3 + fam) * 5 + p) * NFC;
+  }
+  __device__ __forceinline__ void zone_of(int64_t z, int& i, int& j, int& k) const {
+    i = (int)(z % nz) + 1 - g;
+    j = (int)((z / nz) % nz) + 1 - g;
+    k = (int)(z / ((int64_t)nz * nz)) + 1 - g;
+  }
+  __device__ __forceinline__ Gas<double> gas() const {
+    return Gas<double>{gamma, gamma - 1.0, rhofloor, prsfloor, 1.0 / (gamma - 1.0)};
+  }
+};
+
+struct Buf6 {
+  double *U, *ent, *Bx, *By, *Bz;
+  __device__ __forceinline__ double& u(const Dev& D, int s, int c, int64_t z) const {
+    return U[((int64_t)s * 8 + c) * D.NZ + z];
+  }
+  __device__ __forceinline__ double& e(const Dev& D, int s, int64_t z) const { return ent[(int64_t)s * D.NZ + z]; }
+  __device__ __forceinline__ double* b(int fam) const { return fam == 0 ? Bx : (fam == 1 ? By : Bz); }
+};
+
+// a box of zone, face or edge indices of one sector
+struct Box {
+  int i0, j0, k0, ni, nj, nk;
+  __device__ __forceinline__ bool decode(int64_t t, int& i, int& j, int& k) const {
+    if (t >= (int64_t)ni * nj * nk) return false;
+    i = i0 + (int)(t % ni);
+    j = j0 + (int)((t / ni) % nj);
+    k = k0 + (int)(t / ((int64_t)ni * nj));
+    return true;
+  }
+  __host__ __device__ int64_t count() const { return (int64_t)ni * nj * nk; }
+};
+static Box make_box(int i0, int i1, int j0, int j1, int k0, int k1) {
+  return Box{i0, j0, k0, i1 - i0 + 1, j1 - j0 + 1, k1 - k0 + 1};
+}
+
+__device__ __forceinline__ int64_t tid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
+
+__device__ __forceinline__ bool in_weno_range(const Dev& D, int i, int j, int k) {
+  return i >= -2 && i <= D.N + 3 && j >= -2 && j <= D.N + 3 && k >= -2 && k <= D.NR + 3;
+}
+__device__ __forceinline__ bool in_flat_range(const Dev& D, int i, int j, int k) {
+  const int lo = 2 - D.g, hx = D.N + D.g - 1, hr = D.NR + D.g - 1;
+  return i >= lo && i <= hx && j >= lo && j <= hx && k >= lo && k <= hr;
+}
+__device__ __forceinline__ void six(const Dev& D, int i, int j, int k, int64_t* z6) {
+  int a, b;
+  D.nbr(i, j, 0, a, b);
+  z6[0] = D.zi(a, b, k);
+  D.nbr(i, j, 1, a, b);
+  z6[1] = D.zi(a, b, k);
+  D.nbr(i, j, 2, a, b);
+  z6[2] = D.zi(a, b, k);
+  D.nbr(i, j, 3, a, b);
+  z6[3] = D.zi(a, b, k);
+  z6[4] = D.zi(i, j, k + 1);
+  z6[5] = D.zi(i, j, k - 1);
+}
+
+// ---------------------------------------------------------------------------
+// set-up: zone geometry (once)
+__global__ void k_zone_geo(Dev D, Box b, double* zg) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  const int64_t z = D.zi(i, j, k);
+  double c[3], v, l;
+  csg1::zone_geometry(D.G, s, i, j, k, c, &v, &l);
+  // the inactive tri-corner zones get smallnum (410...f step 2, 485...f)
+  if (!D.active(i, j)) l = D.smallnum;
+  for (int q = 0; q < 3; ++q) zg[((int64_t)s * 5 + q) * D.NZ + z] = c[q];
+  zg[((int64_t)s * 5 + 3) * D.NZ + z] = v;
+  zg[((int64_t)s * 5 + 4) * D.NZ + z] = l;
+}
+
+// ---------------------------------------------------------------------------
+// steps 2 and 5 of 410: inactive zones made safe, CONS_TO_PRIM, magnetosonic
+// speed; the slopes of zones outside the WENO range or in inactive columns
+// are cleared (the WENO writes the others)
+__global__ void __launch_bounds__(256) k_prim(Dev D, Box b, Buf6 in) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  const int64_t z = D.zi(i, j, k);
+  const bool act = D.active(i, j);
+  double U[8], Vv[8];
+  if (!act) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) U[c] = 0.0;
+    U[0] = D.rhofloor;
+    U[4] = D.prsfloor / (D.gamma - 1.0);
+  } else {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) U[c] = in.u(D, s, c, z);
+  }
+  double e = in.e(D, s, z);
+  const Gas<double> gs = D.gas();
+  Vv[4] = 0.0;
+  cons_to_prim(gs, U, Vv, e);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    in.u(D, s, c, z) = U[c];
+    D.v(0, s, c, z) = Vv[c];
+  }
+  in.e(D, s, z) = e;
+  D.v(0, s, 8, z) = e;
+  const double b2 = (Vv[5] * Vv[5] + Vv[6] * Vv[6] + Vv[7] * Vv[7]) * C<double>::ob4pi;
+  D.ms[(int64_t)s * D.NZ + z] = sqrt((D.gamma * Vv[4] + b2) / Vv[0]);
+  if (!act || !in_weno_range(D, i, j, k)) {
+#pragma unroll
+    for (int m = 1; m < 4; ++m)
+#pragma unroll
+      for (int c = 0; c < 9; ++c) D.v(m, s, c, z) = 0.0;
+  }
+}
+
+// step 2 (faces): face B between two inactive zones cleared; blockIdx.y = fam + 3 s
+__global__ void __launch_bounds__(256) k_face_zero(Dev D, Buf6 in) {
+  const int fam = blockIdx.y % 3, s = blockIdx.y / 3;
+  const int64_t t = tid();
+  if (t >= D.NFC) return;
+  const int n = D.nf;
+  const int i = (int)(t % n) - D.g, j = (int)((t / n) % n) - D.g, k = (int)(t / ((int64_t)n * n)) - D.g;
+  const int lo = 1 - D.g, hi = D.N + D.g, hr = D.NR + D.g;
+  double* B = in.b(fam) + (int64_t)s * D.NFC;
+  if (fam == 0) {
+    if (j < lo || k < lo || k > hr) return;
+    if (!D.active(max(i, lo), j) && !D.active(min(i + 1, hi), j)) B[t] = 0.0;
+  } else if (fam == 1) {
+    if (i < lo || k < lo || k > hr) return;
+    if (!D.active(i, max(j, lo)) && !D.active(i, min(j + 1, hi))) B[t] = 0.0;
+  } else {
+    if (i < lo || j < lo) return;
+    if (!D.active(i, j)) B[t] = 0.0;
+  }
+}
+
+// step 3: face B synchronised at the four tri-corners of every sector
+__global__ void __launch_bounds__(256) k_tricorner(Dev D, Buf6 in) {
+  const int64_t t = tid();
+  const int g = D.g, nzr = D.nzr, s = blockIdx.y;
+  if (t >= 4 * (int64_t)g * nzr) return;
+  const int corner = (int)(t / ((int64_t)g * nzr)), ii = (int)((t / nzr) % g), k = (int)(t % nzr) + 1 - g;
+  const bool left = corner == 0 || corner == 3, down = corner < 2;
+  const int i = left ? 1 - g + ii : D.N + 1 + ii, jj = down ? 1 : D.N;
+  int a, b;
+  D.nbr(i, jj, down ? 3 : 2, a, b);
+  double& by = in.By[(int64_t)s * D.NFC + D.fci(i, down ? jj - 1 : jj, k)];
+  double& bx = in.Bx[(int64_t)s * D.NFC + D.fci(left ? a - 1 : a, b, k)];
+  if (corner == 0 || corner == 2) {
+    const double h = 0.5 * (by - bx);
+    by = h;
+    bx = -h;
+  } else {
+    const double h = 0.5 * (by + bx);
+    by = h;
+    bx = h;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// step 6: the velocity-divergence flattener (a), its pressure-ratio
+// extension (b) and its spread to the six neighbours (c)
+__global__ void __launch_bounds__(256) k_flat_a(Dev D, Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  const int64_t z = D.zi(i, j, k), so = (int64_t)s * D.NZ;
+  if (!in_flat_range(D, i, j, k)) {
+    D.flat[so + z] = 0.0;
+    return;
+  }
+  int64_t z6[6];
+  six(D, i, j, k, z6);
+  const double obl = D.obl(s, z);
+  double wmin = D.ms[so + z];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) wmin = fmin(wmin, D.ms[so + z6[q]]);
+  // the unit radial direction of the top z-face centroid serves both z-faces
+  double nzv[3];
+  {
+    csg1::zcen_dir(D.G, s, i, j, nzv);
+    const double r = D.G.r(k);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nzv[c] = r * nzv[c];
+    const double t = 1.0 / fmax(sqrt(nzv[0] * nzv[0] + nzv[1] * nzv[1] + nzv[2] * nzv[2]), D.smallnum);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nzv[c] = nzv[c] * t;
+  }
+  double dv = 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    const bool up = (q % 2) == 0;
+    const int64_t zo = up ? z6[q] : z, zn = up ? z : z6[q];
+    double r[3], n[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = (D.cen(s, zo, c) - D.cen(s, zn, c)) * obl;
+    const double a11 = 0.5 / fmax(sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]), D.smallnum);
+    if (q < 2) csg1::normal_x(D.G, s, q == 0 ? i : i - 1, j, n);
+    else if (q < 4) csg1::normal_y(D.G, s, i, q == 2 ? j : j - 1, n);
+    else {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) n[c] = nzv[c];
+    }
+    dv = dv + n[0] * a11 * (D.v(0, s, 1, zo) - D.v(0, s, 1, zn)) + n[1] * a11 * (D.v(0, s, 2, zo) - D.v(0, s, 2, zn)) +
+         n[2] * a11 * (D.v(0, s, 3, zo) - D.v(0, s, 3, zn));
+  }
+  const double a13 = fabs(dv) / (D.flatten_coef * wmin) - 1.0;
+  double f = a13 <= 0.0 ? 0.0 : fmin(1.0, a13);
+  if (!D.active(i, j)) f = 0.0;
+  D.flat[so + z] = f;
+}
+__global__ void __launch_bounds__(256) k_flat_b(Dev D, Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  const int64_t z = D.zi(i, j, k), so = (int64_t)s * D.NZ;
+  double f = D.flat[so + z];
+  if (in_flat_range(D, i, j, k)) {
+    int64_t z6[6];
+    six(D, i, j, k, z6);
+    const double p0 = D.prs_flatten_coef * D.v(0, s, 4, z);
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+      if (D.v(0, s, 4, z6[q]) > p0) f = fmax(f, D.flat[so + z6[q]]);
+  }
+  D.work[so + z] = f;
+}
+__global__ void __launch_bounds__(256) k_flat_c(Dev D, Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  const int64_t z = D.zi(i, j, k), so = (int64_t)s * D.NZ;
+  double f = D.work[so + z];
+  if (in_flat_range(D, i, j, k)) {
+    int64_t z6[6];
+    six(D, i, j, k, z6);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) f = fmax(f, D.work[so + z6[q]]);
+  }
+  D.flat[so + z] = f;
+}
+
+// ---------------------------------------------------------------------------
+// step 7.a: WENO of three of the nine fields (group c0 = 0, 3, 6) of one
+// zone in one pass over the 8 stencils (stencil s: the x-neighbour R for s in
+// {0,1,4,5}, else L; the y-neighbour U for s < 4, else D; the z-neighbour
+// top for even s, else bottom); per stencil its 3 x 3 inverse (formed once
+// for the three fields), each field's gradient X_s and weight
+// w_s = 1 / (|X_s|^2 + smallnum)^4 as running sums; the central candidate
+// (the mean of the 8 gradients, weight 100) at the end; times 1 - flatten.
+__device__ __forceinline__ void weno3(const Dev& D, int s, int i, int j, int k, int64_t z, int c0) {
+  if (!D.active(i, j)) return;
+  int64_t z6[6];
+  six(D, i, j, k, z6);
+  const double obl = D.obl(s, z);
+  double off[6][3];
+#pragma unroll
+  for (int q = 0; q < 6; ++q)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) off[q][c] = (D.cen(s, z6[q], c) - D.cen(s, z, c)) * obl;
+  const double smallnum = D.smallnum, omf = 1.0 - D.flat[(int64_t)s * D.NZ + z];
+  double dR[3], dL[3], dU[3], dD[3], dT[3], dB[3], num[3][3], den[3], sx[3][3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const double* src = &D.v(0, s, c0 + f, 0);
+    const double f0 = src[z];
+    dR[f] = src[z6[0]] - f0;
+    dL[f] = src[z6[1]] - f0;
+    dU[f] = src[z6[2]] - f0;
+    dD[f] = src[z6[3]] - f0;
+    dT[f] = src[z6[4]] - f0;
+    dB[f] = src[z6[5]] - f0;
+    den[f] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) num[f][m] = sx[f][m] = 0.0;
+  }
+#pragma unroll 1
+  for (int st = 0; st < 8; ++st) {
+    const bool bx = (st >> 1) & 1, by = (st >> 2) & 1, bz = st & 1;
+    double m3[3][3], A[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      m3[0][c] = bx ? off[1][c] : off[0][c];
+      m3[1][c] = by ? off[3][c] : off[2][c];
+      m3[2][c] = bz ? off[5][c] : off[4][c];
+    }
+    inv3(m3, A);
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const double r0 = bx ? dL[f] : dR[f], r1 = by ? dD[f] : dU[f], r2 = bz ? dB[f] : dT[f];
+      double X[3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) X[m] = A[m][0] * r0 + A[m][1] * r1 + A[m][2] * r2;
+      double tt = X[0] * X[0] + X[1] * X[1] + X[2] * X[2] + smallnum;
+      tt = tt * tt;
+      const double w = 1.0 / (tt * tt);
+      den[f] = st == 0 ? w : den[f] + w;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        num[f][m] += w * X[m];
+        sx[f][m] += X[m];
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    double X8[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) X8[m] = 0.125 * sx[f][m];
+    double tt = X8[0] * X8[0] + X8[1] * X8[1] + X8[2] * X8[2] + smallnum;
+    tt = tt * tt;
+    const double w8 = 100.0 / (tt * tt);
+    const double os = 1.0 / (den[f] + w8);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) D.v(m + 1, s, c0 + f, z) = (num[f][m] + w8 * X8[m]) * os * omf;
+  }
+}
+// every zone of the WENO range: thread t takes fields 3 (t / n) .. + 2 of zone t % n
+__global__ void __launch_bounds__(128) k_weno(Dev D, Box b) {
+  const int64_t t = tid(), n = b.count();
+  if (t >= 3 * n) return;
+  int i, j, k;
+  b.decode(t % n, i, j, k);
+  weno3(D, blockIdx.y, i, j, k, D.zi(i, j, k), 3 * (int)(t / n));
+}
+// the zones of the replay list (grid-stride)
+__global__ void __launch_bounds__(128) k_weno_list(Dev D) {
+  if (D.ctr[0] == 0 || D.ctr[2]) return;
+  const int64_t n = D.ctr[1];
+  for (int64_t t = tid(); t < 3 * n; t += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = D.list[t % n];
+    const int s = (int)(p / D.NZ);
+    const int64_t z = p - (int64_t)s * D.NZ;
+    int i, j, k;
+    D.zone_of(z, i, j, k);
+    weno3(D, s, i, j, k, z, 3 * (int)(t / n));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// step 7.b: ENFORCE_PRESSURE_POSITIVITY_PRIM (440...f:6-830).  pass < 0: the
+// first pass over the range with the incoming entropy means; flags of every
+// zone set, zones whose slopes were removed counted.  pass >= 0: replay sweep
+// `pass` on the list, taking the entropy means of earlier zones whose prev
+// flag is set as zero (440...f:553-560), flag changes counted.
+__device__ __forceinline__ void columns9(const Dev& D, int i, int j, int64_t* c9) {
+  c9[0] = D.zi(i, j, 0);
+  const int dirs[8] = {0, 1, 2, 3, 8, 9, 10, 11};
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    int ai, bj;
+    D.nbr(i, j, dirs[q], ai, bj);
+    c9[q + 1] = D.zi(ai, bj, 0);
+  }
+}
+
+__device__ __forceinline__ void positivity(const Dev& D, int s, int i, int j, int k, int64_t z, int pass) {
+  const int64_t so = (int64_t)s * D.NZ;
+  int64_t c9[9];
+  columns9(D, i, j, c9);
+  const int64_t ks = (int64_t)D.nz * D.nz;
+  double rmin = 1e300, rmax = -1e300, pmin = 1e300, pmax = -1e300, emin = 1e300, emax = -1e300;
+#pragma unroll 1
+  for (int q = 0; q < 9; ++q)
+#pragma unroll
+    for (int dk = -1; dk <= 1; ++dk) {
+      const int64_t zz = c9[q] + ks * (k + dk);
+      const double r = D.v(0, s, 0, zz), p = D.v(0, s, 4, zz);
+      double e = D.v(0, s, 8, zz);
+      if (pass >= 0 && zz < z && D.prev[so + zz]) e = 0.0;
+      rmin = fmin(rmin, r);
+      rmax = fmax(rmax, r);
+      pmin = fmin(pmin, p);
+      pmax = fmax(pmax, p);
+      emin = fmin(emin, e);
+      emax = fmax(emax, e);
+    }
+  const double expandfac = 0.2;
+  const double ff = fmax(fmin((D.flat[so + z] - 0.1) / 0.4, 1.0), 0.0), omff = 1.0 - ff;
+  rmin = fmax(rmin * (1.0 - expandfac * omff), D.rhofloor);
+  rmax = fmax(rmax * (1.0 + expandfac * omff), D.rhofloor);
+  pmin = fmax(pmin * (1.0 - expandfac * omff), D.prsfloor);
+  pmax = fmax(pmax * (1.0 + expandfac * omff), D.prsfloor);
+  emin = emin * (1.0 - expandfac * omff);
+  emax = emax * (1.0 + expandfac * omff);
+  const double r0 = D.v(0, s, 0, z), p0 = D.v(0, s, 4, z), e0 = D.v(0, s, 8, z);
+  double rm[3], pm[3], em[3];
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    rm[m] = D.v(m + 1, s, 0, z);
+    pm[m] = D.v(m + 1, s, 4, z);
+    em[m] = D.v(m + 1, s, 8, z);
+  }
+  const double obl = D.obl(s, z);
+  double c0[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) c0[c] = D.cen(s, z, c);
+  // the 27 nodes: vertices, face centroids, edge midpoints, the centroid
+  double vx[8][3], fcx[6][3];
+#pragma unroll
+  for (int n = 0; n < 8; ++n) D.G.vert(s, (n & 1) ? i : i - 1, (n & 2) ? j : j - 1, (n & 4) ? k : k - 1, vx[n]);
+  {
+    csg1::FaceG F;
+    csg1::face_x(D.G, s, i, j, k, F);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fcx[0][c] = F.cen[c];
+    csg1::face_x(D.G, s, i - 1, j, k, F);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fcx[1][c] = F.cen[c];
+    csg1::face_y(D.G, s, i, j, k, F);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fcx[2][c] = F.cen[c];
+    csg1::face_y(D.G, s, i, j - 1, k, F);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fcx[3][c] = F.cen[c];
+    csg1::face_z(D.G, s, i, j, k, F);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fcx[4][c] = F.cen[c];
+    csg1::face_z(D.G, s, i, j, k - 1, F);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) fcx[5][c] = F.cen[c];
+  }
+  const double ratio = 0.85769589859089412;  // 0.1^(1/15)
+  auto passes = [&](int n) -> bool {
+    double r[3], p[3], e[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      r[m] = rm[m];
+      p[m] = pm[m];
+      e[m] = em[m];
+    }
+    for (int q = 0; q < n; ++q)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        r[m] = ratio * r[m];
+        p[m] = ratio * p[m];
+        e[m] = ratio * e[m];
+      }
+    bool ok = true;
+    auto node = [&](const double* x0) {
+      double x[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) x[c] = (x0[c] - c0[c]) * obl;
+      const double rn = r0 + r[0] * x[0] + r[1] * x[1] + r[2] * x[2];
+      const double pn = p0 + p[0] * x[0] + p[1] * x[1] + p[2] * x[2];
+      const double en = e0 + e[0] * x[0] + e[1] * x[1] + e[2] * x[2];
+      if (rn < rmin || rn > rmax || pn < pmin || pn > pmax || en < emin || en > emax) ok = false;
+    };
+#pragma unroll
+    for (int v = 0; v < 8; ++v) node(vx[v]);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) node(fcx[f]);
+#pragma unroll
+    for (int q = 0; q < 12; ++q) {
+      const int va = q < 4 ? 2 * q : (q < 8 ? ((q - 4) & 1) + 4 * ((q - 4) >> 1) : q - 8);
+      const int vb = q < 4 ? va + 1 : (q < 8 ? va + 2 : va + 4);
+      double m[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) m[c] = (vx[va][c] + vx[vb][c]) * 0.5;
+      node(m);
+    }
+    node(c0);
+    return ok;
+  };
+  // the reference checks 0, 1, ..., 14 reductions in turn and takes the first
+  // that passes (removing the slopes when none does); the zone's mean lies in
+  // the bounds, so in exact arithmetic passing is monotone in the count and
+  // the first passing count is found by probing 0, then 14, then bisecting
+  int lo = -1, hi = 15, n = 0;
+#pragma unroll 1
+  while (hi - lo > 1) {
+    const bool ok = passes(n);
+    if (ok) hi = n;
+    else lo = n;
+    n = lo < 0 ? 0 : (lo == 0 && hi == 15 ? 14 : (lo + hi) >> 1);
+  }
+  const bool removed = hi == 15;
+  const int nred = removed ? 0 : hi;
+  if (nred > 0 || removed) {
+    for (int c = 0; c < 9; ++c)
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        double v = D.v(m + 1, s, c, z);
+        if (removed) v = 0.0;
+        else
+          for (int r = 0; r < nred; ++r) v = ratio * v;
+        D.v(m + 1, s, c, z) = v;
+      }
+  }
+  const uint8_t flag = removed ? 1 : 0;
+  if (pass < 0) {
+    D.fa[so + z] = flag;
+    D.prev[so + z] = flag;
+    D.next[so + z] = flag;
+    D.ever[so + z] = flag;
+    D.inlist[so + z] = 0;
+    if (removed) atomicAdd(&D.ctr[0], 1u);
+  } else {
+    D.next[so + z] = flag;
+    if (flag != D.prev[so + z]) atomicAdd(&D.ctr[3 + pass], 1u);
+  }
+}
+__global__ void __launch_bounds__(256) k_pos(Dev D, Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  positivity(D, blockIdx.y, i, j, k, D.zi(i, j, k), -1);
+}
+__global__ void __launch_bounds__(256) k_pos_list(Dev D, int sweep) {
+  if (D.ctr[0] == 0 || D.ctr[2]) return;
+  const int64_t n = D.ctr[1];
+  for (int64_t t = tid(); t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = D.list[t];
+    const int s = (int)(p / D.NZ);
+    const int64_t z = p - (int64_t)s * D.NZ;
+    int i, j, k;
+    D.zone_of(z, i, j, k);
+    positivity(D, s, i, j, k, z, sweep);
+  }
+}
+__global__ void k_ctr_reset(Dev D) {
+  if (threadIdx.x < kCtr) D.ctr[threadIdx.x] = 0;
+}
+// the zones of the range with a zone ever flagged earlier in the loop order
+// among their 27 join the list (each once: the list only grows, so a zone
+// that depends on a flag is recomputed in every later sweep)
+__global__ void __launch_bounds__(256) k_cands(Dev D, Box b) {
+  if (D.ctr[0] == 0 || D.ctr[2]) return;
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  const int64_t z = D.zi(i, j, k), so = (int64_t)s * D.NZ, ks = (int64_t)D.nz * D.nz;
+  if (D.inlist[so + z]) return;
+  int64_t c9[9];
+  columns9(D, i, j, c9);
+  bool cand = false;
+  for (int q = 0; q < 9; ++q)
+#pragma unroll
+    for (int dk = -1; dk <= 1; ++dk) {
+      const int64_t zz = c9[q] + ks * (k + dk);
+      cand = cand || (zz < z && D.ever[so + zz]);
+    }
+  if (cand) {
+    D.inlist[so + z] = 1;
+    D.list[atomicAdd(&D.ctr[1], 1u)] = (uint32_t)(so + z);
+  }
+}
+// end of a sweep: the new flags become the previous ones; no change: settled
+__global__ void __launch_bounds__(256) k_commit(Dev D, int sweep) {
+  if (D.ctr[0] == 0) return;
+  const int64_t n = D.ctr[1];
+  for (int64_t t = tid(); t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = D.list[t];
+    const uint8_t f = D.next[p];
+    D.prev[p] = f;
+    D.ever[p] = D.ever[p] | f;
+  }
+  if (tid() == 0 && D.ctr[3 + sweep] == 0) D.ctr[2] = 1;
+}
+// after the sweeps: the entropy means of the zones that lost their slopes
+// are zero (440...f:553-560); a replay that has not settled is an error
+__global__ void __launch_bounds__(256) k_clear_ent(Dev D, Box b, Buf6 in, int* err) {
+  if (D.ctr[0] == 0) return;
+  if (tid() == 0 && blockIdx.y == 0 && !D.ctr[2]) atomicOr(err, 1);
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  const int64_t z = D.zi(i, j, k);
+  if (D.prev[(int64_t)s * D.NZ + z]) {
+    in.e(D, s, z) = 0.0;
+    D.v(0, s, 8, z) = 0.0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// step 8.a-c: normal B at the five B-field points of each face from the two
+// zones' linear B, shifted so that the weighted mean is the face's Bf;
+// blockIdx.y = fam + 3 s
+__global__ void __launch_bounds__(256) k_face_pts(Dev D, Box bx, Box by, Box bz, Buf6 in) {
+  const int fam = blockIdx.y % 3, s = blockIdx.y / 3;
+  const Box& b = fam == 0 ? bx : (fam == 1 ? by : bz);
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int64_t fb = D.fci(i, j, k);
+  int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1);
+  bool skip = false;
+  if (fam < 2) {
+    const bool al = D.active(li, lj), ar = D.active(ri, rj);
+    if (!al && ar) D.nbr(ri, rj, fam == 0 ? 1 : 3, li, lj);
+    else if (al && !ar) D.nbr(li, lj, fam == 0 ? 0 : 2, ri, rj);
+    else if (!al && !ar) skip = true;
+  } else {
+    skip = !D.active(i, j);
+  }
+  csg1::FaceG F;
+  csg1::face_of(D.G, fam, s, i, j, k, F);
+  double q5[5];
+  if (skip) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) q5[q] = 0.0;
+  } else {
+    const int64_t zl = D.zi(li, lj, k), zr = D.zi(ri, rj, fam == 2 ? k + 1 : k);
+    const double oll = D.obl(s, zl), olr = D.obl(s, zr);
+    double vl[3][4], vr[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        vl[c][m] = D.v(m, s, 5 + c, zl);
+        vr[c][m] = D.v(m, s, 5 + c, zr);
+      }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const double* n = fam == 2 ? F.n[q] : F.n[0];
+      double rl[3], rr[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        rl[c] = (F.p[q][c] - D.cen(s, zl, c)) * oll;
+        rr[c] = (F.p[q][c] - D.cen(s, zr, c)) * olr;
+      }
+      double sm[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double bl = vl[c][0] + vl[c][1] * rl[0] + vl[c][2] * rl[1] + vl[c][3] * rl[2];
+        const double br = vr[c][0] + vr[c][1] * rr[0] + vr[c][2] * rr[1] + vr[c][3] * rr[2];
+        sm[c] = bl + br;
+      }
+      q5[q] = 0.5 * (n[0] * sm[0] + n[1] * sm[1] + n[2] * sm[2]);
+    }
+  }
+  double bd = in.b(fam)[(int64_t)s * D.NFC + fb];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) bd = bd - F.bw[q] * q5[q];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) D.bq(s, fam, q)[fb] = q5[q] + bd;
+}
+
+// ---------------------------------------------------------------------------
+// step 8.d: the ADP constrained least squares for the 12 linear B
+// coefficients of a zone (410...f:4643-5360): the six face means are matched
+// exactly (constraint rows n (x) (1, r) at the face centroids), the face-point
+// B at points 2 and 4 of each face and the 12 WENO modes in least squares;
+// then the energy takes the change of magnetic energy (:5336-5357).  Solved
+// through the Schur complement: M = A^T A (>= I) by Cholesky, Z = L^-1 C^T in
+// LDS, S = Z^T Z, x = L^-T (L^-1 A^T b - Z lambda).
+__global__ void __launch_bounds__(64) k_clsq(Dev D, Box b, Buf6 in) {
+  __shared__ double Zs[72][64];  // each thread's own column: no barrier needed
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  const int64_t z = D.zi(i, j, k);
+  if (!in_weno_range(D, i, j, k) || !D.active(i, j)) return;
+  const double obl = D.obl(s, z);
+  double c0[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) c0[c] = D.cen(s, z, c);
+  double L[78], u[12], row[12], dci[6];
+#pragma unroll
+  for (int r = 0; r < 78; ++r) L[r] = 0.0;
+#pragma unroll
+  for (int r = 0; r < 12; ++r) {
+    L[LIX(r, r)] = 1.0;
+    u[r] = D.v(r % 4, s, 5 + r / 4, z);
+  }
+  // face f: 0 right x, 1 left x, 2 up y, 3 down y, 4 top z, 5 bottom z
+#pragma unroll 1
+  for (int f = 0; f < 6; ++f) {
+    const int fam = f >> 1;
+    const bool hi = (f & 1) == 0;
+    const int fi = i - (fam == 0 && !hi), fj = j - (fam == 1 && !hi), fk = k - (fam == 2 && !hi);
+    csg1::FaceG F;
+    csg1::face_of(D.G, fam, s, fi, fj, fk, F);
+    const int64_t fb = D.fci(fi, fj, fk);
+    // least-squares rows: points 2 and 4
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int pt = 1 + 2 * t;
+      double r[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) r[c] = (F.p[pt][c] - c0[c]) * obl;
+      brow(fam == 2 ? F.n[pt] : F.n[0], r, row);
+      const double bv = D.bq(s, fam, pt)[fb];
+#pragma unroll
+      for (int p = 0; p < 12; ++p) {
+        u[p] += row[p] * bv;
+#pragma unroll
+        for (int q = 0; q <= p; ++q) L[LIX(p, q)] += row[p] * row[q];
+      }
+    }
+    // constraint row: the normal of the face's first point, at its centroid
+    double r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) r[c] = (F.cen[c] - c0[c]) * obl;
+    brow(F.n[0], r, row);
+#pragma unroll
+    for (int p = 0; p < 12; ++p) Zs[f * 12 + p][threadIdx.x] = row[p];
+    const double bf = in.b(fam)[(int64_t)s * D.NFC + fb];
+    dci[0] = f == 0 ? bf : dci[0];
+    dci[1] = f == 1 ? bf : dci[1];
+    dci[2] = f == 2 ? bf : dci[2];
+    dci[3] = f == 3 ? bf : dci[3];
+    dci[4] = f == 4 ? bf : dci[4];
+    dci[5] = f == 5 ? bf : dci[5];
+  }
+  chol12(L);
+  fwd12(L, u);
+  double S[21], sv[6];
+#pragma unroll 1
+  for (int ci = 0; ci < 6; ++ci) {
+#pragma unroll
+    for (int p = 0; p < 12; ++p) row[p] = Zs[ci * 12 + p][threadIdx.x];
+    fwd12(L, row);
+    double t = 0.0;
+#pragma unroll
+    for (int p = 0; p < 12; ++p) {
+      t += row[p] * u[p];
+      Zs[ci * 12 + p][threadIdx.x] = row[p];
+    }
+    const double d = ci == 0 ? dci[0] : ci == 1 ? dci[1] : ci == 2 ? dci[2] : ci == 3 ? dci[3] : ci == 4 ? dci[4] : dci[5];
+    sv[0] = ci == 0 ? t - d : sv[0];
+    sv[1] = ci == 1 ? t - d : sv[1];
+    sv[2] = ci == 2 ? t - d : sv[2];
+    sv[3] = ci == 3 ? t - d : sv[3];
+    sv[4] = ci == 4 ? t - d : sv[4];
+    sv[5] = ci == 5 ? t - d : sv[5];
+  }
+#pragma unroll
+  for (int r = 0; r < 6; ++r)
+#pragma unroll
+    for (int c = 0; c <= r; ++c) {
+      double t = 0.0;
+#pragma unroll
+      for (int p = 0; p < 12; ++p) t += Zs[r * 12 + p][threadIdx.x] * Zs[c * 12 + p][threadIdx.x];
+      S[r * (r + 1) / 2 + c] = t;
+    }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double d = S[c * (c + 1) / 2 + c];
+#pragma unroll
+    for (int q = 0; q < c; ++q) d -= S[c * (c + 1) / 2 + q] * S[c * (c + 1) / 2 + q];
+    d = sqrt(d);
+    S[c * (c + 1) / 2 + c] = d;
+#pragma unroll
+    for (int r = c + 1; r < 6; ++r) {
+      double t = S[r * (r + 1) / 2 + c];
+#pragma unroll
+      for (int q = 0; q < c; ++q) t -= S[r * (r + 1) / 2 + q] * S[c * (c + 1) / 2 + q];
+      S[r * (r + 1) / 2 + c] = t / d;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < 6; ++r) {
+    double t = sv[r];
+#pragma unroll
+    for (int q = 0; q < r; ++q) t -= S[r * (r + 1) / 2 + q] * sv[q];
+    sv[r] = t / S[r * (r + 1) / 2 + r];
+  }
+#pragma unroll
+  for (int r = 5; r >= 0; --r) {
+    double t = sv[r];
+#pragma unroll
+    for (int q = r + 1; q < 6; ++q) t -= S[q * (q + 1) / 2 + r] * sv[q];
+    sv[r] = t / S[r * (r + 1) / 2 + r];
+  }
+#pragma unroll
+  for (int p = 0; p < 12; ++p) {
+    double t = u[p];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) t -= sv[c] * Zs[c * 12 + p][threadIdx.x];
+    u[p] = t;
+  }
+  bwd12(L, u);
+  const double b5 = in.u(D, s, 5, z), b6 = in.u(D, s, 6, z), b7 = in.u(D, s, 7, z);
+  const double aa = (b5 * b5 + b6 * b6 + b7 * b7) * C<double>::ob8pi;
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) D.v(m, s, 5 + c, z) = u[4 * c + m];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) in.u(D, s, 5 + c, z) = u[4 * c];
+  const double bb = (u[0] * u[0] + u[4] * u[4] + u[8] * u[8]) * C<double>::ob8pi;
+  in.u(D, s, 4, z) = in.u(D, s, 4, z) - aa + bb;
+}
+
+// ---------------------------------------------------------------------------
+// MAKE_FACIAL_FLUXES at second order: per face of the routine's ranges (one
+// layer beyond the owned zones), the two zones' linear fields at the face's
+// central point, the face frame (x: the normal, the centroid's radial
+// direction, their cross product; y: the same with the second and third axes
+// swapped; z: the point's normal, the vertex-difference direction made
+// orthogonal), the Riemann solve, and the resolved flux rotated back, times
+// the area.  Tri-corner faces take the stencil neighbour as the inactive
+// side's zone (450...f:581-622).  With conducting walls the radial faces on
+// the walls see the inner zone's state mirrored about the face normal.
+__device__ __forceinline__ bool flux_range(const Dev& D, int fam, int i, int j, int k) {
+  return i >= (fam == 0 ? -1 : 0) && i <= D.N + 1 && j >= (fam == 1 ? -1 : 0) && j <= D.N + 1 &&
+         k >= (fam == 2 ? -1 : 0) && k <= D.NR + 1;
+}
+__global__ void __launch_bounds__(128) k_flux(Dev D, Box bx, Box by, Box bz, double* F0, double* F1, double* F2,
+                                              double* S0, double* S1, double* S2) {
+  const int fam = blockIdx.y % 3, s = blockIdx.y / 3;
+  const Box& b = fam == 0 ? bx : (fam == 1 ? by : bz);
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1);
+  if (fam < 2) {
+    const bool al = D.active(li, lj), ar = D.active(ri, rj);
+    if (!al && ar) D.nbr(ri, rj, fam == 0 ? 1 : 3, li, lj);
+    else if (al && !ar) D.nbr(li, lj, fam == 0 ? 0 : 2, ri, rj);
+    else if (!al && !ar) return;
+  } else if (!D.active(i, j)) {
+    return;
+  }
+  const int64_t zl = D.zi(li, lj, k), zr = D.zi(ri, rj, fam == 2 ? k + 1 : k), fb = D.fci(i, j, k);
+  csg1::FaceG F;
+  csg1::face_of(D.G, fam, s, i, j, k, F);
+  const double sm = D.smallnum;
+  double A[3], B[3], Cv[3];
+  if (fam == 0) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      A[c] = F.n[0][c];
+      Cv[c] = F.cen[c];
+    }
+    const double t = 1.0 / fmax(sqrt(Cv[0] * Cv[0] + Cv[1] * Cv[1] + Cv[2] * Cv[2]), sm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Cv[c] = Cv[c] * t;
+    B[0] = Cv[1] * A[2] - Cv[2] * A[1];
+    B[1] = Cv[2] * A[0] - Cv[0] * A[2];
+    B[2] = Cv[0] * A[1] - Cv[1] * A[0];
+  } else if (fam == 1) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      A[c] = F.n[0][c];
+      B[c] = F.cen[c];
+    }
+    const double t = 1.0 / fmax(sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]), sm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) B[c] = B[c] * t;
+    Cv[0] = A[1] * B[2] - A[2] * B[1];
+    Cv[1] = A[2] * B[0] - A[0] * B[2];
+    Cv[2] = A[0] * B[1] - A[1] * B[0];
+  } else {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) A[c] = F.n[0][c];
+    double t = 1.0 / fmax(sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]), sm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) A[c] = A[c] * t;
+    double p00[3], p10[3], p01[3], p11[3];
+    D.G.vert(s, i, j, k, p11);
+    D.G.vert(s, i, j - 1, k, p10);
+    D.G.vert(s, i - 1, j, k, p01);
+    D.G.vert(s, i - 1, j - 1, k, p00);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) B[c] = p11[c] + p10[c] - p01[c] - p00[c];
+    t = B[0] * A[0] + B[1] * A[1] + B[2] * A[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) B[c] = B[c] - t * A[c];
+    t = 1.0 / fmax(sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]), sm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) B[c] = B[c] * t;
+    Cv[0] = A[1] * B[2] - A[2] * B[1];
+    Cv[1] = A[2] * B[0] - A[0] * B[2];
+    Cv[2] = A[0] * B[1] - A[1] * B[0];
+    t = 1.0 / fmax(sqrt(Cv[0] * Cv[0] + Cv[1] * Cv[1] + Cv[2] * Cv[2]), sm);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) Cv[c] = Cv[c] * t;
+    B[0] = Cv[1] * A[2] - Cv[2] * A[1];
+    B[1] = Cv[2] * A[0] - Cv[0] * A[2];
+    B[2] = Cv[0] * A[1] - Cv[1] * A[0];
+  }
+  double VL[8], VR[8], eL, eR;
+  {
+    const double ol = D.obl(s, zl), orr = D.obl(s, zr);
+    double rl[3], rr[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      rl[c] = (F.p[0][c] - D.cen(s, zl, c)) * ol;
+      rr[c] = (F.p[0][c] - D.cen(s, zr, c)) * orr;
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      VL[c] = D.v(0, s, c, zl) + D.v(1, s, c, zl) * rl[0] + D.v(2, s, c, zl) * rl[1] + D.v(3, s, c, zl) * rl[2];
+      VR[c] = D.v(0, s, c, zr) + D.v(1, s, c, zr) * rr[0] + D.v(2, s, c, zr) * rr[1] + D.v(3, s, c, zr) * rr[2];
+    }
+    eL = D.v(0, s, 8, zl) + D.v(1, s, 8, zl) * rl[0] + D.v(2, s, 8, zl) * rl[1] + D.v(3, s, 8, zl) * rl[2];
+    eR = D.v(0, s, 8, zr) + D.v(1, s, 8, zr) * rr[0] + D.v(2, s, 8, zr) * rr[1] + D.v(3, s, 8, zr) * rr[2];
+#pragma unroll
+    for (int o = 1; o <= 5; o += 4) {
+      const double la = A[0] * VL[o] + A[1] * VL[o + 1] + A[2] * VL[o + 2];
+      const double lb = B[0] * VL[o] + B[1] * VL[o + 1] + B[2] * VL[o + 2];
+      const double lc = Cv[0] * VL[o] + Cv[1] * VL[o + 1] + Cv[2] * VL[o + 2];
+      VL[o] = la;
+      VL[o + 1] = lb;
+      VL[o + 2] = lc;
+      const double ra = A[0] * VR[o] + A[1] * VR[o + 1] + A[2] * VR[o + 2];
+      const double rb = B[0] * VR[o] + B[1] * VR[o + 1] + B[2] * VR[o + 2];
+      const double rc = Cv[0] * VR[o] + Cv[1] * VR[o + 1] + Cv[2] * VR[o + 2];
+      VR[o] = ra;
+      VR[o + 1] = rb;
+      VR[o + 2] = rc;
+    }
+  }
+  if (D.wall && fam == 2 && (k == 0 || k == D.NR)) {
+    // the wall: the inner zone's state mirrored (normal velocity and normal B
+    // reversed), so no mass crosses the wall and its normal B is zero
+    double* src = k == 0 ? VR : VL;
+    double* dst = k == 0 ? VL : VR;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) dst[c] = src[c];
+    dst[1] = -src[1];
+    dst[5] = -src[5];
+    if (k == 0) eL = eR;
+    else eR = eL;
+  }
+  const Gas<double> gs = D.gas();
+  double Fs[8], eF, SL, SR;
+  if (D.riemann == CS_RS_HLLD) hlld(gs, VL, VR, eL, eR, Fs, eF, SL, SR);
+  else hll(gs, VL, VR, eL, eR, Fs, eF, SL, SR);
+  const double ar = F.area;
+  double* Fo = (fam == 0 ? F0 : fam == 1 ? F1 : F2) + (int64_t)s * 9 * D.NFC + fb;
+#pragma unroll
+  for (int c = 0; c < 8; ++c) {
+    double v;
+    if (c == 0 || c == 4) {
+      v = Fs[c];
+    } else {
+      const int o = c < 4 ? 1 : 5, q = c - o;
+      v = A[q] * Fs[o] + B[q] * Fs[o + 1] + Cv[q] * Fs[o + 2];
+    }
+    Fo[(int64_t)c * D.NFC] = v * ar;
+  }
+  Fo[(int64_t)8 * D.NFC] = eF * ar;
+  double* So = (fam == 0 ? S0 : fam == 1 ? S1 : S2) + (int64_t)s * 2 * D.NFC + fb;
+  So[0] = SL;
+  So[D.NFC] = SR;
+}
+
+// ---------------------------------------------------------------------------
+// MAKE_EDGE_EFIELDS_P1 at second order (460...f:393-3721 with the tri-corner
+// blocks :3725-7035): every edge of the owned faces takes the mean zone
+// E = -v x B of its four zones along the edge tangent, the normal B at the
+// edge points of the four faces around it, and their wave speeds symmetrised
+// to +-max|S|, into Two_D_RS_Ez_4State_HLL; radial edges with an inactive
+// zone take none, and the radial edge at each tri-corner adds
+// Two_D_RS_Ez_3State_HLL of its three active zones.  With conducting walls
+// the lateral edges on the walls carry no field.  blockIdx.y = fam + 3 s.
+__device__ __forceinline__ double zoneE(const Dev& D, int s, int64_t z, int c) {
+  const double vx = D.v(0, s, 1, z), vy = D.v(0, s, 2, z), vz = D.v(0, s, 3, z);
+  const double Bx = D.v(0, s, 5, z), By = D.v(0, s, 6, z), Bz = D.v(0, s, 7, z);
+  return c == 0 ? -(vy * Bz - vz * By) : (c == 1 ? -(vz * Bx - vx * Bz) : -(vx * By - vy * Bx));
+}
+__device__ __forceinline__ double emean(const Dev& D, int s, int c, const int64_t* z, int n) {
+  return n == 4 ? (zoneE(D, s, z[0], c) + zoneE(D, s, z[1], c) + zoneE(D, s, z[2], c) + zoneE(D, s, z[3], c)) * 0.25
+                : (zoneE(D, s, z[0], c) + zoneE(D, s, z[1], c) + zoneE(D, s, z[2], c)) * (1.0 / 3.0);
+}
+__global__ void __launch_bounds__(256) k_edge(Dev D, Box bx, Box by, Box bz, const double* S0, const double* S1,
+                                              const double* S2, double* E0, double* E1, double* E2) {
+  const int fam = blockIdx.y % 3, s = blockIdx.y / 3;
+  const Box& b = fam == 0 ? bx : (fam == 1 ? by : bz);
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int N = D.N;
+  const int64_t t = D.fci(i, j, k), NF = D.NFC;
+  double* out = (fam == 0 ? E0 : fam == 1 ? E1 : E2) + (int64_t)s * NF;
+  if (D.wall && fam < 2 && (k == 0 || k == D.NR)) {
+    out[t] = 0.0;
+    return;
+  }
+  const double* SL = S0 + (int64_t)s * 2 * NF;  // x faces: S_L, S_R
+  const double* SD = S1 + (int64_t)s * 2 * NF;  // y faces: S_D, S_U
+  const double* SB = S2 + (int64_t)s * 2 * NF;  // z faces: S_B, S_T
+  const double *S_L = SL, *S_R = SL + NF, *S_D = SD, *S_U = SD + NF, *S_B = SB, *S_T = SB + NF;
+  int zc[4][3];
+  int pi = i, pj = j, pk = k;
+  if (fam == 0) {
+    const int q[4][3] = {{i, j, k}, {i, j + 1, k}, {i, j + 1, k + 1}, {i, j, k + 1}};
+    memcpy(zc, q, sizeof(zc));
+    pi = i - 1;
+  } else if (fam == 1) {
+    const int q[4][3] = {{i, j, k}, {i, j, k + 1}, {i + 1, j, k + 1}, {i + 1, j, k}};
+    memcpy(zc, q, sizeof(zc));
+    pj = j - 1;
+  } else {
+    const int q[4][3] = {{i, j, k}, {i + 1, j, k}, {i + 1, j + 1, k}, {i, j + 1, k}};
+    memcpy(zc, q, sizeof(zc));
+    pk = k - 1;
+  }
+  int64_t z[4], f[4];
+  bool act = true;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    z[q] = D.zi(zc[q][0], zc[q][1], zc[q][2]);
+    f[q] = D.fci(zc[q][0], zc[q][1], zc[q][2]);
+    act = act && D.active(zc[q][0], zc[q][1]);
+  }
+  act = fam < 2 || act;
+  double v1[3], v0[3], ec[3], tg[3];
+  D.G.vert(s, i, j, k, v1);
+  D.G.vert(s, pi, pj, pk, v0);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) ec[c] = 0.5 * (v1[c] + v0[c]);
+  double val = 0.0;
+  if (act) {
+    double ct = 0.0, st = 1.0;
+    if (fam < 2) {
+      // tangent: the chord, made normal to the radius at the edge centre
+      // (the centre lifted to the vertices' sphere)
+      const double rv = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+      const double lift = rv / sqrt(ec[0] * ec[0] + ec[1] * ec[1] + ec[2] * ec[2]);
+      double r[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) r[c] = ec[c] * lift;
+      const double ir = 1.0 / sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        r[c] *= ir;
+        tg[c] = v1[c] - v0[c];
+      }
+      const double d = tg[0] * r[0] + tg[1] * r[1] + tg[2] * r[2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) tg[c] -= d * r[c];
+      const double it = 1.0 / sqrt(tg[0] * tg[0] + tg[1] * tg[1] + tg[2] * tg[2]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) tg[c] *= it;
+    } else {
+      // tangent: radial; the angle between the x and y lattice directions
+      const double ir = 1.0 / sqrt(ec[0] * ec[0] + ec[1] * ec[1] + ec[2] * ec[2]);
+      double n1[3], tau[3], xp[3], xm[3], yp[3], ym[3];
+      D.G.vert(s, i + 1, j, k, xp);
+      D.G.vert(s, i - 1, j, k, xm);
+      D.G.vert(s, i, j + 1, k, yp);
+      D.G.vert(s, i, j - 1, k, ym);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        tg[c] = ec[c] * ir;
+        n1[c] = xp[c] - xm[c];
+        tau[c] = yp[c] - ym[c];
+      }
+      const double d1 = n1[0] * tg[0] + n1[1] * tg[1] + n1[2] * tg[2];
+      const double d2 = tau[0] * tg[0] + tau[1] * tg[1] + tau[2] * tg[2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        n1[c] -= d1 * tg[c];
+        tau[c] -= d2 * tg[c];
+      }
+      const double i1 = 1.0 / sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
+      const double i2 = 1.0 / sqrt(tau[0] * tau[0] + tau[1] * tau[1] + tau[2] * tau[2]);
+      ct = (tau[0] * i2) * (n1[0] * i1) + (tau[1] * i2) * (n1[1] * i1) + (tau[2] * i2) * (n1[2] * i1);
+      st = sqrt(1.0 - ct * ct);
+    }
+    const double Et =
+        emean(D, s, 0, z, 4) * tg[0] + emean(D, s, 1, z, 4) * tg[1] + emean(D, s, 2, z, 4) * tg[2];
+    const int fy = fam == 0 ? 2 : fam == 1 ? 0 : 1, fn = fam == 0 ? 1 : fam == 1 ? 2 : 0;
+    const double By_R = D.bq(s, fy, 4)[f[1]], By_L = D.bq(s, fy, 3)[f[0]];
+    const double Bn_U = D.bq(s, fn, 2)[f[3]], Bn_D = D.bq(s, fn, 1)[f[0]];
+    const double *sHi1, *sLo1, *sHi2, *sLo2;
+    int a1, b1, a2, b2;
+    if (fam == 0) {
+      sHi1 = S_U; sLo1 = S_D; sHi2 = S_T; sLo2 = S_B; a1 = 3; b1 = 0; a2 = 1; b2 = 0;
+    } else if (fam == 1) {
+      sHi1 = S_T; sLo1 = S_B; sHi2 = S_R; sLo2 = S_L; a1 = 3; b1 = 0; a2 = 0; b2 = 1;
+    } else {
+      sHi1 = S_R; sLo1 = S_L; sHi2 = S_U; sLo2 = S_D; a1 = 3; b1 = 0; a2 = 1; b2 = 0;
+    }
+    const double sR = fmax(sHi1[f[a1]], sHi1[f[b1]]), sL = fmin(sLo1[f[a1]], sLo1[f[b1]]);
+    const double sU = fmax(sHi2[f[a2]], sHi2[f[b2]]), sD = fmin(sLo2[f[a2]], sLo2[f[b2]]);
+    const double S = fmax(fmax(fabs(sR), fabs(sL)), fmax(fabs(sU), fabs(sD)));
+    val = emf4_hll<double>(S, -S, S, -S, ct, st, Et, Et, Et, Et, Bn_U, Bn_D, By_R, By_L);
+  }
+  if (fam == 2) {
+    const int corner = (i == 0 && j == 0) ? 0 : (i == N && j == 0) ? 1 : (i == N && j == N) ? 2
+                                              : (i == 0 && j == N) ? 3 : -1;
+    if (corner >= 0) {
+      int q1[2], q2[2], q3[2];
+      if (corner == 0) {
+        q1[0] = i + 1; q1[1] = j + 1; q2[0] = i; q2[1] = j + 1; q3[0] = i + 1; q3[1] = j;
+      } else if (corner == 1) {
+        q1[0] = i + 1; q1[1] = j + 1; q2[0] = i; q2[1] = j + 1; q3[0] = i; q3[1] = j;
+      } else if (corner == 2) {
+        q1[0] = i; q1[1] = j + 1; q2[0] = i; q2[1] = j; q3[0] = i + 1; q3[1] = j;
+      } else {
+        q1[0] = i + 1; q1[1] = j + 1; q2[0] = i; q2[1] = j; q3[0] = i + 1; q3[1] = j;
+      }
+      const int64_t z3[3] = {D.zi(q1[0], q1[1], k), D.zi(q2[0], q2[1], k), D.zi(q3[0], q3[1], k)};
+      const double ir = 1.0 / sqrt(ec[0] * ec[0] + ec[1] * ec[1] + ec[2] * ec[2]);
+      const double Et = emean(D, s, 0, z3, 3) * (ec[0] * ir) + emean(D, s, 1, z3, 3) * (ec[1] * ir) +
+                        emean(D, s, 2, z3, 3) * (ec[2] * ir);
+      const int64_t f1 = D.fci(q1[0], q1[1], k), f2 = D.fci(q2[0], q2[1], k), f3 = D.fci(q3[0], q3[1], k),
+                    f2d = D.fci(q2[0], q2[1] - 1, k), f1d = D.fci(q1[0], q1[1] - 1, k);
+      const double *bx1 = D.bq(s, 0, 1), *bx2 = D.bq(s, 0, 2), *by3 = D.bq(s, 1, 3), *by4 = D.bq(s, 1, 4);
+      double bn[3], sr[3], sl[3];
+      if (corner == 0) {
+        bn[0] = -by4[f3]; bn[1] = bx2[f2]; bn[2] = by3[f2d];
+        sr[0] = -S_D[f3]; sl[0] = -S_U[f3]; sr[1] = S_R[f2]; sl[1] = S_L[f2];
+        sr[2] = S_U[f2d]; sl[2] = S_D[f2d];
+      } else if (corner == 1) {
+        bn[0] = -by4[f1d]; bn[1] = bx2[f2]; bn[2] = by3[f3];
+        sr[0] = -S_L[f3]; sl[0] = -S_R[f3]; sr[1] = S_R[f2]; sl[1] = S_L[f2];
+        sr[2] = S_U[f3]; sl[2] = S_D[f3];
+      } else if (corner == 2) {
+        bn[0] = bx2[f1]; bn[1] = by3[f2]; bn[2] = -bx1[f2];
+        sr[0] = S_R[f1]; sl[0] = S_L[f1]; sr[1] = S_U[f2]; sl[1] = S_D[f2];
+        sr[2] = -S_L[f2]; sl[2] = -S_R[f2];
+      } else {
+        bn[0] = -by4[f3]; bn[1] = by3[f2]; bn[2] = -bx1[f2];
+        sr[0] = -S_D[f3]; sl[0] = -S_U[f3]; sr[1] = S_U[f2]; sl[1] = S_D[f2];
+        sr[2] = -S_L[f2]; sl[2] = -S_R[f2];
+      }
+      const double S = fmax(fmax(fmax(fabs(sr[0]), fabs(sl[0])), fmax(fabs(sr[1]), fabs(sl[1]))),
+                            fmax(fabs(sr[2]), fabs(sl[2])));
+      const double Sr[3] = {S, S, S}, Sl[3] = {-S, -S, -S}, E3[3] = {Et, Et, Et};
+      val += emf3_hll(Sr, Sl, E3, bn);
+    }
+  }
+  out[t] = val;
+}
+
+// ---------------------------------------------------------------------------
+// UPDATE_IMEX_RK_STEP_P1 for the owned faces (484...f:630-906): constrained
+// transport with the edges' radial lengths and arc lengths r acos(u1 . u2)
+// (both directions scaled by 1 / r of the face's own vertex), the stage
+// weights applied: out = wa B^n + wb (B_in - dt circ / A).  blockIdx.y = fam + 3 s.
+__device__ __forceinline__ double vrad(const Dev& D, int s, int i, int j, int k) {
+  double x[3];
+  D.G.vert(s, i, j, k, x);
+  return sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+}
+__device__ __forceinline__ double arc(const Dev& D, int s, int i1, int j1, int i2, int j2, int k, double t) {
+  double a[3], b[3];
+  D.G.vert(s, i1, j1, k, a);
+  D.G.vert(s, i2, j2, k, b);
+  return acos((a[0] * t) * (b[0] * t) + (a[1] * t) * (b[1] * t) + (a[2] * t) * (b[2] * t));
+}
+__global__ void __launch_bounds__(256) k_upd_face(Dev D, Buf6 in, Buf6 u0, Buf6 out, const double* E0,
+                                                  const double* E1, const double* E2, const double* dtp, double wa,
+                                                  double wb) {
+  const int fam = blockIdx.y % 3, s = blockIdx.y / 3;
+  const int64_t t = tid();
+  const int N = D.N, NR = D.NR;
+  const int ni = fam == 0 ? N + 1 : N, nj = fam == 1 ? N + 1 : N, nk = fam == 2 ? NR + 1 : NR;
+  if (t >= (int64_t)ni * nj * nk) return;
+  const int i = (int)(t % ni) + (fam == 0 ? 0 : 1), j = (int)((t / ni) % nj) + (fam == 1 ? 0 : 1),
+            k = (int)(t / ((int64_t)ni * nj)) + (fam == 2 ? 0 : 1);
+  const double dt = dtp[0];
+  const int64_t so = (int64_t)s * D.NFC, f = D.fci(i, j, k);
+  const double *Ex = E0 + so, *Ey = E1 + so, *Ez = E2 + so;
+  const double rout = vrad(D, s, i, j, k), tt = 1.0 / rout;
+  double circ, area;
+  if (fam == 0) {
+    const double rin = vrad(D, s, i, j, k - 1), dr = rout - rin, th = arc(D, s, i, j, i, j - 1, k, tt);
+    const double lin = rin * th, lout = rout * th;
+    circ = dr * Ez[f] - dr * Ez[D.fci(i, j - 1, k)] - lout * Ey[f] + lin * Ey[D.fci(i, j, k - 1)];
+    area = csg1::area_x(D.G, s, i, j, k);
+  } else if (fam == 1) {
+    const double rin = vrad(D, s, i, j, k - 1), dr = rout - rin, th = arc(D, s, i, j, i - 1, j, k, tt);
+    const double lin = rin * th, lout = rout * th;
+    circ = -(dr * Ez[f]) + dr * Ez[D.fci(i - 1, j, k)] + lout * Ex[f] - lin * Ex[D.fci(i, j, k - 1)];
+    area = csg1::area_y(D.G, s, i, j, k);
+  } else {
+    const double lxp = rout * arc(D, s, i, j, i - 1, j, k, tt), lxm = rout * arc(D, s, i, j - 1, i - 1, j - 1, k, tt);
+    const double lyp = rout * arc(D, s, i, j, i, j - 1, k, tt), lym = rout * arc(D, s, i - 1, j, i - 1, j - 1, k, tt);
+    circ = lyp * Ey[f] - lym * Ey[D.fci(i - 1, j, k)] - lxp * Ex[f] + lxm * Ex[D.fci(i, j - 1, k)];
+    area = csg1::area_z(D.G, s, i, j, k);
+  }
+  const double bn = in.b(fam)[so + f] - dt * circ / area;
+  out.b(fam)[so + f] = wa != 0.0 ? wa * u0.b(fam)[so + f] + wb * bn : wb * bn;
+}
+
+// UPDATE_IMEX_RK_STEP_P1 for the owned zones (484...f:483-628) with the stage
+// weights and the implicit gravity of the IMEX stages, then
+// CLEANUP_AFTER_UPDATE (485...f:691-865, 1180-1280): the zone B as the
+// least-squares fit of its six new face values over the face normals (both
+// z-faces: the radial direction of the upper z-face centroid), the energy
+// changed by the change of magnetic energy, CONS_TO_PRIM.
+__global__ void __launch_bounds__(256) k_upd_zone(Dev D, Box b, Buf6 in, Buf6 u0, Buf6 out, const double* F0,
+                                                  const double* F1, const double* F2, const double* dtp, double wa,
+                                                  double wb, int mode) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  const int64_t z = D.zi(i, j, k), so = (int64_t)s * 9 * D.NFC;
+  const double dt = dtp[0];
+  const int64_t fxp = D.fci(i, j, k), fxm = D.fci(i - 1, j, k), fym = D.fci(i, j - 1, k), fzm = D.fci(i, j, k - 1);
+  const double ov = 1.0 / D.vol(s, z);
+  double L[9];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) {
+    const int64_t o = so + (int64_t)q * D.NFC;
+    L[q] = -ov * ((F0[o + fxp] - F0[o + fxm]) + (F1[o + fxp] - F1[o + fym]) + (F2[o + fxp] - F2[o + fzm]));
+  }
+  double n[3] = {0.0, 0.0, 0.0}, gr = 0.0;
+  if (D.grav != 0.0) {
+    double c[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) c[q] = D.cen(s, z, q);
+    const double r2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2], ir = 1.0 / sqrt(r2);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) n[q] = c[q] * ir;
+    gr = -D.grav / r2;
+  }
+  double ui[8], uz[8], u[8], en;
+#pragma unroll
+  for (int q = 0; q < 8; ++q) ui[q] = in.u(D, s, q, z);
+  const double ei = in.e(D, s, z);
+  double ez = 0.0;
+  if (mode != 0 || wa != 0.0) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) uz[q] = u0.u(D, s, q, z);
+    ez = u0.e(D, s, z);
+  }
+  rk_combine(ui, ei, uz, ez, dt, wa, wb, mode, gr, n, L, u, en);
+  const Gas<double> gs = D.gas();
+  double Vv[8];
+  cons_to_prim(gs, u, Vv, en);  // the update's CONS_TO_PRIM (484...f:620-628)
+  // the cleanup's least-squares zone B
+  double A[6][3], R[6];
+  csg1::normal_x(D.G, s, i, j, A[0]);
+  csg1::normal_x(D.G, s, i - 1, j, A[1]);
+  csg1::normal_y(D.G, s, i, j, A[2]);
+  csg1::normal_y(D.G, s, i, j - 1, A[3]);
+  {
+    double nv[3];
+    csg1::zcen_dir(D.G, s, i, j, nv);
+    const double r = D.G.r(k);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) nv[c] = r * nv[c];
+    const double t = 1.0 / fmax(D.smallnum, sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      A[4][c] = nv[c] * t;
+      A[5][c] = A[4][c];
+    }
+  }
+  const int64_t fo = (int64_t)s * D.NFC;
+  R[0] = out.Bx[fo + fxp];
+  R[1] = out.Bx[fo + fxm];
+  R[2] = out.By[fo + fxp];
+  R[3] = out.By[fo + fym];
+  R[4] = out.Bz[fo + fxp];
+  R[5] = out.Bz[fo + fzm];
+  double M[3][3], r3[3];
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double t = 0.0;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) t += A[r][p] * A[r][q];
+      M[p][q] = t;
+    }
+    double t = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) t += A[r][p] * R[r];
+    r3[p] = t;
+  }
+  const double det = M[0][0] * (M[1][1] * M[2][2] - M[2][1] * M[1][2]) - M[0][1] * (M[1][0] * M[2][2] - M[2][0] * M[1][2]) +
+                     M[0][2] * (M[1][0] * M[2][1] - M[2][0] * M[1][1]);
+  const double od = copysign(1.0, det) / fmax(1.0e-32, fabs(det));
+  const double bi[3][3] = {{(M[1][1] * M[2][2] - M[1][2] * M[2][1]) * od, (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * od,
+                            (M[0][1] * M[1][2] - M[1][1] * M[0][2]) * od},
+                           {(M[1][2] * M[2][0] - M[1][0] * M[2][2]) * od, (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * od,
+                            (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * od},
+                           {(M[1][0] * M[2][1] - M[1][1] * M[2][0]) * od, (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * od,
+                            (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * od}};
+  const double b2old = u[5] * u[5] + u[6] * u[6] + u[7] * u[7];
+#pragma unroll
+  for (int p = 0; p < 3; ++p) u[5 + p] = bi[p][0] * r3[0] + bi[p][1] * r3[1] + bi[p][2] * r3[2];
+  const double ob8pi = 0.5 * C<double>::ob4pi;
+  u[4] = u[4] - b2old * ob8pi + (u[5] * u[5] + u[6] * u[6] + u[7] * u[7]) * ob8pi;
+  Vv[4] = 0.0;
+  cons_to_prim(gs, u, Vv, en);
+#pragma unroll
+  for (int q = 0; q < 8; ++q) out.u(D, s, q, z) = u[q];
+  out.e(D, s, z) = en;
+}
+
+// IMEX-SSP2(2,2,2) first stage: U1 = Un + gi dt S(U1) in closed form, every
+// zone (the exchange then refreshes the ghosts)
+__global__ void __launch_bounds__(256) k_imex(Dev D, Box b, Buf6 un, Buf6 u1, const double* dtp) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  const int64_t z = D.zi(i, j, k);
+  double u[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) u[q] = un.u(D, s, q, z);
+  if (D.active(i, j) && D.grav != 0.0) {
+    double c[3], n[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) c[q] = D.cen(s, z, q);
+    const double r2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2], ir = 1.0 / sqrt(r2);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) n[q] = c[q] * ir;
+    imex_pre(dtp[0], -D.grav / r2, n, u);
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q) u1.u(D, s, q, z) = u[q];
+  u1.e(D, s, z) = un.e(D, s, z);
+}
+
+// EVALUATE_TIMESTEP (490...f:72-163): CONS_TO_PRIM of the owned active zones
+// (U rewritten, the entropy array left as it is), the magnetosonic speed and
+// min cfl l / (|v| + c_f), one partial minimum per workgroup
+constexpr int kCflThreads = 256;
+__global__ void __launch_bounds__(kCflThreads) k_cfl(Dev D, Buf6 cur, double cfl, double* part) {
+  __shared__ double red[kCflThreads];
+  const Gas<double> gs = D.gas();
+  const int64_t per = (int64_t)D.N * D.N * D.NR, n = 6 * per, stride = (int64_t)gridDim.x * blockDim.x;
+  double best = 1e300;
+  for (int64_t t = tid(); t < n; t += stride) {
+    const int s = (int)(t / per);
+    const int64_t r = t - (int64_t)s * per;
+    const int i = 1 + (int)(r % D.N), j = 1 + (int)((r / D.N) % D.N), k = 1 + (int)(r / ((int64_t)D.N * D.N));
+    const int64_t z = D.zi(i, j, k);
+    double U[8], Vv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) U[c] = cur.u(D, s, c, z);
+    double e = cur.e(D, s, z);
+    Vv[4] = 0.0;
+    cons_to_prim(gs, U, Vv, e);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) cur.u(D, s, c, z) = U[c];
+    const double b2 = Vv[5] * Vv[5] + Vv[6] * Vv[6] + Vv[7] * Vv[7];
+    const double ms = sqrt((D.gamma * Vv[4] + C<double>::ob4pi * b2) / Vv[0]);
+    const double sp = sqrt(Vv[1] * Vv[1] + Vv[2] * Vv[2] + Vv[3] * Vv[3]) + ms;
+    best = fmin(best, cfl * D.scale(s, z) / sp);
+  }
+  red[threadIdx.x] = best;
+  __syncthreads();
+  for (int h = kCflThreads / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + h]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+struct Stage {
+  Dev D;
+  double cfl;
+  std::vector<void*> mem;
+  Box all, wr, own, fx, fy, fz, ffx, ffy, ffz, ex, ey, ez;
+};
+
+static inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+#define P1S_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      cs::set_error("P1 stage: %s: %s", #call, hipGetErrorString(e_));            \
+      return CS_E_HIP;                                                            \
+    }                                                                             \
+  } while (0)
+
+template <typename T>
+static int dalloc(Stage* S, T*& p, int64_t n, size_t* bytes) {
+  void* q = nullptr;
+  const size_t b = sizeof(T) * (size_t)n;
+  if (hipMalloc(&q, b) != hipSuccess) {
+    cs::set_error("P1 stage: hipMalloc of %.3f GB failed", (double)b / 1e9);
+    return CS_E_NOMEM;
+  }
+  S->mem.push_back(q);
+  if (hipMemset(q, 0, b) != hipSuccess) {
+    cs::set_error("P1 stage: hipMemset failed");
+    return CS_E_HIP;
+  }
+  p = (T*)q;
+  *bytes += b;
+  return CS_OK;
+}
+
+void destroy(Stage* S) {
+  if (!S) return;
+  for (void* q : S->mem) (void)hipFree(q);
+  delete S;
+}
+
+int create(const cs_config* cfg, Stage** out, size_t* bytes) {
+  *out = nullptr;
+  *bytes = 0;
+  if (cfg->nghost < 4) {
+    cs::set_error("CS_RECON_P1ADP needs nghost >= 4 (the P1 stencils reach four ghost layers)");
+    return CS_E_INVALID;
+  }
+  if (cfg->nghost >= cfg->ntp) {
+    cs::set_error("CS_RECON_P1ADP: nghost (%d) must be below ntp (%d)", cfg->nghost, cfg->ntp);
+    return CS_E_BAD_DECOMP;
+  }
+  if (cfg->precision != CS_FP64) {
+    cs::set_error("CS_RECON_P1ADP runs in FP64");
+    return CS_E_INVALID;
+  }
+  if (cfg->emf != CS_EMF_LLF) {
+    cs::set_error("CS_RECON_P1ADP uses the reference's LLF edge fields (CS_EMF_LLF)");
+    return CS_E_INVALID;
+  }
+  if ((double)6 * (cfg->ntp + 2 * cfg->nghost) * (cfg->ntp + 2 * cfg->nghost) * (cfg->nr + 2 * cfg->nghost) >=
+      4294967295.0) {
+    cs::set_error("CS_RECON_P1ADP: the shell has too many zones for 32-bit zone lists");
+    return CS_E_INVALID;
+  }
+  cs::P1GeoHost H;
+  int rc = cs::p1geo_host(cfg, H);
+  if (rc) return rc;
+  Stage* S = new Stage();
+  Dev& D = S->D;
+  const int N = cfg->ntp, NR = cfg->nr, g = cfg->nghost;
+  D.N = N;
+  D.NR = NR;
+  D.g = g;
+  D.nz = N + 2 * g;
+  D.nzr = NR + 2 * g;
+  D.nf = D.nz + 1;
+  D.nfr = D.nzr + 1;
+  D.NZ = (int64_t)D.nz * D.nz * D.nzr;
+  D.NFC = (int64_t)D.nf * D.nf * D.nfr;
+  D.NZ2 = (int64_t)D.nz * D.nz;
+  D.gamma = cfg->gamma;
+  D.rhofloor = cfg->rhofloor;
+  D.prsfloor = cfg->prsfloor;
+  D.smallnum = csg1::kSmall;
+  // the driver's settings (1000riemann_geomesh_cs_mhd.f:314, 319)
+  D.flatten_coef = 0.4;
+  D.prs_flatten_coef = 1.5;
+  D.grav = cfg->grav;
+  D.wall = cfg->radial_bc == CS_RBC_WALL ? 1 : 0;
+  D.riemann = cfg->riemann;
+  S->cfl = cfg->cfl;
+  auto fail = [&](int r) {
+    destroy(S);
+    return r;
+  };
+  // geometry tables, zone geometry, mask, stencil
+  double *lat, *rbd, *TX, *TY, *TZ, *zg;
+  int32_t *mask, *sten;
+  if ((rc = dalloc(S, lat, (int64_t)H.lat.size(), bytes)) || (rc = dalloc(S, rbd, (int64_t)H.rb.size(), bytes)) ||
+      (rc = dalloc(S, TX, (int64_t)H.TX.size(), bytes)) || (rc = dalloc(S, TY, (int64_t)H.TY.size(), bytes)) ||
+      (rc = dalloc(S, TZ, (int64_t)H.TZ.size(), bytes)) || (rc = dalloc(S, zg, 6 * 5 * D.NZ, bytes)) ||
+      (rc = dalloc(S, mask, D.NZ2, bytes)) || (rc = dalloc(S, sten, 24 * D.NZ2, bytes)))
+    return fail(rc);
+  std::vector<int32_t> hm((size_t)D.NZ2), hs((size_t)D.NZ2 * 24);
+  for (int j = 1 - g; j <= N + g; ++j)
+    for (int i = 1 - g; i <= N + g; ++i) {
+      int on = 1;
+      for (int a = 0; a < 4; ++a) {
+        double u[3];
+        H.G.u(0, i - (a & 1), j - (a >> 1), u);
+        if (csg1::len3(u) < csg1::kSmall) on = 0;
+      }
+      hm[(size_t)(i - 1 + g) + (size_t)D.nz * (j - 1 + g)] = on;
+    }
+  if ((rc = cs_stencil_indexing(cfg, hs.data()))) return fail(rc);
+  if (hipMemcpy(lat, H.lat.data(), sizeof(double) * H.lat.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(rbd, H.rb.data(), sizeof(double) * H.rb.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(TX, H.TX.data(), sizeof(double) * H.TX.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(TY, H.TY.data(), sizeof(double) * H.TY.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(TZ, H.TZ.data(), sizeof(double) * H.TZ.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(mask, hm.data(), sizeof(int32_t) * hm.size(), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(sten, hs.data(), sizeof(int32_t) * hs.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    cs::set_error("P1 stage: geometry upload failed");
+    return fail(CS_E_HIP);
+  }
+  D.G = H.G;
+  D.G.lat = lat;
+  D.G.rb = rbd;
+  D.G.TX = TX;
+  D.G.TY = TY;
+  D.G.TZ = TZ;
+  D.zg = zg;
+  D.mask = mask;
+  D.stencil = sten;
+  // work arrays
+  const int64_t Z6 = 6 * D.NZ;
+  if ((rc = dalloc(S, D.V, 4 * 9 * Z6, bytes)) || (rc = dalloc(S, D.flat, Z6, bytes)) ||
+      (rc = dalloc(S, D.work, Z6, bytes)) || (rc = dalloc(S, D.ms, Z6, bytes)) ||
+      (rc = dalloc(S, D.Bq, 6 * 3 * 5 * D.NFC, bytes)) || (rc = dalloc(S, D.fa, Z6, bytes)) ||
+      (rc = dalloc(S, D.prev, Z6, bytes)) || (rc = dalloc(S, D.next, Z6, bytes)) ||
+      (rc = dalloc(S, D.ever, Z6, bytes)) || (rc = dalloc(S, D.inlist, Z6, bytes)) ||
+      (rc = dalloc(S, D.list, Z6, bytes)) || (rc = dalloc(S, D.ctr, 64, bytes)))
+    return fail(rc);
+  S->all = make_box(1 - g, N + g, 1 - g, N + g, 1 - g, NR + g);
+  S->wr = make_box(-2, N + 3, -2, N + 3, -2, NR + 3);
+  S->own = make_box(1, N, 1, N, 1, NR);
+  S->fx = make_box(-3, N + 3, -2, N + 3, -2, NR + 3);
+  S->fy = make_box(-2, N + 3, -3, N + 3, -2, NR + 3);
+  S->fz = make_box(-2, N + 3, -2, N + 3, -3, NR + 3);
+  S->ffx = make_box(-1, N + 1, 0, N + 1, 0, NR + 1);
+  S->ffy = make_box(0, N + 1, -1, N + 1, 0, NR + 1);
+  S->ffz = make_box(0, N + 1, 0, N + 1, -1, NR + 1);
+  S->ex = make_box(1, N, 0, N, 0, NR);
+  S->ey = make_box(0, N, 1, N, 0, NR);
+  S->ez = make_box(0, N, 0, N, 1, NR);
+  k_zone_geo<<<dim3(nb(S->all.count(), 128), 6), 128>>>(D, S->all, zg);
+  if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+    cs::set_error("P1 stage: zone geometry kernel failed");
+    return fail(CS_E_HIP);
+  }
+  *out = S;
+  return CS_OK;
+}
+
+static Buf6 b6(const Buf& b) { return Buf6{b.U, b.ent, b.Bx, b.By, b.Bz}; }
+
+int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double* const* E, int* err,
+          hipStream_t st) {
+  const Dev& D = S->D;
+  const Buf6 in = b6(in_);
+  const int64_t nall = S->all.count(), nwr = S->wr.count();
+  k_prim<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all, in);
+  k_face_zero<<<dim3(nb(D.NFC, 256), 18), 256, 0, st>>>(D, in);
+  k_tricorner<<<dim3(nb(4 * (int64_t)D.g * D.nzr, 256), 6), 256, 0, st>>>(D, in);
+  k_flat_a<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all);
+  k_flat_b<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all);
+  k_flat_c<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all);
+  k_ctr_reset<<<1, 64, 0, st>>>(D);
+  k_weno<<<dim3(nb(3 * nwr, 128), 6), 128, 0, st>>>(D, S->wr);
+  k_pos<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr);
+  for (int sw = 0; sw < kSweeps; ++sw) {
+    k_cands<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr);
+    k_weno_list<<<kListBlocks, 128, 0, st>>>(D);
+    k_pos_list<<<kListBlocks, 256, 0, st>>>(D, sw);
+    k_commit<<<kListBlocks, 256, 0, st>>>(D, sw);
+  }
+  k_clear_ent<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr, in, err);
+  const int64_t nfp = std::max(S->fx.count(), std::max(S->fy.count(), S->fz.count()));
+  k_face_pts<<<dim3(nb(nfp, 256), 18), 256, 0, st>>>(D, S->fx, S->fy, S->fz, in);
+  k_clsq<<<dim3(nb(nall, 64), 6), 64, 0, st>>>(D, S->all, in);
+  const int64_t nff = std::max(S->ffx.count(), std::max(S->ffy.count(), S->ffz.count()));
+  k_flux<<<dim3(nb(nff, 128), 18), 128, 0, st>>>(D, S->ffx, S->ffy, S->ffz, F[0], F[1], F[2], Sp[0], Sp[1], Sp[2]);
+  const int64_t ned = std::max(S->ex.count(), std::max(S->ey.count(), S->ez.count()));
+  k_edge<<<dim3(nb(ned, 256), 18), 256, 0, st>>>(D, S->ex, S->ey, S->ez, Sp[0], Sp[1], Sp[2], E[0], E[1], E[2]);
+  P1S_HIP(hipGetLastError());
+  return CS_OK;
+}
+
+int update(Stage* S, const Buf& in, const Buf& u0, const Buf& out, double* const* F, double* const* E,
+           const double* dtp, double wa, double wb, int mode, hipStream_t st) {
+  const Dev& D = S->D;
+  const int64_t nfc = (int64_t)(D.N + 1) * (D.N + 1) * (D.NR + 1);
+  k_upd_face<<<dim3(nb(nfc, 256), 18), 256, 0, st>>>(D, b6(in), b6(u0), b6(out), E[0], E[1], E[2], dtp, wa, wb);
+  k_upd_zone<<<dim3(nb(S->own.count(), 256), 6), 256, 0, st>>>(D, S->own, b6(in), b6(u0), b6(out), F[0], F[1], F[2],
+                                                                dtp, wa, wb, mode);
+  P1S_HIP(hipGetLastError());
+  return CS_OK;
+}
+
+int imex_source(Stage* S, const Buf& un, const Buf& u1, const double* dtp, hipStream_t st) {
+  const Dev& D = S->D;
+  k_imex<<<dim3(nb(S->all.count(), 256), 6), 256, 0, st>>>(D, S->all, b6(un), b6(u1), dtp);
+  const size_t fb = sizeof(double) * (size_t)D.NFC * 6;
+  P1S_HIP(hipMemcpyAsync(u1.Bx, un.Bx, fb, hipMemcpyDeviceToDevice, st));
+  P1S_HIP(hipMemcpyAsync(u1.By, un.By, fb, hipMemcpyDeviceToDevice, st));
+  P1S_HIP(hipMemcpyAsync(u1.Bz, un.Bz, fb, hipMemcpyDeviceToDevice, st));
+  P1S_HIP(hipGetLastError());
+  return CS_OK;
+}
+
+int nblocks_cfl() { return kCflBlocks; }
+
+int cfl_partial(Stage* S, const Buf& cur, double* part, hipStream_t st) {
+  k_cfl<<<kCflBlocks, kCflThreads, 0, st>>>(S->D, b6(cur), S->cfl, part);
+  P1S_HIP(hipGetLastError());
+  return CS_OK;
+}
+
+}  // namespace p1s

@@ -436,4 +436,63 @@ __device__ __forceinline__ R emf3_llf(R S, R Ez_1, R Ez_2, R Ez_3, R Bneta_1, R 
   return ob3 * (Ez_1 + Ez_2 + Ez_3) - (R(4) * Sq3 * S / R(9)) * (Bneta_1 + Bneta_2 + Bneta_3);
 }
 
+
+// The stage update of one zone, u = combination of the stage input ui (ei),
+// U^n uz (ez) and the flux divergence L (L[8]: entropy), with the radial
+// gravity g = gr along the unit vector n:
+//  mode 0: explicit source, Shu-Osher  u = wa uz + wb (ui + dt (L + S(ui))).
+//  IMEX-SSP2(2,2,2) (Pareschi & Russo 2005; explicit part = Heun, the
+//  reference's RK2), gravity implicit, gi = 1 - 1/sqrt(2):
+//  mode 1: ui = U1, uz = Un:  U2 = Un + dt L(U1) + (1 - 2 gi) dt S(U1) + gi dt S(U2).
+//          S leaves rho unchanged and is linear in rho v, so U2 is explicit:
+//          rho v then E take the implicit term in turn.
+//  mode 2: ui = U2, uz = Un:  Un+1 = (Un + U2 + dt L(U2)) / 2 + dt/2 (2 gi S(U1) + (1 - gi) S(U2)),
+//          with U1 = Un + gi dt S(U1) rebuilt from Un.
+// uz and ez are read only when mode != 0 or wa != 0.
+__device__ __forceinline__ void rk_combine(const double* ui, double ei, const double* uz, double ez, double dt,
+                                           double wa, double wb, int mode, double gr, const double* n,
+                                           const double* L, double* u, double& en) {
+  if (mode == 0) {
+    double Lg[5] = {L[0], L[1], L[2], L[3], L[4]};
+    if (gr != 0.0) {
+      const double mr = ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2];
+      for (int q = 0; q < 3; ++q) Lg[1 + q] += ui[0] * gr * n[q];
+      Lg[4] += mr * gr;
+    }
+    for (int q = 0; q < 8; ++q) {
+      double un = ui[q] + dt * (q < 5 ? Lg[q] : L[q]);
+      u[q] = wa != 0.0 ? wa * uz[q] + wb * un : wb * un;
+    }
+    double e1 = ei + dt * L[8];
+    en = wa != 0.0 ? wa * ez + wb * e1 : wb * e1;
+    return;
+  }
+  const double gi = 1.0 - 0.70710678118654752440;
+  if (mode == 1) {
+    const double s1m = ui[0] * gr, s1e = gr * (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]);
+    u[0] = uz[0] + dt * L[0];
+    for (int q = 1; q < 4; ++q) u[q] = uz[q] + dt * L[q] + (1.0 - 2.0 * gi) * dt * s1m * n[q - 1];
+    u[4] = uz[4] + dt * L[4] + (1.0 - 2.0 * gi) * dt * s1e;
+    for (int q = 1; q < 4; ++q) u[q] += gi * dt * u[0] * gr * n[q - 1];
+    u[4] += gi * dt * gr * (u[1] * n[0] + u[2] * n[1] + u[3] * n[2]);
+    for (int q = 5; q < 8; ++q) u[q] = uz[q] + dt * L[q];
+    en = ez + dt * L[8];
+    return;
+  }
+  const double s2m = ui[0] * gr, s2e = gr * (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]);
+  const double s1m = uz[0] * gr, s1e = gr * ((uz[1] * n[0] + uz[2] * n[1] + uz[3] * n[2]) + gi * dt * uz[0] * gr);
+  for (int q = 0; q < 8; ++q) u[q] = 0.5 * uz[q] + 0.5 * (ui[q] + dt * L[q]);
+  for (int q = 1; q < 4; ++q) u[q] += 0.5 * dt * (2.0 * gi * s1m + (1.0 - gi) * s2m) * n[q - 1];
+  u[4] += 0.5 * dt * (2.0 * gi * s1e + (1.0 - gi) * s2e);
+  en = 0.5 * ez + 0.5 * (ei + dt * L[8]);
+}
+
+// The implicit first stage of IMEX-SSP2(2,2,2): U1 = Un + gi dt S(U1) in
+// closed form (rho unchanged, then rho v, then E).
+__device__ __forceinline__ void imex_pre(double dt, double gr, const double* n, double* u) {
+  const double gi = 1.0 - 0.70710678118654752440;
+  for (int q = 1; q < 4; ++q) u[q] += gi * dt * u[0] * gr * n[q - 1];
+  u[4] += gi * dt * gr * (u[1] * n[0] + u[2] * n[1] + u[3] * n[2]);
+}
+
 }  // namespace csp

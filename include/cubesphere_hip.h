@@ -45,6 +45,15 @@ extern "C" {
 /* reconstruction */
 #define CS_RECON_PCM 0   /* piecewise constant (first order)               */
 #define CS_RECON_WENO5 1 /* Jiang-Shu WENO5 of the primitive variables     */
+#define CS_RECON_P1ADP 2 /* the reference's P1 scheme: multi-D WENO of the primitives and
+                            the entropy with the flattener and the positivity limiter, the
+                            divergence-preserving constrained least squares for the zone B
+                            (LIMIT_ALL_SLOPES_P1, 410limit_all_slopes_p1.f:9), face fluxes at
+                            the face centroids (MAKE_FACIAL_FLUXES 450...f:5), edge fields
+                            (MAKE_EDGE_EFIELDS_P1 460...f:5), arc-length CT and cleanup
+                            (484...f:7, 485...f:4), CFL of EVALUATE_TIMESTEP (490...f:7);
+                            on the quadratic isoparametric geometry of 220/230/240; FP64,
+                            CS_EMF_LLF, nghost >= 4 */
 /* face Riemann solver */
 #define CS_RS_HLL 0      /* OneD_RS_MHD_Ptwise restated (610_1d_rs_oned_rs_mhd_ptwise.f:13) */
 #define CS_RS_HLLD 1     /* Miyoshi & Kusano (2005) HLLD                    */
@@ -297,6 +306,16 @@ int cs_stencil_indexing(const cs_config* cfg, int32_t* istencil);
 /* BUILD_MODE_INDICES (204build_mode_indices.f:7): monomial powers of the
  * modes of a reconstruction of the given order (1-4) */
 int cs_mode_indices(int32_t order, int32_t* n_modes, int32_t* ii, int32_t* jj, int32_t* kk);
+/* The geometry CS_RECON_P1ADP evaluates, computed on the host (no GPU):
+ * COORDS_TO_WT_QUADPTS_FACES_ISO2 / _BFLD_ISO2 (220...f:7, 230...f:8) of face
+ * family fam (0 x, 1 y, 2 z) from its separable form (vertices rb(k) u(i, j),
+ * 210...f:300-308) in the layouts of cs_face_quad (pts, wts, normal, area,
+ * centroid) plus the B-field weights bwts (face, 5); and the zone centroid
+ * (zone, 3), volume and scale length of 240...f:7 / 220...f:2212-2290.  Any
+ * output may be NULL. */
+int cs_p1s_face_geometry(const cs_config* cfg, int32_t fam, double* pts, double* wts, double* normal,
+                         double* area, double* centroid, double* bwts);
+int cs_p1s_zone_geometry(const cs_config* cfg, double* centroid, double* volume, double* scale);
 
 /* ---- P1 zone reconstruction of one patch ---------------------------------
  * LIMIT_ALL_SLOPES_P1 (400riemann_integrate/410limit_all_slopes_p1.f:9-160,

@@ -661,6 +661,15 @@ static void sync_edges(const or_grid* G, double** E) {
     }
 }
 
+/* the two averages above, exported: F[fam] and E[fam] in the layouts of
+ * the stage state, six sectors */
+void or_sync_shared(const or_grid* G, double* const* F, double* const* E) {
+  double* f[3] = {F[0], F[1], F[2]};
+  double* e[3] = {E[0], E[1], E[2]};
+  sync_fluxes(G, f);
+  sync_edges(G, e);
+}
+
 or_grid* or_grid_new(const or_config* c) {
   or_grid* G;
   int q;

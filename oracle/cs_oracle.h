@@ -71,6 +71,9 @@ double or_emf3_llf(double S, const double* ez3, const double* bn3);
 void or_rotation_source(const double* om, double rhofloor, const double* r, const double* u, double* L);
 /* CLEANUP_AFTER_UPDATE on the owned zones of a state */
 void or_cleanup(const or_grid* G, double* U, double* ent, double* Bx, double* By, double* Bz);
+/* panel-edge face fluxes and edge fields made unique (the averages taken
+ * before the update) */
+void or_sync_shared(const or_grid* G, double* const* F, double* const* E);
 /* IMEX-SSP2(2,2,2) with the implicit gravity source, u = (rho, rho v, E) */
 void or_imex_pre(double dt, double gr, const double* n, double* u);
 void or_imex_combine(int stage, double dt, double gr, const double* n, const double* ui, const double* uz,
@@ -142,6 +145,14 @@ typedef struct or_p1_flux {
   double *Efx, *Efy, *Efz;
 } or_p1_flux;
 int or_make_facial_fluxes_p1(const or_p1_flux* f);
+/* the P1 stage of a shell (CS_RECON_P1ADP): one resolved flux x area per face
+ * (riemann 0 HLL, 1 HLLD; wall: mirrored states on the radial walls), and the
+ * owned zones' update with the stage weights and the IMEX gravity */
+int or_p1s_face_fluxes(const or_p1_flux* f, int riemann, int wall, double* const* F, double* const* S);
+int or_p1s_update_zones(int N, int NR, int g, const double* const* F, const double* vol, const double* cen,
+                        const double* ui, const double* ei, const double* uz, const double* ez, double dt,
+                        double wa, double wb, int mode, double grav, double gam, double rhof, double prsf,
+                        double* uo, double* eo);
 /* OneD_RS_MHD_Ptwise with all its outputs (face frame, x-direction) */
 void or_rs_ptwise(double gam, double rhof, double prsf, double* VL, double* VR, double entL, double entR,
                   double* SL, double* SR, double* Us, double* Fs, double* entF, double* FlL, double* FlR,

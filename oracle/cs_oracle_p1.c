@@ -822,6 +822,58 @@ static void from_frame(const double* A, const double* B, const double* C, const 
     for (int q = 0; q < 3; ++q) out[o + q] = A[q] * f[o] + B[q] * f[o + 1] + C[q] * f[o + 2];
 }
 
+/* The face frame (A normal, B, C) of 450...f:507-1152 (x), :1212-1883 (y),
+ * :1940-2910 (z): x-faces the normal, the centroid's radial direction C and
+ * B = C x A; y-faces the normal, the centroid's radial direction B and
+ * C = A x B; z-faces the first point's normal, the vertex-difference
+ * direction made orthogonal, and their cross product. */
+static void face_frame(const or_p1_flux* f, const p1dims* d, int fam, int i, int j, int k, long fg, long NF,
+                       const double* nrm, const double* fcen, double* A, double* B, double* C) {
+  const double sm = f->smallnum;
+  double t;
+  int c;
+  if (fam == 0) {
+    for (c = 0; c < 3; ++c) {
+      A[c] = nrm[fg + NF * c];
+      C[c] = fcen[fg + NF * c];
+    }
+    t = 1.0 / omax(sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]), sm);
+    for (c = 0; c < 3; ++c) C[c] = C[c] * t;
+    B[0] = C[1] * A[2] - C[2] * A[1];
+    B[1] = C[2] * A[0] - C[0] * A[2];
+    B[2] = C[0] * A[1] - C[1] * A[0];
+  } else if (fam == 1) {
+    for (c = 0; c < 3; ++c) {
+      A[c] = nrm[fg + NF * c];
+      B[c] = fcen[fg + NF * c];
+    }
+    t = 1.0 / omax(sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]), sm);
+    for (c = 0; c < 3; ++c) B[c] = B[c] * t;
+    C[0] = A[1] * B[2] - A[2] * B[1];
+    C[1] = A[2] * B[0] - A[0] * B[2];
+    C[2] = A[0] * B[1] - A[1] * B[0];
+  } else {
+    for (c = 0; c < 3; ++c) A[c] = nrm[fg + NF * c]; /* the first (central) point's normal */
+    t = 1.0 / omax(sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]), sm);
+    for (c = 0; c < 3; ++c) A[c] = A[c] * t;
+    for (c = 0; c < 3; ++c)
+      B[c] = f->vert[fci(d, i, j, k) + d->NV * c] + f->vert[fci(d, i, j - 1, k) + d->NV * c] -
+             f->vert[fci(d, i - 1, j, k) + d->NV * c] - f->vert[fci(d, i - 1, j - 1, k) + d->NV * c];
+    t = B[0] * A[0] + B[1] * A[1] + B[2] * A[2];
+    for (c = 0; c < 3; ++c) B[c] = B[c] - t * A[c];
+    t = 1.0 / omax(sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]), sm);
+    for (c = 0; c < 3; ++c) B[c] = B[c] * t;
+    C[0] = A[1] * B[2] - A[2] * B[1];
+    C[1] = A[2] * B[0] - A[0] * B[2];
+    C[2] = A[0] * B[1] - A[1] * B[0];
+    t = 1.0 / omax(sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]), sm);
+    for (c = 0; c < 3; ++c) C[c] = C[c] * t;
+    B[0] = C[1] * A[2] - C[2] * A[1];
+    B[1] = C[2] * A[0] - C[0] * A[2];
+    B[2] = C[0] * A[1] - C[1] * A[0];
+  }
+}
+
 /* MAKE_FACIAL_FLUXES (400riemann_integrate/450make_facial_fluxes.f:5) at
  * second order (one Riemann problem per face, at the face centroid, weight 1),
  * MHD, no facial electric fields, no B0. */
@@ -884,48 +936,8 @@ int or_make_facial_fluxes_p1(const or_p1_flux* f) {
             continue;
           }
           long zl = zi(d, li, lj, k), zr = zi(d, ri, rj, fam == 2 ? rk : k);
-          /* the face frame (A normal, B, C) */
-          double A[3], B[3], C[3], t;
-          if (fam == 0) {
-            for (c = 0; c < 3; ++c) {
-              A[c] = nrm[fg + NF * c];
-              C[c] = fcen[fg + NF * c];
-            }
-            t = 1.0 / omax(sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]), sm);
-            for (c = 0; c < 3; ++c) C[c] = C[c] * t;
-            B[0] = C[1] * A[2] - C[2] * A[1];
-            B[1] = C[2] * A[0] - C[0] * A[2];
-            B[2] = C[0] * A[1] - C[1] * A[0];
-          } else if (fam == 1) {
-            for (c = 0; c < 3; ++c) {
-              A[c] = nrm[fg + NF * c];
-              B[c] = fcen[fg + NF * c];
-            }
-            t = 1.0 / omax(sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]), sm);
-            for (c = 0; c < 3; ++c) B[c] = B[c] * t;
-            C[0] = A[1] * B[2] - A[2] * B[1];
-            C[1] = A[2] * B[0] - A[0] * B[2];
-            C[2] = A[0] * B[1] - A[1] * B[0];
-          } else {
-            for (c = 0; c < 3; ++c) A[c] = nrm[fg + NF * c]; /* the first (central) point's normal */
-            t = 1.0 / omax(sqrt(A[0] * A[0] + A[1] * A[1] + A[2] * A[2]), sm);
-            for (c = 0; c < 3; ++c) A[c] = A[c] * t;
-            for (c = 0; c < 3; ++c)
-              B[c] = f->vert[fci(d, i, j, k) + d->NV * c] + f->vert[fci(d, i, j - 1, k) + d->NV * c] -
-                     f->vert[fci(d, i - 1, j, k) + d->NV * c] - f->vert[fci(d, i - 1, j - 1, k) + d->NV * c];
-            t = B[0] * A[0] + B[1] * A[1] + B[2] * A[2];
-            for (c = 0; c < 3; ++c) B[c] = B[c] - t * A[c];
-            t = 1.0 / omax(sqrt(B[0] * B[0] + B[1] * B[1] + B[2] * B[2]), sm);
-            for (c = 0; c < 3; ++c) B[c] = B[c] * t;
-            C[0] = A[1] * B[2] - A[2] * B[1];
-            C[1] = A[2] * B[0] - A[0] * B[2];
-            C[2] = A[0] * B[1] - A[1] * B[0];
-            t = 1.0 / omax(sqrt(C[0] * C[0] + C[1] * C[1] + C[2] * C[2]), sm);
-            for (c = 0; c < 3; ++c) C[c] = C[c] * t;
-            B[0] = C[1] * A[2] - C[2] * A[1];
-            B[1] = C[2] * A[0] - C[0] * A[2];
-            B[2] = C[0] * A[1] - C[1] * A[0];
-          }
+          double A[3], B[3], C[3];
+          face_frame(f, d, fam, i, j, k, fg, NF, nrm, fcen, A, B, C);
           /* the two zones' linear fields at the face centroid (point 1) */
           double rl[3], rr[3], VL[8], VR[8], eL, eR;
           rel(&pp, d, zl, pts + fg, NF, obl_of(&pp, zl), rl);
@@ -1601,5 +1613,166 @@ int or_pad_boundary_r_p1(const or_p1_pad* p) {
 #undef YA
 #undef ZA
 #undef BC
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* The face fluxes of the P1 stage of a shell (the product's CS_RECON_P1ADP):
+ * MAKE_FACIAL_FLUXES's faces, zones (with the tri-corner re-pointing of
+ * 450...f:581-622), frames and face-centre evaluation as above, but each face
+ * keeps one resolved flux: riemann 0 the HLL flux of OneD_RS_MHD_Ptwise
+ * (flux form: (F_L + F_R)/2 + S_L/2 (U* - U_L) - S_R/2 (U_R - U*), with its
+ * entropy fix of U*), 1 HLLD (Miyoshi & Kusano 2005); rotated back to the
+ * global frame and times the face area, with the entropy flux as a ninth
+ * component.  wall: the radial faces k = 0 and k = nr take the inner zone's
+ * face state mirrored (normal velocity and normal B reversed).  F[fam]:
+ * (9, face-B cube), S[fam]: (2, face-B cube) = S_L, S_R (x), S_D, S_U (y),
+ * S_B, S_T (z); entries outside the routine's range are left as they are. */
+int or_p1s_face_fluxes(const or_p1_flux* f, int riemann, int wall, double* const* F, double* const* S) {
+  if (!f || f->nghost < 4 || f->ntp < 1 || f->nr < 1) return 1;
+  or_p1_patch pp;
+  memset(&pp, 0, sizeof pp);
+  pp.ntp = f->ntp;
+  pp.nr = f->nr;
+  pp.nghost = f->nghost;
+  pp.mask = f->mask;
+  pp.stencil = f->stencil;
+  pp.scale = (double*)f->scale;
+  pp.v_centroid = f->v_centroid;
+  pp.V = (double*)f->V;
+  const p1dims D = p1_dims(&pp), *d = &D;
+  const int N = d->N, NR = d->NR;
+  int i, j, k, c;
+  for (int fam = 0; fam < 3; ++fam) {
+    const int ilo = fam == 0 ? -1 : 0, jlo = fam == 1 ? -1 : 0, klo = fam == 2 ? -1 : 0;
+    const double* pts = fam == 0 ? f->xf_pts : (fam == 1 ? f->yf_pts : f->zf_pts);
+    const double* nrm = fam == 0 ? f->xf_normal : (fam == 1 ? f->yf_normal : f->zf_normal);
+    const double* area = fam == 0 ? f->xf_area : (fam == 1 ? f->yf_area : f->zf_area);
+    const double* fcen = fam == 0 ? f->xf_centroid : (fam == 1 ? f->yf_centroid : f->zf_centroid);
+    const long NF = fam == 2 ? d->NZF : d->NXF;
+    for (k = klo; k <= NR + 1; ++k)
+      for (j = jlo; j <= N + 1; ++j)
+        for (i = ilo; i <= N + 1; ++i) {
+          long fb = fci(d, i, j, k), fg = fam == 0 ? xfi(d, i, j, k) : (fam == 1 ? yfi(d, i, j, k) : zfi(d, i, j, k));
+          int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1), rk = k + (fam == 2);
+          if (fam < 2) {
+            int al = active(&pp, d, li, lj, k), ar = active(&pp, d, ri, rj, k);
+            if (al && ar) {
+            } else if (!al && ar) {
+              nbr(&pp, d, ri, rj, fam == 0 ? 1 : 3, &li, &lj);
+            } else if (al && !ar) {
+              nbr(&pp, d, li, lj, fam == 0 ? 0 : 2, &ri, &rj);
+            } else {
+              continue;
+            }
+          } else if (!active(&pp, d, i, j, k)) {
+            continue;
+          }
+          long zl = zi(d, li, lj, k), zr = zi(d, ri, rj, fam == 2 ? rk : k);
+          double A[3], B[3], C[3];
+          face_frame(f, d, fam, i, j, k, fg, NF, nrm, fcen, A, B, C);
+          double rl[3], rr[3], VL[8], VR[8], eL, eR;
+          rel(&pp, d, zl, pts + fg, NF, obl_of(&pp, zl), rl);
+          rel(&pp, d, zr, pts + fg, NF, obl_of(&pp, zr), rr);
+          for (c = 0; c < 8; ++c) {
+            VL[c] = VV(&pp, d, c, 0, zl) + VV(&pp, d, c, 1, zl) * rl[0] + VV(&pp, d, c, 2, zl) * rl[1] +
+                    VV(&pp, d, c, 3, zl) * rl[2];
+            VR[c] = VV(&pp, d, c, 0, zr) + VV(&pp, d, c, 1, zr) * rr[0] + VV(&pp, d, c, 2, zr) * rr[1] +
+                    VV(&pp, d, c, 3, zr) * rr[2];
+          }
+          eL = f->ent[zl] + f->ent[zl + d->NZ] * rl[0] + f->ent[zl + 2 * d->NZ] * rl[1] + f->ent[zl + 3 * d->NZ] * rl[2];
+          eR = f->ent[zr] + f->ent[zr + d->NZ] * rr[0] + f->ent[zr + 2 * d->NZ] * rr[1] + f->ent[zr + 3 * d->NZ] * rr[2];
+          to_frame(A, B, C, VL);
+          to_frame(A, B, C, VR);
+          if (wall && fam == 2 && (k == 0 || k == NR)) {
+            double* src = k == 0 ? VR : VL;
+            double* dst = k == 0 ? VL : VR;
+            for (c = 0; c < 8; ++c) dst[c] = src[c];
+            dst[1] = -src[1];
+            dst[5] = -src[5];
+            if (k == 0) eL = eR;
+            else eR = eL;
+          }
+          double Fs[8], eF, SL, SR, Fg[8];
+          if (riemann == 1) or_hlld(f->gamma, f->rhofloor, f->prsfloor, VL, VR, eL, eR, Fs, &eF, &SL, &SR);
+          else or_hll(f->gamma, f->rhofloor, f->prsfloor, VL, VR, eL, eR, Fs, &eF, &SL, &SR);
+          from_frame(A, B, C, Fs, Fg);
+          for (c = 0; c < 8; ++c) F[fam][fb + d->NFC * c] = Fg[c] * area[fg];
+          F[fam][fb + d->NFC * 8] = eF * area[fg];
+          S[fam][fb] = SL;
+          S[fam][fb + d->NFC] = SR;
+        }
+  }
+  return 0;
+}
+
+/* The zone update of the P1 stage of a shell for the owned zones of one
+ * sector: L = -(1/V) sum over the six faces of the flux-times-area
+ * differences (9 components, the ninth the entropy), the stage combination
+ * with U^n (uz, ez) and the radial gravity g = -grav / r^2 along the zone
+ * centroid's direction: mode 0 explicit, Shu-Osher u = wa uz + wb (ui + dt
+ * (L + S(ui))); modes 1 and 2 the IMEX-SSP2(2,2,2) stages of or_imex_combine
+ * (the induction rows and the entropy as Heun); then CONS_TO_PRIM
+ * (484...f:620-628).  uo, eo receive the owned zones; the others are left. */
+int or_p1s_update_zones(int N, int NR, int g, const double* const* F, const double* vol, const double* cen,
+                        const double* ui_, const double* ei_, const double* uz_, const double* ez_, double dt,
+                        double wa, double wb, int mode, double grav, double gam, double rhof, double prsf,
+                        double* uo, double* eo) {
+  const int nz = N + 2 * g, nzr = NR + 2 * g, nf = nz + 1;
+  const long NZ = (long)nz * nz * nzr, NFC = (long)nf * nf * (nzr + 1);
+  for (int k = 1; k <= NR; ++k)
+    for (int j = 1; j <= N; ++j)
+      for (int i = 1; i <= N; ++i) {
+        const long z = (i - 1 + g) + (long)nz * ((j - 1 + g) + (long)nz * (k - 1 + g));
+        const long fp = (i + g) + (long)nf * ((j + g) + (long)nf * (k + g));
+        const long fxm = fp - 1, fym = fp - nf, fzm = fp - (long)nf * nf;
+        double L[9], ui[8], uz[8], u[8], n[3] = {0.0, 0.0, 0.0}, gr = 0.0, en;
+        int q;
+        for (q = 0; q < 9; ++q) {
+          const long o = (long)q * NFC;
+          L[q] = -(1.0 / vol[z]) * ((F[0][o + fp] - F[0][o + fxm]) + (F[1][o + fp] - F[1][o + fym]) +
+                                    (F[2][o + fp] - F[2][o + fzm]));
+        }
+        if (grav != 0.0) {
+          double c0 = cen[z], c1 = cen[z + NZ], c2 = cen[z + 2 * NZ], r2 = c0 * c0 + c1 * c1 + c2 * c2;
+          double ir = 1.0 / sqrt(r2);
+          n[0] = c0 * ir;
+          n[1] = c1 * ir;
+          n[2] = c2 * ir;
+          gr = -grav / r2;
+        }
+        for (q = 0; q < 8; ++q) {
+          ui[q] = ui_[z + NZ * q];
+          uz[q] = uz_[z + NZ * q];
+        }
+        const double ei = ei_[z], ez = ez_[z];
+        if (mode == 0) {
+          double Lg[5] = {L[0], L[1], L[2], L[3], L[4]};
+          if (gr != 0.0) {
+            for (q = 0; q < 3; ++q) Lg[1 + q] += ui[0] * gr * n[q];
+            Lg[4] += (ui[1] * n[0] + ui[2] * n[1] + ui[3] * n[2]) * gr;
+          }
+          for (q = 0; q < 8; ++q) {
+            double un = ui[q] + dt * (q < 5 ? Lg[q] : L[q]);
+            u[q] = wa != 0.0 ? wa * uz[q] + wb * un : wb * un;
+          }
+          double e1 = ei + dt * L[8];
+          en = wa != 0.0 ? wa * ez + wb * e1 : wb * e1;
+        } else {
+          or_imex_combine(mode, dt, gr, n, ui, uz, L, u);
+          if (mode == 1) {
+            for (q = 5; q < 8; ++q) u[q] = uz[q] + dt * L[q];
+            en = ez + dt * L[8];
+          } else {
+            for (q = 5; q < 8; ++q) u[q] = 0.5 * uz[q] + 0.5 * (ui[q] + dt * L[q]);
+            en = 0.5 * ez + 0.5 * (ei + dt * L[8]);
+          }
+        }
+        double V[8];
+        V[4] = 0.0;
+        or_cons_to_prim(gam, rhof, prsf, &en, u, V);
+        for (q = 0; q < 8; ++q) uo[z + NZ * q] = u[q];
+        eo[z] = en;
+      }
   return 0;
 }

@@ -833,3 +833,245 @@ def p1_shell_step_oracle(grid, probs, As, dt, sources=(1.0, 0.0, (0.0, 0.0, 0.0)
             p1u_oracle(dict(prob, dtmove=move), A)
             p1c_oracle(prob, A)
     return As
+
+
+# ---------------------------------------------------------------------------
+# The P1 stage of a shell (the product's CS_RECON_P1ADP): the reference's
+# routines (the oracle's LIMIT_ALL_SLOPES_P1, MAKE_EDGE_EFIELDS_P1,
+# CLEANUP_AFTER_UPDATE, EVALUATE_TIMESTEP), one resolved flux per face
+# (or_p1s_face_fluxes), the panel-edge averages before the update
+# (or_sync_shared), the stage update (or_p1s_update_zones and the arc-length
+# constrained transport of or_update_imex_rk_step_p1), the geometric exchange.
+
+def p1s_patches(cfg):
+    """Six per-sector P1 patches holding the quadrature geometry (pinned to
+    the reference's routines) and zeroed work arrays; the state lives in the
+    stage layout and is copied in and out per stage."""
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    geo = quad_geometry(cfg)
+    gd = geometry_dims(cfg)
+    d = p1_dims(N, NR, g)
+    st = np.zeros((N + 2 * g) ** 2 * 24, np.int32)
+    L = olib()
+    L.or_stencil_indexing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    L.or_stencil_indexing(N, g, P(st))
+    As = []
+    for s in range(6):
+        sl = lambda a, n, m: np.ascontiguousarray(a[s * n * m:(s + 1) * n * m])
+        A = dict(vert=sl(geo["vert"], gd["vert"], 3), mask=sl(geo["mask"], gd["zone"], 1), stencil=st,
+                 scale=sl(geo["scale"], gd["zone"], 1), v_centroid=sl(geo["v_cen"], gd["zone"], 3),
+                 v_volume=sl(geo["v_vol"], gd["zone"], 1))
+        for f in ("xf", "yf", "zf"):
+            nn = gd[f]
+            A[f + "_normal"] = sl(geo[f]["normal"], nn, 15 if f == "zf" else 3)
+            A[f + "_area"] = sl(geo[f]["area"], nn, 1)
+            A[f + "_centroid"] = sl(geo[f]["centroid"], nn, 3)
+            A[f + "_pts"] = sl(geo[f]["pts"], nn, 15)
+            b = f[0] + "b"
+            A[b + "_pts"] = sl(geo[b]["pts"], nn, 15)
+            A[b + "_wts"] = sl(geo[b]["wts"], nn, 5)
+            A[b + "_normal"] = sl(geo[b]["normal"], nn, 15 if f == "zf" else 3)
+        nz, nfc = d["zone"], d["fc"]
+        sizes = dict(U=32 * nz, ent=4 * nz, Bxf=nfc, Byf=nfc, Bzf=nfc, V=32 * nz, B1_zc=4 * nz, B2_zc=4 * nz,
+                     B3_zc=4 * nz, Ex_zc=4 * nz, Ey_zc=4 * nz, Ez_zc=4 * nz, gamma_eos=nz, gamma_sound=nz,
+                     mean_mol_wt=nz, divvel=nz, msonic=nz, flatten=nz, work=nz, Bxf_q=5 * nfc, Byf_q=5 * nfc,
+                     Bzf_q=5 * nfc, dU_dt=8 * nz, flag_entprs=nz)
+        for k, n in sizes.items():
+            A[k] = np.zeros(n)
+        for k in P1F_OUT:
+            if k not in A:
+                A[k] = np.zeros((8 if k.startswith("Fluct") else 1) * nfc)
+        A["V_min_local"] = np.zeros(9)
+        A["V_max_local"] = np.zeros(9)
+        As.append(A)
+    return As
+
+
+def _p1s_prob(cfg):
+    return dict(N=cfg.ntp, NR=cfg.nr, g=cfg.nghost, gamma=cfg.gamma, molewt=1.0, rhofloor=cfg.rhofloor,
+                prsfloor=cfg.prsfloor, smallnum=1e-12, flatten_coef=0.4, prs_flatten_coef=1.5)
+
+
+def p1s_stage_oracle(grid, cfg, As, st_in, st_u0, st_out, dt, irk):
+    """One stage of the P1 stage of a shell on HostState-like objects with
+    .arrays() = [U, ent, Bx, By, Bz] in the stage layout.  st_in is modified
+    in place (exchange, primitives' floors, the reconstruction's zone B and
+    energy), as the device stage modifies its input buffer."""
+    L = olib()
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    prob = _p1s_prob(cfg)
+    d = p1_dims(N, NR, g)
+    nz, nfc = d["zone"], d["fc"]
+    order, imex = cfg.rk_order, cfg.imex
+    if imex:
+        wa, wb = (0.0, 1.0) if irk == 1 else (0.5, 0.5)
+        mode = irk
+    elif order == 3:
+        wa, wb = ((0.0, 1.0), (0.75, 0.25), (1.0 / 3.0, 2.0 / 3.0))[irk - 1]
+        mode = 0
+    else:
+        wa, wb = ((0.0, 1.0), (0.5, 0.5))[irk - 1]
+        mode = 0
+    grid.halo(st_in)
+    Ui, Ei, Bi = st_in.arrays()[0], st_in.arrays()[1], st_in.arrays()[2:]
+    F = [np.zeros(6 * 9 * nfc) for _ in range(3)]
+    S = [np.zeros(6 * 2 * nfc) for _ in range(3)]
+    E = [np.zeros(6 * nfc) for _ in range(3)]
+    dbl = ctypes.POINTER(ctypes.c_double)
+    L.or_p1s_face_fluxes.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    for s, A in enumerate(As):
+        A["U"][:] = 0.0
+        A["U"][:8 * nz] = Ui[s * 8 * nz:(s + 1) * 8 * nz]
+        A["ent"][:] = 0.0
+        A["ent"][:nz] = Ei[s * nz:(s + 1) * nz]
+        for q, k in enumerate(("Bxf", "Byf", "Bzf")):
+            A[k][:] = Bi[q][s * nfc:(s + 1) * nfc]
+        p1_oracle(prob, A)
+        Ui[s * 8 * nz:(s + 1) * 8 * nz] = A["U"][:8 * nz]
+        Ei[s * nz:(s + 1) * nz] = A["ent"][:nz]
+        for q, k in enumerate(("Bxf", "Byf", "Bzf")):
+            Bi[q][s * nfc:(s + 1) * nfc] = A[k]
+        Fs = [np.zeros(9 * nfc) for _ in range(3)]
+        Ss = [np.zeros(2 * nfc) for _ in range(3)]
+        fp = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in Fs])
+        sp = (ctypes.c_void_p * 3)(*[a.ctypes.data for a in Ss])
+        sf = p1f_struct(prob, A)
+        if L.or_p1s_face_fluxes(ctypes.byref(sf), int(cfg.riemann), int(cfg.radial_bc == 1), fp, sp) != 0:
+            raise RuntimeError("oracle rejected the patch")
+        for q in range(3):
+            F[q][s * 9 * nfc:(s + 1) * 9 * nfc] = Fs[q]
+            S[q][s * 2 * nfc:(s + 1) * 2 * nfc] = Ss[q]
+        A["S_L"][:], A["S_R"][:] = Ss[0][:nfc], Ss[0][nfc:]
+        A["S_D"][:], A["S_U"][:] = Ss[1][:nfc], Ss[1][nfc:]
+        A["S_B"][:], A["S_T"][:] = Ss[2][:nfc], Ss[2][nfc:]
+        p1e_oracle(prob, A)
+        for q, k in enumerate(("Efx", "Efy", "Efz")):
+            e = A[k].copy()
+            if cfg.radial_bc == 1 and q < 2:  # conducting walls: no field on their edges
+                e3 = e.reshape(NR + 2 * g + 1, -1)
+                e3[g] = 0.0
+                e3[g + NR] = 0.0
+            E[q][s * nfc:(s + 1) * nfc] = e
+    L.or_sync_shared.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    L.or_sync_shared(grid.g, (ctypes.c_void_p * 3)(*[a.ctypes.data for a in F]),
+                     (ctypes.c_void_p * 3)(*[a.ctypes.data for a in E]))
+    Uz, Ez, Bz_ = st_u0.arrays()[0], st_u0.arrays()[1], st_u0.arrays()[2:]
+    Uo, Eo, Bo = st_out.arrays()[0], st_out.arrays()[1], st_out.arrays()[2:]
+    L.or_p1s_update_zones.argtypes = ([ctypes.c_int] * 3 + [ctypes.c_void_p] * 7 + [ctypes.c_double] * 3 +
+                                      [ctypes.c_int] + [ctypes.c_double] * 4 + [ctypes.c_void_p] * 2)
+    L.or_update_imex_rk_step_p1.argtypes = [ctypes.c_void_p]
+    for s, A in enumerate(As):
+        # faces: constrained transport of the stage input's faces, then the weights
+        W = p1_copy({k: A[k] for k in P1U_ARRAYS})
+        W["U"] = np.zeros(32 * nz)
+        W["ent"] = np.zeros(4 * nz)
+        W["ent"][:nz] = 1.0
+        W["U"][:nz] = 1.0
+        W["U"][4 * nz:5 * nz] = 1.0
+        W["dU_dt"] = np.zeros(8 * nz)
+        for k in ("EntFlux_x", "EntFlux_y", "EntFlux_z"):
+            W[k] = np.zeros(nfc)
+        for q, k in enumerate(("Efx", "Efy", "Efz")):
+            W[k] = E[q][s * nfc:(s + 1) * nfc].copy()
+        for q, k in enumerate(("Bxf", "Byf", "Bzf")):
+            W[k] = Bi[q][s * nfc:(s + 1) * nfc].copy()
+        p1u_oracle(dict(prob, dtmove=dt), W)
+        # owned faces only: x (0..N, 1..N, 1..NR), y (1..N, 0..N, 1..NR), z (1..N, 1..N, 0..NR)
+        nf = N + 2 * g + 1
+        for q, k in enumerate(("Bxf", "Byf", "Bzf")):
+            i0, j0, k0 = (0 if q == 0 else 1), (0 if q == 1 else 1), (0 if q == 2 else 1)
+            new = W[k].reshape(NR + 2 * g + 1, nf, nf)
+            b0 = Bz_[q][s * nfc:(s + 1) * nfc].reshape(NR + 2 * g + 1, nf, nf)
+            bo = Bo[q][s * nfc:(s + 1) * nfc].reshape(NR + 2 * g + 1, nf, nf)
+            sel = (slice(k0 + g, NR + 1 + g), slice(j0 + g, N + 1 + g), slice(i0 + g, N + 1 + g))
+            bo[sel] = (wa * b0[sel] + wb * new[sel]) if wa != 0.0 else wb * new[sel]
+        # zones
+        ptr = lambda a: a.ctypes.data
+        fsec = [np.ascontiguousarray(F[q][s * 9 * nfc:(s + 1) * 9 * nfc]) for q in range(3)]
+        ui = np.ascontiguousarray(Ui[s * 8 * nz:(s + 1) * 8 * nz])
+        ei = np.ascontiguousarray(Ei[s * nz:(s + 1) * nz])
+        uz = np.ascontiguousarray(Uz[s * 8 * nz:(s + 1) * 8 * nz])
+        ez = np.ascontiguousarray(Ez[s * nz:(s + 1) * nz])
+        uo = np.ascontiguousarray(Uo[s * 8 * nz:(s + 1) * 8 * nz])
+        eo = np.ascontiguousarray(Eo[s * nz:(s + 1) * nz])
+        L.or_p1s_update_zones(N, NR, g, (ctypes.c_void_p * 3)(*[ptr(a) for a in fsec]), ptr(A["v_volume"]),
+                              ptr(A["v_centroid"]), ptr(ui), ptr(ei), ptr(uz), ptr(ez), dt, wa, wb, mode,
+                              float(cfg.grav), cfg.gamma, cfg.rhofloor, cfg.prsfloor, ptr(uo), ptr(eo))
+        Uo[s * 8 * nz:(s + 1) * 8 * nz] = uo
+        Eo[s * nz:(s + 1) * nz] = eo
+    # CLEANUP_AFTER_UPDATE on the output (the new faces), owned zones kept
+    for s, A in enumerate(As):
+        C = p1_copy(A)
+        C["U"][:] = 0.0
+        C["U"][:8 * nz] = Uo[s * 8 * nz:(s + 1) * 8 * nz]
+        C["ent"][:] = 0.0
+        C["ent"][:nz] = Eo[s * nz:(s + 1) * nz]
+        for q, k in enumerate(("Bxf", "Byf", "Bzf")):
+            C[k] = Bo[q][s * nfc:(s + 1) * nfc].copy()
+        p1c_oracle(prob, C)
+        own = np.zeros((NR + 2 * g, N + 2 * g, N + 2 * g), bool)
+        own[g:g + NR, g:g + N, g:g + N] = True
+        own = own.ravel()
+        for q in range(8):
+            blk = Uo[s * 8 * nz + q * nz:s * 8 * nz + (q + 1) * nz]
+            blk[own] = C["U"][q * nz:(q + 1) * nz][own]
+        Eo[s * nz:(s + 1) * nz][own] = C["ent"][:nz][own]
+
+
+def p1s_cfl_oracle(cfg, As, st, dt_old):
+    """EVALUATE_TIMESTEP over the six sectors (U of the owned active zones
+    rewritten by CONS_TO_PRIM, as the routine does); the minimum of the six."""
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    prob = _p1s_prob(cfg)
+    d = p1_dims(N, NR, g)
+    nz = d["zone"]
+    U, E = st.arrays()[0], st.arrays()[1]
+    best = None
+    for s, A in enumerate(As):
+        A["U"][:] = 0.0
+        A["U"][:8 * nz] = U[s * 8 * nz:(s + 1) * 8 * nz]
+        A["ent"][:] = 0.0
+        A["ent"][:nz] = E[s * nz:(s + 1) * nz]
+        A["gamma_sound"][:] = cfg.gamma
+        dtn = p1_timestep_oracle(prob, A, cfg.cfl, dt_old)
+        U[s * 8 * nz:(s + 1) * 8 * nz] = A["U"][:8 * nz]
+        best = dtn if best is None else min(best, dtn)
+    return best
+
+
+def p1s_step_oracle(grid, cfg, As, st, dt):
+    """One step of the P1 stage (stage weights of SSP-RK3, Heun RK2 or
+    IMEX-SSP2(2,2,2)) on st in place; returns the next time step."""
+    other = st.copy()
+    if cfg.imex:
+        U, E = st.arrays()[0], st.arrays()[1]
+        U1, E1 = other.arrays()[0], other.arrays()[1]
+        B, B1 = st.arrays()[2:], other.arrays()[2:]
+        N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+        nz = (N + 2 * g) ** 2 * (NR + 2 * g)
+        L = olib()
+        L.or_imex_pre.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+        U1[:] = U
+        E1[:] = E
+        for q in range(3):
+            B1[q][:] = B[q]
+        for s, A in enumerate(As):
+            mask = A["mask"]
+            cen = A["v_centroid"].reshape(3, nz)
+            for z in np.nonzero(mask == 1)[0]:
+                c = cen[:, z]
+                r2 = float(c @ c)
+                n = np.ascontiguousarray(c / np.sqrt(r2))
+                u = np.ascontiguousarray(U1[s * 8 * nz + z::nz][:8])
+                L.or_imex_pre(dt, -cfg.grav / r2, n.ctypes.data, u.ctypes.data)
+                U1[s * 8 * nz + z::nz][:8] = u
+        p1s_stage_oracle(grid, cfg, As, other, st, other, dt, 1)
+        p1s_stage_oracle(grid, cfg, As, other, st, st, dt, 2)
+    elif cfg.rk_order == 3:
+        p1s_stage_oracle(grid, cfg, As, st, st, other, dt, 1)
+        p1s_stage_oracle(grid, cfg, As, other, st, other, dt, 2)
+        p1s_stage_oracle(grid, cfg, As, other, st, st, dt, 3)
+    else:
+        p1s_stage_oracle(grid, cfg, As, st, st, other, dt, 1)
+        p1s_stage_oracle(grid, cfg, As, other, st, st, dt, 2)
+    return p1s_cfl_oracle(cfg, As, st, dt)

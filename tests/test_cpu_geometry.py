@@ -161,3 +161,66 @@ def test_stencil_indexing(N, g):
                 assert 0.4 * h < dist < 1.6 * h, (x, y, d, vx, vy, dist / h)
                 checked += 1
     assert checked > 4 * N * N
+
+
+# ---------------------------------------------------------------------------
+# The separable form of the same geometry (cs_p1geo.h): the P1 stage of a
+# shell evaluates every face's points, weights, normals, area and centroid
+# from per-column tables and the radial vertices instead of storing them.
+# Exact in exact arithmetic (the element map of a lateral face is r(t) u(s),
+# of a radial face rb u(s, t)); compared here with the oracle's per-face
+# quadrature (pinned bit for bit to the reference's 220/230 above) on every
+# face of an active zone, and the zone volume, centroid and scale length
+# (evaluated from the vertices as 240 does) bit for bit.
+
+SEPARABLE = {
+    "shell6x3_g4": cs.make_config(6, 3, 4, radial_bc=cs.RBC_PAD, rmin=1.0, rmax=1.5),
+    "surface8_g4": cs.make_config(8, 1, 4),
+    "shell5x3_g4_log": cs.make_config(5, 3, 4, radial_bc=cs.RBC_PAD, radial_mesh=cs.RMESH_LOG, rmin=1.0, rmax=2.0),
+}
+
+
+@pytest.mark.parametrize("name", list(SEPARABLE))
+def test_separable_face_geometry_matches_quadrature(name):
+    cfg = SEPARABLE[name]
+    o = orc.quad_geometry(cfg)
+    d = orc.geometry_dims(cfg)
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    nz, nf = N + 2 * g, N + 2 * g + 1
+    act2 = o["mask"].reshape(6, NR + 2 * g, nz, nz)[0, 0].astype(bool)   # zone columns (j, i), same in every sector
+    P = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    for fam, f in enumerate(("xf", "yf", "zf")):
+        n = d[f]
+        G = dict(pts=np.zeros(6 * 15 * n), wts=np.zeros(6 * 5 * n), normal=np.zeros(6 * (15 if fam == 2 else 3) * n),
+                 area=np.zeros(6 * n), centroid=np.zeros(6 * 3 * n), bw=np.zeros(6 * 5 * n))
+        assert cs.lib().cs_p1s_face_geometry(ctypes.byref(cfg), fam, P(G["pts"]), P(G["wts"]), P(G["normal"]),
+                                             P(G["area"]), P(G["centroid"]), P(G["bw"])) == 0
+        # faces with an active zone on either side
+        if fam == 0:
+            a = np.zeros((nz, nf), bool)
+            a[:, 1:] |= act2
+            a[:, :-1] |= act2
+            m = np.broadcast_to(a, (NR + 2 * g, nz, nf)).ravel()
+        elif fam == 1:
+            a = np.zeros((nf, nz), bool)
+            a[1:, :] |= act2
+            a[:-1, :] |= act2
+            m = np.broadcast_to(a, (NR + 2 * g, nf, nz)).ravel()
+        else:
+            m = np.broadcast_to(act2, (NR + 2 * g + 1, nz, nz)).ravel()
+        for k, per in (("pts", 15), ("wts", 5), ("centroid", 3), ("normal", 15 if fam == 2 else 3), ("area", 1)):
+            a = G[k].reshape(6, per, n)[:, :, m]
+            b = o[f][k].reshape(6, per, n)[:, :, m]
+            assert _rel(a, b) < 2e-14, (f, k, _rel(a, b))
+        a = G["bw"].reshape(6, 5, n)[:, :, m]
+        b = o[f[0] + "b"]["wts"].reshape(6, 5, n)[:, :, m]
+        assert np.abs(a - b).max() < 2e-14, (f, "bw")
+    cen, vol, scl = np.zeros(6 * 3 * d["zone"]), np.zeros(6 * d["zone"]), np.zeros(6 * d["zone"])
+    assert cs.lib().cs_p1s_zone_geometry(ctypes.byref(cfg), P(cen), P(vol), P(scl)) == 0
+    act = o["mask"].astype(bool)
+    assert np.array_equal(vol[act], o["v_vol"][act])
+    assert np.array_equal(scl[act], o["scale"][act])
+    c3, o3 = cen.reshape(6, 3, -1), o["v_cen"].reshape(6, 3, -1)
+    am = act.reshape(6, -1)
+    for s in range(6):
+        assert np.array_equal(c3[s][:, am[s]], o3[s][:, am[s]])
