@@ -32,9 +32,10 @@
 // completely; when any do, the zones with such a neighbour earlier in the
 // loop order (k_cands) are recomputed (k_weno9, k_pos on that list) with the
 // means of the flagged earlier zones taken as zero, repeating until the flags
-// no longer change (the dependence only runs forward in the loop order, so
-// this reaches the sequential result), and the flagged means are then
-// cleared.
+// no longer change; a zone flagged for the first time in a sweep brings its
+// later neighbours onto the list before the next sweep (the dependence only
+// runs forward in the loop order, so this reaches the sequential result), and
+// the flagged means are then cleared.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -647,13 +648,16 @@ __global__ void __launch_bounds__(256) k_pos(const P1 P, const Box b, const int6
   }
 }
 
-// the zones of the range with an fa zone earlier in the loop order among their
-// 27: appended to list (order immaterial)
-__global__ void __launch_bounds__(256) k_cands(const P1 P, const Box b, const uint8_t* fa, int64_t* list,
-                                               unsigned int* count) {
+// the zones of the range with a zone ever flagged earlier in the loop order
+// among their 27 and not yet on the list: appended to it (order immaterial;
+// the list only grows, so a zone whose result depends on a flag is
+// recomputed in every later sweep)
+__global__ void __launch_bounds__(256) k_cands(const P1 P, const Box b, const uint8_t* ever, uint8_t* inlist,
+                                               int64_t* list, unsigned int* count) {
   int i, j, k;
   if (!b.decode(tid(), i, j, k)) return;
   const int64_t z = P.zi(i, j, k), ks = (int64_t)P.nz * P.nz;
+  if (inlist[z]) return;
   int64_t col[9];
   columns9(P, i, j, col);
   bool cand = false;
@@ -661,9 +665,23 @@ __global__ void __launch_bounds__(256) k_cands(const P1 P, const Box b, const ui
 #pragma unroll
     for (int dk = -1; dk <= 1; ++dk) {
       const int64_t zz = col[q] + ks * (k + dk);
-      cand = cand || (zz < z && fa[zz]);
+      cand = cand || (zz < z && ever[zz]);
     }
-  if (cand) list[atomicAdd(count, 1u)] = z;
+  if (cand) {
+    inlist[z] = 1;
+    list[atomicAdd(count, 1u)] = z;
+  }
+}
+
+// after a sweep: the list's new flags become the previous ones and join the
+// ever-flagged set
+__global__ void __launch_bounds__(256) k_commit(const int64_t* list, int64_t n, const uint8_t* next, uint8_t* prev,
+                                                uint8_t* ever) {
+  const int64_t t = tid();
+  if (t >= n) return;
+  const int64_t z = list[t];
+  prev[z] = next[z];
+  ever[z] = ever[z] | next[z];
 }
 
 // the entropy means of the zones that lost all their slopes
@@ -1081,7 +1099,7 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms) {
   Scratch& S = g_scratch[dev];
   const size_t flag_bytes = ((size_t)P.NZ + 255) / 256 * 256;
   const size_t list_bytes = (size_t)wr.count() * sizeof(int64_t);
-  const size_t need = 3 * flag_bytes + 256 + (size_t)nblk_diag * 18 * sizeof(double) + 18 * sizeof(double) +
+  const size_t need = 5 * flag_bytes + 256 + (size_t)nblk_diag * 18 * sizeof(double) + 18 * sizeof(double) +
                       list_bytes;
   if (S.bytes < need) {
     if (S.mem) P1_HIP(hipFree(S.mem));
@@ -1098,8 +1116,10 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms) {
   F.fa = base;
   F.prev = base + flag_bytes;
   F.next = base + 2 * flag_bytes;
-  F.counter = (unsigned int*)(base + 3 * flag_bytes);
-  double* part = (double*)(base + 3 * flag_bytes + 256);
+  uint8_t* ever = base + 3 * flag_bytes;    // flagged in the first pass or any sweep
+  uint8_t* inlist = base + 4 * flag_bytes;  // on the replay list
+  F.counter = (unsigned int*)(base + 5 * flag_bytes);
+  double* part = (double*)(base + 5 * flag_bytes + 256);
   double* red = part + (size_t)nblk_diag * 18;
   int64_t* list = (int64_t*)(red + 18);
   unsigned int* lcount = F.counter + 1;
@@ -1152,24 +1172,31 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms) {
   P1_HIP(hipStreamSynchronize(st));
   if (nfail > 0) {
     // replay the loop-order dependence on the zones it can reach until the
-    // flags settle
-    k_cands<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, F.fa, list, lcount);
-    unsigned int ncand = 0;
-    P1_HIP(hipMemcpyAsync(&ncand, lcount, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
-    P1_HIP(hipStreamSynchronize(st));
+    // flags settle: every sweep recomputes the list with the previous sweep's
+    // flags; a zone newly flagged in a sweep brings its later neighbours
+    // onto the list before the next one
     P1_HIP(hipMemcpyAsync(F.prev, F.fa, P.NZ, hipMemcpyDeviceToDevice, st));
     P1_HIP(hipMemcpyAsync(F.next, F.fa, P.NZ, hipMemcpyDeviceToDevice, st));
-    bool settled = ncand == 0;
+    P1_HIP(hipMemcpyAsync(ever, F.fa, P.NZ, hipMemcpyDeviceToDevice, st));
+    P1_HIP(hipMemsetAsync(inlist, 0, P.NZ, st));
+    unsigned int ncand = 0;
+    bool settled = false;
     for (int it = 0; it < 256 && !settled; ++it) {
+      k_cands<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, ever, inlist, list, lcount);
       P1_HIP(hipMemsetAsync(F.counter, 0, sizeof(unsigned int), st));
+      P1_HIP(hipMemcpyAsync(&ncand, lcount, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+      P1_HIP(hipStreamSynchronize(st));
+      if (ncand == 0) {
+        settled = true;
+        break;
+      }
       k_weno9<<<grid_of(3 * (int64_t)ncand, 128), 128, 0, st>>>(P, wr, list, ncand, wv);
       k_pos<<<grid_of(ncand, 256), 256, 0, st>>>(P, wr, list, ncand, F);
+      k_commit<<<grid_of(ncand, 256), 256, 0, st>>>(list, ncand, F.next, F.prev, ever);
       P1_HIP(hipGetLastError());
       unsigned int nchg = 0;
       P1_HIP(hipMemcpyAsync(&nchg, F.counter, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
       P1_HIP(hipStreamSynchronize(st));
-      // next now holds fa outside the list and the new flags on it
-      P1_HIP(hipMemcpyAsync(F.prev, F.next, P.NZ, hipMemcpyDeviceToDevice, st));
       settled = nchg == 0;
     }
     if (!settled) {
