@@ -132,6 +132,7 @@ def lib():
             "cs_stencil_indexing": [cfgp, vp],
             "cs_limit_all_slopes_p1": [vp, vp],
             "cs_limit_all_slopes_p1_timed": [vp, vp, ctypes.POINTER(ctypes.c_float)],
+            "cs_limit_all_slopes_p1_noalloc": [vp, vp],
             "cs_make_facial_fluxes_p1": [vp, vp],
             "cs_update_imex_rk_step_p1": [vp, vp],
             "cs_cleanup_after_update_p1": [vp, vp],
@@ -552,13 +553,16 @@ P1_STAGES = ("prim", "face_zero+tricorner", "flatten", "weno+positivity", "face_
              "minmax")
 
 
-def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=None, minmax=True, timed=False):
+def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=None, minmax=True, timed=False,
+                        noalloc=False):
     """Run LIMIT_ALL_SLOPES_P1 on one patch whose arrays are torch tensors on
     the GPU (float64, int32 for mask and stencil), contiguous, in the
     reference's layouts; they are updated in place.  Every tensor's size is
     checked against the layout before the launch.  Returns (V_min_local,
     V_max_local) as numpy arrays of 9 when minmax, and with timed=True also
-    the per-stage milliseconds (P1_STAGES)."""
+    the per-stage milliseconds (P1_STAGES).  noalloc=True calls
+    cs_limit_all_slopes_p1_noalloc (work memory in the library's fixed
+    per-GPU array, nothing allocated)."""
     import torch
     want = p1_counts(N, NR, g)
     s = P1Patch()
@@ -584,7 +588,10 @@ def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=Non
         ms = (ctypes.c_float * len(P1_STAGES))()
         _check(lib().cs_limit_all_slopes_p1_timed(ctypes.byref(s), st, ms), "cs_limit_all_slopes_p1_timed")
         return vmin, vmax, list(ms)
-    _check(lib().cs_limit_all_slopes_p1(ctypes.byref(s), st), "cs_limit_all_slopes_p1")
+    if noalloc:
+        _check(lib().cs_limit_all_slopes_p1_noalloc(ctypes.byref(s), st), "cs_limit_all_slopes_p1_noalloc")
+    else:
+        _check(lib().cs_limit_all_slopes_p1(ctypes.byref(s), st), "cs_limit_all_slopes_p1")
     return vmin, vmax
 
 

@@ -1067,7 +1067,10 @@ static int p1_check(const cs_p1_patch* p) {
 static inline unsigned grid_of(int64_t n, int threads) { return (unsigned)((n + threads - 1) / threads); }
 
 // kernel_ms (optional): per-stage times from HIP events on the stream
-static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms) {
+// ext (optional): caller-provided device work memory of ext_bytes, used
+// instead of the library's per-device allocation (no allocation then)
+static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t* ext = nullptr,
+                  size_t ext_bytes = 0) {
   int rc = p1_check(p);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream_;
@@ -1094,24 +1097,35 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms) {
 
   int dev = 0;
   P1_HIP(hipGetDevice(&dev));
-  std::lock_guard<std::mutex> lock(g_mu);
-  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
-  Scratch& S = g_scratch[dev];
   const size_t flag_bytes = ((size_t)P.NZ + 255) / 256 * 256;
   const size_t list_bytes = (size_t)wr.count() * sizeof(int64_t);
   const size_t need = 5 * flag_bytes + 256 + (size_t)nblk_diag * 18 * sizeof(double) + 18 * sizeof(double) +
                       list_bytes;
-  if (S.bytes < need) {
-    if (S.mem) P1_HIP(hipFree(S.mem));
-    S.mem = nullptr;
-    S.bytes = 0;
-    if (hipMalloc(&S.mem, need) != hipSuccess) {
-      cs::set_error("cs_limit_all_slopes_p1: cannot allocate %zu bytes of work memory", need);
-      return CS_E_NOMEM;
+  std::unique_lock<std::mutex> lock(g_mu, std::defer_lock);
+  uint8_t* base = nullptr;
+  if (ext) {
+    if (need > ext_bytes) {
+      cs::set_error("cs_limit_all_slopes_p1_noalloc: the patch (%d^2 x %d, nghost %d) needs %zu bytes of work "
+                    "memory, the entry point holds %zu", N, NR, g, need, ext_bytes);
+      return CS_E_INVALID;
     }
-    S.bytes = need;
+    base = ext;
+  } else {
+    lock.lock();
+    if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
+    Scratch& S = g_scratch[dev];
+    if (S.bytes < need) {
+      if (S.mem) P1_HIP(hipFree(S.mem));
+      S.mem = nullptr;
+      S.bytes = 0;
+      if (hipMalloc(&S.mem, need) != hipSuccess) {
+        cs::set_error("cs_limit_all_slopes_p1: cannot allocate %zu bytes of work memory", need);
+        return CS_E_NOMEM;
+      }
+      S.bytes = need;
+    }
+    base = (uint8_t*)S.mem;
   }
-  uint8_t* base = (uint8_t*)S.mem;
   Flags F;
   F.fa = base;
   F.prev = base + flag_bytes;
@@ -1243,6 +1257,43 @@ extern "C" int cs_limit_all_slopes_p1_timed(const cs_p1_patch* p, void* stream, 
     return CS_E_INVALID;
   }
   return run_p1(p, stream, kernel_ms);
+}
+
+// Work memory of cs_limit_all_slopes_p1_noalloc: the positivity flags, the
+// replay list and counters and the min/max partials live in this fixed
+// array instead of a hipMalloc'd buffer.  Each device holds its own copy
+// (from the module load until the process ends) at its own address, which
+// the host must look up: the array's name is no device address in host code.
+constexpr size_t kNoAllocBytes = (size_t)48 << 20;
+__device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_na_mu;
+uint8_t* g_na_ptr[kMaxDevices] = {};  // device address of scratch, per device
+}  // namespace
+
+extern "C" int cs_limit_all_slopes_p1_noalloc(const cs_p1_patch* p, void* stream) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    cs::set_error("cs_limit_all_slopes_p1_noalloc: no usable current device");
+    return CS_E_HIP;
+  }
+  uint8_t* ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_na_mu);
+    if (!g_na_ptr[dev]) {
+      void* q = nullptr;
+      const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+      if (e != hipSuccess || !q) {
+        cs::set_error("cs_limit_all_slopes_p1_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        return CS_E_HIP;
+      }
+      g_na_ptr[dev] = (uint8_t*)q;
+    }
+    ptr = g_na_ptr[dev];
+  }
+  return run_p1(p, stream, nullptr, ptr, kNoAllocBytes);
 }
 
 // ===========================================================================

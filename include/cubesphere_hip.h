@@ -366,6 +366,16 @@ int cs_limit_all_slopes_p1(const cs_p1_patch* p, void* stream);
  * min/max */
 #define CS_P1_STAGES 8
 int cs_limit_all_slopes_p1_timed(const cs_p1_patch* p, void* stream, float* kernel_ms);
+/* The same as cs_limit_all_slopes_p1, allocating nothing: no counterpart in
+ * the reference; for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Its work memory (positivity flags, replay list and counters, min/max
+ * partials) is a fixed 48 MiB device array per GPU, held from the library's
+ * load until the process ends; a patch needing more (about 3.5 M zones with
+ * ghosts, e.g. 192^2 x 64 at nghost 4) is refused with CS_E_INVALID before
+ * anything runs.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU race. */
+int cs_limit_all_slopes_p1_noalloc(const cs_p1_patch* p, void* stream);
 
 /* ---- P1 facial fluxes of one patch -----------------------------------------
  * MAKE_FACIAL_FLUXES (400riemann_integrate/450make_facial_fluxes.f:5-60) at
