@@ -1,11 +1,15 @@
 """Benchmark of the cubed-sphere cell-update path on one MI355X.
 
-Workload (default): 2-D cubed-sphere surface, 6 x 256 x 256 zones, WENO5
-reconstruction + HLLD faces + LLF-symmetrised 2-D HLL edge EMFs + constrained
-transport, SSP-RK3, magnetised blast (p = 10 in a 0.1 rad cap, B = 0.5 east),
-gamma = 5/3.  A step is one full SSP-RK3 time step (3 stages + CFL) of all
-393,216 zones, captured once in a hipGraph and replayed; inputs are resident
-in HBM before the timed region.
+Workload (default, C5): one spherical shell of 6 x 512^2 x 128 zones (201 M
+owned zones, r in [1, 1.5], NGhost 4) advanced by the reference's P1 scheme
+(WENO-ADP: multi-D WENO with the flattener and the pressure-positivity
+limiter, the ADP constrained least squares for the zone B), HLLD face
+fluxes, the 4-state HLL edge fields with arc-length constrained transport,
+and IMEX-SSP2(2,2,2) with the radial gravity g = 1/r^2 implicit; hydrostatic
+state with 1e-2 density noise.  A step is one full IMEX time step (the
+source-only stage, two flux stages, CLEANUP, EVALUATE_TIMESTEP) of all six
+sectors, captured once in a hipGraph and replayed; inputs are resident in HBM
+before the timed region.
 
 One JSON line on stdout; details on stderr.
 """
@@ -630,6 +634,253 @@ def cpu_baseline_p1(max_seconds):
             "sample": f"{calls} calls on a 32^2x16 patch (NGhost 4), single thread, {el:.1f} s"}
 
 
+# Algorithmic HBM bytes per work item of the P1 stage's launch groups (the
+# words an item must read or write once, data shared by neighbouring items
+# counted once): the work items are zones of the stated range, faces or edges.
+#   prim          zone (ghosts incl.): U + entropy read and written (18), V means
+#                 (9) and the magnetosonic speed (1) written
+#   flatten       zone: speed, velocity, centroid, scale, pressure, three flag
+#                 passes (14 words)
+#   weno          zone of the WENO range: V means 9, centroid 3, scale 1,
+#                 flattener 1 read, slopes 27 written
+#   positivity    zone of the WENO range: V means 9 and slopes 27 read,
+#                 centroid 3, scale 1, flattener 1 (rewritten slopes not
+#                 counted: only reduced zones write them)
+#   face_points   face: the B modes of its two zones (each zone's 12 shared by
+#                 its two faces of the family: 12), face B 1, five points written
+#   clsq          zone of the WENO range: B modes 12, zone B and energy 4, its
+#                 three own face B and six own face-point values, centroid 3,
+#                 scale 1 read; B modes 12 and U 4 written
+#   flux          face: its two zones' 36 V words (each zone's shared by its
+#                 two faces of the family: 36), centroid and scale 4; flux x
+#                 area 9 and speeds 2 written
+#   edge          edge: v and B of its zones (6, each zone's shared by its
+#                 four edges of the family: 6), 4 face-point values and 4
+#                 speeds; the field written
+#   update_faces  owned face: B of the stage input and of U^n, the edge
+#                 fields (each shared by 4 faces: 1), B written
+#   update_zones  owned zone: the fluxes of its six faces (each shared by
+#                 two zones: 27), stage input and U^n (9 + 9), its three face
+#                 B, centroid 3, volume 1; U + entropy written (9)
+#   imex          zone (ghosts incl.): U + entropy read and written, centroid
+#                 3, face B copied (3 + 3)
+#   cfl           owned zone: U + entropy 9 and scale 1 read, U 8 written
+P1K_BYTES = {"prim": 8 * 28, "flatten": 8 * 14, "weno": 8 * 41, "positivity": 8 * 41, "face_points": 8 * 18,
+             "clsq": 8 * 45, "flux": 8 * 51, "edge": 8 * 15, "update_faces": 8 * 4, "update_zones": 8 * 61,
+             "imex": 8 * 27, "cfl": 8 * 18}
+
+
+def p1k_units(cfg):
+    """Work items of one launch (group) of each P1 stage kernel over the six
+    sectors: zones with ghosts, the WENO range (-2 .. N+3), faces of the flux
+    and face-point ranges, edges and owned faces and zones."""
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    zones_all = 6 * (N + 2 * g) ** 2 * (NR + 2 * g)
+    zones_wr = 6 * (N + 6) ** 2 * (NR + 6)
+    own = 6 * N * N * NR
+    fpts = 6 * 3 * (N + 7) * (N + 6) * (NR + 6)
+    fflux = 6 * 3 * (N + 3) * (N + 2) * (NR + 2)
+    edges = 6 * 3 * N * (N + 1) * (NR + 1)
+    ofaces = 6 * 3 * (N + 1) * N * NR
+    return {"prim": zones_all, "flatten": zones_all, "weno": zones_wr, "positivity": zones_wr,
+            "face_points": fpts, "clsq": zones_wr, "flux": fflux, "edge": edges, "update_faces": ofaces,
+            "update_zones": own, "imex": zones_all, "cfl": own}
+
+
+# compulsory HBM bytes per cell-update of a whole step (the state of 12 FP64
+# words read 5 times and written 3 times: IMEX-SSP2(2,2,2) reads U^n in its
+# source stage and both flux stages and the two stage inputs, and writes U1,
+# U2 and U^n+1; SSP-RK3 moves the same 8 copies); the metric is separable
+# (O(N^2 + nr) tables), so it adds nothing for nr >= 16
+B_ALG_3D = 8 * 12 * 8
+# the surface (nr = 1): the same plus 3 stages x 26 metric words per zone
+B_ALG_SURFACE = B_ALG_3D + 3 * 26 * 8
+
+
+def host_cpu_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
+
+
+def _p1adp_cpu_worker(args):
+    """One host process: the oracle's step of the same scheme on its own small
+    shell, steps counted for max_seconds (returns (cell-updates, seconds))."""
+    name, N, NR, seed, max_seconds = args
+    import cubesphere_amd as cs
+    from oracle import oracle as orc
+    cfg0, prob, _ = cs.workload(name, cs.FP64)
+    cfg = cs.make_config(N, 1 if cfg0.nr == 1 else NR, cfg0.nghost, cfg0.recon, cfg0.riemann, cfg0.emf, cs.FP64,
+                         cfg0.rk_order, cfg0.radial_bc, cfg0.radial_mesh, cfg0.rmin, cfg0.rmax, grav=cfg0.grav,
+                         imex=cfg0.imex)
+    st = cs.init_problem(cfg, prob, seed=seed)
+    G = orc.Grid(cfg)
+    As = orc.p1s_patches(cfg)
+    dt = orc.p1s_cfl_oracle(cfg, As, st, 1.0)
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        dt = orc.p1s_step_oracle(G, cfg, As, st, dt)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el > max_seconds or steps >= 50:
+            break
+    return 6 * cfg.ntp * cfg.ntp * cfg.nr * steps, el, steps
+
+
+def cpu_baseline_p1adp(cs, name, max_seconds):
+    """The oracle's step of the same scheme (its restatement of the
+    reference's routines; 450, 470 and 484 do not build from the reference's
+    sources, so no reference build of a whole step exists) on the host: one
+    small shell per core, as independent processes on every core the box
+    gives the job (at most 16), their throughputs summed."""
+    import multiprocessing as mp
+    info = host_cpu_info()
+    cores = max(1, min(info["nproc"] or 1, 16))
+    N, NR = (32, 16) if name != "C2" else (48, 1)
+    ctx = mp.get_context("fork")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_p1adp_cpu_worker, [(name, N, NR, 100 + q, max_seconds) for q in range(cores)])
+    wall = time.perf_counter() - t0
+    value = sum(c / el for c, el, _ in res)
+    steps = sum(n for _, _, n in res)
+    log(f"cpu baseline: {cores} processes, {steps} oracle steps in {wall:.1f} s, {value:.3e} cell-updates/s")
+    return {"value": value, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "cpu_model": info["model"], "nproc": info["nproc"],
+            "sample": f"the oracle's step of the same scheme (its C restatement of the reference's routines, "
+                      f"driven per sector) on a 6x{N}^2x{NR if name != 'C2' else 1} shell per process, "
+                      f"{cores} independent processes for about {max_seconds:.0f} s each ({steps} steps in all)"}
+
+
+def reference_cpu_leg(seconds=10.0, N=48, NR=24):
+    """The reference's own LIMIT_ALL_SLOPES_P1 built from its sources with its
+    OpenMP directives (flang -fopenmp) on a P1 patch, zones per second, on the
+    cores the box gives the job; None when that build is absent."""
+    from oracle import oracle as orc
+    info = host_cpu_info()
+    cores = max(1, min(info["nproc"] or 1, 16))
+    os.environ["OMP_NUM_THREADS"] = str(cores)  # read when the OpenMP runtime loads
+    R = orc.rlib(openmp=True)
+    if R is None:
+        return None
+    prob, A0 = orc.p1_problem(N, NR, 4, sector=0, seed=7, shock=True, noise=0.3)
+    calls, t0 = 0, time.perf_counter()
+    while True:
+        orc.p1_reference(prob, orc.p1_copy(A0), lib=R)
+        calls += 1
+        el = time.perf_counter() - t0
+        if el > seconds or calls >= 500:
+            break
+    return {"routine": "LIMIT_ALL_SLOPES_P1 (410...f:9) with 440 and 120", "value": N * N * NR * calls / el,
+            "unit": "zones/s", "cores": cores, "kind": "reference",
+            "sample": f"{calls} calls on a {N}^2x{NR} patch (NGhost 4), flang -O2 -fopenmp, OMP_NUM_THREADS={cores}"}
+
+
+def bench_p1adp(cs, args, cpu):
+    """C5 (or C2) with the reference's P1 scheme: FP64 steps replayed from one
+    hipGraph per step, per-launch times from an eagerly run step with a
+    hipEvent pair around every launch group, the dominant group's roofline,
+    the whole step's share of the HBM roofline, div B and the conservation
+    integrals."""
+    name = args.workload
+    cfg, prob, _ = cs.workload(name, cs.FP64)
+    st = initial_state(cs, name)
+    t0 = time.perf_counter()
+    dev = cs.CubeSphere(cfg)
+    dev.upload(st)
+    _STATES.pop(name, None)
+    del st
+    log(f"{name}: handle created and state uploaded in {time.perf_counter() - t0:.1f} s")
+    try:
+        dev.dt = 0.0
+        dev.evaluate_timestep()
+        divb0 = dev.max_divb()
+        sums0 = dev.conservation_sums()
+        dev.advance(args.warmup, use_graph=True)
+        ms = dev.time_steps(args.steps, use_graph=True)
+        prof = dev.profile_steps_p1(args.profile_steps)
+        divb = dev.max_divb()
+        sums = dev.conservation_sums()
+        dt = dev.dt
+    finally:
+        dev.close()
+    n_int = 6 * cfg.ntp * cfg.ntp * cfg.nr
+    cups = n_int * args.steps / (ms * 1e-3)
+    log(f"{name} fp64: {ms / args.steps:.2f} ms/step, {cups:.3e} cell-updates/s, dt {dt:.3e}, "
+        f"max rel div B {divb0:.2e} -> {divb:.2e}")
+    units = p1k_units(cfg)
+    kern = {}
+    for k, (t, calls) in sorted(prof.items(), key=lambda kv: -kv[1][0] * kv[1][1]):
+        e = {"ms": t, "launches_per_step": calls / args.profile_steps}
+        if k in P1K_BYTES:
+            e["units"] = units[k]
+            e["alg_GBps"] = P1K_BYTES[k] * units[k] / (t * 1e-3) / 1e9
+        kern[k] = e
+        log(f"   {k:14s} {t:9.3f} ms x {e['launches_per_step']:4.1f}"
+            + (f"  units {units[k]:11d}  alg {e['alg_GBps']:8.1f} GB/s" if k in P1K_BYTES else ""))
+    step_ms_prof = sum(e["ms"] * e["launches_per_step"] for e in kern.values())
+    dom = max((k for k in kern if k in P1K_BYTES), key=lambda k: kern[k]["ms"] * kern[k]["launches_per_step"])
+    kd = kern[dom]
+    alg = P1K_BYTES[dom] * units[dom]
+    roof = {"bound": "hbm", "kernel": dom, "achieved": kd["alg_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": kd["alg_GBps"] / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": alg,
+            "alg_bytes_per_unit": P1K_BYTES[dom], "units_per_launch": units[dom],
+            "avg_launch_us": kd["ms"] * 1e3,
+            "launch_time": f"hipEvent pair around each launch group on the library's stream, "
+                           f"{args.profile_steps} step(s) run launch by launch"}
+    b_alg = B_ALG_SURFACE if cfg.nr == 1 else B_ALG_3D
+    step_gbs = cups * b_alg / 1e9
+    # the radial boundaries of a shell are open (outflow pads): its mass
+    # changes by what crosses them; the surface between walls is closed
+    rel_mass = abs(sums[0] / sums0[0] - 1.0) if sums0[0] else None
+    mass_key = "rel_mass_change_open_radial_boundaries" if cfg.nr > 1 else "rel_mass_change_closed"
+    ref_leg = None
+    if not args.no_cpu:
+        try:
+            ref_leg = reference_cpu_leg()
+        except Exception as e:
+            log("reference CPU leg failed:", e)
+    line = {
+        "metric": METRIC,
+        "value": cups,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{name}: 6x{cfg.ntp}x{cfg.ntp}x{cfg.nr} zones, the reference's P1 scheme "
+                               f"(WENO-ADP) + {'HLLD' if cfg.riemann == cs.RS_HLLD else 'HLL'} + 4-state HLL "
+                               "edge fields + arc-length CT, "
+                               + ("IMEX-SSP2(2,2,2) with implicit gravity 1/r^2, shell r in [1, 1.5], hydrostatic "
+                                  "+ 1e-2 density noise" if cfg.imex else
+                                  "SSP-RK3, surface between conducting walls, magnetised blast"),
+                   "zones": n_int, "precision": "fp64", "graph": True},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "reference_cpu_leg": ref_leg,
+        "fp64": {"cell_updates_per_s": cups, "ms_per_step": ms / args.steps,
+                 "max_rel_divb_start": divb0, "max_rel_divb_end": divb, mass_key: rel_mass},
+        "mixed": None,
+        "precision_gate": None,
+        "step_hbm": {"b_alg_per_cell_update": b_alg, "achieved_GBps": step_gbs, "frac": step_gbs / HBM_PEAK_GBS},
+        "kernels": kern,
+        "profiled_step_ms": step_ms_prof,
+    }
+    print(json.dumps(line))
+
+
+
 def probe(cs, name, prec, kernel, iters):
     """Hardware-counter run: `iters` launches of one kernel on the workload's
     state, then one plain device copy of 1 GiB (read) + 1 GiB (write), whose
@@ -658,13 +909,13 @@ def probe(cs, name, prec, kernel, iters):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="C2")
-    ap.add_argument("--profile-steps", type=int, default=20,
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="C5")
+    ap.add_argument("--profile-steps", type=int, default=1,
                     help="steps run launch by launch for the per-kernel durations")
     ap.add_argument("--probe-iters", type=int, default=5, help="launches of the --probe kernel")
-    ap.add_argument("--gate-steps", type=int, default=100)
+    ap.add_argument("--gate-steps", type=int, default=10)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--probe", default=None, metavar="KERNEL",
@@ -678,13 +929,25 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    import cubesphere_amd as cs
+    p1adp = args.workload in ("C2", "C5") and not args.probe
+    cpu = None
+    if p1adp and not args.no_cpu:
+        # before any HIP call: the host processes are forked from a process
+        # that has not touched the GPU
+        try:
+            cpu = cpu_baseline_p1adp(cs, args.workload, args.cpu_seconds)
+        except Exception as e:  # the baseline is reported, not required
+            log("cpu baseline failed:", e)
     if args.workload in ("P1", "P1SHELL"):
         # the P1 legs keep their patches in torch tensors: torch's HIP runtime has
         # to be initialised before the library's first HIP call
         import torch
         torch.zeros(1, device="cuda")
-    import cubesphere_amd as cs
     log("device:", cs.device_info())
+    if p1adp:
+        bench_p1adp(cs, args, cpu)
+        return
     if args.probe:
         probe(cs, args.workload, cs.FP64 if args.precision == "fp64" else cs.MIXED, args.probe,
               args.probe_iters)

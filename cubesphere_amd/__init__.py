@@ -21,6 +21,9 @@ EMF_LLF, EMF_HLL = 0, 1
 FP64, MIXED = 0, 1
 RBC_PAD, RBC_WALL = 0, 1
 RMESH_UNIFORM, RMESH_LOG = 0, 1
+# the launch groups of the P1 stage (CS_P1K_* of the header, in order)
+P1_KERNEL_NAMES = ["exchange", "prim", "face_sync", "flatten", "weno", "positivity", "replay", "face_points",
+                   "clsq", "flux", "edge", "sync", "update_faces", "update_zones", "imex", "cfl"]
 KERNEL_NAMES = ["halo", "prims", "flux_x", "flux_y", "flux_z", "emf_z", "emf_x", "emf_y",
                 "update_cells", "update_faces", "cleanup", "cfl", "step", "sync", "flux_xy", "emf"]
 (K_HALO, K_PRIMS, K_FLUX_X, K_FLUX_Y, K_FLUX_Z, K_EMF_Z, K_EMF_X, K_EMF_Y, K_UPDATE_CELLS,
@@ -125,6 +128,7 @@ def lib():
             "cs_time_steps": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
             "cs_time_kernel": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
             "cs_profile_steps": [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32)],
+            "cs_profile_steps_p1": [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32)],
             "cs_autotune": [vp, i32, ctypes.POINTER(i32)],
             "cs_geometry_sizes": [cfgp, vp],
             "cs_build_geometry": [cfgp, vp, vp],
@@ -364,6 +368,15 @@ class CubeSphere:
         calls = (ctypes.c_int32 * len(KERNEL_NAMES))()
         _check(lib().cs_profile_steps(self.h, nsteps, ms, calls), "cs_profile_steps")
         return {k: (ms[k] / calls[k], calls[k]) for k in range(len(KERNEL_NAMES)) if calls[k]}
+
+    def profile_steps_p1(self, nsteps):
+        """The same for a handle of recon RECON_P1ADP: {P1_KERNEL_NAMES entry:
+        (average ms per launch group, launch groups)}."""
+        n = len(P1_KERNEL_NAMES)
+        ms = (ctypes.c_float * n)()
+        calls = (ctypes.c_int32 * n)()
+        _check(lib().cs_profile_steps_p1(self.h, nsteps, ms, calls), "cs_profile_steps_p1")
+        return {P1_KERNEL_NAMES[k]: (ms[k] / calls[k], calls[k]) for k in range(n) if calls[k]}
 
     def autotune(self, iters=3):
         """Pick each stage launch's workgroup size by timing; returns {kernel name: size}."""

@@ -1320,17 +1320,31 @@ __global__ void k_sums_partial(CsDims d, Rng rg, Geo geo, State st, double* part
     __syncthreads();
   }
 }
-__global__ void k_sums_final(const double* part, int n, double* out) {
-  int q = threadIdx.x;
-  if (q >= 9) return;
+// one workgroup of 256 per variable (blockIdx.x): strided two-sum partials,
+// then a two-sum tree
+__global__ void __launch_bounds__(256) k_sums_final(const double* part, int n, double* out) {
+  __shared__ double ss[256][2];
+  const int q = blockIdx.x;
   double s = 0.0, e = 0.0;
-  for (int b = 0; b < n; ++b) {
+  for (int b = threadIdx.x; b < n; b += blockDim.x) {
     double t, te;
     two_sum(s, part[(b * 9 + q) * 2], t, te);
     s = t;
     e += te + part[(b * 9 + q) * 2 + 1];
   }
-  out[q] = s + e;
+  ss[threadIdx.x][0] = s;
+  ss[threadIdx.x][1] = e;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      double sa, ea;
+      two_sum(ss[threadIdx.x][0], ss[threadIdx.x + o][0], sa, ea);
+      ss[threadIdx.x][0] = sa;
+      ss[threadIdx.x][1] = ss[threadIdx.x][1] + ss[threadIdx.x + o][1] + ea;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[q] = ss[0][0] + ss[0][1];
 }
 
 // relative discrete divergence |sum A_f B_f| / sum |A_f B_f| per zone, max
@@ -1831,9 +1845,13 @@ int launch_stage_p1(cs_handle* h, int irk) {
     return rc;
   // with the average of the two copies of every panel-edge face, as each
   // exchange of the reference (330fetch_buffers_fill_in_ux.f:337-350)
+  p1s::mark_begin(h->p1, h->stream);
   if ((rc = launch_halo(h, b, true, true, true))) return rc;
+  p1s::mark_end(h->p1, h->stream, CS_P1K_EXCHANGE);
   if ((rc = p1s::front(h->p1, p1buf(h, b), h->F, h->S, h->E, h->flags + 2, h->stream))) return rc;
+  p1s::mark_begin(h->p1, h->stream);
   if ((rc = launch_sync(h, true, true))) return rc;
+  p1s::mark_end(h->p1, h->stream, CS_P1K_SYNC);
   return p1s::update(h->p1, p1buf(h, b), p1buf(h, bz), p1buf(h, bt), h->F, h->E, h->dt, wa, wb,
                      h->pr.imex ? irk : 0, h->stream);
 }
@@ -2259,8 +2277,14 @@ extern "C" int cs_conservation_sums(cs_handle* h, double* sums9) {
   const CsDims& d = h->d;
   Rng rc = make_rng(1, d.N, 1, d.N, 1, d.NR);
   unsigned nb = nblk(rc.total);
-  k_sums_partial<<<nb, kBlock, 0, h->stream>>>(d, rc, geo_of(h), state_of(h, h->cur), h->part);
-  k_sums_final<<<1, 64, 0, h->stream>>>(h->part, (int)nb, h->part + (int64_t)h->npart * 18 - 16);
+  if (h->p1) {  // on the P1 geometry the stage conserves
+    nb = (unsigned)p1s::diag_blocks(h->p1);
+    int r = p1s::sums_partial(h->p1, p1buf(h, h->cur), h->part, h->stream);
+    if (r) return r;
+  } else {
+    k_sums_partial<<<nb, kBlock, 0, h->stream>>>(d, rc, geo_of(h), state_of(h, h->cur), h->part);
+  }
+  k_sums_final<<<9, 256, 0, h->stream>>>(h->part, (int)nb, h->part + (int64_t)h->npart * 18 - 16);
   CS_HIP(hipGetLastError());
   CS_HIP(hipMemcpyAsync(sums9, h->part + (int64_t)h->npart * 18 - 16, sizeof(double) * 9,
                         hipMemcpyDeviceToHost, h->stream));
@@ -2274,7 +2298,13 @@ extern "C" int cs_max_divb(cs_handle* h, double* out) {
   const CsDims& d = h->d;
   Rng rc = make_rng(1, d.N, 1, d.N, 1, d.NR);
   unsigned nb = nblk(rc.total);
-  k_divb_partial<<<nb, kBlock, 0, h->stream>>>(d, rc, geo_of(h), state_of(h, h->cur), h->part);
+  if (h->p1) {  // on the P1 geometry the constrained transport keeps
+    nb = (unsigned)p1s::diag_blocks(h->p1);
+    int r = p1s::divb_partial(h->p1, p1buf(h, h->cur), h->part, h->stream);
+    if (r) return r;
+  } else {
+    k_divb_partial<<<nb, kBlock, 0, h->stream>>>(d, rc, geo_of(h), state_of(h, h->cur), h->part);
+  }
   CS_HIP(hipGetLastError());
   std::vector<double> p(nb);
   CS_HIP(hipMemcpyAsync(p.data(), h->part, sizeof(double) * nb, hipMemcpyDeviceToHost, h->stream));
@@ -2337,6 +2367,41 @@ extern "C" int cs_profile_steps(cs_handle* h, int32_t nsteps, float* ms_sum, int
   h->prof = nullptr;
   for (int q = 0; q < made; ++q) hipEventDestroy(p->ev[q]);
   delete p;
+  if (rc) return rc;
+  return check_flags(h);
+}
+
+extern "C" int cs_profile_steps_p1(cs_handle* h, int32_t nsteps, float* ms_sum, int32_t* calls) {
+  CHECK_H(h);
+  if (!h->p1) { set_error("cs_profile_steps_p1: the handle's recon is not CS_RECON_P1ADP"); return CS_E_INVALID; }
+  if (!ms_sum || !calls || nsteps < 1) { set_error("bad arguments"); return CS_E_INVALID; }
+  for (int q = 0; q < CS_P1K_COUNT; ++q) { ms_sum[q] = 0.f; calls[q] = 0; }
+  constexpr int cap = 256;  // launch groups per step (about 16 per stage)
+  std::vector<hipEvent_t> ev(2 * cap);
+  std::vector<int> kid(cap);
+  int made = 0, rc = CS_OK;
+  for (; made < 2 * cap; ++made)
+    if (hipEventCreate(&ev[made]) != hipSuccess) { rc = CS_E_HIP; set_error("hipEventCreate failed"); break; }
+  p1s::Marks m{ev.data(), kid.data(), 0, cap};
+  if (rc == CS_OK) rc = hipStreamSynchronize(h->stream) == hipSuccess ? CS_OK : CS_E_HIP;
+  p1s::set_marks(h->p1, &m);
+  for (int n = 0; n < nsteps && rc == CS_OK; ++n) {
+    m.n = 0;
+    rc = launch_step(h);
+    if (rc == CS_OK && hipStreamSynchronize(h->stream) != hipSuccess) { rc = CS_E_HIP; set_error("step failed"); }
+    for (int q = 0; q < m.n && rc == CS_OK; ++q) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, ev[2 * q], ev[2 * q + 1]) != hipSuccess) {
+        rc = CS_E_HIP;
+        set_error("hipEventElapsedTime failed");
+        break;
+      }
+      ms_sum[kid[q]] += ms;
+      calls[kid[q]] += 1;
+    }
+  }
+  p1s::set_marks(h->p1, nullptr);
+  for (int q = 0; q < made; ++q) hipEventDestroy(ev[q]);
   if (rc) return rc;
   return check_flags(h);
 }

@@ -42,5 +42,24 @@ int imex_source(Stage* S, const Buf& un, const Buf& u1, const double* dtp, hipSt
 // part (nblocks() entries); the caller folds them into dt.
 int cfl_partial(Stage* S, const Buf& cur, double* part, hipStream_t st);
 int nblocks_cfl();
+// Diagnostics on the P1 geometry, one partial per workgroup (diag_blocks()
+// of them): the largest relative discrete divergence |sum A B| / sum |A B|
+// of the owned active zones, and the volume integrals of U and the entropy
+// as two-sum pairs (part[(block * 9 + q) * 2 + {0, 1}]).
+int diag_blocks(Stage* S);
+int divb_partial(Stage* S, const Buf& cur, double* part, hipStream_t st);
+int sums_partial(Stage* S, const Buf& cur, double* part, hipStream_t st);
+
+// Per-launch timing while the handle profiles steps: when set, every launch
+// (or launch group) of the stage is bracketed by an event pair on its stream
+// and tagged with its CS_P1K_* id (include/cubesphere_hip.h).
+struct Marks {
+  hipEvent_t* ev;  // 2 * cap events
+  int* kid;        // cap ids
+  int n, cap;
+};
+void set_marks(Stage* S, Marks* m);
+void mark_begin(Stage* S, hipStream_t st);
+void mark_end(Stage* S, hipStream_t st, int kid);
 
 }  // namespace p1s

@@ -1438,13 +1438,106 @@ __global__ void __launch_bounds__(kCflThreads) k_cfl(Dev D, Buf6 cur, double cfl
 }
 
 // ---------------------------------------------------------------------------
+// diagnostics on the P1 geometry: per owned active zone the relative discrete
+// divergence |sum A_f B_f| / sum |A_f B_f| (maximum per workgroup), and the
+// volume integrals of U and the entropy (two-sum partials per workgroup)
+constexpr int kDiagThreads = 256;
+__global__ void __launch_bounds__(kDiagThreads) k_divb(Dev D, Buf6 cur, double* part) {
+  __shared__ double sm[kDiagThreads / 64];
+  const int64_t per = (int64_t)D.N * D.N * D.NR, t = tid();
+  double v = 0.0;
+  if (t < 6 * per) {
+    const int s = (int)(t / per);
+    const int64_t r = t - (int64_t)s * per;
+    const int i = 1 + (int)(r % D.N), j = 1 + (int)((r / D.N) % D.N), k = 1 + (int)(r / ((int64_t)D.N * D.N));
+    if (D.active(i, j)) {
+      const int64_t so = (int64_t)s * D.NFC;
+      const double p[6] = {csg1::area_x(D.G, s, i, j, k) * cur.Bx[so + D.fci(i, j, k)],
+                           -csg1::area_x(D.G, s, i - 1, j, k) * cur.Bx[so + D.fci(i - 1, j, k)],
+                           csg1::area_y(D.G, s, i, j, k) * cur.By[so + D.fci(i, j, k)],
+                           -csg1::area_y(D.G, s, i, j - 1, k) * cur.By[so + D.fci(i, j - 1, k)],
+                           csg1::area_z(D.G, s, i, j, k) * cur.Bz[so + D.fci(i, j, k)],
+                           -csg1::area_z(D.G, s, i, j, k - 1) * cur.Bz[so + D.fci(i, j, k - 1)]};
+      double sum = 0.0, sa = 0.0;
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        sum += p[q];
+        sa += fabs(p[q]);
+      }
+      v = sa > 0.0 ? fabs(sum) / sa : 0.0;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0;
+    for (int q = 0; q < kDiagThreads / 64; ++q) x = fmax(x, sm[q]);
+    part[blockIdx.x] = x;
+  }
+}
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+__global__ void __launch_bounds__(kDiagThreads) k_sums(Dev D, Buf6 cur, double* part) {
+  __shared__ double ss[kDiagThreads][2];
+  const int64_t per = (int64_t)D.N * D.N * D.NR, t = tid();
+  int s = 0;
+  int64_t z = 0;
+  bool on = false;
+  if (t < 6 * per) {
+    s = (int)(t / per);
+    const int64_t r = t - (int64_t)s * per;
+    const int i = 1 + (int)(r % D.N), j = 1 + (int)((r / D.N) % D.N), k = 1 + (int)(r / ((int64_t)D.N * D.N));
+    z = D.zi(i, j, k);
+    on = D.active(i, j);
+  }
+  for (int q = 0; q < 9; ++q) {
+    const double v = on ? D.vol(s, z) * (q < 8 ? cur.u(D, s, q, z) : cur.e(D, s, z)) : 0.0;
+    ss[threadIdx.x][0] = v;
+    ss[threadIdx.x][1] = 0.0;
+    __syncthreads();
+    for (int o = kDiagThreads / 2; o > 0; o >>= 1) {
+      if ((int)threadIdx.x < o) {
+        double sa, ea;
+        two_sum(ss[threadIdx.x][0], ss[threadIdx.x + o][0], sa, ea);
+        ss[threadIdx.x][0] = sa;
+        ss[threadIdx.x][1] = ss[threadIdx.x][1] + ss[threadIdx.x + o][1] + ea;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      part[(blockIdx.x * 9 + q) * 2 + 0] = ss[0][0];
+      part[(blockIdx.x * 9 + q) * 2 + 1] = ss[0][1];
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // host side
 struct Stage {
   Dev D;
   double cfl;
   std::vector<void*> mem;
   Box all, wr, own, fx, fy, fz, ffx, ffy, ffz, ex, ey, ez;
+  Marks* marks = nullptr;
 };
+
+void set_marks(Stage* S, Marks* m) { S->marks = m; }
+void mark_begin(Stage* S, hipStream_t st) {
+  Marks* m = S->marks;
+  if (m && m->n < m->cap) (void)hipEventRecord(m->ev[2 * m->n], st);
+}
+void mark_end(Stage* S, hipStream_t st, int kid) {
+  Marks* m = S->marks;
+  if (m && m->n < m->cap) {
+    (void)hipEventRecord(m->ev[2 * m->n + 1], st);
+    m->kid[m->n++] = kid;
+  }
+}
 
 static inline unsigned nb(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
 
@@ -1612,15 +1705,26 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
   const Dev& D = S->D;
   const Buf6 in = b6(in_);
   const int64_t nall = S->all.count(), nwr = S->wr.count();
+  mark_begin(S, st);
   k_prim<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all, in);
+  mark_end(S, st, CS_P1K_PRIM);
+  mark_begin(S, st);
   k_face_zero<<<dim3(nb(D.NFC, 256), 18), 256, 0, st>>>(D, in);
   k_tricorner<<<dim3(nb(4 * (int64_t)D.g * D.nzr, 256), 6), 256, 0, st>>>(D, in);
+  mark_end(S, st, CS_P1K_FACE_SYNC);
+  mark_begin(S, st);
   k_flat_a<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all);
   k_flat_b<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all);
   k_flat_c<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all);
   k_ctr_reset<<<1, 64, 0, st>>>(D);
+  mark_end(S, st, CS_P1K_FLATTEN);
+  mark_begin(S, st);
   k_weno<<<dim3(nb(3 * nwr, 128), 6), 128, 0, st>>>(D, S->wr);
+  mark_end(S, st, CS_P1K_WENO);
+  mark_begin(S, st);
   k_pos<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr);
+  mark_end(S, st, CS_P1K_POSITIVITY);
+  mark_begin(S, st);
   for (int sw = 0; sw < kSweeps; ++sw) {
     k_cands<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr);
     k_weno_list<<<kListBlocks, 128, 0, st>>>(D);
@@ -1628,13 +1732,22 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
     k_commit<<<kListBlocks, 256, 0, st>>>(D, sw);
   }
   k_clear_ent<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr, in, err);
+  mark_end(S, st, CS_P1K_REPLAY);
   const int64_t nfp = std::max(S->fx.count(), std::max(S->fy.count(), S->fz.count()));
+  mark_begin(S, st);
   k_face_pts<<<dim3(nb(nfp, 256), 18), 256, 0, st>>>(D, S->fx, S->fy, S->fz, in);
+  mark_end(S, st, CS_P1K_FACE_POINTS);
+  mark_begin(S, st);
   k_clsq<<<dim3(nb(nall, 64), 6), 64, 0, st>>>(D, S->all, in);
+  mark_end(S, st, CS_P1K_CLSQ);
   const int64_t nff = std::max(S->ffx.count(), std::max(S->ffy.count(), S->ffz.count()));
+  mark_begin(S, st);
   k_flux<<<dim3(nb(nff, 128), 18), 128, 0, st>>>(D, S->ffx, S->ffy, S->ffz, F[0], F[1], F[2], Sp[0], Sp[1], Sp[2]);
+  mark_end(S, st, CS_P1K_FLUX);
   const int64_t ned = std::max(S->ex.count(), std::max(S->ey.count(), S->ez.count()));
+  mark_begin(S, st);
   k_edge<<<dim3(nb(ned, 256), 18), 256, 0, st>>>(D, S->ex, S->ey, S->ez, Sp[0], Sp[1], Sp[2], E[0], E[1], E[2]);
+  mark_end(S, st, CS_P1K_EDGE);
   P1S_HIP(hipGetLastError());
   return CS_OK;
 }
@@ -1643,28 +1756,50 @@ int update(Stage* S, const Buf& in, const Buf& u0, const Buf& out, double* const
            const double* dtp, double wa, double wb, int mode, hipStream_t st) {
   const Dev& D = S->D;
   const int64_t nfc = (int64_t)(D.N + 1) * (D.N + 1) * (D.NR + 1);
+  mark_begin(S, st);
   k_upd_face<<<dim3(nb(nfc, 256), 18), 256, 0, st>>>(D, b6(in), b6(u0), b6(out), E[0], E[1], E[2], dtp, wa, wb);
+  mark_end(S, st, CS_P1K_UPDATE_FACES);
+  mark_begin(S, st);
   k_upd_zone<<<dim3(nb(S->own.count(), 256), 6), 256, 0, st>>>(D, S->own, b6(in), b6(u0), b6(out), F[0], F[1], F[2],
                                                                 dtp, wa, wb, mode);
+  mark_end(S, st, CS_P1K_UPDATE_ZONES);
   P1S_HIP(hipGetLastError());
   return CS_OK;
 }
 
 int imex_source(Stage* S, const Buf& un, const Buf& u1, const double* dtp, hipStream_t st) {
   const Dev& D = S->D;
+  mark_begin(S, st);
   k_imex<<<dim3(nb(S->all.count(), 256), 6), 256, 0, st>>>(D, S->all, b6(un), b6(u1), dtp);
   const size_t fb = sizeof(double) * (size_t)D.NFC * 6;
   P1S_HIP(hipMemcpyAsync(u1.Bx, un.Bx, fb, hipMemcpyDeviceToDevice, st));
   P1S_HIP(hipMemcpyAsync(u1.By, un.By, fb, hipMemcpyDeviceToDevice, st));
   P1S_HIP(hipMemcpyAsync(u1.Bz, un.Bz, fb, hipMemcpyDeviceToDevice, st));
+  mark_end(S, st, CS_P1K_IMEX);
   P1S_HIP(hipGetLastError());
   return CS_OK;
 }
 
 int nblocks_cfl() { return kCflBlocks; }
 
+int diag_blocks(Stage* S) { return (int)nb(6 * (int64_t)S->D.N * S->D.N * S->D.NR, kDiagThreads); }
+
+int divb_partial(Stage* S, const Buf& cur, double* part, hipStream_t st) {
+  k_divb<<<diag_blocks(S), kDiagThreads, 0, st>>>(S->D, b6(cur), part);
+  P1S_HIP(hipGetLastError());
+  return CS_OK;
+}
+
+int sums_partial(Stage* S, const Buf& cur, double* part, hipStream_t st) {
+  k_sums<<<diag_blocks(S), kDiagThreads, 0, st>>>(S->D, b6(cur), part);
+  P1S_HIP(hipGetLastError());
+  return CS_OK;
+}
+
 int cfl_partial(Stage* S, const Buf& cur, double* part, hipStream_t st) {
+  mark_begin(S, st);
   k_cfl<<<kCflBlocks, kCflThreads, 0, st>>>(S->D, b6(cur), S->cfl, part);
+  mark_end(S, st, CS_P1K_CFL);
   P1S_HIP(hipGetLastError());
   return CS_OK;
 }

@@ -197,7 +197,10 @@ int cs_download_fluxes(cs_handle* h, int32_t dir, double* flux, double* speeds);
 int cs_download_emf(cs_handle* h, double* Efx, double* Efy, double* Efz);
 
 /* Diagnostics on the device: compensated (two-sum) volume integrals of the
- * 8 conserved variables + entropy, and max |div B| * V^(1/3) / max|B|. */
+ * 8 conserved variables + entropy over the owned zones, and the largest
+ * relative discrete divergence |sum_f A_f B_f| / sum_f |A_f B_f| of an owned
+ * zone.  With recon CS_RECON_P1ADP both use the P1 geometry (the volumes and
+ * face areas the stage conserves on). */
 int cs_conservation_sums(cs_handle* h, double* sums9);
 int cs_max_divb(cs_handle* h, double* out);
 
@@ -233,6 +236,27 @@ int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float* ms_avg);
  * kernel id, as the kernels run inside the step (caches as the step leaves
  * them).  Advances the state by nsteps. */
 int cs_profile_steps(cs_handle* h, int32_t nsteps, float* ms_sum, int32_t* calls);
+/* cs_profile_steps_p1: the same for a handle of recon CS_RECON_P1ADP (no
+ * counterpart in the reference: measurement only), per launch (or launch
+ * group) of its stage, arrays of CS_P1K_COUNT */
+#define CS_P1K_EXCHANGE 0       /* ghost exchange + radial ghosts of the stage input  */
+#define CS_P1K_PRIM 1           /* CONS_TO_PRIM of every zone, magnetosonic speed      */
+#define CS_P1K_FACE_SYNC 2      /* face B cleared between inactive zones, tri-corners  */
+#define CS_P1K_FLATTEN 3        /* the three flattener passes                          */
+#define CS_P1K_WENO 4           /* multi-D WENO of the 8 primitives and the entropy    */
+#define CS_P1K_POSITIVITY 5     /* ENFORCE_PRESSURE_POSITIVITY_PRIM, first pass         */
+#define CS_P1K_REPLAY 6         /* its loop-order replay (bounded sweeps)               */
+#define CS_P1K_FACE_POINTS 7    /* normal B at the B-field points of every face        */
+#define CS_P1K_CLSQ 8           /* ADP constrained least squares, zone B modes         */
+#define CS_P1K_FLUX 9           /* face fluxes (HLL / HLLD) x area, speeds              */
+#define CS_P1K_EDGE 10          /* edge fields (4-state and tri-corner 3-state HLL)    */
+#define CS_P1K_SYNC 11          /* panel-edge faces and edges made one across sectors  */
+#define CS_P1K_UPDATE_FACES 12  /* constrained transport of the owned faces            */
+#define CS_P1K_UPDATE_ZONES 13  /* zone update, RK weights, IMEX source, cleanup       */
+#define CS_P1K_IMEX 14          /* IMEX first (source-only) stage                      */
+#define CS_P1K_CFL 15           /* EVALUATE_TIMESTEP partial minima                    */
+#define CS_P1K_COUNT 16
+int cs_profile_steps_p1(cs_handle* h, int32_t nsteps, float* ms_sum, int32_t* calls);
 int cs_kernel_units(cs_handle* h, int32_t kernel, int64_t* units);
 /* cs_autotune: time every launch of a stage at 64/128/256-thread workgroups
  * (iters launches each, through cs_time_kernel) and keep the fastest; later

@@ -63,9 +63,18 @@ def olib():
     return _o
 
 
-def rlib():
-    """Reference routines (flang build of /root/reference sources) or None."""
-    global _r
+_r_omp = None
+
+
+def rlib(openmp=False):
+    """Reference routines (flang build of /root/reference sources) or None;
+    openmp=True: the same sources built with their OpenMP directives on."""
+    global _r, _r_omp
+    if openmp:
+        path = os.path.join(os.path.dirname(REFLIB), "libcsref_omp.so")
+        if _r_omp is None and os.path.exists(path):
+            _r_omp = ctypes.CDLL(path)
+        return _r_omp
     if _r is None and os.path.exists(REFLIB):
         _r = ctypes.CDLL(REFLIB)
     return _r
@@ -361,9 +370,10 @@ def p1_oracle(prob, A):
     return A
 
 
-def p1_reference(prob, A):
-    """The reference's LIMIT_ALL_SLOPES_P1 (flang build) in place on A."""
-    R = rlib()
+def p1_reference(prob, A, lib=None):
+    """The reference's LIMIT_ALL_SLOPES_P1 (flang build, or the library given)
+    in place on A."""
+    R = lib if lib is not None else rlib()
     if R is None:
         raise RuntimeError("reference build not present")
     R.ref_limit_all_slopes_p1.argtypes = [ctypes.c_void_p]
