@@ -1032,6 +1032,10 @@ class P1Shell:
         import torch
         if len(patches) != 6:
             raise CsError("P1Shell: six patches (one per cube sector) required")
+        if pad and cfg.radial_bc != RBC_PAD:
+            # the outflow pads would overwrite U and face B while the entropy
+            # kept the wall exchange's mirrored ghosts
+            raise CsError("P1Shell: pad=True needs cfg.radial_bc == RBC_PAD (outflow radial pads)")
         self.cfg, self.P, self.sc, self.src, self.itri = cfg, patches, dict(scalars), sources, itri
         self.N, self.NR, self.g = cfg.ntp, cfg.nr, cfg.nghost
         self.solver = CubeSphere(cfg)

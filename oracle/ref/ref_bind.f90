@@ -18,11 +18,12 @@ module cs_ref_bind
     type(c_ptr) :: Bxf_q, Byf_q, Bzf_q, V_min_local, V_max_local
   end type
 
-  ! mirror of or_p1_edge / cs_p1_edge
+  ! mirror of or_p1_pad / cs_p1_pad
   type, bind(C) :: ref_p1_pad
     integer(c_int32_t) :: ntp, nr, nghost, pad
     type(c_ptr) :: mask, bcarr_r, xf_area, yf_area, zf_area, v_volume, U, Bxf, Byf, Bzf
   end type
+  ! mirror of or_p1_edge / cs_p1_edge
   type, bind(C) :: ref_p1_edge
     integer(c_int32_t) :: ntp, nr, nghost, pad
     integer(c_int32_t) :: itri(4)
@@ -391,7 +392,6 @@ contains
          u, ent, bxf, byf, bzf, v, b1, b2, b3, geos, gsnd, mmw, divv, msn, flat)
   end subroutine
 
-  ! MAKE_EDGE_EFIELDS_P1 (460make_edge_fields_p1.f:5) on the fields of a
   ! PAD_BOUNDARY_R (826pad_boundary_r.f:1) on the fields of a ref_p1_pad; U
   ! carries mode 1 only (n_cc_modes 1), the arguments the routine does not read
   ! (vertices, centroids, entropy) are zero-filled arrays of their sizes
@@ -416,6 +416,7 @@ contains
     deallocate(vert, cen, ent)
   end subroutine
 
+  ! MAKE_EDGE_EFIELDS_P1 (460make_edge_fields_p1.f:5) on the fields of a
   ! ref_p1_edge; the arguments the routine declares but does not read at second
   ! order are zero-filled arrays of their declared sizes
   subroutine ref_make_edge_efields_p1(p) bind(C, name="ref_make_edge_efields_p1")

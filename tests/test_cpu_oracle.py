@@ -501,3 +501,99 @@ def test_rotating_frame_coriolis_sign():
                          orc.P(np.array([rho, rho, 0.0, 0.0])), orc.P(Lv))
     assert np.allclose(Lv[1:4], [0.0, -2.0 * rho * w, 0.0], rtol=1e-15, atol=0.0)
     assert Lv[4] == 0.0
+
+
+def _time_order(step, st0, T, ns=(4, 8, 16), nref=64):
+    """error ratios e(T/n) / e(T/2n) of n fixed steps against nref steps (the
+    spatial discretisation is the same, so only the time error remains)"""
+    def run(n):
+        st = st0.copy()
+        for _ in range(n):
+            step(st, T / n)
+        return st.interior()[0]
+    ref = run(nref)
+    errs = [np.abs(run(n) - ref).sum() / np.abs(ref).sum() for n in ns]
+    return [errs[i] / errs[i + 1] for i in range(len(errs) - 1)]
+
+
+def test_ssp_rk3_is_third_order_in_time():
+    """SSP-RK3 (Shu & Osher 1988) on a smooth problem (Gaussian bump with
+    solid-body rotation, WENO5 + HLL): halving the step divides the error by
+    about 8; the same test on Heun RK2 gives about 4, so a mistake in the
+    SSP-RK3 weights that drops an order fails it."""
+    ratios = {}
+    for order in (3, 2):
+        cfg = cs.make_config(12, 1, 3, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_LLF, cs.FP64, order)
+        st0 = cs.init_problem(cfg, 1, seed=2)
+        G = orc.Grid(cfg)
+        T = 4 * G.cfl(st0, 0.0)
+        ratios[order] = _time_order(lambda st, dt: G.advance(st, 1, dt), st0, T)
+    assert all(r > 6.5 for r in ratios[3]), ratios
+    assert all(3.0 < r < 6.0 for r in ratios[2]), ratios
+
+
+def _p1_shell_smooth(N=8, NR=3, g=4):
+    """the six P1 patches of a shell holding the Gaussian bump with
+    solid-body rotation (a smooth state that leaves the limiters idle)"""
+    cfg0, probs, As = orc.p1_shell_problem(N, NR, g, seed=3, noise=0.0)
+    cfg = cs.make_config(N, NR, g, cs.RECON_P1ADP, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
+                         cs.RMESH_UNIFORM, 1.0, 1.6)
+    st = cs.init_problem(cfg, 1, seed=2)
+    nz = (N + 2 * g) ** 2 * (NR + 2 * g)
+    nfc = (N + 2 * g + 1) ** 2 * (NR + 2 * g + 1)
+    for s, A in enumerate(As):
+        A["U"][:] = 0.0
+        A["U"][:8 * nz] = st.U[s * 8 * nz:(s + 1) * 8 * nz]
+        A["ent"][:] = 0.0
+        A["ent"][:nz] = st.ent[s * nz:(s + 1) * nz]
+        for k, b in zip(("Bxf", "Byf", "Bzf"), (st.Bx, st.By, st.Bz)):
+            A[k][:] = b[s * nfc:(s + 1) * nfc]
+    return cfg0, probs, As, nz
+
+
+class _Patches:
+    """the six patches as a state for _time_order: interior() gives the owned
+    zones' U"""
+
+    def __init__(self, As, own, nz):
+        self.As, self.own, self.nz = As, own, nz
+
+    def copy(self):
+        return _Patches([orc.p1_copy(A) for A in self.As], self.own, self.nz)
+
+    def interior(self):
+        return np.concatenate([A["U"][:8 * self.nz].reshape(8, self.nz)[:, self.own] for A in self.As], 1), None
+
+
+def test_p1_shell_heun_is_second_order_in_time():
+    """The driver's Heun RK2 over six P1 patches (DRV:2655-2790: U0 saved,
+    the predictor with dt, then U = (U + U0) / 2 and the corrector with dt / 2),
+    every routine the oracle's: halving the step divides the error by about
+    4 on a smooth state; a forward-Euler driver (the predictor alone, a
+    mistake that drops an order) gives about 2 and fails."""
+    N, NR, g = 8, 3, 4
+    cfg0, probs, As, nz = _p1_shell_smooth(N, NR, g)
+    grid = orc.Grid(cfg0)
+    own = np.zeros((NR + 2 * g, N + 2 * g, N + 2 * g), bool)
+    own[g:g + NR, g:g + N, g:g + N] = True
+    own = own.ravel()
+    st0 = _Patches(As, own, nz)
+    dt0 = min(orc.p1_timestep_oracle(p, orc.p1_copy(A), 0.4, 1.0) for p, A in zip(probs, As))
+
+    def heun(st, dt):
+        orc.p1_shell_step_oracle(grid, probs, st.As, dt, pad=False)
+
+    def euler(st, dt):
+        orc.p1_shell_exchange_oracle(grid, st.As)
+        for prob, A in zip(probs, st.As):
+            orc.p1_oracle(prob, A)
+            orc.p1f_oracle(prob, A)
+            orc.p1e_oracle(prob, A)
+            orc.p1s_oracle(dict(prob, newtonsc=1.0, mstar=0.0, omega=(0.0, 0.0, 0.0)), A)
+            orc.p1u_oracle(dict(prob, dtmove=dt), A)
+            orc.p1c_oracle(prob, A)
+
+    r2 = _time_order(heun, st0, dt0, ns=(2, 4, 8), nref=32)
+    r1 = _time_order(euler, st0, dt0, ns=(2, 4, 8), nref=32)
+    assert all(3.3 < r < 6.0 for r in r2), r2
+    assert all(r < 2.8 for r in r1), r1

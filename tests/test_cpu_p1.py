@@ -224,6 +224,130 @@ def test_facial_fluxes_fill_the_reference_ranges():
     assert np.all(np.isfinite(A["dU_dt"]))
 
 
+def _mhd_flux_vectors(V, gam):
+    """Cartesian flux vectors F_j (j = x, y, z; shape (..., 3, 8)) of the
+    ideal-MHD conserved variables in Gaussian units of primitive states
+    V[..., 8] (rho, v, p, B), written from the equations: mass rho v_j;
+    momentum rho v_i v_j + (p + B^2/8pi) d_ij - B_i B_j / 4pi; energy
+    (E + p + B^2/8pi) v_j - B_j (v . B) / 4pi; induction v_j B_i - B_j v_i."""
+    rho, v, p, B = V[..., 0], V[..., 1:4], V[..., 4], V[..., 5:8]
+    b2 = (B * B).sum(-1)
+    vb = (v * B).sum(-1)
+    E = p / (gam - 1) + 0.5 * rho * (v * v).sum(-1) + b2 / (8 * np.pi)
+    F = np.zeros(V.shape[:-1] + (3, 8))
+    for j in range(3):
+        F[..., j, 0] = rho * v[..., j]
+        for i in range(3):
+            F[..., j, 1 + i] = rho * v[..., i] * v[..., j] - B[..., i] * B[..., j] / (4 * np.pi)
+            F[..., j, 5 + i] = v[..., j] * B[..., i] - B[..., j] * v[..., i]
+        F[..., j, 1 + j] += p + b2 / (8 * np.pi)
+        F[..., j, 4] = (E + p + b2 / (8 * np.pi)) * v[..., j] - B[..., j] * vb / (4 * np.pi)
+    return F
+
+
+def _face_geo(A, N, NR, g):
+    """per family: the first quadrature point, the frame normal (z-faces: that
+    of the first point) and the area, on the face-index grids (k, j, i)"""
+    nzc, nzr = N + 2 * g, NR + 2 * g
+    nf, nfr = nzc + 1, nzr + 1
+    x = (A["xf_pts"].reshape(5, 3, nzr, nzc, nf)[0], A["xf_normal"].reshape(3, nzr, nzc, nf),
+         A["xf_area"].reshape(nzr, nzc, nf))
+    y = (A["yf_pts"].reshape(5, 3, nzr, nf, nzc)[0], A["yf_normal"].reshape(3, nzr, nf, nzc),
+         A["yf_area"].reshape(nzr, nf, nzc))
+    z = (A["zf_pts"].reshape(5, 3, nfr, nzc, nzc)[0], A["zf_normal"].reshape(5, 3, nfr, nzc, nzc)[0],
+         A["zf_area"].reshape(nfr, nzc, nzc))
+    return x, y, z
+
+
+def _expected_dudt(A, N, NR, g, flux_at):
+    """-(1/V) sum over the six faces of each owned zone of +-F(point) . n A,
+    the face's resolved flux when both of its states are the same field:
+    flux_at(point, normal) -> 8 components"""
+    (xp, xn, xa), (yp, yn, ya), (zp, zn, za) = _face_geo(A, N, NR, g)
+    nzc = N + 2 * g
+    vol = A["v_volume"].reshape(NR + 2 * g, nzc, nzc)
+    out = {}
+    for k in range(1, NR + 1):
+        for j in range(1, N + 1):
+            for i in range(1, N + 1):
+                zk, zj, zi = k - 1 + g, j - 1 + g, i - 1 + g
+                tot = (flux_at(xp[:, zk, zj, i + g], xn[:, zk, zj, i + g]) * xa[zk, zj, i + g]
+                       - flux_at(xp[:, zk, zj, i - 1 + g], xn[:, zk, zj, i - 1 + g]) * xa[zk, zj, i - 1 + g]
+                       + flux_at(yp[:, zk, j + g, zi], yn[:, zk, j + g, zi]) * ya[zk, j + g, zi]
+                       - flux_at(yp[:, zk, j - 1 + g, zi], yn[:, zk, j - 1 + g, zi]) * ya[zk, j - 1 + g, zi]
+                       + flux_at(zp[:, k + g, zj, zi], zn[:, k + g, zj, zi]) * za[k + g, zj, zi]
+                       - flux_at(zp[:, k - 1 + g, zj, zi], zn[:, k - 1 + g, zj, zi]) * za[k - 1 + g, zj, zi])
+                out[(zk * nzc + zj) * nzc + zi] = (-tot / vol[zk, zj, zi], xa[zk, zj, i + g] / vol[zk, zj, zi])
+    return out
+
+
+def test_facial_fluxes_of_a_uniform_state_are_its_physical_flux():
+    """A uniform state on every face: each face's share to each of its zones
+    is the physical flux F(U) . n A in the global frame (the frame's rotation
+    in and back, the normal, the area and the zone each share lands on), so an
+    owned zone's dU_dt is -F(U) . (sum of its +-n A) / V.  The faces' n A do
+    not close (the z-faces' frame normal is that of their first point), so a
+    wrong frame, normal, area or destination leaves a residual."""
+    N, NR, g = 8, 3, 4
+    prob, A = orc.p1_flux_problem(N, NR, g, sector=2, seed=5, shock=False, noise=0.0)
+    nz = orc.p1_dims(N, NR, g)["zone"]
+    gam = prob["gamma"]
+    V0 = np.array([1.3, 0.2, -0.1, 0.3, 0.7, 0.4, -0.25, 0.15])
+    V = A["V"].reshape(4, 8, nz)
+    V[:] = 0.0
+    V[0] = V0[:, None]
+    e = A["ent"].reshape(4, nz)
+    e[:] = 0.0
+    e[0] = 0.7 / 1.3 ** (gam - 1)
+    orc.p1f_oracle(prob, A)
+    dU = A["dU_dt"].reshape(8, nz)
+    F = _mhd_flux_vectors(V0, gam)
+    want = _expected_dudt(A, N, NR, g, lambda p, n: F.T @ n)
+    closure = max(np.abs(w).max() for w, _ in want.values())
+    worst = max(np.abs(dU[:, z] - w).max() / (np.abs(F).max() * sc) for z, (w, sc) in want.items())
+    assert closure > 1e-6          # the check is not vacuous
+    assert worst < 1e-13, worst
+
+
+def test_facial_fluxes_evaluate_a_linear_field_at_the_face_point():
+    """A field linear in position (modes = gradient x scale length): both
+    states of every face are the field at the face's first quadrature point
+    (450make_facial_fluxes.f:677-698), so each face passes the physical flux
+    of that point's state and an owned zone's dU_dt is
+    -(1/V) sum +-F(V(p_f)) . n_f A_f.  Evaluating at the face centroid instead
+    differs by ~1e-4 of the flux."""
+    N, NR, g = 8, 3, 4
+    prob, A = orc.p1_flux_problem(N, NR, g, sector=4, seed=5, shock=False, noise=0.0)
+    nz = orc.p1_dims(N, NR, g)["zone"]
+    gam = prob["gamma"]
+    V0 = np.array([1.3, 0.2, -0.1, 0.3, 0.7, 0.4, -0.25, 0.15])
+    G = np.array([[0.05, -0.03, 0.02], [0.01, 0.02, -0.01], [-0.02, 0.01, 0.03], [0.02, -0.01, 0.01],
+                  [0.04, 0.02, -0.03], [0.01, 0.03, 0.02], [-0.02, 0.01, 0.01], [0.03, -0.02, 0.01]])
+    cen = A["v_centroid"].reshape(3, nz)
+    sc = np.maximum(A["scale"], prob["smallnum"])
+    x0 = cen[:, nz // 2].copy()
+    V = A["V"].reshape(4, 8, nz)
+    V[:] = 0.0
+    V[0] = V0[:, None] + G @ (cen - x0[:, None])
+    for m in range(3):
+        V[m + 1] = G[:, m][:, None] * sc[None]
+    e = A["ent"].reshape(4, nz)
+    e[:] = 0.0
+    e[0] = 0.7 / 1.3 ** (gam - 1)
+    orc.p1f_oracle(prob, A)
+    dU = A["dU_dt"].reshape(8, nz)
+    flux_at = lambda p, n: _mhd_flux_vectors(V0 + G @ (p - x0), gam).T @ n
+    want = _expected_dudt(A, N, NR, g, flux_at)
+    worst = max(np.abs(dU[:, z] - w).max() / sc_ for z, (w, sc_) in want.items())
+    assert worst < 1e-13, worst
+    # the check separates the face point from the face centroid
+    (xp, xn, xa), _, _ = _face_geo(A, N, NR, g)
+    xc = A["xf_centroid"].reshape(3, NR + 2 * g, N + 2 * g, N + 2 * g + 1)
+    gap = np.abs(flux_at(xp[:, g, g + 2, g + 3], xn[:, g, g + 2, g + 3]) -
+                 flux_at(xc[:, g, g + 2, g + 3], xn[:, g, g + 2, g + 3])).max()
+    assert gap > 1e-6
+
+
 # ---------------------------------------------------------------------------
 # UPDATE_IMEX_RK_STEP_P1 (484update_imex_rk_step_p1.f:7).  The reference's file
 # does not preprocess (an #endif without #if at :920), so the restatement is

@@ -16,6 +16,9 @@ pytestmark = pytest.mark.gpu
 
 def _cases():
     return {
+        # configuration C1's scheme on its own grid: 6 x 32^2 surface, WENO5 + HLL + SSP-RK3, Gaussian bump
+        # with solid-body rotation and seeded density noise
+        "C1_surf32_weno5_hll_rk3": cs.workload("C1")[:2],
         "surf16_hlld_blast": (cs.make_config(16, 1, 3, cs.RECON_WENO5, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 3), 2),
         "surf12_hll_rk2_g4": (cs.make_config(12, 1, 4, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2), 1),
         "shell10x4_hlld_grav": (cs.make_config(10, 4, 3, cs.RECON_WENO5, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 3,
