@@ -28,7 +28,10 @@ __device__ __forceinline__ void inv3(const double (&m)[3][3], double (&r)[3][3])
   r[2][2] = (-(a12 * a21) + a11 * a22) * od;
 }
 
-// step 8.d: packed lower-triangular 12 x 12 Cholesky and solves
+// step 8.d: packed lower-triangular 12 x 12 Cholesky and solves.  The
+// factor keeps the reciprocal of each diagonal entry in the diagonal slot, so
+// the solves multiply instead of dividing (one reciprocal square root per
+// column in the factorisation, none in the solves).
 #define LIX(r, c) ((r) * ((r) + 1) / 2 + (c))
 
 __device__ __forceinline__ void chol12(double* L) {
@@ -37,8 +40,8 @@ __device__ __forceinline__ void chol12(double* L) {
     double s = L[LIX(j, j)];
 #pragma unroll
     for (int k = 0; k < j; ++k) s -= L[LIX(j, k)] * L[LIX(j, k)];
-    const double d = sqrt(s), od = 1.0 / d;
-    L[LIX(j, j)] = d;
+    const double od = 1.0 / sqrt(s);
+    L[LIX(j, j)] = od;
 #pragma unroll
     for (int i = j + 1; i < 12; ++i) {
       double t = L[LIX(i, j)];
@@ -55,7 +58,7 @@ __device__ __forceinline__ void fwd12(const double* L, double* x) {
     double s = x[i];
 #pragma unroll
     for (int k = 0; k < i; ++k) s -= L[LIX(i, k)] * x[k];
-    x[i] = s / L[LIX(i, i)];
+    x[i] = s * L[LIX(i, i)];
   }
 }
 // x <- L^-T x
@@ -65,25 +68,13 @@ __device__ __forceinline__ void bwd12(const double* L, double* x) {
     double s = x[i];
 #pragma unroll
     for (int k = i + 1; k < 12; ++k) s -= L[LIX(k, i)] * x[k];
-    x[i] = s / L[LIX(i, i)];
+    x[i] = s * L[LIX(i, i)];
   }
 }
 // x <- (L L^T)^-1 x
 __device__ __forceinline__ void cholsolve12(const double* L, double* x) {
-#pragma unroll
-  for (int i = 0; i < 12; ++i) {
-    double s = x[i];
-#pragma unroll
-    for (int k = 0; k < i; ++k) s -= L[LIX(i, k)] * x[k];
-    x[i] = s / L[LIX(i, i)];
-  }
-#pragma unroll
-  for (int i = 11; i >= 0; --i) {
-    double s = x[i];
-#pragma unroll
-    for (int k = i + 1; k < 12; ++k) s -= L[LIX(k, i)] * x[k];
-    x[i] = s / L[LIX(i, i)];
-  }
+  fwd12(L, x);
+  bwd12(L, x);
 }
 
 // the constraint or LSQ row of a face: n (x) (1, r)

@@ -62,10 +62,11 @@ struct Dev {
   int N, NR, g, nz, nzr, nf, nfr;
   int64_t NZ, NFC, NZ2;
   csg1::P1Geo G;
-  const double* zg;        // [s][5][zone]: centroid (3), volume, scale length
+  const double* zg;        // [s][zone / 64][5][64]: centroid (3), volume, scale length
   const int32_t* mask;     // [nz * nz] active zone columns (the same in every sector)
   const int32_t* stencil;  // [24][nz * nz] (STENCIL_INDEXING_TRICORNERS)
-  double* V;               // [m][s][9][zone]: mode m of the 8 primitives and the entropy
+  double* V;               // [s][zone / 64][4 x 9][64]: mode m of the 8 primitives and the entropy
+  int64_t NZB;             // zone blocks of 64 per sector
   double *flat, *work, *ms;  // [s][zone]
   double* Bq;              // [s][fam][5][face]: normal B at the B-field points
   uint8_t *fa, *prev, *next, *ever, *inlist;  // [s][zone]
@@ -89,15 +90,20 @@ struct Dev {
     ni = min(max(stencil[c + NZ2 * (2 * n)], 1 - g), N + g);
     nj = min(max(stencil[c + NZ2 * (2 * n + 1)], 1 - g), N + g);
   }
+  // Zone-blocked layouts: the words of 64 consecutive zones lie together
+  // (a block's field c is 64 contiguous doubles), so a zone's slopes and
+  // geometry share a few pages and a neighbour's words arrive together,
+  // while a wave reading one field of 64 consecutive zones stays coalesced.
   __device__ __forceinline__ double& v(int m, int s, int c, int64_t z) const {
-    return V[(((int64_t)m * 6 + s) * 9 + c) * NZ + z];
+    return V[(((int64_t)s * NZB + (z >> 6)) * 36 + m * 9 + c) * 64 + (z & 63)];
   }
-  __device__ __forceinline__ double cen(int s, int64_t z, int c) const { return zg[((int64_t)s * 5 + c) * NZ + z]; }
-  __device__ __forceinline__ double vol(int s, int64_t z) const { return zg[((int64_t)s * 5 + 3) * NZ + z]; }
-  __device__ __forceinline__ double obl(int s, int64_t z) const {
-    return 1.0 / fmax(zg[((int64_t)s * 5 + 4) * NZ + z], smallnum);
+  __device__ __forceinline__ double zgw(int s, int64_t z, int q) const {
+    return zg[(((int64_t)s * NZB + (z >> 6)) * 5 + q) * 64 + (z & 63)];
   }
-  __device__ __forceinline__ double scale(int s, int64_t z) const { return zg[((int64_t)s * 5 + 4) * NZ + z]; }
+  __device__ __forceinline__ double cen(int s, int64_t z, int c) const { return zgw(s, z, c); }
+  __device__ __forceinline__ double vol(int s, int64_t z) const { return zgw(s, z, 3); }
+  __device__ __forceinline__ double obl(int s, int64_t z) const { return 1.0 / fmax(zgw(s, z, 4), smallnum); }
+  __device__ __forceinline__ double scale(int s, int64_t z) const { return zgw(s, z, 4); }
   __device__ __forceinline__ double* bq(int s, int fam, int p) const {
     return Bq + (((int64_t)s * 3 + fam) * 5 + p) * NFC;
   }
@@ -138,6 +144,24 @@ static Box make_box(int i0, int i1, int j0, int j1, int k0, int k1) {
 
 __device__ __forceinline__ int64_t tid() { return (int64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 
+// XCD-aware workgroup order.  The chip hands workgroup b (row-major over the
+// grid) to XCD b % 8, each XCD with its own L2; renumbering the workgroups
+// so that every XCD receives one contiguous run of the logical grid keeps
+// the workgroups that read each other's zones (the next row, the next
+// plane) on one L2 and close in time.
+constexpr int kXcds = 8;
+struct LBlock {
+  int64_t x;  // logical blockIdx.x
+  int y;      // logical blockIdx.y
+};
+__device__ __forceinline__ LBlock lblock() {
+  const int64_t G = (int64_t)gridDim.x * gridDim.y, b = blockIdx.x + (int64_t)blockIdx.y * gridDim.x;
+  const int64_t x = b % kXcds, q = G / kXcds, r = G % kXcds;
+  const int64_t lb = x * q + (x < r ? x : r) + b / kXcds;
+  return LBlock{lb % gridDim.x, (int)(lb / gridDim.x)};
+}
+__device__ __forceinline__ int64_t ltid(const LBlock& L) { return L.x * blockDim.x + threadIdx.x; }
+
 __device__ __forceinline__ bool in_weno_range(const Dev& D, int i, int j, int k) {
   return i >= -2 && i <= D.N + 3 && j >= -2 && j <= D.N + 3 && k >= -2 && k <= D.NR + 3;
 }
@@ -170,9 +194,10 @@ __global__ void k_zone_geo(Dev D, Box b, double* zg) {
   csg1::zone_geometry(D.G, s, i, j, k, c, &v, &l);
   // the inactive tri-corner zones get smallnum (410...f step 2, 485...f)
   if (!D.active(i, j)) l = D.smallnum;
-  for (int q = 0; q < 3; ++q) zg[((int64_t)s * 5 + q) * D.NZ + z] = c[q];
-  zg[((int64_t)s * 5 + 3) * D.NZ + z] = v;
-  zg[((int64_t)s * 5 + 4) * D.NZ + z] = l;
+  double* o = zg + ((int64_t)s * D.NZB + (z >> 6)) * 5 * 64 + (z & 63);
+  for (int q = 0; q < 3; ++q) o[q * 64] = c[q];
+  o[3 * 64] = v;
+  o[4 * 64] = l;
 }
 
 // ---------------------------------------------------------------------------
@@ -181,8 +206,9 @@ __global__ void k_zone_geo(Dev D, Box b, double* zg) {
 // are cleared (the WENO writes the others)
 __global__ void __launch_bounds__(256) k_prim(Dev D, Box b, Buf6 in) {
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
-  const int s = blockIdx.y;
+  const LBlock wg = lblock();
+  if (!b.decode(ltid(wg), i, j, k)) return;
+  const int s = wg.y;
   const int64_t z = D.zi(i, j, k);
   const bool act = D.active(i, j);
   double U[8], Vv[8];
@@ -265,8 +291,9 @@ __global__ void __launch_bounds__(256) k_tricorner(Dev D, Buf6 in) {
 // extension (b) and its spread to the six neighbours (c)
 __global__ void __launch_bounds__(256) k_flat_a(Dev D, Box b) {
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
-  const int s = blockIdx.y;
+  const LBlock wg = lblock();
+  if (!b.decode(ltid(wg), i, j, k)) return;
+  const int s = wg.y;
   const int64_t z = D.zi(i, j, k), so = (int64_t)s * D.NZ;
   if (!in_flat_range(D, i, j, k)) {
     D.flat[so + z] = 0.0;
@@ -314,8 +341,9 @@ __global__ void __launch_bounds__(256) k_flat_a(Dev D, Box b) {
 }
 __global__ void __launch_bounds__(256) k_flat_b(Dev D, Box b) {
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
-  const int s = blockIdx.y;
+  const LBlock wg = lblock();
+  if (!b.decode(ltid(wg), i, j, k)) return;
+  const int s = wg.y;
   const int64_t z = D.zi(i, j, k), so = (int64_t)s * D.NZ;
   double f = D.flat[so + z];
   if (in_flat_range(D, i, j, k)) {
@@ -330,8 +358,9 @@ __global__ void __launch_bounds__(256) k_flat_b(Dev D, Box b) {
 }
 __global__ void __launch_bounds__(256) k_flat_c(Dev D, Box b) {
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
-  const int s = blockIdx.y;
+  const LBlock wg = lblock();
+  if (!b.decode(ltid(wg), i, j, k)) return;
+  const int s = wg.y;
   const int64_t z = D.zi(i, j, k), so = (int64_t)s * D.NZ;
   double f = D.work[so + z];
   if (in_flat_range(D, i, j, k)) {
@@ -365,14 +394,13 @@ __device__ __forceinline__ void weno3(const Dev& D, int s, int i, int j, int k, 
   double dR[3], dL[3], dU[3], dD[3], dT[3], dB[3], num[3][3], den[3], sx[3][3];
 #pragma unroll
   for (int f = 0; f < 3; ++f) {
-    const double* src = &D.v(0, s, c0 + f, 0);
-    const double f0 = src[z];
-    dR[f] = src[z6[0]] - f0;
-    dL[f] = src[z6[1]] - f0;
-    dU[f] = src[z6[2]] - f0;
-    dD[f] = src[z6[3]] - f0;
-    dT[f] = src[z6[4]] - f0;
-    dB[f] = src[z6[5]] - f0;
+    const double f0 = D.v(0, s, c0 + f, z);
+    dR[f] = D.v(0, s, c0 + f, z6[0]) - f0;
+    dL[f] = D.v(0, s, c0 + f, z6[1]) - f0;
+    dU[f] = D.v(0, s, c0 + f, z6[2]) - f0;
+    dD[f] = D.v(0, s, c0 + f, z6[3]) - f0;
+    dT[f] = D.v(0, s, c0 + f, z6[4]) - f0;
+    dB[f] = D.v(0, s, c0 + f, z6[5]) - f0;
     den[f] = 0.0;
 #pragma unroll
     for (int m = 0; m < 3; ++m) num[f][m] = sx[f][m] = 0.0;
@@ -418,13 +446,98 @@ __device__ __forceinline__ void weno3(const Dev& D, int s, int i, int j, int k, 
     for (int m = 0; m < 3; ++m) D.v(m + 1, s, c0 + f, z) = (num[f][m] + w8 * X8[m]) * os * omf;
   }
 }
-// every zone of the WENO range: thread t takes fields 3 (t / n) .. + 2 of zone t % n
-__global__ void __launch_bounds__(128) k_weno(Dev D, Box b) {
-  const int64_t t = tid(), n = b.count();
-  if (t >= 3 * n) return;
-  int i, j, k;
-  b.decode(t % n, i, j, k);
-  weno3(D, blockIdx.y, i, j, k, D.zi(i, j, k), 3 * (int)(t / n));
+// every zone of the WENO range: a workgroup of three waves takes 64 zones;
+// wave w forms the inverses of stencils w, w + 3, w + 6 of its lane's zone
+// into LDS (each inverse once per zone instead of once per field group),
+// then takes fields 3w .. 3w + 2 of that zone over the eight stencils, as
+// weno3 does
+constexpr int kWenoZones = 64;
+__global__ void __launch_bounds__(3 * kWenoZones) k_weno(Dev D, Box b) {
+  __shared__ double Ainv[8][9][kWenoZones];
+  const int lane = threadIdx.x % kWenoZones, grp = threadIdx.x / kWenoZones;
+  const LBlock wg = lblock();
+  const int s = wg.y;
+  int i = 0, j = 0, k = 0;
+  const bool ok = b.decode(wg.x * kWenoZones + lane, i, j, k) && D.active(i, j);
+  int64_t z = 0, z6[6];
+  double obl = 0.0;
+  if (ok) {
+    z = D.zi(i, j, k);
+    six(D, i, j, k, z6);
+    obl = D.obl(s, z);
+    double off[6][3];
+#pragma unroll
+    for (int q = 0; q < 6; ++q)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) off[q][c] = (D.cen(s, z6[q], c) - D.cen(s, z, c)) * obl;
+#pragma unroll 1
+    for (int st = grp; st < 8; st += 3) {
+      const bool bx = (st >> 1) & 1, by = (st >> 2) & 1, bz = st & 1;
+      double m3[3][3], A[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        m3[0][c] = bx ? off[1][c] : off[0][c];
+        m3[1][c] = by ? off[3][c] : off[2][c];
+        m3[2][c] = bz ? off[5][c] : off[4][c];
+      }
+      inv3(m3, A);
+#pragma unroll
+      for (int q = 0; q < 9; ++q) Ainv[st][q][lane] = A[q / 3][q % 3];
+    }
+  }
+  __syncthreads();
+  if (!ok) return;
+  const int c0 = 3 * grp;
+  const double smallnum = D.smallnum, omf = 1.0 - D.flat[(int64_t)s * D.NZ + z];
+  double dR[3], dL[3], dU[3], dD[3], dT[3], dB[3], num[3][3], den[3], sx[3][3];
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    const double f0 = D.v(0, s, c0 + f, z);
+    dR[f] = D.v(0, s, c0 + f, z6[0]) - f0;
+    dL[f] = D.v(0, s, c0 + f, z6[1]) - f0;
+    dU[f] = D.v(0, s, c0 + f, z6[2]) - f0;
+    dD[f] = D.v(0, s, c0 + f, z6[3]) - f0;
+    dT[f] = D.v(0, s, c0 + f, z6[4]) - f0;
+    dB[f] = D.v(0, s, c0 + f, z6[5]) - f0;
+    den[f] = 0.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) num[f][m] = sx[f][m] = 0.0;
+  }
+#pragma unroll 1
+  for (int st = 0; st < 8; ++st) {
+    const bool bx = (st >> 1) & 1, by = (st >> 2) & 1, bz = st & 1;
+    double A[3][3];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) A[q / 3][q % 3] = Ainv[st][q][lane];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+      const double r0 = bx ? dL[f] : dR[f], r1 = by ? dD[f] : dU[f], r2 = bz ? dB[f] : dT[f];
+      double X[3];
+#pragma unroll
+      for (int m = 0; m < 3; ++m) X[m] = A[m][0] * r0 + A[m][1] * r1 + A[m][2] * r2;
+      double tt = X[0] * X[0] + X[1] * X[1] + X[2] * X[2] + smallnum;
+      tt = tt * tt;
+      const double w = 1.0 / (tt * tt);
+      den[f] = st == 0 ? w : den[f] + w;
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        num[f][m] += w * X[m];
+        sx[f][m] += X[m];
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 3; ++f) {
+    double X8[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) X8[m] = 0.125 * sx[f][m];
+    double tt = X8[0] * X8[0] + X8[1] * X8[1] + X8[2] * X8[2] + smallnum;
+    tt = tt * tt;
+    const double w8 = 100.0 / (tt * tt);
+    const double os = 1.0 / (den[f] + w8);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) D.v(m + 1, s, c0 + f, z) = (num[f][m] + w8 * X8[m]) * os * omf;
+  }
 }
 // the zones of the replay list (grid-stride)
 __global__ void __launch_bounds__(128) k_weno_list(Dev D) {
@@ -605,8 +718,9 @@ __device__ __forceinline__ void positivity(const Dev& D, int s, int i, int j, in
 }
 __global__ void __launch_bounds__(256) k_pos(Dev D, Box b) {
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
-  positivity(D, blockIdx.y, i, j, k, D.zi(i, j, k), -1);
+  const LBlock wg = lblock();
+  if (!b.decode(ltid(wg), i, j, k)) return;
+  positivity(D, wg.y, i, j, k, D.zi(i, j, k), -1);
 }
 __global__ void __launch_bounds__(256) k_pos_list(Dev D, int sweep) {
   if (D.ctr[0] == 0 || D.ctr[2]) return;
@@ -678,11 +792,7 @@ __global__ void __launch_bounds__(256) k_clear_ent(Dev D, Box b, Buf6 in, int* e
 // step 8.a-c: normal B at the five B-field points of each face from the two
 // zones' linear B, shifted so that the weighted mean is the face's Bf;
 // blockIdx.y = fam + 3 s
-__global__ void __launch_bounds__(256) k_face_pts(Dev D, Box bx, Box by, Box bz, Buf6 in) {
-  const int fam = blockIdx.y % 3, s = blockIdx.y / 3;
-  const Box& b = fam == 0 ? bx : (fam == 1 ? by : bz);
-  int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
+__device__ __forceinline__ void face_points(const Dev& D, int fam, int s, int i, int j, int k, const Buf6& in) {
   const int64_t fb = D.fci(i, j, k);
   int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1);
   bool skip = false;
@@ -736,6 +846,27 @@ __global__ void __launch_bounds__(256) k_face_pts(Dev D, Box bx, Box by, Box bz,
 #pragma unroll
   for (int q = 0; q < 5; ++q) D.bq(s, fam, q)[fb] = q5[q] + bd;
 }
+// one column (i, j) of one sector per thread, marching up in k through the
+// faces of index (i, j, k) of the three families' ranges (x: i -3 .. N+3,
+// j, k -2 .. N+3, NR+3; y and z likewise), so each zone's B modes are read
+// from HBM about once
+__global__ void __launch_bounds__(256) k_face_pts(Dev D, Buf6 in) {
+  const LBlock wg = lblock();
+  const int s = wg.y, nI = D.N + 7;
+  const int64_t t = ltid(wg);
+  if (t >= (int64_t)nI * nI) return;
+  const int i = -3 + (int)(t % nI), j = -3 + (int)(t / nI);
+  const bool fx = j >= -2, fy = i >= -2, fz = i >= -2 && j >= -2;
+  const int nq = 3 * (D.NR + 7);
+#pragma unroll 1
+  for (int q = 0; q < nq; ++q) {
+    const int k = -3 + q / 3, fam = q % 3;
+    const bool on = fam == 0 ? (k >= -2 && fx) : fam == 1 ? (k >= -2 && fy) : fz;
+    int ii = i, jj = j;  // opaque, as in k_flux
+    asm volatile("" : "+v"(ii), "+v"(jj));
+    if (on) face_points(D, fam, s, ii, jj, k, in);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // step 8.d: the ADP constrained least squares for the 12 linear B
@@ -748,8 +879,9 @@ __global__ void __launch_bounds__(256) k_face_pts(Dev D, Box bx, Box by, Box bz,
 __global__ void __launch_bounds__(64) k_clsq(Dev D, Box b, Buf6 in) {
   __shared__ double Zs[72][64];  // each thread's own column: no barrier needed
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
-  const int s = blockIdx.y;
+  const LBlock wg = lblock();
+  if (!b.decode(ltid(wg), i, j, k)) return;
+  const int s = wg.y;
   const int64_t z = D.zi(i, j, k);
   if (!in_weno_range(D, i, j, k) || !D.active(i, j)) return;
   const double obl = D.obl(s, z);
@@ -773,20 +905,52 @@ __global__ void __launch_bounds__(64) k_clsq(Dev D, Box b, Buf6 in) {
     csg1::FaceG F;
     csg1::face_of(D.G, fam, s, fi, fj, fk, F);
     const int64_t fb = D.fci(fi, fj, fk);
-    // least-squares rows: points 2 and 4
+    // least-squares rows: points 2 and 4, each n (x) (1, r)
+    if (fam < 2) {
+      // one normal for the whole lateral face: the two rows add
+      // (n n^T) (x) (rho_2 rho_2^T + rho_4 rho_4^T) to the normal matrix
+      double R[10], h[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int pt = 1 + 2 * t;
-      double r[3];
+      for (int q = 0; q < 10; ++q) R[q] = 0.0;
 #pragma unroll
-      for (int c = 0; c < 3; ++c) r[c] = (F.p[pt][c] - c0[c]) * obl;
-      brow(fam == 2 ? F.n[pt] : F.n[0], r, row);
-      const double bv = D.bq(s, fam, pt)[fb];
+      for (int t = 0; t < 2; ++t) {
+        const int pt = 1 + 2 * t;
+        const double bv = D.bq(s, fam, pt)[fb];
+        const double rho[4] = {1.0, (F.p[pt][0] - c0[0]) * obl, (F.p[pt][1] - c0[1]) * obl,
+                               (F.p[pt][2] - c0[2]) * obl};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          h[m] += rho[m] * bv;
+#pragma unroll
+          for (int m2 = 0; m2 <= m; ++m2) R[LIX(m, m2)] += rho[m] * rho[m2];
+        }
+      }
+      const double* n = F.n[0];
 #pragma unroll
       for (int p = 0; p < 12; ++p) {
-        u[p] += row[p] * bv;
+        const int c1 = p >> 2, m1 = p & 3;
+        u[p] += n[c1] * h[m1];
 #pragma unroll
-        for (int q = 0; q <= p; ++q) L[LIX(p, q)] += row[p] * row[q];
+        for (int q = 0; q <= p; ++q) {
+          const int c2 = q >> 2, m2 = q & 3;
+          L[LIX(p, q)] += (n[c1] * n[c2]) * R[m1 >= m2 ? LIX(m1, m2) : LIX(m2, m1)];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const int pt = 1 + 2 * t;
+        double r[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r[c] = (F.p[pt][c] - c0[c]) * obl;
+        brow(F.n[pt], r, row);
+        const double bv = D.bq(s, fam, pt)[fb];
+#pragma unroll
+        for (int p = 0; p < 12; ++p) {
+          u[p] += row[p] * bv;
+#pragma unroll
+          for (int q = 0; q <= p; ++q) L[LIX(p, q)] += row[p] * row[q];
+        }
       }
     }
     // constraint row: the normal of the face's first point, at its centroid
@@ -835,19 +999,20 @@ __global__ void __launch_bounds__(64) k_clsq(Dev D, Box b, Buf6 in) {
       for (int p = 0; p < 12; ++p) t += Zs[r * 12 + p][threadIdx.x] * Zs[c * 12 + p][threadIdx.x];
       S[r * (r + 1) / 2 + c] = t;
     }
+  // 6 x 6 Cholesky of S, the reciprocal of each diagonal entry kept
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     double d = S[c * (c + 1) / 2 + c];
 #pragma unroll
     for (int q = 0; q < c; ++q) d -= S[c * (c + 1) / 2 + q] * S[c * (c + 1) / 2 + q];
-    d = sqrt(d);
-    S[c * (c + 1) / 2 + c] = d;
+    const double od = 1.0 / sqrt(d);
+    S[c * (c + 1) / 2 + c] = od;
 #pragma unroll
     for (int r = c + 1; r < 6; ++r) {
       double t = S[r * (r + 1) / 2 + c];
 #pragma unroll
       for (int q = 0; q < c; ++q) t -= S[r * (r + 1) / 2 + q] * S[c * (c + 1) / 2 + q];
-      S[r * (r + 1) / 2 + c] = t / d;
+      S[r * (r + 1) / 2 + c] = t * od;
     }
   }
 #pragma unroll
@@ -855,14 +1020,14 @@ __global__ void __launch_bounds__(64) k_clsq(Dev D, Box b, Buf6 in) {
     double t = sv[r];
 #pragma unroll
     for (int q = 0; q < r; ++q) t -= S[r * (r + 1) / 2 + q] * sv[q];
-    sv[r] = t / S[r * (r + 1) / 2 + r];
+    sv[r] = t * S[r * (r + 1) / 2 + r];
   }
 #pragma unroll
   for (int r = 5; r >= 0; --r) {
     double t = sv[r];
 #pragma unroll
     for (int q = r + 1; q < 6; ++q) t -= S[q * (q + 1) / 2 + r] * sv[q];
-    sv[r] = t / S[r * (r + 1) / 2 + r];
+    sv[r] = t * S[r * (r + 1) / 2 + r];
   }
 #pragma unroll
   for (int p = 0; p < 12; ++p) {
@@ -898,12 +1063,8 @@ __device__ __forceinline__ bool flux_range(const Dev& D, int fam, int i, int j, 
   return i >= (fam == 0 ? -1 : 0) && i <= D.N + 1 && j >= (fam == 1 ? -1 : 0) && j <= D.N + 1 &&
          k >= (fam == 2 ? -1 : 0) && k <= D.NR + 1;
 }
-__global__ void __launch_bounds__(128) k_flux(Dev D, Box bx, Box by, Box bz, double* F0, double* F1, double* F2,
-                                              double* S0, double* S1, double* S2) {
-  const int fam = blockIdx.y % 3, s = blockIdx.y / 3;
-  const Box& b = fam == 0 ? bx : (fam == 1 ? by : bz);
-  int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
+__device__ __forceinline__ void face_flux(const Dev& D, int fam, int s, int i, int j, int k, double* F0, double* F1,
+                                          double* F2, double* S0, double* S1, double* S2) {
   int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1);
   if (fam < 2) {
     const bool al = D.active(li, lj), ar = D.active(ri, rj);
@@ -1006,13 +1167,14 @@ __global__ void __launch_bounds__(128) k_flux(Dev D, Box bx, Box by, Box bz, dou
   if (D.wall && fam == 2 && (k == 0 || k == D.NR)) {
     // the wall: the inner zone's state mirrored (normal velocity and normal B
     // reversed), so no mass crosses the wall and its normal B is zero
-    double* src = k == 0 ? VR : VL;
-    double* dst = k == 0 ? VL : VR;
+    const bool lo = k == 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) dst[c] = src[c];
-    dst[1] = -src[1];
-    dst[5] = -src[5];
-    if (k == 0) eL = eR;
+    for (int c = 0; c < 8; ++c) {
+      const double m = (c == 1 || c == 5) ? -1.0 : 1.0;
+      if (lo) VL[c] = m * VR[c];
+      else VR[c] = m * VL[c];
+    }
+    if (lo) eL = eR;
     else eR = eL;
   }
   const Gas<double> gs = D.gas();
@@ -1037,6 +1199,32 @@ __global__ void __launch_bounds__(128) k_flux(Dev D, Box bx, Box by, Box bz, dou
   So[0] = SL;
   So[D.NFC] = SR;
 }
+// One column (i, j) of one sector per thread, marching up in k: at each
+// level the x-, y- and z-faces of index (i, j, k) in their families'
+// ranges, so a zone's 36 slope words are read from HBM about once (the
+// faces of one zone and its upper neighbour follow each other in time and
+// hit the caches) instead of once per family.
+__global__ void __launch_bounds__(128) k_flux(Dev D, double* F0, double* F1, double* F2, double* S0, double* S1,
+                                              double* S2) {
+  const LBlock wg = lblock();
+  const int s = wg.y, nI = D.N + 3;
+  const int64_t t = ltid(wg);
+  if (t >= (int64_t)nI * nI) return;
+  const int i = -1 + (int)(t % nI), j = -1 + (int)(t / nI);
+  const bool fx = j >= 0, fy = i >= 0, fz = i >= 0 && j >= 0;
+  // one call site (the three families in turn), so the solver is inlined once
+  const int nq = 3 * (D.NR + 3);
+#pragma unroll 1
+  for (int q = 0; q < nq; ++q) {
+    const int k = -1 + q / 3, fam = q % 3;
+    const bool on = fam == 0 ? (k >= 0 && fx) : fam == 1 ? (k >= 0 && fy) : fz;
+    // the column's indices opaque to the optimiser: the k-invariant table
+    // loads stay in the loop body instead of being hoisted and held live
+    int ii = i, jj = j;
+    asm volatile("" : "+v"(ii), "+v"(jj));
+    if (on) face_flux(D, fam, s, ii, jj, k, F0, F1, F2, S0, S1, S2);
+  }
+}
 
 // ---------------------------------------------------------------------------
 // MAKE_EDGE_EFIELDS_P1 at second order (460...f:393-3721 with the tri-corner
@@ -1058,10 +1246,11 @@ __device__ __forceinline__ double emean(const Dev& D, int s, int c, const int64_
 }
 __global__ void __launch_bounds__(256) k_edge(Dev D, Box bx, Box by, Box bz, const double* S0, const double* S1,
                                               const double* S2, double* E0, double* E1, double* E2) {
-  const int fam = blockIdx.y % 3, s = blockIdx.y / 3;
+  const LBlock wg = lblock();
+  const int fam = wg.y % 3, s = wg.y / 3;
   const Box& b = fam == 0 ? bx : (fam == 1 ? by : bz);
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
+  if (!b.decode(ltid(wg), i, j, k)) return;
   const int N = D.N;
   const int64_t t = D.fci(i, j, k), NF = D.NFC;
   double* out = (fam == 0 ? E0 : fam == 1 ? E1 : E2) + (int64_t)s * NF;
@@ -1237,8 +1426,9 @@ __device__ __forceinline__ double arc(const Dev& D, int s, int i1, int j1, int i
 __global__ void __launch_bounds__(256) k_upd_face(Dev D, Buf6 in, Buf6 u0, Buf6 out, const double* E0,
                                                   const double* E1, const double* E2, const double* dtp, double wa,
                                                   double wb) {
-  const int fam = blockIdx.y % 3, s = blockIdx.y / 3;
-  const int64_t t = tid();
+  const LBlock wg = lblock();
+  const int fam = wg.y % 3, s = wg.y / 3;
+  const int64_t t = ltid(wg);
   const int N = D.N, NR = D.NR;
   const int ni = fam == 0 ? N + 1 : N, nj = fam == 1 ? N + 1 : N, nk = fam == 2 ? NR + 1 : NR;
   if (t >= (int64_t)ni * nj * nk) return;
@@ -1279,8 +1469,9 @@ __global__ void __launch_bounds__(256) k_upd_zone(Dev D, Box b, Buf6 in, Buf6 u0
                                                   const double* F1, const double* F2, const double* dtp, double wa,
                                                   double wb, int mode) {
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
-  const int s = blockIdx.y;
+  const LBlock wg = lblock();
+  if (!b.decode(ltid(wg), i, j, k)) return;
+  const int s = wg.y;
   const int64_t z = D.zi(i, j, k), so = (int64_t)s * 9 * D.NFC;
   const double dt = dtp[0];
   const int64_t fxp = D.fci(i, j, k), fxm = D.fci(i - 1, j, k), fym = D.fci(i, j - 1, k), fzm = D.fci(i, j, k - 1);
@@ -1381,8 +1572,9 @@ __global__ void __launch_bounds__(256) k_upd_zone(Dev D, Box b, Buf6 in, Buf6 u0
 // zone (the exchange then refreshes the ghosts)
 __global__ void __launch_bounds__(256) k_imex(Dev D, Box b, Buf6 un, Buf6 u1, const double* dtp) {
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
-  const int s = blockIdx.y;
+  const LBlock wg = lblock();
+  if (!b.decode(ltid(wg), i, j, k)) return;
+  const int s = wg.y;
   const int64_t z = D.zi(i, j, k);
   double u[8];
 #pragma unroll
@@ -1632,9 +1824,10 @@ int create(const cs_config* cfg, Stage** out, size_t* bytes) {
   // geometry tables, zone geometry, mask, stencil
   double *lat, *rbd, *TX, *TY, *TZ, *zg;
   int32_t *mask, *sten;
+  D.NZB = (D.NZ + 63) / 64;
   if ((rc = dalloc(S, lat, (int64_t)H.lat.size(), bytes)) || (rc = dalloc(S, rbd, (int64_t)H.rb.size(), bytes)) ||
       (rc = dalloc(S, TX, (int64_t)H.TX.size(), bytes)) || (rc = dalloc(S, TY, (int64_t)H.TY.size(), bytes)) ||
-      (rc = dalloc(S, TZ, (int64_t)H.TZ.size(), bytes)) || (rc = dalloc(S, zg, 6 * 5 * D.NZ, bytes)) ||
+      (rc = dalloc(S, TZ, (int64_t)H.TZ.size(), bytes)) || (rc = dalloc(S, zg, 6 * 5 * 64 * D.NZB, bytes)) ||
       (rc = dalloc(S, mask, D.NZ2, bytes)) || (rc = dalloc(S, sten, 24 * D.NZ2, bytes)))
     return fail(rc);
   std::vector<int32_t> hm((size_t)D.NZ2), hs((size_t)D.NZ2 * 24);
@@ -1670,7 +1863,7 @@ int create(const cs_config* cfg, Stage** out, size_t* bytes) {
   D.stencil = sten;
   // work arrays
   const int64_t Z6 = 6 * D.NZ;
-  if ((rc = dalloc(S, D.V, 4 * 9 * Z6, bytes)) || (rc = dalloc(S, D.flat, Z6, bytes)) ||
+  if ((rc = dalloc(S, D.V, 6 * 36 * 64 * D.NZB, bytes)) || (rc = dalloc(S, D.flat, Z6, bytes)) ||
       (rc = dalloc(S, D.work, Z6, bytes)) || (rc = dalloc(S, D.ms, Z6, bytes)) ||
       (rc = dalloc(S, D.Bq, 6 * 3 * 5 * D.NFC, bytes)) || (rc = dalloc(S, D.fa, Z6, bytes)) ||
       (rc = dalloc(S, D.prev, Z6, bytes)) || (rc = dalloc(S, D.next, Z6, bytes)) ||
@@ -1719,7 +1912,7 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
   k_ctr_reset<<<1, 64, 0, st>>>(D);
   mark_end(S, st, CS_P1K_FLATTEN);
   mark_begin(S, st);
-  k_weno<<<dim3(nb(3 * nwr, 128), 6), 128, 0, st>>>(D, S->wr);
+  k_weno<<<dim3(nb(nwr, kWenoZones), 6), 3 * kWenoZones, 0, st>>>(D, S->wr);
   mark_end(S, st, CS_P1K_WENO);
   mark_begin(S, st);
   k_pos<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr);
@@ -1733,16 +1926,16 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
   }
   k_clear_ent<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr, in, err);
   mark_end(S, st, CS_P1K_REPLAY);
-  const int64_t nfp = std::max(S->fx.count(), std::max(S->fy.count(), S->fz.count()));
+  const int64_t ncolp = (int64_t)(D.N + 7) * (D.N + 7);
   mark_begin(S, st);
-  k_face_pts<<<dim3(nb(nfp, 256), 18), 256, 0, st>>>(D, S->fx, S->fy, S->fz, in);
+  k_face_pts<<<dim3(nb(ncolp, 256), 6), 256, 0, st>>>(D, in);
   mark_end(S, st, CS_P1K_FACE_POINTS);
   mark_begin(S, st);
   k_clsq<<<dim3(nb(nall, 64), 6), 64, 0, st>>>(D, S->all, in);
   mark_end(S, st, CS_P1K_CLSQ);
-  const int64_t nff = std::max(S->ffx.count(), std::max(S->ffy.count(), S->ffz.count()));
+  const int64_t ncol = (int64_t)(D.N + 3) * (D.N + 3);
   mark_begin(S, st);
-  k_flux<<<dim3(nb(nff, 128), 18), 128, 0, st>>>(D, S->ffx, S->ffy, S->ffz, F[0], F[1], F[2], Sp[0], Sp[1], Sp[2]);
+  k_flux<<<dim3(nb(ncol, 128), 6), 128, 0, st>>>(D, F[0], F[1], F[2], Sp[0], Sp[1], Sp[2]);
   mark_end(S, st, CS_P1K_FLUX);
   const int64_t ned = std::max(S->ex.count(), std::max(S->ey.count(), S->ez.count()));
   mark_begin(S, st);

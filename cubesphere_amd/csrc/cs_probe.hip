@@ -5,9 +5,13 @@
 // whose FETCH_SIZE / WRITE_SIZE readings calibrate the byte counters.
 //
 //   cs_probe <kernel id> [iters=5] [workload=C2] [precision=0|1] [copy MiB=1024]
+//   cs_probe step [steps=1] [workload=C5] [0] [copy MiB=1024]
 //
 // Workloads: C1 6x32^2 surface HLL, C2 6x256^2 surface HLLD (magnetised
-// blast), C3 6x192^2x64 shell with gravity.
+// blast), C3 6x192^2x64 shell with gravity (all WENO5, the stage path's own
+// numerics); C3P1 and C5 the 6x192^2x64 and 6x512^2x128 shells with the
+// reference's P1 scheme (CS_RECON_P1ADP, NGhost 4) + HLLD + IMEX gravity.
+// "step" runs whole steps launch by launch (every kernel of the P1 stage).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -35,9 +39,10 @@ int main(int argc, char** argv) {
     fprintf(stderr, "usage: %s <kernel id> [iters] [C1|C2|C3] [precision 0|1] [copy MiB]\n", argv[0]);
     return 2;
   }
-  int kid = atoi(argv[1]);
-  int iters = argc > 2 ? atoi(argv[2]) : 5;
-  const char* wl = argc > 3 ? argv[3] : "C2";
+  const bool steps = !strcmp(argv[1], "step");
+  int kid = steps ? 0 : atoi(argv[1]);
+  int iters = argc > 2 ? atoi(argv[2]) : (steps ? 1 : 5);
+  const char* wl = argc > 3 ? argv[3] : (steps ? "C5" : "C2");
   int prec = argc > 4 ? atoi(argv[4]) : 0;
   size_t mib = argc > 5 ? (size_t)atol(argv[5]) : 1024;
   if (kid < 0 || kid >= CS_K_COUNT || iters < 1 || (prec != 0 && prec != 1) || mib < 1 || mib > 8192) {
@@ -57,9 +62,14 @@ int main(int argc, char** argv) {
     c.rmax = c.rmin + 0.5 * M_PI / c.ntp;
     if (!strcmp(wl, "C1")) c.riemann = CS_RS_HLL;
     problem = !strcmp(wl, "C1") ? 1 : 2;
-  } else if (!strcmp(wl, "C3")) {
-    c.ntp = 192; c.nr = 64; c.radial_bc = CS_RBC_PAD; c.rmax = 1.5; c.grav = 1.0;
+  } else if (!strcmp(wl, "C3") || !strcmp(wl, "C3P1") || !strcmp(wl, "C5")) {
+    c.ntp = !strcmp(wl, "C5") ? 512 : 192; c.nr = !strcmp(wl, "C5") ? 128 : 64;
+    c.radial_bc = CS_RBC_PAD; c.rmax = 1.5; c.grav = 1.0;
     c.rk_order = 2; c.imex = 1;  // IMEX-SSP2(2,2,2), gravity implicit
+    if (strcmp(wl, "C3")) {
+      c.recon = CS_RECON_P1ADP;
+      c.nghost = 4;
+    }
     problem = 3;
   } else {
     fprintf(stderr, "unknown workload %s\n", wl);
@@ -76,11 +86,17 @@ int main(int argc, char** argv) {
   if ((rc = cs_upload_state(h, U.data(), ent.data(), bx.data(), by.data(), bz.data()))) fail("upload", rc);
   if ((rc = cs_set_dt(h, 0.0))) fail("cs_set_dt", rc);
   if ((rc = cs_evaluate_timestep(h))) fail("cs_evaluate_timestep", rc);
-  if ((rc = cs_advance(h, 2, 0))) fail("cs_advance", rc);
   float ms = 0.f;
-  if ((rc = cs_time_kernel(h, kid, iters, &ms))) fail("cs_time_kernel", rc);
   int64_t units = 0;
-  if ((rc = cs_kernel_units(h, kid, &units))) fail("cs_kernel_units", rc);
+  if (steps) {
+    if ((rc = cs_time_steps(h, iters, 0, &ms))) fail("cs_time_steps", rc);
+    ms /= iters;
+    units = 6ll * c.ntp * c.ntp * c.nr;
+  } else {
+    if ((rc = cs_advance(h, 2, 0))) fail("cs_advance", rc);
+    if ((rc = cs_time_kernel(h, kid, iters, &ms))) fail("cs_time_kernel", rc);
+    if ((rc = cs_kernel_units(h, kid, &units))) fail("cs_kernel_units", rc);
+  }
   cs_destroy(h);
 
   size_t bytes = mib << 20, n = bytes / sizeof(double2);
@@ -97,8 +113,13 @@ int main(int argc, char** argv) {
   }
   hipFree(a);
   hipFree(b);
-  printf("probe kernel %d workload %s precision %d: %d launches, %.2f us avg, %lld units; "
-         "calibration copy %zu bytes read + %zu written\n",
-         kid, wl, prec, iters, ms * 1e3, (long long)units, bytes, bytes);
+  if (steps)
+    printf("probe: %d step(s) of %s launch by launch, %.2f ms per step, %lld owned zones; "
+           "calibration copy %zu bytes read + %zu written\n",
+           iters, wl, ms, (long long)units, bytes, bytes);
+  else
+    printf("probe kernel %d workload %s precision %d: %d launches, %.2f us avg, %lld units; "
+           "calibration copy %zu bytes read + %zu written\n",
+           kid, wl, prec, iters, ms * 1e3, (long long)units, bytes, bytes);
   return 0;
 }
