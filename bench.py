@@ -719,7 +719,8 @@ def _p1adp_cpu_worker(args):
     cfg = cs.make_config(N, 1 if cfg0.nr == 1 else NR, cfg0.nghost, cfg0.recon, cfg0.riemann, cfg0.emf, cs.FP64,
                          cfg0.rk_order, cfg0.radial_bc, cfg0.radial_mesh, cfg0.rmin, cfg0.rmax, grav=cfg0.grav,
                          imex=cfg0.imex)
-    st = cs.init_problem(cfg, prob, seed=seed)
+    st = cs.HostState(cfg)
+    st.U[:], st.ent[:], st.Bx[:], st.By[:], st.Bz[:] = orc.init_problem_p1(cfg, P1ADP_PROBLEM[name], seed=seed)
     G = orc.Grid(cfg)
     As = orc.p1s_patches(cfg)
     dt = orc.p1s_cfl_oracle(cfg, As, st, 1.0)
@@ -782,6 +783,12 @@ def reference_cpu_leg(seconds=10.0, N=48, NR=24):
             "sample": f"{calls} calls on a {N}^2x{NR} patch (NGhost 4), flang -O2 -fopenmp, OMP_NUM_THREADS={cores}"}
 
 
+# the device problem of each P1 workload (cs_init_problem_p1): the surface
+# blast with B = 0.5 east (C2), the isothermal hydrostatic shell with gravity
+# 1/r^2 and 1e-2 density noise (C5)
+P1ADP_PROBLEM = {"C2": 102, "C5": 103}
+
+
 def bench_p1adp(cs, args, cpu):
     """C5 (or C2) with the reference's P1 scheme: FP64 steps replayed from one
     hipGraph per step, per-launch times from an eagerly run step with a
@@ -790,13 +797,13 @@ def bench_p1adp(cs, args, cpu):
     integrals."""
     name = args.workload
     cfg, prob, _ = cs.workload(name, cs.FP64)
-    st = initial_state(cs, name)
     t0 = time.perf_counter()
     dev = cs.CubeSphere(cfg)
-    dev.upload(st)
-    _STATES.pop(name, None)
-    del st
-    log(f"{name}: handle created and state uploaded in {time.perf_counter() - t0:.1f} s")
+    # the initial state on the device (INIT_PATCH_CUBE_SPHERE with the
+    # workload's problem): face B from the vector potential's circulation,
+    # so div B starts at rounding on the P1 geometry
+    dev.init_problem_p1(P1ADP_PROBLEM[name], seed=20261202)
+    log(f"{name}: handle created and initial state built on the device in {time.perf_counter() - t0:.1f} s")
     try:
         dev.dt = 0.0
         dev.evaluate_timestep()
@@ -862,9 +869,13 @@ def bench_p1adp(cs, args, cpu):
         "config": {"workload": f"{name}: 6x{cfg.ntp}x{cfg.ntp}x{cfg.nr} zones, the reference's P1 scheme "
                                f"(WENO-ADP) + {'HLLD' if cfg.riemann == cs.RS_HLLD else 'HLL'} + 4-state HLL "
                                "edge fields + arc-length CT, "
-                               + ("IMEX-SSP2(2,2,2) with implicit gravity 1/r^2, shell r in [1, 1.5], hydrostatic "
-                                  "+ 1e-2 density noise" if cfg.imex else
-                                  "SSP-RK3, surface between conducting walls, magnetised blast"),
+                               + ("IMEX-SSP2(2,2,2) with implicit gravity 1/r^2, shell r in [1, 1.5], isothermal "
+                                  "hydrostatic state + seeded 1e-2 density noise, B from the vector potential "
+                                  "0.05/r^2 (yz, zx + xy/2, xy)" if cfg.imex else
+                                  "SSP-RK3, surface between conducting walls, magnetised blast (p 10 in a 0.1 rad "
+                                  "cap, B = 0.5 east)")
+                               + "; initial state built on the device by INIT_PATCH_CUBE_SPHERE "
+                                 "(cs_init_problem_p1)",
                    "zones": n_int, "precision": "fp64", "graph": True},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -129,6 +129,8 @@ def lib():
             "cs_time_kernel": [vp, i32, i32, ctypes.POINTER(ctypes.c_float)],
             "cs_profile_steps": [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32)],
             "cs_profile_steps_p1": [vp, i32, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(i32)],
+            "cs_init_problem_p1": [vp, i32, ctypes.c_uint64, ctypes.c_double],
+            "cs_add_boundary_p1": [vp, i32, i32, i32, ctypes.c_uint64, ctypes.c_double],
             "cs_autotune": [vp, i32, ctypes.POINTER(i32)],
             "cs_geometry_sizes": [cfgp, vp],
             "cs_build_geometry": [cfgp, vp, vp],
@@ -368,6 +370,17 @@ class CubeSphere:
         calls = (ctypes.c_int32 * len(KERNEL_NAMES))()
         _check(lib().cs_profile_steps(self.h, nsteps, ms, calls), "cs_profile_steps")
         return {k: (ms[k] / calls[k], calls[k]) for k in range(len(KERNEL_NAMES)) if calls[k]}
+
+    def init_problem_p1(self, problem, seed=0, time=0.0):
+        """INIT_PATCH_CUBE_SPHERE with PROBLEM_GENERATOR_U_A on the device
+        (recon RECON_P1ADP): problems 0, 2 (the reference's), 102, 103."""
+        _check(lib().cs_init_problem_p1(self.h, int(problem), int(seed), float(time)), "cs_init_problem_p1")
+
+    def add_boundary_p1(self, inner, outer, problem, seed=0, time=0.0):
+        """ADD_BOUNDARY: the problem's state on the inner and / or outer
+        radial ghost layers."""
+        _check(lib().cs_add_boundary_p1(self.h, int(bool(inner)), int(bool(outer)), int(problem), int(seed),
+                                        float(time)), "cs_add_boundary_p1")
 
     def profile_steps_p1(self, nsteps):
         """The same for a handle of recon RECON_P1ADP: {P1_KERNEL_NAMES entry:

@@ -2371,6 +2371,29 @@ extern "C" int cs_profile_steps(cs_handle* h, int32_t nsteps, float* ms_sum, int
   return check_flags(h);
 }
 
+extern "C" int cs_init_problem_p1(cs_handle* h, int32_t problem, uint64_t seed, double time) {
+  CHECK_H(h);
+  if (!h->p1) { set_error("cs_init_problem_p1: the handle's recon is not CS_RECON_P1ADP"); return CS_E_INVALID; }
+  int rc = p1s::init_problem(h->p1, p1buf(h, h->cur), problem, seed, time, 0, h->stream);
+  if (rc) return rc;
+  h->vbuf = -1;
+  h->vghost = -1;
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
+extern "C" int cs_add_boundary_p1(cs_handle* h, int32_t ibdy_inner, int32_t ibdy_outer, int32_t problem,
+                                  uint64_t seed, double time) {
+  CHECK_H(h);
+  if (!h->p1) { set_error("cs_add_boundary_p1: the handle's recon is not CS_RECON_P1ADP"); return CS_E_INVALID; }
+  const int layers = (ibdy_inner ? 1 : 0) | (ibdy_outer ? 2 : 0);
+  if (!layers) return CS_OK;
+  int rc = p1s::init_problem(h->p1, p1buf(h, h->cur), problem, seed, time, layers, h->stream);
+  if (rc) return rc;
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+
 extern "C" int cs_profile_steps_p1(cs_handle* h, int32_t nsteps, float* ms_sum, int32_t* calls) {
   CHECK_H(h);
   if (!h->p1) { set_error("cs_profile_steps_p1: the handle's recon is not CS_RECON_P1ADP"); return CS_E_INVALID; }

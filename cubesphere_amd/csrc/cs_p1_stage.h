@@ -42,6 +42,10 @@ int imex_source(Stage* S, const Buf& un, const Buf& u1, const double* dtp, hipSt
 // part (nblocks() entries); the caller folds them into dt.
 int cfl_partial(Stage* S, const Buf& cur, double* part, hipStream_t st);
 int nblocks_cfl();
+// INIT_PATCH_CUBE_SPHERE (layers 0: the whole buffer, cleared first) or
+// ADD_BOUNDARY (layers 1 inner, 2 outer, 3 both radial ghost layers) of a
+// problem of PROBLEM_GENERATOR_U_A into buffer `out` (cs_problems.h).
+int init_problem(Stage* S, const Buf& out, int problem, uint64_t seed, double time, int layers, hipStream_t st);
 // Diagnostics on the P1 geometry, one partial per workgroup (diag_blocks()
 // of them): the largest relative discrete divergence |sum A B| / sum |A B|
 // of the owned active zones, and the volume integrals of U and the entropy

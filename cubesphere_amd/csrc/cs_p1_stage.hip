@@ -39,6 +39,7 @@
 #include "cs_p1_common.h"
 #include "cs_p1_stage.h"
 #include "cs_p1geo.h"
+#include "cs_problems.h"
 #include "cs_p1s_host.h"
 #include "cs_physics.h"
 
@@ -1630,6 +1631,100 @@ __global__ void __launch_bounds__(kCflThreads) k_cfl(Dev D, Buf6 cur, double cfl
 }
 
 // ---------------------------------------------------------------------------
+// INIT_PATCH_CUBE_SPHERE (820init_patch_cube_sphere.f:303-1177) and, with
+// layers (1 inner, 2 outer), ADD_BOUNDARY (825add_boundary.f:320-1267): U the
+// average of PROBLEM_GENERATOR_U_A's U over the zone's 8 Gauss points, the
+// entropy of that mean (:431-455); face B the circulation of A over the
+// face's four edges over its area.  Zones and faces of inactive columns keep
+// their values (the whole-state form clears the buffer first).
+__device__ __forceinline__ bool in_layers(const Dev& D, int k, int layers, bool zface) {
+  if (!layers) return true;
+  const bool inner = zface ? (k >= -D.g && k <= -1) : (k >= 1 - D.g && k <= 0);
+  const bool outer = k >= D.NR + 1 && k <= D.NR + D.g;
+  return ((layers & 1) && inner) || ((layers & 2) && outer);
+}
+__global__ void __launch_bounds__(128) k_init_zone(Dev D, Box b, Buf6 out, cspg::Problem P, int layers) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int s = blockIdx.y;
+  if (!in_layers(D, k, layers, false)) return;
+  const int64_t z = D.zi(i, j, k);
+  if (layers) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) out.u(D, s, c, z) = 0.0;
+  }
+  if (!D.active(i, j)) return;
+  double X[20][3], Pq[8][3], w[8], U[8];
+  const double v = csg1::zone_gauss(D.G, s, i, j, k, X, Pq, w);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) U[c] = 0.0;
+  for (int q = 0; q < 8; ++q) {
+    double Up[8], A[3];
+    cspg::point(P, Pq[q][0], Pq[q][1], Pq[q][2], Up, A);
+    const double wq = w[q] / v;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) U[c] = U[c] + Up[c] * wq;
+  }
+  if (P.id == 103) U[0] = U[0] * (1.0 + 1.0e-2 * cspg::normal(P.seed, (uint64_t)s * (uint64_t)D.NZ + (uint64_t)z));
+  const double rho = fmax(U[0], P.rhofloor), v0 = rho / (rho * rho + P.rhofloor * P.rhofloor);
+  const double vx = U[1] * v0, vy = U[2] * v0, vz = U[3] * v0;
+  double prs = (P.gamma - 1.0) * (U[4] - 0.5 * rho * (vx * vx + vy * vy + vz * vz) -
+                                  (U[5] * U[5] + U[6] * U[6] + U[7] * U[7]) * (0.5 * C<double>::ob4pi));
+  prs = fmax(prs, P.prsfloor);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) out.u(D, s, c, z) = U[c];
+  out.e(D, s, z) = prs / pow(rho, P.gamma - 1.0);
+}
+// blockIdx.y = fam + 3 s; every face of the family's index range
+__global__ void __launch_bounds__(128) k_init_face(Dev D, Buf6 out, cspg::Problem P, int layers) {
+  const int fam = blockIdx.y % 3, s = blockIdx.y / 3;
+  const int64_t t = tid();
+  if (t >= D.NFC) return;
+  const int n = D.nf, g = D.g, N = D.N;
+  const int i = (int)(t % n) - g, j = (int)((t / n) % n) - g, k = (int)(t / ((int64_t)n * n)) - g;
+  double v1[3], v2[3], v3[3], v4[3];
+  double sum = 0.0, area;
+  if (fam == 0) {
+    if (j < 1 - g || k < 1 - g || !in_layers(D, k, layers, false)) return;
+    if (!D.active(max(i, 1 - g), j) && !D.active(min(i + 1, N + g), j)) return;
+    D.G.vert(s, i, j - 1, k - 1, v1);
+    D.G.vert(s, i, j, k - 1, v2);
+    D.G.vert(s, i, j - 1, k, v3);
+    D.G.vert(s, i, j, k, v4);
+    sum = sum + cspg::edge(P, v2, v4, false, 0.0);
+    sum = sum + cspg::edge(P, v3, v1, false, 0.0);
+    sum = sum + cspg::edge(P, v4, v3, true, csg1::len3(v3));
+    sum = sum + cspg::edge(P, v1, v2, true, csg1::len3(v1));
+    area = csg1::area_x(D.G, s, i, j, k);
+  } else if (fam == 1) {
+    if (i < 1 - g || k < 1 - g || !in_layers(D, k, layers, false)) return;
+    if (!D.active(i, max(j, 1 - g)) && !D.active(i, min(j + 1, N + g))) return;
+    D.G.vert(s, i - 1, j, k - 1, v1);
+    D.G.vert(s, i - 1, j, k, v2);
+    D.G.vert(s, i, j, k - 1, v3);
+    D.G.vert(s, i, j, k, v4);
+    sum = sum + cspg::edge(P, v2, v4, true, csg1::len3(v2));
+    sum = sum + cspg::edge(P, v3, v1, true, csg1::len3(v1));
+    sum = sum + cspg::edge(P, v4, v3, false, 0.0);
+    sum = sum + cspg::edge(P, v1, v2, false, 0.0);
+    area = csg1::area_y(D.G, s, i, j, k);
+  } else {
+    if (i < 1 - g || j < 1 - g || !in_layers(D, k, layers, true)) return;
+    if (!D.active(i, j)) return;
+    D.G.vert(s, i - 1, j - 1, k, v1);
+    D.G.vert(s, i, j - 1, k, v2);
+    D.G.vert(s, i - 1, j, k, v3);
+    D.G.vert(s, i, j, k, v4);
+    sum = sum + cspg::edge(P, v2, v4, true, csg1::len3(v2));
+    sum = sum + cspg::edge(P, v3, v1, true, csg1::len3(v1));
+    sum = sum + cspg::edge(P, v4, v3, true, csg1::len3(v3));
+    sum = sum + cspg::edge(P, v1, v2, true, csg1::len3(v1));
+    area = csg1::area_z(D.G, s, i, j, k);
+  }
+  out.b(fam)[(int64_t)s * D.NFC + t] = sum / area;
+}
+
+// ---------------------------------------------------------------------------
 // diagnostics on the P1 geometry: per owned active zone the relative discrete
 // divergence |sum A_f B_f| / sum |A_f B_f| (maximum per workgroup), and the
 // volume integrals of U and the entropy (two-sum partials per workgroup)
@@ -1712,7 +1807,7 @@ __global__ void __launch_bounds__(kDiagThreads) k_sums(Dev D, Buf6 cur, double* 
 // host side
 struct Stage {
   Dev D;
-  double cfl;
+  double cfl, rmin, rmax;
   std::vector<void*> mem;
   Box all, wr, own, fx, fy, fz, ffx, ffy, ffz, ex, ey, ez;
   Marks* marks = nullptr;
@@ -1817,6 +1912,8 @@ int create(const cs_config* cfg, Stage** out, size_t* bytes) {
   D.wall = cfg->radial_bc == CS_RBC_WALL ? 1 : 0;
   D.riemann = cfg->riemann;
   S->cfl = cfg->cfl;
+  S->rmin = cfg->rmin;
+  S->rmax = cfg->rmax;
   auto fail = [&](int r) {
     destroy(S);
     return r;
@@ -1974,6 +2071,31 @@ int imex_source(Stage* S, const Buf& un, const Buf& u1, const double* dtp, hipSt
 }
 
 int nblocks_cfl() { return kCflBlocks; }
+
+int init_problem(Stage* S, const Buf& out, int problem, uint64_t seed, double time, int layers, hipStream_t st) {
+  if (!cspg::known(problem)) {
+    cs::set_error("cs_init_problem_p1: unknown problem %d (0, 2, 102, 103)", problem);
+    return CS_E_INVALID;
+  }
+  if (layers < 0 || layers > 3) {
+    cs::set_error("cs_add_boundary_p1: layers must be 1 (inner), 2 (outer) or 3");
+    return CS_E_INVALID;
+  }
+  const Dev& D = S->D;
+  const cspg::Problem P{problem, time, S->rmin, S->rmax, D.gamma, D.rhofloor, D.prsfloor, D.grav, seed};
+  if (!layers) {
+    const size_t zb = sizeof(double) * (size_t)D.NZ * 6, fb = sizeof(double) * (size_t)D.NFC * 6;
+    P1S_HIP(hipMemsetAsync(out.U, 0, zb * 8, st));
+    P1S_HIP(hipMemsetAsync(out.ent, 0, zb, st));
+    P1S_HIP(hipMemsetAsync(out.Bx, 0, fb, st));
+    P1S_HIP(hipMemsetAsync(out.By, 0, fb, st));
+    P1S_HIP(hipMemsetAsync(out.Bz, 0, fb, st));
+  }
+  k_init_zone<<<dim3(nb(S->all.count(), 128), 6), 128, 0, st>>>(D, S->all, b6(out), P, layers);
+  k_init_face<<<dim3(nb(D.NFC, 128), 18), 128, 0, st>>>(D, b6(out), P, layers);
+  P1S_HIP(hipGetLastError());
+  return CS_OK;
+}
 
 int diag_blocks(Stage* S) { return (int)nb(6 * (int64_t)S->D.N * S->D.N * S->D.NR, kDiagThreads); }
 

@@ -262,17 +262,12 @@ CS_HD void node_mid(const double* a, const double* b, bool onto_sphere, double r
   }
 }
 
-// Zone (i, j, k) of sector s: volume and centroid of the 20-node element by
-// the 2 x 2 x 2 Gauss rule (COORDS_TO_WT_QUADPTS_VOLUME_ISO2,
-// 240coords_to_wt_quadpts_volume_iso2.f:97-300: weights det J / 8, centroid
-// as the vertex mean plus the weighted offsets) and the scale length (the
-// radial edge and the four lateral edges of the lower face,
-// 220...f:2212-2290).
-CS_HD void zone_geometry(const P1Geo& G, int s, int i, int j, int k, double* cen, double* vol, double* scale) {
+// The 8 Gauss points of zone (i, j, k) (P) and their weights det J / 8 (w);
+// returns the volume, their sum.  X: the 20 nodes, filled on return.
+CS_HD double zone_gauss(const P1Geo& G, int s, int i, int j, int k, double (*X)[3], double (*P)[3], double* w) {
 #if defined(__clang__)
 #pragma clang fp contract(off)
 #endif
-  double X[20][3];
   for (int n = 0; n < 8; ++n) {
     const int di = (n == 1 || n == 2 || n == 5 || n == 6) ? 1 : 0, dj = (n == 2 || n == 3 || n == 6 || n == 7) ? 1 : 0;
     G.vert(s, i - 1 + di, j - 1 + dj, k - 1 + (n >= 4 ? 1 : 0), X[n]);
@@ -291,7 +286,7 @@ CS_HD void zone_geometry(const P1Geo& G, int s, int i, int j, int k, double* cen
   node_mid(X[2], X[6], false, 0.0, X[18]);
   node_mid(X[3], X[7], false, 0.0, X[19]);
   const double xg = 0.5 * sqrt(1.0 / 3.0);
-  double P[8][3], w[8], v = 0.0;
+  double v = 0.0;
   for (int p = 0; p < 8; ++p) {
     const double x[3] = {(p & 1) ? xg : -xg, (p & 2) ? xg : -xg, (p & 4) ? xg : -xg};
     double T[3][3];
@@ -316,7 +311,21 @@ CS_HD void zone_geometry(const P1Geo& G, int s, int i, int j, int k, double* cen
     w[p] = 0.125 * fabs(T[2][0] * Dx + T[2][1] * Dy + T[2][2] * Dz);
   }
   for (int p = 0; p < 8; ++p) v = v + w[p];
-  v = fmax(kSmall, v);
+  return fmax(kSmall, v);
+}
+
+// Zone (i, j, k) of sector s: volume and centroid of the 20-node element by
+// the 2 x 2 x 2 Gauss rule (COORDS_TO_WT_QUADPTS_VOLUME_ISO2,
+// 240coords_to_wt_quadpts_volume_iso2.f:97-300: weights det J / 8, centroid
+// as the vertex mean plus the weighted offsets) and the scale length (the
+// radial edge and the four lateral edges of the lower face,
+// 220...f:2212-2290).
+CS_HD void zone_geometry(const P1Geo& G, int s, int i, int j, int k, double* cen, double* vol, double* scale) {
+#if defined(__clang__)
+#pragma clang fp contract(off)
+#endif
+  double X[20][3], P[8][3], w[8];
+  const double v = zone_gauss(G, s, i, j, k, X, P, w);
   *vol = v;
   for (int q = 0; q < 3; ++q) {
     const double c0 = 0.125 * (X[0][q] + X[1][q] + X[2][q] + X[3][q] + X[4][q] + X[5][q] + X[6][q] + X[7][q]);

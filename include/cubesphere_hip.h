@@ -236,6 +236,25 @@ int cs_time_kernel(cs_handle* h, int32_t kernel, int32_t iters, float* ms_avg);
  * kernel id, as the kernels run inside the step (caches as the step leaves
  * them).  Advances the state by nsteps. */
 int cs_profile_steps(cs_handle* h, int32_t nsteps, float* ms_sum, int32_t* calls);
+/* ---- problem generator and radial boundary of a P1 handle -----------------
+ * cs_init_problem_p1 replaces INIT_PATCH_CUBE_SPHERE
+ * (800applications_cs/820init_patch_cube_sphere.f:8) with
+ * PROBLEM_GENERATOR_U_A (801problem_generator_u_a.f:10) on every sector of a
+ * handle of recon CS_RECON_P1ADP, on the device: the state (U, entropy, face
+ * B) is cleared, every active zone takes the mean of the problem's pointwise
+ * U over its 8 Gauss points and the entropy of that mean, every active face
+ * the circulation of the problem's vector potential over its four edges
+ * (chords radially, arcs along the sphere) over its area, so the discrete
+ * divergence of every zone vanishes to rounding.  Problems: 0 and 2 the
+ * reference's (coordinate fields :105-229, magnetised blast :249-371), 102
+ * and 103 this build's (surface blast with B = 0.5 east; isothermal
+ * hydrostatic shell with gravity 1/r^2, A = 0.05 / r^2 (yz, zx + xy/2, xy)
+ * and seeded 1e-2 density noise).  cs_add_boundary_p1 replaces ADD_BOUNDARY
+ * (825add_boundary.f:12): the same on the inner (ibdy_inner) and / or outer
+ * (ibdy_outer) radial ghost layers only, the rest of the state untouched. */
+int cs_init_problem_p1(cs_handle* h, int32_t problem, uint64_t seed, double time);
+int cs_add_boundary_p1(cs_handle* h, int32_t ibdy_inner, int32_t ibdy_outer, int32_t problem, uint64_t seed,
+                       double time);
 /* cs_profile_steps_p1: the same for a handle of recon CS_RECON_P1ADP (no
  * counterpart in the reference: measurement only), per launch (or launch
  * group) of its stage, arrays of CS_P1K_COUNT */

@@ -216,4 +216,39 @@ typedef struct or_p1_pad {
 } or_p1_pad;
 int or_pad_boundary_r_p1(const or_p1_pad* p);
 
+/* PROBLEM_GENERATOR_U_A (800applications_cs/801problem_generator_u_a.f:10):
+ * the pointwise conserved state U_pt(8) and vector potential A_pt(3) at x,
+ * MHD branch, for iproblem 0 (the coordinate fields of the messaging test,
+ * :105-229 with TESTING_OF_MPI_MESSAGING) and 2 (the magnetised blast,
+ * :249-371), plus two problems of this build with no reference counterpart:
+ * 102 (the surface blast of the benchmark: p 10 in a 0.1 rad cap around
+ * (1, 0, 0), B = 0.5 east) and 103 (the isothermal hydrostatic shell with
+ * gravity 1/r^2 and a field from A = 0.05 / r^2 (yz, zx + xy / 2, xy)). */
+typedef struct or_prob {
+  int32_t iproblem, pad;
+  double time, rmin, rmax, gamma, rhofloor, prsfloor, grav;
+  uint64_t seed;   /* 103: the 1e-2 density noise of zone (sector, i, j, k) */
+} or_prob;
+void or_problem_generator_u_a(const or_prob* pb, double x, double y, double z, double* U_pt, double* A_pt);
+
+/* INIT_PATCH_CUBE_SPHERE (800applications_cs/820init_patch_cube_sphere.f:8)
+ * of one patch, MHD, second order, no B0 field: U (mode 1) the volume
+ * average of U_pt over the 8 Gauss points of every active zone (v_pts,
+ * normalised v_wts of COORDS_TO_WT_QUADPTS_VOLUME_ISO2), the entropy
+ * (mode 1) from it; face B of every active face as the sum of A_pt . dl over
+ * its four edges (chords radially, arcs r acos(u1 . u2) along the sphere)
+ * over the face area.  U, ent, Bxf, Byf, Bzf are cleared first.  With
+ * layers = 1 (inner) | 2 (outer) only the radial ghost layers are written
+ * and nothing is cleared elsewhere: ADD_BOUNDARY
+ * (800applications_cs/825add_boundary.f:12) with ibdy_inner / ibdy_outer. */
+typedef struct or_p1_init {
+  int32_t ntp, nr, nghost, sector;
+  const double* vert;        /* (face-B layout) x 3 */
+  const int32_t* mask;
+  const double *v_pts, *v_wts; /* (zone, 3, 8), (zone, 8) */
+  const double *xf_area, *yf_area, *zf_area;
+  double *U, *ent, *Bxf, *Byf, *Bzf; /* U (zone, 8) mode 1, ent (zone) mode 1 */
+} or_p1_init;
+int or_init_patch_cube_sphere(const or_p1_init* p, const or_prob* pb, int32_t layers);
+
 #endif

@@ -1776,3 +1776,257 @@ int or_p1s_update_zones(int N, int NR, int g, const double* const* F, const doub
       }
   return 0;
 }
+
+/* ---------------------------------------------------------------------------
+ * PROBLEM_GENERATOR_U_A (801problem_generator_u_a.f:10) and
+ * INIT_PATCH_CUBE_SPHERE (820init_patch_cube_sphere.f:8) / ADD_BOUNDARY
+ * (825add_boundary.f:12) on one P1 patch. */
+
+/* splitmix64 and the standard normal of (seed, id) by Box-Muller: the
+ * density noise of problem 103, one draw per zone */
+static uint64_t pg_mix(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+static double pg_unit(uint64_t seed, uint64_t id, uint64_t stream) {
+  const uint64_t h = pg_mix(pg_mix(seed ^ (stream * 0xD1B54A32D192ED03ull)) + id);
+  return ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+}
+static double pg_normal(uint64_t seed, uint64_t id) {
+  const double u1 = pg_unit(seed, id, 21), u2 = pg_unit(seed, id, 22);
+  return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+}
+
+void or_problem_generator_u_a(const or_prob* pb, double x, double y, double z, double* U, double* A) {
+  const double gam = pb->gamma;
+  int c;
+  for (c = 0; c < 8; ++c) U[c] = 0.0;
+  A[0] = A[1] = A[2] = 0.0;
+  if (pb->iproblem == 0) {
+    /* the coordinate fields of the messaging test (:158-196): U directly */
+    A[0] = y * z;
+    A[1] = z * x * 2.0;
+    A[2] = x * y * 3.0;
+    U[0] = fabs(x);
+    U[1] = x;
+    U[2] = y;
+    U[3] = z;
+    U[4] = fabs(z);
+    U[5] = 1.0 * x;
+    U[6] = -2.0 * y;
+    U[7] = 1.0 * z;
+  } else if (pb->iproblem == 2) {
+    /* the magnetised blast (:275-364): p 1000 within radius 1 of
+     * ((rmax + rmin) / 2, 0, 0), 0.1 outside, B = 100 / sqrt(3) (1, 1, 1) */
+    const double b = 100.0 / sqrt(3.0);
+    const double dx = x - (pb->rmax + pb->rmin) * 0.5;
+    const double dist = sqrt(dx * dx + y * y + z * z);
+    double V[8] = {1.0, 0.0, 0.0, 0.0, 0.1, b, b, b};
+    if (dist <= 1.0) V[4] = 1000.0;
+    A[0] = b * z;
+    A[1] = b * x;
+    A[2] = b * y;
+    const double ent = V[4] / pow(V[0], gam - 1.0);
+    or_prim_to_cons(gam, pb->rhofloor, pb->prsfloor, ent, V, U);
+  } else if (pb->iproblem == 102) {
+    /* the surface blast: p 10 within 0.1 rad of (1, 0, 0), B = 0.5 east,
+     * A = -0.5 r theta rhat (theta the colatitude) */
+    const double r = sqrt(x * x + y * y + z * z);
+    const double d = atan2(sqrt(y * y + z * z), x);
+    const double rho_xy = sqrt(x * x + y * y);
+    double V[8] = {1.0, 0.0, 0.0, 0.0, d < 0.1 ? 10.0 : 1.0, 0.0, 0.0, 0.0};
+    if (rho_xy > 0.0) {
+      V[5] = -0.5 * y / rho_xy;
+      V[6] = 0.5 * x / rho_xy;
+    }
+    const double th = atan2(rho_xy, z), ar = -0.5 * th;
+    A[0] = ar * x;
+    A[1] = ar * y;
+    A[2] = ar * z;
+    (void)r;
+    const double ent = V[4] / pow(V[0], gam - 1.0);
+    or_prim_to_cons(gam, pb->rhofloor, pb->prsfloor, ent, V, U);
+  } else if (pb->iproblem == 103) {
+    /* isothermal hydrostatic: rho = p = exp(g0 (1/r - 1/rmin)), at rest;
+     * A = s (yz, zx + xy / 2, xy), s = 0.05 / r^2, whose curl is
+     * s (0, 0, y / 2) + grad s x a with grad s = -0.1 x / r^4 */
+    const double r2 = x * x + y * y + z * z, r = sqrt(r2);
+    const double g0 = pb->grav > 0.0 ? pb->grav : 1.0;
+    const double rho = exp(g0 * (1.0 / r - 1.0 / pb->rmin));
+    const double s = 0.05 / r2, gs = -0.1 / (r2 * r2);
+    const double a[3] = {y * z, z * x + 0.5 * x * y, x * y};
+    A[0] = s * a[0];
+    A[1] = s * a[1];
+    A[2] = s * a[2];
+    double V[8] = {rho, 0.0, 0.0, 0.0, rho, 0.0, 0.0, 0.0};
+    V[5] = gs * (y * a[2] - z * a[1]);
+    V[6] = gs * (z * a[0] - x * a[2]);
+    V[7] = s * 0.5 * y + gs * (x * a[1] - y * a[0]);
+    const double ent = V[4] / pow(V[0], gam - 1.0);
+    or_prim_to_cons(gam, pb->rhofloor, pb->prsfloor, ent, V, U);
+  }
+}
+
+/* A . dl of edge a -> b of a face: along a radial edge the chord with A at
+ * its midpoint; along a lateral edge the arc at the radius of vertex r0 (the
+ * midpoint lifted onto that sphere, the unit chord times r0 acos(a . b / |a||b|)) */
+static double pg_edge(const or_prob* pb, const double* a, const double* b, int lateral, double rref) {
+  double cen[3], dl[3], U[8], A[3];
+  int c;
+  for (c = 0; c < 3; ++c) cen[c] = (a[c] + b[c]) * 0.5;
+  if (!lateral) {
+    for (c = 0; c < 3; ++c) dl[c] = b[c] - a[c];
+  } else {
+    const double cm = sqrt(cen[0] * cen[0] + cen[1] * cen[1] + cen[2] * cen[2]);
+    for (c = 0; c < 3; ++c) cen[c] = rref * cen[c] / cm;
+    double u[3];
+    for (c = 0; c < 3; ++c) u[c] = b[c] - a[c];
+    const double um = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+    for (c = 0; c < 3; ++c) u[c] = u[c] / um;
+    const double am = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    const double bm = sqrt(b[0] * b[0] + b[1] * b[1] + b[2] * b[2]);
+    const double ab = a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+    const double ang = acos(ab / (am * bm));
+    for (c = 0; c < 3; ++c) dl[c] = rref * ang * u[c];
+  }
+  or_problem_generator_u_a(pb, cen[0], cen[1], cen[2], U, A);
+  return A[0] * dl[0] + A[1] * dl[1] + A[2] * dl[2];
+}
+
+int or_init_patch_cube_sphere(const or_p1_init* p, const or_prob* pb, int32_t layers) {
+  const int N = p->ntp, NR = p->nr, g = p->nghost;
+  if (N < 1 || NR < 1 || g < 1 || layers < 0 || layers > 3) return -1;
+  const int64_t nz = N + 2 * g, nf = nz + 1, nzr = NR + 2 * g, nfr = nzr + 1;
+  const int64_t NZ = nz * nz * nzr, NFC = nf * nf * nfr;
+  const double ob8pi = 1.0 / (8.0 * (4.0 * atan(1.0)));
+  const double gam = pb->gamma;
+#define ZI(i, j, k) ((int64_t)((i) - 1 + g) + nz * (((j) - 1 + g) + nz * ((k) - 1 + g)))
+#define FC(i, j, k) ((int64_t)((i) + g) + nf * (((j) + g) + nf * ((k) + g)))
+#define XA(i, j, k) ((int64_t)((i) + g) + nf * (((j) - 1 + g) + nz * ((k) - 1 + g)))
+#define YA(i, j, k) ((int64_t)((i) - 1 + g) + nz * (((j) + g) + nf * ((k) - 1 + g)))
+#define ZA(i, j, k) ((int64_t)((i) - 1 + g) + nz * (((j) - 1 + g) + nz * ((k) + g)))
+#define LAYER(k) (((layers & 1) && (k) >= 1 - g && (k) <= 0) || ((layers & 2) && (k) >= NR + 1 && (k) <= NR + g))
+#define ZLAYER(k) (((layers & 1) && (k) >= -g && (k) <= -1) || ((layers & 2) && (k) >= NR + 1 && (k) <= NR + g))
+  int i, j, k, c, q;
+  if (!layers) {
+    memset(p->U, 0, sizeof(double) * NZ * 8);
+    memset(p->ent, 0, sizeof(double) * NZ);
+    memset(p->Bxf, 0, sizeof(double) * NFC);
+    memset(p->Byf, 0, sizeof(double) * NFC);
+    memset(p->Bzf, 0, sizeof(double) * NFC);
+  } else {
+    for (k = 1 - g; k <= NR + g; ++k)
+      if (LAYER(k))
+        for (j = 1 - g; j <= N + g; ++j)
+          for (i = 1 - g; i <= N + g; ++i)
+            for (c = 0; c < 8; ++c) p->U[ZI(i, j, k) + NZ * c] = 0.0;
+  }
+  /* zones: the normalised Gauss weights of the 8 points (:303-460) */
+  for (k = 1 - g; k <= NR + g; ++k) {
+    if (layers && !LAYER(k)) continue;
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        const int64_t z0 = ZI(i, j, k);
+        if (p->mask[z0] != 1) continue;
+        for (q = 0; q < 8; ++q) {
+          double U[8], A[3];
+          or_problem_generator_u_a(pb, p->v_pts[z0 + NZ * (0 + 3 * q)], p->v_pts[z0 + NZ * (1 + 3 * q)],
+                                   p->v_pts[z0 + NZ * (2 + 3 * q)], U, A);
+          for (c = 0; c < 8; ++c) p->U[z0 + NZ * c] = p->U[z0 + NZ * c] + U[c] * p->v_wts[z0 + NZ * q];
+        }
+        if (pb->iproblem == 103) {
+          /* the zone's density noise (no reference counterpart) */
+          const uint64_t gid = (((uint64_t)p->sector * (uint64_t)nzr + (uint64_t)(k - 1 + g)) * (uint64_t)nz +
+                                (uint64_t)(j - 1 + g)) * (uint64_t)nz + (uint64_t)(i - 1 + g);
+          p->U[z0] = p->U[z0] * (1.0 + 1.0e-2 * pg_normal(pb->seed, gid));
+        }
+        /* entropy of the zone mean (:431-455) */
+        const double rho = omax(p->U[z0], pb->rhofloor);
+        const double v0 = rho / (rho * rho + pb->rhofloor * pb->rhofloor);
+        const double vx = p->U[z0 + NZ] * v0, vy = p->U[z0 + 2 * NZ] * v0, vz = p->U[z0 + 3 * NZ] * v0;
+        const double bx = p->U[z0 + 5 * NZ], by = p->U[z0 + 6 * NZ], bz = p->U[z0 + 7 * NZ];
+        double prs = (gam - 1.0) * (p->U[z0 + 4 * NZ] - 0.5 * rho * (vx * vx + vy * vy + vz * vz) -
+                                    (bx * bx + by * by + bz * bz) * ob8pi);
+        prs = omax(prs, pb->prsfloor);
+        p->ent[z0] = prs / pow(rho, gam - 1.0);
+      }
+  }
+  double v1[3], v2[3], v3[3], v4[3];
+#define VX(out, ii, jj, kk) \
+  for (c = 0; c < 3; ++c) (out)[c] = p->vert[FC(ii, jj, kk) + NFC * c]
+#define MAG(v) sqrt((v)[0] * (v)[0] + (v)[1] * (v)[1] + (v)[2] * (v)[2])
+  /* x-faces (:464-693): v1 (i, j-1, k-1), v2 (i, j, k-1), v3 (i, j-1, k),
+   * v4 (i, j, k); e1 v2 -> v4 and e2 v3 -> v1 radial, e3 v4 -> v3 at |v3|
+   * and e4 v1 -> v2 at |v1| lateral */
+  for (k = 1 - g; k <= NR + g; ++k) {
+    if (layers && !LAYER(k)) continue;
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = -g; i <= N + g; ++i) {
+        const int ii = i > 1 - g ? i : 1 - g, ip = i + 1 < N + g ? i + 1 : N + g;
+        if (!(p->mask[ZI(ii, j, k)] == 1 || p->mask[ZI(ip, j, k)] == 1)) continue;
+        VX(v1, i, j - 1, k - 1);
+        VX(v2, i, j, k - 1);
+        VX(v3, i, j - 1, k);
+        VX(v4, i, j, k);
+        double s = 0.0;
+        s = s + pg_edge(pb, v2, v4, 0, 0.0);
+        s = s + pg_edge(pb, v3, v1, 0, 0.0);
+        s = s + pg_edge(pb, v4, v3, 1, MAG(v3));
+        s = s + pg_edge(pb, v1, v2, 1, MAG(v1));
+        p->Bxf[FC(i, j, k)] = s / p->xf_area[XA(i, j, k)];
+      }
+  }
+  /* y-faces (:695-923): v1 (i-1, j, k-1), v2 (i-1, j, k), v3 (i, j, k-1),
+   * v4 (i, j, k); e1 v2 -> v4 at |v2| and e2 v3 -> v1 at |v1| lateral, e3
+   * v4 -> v3 and e4 v1 -> v2 radial */
+  for (k = 1 - g; k <= NR + g; ++k) {
+    if (layers && !LAYER(k)) continue;
+    for (j = -g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        const int jj = j > 1 - g ? j : 1 - g, jp = j + 1 < N + g ? j + 1 : N + g;
+        if (!(p->mask[ZI(i, jj, k)] == 1 || p->mask[ZI(i, jp, k)] == 1)) continue;
+        VX(v1, i - 1, j, k - 1);
+        VX(v2, i - 1, j, k);
+        VX(v3, i, j, k - 1);
+        VX(v4, i, j, k);
+        double s = 0.0;
+        s = s + pg_edge(pb, v2, v4, 1, MAG(v2));
+        s = s + pg_edge(pb, v3, v1, 1, MAG(v1));
+        s = s + pg_edge(pb, v4, v3, 0, 0.0);
+        s = s + pg_edge(pb, v1, v2, 0, 0.0);
+        p->Byf[FC(i, j, k)] = s / p->yf_area[YA(i, j, k)];
+      }
+  }
+  /* z-faces (:925-1177): v1 (i-1, j-1, k), v2 (i, j-1, k), v3 (i-1, j, k),
+   * v4 (i, j, k); all four edges lateral: e1 v2 -> v4 at |v2|, e2 v3 -> v1
+   * at |v1|, e3 v4 -> v3 at |v3|, e4 v1 -> v2 at |v1| */
+  for (k = -g; k <= NR + g; ++k) {
+    if (layers && !ZLAYER(k)) continue;
+    for (j = 1 - g; j <= N + g; ++j)
+      for (i = 1 - g; i <= N + g; ++i) {
+        if (p->mask[ZI(i, j, 1)] != 1) continue;
+        VX(v1, i - 1, j - 1, k);
+        VX(v2, i, j - 1, k);
+        VX(v3, i - 1, j, k);
+        VX(v4, i, j, k);
+        double s = 0.0;
+        s = s + pg_edge(pb, v2, v4, 1, MAG(v2));
+        s = s + pg_edge(pb, v3, v1, 1, MAG(v1));
+        s = s + pg_edge(pb, v4, v3, 1, MAG(v3));
+        s = s + pg_edge(pb, v1, v2, 1, MAG(v1));
+        p->Bzf[FC(i, j, k)] = s / p->zf_area[ZA(i, j, k)];
+      }
+  }
+#undef VX
+#undef MAG
+#undef ZI
+#undef FC
+#undef XA
+#undef YA
+#undef ZA
+#undef LAYER
+#undef ZLAYER
+  return 0;
+}

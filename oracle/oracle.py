@@ -1085,3 +1085,84 @@ def p1s_step_oracle(grid, cfg, As, st, dt):
         p1s_stage_oracle(grid, cfg, As, st, st, other, dt, 1)
         p1s_stage_oracle(grid, cfg, As, other, st, st, dt, 2)
     return p1s_cfl_oracle(cfg, As, st, dt)
+
+
+# ---------------------------------------------------------------------------
+# PROBLEM_GENERATOR_U_A + INIT_PATCH_CUBE_SPHERE / ADD_BOUNDARY over the six
+# P1 patches of a shell (or_init_patch_cube_sphere, or the reference's
+# routines built from source: ref_init_patch)
+
+class OrProb(ctypes.Structure):
+    _fields_ = [("iproblem", ctypes.c_int32), ("pad", ctypes.c_int32), ("time", ctypes.c_double),
+                ("rmin", ctypes.c_double), ("rmax", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("rhofloor", ctypes.c_double), ("prsfloor", ctypes.c_double), ("grav", ctypes.c_double),
+                ("seed", ctypes.c_uint64)]
+
+
+P1INIT_ARRAYS = ("vert", "mask", "v_pts", "v_wts", "xf_area", "yf_area", "zf_area", "U", "ent", "Bxf", "Byf",
+                 "Bzf")
+
+
+class OrP1Init(ctypes.Structure):
+    _fields_ = ([("ntp", ctypes.c_int32), ("nr", ctypes.c_int32), ("nghost", ctypes.c_int32),
+                 ("sector", ctypes.c_int32)] + [(k, ctypes.c_void_p) for k in P1INIT_ARRAYS])
+
+
+def or_prob(cfg, problem, seed=0, time=0.0):
+    return OrProb(int(problem), 0, float(time), float(cfg.rmin), float(cfg.rmax), float(cfg.gamma),
+                  float(cfg.rhofloor), float(cfg.prsfloor), float(cfg.grav), int(seed))
+
+
+def problem_point(cfg, problem, x, seed=0, time=0.0):
+    """PROBLEM_GENERATOR_U_A at one point: (U_pt(8), A_pt(3))"""
+    L = olib()
+    L.or_problem_generator_u_a.argtypes = [ctypes.c_void_p] + [ctypes.c_double] * 3 + [ctypes.c_void_p] * 2
+    U, A = np.zeros(8), np.zeros(3)
+    pb = or_prob(cfg, problem, seed, time)
+    L.or_problem_generator_u_a(ctypes.byref(pb), float(x[0]), float(x[1]), float(x[2]), P(U), P(A))
+    return U, A
+
+
+def init_problem_p1(cfg, problem, seed=0, time=0.0, layers=0, state=None, geo=None, use_reference=False):
+    """INIT_PATCH_CUBE_SPHERE (layers 0) or ADD_BOUNDARY (layers 1 inner,
+    2 outer, 3 both; on a copy of `state`) over the six sectors on the
+    quadrature geometry; returns (U, ent, Bx, By, Bz) in the stage layout
+    (sector-major)."""
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    geo = quad_geometry(cfg) if geo is None else geo
+    gd = geometry_dims(cfg)
+    nz = (N + 2 * g) ** 2 * (NR + 2 * g)
+    nfc = (N + 2 * g + 1) ** 2 * (NR + 2 * g + 1)
+    if state is None:
+        U, ent, Bx, By, Bz = np.zeros(6 * 8 * nz), np.zeros(6 * nz), np.zeros(6 * nfc), np.zeros(6 * nfc), \
+            np.zeros(6 * nfc)
+    else:
+        U, ent, Bx, By, Bz = (np.array(a, dtype=np.float64, copy=True) for a in state)
+    pb = or_prob(cfg, problem, seed, time)
+    if use_reference:
+        R = rlib()
+        if R is None:
+            raise RuntimeError("reference build not present")
+        fn = R.ref_init_patch
+    else:
+        fn = olib().or_init_patch_cube_sphere
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    for s in range(6):
+        sl = lambda a, n, m: np.ascontiguousarray(a[s * n * m:(s + 1) * n * m])
+        A = dict(vert=sl(geo["vert"], gd["vert"], 3), mask=sl(geo["mask"], gd["zone"], 1),
+                 v_pts=sl(geo["v_pts"], gd["zone"], 24), v_wts=sl(geo["v_wts"], gd["zone"], 8),
+                 xf_area=sl(geo["xf"]["area"], gd["xf"], 1), yf_area=sl(geo["yf"]["area"], gd["yf"], 1),
+                 zf_area=sl(geo["zf"]["area"], gd["zf"], 1),
+                 U=np.ascontiguousarray(U[s * 8 * nz:(s + 1) * 8 * nz]), ent=np.ascontiguousarray(ent[s * nz:(s + 1) * nz]),
+                 Bxf=np.ascontiguousarray(Bx[s * nfc:(s + 1) * nfc]), Byf=np.ascontiguousarray(By[s * nfc:(s + 1) * nfc]),
+                 Bzf=np.ascontiguousarray(Bz[s * nfc:(s + 1) * nfc]))
+        st = OrP1Init(N, NR, g, s, *[A[k].ctypes.data for k in P1INIT_ARRAYS])
+        rc = fn(ctypes.byref(st), ctypes.byref(pb), int(layers))
+        if rc not in (None, 0):
+            raise RuntimeError("init rejected the patch")
+        U[s * 8 * nz:(s + 1) * 8 * nz] = A["U"]
+        ent[s * nz:(s + 1) * nz] = A["ent"]
+        Bx[s * nfc:(s + 1) * nfc] = A["Bxf"]
+        By[s * nfc:(s + 1) * nfc] = A["Byf"]
+        Bz[s * nfc:(s + 1) * nfc] = A["Bzf"]
+    return U, ent, Bx, By, Bz

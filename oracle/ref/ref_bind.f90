@@ -23,6 +23,16 @@ module cs_ref_bind
     integer(c_int32_t) :: ntp, nr, nghost, pad
     type(c_ptr) :: mask, bcarr_r, xf_area, yf_area, zf_area, v_volume, U, Bxf, Byf, Bzf
   end type
+  ! mirrors of or_p1_init and or_prob
+  type, bind(C) :: ref_p1_init
+    integer(c_int32_t) :: ntp, nr, nghost, sector
+    type(c_ptr) :: vert, mask, v_pts, v_wts, xf_area, yf_area, zf_area, U, ent, Bxf, Byf, Bzf
+  end type
+  type, bind(C) :: ref_prob
+    integer(c_int32_t) :: iproblem, pad
+    real(c_double) :: time, rmin, rmax, gamma, rhofloor, prsfloor, grav
+    integer(c_int64_t) :: seed
+  end type
   ! mirror of or_p1_edge / cs_p1_edge
   type, bind(C) :: ref_p1_edge
     integer(c_int32_t) :: ntp, nr, nghost, pad
@@ -456,6 +466,56 @@ contains
          zone32, ent, bf, bf, bf, zone32, zone4, zone4, zone4, ex, ey, ez, &
          zone1, zone1, zone1, zone1, zone1, zone1, bxq, byq, bzq, sl, sr, sd, su, sb, st_, efx, efy, efz)
     deallocate(fq, fw, fn, fa, fc, zq, zw, zn, za, zc, zone1, zone3, zone4, zone32, ent, bf, st)
+  end subroutine
+
+
+  ! INIT_PATCH_CUBE_SPHERE (820init_patch_cube_sphere.f:8) with layers = 0, else
+  ! ADD_BOUNDARY (825add_boundary.f:12) with ibdy_inner = layers & 1 and
+  ! ibdy_outer = layers >> 1, on one patch of a ref_p1_init (U and the face B
+  ! carry one mode; the entropy mode 1 is copied in and out); the geometry
+  ! arguments the routines do not read at these problems are zero-filled
+  subroutine ref_init_patch(p, pb, layers) bind(C, name="ref_init_patch")
+    type(ref_p1_init), intent(in) :: p
+    type(ref_prob), intent(in) :: pb
+    integer(c_int), value :: layers
+    integer :: n, nr, g, nz, nzr, nf, nfr, nzone, nxf, nzf, nfc, ib_in, ib_out
+    real(c_double), pointer :: vert(:), vpts(:), vwts(:), xa(:), ya(:), za(:), u(:), ent(:)
+    real(c_double), pointer :: bxf(:), byf(:), bzf(:)
+    integer(c_int), pointer :: mask(:)
+    real(c_double), allocatable :: scl(:), xq(:), xw(:), xn(:), xc(:), yq(:), yw(:), yn(:), yc(:)
+    real(c_double), allocatable :: zq(:), zw(:), zn(:), zc(:), v(:), entm(:), b0(:), geos(:), vcen(:)
+    real(c_double) :: vmin(9), vmax(9)
+    integer(c_int), allocatable :: bc(:)
+    external INIT_PATCH_CUBE_SPHERE, ADD_BOUNDARY, REF_INIT_COMMON, REF_SET_P1_COMMON
+    n = p%ntp; nr = p%nr; g = p%nghost
+    nz = n + 2 * g; nzr = nr + 2 * g; nf = nz + 1; nfr = nzr + 1
+    nzone = nz * nz * nzr; nxf = nf * nz * nzr; nzf = nz * nz * nfr; nfc = nf * nf * nfr
+    call c_f_pointer(p%vert, vert, [3 * nfc]); call c_f_pointer(p%mask, mask, [nzone])
+    call c_f_pointer(p%v_pts, vpts, [24 * nzone]); call c_f_pointer(p%v_wts, vwts, [8 * nzone])
+    call c_f_pointer(p%xf_area, xa, [nxf]); call c_f_pointer(p%yf_area, ya, [nxf])
+    call c_f_pointer(p%zf_area, za, [nzf])
+    call c_f_pointer(p%U, u, [8 * nzone]); call c_f_pointer(p%ent, ent, [nzone])
+    call c_f_pointer(p%Bxf, bxf, [nfc]); call c_f_pointer(p%Byf, byf, [nfc]); call c_f_pointer(p%Bzf, bzf, [nfc])
+    allocate(scl(nzone), xq(15 * nxf), xw(5 * nxf), xn(3 * nxf), xc(3 * nxf), yq(15 * nxf), yw(5 * nxf), &
+             yn(3 * nxf), yc(3 * nxf), zq(15 * nzf), zw(5 * nzf), zn(15 * nzf), zc(3 * nzf), v(8 * nzone), &
+             entm(4 * nzone), b0(3 * nzone), geos(nzone), vcen(3 * nzone), bc(2 * nz * nz))
+    scl = 0; xq = 0; xw = 0; xn = 0; xc = 0; yq = 0; yw = 0; yn = 0; yc = 0
+    zq = 0; zw = 0; zn = 0; zc = 0; v = 0; entm = 0; b0 = 0; geos = pb%gamma; vcen = 0; bc = 0
+    entm(1:nzone) = ent(1:nzone)
+    call REF_INIT_COMMON()
+    call REF_SET_P1_COMMON(pb%gamma, 1.0d0, pb%rhofloor, pb%prsfloor, 1.0d-12, 0.4d0, 1.5d0)
+    if (layers == 0) then
+      call INIT_PATCH_CUBE_SPHERE(nr, n, g, 8, 1, 1, pb%iproblem, pb%time, 0.0d0, pb%rmin, pb%rmax, vert, mask, &
+           scl, xq, xw, xn, xa, xc, yq, yw, yn, ya, yc, zq, zw, zn, za, zc, vpts, vwts, &
+           u, v, entm, bxf, byf, bzf, b0, geos, vmin, vmax)
+    else
+      ib_in = iand(layers, 1); ib_out = iand(ishft(layers, -1), 1)
+      call ADD_BOUNDARY(ib_in, ib_out, nr, n, g, 8, 1, 1, pb%iproblem, pb%time, 0.0d0, pb%rmin, pb%rmax, &
+           vert, mask, scl, xq, xw, xn, xa, xc, yq, yw, yn, ya, yc, zq, zw, zn, za, zc, vpts, vwts, vcen, &
+           u, v, entm, bxf, byf, bzf, geos, bc, vmin, vmax)
+    end if
+    ent(1:nzone) = entm(1:nzone)
+    deallocate(scl, xq, xw, xn, xc, yq, yw, yn, yc, zq, zw, zn, zc, v, entm, b0, geos, vcen, bc)
   end subroutine
 
 end module cs_ref_bind

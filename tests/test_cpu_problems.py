@@ -1,0 +1,156 @@
+"""PROBLEM_GENERATOR_U_A (801problem_generator_u_a.f:10), INIT_PATCH_CUBE_SPHERE
+(820init_patch_cube_sphere.f:8) and ADD_BOUNDARY (825add_boundary.f:12) in
+the oracle (or_problem_generator_u_a, or_init_patch_cube_sphere): bit-exact
+with the reference's routines built from source (problems 0 and 2), and the
+closed forms the two problems of this build without a reference counterpart
+(102 surface blast, 103 hydrostatic shell) must satisfy."""
+import numpy as np
+import pytest
+
+import cubesphere_amd as cs
+from oracle import oracle as orc
+
+
+def _cfg(N=6, NR=3, rmax=1.5):
+    return cs.make_config(N, NR, 4, cs.RECON_P1ADP, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
+                          cs.RMESH_UNIFORM, 1.0, rmax, grav=1.0, imex=1)
+
+
+def _ref_or_skip():
+    if orc.rlib() is None:
+        pytest.skip("reference build not present (oracle/ref/Makefile)")
+
+
+@pytest.mark.parametrize("problem", [0, 2])
+def test_init_patch_bit_exact_with_reference(problem):
+    _ref_or_skip()
+    cfg = _cfg()
+    geo = orc.quad_geometry(cfg)
+    o = orc.init_problem_p1(cfg, problem, geo=geo)
+    r = orc.init_problem_p1(cfg, problem, geo=geo, use_reference=True)
+    for nm, a, b in zip("U ent Bx By Bz".split(), o, r):
+        assert np.array_equal(a, b), nm
+        assert np.abs(b).max() > 0.0, nm
+
+
+@pytest.mark.parametrize("layers", [1, 2, 3])
+def test_add_boundary_bit_exact_with_reference(layers):
+    """ADD_BOUNDARY replaces the inner (1), outer (2) or both radial ghost
+    layers of a state with the problem's: the blast written over the
+    coordinate fields."""
+    _ref_or_skip()
+    cfg = _cfg()
+    geo = orc.quad_geometry(cfg)
+    base = orc.init_problem_p1(cfg, 0, geo=geo)
+    o = orc.init_problem_p1(cfg, 2, geo=geo, layers=layers, state=base)
+    r = orc.init_problem_p1(cfg, 2, geo=geo, layers=layers, state=base, use_reference=True)
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    nz = N + 2 * g
+    for nm, a, b, c in zip("U ent Bx By Bz".split(), o, r, base):
+        assert np.array_equal(a, b), nm
+    # only the radial ghost layers changed
+    U = o[0].reshape(6, 8, NR + 2 * g, nz, nz)
+    U0 = base[0].reshape(6, 8, NR + 2 * g, nz, nz)
+    inner, outer = slice(0, g), slice(NR + g, NR + 2 * g)
+    np.testing.assert_array_equal(U[:, :, g:NR + g], U0[:, :, g:NR + g])
+    assert (not np.array_equal(U[:, :, inner], U0[:, :, inner])) == bool(layers & 1)
+    assert (not np.array_equal(U[:, :, outer], U0[:, :, outer])) == bool(layers & 2)
+
+
+def _face_divergence(cfg, geo, st):
+    """per active zone |sum A_f B_f| / sum |A_f B_f| on the quadrature areas"""
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    nz, nf = N + 2 * g, N + 2 * g + 1
+    Bx, By, Bz = (b.reshape(6, NR + 2 * g + 1, nf, nf) for b in st[2:])
+    xa = geo["xf"]["area"].reshape(6, NR + 2 * g, nz, nf)
+    ya = geo["yf"]["area"].reshape(6, NR + 2 * g, nf, nz)
+    za = geo["zf"]["area"].reshape(6, NR + 2 * g + 1, nz, nz)
+    mask = geo["mask"].reshape(6, NR + 2 * g, nz, nz)
+    worst = 0.0
+    for s in range(6):
+        for k in range(1 - g + 1, NR + g):
+            for j in range(1 - g + 1, N + g):
+                for i in range(1 - g + 1, N + g):
+                    zk, zj, zi = k - 1 + g, j - 1 + g, i - 1 + g
+                    if mask[s, zk, zj, zi] != 1:
+                        continue
+                    t = [xa[s, zk, zj, i + g] * Bx[s, k + g, j + g, i + g],
+                         -xa[s, zk, zj, i - 1 + g] * Bx[s, k + g, j + g, i - 1 + g],
+                         ya[s, zk, j + g, zi] * By[s, k + g, j + g, i + g],
+                         -ya[s, zk, j - 1 + g, zi] * By[s, k + g, j - 1 + g, i + g],
+                         za[s, k + g, zj, zi] * Bz[s, k + g, j + g, i + g],
+                         -za[s, k - 1 + g, zj, zi] * Bz[s, k - 1 + g, j + g, i + g]]
+                    den = sum(abs(x) for x in t)
+                    if den > 0:
+                        worst = max(worst, abs(sum(t)) / den)
+    return worst
+
+
+@pytest.mark.parametrize("problem", [0, 2, 103])
+def test_init_patch_face_b_is_divergence_free(problem):
+    """face B as the circulation of A over each face's four edges: every
+    edge enters its faces with opposite signs, so each zone's net flux
+    vanishes to rounding"""
+    cfg = _cfg()
+    geo = orc.quad_geometry(cfg)
+    st = orc.init_problem_p1(cfg, problem, seed=3, geo=geo)
+    assert _face_divergence(cfg, geo, st) < 1e-13
+
+
+def test_init_patch_surface_blast_is_divergence_free():
+    cfg = cs.make_config(8, 1, 4, cs.RECON_P1ADP, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 3)
+    geo = orc.quad_geometry(cfg)
+    st = orc.init_problem_p1(cfg, 102, geo=geo)
+    assert _face_divergence(cfg, geo, st) < 1e-13
+
+
+def test_coordinate_fields_average_to_the_centroid():
+    """problem 0: U = (|x|, x, y, z, |z|, x, -2y, z) pointwise; the Gauss rule
+    of the zone integrates the linear components exactly, so their zone
+    values are the zone centroid's"""
+    cfg = _cfg()
+    geo = orc.quad_geometry(cfg)
+    U = orc.init_problem_p1(cfg, 0, geo=geo)[0].reshape(6, 8, -1)
+    cen = geo["v_cen"].reshape(6, 3, -1)
+    act = geo["mask"].reshape(6, -1) == 1
+    for c, (q, f) in enumerate(((1, 1.0), (2, 1.0), (3, 1.0), (5, 1.0), (6, -2.0), (7, 1.0))):
+        comp = {1: 0, 2: 1, 3: 2, 5: 0, 6: 1, 7: 2}[q]
+        np.testing.assert_allclose(U[:, q][act], f * cen[:, comp][act], rtol=0, atol=1e-13)
+
+
+def test_hydrostatic_field_is_the_curl_of_its_potential():
+    """problem 103: the pointwise B equals the curl of the pointwise A
+    (central differences), and rho = p = exp(g (1/r - 1/rmin)) at rest"""
+    cfg = _cfg()
+    rng = np.random.default_rng(4)
+    h = 1e-5
+    for _ in range(20):
+        d = rng.normal(size=3)
+        x = d / np.linalg.norm(d) * rng.uniform(1.0, 1.5)
+        U, _ = orc.problem_point(cfg, 103, x)
+        J = np.zeros((3, 3))
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = h
+            J[:, a] = (orc.problem_point(cfg, 103, x + e)[1] - orc.problem_point(cfg, 103, x - e)[1]) / (2 * h)
+        curl = np.array([J[2, 1] - J[1, 2], J[0, 2] - J[2, 0], J[1, 0] - J[0, 1]])
+        np.testing.assert_allclose(U[5:8], curl, rtol=0, atol=1e-9)
+        r = np.linalg.norm(x)
+        rho = np.exp(1.0 / r - 1.0 / cfg.rmin)
+        assert abs(U[0] - rho) < 1e-14 and np.all(U[1:4] == 0.0)
+        b2 = (U[5:8] ** 2).sum()
+        assert abs(U[4] - (rho / (cfg.gamma - 1) + b2 / (8 * np.pi))) < 1e-13
+
+
+def test_hydrostatic_density_noise():
+    """problem 103: each zone's density carries one seeded N(0, 1e-2)
+    factor; a different seed draws different noise"""
+    cfg = _cfg(N=10, NR=4)
+    geo = orc.quad_geometry(cfg)
+    a = orc.init_problem_p1(cfg, 103, seed=1, geo=geo)[0].reshape(6, 8, -1)[:, 0]
+    b = orc.init_problem_p1(cfg, 103, seed=2, geo=geo)[0].reshape(6, 8, -1)[:, 0]
+    act = geo["mask"].reshape(6, -1) == 1
+    # the noise-free density of a zone is the same for both seeds
+    ratio = a[act] / b[act]
+    assert 0.005 < np.std(np.log(ratio)) / np.sqrt(2.0) < 0.015
+    assert abs(np.mean(np.log(ratio))) < 2e-3
