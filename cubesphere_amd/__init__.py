@@ -19,7 +19,7 @@ RECON_PCM, RECON_WENO5, RECON_P1ADP = 0, 1, 2
 RS_HLL, RS_HLLD = 0, 1
 EMF_LLF, EMF_HLL = 0, 1
 FP64, MIXED = 0, 1
-RBC_PAD, RBC_WALL = 0, 1
+RBC_PAD, RBC_WALL, RBC_PROBLEM = 0, 1, 2
 RMESH_UNIFORM, RMESH_LOG = 0, 1
 # the launch groups of the P1 stage (CS_P1K_* of the header, in order)
 P1_KERNEL_NAMES = ["exchange", "prim", "face_sync", "flatten", "weno", "positivity", "replay", "face_points",
@@ -37,7 +37,7 @@ class Config(ctypes.Structure):
                   "rk_order", "radial_bc", "radial_mesh")] +
                 [(n, ctypes.c_double) for n in
                  ("rmin", "rmax", "gamma", "rhofloor", "prsfloor", "cfl", "grav")] +
-                [("imex", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                [("imex", ctypes.c_int32), ("bc_problem", ctypes.c_int32),
                  ("omega", ctypes.c_double * 3)])
 
     def as_dict(self):
@@ -52,15 +52,17 @@ class Sizes(ctypes.Structure):
 def make_config(ntp, nr=1, nghost=3, recon=RECON_WENO5, riemann=RS_HLLD, emf=EMF_LLF,
                 precision=FP64, rk_order=3, radial_bc=None, radial_mesh=RMESH_UNIFORM,
                 rmin=1.0, rmax=None, gamma=5.0 / 3.0, rhofloor=1e-12, prsfloor=1e-9,
-                cfl=0.4, grav=0.0, imex=0, omega=(0.0, 0.0, 0.0)):
+                cfl=0.4, grav=0.0, imex=0, omega=(0.0, 0.0, 0.0), bc_problem=0):
     """Build a Config.  nr == 1 is the thin-shell surface mode: rmax defaults
-    to rmin + (pi/2)/ntp and the radial boundaries are conducting walls."""
+    to rmin + (pi/2)/ntp and the radial boundaries are conducting walls.
+    bc_problem: the problem whose state RBC_PROBLEM writes into the radial
+    ghost layers (ADD_BOUNDARY)."""
     if rmax is None:
         rmax = rmin + (0.5 * math.pi) / ntp if nr == 1 else 1.5 * rmin
     if radial_bc is None:
         radial_bc = RBC_WALL if nr == 1 else RBC_PAD
     return Config(ntp, nr, nghost, recon, riemann, emf, precision, rk_order, radial_bc,
-                  radial_mesh, rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav, imex, 0,
+                  radial_mesh, rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav, imex, bc_problem,
                   (ctypes.c_double * 3)(*omega))
 
 

@@ -1847,6 +1847,13 @@ int launch_stage_p1(cs_handle* h, int irk) {
   // exchange of the reference (330fetch_buffers_fill_in_ux.f:337-350)
   p1s::mark_begin(h->p1, h->stream);
   if ((rc = launch_halo(h, b, true, true, true))) return rc;
+  // the radial ghost layers as the driver's radial boundary routines leave
+  // them: PAD_BOUNDARY_R (outflow pads) or ADD_BOUNDARY (the problem's state)
+  if (h->cfg.radial_bc == CS_RBC_PAD && h->cfg.nr > 1 && (rc = p1s::pad_radial(h->p1, p1buf(h, b), h->stream)))
+    return rc;
+  if (h->cfg.radial_bc == CS_RBC_PROBLEM &&
+      (rc = p1s::init_problem(h->p1, p1buf(h, b), h->cfg.bc_problem, 0, 0.0, 3, h->stream)))
+    return rc;
   p1s::mark_end(h->p1, h->stream, CS_P1K_EXCHANGE);
   if ((rc = p1s::front(h->p1, p1buf(h, b), h->F, h->S, h->E, h->flags + 2, h->stream))) return rc;
   p1s::mark_begin(h->p1, h->stream);

@@ -390,10 +390,17 @@ static int check_cfg(const cs_config* c) {
     return CS_E_INVALID;
   }
   if (c->recon < 0 || c->recon > 2 || c->riemann < 0 || c->riemann > 1 || c->emf < 0 || c->emf > 1 ||
-      c->precision < 0 || c->precision > 1 || c->radial_bc < 0 || c->radial_bc > 1 ||
+      c->precision < 0 || c->precision > 1 || c->radial_bc < 0 || c->radial_bc > 2 ||
       c->radial_mesh < 0 || c->radial_mesh > 1) {
     set_error("bad scheme selector");
     return CS_E_INVALID;
+  }
+  if (c->radial_bc == CS_RBC_PROBLEM) {
+    const int p = c->bc_problem;
+    if (c->recon != CS_RECON_P1ADP || c->nr < 2 || !(p == 0 || p == 2 || p == 102 || p == 103)) {
+      set_error("CS_RBC_PROBLEM: a CS_RECON_P1ADP shell (nr > 1) and bc_problem 0, 2, 102 or 103");
+      return CS_E_INVALID;
+    }
   }
   return CS_OK;
 }

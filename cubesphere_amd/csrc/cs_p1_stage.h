@@ -46,6 +46,9 @@ int nblocks_cfl();
 // ADD_BOUNDARY (layers 1 inner, 2 outer, 3 both radial ghost layers) of a
 // problem of PROBLEM_GENERATOR_U_A into buffer `out` (cs_problems.h).
 int init_problem(Stage* S, const Buf& out, int problem, uint64_t seed, double time, int layers, hipStream_t st);
+// PAD_BOUNDARY_R with outflow at both radial ends on buffer b (after the
+// exchange; U and face B of the radial ghost layers)
+int pad_radial(Stage* S, const Buf& b, hipStream_t st);
 // Diagnostics on the P1 geometry, one partial per workgroup (diag_blocks()
 // of them): the largest relative discrete divergence |sum A B| / sum |A B|
 // of the owned active zones, and the volume integrals of U and the entropy

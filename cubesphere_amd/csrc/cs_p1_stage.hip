@@ -1631,6 +1631,48 @@ __global__ void __launch_bounds__(kCflThreads) k_cfl(Dev D, Buf6 cur, double cfl
 }
 
 // ---------------------------------------------------------------------------
+// PAD_BOUNDARY_R (826pad_boundary_r.f:201-420) with outflow at both radial
+// ends, after the exchange: every active column's inner ghost layers copy
+// layer 1 (U, the lateral face B of the layer, the radial face B of face 0),
+// its outer ghost layers take layer nr scaled by the volume ratio V(nr) /
+// V(nr + l) and the face B of layer nr (radial: face nr) scaled by the ratio
+// of the face areas.  The entropy is not written (the exchange's copy stays).
+__global__ void __launch_bounds__(256) k_pad6(Dev D, Buf6 b) {
+  const int s = blockIdx.y, g = D.g, N = D.N, NR = D.NR, nz = D.nz;
+  const int64_t t = tid();
+  if (t >= (int64_t)nz * nz * g) return;
+  const int ix = (int)(t % nz) + 1 - g, iy = (int)((t / nz) % nz) + 1 - g, izz = (int)(t / ((int64_t)nz * nz)) + 1;
+  if (!D.active(ix, iy)) return;
+  const int64_t so = (int64_t)s * D.NFC;
+  double *Bx = b.Bx + so, *By = b.By + so, *Bz = b.Bz + so;
+  {
+    const int l = 1 - izz, r = 1;
+    const int64_t zl = D.zi(ix, iy, l), zr = D.zi(ix, iy, r);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) b.u(D, s, c, zl) = b.u(D, s, c, zr);
+    Bx[D.fci(ix, iy, l)] = Bx[D.fci(ix, iy, r)];
+    Bx[D.fci(ix - 1, iy, l)] = Bx[D.fci(ix - 1, iy, r)];
+    By[D.fci(ix, iy, l)] = By[D.fci(ix, iy, r)];
+    By[D.fci(ix, iy - 1, l)] = By[D.fci(ix, iy - 1, r)];
+    Bz[D.fci(ix, iy, l - 1)] = Bz[D.fci(ix, iy, r - 1)];
+  }
+  {
+    const int l = NR + izz, r = NR;
+    const int64_t zl = D.zi(ix, iy, l), zr = D.zi(ix, iy, r);
+    const double vr = D.vol(s, zr), vl = D.vol(s, zl);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) b.u(D, s, c, zl) = b.u(D, s, c, zr) * vr / vl;
+    using namespace csg1;
+    Bz[D.fci(ix, iy, l)] = Bz[D.fci(ix, iy, r)] * area_z(D.G, s, ix, iy, r) / area_z(D.G, s, ix, iy, l);
+    Bx[D.fci(ix, iy, l)] = Bx[D.fci(ix, iy, r)] * area_x(D.G, s, ix, iy, r) / area_x(D.G, s, ix, iy, l);
+    By[D.fci(ix, iy, l)] = By[D.fci(ix, iy, r)] * area_y(D.G, s, ix, iy, r) / area_y(D.G, s, ix, iy, l);
+    Bx[D.fci(ix - 1, iy, l)] = Bx[D.fci(ix - 1, iy, r)] * area_x(D.G, s, ix - 1, iy, r) / area_x(D.G, s, ix - 1, iy, l);
+    By[D.fci(ix, iy - 1, l)] = By[D.fci(ix, iy - 1, r)] * area_y(D.G, s, ix, iy - 1, r) / area_y(D.G, s, ix, iy - 1, l);
+  }
+  (void)N;
+}
+
+// ---------------------------------------------------------------------------
 // INIT_PATCH_CUBE_SPHERE (820init_patch_cube_sphere.f:303-1177) and, with
 // layers (1 inner, 2 outer), ADD_BOUNDARY (825add_boundary.f:320-1267): U the
 // average of PROBLEM_GENERATOR_U_A's U over the zone's 8 Gauss points, the
@@ -2071,6 +2113,13 @@ int imex_source(Stage* S, const Buf& un, const Buf& u1, const double* dtp, hipSt
 }
 
 int nblocks_cfl() { return kCflBlocks; }
+
+int pad_radial(Stage* S, const Buf& b, hipStream_t st) {
+  const Dev& D = S->D;
+  k_pad6<<<dim3(nb((int64_t)D.nz * D.nz * D.g, 256), 6), 256, 0, st>>>(D, b6(b));
+  P1S_HIP(hipGetLastError());
+  return CS_OK;
+}
 
 int init_problem(Stage* S, const Buf& out, int problem, uint64_t seed, double time, int layers, hipStream_t st) {
   if (!cspg::known(problem)) {

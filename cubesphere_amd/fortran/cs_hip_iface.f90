@@ -34,7 +34,7 @@ module cs_hip_iface
   type, bind(C) :: cs_config
     integer(c_int32_t) :: ntp, nr, nghost, recon, riemann, emf, precision, rk_order, radial_bc, radial_mesh
     real(c_double) :: rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav
-    integer(c_int32_t) :: imex, reserved
+    integer(c_int32_t) :: imex, bc_problem
     real(c_double) :: omega(3)
   end type
 

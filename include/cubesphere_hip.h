@@ -66,6 +66,9 @@ extern "C" {
 /* radial boundary */
 #define CS_RBC_PAD 0     /* zero-gradient pad (826pad_boundary_r.f:201 form) */
 #define CS_RBC_WALL 1    /* reflecting, perfectly conducting wall (surface mode) */
+#define CS_RBC_PROBLEM 2 /* CS_RECON_P1ADP shells: the radial ghost layers from the problem
+                            generator after each exchange, ADD_BOUNDARY (825add_boundary.f:12)
+                            with problem bc_problem (cs_init_problem_p1) */
 /* radial mesh (202generate_global_radial_mesh.f) */
 #define CS_RMESH_UNIFORM 0
 #define CS_RMESH_LOG 1
@@ -88,7 +91,7 @@ typedef struct cs_config {
   double cfl;          /* Courant number (cfl_coef)                         */
   double grav;         /* radial gravity g = grav / r^2 (0 = off)           */
   int32_t imex;        /* 1: gravity handled implicitly in the stage solve  */
-  int32_t reserved;
+  int32_t bc_problem;  /* the problem of CS_RBC_PROBLEM (else unused)        */
   double omega[3];     /* angular velocity of the rotating frame (omega_passive,
                           470build_non_stiff_sources.f): centrifugal and
                           Coriolis sources, explicit; 0 = off               */

@@ -15,7 +15,7 @@
 typedef struct or_config { /* field-for-field the same as cs_config */
   int32_t ntp, nr, nghost, recon, riemann, emf, precision, rk_order, radial_bc, radial_mesh;
   double rmin, rmax, gamma, rhofloor, prsfloor, cfl, grav;
-  int32_t imex, reserved;
+  int32_t imex, bc_problem;
   double omega[3];
 } or_config;
 

@@ -902,6 +902,36 @@ def _p1s_prob(cfg):
                 prsfloor=cfg.prsfloor, smallnum=1e-12, flatten_coef=0.4, prs_flatten_coef=1.5)
 
 
+def _p1s_radial_boundary(cfg, As, st):
+    """the radial ghost layers after the exchange: PAD_BOUNDARY_R with outflow
+    at both ends (shells with radial_bc 0) or ADD_BOUNDARY with the problem
+    bc_problem (radial_bc 2)"""
+    if cfg.nr < 2 or cfg.radial_bc == 1:
+        return
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    arrs = st.arrays()
+    if cfg.radial_bc == 2:
+        new = init_problem_p1(cfg, cfg.bc_problem, layers=3, state=arrs)
+        for a, b in zip(arrs, new):
+            a[:] = b
+        return
+    nz = (N + 2 * g) ** 2 * (NR + 2 * g)
+    nfc = (N + 2 * g + 1) ** 2 * (NR + 2 * g + 1)
+    prob = dict(N=N, NR=NR, g=g)
+    for s, A in enumerate(As):
+        P = dict(mask=A["mask"], bcarr_r=p1_bcarr_r(N, g), xf_area=A["xf_area"], yf_area=A["yf_area"],
+                 zf_area=A["zf_area"], v_volume=A["v_volume"],
+                 U=np.ascontiguousarray(arrs[0][s * 8 * nz:(s + 1) * 8 * nz]),
+                 Bxf=np.ascontiguousarray(arrs[2][s * nfc:(s + 1) * nfc]),
+                 Byf=np.ascontiguousarray(arrs[3][s * nfc:(s + 1) * nfc]),
+                 Bzf=np.ascontiguousarray(arrs[4][s * nfc:(s + 1) * nfc]))
+        p1p_oracle(prob, P)
+        arrs[0][s * 8 * nz:(s + 1) * 8 * nz] = P["U"]
+        arrs[2][s * nfc:(s + 1) * nfc] = P["Bxf"]
+        arrs[3][s * nfc:(s + 1) * nfc] = P["Byf"]
+        arrs[4][s * nfc:(s + 1) * nfc] = P["Bzf"]
+
+
 def p1s_stage_oracle(grid, cfg, As, st_in, st_u0, st_out, dt, irk):
     """One stage of the P1 stage of a shell on HostState-like objects with
     .arrays() = [U, ent, Bx, By, Bz] in the stage layout.  st_in is modified
@@ -923,6 +953,7 @@ def p1s_stage_oracle(grid, cfg, As, st_in, st_u0, st_out, dt, irk):
         wa, wb = ((0.0, 1.0), (0.5, 0.5))[irk - 1]
         mode = 0
     grid.halo(st_in)
+    _p1s_radial_boundary(cfg, As, st_in)
     Ui, Ei, Bi = st_in.arrays()[0], st_in.arrays()[1], st_in.arrays()[2:]
     F = [np.zeros(6 * 9 * nfc) for _ in range(3)]
     S = [np.zeros(6 * 2 * nfc) for _ in range(3)]

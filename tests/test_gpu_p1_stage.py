@@ -9,7 +9,9 @@ pieces are bit-exact with the reference's routines built from source.
 Cases named for the benchmark configurations they shrink:
   C2: 6x16^2 surface between conducting walls, HLLD, SSP-RK3, magnetised blast;
   C5: 6x8^2x3 shell r in [1, 1.5], HLLD, IMEX-SSP2(2,2,2) with the radial
-      gravity implicit, hydrostatic state with density noise;
+      gravity implicit, hydrostatic state with density noise (radial ghost
+      layers by PAD_BOUNDARY_R, outflow at both ends, and, in a second case,
+      by ADD_BOUNDARY with the problem's own state);
   plus the reference's own combination (HLL, Heun RK2) on a shell, and a
   blast with log-normal noise so the positivity limiter removes slopes and
   its loop-order replay runs on the device.
@@ -40,15 +42,23 @@ def _cfg(name):
     if name == "shell8x3_hll_rk2":
         return (cs.make_config(8, 3, 4, cs.RECON_P1ADP, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
                                cs.RMESH_UNIFORM, 1.0, 1.3), 2, 0.0)
+    if name == "shell8x3_problem_boundary":
+        return (cs.make_config(8, 3, 4, cs.RECON_P1ADP, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PROBLEM,
+                               cs.RMESH_UNIFORM, 1.0, 1.5, grav=1.0, imex=1, bc_problem=103), 103, 0.0)
     if name == "surface12_blast_noise":
         return cs.make_config(12, 1, 4, cs.RECON_P1ADP, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 3), 2, 1.5
     raise ValueError(name)
 
 
-CASES = ["C2_surface16_hlld_rk3", "C5_shell8x3_hlld_imex", "shell8x3_hll_rk2", "surface12_blast_noise"]
+CASES = ["C2_surface16_hlld_rk3", "C5_shell8x3_hlld_imex", "shell8x3_hll_rk2", "shell8x3_problem_boundary",
+         "surface12_blast_noise"]
 
 
 def _state(cfg, prob, noise, seed=11):
+    if prob >= 100:  # a problem of PROBLEM_GENERATOR_U_A on the quadrature geometry
+        st = cs.init_problem(cfg, 0, seed=seed)
+        st.U[:], st.ent[:], st.Bx[:], st.By[:], st.Bz[:] = orc.init_problem_p1(cfg, prob, seed=seed)
+        return st
     st = cs.init_problem(cfg, prob, seed=seed)
     if noise > 0.0:
         rng = np.random.default_rng(seed)
