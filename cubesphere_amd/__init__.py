@@ -100,6 +100,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIBPATH):
             raise CsError(f"{LIBPATH} is missing: run `python -m cubesphere_amd.build`")
+        # One HIP runtime per process: torch ships its own libamdhip64.so.7,
+        # and a library loaded before it would bring in a second runtime that
+        # leaves torch without devices.  Loading torch first makes the library
+        # bind to the runtime already in the process.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIBPATH)
         vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
         cfgp = ctypes.POINTER(Config)

@@ -15,6 +15,7 @@ ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libcubesphere_hip.so")
 BUILD = os.path.join(HERE, "_build")
 
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CS_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = [
@@ -70,6 +71,7 @@ PROBE = os.path.join(HERE, "cs_probe")
 
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 DEMO = os.path.join(HERE, "cs_dropin_demo")
+P1DEMO = os.path.join(HERE, "cs_p1_dropin")
 
 
 def build_fortran(verbose=False):
@@ -84,8 +86,11 @@ def build_fortran(verbose=False):
     _run([FLANG, "-c", "-O2", "-J", BUILD, os.path.join(fdir, "cs_hip_iface.f90"), "-o", obj])
     _run([FLANG, "-O2", "-I", BUILD, os.path.join(fdir, "cs_dropin_demo.f90"), obj, "-L", HERE,
           "-lcubesphere_hip", "-Wl,-rpath,$ORIGIN", "-o", DEMO])
+    _run([FLANG, "-O2", "-I", BUILD, os.path.join(fdir, "cs_p1_dropin.f90"), obj, "-L", HERE,
+          "-lcubesphere_hip", "-L", os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+          "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-o", P1DEMO])
     if verbose:
-        print("built", DEMO)
+        print("built", DEMO, P1DEMO)
     return DEMO
 
 

@@ -25,11 +25,11 @@ module cs_hip_iface
   use iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: CS_RECON_PCM = 0, CS_RECON_WENO5 = 1
+  integer(c_int), parameter :: CS_RECON_PCM = 0, CS_RECON_WENO5 = 1, CS_RECON_P1ADP = 2
   integer(c_int), parameter :: CS_RS_HLL = 0, CS_RS_HLLD = 1
   integer(c_int), parameter :: CS_EMF_LLF = 0, CS_EMF_HLL = 1
   integer(c_int), parameter :: CS_FP64 = 0, CS_MIXED = 1
-  integer(c_int), parameter :: CS_RBC_PAD = 0, CS_RBC_WALL = 1
+  integer(c_int), parameter :: CS_RBC_PAD = 0, CS_RBC_WALL = 1, CS_RBC_PROBLEM = 2
 
   type, bind(C) :: cs_config
     integer(c_int32_t) :: ntp, nr, nghost, recon, riemann, emf, precision, rk_order, radial_bc, radial_mesh

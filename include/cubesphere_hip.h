@@ -64,7 +64,9 @@ extern "C" {
 #define CS_FP64 0        /* everything in FP64 (the drop-in)               */
 #define CS_MIXED 1       /* FP32 reconstruction + Riemann solves; FP64 metric, CT, update, sums */
 /* radial boundary */
-#define CS_RBC_PAD 0     /* zero-gradient pad (826pad_boundary_r.f:201 form) */
+#define CS_RBC_PAD 0     /* zero-gradient pad (826pad_boundary_r.f:201 form); CS_RECON_P1ADP shells:
+                            PAD_BOUNDARY_R with outflow at both ends, the inner layers a copy of
+                            layer 1, the outer ones layer nr rescaled by volume and face area */
 #define CS_RBC_WALL 1    /* reflecting, perfectly conducting wall (surface mode) */
 #define CS_RBC_PROBLEM 2 /* CS_RECON_P1ADP shells: the radial ghost layers from the problem
                             generator after each exchange, ADD_BOUNDARY (825add_boundary.f:12)

@@ -1,7 +1,11 @@
-"""The reference's language on the drop-in boundary: a Fortran program using
-the ISO_C_BINDING module (cubesphere_amd/fortran/) runs one SSP-RK3 step in
-the driver's array layout through the C-ABI; its zone value must equal the
-value the Python host path produces for the same inputs and call sequence."""
+"""The reference's language on the drop-in boundary, through the
+ISO_C_BINDING module (cubesphere_amd/fortran/cs_hip_iface.f90):
+  * cs_dropin_demo runs one SSP-RK3 step of a handle in the driver's array
+    layout; its zone value must equal the value the Python host path produces
+    for the same inputs and call sequence;
+  * cs_p1_dropin fills the bind(C) structures of the reference's P1 routines
+    with device arrays and calls the seven entry points in the driver's order;
+    every array they write must equal the Python path's bit for bit."""
 import os
 import re
 import subprocess
@@ -39,3 +43,72 @@ def test_fortran_dropin_matches_python_path():
     rho_p = U[2, 0, g, g, g]          # panel 3, zone (1,1,1)
     assert abs(rho_f - rho_p) <= 1e-4 * abs(rho_p)   # printed with 5 significant digits
     assert abs(drift) < 1e-12 and divb < 1e-12
+
+
+# the arrays cs_p1_dropin reads, and those it writes back (out_<name>.bin)
+P1_FORTRAN_IN = (cs.P1_ARRAYS + ("xf_pts", "yf_pts", "zf_pts", "zf_normal", "v_volume") + cs.P1F_OUT
+                 + ("Source",))
+P1_FORTRAN_OUT = ("U", "ent", "V", "Bxf", "Byf", "Bzf", "Efx", "Efy", "Efz", "Source", "dU_dt", "flag_entprs",
+                  "msonic")
+
+
+def write_p1_dropin_inputs(path, prob, A, dtmove, newtonsc, mstar, omega, cfl, dtcur):
+    """The input directory of cs_p1_dropin: scalars.bin and one raw file per
+    array, in the reference's layouts."""
+    hdr = np.array([prob["N"], prob["NR"], prob["g"], 0], np.int32)
+    r = np.array([prob["gamma"], prob["molewt"], prob["rhofloor"], prob["prsfloor"], prob["smallnum"],
+                  prob["flatten_coef"], prob["prs_flatten_coef"], dtmove, newtonsc, mstar, *omega, cfl, dtcur],
+                 np.float64)
+    with open(os.path.join(path, "scalars.bin"), "wb") as fh:
+        fh.write(hdr.tobytes())
+        fh.write(r.tobytes())
+    for k in P1_FORTRAN_IN:
+        np.ascontiguousarray(A[k]).tofile(os.path.join(path, k + ".bin"))
+
+
+def test_fortran_p1_stage_matches_python_path(tmp_path):
+    """The reference driver's per-patch P1 stage (DRV:2396-2887) called from
+    Fortran through the bind(C) structures of cs_hip_iface.f90 (cs_p1_patch,
+    cs_p1_flux, cs_p1_edge, cs_p1_source, cs_p1_update) on device arrays:
+    every array it writes equals, bit for bit, what the same seven entry points
+    write when called through ctypes from Python on the same inputs."""
+    torch = pytest.importorskip("torch")
+    from oracle import oracle as orc
+    exe = os.path.join(os.path.dirname(cs.__file__), "cs_p1_dropin")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran P1 program not built (no flang)")
+    N, NR, g = 12, 5, 4
+    print(f"\nFortran P1 stage: N={N} NR={NR} sector 1 seed 3 noise 0.3", flush=True)
+    prob, A0 = orc.p1_problem(N, NR, g, sector=1, seed=3, shock=True, noise=0.3)
+    d = orc.p1_dims(N, NR, g)
+    for k in cs.P1F_OUT:
+        A0[k] = np.zeros((8 if k.startswith("Fluct") else 1) * d["fc"])
+    A0["dU_dt"] = np.zeros(8 * d["zone"])
+    A0["Source"] = np.zeros(8 * d["zone"])
+    dtmove, newtonsc, mstar, omega, cfl, dtcur = 2e-3, 1.0, 0.5, (0.05, 0.0, 0.2), 0.4, 1.0
+    write_p1_dropin_inputs(str(tmp_path), prob, A0, dtmove, newtonsc, mstar, omega, cfl, dtcur)
+    r = subprocess.run([exe, str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(r.stdout.strip(), flush=True)
+
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(v).to(dev) for k, v in A0.items()}
+    T["flag_entprs"] = torch.zeros(d["zone"], dtype=torch.float64, device=dev)
+    vmin, vmax = cs.limit_all_slopes_p1(N, NR, g, prob, T)
+    cs.make_facial_fluxes_p1(N, NR, g, prob, T)
+    cs.make_edge_efields_p1(N, NR, g, T)
+    cs.build_non_stiff_sources_p1(N, NR, g, newtonsc, mstar, prob["smallnum"], omega, T)
+    cs.update_imex_rk_step_p1(N, NR, g, dict(prob, dtmove=dtmove), T)
+    cs.cleanup_after_update_p1(N, NR, g, prob, T)
+    dt_p = float(cs.evaluate_timestep_p1(N, NR, g, prob, T, cfl, dtcur).item())
+    torch.cuda.synchronize()
+    for k in P1_FORTRAN_OUT:
+        want = T[k].cpu().numpy()
+        got = np.fromfile(os.path.join(str(tmp_path), "out_" + k + ".bin"), dtype=want.dtype)
+        assert got.shape == want.shape, k
+        bad = np.flatnonzero(got != want)
+        assert bad.size == 0, f"{k} differs at {bad.size} entries, first {bad[:5]}"
+    dt_f = np.fromfile(os.path.join(str(tmp_path), "out_dt.bin"), dtype=np.float64)
+    assert dt_f.shape == (1,) and dt_f[0] == dt_p, (dt_f, dt_p)
+    mm = np.fromfile(os.path.join(str(tmp_path), "out_vminmax.bin"), dtype=np.float64)
+    np.testing.assert_array_equal(mm, np.concatenate([vmin, vmax]))
