@@ -90,8 +90,16 @@ def workload(name, precision=FP64):
 _lib = None
 
 
+# status codes of the C-ABI (CS_E_*)
+E_INVALID, E_BAD_DECOMP, E_HIP, E_NONPOSITIVE, E_NAN, E_NOMEM, E_IO, E_FORMAT = -1, -2, -3, -4, -5, -6, -7, -8
+
+
 class CsError(RuntimeError):
-    pass
+    """A failed C-ABI call; code is its CS_E_* status."""
+
+    def __init__(self, msg, code=None):
+        super().__init__(msg)
+        self.code = code
 
 
 def lib():
@@ -167,6 +175,11 @@ def lib():
             "cs_kernel_units": [vp, i32, ctypes.POINTER(i64)],
             "cs_last_error": [ctypes.c_char_p, ctypes.c_size_t],
             "cs_device_info": [ctypes.c_char_p, ctypes.c_size_t],
+            "cs_dump_write": [ctypes.c_char_p, cfgp, ctypes.POINTER(DumpMeta), vp, vp, vp, vp, vp],
+            "cs_dump_read": [ctypes.c_char_p, cfgp, ctypes.POINTER(DumpMeta), vp, vp, vp, vp, vp],
+            "cs_dump_info": [ctypes.c_char_p, ctypes.POINTER(DumpMeta)],
+            "cs_write_dump": [vp, ctypes.c_char_p, ctypes.POINTER(DumpMeta)],
+            "cs_read_dump": [vp, ctypes.c_char_p, ctypes.POINTER(DumpMeta)],
         }
         for name, args in sig.items():
             f = getattr(L, name)
@@ -184,7 +197,7 @@ def last_error():
 
 def _check(rc, what):
     if rc != 0:
-        raise CsError(f"{what} failed ({rc}): {last_error()}")
+        raise CsError(f"{what} failed ({rc}): {last_error()}", rc)
 
 
 def _p(a):
@@ -233,6 +246,43 @@ class HostState:
         g, N, NR = c.nghost, c.ntp, c.nr
         U, ent = self.views()[:2]
         return U[:, :, g:g + NR, g:g + N, g:g + N], ent[:, g:g + NR, g:g + N, g:g + N]
+
+
+class DumpMeta(ctypes.Structure):
+    """cs_dump_meta: the header fields of a checkpoint (riemann.in's time,
+    dtcur, n_cc_components and dump number, the step count, the format
+    version and the configuration the state was written with)."""
+    _fields_ = [("time", ctypes.c_double), ("dtcur", ctypes.c_double), ("filenum", ctypes.c_int64),
+                ("step", ctypes.c_int64), ("n_cc_components", ctypes.c_int32), ("version", ctypes.c_int32),
+                ("cfg", Config)]
+
+
+def _path(path):
+    return os.fsencode(os.fspath(path))
+
+
+def write_dump(path, st, time=0.0, dtcur=0.0, filenum=0, step=0):
+    """Checkpoint of a host state (cs_dump_write): the riemann.in fields and
+    the five arrays in one versioned, checksummed file."""
+    m = DumpMeta(float(time), float(dtcur), int(filenum), int(step), 8, 0)
+    _check(lib().cs_dump_write(_path(path), ctypes.byref(st.cfg), ctypes.byref(m), *[_p(a) for a in st.arrays()]),
+           "cs_dump_write")
+
+
+def read_dump(path, cfg):
+    """(HostState, DumpMeta) of a checkpoint written on cfg's grid (cs_dump_read)."""
+    st = HostState(cfg)
+    m = DumpMeta()
+    _check(lib().cs_dump_read(_path(path), ctypes.byref(cfg), ctypes.byref(m), *[_p(a) for a in st.arrays()]),
+           "cs_dump_read")
+    return st, m
+
+
+def dump_info(path):
+    """The checked header of a checkpoint (cs_dump_info)."""
+    m = DumpMeta()
+    _check(lib().cs_dump_info(_path(path), ctypes.byref(m)), "cs_dump_info")
+    return m
 
 
 def init_problem(cfg, problem, seed=1234):
@@ -297,6 +347,20 @@ class CubeSphere:
         st = HostState(self.cfg)
         _check(lib().cs_download_state(self.h, *[_p(a) for a in st.arrays()]), "cs_download_state")
         return st
+
+    def write_dump(self, path, time=0.0, filenum=0, step=0):
+        """Checkpoint of U^n straight from the device (cs_write_dump); dtcur
+        is the handle's current time step."""
+        m = DumpMeta(float(time), 0.0, int(filenum), int(step), 8, 0)
+        _check(lib().cs_write_dump(self.h, _path(path), ctypes.byref(m)), "cs_write_dump")
+
+    def read_dump(self, path):
+        """Restart from a checkpoint (cs_read_dump): the state uploaded as by
+        upload() and the time step set to the dump's dtcur; returns its
+        DumpMeta."""
+        m = DumpMeta()
+        _check(lib().cs_read_dump(self.h, _path(path), ctypes.byref(m)), "cs_read_dump")
+        return m
 
     def halo_exchange(self):
         _check(lib().cs_halo_exchange(self.h), "cs_halo_exchange")

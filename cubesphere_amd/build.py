@@ -42,6 +42,8 @@ def build(verbose=False):
     p1h_o = os.path.join(BUILD, "cs_p1s_host.o")
     _run(["g++", "-O2", "-std=c++17", "-fPIC", "-pthread", "-ffp-contract=off", "-c",
           os.path.join(CSRC, "cs_p1s_host.cpp"), "-o", p1h_o])
+    dump_o = os.path.join(BUILD, "cs_dump.o")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, "cs_dump.cpp"), "-o", dump_o])
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "cs_kernels.hip"), "-o", kern_o])
     geom_o = os.path.join(BUILD, "cs_geometry.o")
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "cs_geometry.hip"), "-o", geom_o])
@@ -52,7 +54,8 @@ def build(verbose=False):
     _run([HIPCC, *HIP_FLAGS, "-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, "cs_p1_stage.hip"),
           "-o", p1s_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", mesh_o, p1h_o, kern_o, geom_o, p1_o, p1s_o, "-pthread", "-o",
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", mesh_o, p1h_o, dump_o, kern_o, geom_o, p1_o, p1s_o,
+          "-pthread", "-o",
           tmp])
     os.replace(tmp, LIB)
     if verbose:

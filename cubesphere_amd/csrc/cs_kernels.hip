@@ -20,6 +20,7 @@
 #include "cs_halo_maps.h"
 #include "cs_p1_stage.h"
 #include "cs_internal.h"
+#include "cs_dump.h"
 #include "cs_physics.h"
 
 using namespace csp;
@@ -2047,6 +2048,26 @@ extern "C" int cs_destroy(cs_handle* h) {
     if ((h)->p1) { set_error("%s: not available with CS_RECON_P1ADP", name); return CS_E_INVALID; } \
   } while (0)
 
+namespace {
+// After U^n has been copied into stage buffer 0 (cs_upload_state,
+// cs_read_dump): the two copies of each panel-edge face are made one
+// (negation-consistent) value; from then on the face update keeps them so.
+// The ghost zones and faces are filled once: faces the updates never write
+// (conducting walls) keep these ghost copies in both stage buffers.
+int finish_upload(cs_handle* h) {
+  const CsDims& d = h->d;
+  size_t cb = sizeof(double) * d.cpp * 6, fb = sizeof(double) * d.fpp * 6;
+  int rc = launch_halo(h, 0, true, true, true);
+  if (rc) return rc;
+  // the scratch stage buffer starts as a copy so its ghost zones are defined
+  for (int q = 0; q < 5; ++q)
+    CS_HIP(hipMemcpyAsync(h->buf[1][q], h->buf[0][q], q == 0 ? cb * 8 : (q == 1 ? cb : fb),
+                          hipMemcpyDeviceToDevice, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  return CS_OK;
+}
+}  // namespace
+
 extern "C" int cs_upload_state(cs_handle* h, const double* U, const double* ent, const double* Bxf,
                                const double* Byf, const double* Bzf) {
   CHECK_H(h);
@@ -2060,18 +2081,7 @@ extern "C" int cs_upload_state(cs_handle* h, const double* U, const double* ent,
   CS_HIP(hipMemcpyAsync(h->buf[0][2], Bxf, fb, hipMemcpyHostToDevice, h->stream));
   CS_HIP(hipMemcpyAsync(h->buf[0][3], Byf, fb, hipMemcpyHostToDevice, h->stream));
   CS_HIP(hipMemcpyAsync(h->buf[0][4], Bzf, fb, hipMemcpyHostToDevice, h->stream));
-  // the two copies of each panel-edge face are made one (negation-consistent)
-  // value here; from then on the face update keeps them so.  The ghost zones
-  // and faces are filled once: faces the updates never write (conducting
-  // walls) keep these ghost copies in both stage buffers.
-  int rc = launch_halo(h, 0, true, true, true);
-  if (rc) return rc;
-  // the scratch stage buffer starts as a copy so its ghost zones are defined
-  for (int q = 0; q < 5; ++q)
-    CS_HIP(hipMemcpyAsync(h->buf[1][q], h->buf[0][q], q == 0 ? cb * 8 : (q == 1 ? cb : fb),
-                          hipMemcpyDeviceToDevice, h->stream));
-  CS_HIP(hipStreamSynchronize(h->stream));
-  return CS_OK;
+  return finish_upload(h);
 }
 
 extern "C" int cs_download_state(cs_handle* h, double* U, double* ent, double* Bxf, double* Byf,
@@ -2087,6 +2097,66 @@ extern "C" int cs_download_state(cs_handle* h, double* U, double* ent, double* B
   if (Bzf) CS_HIP(hipMemcpyAsync(Bzf, h->buf[b][4], fb, hipMemcpyDeviceToHost, h->stream));
   CS_HIP(hipStreamSynchronize(h->stream));
   return CS_OK;
+}
+
+// ---------------------------------------------------------------- checkpoints
+// The dump format and its checks are in cs_dump.cpp; the state moves between
+// device memory and the file through one pinned staging buffer, a chunk at a
+// time, so no host copy of the whole state is made.
+namespace {
+constexpr uint64_t kDumpChunk = 8u << 20;  // float64 words per staged transfer (64 MiB)
+struct PinnedStage {
+  double* p = nullptr;
+  ~PinnedStage() {
+    if (p) hipHostFree(p);
+  }
+};
+}  // namespace
+
+extern "C" int cs_write_dump(cs_handle* h, const char* path, const cs_dump_meta* meta) {
+  CHECK_H(h);
+  if (!path || !meta) { set_error("cs_write_dump: null argument"); return CS_E_INVALID; }
+  PinnedStage st;
+  CS_HIP(hipHostMalloc((void**)&st.p, sizeof(double) * kDumpChunk, hipHostMallocDefault));
+  cs_dump_meta m = *meta;
+  CS_HIP(hipMemcpyAsync(&m.dtcur, h->dt, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  CS_HIP(hipStreamSynchronize(h->stream));
+  const int b = h->cur;
+  DumpIo io;
+  io.chunk = st.p;
+  io.chunk_n = kDumpChunk;
+  io.get = [&](int q, uint64_t off, uint64_t n, double* dst) -> int {
+    CS_HIP(hipMemcpyAsync(dst, h->buf[b][q] + off, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream));
+    CS_HIP(hipStreamSynchronize(h->stream));
+    return CS_OK;
+  };
+  return dump_write(path, &h->cfg, &m, io);
+}
+
+extern "C" int cs_read_dump(cs_handle* h, const char* path, cs_dump_meta* meta) {
+  CHECK_H(h);
+  if (!path) { set_error("cs_read_dump: null path"); return CS_E_INVALID; }
+  cs_dump_meta local;
+  if (!meta) meta = &local;
+  // header and grid first: a refused file leaves the handle's state as it was
+  DumpIo probe;
+  int rc = dump_read(path, &h->cfg, meta, probe);
+  if (rc) return rc;
+  PinnedStage st;
+  CS_HIP(hipHostMalloc((void**)&st.p, sizeof(double) * kDumpChunk, hipHostMallocDefault));
+  h->cur = 0;
+  h->vbuf = h->vghost = -1;
+  DumpIo io;
+  io.chunk = st.p;
+  io.chunk_n = kDumpChunk;
+  io.put = [&](int q, uint64_t off, uint64_t n, const double* src) -> int {
+    CS_HIP(hipMemcpyAsync(h->buf[0][q] + off, src, sizeof(double) * n, hipMemcpyHostToDevice, h->stream));
+    CS_HIP(hipStreamSynchronize(h->stream));
+    return CS_OK;
+  };
+  if ((rc = dump_read(path, &h->cfg, meta, io))) return rc;
+  if ((rc = finish_upload(h))) return rc;
+  return cs_set_dt(h, meta->dtcur);
 }
 
 extern "C" int cs_halo_exchange(cs_handle* h) {

@@ -19,6 +19,7 @@
 !   cs_update_stage        <- UPDATE_IMEX_RK_STEP_P1 + CLEANUP_AFTER_UPDATE (DRV:2782, 2815)
 !   cs_evaluate_timestep   <- EVALUATE_TIMESTEP                             (DRV:2887)
 !   cs_rk_stage            <- the whole per-patch loop body                (DRV:2007-3120)
+!   cs_write_dump / cs_read_dump <- riemann.in + WRITEDATA / READDATA      (DRV:1527-1666, 1783-1967)
 ! State arrays keep the driver's layout U_amr(1-g:N+g,1-g:N+g,1-g:nr+g,8,6),
 ! entropygr_amr(...,6), Bxf_amr(0-g:N+g,0-g:N+g,0-g:nr+g,6) (MOD:502-552).
 module cs_hip_iface
@@ -112,7 +113,36 @@ module cs_hip_iface
     type(c_ptr) :: EntFlux_x, EntFlux_y, EntFlux_z, Efx, Efy, Efz, flag_entprs
   end type
 
+  ! cs_dump_meta: riemann.in's time, dtcur, n_cc_components and dump number
+  ! (DRV:1527-1547, 1798-1802), the step count, the format version and the
+  ! configuration of a checkpoint
+  type, bind(C) :: cs_dump_meta
+    real(c_double) :: time, dtcur
+    integer(c_int64_t) :: filenum, step
+    integer(c_int32_t) :: n_cc_components, version
+    type(cs_config) :: cfg
+  end type
+
   interface
+    ! checkpoint / restart in place of riemann.in + WRITEDATA / READDATA
+    ! (DRV:1564-1605, 1845-1930); path: trim(name)//c_null_char
+    integer(c_int) function cs_write_dump(h, path, meta) bind(C, name="cs_write_dump")
+      import :: c_int, c_ptr, c_char, cs_dump_meta
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      type(cs_dump_meta), intent(in) :: meta
+    end function
+    integer(c_int) function cs_read_dump(h, path, meta) bind(C, name="cs_read_dump")
+      import :: c_int, c_ptr, c_char, cs_dump_meta
+      type(c_ptr), value :: h
+      character(kind=c_char), intent(in) :: path(*)
+      type(cs_dump_meta), intent(out) :: meta
+    end function
+    integer(c_int) function cs_dump_info(path, meta) bind(C, name="cs_dump_info")
+      import :: c_int, c_char, cs_dump_meta
+      character(kind=c_char), intent(in) :: path(*)
+      type(cs_dump_meta), intent(out) :: meta
+    end function
     integer(c_int) function cs_sync_shared_p1(p, stream) bind(C, name="cs_sync_shared_p1")
       import :: c_int, c_ptr, cs_p1_shell_sync
       type(cs_p1_shell_sync), intent(in) :: p

@@ -41,6 +41,8 @@ extern "C" {
 #define CS_E_NONPOSITIVE (-4) /* non-positive density/pressure after floors   */
 #define CS_E_NAN (-5)         /* NaN/Inf found in the state                   */
 #define CS_E_NOMEM (-6)       /* device allocation failed                     */
+#define CS_E_IO (-7)          /* a checkpoint file could not be opened, read or written */
+#define CS_E_FORMAT (-8)      /* a checkpoint file is not a dump of this format or is damaged */
 
 /* reconstruction */
 #define CS_RECON_PCM 0   /* piecewise constant (first order)               */
@@ -608,6 +610,48 @@ int cs_cleanup_after_update_p1(const cs_p1_patch* p, void* stream);
  * minimum over many patches can stay on the device.  Reads mask, scale,
  * gamma_sound; writes U, V, msonic. */
 int cs_evaluate_timestep_p1(const cs_p1_patch* p, double cfl, double dtcur, double* dt_new, void* stream);
+
+/* ------------------------------------------------------------ checkpoints */
+/* Checkpoint / restart.  Replaces the driver's riemann.in (time, dtcur,
+ * n_cc_components, dump number: read DRV:1527-1547 and checked against the
+ * dump DRV:1618-1647, written DRV:1798-1802) and its Silo READDATA /
+ * WRITEDATA of U, entropy and face B (DRV:1564-1605, DRV:1845-1930) with one
+ * versioned binary file: a 4096-byte header (format version, byte-order
+ * mark, the cs_config, the fields below, a checksum per section and one of
+ * the header), then U, ent, Bxf, Byf, Bzf in exactly the layouts of
+ * cs_upload_state above, ghosts included, each 4096-byte aligned.  Files
+ * are written as path.tmp and renamed into place, so a dump that exists is
+ * complete.  A reader refuses (CS_E_FORMAT) a bad magic, another version,
+ * another byte order, a header whose sections do not fit its grid or the
+ * file, and any checksum mismatch; it refuses (CS_E_INVALID) a grid (ntp,
+ * nr, nghost, radial mesh, radii) other than the caller's.  The scheme
+ * fields of the configuration may differ: a run may restart with another
+ * scheme on the same grid. */
+typedef struct cs_dump_meta {
+  double time;      /* simulation time (riemann.in's time)                */
+  double dtcur;     /* current time step (riemann.in's dtcur)             */
+  int64_t filenum;  /* dump number (riemann.in's filenum)                 */
+  int64_t step;     /* completed time steps                               */
+  int32_t n_cc_components; /* 8 (filled by the readers)                   */
+  int32_t version;  /* format version (filled by the readers)             */
+  cs_config cfg;    /* configuration the state was written with (readers) */
+} cs_dump_meta;
+/* host arrays (no GPU): write / read a state held in the upload layout */
+int cs_dump_write(const char* path, const cs_config* cfg, const cs_dump_meta* meta, const double* U,
+                  const double* ent, const double* Bxf, const double* Byf, const double* Bzf);
+int cs_dump_read(const char* path, const cs_config* cfg, cs_dump_meta* meta, double* U, double* ent,
+                 double* Bxf, double* Byf, double* Bzf);
+/* the header alone (checked as by cs_dump_read, no grid comparison) */
+int cs_dump_info(const char* path, cs_dump_meta* meta);
+/* a handle's state U^n straight from device memory through a pinned
+ * 64 MiB staging buffer (no host copy of the state); dtcur is the handle's
+ * current time step, meta->dtcur is not used */
+int cs_write_dump(cs_handle* h, const char* path, const cs_dump_meta* meta);
+/* into a handle, as cs_upload_state (exchange included), and the handle's
+ * time step set to the dump's dtcur; meta receives the header.  A file
+ * refused by its header or grid leaves the handle as it was; one whose
+ * section fails its checksum leaves the handle's state undefined. */
+int cs_read_dump(cs_handle* h, const char* path, cs_dump_meta* meta);
 
 int cs_last_error(char* buf, size_t len);
 int cs_device_info(char* buf, size_t len);

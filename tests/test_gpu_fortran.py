@@ -19,10 +19,11 @@ pytestmark = pytest.mark.gpu
 DEMO = os.path.join(os.path.dirname(cs.__file__), "cs_dropin_demo")
 
 
-def test_fortran_dropin_matches_python_path():
+def test_fortran_dropin_matches_python_path(tmp_path):
     if not os.path.exists(DEMO):
         pytest.skip("Fortran demo not built (no flang)")
-    r = subprocess.run([DEMO], capture_output=True, text=True, timeout=120)
+    dump = str(tmp_path / "demo.csd")
+    r = subprocess.run([DEMO, dump], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     m = re.search(r"rho\(1,1,1,panel 3\) =\s*(\S+)\s+mass drift =\s*(\S+)\s+divB =\s*(\S+)", r.stdout)
     assert m, r.stdout
@@ -43,6 +44,14 @@ def test_fortran_dropin_matches_python_path():
     rho_p = U[2, 0, g, g, g]          # panel 3, zone (1,1,1)
     assert abs(rho_f - rho_p) <= 1e-4 * abs(rho_p)   # printed with 5 significant digits
     assert abs(drift) < 1e-12 and divb < 1e-12
+    # the checkpoint the Fortran program wrote through cs_write_dump holds the
+    # state the Python path reaches, bit for bit
+    m = re.search(r"dump: version (\d+) step (\d+) ntp (\d+) dtcur\s*(\S+)", r.stdout)
+    assert m and m.group(1) == "1" and m.group(2) == "1" and m.group(3) == "16", r.stdout
+    saved, meta = cs.read_dump(dump, cfg)
+    for x, y in zip(saved.arrays(), got.arrays()):
+        assert np.array_equal(x, y)
+    assert abs(meta.dtcur - float(m.group(4))) <= 1e-15 * meta.dtcur and meta.filenum == 1
 
 
 # the arrays cs_p1_dropin reads, and those it writes back (out_<name>.bin)
