@@ -853,10 +853,12 @@ __device__ __forceinline__ void face_points(const Dev& D, int fam, int s, int i,
 // from HBM about once
 __global__ void __launch_bounds__(256) k_face_pts(Dev D, Buf6 in) {
   const LBlock wg = lblock();
-  const int s = wg.y, nI = D.N + 7;
-  const int64_t t = ltid(wg);
-  if (t >= (int64_t)nI * nI) return;
-  const int i = -3 + (int)(t % nI), j = -3 + (int)(t / nI);
+  const int s = wg.y, nI = D.N + 7, nti = (nI + 63) / 64;
+  // a workgroup's four waves take four neighbouring rows of 64 columns (the
+  // upper neighbours of a row's y-faces are the next wave's own zones)
+  const int i = -3 + 64 * (int)(wg.x % nti) + (int)(threadIdx.x & 63);
+  const int j = -3 + 4 * (int)(wg.x / nti) + (int)(threadIdx.x >> 6);
+  if (i > D.N + 3 || j > D.N + 3) return;
   const bool fx = j >= -2, fy = i >= -2, fz = i >= -2 && j >= -2;
   const int nq = 3 * (D.NR + 7);
 #pragma unroll 1
@@ -1208,10 +1210,12 @@ __device__ __forceinline__ void face_flux(const Dev& D, int fam, int s, int i, i
 __global__ void __launch_bounds__(128) k_flux(Dev D, double* F0, double* F1, double* F2, double* S0, double* S1,
                                               double* S2) {
   const LBlock wg = lblock();
-  const int s = wg.y, nI = D.N + 3;
-  const int64_t t = ltid(wg);
-  if (t >= (int64_t)nI * nI) return;
-  const int i = -1 + (int)(t % nI), j = -1 + (int)(t / nI);
+  const int s = wg.y, nI = D.N + 3, nti = (nI + 63) / 64;
+  // a workgroup's two waves take two neighbouring rows of 64 columns, so the
+  // upper neighbour a y-face reads is the other wave's own zone
+  const int i = -1 + 64 * (int)(wg.x % nti) + (int)(threadIdx.x & 63);
+  const int j = -1 + 2 * (int)(wg.x / nti) + (int)(threadIdx.x >> 6);
+  if (i > D.N + 1 || j > D.N + 1) return;
   const bool fx = j >= 0, fy = i >= 0, fz = i >= 0 && j >= 0;
   // one call site (the three families in turn), so the solver is inlined once
   const int nq = 3 * (D.NR + 3);
@@ -2065,16 +2069,16 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
   }
   k_clear_ent<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr, in, err);
   mark_end(S, st, CS_P1K_REPLAY);
-  const int64_t ncolp = (int64_t)(D.N + 7) * (D.N + 7);
+  const int64_t nfp = (int64_t)((D.N + 7 + 63) / 64) * ((D.N + 7 + 3) / 4);
   mark_begin(S, st);
-  k_face_pts<<<dim3(nb(ncolp, 256), 6), 256, 0, st>>>(D, in);
+  k_face_pts<<<dim3((unsigned)nfp, 6), 256, 0, st>>>(D, in);
   mark_end(S, st, CS_P1K_FACE_POINTS);
   mark_begin(S, st);
   k_clsq<<<dim3(nb(nall, 64), 6), 64, 0, st>>>(D, S->all, in);
   mark_end(S, st, CS_P1K_CLSQ);
-  const int64_t ncol = (int64_t)(D.N + 3) * (D.N + 3);
+  const int64_t nfl = (int64_t)((D.N + 3 + 63) / 64) * ((D.N + 3 + 1) / 2);
   mark_begin(S, st);
-  k_flux<<<dim3(nb(ncol, 128), 6), 128, 0, st>>>(D, F[0], F[1], F[2], Sp[0], Sp[1], Sp[2]);
+  k_flux<<<dim3((unsigned)nfl, 6), 128, 0, st>>>(D, F[0], F[1], F[2], Sp[0], Sp[1], Sp[2]);
   mark_end(S, st, CS_P1K_FLUX);
   const int64_t ned = std::max(S->ex.count(), std::max(S->ey.count(), S->ez.count()));
   mark_begin(S, st);
