@@ -847,27 +847,114 @@ __device__ __forceinline__ void face_points(const Dev& D, int fam, int s, int i,
 #pragma unroll
   for (int q = 0; q < 5; ++q) D.bq(s, fam, q)[fb] = q5[q] + bd;
 }
-// one column (i, j) of one sector per thread, marching up in k through the
-// faces of index (i, j, k) of the three families' ranges (x: i -3 .. N+3,
-// j, k -2 .. N+3, NR+3; y and z likewise), so each zone's B modes are read
-// from HBM about once
-__global__ void __launch_bounds__(256) k_face_pts(Dev D, Buf6 in) {
+// A zone's linear B in LDS (kBWords words per lane: the 3 x 4 modes, the
+// centroid, the reciprocal scale length) and its value at a point, with the
+// arithmetic of face_points.
+constexpr int kFpTI = 16, kFpTJ = 8;  // columns per workgroup along i and j
+constexpr int kBWords = 16;
+__device__ __forceinline__ void b_at(const double* w, const double* p, double* b) {
+  double r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r[c] = (p[c] - w[12 + c]) * w[15];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) b[c] = w[4 * c] + w[4 * c + 1] * r[0] + w[4 * c + 2] * r[1] + w[4 * c + 3] * r[2];
+}
+__device__ __forceinline__ void lds_b(const double (*Bs)[kFpTI * kFpTJ], int slot, const double* p, double* b) {
+  double w[kBWords];
+#pragma unroll
+  for (int q = 0; q < kBWords; ++q) w[q] = Bs[q][slot];
+  b_at(w, p, b);
+}
+// The face-point B of the three families' ranges (x: i -3 .. N+3, j -2 ..
+// N+3, k -2 .. NR+3; y likewise; z: i, j -2 .. N+3, k -3 .. NR+3), one
+// column (i, j) per thread marching up in k, a workgroup holding a 16 x 8
+// block of columns whose last column and row are the next blocks' first
+// ones: at each level every lane copies its zone's linear B (16 words) into
+// LDS once, the lanes inside the 15 x 7 computed columns take both sides of
+// their x- and y-faces from LDS and the lower side of their z-face from the
+// zone below's words, kept in registers from the previous level.  Tri-corner faces with one inactive side read the zones from
+// memory (face_points).
+__global__ void __launch_bounds__(kFpTI * kFpTJ) k_face_pts(Dev D, Buf6 in) {
+  __shared__ double Bs[kBWords][kFpTI * kFpTJ];
   const LBlock wg = lblock();
-  const int s = wg.y, nI = D.N + 7, nti = (nI + 63) / 64;
-  // a workgroup's four waves take four neighbouring rows of 64 columns (the
-  // upper neighbours of a row's y-faces are the next wave's own zones)
-  const int i = -3 + 64 * (int)(wg.x % nti) + (int)(threadIdx.x & 63);
-  const int j = -3 + 4 * (int)(wg.x / nti) + (int)(threadIdx.x >> 6);
-  if (i > D.N + 3 || j > D.N + 3) return;
-  const bool fx = j >= -2, fy = i >= -2, fz = i >= -2 && j >= -2;
-  const int nq = 3 * (D.NR + 7);
+  const int s = wg.y, N = D.N, NR = D.NR, nti = (N + 7 + kFpTI - 2) / (kFpTI - 1);
+  const int t = threadIdx.x, li = t % kFpTI, lj = t / kFpTI;
+  const int i = -3 + (kFpTI - 1) * (int)(wg.x % nti) + li;
+  const int j = -3 + (kFpTJ - 1) * (int)(wg.x / nti) + lj;
+  const bool have = i <= N + D.g && j <= N + D.g;  // the column's zones exist
+  const bool mine = have && li < kFpTI - 1 && lj < kFpTJ - 1 && i <= N + 3 && j <= N + 3;
+  const bool act = have && D.active(i, j);
+  const bool fx = mine && j >= -2, fy = mine && i >= -2, fz = mine && i >= -2 && j >= -2;
+  const bool actx = fx && D.active(i + 1, j), acty = fy && D.active(i, j + 1);
+  const bool fastx = act && actx, fasty = act && acty;
+  double below[kBWords];  // the words of the zone below the z-face about to be made
+#pragma unroll
+  for (int q = 0; q < kBWords; ++q) below[q] = 0.0;
+  // one flat loop over (level, family) with one call site per piece
+  const int nq = 3 * (NR + 8);
 #pragma unroll 1
   for (int q = 0; q < nq; ++q) {
     const int k = -3 + q / 3, fam = q % 3;
-    const bool on = fam == 0 ? (k >= -2 && fx) : fam == 1 ? (k >= -2 && fy) : fz;
     int ii = i, jj = j;  // opaque, as in k_flux
     asm volatile("" : "+v"(ii), "+v"(jj));
-    if (on) face_points(D, fam, s, ii, jj, k, in);
+    if (fam == 0) {
+      __syncthreads();  // the previous level's LDS words are no longer read
+      if (have) {
+        const int64_t z = D.zi(ii, jj, k);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+          for (int m = 0; m < 4; ++m) Bs[4 * c + m][t] = D.v(m, s, 5 + c, z);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Bs[12 + c][t] = D.cen(s, z, c);
+        Bs[15][t] = D.obl(s, z);
+      }
+      __syncthreads();
+    }
+    // the face of this family at this level: x and y (i, j, k), z (i, j, k - 1)
+    const bool on = fam == 0 ? (k >= -2 && k <= NR + 3 && fx)
+                             : fam == 1 ? (k >= -2 && k <= NR + 3 && fy) : (k >= -2 && fz);
+    if (on) {
+      const bool fast = fam == 0 ? fastx : fam == 1 ? fasty : act;
+      const bool some = fam == 0 ? (act || actx) : fam == 1 ? (act || acty) : act;
+      const int fk = fam == 2 ? k - 1 : k;
+      if (fast || !some) {
+        csg1::FaceG F;
+        csg1::face_of(D.G, fam, s, ii, jj, fk, F);
+        const int64_t fb = D.fci(ii, jj, fk);
+        double q5[5];
+#pragma unroll
+        for (int pt = 0; pt < 5; ++pt) {
+          double bl[3], br[3];
+          if (!some) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) bl[c] = br[c] = 0.0;
+          } else if (fam == 2) {
+            b_at(below, F.p[pt], bl);
+            lds_b(Bs, t, F.p[pt], br);
+          } else {
+            lds_b(Bs, t, F.p[pt], bl);
+            lds_b(Bs, t + (fam == 0 ? 1 : kFpTI), F.p[pt], br);
+          }
+          const double* n = fam == 2 ? F.n[pt] : F.n[0];
+          double sm[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) sm[c] = bl[c] + br[c];
+          q5[pt] = some ? 0.5 * (n[0] * sm[0] + n[1] * sm[1] + n[2] * sm[2]) : 0.0;
+        }
+        double bd = in.b(fam)[(int64_t)s * D.NFC + fb];
+#pragma unroll
+        for (int pt = 0; pt < 5; ++pt) bd = bd - F.bw[pt] * q5[pt];
+#pragma unroll
+        for (int pt = 0; pt < 5; ++pt) D.bq(s, fam, pt)[fb] = q5[pt] + bd;
+      } else {
+        face_points(D, fam, s, ii, jj, fk, in);  // a tri-corner face: one side re-pointed
+      }
+    }
+    if (fam == 2 && fz && act) {  // this level's zone is the next z-face's lower side
+#pragma unroll
+      for (int q = 0; q < kBWords; ++q) below[q] = Bs[q][t];
+    }
   }
 }
 
@@ -1062,26 +1149,13 @@ __global__ void __launch_bounds__(64) k_clsq(Dev D, Box b, Buf6 in) {
 // the area.  Tri-corner faces take the stencil neighbour as the inactive
 // side's zone (450...f:581-622).  With conducting walls the radial faces on
 // the walls see the inner zone's state mirrored about the face normal.
-__device__ __forceinline__ bool flux_range(const Dev& D, int fam, int i, int j, int k) {
-  return i >= (fam == 0 ? -1 : 0) && i <= D.N + 1 && j >= (fam == 1 ? -1 : 0) && j <= D.N + 1 &&
-         k >= (fam == 2 ? -1 : 0) && k <= D.NR + 1;
-}
-__device__ __forceinline__ void face_flux(const Dev& D, int fam, int s, int i, int j, int k, double* F0, double* F1,
-                                          double* F2, double* S0, double* S1, double* S2) {
-  int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1);
-  if (fam < 2) {
-    const bool al = D.active(li, lj), ar = D.active(ri, rj);
-    if (!al && ar) D.nbr(ri, rj, fam == 0 ? 1 : 3, li, lj);
-    else if (al && !ar) D.nbr(li, lj, fam == 0 ? 0 : 2, ri, rj);
-    else if (!al && !ar) return;
-  } else if (!D.active(i, j)) {
-    return;
-  }
-  const int64_t zl = D.zi(li, lj, k), zr = D.zi(ri, rj, fam == 2 ? k + 1 : k), fb = D.fci(i, j, k);
-  csg1::FaceG F;
-  csg1::face_of(D.G, fam, s, i, j, k, F);
+// The face frame (450...f:890-1030): A the unit normal; for x-faces C the
+// radial direction of the centroid and B = C x A, for y-faces B the radial
+// direction and C = A x B, for z-faces B along the face's i diagonal made
+// orthogonal to A, C = A x B, B = C x A.
+__device__ __forceinline__ void face_frame(const Dev& D, int fam, int s, int i, int j, int k, const csg1::FaceG& F,
+                                           double* A, double* B, double* Cv) {
   const double sm = D.smallnum;
-  double A[3], B[3], Cv[3];
   if (fam == 0) {
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -1135,56 +1209,94 @@ __device__ __forceinline__ void face_flux(const Dev& D, int fam, int s, int i, i
     B[1] = Cv[2] * A[0] - Cv[0] * A[2];
     B[2] = Cv[0] * A[1] - Cv[1] * A[0];
   }
-  double VL[8], VR[8], eL, eR;
-  {
-    const double ol = D.obl(s, zl), orr = D.obl(s, zr);
-    double rl[3], rr[3];
+}
+
+// A zone's linear reconstruction: the four modes of the nine fields, its
+// centroid and its reciprocal scale length; state_at evaluates it at a
+// point (the 8 primitives and the entropy).
+struct ZoneSlopes {
+  double m[4][9], c[3], o;
+};
+__device__ __forceinline__ void load_slopes(const Dev& D, int s, int64_t z, ZoneSlopes& Z) {
+  Z.o = D.obl(s, z);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      rl[c] = (F.p[0][c] - D.cen(s, zl, c)) * ol;
-      rr[c] = (F.p[0][c] - D.cen(s, zr, c)) * orr;
-    }
+  for (int c = 0; c < 3; ++c) Z.c[c] = D.cen(s, z, c);
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      VL[c] = D.v(0, s, c, zl) + D.v(1, s, c, zl) * rl[0] + D.v(2, s, c, zl) * rl[1] + D.v(3, s, c, zl) * rl[2];
-      VR[c] = D.v(0, s, c, zr) + D.v(1, s, c, zr) * rr[0] + D.v(2, s, c, zr) * rr[1] + D.v(3, s, c, zr) * rr[2];
-    }
-    eL = D.v(0, s, 8, zl) + D.v(1, s, 8, zl) * rl[0] + D.v(2, s, 8, zl) * rl[1] + D.v(3, s, 8, zl) * rl[2];
-    eR = D.v(0, s, 8, zr) + D.v(1, s, 8, zr) * rr[0] + D.v(2, s, 8, zr) * rr[1] + D.v(3, s, 8, zr) * rr[2];
+  for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int o = 1; o <= 5; o += 4) {
-      const double la = A[0] * VL[o] + A[1] * VL[o + 1] + A[2] * VL[o + 2];
-      const double lb = B[0] * VL[o] + B[1] * VL[o + 1] + B[2] * VL[o + 2];
-      const double lc = Cv[0] * VL[o] + Cv[1] * VL[o + 1] + Cv[2] * VL[o + 2];
-      VL[o] = la;
-      VL[o + 1] = lb;
-      VL[o + 2] = lc;
-      const double ra = A[0] * VR[o] + A[1] * VR[o + 1] + A[2] * VR[o + 2];
-      const double rb = B[0] * VR[o] + B[1] * VR[o + 1] + B[2] * VR[o + 2];
-      const double rc = Cv[0] * VR[o] + Cv[1] * VR[o + 1] + Cv[2] * VR[o + 2];
-      VR[o] = ra;
-      VR[o + 1] = rb;
-      VR[o + 2] = rc;
-    }
+    for (int c = 0; c < 9; ++c) Z.m[m][c] = D.v(m, s, c, z);
+}
+__device__ __forceinline__ void state_at(const ZoneSlopes& Z, const double* p, double* w) {
+  double r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r[c] = (p[c] - Z.c[c]) * Z.o;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) w[c] = Z.m[0][c] + Z.m[1][c] * r[0] + Z.m[2][c] * r[1] + Z.m[3][c] * r[2];
+}
+
+// zone z's linear fields at point p, straight from memory (the arithmetic of
+// state_at)
+__device__ __forceinline__ void zone_state(const Dev& D, int s, int64_t z, const double* p, double* w) {
+  const double o = D.obl(s, z);
+  double r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r[c] = (p[c] - D.cen(s, z, c)) * o;
+#pragma unroll
+  for (int c = 0; c < 9; ++c) w[c] = D.v(0, s, c, z) + D.v(1, s, c, z) * r[0] + D.v(2, s, c, z) * r[1] + D.v(3, s, c, z) * r[2];
+}
+
+// The two states of a lateral face from memory: the left and right zones
+// (a tri-corner face takes the stencil neighbour as its inactive side's
+// zone, 450...f:581-622) evaluated at the face's first point.
+__device__ __forceinline__ void lateral_states(const Dev& D, int fam, int s, int i, int j, int k, const double* p,
+                                               double* VL, double* VR) {
+  int li = i, lj = j, ri = i + (fam == 0), rj = j + (fam == 1);
+  const bool al = D.active(li, lj), ar = D.active(ri, rj);
+  if (!al && ar) D.nbr(ri, rj, fam == 0 ? 1 : 3, li, lj);
+  else if (al && !ar) D.nbr(li, lj, fam == 0 ? 0 : 2, ri, rj);
+  zone_state(D, s, D.zi(li, lj, k), p, VL);
+  zone_state(D, s, D.zi(ri, rj, k), p, VR);
+}
+
+// One face from its two states at the face's first point: rotated into the
+// face frame, the facial B in the normal slot, the wall mirror for radial
+// faces on conducting walls, HLLD or HLL, the resolved flux rotated back to
+// the global frame and times the area (450...f:890-1030), with the entropy
+// flux as the ninth component and the wave speeds S_L, S_R.
+__device__ __forceinline__ void face_solve(const Dev& D, int fam, int s, int k, int64_t fb, double area,
+                                           const double* A, const double* B, const double* Cv, double* VL,
+                                           double* VR, double* F0, double* F1, double* F2, double* S0, double* S1,
+                                           double* S2) {
+#pragma unroll
+  for (int o = 1; o <= 5; o += 4) {
+    const double la = A[0] * VL[o] + A[1] * VL[o + 1] + A[2] * VL[o + 2];
+    const double lb = B[0] * VL[o] + B[1] * VL[o + 1] + B[2] * VL[o + 2];
+    const double lc = Cv[0] * VL[o] + Cv[1] * VL[o + 1] + Cv[2] * VL[o + 2];
+    VL[o] = la;
+    VL[o + 1] = lb;
+    VL[o + 2] = lc;
+    const double ra = A[0] * VR[o] + A[1] * VR[o + 1] + A[2] * VR[o + 2];
+    const double rb = B[0] * VR[o] + B[1] * VR[o + 1] + B[2] * VR[o + 2];
+    const double rc = Cv[0] * VR[o] + Cv[1] * VR[o + 1] + Cv[2] * VR[o + 2];
+    VR[o] = ra;
+    VR[o + 1] = rb;
+    VR[o + 2] = rc;
   }
   if (D.wall && fam == 2 && (k == 0 || k == D.NR)) {
     // the wall: the inner zone's state mirrored (normal velocity and normal B
     // reversed), so no mass crosses the wall and its normal B is zero
     const bool lo = k == 0;
 #pragma unroll
-    for (int c = 0; c < 8; ++c) {
-      const double m = (c == 1 || c == 5) ? -1.0 : 1.0;
-      if (lo) VL[c] = m * VR[c];
-      else VR[c] = m * VL[c];
+    for (int c = 0; c < 9; ++c) {
+      const double m = (c == 1 || c == 5) ? -1.0 : 1.0, l = VL[c], r = VR[c];
+      VL[c] = lo ? m * r : l;
+      VR[c] = lo ? r : m * l;
     }
-    if (lo) eL = eR;
-    else eR = eL;
   }
   const Gas<double> gs = D.gas();
   double Fs[8], eF, SL, SR;
-  if (D.riemann == CS_RS_HLLD) hlld(gs, VL, VR, eL, eR, Fs, eF, SL, SR);
-  else hll(gs, VL, VR, eL, eR, Fs, eF, SL, SR);
-  const double ar = F.area;
+  if (D.riemann == CS_RS_HLLD) hlld(gs, VL, VR, VL[8], VR[8], Fs, eF, SL, SR);
+  else hll(gs, VL, VR, VL[8], VR[8], Fs, eF, SL, SR);
   double* Fo = (fam == 0 ? F0 : fam == 1 ? F1 : F2) + (int64_t)s * 9 * D.NFC + fb;
 #pragma unroll
   for (int c = 0; c < 8; ++c) {
@@ -1195,39 +1307,106 @@ __device__ __forceinline__ void face_flux(const Dev& D, int fam, int s, int i, i
       const int o = c < 4 ? 1 : 5, q = c - o;
       v = A[q] * Fs[o] + B[q] * Fs[o + 1] + Cv[q] * Fs[o + 2];
     }
-    Fo[(int64_t)c * D.NFC] = v * ar;
+    Fo[(int64_t)c * D.NFC] = v * area;
   }
-  Fo[(int64_t)8 * D.NFC] = eF * ar;
+  Fo[(int64_t)8 * D.NFC] = eF * area;
   double* So = (fam == 0 ? S0 : fam == 1 ? S1 : S2) + (int64_t)s * 2 * D.NFC + fb;
   So[0] = SL;
   So[D.NFC] = SR;
 }
-// One column (i, j) of one sector per thread, marching up in k: at each
-// level the x-, y- and z-faces of index (i, j, k) in their families'
-// ranges, so a zone's 36 slope words are read from HBM about once (the
-// faces of one zone and its upper neighbour follow each other in time and
-// hit the caches) instead of once per family.
-__global__ void __launch_bounds__(128) k_flux(Dev D, double* F0, double* F1, double* F2, double* S0, double* S1,
-                                              double* S2) {
+
+// The face fluxes of a sector (the x-faces i -1 .. N+1, j 0 .. N+1, k 0 ..
+// NR+1; y likewise; z-faces i, j 0 .. N+1, k -1 .. NR+1), one column (i, j)
+// per thread marching up in k.  A workgroup holds a 16 x 8 block of
+// columns whose last column and last row are the next blocks' first ones
+// (blocks step by 15 and 7): at each level every lane copies its zone's
+// slopes, centroid and scale (40 words) from HBM into LDS, and the lanes
+// inside the block's 15 x 7 columns evaluate both states of their x- and
+// y-faces from their own and their neighbour's LDS words; the lower state of
+// a z-face is the column's previous top state (kept in registers).  Only
+// tri-corner faces whose neighbour is an inactive column read the zones from
+// memory.  So a zone's words come from HBM about 1.2 times per launch
+// (16 x 8 loaded for 15 x 7 computed), not once per face side.
+constexpr int kFluxTI = 16, kFluxTJ = 8;  // columns per workgroup along i and j
+constexpr int kSlopeWords = 40;           // 4 x 9 modes, the centroid, the reciprocal scale length
+__device__ __forceinline__ void lds_state(const double (*Zs)[kFluxTI * kFluxTJ], int slot, const double* p,
+                                          double* w) {
+  double r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r[c] = (p[c] - Zs[36 + c][slot]) * Zs[39][slot];
+#pragma unroll
+  for (int c = 0; c < 9; ++c)
+    w[c] = Zs[c][slot] + Zs[9 + c][slot] * r[0] + Zs[18 + c][slot] * r[1] + Zs[27 + c][slot] * r[2];
+}
+__global__ void __launch_bounds__(kFluxTI * kFluxTJ) k_flux(Dev D, double* F0, double* F1, double* F2, double* S0,
+                                                           double* S1, double* S2) {
+  __shared__ double Zs[kSlopeWords][kFluxTI * kFluxTJ];
   const LBlock wg = lblock();
-  const int s = wg.y, nI = D.N + 3, nti = (nI + 63) / 64;
-  // a workgroup's two waves take two neighbouring rows of 64 columns, so the
-  // upper neighbour a y-face reads is the other wave's own zone
-  const int i = -1 + 64 * (int)(wg.x % nti) + (int)(threadIdx.x & 63);
-  const int j = -1 + 2 * (int)(wg.x / nti) + (int)(threadIdx.x >> 6);
-  if (i > D.N + 1 || j > D.N + 1) return;
-  const bool fx = j >= 0, fy = i >= 0, fz = i >= 0 && j >= 0;
-  // one call site (the three families in turn), so the solver is inlined once
-  const int nq = 3 * (D.NR + 3);
+  const int s = wg.y, N = D.N, NR = D.NR, nti = (N + 3 + kFluxTI - 2) / (kFluxTI - 1);
+  const int t = threadIdx.x, li = t % kFluxTI, lj = t / kFluxTI;
+  const int i = -1 + (kFluxTI - 1) * (int)(wg.x % nti) + li;
+  const int j = -1 + (kFluxTJ - 1) * (int)(wg.x / nti) + lj;
+  const bool have = i <= N + 2 && j <= N + 2;  // the column's zones exist
+  const bool mine = have && li < kFluxTI - 1 && lj < kFluxTJ - 1;  // the block computes this column's faces
+  const bool act = have && D.active(i, j);
+  const bool inx = mine && i <= N + 1 && j >= 0 && j <= N + 1, iny = mine && j <= N + 1 && i >= 0 && i <= N + 1;
+  const bool actx = inx && D.active(i + 1, j), acty = iny && D.active(i, j + 1);
+  const bool onx = inx && (act || actx), ony = iny && (act || acty);
+  const bool fastx = act && actx, fasty = act && acty;
+  const bool onz = mine && act && i >= 0 && i <= N + 1 && j >= 0 && j <= N + 1;
+  double zb[9];  // the column's state at the bottom point of the next z-face (the previous level's top)
+#pragma unroll
+  for (int c = 0; c < 9; ++c) zb[c] = 0.0;
+  // one flat loop over (level, family): a single call site, so the solver is
+  // inlined once (nested loops cost a second wave of registers)
+  const int nq = 3 * (NR + 4);
 #pragma unroll 1
   for (int q = 0; q < nq; ++q) {
     const int k = -1 + q / 3, fam = q % 3;
-    const bool on = fam == 0 ? (k >= 0 && fx) : fam == 1 ? (k >= 0 && fy) : fz;
     // the column's indices opaque to the optimiser: the k-invariant table
     // loads stay in the loop body instead of being hoisted and held live
     int ii = i, jj = j;
     asm volatile("" : "+v"(ii), "+v"(jj));
-    if (on) face_flux(D, fam, s, ii, jj, k, F0, F1, F2, S0, S1, S2);
+    if (fam == 0) {
+      __syncthreads();  // the previous level's LDS words are no longer read
+      if (have) {
+        const int64_t z = D.zi(ii, jj, k);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int c = 0; c < 9; ++c) Zs[9 * m + c][t] = D.v(m, s, c, z);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) Zs[36 + c][t] = D.cen(s, z, c);
+        Zs[39][t] = D.obl(s, z);
+      }
+      __syncthreads();
+    }
+    const bool on = fam == 0 ? (k >= 0 && k <= NR + 1 && onx)
+                             : fam == 1 ? (k >= 0 && k <= NR + 1 && ony) : (k >= 0 && onz);
+    if (on) {
+      const int fk = fam == 2 ? k - 1 : k;
+      csg1::FaceG F;
+      csg1::face_of(D.G, fam, s, ii, jj, fk, F);
+      double A[3], B[3], Cv[3];
+      face_frame(D, fam, s, ii, jj, fk, F, A, B, Cv);
+      double VL[9], VR[9];
+      if (fam == 2) {
+#pragma unroll
+        for (int c = 0; c < 9; ++c) VL[c] = zb[c];
+        lds_state(Zs, t, F.p[0], VR);
+      } else if (fam == 0 ? fastx : fasty) {
+        lds_state(Zs, t, F.p[0], VL);
+        lds_state(Zs, t + (fam == 0 ? 1 : kFluxTI), F.p[0], VR);
+      } else {
+        lateral_states(D, fam, s, ii, jj, k, F.p[0], VL, VR);
+      }
+      face_solve(D, fam, s, fk, D.fci(ii, jj, fk), F.area, A, B, Cv, VL, VR, F0, F1, F2, S0, S1, S2);
+    }
+    if (fam == 2 && onz) {  // the top state of this level's zone: the bottom of the next z-face
+      double p[3];
+      csg1::face_p0(D.G, 2, s, ii, jj, k, p);
+      lds_state(Zs, t, p, zb);
+    }
   }
 }
 
@@ -2069,16 +2248,16 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
   }
   k_clear_ent<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr, in, err);
   mark_end(S, st, CS_P1K_REPLAY);
-  const int64_t nfp = (int64_t)((D.N + 7 + 63) / 64) * ((D.N + 7 + 3) / 4);
+  const int64_t nfpi = (D.N + 7 + kFpTI - 2) / (kFpTI - 1), nfpj = (D.N + 7 + kFpTJ - 2) / (kFpTJ - 1);
   mark_begin(S, st);
-  k_face_pts<<<dim3((unsigned)nfp, 6), 256, 0, st>>>(D, in);
+  k_face_pts<<<dim3((unsigned)(nfpi * nfpj), 6), kFpTI * kFpTJ, 0, st>>>(D, in);
   mark_end(S, st, CS_P1K_FACE_POINTS);
   mark_begin(S, st);
   k_clsq<<<dim3(nb(nall, 64), 6), 64, 0, st>>>(D, S->all, in);
   mark_end(S, st, CS_P1K_CLSQ);
-  const int64_t nfl = (int64_t)((D.N + 3 + 63) / 64) * ((D.N + 3 + 1) / 2);
+  const int64_t nfi = (D.N + 3 + kFluxTI - 2) / (kFluxTI - 1), nfj = (D.N + 3 + kFluxTJ - 2) / (kFluxTJ - 1);
   mark_begin(S, st);
-  k_flux<<<dim3((unsigned)nfl, 6), 128, 0, st>>>(D, F[0], F[1], F[2], Sp[0], Sp[1], Sp[2]);
+  k_flux<<<dim3((unsigned)(nfi * nfj), 6), kFluxTI * kFluxTJ, 0, st>>>(D, F[0], F[1], F[2], Sp[0], Sp[1], Sp[2]);
   mark_end(S, st, CS_P1K_FLUX);
   const int64_t ned = std::max(S->ex.count(), std::max(S->ey.count(), S->ez.count()));
   mark_begin(S, st);

@@ -177,6 +177,20 @@ CS_HD void face_of(const P1Geo& G, int fam, int s, int i, int j, int k, FaceG& F
   else if (fam == 1) face_y(G, s, i, j, k, F);
   else face_z(G, s, i, j, k, F);
 }
+// the first point alone, computed as face_of computes F.p[0]
+CS_HD void face_p0(const P1Geo& G, int fam, int s, int i, int j, int k, double* p) {
+  if (fam == 2) {
+    const int64_t q = G.iz(i, j), st = 6 * G.tz;
+    const double* T = G.TZ + (int64_t)s * G.tz;
+    const double r = G.r(k);
+    for (int c = 0; c < 3; ++c) p[c] = r * T[(TZ_U + c) * st + q];
+    return;
+  }
+  const double* T = (fam == 0 ? G.TX : G.TY) + (int64_t)s * G.tl;
+  const int64_t q = fam == 0 ? G.ix(i, j) : G.iy(i, j), st = 6 * G.tl;
+  const double rm = 0.5 * (G.r(k - 1) + G.r(k));
+  for (int c = 0; c < 3; ++c) p[c] = rm * T[(TL_M + c) * st + q];
+}
 
 // Only the normal (lateral faces) or the centroid direction (radial faces):
 // the pieces the flattener and the cleanup read.
