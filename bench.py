@@ -97,27 +97,33 @@ def describe(cs, cfg, problem):
     return f"{rec} + {rs} + 2-D HLL/LLF EMF + CT, {tint}, {ic}"
 
 
+TRAFFIC_PROFILES = ("c2_surface256_hbm_traffic.txt", "c5_shell512x128_p1adp_counters.md")
+
+
 def measured_traffic(kernel, prec, workload):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    """HBM bytes per launch of `kernel` from a committed rocprofv3 --pmc
     summary (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter read
-    for a copy kernel of known size), or None when it holds no such line."""
-    path = os.path.join(ROOT, "profiles", "c2_surface256_hbm_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    cal, rec = {}, None
-    for line in open(path):
-        f = line.split()
-        if not f or f[0].startswith("#"):
+    for a copy kernel of known size in the same summary), or None when none
+    holds such a line."""
+    for name in TRAFFIC_PROFILES:
+        path = os.path.join(ROOT, "profiles", name)
+        if not os.path.exists(path):
             continue
-        if f[0].startswith("calibration_"):
-            cal[f[0]] = float(f[1])
-        elif f[0] == "kernel" and f[1:4] == [kernel, prec, workload]:
-            rec = {f[4]: float(f[5]), f[6]: float(f[7])}
-    if rec is None or len(cal) != 3:
-        return None
-    copy = cal["calibration_copy_bytes"]
-    return (rec["fetch_kib"] * 1024 * copy / (cal["calibration_fetch_kib"] * 1024) +
-            rec["write_kib"] * 1024 * copy / (cal["calibration_write_kib"] * 1024))
+        cal, rec = {}, None
+        for line in open(path):
+            f = line.split()
+            if not f or f[0].startswith("#"):
+                continue
+            if f[0].startswith("calibration_"):
+                cal[f[0]] = float(f[1])
+            elif f[0] == "kernel" and f[1:4] == [kernel, prec, workload]:
+                rec = {f[4]: float(f[5]), f[6]: float(f[7])}
+        if rec is None or len(cal) != 3:
+            continue
+        copy = cal["calibration_copy_bytes"]
+        return (rec["fetch_kib"] * 1024 * copy / (cal["calibration_fetch_kib"] * 1024) +
+                rec["write_kib"] * 1024 * copy / (cal["calibration_write_kib"] * 1024))
+    return None
 
 
 _STATES = {}
@@ -836,7 +842,8 @@ def bench_p1adp(cs, args, cpu):
     kd = kern[dom]
     alg = P1K_BYTES[dom] * units[dom]
     roof = {"bound": "hbm", "kernel": dom, "achieved": kd["alg_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": kd["alg_GBps"] / HBM_PEAK_GBS, "traffic": None, "alg_bytes_per_launch": alg,
+            "frac": kd["alg_GBps"] / HBM_PEAK_GBS, "traffic": measured_traffic(dom, "fp64", name),
+            "alg_bytes_per_launch": alg,
             "alg_bytes_per_unit": P1K_BYTES[dom], "units_per_launch": units[dom],
             "avg_launch_us": kd["ms"] * 1e3,
             "launch_time": f"hipEvent pair around each launch group on the library's stream, "

@@ -20,6 +20,8 @@
 !   cs_evaluate_timestep   <- EVALUATE_TIMESTEP                             (DRV:2887)
 !   cs_rk_stage            <- the whole per-patch loop body                (DRV:2007-3120)
 !   cs_write_dump / cs_read_dump <- riemann.in + WRITEDATA / READDATA      (DRV:1527-1666, 1783-1967)
+!   cs_init_problem_p1     <- INIT_PATCH_CUBE_SPHERE (820...f:303) on the device (DRV:1386)
+!   cs_add_boundary_p1     <- ADD_BOUNDARY (825...f:320) on the device (DRV:2304)
 ! State arrays keep the driver's layout U_amr(1-g:N+g,1-g:N+g,1-g:nr+g,8,6),
 ! entropygr_amr(...,6), Bxf_amr(0-g:N+g,0-g:N+g,0-g:nr+g,6) (MOD:502-552).
 module cs_hip_iface
@@ -142,6 +144,23 @@ module cs_hip_iface
       import :: c_int, c_char, cs_dump_meta
       character(kind=c_char), intent(in) :: path(*)
       type(cs_dump_meta), intent(out) :: meta
+    end function
+    ! INIT_PATCH_CUBE_SPHERE (820...f:303) and ADD_BOUNDARY (825...f:320) with
+    ! PROBLEM_GENERATOR_U_A on a CS_RECON_P1ADP handle's device state
+    integer(c_int) function cs_init_problem_p1(h, problem, seed, time) bind(C, name="cs_init_problem_p1")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: problem
+      integer(c_int64_t), value :: seed
+      real(c_double), value :: time
+    end function
+    integer(c_int) function cs_add_boundary_p1(h, ibdy_inner, ibdy_outer, problem, seed, time) &
+        bind(C, name="cs_add_boundary_p1")
+      import :: c_int, c_ptr, c_int32_t, c_int64_t, c_double
+      type(c_ptr), value :: h
+      integer(c_int32_t), value :: ibdy_inner, ibdy_outer, problem
+      integer(c_int64_t), value :: seed
+      real(c_double), value :: time
     end function
     integer(c_int) function cs_sync_shared_p1(p, stream) bind(C, name="cs_sync_shared_p1")
       import :: c_int, c_ptr, cs_p1_shell_sync
