@@ -3,7 +3,9 @@
 the oracle (or_problem_generator_u_a, or_init_patch_cube_sphere): bit-exact
 with the reference's routines built from source (problems 0 and 2), and the
 closed forms the two problems of this build without a reference counterpart
-(102 surface blast, 103 hydrostatic shell) must satisfy."""
+(102 surface blast, 103 hydrostatic shell) must satisfy.  The coordinate
+fields of problem 0 also pin the ghost exchange (330-333) on every
+component and face family against geometry."""
 import numpy as np
 import pytest
 
@@ -154,3 +156,92 @@ def test_hydrostatic_density_noise():
     ratio = a[act] / b[act]
     assert 0.005 < np.std(np.log(ratio)) / np.sqrt(2.0) < 0.015
     assert abs(np.mean(np.log(ratio))) < 2e-3
+
+
+def closed_form_exchange_check(cfg, exchanged, full):
+    """Compare the lateral ghost strips of an exchanged state with the same
+    zones and faces filled directly by the coordinate fields of problem 0
+    (801...f:105-229: U = (|x|, x, y, z, |z|, x, -2y, z), A = (yz, 2zx, 3xy),
+    so face B is the flux of curl A = (x, -2y, z)).  A ghost zone or face
+    occupies the place of the neighbouring sector's zone or face it copies,
+    so after the exchange it must hold its own closed-form value: every
+    component, every face-B swap and sign flip of the exchange is checked
+    against geometry.  Corner blocks (no sector owns them) are left out.
+    Returns the largest relative deviations (zones, faces) and the number of
+    entries compared."""
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    nz, nf, nzr, nfr = N + 2 * g, N + 2 * g + 1, NR + 2 * g, NR + 2 * g + 1
+    zi = np.arange(nz) - g + 1           # zone index of a zone-array position
+    fi = np.arange(nf) - g               # index of a face-array position (every dimension)
+    zg = (zi < 1) | (zi > N)
+    strip = (zg[:, None] ^ zg[None, :])  # [j, i]: one index a ghost, not both
+    shapes = [(6, 8, nzr, nz, nz), (6, nzr, nz, nz), (6, nfr, nf, nf), (6, nfr, nf, nf), (6, nfr, nf, nf)]
+    E = [np.asarray(a).reshape(sh) for a, sh in zip(exchanged, shapes)]
+    F = [np.asarray(a).reshape(sh) for a, sh in zip(full, shapes)]
+    ks = slice(g, g + NR)
+    dz, n = 0.0, 0
+    for c in range(8):
+        a, b = E[0][:, c, ks][..., strip], F[0][:, c, ks][..., strip]
+        assert np.isfinite(b).all() and np.isfinite(a).all(), c
+        dz = max(dz, float(np.abs(a - b).max()) / float(np.abs(b).max()))
+        n += a.size
+    zone_j = (fi >= 1 - g) & (fi <= N + g)   # face arrays are one wider than the zone range
+    zfg = (fi < 1) | (fi > N)
+    ffg = (fi < 0) | (fi > N)
+    masks = {2: zone_j[:, None] & (zfg[:, None] ^ ffg[None, :]),          # x-faces: j zone, i face
+             3: zone_j[None, :] & (ffg[:, None] ^ zfg[None, :]),          # y-faces: j face, i zone
+             4: zone_j[:, None] & zone_j[None, :] & (zfg[:, None] ^ zfg[None, :])}  # z-faces
+    kz = {2: slice(g + 1, g + NR + 1), 3: slice(g + 1, g + NR + 1), 4: slice(g, g + NR + 1)}
+    # (faces of the inactive corner columns have no area: the generator
+    # leaves them NaN, and no strip entry is one of them)
+    bscale = max(np.nanmax(np.abs(F[q])) for q in (2, 3, 4))
+    df = 0.0
+    for q, m in masks.items():
+        a, b = E[q][:, kz[q]][..., m], F[q][:, kz[q]][..., m]
+        assert np.isfinite(b).all() and np.isfinite(a).all(), q
+        df = max(df, float(np.abs(a - b).max()) / bscale)
+        n += a.size
+    return dz, df, n
+
+
+def _poison_lateral_ghosts(cfg, arrays):
+    """NaN in every lateral ghost zone and every face outside the owned range,
+    so anything the exchange does not write stays visible."""
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    nz, nf, nzr, nfr = N + 2 * g, N + 2 * g + 1, NR + 2 * g, NR + 2 * g + 1
+    zi = np.arange(nz) - g + 1
+    fi = np.arange(nf) - g
+    oz = (zi >= 1) & (zi <= N)
+    of = (fi >= 0) & (fi <= N)
+    ozf = (fi >= 1) & (fi <= N)
+    U, ent, Bx, By, Bz = (np.array(a, copy=True) for a in arrays)
+    U.reshape(6, 8, nzr, nz, nz)[..., ~(oz[:, None] & oz[None, :])] = np.nan
+    ent.reshape(6, nzr, nz, nz)[..., ~(oz[:, None] & oz[None, :])] = np.nan
+    Bx.reshape(6, nfr, nf, nf)[..., ~(ozf[:, None] & of[None, :])] = np.nan
+    By.reshape(6, nfr, nf, nf)[..., ~(of[:, None] & ozf[None, :])] = np.nan
+    Bz.reshape(6, nfr, nf, nf)[..., ~(ozf[:, None] & ozf[None, :])] = np.nan
+    return U, ent, Bx, By, Bz
+
+
+@pytest.mark.parametrize("N,NR", [(8, 3), (12, 2)])
+def test_exchange_of_closed_form_fields_every_component(N, NR):
+    """The oracle's geometric exchange against the closed forms of problem 0
+    on every component of U and all three face-B families (a transposed
+    strip, a missed sign flip or swap of the face-B copies, a wrong
+    component would each fail it by O(1))."""
+    cfg = cs.make_config(N, NR, 4, cs.RECON_P1ADP, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
+                         cs.RMESH_UNIFORM, 1.0, 1.5)
+    full = orc.init_problem_p1(cfg, 0)
+    st = cs.HostState(cfg)
+    for a, b in zip(st.arrays(), _poison_lateral_ghosts(cfg, full)):
+        a[:] = b
+    orc.Grid(cfg).halo(st)
+    dz, df, n = closed_form_exchange_check(cfg, st.arrays(), full)
+    assert n > 0
+    # zone means of the same Gauss points summed in another order; face
+    # circulations of O(1) potentials over O(h) edges
+    assert dz < 1e-14 and df < 1e-12, (dz, df)
+    # the same comparison fails for a planted sign error in one face family
+    bad = [a.copy() for a in st.arrays()]
+    bad[3] *= -1.0
+    assert closed_form_exchange_check(cfg, bad, full)[1] > 0.1

@@ -85,3 +85,29 @@ def test_steps_from_the_device_initial_state_keep_div_b():
     finally:
         h.close()
     assert d0 < 1e-13 and d1 < 1e-13, (d0, d1)
+
+
+@pytest.mark.parametrize("N,NR", [(8, 3), (12, 2)])
+def test_device_exchange_of_closed_form_fields(N, NR):
+    """The device exchange (the one every upload and every P1 stage runs)
+    against the closed forms of problem 0 built on the device: after an
+    upload whose lateral ghosts are NaN, every ghost strip zone and face
+    holds its own closed-form value (every component, every face-B swap and
+    sign flip; tests/test_cpu_problems.py holds the comparison)."""
+    from test_cpu_problems import _poison_lateral_ghosts, closed_form_exchange_check
+    cfg = cs.make_config(N, NR, 4, cs.RECON_P1ADP, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
+                         cs.RMESH_UNIFORM, 1.0, 1.5)
+    print(f"case N={N} NR={NR}", flush=True)
+    h = cs.CubeSphere(cfg)
+    try:
+        h.init_problem_p1(0)
+        full = h.download()
+        st = cs.HostState(cfg)
+        for a, b in zip(st.arrays(), _poison_lateral_ghosts(cfg, full.arrays())):
+            a[:] = b
+        h.upload(st)
+        got = h.download()
+    finally:
+        h.close()
+    dz, df, n = closed_form_exchange_check(cfg, got.arrays(), full.arrays())
+    assert n > 0 and dz < 1e-14 and df < 1e-12, (dz, df)
