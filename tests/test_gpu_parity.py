@@ -194,6 +194,13 @@ def test_divb_and_conservation():
 
 
 def test_mixed_precision_close_to_fp64():
+    """A regression bound on the stage path's mixed form (FP32 WENO5 and
+    Riemann solves, FP64 metric, CT, update and sums), not the benchmark's
+    1e-6 gate: this form does not meet that gate (FP32 rounding of the
+    primitives puts ~6e-8 p of noise into the momentum flux, large against
+    the small momentum of the quiescent region), and the benchmark reports
+    its gate as failed.  20 steps of the magnetised blast stay within 1e-5
+    relative L1 of FP64."""
     base = dict(ntp=24, nr=1, nghost=3, recon=cs.RECON_WENO5, riemann=cs.RS_HLLD, emf=cs.EMF_LLF, rk_order=3)
     res = {}
     for prec in (cs.FP64, cs.MIXED):
