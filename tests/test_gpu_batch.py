@@ -3,7 +3,8 @@ all shells.  The batched kernels run the same device functions on the same
 data as the single-shell path, so after a few steps every shell must be bit
 for bit the state the single-shell path produces (whose parity with the
 oracle is tested in test_gpu_parity.py), with explicit and implicit gravity,
-and after the autotuner has changed the workgroup sizes."""
+in FP64 and in the mixed form (the 48-shell configuration's two legs), and
+after the autotuner has changed the workgroup sizes."""
 import numpy as np
 import pytest
 
@@ -12,8 +13,8 @@ import cubesphere_amd as cs
 pytestmark = pytest.mark.gpu
 
 
-def _shells(imex):
-    kw = dict(recon=cs.RECON_WENO5, riemann=cs.RS_HLLD, emf=cs.EMF_LLF, precision=cs.FP64,
+def _shells(imex, precision=cs.FP64):
+    kw = dict(recon=cs.RECON_WENO5, riemann=cs.RS_HLLD, emf=cs.EMF_LLF, precision=precision,
               rk_order=2 if imex else 3, radial_bc=cs.RBC_PAD, rmin=1.0, rmax=1.5, grav=1.0, imex=imex)
     cfgs = [cs.make_config(8, 3, **kw), cs.make_config(10, 4, **kw), cs.make_config(12, 2, **kw),
             cs.make_config(9, 5, riemann=cs.RS_HLL, **{k: v for k, v in kw.items() if k != "riemann"})]
@@ -34,11 +35,11 @@ def _one_by_one(cfgs, states, nsteps):
     return out
 
 
-@pytest.mark.parametrize("imex", [0, 1])
-@pytest.mark.parametrize("tune", [False, True])
-def test_batch_equals_single_shells(imex, tune):
-    print(f"batch imex={imex} autotune={tune}", flush=True)
-    cfgs, states = _shells(imex)
+@pytest.mark.parametrize("imex,tune,precision", [(0, False, cs.FP64), (1, False, cs.FP64), (0, True, cs.FP64),
+                                                (1, True, cs.FP64), (0, False, cs.MIXED), (1, True, cs.MIXED)])
+def test_batch_equals_single_shells(imex, tune, precision):
+    print(f"batch imex={imex} autotune={tune} precision={precision}", flush=True)
+    cfgs, states = _shells(imex, precision)
     ref = _one_by_one(cfgs, states, 3)
     B = cs.Batch(cfgs)
     for sh, st in zip(B.shells, states):

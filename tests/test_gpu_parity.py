@@ -220,6 +220,40 @@ def test_mixed_precision_close_to_fp64():
     assert rel < 1e-5
 
 
+def test_mixed_shell_imex_close_to_fp64():
+    """The 3-D shell configuration's precision comparison at a small size
+    (6x10^2x4, WENO5 + HLLD + CT, IMEX with implicit gravity, hydrostatic
+    state with noise): after 10 steps density, energy and zone B of the mixed
+    form agree with FP64 to 1e-6 relative L1 (measured 2e-8 to 1e-7) and div B
+    stays at rounding in both; the momentum, nearly zero in hydrostatic
+    balance, carries FP32 rounding of the pressure-gravity balance against its
+    tiny magnitude (measured 6e-7) and is bounded at 1e-5.  Over the full
+    configuration's grid and step count the momentum error grows past the
+    1e-6 gate, which the benchmark reports."""
+    base = dict(ntp=10, nr=4, nghost=3, recon=cs.RECON_WENO5, riemann=cs.RS_HLLD, emf=cs.EMF_LLF, rk_order=2,
+                radial_bc=cs.RBC_PAD, rmin=1.0, rmax=1.5, grav=1.0, imex=1)
+    res, divb = {}, {}
+    for prec in (cs.FP64, cs.MIXED):
+        cfg = cs.make_config(precision=prec, **base)
+        st = cs.init_problem(cfg, 3, seed=7)
+        dev = cs.CubeSphere(cfg)
+        dev.upload(st)
+        dev.dt = 0.0
+        dev.evaluate_timestep()
+        dev.advance(10)
+        res[prec] = dev.download()
+        divb[prec] = dev.max_divb()
+        dev.close()
+    U64, _ = res[cs.FP64].interior()
+    U32, _ = res[cs.MIXED].interior()
+    rel = [float(np.abs(U32[:, c] - U64[:, c]).sum() / np.abs(U64[:, c]).sum()) for c in range(8)]
+    print("mixed vs fp64 rel L1 per component after 10 steps:", " ".join(f"{r:.2e}" for r in rel))
+    for c in (0, 4, 5, 6, 7):
+        assert rel[c] < 1e-6, (c, rel)
+    assert max(rel[1:4]) < 1e-5, rel
+    assert divb[cs.FP64] < 1e-12 and divb[cs.MIXED] < 1e-12, divb
+
+
 @pytest.mark.parametrize("name", ["surf16_hlld_blast", "shell10x4_hlld_grav"])
 def test_autotune_keeps_results(name):
     """Workgroup sizes change only the schedule: the same steps give the same bits."""
