@@ -163,6 +163,28 @@ __device__ __forceinline__ LBlock lblock() {
 }
 __device__ __forceinline__ int64_t ltid(const LBlock& L) { return L.x * blockDim.x + threadIdx.x; }
 
+// Zone work items in column tiles: workgroup w takes a TI x TJ tile of
+// columns at one level, and consecutive workgroups (one XCD's run of lblock
+// numbers) step through the levels of one tile before the next tile, so a
+// zone's neighbours in the level below and above, and in the tile's other
+// rows, were read moments before through the same L2.
+template <int TI, int TJ>
+__device__ __forceinline__ bool tile_decode(const Box& b, int64_t w, int lane, int& i, int& j, int& k) {
+  const int nti = (b.ni + TI - 1) / TI, ntj = (b.nj + TJ - 1) / TJ;
+  const int64_t tile = w / b.nk;
+  if (tile >= (int64_t)nti * ntj) return false;
+  const int ii = (int)(tile % nti) * TI + lane % TI, jj = (int)(tile / nti) * TJ + lane / TI;
+  if (ii >= b.ni || jj >= b.nj) return false;
+  i = b.i0 + ii;
+  j = b.j0 + jj;
+  k = b.k0 + (int)(w % b.nk);
+  return true;
+}
+template <int TI, int TJ>
+static unsigned tile_blocks(const Box& b) {
+  return (unsigned)((int64_t)((b.ni + TI - 1) / TI) * ((b.nj + TJ - 1) / TJ) * b.nk);
+}
+
 __device__ __forceinline__ bool in_weno_range(const Dev& D, int i, int j, int k) {
   return i >= -2 && i <= D.N + 3 && j >= -2 && j <= D.N + 3 && k >= -2 && k <= D.NR + 3;
 }
@@ -293,7 +315,7 @@ __global__ void __launch_bounds__(256) k_tricorner(Dev D, Buf6 in) {
 __global__ void __launch_bounds__(256) k_flat_a(Dev D, Box b) {
   int i, j, k;
   const LBlock wg = lblock();
-  if (!b.decode(ltid(wg), i, j, k)) return;
+  if (!tile_decode<32, 8>(b, wg.x, threadIdx.x, i, j, k)) return;
   const int s = wg.y;
   const int64_t z = D.zi(i, j, k), so = (int64_t)s * D.NZ;
   if (!in_flat_range(D, i, j, k)) {
@@ -1434,7 +1456,7 @@ __global__ void __launch_bounds__(256) k_edge(Dev D, Box bx, Box by, Box bz, con
   const int fam = wg.y % 3, s = wg.y / 3;
   const Box& b = fam == 0 ? bx : (fam == 1 ? by : bz);
   int i, j, k;
-  if (!b.decode(ltid(wg), i, j, k)) return;
+  if (!tile_decode<32, 8>(b, wg.x, threadIdx.x, i, j, k)) return;
   const int N = D.N;
   const int64_t t = D.fci(i, j, k), NF = D.NFC;
   double* out = (fam == 0 ? E0 : fam == 1 ? E1 : E2) + (int64_t)s * NF;
@@ -1654,7 +1676,7 @@ __global__ void __launch_bounds__(256) k_upd_zone(Dev D, Box b, Buf6 in, Buf6 u0
                                                   double wb, int mode) {
   int i, j, k;
   const LBlock wg = lblock();
-  if (!b.decode(ltid(wg), i, j, k)) return;
+  if (!tile_decode<32, 8>(b, wg.x, threadIdx.x, i, j, k)) return;
   const int s = wg.y;
   const int64_t z = D.zi(i, j, k), so = (int64_t)s * 9 * D.NFC;
   const double dt = dtp[0];
@@ -2228,7 +2250,7 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
   k_tricorner<<<dim3(nb(4 * (int64_t)D.g * D.nzr, 256), 6), 256, 0, st>>>(D, in);
   mark_end(S, st, CS_P1K_FACE_SYNC);
   mark_begin(S, st);
-  k_flat_a<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all);
+  k_flat_a<<<dim3(tile_blocks<32, 8>(S->all), 6), 256, 0, st>>>(D, S->all);
   k_flat_b<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all);
   k_flat_c<<<dim3(nb(nall, 256), 6), 256, 0, st>>>(D, S->all);
   k_ctr_reset<<<1, 64, 0, st>>>(D);
@@ -2259,9 +2281,9 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
   mark_begin(S, st);
   k_flux<<<dim3((unsigned)(nfi * nfj), 6), kFluxTI * kFluxTJ, 0, st>>>(D, F[0], F[1], F[2], Sp[0], Sp[1], Sp[2]);
   mark_end(S, st, CS_P1K_FLUX);
-  const int64_t ned = std::max(S->ex.count(), std::max(S->ey.count(), S->ez.count()));
+  const unsigned ned = std::max(tile_blocks<32, 8>(S->ex), std::max(tile_blocks<32, 8>(S->ey), tile_blocks<32, 8>(S->ez)));
   mark_begin(S, st);
-  k_edge<<<dim3(nb(ned, 256), 18), 256, 0, st>>>(D, S->ex, S->ey, S->ez, Sp[0], Sp[1], Sp[2], E[0], E[1], E[2]);
+  k_edge<<<dim3(ned, 18), 256, 0, st>>>(D, S->ex, S->ey, S->ez, Sp[0], Sp[1], Sp[2], E[0], E[1], E[2]);
   mark_end(S, st, CS_P1K_EDGE);
   P1S_HIP(hipGetLastError());
   return CS_OK;
@@ -2275,7 +2297,7 @@ int update(Stage* S, const Buf& in, const Buf& u0, const Buf& out, double* const
   k_upd_face<<<dim3(nb(nfc, 256), 18), 256, 0, st>>>(D, b6(in), b6(u0), b6(out), E[0], E[1], E[2], dtp, wa, wb);
   mark_end(S, st, CS_P1K_UPDATE_FACES);
   mark_begin(S, st);
-  k_upd_zone<<<dim3(nb(S->own.count(), 256), 6), 256, 0, st>>>(D, S->own, b6(in), b6(u0), b6(out), F[0], F[1], F[2],
+  k_upd_zone<<<dim3(tile_blocks<32, 8>(S->own), 6), 256, 0, st>>>(D, S->own, b6(in), b6(u0), b6(out), F[0], F[1], F[2],
                                                                 dtp, wa, wb, mode);
   mark_end(S, st, CS_P1K_UPDATE_ZONES);
   P1S_HIP(hipGetLastError());
