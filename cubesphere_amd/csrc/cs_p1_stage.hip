@@ -139,7 +139,7 @@ struct Box {
   }
   __host__ __device__ int64_t count() const { return (int64_t)ni * nj * nk; }
 };
-static Box make_box(int i0, int i1, int j0, int j1, int k0, int k1) {
+__host__ __device__ static inline Box make_box(int i0, int i1, int j0, int j1, int k0, int k1) {
   return Box{i0, j0, k0, i1 - i0 + 1, j1 - j0 + 1, k1 - k0 + 1};
 }
 
@@ -1634,12 +1634,11 @@ __global__ void __launch_bounds__(256) k_upd_face(Dev D, Buf6 in, Buf6 u0, Buf6 
                                                   double wb) {
   const LBlock wg = lblock();
   const int fam = wg.y % 3, s = wg.y / 3;
-  const int64_t t = ltid(wg);
   const int N = D.N, NR = D.NR;
-  const int ni = fam == 0 ? N + 1 : N, nj = fam == 1 ? N + 1 : N, nk = fam == 2 ? NR + 1 : NR;
-  if (t >= (int64_t)ni * nj * nk) return;
-  const int i = (int)(t % ni) + (fam == 0 ? 0 : 1), j = (int)((t / ni) % nj) + (fam == 1 ? 0 : 1),
-            k = (int)(t / ((int64_t)ni * nj)) + (fam == 2 ? 0 : 1);
+  // the owned faces of the family: x i 0 .. N, j 1 .. N, k 1 .. NR; y, z likewise
+  const Box fb = make_box(fam == 0 ? 0 : 1, N, fam == 1 ? 0 : 1, N, fam == 2 ? 0 : 1, NR);
+  int i, j, k;
+  if (!tile_decode<32, 8>(fb, wg.x, threadIdx.x, i, j, k)) return;
   const double dt = dtp[0];
   const int64_t so = (int64_t)s * D.NFC, f = D.fci(i, j, k);
   const double *Ex = E0 + so, *Ey = E1 + so, *Ez = E2 + so;
@@ -2292,9 +2291,9 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
 int update(Stage* S, const Buf& in, const Buf& u0, const Buf& out, double* const* F, double* const* E,
            const double* dtp, double wa, double wb, int mode, hipStream_t st) {
   const Dev& D = S->D;
-  const int64_t nfc = (int64_t)(D.N + 1) * (D.N + 1) * (D.NR + 1);
+  const unsigned nfc = tile_blocks<32, 8>(make_box(0, D.N, 0, D.N, 0, D.NR));  // covers every family's box
   mark_begin(S, st);
-  k_upd_face<<<dim3(nb(nfc, 256), 18), 256, 0, st>>>(D, b6(in), b6(u0), b6(out), E[0], E[1], E[2], dtp, wa, wb);
+  k_upd_face<<<dim3(nfc, 18), 256, 0, st>>>(D, b6(in), b6(u0), b6(out), E[0], E[1], E[2], dtp, wa, wb);
   mark_end(S, st, CS_P1K_UPDATE_FACES);
   mark_begin(S, st);
   k_upd_zone<<<dim3(tile_blocks<32, 8>(S->own), 6), 256, 0, st>>>(D, S->own, b6(in), b6(u0), b6(out), F[0], F[1], F[2],
