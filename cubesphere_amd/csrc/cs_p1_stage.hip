@@ -983,13 +983,47 @@ __global__ void __launch_bounds__(kFpTI * kFpTJ) k_face_pts(Dev D, Buf6 in) {
 // ---------------------------------------------------------------------------
 // step 8.d: the ADP constrained least squares for the 12 linear B
 // coefficients of a zone (410...f:4643-5360): the six face means are matched
-// exactly (constraint rows n (x) (1, r) at the face centroids), the face-point
-// B at points 2 and 4 of each face and the 12 WENO modes in least squares;
-// then the energy takes the change of magnetic energy (:5336-5357).  Solved
-// through the Schur complement: M = A^T A (>= I) by Cholesky, Z = L^-1 C^T in
-// LDS, S = Z^T Z, x = L^-T (L^-1 A^T b - Z lambda).
+// exactly (constraint rows n (x) (1, r) at the face centroids, the normal of
+// the face's first point), the face-point B at points 2 and 4 of each face
+// and the 12 WENO modes in least squares; then the energy takes the change
+// of magnetic energy (:5336-5357).  The reference solves the 18 x 18 KKT
+// system by LU; here six unknowns are eliminated.  In a frame of the zone
+// (the mean of its x-face normals, the mean of its y-face normals made
+// orthogonal to it, their cross product: a rotation R), the six
+// face-mean constraints fix each B component's mean and its derivative along
+// its own axis, the pivots (b0x, Gxx, b0y, Gyy, b0z, Gzz) = e - W x_f of the
+// six other coefficients x_f (Gaussian elimination of the 6 x 6 pivot block,
+// which in this frame is close to block diagonal, [[1, h], [1, -h]] per
+// component).  The least-squares rows (the face points 2 and 4 of each face,
+// the 12 WENO modes) in x_f are then a 6 x 6 normal system.  B' = R B and
+// rho' = R rho, so n . B(rho) = n' . B'(rho') and the WENO rows' squared
+// distance is unchanged: the minimiser is the one of the 12-unknown form.
+// About 100 live words instead of a 12 x 12 Cholesky factor and a 6 x 12
+// block in LDS, so two waves fit per SIMD, and no LDS.
+__device__ __forceinline__ void put6(double* a, int f, double v) {
+#pragma unroll
+  for (int q = 0; q < 6; ++q) a[q] = f == q ? v : a[q];
+}
+__device__ __forceinline__ void rot3(const double* R, const double* v, double* o) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) o[r] = R[3 * r] * v[0] + R[3 * r + 1] * v[1] + R[3 * r + 2] * v[2];
+}
+// the row n' (x) (1, rho') split into the pivot and the free coefficients
+__device__ __forceinline__ void split_row(const double* n, const double* r, double* rp, double* rf) {
+  rp[0] = n[0];
+  rp[1] = n[0] * r[0];
+  rp[2] = n[1];
+  rp[3] = n[1] * r[1];
+  rp[4] = n[2];
+  rp[5] = n[2] * r[2];
+  rf[0] = n[0] * r[1];
+  rf[1] = n[0] * r[2];
+  rf[2] = n[1] * r[0];
+  rf[3] = n[1] * r[2];
+  rf[4] = n[2] * r[0];
+  rf[5] = n[2] * r[1];
+}
 __global__ void __launch_bounds__(64) k_clsq(Dev D, Box b, Buf6 in) {
-  __shared__ double Zs[72][64];  // each thread's own column: no barrier needed
   int i, j, k;
   const LBlock wg = lblock();
   if (!b.decode(ltid(wg), i, j, k)) return;
@@ -1000,15 +1034,96 @@ __global__ void __launch_bounds__(64) k_clsq(Dev D, Box b, Buf6 in) {
   double c0[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) c0[c] = D.cen(s, z, c);
-  double L[78], u[12], row[12], dci[6];
-#pragma unroll
-  for (int r = 0; r < 78; ++r) L[r] = 0.0;
-#pragma unroll
-  for (int r = 0; r < 12; ++r) {
-    L[LIX(r, r)] = 1.0;
-    u[r] = D.v(r % 4, s, 5 + r / 4, z);
-  }
   // face f: 0 right x, 1 left x, 2 up y, 3 down y, 4 top z, 5 bottom z
+  // pass 1: the constraint rows (the first point's normal at the centroid) and the face B
+  double cn[3][6], cr[3][6], cd[6];
+#pragma unroll 1
+  for (int f = 0; f < 6; ++f) {
+    const int fam = f >> 1;
+    const bool hi = (f & 1) == 0;
+    const int fi = i - (fam == 0 && !hi), fj = j - (fam == 1 && !hi), fk = k - (fam == 2 && !hi);
+    csg1::FaceG F;
+    csg1::face_of(D.G, fam, s, fi, fj, fk, F);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      put6(cn[c], f, F.n[0][c]);
+      put6(cr[c], f, (F.cen[c] - c0[c]) * obl);
+    }
+    put6(cd, f, in.b(fam)[(int64_t)s * D.NFC + D.fci(fi, fj, fk)]);
+  }
+  // the zone's frame
+  double R[9];
+  {
+    double a[3], e2[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      a[c] = cn[c][0] + cn[c][1];
+      e2[c] = cn[c][2] + cn[c][3];
+    }
+    double t = 1.0 / sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a[c] *= t;
+    t = e2[0] * a[0] + e2[1] * a[1] + e2[2] * a[2];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e2[c] -= t * a[c];
+    t = 1.0 / sqrt(e2[0] * e2[0] + e2[1] * e2[1] + e2[2] * e2[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      e2[c] *= t;
+      R[c] = a[c];
+      R[3 + c] = e2[c];
+    }
+    R[6] = a[1] * e2[2] - a[2] * e2[1];
+    R[7] = a[2] * e2[0] - a[0] * e2[2];
+    R[8] = a[0] * e2[1] - a[1] * e2[0];
+  }
+  // the pivot block and its right-hand sides [free | d], eliminated in place
+  double Ap[6][6], Wd[6][7];
+#pragma unroll
+  for (int f = 0; f < 6; ++f) {
+    double n[3], r[3], nn[3], rr[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      n[c] = cn[c][f];
+      r[c] = cr[c][f];
+    }
+    rot3(R, n, nn);
+    rot3(R, r, rr);
+    double rf[6];
+    split_row(nn, rr, Ap[f], rf);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) Wd[f][q] = rf[q];
+    Wd[f][6] = cd[f];
+  }
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double od = 1.0 / Ap[c][c];
+#pragma unroll
+    for (int r = c + 1; r < 6; ++r) {
+      const double m = Ap[r][c] * od;
+#pragma unroll
+      for (int q = c + 1; q < 6; ++q) Ap[r][q] -= m * Ap[c][q];
+#pragma unroll
+      for (int q = 0; q < 7; ++q) Wd[r][q] -= m * Wd[c][q];
+    }
+  }
+#pragma unroll
+  for (int r = 5; r >= 0; --r) {
+    const double od = 1.0 / Ap[r][r];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) {
+      double t = Wd[r][q];
+#pragma unroll
+      for (int c = r + 1; c < 6; ++c) t -= Ap[r][c] * Wd[c][q];
+      Wd[r][q] = t * od;
+    }
+  }
+  // pass 2: the least-squares rows in the free coefficients
+  double N[21], h[6];
+#pragma unroll
+  for (int q = 0; q < 21; ++q) N[q] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 6; ++q) h[q] = 0.0;
 #pragma unroll 1
   for (int f = 0; f < 6; ++f) {
     const int fam = f >> 1;
@@ -1017,147 +1132,146 @@ __global__ void __launch_bounds__(64) k_clsq(Dev D, Box b, Buf6 in) {
     csg1::FaceG F;
     csg1::face_of(D.G, fam, s, fi, fj, fk, F);
     const int64_t fb = D.fci(fi, fj, fk);
-    // least-squares rows: points 2 and 4, each n (x) (1, r)
-    if (fam < 2) {
-      // one normal for the whole lateral face: the two rows add
-      // (n n^T) (x) (rho_2 rho_2^T + rho_4 rho_4^T) to the normal matrix
-      double R[10], h[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int q = 0; q < 10; ++q) R[q] = 0.0;
+    for (int t = 0; t < 2; ++t) {
+      const int pt = 1 + 2 * t;
+      const double* n = fam == 2 ? F.n[pt] : F.n[0];
+      double r[3], nn[3], rr[3];
 #pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int pt = 1 + 2 * t;
-        const double bv = D.bq(s, fam, pt)[fb];
-        const double rho[4] = {1.0, (F.p[pt][0] - c0[0]) * obl, (F.p[pt][1] - c0[1]) * obl,
-                               (F.p[pt][2] - c0[2]) * obl};
+      for (int c = 0; c < 3; ++c) r[c] = (F.p[pt][c] - c0[c]) * obl;
+      rot3(R, n, nn);
+      rot3(R, r, rr);
+      double rp[6], red[6];
+      split_row(nn, rr, rp, red);
+      double tgt = D.bq(s, fam, pt)[fb];
 #pragma unroll
-        for (int m = 0; m < 4; ++m) {
-          h[m] += rho[m] * bv;
+      for (int q = 0; q < 6; ++q) {
+        tgt -= rp[q] * Wd[q][6];
 #pragma unroll
-          for (int m2 = 0; m2 <= m; ++m2) R[LIX(m, m2)] += rho[m] * rho[m2];
-        }
+        for (int c = 0; c < 6; ++c) red[c] -= rp[q] * Wd[q][c];
       }
-      const double* n = F.n[0];
 #pragma unroll
-      for (int p = 0; p < 12; ++p) {
-        const int c1 = p >> 2, m1 = p & 3;
-        u[p] += n[c1] * h[m1];
+      for (int r2 = 0; r2 < 6; ++r2) {
+        h[r2] += red[r2] * tgt;
 #pragma unroll
-        for (int q = 0; q <= p; ++q) {
-          const int c2 = q >> 2, m2 = q & 3;
-          L[LIX(p, q)] += (n[c1] * n[c2]) * R[m1 >= m2 ? LIX(m1, m2) : LIX(m2, m1)];
-        }
-      }
-    } else {
-#pragma unroll
-      for (int t = 0; t < 2; ++t) {
-        const int pt = 1 + 2 * t;
-        double r[3];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) r[c] = (F.p[pt][c] - c0[c]) * obl;
-        brow(F.n[pt], r, row);
-        const double bv = D.bq(s, fam, pt)[fb];
-#pragma unroll
-        for (int p = 0; p < 12; ++p) {
-          u[p] += row[p] * bv;
-#pragma unroll
-          for (int q = 0; q <= p; ++q) L[LIX(p, q)] += row[p] * row[q];
-        }
+        for (int c = 0; c <= r2; ++c) N[LIX(r2, c)] += red[r2] * red[c];
       }
     }
-    // constraint row: the normal of the face's first point, at its centroid
-    double r[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) r[c] = (F.cen[c] - c0[c]) * obl;
-    brow(F.n[0], r, row);
-#pragma unroll
-    for (int p = 0; p < 12; ++p) Zs[f * 12 + p][threadIdx.x] = row[p];
-    const double bf = in.b(fam)[(int64_t)s * D.NFC + fb];
-    dci[0] = f == 0 ? bf : dci[0];
-    dci[1] = f == 1 ? bf : dci[1];
-    dci[2] = f == 2 ? bf : dci[2];
-    dci[3] = f == 3 ? bf : dci[3];
-    dci[4] = f == 4 ? bf : dci[4];
-    dci[5] = f == 5 ? bf : dci[5];
   }
-  chol12(L);
-  fwd12(L, u);
-  double S[21], sv[6];
-#pragma unroll 1
-  for (int ci = 0; ci < 6; ++ci) {
+  // the WENO modes u, in the zone's frame: b0' = R b0, G' = R G R^T
+  double up[6], uf[6];
+  {
+    double b0[3], G[3][3], t[3][3];
 #pragma unroll
-    for (int p = 0; p < 12; ++p) row[p] = Zs[ci * 12 + p][threadIdx.x];
-    fwd12(L, row);
-    double t = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      b0[c] = D.v(0, s, 5 + c, z);
 #pragma unroll
-    for (int p = 0; p < 12; ++p) {
-      t += row[p] * u[p];
-      Zs[ci * 12 + p][threadIdx.x] = row[p];
+      for (int q = 0; q < 3; ++q) G[c][q] = D.v(1 + q, s, 5 + c, z);
     }
-    const double d = ci == 0 ? dci[0] : ci == 1 ? dci[1] : ci == 2 ? dci[2] : ci == 3 ? dci[3] : ci == 4 ? dci[4] : dci[5];
-    sv[0] = ci == 0 ? t - d : sv[0];
-    sv[1] = ci == 1 ? t - d : sv[1];
-    sv[2] = ci == 2 ? t - d : sv[2];
-    sv[3] = ci == 3 ? t - d : sv[3];
-    sv[4] = ci == 4 ? t - d : sv[4];
-    sv[5] = ci == 5 ? t - d : sv[5];
+    double bp[3];
+    rot3(R, b0, bp);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) t[r][q] = R[3 * r] * G[0][q] + R[3 * r + 1] * G[1][q] + R[3 * r + 2] * G[2][q];
+    double Gp[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) Gp[r][q] = t[r][0] * R[3 * q] + t[r][1] * R[3 * q + 1] + t[r][2] * R[3 * q + 2];
+    up[0] = bp[0];
+    up[1] = Gp[0][0];
+    up[2] = bp[1];
+    up[3] = Gp[1][1];
+    up[4] = bp[2];
+    up[5] = Gp[2][2];
+    uf[0] = Gp[0][1];
+    uf[1] = Gp[0][2];
+    uf[2] = Gp[1][0];
+    uf[3] = Gp[1][2];
+    uf[4] = Gp[2][0];
+    uf[5] = Gp[2][1];
+  }
+  // the WENO rows: free ones directly, pivot ones through x_p = e - W x_f
+#pragma unroll
+  for (int q = 0; q < 6; ++q) {
+    N[LIX(q, q)] += 1.0;
+    h[q] += uf[q];
   }
 #pragma unroll
-  for (int r = 0; r < 6; ++r)
+  for (int q = 0; q < 6; ++q) {
+    const double tgt = up[q] - Wd[q][6];
 #pragma unroll
-    for (int c = 0; c <= r; ++c) {
-      double t = 0.0;
+    for (int r2 = 0; r2 < 6; ++r2) {
+      h[r2] -= Wd[q][r2] * tgt;
 #pragma unroll
-      for (int p = 0; p < 12; ++p) t += Zs[r * 12 + p][threadIdx.x] * Zs[c * 12 + p][threadIdx.x];
-      S[r * (r + 1) / 2 + c] = t;
+      for (int c = 0; c <= r2; ++c) N[LIX(r2, c)] += Wd[q][r2] * Wd[q][c];
     }
-  // 6 x 6 Cholesky of S, the reciprocal of each diagonal entry kept
+  }
+  // 6 x 6 Cholesky with reciprocal diagonals, then the two solves
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
-    double d = S[c * (c + 1) / 2 + c];
+    double d = N[LIX(c, c)];
 #pragma unroll
-    for (int q = 0; q < c; ++q) d -= S[c * (c + 1) / 2 + q] * S[c * (c + 1) / 2 + q];
+    for (int q = 0; q < c; ++q) d -= N[LIX(c, q)] * N[LIX(c, q)];
     const double od = 1.0 / sqrt(d);
-    S[c * (c + 1) / 2 + c] = od;
+    N[LIX(c, c)] = od;
 #pragma unroll
     for (int r = c + 1; r < 6; ++r) {
-      double t = S[r * (r + 1) / 2 + c];
+      double t = N[LIX(r, c)];
 #pragma unroll
-      for (int q = 0; q < c; ++q) t -= S[r * (r + 1) / 2 + q] * S[c * (c + 1) / 2 + q];
-      S[r * (r + 1) / 2 + c] = t * od;
+      for (int q = 0; q < c; ++q) t -= N[LIX(r, q)] * N[LIX(c, q)];
+      N[LIX(r, c)] = t * od;
     }
   }
 #pragma unroll
   for (int r = 0; r < 6; ++r) {
-    double t = sv[r];
+    double t = h[r];
 #pragma unroll
-    for (int q = 0; q < r; ++q) t -= S[r * (r + 1) / 2 + q] * sv[q];
-    sv[r] = t * S[r * (r + 1) / 2 + r];
+    for (int q = 0; q < r; ++q) t -= N[LIX(r, q)] * h[q];
+    h[r] = t * N[LIX(r, r)];
   }
 #pragma unroll
   for (int r = 5; r >= 0; --r) {
-    double t = sv[r];
+    double t = h[r];
 #pragma unroll
-    for (int q = r + 1; q < 6; ++q) t -= S[q * (q + 1) / 2 + r] * sv[q];
-    sv[r] = t * S[r * (r + 1) / 2 + r];
+    for (int q = r + 1; q < 6; ++q) t -= N[LIX(q, r)] * h[q];
+    h[r] = t * N[LIX(r, r)];
   }
+  // the pivots, then back to the global frame: b0 = R^T b0', G = R^T G' R
+  double xp[6];
 #pragma unroll
-  for (int p = 0; p < 12; ++p) {
-    double t = u[p];
+  for (int q = 0; q < 6; ++q) {
+    double t = Wd[q][6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) t -= sv[c] * Zs[c * 12 + p][threadIdx.x];
-    u[p] = t;
+    for (int c = 0; c < 6; ++c) t -= Wd[q][c] * h[c];
+    xp[q] = t;
   }
-  bwd12(L, u);
+  double x[12];
+  {
+    const double bp[3] = {xp[0], xp[2], xp[4]};
+    const double Gp[3][3] = {{xp[1], h[0], h[1]}, {h[2], xp[3], h[3]}, {h[4], h[5], xp[5]}};
+    double t[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[4 * c] = R[c] * bp[0] + R[3 + c] * bp[1] + R[6 + c] * bp[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) t[r][q] = R[r] * Gp[0][q] + R[3 + r] * Gp[1][q] + R[6 + r] * Gp[2][q];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) x[4 * r + 1 + q] = t[r][0] * R[q] + t[r][1] * R[3 + q] + t[r][2] * R[6 + q];
+  }
   const double b5 = in.u(D, s, 5, z), b6 = in.u(D, s, 6, z), b7 = in.u(D, s, 7, z);
   const double aa = (b5 * b5 + b6 * b6 + b7 * b7) * C<double>::ob8pi;
 #pragma unroll
   for (int m = 0; m < 4; ++m)
 #pragma unroll
-    for (int c = 0; c < 3; ++c) D.v(m, s, 5 + c, z) = u[4 * c + m];
+    for (int c = 0; c < 3; ++c) D.v(m, s, 5 + c, z) = x[4 * c + m];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) in.u(D, s, 5 + c, z) = u[4 * c];
-  const double bb = (u[0] * u[0] + u[4] * u[4] + u[8] * u[8]) * C<double>::ob8pi;
+  for (int c = 0; c < 3; ++c) in.u(D, s, 5 + c, z) = x[4 * c];
+  const double bb = (x[0] * x[0] + x[4] * x[4] + x[8] * x[8]) * C<double>::ob8pi;
   in.u(D, s, 4, z) = in.u(D, s, 4, z) - aa + bb;
 }
 
