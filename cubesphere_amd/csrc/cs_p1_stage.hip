@@ -469,32 +469,27 @@ __device__ __forceinline__ void weno3(const Dev& D, int s, int i, int j, int k, 
     for (int m = 0; m < 3; ++m) D.v(m + 1, s, c0 + f, z) = (num[f][m] + w8 * X8[m]) * os * omf;
   }
 }
-// every zone of the WENO range: a workgroup of three waves takes 64 zones;
-// wave w forms the inverses of stencils w, w + 3, w + 6 of its lane's zone
-// into LDS (each inverse once per zone instead of once per field group),
-// then takes fields 3w .. 3w + 2 of that zone over the eight stencils, as
-// weno3 does
-constexpr int kWenoZones = 64;
-__global__ void __launch_bounds__(3 * kWenoZones) k_weno(Dev D, Box b) {
-  __shared__ double Ainv[8][9][kWenoZones];
-  const int lane = threadIdx.x % kWenoZones, grp = threadIdx.x / kWenoZones;
+// every zone of the WENO range, one zone per thread: its eight stencil
+// inverses once, in registers, then the nine fields in turn over the eight
+// stencils, with weno3's arithmetic
+__global__ void __launch_bounds__(128) k_weno(Dev D, Box b) {
+  int i, j, k;
   const LBlock wg = lblock();
+  if (!b.decode(ltid(wg), i, j, k) || !D.active(i, j)) return;
   const int s = wg.y;
-  int i = 0, j = 0, k = 0;
-  const bool ok = b.decode(wg.x * kWenoZones + lane, i, j, k) && D.active(i, j);
-  int64_t z = 0, z6[6];
-  double obl = 0.0;
-  if (ok) {
-    z = D.zi(i, j, k);
-    six(D, i, j, k, z6);
-    obl = D.obl(s, z);
+  const int64_t z = D.zi(i, j, k);
+  int64_t z6[6];
+  six(D, i, j, k, z6);
+  double Ainv[8][9];
+  {
+    const double obl = D.obl(s, z);
     double off[6][3];
 #pragma unroll
     for (int q = 0; q < 6; ++q)
 #pragma unroll
       for (int c = 0; c < 3; ++c) off[q][c] = (D.cen(s, z6[q], c) - D.cen(s, z, c)) * obl;
-#pragma unroll 1
-    for (int st = grp; st < 8; st += 3) {
+#pragma unroll
+    for (int st = 0; st < 8; ++st) {
       const bool bx = (st >> 1) & 1, by = (st >> 2) & 1, bz = st & 1;
       double m3[3][3], A[3][3];
 #pragma unroll
@@ -505,61 +500,42 @@ __global__ void __launch_bounds__(3 * kWenoZones) k_weno(Dev D, Box b) {
       }
       inv3(m3, A);
 #pragma unroll
-      for (int q = 0; q < 9; ++q) Ainv[st][q][lane] = A[q / 3][q % 3];
+      for (int q = 0; q < 9; ++q) Ainv[st][q] = A[q / 3][q % 3];
     }
   }
-  __syncthreads();
-  if (!ok) return;
-  const int c0 = 3 * grp;
   const double smallnum = D.smallnum, omf = 1.0 - D.flat[(int64_t)s * D.NZ + z];
-  double dR[3], dL[3], dU[3], dD[3], dT[3], dB[3], num[3][3], den[3], sx[3][3];
-#pragma unroll
-  for (int f = 0; f < 3; ++f) {
-    const double f0 = D.v(0, s, c0 + f, z);
-    dR[f] = D.v(0, s, c0 + f, z6[0]) - f0;
-    dL[f] = D.v(0, s, c0 + f, z6[1]) - f0;
-    dU[f] = D.v(0, s, c0 + f, z6[2]) - f0;
-    dD[f] = D.v(0, s, c0 + f, z6[3]) - f0;
-    dT[f] = D.v(0, s, c0 + f, z6[4]) - f0;
-    dB[f] = D.v(0, s, c0 + f, z6[5]) - f0;
-    den[f] = 0.0;
-#pragma unroll
-    for (int m = 0; m < 3; ++m) num[f][m] = sx[f][m] = 0.0;
-  }
 #pragma unroll 1
-  for (int st = 0; st < 8; ++st) {
-    const bool bx = (st >> 1) & 1, by = (st >> 2) & 1, bz = st & 1;
-    double A[3][3];
+  for (int f = 0; f < 9; ++f) {
+    const double f0 = D.v(0, s, f, z);
+    const double dR = D.v(0, s, f, z6[0]) - f0, dL = D.v(0, s, f, z6[1]) - f0, dU = D.v(0, s, f, z6[2]) - f0,
+                 dD = D.v(0, s, f, z6[3]) - f0, dT = D.v(0, s, f, z6[4]) - f0, dB = D.v(0, s, f, z6[5]) - f0;
+    double num[3] = {0.0, 0.0, 0.0}, sx[3] = {0.0, 0.0, 0.0}, den = 0.0;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) A[q / 3][q % 3] = Ainv[st][q][lane];
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-      const double r0 = bx ? dL[f] : dR[f], r1 = by ? dD[f] : dU[f], r2 = bz ? dB[f] : dT[f];
+    for (int st = 0; st < 8; ++st) {
+      const bool bx = (st >> 1) & 1, by = (st >> 2) & 1, bz = st & 1;
+      const double r0 = bx ? dL : dR, r1 = by ? dD : dU, r2 = bz ? dB : dT;
       double X[3];
 #pragma unroll
-      for (int m = 0; m < 3; ++m) X[m] = A[m][0] * r0 + A[m][1] * r1 + A[m][2] * r2;
+      for (int m = 0; m < 3; ++m) X[m] = Ainv[st][3 * m] * r0 + Ainv[st][3 * m + 1] * r1 + Ainv[st][3 * m + 2] * r2;
       double tt = X[0] * X[0] + X[1] * X[1] + X[2] * X[2] + smallnum;
       tt = tt * tt;
       const double w = 1.0 / (tt * tt);
-      den[f] = st == 0 ? w : den[f] + w;
+      den = st == 0 ? w : den + w;
 #pragma unroll
       for (int m = 0; m < 3; ++m) {
-        num[f][m] += w * X[m];
-        sx[f][m] += X[m];
+        num[m] += w * X[m];
+        sx[m] += X[m];
       }
     }
-  }
-#pragma unroll
-  for (int f = 0; f < 3; ++f) {
     double X8[3];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) X8[m] = 0.125 * sx[f][m];
+    for (int m = 0; m < 3; ++m) X8[m] = 0.125 * sx[m];
     double tt = X8[0] * X8[0] + X8[1] * X8[1] + X8[2] * X8[2] + smallnum;
     tt = tt * tt;
     const double w8 = 100.0 / (tt * tt);
-    const double os = 1.0 / (den[f] + w8);
+    const double os = 1.0 / (den + w8);
 #pragma unroll
-    for (int m = 0; m < 3; ++m) D.v(m + 1, s, c0 + f, z) = (num[f][m] + w8 * X8[m]) * os * omf;
+    for (int m = 0; m < 3; ++m) D.v(m + 1, s, f, z) = (num[m] + w8 * X8[m]) * os * omf;
   }
 }
 // the zones of the replay list (grid-stride)
@@ -2369,7 +2345,7 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
   k_ctr_reset<<<1, 64, 0, st>>>(D);
   mark_end(S, st, CS_P1K_FLATTEN);
   mark_begin(S, st);
-  k_weno<<<dim3(nb(nwr, kWenoZones), 6), 3 * kWenoZones, 0, st>>>(D, S->wr);
+  k_weno<<<dim3(nb(nwr, 128), 6), 128, 0, st>>>(D, S->wr);
   mark_end(S, st, CS_P1K_WENO);
   mark_begin(S, st);
   k_pos<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr);
