@@ -849,6 +849,7 @@ __device__ __forceinline__ void face_points(const Dev& D, int fam, int s, int i,
 // centroid, the reciprocal scale length) and its value at a point, with the
 // arithmetic of face_points.
 constexpr int kFpTI = 16, kFpTJ = 8;  // columns per workgroup along i and j
+constexpr int kFpLevels = 16;         // levels per workgroup at most
 constexpr int kBWords = 16;
 __device__ __forceinline__ void b_at(const double* w, const double* p, double* b) {
   double r[3];
@@ -872,13 +873,22 @@ __device__ __forceinline__ void lds_b(const double (*Bs)[kFpTI * kFpTJ], int slo
 // their x- and y-faces from LDS and the lower side of their z-face from the
 // zone below's words, kept in registers from the previous level.  Tri-corner faces with one inactive side read the zones from
 // memory (face_points).
-__global__ void __launch_bounds__(kFpTI * kFpTJ) k_face_pts(Dev D, Buf6 in) {
+//
+// The faces' geometry (about 60 words of angular tables per column) is the
+// same at every level, and one workgroup reads it again at each level; the
+// march is split into chunks of nlev levels, and the chunks of one block of
+// columns are consecutive workgroups of one XCD, so they run together and
+// read the tables through that XCD's L2 instead of HBM.  A chunk after the
+// first takes the zone below its first z-face from memory.
+__global__ void __launch_bounds__(kFpTI * kFpTJ) k_face_pts(Dev D, Buf6 in, int nch, int nlev) {
   __shared__ double Bs[kBWords][kFpTI * kFpTJ];
   const LBlock wg = lblock();
   const int s = wg.y, N = D.N, NR = D.NR, nti = (N + 7 + kFpTI - 2) / (kFpTI - 1);
   const int t = threadIdx.x, li = t % kFpTI, lj = t / kFpTI;
-  const int i = -3 + (kFpTI - 1) * (int)(wg.x % nti) + li;
-  const int j = -3 + (kFpTJ - 1) * (int)(wg.x / nti) + lj;
+  const int64_t blk = wg.x / nch;
+  const int ch = (int)(wg.x % nch);
+  const int i = -3 + (kFpTI - 1) * (int)(blk % nti) + li;
+  const int j = -3 + (kFpTJ - 1) * (int)(blk / nti) + lj;
   const bool have = i <= N + D.g && j <= N + D.g;  // the column's zones exist
   const bool mine = have && li < kFpTI - 1 && lj < kFpTJ - 1 && i <= N + 3 && j <= N + 3;
   const bool act = have && D.active(i, j);
@@ -888,11 +898,24 @@ __global__ void __launch_bounds__(kFpTI * kFpTJ) k_face_pts(Dev D, Buf6 in) {
   double below[kBWords];  // the words of the zone below the z-face about to be made
 #pragma unroll
   for (int q = 0; q < kBWords; ++q) below[q] = 0.0;
+  // levels k0 .. k1 - 1 of -3 .. NR + 4
+  const int k0 = -3 + ch * nlev, k1 = min(k0 + nlev, NR + 5);
+  if (k0 >= k1) return;  // the whole workgroup: an empty last chunk
+  if (ch > 0 && fz && act) {
+    const int64_t z = D.zi(i, j, k0 - 1);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) below[4 * c + m] = D.v(m, s, 5 + c, z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) below[12 + c] = D.cen(s, z, c);
+    below[15] = D.obl(s, z);
+  }
   // one flat loop over (level, family) with one call site per piece
-  const int nq = 3 * (NR + 8);
+  const int nq = 3 * (k1 - k0);
 #pragma unroll 1
   for (int q = 0; q < nq; ++q) {
-    const int k = -3 + q / 3, fam = q % 3;
+    const int k = k0 + q / 3, fam = q % 3;
     int ii = i, jj = j;  // opaque, as in k_flux
     asm volatile("" : "+v"(ii), "+v"(jj));
     if (fam == 0) {
@@ -1438,8 +1461,13 @@ __device__ __forceinline__ void face_solve(const Dev& D, int fam, int s, int k, 
 // a z-face is the column's previous top state (kept in registers).  Only
 // tri-corner faces whose neighbour is an inactive column read the zones from
 // memory.  So a zone's words come from HBM about 1.2 times per launch
-// (16 x 8 loaded for 15 x 7 computed), not once per face side.
+// (16 x 8 loaded for 15 x 7 computed), not once per face side.  As in
+// k_face_pts the march is split into chunks of nlev levels whose workgroups
+// run together on one XCD and share the faces' geometry tables through its
+// L2; a chunk after the first starts one level early with its faces off, so
+// that level leaves the bottom state of the chunk's first z-face.
 constexpr int kFluxTI = 16, kFluxTJ = 8;  // columns per workgroup along i and j
+constexpr int kFluxLevels = 16;           // levels per workgroup at most
 constexpr int kSlopeWords = 40;           // 4 x 9 modes, the centroid, the reciprocal scale length
 __device__ __forceinline__ void lds_state(const double (*Zs)[kFluxTI * kFluxTJ], int slot, const double* p,
                                           double* w) {
@@ -1451,13 +1479,15 @@ __device__ __forceinline__ void lds_state(const double (*Zs)[kFluxTI * kFluxTJ],
     w[c] = Zs[c][slot] + Zs[9 + c][slot] * r[0] + Zs[18 + c][slot] * r[1] + Zs[27 + c][slot] * r[2];
 }
 __global__ void __launch_bounds__(kFluxTI * kFluxTJ) k_flux(Dev D, double* F0, double* F1, double* F2, double* S0,
-                                                           double* S1, double* S2) {
+                                                           double* S1, double* S2, int nch, int nlev) {
   __shared__ double Zs[kSlopeWords][kFluxTI * kFluxTJ];
   const LBlock wg = lblock();
   const int s = wg.y, N = D.N, NR = D.NR, nti = (N + 3 + kFluxTI - 2) / (kFluxTI - 1);
   const int t = threadIdx.x, li = t % kFluxTI, lj = t / kFluxTI;
-  const int i = -1 + (kFluxTI - 1) * (int)(wg.x % nti) + li;
-  const int j = -1 + (kFluxTJ - 1) * (int)(wg.x / nti) + lj;
+  const int64_t blk = wg.x / nch;
+  const int ch = (int)(wg.x % nch);
+  const int i = -1 + (kFluxTI - 1) * (int)(blk % nti) + li;
+  const int j = -1 + (kFluxTJ - 1) * (int)(blk / nti) + lj;
   const bool have = i <= N + 2 && j <= N + 2;  // the column's zones exist
   const bool mine = have && li < kFluxTI - 1 && lj < kFluxTJ - 1;  // the block computes this column's faces
   const bool act = have && D.active(i, j);
@@ -1469,12 +1499,15 @@ __global__ void __launch_bounds__(kFluxTI * kFluxTJ) k_flux(Dev D, double* F0, d
   double zb[9];  // the column's state at the bottom point of the next z-face (the previous level's top)
 #pragma unroll
   for (int c = 0; c < 9; ++c) zb[c] = 0.0;
+  // faces of levels k0 .. k1 - 1 of -1 .. NR + 2, the loop from kb
+  const int k0 = -1 + ch * nlev, k1 = min(k0 + nlev, NR + 3), kb = ch > 0 ? k0 - 1 : k0;
+  if (k0 >= k1) return;  // the whole workgroup: an empty last chunk
   // one flat loop over (level, family): a single call site, so the solver is
   // inlined once (nested loops cost a second wave of registers)
-  const int nq = 3 * (NR + 4);
+  const int nq = 3 * (k1 - kb);
 #pragma unroll 1
   for (int q = 0; q < nq; ++q) {
-    const int k = -1 + q / 3, fam = q % 3;
+    const int k = kb + q / 3, fam = q % 3;
     // the column's indices opaque to the optimiser: the k-invariant table
     // loads stay in the loop body instead of being hoisted and held live
     int ii = i, jj = j;
@@ -1493,8 +1526,8 @@ __global__ void __launch_bounds__(kFluxTI * kFluxTJ) k_flux(Dev D, double* F0, d
       }
       __syncthreads();
     }
-    const bool on = fam == 0 ? (k >= 0 && k <= NR + 1 && onx)
-                             : fam == 1 ? (k >= 0 && k <= NR + 1 && ony) : (k >= 0 && onz);
+    const bool on = k >= k0 && (fam == 0 ? (k >= 0 && k <= NR + 1 && onx)
+                                          : fam == 1 ? (k >= 0 && k <= NR + 1 && ony) : (k >= 0 && onz));
     if (on) {
       const int fk = fam == 2 ? k - 1 : k;
       csg1::FaceG F;
@@ -2360,15 +2393,18 @@ int front(Stage* S, const Buf& in_, double* const* F, double* const* Sp, double*
   k_clear_ent<<<dim3(nb(nwr, 256), 6), 256, 0, st>>>(D, S->wr, in, err);
   mark_end(S, st, CS_P1K_REPLAY);
   const int64_t nfpi = (D.N + 7 + kFpTI - 2) / (kFpTI - 1), nfpj = (D.N + 7 + kFpTJ - 2) / (kFpTJ - 1);
+  const int fpl = D.NR + 8, fpch = (fpl + kFpLevels - 1) / kFpLevels, fpnl = (fpl + fpch - 1) / fpch;
   mark_begin(S, st);
-  k_face_pts<<<dim3((unsigned)(nfpi * nfpj), 6), kFpTI * kFpTJ, 0, st>>>(D, in);
+  k_face_pts<<<dim3((unsigned)(nfpi * nfpj * fpch), 6), kFpTI * kFpTJ, 0, st>>>(D, in, fpch, fpnl);
   mark_end(S, st, CS_P1K_FACE_POINTS);
   mark_begin(S, st);
   k_clsq<<<dim3(nb(nall, 64), 6), 64, 0, st>>>(D, S->all, in);
   mark_end(S, st, CS_P1K_CLSQ);
   const int64_t nfi = (D.N + 3 + kFluxTI - 2) / (kFluxTI - 1), nfj = (D.N + 3 + kFluxTJ - 2) / (kFluxTJ - 1);
+  const int fll = D.NR + 4, flch = (fll + kFluxLevels - 1) / kFluxLevels, flnl = (fll + flch - 1) / flch;
   mark_begin(S, st);
-  k_flux<<<dim3((unsigned)(nfi * nfj), 6), kFluxTI * kFluxTJ, 0, st>>>(D, F[0], F[1], F[2], Sp[0], Sp[1], Sp[2]);
+  k_flux<<<dim3((unsigned)(nfi * nfj * flch), 6), kFluxTI * kFluxTJ, 0, st>>>(D, F[0], F[1], F[2], Sp[0], Sp[1],
+                                                                              Sp[2], flch, flnl);
   mark_end(S, st, CS_P1K_FLUX);
   const unsigned ned = std::max(tile_blocks<32, 8>(S->ex), std::max(tile_blocks<32, 8>(S->ey), tile_blocks<32, 8>(S->ez)));
   mark_begin(S, st);

@@ -12,14 +12,16 @@ Cases named for the benchmark configurations they shrink:
       gravity implicit, hydrostatic state with density noise (radial ghost
       layers by PAD_BOUNDARY_R, outflow at both ends, and, in a second case,
       by ADD_BOUNDARY with the problem's own state);
-  plus the reference's own combination (HLL, Heun RK2) on a shell, and a
-  blast with log-normal noise so the positivity limiter removes slopes and
-  its loop-order replay runs on the device.
+  plus the reference's own combination (HLL, Heun RK2) on a shell, a
+  6x8^2x20 shell tall enough that the face kernels split every column's
+  march into two chunks of levels, and a blast with log-normal noise so the
+  positivity limiter removes slopes and its loop-order replay runs on the
+  device.
 
 Tolerance: the device evaluates the separable form of the face geometry (a
 few ulps from the oracle's per-face quadrature), lets the compiler fuse
-multiply-adds and solves the constrained least squares through a Cholesky /
-Schur complement, so after two steps every conserved variable agrees with
+multiply-adds and solves the constrained least squares with six unknowns
+eliminated in the zone's own frame, so after two steps every conserved variable agrees with
 the oracle to 1e-9 relative L1 over the owned zones (and the owned face B
 to 1e-9 of its largest magnitude); the next time step to 1e-9."""
 import ctypes
@@ -45,13 +47,16 @@ def _cfg(name):
     if name == "shell8x3_problem_boundary":
         return (cs.make_config(8, 3, 4, cs.RECON_P1ADP, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PROBLEM,
                                cs.RMESH_UNIFORM, 1.0, 1.5, grav=1.0, imex=1, bc_problem=103), 103, 0.0)
+    if name == "shell8x20_hlld_imex":
+        return (cs.make_config(8, 20, 4, cs.RECON_P1ADP, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 2, cs.RBC_PAD,
+                               cs.RMESH_UNIFORM, 1.0, 2.0, grav=1.0, imex=1), 3, 0.0)
     if name == "surface12_blast_noise":
         return cs.make_config(12, 1, 4, cs.RECON_P1ADP, cs.RS_HLLD, cs.EMF_LLF, cs.FP64, 3), 2, 1.5
     raise ValueError(name)
 
 
 CASES = ["C2_surface16_hlld_rk3", "C5_shell8x3_hlld_imex", "shell8x3_hll_rk2", "shell8x3_problem_boundary",
-         "surface12_blast_noise"]
+         "shell8x20_hlld_imex", "surface12_blast_noise"]
 
 
 def _state(cfg, prob, noise, seed=11):
