@@ -1886,19 +1886,37 @@ static void h20_basis(const double* x, double* N, double (*D)[3]) {
   }
 }
 
-/* COORDS_TO_WT_QUADPTS_VOLUME_ISO2 (240coords_to_wt_quadpts_volume_iso2.f:7):
- * 2x2x2 Gauss points (+-1/(2 sqrt 3)), quadrature points, weights Det/8
- * normalised by the volume, the three tangent vectors d x/d(xi, eta, zeta)
- * at each point, volume, and the centroid (vertex mean + weighted offsets). */
-void or_quad_volume(const or_config* c, const double* vert, double* pts, double* wts, double* tan,
-                    double* vol, double* cen) {
+/* COORDS_TO_WT_QUADPTS_VOLUME_ISO2 (240coords_to_wt_quadpts_volume_iso2.f:7)
+ * at NUMORDER nq (2 or 3): nq^3 Gauss points, x fastest (:184-200), at
+ * +-1/(2 sqrt 3) with linear weights 1/2 (:121-132), or at 0, +-sqrt(3/5)/2
+ * with 8/18, 5/18 (:134-147); the tensor weight of point (ix, iy, iz) is
+ * lin(iz) lin(iy) lin(ix), multiplied in that order (the weight table is
+ * filled with x slowest, :169-181).  Outputs: quadrature points, weights
+ * times |det J| normalised by the volume, the three tangent vectors
+ * d x/d(xi, eta, zeta) at each point, volume, and the centroid (vertex mean
+ * plus the weighted offsets). */
+static void quad_volume_n(const or_config* c, int nq, const double* vert, double* pts, double* wts, double* tan,
+                          double* vol, double* cen) {
   geo_dims d = geo_dims_of(c);
   int N = d.N, NR = d.NR, g = d.g, s, i, j, k, p, n, q;
-  const double xg = 0.5 * sqrt(1.0 / 3.0);
-  double Nb[8][20], Db[8][20][3];
-  for (p = 0; p < 8; ++p) {
-    double x[3] = {(p & 1) ? xg : -xg, (p & 2) ? xg : -xg, (p & 4) ? xg : -xg};
+  const int np = nq * nq * nq;
+  double xg[3], lw[3], Nb[27][20], Db[27][20][3], wq[27];
+  if (nq == 2) {
+    xg[0] = -0.5 * sqrt(1.0 / 3.0);
+    xg[1] = 0.5 * sqrt(1.0 / 3.0);
+    lw[0] = lw[1] = 0.5;
+  } else {
+    xg[0] = -0.5 * sqrt(3.0 / 5.0);
+    xg[1] = 0.0;
+    xg[2] = 0.5 * sqrt(3.0 / 5.0);
+    lw[0] = lw[2] = 5.0 / 18.0;
+    lw[1] = 8.0 / 18.0;
+  }
+  for (p = 0; p < np; ++p) {
+    int ix = p % nq, iy = (p / nq) % nq, iz = p / (nq * nq);
+    double x[3] = {xg[ix], xg[iy], xg[iz]};
     h20_basis(x, Nb[p], Db[p]);
+    wq[p] = lw[iz] * lw[iy] * lw[ix];
   }
   for (s = 0; s < 6; ++s)
     for (k = 1 - g; k <= NR + g; ++k)
@@ -1918,7 +1936,7 @@ void or_quad_volume(const or_config* c, const double* vert, double* pts, double*
           midpoint(X[5], X[6], 1, r2, X[14]); midpoint(X[1], X[2], 1, r1, X[15]);
           midpoint(X[0], X[4], 0, 0.0, X[16]); midpoint(X[1], X[5], 0, 0.0, X[17]);
           midpoint(X[2], X[6], 0, 0.0, X[18]); midpoint(X[3], X[7], 0, 0.0, X[19]);
-          for (p = 0; p < 8; ++p) {
+          for (p = 0; p < np; ++p) {
             double T[3][3], Dx, Dy, Dz, det;
             for (q = 0; q < 3; ++q) {
               double x = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
@@ -1928,64 +1946,83 @@ void or_quad_volume(const or_config* c, const double* vert, double* pts, double*
                 t1 = t1 + X[n][q] * Db[p][n][1];
                 t2 = t2 + X[n][q] * Db[p][n][2];
               }
-              pts[b * 24 + o + (q + 3 * p) * S] = x;
+              pts[b * 3 * np + o + (q + 3 * p) * S] = x;
               T[0][q] = t0; T[1][q] = t1; T[2][q] = t2;
             }
             Dx = T[0][1] * T[1][2] - T[0][2] * T[1][1];
             Dy = T[0][2] * T[1][0] - T[0][0] * T[1][2];
             Dz = T[0][0] * T[1][1] - T[0][1] * T[1][0];
             det = fabs(T[2][0] * Dx + T[2][1] * Dy + T[2][2] * Dz);
-            wts[b * 8 + o + p * S] = 0.125 * det;
-            for (q = 0; q < 3; ++q)
-              for (n = 0; n < 3; ++n) tan[b * 72 + o + (q + 3 * (n + 3 * p)) * S] = T[n][q];
+            wts[b * np + o + p * S] = wq[p] * det;
+            if (tan)
+              for (q = 0; q < 3; ++q)
+                for (n = 0; n < 3; ++n) tan[b * 9 * np + o + (q + 3 * (n + 3 * p)) * S] = T[n][q];
           }
-          for (p = 0; p < 8; ++p) v = v + wts[b * 8 + o + p * S];
+          for (p = 0; p < np; ++p) v = v + wts[b * np + o + p * S];
           v = fmax(SMALLNUM, v);
           vol[b + o] = v;
-          for (p = 0; p < 8; ++p) wts[b * 8 + o + p * S] = wts[b * 8 + o + p * S] / v;
+          for (p = 0; p < np; ++p) wts[b * np + o + p * S] = wts[b * np + o + p * S] / v;
           for (q = 0; q < 3; ++q)
             c0[q] = 0.125 * (X[0][q] + X[1][q] + X[2][q] + X[3][q] + X[4][q] + X[5][q] + X[6][q] + X[7][q]);
-          for (p = 0; p < 8; ++p)
-            for (q = 0; q < 3; ++q) tmp[q] = tmp[q] + wts[b * 8 + o + p * S] * (pts[b * 24 + o + (q + 3 * p) * S] - c0[q]);
+          for (p = 0; p < np; ++p)
+            for (q = 0; q < 3; ++q)
+              tmp[q] = tmp[q] + wts[b * np + o + p * S] * (pts[b * 3 * np + o + (q + 3 * p) * S] - c0[q]);
           for (q = 0; q < 3; ++q) cen[b * 3 + o + q * S] = c0[q] + tmp[q];
         }
 }
+void or_quad_volume(const or_config* c, const double* vert, double* pts, double* wts, double* tan,
+                    double* vol, double* cen) {
+  quad_volume_n(c, 2, vert, pts, wts, tan, vol, cen);
+}
+void or_quad_volume3(const or_config* c, const double* vert, double* pts, double* wts, double* tan,
+                     double* vol, double* cen) {
+  quad_volume_n(c, 3, vert, pts, wts, tan, vol, cen);
+}
 
-/* BUILD_MODE_INDICES (204build_mode_indices.f:7) at second order and
- * INIT_BASIS_FUNCTION_COEFS (250init_basis_function_coefs.f:5, :150-267):
- * moments <x^a y^b z^c> of the zone's quadrature points about the centroid in
- * units of the scale length, mode 1 zero; inactive zones keep zeros. */
+/* BUILD_MODE_INDICES (204build_mode_indices.f:7) and INIT_BASIS_FUNCTION_COEFS
+ * (250init_basis_function_coefs.f:5, :150-267): moments <x^a y^b z^c> of the
+ * zone's np quadrature points about the centroid in units of the scale
+ * length, summed over the points in order, mode 1 zero; inactive zones keep
+ * zeros.  Second order: modes 1, x, y, z; third order adds x^2, y^2, z^2,
+ * xy, yz, xz (204...f:55-95). */
+static const int kModePow[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0},
+                                    {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}};
 void or_mode_indices(int32_t* ii, int32_t* jj, int32_t* kk) {
   int m;
-  for (m = 0; m < 4; ++m) { ii[m] = m == 1; jj[m] = m == 2; kk[m] = m == 3; }
+  for (m = 0; m < 4; ++m) { ii[m] = kModePow[m][0]; jj[m] = kModePow[m][1]; kk[m] = kModePow[m][2]; }
 }
-void or_basis_moments(const or_config* c, const int32_t* mask, const double* pts, const double* wts,
-                      const double* cen, const double* scl, double* coef) {
+static double ipow(double x, int e) { return e == 0 ? 1.0 : (e == 1 ? x : x * x); }
+static void basis_moments_n(const or_config* c, int np, int nm, const int32_t* mask, const double* pts,
+                            const double* wts, const double* cen, const double* scl, double* coef) {
   geo_dims d = geo_dims_of(c);
-  int32_t ii[4], jj[4], kk[4];
   size_t S = d.zs, z, s;
   int m, p, q;
-  or_mode_indices(ii, jj, kk);
-  memset(coef, 0, sizeof(double) * S * 6 * 4);
+  memset(coef, 0, sizeof(double) * S * 6 * nm);
   for (s = 0; s < 6; ++s)
     for (z = 0; z < S; ++z) {
       size_t o = s * S + z;
       double aa;
       if (mask[o] != 1) continue;
       aa = 1.0 / fmax(SMALLNUM, fabs(scl[o]));
-      for (m = 1; m < 4; ++m) {
+      for (m = 1; m < nm; ++m) {
         double t = 0.0;
-        for (p = 0; p < 8; ++p) {
-          double x[3], v = 1.0;
-          for (q = 0; q < 3; ++q) x[q] = (pts[s * S * 24 + z + (q + 3 * p) * S] - cen[s * S * 3 + z + q * S]) * aa;
-          if (ii[m]) v *= x[0];
-          if (jj[m]) v *= x[1];
-          if (kk[m]) v *= x[2];
-          t = t + wts[s * S * 8 + z + p * S] * v;
+        for (p = 0; p < np; ++p) {
+          double x[3];
+          for (q = 0; q < 3; ++q) x[q] = (pts[s * S * 3 * np + z + (q + 3 * p) * S] - cen[s * S * 3 + z + q * S]) * aa;
+          const double v = ipow(x[0], kModePow[m][0]) * ipow(x[1], kModePow[m][1]) * ipow(x[2], kModePow[m][2]);
+          t = t + wts[s * S * np + z + p * S] * v;
         }
-        coef[s * S * 4 + z + m * S] = coef[s * S * 4 + z + m * S] + t;
+        coef[s * S * nm + z + m * S] = coef[s * S * nm + z + m * S] + t;
       }
     }
+}
+void or_basis_moments(const or_config* c, const int32_t* mask, const double* pts, const double* wts,
+                      const double* cen, const double* scl, double* coef) {
+  basis_moments_n(c, 8, 4, mask, pts, wts, cen, scl, coef);
+}
+void or_basis_moments10(const or_config* c, const int32_t* mask, const double* pts, const double* wts,
+                        const double* cen, const double* scl, double* coef) {
+  basis_moments_n(c, 27, 10, mask, pts, wts, cen, scl, coef);
 }
 
 /* STENCIL_INDEXING_TRICORNERS (211stencil_indexing_tricorners.f:6) for a

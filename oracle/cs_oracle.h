@@ -94,6 +94,11 @@ void or_mode_indices(int32_t* ii, int32_t* jj, int32_t* kk);
 void or_stencil_indexing(int N, int g, int32_t* st);
 void or_basis_moments(const or_config* c, const int32_t* mask, const double* pts, const double* wts,
                       const double* cen, const double* scl, double* coef);
+/* the same at third order (NUMORDER 3): 27 volume points, 10 modes */
+void or_quad_volume3(const or_config* c, const double* vert, double* pts, double* wts, double* tan,
+                     double* vol, double* cen);
+void or_basis_moments10(const or_config* c, const int32_t* mask, const double* pts, const double* wts,
+                        const double* cen, const double* scl, double* coef);
 
 /* LIMIT_ALL_SLOPES_P1 (410limit_all_slopes_p1.f:9) with the pressure
  * positivity of 440 for one patch (cs_oracle_p1.c).  Field order and types
@@ -251,4 +256,16 @@ typedef struct or_p1_init {
 } or_p1_init;
 int or_init_patch_cube_sphere(const or_p1_init* p, const or_prob* pb, int32_t layers);
 
+
+/* third order (cs_oracle_p1.c): MATINV (121matinv.f:8) of a row-major
+ * n x n matrix (n <= 18; m destroyed), RECONSTRUCTION_COEFS (251...f:3) at
+ * order 3, step 7.b of LIMIT_ALL_SLOPES_P12 (421...f:3811-4557) and steps 2
+ * to 7.b of it; U and V of the patch hold ten modes, mom the ten basis
+ * moments of each zone (or_basis_moments10) */
+int or_matinv(int n, double* m, double* minv);
+void or_reconstruction_row3(double x, double y, double z, double l0, double lj, const double* c0, const double* cj,
+                            double* row);
+int or_weno_ao_p12(const or_p1_patch* p, const double* mom);
+int or_limit_slopes_p12_zones(const or_p1_patch* p, const double* mom);
+int or_ao_fit_p12(const or_p1_patch* p, const double* mom, int i, int j, int k, int c, double* X);
 #endif

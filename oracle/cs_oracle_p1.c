@@ -175,8 +175,8 @@ static void weno_apply(const weno_mats* W, double f0, const double* f6, double s
  * pivoting (Crout), then forward/back substitution: Press et al., Numerical
  * Recipes, 2nd ed., sec. 2.3, as called through MATSOLVE (120matsolve.f). */
 #define LUMAX 18
-static int lu_solve(int n, double a[LUMAX][LUMAX], double* b) {
-  int indx[LUMAX], imax = 0;
+static int lu_dcmp(int n, double a[LUMAX][LUMAX], int* indx) {
+  int imax = 0;
   double vv[LUMAX];
   for (int i = 0; i < n; ++i) {
     double big = 0.0;
@@ -217,6 +217,9 @@ static int lu_solve(int n, double a[LUMAX][LUMAX], double* b) {
       for (int i = j + 1; i < n; ++i) a[i][j] = a[i][j] * r;
     }
   }
+  return 0;
+}
+static void lu_bksb(int n, double a[LUMAX][LUMAX], const int* indx, double* b) {
   int ii = -1;
   for (int i = 0; i < n; ++i) {
     int ip = indx[i];
@@ -233,6 +236,32 @@ static int lu_solve(int n, double a[LUMAX][LUMAX], double* b) {
     for (int j = i + 1; j < n; ++j) s = s - a[i][j] * b[j];
     b[i] = s / a[i][i];
   }
+}
+static int lu_solve(int n, double a[LUMAX][LUMAX], double* b) {
+  int indx[LUMAX];
+  if (lu_dcmp(n, a, indx)) return 1;
+  lu_bksb(n, a, indx, b);
+  return 0;
+}
+/* MATINV (100math/121matinv.f:8): the LU above, then one back-substitution
+ * per column of the identity; m (row-major n x n) is destroyed.  On a
+ * singular matrix the reference continues with its partial factors; here
+ * the inverse is left zero and 1 is returned. */
+int or_matinv(int n, double* m, double* minv) {
+  double a[LUMAX][LUMAX], col[LUMAX];
+  int indx[LUMAX];
+  if (n < 1 || n > LUMAX) return 1;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) a[i][j] = m[i * n + j];
+  for (int i = 0; i < n * n; ++i) minv[i] = 0.0;
+  if (lu_dcmp(n, a, indx)) return 1;
+  for (int j = 0; j < n; ++j) {
+    for (int i = 0; i < n; ++i) col[i] = i == j ? 1.0 : 0.0;
+    lu_bksb(n, a, indx, col);
+    for (int i = 0; i < n; ++i) minv[i * n + j] = col[i];
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) m[i * n + j] = a[i][j];
   return 0;
 }
 
@@ -255,11 +284,13 @@ void or_p1_positivity_stats(long* out) {
 }
 
 /* ------------------------------------------------------------------------ */
-int or_limit_all_slopes_p1(const or_p1_patch* p) {
-  if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1) return 1;
-  const p1dims D = p1_dims(p), *d = &D;
+/* Steps 2 to 7.a of LIMIT_ALL_SLOPES_P1 (410...f:529-3761), shared with
+ * the third-order LIMIT_ALL_SLOPES_P12 (421...f:573-3810): U and V hold nm
+ * modes (4 or 10); the third-order routine leaves the WENO gradients of V
+ * unflattened (421...f:3435-3448: the flattener comes after the quadratic
+ * hybridisation) and flattens only the entropy's (flat_v 0). */
+static void p1_front(const or_p1_patch* p, const p1dims* d, int nm, int flat_v) {
   const int N = d->N, NR = d->NR, g = d->g;
-  g_p1_stats[0] = g_p1_stats[1] = 0;
   const double pi = 4.0 * atan(1.0), ob8pi = 1.0 / (8.0 * pi), ob4pi = 1.0 / (4.0 * pi), gam = p->gamma;
   int i, j, k, c, m, q;
 
@@ -274,10 +305,10 @@ int or_limit_all_slopes_p1(const or_p1_patch* p) {
         p->mean_mol_wt[z] = p->molewt;
         p->msonic[z] = p->smallnum;
         for (c = 0; c < 8; ++c)
-          for (m = 1; m < 4; ++m) UU(p, d, c, m, z) = 0.0;
+          for (m = 1; m < nm; ++m) UU(p, d, c, m, z) = 0.0;
         for (m = 1; m < 4; ++m) p->ent[z + d->NZ * m] = 0.0;
         for (c = 0; c < 8; ++c)
-          for (m = 0; m < 4; ++m) VV(p, d, c, m, z) = 0.0;
+          for (m = 0; m < nm; ++m) VV(p, d, c, m, z) = 0.0;
         if (p->mask[z] == 0) {
           for (c = 0; c < 8; ++c) UU(p, d, c, 0, z) = 0.0;
           UU(p, d, 0, 0, z) = p->rhofloor;
@@ -427,7 +458,7 @@ int or_limit_all_slopes_p1(const or_p1_patch* p) {
             f0v = p->ent[z];
             for (q = 0; q < 6; ++q) f6[q] = p->ent[z6[q]];
           }
-          weno_apply(&W, f0v, f6, p->smallnum, p->flatten[z], g3);
+          weno_apply(&W, f0v, f6, p->smallnum, (c < 8 && !flat_v) ? 0.0 : p->flatten[z], g3);
           for (m = 0; m < 3; ++m) {
             if (c < 8)
               VV(p, d, c, m + 1, z) = g3[m];
@@ -436,6 +467,17 @@ int or_limit_all_slopes_p1(const or_p1_patch* p) {
           }
         }
       }
+
+}
+
+int or_limit_all_slopes_p1(const or_p1_patch* p) {
+  if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1) return 1;
+  const p1dims D = p1_dims(p), *d = &D;
+  const int N = d->N, NR = d->NR, g = d->g;
+  g_p1_stats[0] = g_p1_stats[1] = 0;
+  const double pi = 4.0 * atan(1.0), ob8pi = 1.0 / (8.0 * pi), ob4pi = 1.0 / (4.0 * pi), gam = p->gamma;
+  int i, j, k, c, m, q;
+  p1_front(p, d, 4, 1);
 
   /* STEP 7.b: ENFORCE_PRESSURE_POSITIVITY_PRIM (440...f:6-830) */
   {
@@ -2029,4 +2071,164 @@ int or_init_patch_cube_sphere(const or_p1_init* p, const or_prob* pb, int32_t la
 #undef LAYER
 #undef ZLAYER
   return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Third order: LIMIT_ALL_SLOPES_P12 (421limit_all_slopes_p12.f:9), steps 2
+ * to 7.b, for one patch whose U and V hold ten modes (mean, x, y, z, x^2,
+ * y^2, z^2, xy, yz, xz: 204build_mode_indices.f:55-95) and whose basis
+ * moments (moments, zone x 10) are those of the 27-point volume rule.
+ * Steps 7.c (positivity at third order), 8 (face B and the 30-unknown
+ * constrained least squares) and 9 (zone E) are not restated. */
+
+/* RECONSTRUCTION_COEFS (200mesh/251reconstruction_coefs.f:3) at order 3: the
+ * row of zone j in the quadratic fit about zone 0, (x, y, z) the scaled
+ * centroid offset, l0, lj the two scale lengths, c0, cj the two zones' ten
+ * basis moments: the average over zone j of zone 0's basis functions */
+void or_reconstruction_row3(double x, double y, double z, double l0, double lj, const double* c0, const double* cj,
+                            double* row) {
+  const double rho = lj / l0, r2 = rho * rho;
+  row[0] = x;
+  row[1] = y;
+  row[2] = z;
+  row[3] = x * x - c0[4] + cj[4] * r2;
+  row[4] = y * y - c0[5] + cj[5] * r2;
+  row[5] = z * z - c0[6] + cj[6] * r2;
+  row[6] = x * y - c0[7] + cj[7] * r2;
+  row[7] = y * z - c0[8] + cj[8] * r2;
+  row[8] = x * z - c0[9] + cj[9] * r2;
+}
+
+/* the 18 zones of the central quadratic stencil (421...f:3888-4350, in that
+ * order): stencil directions (0-based R L U D .. RU LU LD RD) and k offsets */
+static const int kAoDir[18] = {0, 1, 2, 3, -1, -1, 8, 9, 10, 11, 0, 1, 0, 1, 2, 3, 2, 3};
+static const int kAoDk[18] = {0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+
+/* the least-squares operator of the quadratic stencil of zone (i, j, k):
+ * the 18 x 9 rows A and (A^T A)^-1 (421...f:3870-4370); returns the zones */
+static void ao_build(const or_p1_patch* p, const p1dims* d, const double* mom, int i, int j, int k, long* z18,
+                     double A[18][9], double inv[81]) {
+  const long z = zi(d, i, j, k);
+  const double ell0 = omax(p->scale[z], p->smallnum), obl0 = 1.0 / ell0;
+  double c0[10], cj[10], ata[81];
+  for (int m = 0; m < 10; ++m) c0[m] = mom[z + d->NZ * m];
+  for (int r = 0; r < 18; ++r) {
+    int a = i, b = j;
+    if (kAoDir[r] >= 0) nbr(p, d, i, j, kAoDir[r], &a, &b);
+    const long zj = zi(d, a, b, k + kAoDk[r]);
+    z18[r] = zj;
+    double x[3];
+    for (int c = 0; c < 3; ++c) x[c] = (p->v_centroid[zj + d->NZ * c] - p->v_centroid[z + d->NZ * c]) * obl0;
+    const double ellj = omax(p->scale[zj], p->smallnum);
+    for (int m = 0; m < 10; ++m) cj[m] = mom[zj + d->NZ * m];
+    or_reconstruction_row3(x[0], x[1], x[2], ell0, ellj, c0, cj, A[r]);
+  }
+  for (int a = 0; a < 9; ++a)
+    for (int b = 0; b < 9; ++b) {
+      double t = 0.0;
+      for (int r = 0; r < 18; ++r) t = t + A[r][a] * A[r][b];
+      ata[a * 9 + b] = t;
+    }
+  or_matinv(9, ata, inv);
+}
+
+/* Step 7.b (421...f:3811-4557): on the active zones of -1 .. N+2 (k -1 ..
+ * NR+2), the 9 modes X of the quadratic least-squares fit to the 18 mean
+ * differences, hybridised with the WENO gradients of 7.a (MR-WENO: linear
+ * weights 0.95 / 0.05 for smooth flow, 0.6 / 0.4 where the flattener is 1,
+ * smoothness indicators of the two polynomials, weights (1 + tau / IS)),
+ * then every mode above the mean times (1 - flattener).  The eight
+ * primitives only; the entropy keeps its flattened gradients. */
+int or_weno_ao_p12(const or_p1_patch* p, const double* mom) {
+  if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1 || !mom) return 1;
+  const p1dims D = p1_dims(p), *d = &D;
+  const int N = d->N, NR = d->NR;
+  const double ob12 = 1.0 / 12.0, ob180 = 1.0 / 180.0, ob144 = 1.0 / 144.0, sm = p->smallnum;
+  for (int k = -1; k <= NR + 2; ++k)
+    for (int j = -1; j <= N + 2; ++j)
+      for (int i = -1; i <= N + 2; ++i) {
+        if (!active(p, d, i, j, 1)) continue;
+        const long z = zi(d, i, j, k);
+        long z18[18];
+        double A[18][9], inv[81];
+        ao_build(p, d, mom, i, j, k, z18, A, inv);
+        const double f = p->flatten[z];
+        const double betalo = f + (1.0 - f) * 2.0;
+        const double gammahi = f * 0.6 + (1.0 - f) * 0.95, gammalo = 1.0 - gammahi;
+        const double obgammahi = 1.0 / gammahi, gammaratio = gammalo / gammahi;
+        const double* c0 = mom;
+        for (int c = 0; c < 8; ++c) {
+          double R[18], Y[9], X[9], Phi[9];
+          const double v0 = VV(p, d, c, 0, z);
+          for (int r = 0; r < 18; ++r) R[r] = VV(p, d, c, 0, z18[r]) - v0;
+          for (int a = 0; a < 9; ++a) {
+            double t = 0.0;
+            for (int r = 0; r < 18; ++r) t = t + A[r][a] * R[r];
+            Y[a] = t;
+          }
+          for (int a = 0; a < 9; ++a) {
+            double t = 0.0;
+            for (int b = 0; b < 9; ++b) t = t + inv[a * 9 + b] * Y[b];
+            X[a] = t;
+          }
+          const double g1 = VV(p, d, c, 1, z), g2 = VV(p, d, c, 2, z), g3 = VV(p, d, c, 3, z);
+          const double ISlo = betalo * (g1 * g1 + g2 * g2 + g3 * g3) * ob12;
+          Phi[0] = X[0] * obgammahi - g1 * gammaratio;
+          Phi[1] = X[1] * obgammahi - g2 * gammaratio;
+          Phi[2] = X[2] * obgammahi - g3 * gammaratio;
+          for (int a = 3; a < 9; ++a) Phi[a] = X[a] * obgammahi;
+          const double m5 = c0[z + d->NZ * 4] - ob12, m6 = c0[z + d->NZ * 5] - ob12, m7 = c0[z + d->NZ * 6] - ob12;
+          const double m8 = c0[z + d->NZ * 7], m9 = c0[z + d->NZ * 8], m10 = c0[z + d->NZ * 9];
+          const double IShi = (Phi[0] * Phi[0] + Phi[1] * Phi[1] + Phi[2] * Phi[2]) * ob12 +
+                              (Phi[3] * Phi[3]) * (m5 * m5 + ob180) + (Phi[4] * Phi[4]) * (m6 * m6 + ob180) +
+                              (Phi[5] * Phi[5]) * (m7 * m7 + ob180) + (Phi[6] * Phi[6]) * (m8 * m8 + ob144) +
+                              (Phi[7] * Phi[7]) * (m9 * m9 + ob144) + (Phi[8] * Phi[8]) * (m10 * m10 + ob144);
+          const double tau = fabs(IShi - ISlo);
+          double wlo = gammalo * (1.0 + tau / (ISlo + sm)), whi = gammahi * (1.0 + tau / (IShi + sm));
+          const double aa = 1.0 / (wlo + whi);
+          wlo = wlo * aa;
+          whi = whi * aa;
+          double out[9];
+          out[0] = whi * Phi[0] + wlo * g1;
+          out[1] = whi * Phi[1] + wlo * g2;
+          out[2] = whi * Phi[2] + wlo * g3;
+          for (int a = 3; a < 9; ++a) out[a] = whi * Phi[a];
+          const double x = 1.0 - f;
+          for (int a = 0; a < 9; ++a) VV(p, d, c, a + 1, z) = out[a] * x;
+        }
+      }
+  return 0;
+}
+
+/* the quadratic least-squares modes X (9) of component c of zone (i, j, k)
+ * from the means in V (mode 1), before the hybridisation: the fit of step
+ * 7.b alone, for tests of its exactness */
+int or_ao_fit_p12(const or_p1_patch* p, const double* mom, int i, int j, int k, int c, double* X) {
+  if (!p || !mom || c < 0 || c > 7) return 1;
+  const p1dims D = p1_dims(p), *d = &D;
+  long z18[18];
+  double A[18][9], inv[81], R[18], Y[9];
+  const long z = zi(d, i, j, k);
+  ao_build(p, d, mom, i, j, k, z18, A, inv);
+  for (int r = 0; r < 18; ++r) R[r] = VV(p, d, c, 0, z18[r]) - VV(p, d, c, 0, z);
+  for (int a = 0; a < 9; ++a) {
+    double t = 0.0;
+    for (int r = 0; r < 18; ++r) t = t + A[r][a] * R[r];
+    Y[a] = t;
+  }
+  for (int a = 0; a < 9; ++a) {
+    double t = 0.0;
+    for (int b = 0; b < 9; ++b) t = t + inv[a * 9 + b] * Y[b];
+    X[a] = t;
+  }
+  return 0;
+}
+
+/* the zone-centred part of LIMIT_ALL_SLOPES_P12: steps 2 to 7.a with ten
+ * modes and unflattened WENO gradients, then 7.b */
+int or_limit_slopes_p12_zones(const or_p1_patch* p, const double* mom) {
+  if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1 || !mom) return 1;
+  const p1dims D = p1_dims(p);
+  p1_front(p, &D, 10, 0);
+  return or_weno_ao_p12(p, mom);
 }

@@ -80,6 +80,19 @@ def rlib(openmp=False):
     return _r
 
 
+_r3 = None
+
+
+def rlib3():
+    """The reference's third-order routines (NUMORDER 3 build, libcsref3.so)
+    or None."""
+    global _r3
+    path = os.path.join(os.path.dirname(REFLIB), "libcsref3.so")
+    if _r3 is None and os.path.exists(path):
+        _r3 = ctypes.CDLL(path)
+    return _r3
+
+
 def P(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
@@ -1197,3 +1210,100 @@ def init_problem_p1(cfg, problem, seed=0, time=0.0, layers=0, state=None, geo=No
         By[s * nfc:(s + 1) * nfc] = A["Byf"]
         Bz[s * nfc:(s + 1) * nfc] = A["Bzf"]
     return U, ent, Bx, By, Bz
+
+
+# ---------------------------------------------------------------------------
+# Third order (NUMORDER 3): the 27-point volume rule, ten basis moments and
+# the zone-centred part of LIMIT_ALL_SLOPES_P12 on one patch.
+
+def volume_geometry3(cfg, geo, use_reference=False):
+    """COORDS_TO_WT_QUADPTS_VOLUME_ISO2 at NUMORDER 3 (27 points) and
+    INIT_BASIS_FUNCTION_COEFS with ten modes, every sector, from the vertices,
+    mask and scale lengths of quad_geometry(cfg): v_pts (zone, 3, 27), v_wts
+    (zone, 27), v_tan (zone, 3, 3, 27), v_vol, v_cen (zone, 3), moments
+    (zone, 10), sector slowest."""
+    gd = geometry_dims(cfg)
+    nzn = gd["zone"]
+    a = dict(v_pts=np.zeros(6 * nzn * 81), v_wts=np.zeros(6 * nzn * 27), v_tan=np.zeros(6 * nzn * 243),
+             v_vol=np.zeros(6 * nzn), v_cen=np.zeros(6 * nzn * 3), moments=np.zeros(6 * nzn * 10))
+    N, NR, g = cfg.ntp, cfg.nr, cfg.nghost
+    vp = ctypes.c_void_p
+    if use_reference:
+        R = rlib3()
+        if R is None:
+            raise RuntimeError("third-order reference build not present")
+        R.ref3_quad_volume(N, NR, g, P(geo["vert"]), P(geo["mask"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_tan"]),
+                           P(a["v_vol"]), P(a["v_cen"]))
+        R.ref3_basis_moments(N, NR, g, P(geo["mask"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_vol"]), P(a["v_cen"]),
+                             P(geo["scale"]), P(a["moments"]))
+        return a
+    L = olib()
+    L.or_quad_volume3.argtypes = [vp] * 7
+    L.or_basis_moments10.argtypes = [vp] * 7
+    L.or_quad_volume3(ctypes.byref(cfg), P(geo["vert"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_tan"]), P(a["v_vol"]),
+                      P(a["v_cen"]))
+    L.or_basis_moments10(ctypes.byref(cfg), P(geo["mask"]), P(a["v_pts"]), P(a["v_wts"]), P(a["v_cen"]),
+                         P(geo["scale"]), P(a["moments"]))
+    return a
+
+
+P12_EXTRA = ("xf_pts", "xf_wts", "yf_pts", "yf_wts", "zf_pts", "zf_wts", "zf_normal", "v_volume", "moments", "Wzc")
+
+
+class P12Extra(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_void_p) for k in P12_EXTRA]
+
+
+def p12_problem(N, NR, g=4, sector=0, seed=0, shock=False, rmin=1.0, rmax=1.6, gamma=5.0 / 3.0):
+    """One sector as a patch for the third-order routine: the arrays of
+    p1_problem with ten-mode U, V, B*_zc, E*_zc and Wzc (seeded junk above
+    the means), the 27-point centroids and volumes, the ten basis moments,
+    and the face quadrature arrays the routine declares."""
+    import cubesphere_amd as cs  # Config layout only
+    prob, A = p1_problem(N, NR, g, sector=sector, seed=seed, shock=shock, rmin=rmin, rmax=rmax, gamma=gamma)
+    cfg = cs.make_config(N, NR, g, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2, rmin=rmin, rmax=rmax,
+                         gamma=gamma)
+    geo = quad_geometry(cfg)
+    g3 = volume_geometry3(cfg, geo)
+    gd = geometry_dims(cfg)
+    d = p1_dims(N, NR, g)
+    nzone = d["zone"]
+    sl = lambda a, n, m: np.ascontiguousarray(a[sector * n * m:(sector + 1) * n * m])
+    rng = np.random.default_rng(seed + 1000)
+    U = rng.normal(size=(10, 8 * nzone))
+    U[0] = A["U"][:8 * nzone]
+    A["U"] = U.ravel().copy()
+    A["V"] = rng.normal(size=80 * nzone)
+    for k in ("B1_zc", "B2_zc", "B3_zc", "Ex_zc", "Ey_zc", "Ez_zc", "Wzc"):
+        A[k] = rng.normal(size=10 * nzone)
+    A["v_centroid"] = sl(g3["v_cen"], gd["zone"], 3)
+    A["v_volume"] = sl(g3["v_vol"], gd["zone"], 1)
+    A["moments"] = sl(g3["moments"], gd["zone"], 10)
+    for f in ("xf", "yf", "zf"):
+        A[f + "_wts"] = sl(geo[f]["wts"], gd[f], 5)
+    return prob, A
+
+
+def p12_oracle(prob, A):
+    """or_limit_slopes_p12_zones in place on A (steps 2 to 7.b)."""
+    L = olib()
+    L.or_limit_slopes_p12_zones.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    s = p1_struct(prob, A)
+    if L.or_limit_slopes_p12_zones(ctypes.byref(s), P(A["moments"])) != 0:
+        raise RuntimeError("oracle rejected the patch")
+    return A
+
+
+def p12_reference(prob, A):
+    """The reference's whole LIMIT_ALL_SLOPES_P12 (NUMORDER 3 build) in place
+    on A."""
+    R = rlib3()
+    if R is None:
+        raise RuntimeError("third-order reference build not present")
+    R.ref3_limit_all_slopes_p12.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    s = p1_struct(prob, A)
+    e = P12Extra()
+    for k in P12_EXTRA:
+        setattr(e, k, A[k].ctypes.data)
+    R.ref3_limit_all_slopes_p12(ctypes.byref(s), ctypes.byref(e))
+    return A
