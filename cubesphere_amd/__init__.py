@@ -155,6 +155,8 @@ def lib():
             "cs_mode_indices": [i32, ctypes.POINTER(i32), vp, vp, vp],
             "cs_stencil_indexing": [cfgp, vp],
             "cs_limit_all_slopes_p1": [vp, vp],
+            "cs_build_volume_p12": [cfgp, vp],
+            "cs_limit_slopes_p12_zones": [vp, vp],
             "cs_limit_all_slopes_p1_timed": [vp, vp, ctypes.POINTER(ctypes.c_float)],
             "cs_limit_all_slopes_p1_noalloc": [vp, vp],
             "cs_make_facial_fluxes_p1": [vp, vp],
@@ -693,6 +695,69 @@ def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=Non
     else:
         _check(lib().cs_limit_all_slopes_p1(ctypes.byref(s), st), "cs_limit_all_slopes_p1")
     return vmin, vmax
+
+
+# ---------------------------------------------------------------------------
+# Third order: the NUMORDER 3 volume geometry and the zone-centred part of
+# LIMIT_ALL_SLOPES_P12 (cs_build_volume_p12, cs_limit_slopes_p12_zones)
+
+class VolumeP12(ctypes.Structure):
+    """cs_volume_p12: host outputs of the third-order volume geometry."""
+    _fields_ = [(k, ctypes.c_void_p) for k in ("v_pts", "v_wts", "v_vol", "v_cen", "moments")]
+
+
+def build_volume_p12(cfg):
+    """COORDS_TO_WT_QUADPTS_VOLUME_ISO2 with the 27-point rule and the ten basis
+    moments of INIT_BASIS_FUNCTION_COEFS at third order, for the six sectors,
+    built on the GPU: dict of v_pts (zone, 3, 27), v_wts (zone, 27), v_vol,
+    v_cen (zone, 3), moments (zone, 10), sector slowest."""
+    nz = geometry_sizes(cfg)["zone"]
+    a = dict(v_pts=np.zeros(81 * nz), v_wts=np.zeros(27 * nz), v_vol=np.zeros(nz), v_cen=np.zeros(3 * nz),
+             moments=np.zeros(10 * nz))
+    o = VolumeP12()
+    for k, v in a.items():
+        setattr(o, k, v.ctypes.data)
+    _check(lib().cs_build_volume_p12(ctypes.byref(cfg), ctypes.byref(o)), "cs_build_volume_p12")
+    return a
+
+
+class P12Patch(ctypes.Structure):
+    """cs_p12_patch: a cs_p1_patch whose U and V hold ten modes, plus the ten
+    basis moments."""
+    _fields_ = [("p1", P1Patch), ("moments", ctypes.c_void_p)]
+
+
+def limit_slopes_p12_zones(N, NR, g, scalars, arrays, moments, itri=(1, 1, 1, 1), stream=None):
+    """Steps 2 to 7.b of LIMIT_ALL_SLOPES_P12 on one patch (the P1 steps up to
+    the WENO, then the WENO-AO hybridisation with the quadratic central
+    stencil), in place on torch CUDA tensors in the reference's layouts: as
+    limit_all_slopes_p1, with U and V of ten modes (zone, 8, 10), v_centroid
+    from the 27-point rule, and moments (zone, 10) a float64 CUDA tensor."""
+    import torch
+    want = p1_counts(N, NR, g)
+    nzone = want["mask"]
+    want["U"] = want["V"] = 80 * nzone
+    s = P12Patch()
+    q = s.p1
+    q.ntp, q.nr, q.nghost = N, NR, g
+    for i in range(4):
+        q.itri[i] = int(itri[i])
+    for k in P1_SCALARS:
+        setattr(q, k, float(scalars[k]))
+    tensors = dict(arrays, moments=moments)
+    want["moments"] = 10 * nzone
+    for k in P1_ARRAYS + ("moments",):
+        t = tensors[k]
+        dt = torch.int32 if k in ("mask", "stencil") else torch.float64
+        if not (t.is_cuda and t.dtype == dt and t.is_contiguous() and t.numel() >= want[k]):
+            raise CsError(f"limit_slopes_p12_zones: {k} must be a contiguous {dt} CUDA tensor of at least "
+                          f"{want[k]} elements (got {t.dtype}, {t.numel()}, cuda={t.is_cuda})")
+        if k == "moments":
+            s.moments = t.data_ptr()
+        else:
+            setattr(q, k, t.data_ptr())
+    st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    _check(lib().cs_limit_slopes_p12_zones(ctypes.byref(s), st), "cs_limit_slopes_p12_zones")
 
 
 # ---------------------------------------------------------------------------

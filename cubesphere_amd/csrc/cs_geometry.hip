@@ -31,6 +31,8 @@ constexpr double kSmall = 1.0e-12;  // RIEMANN_COM smallnum (090init_riemann.f:1
 __constant__ double c_fq_w[5];
 __constant__ double c_q8[5][8][3];    // [point][node][value, d/dx, d/dy]
 __constant__ double c_h20[8][20][4];  // [point][node][value, d/dxi, d/deta, d/dzeta]
+__constant__ double c_h27[27][20][4]; // the same at the 27 points of the third-order rule
+__constant__ double c_w27[27];        // their tensor weights
 
 // 8-node quadrilateral: corners (--, +-, ++, -+), then the edge nodes
 // (0,-), (+,0), (0,+), (-,0)
@@ -105,9 +107,27 @@ int load_tables() {
       for (int a = 0; a < 3; ++a) h20[p][n][1 + a] = g[a];
     }
   }
+  // 3x3x3 Gauss points 0, +-sqrt(3/5)/2 with weights 8/18, 5/18, x fastest;
+  // the weight of point (ix, iy, iz) is lin(iz) lin(iy) lin(ix) in that
+  // order (240…:134-181: the weight table is filled with x slowest)
+  static double h27[27][20][4], w27[27];
+  const double xg3[3] = {-0.5 * sqrt(3.0 / 5.0), 0.0, 0.5 * sqrt(3.0 / 5.0)};
+  const double lw3[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};
+  for (int p = 0; p < 27; ++p) {
+    const int ix = p % 3, iy = (p / 3) % 3, iz = p / 9;
+    double x[3] = {xg3[ix], xg3[iy], xg3[iz]};
+    for (int n = 0; n < 20; ++n) {
+      double g[3];
+      serendipity(3, kH20[n], x, &h27[p][n][0], g);
+      for (int a = 0; a < 3; ++a) h27[p][n][1 + a] = g[a];
+    }
+    w27[p] = lw3[iz] * lw3[iy] * lw3[ix];
+  }
   if (hipMemcpyToSymbol(HIP_SYMBOL(c_fq_w), w, sizeof(w)) != hipSuccess ||
       hipMemcpyToSymbol(HIP_SYMBOL(c_q8), q8, sizeof(q8)) != hipSuccess ||
-      hipMemcpyToSymbol(HIP_SYMBOL(c_h20), h20, sizeof(h20)) != hipSuccess)
+      hipMemcpyToSymbol(HIP_SYMBOL(c_h20), h20, sizeof(h20)) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_h27), h27, sizeof(h27)) != hipSuccess ||
+      hipMemcpyToSymbol(HIP_SYMBOL(c_w27), w27, sizeof(w27)) != hipSuccess)
     return CS_E_HIP;
   if (dev >= 0 && dev < 64) done[dev] = true;
   return CS_OK;
@@ -346,6 +366,82 @@ __global__ void __launch_bounds__(256) k_volume(GDims d, const double* __restric
   }
 }
 
+// The same at NUMORDER 3: 27 points (no tangents are kept).
+__global__ void __launch_bounds__(256) k_volume3(GDims d, const double* __restrict__ vert, VolOut v) {
+  const uint32_t t = tid();
+  if (t >= (uint64_t)6 * d.zs) return;
+  const int s = (int)(t / d.zs);
+  const int64_t o = t - (int64_t)s * d.zs, S = d.zs, b = (int64_t)s * S;
+  const int i = 1 - d.g + (int)(o % d.nc), j = 1 - d.g + (int)((o / d.nc) % d.nc);
+  const int k = 1 - d.g + (int)(o / ((int64_t)d.nc * d.nc));
+  double X[20][3];
+  for (int n = 0; n < 8; ++n)
+    vload(d, vert, s, i - 1 + ((n + 1) >> 1 & 1), j - 1 + ((n >> 1) & 1), k - 1 + (n >> 2), X[n]);
+  const double r1 = len3(X[0]), r2 = len3(X[4]);
+  mid(X[0], X[1], r1, X[8]);  mid(X[2], X[3], r1, X[9]);  mid(X[6], X[7], r2, X[10]); mid(X[4], X[5], r2, X[11]);
+  mid(X[0], X[3], r1, X[12]); mid(X[4], X[7], r2, X[13]); mid(X[5], X[6], r2, X[14]); mid(X[1], X[2], r1, X[15]);
+  for (int n = 0; n < 4; ++n) mid(X[n], X[n + 4], -1.0, X[16 + n]);
+  double vol = 0.0;
+  for (int p = 0; p < 27; ++p) {
+    double T[3][3];
+    for (int c = 0; c < 3; ++c) {
+      double x = 0.0, t0 = 0.0, t1 = 0.0, t2 = 0.0;
+      for (int n = 0; n < 20; ++n) {
+        x = x + X[n][c] * c_h27[p][n][0];
+        t0 = t0 + X[n][c] * c_h27[p][n][1];
+        t1 = t1 + X[n][c] * c_h27[p][n][2];
+        t2 = t2 + X[n][c] * c_h27[p][n][3];
+      }
+      v.pts[81 * b + o + (c + 3 * p) * S] = x;
+      T[0][c] = t0; T[1][c] = t1; T[2][c] = t2;
+    }
+    const double D0 = T[0][1] * T[1][2] - T[0][2] * T[1][1], D1 = T[0][2] * T[1][0] - T[0][0] * T[1][2],
+                 D2 = T[0][0] * T[1][1] - T[0][1] * T[1][0];
+    const double w = c_w27[p] * fabs(T[2][0] * D0 + T[2][1] * D1 + T[2][2] * D2);
+    v.wts[27 * b + o + p * S] = w;
+    vol = vol + w;
+  }
+  vol = fmax(kSmall, vol);
+  v.vol[b + o] = vol;
+  double m[3] = {0.0, 0.0, 0.0}, c0[3];
+  for (int c = 0; c < 3; ++c)
+    c0[c] = 0.125 * (X[0][c] + X[1][c] + X[2][c] + X[3][c] + X[4][c] + X[5][c] + X[6][c] + X[7][c]);
+  for (int p = 0; p < 27; ++p) {
+    const double w = v.wts[27 * b + o + p * S] / vol;
+    v.wts[27 * b + o + p * S] = w;
+    for (int c = 0; c < 3; ++c) m[c] = m[c] + w * (v.pts[81 * b + o + (c + 3 * p) * S] - c0[c]);
+  }
+  for (int c = 0; c < 3; ++c) v.cen[3 * b + o + c * S] = c0[c] + m[c];
+}
+
+// ------------------------------------------------------------------ 250
+// Third-order modes (204…:55-95): 1, x, y, z, x^2, y^2, z^2, xy, yz, xz; the
+// moments of modes 2-10 over the 27 points about the centroid in units of
+// the scale length (inactive zones keep zeros).
+__global__ void k_moments10(GDims d, const int32_t* __restrict__ mask, const double* __restrict__ pts,
+                            const double* __restrict__ wts, const double* __restrict__ cen,
+                            const double* __restrict__ scale, double* mom) {
+  const uint32_t t = tid();
+  if (t >= (uint64_t)6 * d.zs) return;
+  const int s = (int)(t / d.zs);
+  const int64_t o = t - (int64_t)s * d.zs, S = d.zs, b = (int64_t)s * S;
+  double m[10] = {};
+  if (mask[t] == 1) {
+    const double aa = 1.0 / fmax(kSmall, fabs(scale[t]));
+    double ce[3];
+    for (int c = 0; c < 3; ++c) ce[c] = cen[3 * b + o + c * S];
+    for (int p = 0; p < 27; ++p) {
+      const double w = wts[27 * b + o + p * S];
+      double x[3];
+      for (int c = 0; c < 3; ++c) x[c] = (pts[81 * b + o + (c + 3 * p) * S] - ce[c]) * aa;
+      const double f[9] = {x[0], x[1], x[2], x[0] * x[0], x[1] * x[1], x[2] * x[2], x[0] * x[1], x[1] * x[2],
+                           x[0] * x[2]};
+      for (int q = 0; q < 9; ++q) m[1 + q] = m[1 + q] + w * f[q];
+    }
+  }
+  for (int q = 0; q < 10; ++q) mom[10 * b + o + q * S] = m[q];
+}
+
 // ------------------------------------------------------------------ 250
 // Second-order modes: 1, x, y, z; the moments of modes 2-4 about the centroid
 // in units of the scale length (inactive zones keep zeros).
@@ -425,6 +521,78 @@ int copy_out(void* host, const void* dev, size_t bytes, hipStream_t st) {
   return hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, st) == hipSuccess ? CS_OK : CS_E_HIP;
 }
 }  // namespace
+
+// the vertices, mask and scale lengths of the six sectors on the device
+// (201-203 inputs, 210, 220…:2212-2290), shared by the geometry builders
+static int build_vertices(const cs_config* cfg, const GDims& d, DevBuf& B, hipStream_t st, double** vert,
+                          int32_t** mask, double** scale) {
+  std::vector<double> lat((size_t)d.nv * d.nv * 18), rbh((size_t)d.nvr);
+  int rc = cs_build_unit_lattice(cfg, lat.data());
+  if (rc) return rc;
+  for (int s = 0; s < 6; ++s)
+    for (int c = 0; c < 3; ++c)
+      for (int j = 0; j < d.nv; ++j)
+        for (int i = 0; i < d.nv; ++i) {
+          const bool ci = i < d.g || i > d.g + d.N, cj = j < d.g || j > d.g + d.N;
+          if (ci && cj) lat[(size_t)i + (size_t)d.nv * (j + (size_t)d.nv * (c + 3 * (size_t)s))] = 0.0;
+        }
+  if ((rc = cs_build_radial_mesh(cfg, rbh.data()))) return rc;
+  double* dlat = B.get<double>((int64_t)lat.size());
+  double* drb = B.get<double>(d.nvr);
+  *vert = B.get<double>(18 * d.vs);
+  *mask = B.get<int32_t>(6 * d.zs);
+  *scale = B.get<double>(6 * d.zs);
+  if (!dlat || !drb || !*vert || !*mask || !*scale) return CS_E_NOMEM;
+  if (hipMemcpyAsync(dlat, lat.data(), sizeof(double) * lat.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+      hipMemcpyAsync(drb, rbh.data(), sizeof(double) * rbh.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+    return CS_E_HIP;
+  k_vert<<<nb(6 * d.vs), 256, 0, st>>>(d, dlat, drb, *vert);
+  k_mask<<<nb(6 * d.zs), 256, 0, st>>>(d, dlat, *mask);
+  k_scale<<<nb(6 * d.zs), 256, 0, st>>>(d, *vert, *scale);
+  // the lattice and radii must outlive the copies: wait before they go
+  return hipStreamSynchronize(st) == hipSuccess ? CS_OK : CS_E_HIP;
+}
+
+extern "C" int cs_build_volume_p12(const cs_config* cfg, const cs_volume_p12* out) {
+  cs_sizes sz;
+  int rc = cs_sizes_of(cfg, &sz);
+  if (rc) return rc;
+  if (!out) return CS_E_INVALID;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CS_E_HIP;
+  if ((rc = load_tables())) return rc;
+  const GDims d = gdims(cfg);
+  DevBuf B;
+  hipStream_t st;
+  if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return CS_E_HIP;
+  double *vert = nullptr, *scale = nullptr;
+  int32_t* mask = nullptr;
+  rc = build_vertices(cfg, d, B, st, &vert, &mask, &scale);
+  VolOut Vo{};
+  double* mom = nullptr;
+  if (!rc) {
+    Vo.pts = B.get<double>(6 * 81 * d.zs);
+    Vo.wts = B.get<double>(6 * 27 * d.zs);
+    Vo.vol = B.get<double>(6 * d.zs);
+    Vo.cen = B.get<double>(18 * d.zs);
+    mom = B.get<double>(60 * d.zs);
+    if (!Vo.pts || !Vo.wts || !Vo.vol || !Vo.cen || !mom) rc = CS_E_NOMEM;
+  }
+  if (!rc) {
+    k_volume3<<<nb(6 * d.zs), 256, 0, st>>>(d, vert, Vo);
+    k_moments10<<<nb(6 * d.zs), 256, 0, st>>>(d, mask, Vo.pts, Vo.wts, Vo.cen, scale, mom);
+    if (hipGetLastError() != hipSuccess) rc = CS_E_HIP;
+  }
+  const size_t zb = sizeof(double) * 6 * d.zs;
+  if (!rc) rc = copy_out(out->v_pts, Vo.pts, 81 * zb, st);
+  if (!rc) rc = copy_out(out->v_wts, Vo.wts, 27 * zb, st);
+  if (!rc) rc = copy_out(out->v_vol, Vo.vol, zb, st);
+  if (!rc) rc = copy_out(out->v_cen, Vo.cen, 3 * zb, st);
+  if (!rc) rc = copy_out(out->moments, mom, 10 * zb, st);
+  if (hipStreamSynchronize(st) != hipSuccess && !rc) rc = CS_E_HIP;
+  hipStreamDestroy(st);
+  return rc;
+}
 
 extern "C" int cs_build_geometry(const cs_config* cfg, const cs_geometry* out, float* kernel_ms) {
   cs_sizes sz;

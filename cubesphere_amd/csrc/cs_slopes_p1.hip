@@ -336,6 +336,7 @@ struct WenoFields {
   double* dst[9];
   int64_t stride[9];
   int n;
+  uint32_t noflat;  // bit f: field f's gradients are not multiplied by (1 - flattener)
 };
 
 // The WENO of up to three fields of one zone in one pass over the 8
@@ -443,8 +444,9 @@ __global__ void __launch_bounds__(128) k_weno9(const P1 P, const Box b, const in
     const double os = 1.0 / (den[f] + w8);
     double* dst = dsts[f];
     const int64_t stride = strides[f];
+    const double of = ((W9.noflat >> (c0 + f)) & 1u) ? 1.0 : omf;
 #pragma unroll
-    for (int m = 0; m < 3; ++m) dst[z + stride * m] = (num[f][m] + w8 * X8[m]) * os * omf;
+    for (int m = 0; m < 3; ++m) dst[z + stride * m] = (num[f][m] + w8 * X8[m]) * os * of;
   }
 }
 
@@ -1163,6 +1165,7 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
   P1_HIP(hipMemsetAsync(F.counter, 0, 2 * sizeof(unsigned int), st));
   WenoFields wv, we;
   wv.n = 9;
+  wv.noflat = we.noflat = 0;
   for (int c = 0; c < 8; ++c) {
     wv.src[c] = p->V + P.NZ * c;
     wv.dst[c] = p->V + P.NZ * (c + 8);
@@ -2661,4 +2664,231 @@ extern "C" int cs_pad_boundary_r_p1(const cs_p1_pad* p, void* stream_) {
                                                                                        p->nghost, NZ);
   P1_HIP(hipGetLastError());
   return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Third order: steps 2 to 7.b of LIMIT_ALL_SLOPES_P12 (421limit_all_slopes_p12.f:9)
+// on one patch whose U and V hold ten modes.  Steps 2-7.a are the P1 kernels
+// above (they address modes 1-4, which a ten-mode array lays out exactly as
+// a four-mode one); k_zero_hi clears modes 5-10, k_weno_ao is step 7.b.
+namespace csp1 {
+
+__global__ void __launch_bounds__(256) k_zero_hi(const P1 P, const Box b) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int64_t z = P.zi(i, j, k);
+#pragma unroll
+  for (int m = 4; m < 10; ++m)
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+      P.Ur(c, m, z) = 0.0;
+      P.Vr(c, m, z) = 0.0;
+    }
+}
+
+// the 18 zones of the central quadratic stencil (421...f:3888-4350, in that
+// order): stencil direction (R L U D, then RU LU LD RD as 4-7) and k offset
+__constant__ int c_ao_dir[18] = {0, 1, 2, 3, -1, -1, 4, 5, 6, 7, 0, 1, 0, 1, 2, 3, 2, 3};
+__constant__ int c_ao_dk[18] = {0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1};
+
+// packed lower triangle of a symmetric 9 x 9 matrix
+__device__ __forceinline__ constexpr int lt(int r, int c) { return r * (r + 1) / 2 + c; }
+
+// Step 7.b (421...f:3811-4557), one thread per zone: the 18 x 9 rows of
+// RECONSTRUCTION_COEFS (251reconstruction_coefs.f:3) — the average over the
+// stencil zone of this zone's quadratic basis, from the centroid offset, the
+// two scale lengths and the ten basis moments — give the normal equations
+// (A^T A) X = A^T R; A^T A is factored once (Cholesky, in registers) and
+// solved for the eight primitives, four at a time with the rows formed
+// again for the second four.  The fit is hybridised with the P1 gradients
+// (MR-WENO linear weights 0.95 / 0.05 for smooth flow, 0.6 / 0.4 where the
+// flattener is 1; smoothness indicators of the two polynomials; weights
+// (1 + tau / IS)), and every mode above the mean is multiplied by
+// (1 - flattener).  The reference inverts A^T A by LU (MATINV); the two
+// agree to rounding.
+__global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  if (!P.active(i, j, 1)) return;
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k), NZ = P.NZ;
+  const double sm = a.smallnum, ell0 = fmax(a.scale[z], sm), obl0 = 1.0 / ell0;
+  double c0[6], cz[3];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) c0[m] = mom[z + NZ * (4 + m)];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) cz[c] = a.v_centroid[z + NZ * c];
+  int ni[8], nj[8];
+  {
+    const int dirs[8] = {0, 1, 2, 3, 8, 9, 10, 11};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) P.nbr(i, j, dirs[q], ni[q], nj[q]);
+  }
+  const double f = a.flatten[z];
+  const double betalo = f + (1.0 - f) * 2.0;
+  const double gammahi = f * 0.6 + (1.0 - f) * 0.95, gammalo = 1.0 - gammahi;
+  const double obgammahi = 1.0 / gammahi, gammaratio = gammalo / gammahi;
+  const double ob12 = 1.0 / 12.0, ob180 = 1.0 / 180.0, ob144 = 1.0 / 144.0;
+  const double m5 = c0[0] - ob12, m6 = c0[1] - ob12, m7 = c0[2] - ob12;
+  const double q4 = m5 * m5 + ob180, q5 = m6 * m6 + ob180, q6 = m7 * m7 + ob180;
+  const double q7 = c0[3] * c0[3] + ob144, q8 = c0[4] * c0[4] + ob144, q9 = c0[5] * c0[5] + ob144;
+  double L[45];
+#pragma unroll
+  for (int e = 0; e < 45; ++e) L[e] = 0.0;
+#pragma unroll 1
+  for (int grp = 0; grp < 2; ++grp) {
+    double Y[4][9];
+    double v0[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      v0[c] = P.V(4 * grp + c, 0, z);
+#pragma unroll
+      for (int m = 0; m < 9; ++m) Y[c][m] = 0.0;
+    }
+#pragma unroll 1
+    for (int r = 0; r < 18; ++r) {
+      const int dir = c_ao_dir[r];
+      int ii = i, jj = j;
+      if (dir >= 0) {
+        // selects, not an indexed read of ni / nj (which would go to scratch)
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+          if (q == dir) {
+            ii = ni[q];
+            jj = nj[q];
+          }
+      }
+      const int64_t zr = P.zi(ii, jj, k + c_ao_dk[r]);
+      double x[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) x[c] = (a.v_centroid[zr + NZ * c] - cz[c]) * obl0;
+      const double rho = fmax(a.scale[zr], sm) / ell0, r2 = rho * rho;
+      double row[9];
+      row[0] = x[0];
+      row[1] = x[1];
+      row[2] = x[2];
+      row[3] = x[0] * x[0] - c0[0] + mom[zr + NZ * 4] * r2;
+      row[4] = x[1] * x[1] - c0[1] + mom[zr + NZ * 5] * r2;
+      row[5] = x[2] * x[2] - c0[2] + mom[zr + NZ * 6] * r2;
+      row[6] = x[0] * x[1] - c0[3] + mom[zr + NZ * 7] * r2;
+      row[7] = x[1] * x[2] - c0[4] + mom[zr + NZ * 8] * r2;
+      row[8] = x[0] * x[2] - c0[5] + mom[zr + NZ * 9] * r2;
+      if (grp == 0) {
+#pragma unroll
+        for (int p = 0; p < 9; ++p)
+#pragma unroll
+          for (int q = 0; q <= p; ++q) L[lt(p, q)] += row[p] * row[q];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const double R = P.V(4 * grp + c, 0, zr) - v0[c];
+#pragma unroll
+        for (int m = 0; m < 9; ++m) Y[c][m] += row[m] * R;
+      }
+    }
+    if (grp == 0) {  // Cholesky factor of A^T A in place
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {
+        double d = L[lt(p, p)];
+#pragma unroll
+        for (int q = 0; q < p; ++q) d -= L[lt(p, q)] * L[lt(p, q)];
+        d = sqrt(fmax(d, 1.0e-300));
+        L[lt(p, p)] = d;
+        const double od = 1.0 / d;
+#pragma unroll
+        for (int r = p + 1; r < 9; ++r) {
+          double t = L[lt(r, p)];
+#pragma unroll
+          for (int q = 0; q < p; ++q) t -= L[lt(r, q)] * L[lt(p, q)];
+          L[lt(r, p)] = t * od;
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double X[9];
+#pragma unroll
+      for (int p = 0; p < 9; ++p) {  // L y = Y
+        double t = Y[c][p];
+#pragma unroll
+        for (int q = 0; q < p; ++q) t -= L[lt(p, q)] * X[q];
+        X[p] = t / L[lt(p, p)];
+      }
+#pragma unroll
+      for (int p = 8; p >= 0; --p) {  // L^T x = y
+        double t = X[p];
+#pragma unroll
+        for (int q = p + 1; q < 9; ++q) t -= L[lt(q, p)] * X[q];
+        X[p] = t / L[lt(p, p)];
+      }
+      const int comp = 4 * grp + c;
+      const double g1 = P.V(comp, 1, z), g2 = P.V(comp, 2, z), g3 = P.V(comp, 3, z);
+      const double ISlo = betalo * (g1 * g1 + g2 * g2 + g3 * g3) * ob12;
+      double Phi[9];
+      Phi[0] = X[0] * obgammahi - g1 * gammaratio;
+      Phi[1] = X[1] * obgammahi - g2 * gammaratio;
+      Phi[2] = X[2] * obgammahi - g3 * gammaratio;
+#pragma unroll
+      for (int m = 3; m < 9; ++m) Phi[m] = X[m] * obgammahi;
+      const double IShi = (Phi[0] * Phi[0] + Phi[1] * Phi[1] + Phi[2] * Phi[2]) * ob12 + Phi[3] * Phi[3] * q4 +
+                          Phi[4] * Phi[4] * q5 + Phi[5] * Phi[5] * q6 + Phi[6] * Phi[6] * q7 +
+                          Phi[7] * Phi[7] * q8 + Phi[8] * Phi[8] * q9;
+      const double tau = fabs(IShi - ISlo);
+      double wlo = gammalo * (1.0 + tau / (ISlo + sm)), whi = gammahi * (1.0 + tau / (IShi + sm));
+      const double aa = 1.0 / (wlo + whi);
+      wlo *= aa;
+      whi *= aa;
+      const double x1 = 1.0 - f;
+      P.Vr(comp, 1, z) = (whi * Phi[0] + wlo * g1) * x1;
+      P.Vr(comp, 2, z) = (whi * Phi[1] + wlo * g2) * x1;
+      P.Vr(comp, 3, z) = (whi * Phi[2] + wlo * g3) * x1;
+#pragma unroll
+      for (int m = 3; m < 9; ++m) P.Vr(comp, m + 1, z) = whi * Phi[m] * x1;
+    }
+  }
+}
+
+}  // namespace csp1
+
+extern "C" int cs_limit_slopes_p12_zones(const cs_p12_patch* q, void* stream_) {
+  if (!q || !q->moments) {
+    cs::set_error("cs_limit_slopes_p12_zones: null patch or moments");
+    return CS_E_INVALID;
+  }
+  const cs_p1_patch* p = &q->p1;
+  int rc = p1_check(p);
+  if (rc) return rc;
+  P1 P;
+  if ((rc = p1_dims_check(p, "cs_limit_slopes_p12_zones", P))) return rc;
+  hipStream_t st = (hipStream_t)stream_;
+  const int N = P.N, NR = P.NR, g = P.g;
+  const Box all = make_box(1 - g, N + g, 1 - g, N + g, 1 - g, NR + g);
+  const Box wr = make_box(-2, N + 3, -2, N + 3, -2, NR + 3);
+  const Box ao = make_box(-1, N + 2, -1, N + 2, -1, NR + 2);
+  k_zero_hi<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  k_prim<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  k_face_zero<<<dim3(grid_of(P.NFC, 256), 3), 256, 0, st>>>(P);
+  k_tricorner<<<grid_of(4 * (int64_t)g * P.nzr, 256), 256, 0, st>>>(P);
+  k_flat_a<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  k_flat_b<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  k_flat_c<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
+  WenoFields wv;
+  wv.n = 9;
+  wv.noflat = 0xffu;  // the eight primitives: flattened after 7.b
+  for (int c = 0; c < 8; ++c) {
+    wv.src[c] = p->V + P.NZ * c;
+    wv.dst[c] = p->V + P.NZ * (c + 8);
+    wv.stride[c] = 8 * P.NZ;
+  }
+  wv.src[8] = p->ent;
+  wv.dst[8] = p->ent + P.NZ;
+  wv.stride[8] = P.NZ;
+  k_weno9<<<grid_of(3 * wr.count(), 128), 128, 0, st>>>(P, wr, nullptr, 0, wv);
+  k_weno_ao<<<grid_of(ao.count(), 128), 128, 0, st>>>(P, ao, q->moments);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    cs::set_error("cs_limit_slopes_p12_zones: %s", hipGetErrorString(e));
+    return CS_E_HIP;
+  }
+  return CS_OK;
 }

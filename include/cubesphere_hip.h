@@ -427,6 +427,41 @@ int cs_limit_all_slopes_p1_timed(const cs_p1_patch* p, void* stream, float* kern
  * overlap on one GPU race. */
 int cs_limit_all_slopes_p1_noalloc(const cs_p1_patch* p, void* stream);
 
+/* ---- third order: the zone-centred reconstruction of LIMIT_ALL_SLOPES_P12 --
+ * The geometry of NUMORDER 3 (the driver builds it once, as at second
+ * order): COORDS_TO_WT_QUADPTS_VOLUME_ISO2 (240coords_to_wt_quadpts_volume_iso2.f:7)
+ * with the 3x3x3 Gauss rule (:134-147) and INIT_BASIS_FUNCTION_COEFS
+ * (250init_basis_function_coefs.f:5) with the ten modes of
+ * BUILD_MODE_INDICES at order 3 (204build_mode_indices.f:55-95: 1, x, y, z,
+ * x^2, y^2, z^2, xy, yz, xz), for the six sectors on the GPU, copied back to
+ * HOST arrays in the layouts of cs_geometry: v_pts (zone, 3, 27), v_wts
+ * (zone, 27), v_vol, v_cen (zone, 3), moments (zone, 10).  Any null output
+ * is skipped. */
+typedef struct cs_volume_p12 {
+  double *v_pts, *v_wts, *v_vol, *v_cen, *moments;
+} cs_volume_p12;
+int cs_build_volume_p12(const cs_config* cfg, const cs_volume_p12* out);
+/* Steps 2 to 7.b of LIMIT_ALL_SLOPES_P12 (400riemann_integrate/
+ * 421limit_all_slopes_p12.f:9, called from the driver's stage loop at
+ * NUMORDER 3, DRV:2395-2398) on one patch: the steps of LIMIT_ALL_SLOPES_P1
+ * up to its WENO (the P1 gradients of the eight primitives left unflattened,
+ * 421…:3435-3448), then step 7.b (:3811-4557): on the active zones of
+ * -1 .. N+2 (k -1 .. NR+2) the nine modes of the least-squares quadratic
+ * fit to the 18-zone central stencil (RECONSTRUCTION_COEFS,
+ * 251reconstruction_coefs.f:3), non-linearly hybridised with the P1
+ * gradients (WENO-AO), then times (1 - flattener).  The patch is a
+ * cs_p1_patch whose U, V (zone, 8, 10) hold ten modes and whose v_centroid
+ * comes from the 27-point rule, plus the ten basis moments (DEVICE, zone x
+ * 10).  Outputs: V (all ten modes), the entropy's gradients, the flattener
+ * and the other work arrays of steps 2-6.  The third-order positivity limiter
+ * (7.c), the facial B (8) and the zone E (9) are not part of this call.
+ * No work memory; stream as cs_limit_all_slopes_p1. */
+typedef struct cs_p12_patch {
+  cs_p1_patch p1;
+  const double* moments; /* Coef_Basis_Moments (zone, 10) */
+} cs_p12_patch;
+int cs_limit_slopes_p12_zones(const cs_p12_patch* p, void* stream);
+
 /* ---- P1 facial fluxes of one patch -----------------------------------------
  * MAKE_FACIAL_FLUXES (400riemann_integrate/450make_facial_fluxes.f:5-60) at
  * second order: per face, the two zones' P1 fields (V, entropygr from
