@@ -728,15 +728,18 @@ class P12Patch(ctypes.Structure):
 
 
 def limit_slopes_p12_zones(N, NR, g, scalars, arrays, moments, itri=(1, 1, 1, 1), stream=None):
-    """Steps 2 to 7.b of LIMIT_ALL_SLOPES_P12 on one patch (the P1 steps up to
-    the WENO, then the WENO-AO hybridisation with the quadratic central
-    stencil), in place on torch CUDA tensors in the reference's layouts: as
-    limit_all_slopes_p1, with U and V of ten modes (zone, 8, 10), v_centroid
-    from the 27-point rule, and moments (zone, 10) a float64 CUDA tensor."""
+    """Steps 2 to 7.c of LIMIT_ALL_SLOPES_P12 on one patch (the P1 steps up to
+    the WENO, the WENO-AO hybridisation with the quadratic central stencil,
+    the positivity limiter with the quadratic modes), in place on torch CUDA
+    tensors in the reference's layouts: as limit_all_slopes_p1, with U and V
+    of ten modes (zone, 8, 10), B1_zc, B2_zc, B3_zc of ten, v_centroid from
+    the 27-point rule, and moments (zone, 10) a float64 CUDA tensor."""
     import torch
     want = p1_counts(N, NR, g)
     nzone = want["mask"]
     want["U"] = want["V"] = 80 * nzone
+    for k in ("B1_zc", "B2_zc", "B3_zc"):
+        want[k] = 10 * nzone
     s = P12Patch()
     q = s.p1
     q.ntp, q.nr, q.nghost = N, NR, g

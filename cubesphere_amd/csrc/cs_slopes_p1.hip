@@ -476,8 +476,12 @@ __device__ __forceinline__ void columns9(const P1& P, int i, int j, int64_t* col
   }
 }
 
+// NM = 4: LIMIT_ALL_SLOPES_P1; NM = 10: LIMIT_ALL_SLOPES_P12 (step 7.c), the
+// node values carrying the quadratic modes about the zone's own moments mom
+// (440...f:594-600) and every mode above the mean reduced
+template <int NM>
 __global__ void __launch_bounds__(256) k_pos(const P1 P, const Box b, const int64_t* list, int64_t nlist,
-                                             const Flags F) {
+                                             const Flags F, const double* __restrict__ mom) {
   const int64_t t = tid();
   int i, j, k;
   int64_t z;
@@ -521,13 +525,16 @@ __global__ void __launch_bounds__(256) k_pos(const P1 P, const Box b, const int6
   emax = emax * (1.0 + expandfac * omff);
   // the modes checked at the 27 nodes: rho, p and the entropy
   const double r0 = P.V(0, 0, z), p0 = P.V(4, 0, z), e0 = a.ent[z];
-  double rm[3], pm[3], em[3];
+  double rm[NM - 1], pm[NM - 1], em[3], q0[6];
 #pragma unroll
-  for (int m = 0; m < 3; ++m) {
+  for (int m = 0; m < NM - 1; ++m) {
     rm[m] = P.V(0, m + 1, z);
     pm[m] = P.V(4, m + 1, z);
-    em[m] = a.ent[z + P.NZ * (m + 1)];
   }
+#pragma unroll
+  for (int m = 0; m < 3; ++m) em[m] = a.ent[z + P.NZ * (m + 1)];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) q0[m] = NM == 10 ? mom[z + P.NZ * (4 + m)] : 0.0;
   const double obl = P.obl(z);
   double c0[3];
 #pragma unroll
@@ -554,28 +561,45 @@ __global__ void __launch_bounds__(256) k_pos(const P1 P, const Box b, const int6
   // does every node pass with the slopes reduced n times?  The reduced slopes
   // are formed by n multiplications, as the reference's loop forms them
   auto passes = [&](int n) -> bool {
-    double r[3], p[3], e[3];
+    double r[NM - 1], p[NM - 1], e[3];
 #pragma unroll
-    for (int m = 0; m < 3; ++m) {
+    for (int m = 0; m < NM - 1; ++m) {
       r[m] = rm[m];
       p[m] = pm[m];
-      e[m] = em[m];
     }
-    for (int q = 0; q < n; ++q)
 #pragma unroll
-      for (int m = 0; m < 3; ++m) {
+    for (int m = 0; m < 3; ++m) e[m] = em[m];
+    for (int q = 0; q < n; ++q) {
+#pragma unroll
+      for (int m = 0; m < NM - 1; ++m) {
         r[m] = ratio * r[m];
         p[m] = ratio * p[m];
-        e[m] = ratio * e[m];
       }
+#pragma unroll
+      for (int m = 0; m < 3; ++m) e[m] = ratio * e[m];
+    }
     bool pass = true;
+    // the scale opaque to the optimiser at third order: the nodes' offsets and
+    // quadratic basis values are formed in each check instead of being
+    // hoisted out of the bisection and held for all 27 nodes at once
+    double ob = obl;
+    if constexpr (NM == 10) asm volatile("" : "+v"(ob));
     // one node: scaled offset from the centroid, then the three checks
     auto node = [&](const double* x0) {
       double x[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) x[c] = (x0[c] - c0[c]) * obl;
-      const double rn = r0 + r[0] * x[0] + r[1] * x[1] + r[2] * x[2];
-      const double pn = p0 + p[0] * x[0] + p[1] * x[1] + p[2] * x[2];
+      for (int c = 0; c < 3; ++c) x[c] = (x0[c] - c0[c]) * ob;
+      double rn = r0 + r[0] * x[0] + r[1] * x[1] + r[2] * x[2];
+      double pn = p0 + p[0] * x[0] + p[1] * x[1] + p[2] * x[2];
+      if constexpr (NM == 10) {
+        const double b[6] = {x[0] * x[0] - q0[0], x[1] * x[1] - q0[1], x[2] * x[2] - q0[2],
+                             x[0] * x[1] - q0[3], x[1] * x[2] - q0[4], x[0] * x[2] - q0[5]};
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+          rn = rn + r[3 + m] * b[m];
+          pn = pn + p[3 + m] * b[m];
+        }
+      }
       const double en = e0 + e[0] * x[0] + e[1] * x[1] + e[2] * x[2];
       if (rn < rmin || rn > rmax || pn < pmin || pn > pmax || en < emin || en > emax) pass = false;
     };
@@ -622,7 +646,7 @@ __global__ void __launch_bounds__(256) k_pos(const P1 P, const Box b, const int6
   if (nred > 0 || removed) {
     for (int c = 0; c < 8; ++c)
 #pragma unroll
-      for (int m = 0; m < 3; ++m) {
+      for (int m = 0; m < NM - 1; ++m) {
         double v = P.V(c, m + 1, z);
         if (removed) {
           v = 0.0;
@@ -635,7 +659,7 @@ __global__ void __launch_bounds__(256) k_pos(const P1 P, const Box b, const int6
     for (int m = 0; m < 3; ++m) a.ent[z + P.NZ * (m + 1)] = removed ? 0.0 : em[m];
   }
 #pragma unroll
-  for (int m = 0; m < 4; ++m) {
+  for (int m = 0; m < NM; ++m) {
     a.B1_zc[z + P.NZ * m] = P.V(5, m, z);
     a.B2_zc[z + P.NZ * m] = P.V(6, m, z);
     a.B3_zc[z + P.NZ * m] = P.V(7, m, z);
@@ -1071,8 +1095,18 @@ static inline unsigned grid_of(int64_t n, int threads) { return (unsigned)((n + 
 // kernel_ms (optional): per-stage times from HIP events on the stream
 // ext (optional): caller-provided device work memory of ext_bytes, used
 // instead of the library's per-device allocation (no allocation then)
+namespace csp1 {
+__global__ void __launch_bounds__(256) k_zero_hi(const P1 P, const Box b);
+__global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom,
+                                                 const int64_t* list, int64_t nlist);
+}  // namespace csp1
+
+// mom (optional, DEVICE zone x 10): the third-order form (LIMIT_ALL_SLOPES_P12
+// steps 2-7.c): U and V of ten modes, the WENO gradients left unflattened
+// and followed by the WENO-AO hybridisation of step 7.b, the positivity
+// limiter with the quadratic modes; the call ends after step 7
 static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t* ext = nullptr,
-                  size_t ext_bytes = 0) {
+                  size_t ext_bytes = 0, const double* mom = nullptr) {
   int rc = p1_check(p);
   if (rc) return rc;
   hipStream_t st = (hipStream_t)stream_;
@@ -1151,6 +1185,8 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
     return 0;
   };
   if ((rc = mark())) return rc;
+  const Box ao = make_box(-1, N + 2, -1, N + 2, -1, NR + 2);
+  if (mom) k_zero_hi<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
   k_prim<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
   if ((rc = mark())) return rc;
   k_face_zero<<<dim3(grid_of(P.NFC, 256), 3), 256, 0, st>>>(P);
@@ -1165,7 +1201,8 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
   P1_HIP(hipMemsetAsync(F.counter, 0, 2 * sizeof(unsigned int), st));
   WenoFields wv, we;
   wv.n = 9;
-  wv.noflat = we.noflat = 0;
+  wv.noflat = mom ? 0xffu : 0u;  // third order: V flattened after the hybridisation
+  we.noflat = 0;
   for (int c = 0; c < 8; ++c) {
     wv.src[c] = p->V + P.NZ * c;
     wv.dst[c] = p->V + P.NZ * (c + 8);
@@ -1182,7 +1219,12 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
     we.stride[c] = P.NZ;
   }
   k_weno9<<<grid_of(3 * wr.count(), 128), 128, 0, st>>>(P, wr, nullptr, 0, wv);
-  k_pos<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, nullptr, 0, F);
+  if (mom) {
+    k_weno_ao<<<grid_of(ao.count(), 128), 128, 0, st>>>(P, ao, mom, nullptr, 0);
+    k_pos<10><<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, nullptr, 0, F, mom);
+  } else {
+    k_pos<4><<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, nullptr, 0, F, nullptr);
+  }
   P1_HIP(hipGetLastError());
   unsigned int nfail = 0;
   P1_HIP(hipMemcpyAsync(&nfail, F.counter, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
@@ -1208,7 +1250,12 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
         break;
       }
       k_weno9<<<grid_of(3 * (int64_t)ncand, 128), 128, 0, st>>>(P, wr, list, ncand, wv);
-      k_pos<<<grid_of(ncand, 256), 256, 0, st>>>(P, wr, list, ncand, F);
+      if (mom) {
+        k_weno_ao<<<grid_of(ncand, 128), 128, 0, st>>>(P, ao, mom, list, ncand);
+        k_pos<10><<<grid_of(ncand, 256), 256, 0, st>>>(P, wr, list, ncand, F, mom);
+      } else {
+        k_pos<4><<<grid_of(ncand, 256), 256, 0, st>>>(P, wr, list, ncand, F, nullptr);
+      }
       k_commit<<<grid_of(ncand, 256), 256, 0, st>>>(list, ncand, F.next, F.prev, ever);
       P1_HIP(hipGetLastError());
       unsigned int nchg = 0;
@@ -1221,6 +1268,10 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
       return CS_E_INVALID;
     }
     k_clear_ent<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, F.prev);
+  }
+  if (mom) {  // the third-order steps 8-9 are not part of this call
+    P1_HIP(hipGetLastError());
+    return CS_OK;
   }
   if ((rc = mark())) return rc;
   k_face_pts<<<dim3(grid_of(std::max(fx.count(), std::max(fy.count(), fz.count())), 256), 3), 256, 0, st>>>(
@@ -2706,9 +2757,17 @@ __device__ __forceinline__ constexpr int lt(int r, int c) { return r * (r + 1) /
 // (1 + tau / IS)), and every mode above the mean is multiplied by
 // (1 - flattener).  The reference inverts A^T A by LU (MATINV); the two
 // agree to rounding.
-__global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom) {
+__global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom,
+                                                 const int64_t* list, int64_t nlist) {
   int i, j, k;
-  if (!b.decode(tid(), i, j, k)) return;
+  if (list) {  // the positivity replay's zones: those inside the range
+    const int64_t t = tid();
+    if (t >= nlist) return;
+    zone_of(P, list[t], i, j, k);
+    if (i < b.i0 || i >= b.i0 + b.ni || j < b.j0 || j >= b.j0 + b.nj || k < b.k0 || k >= b.k0 + b.nk) return;
+  } else if (!b.decode(tid(), i, j, k)) {
+    return;
+  }
   if (!P.active(i, j, 1)) return;
   const cs_p1_patch& a = P.a;
   const int64_t z = P.zi(i, j, k), NZ = P.NZ;
@@ -2855,40 +2914,5 @@ extern "C" int cs_limit_slopes_p12_zones(const cs_p12_patch* q, void* stream_) {
     cs::set_error("cs_limit_slopes_p12_zones: null patch or moments");
     return CS_E_INVALID;
   }
-  const cs_p1_patch* p = &q->p1;
-  int rc = p1_check(p);
-  if (rc) return rc;
-  P1 P;
-  if ((rc = p1_dims_check(p, "cs_limit_slopes_p12_zones", P))) return rc;
-  hipStream_t st = (hipStream_t)stream_;
-  const int N = P.N, NR = P.NR, g = P.g;
-  const Box all = make_box(1 - g, N + g, 1 - g, N + g, 1 - g, NR + g);
-  const Box wr = make_box(-2, N + 3, -2, N + 3, -2, NR + 3);
-  const Box ao = make_box(-1, N + 2, -1, N + 2, -1, NR + 2);
-  k_zero_hi<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
-  k_prim<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
-  k_face_zero<<<dim3(grid_of(P.NFC, 256), 3), 256, 0, st>>>(P);
-  k_tricorner<<<grid_of(4 * (int64_t)g * P.nzr, 256), 256, 0, st>>>(P);
-  k_flat_a<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
-  k_flat_b<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
-  k_flat_c<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
-  WenoFields wv;
-  wv.n = 9;
-  wv.noflat = 0xffu;  // the eight primitives: flattened after 7.b
-  for (int c = 0; c < 8; ++c) {
-    wv.src[c] = p->V + P.NZ * c;
-    wv.dst[c] = p->V + P.NZ * (c + 8);
-    wv.stride[c] = 8 * P.NZ;
-  }
-  wv.src[8] = p->ent;
-  wv.dst[8] = p->ent + P.NZ;
-  wv.stride[8] = P.NZ;
-  k_weno9<<<grid_of(3 * wr.count(), 128), 128, 0, st>>>(P, wr, nullptr, 0, wv);
-  k_weno_ao<<<grid_of(ao.count(), 128), 128, 0, st>>>(P, ao, q->moments);
-  const hipError_t e = hipGetLastError();
-  if (e != hipSuccess) {
-    cs::set_error("cs_limit_slopes_p12_zones: %s", hipGetErrorString(e));
-    return CS_E_HIP;
-  }
-  return CS_OK;
+  return run_p1(&q->p1, stream_, nullptr, nullptr, 0, q->moments);
 }

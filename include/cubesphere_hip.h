@@ -441,21 +441,25 @@ typedef struct cs_volume_p12 {
   double *v_pts, *v_wts, *v_vol, *v_cen, *moments;
 } cs_volume_p12;
 int cs_build_volume_p12(const cs_config* cfg, const cs_volume_p12* out);
-/* Steps 2 to 7.b of LIMIT_ALL_SLOPES_P12 (400riemann_integrate/
+/* Steps 2 to 7.c of LIMIT_ALL_SLOPES_P12 (400riemann_integrate/
  * 421limit_all_slopes_p12.f:9, called from the driver's stage loop at
  * NUMORDER 3, DRV:2395-2398) on one patch: the steps of LIMIT_ALL_SLOPES_P1
  * up to its WENO (the P1 gradients of the eight primitives left unflattened,
- * 421…:3435-3448), then step 7.b (:3811-4557): on the active zones of
- * -1 .. N+2 (k -1 .. NR+2) the nine modes of the least-squares quadratic
- * fit to the 18-zone central stencil (RECONSTRUCTION_COEFS,
- * 251reconstruction_coefs.f:3), non-linearly hybridised with the P1
- * gradients (WENO-AO), then times (1 - flattener).  The patch is a
- * cs_p1_patch whose U, V (zone, 8, 10) hold ten modes and whose v_centroid
- * comes from the 27-point rule, plus the ten basis moments (DEVICE, zone x
- * 10).  Outputs: V (all ten modes), the entropy's gradients, the flattener
- * and the other work arrays of steps 2-6.  The third-order positivity limiter
- * (7.c), the facial B (8) and the zone E (9) are not part of this call.
- * No work memory; stream as cs_limit_all_slopes_p1. */
+ * 421…:3435-3448); step 7.b (:3811-4557): on the active zones of -1 .. N+2
+ * (k -1 .. NR+2) the nine modes of the least-squares quadratic fit to the
+ * 18-zone central stencil (RECONSTRUCTION_COEFS, 251reconstruction_coefs.f:3),
+ * non-linearly hybridised with the P1 gradients (WENO-AO), then times
+ * (1 - flattener); step 7.c (:4558-4568): ENFORCE_PRESSURE_POSITIVITY_PRIM
+ * (440enforce_pressure_positivity_prim.f:6) with the quadratic modes in the
+ * node values (:594-600), reducing all ten modes, with the loop-order replay
+ * of cs_limit_all_slopes_p1.  The patch is a cs_p1_patch whose U, V (zone,
+ * 8, 10) and B1_zc, B2_zc, B3_zc (zone, 10) hold ten modes and whose
+ * v_centroid comes from the 27-point rule, plus the ten basis moments
+ * (DEVICE, zone x 10).  Outputs: V, the entropy, B1..3_zc, the flattener and
+ * the work arrays of steps 2-6.  The facial B (step 8, with its 30-unknown
+ * constrained least squares) and the zone E (9) are not part of this call.
+ * Work memory and stream as cs_limit_all_slopes_p1 (the call waits on the
+ * stream once mid-way). */
 typedef struct cs_p12_patch {
   cs_p1_patch p1;
   const double* moments; /* Coef_Basis_Moments (zone, 10) */

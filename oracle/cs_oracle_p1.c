@@ -470,17 +470,14 @@ static void p1_front(const or_p1_patch* p, const p1dims* d, int nm, int flat_v) 
 
 }
 
-int or_limit_all_slopes_p1(const or_p1_patch* p) {
-  if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1) return 1;
-  const p1dims D = p1_dims(p), *d = &D;
-  const int N = d->N, NR = d->NR, g = d->g;
-  g_p1_stats[0] = g_p1_stats[1] = 0;
-  const double pi = 4.0 * atan(1.0), ob8pi = 1.0 / (8.0 * pi), ob4pi = 1.0 / (4.0 * pi), gam = p->gamma;
+/* STEP 7.b of LIMIT_ALL_SLOPES_P1, 7.c of LIMIT_ALL_SLOPES_P12:
+ * ENFORCE_PRESSURE_POSITIVITY_PRIM (440...f:6-830), zone by zone in the
+ * loop order; nm modes (4, or 10 with the zone's moments mom: the node
+ * values then carry the quadratic modes about the zone's own moments,
+ * 440...f:594-600, and every mode above the mean is reduced) */
+static void p1_positivity(const or_p1_patch* p, const p1dims* d, int nm, const double* mom) {
+  const int N = d->N, NR = d->NR;
   int i, j, k, c, m, q;
-  p1_front(p, d, 4, 1);
-
-  /* STEP 7.b: ENFORCE_PRESSURE_POSITIVITY_PRIM (440...f:6-830) */
-  {
     const int n_iter = 15;
     const double ratio = pow(0.10, 1.0 / (double)n_iter), expandfac = 0.2;
     for (k = -2; k <= NR + 3; ++k)
@@ -537,16 +534,18 @@ int or_limit_all_slopes_p1(const or_p1_patch* p) {
           for (c = 0; c < 3; ++c) node[26][c] = cen[c];
           for (int n = 0; n < 27; ++n)
             for (c = 0; c < 3; ++c) node[n][c] = (node[n][c] - cen[c]) * obl;
-          double Vm[8][4], em[4];
+          double Vm[8][10], em[4], q0[10] = {0};
           for (c = 0; c < 8; ++c)
-            for (m = 0; m < 4; ++m) Vm[c][m] = VV(p, d, c, m, z);
+            for (m = 0; m < nm; ++m) Vm[c][m] = VV(p, d, c, m, z);
+          if (mom)
+            for (m = 4; m < 10; ++m) q0[m] = mom[z + d->NZ * m];
           for (m = 0; m < 4; ++m) em[m] = p->ent[z + d->NZ * m];
           for (int it = 1; it <= n_iter + 1; ++it) {
             if (it == 2) ++g_p1_stats[0];
             if (it == n_iter + 1) {
               ++g_p1_stats[1];
               for (c = 0; c < 8; ++c)
-                for (m = 1; m < 4; ++m) Vm[c][m] = 0.0;
+                for (m = 1; m < nm; ++m) Vm[c][m] = 0.0;
               for (m = 0; m < 4; ++m) em[m] = 0.0;
               break;
             }
@@ -555,6 +554,12 @@ int or_limit_all_slopes_p1(const or_p1_patch* p) {
               double x = node[n][0], y = node[n][1], zz = node[n][2];
               double rn = Vm[0][0] + Vm[0][1] * x + Vm[0][2] * y + Vm[0][3] * zz;
               double pn = Vm[4][0] + Vm[4][1] * x + Vm[4][2] * y + Vm[4][3] * zz;
+              if (nm == 10) { /* 440...f:594-600: the quadratic modes about the zone's moments */
+                const double b5 = x * x - q0[4], b6 = y * y - q0[5], b7 = zz * zz - q0[6];
+                const double b8 = x * y - q0[7], b9 = y * zz - q0[8], b10 = x * zz - q0[9];
+                rn = rn + Vm[0][4] * b5 + Vm[0][5] * b6 + Vm[0][6] * b7 + Vm[0][7] * b8 + Vm[0][8] * b9 + Vm[0][9] * b10;
+                pn = pn + Vm[4][4] * b5 + Vm[4][5] * b6 + Vm[4][6] * b7 + Vm[4][7] * b8 + Vm[4][8] * b9 + Vm[4][9] * b10;
+              }
               double en = em[0] + em[1] * x + em[2] * y + em[3] * zz;
               if (rn < rmin || rn > rmax) pass = 0;
               if (pn < pmin || pn > pmax) pass = 0;
@@ -562,19 +567,30 @@ int or_limit_all_slopes_p1(const or_p1_patch* p) {
             }
             if (pass) break;
             for (c = 0; c < 8; ++c)
-              for (m = 1; m < 4; ++m) Vm[c][m] = ratio * Vm[c][m];
+              for (m = 1; m < nm; ++m) Vm[c][m] = ratio * Vm[c][m];
             for (m = 1; m < 4; ++m) em[m] = ratio * em[m];
           }
           for (c = 0; c < 8; ++c)
-            for (m = 0; m < 4; ++m) VV(p, d, c, m, z) = Vm[c][m];
-          for (m = 0; m < 4; ++m) {
-            p->ent[z + d->NZ * m] = em[m];
+            for (m = 0; m < nm; ++m) VV(p, d, c, m, z) = Vm[c][m];
+          for (m = 0; m < 4; ++m) p->ent[z + d->NZ * m] = em[m];
+          for (m = 0; m < nm; ++m) {
             p->B1_zc[z + d->NZ * m] = Vm[5][m];
             p->B2_zc[z + d->NZ * m] = Vm[6][m];
             p->B3_zc[z + d->NZ * m] = Vm[7][m];
           }
         }
-  }
+}
+
+int or_limit_all_slopes_p1(const or_p1_patch* p) {
+  if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1) return 1;
+  const p1dims D = p1_dims(p), *d = &D;
+  const int N = d->N, NR = d->NR, g = d->g;
+  g_p1_stats[0] = g_p1_stats[1] = 0;
+  const double pi = 4.0 * atan(1.0), ob8pi = 1.0 / (8.0 * pi), ob4pi = 1.0 / (4.0 * pi), gam = p->gamma;
+  int i, j, k, c, m, q;
+  p1_front(p, d, 4, 1);
+
+  p1_positivity(p, d, 4, NULL);
 
   /* STEP 8.a-c (:3874-4640): normal B at the five B-field points of each face
      from the two zones' linear fields, averaged, then shifted so the weighted
@@ -2225,10 +2241,14 @@ int or_ao_fit_p12(const or_p1_patch* p, const double* mom, int i, int j, int k, 
 }
 
 /* the zone-centred part of LIMIT_ALL_SLOPES_P12: steps 2 to 7.a with ten
- * modes and unflattened WENO gradients, then 7.b */
+ * modes and unflattened WENO gradients, 7.b, then the positivity limiter of
+ * 7.c with the quadratic modes */
 int or_limit_slopes_p12_zones(const or_p1_patch* p, const double* mom) {
   if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1 || !mom) return 1;
   const p1dims D = p1_dims(p);
   p1_front(p, &D, 10, 0);
-  return or_weno_ao_p12(p, mom);
+  if (or_weno_ao_p12(p, mom)) return 1;
+  g_p1_stats[0] = g_p1_stats[1] = 0;
+  p1_positivity(p, &D, 10, mom);
+  return 0;
 }

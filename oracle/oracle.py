@@ -1254,13 +1254,14 @@ class P12Extra(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in P12_EXTRA]
 
 
-def p12_problem(N, NR, g=4, sector=0, seed=0, shock=False, rmin=1.0, rmax=1.6, gamma=5.0 / 3.0):
+def p12_problem(N, NR, g=4, sector=0, seed=0, shock=False, rmin=1.0, rmax=1.6, gamma=5.0 / 3.0, noise=0.0):
     """One sector as a patch for the third-order routine: the arrays of
     p1_problem with ten-mode U, V, B*_zc, E*_zc and Wzc (seeded junk above
     the means), the 27-point centroids and volumes, the ten basis moments,
     and the face quadrature arrays the routine declares."""
     import cubesphere_amd as cs  # Config layout only
-    prob, A = p1_problem(N, NR, g, sector=sector, seed=seed, shock=shock, rmin=rmin, rmax=rmax, gamma=gamma)
+    prob, A = p1_problem(N, NR, g, sector=sector, seed=seed, shock=shock, rmin=rmin, rmax=rmax, gamma=gamma,
+                         noise=noise)
     cfg = cs.make_config(N, NR, g, cs.RECON_WENO5, cs.RS_HLL, cs.EMF_LLF, cs.FP64, 2, rmin=rmin, rmax=rmax,
                          gamma=gamma)
     geo = quad_geometry(cfg)
@@ -1285,7 +1286,7 @@ def p12_problem(N, NR, g=4, sector=0, seed=0, shock=False, rmin=1.0, rmax=1.6, g
 
 
 def p12_oracle(prob, A):
-    """or_limit_slopes_p12_zones in place on A (steps 2 to 7.b)."""
+    """or_limit_slopes_p12_zones in place on A (steps 2 to 7.c)."""
     L = olib()
     L.or_limit_slopes_p12_zones.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     s = p1_struct(prob, A)

@@ -2,13 +2,13 @@
 on the CPU: the oracle's restatement against the reference's own routines
 built at NUMORDER 3 (oracle/_ref/libcsref3.so), and against closed forms.
 
-The oracle restates steps 2 to 7.b: the P1 WENO gradients (7.a) left
-unflattened, then the WENO-AO hybridisation of 7.b with the 18-zone
-quadratic least-squares stencil (RECONSTRUCTION_COEFS, 251...f:3; MATINV,
-121matinv.f:8), then the flattener.  The reference runs its whole routine,
-so only what 7.c-9 leave alone is compared: the five hydrodynamic
-primitives (step 8.d rewrites the magnetic ones) and the entropy, on zones
-the third-order positivity limiter (7.c) did not reduce."""
+The oracle restates steps 2 to 7.c: the P1 WENO gradients (7.a) left
+unflattened, the WENO-AO hybridisation of 7.b with the 18-zone quadratic
+least-squares stencil (RECONSTRUCTION_COEFS, 251...f:3; MATINV,
+121matinv.f:8) and the flattener, then the positivity limiter with the
+quadratic modes (7.c, 440...f:594-600).  The reference runs its whole
+routine, so what steps 8-9 leave alone is compared: the five hydrodynamic
+primitives (step 8.d rewrites the magnetic ones) and the entropy."""
 import ctypes
 
 import numpy as np
@@ -18,7 +18,6 @@ import cubesphere_amd as cs
 from oracle import oracle as orc
 
 GEO_KEYS = ("v_pts", "v_wts", "v_tan", "v_vol", "v_cen", "moments")
-RATIO = 0.1 ** (1.0 / 15.0)  # the positivity limiter's reduction per step (440...f)
 
 
 def _ref3_or_skip():
@@ -122,33 +121,29 @@ def test_p12_zones_bit_exact_with_reference_smooth(case):
     assert np.abs(_hydro(A, nz)[4:]).max() > 1e-4
 
 
-@pytest.mark.parametrize("sector", [0, 3])
-def test_p12_zones_with_flattener_match_reference(sector):
-    """A compressive front: the flattener acts (the linear weights move to
-    0.6 / 0.4 and the modes are scaled), and the reference's third-order
-    positivity limiter reduces a few zones by RATIO^n.  Every other zone is
-    bit-exact; the reduced ones are the oracle's modes times RATIO^n."""
+@pytest.mark.parametrize("case", [(0, 3, 0.0), (3, 3, 0.0), (2, 5, 0.6), (5, 7, 1.5)],
+                         ids=lambda c: "s%d_seed%d_noise%g" % c)
+def test_p12_zones_with_flattener_and_positivity_bit_exact(case):
+    """A compressive front (the flattener acts: the linear weights move to
+    0.6 / 0.4 and every mode is scaled) and log-normal noise (the positivity
+    limiter reduces slopes by 0.1^(n/15) or removes them, and its loop order
+    matters): bit-exact with the reference on the hydrodynamic primitives
+    and the entropy, every zone."""
     _ref3_or_skip()
+    sector, seed, noise = case
     N, NR = 8, 3
-    prob, A0 = orc.p12_problem(N, NR, 4, sector=sector, seed=3, shock=True)
+    prob, A0 = orc.p12_problem(N, NR, 4, sector=sector, seed=seed, shock=True, noise=noise)
     nz = orc.p1_dims(N, NR, 4)["zone"]
     A = orc.p12_oracle(prob, orc.p1_copy(A0))
+    st = np.zeros(2, np.int64)
+    orc.olib().or_p1_positivity_stats(orc.P(st))
     B = orc.p12_reference(prob, orc.p1_copy(A0))
-    VA, VB = _hydro(A, nz), _hydro(B, nz)
-    assert np.array_equal(VA[0], VB[0])
-    diff = (VA != VB).any(axis=(0, 1))
-    flat = A["flatten"] > 0
-    assert (flat & ~diff).sum() > 100, "the flattened zones must be compared"
-    assert diff.sum() < 0.05 * nz
-    for z in np.where(diff)[0]:
-        a, b = VA[1:, :, z], VB[1:, :, z]
-        if not b.any():
-            continue  # slopes removed
-        nzm = a != 0
-        r = b[nzm] / a[nzm]
-        n = np.log(r.mean()) / np.log(RATIO)
-        assert abs(n - round(n)) < 1e-9 and 1 <= round(n) <= 15, (z, n)
-        assert np.allclose(r, RATIO ** round(n), rtol=1e-14, atol=0)
+    assert np.array_equal(_hydro(A, nz), _hydro(B, nz))
+    assert np.array_equal(A["ent"], B["ent"])
+    assert (A["flatten"] > 0).sum() > 100
+    assert st[0] > 5, "the positivity limiter must act"
+    if noise > 1.0:
+        assert st[1] > 5, "and remove slopes"
 
 
 def _quad_problem(N=8, NR=4, sector=1):
