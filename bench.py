@@ -545,6 +545,91 @@ def bench_p1(cs, args):
     print(json.dumps(line))
 
 
+def bench_p12(cs, args):
+    """Steps 2 to 7 of the third-order LIMIT_ALL_SLOPES_P12 (P1 WENO,
+    WENO-AO hybridisation with the 18-zone quadratic stencil, positivity with
+    the quadratic modes) on one sector of the 6 x 192^2 x 64 shell as a patch
+    (NGhost 4): owned-zone reconstructions per second; the oracle and the
+    reference's own routine (built at NUMORDER 3) timed on a small patch."""
+    import torch
+    N, NR, g = 192, 64, 4
+    cfg, scal, T = p1_patch_on_gpu(cs, N, NR, g)
+    dev = T["U"].device
+    cnt = cs.p1_counts(N, NR, g)
+    nz = cnt["mask"]
+    vol = cs.build_volume_p12(cfg, skip=("v_pts", "v_wts"))
+    sec = lambda a, m: torch.from_numpy(a[:nz * m].copy()).to(dev)  # sector 0
+    T["v_centroid"] = sec(vol["v_cen"], 3)
+    mom = sec(vol["moments"], 10)
+    del vol
+    U4 = T["U"].view(4, 8 * nz)
+    U = torch.zeros(10, 8 * nz, dtype=torch.float64, device=dev)
+    U[0] = U4[0]
+    T["U"] = U.view(-1)
+    T["V"] = torch.zeros(80 * nz, dtype=torch.float64, device=dev)
+    for k in ("B1_zc", "B2_zc", "B3_zc"):
+        T[k] = torch.zeros(10 * nz, dtype=torch.float64, device=dev)
+    U0, e0, sc0 = T["U"].clone(), T["ent"].clone(), T["scale"].clone()
+
+    def reset():
+        T["U"].copy_(U0)
+        T["ent"].copy_(e0)
+        T["scale"].copy_(sc0)
+
+    for _ in range(max(1, args.warmup)):
+        reset()
+        cs.limit_slopes_p12_zones(N, NR, g, scal, T, mom)
+    torch.cuda.synchronize()
+    steps = max(1, min(args.steps, 50))
+    tot = 0.0
+    for _ in range(steps):
+        reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cs.limit_slopes_p12_zones(N, NR, g, scal, T, mom)
+        torch.cuda.synchronize()
+        tot += time.perf_counter() - t0
+    ms_call = tot / steps * 1e3
+    own = N * N * NR
+    log(f"P12 steps 2-7 on a {N}^2 x {NR} patch: {ms_call:.3f} ms per call, {own / (ms_call * 1e-3):.3e} zones/s")
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_p12(args.cpu_seconds)
+    line = {
+        "metric": "P12 zone reconstructions/s", "value": own / (ms_call * 1e-3), "unit": "zones/s", "n_gpus": 1,
+        "steps": steps, "warmup": args.warmup, "ms_per_step": ms_call, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"P12: LIMIT_ALL_SLOPES_P12 steps 2-7 (third-order WENO-AO with positivity) on "
+                               f"one sector of the 6x{N}^2x{NR} shell as a patch, NGhost {g} ({own} owned zones, "
+                               f"{nz} with ghosts); front + log-normal noise 0.3", "zones": own},
+        "roofline": None, "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+def cpu_baseline_p12(max_seconds):
+    """The oracle's steps 2-7 of LIMIT_ALL_SLOPES_P12 (one host core) and the
+    reference's whole routine built at NUMORDER 3 (one host core, steps 8-9
+    included) on a 24^2 x 8 patch."""
+    from oracle import oracle as orc
+    N, NR = 24, 8
+    prob, A0 = orc.p12_problem(N, NR, 4, sector=1, seed=3, shock=True, noise=0.3)
+    own = N * N * NR
+    out = {"unit": "zones/s", "cores": 1, "kind": "port", "sample": f"{N}^2 x {NR} patch, NGhost 4"}
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < max_seconds / 2 or n == 0:
+        orc.p12_oracle(prob, orc.p1_copy(A0))
+        n += 1
+    out["value"] = own * n / (time.perf_counter() - t0)
+    if orc.rlib3() is not None:
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < max_seconds / 2 or n == 0:
+            orc.p12_reference(prob, orc.p1_copy(A0))
+            n += 1
+        out["reference_whole_routine"] = own * n / (time.perf_counter() - t0)
+    return out
+
+
 def bench_p1_shell(cs, args):
     """The reference's own time step on the 6 x 192^2 x 64 shell (the C3
     size): six P1 patches, the driver's loop (cubesphere_amd.P1Shell) of
@@ -957,7 +1042,7 @@ def main():
             cpu = cpu_baseline_p1adp(cs, args.workload, args.cpu_seconds)
         except Exception as e:  # the baseline is reported, not required
             log("cpu baseline failed:", e)
-    if args.workload in ("P1", "P1SHELL"):
+    if args.workload in ("P1", "P1SHELL", "P12"):
         # the P1 legs keep their patches in torch tensors: torch's HIP runtime has
         # to be initialised before the library's first HIP call
         import torch
@@ -981,6 +1066,9 @@ def main():
         return
     if args.workload == "P1SHELL":
         bench_p1_shell(cs, args)
+        return
+    if args.workload == "P12":
+        bench_p12(cs, args)
         return
     res = {}
     for prec in (cs.FP64, cs.MIXED):

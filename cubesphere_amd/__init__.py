@@ -706,17 +706,18 @@ class VolumeP12(ctypes.Structure):
     _fields_ = [(k, ctypes.c_void_p) for k in ("v_pts", "v_wts", "v_vol", "v_cen", "moments")]
 
 
-def build_volume_p12(cfg):
+def build_volume_p12(cfg, skip=()):
     """COORDS_TO_WT_QUADPTS_VOLUME_ISO2 with the 27-point rule and the ten basis
     moments of INIT_BASIS_FUNCTION_COEFS at third order, for the six sectors,
     built on the GPU: dict of v_pts (zone, 3, 27), v_wts (zone, 27), v_vol,
-    v_cen (zone, 3), moments (zone, 10), sector slowest."""
+    v_cen (zone, 3), moments (zone, 10), sector slowest; arrays named in skip
+    are not copied back."""
     nz = geometry_sizes(cfg)["zone"]
-    a = dict(v_pts=np.zeros(81 * nz), v_wts=np.zeros(27 * nz), v_vol=np.zeros(nz), v_cen=np.zeros(3 * nz),
-             moments=np.zeros(10 * nz))
+    per = dict(v_pts=81, v_wts=27, v_vol=1, v_cen=3, moments=10)
+    a = {k: np.zeros(m * nz) for k, m in per.items() if k not in skip}
     o = VolumeP12()
-    for k, v in a.items():
-        setattr(o, k, v.ctypes.data)
+    for k in per:
+        setattr(o, k, a[k].ctypes.data if k in a else None)
     _check(lib().cs_build_volume_p12(ctypes.byref(cfg), ctypes.byref(o)), "cs_build_volume_p12")
     return a
 
