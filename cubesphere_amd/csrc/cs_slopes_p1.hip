@@ -720,7 +720,11 @@ __global__ void __launch_bounds__(256) k_clear_ent(const P1 P, const Box b, cons
 
 // ---------------------------------------------------------------------------
 // step 8.a-c: blockIdx.y = family
-__global__ void __launch_bounds__(256) k_face_pts(const P1 P, const Box bx, const Box by, const Box bz) {
+// NM = 10: the third-order form (421...f:4817-4879), each side's field with
+// its quadratic modes about that zone's own moments mom
+template <int NM>
+__global__ void __launch_bounds__(256) k_face_pts(const P1 P, const Box bx, const Box by, const Box bz,
+                                                  const double* __restrict__ mom) {
   const int fam = blockIdx.y;
   const Box& b = fam == 0 ? bx : (fam == 1 ? by : bz);
   int i, j, k;
@@ -754,14 +758,19 @@ __global__ void __launch_bounds__(256) k_face_pts(const P1 P, const Box bx, cons
   } else {
     const int64_t zl = P.zi(li, lj, k), zr = P.zi(ri, rj, fam == 2 ? k + 1 : k);
     const double oll = P.obl(zl), olr = P.obl(zr);
-    double vl[3][4], vr[3][4];
+    double vl[3][NM], vr[3][NM], ql[6], qr[6];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
+      for (int m = 0; m < NM; ++m) {
         vl[c][m] = P.V(5 + c, m, zl);
         vr[c][m] = P.V(5 + c, m, zr);
       }
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+      ql[m] = NM == 10 ? mom[zl + P.NZ * (4 + m)] : 0.0;
+      qr[m] = NM == 10 ? mom[zr + P.NZ * (4 + m)] : 0.0;
+    }
     double n[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) n[c] = nrm[fg + NF * c];
@@ -781,8 +790,19 @@ __global__ void __launch_bounds__(256) k_face_pts(const P1 P, const Box bx, cons
       double s[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const double bl = vl[c][0] + vl[c][1] * rl[0] + vl[c][2] * rl[1] + vl[c][3] * rl[2];
-        const double br = vr[c][0] + vr[c][1] * rr[0] + vr[c][2] * rr[1] + vr[c][3] * rr[2];
+        double bl = vl[c][0] + vl[c][1] * rl[0] + vl[c][2] * rl[1] + vl[c][3] * rl[2];
+        double br = vr[c][0] + vr[c][1] * rr[0] + vr[c][2] * rr[1] + vr[c][3] * rr[2];
+        if constexpr (NM == 10) {
+          const double pl[6] = {rl[0] * rl[0] - ql[0], rl[1] * rl[1] - ql[1], rl[2] * rl[2] - ql[2],
+                                rl[0] * rl[1] - ql[3], rl[1] * rl[2] - ql[4], rl[0] * rl[2] - ql[5]};
+          const double pr[6] = {rr[0] * rr[0] - qr[0], rr[1] * rr[1] - qr[1], rr[2] * rr[2] - qr[2],
+                                rr[0] * rr[1] - qr[3], rr[1] * rr[2] - qr[4], rr[0] * rr[2] - qr[5]};
+#pragma unroll
+          for (int m = 0; m < 6; ++m) {
+            bl = bl + vl[c][4 + m] * pl[m];
+            br = br + vr[c][4 + m] * pr[m];
+          }
+        }
         s[c] = bl + br;
       }
       q5[q] = 0.5 * (n[0] * s[0] + n[1] * s[1] + n[2] * s[2]);
@@ -1102,9 +1122,9 @@ __global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const 
 }  // namespace csp1
 
 // mom (optional, DEVICE zone x 10): the third-order form (LIMIT_ALL_SLOPES_P12
-// steps 2-7.c): U and V of ten modes, the WENO gradients left unflattened
+// steps 2-8.c): U and V of ten modes, the WENO gradients left unflattened
 // and followed by the WENO-AO hybridisation of step 7.b, the positivity
-// limiter with the quadratic modes; the call ends after step 7
+// limiter and the face-point B with the quadratic modes; the call ends there
 static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t* ext = nullptr,
                   size_t ext_bytes = 0, const double* mom = nullptr) {
   int rc = p1_check(p);
@@ -1269,13 +1289,15 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
     }
     k_clear_ent<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, F.prev);
   }
-  if (mom) {  // the third-order steps 8-9 are not part of this call
+  if (mom) {  // third order: the face-point B (8.a-c); 8.d and 9 are not part of this call
+    k_face_pts<10><<<dim3(grid_of(std::max(fx.count(), std::max(fy.count(), fz.count())), 256), 3), 256, 0, st>>>(
+        P, fx, fy, fz, mom);
     P1_HIP(hipGetLastError());
     return CS_OK;
   }
   if ((rc = mark())) return rc;
-  k_face_pts<<<dim3(grid_of(std::max(fx.count(), std::max(fy.count(), fz.count())), 256), 3), 256, 0, st>>>(
-      P, fx, fy, fz);
+  k_face_pts<4><<<dim3(grid_of(std::max(fx.count(), std::max(fy.count(), fz.count())), 256), 3), 256, 0, st>>>(
+      P, fx, fy, fz, nullptr);
   if ((rc = mark())) return rc;
   k_clsq_e<<<grid_of(all.count(), 64), 64, 0, st>>>(P, all);
   if ((rc = mark())) return rc;

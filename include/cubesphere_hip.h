@@ -441,7 +441,7 @@ typedef struct cs_volume_p12 {
   double *v_pts, *v_wts, *v_vol, *v_cen, *moments;
 } cs_volume_p12;
 int cs_build_volume_p12(const cs_config* cfg, const cs_volume_p12* out);
-/* Steps 2 to 7.c of LIMIT_ALL_SLOPES_P12 (400riemann_integrate/
+/* Steps 2 to 8.c of LIMIT_ALL_SLOPES_P12 (400riemann_integrate/
  * 421limit_all_slopes_p12.f:9, called from the driver's stage loop at
  * NUMORDER 3, DRV:2395-2398) on one patch: the steps of LIMIT_ALL_SLOPES_P1
  * up to its WENO (the P1 gradients of the eight primitives left unflattened,
@@ -452,12 +452,16 @@ int cs_build_volume_p12(const cs_config* cfg, const cs_volume_p12* out);
  * (1 - flattener); step 7.c (:4558-4568): ENFORCE_PRESSURE_POSITIVITY_PRIM
  * (440enforce_pressure_positivity_prim.f:6) with the quadratic modes in the
  * node values (:594-600), reducing all ten modes, with the loop-order replay
- * of cs_limit_all_slopes_p1.  The patch is a cs_p1_patch whose U, V (zone,
+ * of cs_limit_all_slopes_p1; steps 8.a-c (:4668-5573): the normal B at the
+ * five B-field points of each face from both zones' quadratic fields about
+ * their own moments (:4817-4879), shifted to the face's mean B.  The patch
+ * is a cs_p1_patch whose U, V (zone,
  * 8, 10) and B1_zc, B2_zc, B3_zc (zone, 10) hold ten modes and whose
  * v_centroid comes from the 27-point rule, plus the ten basis moments
- * (DEVICE, zone x 10).  Outputs: V, the entropy, B1..3_zc, the flattener and
- * the work arrays of steps 2-6.  The facial B (step 8, with its 30-unknown
- * constrained least squares) and the zone E (9) are not part of this call.
+ * (DEVICE, zone x 10).  Outputs: V, the entropy, B1..3_zc, Bxf_q, Byf_q,
+ * Bzf_q, the flattener and the work arrays of steps 2-6.  The constrained
+ * least squares of step 8.d (30 unknowns) and the zone E (9) are not part
+ * of this call.
  * Work memory and stream as cs_limit_all_slopes_p1 (the call waits on the
  * stream once mid-way). */
 typedef struct cs_p12_patch {

@@ -581,20 +581,14 @@ static void p1_positivity(const or_p1_patch* p, const p1dims* d, int nm, const d
         }
 }
 
-int or_limit_all_slopes_p1(const or_p1_patch* p) {
-  if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1) return 1;
-  const p1dims D = p1_dims(p), *d = &D;
-  const int N = d->N, NR = d->NR, g = d->g;
-  g_p1_stats[0] = g_p1_stats[1] = 0;
-  const double pi = 4.0 * atan(1.0), ob8pi = 1.0 / (8.0 * pi), ob4pi = 1.0 / (4.0 * pi), gam = p->gamma;
-  int i, j, k, c, m, q;
-  p1_front(p, d, 4, 1);
-
-  p1_positivity(p, d, 4, NULL);
-
-  /* STEP 8.a-c (:3874-4640): normal B at the five B-field points of each face
-     from the two zones' linear fields, averaged, then shifted so the weighted
-     mean is the face's own Bf */
+/* STEP 8.a-c of LIMIT_ALL_SLOPES_P1 (410...f:3874-4640) and of
+ * LIMIT_ALL_SLOPES_P12 (421...f:4668-5573): normal B at the five B-field
+ * points of each face from the two zones' fields (linear, or with nm 10 the
+ * quadratic modes about each zone's moments mom), averaged, then shifted so
+ * the weighted mean is the face's own Bf */
+static void p1_face_points(const or_p1_patch* p, const p1dims* d, int nm, const double* mom) {
+  const int N = d->N, NR = d->NR;
+  int i, j, k, c, q;
   for (int fam = 0; fam < 3; ++fam) {
     const int ilo = fam == 0 ? -3 : -2, jlo = fam == 1 ? -3 : -2, klo = fam == 2 ? -3 : -2;
     double* Bq = fam == 0 ? p->Bxf_q : (fam == 1 ? p->Byf_q : p->Bzf_q);
@@ -638,6 +632,23 @@ int or_limit_all_slopes_p1(const or_p1_patch* p) {
                       VV(p, d, 5 + c, 3, zl) * rl[2];
               br[c] = VV(p, d, 5 + c, 0, zr) + VV(p, d, 5 + c, 1, zr) * rr[0] + VV(p, d, 5 + c, 2, zr) * rr[1] +
                       VV(p, d, 5 + c, 3, zr) * rr[2];
+              if (nm == 10) { /* 421...f:4817-4879: the quadratic modes about each zone's own moments */
+                const double* ml = mom + zl;
+                const double* mr = mom + zr;
+                const long S = d->NZ;
+                bl[c] = bl[c] + VV(p, d, 5 + c, 4, zl) * (rl[0] * rl[0] - ml[4 * S]) +
+                        VV(p, d, 5 + c, 5, zl) * (rl[1] * rl[1] - ml[5 * S]) +
+                        VV(p, d, 5 + c, 6, zl) * (rl[2] * rl[2] - ml[6 * S]) +
+                        VV(p, d, 5 + c, 7, zl) * (rl[0] * rl[1] - ml[7 * S]) +
+                        VV(p, d, 5 + c, 8, zl) * (rl[1] * rl[2] - ml[8 * S]) +
+                        VV(p, d, 5 + c, 9, zl) * (rl[0] * rl[2] - ml[9 * S]);
+                br[c] = br[c] + VV(p, d, 5 + c, 4, zr) * (rr[0] * rr[0] - mr[4 * S]) +
+                        VV(p, d, 5 + c, 5, zr) * (rr[1] * rr[1] - mr[5 * S]) +
+                        VV(p, d, 5 + c, 6, zr) * (rr[2] * rr[2] - mr[6 * S]) +
+                        VV(p, d, 5 + c, 7, zr) * (rr[0] * rr[1] - mr[7 * S]) +
+                        VV(p, d, 5 + c, 8, zr) * (rr[1] * rr[2] - mr[8 * S]) +
+                        VV(p, d, 5 + c, 9, zr) * (rr[0] * rr[2] - mr[9 * S]);
+              }
             }
             Bq[fb + d->NFC * q] = 0.5 * (n[0] * (bl[0] + br[0]) + n[1] * (bl[1] + br[1]) + n[2] * (bl[2] + br[2]));
           }
@@ -646,6 +657,21 @@ int or_limit_all_slopes_p1(const or_p1_patch* p) {
           for (q = 0; q < 5; ++q) Bq[fb + d->NFC * q] = Bq[fb + d->NFC * q] + bd;
         }
   }
+
+}
+
+int or_limit_all_slopes_p1(const or_p1_patch* p) {
+  if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1) return 1;
+  const p1dims D = p1_dims(p), *d = &D;
+  const int N = d->N, NR = d->NR, g = d->g;
+  g_p1_stats[0] = g_p1_stats[1] = 0;
+  const double pi = 4.0 * atan(1.0), ob8pi = 1.0 / (8.0 * pi), ob4pi = 1.0 / (4.0 * pi), gam = p->gamma;
+  int i, j, k, c, m, q;
+  p1_front(p, d, 4, 1);
+
+  p1_positivity(p, d, 4, NULL);
+
+  p1_face_points(p, d, 4, NULL);
 
   /* STEP 8.d (:4643-5360): ADP constrained least squares for the 12 linear
      coefficients of B: the six face means exactly (constraints), points 2 and
@@ -2240,9 +2266,9 @@ int or_ao_fit_p12(const or_p1_patch* p, const double* mom, int i, int j, int k, 
   return 0;
 }
 
-/* the zone-centred part of LIMIT_ALL_SLOPES_P12: steps 2 to 7.a with ten
- * modes and unflattened WENO gradients, 7.b, then the positivity limiter of
- * 7.c with the quadratic modes */
+/* LIMIT_ALL_SLOPES_P12 up to its facial B points: steps 2 to 7.a with ten
+ * modes and unflattened WENO gradients, 7.b, the positivity limiter of 7.c
+ * with the quadratic modes, and 8.a-c with them */
 int or_limit_slopes_p12_zones(const or_p1_patch* p, const double* mom) {
   if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1 || !mom) return 1;
   const p1dims D = p1_dims(p);
@@ -2250,5 +2276,6 @@ int or_limit_slopes_p12_zones(const or_p1_patch* p, const double* mom) {
   if (or_weno_ao_p12(p, mom)) return 1;
   g_p1_stats[0] = g_p1_stats[1] = 0;
   p1_positivity(p, &D, 10, mom);
+  p1_face_points(p, &D, 10, mom);
   return 0;
 }
