@@ -729,10 +729,10 @@ class P12Patch(ctypes.Structure):
 
 
 def limit_slopes_p12_zones(N, NR, g, scalars, arrays, moments, itri=(1, 1, 1, 1), stream=None):
-    """Steps 2 to 8.c of LIMIT_ALL_SLOPES_P12 on one patch (the P1 steps up to
+    """Steps 2 to 8.d of LIMIT_ALL_SLOPES_P12 on one patch (the P1 steps up to
     the WENO, the WENO-AO hybridisation with the quadratic central stencil,
-    the positivity limiter and the face-point B with the quadratic modes), in
-    place on torch CUDA
+    the positivity limiter and the face-point B with the quadratic modes, the
+    30-unknown constrained least squares of the zone B), in place on torch CUDA
     tensors in the reference's layouts: as limit_all_slopes_p1, with U and V
     of ten modes (zone, 8, 10), B1_zc, B2_zc, B3_zc of ten, v_centroid from
     the 27-point rule, and moments (zone, 10) a float64 CUDA tensor."""

@@ -1117,14 +1117,16 @@ static inline unsigned grid_of(int64_t n, int threads) { return (unsigned)((n + 
 // instead of the library's per-device allocation (no allocation then)
 namespace csp1 {
 __global__ void __launch_bounds__(256) k_zero_hi(const P1 P, const Box b);
+__global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const double* __restrict__ mom);
 __global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom,
                                                  const int64_t* list, int64_t nlist);
 }  // namespace csp1
 
 // mom (optional, DEVICE zone x 10): the third-order form (LIMIT_ALL_SLOPES_P12
-// steps 2-8.c): U and V of ten modes, the WENO gradients left unflattened
+// steps 2-8.d): U and V of ten modes, the WENO gradients left unflattened
 // and followed by the WENO-AO hybridisation of step 7.b, the positivity
-// limiter and the face-point B with the quadratic modes; the call ends there
+// limiter and the face-point B with the quadratic modes, the 30-unknown
+// constrained least squares; the call ends there
 static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t* ext = nullptr,
                   size_t ext_bytes = 0, const double* mom = nullptr) {
   int rc = p1_check(p);
@@ -1289,9 +1291,10 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
     }
     k_clear_ent<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, F.prev);
   }
-  if (mom) {  // third order: the face-point B (8.a-c); 8.d and 9 are not part of this call
+  if (mom) {  // third order: the face-point B (8.a-c) and the zone B (8.d); 9 is not part of this call
     k_face_pts<10><<<dim3(grid_of(std::max(fx.count(), std::max(fy.count(), fz.count())), 256), 3), 256, 0, st>>>(
         P, fx, fy, fz, mom);
+    k_clsq12<<<(unsigned)wr.count(), 64, 0, st>>>(P, wr, mom);
     P1_HIP(hipGetLastError());
     return CS_OK;
   }
@@ -2926,6 +2929,172 @@ __global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const 
 #pragma unroll
       for (int m = 3; m < 9; ++m) P.Vr(comp, m + 1, z) = whi * Phi[m] * x1;
     }
+  }
+}
+
+// Step 8.d at third order (421...f:5574-6680), one wave per zone: the 36 x 36
+// system of the normal equations of the 64 least-squares rows (n . B at the
+// 30 face points against their face-point B, the 30 WENO-AO modes, the
+// divergence and its gradient) and the 6 face-mean constraints.  Lanes 0-29
+// put one face point each (normal, the ten basis values about the zone's
+// moments, B-field weight, face-point B) in LDS; lane r < 36 forms row r of
+// the augmented system in registers (A^T A = P^T P + I + D^T D, P the point
+// rows, D the divergence rows); Gaussian elimination with scaled partial
+// pivoting across the lanes (the pivot row broadcast, no row exchange), then
+// back substitution in pivot order.  The reference solves the same system by
+// Numerical Recipes' LU (MATSOLVE): the two agree to rounding.
+__constant__ double c_div_rows[4][30] = {
+    {0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0},
+    {0, 0, 0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1},
+    {0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0},
+    {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0}};
+
+__global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const double* __restrict__ mom) {
+  __shared__ double pt[30][16];  // n (3), basis (10), weight, face-point B
+  const int lane = threadIdx.x;
+  int i, j, k;
+  if (!b.decode(blockIdx.x, i, j, k)) return;  // one zone per workgroup: uniform
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k), NZ = P.NZ;
+  if (a.mask[z] == 0) return;
+  const double obl = P.obl(z);
+  double c0[10];
+#pragma unroll
+  for (int m = 0; m < 10; ++m) c0[m] = mom[z + NZ * m];
+  if (lane < 30) {
+    const int f = lane / 5, q = lane % 5, fam = f >> 1;
+    const bool hi = (f & 1) == 0;
+    const int fi = i - (fam == 0 && !hi), fj = j - (fam == 1 && !hi), fk = k - (fam == 2 && !hi);
+    const int64_t fg = fam == 0 ? P.xfi(fi, fj, fk) : (fam == 1 ? P.yfi(fi, fj, fk) : P.zfi(fi, fj, fk));
+    const int64_t NF = fam == 2 ? P.NZF : P.NXF, fb = P.fci(fi, fj, fk);
+    const double* nrm = fam == 0 ? a.xb_normal : (fam == 1 ? a.yb_normal : a.zb_normal);
+    const double* pts = fam == 0 ? a.xb_pts : (fam == 1 ? a.yb_pts : a.zb_pts);
+    const double* wts = fam == 0 ? a.xb_wts : (fam == 1 ? a.yb_wts : a.zb_wts);
+    const double* Bq = fam == 0 ? a.Bxf_q : (fam == 1 ? a.Byf_q : a.Bzf_q);
+    double r[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      pt[lane][c] = fam == 2 ? nrm[fg + NF * (c + 3 * q)] : nrm[fg + NF * c];
+      r[c] = (pts[fg + NF * (c + 3 * q)] - P.cen(z, c)) * obl;
+    }
+    pt[lane][3] = 1.0;
+    pt[lane][4] = r[0];
+    pt[lane][5] = r[1];
+    pt[lane][6] = r[2];
+    pt[lane][7] = r[0] * r[0] - c0[4];
+    pt[lane][8] = r[1] * r[1] - c0[5];
+    pt[lane][9] = r[2] * r[2] - c0[6];
+    pt[lane][10] = r[0] * r[1] - c0[7];
+    pt[lane][11] = r[1] * r[2] - c0[8];
+    pt[lane][12] = r[0] * r[2] - c0[9];
+    pt[lane][13] = wts[fg + NF * q];
+    pt[lane][14] = Bq[fb + P.NFC * q];
+  }
+  __syncthreads();
+  // row `lane` of the augmented system [K | rhs]
+  double row[37];
+#pragma unroll
+  for (int e = 0; e < 37; ++e) row[e] = 0.0;
+  if (lane < 30) {
+    const int c = lane / 10, m = lane % 10;
+#pragma unroll
+    for (int e = 0; e < 30; ++e) {
+      double t = e == lane ? 1.0 : 0.0;
+#pragma unroll
+      for (int d = 0; d < 4; ++d) t += c_div_rows[d][lane] * c_div_rows[d][e];
+      row[e] = t;
+    }
+    row[36] = P.V(5 + c, m, z);
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+      double cf = 0.0;  // the constraint column of face f
+#pragma unroll 1
+      for (int q = 5 * f; q < 5 * f + 5; ++q) {
+        const double pa = pt[q][c] * pt[q][3 + m];
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+          const double pn = pa * pt[q][cc];
+#pragma unroll
+          for (int mm = 0; mm < 10; ++mm) row[10 * cc + mm] += pn * pt[q][3 + mm];
+        }
+        row[36] += pa * pt[q][14];
+        cf += pt[q][13] * pa;
+      }
+      row[30 + f] = cf;
+    }
+  } else if (lane < 36) {
+    const int f = lane - 30;
+#pragma unroll
+    for (int q5 = 0; q5 < 5; ++q5) {
+      const int q = 5 * f + q5;
+      const double w = pt[q][13];
+#pragma unroll
+      for (int cc = 0; cc < 3; ++cc) {
+        const double wn = w * pt[q][cc];
+#pragma unroll
+        for (int mm = 0; mm < 10; ++mm) row[10 * cc + mm] += wn * pt[q][3 + mm];
+      }
+      row[36] += w * pt[q][14];
+    }
+  }
+  // elimination with scaled partial pivoting; lane piv[jj] holds pivot row jj
+  double big = 0.0;
+#pragma unroll
+  for (int e = 0; e < 36; ++e) big = fmax(big, fabs(row[e]));
+  const double vv = lane < 36 && big > 0.0 ? 1.0 / big : 0.0;
+  bool done = lane >= 36;
+  int piv[36];
+#pragma unroll
+  for (int jj = 0; jj < 36; ++jj) {
+    double best = done ? -1.0 : vv * fabs(row[jj]);
+    int bl = lane;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const double ob = __shfl_xor(best, off);
+      const int ol = __shfl_xor(bl, off);
+      if (ob > best || (ob == best && ol < bl)) {
+        best = ob;
+        bl = ol;
+      }
+    }
+    const int p = __builtin_amdgcn_readfirstlane(bl);
+    piv[jj] = p;
+    double pr[37];
+#pragma unroll
+    for (int e = jj; e < 37; ++e) pr[e] = __shfl(row[e], p);
+    if (lane == p) done = true;
+    if (!done) {
+      const double fct = row[jj] / pr[jj];
+#pragma unroll
+      for (int e = jj + 1; e < 37; ++e) row[e] -= fct * pr[e];
+    }
+  }
+  double x[36];
+#pragma unroll
+  for (int jj = 35; jj >= 0; --jj) {
+    double t = row[36];
+#pragma unroll
+    for (int e = jj + 1; e < 36; ++e) t -= row[e] * x[e];
+    x[jj] = __shfl(t / row[jj], piv[jj]);
+  }
+  if (lane < 30) {
+    const int c = lane / 10, m = lane % 10;
+    double* Bz = c == 0 ? a.B1_zc : (c == 1 ? a.B2_zc : a.B3_zc);
+    double mine = x[0];
+#pragma unroll
+    for (int e = 1; e < 30; ++e)
+      if (e == lane) mine = x[e];
+    Bz[z + NZ * m] = mine;
+    P.Vr(5 + c, m, z) = mine;
+  }
+  if (lane == 0) {
+    const double ob8pi = 1.0 / (8.0 * M_PI);
+    const double aa = (P.Ur(5, 0, z) * P.Ur(5, 0, z) + P.Ur(6, 0, z) * P.Ur(6, 0, z) + P.Ur(7, 0, z) * P.Ur(7, 0, z)) * ob8pi;
+    P.Ur(5, 0, z) = x[0];
+    P.Ur(6, 0, z) = x[10];
+    P.Ur(7, 0, z) = x[20];
+    const double bb = (x[0] * x[0] + x[10] * x[10] + x[20] * x[20]) * ob8pi;
+    P.Ur(4, 0, z) = P.Ur(4, 0, z) - aa + bb;
   }
 }
 

@@ -441,7 +441,7 @@ typedef struct cs_volume_p12 {
   double *v_pts, *v_wts, *v_vol, *v_cen, *moments;
 } cs_volume_p12;
 int cs_build_volume_p12(const cs_config* cfg, const cs_volume_p12* out);
-/* Steps 2 to 8.c of LIMIT_ALL_SLOPES_P12 (400riemann_integrate/
+/* Steps 2 to 8.d of LIMIT_ALL_SLOPES_P12 (400riemann_integrate/
  * 421limit_all_slopes_p12.f:9, called from the driver's stage loop at
  * NUMORDER 3, DRV:2395-2398) on one patch: the steps of LIMIT_ALL_SLOPES_P1
  * up to its WENO (the P1 gradients of the eight primitives left unflattened,
@@ -454,14 +454,17 @@ int cs_build_volume_p12(const cs_config* cfg, const cs_volume_p12* out);
  * node values (:594-600), reducing all ten modes, with the loop-order replay
  * of cs_limit_all_slopes_p1; steps 8.a-c (:4668-5573): the normal B at the
  * five B-field points of each face from both zones' quadratic fields about
- * their own moments (:4817-4879), shifted to the face's mean B.  The patch
- * is a cs_p1_patch whose U, V (zone,
+ * their own moments (:4817-4879), shifted to the face's mean B; step 8.d
+ * (:5574-6680): the zone's 30 quadratic B coefficients from the
+ * constrained least squares (the six face means exactly; n . B at the 30
+ * face points, the WENO-AO modes and a zero divergence and divergence
+ * gradient in the least-squares sense), U's B means and energy updated.
+ * The patch is a cs_p1_patch whose U, V (zone,
  * 8, 10) and B1_zc, B2_zc, B3_zc (zone, 10) hold ten modes and whose
  * v_centroid comes from the 27-point rule, plus the ten basis moments
  * (DEVICE, zone x 10).  Outputs: V, the entropy, B1..3_zc, Bxf_q, Byf_q,
- * Bzf_q, the flattener and the work arrays of steps 2-6.  The constrained
- * least squares of step 8.d (30 unknowns) and the zone E (9) are not part
- * of this call.
+ * Bzf_q, the flattener and the work arrays of steps 2-6.  The zone E and
+ * its WENO-AO (step 9) are not part of this call.
  * Work memory and stream as cs_limit_all_slopes_p1 (the call waits on the
  * stream once mid-way). */
 typedef struct cs_p12_patch {

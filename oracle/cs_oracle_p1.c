@@ -174,7 +174,7 @@ static void weno_apply(const weno_mats* W, double f0, const double* f6, double s
 /* Numerical Recipes' LU decomposition with implicit (row-scaled) partial
  * pivoting (Crout), then forward/back substitution: Press et al., Numerical
  * Recipes, 2nd ed., sec. 2.3, as called through MATSOLVE (120matsolve.f). */
-#define LUMAX 18
+#define LUMAX 36
 static int lu_dcmp(int n, double a[LUMAX][LUMAX], int* indx) {
   int imax = 0;
   double vv[LUMAX];
@@ -2266,9 +2266,115 @@ int or_ao_fit_p12(const or_p1_patch* p, const double* mom, int i, int j, int k, 
   return 0;
 }
 
-/* LIMIT_ALL_SLOPES_P12 up to its facial B points: steps 2 to 7.a with ten
- * modes and unflattened WENO gradients, 7.b, the positivity limiter of 7.c
- * with the quadratic modes, and 8.a-c with them */
+/* CMat_Row_Coefs_P2 (200mesh/253cmat_row_coefs_p2.f:7): n (x) the ten
+ * basis functions at the scaled point (x, y, z) about a zone with moments c0 */
+static void cmat_row_p2(double x, double y, double z, const double* n, const double* c0, double* row) {
+  const double b[10] = {1.0, x, y, z, x * x - c0[4], y * y - c0[5], z * z - c0[6], x * y - c0[7], y * z - c0[8],
+                        x * z - c0[9]};
+  for (int a = 0; a < 3; ++a) {
+    row[10 * a] = n[a];
+    for (int m = 1; m < 10; ++m) row[10 * a + m] = n[a] * b[m];
+  }
+}
+
+/* STEP 8.d of LIMIT_ALL_SLOPES_P12 (421...f:5574-6680): the almost
+ * divergence-preserving constrained least squares for the 30 quadratic
+ * coefficients of B.  Constraints (6): each face's weighted mean of n . B
+ * over its five B-field points equals that of its face-point B.  Least
+ * squares (64 rows): n . B at every face point against the face-point B (30),
+ * the 30 WENO-AO modes, and the divergence and its gradient set to zero (4).
+ * The 36 x 36 system of normal equations and constraints is solved by
+ * MATSOLVE; B_zc, V's B modes and U's B means take the result and the energy
+ * the change of magnetic energy. */
+static void p12_clsq(const or_p1_patch* p, const p1dims* d, const double* mom) {
+  const int N = d->N, NR = d->NR;
+  const double pi = 4.0 * atan(1.0), ob8pi = 1.0 / (8.0 * pi);
+  for (int k = -2; k <= NR + 3; ++k)
+    for (int j = -2; j <= N + 3; ++j)
+      for (int i = -2; i <= N + 3; ++i) {
+        const long z = zi(d, i, j, k);
+        if (p->mask[z] == 0) continue;
+        const double obl = obl_of(p, z);
+        double c0[10], Cm[6][30], dv[6], Am[64][30], bv[64];
+        for (int m = 0; m < 10; ++m) c0[m] = mom[z + d->NZ * m];
+        memset(Cm, 0, sizeof Cm);
+        memset(dv, 0, sizeof dv);
+        memset(Am, 0, sizeof Am);
+        memset(bv, 0, sizeof bv);
+        /* faces in the order right x, left x, up y, down y, top z, bottom z */
+        for (int f = 0; f < 6; ++f) {
+          const int fam = f / 2, hi = f % 2 == 0;
+          const int fi_ = i - (fam == 0 && !hi), fj_ = j - (fam == 1 && !hi), fk_ = k - (fam == 2 && !hi);
+          const long fg = fam == 0 ? xfi(d, fi_, fj_, fk_) : (fam == 1 ? yfi(d, fi_, fj_, fk_) : zfi(d, fi_, fj_, fk_));
+          const long NF = fam == 0 ? d->NXF : (fam == 1 ? d->NYF : d->NZF);
+          const long fb = fci(d, fi_, fj_, fk_);
+          const double* nrm = fam == 0 ? p->xb_normal : (fam == 1 ? p->yb_normal : p->zb_normal);
+          const double* pts = fam == 0 ? p->xb_pts : (fam == 1 ? p->yb_pts : p->zb_pts);
+          const double* wts = fam == 0 ? p->xb_wts : (fam == 1 ? p->yb_wts : p->zb_wts);
+          const double* Bq = fam == 0 ? p->Bxf_q : (fam == 1 ? p->Byf_q : p->Bzf_q);
+          for (int q = 0; q < 5; ++q) {
+            double n[3], r[3], row[30];
+            for (int c = 0; c < 3; ++c) n[c] = fam == 2 ? nrm[fg + NF * (c + 3 * q)] : nrm[fg + NF * c];
+            const double wt = wts[fg + NF * q], bq = Bq[fb + d->NFC * q];
+            rel(p, d, z, pts + fg + NF * (3 * q), NF, obl, r);
+            cmat_row_p2(r[0], r[1], r[2], n, c0, row);
+            for (int u = 0; u < 30; ++u) {
+              Cm[f][u] = Cm[f][u] + wt * row[u];
+              Am[5 * f + q][u] = row[u];
+            }
+            dv[f] = dv[f] + wt * bq;
+            bv[5 * f + q] = bq;
+          }
+        }
+        for (int q = 0; q < 30; ++q) {  /* the WENO-AO modes */
+          Am[30 + q][q] = 1.0;
+          bv[30 + q] = VV(p, d, 5 + q / 10, q % 10, z);
+        }
+        /* div B = 0 and its gradient: unknowns c * 10 + mode (1 x, 2 y, 3 z,
+           4 xx, 5 yy, 6 zz, 7 xy, 8 yz, 9 xz) */
+        Am[60][1] = 1.0; Am[60][12] = 1.0; Am[60][23] = 1.0;
+        Am[61][4] = 2.0; Am[61][17] = 1.0; Am[61][29] = 1.0;
+        Am[62][7] = 1.0; Am[62][15] = 2.0; Am[62][28] = 1.0;
+        Am[63][9] = 1.0; Am[63][18] = 1.0; Am[63][26] = 2.0;
+        double K[LUMAX][LUMAX], rhs[LUMAX];
+        for (int a = 0; a < 30; ++a) {
+          for (int b = 0; b < 30; ++b) {
+            double t = 0.0;
+            for (int q = 0; q < 64; ++q) t = t + Am[q][a] * Am[q][b];
+            K[a][b] = t;
+          }
+          double t = 0.0;
+          for (int q = 0; q < 64; ++q) t = t + Am[q][a] * bv[q];
+          rhs[a] = t;
+          for (int f = 0; f < 6; ++f) {
+            K[a][30 + f] = Cm[f][a];
+            K[30 + f][a] = Cm[f][a];
+          }
+        }
+        for (int f = 0; f < 6; ++f) {
+          for (int h = 0; h < 6; ++h) K[30 + f][30 + h] = 0.0;
+          rhs[30 + f] = dv[f];
+        }
+        lu_solve(36, K, rhs);
+        const double aa = (UU(p, d, 5, 0, z) * UU(p, d, 5, 0, z) + UU(p, d, 6, 0, z) * UU(p, d, 6, 0, z) +
+                           UU(p, d, 7, 0, z) * UU(p, d, 7, 0, z)) * ob8pi;
+        for (int m = 0; m < 10; ++m) {
+          p->B1_zc[z + d->NZ * m] = rhs[m];
+          p->B2_zc[z + d->NZ * m] = rhs[10 + m];
+          p->B3_zc[z + d->NZ * m] = rhs[20 + m];
+          for (int c = 0; c < 3; ++c) VV(p, d, 5 + c, m, z) = rhs[10 * c + m];
+        }
+        for (int c = 0; c < 3; ++c) UU(p, d, 5 + c, 0, z) = VV(p, d, 5 + c, 0, z);
+        const double bb = (UU(p, d, 5, 0, z) * UU(p, d, 5, 0, z) + UU(p, d, 6, 0, z) * UU(p, d, 6, 0, z) +
+                           UU(p, d, 7, 0, z) * UU(p, d, 7, 0, z)) * ob8pi;
+        UU(p, d, 4, 0, z) = UU(p, d, 4, 0, z) - aa + bb;
+      }
+}
+
+/* LIMIT_ALL_SLOPES_P12 up to its zone B: steps 2 to 7.a with ten modes and
+ * unflattened WENO gradients, 7.b, the positivity limiter of 7.c and the
+ * face-point B of 8.a-c with the quadratic modes, and the constrained least
+ * squares of 8.d */
 int or_limit_slopes_p12_zones(const or_p1_patch* p, const double* mom) {
   if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1 || !mom) return 1;
   const p1dims D = p1_dims(p);
@@ -2277,5 +2383,6 @@ int or_limit_slopes_p12_zones(const or_p1_patch* p, const double* mom) {
   g_p1_stats[0] = g_p1_stats[1] = 0;
   p1_positivity(p, &D, 10, mom);
   p1_face_points(p, &D, 10, mom);
+  p12_clsq(p, &D, mom);
   return 0;
 }

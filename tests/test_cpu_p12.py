@@ -2,14 +2,15 @@
 on the CPU: the oracle's restatement against the reference's own routines
 built at NUMORDER 3 (oracle/_ref/libcsref3.so), and against closed forms.
 
-The oracle restates steps 2 to 8.c: the P1 WENO gradients (7.a) left
+The oracle restates steps 2 to 8.d: the P1 WENO gradients (7.a) left
 unflattened, the WENO-AO hybridisation of 7.b with the 18-zone quadratic
 least-squares stencil (RECONSTRUCTION_COEFS, 251...f:3; MATINV,
 121matinv.f:8) and the flattener, the positivity limiter with the quadratic
-modes (7.c, 440...f:594-600), and the face-point B of 8.a-c with them
-(421...f:4817-4879).  The reference runs its whole routine, so what steps
-8.d-9 leave alone is compared: the five hydrodynamic primitives (step 8.d
-rewrites the magnetic ones), the entropy and the face-point B."""
+modes (7.c, 440...f:594-600), the face-point B of 8.a-c with them
+(421...f:4817-4879), and the 30-unknown constrained least squares of 8.d
+(CMat_Row_Coefs_P2, 253...f:7; MATSOLVE).  The reference runs its whole
+routine; what step 9 (the zone E) leaves alone is compared: V, U, B_zc, the
+face-point B and the entropy."""
 import ctypes
 
 import numpy as np
@@ -118,7 +119,9 @@ def test_p12_zones_bit_exact_with_reference_smooth(case):
     assert np.array_equal(_hydro(A, nz), _hydro(B, nz))
     assert np.array_equal(A["ent"], B["ent"])
     assert np.array_equal(A["flatten"], B["flatten"])
-    for k in ("Bxf_q", "Byf_q", "Bzf_q"):  # step 8.a-c: face-point B with the quadratic modes
+    # step 8.a-c: face-point B with the quadratic modes; 8.d: the zone B, U's
+    # B means and energy from the 30-unknown constrained least squares
+    for k in ("Bxf_q", "Byf_q", "Bzf_q", "V", "U", "B1_zc", "B2_zc", "B3_zc"):
         assert np.array_equal(A[k], B[k]), k
     # the quadratic modes are there (not all zero)
     assert np.abs(_hydro(A, nz)[4:]).max() > 1e-4
@@ -143,7 +146,7 @@ def test_p12_zones_with_flattener_and_positivity_bit_exact(case):
     B = orc.p12_reference(prob, orc.p1_copy(A0))
     assert np.array_equal(_hydro(A, nz), _hydro(B, nz))
     assert np.array_equal(A["ent"], B["ent"])
-    for k in ("Bxf_q", "Byf_q", "Bzf_q"):
+    for k in ("Bxf_q", "Byf_q", "Bzf_q", "V", "U", "B1_zc", "B2_zc", "B3_zc"):
         assert np.array_equal(A[k], B[k]), k
     assert (A["flatten"] > 0).sum() > 100
     assert st[0] > 5, "the positivity limiter must act"

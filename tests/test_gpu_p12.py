@@ -1,12 +1,13 @@
 """GPU parity of the third-order entry points with the oracle, which is
 bit-exact with the reference built at NUMORDER 3 (tests/test_cpu_p12.py):
 cs_build_volume_p12 (COORDS_TO_WT_QUADPTS_VOLUME_ISO2 with 27 points and ten
-basis moments) and cs_limit_slopes_p12_zones (steps 2 to 8.c of
+basis moments) and cs_limit_slopes_p12_zones (steps 2 to 8.d of
 LIMIT_ALL_SLOPES_P12, 421limit_all_slopes_p12.f:9).
 
 Tolerance: the HIP path fuses multiply-adds and solves the 9 x 9 normal
 equations of the quadratic stencil by Cholesky where the reference inverts
-them by LU, so the geometry agrees to 1e-13 and each (component, mode) row of
+them by LU, and the 36 x 36 system of step 8.d by elimination across a
+wave's lanes where the reference uses LU, so the geometry agrees to 1e-13 and each (component, mode) row of
 V, B*_zc and the face-point B to 1e-9 of its largest magnitude; the flattener, the entropy and
 the integer arrays to 1e-10 / exactly.  The positivity limiter's discrete
 decisions (how many reductions, removal) must be the same: a different
@@ -69,7 +70,7 @@ def test_p12_zones_match_oracle(case):
                 bad.append((k, "int"))
             continue
         if k in ("Ex_zc", "Ey_zc", "Ez_zc"):
-            continue  # steps 8.d-9 are not part of the call: both keep their inputs
+            continue  # step 9 is not part of the call: both keep their inputs
         rows = 80 if k in ("U", "V") else (4 if k == "ent" else (10 if k.endswith("_zc") else
                                                                    (5 if k.endswith("_q") else 1)))
         rtol = 1e-9 if (k == "V" or k.endswith("_zc") or k.endswith("_q")) else 1e-10
