@@ -156,7 +156,7 @@ def lib():
             "cs_stencil_indexing": [cfgp, vp],
             "cs_limit_all_slopes_p1": [vp, vp],
             "cs_build_volume_p12": [cfgp, vp],
-            "cs_limit_slopes_p12_zones": [vp, vp],
+            "cs_limit_all_slopes_p12": [vp, vp],
             "cs_limit_all_slopes_p1_timed": [vp, vp, ctypes.POINTER(ctypes.c_float)],
             "cs_limit_all_slopes_p1_noalloc": [vp, vp],
             "cs_make_facial_fluxes_p1": [vp, vp],
@@ -699,7 +699,7 @@ def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=Non
 
 # ---------------------------------------------------------------------------
 # Third order: the NUMORDER 3 volume geometry and the zone-centred part of
-# LIMIT_ALL_SLOPES_P12 (cs_build_volume_p12, cs_limit_slopes_p12_zones)
+# LIMIT_ALL_SLOPES_P12 (cs_build_volume_p12, cs_limit_all_slopes_p12)
 
 class VolumeP12(ctypes.Structure):
     """cs_volume_p12: host outputs of the third-order volume geometry."""
@@ -728,19 +728,21 @@ class P12Patch(ctypes.Structure):
     _fields_ = [("p1", P1Patch), ("moments", ctypes.c_void_p)]
 
 
-def limit_slopes_p12_zones(N, NR, g, scalars, arrays, moments, itri=(1, 1, 1, 1), stream=None):
-    """Steps 2 to 8.d of LIMIT_ALL_SLOPES_P12 on one patch (the P1 steps up to
-    the WENO, the WENO-AO hybridisation with the quadratic central stencil,
-    the positivity limiter and the face-point B with the quadratic modes, the
-    30-unknown constrained least squares of the zone B), in place on torch CUDA
-    tensors in the reference's layouts: as limit_all_slopes_p1, with U and V
-    of ten modes (zone, 8, 10), B1_zc, B2_zc, B3_zc of ten, v_centroid from
-    the 27-point rule, and moments (zone, 10) a float64 CUDA tensor."""
+def limit_all_slopes_p12(N, NR, g, scalars, arrays, moments, itri=(1, 1, 1, 1), stream=None):
+    """LIMIT_ALL_SLOPES_P12 on one patch, the third-order WENO-AO
+    reconstruction (the P1 steps up to the WENO, the hybridisation with the
+    quadratic central stencil, the positivity limiter and the face-point B
+    with the quadratic modes, the 30-unknown constrained least squares of the
+    zone B, the zone E and its WENO-AO, the min/max diagnostics), in place on
+    torch CUDA tensors in the reference's layouts: as limit_all_slopes_p1,
+    with U and V of ten modes (zone, 8, 10), B1..3_zc and Ex..Ez_zc of ten,
+    v_centroid from the 27-point rule, and moments (zone, 10) a float64 CUDA
+    tensor.  Returns (V_min_local, V_max_local) as numpy arrays of 9."""
     import torch
     want = p1_counts(N, NR, g)
     nzone = want["mask"]
     want["U"] = want["V"] = 80 * nzone
-    for k in ("B1_zc", "B2_zc", "B3_zc"):
+    for k in ("B1_zc", "B2_zc", "B3_zc", "Ex_zc", "Ey_zc", "Ez_zc"):
         want[k] = 10 * nzone
     s = P12Patch()
     q = s.p1
@@ -755,14 +757,17 @@ def limit_slopes_p12_zones(N, NR, g, scalars, arrays, moments, itri=(1, 1, 1, 1)
         t = tensors[k]
         dt = torch.int32 if k in ("mask", "stencil") else torch.float64
         if not (t.is_cuda and t.dtype == dt and t.is_contiguous() and t.numel() >= want[k]):
-            raise CsError(f"limit_slopes_p12_zones: {k} must be a contiguous {dt} CUDA tensor of at least "
+            raise CsError(f"limit_all_slopes_p12: {k} must be a contiguous {dt} CUDA tensor of at least "
                           f"{want[k]} elements (got {t.dtype}, {t.numel()}, cuda={t.is_cuda})")
         if k == "moments":
             s.moments = t.data_ptr()
         else:
             setattr(q, k, t.data_ptr())
+    vmin, vmax = np.zeros(9), np.zeros(9)
+    q.V_min_local, q.V_max_local = vmin.ctypes.data, vmax.ctypes.data
     st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
-    _check(lib().cs_limit_slopes_p12_zones(ctypes.byref(s), st), "cs_limit_slopes_p12_zones")
+    _check(lib().cs_limit_all_slopes_p12(ctypes.byref(s), st), "cs_limit_all_slopes_p12")
+    return vmin, vmax
 
 
 # ---------------------------------------------------------------------------

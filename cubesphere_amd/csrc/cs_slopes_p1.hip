@@ -1118,15 +1118,23 @@ static inline unsigned grid_of(int64_t n, int threads) { return (unsigned)((n + 
 namespace csp1 {
 __global__ void __launch_bounds__(256) k_zero_hi(const P1 P, const Box b);
 __global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const double* __restrict__ mom);
+__global__ void __launch_bounds__(256) k_e12(const P1 P, const Box b, const double* __restrict__ mom);
+__global__ void __launch_bounds__(256) k_ao_order(const P1 P, const Box b, unsigned int* bad);
+constexpr int kAoRec = 200;  // doubles per zone record of k_ao_e_pre: G (19 x 10), 18 int32 indices
+constexpr int kAoWin = 8, kAoLag = kAoWin + 8;  // levels per launch of k_weno_ao_e_win, and the plane lag
+__global__ void __launch_bounds__(128) k_ao_e_pre(const P1 P, const Box b, const double* __restrict__ mom,
+                                                  double* __restrict__ rec);
+__global__ void __launch_bounds__(256) k_weno_ao_e_win(const P1 P, const Box b, const double* __restrict__ mom,
+                                                       const double* __restrict__ rec, int s, int klo);
 __global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom,
                                                  const int64_t* list, int64_t nlist);
 }  // namespace csp1
 
-// mom (optional, DEVICE zone x 10): the third-order form (LIMIT_ALL_SLOPES_P12
-// steps 2-8.d): U and V of ten modes, the WENO gradients left unflattened
-// and followed by the WENO-AO hybridisation of step 7.b, the positivity
-// limiter and the face-point B with the quadratic modes, the 30-unknown
-// constrained least squares; the call ends there
+// mom (optional, DEVICE zone x 10): the third-order form, LIMIT_ALL_SLOPES_P12:
+// U, V, B_zc and E_zc of ten modes, the WENO gradients left unflattened and
+// followed by the WENO-AO hybridisation of step 7.b, the positivity limiter
+// and the face-point B with the quadratic modes, the 30-unknown constrained
+// least squares, the zone E at the centroid with its own WENO-AO (step 9)
 static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t* ext = nullptr,
                   size_t ext_bytes = 0, const double* mom = nullptr) {
   int rc = p1_check(p);
@@ -1157,8 +1165,15 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
   P1_HIP(hipGetDevice(&dev));
   const size_t flag_bytes = ((size_t)P.NZ + 255) / 256 * 256;
   const size_t list_bytes = (size_t)wr.count() * sizeof(int64_t);
-  const size_t need = 5 * flag_bytes + 256 + (size_t)nblk_diag * 18 * sizeof(double) + 18 * sizeof(double) +
-                      list_bytes;
+  const Box ao = make_box(-1, N + 2, -1, N + 2, -1, NR + 2);
+  const size_t base_bytes = (5 * flag_bytes + 256 + (size_t)nblk_diag * 18 * sizeof(double) + 18 * sizeof(double) +
+                             list_bytes + 255) / 256 * 256;
+  const size_t need = base_bytes + (mom ? (size_t)ao.count() * kAoRec * sizeof(double) : 0);
+  if (mom && P.NZ >= ((int64_t)1 << 31)) {
+    cs::set_error("cs_limit_all_slopes_p12: the patch has %lld zones, more than int32 indices reach",
+                  (long long)P.NZ);
+    return CS_E_INVALID;
+  }
   std::unique_lock<std::mutex> lock(g_mu, std::defer_lock);
   uint8_t* base = nullptr;
   if (ext) {
@@ -1195,6 +1210,7 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
   double* red = part + (size_t)nblk_diag * 18;
   int64_t* list = (int64_t*)(red + 18);
   unsigned int* lcount = F.counter + 1;
+  double* aorec = (double*)(base + base_bytes);  // third order: the records of k_ao_e_pre
 
   constexpr int NEV = CS_P1_STAGES + 1;
   hipEvent_t ev[NEV];
@@ -1207,7 +1223,6 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
     return 0;
   };
   if ((rc = mark())) return rc;
-  const Box ao = make_box(-1, N + 2, -1, N + 2, -1, NR + 2);
   if (mom) k_zero_hi<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
   k_prim<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all);
   if ((rc = mark())) return rc;
@@ -1224,7 +1239,7 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
   WenoFields wv, we;
   wv.n = 9;
   wv.noflat = mom ? 0xffu : 0u;  // third order: V flattened after the hybridisation
-  we.noflat = 0;
+  we.noflat = mom ? 0x7u : 0u;  // third order: E flattened after its hybridisation
   for (int c = 0; c < 8; ++c) {
     wv.src[c] = p->V + P.NZ * c;
     wv.dst[c] = p->V + P.NZ * (c + 8);
@@ -1291,20 +1306,39 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
     }
     k_clear_ent<<<grid_of(wr.count(), 256), 256, 0, st>>>(P, wr, F.prev);
   }
-  if (mom) {  // third order: the face-point B (8.a-c) and the zone B (8.d); 9 is not part of this call
+  if ((rc = mark())) return rc;
+  if (mom)
     k_face_pts<10><<<dim3(grid_of(std::max(fx.count(), std::max(fy.count(), fz.count())), 256), 3), 256, 0, st>>>(
         P, fx, fy, fz, mom);
+  else
+    k_face_pts<4><<<dim3(grid_of(std::max(fx.count(), std::max(fy.count(), fz.count())), 256), 3), 256, 0, st>>>(
+        P, fx, fy, fz, nullptr);
+  if ((rc = mark())) return rc;
+  if (mom) {
     k_clsq12<<<(unsigned)wr.count(), 64, 0, st>>>(P, wr, mom);
-    P1_HIP(hipGetLastError());
-    return CS_OK;
+    k_e12<<<grid_of(all.count(), 256), 256, 0, st>>>(P, all, mom);
+  } else {
+    k_clsq_e<<<grid_of(all.count(), 64), 64, 0, st>>>(P, all);
   }
   if ((rc = mark())) return rc;
-  k_face_pts<4><<<dim3(grid_of(std::max(fx.count(), std::max(fy.count(), fz.count())), 256), 3), 256, 0, st>>>(
-      P, fx, fy, fz, nullptr);
-  if ((rc = mark())) return rc;
-  k_clsq_e<<<grid_of(all.count(), 64), 64, 0, st>>>(P, all);
-  if ((rc = mark())) return rc;
   k_weno9<<<grid_of(wr.count(), 128), 128, 0, st>>>(P, wr, nullptr, 0, we);
+  if (mom) {
+    // 9.c in the loop order of the reference's serial build: level
+    // (i+1) + 3 (j+1) + 9 (k+1) of the zone box; k_ao_order flags a stencil
+    // table for which a zone's earlier neighbours are not all on lower
+    // levels and its later ones all on higher levels
+    P1_HIP(hipMemsetAsync(F.counter + 2, 0, sizeof(unsigned int), st));
+    k_ao_order<<<grid_of((int64_t)ao.ni * ao.nj, 256), 256, 0, st>>>(P, ao, F.counter + 2);
+    k_ao_e_pre<<<grid_of(ao.count(), 128), 128, 0, st>>>(P, ao, mom, aorec);
+    const int top = (ao.ni - 1) + 3 * (ao.nj - 1);
+    const int nsup = (top + kAoLag * (ao.nk - 1)) / kAoWin;
+    for (int sup = 0; sup <= nsup; ++sup) {
+      // planes whose window [kAoWin sup - kAoLag kr, + kAoWin) meets [0, top]
+      const int klo = std::max(0, (kAoWin * sup - top + kAoLag - 1) / kAoLag);
+      const int khi = std::min(ao.nk - 1, (kAoWin * sup + kAoWin - 1) / kAoLag);
+      if (khi >= klo) k_weno_ao_e_win<<<khi - klo + 1, 256, 0, st>>>(P, ao, mom, aorec, sup, klo);
+    }
+  }
   if ((rc = mark())) return rc;
   k_diag<<<nblk_diag, DIAG_THREADS, 0, st>>>(P, all, part);
   k_diag_final<<<18, 256, 0, st>>>(part, nblk_diag, red);
@@ -1314,6 +1348,16 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
     P1_HIP(hipEventSynchronize(ev[nev - 1]));
     for (int q = 0; q + 1 < nev; ++q) P1_HIP(hipEventElapsedTime(&kernel_ms[q], ev[q], ev[q + 1]));
     for (int q = 0; q < NEV; ++q) (void)hipEventDestroy(ev[q]);
+  }
+  if (mom) {
+    unsigned int bad = 0;
+    P1_HIP(hipMemcpyAsync(&bad, F.counter + 2, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
+    P1_HIP(hipStreamSynchronize(st));
+    if (bad) {
+      cs::set_error("cs_limit_all_slopes_p12: the stencil table reaches beyond two zones in i or j, so step 9.c "
+                    "cannot follow the serial loop order level by level");
+      return CS_E_INVALID;
+    }
   }
   if (p->V_min_local || p->V_max_local) {
     double h[18];
@@ -2939,10 +2983,18 @@ __global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const 
 // put one face point each (normal, the ten basis values about the zone's
 // moments, B-field weight, face-point B) in LDS; lane r < 36 forms row r of
 // the augmented system in registers (A^T A = P^T P + I + D^T D, P the point
-// rows, D the divergence rows); Gaussian elimination with scaled partial
-// pivoting across the lanes (the pivot row broadcast, no row exchange), then
-// back substitution in pivot order.  The reference solves the same system by
-// Numerical Recipes' LU (MATSOLVE): the two agree to rounding.
+// rows, D the divergence rows); Gaussian elimination across the lanes in
+// natural order, then back substitution.  The reference solves the same
+// system by Numerical Recipes' LU with partial pivoting (MATSOLVE): the two
+// agree to rounding.
+// lane l's value of v, uniform across the wave (l uniform)
+__device__ __forceinline__ double rdlane(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const unsigned int lo = (unsigned int)__builtin_amdgcn_readlane((int)(unsigned int)b, l);
+  const unsigned int hi = (unsigned int)__builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __constant__ double c_div_rows[4][30] = {
     {0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0},
     {0, 0, 0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1},
@@ -3037,45 +3089,28 @@ __global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const do
       row[36] += w * pt[q][14];
     }
   }
-  // elimination with scaled partial pivoting; lane piv[jj] holds pivot row jj
-  double big = 0.0;
-#pragma unroll
-  for (int e = 0; e < 36; ++e) big = fmax(big, fabs(row[e]));
-  const double vv = lane < 36 && big > 0.0 ? 1.0 / big : 0.0;
-  bool done = lane >= 36;
-  int piv[36];
+  // Elimination in natural order, lane jj holding pivot row jj: the leading
+  // 30 x 30 block is I + D^T D + P^T P, symmetric positive definite with
+  // every eigenvalue at least 1, so its pivots are too; the trailing 6 x 6
+  // block then holds -C K^-1 C^T, negative definite for the six independent
+  // face-mean rows C.  No pivot search, and the pivot row comes from a known
+  // lane, broadcast through scalar registers.
 #pragma unroll
   for (int jj = 0; jj < 36; ++jj) {
-    double best = done ? -1.0 : vv * fabs(row[jj]);
-    int bl = lane;
+    const double d = rdlane(row[jj], jj);
+    const double fct = row[jj] / d;
+    if (lane > jj) {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const double ob = __shfl_xor(best, off);
-      const int ol = __shfl_xor(bl, off);
-      if (ob > best || (ob == best && ol < bl)) {
-        best = ob;
-        bl = ol;
-      }
-    }
-    const int p = __builtin_amdgcn_readfirstlane(bl);
-    piv[jj] = p;
-    double pr[37];
-#pragma unroll
-    for (int e = jj; e < 37; ++e) pr[e] = __shfl(row[e], p);
-    if (lane == p) done = true;
-    if (!done) {
-      const double fct = row[jj] / pr[jj];
-#pragma unroll
-      for (int e = jj + 1; e < 37; ++e) row[e] -= fct * pr[e];
+      for (int e = jj + 1; e < 37; ++e) row[e] -= fct * rdlane(row[e], jj);
     }
   }
+  // back substitution by columns: x[jj] from lane jj, then every lane above
+  // takes it out of its right-hand side
   double x[36];
 #pragma unroll
   for (int jj = 35; jj >= 0; --jj) {
-    double t = row[36];
-#pragma unroll
-    for (int e = jj + 1; e < 36; ++e) t -= row[e] * x[e];
-    x[jj] = __shfl(t / row[jj], piv[jj]);
+    x[jj] = rdlane(row[36] / row[jj], jj);
+    if (lane < jj) row[36] -= row[jj] * x[jj];
   }
   if (lane < 30) {
     const int c = lane / 10, m = lane % 10;
@@ -3098,11 +3133,283 @@ __global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const do
   }
 }
 
+// Step 9.a at third order (421...f:6684-6795): E = -v x B at the zone's
+// centroid, v and B the quadratic fields there (the mean minus the quadratic
+// modes times the moments), every zone.
+__global__ void __launch_bounds__(256) k_e12(const P1 P, const Box b, const double* __restrict__ mom) {
+  int i, j, k;
+  if (!b.decode(tid(), i, j, k)) return;
+  const int64_t z = P.zi(i, j, k), NZ = P.NZ;
+  double q[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) q[m] = mom[z + NZ * (4 + m)];
+  auto at = [&](int c) {
+    double v = P.V(c, 0, z);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) v = v - P.V(c, 4 + m, z) * q[m];
+    return v;
+  };
+  const double Bx = at(5), By = at(6), Bz = at(7), vx = at(1), vy = at(2), vz = at(3);
+  const cs_p1_patch& a = P.a;
+  a.Ex_zc[z] = -(vy * Bz - vz * By);
+  a.Ey_zc[z] = -(vz * Bx - vx * Bz);
+  a.Ez_zc[z] = -(vx * By - vy * Bx);
+}
+
+// The reference's step 9.c overwrites each zone's point value of E
+// (421...f:9030) while later zones of the same loop read it as a stencil
+// value (:8899-8935): its serial build gives a zone the new point values of
+// the active box zones before it in (k, j, i) order and the old ones of those
+// after it.  Level (i+1) + 3 (j+1) + 9 (k+1) reproduces that order when every
+// stencil entry lies within two zones in i and j: a neighbour in the plane
+// differs by at most 2 in i, so a row step of 3 outweighs it, and a neighbour
+// above or below by at most 2 + 3 * 2 = 8 within the plane, which a plane step
+// of 9 outweighs.  One thread per (i, j) of the box sets *bad when a pair of
+// zones of the box breaks the order.
+__global__ void __launch_bounds__(256) k_ao_order(const P1 P, const Box b, unsigned int* bad) {
+  const int64_t t = tid();
+  if (t >= (int64_t)b.ni * b.nj) return;
+  const int i = b.i0 + (int)(t % b.ni), j = b.j0 + (int)(t / b.ni);
+  if (!P.active(i, j, 1)) return;
+  int ni[8], nj[8];
+  {
+    const int dirs[8] = {0, 1, 2, 3, 8, 9, 10, 11};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) P.nbr(i, j, dirs[q], ni[q], nj[q]);
+  }
+  bool ok = true;
+#pragma unroll
+  for (int r = 0; r < 18; ++r) {
+    const int dir = c_ao_dir[r], dk = c_ao_dk[r];
+    int ii = i, jj = j;
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (q == dir) {
+        ii = ni[q];
+        jj = nj[q];
+      }
+    if (ii < b.i0 || ii >= b.i0 + b.ni || jj < b.j0 || jj >= b.j0 + b.nj || !P.active(ii, jj, 1)) continue;
+    const int di = ii - i, dj = jj - j;
+    const bool earlier = dk < 0 || (dk == 0 && (dj < 0 || (dj == 0 && di < 0)));
+    const int dl = di + 3 * dj + 9 * dk;
+    if (dk == 0 && dj == 0 && di == 0) continue;
+    if (earlier ? dl >= 0 : dl <= 0) ok = false;
+  }
+  if (!ok) atomicOr(bad, 1u);
+}
+
+// Step 9.c's fit (421...f:8406-8936) depends on the stencil's point values
+// of E only through its right-hand side: the ten modes are X = G R, R the 19
+// point values, G = (A^T A)^-1 A^T, A the 19 x 10 matrix of
+// INTERPOLATION_COEFS (252interpolation_coefs.f:5: this zone's basis at each
+// stencil zone's centroid, row 0 the zone itself).  k_ao_e_pre forms G for
+// every active zone of the box, one thread per zone, with a Cholesky factor
+// of A^T A, and stores it with the 18 stencil zones' indices as one record of
+// kAoRec doubles per zone (G row-major by stencil row, then the indices as
+// int32), so that the ordered sweep reads one contiguous record per zone.
+
+__global__ void __launch_bounds__(128) k_ao_e_pre(const P1 P, const Box b, const double* __restrict__ mom,
+                                                  double* __restrict__ rec) {
+  const int64_t t = tid();
+  int i, j, k;
+  if (!b.decode(t, i, j, k)) return;
+  if (!P.active(i, j, 1)) return;
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k), NZ = P.NZ;
+  const double obl0 = 1.0 / fmax(a.scale[z], a.smallnum);
+  double c0[6], cz[3];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) c0[m] = mom[z + NZ * (4 + m)];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) cz[c] = a.v_centroid[z + NZ * c];
+  int ni[8], nj[8];
+  {
+    const int dirs[8] = {0, 1, 2, 3, 8, 9, 10, 11};
+#pragma unroll
+    for (int q = 0; q < 8; ++q) P.nbr(i, j, dirs[q], ni[q], nj[q]);
+  }
+  double* G = rec + t * kAoRec;
+  int* idx = (int*)(G + 190);
+  // row r of A: r = 0 the zone itself, r = 1..18 the stencil zones
+  auto arow = [&](int r, double* row) {
+    if (r == 0) {
+      row[0] = 1.0;
+      row[1] = row[2] = row[3] = 0.0;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) row[4 + m] = -c0[m];
+      return;
+    }
+    const int dir = c_ao_dir[r - 1];
+    int ii = i, jj = j;
+    if (dir >= 0) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (q == dir) {
+          ii = ni[q];
+          jj = nj[q];
+        }
+    }
+    const int64_t zr = P.zi(ii, jj, k + c_ao_dk[r - 1]);
+    double x[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) x[c] = (a.v_centroid[zr + NZ * c] - cz[c]) * obl0;
+    row[0] = 1.0;
+    row[1] = x[0];
+    row[2] = x[1];
+    row[3] = x[2];
+    row[4] = x[0] * x[0] - c0[0];
+    row[5] = x[1] * x[1] - c0[1];
+    row[6] = x[2] * x[2] - c0[2];
+    row[7] = x[0] * x[1] - c0[3];
+    row[8] = x[1] * x[2] - c0[4];
+    row[9] = x[0] * x[2] - c0[5];
+    idx[r - 1] = (int)zr;
+  };
+  double L[55];
+#pragma unroll
+  for (int e = 0; e < 55; ++e) L[e] = 0.0;
+#pragma unroll 1
+  for (int r = 0; r < 19; ++r) {
+    double row[10];
+    arow(r, row);
+#pragma unroll
+    for (int p = 0; p < 10; ++p)
+#pragma unroll
+      for (int q = 0; q <= p; ++q) L[p * (p + 1) / 2 + q] += row[p] * row[q];
+  }
+  double od[10];
+#pragma unroll
+  for (int p = 0; p < 10; ++p) {  // Cholesky factor in place, reciprocal diagonal in od
+    double d = L[p * (p + 1) / 2 + p];
+#pragma unroll
+    for (int q = 0; q < p; ++q) d -= L[p * (p + 1) / 2 + q] * L[p * (p + 1) / 2 + q];
+    d = sqrt(fmax(d, 1.0e-300));
+    L[p * (p + 1) / 2 + p] = d;
+    od[p] = 1.0 / d;
+#pragma unroll
+    for (int r = p + 1; r < 10; ++r) {
+      double t2 = L[r * (r + 1) / 2 + p];
+#pragma unroll
+      for (int q = 0; q < p; ++q) t2 -= L[r * (r + 1) / 2 + q] * L[p * (p + 1) / 2 + q];
+      L[r * (r + 1) / 2 + p] = t2 * od[p];
+    }
+  }
+#pragma unroll 1
+  for (int r = 0; r < 19; ++r) {  // column r of G: (A^T A)^-1 times row r
+    double X[10];
+    arow(r, X);
+#pragma unroll
+    for (int p = 0; p < 10; ++p) {
+      double t2 = X[p];
+#pragma unroll
+      for (int q = 0; q < p; ++q) t2 -= L[p * (p + 1) / 2 + q] * X[q];
+      X[p] = t2 * od[p];
+    }
+#pragma unroll
+    for (int p = 9; p >= 0; --p) {
+      double t2 = X[p];
+#pragma unroll
+      for (int q = p + 1; q < 10; ++q) t2 -= L[q * (q + 1) / 2 + p] * X[q];
+      X[p] = t2 * od[p];
+    }
+#pragma unroll
+    for (int m = 0; m < 10; ++m) G[10 * r + m] = X[m];
+  }
+}
+
+// Step 9.c at third order (421...f:8406-9353) for component c of E at box
+// zone (ir, jr, kr): the ten modes of the quadratic interpolant X = G R from
+// the zone's record (k_ao_e_pre) and the point values of E at the zone and
+// its 18 stencil zones; the hybridisation with the WENO gradients of 9.b as
+// in 7.b, then the point value E_pt f + E (1 - f) and the modes times (1 - f).
+__device__ __forceinline__ void ao_e_zone(const P1& P, const Box& b, const double* __restrict__ mom,
+                                          const double* __restrict__ rec, int ir, int jr, int kr, int c) {
+  const int i = b.i0 + ir, j = b.j0 + jr, k = b.k0 + kr;
+  if (!P.active(i, j, 1)) return;
+  const cs_p1_patch& a = P.a;
+  const int64_t z = P.zi(i, j, k), NZ = P.NZ;
+  const double* G = rec + ((int64_t)ir + (int64_t)b.ni * (jr + (int64_t)b.nj * kr)) * kAoRec;
+  const int* idx = (const int*)(G + 190);
+  double* E = c == 0 ? a.Ex_zc : (c == 1 ? a.Ey_zc : a.Ez_zc);
+  double X[10];
+  const double Ept = E[z];
+#pragma unroll
+  for (int m = 0; m < 10; ++m) X[m] = G[m] * Ept;
+#pragma unroll
+  for (int r = 1; r < 19; ++r) {
+    const double R = E[idx[r - 1]];
+#pragma unroll
+    for (int m = 0; m < 10; ++m) X[m] += G[10 * r + m] * R;
+  }
+  double c0[6];
+#pragma unroll
+  for (int m = 0; m < 6; ++m) c0[m] = mom[z + NZ * (4 + m)];
+  const double sm = a.smallnum, f = a.flatten[z];
+  const double betalo = f + (1.0 - f) * 2.0;
+  const double gammahi = f * 0.6 + (1.0 - f) * 0.95, gammalo = 1.0 - gammahi;
+  const double obgammahi = 1.0 / gammahi, gammaratio = gammalo / gammahi;
+  const double ob12 = 1.0 / 12.0, ob180 = 1.0 / 180.0, ob144 = 1.0 / 144.0;
+  const double m5 = c0[0] - ob12, m6 = c0[1] - ob12, m7 = c0[2] - ob12;
+  double e[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) e[m] = E[z + NZ * m];
+  const double ISlo = betalo * (e[1] * e[1] + e[2] * e[2] + e[3] * e[3]) * ob12;
+  double Phi[10];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) Phi[m] = X[m] * obgammahi - e[m] * gammaratio;
+#pragma unroll
+  for (int m = 4; m < 10; ++m) Phi[m] = X[m] * obgammahi;
+  const double IShi = (Phi[1] * Phi[1] + Phi[2] * Phi[2] + Phi[3] * Phi[3]) * ob12 +
+                      Phi[4] * Phi[4] * (m5 * m5 + ob180) + Phi[5] * Phi[5] * (m6 * m6 + ob180) +
+                      Phi[6] * Phi[6] * (m7 * m7 + ob180) + Phi[7] * Phi[7] * (c0[3] * c0[3] + ob144) +
+                      Phi[8] * Phi[8] * (c0[4] * c0[4] + ob144) + Phi[9] * Phi[9] * (c0[5] * c0[5] + ob144);
+  const double tau = fabs(IShi - ISlo);
+  double wlo = gammalo * (1.0 + tau / (ISlo + sm)), whi = gammahi * (1.0 + tau / (IShi + sm));
+  const double aa = 1.0 / (wlo + whi);
+  wlo *= aa;
+  whi *= aa;
+  const double x1 = 1.0 - f;
+  E[z] = Ept * f + (whi * Phi[0] + wlo * e[0]) * x1;
+#pragma unroll
+  for (int m = 1; m < 4; ++m) E[z + NZ * m] = (whi * Phi[m] + wlo * e[m]) * x1;
+#pragma unroll
+  for (int m = 4; m < 10; ++m) E[z + NZ * m] = whi * Phi[m] * x1;
+}
+
+// Step 9.c in the serial loop order (k_ao_order) over windows of in-plane
+// levels: in launch s, plane kr of the box takes its in-plane levels
+// l = (i+1) + 3 (j+1) from a = kAoWin s - kAoLag kr to a + kAoWin - 1 in turn,
+// one workgroup per plane (planes klo on), three lanes per zone (one per
+// component of E), a barrier between levels; the zones of a level read none
+// of each other.  A zone's neighbours in the plane below that come earlier in
+// the loop are on in-plane levels up to l+8 there (a plane step is 9 levels),
+// which that plane finished in earlier launches (its window leads by kAoLag =
+// kAoWin + 8 levels); the zones of the plane below that read this zone's old
+// value are on levels up to l+8 as well, done before this launch or outside
+// this launch's window there.  The launches order the planes, the barriers
+// the levels of one plane: no workgroup waits for another.
+__global__ void __launch_bounds__(256) k_weno_ao_e_win(const P1 P, const Box b, const double* __restrict__ mom,
+                                                       const double* __restrict__ rec, int s, int klo) {
+  const int kr = klo + blockIdx.x;
+  const int top = (b.ni - 1) + 3 * (b.nj - 1);
+  const int a = kAoWin * s - kAoLag * kr;
+  const int l0 = max(a, 0), l1 = min(a + kAoWin - 1, top);
+  for (int l = l0; l <= l1; ++l) {
+    const int jlo = max(0, (l - (b.ni - 1) + 2) / 3), jhi = min(b.nj - 1, l / 3);
+    const int nl = 3 * (jhi - jlo + 1);
+    for (int q = threadIdx.x; q < nl; q += blockDim.x) {
+      const int jr = jlo + q / 3;
+      ao_e_zone(P, b, mom, rec, l - 3 * jr, jr, kr, q % 3);
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace csp1
 
-extern "C" int cs_limit_slopes_p12_zones(const cs_p12_patch* q, void* stream_) {
+extern "C" int cs_limit_all_slopes_p12(const cs_p12_patch* q, void* stream_) {
   if (!q || !q->moments) {
-    cs::set_error("cs_limit_slopes_p12_zones: null patch or moments");
+    cs::set_error("cs_limit_all_slopes_p12: null patch or moments");
     return CS_E_INVALID;
   }
   return run_p1(&q->p1, stream_, nullptr, nullptr, 0, q->moments);

@@ -8,7 +8,7 @@
 !   cs_halo_exchange_p1    <- FETCH_BUFFERS_FILL_IN_* on six P1 patches' arrays (DRV:1746-1756)
 !   cs_sync_shared_p1      <- the defect correction over six P1 patches, before the update (DRV:3027-3461)
 !   cs_limit_all_slopes_p1 <- LIMIT_ALL_SLOPES_P1 alone, argument for argument (DRV:2396)
-!   cs_limit_slopes_p12_zones <- LIMIT_ALL_SLOPES_P12 steps 2-7 at NUMORDER 3 (DRV:2398)
+!   cs_limit_all_slopes_p12 <- LIMIT_ALL_SLOPES_P12 alone at NUMORDER 3, argument for argument (DRV:2398)
 !   cs_build_volume_p12    <- COORDS_TO_WT_QUADPTS_VOLUME_ISO2 + INIT_BASIS_FUNCTION_COEFS at NUMORDER 3
 !   cs_make_facial_fluxes_p1 <- MAKE_FACIAL_FLUXES alone, argument for argument (DRV:2485)
 !   cs_make_edge_efields   <- MAKE_EDGE_EFIELDS_P1                          (DRV:2548)
@@ -57,8 +57,8 @@ module cs_hip_iface
     type(c_ptr) :: Bxf_q, Byf_q, Bzf_q, V_min_local, V_max_local
   end type
 
-  ! cs_p12_patch: a cs_p1_patch whose U, V and B1..3_zc hold ten modes (NUMORDER
-  ! 3), plus the device address of Coef_Basis_Moments (zone, 10)
+  ! cs_p12_patch: a cs_p1_patch whose U, V, B1..3_zc and Ex..Ez_zc hold ten
+  ! modes (NUMORDER 3), plus the device address of Coef_Basis_Moments (zone, 10)
   type, bind(C) :: cs_p12_patch
     type(cs_p1_patch) :: p1
     type(c_ptr) :: moments
@@ -208,7 +208,7 @@ module cs_hip_iface
       type(cs_p1_update), intent(in) :: p
       type(c_ptr), value :: stream
     end function
-    integer(c_int) function cs_limit_slopes_p12_zones(p, stream) bind(C, name="cs_limit_slopes_p12_zones")
+    integer(c_int) function cs_limit_all_slopes_p12(p, stream) bind(C, name="cs_limit_all_slopes_p12")
       import :: c_int, c_ptr, cs_p12_patch
       type(cs_p12_patch), intent(in) :: p
       type(c_ptr), value :: stream

@@ -441,37 +441,43 @@ typedef struct cs_volume_p12 {
   double *v_pts, *v_wts, *v_vol, *v_cen, *moments;
 } cs_volume_p12;
 int cs_build_volume_p12(const cs_config* cfg, const cs_volume_p12* out);
-/* Steps 2 to 8.d of LIMIT_ALL_SLOPES_P12 (400riemann_integrate/
- * 421limit_all_slopes_p12.f:9, called from the driver's stage loop at
- * NUMORDER 3, DRV:2395-2398) on one patch: the steps of LIMIT_ALL_SLOPES_P1
- * up to its WENO (the P1 gradients of the eight primitives left unflattened,
- * 421…:3435-3448); step 7.b (:3811-4557): on the active zones of -1 .. N+2
- * (k -1 .. NR+2) the nine modes of the least-squares quadratic fit to the
- * 18-zone central stencil (RECONSTRUCTION_COEFS, 251reconstruction_coefs.f:3),
- * non-linearly hybridised with the P1 gradients (WENO-AO), then times
- * (1 - flattener); step 7.c (:4558-4568): ENFORCE_PRESSURE_POSITIVITY_PRIM
- * (440enforce_pressure_positivity_prim.f:6) with the quadratic modes in the
- * node values (:594-600), reducing all ten modes, with the loop-order replay
- * of cs_limit_all_slopes_p1; steps 8.a-c (:4668-5573): the normal B at the
- * five B-field points of each face from both zones' quadratic fields about
- * their own moments (:4817-4879), shifted to the face's mean B; step 8.d
- * (:5574-6680): the zone's 30 quadratic B coefficients from the
- * constrained least squares (the six face means exactly; n . B at the 30
- * face points, the WENO-AO modes and a zero divergence and divergence
- * gradient in the least-squares sense), U's B means and energy updated.
- * The patch is a cs_p1_patch whose U, V (zone,
- * 8, 10) and B1_zc, B2_zc, B3_zc (zone, 10) hold ten modes and whose
- * v_centroid comes from the 27-point rule, plus the ten basis moments
- * (DEVICE, zone x 10).  Outputs: V, the entropy, B1..3_zc, Bxf_q, Byf_q,
- * Bzf_q, the flattener and the work arrays of steps 2-6.  The zone E and
- * its WENO-AO (step 9) are not part of this call.
- * Work memory and stream as cs_limit_all_slopes_p1 (the call waits on the
- * stream once mid-way). */
+/* LIMIT_ALL_SLOPES_P12 (400riemann_integrate/421limit_all_slopes_p12.f:9,
+ * called from the driver's stage loop at NUMORDER 3, DRV:2395-2398) on one
+ * patch, the third-order WENO-AO reconstruction: the steps of
+ * LIMIT_ALL_SLOPES_P1 up to its WENO (the P1 gradients of the eight
+ * primitives left unflattened, 421…:3435-3448); step 7.b (:3811-4557): on
+ * the active zones of -1 .. N+2 (k -1 .. NR+2) the nine modes of the
+ * least-squares quadratic fit to the 18-zone central stencil
+ * (RECONSTRUCTION_COEFS, 251reconstruction_coefs.f:3), non-linearly
+ * hybridised with the P1 gradients, then times (1 - flattener); step 7.c:
+ * ENFORCE_PRESSURE_POSITIVITY_PRIM (440enforce_pressure_positivity_prim.f:6)
+ * with the quadratic modes in the node values (:594-600), reducing all ten
+ * modes, with the loop-order replay of cs_limit_all_slopes_p1; steps 8.a-c:
+ * the normal B at the five B-field points of each face from both zones'
+ * quadratic fields about their own moments (:4817-4879), shifted to the
+ * face's mean B; step 8.d (:5574-6680): the zone's 30 quadratic B
+ * coefficients from the constrained least squares (the six face means
+ * exactly; n . B at the 30 face points, the WENO-AO modes and a zero
+ * divergence and divergence gradient in the least-squares sense), U's B
+ * means and energy updated; step 9 (:6681-9353): E = -v x B at the centroid
+ * from the quadratic fields, its P1 WENO gradients, and the WENO-AO with
+ * the quadratic interpolant through the 19 zones' point values
+ * (INTERPOLATION_COEFS, 252interpolation_coefs.f:5), in the (k, j, i) loop
+ * order of the reference's serial build, whose loop overwrites each point
+ * value that later zones read as stencil values; step 10: the min/max and
+ * divergence diagnostics of cs_limit_all_slopes_p1.  The patch is a
+ * cs_p1_patch whose U, V (zone, 8, 10), B1..3_zc and Ex..Ez_zc (zone, 10)
+ * hold ten modes and whose v_centroid comes from the 27-point rule, plus
+ * the ten basis moments (DEVICE, zone x 10).  Work memory, stream and the
+ * host min/max as cs_limit_all_slopes_p1 (the call waits on the stream
+ * mid-way and at the end).  CS_E_INVALID, with the outputs undefined, when a
+ * stencil-table entry reaches beyond two zones in i or j (the cubed-sphere
+ * table reaches two, at the corner strips). */
 typedef struct cs_p12_patch {
   cs_p1_patch p1;
   const double* moments; /* Coef_Basis_Moments (zone, 10) */
 } cs_p12_patch;
-int cs_limit_slopes_p12_zones(const cs_p12_patch* p, void* stream);
+int cs_limit_all_slopes_p12(const cs_p12_patch* p, void* stream);
 
 /* ---- P1 facial fluxes of one patch -----------------------------------------
  * MAKE_FACIAL_FLUXES (400riemann_integrate/450make_facial_fluxes.f:5-60) at

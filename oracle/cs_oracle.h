@@ -259,13 +259,15 @@ int or_init_patch_cube_sphere(const or_p1_init* p, const or_prob* pb, int32_t la
 
 /* third order (cs_oracle_p1.c): MATINV (121matinv.f:8) of a row-major
  * n x n matrix (n <= 18; m destroyed), RECONSTRUCTION_COEFS (251...f:3) at
- * order 3, step 7.b of LIMIT_ALL_SLOPES_P12 (421...f:3811-4557) and steps 2
- * to 7.c of it (7.c: the positivity limiter with the quadratic modes); U and V of the patch hold ten modes, mom the ten basis
+ * order 3, step 7.b of LIMIT_ALL_SLOPES_P12 (421...f:3811-4557) and the
+ * whole routine (steps 2 to 10); U and V of the patch hold ten modes, mom the ten basis
  * moments of each zone (or_basis_moments10) */
 int or_matinv(int n, double* m, double* minv);
 void or_reconstruction_row3(double x, double y, double z, double l0, double lj, const double* c0, const double* cj,
                             double* row);
 int or_weno_ao_p12(const or_p1_patch* p, const double* mom);
-int or_limit_slopes_p12_zones(const or_p1_patch* p, const double* mom);
+int or_limit_all_slopes_p12(const or_p1_patch* p, const double* mom);
 int or_ao_fit_p12(const or_p1_patch* p, const double* mom, int i, int j, int k, int c, double* X);
+/* INTERPOLATION_COEFS (252interpolation_coefs.f:5) at order 3 */
+void or_interpolation_row3(double x, double y, double z, const double* c0, double* row);
 #endif

@@ -660,6 +660,53 @@ static void p1_face_points(const or_p1_patch* p, const p1dims* d, int nm, const 
 
 }
 
+/* STEP 10 of LIMIT_ALL_SLOPES_P1 and _P12 (410...f:7101-7700,
+ * 421...f:9354-): the same in both routines */
+static void p1_step10(const or_p1_patch* p, const p1dims* d) {
+  const int N = d->N, NR = d->NR;
+  const double pi = 4.0 * atan(1.0), ob4pi = 1.0 / (4.0 * pi);
+  int i, j, k, c, q;
+  /* min and max of the primitives over the owned zones, the face-B
+     divergence (kept in work) normalised by |B| and the magnetosonic speed
+     (recomputed from the new primitives) */
+  double vmin[9], vmax[9];
+  for (q = 0; q < 9; ++q) {
+    vmin[q] = 1.0e30;
+    vmax[q] = -1.0e30;
+  }
+  for (long z = 0; z < d->NZ; ++z) p->work[z] = 0.0;
+  for (k = 1; k <= NR; ++k)
+    for (j = 1; j <= N; ++j)
+      for (i = 1; i <= N; ++i) {
+        long z = zi(d, i, j, k);
+        if (p->mask[z] != 1) continue;
+        for (c = 0; c < 8; ++c) {
+          vmin[c] = omin(vmin[c], VV(p, d, c, 0, z));
+          vmax[c] = omax(vmax[c], VV(p, d, c, 0, z));
+        }
+        long xp = xfi(d, i, j, k), xm = xfi(d, i - 1, j, k), yp = yfi(d, i, j, k), ym = yfi(d, i, j - 1, k),
+             zp = zfi(d, i, j, k), zm = zfi(d, i, j, k - 1);
+        double bxp = p->Bxf[fci(d, i, j, k)], bxm = p->Bxf[fci(d, i - 1, j, k)], byp = p->Byf[fci(d, i, j, k)],
+               bym = p->Byf[fci(d, i, j - 1, k)], bzp = p->Bzf[fci(d, i, j, k)], bzm = p->Bzf[fci(d, i, j, k - 1)];
+        double a21 = p->xf_area[xp] * bxp + p->yf_area[yp] * byp + p->zf_area[zp] * bzp - p->xf_area[xm] * bxm -
+                     p->yf_area[ym] * bym - p->zf_area[zm] * bzm;
+        p->work[z] = a21;
+        double v6 = VV(p, d, 5, 0, z), v7 = VV(p, d, 6, 0, z), v8 = VV(p, d, 7, 0, z);
+        double a11 = (v6 + 0.0) * (v6 + 0.0) + (v7 + 0.0) * (v7 + 0.0) + (v8 + 0.0) * (v8 + 0.0);
+        double ms = p->gamma_sound[z] * VV(p, d, 4, 0, z) + ob4pi * a11;
+        p->msonic[z] = sqrt(ms / VV(p, d, 0, 0, z));
+        double den = omax(omax(sqrt(bxp * bxp + byp * byp + bzp * bzp), sqrt(bxm * bxm + bym * bym + bzm * bzm)),
+                          p->msonic[z] * sqrt(4.0 * 3.14 * VV(p, d, 0, 0, z)));
+        a21 = a21 / den;
+        vmin[8] = omin(vmin[8], a21);
+        vmax[8] = omax(vmax[8], a21);
+      }
+  if (p->V_min_local)
+    for (q = 0; q < 9; ++q) p->V_min_local[q] = vmin[q];
+  if (p->V_max_local)
+    for (q = 0; q < 9; ++q) p->V_max_local[q] = vmax[q];
+}
+
 int or_limit_all_slopes_p1(const or_p1_patch* p) {
   if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1) return 1;
   const p1dims D = p1_dims(p), *d = &D;
@@ -773,45 +820,7 @@ int or_limit_all_slopes_p1(const or_p1_patch* p) {
         }
       }
 
-  /* STEP 10 (:7101-7700): min and max of the primitives over the owned zones,
-     the face-B divergence (kept in work) normalised by |B| and the
-     magnetosonic speed (recomputed from the new primitives) */
-  double vmin[9], vmax[9];
-  for (q = 0; q < 9; ++q) {
-    vmin[q] = 1.0e30;
-    vmax[q] = -1.0e30;
-  }
-  for (long z = 0; z < d->NZ; ++z) p->work[z] = 0.0;
-  for (k = 1; k <= NR; ++k)
-    for (j = 1; j <= N; ++j)
-      for (i = 1; i <= N; ++i) {
-        long z = zi(d, i, j, k);
-        if (p->mask[z] != 1) continue;
-        for (c = 0; c < 8; ++c) {
-          vmin[c] = omin(vmin[c], VV(p, d, c, 0, z));
-          vmax[c] = omax(vmax[c], VV(p, d, c, 0, z));
-        }
-        long xp = xfi(d, i, j, k), xm = xfi(d, i - 1, j, k), yp = yfi(d, i, j, k), ym = yfi(d, i, j - 1, k),
-             zp = zfi(d, i, j, k), zm = zfi(d, i, j, k - 1);
-        double bxp = p->Bxf[fci(d, i, j, k)], bxm = p->Bxf[fci(d, i - 1, j, k)], byp = p->Byf[fci(d, i, j, k)],
-               bym = p->Byf[fci(d, i, j - 1, k)], bzp = p->Bzf[fci(d, i, j, k)], bzm = p->Bzf[fci(d, i, j, k - 1)];
-        double a21 = p->xf_area[xp] * bxp + p->yf_area[yp] * byp + p->zf_area[zp] * bzp - p->xf_area[xm] * bxm -
-                     p->yf_area[ym] * bym - p->zf_area[zm] * bzm;
-        p->work[z] = a21;
-        double v6 = VV(p, d, 5, 0, z), v7 = VV(p, d, 6, 0, z), v8 = VV(p, d, 7, 0, z);
-        double a11 = (v6 + 0.0) * (v6 + 0.0) + (v7 + 0.0) * (v7 + 0.0) + (v8 + 0.0) * (v8 + 0.0);
-        double ms = p->gamma_sound[z] * VV(p, d, 4, 0, z) + ob4pi * a11;
-        p->msonic[z] = sqrt(ms / VV(p, d, 0, 0, z));
-        double den = omax(omax(sqrt(bxp * bxp + byp * byp + bzp * bzp), sqrt(bxm * bxm + bym * bym + bzm * bzm)),
-                          p->msonic[z] * sqrt(4.0 * 3.14 * VV(p, d, 0, 0, z)));
-        a21 = a21 / den;
-        vmin[8] = omin(vmin[8], a21);
-        vmax[8] = omax(vmax[8], a21);
-      }
-  if (p->V_min_local)
-    for (q = 0; q < 9; ++q) p->V_min_local[q] = vmin[q];
-  if (p->V_max_local)
-    for (q = 0; q < 9; ++q) p->V_max_local[q] = vmax[q];
+  p1_step10(p, d);
   return 0;
 }
 
@@ -2371,11 +2380,141 @@ static void p12_clsq(const or_p1_patch* p, const p1dims* d, const double* mom) {
       }
 }
 
-/* LIMIT_ALL_SLOPES_P12 up to its zone B: steps 2 to 7.a with ten modes and
+/* INTERPOLATION_COEFS (200mesh/252interpolation_coefs.f:5) at order 3: the
+ * ten basis functions of a zone with moments c0 at the scaled point
+ * (x, y, z) */
+void or_interpolation_row3(double x, double y, double z, const double* c0, double* row) {
+  row[0] = 1.0;
+  row[1] = x;
+  row[2] = y;
+  row[3] = z;
+  row[4] = x * x - c0[4];
+  row[5] = y * y - c0[5];
+  row[6] = z * z - c0[6];
+  row[7] = x * y - c0[7];
+  row[8] = y * z - c0[8];
+  row[9] = x * z - c0[9];
+}
+
+/* STEP 9 of LIMIT_ALL_SLOPES_P12 (421...f:6681-9353): the zone electric
+ * field.  9.a: E = -v x B at the centroid, v and B the quadratic fields there
+ * (the mean minus the quadratic modes times the moments), every zone; 9.b:
+ * the P1 WENO gradients of E, unflattened (:6796-8405); 9.c (:8406-9353): on
+ * the active zones of -1 .. N+2 the ten modes of the least-squares quadratic
+ * interpolant (INTERPOLATION_COEFS) through the zone's and its 18
+ * neighbours' point values, hybridised with the WENO gradients as in 7.b,
+ * then E = E_pt f + E (1 - f) for the point value and the modes times
+ * (1 - f). */
+static void p12_efield(const or_p1_patch* p, const p1dims* d, const double* mom) {
+  const int N = d->N, NR = d->NR;
+  const long S = d->NZ;
+  double* E[3] = {p->Ex_zc, p->Ey_zc, p->Ez_zc};
+  for (long z = 0; z < S; ++z) {
+    double q[7][1];
+    int cs_[6] = {5, 6, 7, 1, 2, 3};
+    double val[6];
+    for (int t = 0; t < 6; ++t) {
+      const int c = cs_[t];
+      double v = (t < 3 ? 0.0 + VV(p, d, c, 0, z) : VV(p, d, c, 0, z));
+      for (int m = 4; m < 10; ++m) v = v - VV(p, d, c, m, z) * mom[z + S * m];
+      val[t] = v;
+    }
+    (void)q;
+    const double Bx = val[0], By = val[1], Bz = val[2], vx = val[3], vy = val[4], vz = val[5];
+    p->Ex_zc[z] = -(vy * Bz - vz * By);
+    p->Ey_zc[z] = -(vz * Bx - vx * Bz);
+    p->Ez_zc[z] = -(vx * By - vy * Bx);
+  }
+  for (int k = -2; k <= NR + 3; ++k)
+    for (int j = -2; j <= N + 3; ++j)
+      for (int i = -2; i <= N + 3; ++i) {
+        if (!active(p, d, i, j, 1)) continue;
+        long z = zi(d, i, j, k), z6[6];
+        weno_mats W;
+        six_nbrs(p, d, i, j, k, z6);
+        weno_build(p, d, z, z6, &W);
+        for (int c = 0; c < 3; ++c) {
+          double f6[6], g3[3];
+          for (int q = 0; q < 6; ++q) f6[q] = E[c][z6[q]];
+          weno_apply(&W, E[c][z], f6, p->smallnum, 0.0, g3);
+          for (int m = 0; m < 3; ++m) E[c][z + S * (m + 1)] = g3[m];
+        }
+      }
+  const double ob12 = 1.0 / 12.0, ob180 = 1.0 / 180.0, ob144 = 1.0 / 144.0, sm = p->smallnum;
+  for (int k = -1; k <= NR + 2; ++k)
+    for (int j = -1; j <= N + 2; ++j)
+      for (int i = -1; i <= N + 2; ++i) {
+        if (!active(p, d, i, j, 1)) continue;
+        const long z = zi(d, i, j, k);
+        const double ell0 = omax(p->scale[z], sm), obl0 = 1.0 / ell0;
+        double c0[10], A[19][10], ata[100], inv[100];
+        long z19[19];
+        for (int m = 0; m < 10; ++m) c0[m] = mom[z + S * m];
+        z19[0] = z;
+        or_interpolation_row3(0.0, 0.0, 0.0, c0, A[0]);
+        for (int r = 0; r < 18; ++r) {
+          int a = i, b = j;
+          if (kAoDir[r] >= 0) nbr(p, d, i, j, kAoDir[r], &a, &b);
+          const long zj = zi(d, a, b, k + kAoDk[r]);
+          z19[r + 1] = zj;
+          double x[3];
+          for (int c = 0; c < 3; ++c) x[c] = (p->v_centroid[zj + S * c] - p->v_centroid[z + S * c]) * obl0;
+          or_interpolation_row3(x[0], x[1], x[2], c0, A[r + 1]);
+        }
+        for (int a = 0; a < 10; ++a)
+          for (int b = 0; b < 10; ++b) {
+            double t = 0.0;
+            for (int r = 0; r < 19; ++r) t = t + A[r][a] * A[r][b];
+            ata[a * 10 + b] = t;
+          }
+        or_matinv(10, ata, inv);
+        const double f = p->flatten[z];
+        const double betalo = f + (1.0 - f) * 2.0;
+        const double gammahi = f * 0.6 + (1.0 - f) * 0.95, gammalo = 1.0 - gammahi;
+        const double obgammahi = 1.0 / gammahi, gammaratio = gammalo / gammahi;
+        for (int c = 0; c < 3; ++c) {
+          double R[19], Y[10], X[10], Phi[10], e[4];
+          const double Ept = E[c][z];
+          for (int r = 0; r < 19; ++r) R[r] = E[c][z19[r]];
+          for (int a = 0; a < 10; ++a) {
+            double t = 0.0;
+            for (int r = 0; r < 19; ++r) t = t + A[r][a] * R[r];
+            Y[a] = t;
+          }
+          for (int a = 0; a < 10; ++a) {
+            double t = 0.0;
+            for (int b = 0; b < 10; ++b) t = t + inv[a * 10 + b] * Y[b];
+            X[a] = t;
+          }
+          for (int m = 0; m < 4; ++m) e[m] = E[c][z + S * m];
+          const double ISlo = betalo * (e[1] * e[1] + e[2] * e[2] + e[3] * e[3]) * ob12;
+          for (int m = 0; m < 4; ++m) Phi[m] = X[m] * obgammahi - e[m] * gammaratio;
+          for (int m = 4; m < 10; ++m) Phi[m] = X[m] * obgammahi;
+          const double m5 = c0[4] - ob12, m6 = c0[5] - ob12, m7 = c0[6] - ob12;
+          const double IShi = (Phi[1] * Phi[1] + Phi[2] * Phi[2] + Phi[3] * Phi[3]) * ob12 +
+                              (Phi[4] * Phi[4]) * (m5 * m5 + ob180) + (Phi[5] * Phi[5]) * (m6 * m6 + ob180) +
+                              (Phi[6] * Phi[6]) * (m7 * m7 + ob180) + (Phi[7] * Phi[7]) * (c0[7] * c0[7] + ob144) +
+                              (Phi[8] * Phi[8]) * (c0[8] * c0[8] + ob144) + (Phi[9] * Phi[9]) * (c0[9] * c0[9] + ob144);
+          const double tau = fabs(IShi - ISlo);
+          double wlo = gammalo * (1.0 + tau / (ISlo + sm)), whi = gammahi * (1.0 + tau / (IShi + sm));
+          const double aa = 1.0 / (wlo + whi);
+          wlo = wlo * aa;
+          whi = whi * aa;
+          double out[10];
+          for (int m = 0; m < 4; ++m) out[m] = whi * Phi[m] + wlo * e[m];
+          for (int m = 4; m < 10; ++m) out[m] = whi * Phi[m];
+          const double x1 = 1.0 - f;
+          E[c][z] = Ept * f + out[0] * x1;
+          for (int m = 1; m < 10; ++m) E[c][z + S * m] = out[m] * x1;
+        }
+      }
+}
+
+/* LIMIT_ALL_SLOPES_P12, steps 2 to 10: 2 to 7.a with ten modes and
  * unflattened WENO gradients, 7.b, the positivity limiter of 7.c and the
- * face-point B of 8.a-c with the quadratic modes, and the constrained least
- * squares of 8.d */
-int or_limit_slopes_p12_zones(const or_p1_patch* p, const double* mom) {
+ * face-point B of 8.a-c with the quadratic modes, the constrained least
+ * squares of 8.d, the zone E of 9, and the diagnostics of 10 */
+int or_limit_all_slopes_p12(const or_p1_patch* p, const double* mom) {
   if (!p || p->nghost < 4 || p->ntp < 1 || p->nr < 1 || !mom) return 1;
   const p1dims D = p1_dims(p);
   p1_front(p, &D, 10, 0);
@@ -2384,5 +2523,7 @@ int or_limit_slopes_p12_zones(const or_p1_patch* p, const double* mom) {
   p1_positivity(p, &D, 10, mom);
   p1_face_points(p, &D, 10, mom);
   p12_clsq(p, &D, mom);
+  p12_efield(p, &D, mom);
+  p1_step10(p, &D);
   return 0;
 }

@@ -1214,7 +1214,7 @@ def init_problem_p1(cfg, problem, seed=0, time=0.0, layers=0, state=None, geo=No
 
 # ---------------------------------------------------------------------------
 # Third order (NUMORDER 3): the 27-point volume rule, ten basis moments and
-# the zone-centred part of LIMIT_ALL_SLOPES_P12 on one patch.
+# LIMIT_ALL_SLOPES_P12 on one patch.
 
 def volume_geometry3(cfg, geo, use_reference=False):
     """COORDS_TO_WT_QUADPTS_VOLUME_ISO2 at NUMORDER 3 (27 points) and
@@ -1286,11 +1286,11 @@ def p12_problem(N, NR, g=4, sector=0, seed=0, shock=False, rmin=1.0, rmax=1.6, g
 
 
 def p12_oracle(prob, A):
-    """or_limit_slopes_p12_zones in place on A (steps 2 to 7.c)."""
+    """or_limit_all_slopes_p12 in place on A (the whole routine)."""
     L = olib()
-    L.or_limit_slopes_p12_zones.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.or_limit_all_slopes_p12.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     s = p1_struct(prob, A)
-    if L.or_limit_slopes_p12_zones(ctypes.byref(s), P(A["moments"])) != 0:
+    if L.or_limit_all_slopes_p12(ctypes.byref(s), P(A["moments"])) != 0:
         raise RuntimeError("oracle rejected the patch")
     return A
 

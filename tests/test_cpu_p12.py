@@ -2,15 +2,15 @@
 on the CPU: the oracle's restatement against the reference's own routines
 built at NUMORDER 3 (oracle/_ref/libcsref3.so), and against closed forms.
 
-The oracle restates steps 2 to 8.d: the P1 WENO gradients (7.a) left
+The oracle restates the whole routine: the P1 WENO gradients (7.a) left
 unflattened, the WENO-AO hybridisation of 7.b with the 18-zone quadratic
 least-squares stencil (RECONSTRUCTION_COEFS, 251...f:3; MATINV,
 121matinv.f:8) and the flattener, the positivity limiter with the quadratic
 modes (7.c, 440...f:594-600), the face-point B of 8.a-c with them
-(421...f:4817-4879), and the 30-unknown constrained least squares of 8.d
-(CMat_Row_Coefs_P2, 253...f:7; MATSOLVE).  The reference runs its whole
-routine; what step 9 (the zone E) leaves alone is compared: V, U, B_zc, the
-face-point B and the entropy."""
+(421...f:4817-4879), the 30-unknown constrained least squares of 8.d
+(CMat_Row_Coefs_P2, 253...f:7; MATSOLVE), the zone E and its WENO-AO with
+the quadratic interpolant of 9 (INTERPOLATION_COEFS, 252...f:5), and the
+diagnostics of 10; every output array is compared."""
 import ctypes
 
 import numpy as np
@@ -107,22 +107,18 @@ def _hydro(A, nz):
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "N%d_nr%d_s%d_seed%d" % c)
 def test_p12_zones_bit_exact_with_reference_smooth(case):
-    """Smooth flow: the positivity limiter does not act, so the oracle's steps
-    2-7.b equal the reference's whole routine on the five hydrodynamic
-    primitives (all ten modes, every zone), the entropy and the flattener."""
+    """Smooth flow: the oracle's routine equals the reference's on every
+    output array."""
     _ref3_or_skip()
     N, NR, s, seed = case
     prob, A0 = orc.p12_problem(N, NR, 4, sector=s, seed=seed, shock=False)
     nz = orc.p1_dims(N, NR, 4)["zone"]
     A = orc.p12_oracle(prob, orc.p1_copy(A0))
     B = orc.p12_reference(prob, orc.p1_copy(A0))
-    assert np.array_equal(_hydro(A, nz), _hydro(B, nz))
-    assert np.array_equal(A["ent"], B["ent"])
-    assert np.array_equal(A["flatten"], B["flatten"])
-    # step 8.a-c: face-point B with the quadratic modes; 8.d: the zone B, U's
-    # B means and energy from the 30-unknown constrained least squares
-    for k in ("Bxf_q", "Byf_q", "Bzf_q", "V", "U", "B1_zc", "B2_zc", "B3_zc"):
-        assert np.array_equal(A[k], B[k]), k
+    # every output array of the routine: V, U, entropy, B_zc, E_zc, the
+    # face-point B, the flattener and work arrays, the min/max
+    bad = [k for k in A if not np.array_equal(A[k], B[k])]
+    assert not bad, bad
     # the quadratic modes are there (not all zero)
     assert np.abs(_hydro(A, nz)[4:]).max() > 1e-4
 
@@ -133,8 +129,7 @@ def test_p12_zones_with_flattener_and_positivity_bit_exact(case):
     """A compressive front (the flattener acts: the linear weights move to
     0.6 / 0.4 and every mode is scaled) and log-normal noise (the positivity
     limiter reduces slopes by 0.1^(n/15) or removes them, and its loop order
-    matters): bit-exact with the reference on the hydrodynamic primitives
-    and the entropy, every zone."""
+    matters): bit-exact with the reference on every output array."""
     _ref3_or_skip()
     sector, seed, noise = case
     N, NR = 8, 3
@@ -144,10 +139,8 @@ def test_p12_zones_with_flattener_and_positivity_bit_exact(case):
     st = np.zeros(2, np.int64)
     orc.olib().or_p1_positivity_stats(orc.P(st))
     B = orc.p12_reference(prob, orc.p1_copy(A0))
-    assert np.array_equal(_hydro(A, nz), _hydro(B, nz))
-    assert np.array_equal(A["ent"], B["ent"])
-    for k in ("Bxf_q", "Byf_q", "Bzf_q", "V", "U", "B1_zc", "B2_zc", "B3_zc"):
-        assert np.array_equal(A[k], B[k]), k
+    bad = [k for k in A if not np.array_equal(A[k], B[k])]
+    assert not bad, bad
     assert (A["flatten"] > 0).sum() > 100
     assert st[0] > 5, "the positivity limiter must act"
     if noise > 1.0:
@@ -228,3 +221,58 @@ def test_p12_linear_field_is_exact():
                 worst_g = max(worst_g, np.abs(V[1:4, 0, z] - want).max() / np.abs(l0 * gvec).max())
                 worst_q = max(worst_q, np.abs(V[4:, 0, z]).max() / np.abs(l0 * gvec).max())
     assert worst_g < 1e-10 and worst_q < 1e-10, (worst_g, worst_q)
+
+
+# zone offsets of the 18-zone stencil: direction in the stencil table (-1 for
+# the zone's own column) and the radial offset, as INTERPOLATION_COEFS' rows
+# are filled at 421limit_all_slopes_p12.f:8899-8935
+_AO_DIR = (0, 1, 2, 3, -1, -1, 8, 9, 10, 11, 0, 1, 0, 1, 2, 3, 2, 3)
+_AO_DK = (0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, 1, -1, -1, 1, 1, -1, -1)
+
+
+@pytest.mark.parametrize("N", [8, 12, 24, 192])
+def test_step9c_levels_follow_the_serial_loop_order(N):
+    """Step 9.c overwrites each zone's point value of E while later zones of
+    the loop read it; the GPU runs it level by level, level
+    (i+1) + 3 (j+1) + 9 (k+1) over the box -1..N+2.  On the stencil table of
+    the cubed-sphere patch (corner strips included) every active neighbour
+    that the serial (k, j, i) loop visits earlier has a lower level and every
+    later one a higher level, and no entry reaches beyond two zones in i or j
+    (the bound that the GPU's own check enforces); at the corner strips the
+    plainer level i + 2j + 4k breaks this order."""
+    g = 4
+    nz = N + 2 * g
+    st = np.zeros(nz * nz * 24, np.int32)
+    L = orc.olib()
+    L.or_stencil_indexing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p]
+    L.or_stencil_indexing(N, g, st.ctypes.data)
+    st = st.reshape(24, nz, nz)
+    prob, A = orc.p1_problem(N, 2, g, sector=0, seed=1)
+    mask = A["mask"].reshape(-1, nz, nz)[g]
+
+    def act(i, j):
+        return -1 <= i <= N + 2 and -1 <= j <= N + 2 and mask[j - 1 + g, i - 1 + g] == 1
+
+    def violations(wj, wk):
+        bad, reach = 0, 0
+        for j in range(-1, N + 3):
+            for i in range(-1, N + 3):
+                if not act(i, j):
+                    continue
+                for r in range(18):
+                    a, b = i, j
+                    if _AO_DIR[r] >= 0:
+                        a = int(st[2 * _AO_DIR[r], j - 1 + g, i - 1 + g])
+                        b = int(st[2 * _AO_DIR[r] + 1, j - 1 + g, i - 1 + g])
+                    if not act(a, b):
+                        continue
+                    dk = _AO_DK[r]
+                    reach = max(reach, abs(a - i), abs(b - j))
+                    earlier = (dk, b - j, a - i) < (0, 0, 0)
+                    dl = (a - i) + wj * (b - j) + wk * dk
+                    bad += int(dl >= 0 if earlier else dl <= 0)
+        return bad, reach
+
+    bad, reach = violations(3, 9)
+    assert bad == 0 and reach == 2
+    assert violations(2, 4)[0] > 0

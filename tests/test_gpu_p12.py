@@ -1,14 +1,14 @@
 """GPU parity of the third-order entry points with the oracle, which is
 bit-exact with the reference built at NUMORDER 3 (tests/test_cpu_p12.py):
 cs_build_volume_p12 (COORDS_TO_WT_QUADPTS_VOLUME_ISO2 with 27 points and ten
-basis moments) and cs_limit_slopes_p12_zones (steps 2 to 8.d of
-LIMIT_ALL_SLOPES_P12, 421limit_all_slopes_p12.f:9).
+basis moments) and cs_limit_all_slopes_p12 (LIMIT_ALL_SLOPES_P12,
+421limit_all_slopes_p12.f:9).
 
 Tolerance: the HIP path fuses multiply-adds and solves the 9 x 9 normal
 equations of the quadratic stencil by Cholesky where the reference inverts
 them by LU, and the 36 x 36 system of step 8.d by elimination across a
 wave's lanes where the reference uses LU, so the geometry agrees to 1e-13 and each (component, mode) row of
-V, B*_zc and the face-point B to 1e-9 of its largest magnitude; the flattener, the entropy and
+V, B*_zc, E*_zc and the face-point B to 1e-9 of its largest magnitude; the flattener, the entropy and
 the integer arrays to 1e-10 / exactly.  The positivity limiter's discrete
 decisions (how many reductions, removal) must be the same: a different
 decision would move a mode by 14 % or more, far outside the tolerance."""
@@ -48,9 +48,11 @@ def _run_gpu(prob, A):
     dev = torch.device("cuda:0")
     T = {k: torch.from_numpy(A[k]).to(dev) for k in cs.P1_ARRAYS}
     mom = torch.from_numpy(A["moments"]).to(dev)
-    cs.limit_slopes_p12_zones(prob["N"], prob["NR"], prob["g"], prob, T, mom)
+    vmin, vmax = cs.limit_all_slopes_p12(prob["N"], prob["NR"], prob["g"], prob, T, mom)
     torch.cuda.synchronize()
-    return {k: T[k].cpu().numpy() for k in cs.P1_ARRAYS}
+    out = {k: T[k].cpu().numpy() for k in cs.P1_ARRAYS}
+    out["V_min_local"], out["V_max_local"] = vmin, vmax
+    return out
 
 
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "N%d_nr%d_s%d_seed%d_%s_noise%g" % (
@@ -63,16 +65,15 @@ def test_p12_zones_match_oracle(case):
     G = _run_gpu(prob, orc.p1_copy(A0))
     nz = orc.p1_dims(N, NR, 4)["zone"]
     bad = []
-    for k in cs.P1_ARRAYS:
+    for k in cs.P1_ARRAYS + ("V_min_local", "V_max_local"):
         g, o = G[k], O[k]
         if o.dtype.kind != "f":
             if not np.array_equal(g, o):
                 bad.append((k, "int"))
             continue
-        if k in ("Ex_zc", "Ey_zc", "Ez_zc"):
-            continue  # step 9 is not part of the call: both keep their inputs
         rows = 80 if k in ("U", "V") else (4 if k == "ent" else (10 if k.endswith("_zc") else
-                                                                   (5 if k.endswith("_q") else 1)))
+                                                                   (5 if k.endswith("_q") else
+                                                                    (9 if k.startswith("V_m") else 1))))
         rtol = 1e-9 if (k == "V" or k.endswith("_zc") or k.endswith("_q")) else 1e-10
         for r, (gr, orow) in enumerate(zip(g.reshape(rows, -1), o.reshape(rows, -1))):
             scale = max(np.abs(orow).max(), 1e-300)
