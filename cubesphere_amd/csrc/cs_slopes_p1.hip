@@ -1121,11 +1121,10 @@ __global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const do
 __global__ void __launch_bounds__(256) k_e12(const P1 P, const Box b, const double* __restrict__ mom);
 __global__ void __launch_bounds__(256) k_ao_order(const P1 P, const Box b, unsigned int* bad);
 constexpr int kAoRec = 200;  // doubles per zone record of k_ao_e_pre: G (19 x 10), 18 int32 indices
-constexpr int kAoWin = 8, kAoLag = kAoWin + 8;  // levels per launch of k_weno_ao_e_win, and the plane lag
 __global__ void __launch_bounds__(128) k_ao_e_pre(const P1 P, const Box b, const double* __restrict__ mom,
                                                   double* __restrict__ rec);
-__global__ void __launch_bounds__(256) k_weno_ao_e_win(const P1 P, const Box b, const double* __restrict__ mom,
-                                                       const double* __restrict__ rec, int s, int klo);
+__global__ void __launch_bounds__(192) k_weno_ao_e(const P1 P, const Box b, const double* __restrict__ mom,
+                                                   const double* __restrict__ rec, int level, int klo);
 __global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom,
                                                  const int64_t* list, int64_t nlist);
 }  // namespace csp1
@@ -1330,13 +1329,11 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
     P1_HIP(hipMemsetAsync(F.counter + 2, 0, sizeof(unsigned int), st));
     k_ao_order<<<grid_of((int64_t)ao.ni * ao.nj, 256), 256, 0, st>>>(P, ao, F.counter + 2);
     k_ao_e_pre<<<grid_of(ao.count(), 128), 128, 0, st>>>(P, ao, mom, aorec);
-    const int top = (ao.ni - 1) + 3 * (ao.nj - 1);
-    const int nsup = (top + kAoLag * (ao.nk - 1)) / kAoWin;
-    for (int sup = 0; sup <= nsup; ++sup) {
-      // planes whose window [kAoWin sup - kAoLag kr, + kAoWin) meets [0, top]
-      const int klo = std::max(0, (kAoWin * sup - top + kAoLag - 1) / kAoLag);
-      const int khi = std::min(ao.nk - 1, (kAoWin * sup + kAoWin - 1) / kAoLag);
-      if (khi >= klo) k_weno_ao_e_win<<<khi - klo + 1, 256, 0, st>>>(P, ao, mom, aorec, sup, klo);
+    const int top = (ao.ni - 1) + 3 * (ao.nj - 1) + 9 * (ao.nk - 1);
+    for (int lv = 0; lv <= top; ++lv) {
+      const int klo = std::max(0, (lv - (ao.ni - 1) - 3 * (ao.nj - 1) + 8) / 9);
+      const int khi = std::min(ao.nk - 1, lv / 9);
+      k_weno_ao_e<<<grid_of(3 * (int64_t)ao.nj * (khi - klo + 1), 192), 192, 0, st>>>(P, ao, mom, aorec, lv, klo);
     }
   }
   if ((rc = mark())) return rc;
@@ -3213,9 +3210,14 @@ __global__ void __launch_bounds__(128) k_ao_e_pre(const P1 P, const Box b, const
   const int64_t t = tid();
   int i, j, k;
   if (!b.decode(t, i, j, k)) return;
-  if (!P.active(i, j, 1)) return;
   const cs_p1_patch& a = P.a;
   const int64_t z = P.zi(i, j, k), NZ = P.NZ;
+  if (!P.active(i, j, 1)) {  // the sweep loads an inactive zone's stencil too, and discards it
+    int* idx = (int*)(rec + t * kAoRec + 190);
+#pragma unroll
+    for (int r = 0; r < 18; ++r) idx[r] = (int)z;
+    return;
+  }
   const double obl0 = 1.0 / fmax(a.scale[z], a.smallnum);
   double c0[6], cz[3];
 #pragma unroll
@@ -3325,9 +3327,11 @@ __global__ void __launch_bounds__(128) k_ao_e_pre(const P1 P, const Box b, const
 __device__ __forceinline__ void ao_e_zone(const P1& P, const Box& b, const double* __restrict__ mom,
                                           const double* __restrict__ rec, int ir, int jr, int kr, int c) {
   const int i = b.i0 + ir, j = b.j0 + jr, k = b.k0 + kr;
-  if (!P.active(i, j, 1)) return;
   const cs_p1_patch& a = P.a;
+  // the loads go out before the mask is known: an inactive zone's record
+  // points at the zone itself, and only an active zone writes
   const int64_t z = P.zi(i, j, k), NZ = P.NZ;
+  const bool act = P.active(i, j, 1);
   const double* G = rec + ((int64_t)ir + (int64_t)b.ni * (jr + (int64_t)b.nj * kr)) * kAoRec;
   const int* idx = (const int*)(G + 190);
   double* E = c == 0 ? a.Ex_zc : (c == 1 ? a.Ey_zc : a.Ez_zc);
@@ -3369,6 +3373,7 @@ __device__ __forceinline__ void ao_e_zone(const P1& P, const Box& b, const doubl
   wlo *= aa;
   whi *= aa;
   const double x1 = 1.0 - f;
+  if (!act) return;
   E[z] = Ept * f + (whi * Phi[0] + wlo * e[0]) * x1;
 #pragma unroll
   for (int m = 1; m < 4; ++m) E[z + NZ * m] = (whi * Phi[m] + wlo * e[m]) * x1;
@@ -3376,33 +3381,20 @@ __device__ __forceinline__ void ao_e_zone(const P1& P, const Box& b, const doubl
   for (int m = 4; m < 10; ++m) E[z + NZ * m] = whi * Phi[m] * x1;
 }
 
-// Step 9.c in the serial loop order (k_ao_order) over windows of in-plane
-// levels: in launch s, plane kr of the box takes its in-plane levels
-// l = (i+1) + 3 (j+1) from a = kAoWin s - kAoLag kr to a + kAoWin - 1 in turn,
-// one workgroup per plane (planes klo on), three lanes per zone (one per
-// component of E), a barrier between levels; the zones of a level read none
-// of each other.  A zone's neighbours in the plane below that come earlier in
-// the loop are on in-plane levels up to l+8 there (a plane step is 9 levels),
-// which that plane finished in earlier launches (its window leads by kAoLag =
-// kAoWin + 8 levels); the zones of the plane below that read this zone's old
-// value are on levels up to l+8 as well, done before this launch or outside
-// this launch's window there.  The launches order the planes, the barriers
-// the levels of one plane: no workgroup waits for another.
-__global__ void __launch_bounds__(256) k_weno_ao_e_win(const P1 P, const Box b, const double* __restrict__ mom,
-                                                       const double* __restrict__ rec, int s, int klo) {
-  const int kr = klo + blockIdx.x;
-  const int top = (b.ni - 1) + 3 * (b.nj - 1);
-  const int a = kAoWin * s - kAoLag * kr;
-  const int l0 = max(a, 0), l1 = min(a + kAoWin - 1, top);
-  for (int l = l0; l <= l1; ++l) {
-    const int jlo = max(0, (l - (b.ni - 1) + 2) / 3), jhi = min(b.nj - 1, l / 3);
-    const int nl = 3 * (jhi - jlo + 1);
-    for (int q = threadIdx.x; q < nl; q += blockDim.x) {
-      const int jr = jlo + q / 3;
-      ao_e_zone(P, b, mom, rec, l - 3 * jr, jr, kr, q % 3);
-    }
-    __syncthreads();
-  }
+// Step 9.c in the serial loop order (k_ao_order): one launch per level
+// (i+1) + 3 (j+1) + 9 (k+1) of the box, three lanes per zone (one per
+// component of E), the box's (j, k) rows from plane klo on.  The zones of a
+// level read none of each other; the launches order the levels.
+__global__ void __launch_bounds__(192) k_weno_ao_e(const P1 P, const Box b, const double* __restrict__ mom,
+                                                   const double* __restrict__ rec, int level, int klo) {
+  const int64_t t = tid();
+  const int c = (int)(t % 3);
+  const int64_t tz = t / 3;
+  const int jr = (int)(tz % b.nj), kr = klo + (int)(tz / b.nj);
+  if (kr >= b.nk) return;
+  const int ir = level - 3 * jr - 9 * kr;
+  if (ir < 0 || ir >= b.ni) return;
+  ao_e_zone(P, b, mom, rec, ir, jr, kr, c);
 }
 
 }  // namespace csp1
