@@ -1351,8 +1351,8 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
     P1_HIP(hipMemcpyAsync(&bad, F.counter + 2, sizeof(unsigned int), hipMemcpyDeviceToHost, st));
     P1_HIP(hipStreamSynchronize(st));
     if (bad) {
-      cs::set_error("cs_limit_all_slopes_p12: the stencil table reaches beyond two zones in i or j, so step 9.c "
-                    "cannot follow the serial loop order level by level");
+      cs::set_error("cs_limit_all_slopes_p12: the stencil table breaks the level order of step 9.c (an entry "
+                    "beyond two zones in i or j), so it cannot follow the serial loop order level by level");
       return CS_E_INVALID;
     }
   }

@@ -470,9 +470,10 @@ int cs_build_volume_p12(const cs_config* cfg, const cs_volume_p12* out);
  * hold ten modes and whose v_centroid comes from the 27-point rule, plus
  * the ten basis moments (DEVICE, zone x 10).  Work memory, stream and the
  * host min/max as cs_limit_all_slopes_p1 (the call waits on the stream
- * mid-way and at the end).  CS_E_INVALID, with the outputs undefined, when a
- * stencil-table entry reaches beyond two zones in i or j (the cubed-sphere
- * table reaches two, at the corner strips). */
+ * mid-way and at the end).  CS_E_INVALID, with the outputs undefined, when the
+ * stencil table breaks that order as the call runs it (level
+ * (i+1) + 3 (j+1) + 9 (k+1)), which entries within two zones in i and j never
+ * do; the cubed-sphere table reaches two, at the corner strips. */
 typedef struct cs_p12_patch {
   cs_p1_patch p1;
   const double* moments; /* Coef_Basis_Moments (zone, 10) */

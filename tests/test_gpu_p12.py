@@ -86,3 +86,20 @@ def test_p12_zones_match_oracle(case):
     assert np.abs(V[4:]).max() > 1e-5
     if shock:
         assert (G["flatten"] > 0).sum() > 50
+
+
+def test_p12_rejects_a_stencil_that_breaks_the_step9c_order():
+    """Zone (1, 1)'s right neighbour moved to (1, 4), three rows up (an index
+    inside the patch, so every kernel still reads in bounds): its stencil zone
+    below the plane is then on the same level as the zone, and the call
+    reports that it cannot keep the serial loop order."""
+    N, NR, g = 8, 3, 4
+    print("case p12 stencil order", flush=True)
+    prob, A0 = orc.p12_problem(N, NR, g, sector=0, seed=3)
+    A = orc.p1_copy(A0)
+    nz = N + 2 * g
+    s = (1 - 1 + g) + nz * (1 - 1 + g)
+    assert A["stencil"][s] == 2 and A["stencil"][s + nz * nz] == 1  # direction 0: (2, 1)
+    A["stencil"][s + nz * nz] = 4
+    with pytest.raises(cs.CsError, match="level order"):
+        _run_gpu(prob, A)
