@@ -594,6 +594,27 @@ def bench_p12(cs, args):
     ms_call = tot / steps * 1e3
     own = N * N * NR
     log(f"LIMIT_ALL_SLOPES_P12 on a {N}^2 x {NR} patch: {ms_call:.3f} ms per call, {own / (ms_call * 1e-3):.3e} zones/s")
+    # the stages of the same call, HIP events on its stream, a few calls more
+    stage = None
+    nst = max(1, min(steps, 5))
+    for _ in range(nst):
+        reset()
+        _, _, ms = cs.limit_all_slopes_p12(N, NR, g, scal, T, mom, timed=True)
+        stage = ms if stage is None else [a + b for a, b in zip(stage, ms)]
+    stage = {name: t / nst for name, t in zip(cs.P1_STAGES, stage)}
+    for name, t in stage.items():
+        log(f"   {name:20s} {t:8.3f} ms")
+    # the dominant stage: 8.d's constrained least squares (k_clsq12) on every
+    # active zone of the box -2..N+3, P12_CLSQ_FLOP algorithmic FP64 flops each
+    box = _box_zones(N, NR, g, -2, N + 3, -2, NR + 3).to(T["mask"].device)
+    nclsq = int(torch.count_nonzero(T["mask"].view(-1)[box]))
+    t_clsq = stage["clsq+E"]
+    ach = P12_CLSQ_FLOP * nclsq / (t_clsq * 1e-3) / 1e12
+    roof = {"bound": "mfma", "kernel": "clsq+E (k_clsq12, k_e12)", "achieved": ach, "peak": FP64_PEAK_TFS,
+            "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFS, "traffic": None, "ms_per_launch": t_clsq,
+            "flop_per_zone": P12_CLSQ_FLOP, "zones_per_launch": nclsq,
+            "note": "FP64 VALU (one wave per zone); 78.6 TFLOP/s is the MI355X FP64 peak, vector and matrix alike"}
+    log(f"   k_clsq12: {nclsq} zones, {ach:.2f} TFLOP/s algorithmic ({ach / FP64_PEAK_TFS:.3f} of FP64 peak)")
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_p12(args.cpu_seconds)
@@ -605,9 +626,28 @@ def bench_p12(cs, args):
                                f"least-squares B, zone E) on "
                                f"one sector of the 6x{N}^2x{NR} shell as a patch, NGhost {g} ({own} owned zones, "
                                f"{nz} with ghosts); front + log-normal noise 0.3", "zones": own},
-        "roofline": None, "cpu_baseline": cpu,
+        "roofline": roof, "cpu_baseline": cpu, "stages_ms": stage,
     }
     print(json.dumps(line))
+
+
+# LIMIT_ALL_SLOPES_P12 step 8.d per zone, algorithmic FP64 flops: the
+# symmetric Gram of the 30 point rows of length 30 (30 x 465 multiply-adds),
+# their right-hand side and the 6 face-mean constraint rows (2 x 900), the
+# 36 x 36 KKT system by LU without pivoting (2/3 36^3) and its two triangular
+# solves (2 x 36^2)
+P12_CLSQ_FLOP = 2 * 30 * 465 + 2 * 2 * 900 + (2 * 36 ** 3) // 3 + 2 * 36 ** 2
+FP64_PEAK_TFS = 78.6
+
+
+def _box_zones(N, NR, g, i0, i1, k0, k1):
+    """flat zone indices of the box i, j in [i0, i1], k in [k0, k1] of a patch"""
+    import torch
+    nz = N + 2 * g
+    ii = torch.arange(i0, i1 + 1)
+    kk = torch.arange(k0, k1 + 1)
+    zi = (ii[None, None, :] - 1 + g) + nz * ((ii[None, :, None] - 1 + g) + nz * (kk[:, None, None] - 1 + g))
+    return zi.reshape(-1)
 
 
 def cpu_baseline_p12(max_seconds):

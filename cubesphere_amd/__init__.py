@@ -157,6 +157,7 @@ def lib():
             "cs_limit_all_slopes_p1": [vp, vp],
             "cs_build_volume_p12": [cfgp, vp],
             "cs_limit_all_slopes_p12": [vp, vp],
+            "cs_limit_all_slopes_p12_timed": [vp, vp, ctypes.POINTER(ctypes.c_float)],
             "cs_limit_all_slopes_p1_timed": [vp, vp, ctypes.POINTER(ctypes.c_float)],
             "cs_limit_all_slopes_p1_noalloc": [vp, vp],
             "cs_make_facial_fluxes_p1": [vp, vp],
@@ -728,7 +729,7 @@ class P12Patch(ctypes.Structure):
     _fields_ = [("p1", P1Patch), ("moments", ctypes.c_void_p)]
 
 
-def limit_all_slopes_p12(N, NR, g, scalars, arrays, moments, itri=(1, 1, 1, 1), stream=None):
+def limit_all_slopes_p12(N, NR, g, scalars, arrays, moments, itri=(1, 1, 1, 1), stream=None, timed=False):
     """LIMIT_ALL_SLOPES_P12 on one patch, the third-order WENO-AO
     reconstruction (the P1 steps up to the WENO, the hybridisation with the
     quadratic central stencil, the positivity limiter and the face-point B
@@ -737,7 +738,8 @@ def limit_all_slopes_p12(N, NR, g, scalars, arrays, moments, itri=(1, 1, 1, 1), 
     torch CUDA tensors in the reference's layouts: as limit_all_slopes_p1,
     with U and V of ten modes (zone, 8, 10), B1..3_zc and Ex..Ez_zc of ten,
     v_centroid from the 27-point rule, and moments (zone, 10) a float64 CUDA
-    tensor.  Returns (V_min_local, V_max_local) as numpy arrays of 9."""
+    tensor.  Returns (V_min_local, V_max_local) as numpy arrays of 9, and with
+    timed=True also the per-stage milliseconds (P1_STAGES)."""
     import torch
     want = p1_counts(N, NR, g)
     nzone = want["mask"]
@@ -766,6 +768,10 @@ def limit_all_slopes_p12(N, NR, g, scalars, arrays, moments, itri=(1, 1, 1, 1), 
     vmin, vmax = np.zeros(9), np.zeros(9)
     q.V_min_local, q.V_max_local = vmin.ctypes.data, vmax.ctypes.data
     st = None if stream is None else ctypes.c_void_p(stream.cuda_stream)
+    if timed:
+        ms = (ctypes.c_float * len(P1_STAGES))()
+        _check(lib().cs_limit_all_slopes_p12_timed(ctypes.byref(s), st, ms), "cs_limit_all_slopes_p12_timed")
+        return vmin, vmax, list(ms)
     _check(lib().cs_limit_all_slopes_p12(ctypes.byref(s), st), "cs_limit_all_slopes_p12")
     return vmin, vmax
 

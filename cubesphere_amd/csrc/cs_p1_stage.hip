@@ -680,7 +680,12 @@ __device__ __forceinline__ void positivity(const Dev& D, int s, int i, int j, in
   // the reference checks 0, 1, ..., 14 reductions in turn and takes the first
   // that passes (removing the slopes when none does); the zone's mean lies in
   // the bounds, so in exact arithmetic passing is monotone in the count and
-  // the first passing count is found by probing 0, then 14, then bisecting
+  // the first passing count is found by probing 0, then 14, then bisecting.
+  // The count found passes and the one below it fails (both are checked);
+  // only a node value that rounds across a bound at a smaller count, with
+  // cancellation at least as large as the bound's own rounding, could make
+  // the reference's scan stop earlier (neighbouring counts' node values
+  // differ by 14 % of the slope part: each reduction scales it by 0.1^(1/15))
   int lo = -1, hi = 15, n = 0;
 #pragma unroll 1
   while (hi - lo > 1) {

@@ -623,8 +623,12 @@ __global__ void __launch_bounds__(256) k_pos(const P1 P, const Box b, const int6
   // the reference (440...f:553-600) checks 0, 1, ..., 14 reductions in turn and
   // takes the first that passes, removing the slopes when none does.  With
   // the zone's mean inside the bounds, passing is monotone in the number of
-  // reductions, so the first passing count is found by bisection: 2 to 6
-  // checks instead of up to 15
+  // reductions (in exact arithmetic), so the first passing count is found by
+  // bisection: 2 to 6 checks instead of up to 15.  The count found passes and
+  // the one below it fails (both are checked); only a node value that rounds
+  // across a bound at a smaller count, although the counts' node values
+  // differ by 14 % of the slope part, could make the reference's scan stop
+  // earlier
   // one call site (the check is large): probe 0, then 14, then bisect; lo is
   // the largest count known to fail, hi the smallest known to pass (15: none)
   int lo = -1, hi = 15, n = 0;
@@ -3405,4 +3409,12 @@ extern "C" int cs_limit_all_slopes_p12(const cs_p12_patch* q, void* stream_) {
     return CS_E_INVALID;
   }
   return run_p1(&q->p1, stream_, nullptr, nullptr, 0, q->moments);
+}
+
+extern "C" int cs_limit_all_slopes_p12_timed(const cs_p12_patch* q, void* stream_, float* kernel_ms) {
+  if (!q || !q->moments || !kernel_ms) {
+    cs::set_error("cs_limit_all_slopes_p12_timed: null patch, moments or kernel_ms");
+    return CS_E_INVALID;
+  }
+  return run_p1(&q->p1, stream_, kernel_ms, nullptr, 0, q->moments);
 }

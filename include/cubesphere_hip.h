@@ -479,6 +479,13 @@ typedef struct cs_p12_patch {
   const double* moments; /* Coef_Basis_Moments (zone, 10) */
 } cs_p12_patch;
 int cs_limit_all_slopes_p12(const cs_p12_patch* p, void* stream);
+/* the same, timing its stages with HIP events on the stream (no counterpart
+ * in the reference: measurement only), kernel_ms[CS_P1_STAGES] as for
+ * cs_limit_all_slopes_p1_timed: primitives; face-B clearing and tri-corner
+ * sync; flattener; WENO, WENO-AO and positivity (with any replay); face B
+ * points; constrained least squares + zone E; WENO and WENO-AO of E;
+ * min/max */
+int cs_limit_all_slopes_p12_timed(const cs_p12_patch* p, void* stream, float* kernel_ms);
 
 /* ---- P1 facial fluxes of one patch -----------------------------------------
  * MAKE_FACIAL_FLUXES (400riemann_integrate/450make_facial_fluxes.f:5-60) at

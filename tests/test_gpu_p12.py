@@ -103,3 +103,22 @@ def test_p12_rejects_a_stencil_that_breaks_the_step9c_order():
     A["stencil"][s + nz * nz] = 4
     with pytest.raises(cs.CsError, match="level order"):
         _run_gpu(prob, A)
+
+
+def test_p12_timed_matches_untimed():
+    """cs_limit_all_slopes_p12_timed: the same arrays as the plain call, and
+    eight stage times in milliseconds."""
+    N, NR = 10, 4
+    print("case p12 timed", flush=True)
+    prob, A0 = orc.p12_problem(N, NR, 4, sector=2, seed=4, shock=True, noise=0.5)
+    G = _run_gpu(prob, orc.p1_copy(A0))
+    dev = torch.device("cuda:0")
+    A = orc.p1_copy(A0)
+    T = {k: torch.from_numpy(A[k]).to(dev) for k in cs.P1_ARRAYS}
+    mom = torch.from_numpy(A["moments"]).to(dev)
+    vmin, vmax, ms = cs.limit_all_slopes_p12(prob["N"], prob["NR"], prob["g"], prob, T, mom, timed=True)
+    torch.cuda.synchronize()
+    assert len(ms) == 8 and all(t >= 0.0 for t in ms) and sum(ms) > 0.0
+    for k in cs.P1_ARRAYS:
+        assert np.array_equal(T[k].cpu().numpy(), G[k]), k
+    assert np.array_equal(vmin, G["V_min_local"]) and np.array_equal(vmax, G["V_max_local"])
