@@ -2984,9 +2984,10 @@ __global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const 
 // put one face point each (normal, the ten basis values about the zone's
 // moments, B-field weight, face-point B) in LDS; lane r < 36 forms row r of
 // the augmented system in registers (A^T A = P^T P + I + D^T D, P the point
-// rows, D the divergence rows); Gaussian elimination across the lanes in
-// natural order, then back substitution.  The reference solves the same
-// system by Numerical Recipes' LU with partial pivoting (MATSOLVE): the two
+// rows, D the divergence rows; P^T P by FP64 MFMA across the wave); Gaussian
+// elimination across the lanes in natural order, then back substitution.  The
+// reference solves the same system by Numerical Recipes' LU with partial
+// pivoting (MATSOLVE): the two
 // agree to rounding.
 // lane l's value of v, uniform across the wave (l uniform)
 __device__ __forceinline__ double rdlane(double v, int l) {
@@ -3061,15 +3062,9 @@ __global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const do
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
       double cf = 0.0;  // the constraint column of face f
-#pragma unroll 1
+#pragma unroll
       for (int q = 5 * f; q < 5 * f + 5; ++q) {
         const double pa = pt[q][c] * pt[q][3 + m];
-#pragma unroll
-        for (int cc = 0; cc < 3; ++cc) {
-          const double pn = pa * pt[q][cc];
-#pragma unroll
-          for (int mm = 0; mm < 10; ++mm) row[10 * cc + mm] += pn * pt[q][3 + mm];
-        }
         row[36] += pa * pt[q][14];
         cf += pt[q][13] * pa;
       }
@@ -3089,6 +3084,43 @@ __global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const do
       }
       row[36] += w * pt[q][14];
     }
+  }
+  // P^T P, the Gram of the 30 point rows P[q][10 c + m] = n_c(q) basis_m(q),
+  // by FP64 MFMA 16 x 16 x 4 over the zero-padded 32 x 32: lane l brings
+  // P[4 kc + l / 16][16 I + l % 16] as both operands (A = P^T by columns, B = P
+  // by rows), and result register r of lane l is entry (l / 16 + 4 r, l % 16)
+  // of its 16 x 16 block.  Blocks (0, 0), (0, 1) and (1, 1) are formed, (1, 0)
+  // is (0, 1) transposed; the sum goes through LDS to the row lanes.
+  __shared__ double gram[32][33];
+  {
+    typedef double d4 __attribute__((ext_vector_type(4)));
+    d4 g00 = {0.0, 0.0, 0.0, 0.0}, g01 = g00, g11 = g00;
+    const int l16 = lane % 16, lq = lane / 16;
+#pragma unroll
+    for (int kc = 0; kc < 8; ++kc) {
+      const int q = 4 * kc + lq;
+      double p0 = 0.0, p1 = 0.0;
+      if (q < 30) {
+        p0 = pt[q][l16 / 10] * pt[q][3 + l16 % 10];
+        if (16 + l16 < 30) p1 = pt[q][(16 + l16) / 10] * pt[q][3 + (16 + l16) % 10];
+      }
+      g00 = __builtin_amdgcn_mfma_f64_16x16x4f64(p0, p0, g00, 0, 0, 0);
+      g01 = __builtin_amdgcn_mfma_f64_16x16x4f64(p0, p1, g01, 0, 0, 0);
+      g11 = __builtin_amdgcn_mfma_f64_16x16x4f64(p1, p1, g11, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int gr = lq + 4 * r;
+      gram[gr][l16] = g00[r];
+      gram[gr][16 + l16] = g01[r];
+      gram[16 + l16][gr] = g01[r];
+      gram[16 + gr][16 + l16] = g11[r];
+    }
+  }
+  __syncthreads();
+  if (lane < 30) {
+#pragma unroll
+    for (int e = 0; e < 30; ++e) row[e] += gram[lane][e];
   }
   // Elimination in natural order, lane jj holding pivot row jj: the leading
   // 30 x 30 block is I + D^T D + P^T P, symmetric positive definite with
