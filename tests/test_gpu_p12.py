@@ -122,3 +122,40 @@ def test_p12_timed_matches_untimed():
     for k in cs.P1_ARRAYS:
         assert np.array_equal(T[k].cpu().numpy(), G[k]), k
     assert np.array_equal(vmin, G["V_min_local"]) and np.array_equal(vmax, G["V_max_local"])
+
+
+def test_p12_linear_field_is_exact_at_48x12():
+    """A size-independent property, at a size the oracle would take minutes
+    over: a linear density at rest (uniform pressure, no B) is reconstructed
+    exactly by both the WENO gradients and the 18-zone quadratic fit, so every
+    owned zone gets the gradient (times the scale length and 1 - flattener)
+    and no quadratic modes, whatever the nonlinear weights; B and E stay
+    zero."""
+    N, NR, g = 48, 12, 4
+    print("case p12 linear field 48x12", flush=True)
+    prob, A = orc.p12_problem(N, NR, g, sector=1, seed=1)
+    nz = orc.p1_dims(N, NR, g)["zone"]
+    cen = A["v_centroid"].reshape(3, nz)
+    gvec = np.array([0.4, -0.2, 0.3])
+    U = A["U"].reshape(10, 8, nz)
+    U[:] = 0.0
+    U[0, 0] = 2.0 + gvec @ cen
+    U[0, 4] = 1.0 / (prob["gamma"] - 1.0)
+    for k in ("Bxf", "Byf", "Bzf"):
+        A[k][:] = 0.0
+    A["ent"][:] = 0.0
+    A["ent"][:nz] = 1.0
+    G = _run_gpu(prob, A)
+    V = G["V"].reshape(10, 8, nz)
+    nzz = N + 2 * g
+    ii = np.arange(1, N + 1)
+    kk = np.arange(1, NR + 1)
+    own = ((ii[None, None, :] - 1 + g) + nzz * ((ii[None, :, None] - 1 + g) + nzz * (kk[:, None, None] - 1 + g))).ravel()
+    l0 = np.maximum(G["scale"][own], prob["smallnum"])
+    want = l0[None, :] * gvec[:, None] * (1.0 - G["flatten"][own])[None, :]
+    ref = np.abs(l0[None, :] * gvec[:, None]).max()
+    err_g = np.abs(V[1:4, 0, own] - want).max() / ref
+    err_q = np.abs(V[4:, 0, own]).max() / ref
+    assert err_g < 1e-9 and err_q < 1e-9, (err_g, err_q)
+    for k in ("B1_zc", "B2_zc", "B3_zc", "Ex_zc", "Ey_zc", "Ez_zc"):
+        assert np.abs(G[k].reshape(10, nz)[:, own]).max() == 0.0, k
