@@ -17,10 +17,14 @@
 ! beside the inputs, the new time step as out_dt.bin and the reconstruction's
 ! V_min_local, V_max_local as out_vminmax.bin.
 !
-! Usage: cs_p1_dropin <dir>, where <dir>/scalars.bin holds int32 ntp, nr,
-! nghost, 0 and the float64 gamma, molewt, rhofloor, prsfloor, smallnum,
+! Usage: cs_p1_dropin <dir> [p12], where <dir>/scalars.bin holds int32 ntp,
+! nr, nghost, 0 and the float64 gamma, molewt, rhofloor, prsfloor, smallnum,
 ! flatten_coef, prs_flatten_coef, dtmove, newtonsc, mstar, omega(3), cfl,
-! dtcur.
+! dtcur.  With p12 the driver's third-order reconstruction runs instead
+! (DRV:2398): U, V, B1..3_zc and Ex..Ez_zc hold ten modes, moments.bin the
+! basis moments (zone, 10), and LIMIT_ALL_SLOPES_P12 -> cs_limit_all_slopes_p12
+! (cs_p12_patch) is the only call; the arrays it writes are stored as
+! out_<name>.bin and its V_min_local, V_max_local as out_vminmax.bin.
 program cs_p1_dropin
   use iso_c_binding
   use cs_hip_iface
@@ -59,8 +63,16 @@ program cs_p1_dropin
   integer, parameter :: nout = 13
   character(len=12), parameter :: outs(nout) = [character(len=12) :: &
       "U", "ent", "V", "Bxf", "Byf", "Bzf", "Efx", "Efy", "Efz", "Source", "dU_dt", "flag_entprs", "msonic"]
+  integer, parameter :: nout12 = 14
+  character(len=12), parameter :: outs12(nout12) = [character(len=12) :: &
+      "U", "ent", "V", "B1_zc", "B2_zc", "B3_zc", "Ex_zc", "Ey_zc", "Ez_zc", "Bxf_q", "Byf_q", "Bzf_q", &
+      "flatten", "work"]
   character(len=4096) :: dir
-  type(c_ptr) :: dev(nn + 1), dt_dev
+  character(len=8) :: mode
+  logical :: p12
+  integer(c_int64_t) :: nmode
+  type(c_ptr) :: dev(nn + 1), dt_dev, mom_dev
+  type(cs_p12_patch) :: pq
   integer(c_size_t) :: nbytes(nn + 1)
   integer(c_int32_t) :: ihdr(4)
   real(c_double) :: rhdr(15)
@@ -75,9 +87,13 @@ program cs_p1_dropin
 
   call get_command_argument(1, dir)
   if (len_trim(dir) == 0) then
-    write (6, *) "usage: cs_p1_dropin <dir>"
+    write (6, *) "usage: cs_p1_dropin <dir> [p12]"
     stop 2
   end if
+  call get_command_argument(2, mode)
+  p12 = trim(mode) == "p12"
+  nmode = 4
+  if (p12) nmode = 10
   open (newunit=u, file=trim(dir)//"/scalars.bin", access="stream", form="unformatted", status="old", action="read")
   read (u) ihdr, rhdr
   close (u)
@@ -103,6 +119,8 @@ program cs_p1_dropin
   call hip_check(hipMalloc(dev(nn + 1), nbytes(nn + 1)), "hipMalloc")
   call zero_dev(dev(nn + 1), nbytes(nn + 1))
   call hip_check(hipMalloc(dt_dev, 8_c_size_t), "hipMalloc")
+  mom_dev = c_null_ptr
+  if (p12) call load_moments()
 
   ! LIMIT_ALL_SLOPES_P1 (410limit_all_slopes_p1.f:9-60), CLEANUP_AFTER_UPDATE and EVALUATE_TIMESTEP
   p%ntp = ntp; p%nr = nr; p%nghost = g; p%itri = 1; p%pad = 0
@@ -161,6 +179,27 @@ program cs_p1_dropin
   up%EntFlux_x = d("EntFlux_x"); up%EntFlux_y = d("EntFlux_y"); up%EntFlux_z = d("EntFlux_z")
   up%Efx = d("Efx"); up%Efy = d("Efy"); up%Efz = d("Efz"); up%flag_entprs = dev(nn + 1)
 
+  if (p12) then
+    ! LIMIT_ALL_SLOPES_P12 (421limit_all_slopes_p12.f:9), the patch's own
+    ! arrays of ten modes and the basis moments
+    pq%p1 = p
+    pq%moments = mom_dev
+    call cs_check(cs_limit_all_slopes_p12(pq, c_null_ptr), "LIMIT_ALL_SLOPES_P12")
+    call hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize")
+    do i = 1, nout12
+      call store(outs12(i))
+    end do
+    open (newunit=u, file=trim(dir)//"/out_vminmax.bin", access="stream", form="unformatted", status="replace")
+    write (u) vmin, vmax
+    close (u)
+    do i = 1, nn + 1
+      call hip_check(hipFree(dev(i)), "hipFree")
+    end do
+    call hip_check(hipFree(dt_dev), "hipFree")
+    call hip_check(hipFree(mom_dev), "hipFree")
+    write (6, '(a, i0, a, i0, a, i0)') "p12 dropin: ntp=", ntp, " nr=", nr, " nghost=", g
+    stop
+  end if
   call cs_check(cs_limit_all_slopes_p1(p, c_null_ptr), "LIMIT_ALL_SLOPES_P1")
   call cs_check(cs_make_facial_fluxes_p1(f, c_null_ptr), "MAKE_FACIAL_FLUXES")
   call cs_check(cs_make_edge_efields_p1(e, c_null_ptr), "MAKE_EDGE_EFIELDS_P1")
@@ -206,8 +245,9 @@ contains
     case ("zb_pts", "zb_normal", "zf_pts", "zf_normal"); n = 15 * zf
     case ("zb_wts"); n = 5 * zf
     case ("v_centroid"); n = 3 * zone
-    case ("U", "V"); n = 32 * zone
-    case ("ent", "B1_zc", "B2_zc", "B3_zc", "Ex_zc", "Ey_zc", "Ez_zc"); n = 4 * zone
+    case ("U", "V"); n = 8 * nmode * zone
+    case ("ent"); n = 4 * zone
+    case ("B1_zc", "B2_zc", "B3_zc", "Ex_zc", "Ey_zc", "Ez_zc"); n = nmode * zone
     case ("Bxf", "Byf", "Bzf", "S_L", "S_R", "S_D", "S_U", "S_B", "S_T", "EntFlux_x", "EntFlux_y", "EntFlux_z", &
           "Efx", "Efy", "Efz"); n = fc
     case ("Bxf_q", "Byf_q", "Bzf_q"); n = 5 * fc
@@ -268,6 +308,30 @@ contains
     call hip_check(hipMemcpy(dev(k), c_loc(buf), nbytes(k), H2D), "hipMemcpy")
   end subroutine
 
+  ! the third order's basis moments (zone, 10) from moments.bin
+  subroutine load_moments()
+    integer :: v, ios
+    integer(c_int64_t) :: sz
+    integer(c_size_t) :: nb
+    real(c_double), allocatable, target :: buf(:)
+    nb = 80_c_size_t * zone
+    open (newunit=v, file=trim(dir)//"/moments.bin", access="stream", form="unformatted", status="old", &
+          action="read", iostat=ios)
+    if (ios /= 0) then
+      write (6, *) "cannot open moments.bin"
+      stop 2
+    end if
+    inquire (unit=v, size=sz)
+    if (sz /= nb) then
+      write (6, *) "moments.bin holds ", sz, " bytes, its layout ", nb
+      stop 2
+    end if
+    allocate (buf(10 * zone))
+    read (v) buf
+    close (v)
+    call hip_check(hipMalloc(mom_dev, nb), "hipMalloc")
+    call hip_check(hipMemcpy(mom_dev, c_loc(buf), nb, H2D), "hipMemcpy")
+  end subroutine
   subroutine zero_dev(ptr, n)
     type(c_ptr), intent(in) :: ptr
     integer(c_size_t), intent(in) :: n

@@ -5,7 +5,9 @@ ISO_C_BINDING module (cubesphere_amd/fortran/cs_hip_iface.f90):
     for the same inputs and call sequence;
   * cs_p1_dropin fills the bind(C) structures of the reference's P1 routines
     with device arrays and calls the seven entry points in the driver's order;
-    every array they write must equal the Python path's bit for bit."""
+    every array they write must equal the Python path's bit for bit; in its
+    p12 mode it calls the third-order reconstruction through cs_p12_patch,
+    with the same bar."""
 import os
 import re
 import subprocess
@@ -119,5 +121,50 @@ def test_fortran_p1_stage_matches_python_path(tmp_path):
         assert bad.size == 0, f"{k} differs at {bad.size} entries, first {bad[:5]}"
     dt_f = np.fromfile(os.path.join(str(tmp_path), "out_dt.bin"), dtype=np.float64)
     assert dt_f.shape == (1,) and dt_f[0] == dt_p, (dt_f, dt_p)
+    mm = np.fromfile(os.path.join(str(tmp_path), "out_vminmax.bin"), dtype=np.float64)
+    np.testing.assert_array_equal(mm, np.concatenate([vmin, vmax]))
+
+
+# the arrays LIMIT_ALL_SLOPES_P12 writes
+P12_FORTRAN_OUT = ("U", "ent", "V", "B1_zc", "B2_zc", "B3_zc", "Ex_zc", "Ey_zc", "Ez_zc", "Bxf_q", "Byf_q",
+                   "Bzf_q", "flatten", "work")
+
+
+def test_fortran_p12_matches_python_path(tmp_path):
+    """The driver's third-order reconstruction (DRV:2398) called from Fortran
+    through the bind(C) cs_p12_patch of cs_hip_iface.f90 (a cs_p1_patch of
+    ten-mode arrays and the basis moments) on device arrays: every array
+    cs_limit_all_slopes_p12 writes, and the min/max, equal bit for bit what the
+    same entry point writes when called through ctypes from Python."""
+    torch = pytest.importorskip("torch")
+    from oracle import oracle as orc
+    exe = os.path.join(os.path.dirname(cs.__file__), "cs_p1_dropin")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran P1 program not built (no flang)")
+    N, NR, g = 10, 4, 4
+    print(f"\nFortran P12: N={N} NR={NR} sector 2 seed 5 noise 0.5", flush=True)
+    prob, A0 = orc.p12_problem(N, NR, g, sector=2, seed=5, shock=True, noise=0.5)
+    d = orc.p1_dims(N, NR, g)
+    for k in cs.P1F_OUT:
+        A0[k] = np.zeros((8 if k.startswith("Fluct") else 1) * d["fc"])
+    A0["dU_dt"] = np.zeros(8 * d["zone"])
+    A0["Source"] = np.zeros(8 * d["zone"])
+    write_p1_dropin_inputs(str(tmp_path), prob, A0, 0.0, 0.0, 0.0, (0.0, 0.0, 0.0), 0.4, 1.0)
+    np.ascontiguousarray(A0["moments"]).tofile(os.path.join(str(tmp_path), "moments.bin"))
+    r = subprocess.run([exe, str(tmp_path), "p12"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    print(r.stdout.strip(), flush=True)
+
+    dev = torch.device("cuda:0")
+    T = {k: torch.from_numpy(A0[k]).to(dev) for k in cs.P1_ARRAYS}
+    mom = torch.from_numpy(A0["moments"]).to(dev)
+    vmin, vmax = cs.limit_all_slopes_p12(N, NR, g, prob, T, mom)
+    torch.cuda.synchronize()
+    for k in P12_FORTRAN_OUT:
+        want = T[k].cpu().numpy()
+        got = np.fromfile(os.path.join(str(tmp_path), "out_" + k + ".bin"), dtype=want.dtype)
+        assert got.shape == want.shape, k
+        bad = np.flatnonzero(got != want)
+        assert bad.size == 0, f"{k} differs at {bad.size} entries, first {bad[:5]}"
     mm = np.fromfile(os.path.join(str(tmp_path), "out_vminmax.bin"), dtype=np.float64)
     np.testing.assert_array_equal(mm, np.concatenate([vmin, vmax]))
