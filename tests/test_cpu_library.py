@@ -108,3 +108,19 @@ def test_ghost_push_maps_invert_the_exchange(N, g, tmp_path):
     subprocess.run([gxx, "-std=c++17", "-O1", "-I", inc, src, "-o", str(exe)], check=True)
     r = subprocess.run([str(exe), str(N), str(g)], capture_output=True, text=True)
     assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout + r.stderr
+
+
+def test_third_order_entry_points_reject_null_arguments():
+    """cs_limit_all_slopes_p12 and its timed form return CS_E_INVALID with a
+    message, before any device call, when the patch, its moments or the
+    timing array is missing."""
+    L = cs.lib()
+    assert L.cs_limit_all_slopes_p12(None, None) == -1
+    assert "cs_limit_all_slopes_p12" in cs.last_error()
+    q = cs.P12Patch()  # moments left null
+    assert L.cs_limit_all_slopes_p12(ctypes.byref(q), None) == -1
+    assert "moments" in cs.last_error()
+    ms = (ctypes.c_float * 8)()
+    assert L.cs_limit_all_slopes_p12_timed(ctypes.byref(q), None, ms) == -1
+    assert L.cs_limit_all_slopes_p12_timed(None, None, None) == -1
+    assert "kernel_ms" in cs.last_error()
