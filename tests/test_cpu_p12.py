@@ -147,6 +147,18 @@ def test_p12_zones_with_flattener_and_positivity_bit_exact(case):
         assert st[1] > 5, "and remove slopes"
 
 
+def test_p12_zones_bit_exact_with_nghost5():
+    """The same routine on a patch with NGhost 5 (front and noise): the
+    oracle's layouts and loop ranges follow the ghost width as the
+    reference's do, bit-exact on every output array."""
+    _ref3_or_skip()
+    prob, A0 = orc.p12_problem(8, 3, 5, sector=3, seed=9, shock=True, noise=0.6)
+    A = orc.p12_oracle(prob, orc.p1_copy(A0))
+    B = orc.p12_reference(prob, orc.p1_copy(A0))
+    bad = [k for k in A if not np.array_equal(A[k], B[k])]
+    assert not bad, bad
+
+
 def _quad_problem(N=8, NR=4, sector=1):
     prob, A = orc.p12_problem(N, NR, 4, sector=sector, seed=1)
     cfg = _cfg(N, NR)

@@ -41,6 +41,7 @@ CASES = [
     (12, 5, 4, 2, True, 0.6),
     (16, 6, 2, 5, False, 1.5),
     (10, 4, 5, 7, True, 1.0),
+    (8, 3, 3, 9, True, 0.6, 5),  # NGhost 5
 ]
 
 
@@ -55,15 +56,16 @@ def _run_gpu(prob, A):
     return out
 
 
-@pytest.mark.parametrize("case", CASES, ids=lambda c: "N%d_nr%d_s%d_seed%d_%s_noise%g" % (
-    c[0], c[1], c[2], c[3], "front" if c[4] else "smooth", c[5]))
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "N%d_nr%d_s%d_seed%d_%s_noise%g_g%d" % (
+    c[0], c[1], c[2], c[3], "front" if c[4] else "smooth", c[5], c[6] if len(c) > 6 else 4))
 def test_p12_zones_match_oracle(case):
-    N, NR, s, seed, shock, noise = case
+    N, NR, s, seed, shock, noise = case[:6]
+    g = case[6] if len(case) > 6 else 4
     print(f"case p12 zones {case}", flush=True)
-    prob, A0 = orc.p12_problem(N, NR, 4, sector=s, seed=seed, shock=shock, noise=noise)
+    prob, A0 = orc.p12_problem(N, NR, g, sector=s, seed=seed, shock=shock, noise=noise)
     O = orc.p12_oracle(prob, orc.p1_copy(A0))
     G = _run_gpu(prob, orc.p1_copy(A0))
-    nz = orc.p1_dims(N, NR, 4)["zone"]
+    nz = orc.p1_dims(N, NR, g)["zone"]
     bad = []
     for k in cs.P1_ARRAYS + ("V_min_local", "V_max_local"):
         g, o = G[k], O[k]
