@@ -699,8 +699,8 @@ def limit_all_slopes_p1(N, NR, g, scalars, arrays, itri=(1, 1, 1, 1), stream=Non
 
 
 # ---------------------------------------------------------------------------
-# Third order: the NUMORDER 3 volume geometry and the zone-centred part of
-# LIMIT_ALL_SLOPES_P12 (cs_build_volume_p12, cs_limit_all_slopes_p12)
+# Third order: the NUMORDER 3 volume geometry and LIMIT_ALL_SLOPES_P12
+# (cs_build_volume_p12, cs_limit_all_slopes_p12)
 
 class VolumeP12(ctypes.Structure):
     """cs_volume_p12: host outputs of the third-order volume geometry."""

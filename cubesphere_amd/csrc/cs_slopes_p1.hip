@@ -2788,10 +2788,12 @@ extern "C" int cs_pad_boundary_r_p1(const cs_p1_pad* p, void* stream_) {
 }
 
 // ---------------------------------------------------------------------------
-// Third order: steps 2 to 7.b of LIMIT_ALL_SLOPES_P12 (421limit_all_slopes_p12.f:9)
-// on one patch whose U and V hold ten modes.  Steps 2-7.a are the P1 kernels
-// above (they address modes 1-4, which a ten-mode array lays out exactly as
-// a four-mode one); k_zero_hi clears modes 5-10, k_weno_ao is step 7.b.
+// Third order: LIMIT_ALL_SLOPES_P12 (421limit_all_slopes_p12.f:9) on one patch
+// whose U and V hold ten modes.  Steps 2-7.a are the P1 kernels above (they
+// address modes 1-4, which a ten-mode array lays out exactly as a four-mode
+// one); k_zero_hi clears modes 5-10, k_weno_ao is step 7.b, k_pos<10> 7.c,
+// k_face_pts<10> 8.a-c, k_clsq12 8.d, k_e12 9.a, k_weno9 on E 9.b,
+// k_ao_order / k_ao_e_pre / k_weno_ao_e 9.c, and the P1 diagnostics step 10.
 namespace csp1 {
 
 __global__ void __launch_bounds__(256) k_zero_hi(const P1 P, const Box b) {
