@@ -98,7 +98,7 @@ def test_reconstruction_row_and_matinv_bit_exact_with_reference():
             assert np.abs(a @ M - np.eye(n)).max() < 1e-10
 
 
-CASES = [(8, 3, 0, 3), (8, 3, 1, 3), (6, 4, 2, 7), (8, 3, 5, 11)]
+CASES = [(8, 3, 0, 3), (8, 3, 1, 3), (6, 4, 2, 7), (8, 3, 5, 11), (5, 2, 0, 11), (9, 3, 4, 13)]
 
 
 def _hydro(A, nz):

@@ -42,6 +42,8 @@ CASES = [
     (16, 6, 2, 5, False, 1.5),
     (10, 4, 5, 7, True, 1.0),
     (8, 3, 3, 9, True, 0.6, 5),  # NGhost 5
+    (5, 2, 0, 11, True, 0.8),  # a patch barely wider than its ghosts
+    (9, 3, 4, 13, False, 0.3),  # odd N
 ]
 
 
