@@ -3132,11 +3132,15 @@ __global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const do
   // lane, broadcast through scalar registers.
 #pragma unroll
   for (int jj = 0; jj < 36; ++jj) {
-    const double d = rdlane(row[jj], jj);
-    const double fct = row[jj] / d;
+    // the whole pivot row first, then the updates: the scalar registers the
+    // broadcasts write are read well after, without wait states between
+    double pr[37];
+#pragma unroll
+    for (int e = jj; e < 37; ++e) pr[e] = rdlane(row[e], jj);
+    const double fct = row[jj] / pr[jj];
     if (lane > jj) {
 #pragma unroll
-      for (int e = jj + 1; e < 37; ++e) row[e] -= fct * rdlane(row[e], jj);
+      for (int e = jj + 1; e < 37; ++e) row[e] -= fct * pr[e];
     }
   }
   // back substitution by columns: x[jj] from lane jj, then every lane above
