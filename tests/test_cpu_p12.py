@@ -12,6 +12,7 @@ modes (7.c, 440...f:594-600), the face-point B of 8.a-c with them
 the quadratic interpolant of 9 (INTERPOLATION_COEFS, 252...f:5), and the
 diagnostics of 10; every output array is compared."""
 import ctypes
+import os
 
 import numpy as np
 import pytest
@@ -288,3 +289,29 @@ def test_step9c_levels_follow_the_serial_loop_order(N):
     bad, reach = violations(3, 9)
     assert bad == 0 and reach == 2
     assert violations(2, 4)[0] > 0
+
+
+def test_p12_oracle_matches_reference_digests():
+    """Without the reference build: the oracle's LIMIT_ALL_SLOPES_P12 leaves,
+    bit for bit, the arrays the reference's own routine left on the same
+    inputs (SHA-256 of every array, tests/golden/p12_digests.json, written by
+    tests/golden/make_golden_p12.py from the reference built at NUMORDER 3).
+    The inputs' digests are checked first, so a change in the problem
+    generator is told apart from a change in the routine."""
+    import hashlib
+    import json
+    path = os.path.join(os.path.dirname(__file__), "golden", "p12_digests.json")
+    cases = json.load(open(path))
+    assert len(cases) == 3
+
+    def dig(a):
+        return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+    for c in cases:
+        N, NR, g, s, seed, shock, noise = c["case"]
+        prob, A0 = orc.p12_problem(N, NR, g, sector=s, seed=seed, shock=shock, noise=noise)
+        bad_in = [k for k in sorted(A0) if dig(A0[k]) != c["inputs"].get(k)]
+        assert not bad_in and set(A0) == set(c["inputs"]), ("inputs differ", c["case"], bad_in)
+        A = orc.p12_oracle(prob, orc.p1_copy(A0))
+        bad = [k for k in sorted(A) if dig(A[k]) != c["reference"].get(k)]
+        assert not bad and set(A) == set(c["reference"]), (c["case"], bad)
