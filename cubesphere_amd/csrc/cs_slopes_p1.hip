@@ -1127,7 +1127,7 @@ __global__ void __launch_bounds__(256) k_ao_order(const P1 P, const Box b, unsig
 constexpr int kAoRec = 200;  // doubles per zone record of k_ao_e_pre: G (19 x 10), 18 int32 indices
 __global__ void __launch_bounds__(128) k_ao_e_pre(const P1 P, const Box b, const double* __restrict__ mom,
                                                   double* __restrict__ rec);
-__global__ void __launch_bounds__(192) k_weno_ao_e(const P1 P, const Box b, const double* __restrict__ mom,
+__global__ void __launch_bounds__(64) k_weno_ao_e(const P1 P, const Box b, const double* __restrict__ mom,
                                                    const double* __restrict__ rec, int level, int klo);
 __global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom,
                                                  const int64_t* list, int64_t nlist);
@@ -1337,7 +1337,7 @@ static int run_p1(const cs_p1_patch* p, void* stream_, float* kernel_ms, uint8_t
     for (int lv = 0; lv <= top; ++lv) {
       const int klo = std::max(0, (lv - (ao.ni - 1) - 3 * (ao.nj - 1) + 8) / 9);
       const int khi = std::min(ao.nk - 1, lv / 9);
-      k_weno_ao_e<<<grid_of(3 * (int64_t)ao.nj * (khi - klo + 1), 192), 192, 0, st>>>(P, ao, mom, aorec, lv, klo);
+      k_weno_ao_e<<<grid_of(3 * (int64_t)ao.nj * (khi - klo + 1), 64), 64, 0, st>>>(P, ao, mom, aorec, lv, klo);
     }
   }
   if ((rc = mark())) return rc;
@@ -3427,7 +3427,7 @@ __device__ __forceinline__ void ao_e_zone(const P1& P, const Box& b, const doubl
 // (i+1) + 3 (j+1) + 9 (k+1) of the box, three lanes per zone (one per
 // component of E), the box's (j, k) rows from plane klo on.  The zones of a
 // level read none of each other; the launches order the levels.
-__global__ void __launch_bounds__(192) k_weno_ao_e(const P1 P, const Box b, const double* __restrict__ mom,
+__global__ void __launch_bounds__(64) k_weno_ao_e(const P1 P, const Box b, const double* __restrict__ mom,
                                                    const double* __restrict__ rec, int level, int klo) {
   const int64_t t = tid();
   const int c = (int)(t % 3);
