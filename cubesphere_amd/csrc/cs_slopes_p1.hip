@@ -478,10 +478,12 @@ __device__ __forceinline__ void columns9(const P1& P, int i, int j, int64_t* col
 
 // NM = 4: LIMIT_ALL_SLOPES_P1; NM = 10: LIMIT_ALL_SLOPES_P12 (step 7.c), the
 // node values carrying the quadratic modes about the zone's own moments mom
-// (440...f:594-600) and every mode above the mean reduced
+// (440...f:594-600) and every mode above the mean reduced.  Two waves per
+// SIMD: at NM = 10 the registers are held to 256 with a small spill, which
+// the second wave more than pays for (the NM = 4 form fits without one)
 template <int NM>
-__global__ void __launch_bounds__(256) k_pos(const P1 P, const Box b, const int64_t* list, int64_t nlist,
-                                             const Flags F, const double* __restrict__ mom) {
+__global__ void __attribute__((amdgpu_waves_per_eu(2))) __launch_bounds__(256)
+    k_pos(const P1 P, const Box b, const int64_t* list, int64_t nlist, const Flags F, const double* __restrict__ mom) {
   const int64_t t = tid();
   int i, j, k;
   int64_t z;
@@ -1129,8 +1131,8 @@ __global__ void __launch_bounds__(128) k_ao_e_pre(const P1 P, const Box b, const
                                                   double* __restrict__ rec);
 __global__ void __launch_bounds__(64) k_weno_ao_e(const P1 P, const Box b, const double* __restrict__ mom,
                                                    const double* __restrict__ rec, int level, int klo);
-__global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom,
-                                                 const int64_t* list, int64_t nlist);
+__global__ void __attribute__((amdgpu_waves_per_eu(2))) __launch_bounds__(128)
+    k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom, const int64_t* list, int64_t nlist);
 }  // namespace csp1
 
 // mom (optional, DEVICE zone x 10): the third-order form, LIMIT_ALL_SLOPES_P12:
@@ -2828,9 +2830,10 @@ __device__ __forceinline__ constexpr int lt(int r, int c) { return r * (r + 1) /
 // flattener is 1; smoothness indicators of the two polynomials; weights
 // (1 + tau / IS)), and every mode above the mean is multiplied by
 // (1 - flattener).  The reference inverts A^T A by LU (MATINV); the two
-// agree to rounding.
-__global__ void __launch_bounds__(128) k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom,
-                                                 const int64_t* list, int64_t nlist) {
+// agree to rounding.  Two waves per SIMD, the registers held to 256 with a
+// small spill.
+__global__ void __attribute__((amdgpu_waves_per_eu(2))) __launch_bounds__(128)
+    k_weno_ao(const P1 P, const Box b, const double* __restrict__ mom, const int64_t* list, int64_t nlist) {
   int i, j, k;
   if (list) {  // the positivity replay's zones: those inside the range
     const int64_t t = tid();
