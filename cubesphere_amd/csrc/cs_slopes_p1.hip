@@ -3009,7 +3009,11 @@ __constant__ double c_div_rows[4][30] = {
     {0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0, 0, 2, 0, 0, 0}};
 
 __global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const double* __restrict__ mom) {
-  __shared__ double pt[30][16];  // n (3), basis (10), weight, face-point B
+  // one LDS block: the face points' words first (pt), then the Gram (gram),
+  // which is written after the last read of pt; 8.4 KB per zone instead of
+  // 12.3, so more zones fit a CU than its registers allow anyway
+  __shared__ double sh[32 * 33];
+  double(*pt)[16] = reinterpret_cast<double(*)[16]>(sh);  // n (3), basis (10), weight, face-point B
   const int lane = threadIdx.x;
   int i, j, k;
   if (!b.decode(blockIdx.x, i, j, k)) return;  // one zone per workgroup: uniform
@@ -3096,7 +3100,7 @@ __global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const do
   // by rows), and result register r of lane l is entry (l / 16 + 4 r, l % 16)
   // of its 16 x 16 block.  Blocks (0, 0), (0, 1) and (1, 1) are formed, (1, 0)
   // is (0, 1) transposed; the sum goes through LDS to the row lanes.
-  __shared__ double gram[32][33];
+  double(*gram)[33] = reinterpret_cast<double(*)[33]>(sh);
   {
     typedef double d4 __attribute__((ext_vector_type(4)));
     d4 g00 = {0.0, 0.0, 0.0, 0.0}, g01 = g00, g11 = g00;
@@ -3113,6 +3117,7 @@ __global__ void __launch_bounds__(64) k_clsq12(const P1 P, const Box b, const do
       g01 = __builtin_amdgcn_mfma_f64_16x16x4f64(p0, p1, g01, 0, 0, 0);
       g11 = __builtin_amdgcn_mfma_f64_16x16x4f64(p1, p1, g11, 0, 0, 0);
     }
+    __syncthreads();  // pt is read no more: the Gram takes its place
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int gr = lq + 4 * r;
